@@ -1,0 +1,281 @@
+/*
+ * mpt.h - C ABI of the MI355X wavefront path-tracing core.
+ *
+ * This is the drop-in boundary for the reference renderer's hot path,
+ * camera::render (include/camera.h:79-105 of adriendjn/mypbr) and the
+ * closest-hit query under it (bvh::hit, include/bvh.h:45-86). Everything is
+ * plain C: opaque handles, plain pointers and sizes, no C++ or torch types.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative MPT_E_* code on
+ *    error; mpt_last_error() then returns a message (thread-local).
+ *  - Pointers to outputs are caller-allocated host memory unless the name
+ *    says "dev" (device memory the caller owns, for callers that keep data
+ *    in HBM).
+ *  - Geometry and shading arithmetic is IEEE double, as in the reference
+ *    (vec3.h:11); only the traversal bookkeeping uses integers.
+ *  - Unsupported objects or materials make mpt_scene_create fail; nothing
+ *    falls back to a CPU path.
+ */
+#ifndef MPT_H
+#define MPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPT_ABI_VERSION 1
+
+enum {
+    MPT_OK = 0,
+    MPT_E_INVALID = -1,     /* bad argument or malformed description */
+    MPT_E_UNSUPPORTED = -2, /* object/material the GPU path does not model */
+    MPT_E_HIP = -3,         /* a HIP runtime call failed */
+    MPT_E_IO = -4,          /* a scene or data file could not be read */
+    MPT_E_CAPACITY = -5     /* a queue or table would overflow */
+};
+
+/* ---------------------------------------------------------------------- */
+/* Flattened scene description (host SoA handed to mpt_scene_create).     */
+/* ---------------------------------------------------------------------- */
+
+/* Primitive kinds (geometry a leaf tests). */
+enum { MPT_PRIM_SPHERE = 0, MPT_PRIM_QUAD = 1, MPT_PRIM_TRIANGLE = 2 };
+
+/* Object kinds: one entry of the world list the top-level BVH is built on
+ * (scene.h:406-420 adds each scene line as one such object). */
+enum {
+    MPT_OBJ_PRIM = 0,   /* a single primitive (sphere.h, quad.h) */
+    MPT_OBJ_LIST = 1,   /* hittable_list tested linearly (hittable_list.h:25-39), e.g. box() */
+    MPT_OBJ_MESH = 2,   /* triangle mesh with its own BVH (mesh.h:175-191) */
+    MPT_OBJ_MEDIUM = 3  /* constant_medium over a list/sphere boundary (medium.h:60-88) */
+};
+
+/* Material types (material.h, medium.h). */
+enum {
+    MPT_MAT_LAMBERTIAN = 0,
+    MPT_MAT_METAL = 1,
+    MPT_MAT_DIELECTRIC = 2,
+    MPT_MAT_PRINCIPLED = 3,
+    MPT_MAT_DIFFUSE_LIGHT = 4,
+    MPT_MAT_PHASE = 5
+};
+
+/* Texture types (texture.h:24-56). */
+enum { MPT_TEX_SOLID = 0, MPT_TEX_CHECKER = 1 };
+
+typedef struct mpt_texture {
+    int32_t type;        /* MPT_TEX_* */
+    int32_t pad;
+    double inv_scale;    /* checker: 1 / cube side */
+    double even[3];      /* solid: the color; checker: even cells */
+    double odd[3];       /* checker: odd cells */
+} mpt_texture;
+
+typedef struct mpt_material {
+    int32_t type;        /* MPT_MAT_* */
+    int32_t texture;     /* albedo / base color texture index */
+    double roughness;    /* metal, dielectric, principled (clamped to [0,1]) */
+    double alpha;        /* roughness_to_alpha(roughness), microfacet.h:17-19 */
+    double ior;          /* dielectric index; principled ior */
+    double metallic;     /* principled */
+    double f0;           /* principled: dielectric f0 = ((ior-1)/(ior+1))^2 */
+    double g;            /* phase: Henyey-Greenstein anisotropy */
+    double spread;       /* diffuse_light: grid spread in degrees (180 = bare) */
+    double tan_half;     /* diffuse_light: tan(spread/2) */
+    double albedo[3];    /* metal albedo (Schlick f0), phase albedo */
+    double f_avg[3];     /* metal: Schlick F_avg = f0 + (1-f0)/21 */
+    double emit[3];      /* diffuse_light radiance */
+    double sigma_a[3];   /* dielectric absorption (Beer-Lambert) */
+    double avg_emission[3]; /* diffuse_light: emit * directional share */
+} mpt_material;
+
+/* An affine transform with its stored inverse, as transform.h:120-123:
+ * m (row-major 3x3), t, inv_m, inv_t. */
+typedef struct mpt_xform {
+    double m[9];
+    double t[3];
+    double inv_m[9];
+    double inv_t[3];
+} mpt_xform;
+
+typedef struct mpt_scene_desc {
+    /* Primitives. prim_data holds 9 doubles per primitive:
+     *   sphere:   center xyz, radius, 5 unused
+     *   quad:     Q xyz, u xyz, v xyz
+     *   triangle: p0 xyz, p1 xyz, p2 xyz                                  */
+    int64_t n_prims;
+    const int32_t* prim_kind;      /* [n_prims] MPT_PRIM_* */
+    const int32_t* prim_material;  /* [n_prims] material index (-1 = none) */
+    const double* prim_data;       /* [n_prims * 9] */
+
+    /* World objects, in world-list order. An object covers the primitive
+     * range [first, first + count); several objects may share a range
+     * (instances of one mesh). Canonical primitive IDs are id_base + k. */
+    int32_t n_objects;
+    const int32_t* object_kind;    /* [n_objects] MPT_OBJ_* */
+    const int64_t* object_first;   /* [n_objects] */
+    const int64_t* object_count;   /* [n_objects] */
+    const int32_t* object_xform;   /* [n_objects] -1 or index into xforms */
+    const int64_t* object_id_base; /* [n_objects] first canonical ID */
+    const double* object_density;  /* [n_objects] medium density (else 0) */
+    const int32_t* object_material;/* [n_objects] medium phase material (else -1) */
+    const int32_t* object_boundary_kind; /* [n_objects] medium boundary: MPT_OBJ_PRIM or MPT_OBJ_LIST */
+
+    int32_t n_xforms;
+    const mpt_xform* xforms;
+
+    int32_t n_materials;
+    const mpt_material* materials;
+    int32_t n_textures;
+    const mpt_texture* textures;
+
+    /* Lights for next event estimation, in light-list order
+     * (light_list.h:33-134): an object index, or -1 for the environment. */
+    int32_t n_lights;
+    const int32_t* light_object;
+    int32_t light_strategy;        /* 0 = uniform, 1 = power (light_list.h:35-38), 2 = a plain
+                                      hittable_list: pick min(int(xi n), n-1) (hittable_list.h:58-62) */
+    double light_uniform_fraction; /* 0.2 by default */
+
+    /* Environment (environment.h): an equirectangular RGB image, row 0 at +y,
+     * or width = 0 for none. */
+    int32_t env_width, env_height;
+    const float* env_rgb;          /* [env_width * env_height * 3] */
+    double env_intensity;
+    double env_rotation_degrees;
+} mpt_scene_desc;
+
+/* Camera and render settings: the public fields of the reference camera
+ * (camera.h:21-56). */
+typedef struct mpt_camera {
+    double aspect_ratio;
+    int32_t image_width;
+    int32_t samples_per_pixel;
+    int32_t max_depth;
+    int32_t max_diffuse, max_specular, max_scatter;
+    int32_t transparent_shadows;
+    double vfov;
+    double lookfrom[3], lookat[3], vup[3];
+    int32_t use_sky;               /* gradient sky when no environment and use_sky */
+    int32_t collect_features;      /* fill albedo/normal/variance */
+    double background[3];
+    double ortho_height;
+    double defocus_angle;
+    double focus_dist;
+    int32_t blades;
+    int32_t pad;
+    double blade_rotation;
+} mpt_camera;
+
+/* Knobs of the GPU path that do not change the estimate. Zero means
+ * "choose" (autotuned or default). */
+typedef struct mpt_render_opts {
+    uint64_t seed;                 /* counter-based RNG seed */
+    int32_t first_pass;            /* pass index of the first sample rendered (resume) */
+    int32_t save_every;            /* call on_pass every this many passes (0 = never) */
+    int32_t passes_in_flight;      /* samples per pixel traced together (0 = auto) */
+    int32_t block_size;            /* threads per block of the stage kernels (0 = auto) */
+    int32_t sort_by_material;      /* -1 = off, 1 = on, 0 = auto */
+    int32_t use_graph;             /* -1 = off, 1 = on, 0 = auto */
+    int32_t scene_index;           /* keys the RNG for batched multi-scene launches */
+    int32_t pad;
+} mpt_render_opts;
+
+/* Per-render statistics. */
+typedef struct mpt_render_stats {
+    int64_t paths;                 /* pixels x passes rendered */
+    int64_t extension_rays;        /* extension rays traced */
+    int64_t shadow_rays;           /* shadow rays traced */
+    double seconds;                /* wall time of the GPU part */
+} mpt_render_stats;
+
+typedef struct mpt_scene mpt_scene;
+
+/* Progress callback, invoked on the calling thread after every save_every
+ * passes with the number of passes done (camera.h:99-101). The averaged
+ * frame so far can be read with mpt_read_frame inside the callback. */
+typedef void (*mpt_pass_callback)(int32_t done, void* user);
+
+const char* mpt_last_error(void);
+int32_t mpt_abi_version(void);
+
+/* Device information for logs: name, CU count and HBM bytes. */
+int32_t mpt_device_info(char* name, int32_t name_len, int32_t* cu_count, int64_t* hbm_bytes);
+
+/* Copies the description to HBM and builds BVHs, light CDFs and tables.
+ * Replaces: bvh(list) construction (bvh.h:22-43, main.cpp:70), mesh's inner
+ * BVH (mesh.h:177-179), light_list::build (light_list.h:45-96) and the
+ * ggx_albedo_table (microfacet.h:265-338). */
+int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out);
+
+/* Loads a .scene file (the reference format, scene.h:42-243, for the subset
+ * listed in the library's docs) and creates the scene; fills cam with the
+ * file's camera settings when cam is not NULL. Replaces load_scene
+ * (scene.h:244) followed by the BVH build (main.cpp:62-74). */
+int32_t mpt_scene_load(const char* path, mpt_scene** out, mpt_camera* cam);
+
+void mpt_scene_destroy(mpt_scene* scene);
+
+/* Parses a .scene file and flattens it into *desc without touching the GPU
+ * (load_scene, scene.h:244, then the world/lights flattening camera::render
+ * does). The arrays *desc points to stay valid until mpt_desc_free(*holder).
+ * cam, when not NULL, receives the file's camera settings. */
+int32_t mpt_scene_describe(const char* path, mpt_scene_desc* desc, mpt_camera* cam, void** holder);
+void mpt_desc_free(void* holder);
+
+/* Counts of the created scene (for logs and tests). */
+int32_t mpt_scene_counts(const mpt_scene* scene, int64_t* n_prims, int32_t* n_objects,
+                         int64_t* n_tlas_nodes, int64_t* n_blas_nodes, int32_t* n_lights);
+
+/* Closest-hit ray query (bvh::hit over the world, bvh.h:45-86), one result
+ * per ray: the canonical primitive ID (-1 on a miss), t and front_face.
+ * rays: n x 6 doubles (origin, direction, direction need not be unit);
+ * tmin_tmax: n x 2 doubles (the interval ray_t). Media are not tested. */
+int32_t mpt_trace_closest(mpt_scene* scene, const double* rays, const double* tmin_tmax, int64_t n,
+                          int64_t* prim_id, double* t, uint8_t* front_face);
+
+/* The same query on device-resident buffers (no host copies). */
+int32_t mpt_trace_closest_dev(mpt_scene* scene, const double* rays_dev, const double* tmin_tmax_dev,
+                              int64_t n, int64_t* prim_id_dev, double* t_dev, uint8_t* front_face_dev,
+                              void* hip_stream);
+
+/* camera::render (camera.h:79-105): renders cam.samples_per_pixel passes
+ * and writes the averaged frame (denoiser.h:14-20), row-major from the top
+ * left, height = max(1, int(width / aspect)). radiance/albedo/normal are
+ * width*height*3 doubles, variance width*height doubles; albedo, normal and
+ * variance may be NULL and are only filled with cam->collect_features. */
+int32_t mpt_render(mpt_scene* scene, const mpt_camera* cam, const mpt_render_opts* opts,
+                   double* radiance, double* albedo, double* normal, double* variance,
+                   mpt_pass_callback on_pass, void* user, mpt_render_stats* stats);
+
+/* Inside an on_pass callback: the frame averaged over the passes done. */
+int32_t mpt_read_frame(mpt_scene* scene, double* radiance, double* albedo, double* normal,
+                       double* variance);
+
+/* Image height camera::initialize computes (camera.h:143-144). */
+int32_t mpt_image_height(const mpt_camera* cam);
+
+/* The 32x32 E(mu, alpha) table and its 32 E_avg values the scene uses for
+ * Kulla-Conty compensation (microfacet.h:265-338). */
+int32_t mpt_ggx_albedo_table(double* e_1024, double* e_avg_32);
+
+/* Light selection probabilities light_list::build assigned (light_list.h:45-96). */
+int32_t mpt_scene_light_probabilities(const mpt_scene* scene, double* probabilities);
+
+/* BSDF evaluation on the GPU for fixed directions (test hook for the
+ * material::eval / pdf value pairs, material.h:22-72): for each of n queries
+ * with material index mats[i], unit shading normal n, incoming ray direction
+ * r_in and outgoing direction wi (9 doubles each), front_face flags, writes
+ * eval (3 doubles) and the scatter pdf's value (1 double). */
+int32_t mpt_bsdf_eval(mpt_scene* scene, const int32_t* mats, const double* n_rin_wi,
+                      const uint8_t* front_face, int64_t n, double* eval_rgb, double* pdf);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPT_H */

@@ -1,0 +1,368 @@
+// mpt.hpp - C++ host API of the MI355X path-tracing core, mirroring the
+// reference renderer's scene API so that code written against it switches
+// over by changing the include and namespace:
+//
+//   reference (adriendjn/mypbr)                this header
+//   hittable / hit_record  hittable.h:12-60    mpt::hittable (flattened, never hit on the host)
+//   hittable_list          hittable_list.h     mpt::hittable_list
+//   bvh                    bvh.h:20-43         mpt::bvh (the BVH itself is built on upload)
+//   sphere, quad, box()    sphere.h, quad.h    mpt::sphere, mpt::quad, mpt::box()
+//   mesh / triangle        mesh.h:175-233      mpt::mesh (positions + index triples)
+//   instance, transform    instance.h, transform.h
+//   constant_medium        medium.h:57-96      mpt::constant_medium
+//   material classes       material.h, medium.h (lambertian, metal, dielectric,
+//                                                 principled, diffuse_light, phase_material)
+//   texture classes        texture.h:24-56     mpt::solid_color, mpt::checker_texture
+//   environment_light      environment.h       mpt::environment_light (in-memory pixels)
+//   light_list             light_list.h:33-134 mpt::light_list
+//   camera::render         camera.h:79-105     mpt::camera::render -> mpt_render (include/mpt.h)
+//   frame                  denoiser.h:14-20    mpt::frame
+//   scene / load_scene     scene.h:32-244      mpt::scene / mpt::load_scene
+//
+// Objects are plain descriptions; camera::render flattens world and lights
+// into the C ABI's structure-of-arrays description, uploads it once per
+// (world, lights) pair and renders on the GPU. Types the GPU path does not
+// model make render() throw std::runtime_error instead of falling back.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../mpt.h"
+
+namespace mpt {
+
+class vec3 {
+public:
+    double v[3];
+    vec3() : v{0, 0, 0} {}
+    vec3(double x, double y, double z) : v{x, y, z} {}
+    double x() const { return v[0]; }
+    double y() const { return v[1]; }
+    double z() const { return v[2]; }
+    double operator[](int i) const { return v[i]; }
+    double& operator[](int i) { return v[i]; }
+    vec3 operator-() const { return vec3(-v[0], -v[1], -v[2]); }
+    double length_squared() const { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; }
+    double length() const { return std::sqrt(length_squared()); }
+    bool near_zero() const {
+        const double s = 1e-8;
+        return std::fabs(v[0]) < s && std::fabs(v[1]) < s && std::fabs(v[2]) < s;
+    }
+};
+using point3 = vec3;
+using color = vec3;
+inline vec3 operator+(const vec3& a, const vec3& b) { return vec3(a[0] + b[0], a[1] + b[1], a[2] + b[2]); }
+inline vec3 operator-(const vec3& a, const vec3& b) { return vec3(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
+inline vec3 operator*(const vec3& a, const vec3& b) { return vec3(a[0] * b[0], a[1] * b[1], a[2] * b[2]); }
+inline vec3 operator*(double t, const vec3& a) { return vec3(t * a[0], t * a[1], t * a[2]); }
+inline vec3 operator*(const vec3& a, double t) { return vec3(t * a[0], t * a[1], t * a[2]); }
+inline vec3 operator/(const vec3& a, double t) { return (1 / t) * a; }
+inline double dot(const vec3& a, const vec3& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline vec3 cross(const vec3& u, const vec3& v) {
+    return vec3(u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]);
+}
+inline vec3 unit_vector(const vec3& a) { return a / a.length(); }
+
+// An affine transform p -> M p + t with its inverse (transform.h).
+class transform {
+public:
+    transform();
+    static transform translate(const vec3& offset);
+    static transform scale(const vec3& s);
+    static transform rotate(double degrees, const vec3& axis);
+    static transform orthogonal(const double r[3][3]);
+    transform then_after(const transform& first) const;
+    point3 apply_point(const point3& p) const;
+    vec3 apply_vector(const vec3& v) const;
+    point3 apply_inverse_point(const point3& p) const;
+    vec3 apply_inverse_vector(const vec3& v) const;
+    bool is_similarity() const;
+    double area_scale() const;
+    mpt_xform to_abi() const;
+
+private:
+    double m[3][3];
+    vec3 t;
+    double inv_m[3][3];
+    vec3 inv_t;
+};
+
+// ------------------------------------------------------------------ textures
+class texture {
+public:
+    virtual ~texture() = default;
+    virtual mpt_texture to_abi() const = 0;
+};
+class solid_color : public texture {
+public:
+    explicit solid_color(const color& albedo) : albedo(albedo) {}
+    mpt_texture to_abi() const override;
+    color albedo;
+};
+class checker_texture : public texture {
+public:
+    checker_texture(double scale, std::shared_ptr<texture> even, std::shared_ptr<texture> odd);
+    mpt_texture to_abi() const override;
+    double inv_scale;
+    std::shared_ptr<texture> even, odd;
+};
+
+// ------------------------------------------------------------------ materials
+class material {
+public:
+    virtual ~material() = default;
+    // Fills the ABI record; texture is the index already assigned to the
+    // material's texture (or -1).
+    virtual mpt_material to_abi(int32_t texture_index) const = 0;
+    virtual std::shared_ptr<texture> base_texture() const { return nullptr; }
+    virtual color average_emission() const { return color(0, 0, 0); }
+};
+class lambertian : public material {
+public:
+    explicit lambertian(const color& albedo) : tex(std::make_shared<solid_color>(albedo)) {}
+    explicit lambertian(std::shared_ptr<texture> t) : tex(std::move(t)) {}
+    mpt_material to_abi(int32_t ti) const override;
+    std::shared_ptr<texture> base_texture() const override { return tex; }
+    std::shared_ptr<texture> tex;
+};
+class metal : public material {
+public:
+    metal(const color& albedo, double roughness);
+    mpt_material to_abi(int32_t ti) const override;
+    color albedo;
+    double roughness;
+};
+class dielectric : public material {
+public:
+    dielectric(double refraction_index, double roughness = 0, const color& absorption = color(0, 0, 0));
+    static color absorption_for(const color& tint, double distance);
+    mpt_material to_abi(int32_t ti) const override;
+    double refraction_index, roughness;
+    color sigma_a;
+};
+class principled : public material {
+public:
+    principled(std::shared_ptr<texture> base_color, double metallic, double roughness, double ior);
+    mpt_material to_abi(int32_t ti) const override;
+    std::shared_ptr<texture> base_texture() const override { return base_color; }
+    std::shared_ptr<texture> base_color;
+    double metallic, roughness, ior;
+};
+class diffuse_light : public material {
+public:
+    explicit diffuse_light(const color& emit, double spread_degrees = 180);
+    mpt_material to_abi(int32_t ti) const override;
+    color average_emission() const override;
+    color emit;
+    double spread;
+};
+class phase_material : public material {
+public:
+    explicit phase_material(const color& albedo, double g = 0) : albedo(albedo), g(g) {}
+    mpt_material to_abi(int32_t ti) const override;
+    color albedo;
+    double g;
+};
+
+// ------------------------------------------------------------------ geometry
+class flat_builder;
+
+class hittable {
+public:
+    virtual ~hittable() = default;
+    // Appends this object to the world list of the description; xform is the
+    // index of the instance transform it sits behind (-1 for none).
+    virtual void flatten(flat_builder& b, int32_t xform) const = 0;
+};
+
+class sphere : public hittable {
+public:
+    sphere(const point3& center, double radius, std::shared_ptr<material> mat)
+        : center(center), radius(radius), mat(std::move(mat)) {}
+    void flatten(flat_builder& b, int32_t xform) const override;
+    point3 center;
+    double radius;
+    std::shared_ptr<material> mat;
+};
+
+class quad : public hittable {
+public:
+    quad(const point3& Q, const vec3& u, const vec3& v, std::shared_ptr<material> mat)
+        : Q(Q), u(u), v(v), mat(std::move(mat)) {}
+    void flatten(flat_builder& b, int32_t xform) const override;
+    point3 Q;
+    vec3 u, v;
+    std::shared_ptr<material> mat;
+};
+
+class hittable_list : public hittable {
+public:
+    std::vector<std::shared_ptr<hittable>> objects;
+    hittable_list() = default;
+    explicit hittable_list(std::shared_ptr<hittable> object) { add(std::move(object)); }
+    void clear() { objects.clear(); }
+    void add(std::shared_ptr<hittable> object) { objects.push_back(std::move(object)); }
+    bool empty() const { return objects.empty(); }
+    void flatten(flat_builder& b, int32_t xform) const override;
+};
+
+// A BVH over a list. The GPU path builds its own binned-SAH BVH over the
+// world when the scene is uploaded, so this only marks the list.
+class bvh : public hittable {
+public:
+    explicit bvh(const hittable_list& list) : objects(list.objects) {}
+    explicit bvh(const std::vector<std::shared_ptr<hittable>>& objs) : objects(objs) {}
+    void flatten(flat_builder& b, int32_t xform) const override;
+    std::vector<std::shared_ptr<hittable>> objects;
+};
+
+// Returns the six sides of the box with opposite corners a and b (quad.h:95-113).
+std::shared_ptr<hittable_list> box(const point3& a, const point3& b, std::shared_ptr<material> mat);
+
+class mesh : public hittable {
+public:
+    // positions and counter-clockwise index triples; one material.
+    mesh(std::vector<point3> positions, std::vector<int32_t> indices, std::shared_ptr<material> mat)
+        : positions(std::move(positions)), indices(std::move(indices)), mat(std::move(mat)) {}
+    void flatten(flat_builder& b, int32_t xform) const override;
+    size_t size() const { return indices.size() / 3; }
+    std::vector<point3> positions;
+    std::vector<int32_t> indices;
+    std::shared_ptr<material> mat;
+};
+
+class instance : public hittable {
+public:
+    instance(std::shared_ptr<hittable> object, const transform& xform) : object(std::move(object)), xform(xform) {}
+    void flatten(flat_builder& b, int32_t xform) const override;
+    std::shared_ptr<hittable> object;
+    transform xform;
+};
+
+class constant_medium : public hittable {
+public:
+    constant_medium(std::shared_ptr<hittable> boundary, double density, std::shared_ptr<material> phase)
+        : boundary(std::move(boundary)), density(density), phase(std::move(phase)) {}
+    void flatten(flat_builder& b, int32_t xform) const override;
+    std::shared_ptr<hittable> boundary;
+    double density;
+    std::shared_ptr<material> phase;
+};
+
+// Equirectangular sky from pixels in memory, row 0 at +y (environment.h:44-50).
+class environment_light : public hittable {
+public:
+    environment_light(int width, int height, std::vector<color> pixels, double intensity = 1, double rotation_degrees = 0)
+        : width(width), height(height), pixels(std::move(pixels)), intensity(intensity), rotation_degrees(rotation_degrees) {}
+    void flatten(flat_builder& b, int32_t xform) const override;
+    int width, height;
+    std::vector<color> pixels;
+    double intensity, rotation_degrees;
+};
+
+// light_list (light_list.h:33-134): the selection probabilities are computed
+// by the core on upload from this strategy; build() only records the bounds
+// flag, as the core recomputes the scene radius itself.
+class light_list : public hittable_list {
+public:
+    enum class selection { uniform, power };
+    selection strategy = selection::power;
+    double uniform_fraction = 0.2;
+    void build() { built = true; }
+    bool built = false;
+};
+
+// ------------------------------------------------------------------ camera
+struct frame {  // denoiser.h:14-20
+    std::vector<color> radiance, albedo, normal;
+    std::vector<double> variance;
+};
+
+class camera {
+public:
+    double aspect_ratio = 1.0;
+    int image_width = 100;
+    int samples_per_pixel = 10;
+    int max_depth = 10;
+    int max_diffuse = 0, max_specular = 0, max_scatter = 0;
+    bool transparent_shadows = false;
+    double vfov = 90;
+    point3 lookfrom = point3(0, 0, 0);
+    point3 lookat = point3(0, 0, -1);
+    vec3 vup = vec3(0, 1, 0);
+    bool use_sky = true;
+    color background = color(0, 0, 0);
+    std::shared_ptr<environment_light> environment;
+    double ortho_height = 0;
+    double defocus_angle = 0;
+    double focus_dist = 10;
+    int blades = 0;
+    double blade_rotation = 0;
+    int threads = 0;  // kept for source compatibility; the GPU ignores it
+    bool collect_features = false;
+    uint64_t seed = 1;  // the counter-based RNG's seed
+
+    int height() const { return image_height; }
+    using pass_callback = std::function<void(int, const frame&)>;
+
+    // camera::render (camera.h:79-105) on the GPU.
+    frame render(const hittable& world, const hittable_list& lights, int save_every = 0,
+                 const pass_callback& on_pass = nullptr);
+
+    mpt_camera to_abi() const;
+
+private:
+    int image_height = 1;
+};
+
+// A flattened scene and the GPU scene made from it.
+class gpu_scene {
+public:
+    gpu_scene(const hittable& world, const hittable_list& lights, const std::shared_ptr<environment_light>& env);
+    ~gpu_scene();
+    gpu_scene(const gpu_scene&) = delete;
+    gpu_scene& operator=(const gpu_scene&) = delete;
+    mpt_scene* get() const { return handle; }
+
+private:
+    mpt_scene* handle = nullptr;
+};
+
+struct scene {  // scene.h:32-40 (display settings are parsed and ignored)
+    hittable_list world;
+    light_list lights;
+    camera cam;
+};
+
+// Loads a .scene file (scene.h:42-243) for the subset the GPU path models.
+// Returns false and writes "path:line: message" to err on failure.
+bool load_scene(const std::string& path, scene& result, std::string* err = nullptr);
+
+// Flattens world + lights (+ environment) into the ABI description; the
+// arrays live in the builder.
+class flat_builder {
+public:
+    flat_builder();
+    ~flat_builder();
+    void add_world(const hittable& world);
+    void add_lights(const hittable_list& lights, const std::shared_ptr<environment_light>& env);
+    const mpt_scene_desc& desc();
+
+    // used by hittable::flatten
+    int32_t material_index(const std::shared_ptr<material>& m);
+    int32_t xform_index(const transform& x);
+    int64_t add_prim(int32_t kind, int32_t mat, const double* data9);
+    int32_t add_object(int32_t kind, int64_t first, int64_t count, int32_t xform, const hittable* owner);
+    std::map<const void*, std::pair<int64_t, int64_t>> mesh_ranges;
+    struct impl;
+    impl* p;
+};
+
+}  // namespace mpt

@@ -1,0 +1,222 @@
+"""mypbr_amd - MI355X wavefront path-tracing core (Python host side).
+
+A thin ctypes layer over libmpt.so, the C ABI in include/mpt.h. It mirrors
+the reference renderer's render entry (camera::render, include/camera.h:79)
+for Python callers, tests and the benchmark:
+
+    scene = Scene.load("scenes/cornell.scene")      # load_scene + bvh (scene.h:244, main.cpp:70)
+    frame = scene.render(seed=1)                    # camera::render -> Frame(radiance, ...)
+    ids, t, front = scene.trace_closest(rays, tt)   # bvh::hit for a batch of rays
+
+Every compute call goes to the HIP kernels in libmpt.so; if the library is
+missing this module raises at import time instead of falling back.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import Camera, RenderOpts, RenderStats, SceneDesc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmpt.so")
+
+__all__ = ["Scene", "Frame", "MptError", "lib", "abi", "Camera", "RenderOpts", "device_info",
+           "ggx_albedo_table", "describe"]
+
+
+class MptError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mpt error {code}: {msg}")
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run python -c 'import __graft_entry__ as g; g.build()' "
+                          "(or python mypbr_amd/build.py)")
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    sig = {
+        "mpt_last_error": (C.c_char_p, []),
+        "mpt_abi_version": (C.c_int32, []),
+        "mpt_device_info": (C.c_int32, [C.c_char_p, C.c_int32, P(C.c_int32), P(C.c_int64)]),
+        "mpt_scene_create": (C.c_int32, [P(SceneDesc), P(vp)]),
+        "mpt_scene_load": (C.c_int32, [C.c_char_p, P(vp), P(Camera)]),
+        "mpt_scene_destroy": (None, [vp]),
+        "mpt_scene_describe": (C.c_int32, [C.c_char_p, P(SceneDesc), P(Camera), P(vp)]),
+        "mpt_desc_free": (None, [vp]),
+        "mpt_scene_counts": (C.c_int32, [vp, P(C.c_int64), P(C.c_int32), P(C.c_int64), P(C.c_int64), P(C.c_int32)]),
+        "mpt_trace_closest": (C.c_int32, [vp, P(C.c_double), P(C.c_double), C.c_int64, P(C.c_int64),
+                                          P(C.c_double), P(C.c_uint8)]),
+        "mpt_trace_closest_dev": (C.c_int32, [vp, vp, vp, C.c_int64, vp, vp, vp, vp]),
+        "mpt_render": (C.c_int32, [vp, P(Camera), P(RenderOpts), P(C.c_double), P(C.c_double), P(C.c_double),
+                                   P(C.c_double), abi.PASS_CALLBACK, vp, P(RenderStats)]),
+        "mpt_read_frame": (C.c_int32, [vp, P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double)]),
+        "mpt_image_height": (C.c_int32, [P(Camera)]),
+        "mpt_ggx_albedo_table": (C.c_int32, [P(C.c_double), P(C.c_double)]),
+        "mpt_scene_light_probabilities": (C.c_int32, [vp, P(C.c_double)]),
+        "mpt_bsdf_eval": (C.c_int32, [vp, P(C.c_int32), P(C.c_double), P(C.c_uint8), C.c_int64, P(C.c_double),
+                                      P(C.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _check(rc):
+    if rc != abi.MPT_OK:
+        raise MptError(rc, lib.mpt_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def device_info():
+    name = C.create_string_buffer(256)
+    cu = C.c_int32()
+    mem = C.c_int64()
+    _check(lib.mpt_device_info(name, 256, C.byref(cu), C.byref(mem)))
+    return {"name": name.value.decode(), "cu_count": cu.value, "hbm_bytes": mem.value}
+
+
+def ggx_albedo_table():
+    e = np.zeros(1024)
+    ea = np.zeros(32)
+    _check(lib.mpt_ggx_albedo_table(_dp(e), _dp(ea)))
+    return e.reshape(32, 32), ea
+
+
+class Description:
+    """A scene flattened on the host (no GPU): the mpt_scene_desc plus the
+    camera of the file. Keeps the arrays alive until closed."""
+
+    def __init__(self, path):
+        self.desc = SceneDesc()
+        self.camera = Camera()
+        self._holder = C.c_void_p()
+        _check(lib.mpt_scene_describe(os.fsencode(path), C.byref(self.desc), C.byref(self.camera),
+                                      C.byref(self._holder)))
+
+    def close(self):
+        if self._holder:
+            lib.mpt_desc_free(self._holder)
+            self._holder = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+def describe(path):
+    return Description(path)
+
+
+class Frame:
+    """The averaged frame camera::render returns (denoiser.h:14-20)."""
+
+    def __init__(self, radiance, albedo=None, normal=None, variance=None, stats=None):
+        self.radiance, self.albedo, self.normal, self.variance = radiance, albedo, normal, variance
+        self.stats = stats
+
+
+class Scene:
+    """A scene resident in HBM (mpt_scene)."""
+
+    def __init__(self, handle, camera=None, keepalive=None):
+        self._h = handle
+        self.camera = camera
+        self._keepalive = keepalive
+
+    @classmethod
+    def load(cls, path):
+        h = C.c_void_p()
+        cam = Camera()
+        _check(lib.mpt_scene_load(os.fsencode(path), C.byref(h), C.byref(cam)))
+        return cls(h, cam)
+
+    @classmethod
+    def from_desc(cls, desc, camera=None, keepalive=None):
+        h = C.c_void_p()
+        _check(lib.mpt_scene_create(C.byref(desc), C.byref(h)))
+        return cls(h, camera, keepalive)
+
+    def close(self):
+        if self._h:
+            lib.mpt_scene_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def counts(self):
+        np_, no, nt, nb, nl = C.c_int64(), C.c_int32(), C.c_int64(), C.c_int64(), C.c_int32()
+        _check(lib.mpt_scene_counts(self._h, C.byref(np_), C.byref(no), C.byref(nt), C.byref(nb), C.byref(nl)))
+        return {"prims": np_.value, "objects": no.value, "tlas_nodes": nt.value, "blas_nodes": nb.value,
+                "lights": nl.value}
+
+    def light_probabilities(self):
+        n = self.counts()["lights"]
+        out = np.zeros(max(n, 1))
+        _check(lib.mpt_scene_light_probabilities(self._h, _dp(out)))
+        return out[:n]
+
+    def trace_closest(self, rays, tt):
+        """bvh::hit for each ray: (canonical id or -1, t, front_face)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float64)
+        tt = np.ascontiguousarray(tt, dtype=np.float64)
+        n = rays.shape[0]
+        assert rays.shape == (n, 6) and tt.shape == (n, 2)
+        ids = np.zeros(n, dtype=np.int64)
+        t = np.zeros(n)
+        front = np.zeros(n, dtype=np.uint8)
+        _check(lib.mpt_trace_closest(self._h, _dp(rays), _dp(tt), n, ids.ctypes.data_as(C.POINTER(C.c_int64)),
+                                     _dp(t), front.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return ids, t, front
+
+    def bsdf_eval(self, mats, q, front):
+        mats = np.ascontiguousarray(mats, dtype=np.int32)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        front = np.ascontiguousarray(front, dtype=np.uint8)
+        n = mats.shape[0]
+        ev = np.zeros((n, 3))
+        pdf = np.zeros(n)
+        _check(lib.mpt_bsdf_eval(self._h, mats.ctypes.data_as(C.POINTER(C.c_int32)), _dp(q),
+                                 front.ctypes.data_as(C.POINTER(C.c_uint8)), n, _dp(ev), _dp(pdf)))
+        return ev, pdf
+
+    def render(self, camera=None, seed=1, features=None, passes_in_flight=0, save_every=0, on_pass=None,
+               first_pass=0, scene_index=0):
+        cam = abi.camera_copy(camera if camera is not None else self.camera)
+        if features is not None:
+            cam.collect_features = int(bool(features))
+        w, h = cam.image_width, cam.height()
+        rad = np.zeros((h, w, 3))
+        feats = bool(cam.collect_features)
+        alb = np.zeros((h, w, 3)) if feats else None
+        nrm = np.zeros((h, w, 3)) if feats else None
+        var = np.zeros((h, w)) if feats else None
+        opts = RenderOpts(seed=seed, first_pass=first_pass, save_every=save_every,
+                          passes_in_flight=passes_in_flight, scene_index=scene_index)
+        stats = RenderStats()
+        cb = abi.PASS_CALLBACK(lambda done, user: on_pass(done, self)) if on_pass else abi.PASS_CALLBACK()
+        _check(lib.mpt_render(self._h, C.byref(cam), C.byref(opts), _dp(rad),
+                              _dp(alb) if feats else None, _dp(nrm) if feats else None,
+                              _dp(var) if feats else None, cb, None, C.byref(stats)))
+        return Frame(rad, alb, nrm, var, stats)
+
+    def read_frame(self, camera):
+        w, h = camera.image_width, camera.height()
+        rad = np.zeros((h, w, 3))
+        _check(lib.mpt_read_frame(self._h, _dp(rad), None, None, None))
+        return rad

@@ -1,0 +1,121 @@
+"""ctypes mirror of the C ABI structs in include/mpt.h.
+
+Kept separate from the loader so that the oracle wrapper in tests/ can
+reuse the same struct layouts without loading the GPU library.
+"""
+import ctypes as C
+
+MPT_OK = 0
+MPT_E_INVALID = -1
+MPT_E_UNSUPPORTED = -2
+MPT_E_HIP = -3
+MPT_E_IO = -4
+MPT_E_CAPACITY = -5
+
+PRIM_SPHERE, PRIM_QUAD, PRIM_TRIANGLE = 0, 1, 2
+OBJ_PRIM, OBJ_LIST, OBJ_MESH, OBJ_MEDIUM = 0, 1, 2, 3
+MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_PRINCIPLED, MAT_DIFFUSE_LIGHT, MAT_PHASE = range(6)
+TEX_SOLID, TEX_CHECKER = 0, 1
+
+D3 = C.c_double * 3
+
+
+class Texture(C.Structure):
+    _fields_ = [("type", C.c_int32), ("pad", C.c_int32), ("inv_scale", C.c_double),
+                ("even", D3), ("odd", D3)]
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_int32), ("texture", C.c_int32), ("roughness", C.c_double),
+                ("alpha", C.c_double), ("ior", C.c_double), ("metallic", C.c_double),
+                ("f0", C.c_double), ("g", C.c_double), ("spread", C.c_double),
+                ("tan_half", C.c_double), ("albedo", D3), ("f_avg", D3), ("emit", D3),
+                ("sigma_a", D3), ("avg_emission", D3)]
+
+
+class Xform(C.Structure):
+    _fields_ = [("m", C.c_double * 9), ("t", D3), ("inv_m", C.c_double * 9), ("inv_t", D3)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("n_prims", C.c_int64),
+        ("prim_kind", C.POINTER(C.c_int32)),
+        ("prim_material", C.POINTER(C.c_int32)),
+        ("prim_data", C.POINTER(C.c_double)),
+        ("n_objects", C.c_int32),
+        ("object_kind", C.POINTER(C.c_int32)),
+        ("object_first", C.POINTER(C.c_int64)),
+        ("object_count", C.POINTER(C.c_int64)),
+        ("object_xform", C.POINTER(C.c_int32)),
+        ("object_id_base", C.POINTER(C.c_int64)),
+        ("object_density", C.POINTER(C.c_double)),
+        ("object_material", C.POINTER(C.c_int32)),
+        ("object_boundary_kind", C.POINTER(C.c_int32)),
+        ("n_xforms", C.c_int32),
+        ("xforms", C.POINTER(Xform)),
+        ("n_materials", C.c_int32),
+        ("materials", C.POINTER(Material)),
+        ("n_textures", C.c_int32),
+        ("textures", C.POINTER(Texture)),
+        ("n_lights", C.c_int32),
+        ("light_object", C.POINTER(C.c_int32)),
+        ("light_strategy", C.c_int32),
+        ("light_uniform_fraction", C.c_double),
+        ("env_width", C.c_int32),
+        ("env_height", C.c_int32),
+        ("env_rgb", C.POINTER(C.c_float)),
+        ("env_intensity", C.c_double),
+        ("env_rotation_degrees", C.c_double),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("aspect_ratio", C.c_double),
+        ("image_width", C.c_int32),
+        ("samples_per_pixel", C.c_int32),
+        ("max_depth", C.c_int32),
+        ("max_diffuse", C.c_int32),
+        ("max_specular", C.c_int32),
+        ("max_scatter", C.c_int32),
+        ("transparent_shadows", C.c_int32),
+        ("vfov", C.c_double),
+        ("lookfrom", D3), ("lookat", D3), ("vup", D3),
+        ("use_sky", C.c_int32),
+        ("collect_features", C.c_int32),
+        ("background", D3),
+        ("ortho_height", C.c_double),
+        ("defocus_angle", C.c_double),
+        ("focus_dist", C.c_double),
+        ("blades", C.c_int32),
+        ("pad", C.c_int32),
+        ("blade_rotation", C.c_double),
+    ]
+
+    def height(self):
+        return max(1, int(self.image_width / self.aspect_ratio))
+
+
+class RenderOpts(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("first_pass", C.c_int32), ("save_every", C.c_int32),
+                ("passes_in_flight", C.c_int32), ("block_size", C.c_int32),
+                ("sort_by_material", C.c_int32), ("use_graph", C.c_int32),
+                ("scene_index", C.c_int32), ("pad", C.c_int32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("paths", C.c_int64), ("extension_rays", C.c_int64), ("shadow_rays", C.c_int64),
+                ("seconds", C.c_double)]
+
+
+PASS_CALLBACK = C.CFUNCTYPE(None, C.c_int32, C.c_void_p)
+
+
+def camera_copy(cam, **overrides):
+    """A copy of a Camera with fields overridden."""
+    c = Camera()
+    C.pointer(c)[0] = cam
+    for k, v in overrides.items():
+        setattr(c, k, v)
+    return c

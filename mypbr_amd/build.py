@@ -1,0 +1,58 @@
+"""Builds libmpt.so, the MI355X path-tracing core, in-tree.
+
+hipcc compiles the HIP stages for gfx950 and the host C++ (scene parsing,
+flattening, BVH build, tables) into one shared library next to this file.
+Geometry is IEEE double with FP contraction off everywhere, so that device
+and host arithmetic match the reference renderer's operation order bit for
+bit; packed-FP32 instructions are disabled for the device.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libmpt.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("MPT_OFFLOAD_ARCH", "gfx950")
+
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
+          "-Wno-unused-parameter", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-munsafe-fp-atomics"]
+
+HOST_SOURCES = ["scene_host.cpp", "scene_api.cpp"]
+DEVICE_SOURCES = ["kernels.hip"]
+
+
+def run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def build(verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for src in HOST_SOURCES:
+        obj = os.path.join(BUILD, src + ".o")
+        run([HIPCC, "-x", "c++", *COMMON, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    for src in DEVICE_SOURCES:
+        obj = os.path.join(BUILD, src + ".o")
+        run([HIPCC, "-x", "hip", *COMMON, *DEVICE, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, *objs, "-o", tmp, "-lpthread"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    try:
+        build()
+    except subprocess.CalledProcessError as e:
+        sys.exit(e.returncode)

@@ -1,0 +1,598 @@
+// dev_shade.hpp - device-side materials, scatter densities, lights and the
+// environment, in the reference's operand order (material.h, pdf.h,
+// microfacet.h, light_list.h, environment.h, quad.h, sphere.h,
+// instance.h). The reference allocates a pdf object per scatter
+// (material.h:82, 134, 148); here a scatter density is a small tagged record
+// in registers (lobe kind, frame normal, wo, alpha, eta, mixture weight).
+#pragma once
+
+#include "dev_trace.hpp"
+#include "microfacet.hpp"
+
+namespace mpt {
+
+// A path's random stream: key and draw counter.
+struct rstream {
+    uint64_t key;
+    uint32_t ctr;
+    __device__ __forceinline__ double next() { return draw(key, ++ctr); }
+    __device__ __forceinline__ double next(double lo, double hi) { return lo + (hi - lo) * next(); }
+};
+
+__device__ __forceinline__ v3 tex_value(const dscene& S, int ti, const v3& p) {  // texture.h:24-56
+    const mpt_texture& t = S.tex[ti];
+    if (t.type == MPT_TEX_SOLID) return v3{t.even[0], t.even[1], t.even[2]};
+    int x = int(std::floor(t.inv_scale * p.x));
+    int y = int(std::floor(t.inv_scale * p.y));
+    int z = int(std::floor(t.inv_scale * p.z));
+    bool even = (x + y + z) % 2 == 0;
+    return even ? v3{t.even[0], t.even[1], t.even[2]} : v3{t.odd[0], t.odd[1], t.odd[2]};
+}
+
+__device__ __forceinline__ double avg3(const v3& c) { return (c.x + c.y + c.z) / 3; }
+__device__ __forceinline__ v3 m3(const double* a) { return v3{a[0], a[1], a[2]}; }
+
+// ---------------------------------------------------------------- densities
+enum { LOBE_COSINE = 0, LOBE_GGX = 1, LOBE_MIX = 2, LOBE_ROUGH_GLASS = 3, LOBE_HG = 4 };
+
+struct sdensity {
+    int kind;
+    v3 n;          // frame normal (cosine/GGX/glass), HG propagation direction
+    v3 wo_world;   // GGX / rough glass: direction towards the viewer (world)
+    double alpha;
+    double eta;    // rough glass
+    double w0;     // mixture weight of the GGX lobe; HG g
+};
+
+__device__ __forceinline__ double cosine_value(const v3& n, const v3& dir) {  // pdf.h:41-44
+    onb f = make_onb(n);
+    double c = dot(unit_vector(dir), f.w);
+    return std::fmax(0, c / kPi);
+}
+__device__ __forceinline__ v3 cosine_generate(const v3& n, rstream& rs) {  // pdf.h:46-48, vec3.h:126-136
+    onb f = make_onb(n);
+    double r1 = rs.next();
+    double r2 = rs.next();
+    double phi = 2 * kPi * r1;
+    double x = std::cos(phi) * std::sqrt(r2);
+    double y = std::sin(phi) * std::sqrt(r2);
+    double z = std::sqrt(1 - r2);
+    return onb_transform(f, v3{x, y, z});
+}
+__device__ __forceinline__ double ggx_value(const v3& n, const v3& wo_world, double alpha, const v3& dir) {
+    // ggx_reflection_pdf::value (microfacet.h:220-228)
+    if (length_squared(dir) == 0) return 0;
+    onb f = make_onb(n);
+    v3 wo = unit_vector(onb_to_local(f, wo_world));
+    v3 wi = unit_vector(onb_to_local(f, dir));
+    if (wi.z <= 0 || wo.z <= 0) return 0;
+    v3 h = unit_vector(wo + wi);
+    return ggx_visible_pdf(alpha, wo, h) / (4 * std::fabs(dot(wo, h)));
+}
+__device__ __forceinline__ v3 ggx_generate(const v3& n, const v3& wo_world, double alpha, rstream& rs) {
+    onb f = make_onb(n);
+    v3 wo = unit_vector(onb_to_local(f, wo_world));
+    if (wo.z <= 0) return v3{0, 0, 0};
+    double u1 = rs.next();
+    double u2 = rs.next();
+    v3 h = ggx_sample_visible(alpha, wo, u1, u2);
+    v3 wi = -wo + 2 * dot(wo, h) * h;
+    if (wi.z <= 0) return v3{0, 0, 0};
+    return onb_transform(f, wi);
+}
+
+// rough_dielectric_bsdf (microfacet.h:384-470)
+struct rglass {
+    onb f;
+    v3 wo;
+    double eta, alpha;
+};
+__device__ __forceinline__ rglass make_rglass(const v3& n, const v3& wo_world, double eta, double alpha) {
+    rglass g;
+    g.f = make_onb(n);
+    g.wo = unit_vector(onb_to_local(g.f, wo_world));
+    g.eta = eta;
+    g.alpha = alpha;
+    return g;
+}
+__device__ __forceinline__ bool rglass_half(const rglass& g, const v3& wi_world, v3& wi, v3& wm, bool& refl) {
+    if (length_squared(wi_world) == 0 || g.wo.z <= 0) return false;
+    wi = unit_vector(onb_to_local(g.f, wi_world));
+    if (wi.z == 0) return false;
+    refl = wi.z > 0;
+    wm = refl ? g.wo + wi : wi * g.eta + g.wo;
+    if (length_squared(wm) == 0) return false;
+    wm = unit_vector(wm);
+    if (wm.z < 0) wm = -wm;
+    if (dot(wm, wi) * wi.z < 0 || dot(wm, g.wo) * g.wo.z < 0) return false;
+    return true;
+}
+__device__ __forceinline__ double rglass_eval(const rglass& g, const v3& wi_world) {
+    v3 wi, wm;
+    bool refl;
+    if (!rglass_half(g, wi_world, wi, wm, refl)) return 0;
+    double F = fresnel_dielectric(dot(g.wo, wm), g.eta);
+    if (refl) return F * ggx_D(g.alpha, wm) * ggx_G(g.alpha, g.wo, wi) / (4 * g.wo.z);
+    double s = dot(wi, wm) + dot(g.wo, wm) / g.eta;
+    double denom = s * s;
+    double ft = (1 - F) * ggx_D(g.alpha, wm) * ggx_G(g.alpha, g.wo, wi) *
+                std::fabs(dot(wi, wm) * dot(g.wo, wm) / (wi.z * g.wo.z * denom));
+    ft /= g.eta * g.eta;
+    return ft * std::fabs(wi.z);
+}
+__device__ __forceinline__ double rglass_pdf(const rglass& g, const v3& wi_world) {
+    v3 wi, wm;
+    bool refl;
+    if (!rglass_half(g, wi_world, wi, wm, refl)) return 0;
+    double F = fresnel_dielectric(dot(g.wo, wm), g.eta);
+    if (refl) return F * ggx_visible_pdf(g.alpha, g.wo, wm) / (4 * std::fabs(dot(g.wo, wm)));
+    double s = dot(wi, wm) + dot(g.wo, wm) / g.eta;
+    double denom = s * s;
+    return (1 - F) * ggx_visible_pdf(g.alpha, g.wo, wm) * std::fabs(dot(wi, wm)) / denom;
+}
+__device__ __forceinline__ v3 rglass_sample(const rglass& g, rstream& rs) {
+    if (g.wo.z <= 0) return v3{0, 0, 0};
+    double u1 = rs.next();
+    double u2 = rs.next();
+    v3 wm = ggx_sample_visible(g.alpha, g.wo, u1, u2);
+    double F = fresnel_dielectric(dot(g.wo, wm), g.eta);
+    v3 wi;
+    if (rs.next() < F) {
+        wi = -g.wo + 2 * dot(g.wo, wm) * wm;
+        if (wi.z <= 0) return v3{0, 0, 0};
+    } else {
+        wi = refract(-g.wo, wm, 1 / g.eta);
+        if (wi.z >= 0) return v3{0, 0, 0};
+    }
+    return onb_transform(g.f, wi);
+}
+
+__device__ __forceinline__ double hg_phase(double c, double g) {  // pdf.h:85-88
+    double denom = 1 + g * g - 2 * g * c;
+    return (1 - g * g) / (4 * kPi * denom * std::sqrt(denom));
+}
+
+__device__ __forceinline__ double density_value(const sdensity& s, const v3& dir) {
+    switch (s.kind) {
+        case LOBE_COSINE: return cosine_value(s.n, dir);
+        case LOBE_GGX: return ggx_value(s.n, s.wo_world, s.alpha, dir);
+        case LOBE_MIX:
+            return s.w0 * ggx_value(s.n, s.wo_world, s.alpha, dir) + (1 - s.w0) * cosine_value(s.n, dir);
+        case LOBE_ROUGH_GLASS: return rglass_pdf(make_rglass(s.n, s.wo_world, s.eta, s.alpha), dir);
+        default: {
+            onb f = make_onb(s.n);
+            return hg_phase(dot(unit_vector(dir), f.w), s.w0);
+        }
+    }
+}
+
+__device__ __forceinline__ v3 density_generate(const sdensity& s, rstream& rs) {
+    switch (s.kind) {
+        case LOBE_COSINE: return cosine_generate(s.n, rs);
+        case LOBE_GGX: return ggx_generate(s.n, s.wo_world, s.alpha, rs);
+        case LOBE_MIX:
+            if (rs.next() < s.w0) return ggx_generate(s.n, s.wo_world, s.alpha, rs);
+            return cosine_generate(s.n, rs);
+        case LOBE_ROUGH_GLASS: return rglass_sample(make_rglass(s.n, s.wo_world, s.eta, s.alpha), rs);
+        default: {  // henyey_greenstein_pdf::generate (pdf.h:67-80)
+            onb f = make_onb(s.n);
+            double g = s.w0;
+            double xi = rs.next();
+            double c;
+            if (std::fabs(g) < 1e-3) {
+                c = 1 - 2 * xi;
+            } else {
+                double sq = (1 - g * g) / (1 - g + 2 * g * xi);
+                c = (1 + g * g - sq * sq) / (2 * g);
+            }
+            c = clampd(c, -1.0, 1.0);
+            double st = std::sqrt(1 - c * c);
+            double phi = 2 * kPi * rs.next();
+            return onb_transform(f, v3{std::cos(phi) * st, std::sin(phi) * st, c});
+        }
+    }
+}
+
+// ---------------------------------------------------------------- materials
+struct scatter_out {
+    bool scattered;
+    bool skip;
+    v3 attenuation;
+    v3 skip_dir;
+    sdensity dens;
+};
+
+__device__ __forceinline__ v3 principled_f0(const mpt_material& m, const v3& base) {
+    return (1 - m.metallic) * v3{m.f0, m.f0, m.f0} + m.metallic * base;
+}
+
+// material::scatter (material.h:79-85, 123-150, 210-242, 387-421; medium.h:21-27)
+__device__ scatter_out mat_scatter(const dscene& S, const mpt_material& m, const v3& rd, const hitrec& rec,
+                                   rstream& rs) {
+    scatter_out o;
+    o.scattered = true;
+    o.skip = false;
+    o.attenuation = v3{1, 1, 1};
+    o.skip_dir = v3{0, 0, 0};
+    o.dens.kind = LOBE_COSINE;
+    o.dens.n = rec.normal;
+    o.dens.wo_world = v3{0, 0, 0};
+    o.dens.alpha = 0;
+    o.dens.eta = 1;
+    o.dens.w0 = 1;
+    switch (m.type) {
+        case MPT_MAT_LAMBERTIAN:
+            o.attenuation = tex_value(S, m.texture, rec.p);
+            break;
+        case MPT_MAT_METAL: {
+            v3 albedo = m3(m.albedo);
+            if (m.roughness == 0) {
+                o.attenuation = albedo;
+                o.skip = true;
+                o.skip_dir = reflect(unit_vector(rd), rec.normal);
+                break;
+            }
+            o.attenuation = albedo;
+            o.dens.kind = LOBE_GGX;
+            o.dens.wo_world = -rd;
+            o.dens.alpha = m.alpha;
+            double cos_o = std::fmax(0.0, dot(rec.normal, unit_vector(-rd)));
+            double ms_w = ggx_ms_albedo(S.ggx_e, S.ggx_eavg, cos_o, m.alpha) *
+                          avg3(ggx_ms_fresnel_avg(S.ggx_eavg, m3(m.f_avg), m.alpha));
+            double ss_w = ggx_E(S.ggx_e, cos_o, m.alpha) * avg3(fresnel_schlick3(cos_o, albedo));
+            if (ms_w <= 0) break;
+            o.dens.kind = LOBE_MIX;
+            o.dens.w0 = clampd(ss_w / (ss_w + ms_w), 0.1, 1.0);
+            break;
+        }
+        case MPT_MAT_DIELECTRIC: {
+            if (m.roughness > 0) {
+                o.dens.kind = LOBE_ROUGH_GLASS;
+                o.dens.wo_world = -rd;
+                o.dens.eta = rec.front ? m.ior : 1.0 / m.ior;
+                o.dens.alpha = m.alpha;
+                break;
+            }
+            o.skip = true;
+            double ri = rec.front ? (1.0 / m.ior) : m.ior;
+            v3 ud = unit_vector(rd);
+            double cos_theta = std::fmin(dot(-ud, rec.normal), 1.0);
+            double sin_theta = std::sqrt(1.0 - cos_theta * cos_theta);
+            bool cannot_refract = ri * sin_theta > 1.0;
+            if (cannot_refract || fresnel_dielectric(cos_theta, 1.0 / ri) > rs.next())
+                o.skip_dir = reflect(ud, rec.normal);
+            else
+                o.skip_dir = refract(ud, rec.normal, ri);
+            break;
+        }
+        case MPT_MAT_PRINCIPLED: {
+            v3 wo = unit_vector(-rd);
+            v3 base = tex_value(S, m.texture, rec.p);
+            double metallic = m.metallic;
+            double alpha = m.alpha;
+            double cos_o = std::fmax(0.0, dot(rec.normal, wo));
+            v3 sf0 = principled_f0(m, base);
+            double spec_w = avg3(fresnel_schlick3(cos_o, sf0));
+            double diff_w = (1 - metallic) * avg3(base) * (1 - fresnel_schlick(cos_o, m.f0));
+            diff_w += ggx_ms_albedo(S.ggx_e, S.ggx_eavg, cos_o, alpha) * avg3(ggx_ms_fresnel(S.ggx_eavg, sf0, alpha));
+            double p_spec = spec_w + diff_w > 0 ? spec_w / (spec_w + diff_w) : 1;
+            o.dens.kind = LOBE_MIX;
+            o.dens.w0 = clampd(p_spec, 0.1, 1.0);
+            o.dens.wo_world = wo;
+            o.dens.alpha = alpha;
+            o.attenuation = base;
+            break;
+        }
+        case MPT_MAT_PHASE:
+            o.attenuation = m3(m.albedo);
+            o.dens.kind = LOBE_HG;
+            o.dens.n = rd;
+            o.dens.w0 = m.g;
+            break;
+        default:
+            o.scattered = false;
+            break;
+    }
+    return o;
+}
+
+// material::eval: f * |cos| for a non-specular direction.
+__device__ v3 mat_eval(const dscene& S, const mpt_material& m, const v3& rd, const hitrec& rec, const v3& dir) {
+    switch (m.type) {
+        case MPT_MAT_LAMBERTIAN: {  // material.h:88-97
+            double c = dot(rec.normal, unit_vector(dir));
+            double pdf = c < 0 ? 0 : c / kPi;
+            return tex_value(S, m.texture, rec.p) * pdf;
+        }
+        case MPT_MAT_METAL: {  // material.h:154-164
+            onb f = make_onb(rec.normal);
+            v3 wo = unit_vector(onb_to_local(f, -rd));
+            v3 wi = unit_vector(onb_to_local(f, dir));
+            if (wo.z <= 0 || wi.z <= 0) return v3{0, 0, 0};
+            v3 h = unit_vector(wo + wi);
+            v3 spec = fresnel_schlick3(dot(wi, h), m3(m.albedo)) *
+                      (ggx_D(m.alpha, h) * ggx_G(m.alpha, wo, wi) / (4 * wo.z));
+            return spec + ggx_ms_fresnel_avg(S.ggx_eavg, m3(m.f_avg), m.alpha) *
+                              ggx_ms_eval(S.ggx_e, S.ggx_eavg, wo.z, wi.z, m.alpha);
+        }
+        case MPT_MAT_DIELECTRIC: {  // material.h:244-249
+            if (m.roughness == 0) return v3{0, 0, 0};
+            double eta = rec.front ? m.ior : 1.0 / m.ior;
+            double f = rglass_eval(make_rglass(rec.normal, -rd, eta, m.alpha), dir);
+            return v3{f, f, f};
+        }
+        case MPT_MAT_PRINCIPLED: {  // material.h:423-431, 486-506
+            if (dot(dir, rec.normal) <= 0) return v3{0, 0, 0};
+            v3 wo_w = unit_vector(-rd);
+            v3 wi_w = unit_vector(dir);
+            onb f = make_onb(rec.normal);
+            v3 wo = unit_vector(onb_to_local(f, wo_w));
+            v3 wi = unit_vector(onb_to_local(f, wi_w));
+            if (wo.z <= 0 || wi.z <= 0) return v3{0, 0, 0};
+            double metallic = m.metallic, alpha = m.alpha;
+            v3 base = tex_value(S, m.texture, rec.p);
+            v3 h = unit_vector(wo + wi);
+            v3 sf0 = principled_f0(m, base);
+            v3 specular = fresnel_schlick3(dot(wi, h), sf0) * (ggx_D(alpha, h) * ggx_G(alpha, wo, wi) / (4 * wo.z)) +
+                          ggx_ms_fresnel(S.ggx_eavg, sf0, alpha) * ggx_ms_eval(S.ggx_e, S.ggx_eavg, wo.z, wi.z, alpha);
+            double transmitted = (1 - fresnel_schlick(wi.z, m.f0)) * (1 - fresnel_schlick(wo.z, m.f0));
+            v3 diffuse = ((1 - metallic) * transmitted * wi.z / kPi) * base;
+            return specular + diffuse;
+        }
+        case MPT_MAT_PHASE: {  // medium.h:33-36
+            double c = dot(unit_vector(rd), unit_vector(dir));
+            return m3(m.albedo) * hg_phase(c, m.g);
+        }
+        default:
+            return v3{0, 0, 0};
+    }
+}
+
+__device__ __forceinline__ v3 mat_emitted(const mpt_material& m, const v3& rd, const hitrec& rec) {
+    if (m.type != MPT_MAT_DIFFUSE_LIGHT || !rec.front) return v3{0, 0, 0};  // material.h:578-584
+    v3 e = m3(m.emit);
+    if (m.spread >= 180) return e;
+    double c = -dot(unit_vector(rd), rec.normal);
+    double gf;
+    if (c <= 0) gf = 0;
+    else {
+        double tan_theta = std::sqrt(std::fmax(0.0, 1 - c * c)) / c;
+        gf = std::fmax(0.0, 1 - tan_theta / m.tan_half);
+    }
+    return e * gf;
+}
+
+__device__ __forceinline__ v3 mat_interior(const mpt_material& m, const v3& rd, const hitrec& rec) {
+    if (m.type != MPT_MAT_DIELECTRIC) return v3{1, 1, 1};  // material.h:201-206
+    if (rec.front || (m.sigma_a[0] == 0 && m.sigma_a[1] == 0 && m.sigma_a[2] == 0)) return v3{1, 1, 1};
+    double dd = rec.t * length(rd);
+    return v3{std::exp(-m.sigma_a[0] * dd), std::exp(-m.sigma_a[1] * dd), std::exp(-m.sigma_a[2] * dd)};
+}
+
+__device__ __forceinline__ bool mat_lets_shadows(const mpt_material& m) {
+    return m.type == MPT_MAT_DIELECTRIC && m.roughness == 0;
+}
+
+// ---------------------------------------------------------------- environment
+__device__ __forceinline__ void env_angles(const dscene& S, const v3& dir, double& u, double& theta) {
+    v3 d = unit_vector(dir);
+    theta = std::acos(clampd(d.y, -1.0, 1.0));
+    u = 0.5 + std::atan2(d.x, -d.z) / (2 * kPi) + S.env_rotation;
+    u -= std::floor(u);
+}
+
+__device__ v3 env_radiance(const dscene& S, const v3& dir) {  // environment.h:66-79
+    double u, theta;
+    env_angles(S, dir, u, theta);
+    const int w = S.env_w, h = S.env_h;
+    double x = u * w - 0.5;
+    double y = clampd(theta / kPi * h - 0.5, 0.0, h - 1.0);
+    double x0 = std::floor(x), y0 = std::floor(y);
+    double fx = x - x0, fy = y - y0;
+    int i0 = (int(x0) + w) % w, i1 = (i0 + 1) % w;
+    int j0 = int(y0), j1 = j0 + 1 < h - 1 ? j0 + 1 : h - 1;
+    auto px = [&](int i, int j) { return m3(&S.env_rgb[(size_t(j) * w + i) * 3]); };
+    v3 top = (1 - fx) * px(i0, j0) + fx * px(i1, j0);
+    v3 bottom = (1 - fx) * px(i0, j1) + fx * px(i1, j1);
+    return S.env_intensity * ((1 - fy) * top + fy * bottom);
+}
+
+__device__ double env_pdf(const dscene& S, const v3& dir) {  // environment.h:84-92
+    double u, theta;
+    env_angles(S, dir, u, theta);
+    double sin_theta = std::sin(theta);
+    const int w = S.env_w, h = S.env_h;
+    int i = int(u * w);
+    i = i < 0 ? 0 : (i > w - 1 ? w - 1 : i);
+    int j = int(theta / kPi * h);
+    j = j < 0 ? 0 : (j > h - 1 ? h - 1 : j);
+    if (sin_theta <= 0) return 0;
+    double row = S.env_roww[j] / S.env_total;
+    const double* cdf = &S.env_cond[size_t(j) * (w + 1)];
+    double column = cdf[i + 1] - cdf[i];
+    double pix = row * column * w * h;
+    return pix / (2 * kPi * kPi * sin_theta);
+}
+
+__device__ __forceinline__ double sample_cdf(const double* cdf, int n, double xi) {  // environment.h:180-186
+    // upper_bound over cdf[0..n]
+    int lo = 0, hi = n + 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (xi < cdf[mid]) hi = mid;
+        else lo = mid + 1;
+    }
+    int k = lo - 1;
+    k = k < 0 ? 0 : (k > n - 1 ? n - 1 : k);
+    double span = cdf[k + 1] - cdf[k];
+    double fraction = span > 0 ? (xi - cdf[k]) / span : 0.5;
+    return k + clampd(fraction, 0.0, 1.0 - 1e-9);
+}
+
+__device__ v3 env_random(const dscene& S, rstream& rs) {  // environment.h:94-107
+    const int w = S.env_w, h = S.env_h;
+    double y = sample_cdf(S.env_marg, h, rs.next());
+    int j = int(y) < h - 1 ? int(y) : h - 1;
+    double x = sample_cdf(&S.env_cond[size_t(j) * (w + 1)], w, rs.next());
+    double theta = kPi * y / h;
+    double phi = 2 * kPi * (x / w - 0.5 - S.env_rotation);
+    double st = std::sin(theta);
+    return v3{st * std::sin(phi), std::cos(theta), -st * std::cos(phi)};
+}
+
+// ---------------------------------------------------------------- lights
+__device__ __forceinline__ double prim_light_pdf(const dscene& S, int64_t p, const v3& o, const v3& d) {
+    const double* q = prim_ptr(S, p);
+    int k = S.pkind[p];
+    double t;
+    if (k == MPT_PRIM_QUAD) {  // quad.h:65-74
+        if (!quad_t(q, o, d, 0.001, kInf, t)) return 0;
+        double dist2 = t * t * length_squared(d);
+        v3 nrm{q[9], q[10], q[11]};
+        v3 n = dot(d, nrm) < 0 ? nrm : -nrm;
+        double cosine = std::fabs(dot(d, n) / length(d));
+        double area = length(cross(v3{q[3], q[4], q[5]}, v3{q[6], q[7], q[8]}));
+        return dist2 / (cosine * area);
+    }
+    if (k == MPT_PRIM_SPHERE) {  // sphere.h:56-73
+        if (!sphere_t(q, o, d, 0.001, kInf, t)) return 0;
+        v3 c{q[0], q[1], q[2]};
+        double r = std::fabs(q[3]);
+        double dist2 = length_squared(c - o);
+        if (dist2 <= r * r) {
+            double hd2 = t * t * length_squared(d);
+            v3 p = o + t * d;
+            double cosine = std::fabs(dot(unit_vector(d), (p - c) / r));
+            return hd2 / (cosine * 4 * kPi * r * r);
+        }
+        double cos_theta_max = std::sqrt(1 - r * r / dist2);
+        double solid_angle = 2 * kPi * (1 - cos_theta_max);
+        return 1 / solid_angle;
+    }
+    // triangle (mesh.h:115-124)
+    if (!tri_t(q, o, d, 0.001, kInf, t)) return 0;
+    double dist2 = t * t * length_squared(d);
+    v3 g{q[9], q[10], q[11]};
+    double cosine = std::fabs(dot(d, g) / length(d));
+    return dist2 / (cosine * q[12]);
+}
+
+__device__ __forceinline__ v3 random_unit_vector(rstream& rs) {  // vec3.h:116-123
+    while (true) {
+        double x = rs.next(-1, 1);
+        double y = rs.next(-1, 1);
+        double z = rs.next(-1, 1);
+        v3 p{x, y, z};
+        double lensq = length_squared(p);
+        if (1e-160 < lensq && lensq <= 1) return p / std::sqrt(lensq);
+    }
+}
+
+__device__ __forceinline__ v3 prim_light_random(const dscene& S, int64_t p, const v3& o, rstream& rs) {
+    const double* q = prim_ptr(S, p);
+    int k = S.pkind[p];
+    if (k == MPT_PRIM_QUAD) {  // quad.h:76-79
+        double a = rs.next();
+        double b = rs.next();
+        v3 pt = v3{q[0], q[1], q[2]} + (a * v3{q[3], q[4], q[5]}) + (b * v3{q[6], q[7], q[8]});
+        return pt - o;
+    }
+    if (k == MPT_PRIM_SPHERE) {  // sphere.h:75-83, 122-132
+        v3 c{q[0], q[1], q[2]};
+        double r = std::fabs(q[3]);
+        v3 dir = c - o;
+        double dist2 = length_squared(dir);
+        if (dist2 <= r * r) return c + r * random_unit_vector(rs) - o;
+        onb f = make_onb(dir);
+        double r1 = rs.next();
+        double r2 = rs.next();
+        double z = 1 + r2 * (std::sqrt(1 - r * r / dist2) - 1);
+        double phi = 2 * kPi * r1;
+        double x = std::cos(phi) * std::sqrt(1 - z * z);
+        double y = std::sin(phi) * std::sqrt(1 - z * z);
+        return onb_transform(f, v3{x, y, z});
+    }
+    // triangle (mesh.h:127-139)
+    double r1 = rs.next(), r2 = rs.next();
+    if (r1 + r2 > 1) {
+        r1 = 1 - r1;
+        r2 = 1 - r2;
+    }
+    v3 p0{q[0], q[1], q[2]};
+    v3 pt = p0 + r1 * v3{q[3], q[4], q[5]} + r2 * v3{q[6], q[7], q[8]};
+    return pt - o;
+}
+
+// hittable::pdf_value of light object ob (instance.h:47-50, hittable_list.h:46-56).
+__device__ double object_light_pdf(const dscene& S, int ob, v3 o, v3 d) {
+    const obj_rec r = S.obj[ob];
+    if (r.xform >= 0) {
+        const xform& x = S.xf[r.xform];
+        v3 lo = xf_inv_point(x, o);
+        v3 ld = xf_inv_vector(x, d);
+        o = lo;
+        d = ld;
+    }
+    if (r.kind == MPT_OBJ_PRIM) return prim_light_pdf(S, r.first, o, d);
+    double weight = 1.0 / double(r.count);
+    double sum = 0.0;
+    for (int64_t k = 0; k < r.count; k++) sum += weight * prim_light_pdf(S, r.first + k, o, d);
+    return sum;
+}
+
+__device__ v3 object_light_random(const dscene& S, int ob, const v3& o, rstream& rs) {
+    const obj_rec r = S.obj[ob];
+    v3 lo = o;
+    const xform* x = r.xform >= 0 ? &S.xf[r.xform] : nullptr;
+    if (x) lo = xf_inv_point(*x, o);
+    v3 res;
+    if (r.kind == MPT_OBJ_PRIM) {
+        res = prim_light_random(S, r.first, lo, rs);
+    } else {
+        int n = int(r.count);
+        int idx = int(rs.next() * n);
+        idx = idx < n - 1 ? idx : n - 1;
+        res = prim_light_random(S, r.first + idx, lo, rs);
+    }
+    if (x) res = xf_vector(*x, res);
+    return res;
+}
+
+// light_list::pdf_value (light_list.h:106-113): sum of p_k pdf_k.
+__device__ double lights_pdf(const dscene& S, const v3& o, const v3& d) {
+    if (S.nlights == 0) return 0.0;
+    double sum = 0;
+    for (int k = 0; k < S.nlights; k++) {
+        const light_rec L = S.lights[k];
+        double pk = L.object < 0 ? env_pdf(S, d) : object_light_pdf(S, L.object, o, d);
+        sum += L.probability * pk;
+    }
+    return sum;
+}
+
+// light_list::random (light_list.h:115-127), or hittable_list::random for an
+// unbuilt list (strategy 2).
+__device__ v3 lights_random(const dscene& S, const v3& o, rstream& rs) {
+    double xi = rs.next();
+    int n = S.nlights;
+    int k;
+    if (S.lstrategy == 2) {
+        k = int(xi * n);
+        k = k < n - 1 ? k : n - 1;
+    } else {
+        int lo = 0, hi = n + 1;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (xi < S.lcdf[mid]) hi = mid;
+            else lo = mid + 1;
+        }
+        k = lo;
+        k = k < 1 ? 1 : (k > n ? n : k);
+        k -= 1;
+    }
+    const light_rec L = S.lights[k];
+    if (L.object < 0) return env_random(S, rs);
+    return object_light_random(S, L.object, o, rs);
+}
+
+}  // namespace mpt

@@ -1,0 +1,382 @@
+// dev_trace.hpp - device-side scene view and closest-hit traversal.
+//
+// Two-level BVH in the reference's layout: a top-level BVH over the world
+// list (main.cpp:70) whose leaves are objects (a primitive, a linear list, a
+// mesh with its own BVH, or a medium), each optionally behind an affine
+// transform (instance.h:29-41). Traversal follows bvh::hit (bvh.h:45-86)
+// step for step in double precision: the same slab test (aabb.h:40-54),
+// near-first order by the split axis sign, leaf tests that shrink the
+// interval on every hit. Identical node order and identical arithmetic give
+// identical closest hits, ties included.
+//
+// The traversal stack lives in LDS (a short stack of kLdsDepth entries per
+// lane, strided by the block size so lanes hit distinct banks) and spills to
+// a per-thread area in HBM beyond that.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "geom.hpp"
+#include "rng.hpp"
+#include "scene_host.hpp"
+
+namespace mpt {
+
+struct dscene {
+    const double* prim;
+    const int32_t* pkind;
+    const int32_t* pmat;
+    const int64_t* pcanon;
+    const obj_rec* obj;
+    const bvh_node* tlas;
+    const int32_t* titems;
+    const bvh_node* blas;
+    const xform* xf;
+    const mpt_material* mat;
+    const mpt_texture* tex;
+    const light_rec* lights;
+    const double* lcdf;
+    const double* env_rgb;
+    const double* env_cond;
+    const double* env_marg;
+    const double* env_roww;
+    const double* ggx_e;
+    const double* ggx_eavg;
+    double env_total;
+    double env_intensity;
+    double env_rotation;
+    int32_t nobj;
+    int32_t nlights;
+    int32_t lstrategy;
+    int32_t env_w, env_h;
+    int32_t pad;
+};
+
+// Per-lane traversal stack: LDS for the first kLdsDepth entries, then HBM.
+struct tstack {
+    int32_t* lds;        // this block's LDS stack, [depth][block]
+    int32_t* ovf;        // spill area, [depth][total threads]
+    int32_t tid;         // lane index within the block
+    int32_t bs;          // block size
+    int64_t gtid;        // global thread index
+    int64_t nthreads;    // total threads of the persistent grid
+    int32_t lds_depth;
+    __device__ __forceinline__ void put(int sp, int32_t v) {
+        if (sp < lds_depth) lds[sp * bs + tid] = v;
+        else ovf[int64_t(sp - lds_depth) * nthreads + gtid] = v;
+    }
+    __device__ __forceinline__ int32_t get(int sp) const {
+        if (sp < lds_depth) return lds[sp * bs + tid];
+        return ovf[int64_t(sp - lds_depth) * nthreads + gtid];
+    }
+};
+
+struct hitinfo {
+    double t;
+    int32_t obj;    // object index, -1 = miss
+    int32_t prim;   // device primitive index, -1 for a medium
+};
+
+// Slab test of aabb::hit (aabb.h:40-54) on node n.
+__device__ __forceinline__ bool node_hit(const bvh_node& n, const v3& o, const v3& inv, double tmin, double tmax) {
+    const double oo[3] = {o.x, o.y, o.z};
+    const double ii[3] = {inv.x, inv.y, inv.z};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        double t0 = (n.b[2 * a] - oo[a]) * ii[a];
+        double t1 = (n.b[2 * a + 1] - oo[a]) * ii[a];
+        if (t0 > t1) {
+            double s = t0;
+            t0 = t1;
+            t1 = s;
+        }
+        if (t0 > tmin) tmin = t0;
+        if (t1 < tmax) tmax = t1;
+        if (tmax <= tmin) return false;
+    }
+    return true;
+}
+
+__device__ __forceinline__ bvh_node load_node(const bvh_node* nodes, int i) {
+    const double2* p = reinterpret_cast<const double2*>(nodes + i);
+    double2 a = p[0], b = p[1], c = p[2];
+    const int4 q = reinterpret_cast<const int4*>(nodes + i)[3];
+    bvh_node n;
+    n.b[0] = a.x, n.b[1] = a.y, n.b[2] = b.x, n.b[3] = b.y, n.b[4] = c.x, n.b[5] = c.y;
+    n.offset = q.x, n.count = q.y, n.axis = q.z, n.pad = 0;
+    return n;
+}
+
+__device__ __forceinline__ const double* prim_ptr(const dscene& S, int64_t p) { return S.prim + p * kPrimStride; }
+
+// sphere::hit (sphere.h:23-41), t only.
+__device__ __forceinline__ bool sphere_t(const double* q, const v3& o, const v3& d, double tmin, double tmax,
+                                         double& t) {
+    v3 c{q[0], q[1], q[2]};
+    double radius = q[3];
+    v3 oc = c - o;
+    double a = length_squared(d);
+    double h = dot(d, oc);
+    double cc = length_squared(oc) - radius * radius;
+    double disc = h * h - a * cc;
+    if (disc < 0) return false;
+    double sq = std::sqrt(disc);
+    double root = (h - sq) / a;
+    if (!(tmin < root && root < tmax)) {
+        root = (h + sq) / a;
+        if (!(tmin < root && root < tmax)) return false;
+    }
+    t = root;
+    return true;
+}
+
+// quad::hit (quad.h:30-52), t only; the interval is inclusive.
+__device__ __forceinline__ bool quad_t(const double* q, const v3& o, const v3& d, double tmin, double tmax,
+                                       double& t) {
+    v3 nrm{q[9], q[10], q[11]};
+    double denom = dot(nrm, d);
+    if (std::fabs(denom) < 1e-8) return false;
+    double tt = (q[15] - dot(nrm, o)) / denom;
+    if (!(tmin <= tt && tt <= tmax)) return false;
+    v3 ip = o + tt * d;
+    v3 Q{q[0], q[1], q[2]}, u{q[3], q[4], q[5]}, v{q[6], q[7], q[8]}, w{q[12], q[13], q[14]};
+    v3 ph = ip - Q;
+    double alpha = dot(w, cross(ph, v));
+    double beta = dot(w, cross(u, ph));
+    if (!(0 <= alpha && alpha <= 1) || !(0 <= beta && beta <= 1)) return false;
+    t = tt;
+    return true;
+}
+
+// triangle::hit (mesh.h:47-69), Moller-Trumbore, t only.
+__device__ __forceinline__ bool tri_t(const double* q, const v3& o, const v3& d, double tmin, double tmax,
+                                      double& t) {
+    v3 p0{q[0], q[1], q[2]}, e1{q[3], q[4], q[5]}, e2{q[6], q[7], q[8]};
+    v3 pvec = cross(d, e2);
+    double det = dot(e1, pvec);
+    if (std::fabs(det) < 1e-12) return false;
+    double inv_det = 1 / det;
+    v3 tvec = o - p0;
+    double b1 = dot(tvec, pvec) * inv_det;
+    if (b1 < 0 || b1 > 1) return false;
+    v3 qvec = cross(tvec, e1);
+    double b2 = dot(d, qvec) * inv_det;
+    if (b2 < 0 || b1 + b2 > 1) return false;
+    double tt = dot(e2, qvec) * inv_det;
+    if (!(tmin < tt && tt < tmax)) return false;
+    t = tt;
+    return true;
+}
+
+__device__ __forceinline__ bool prim_t(const dscene& S, int64_t p, const v3& o, const v3& d, double tmin,
+                                       double tmax, double& t) {
+    const double* q = prim_ptr(S, p);
+    int k = S.pkind[p];
+    if (k == MPT_PRIM_SPHERE) return sphere_t(q, o, d, tmin, tmax, t);
+    if (k == MPT_PRIM_QUAD) return quad_t(q, o, d, tmin, tmax, t);
+    return tri_t(q, o, d, tmin, tmax, t);
+}
+
+// hittable_list::hit (hittable_list.h:25-39) over [first, first + count).
+__device__ __forceinline__ bool list_t(const dscene& S, int64_t first, int64_t count, const v3& o, const v3& d,
+                                       double tmin, double tmax, double& t, int32_t& which) {
+    bool any = false;
+    double closest = tmax;
+    for (int64_t k = 0; k < count; k++) {
+        double tt;
+        if (prim_t(S, first + k, o, d, tmin, closest, tt)) {
+            any = true;
+            closest = tt;
+            which = int32_t(first + k);
+        }
+    }
+    t = closest;
+    return any;
+}
+
+// Inner bvh::hit of a mesh; the stack continues above sp_base.
+__device__ bool mesh_t(const dscene& S, int root, const v3& o, const v3& d, double tmin, double tmax, double& t,
+                       int32_t& which, tstack& st, int sp_base) {
+    v3 inv{1 / d.x, 1 / d.y, 1 / d.z};
+    bool neg[3] = {inv.x < 0, inv.y < 0, inv.z < 0};
+    bool any = false;
+    int sp = sp_base;
+    int cur = root;
+    while (true) {
+        bvh_node n = load_node(S.blas, cur);
+        if (node_hit(n, o, inv, tmin, tmax)) {
+            if (n.count > 0) {
+                for (int k = 0; k < n.count; k++) {
+                    double tt;
+                    if (tri_t(prim_ptr(S, n.offset + k), o, d, tmin, tmax, tt)) {
+                        any = true;
+                        tmax = tt;
+                        which = n.offset + k;
+                    }
+                }
+                if (sp == sp_base) break;
+                cur = st.get(--sp);
+            } else if (neg[n.axis]) {
+                st.put(sp++, cur + 1);
+                cur = n.offset;
+            } else {
+                st.put(sp++, n.offset);
+                cur = cur + 1;
+            }
+        } else {
+            if (sp == sp_base) break;
+            cur = st.get(--sp);
+        }
+    }
+    t = tmax;
+    return any;
+}
+
+// Hit of an object's boundary (a primitive or a list) for media.
+__device__ __forceinline__ bool boundary_t(const dscene& S, const obj_rec& r, const v3& o, const v3& d, double tmin,
+                                           double tmax, double& t) {
+    int32_t w;
+    if (r.bkind == MPT_OBJ_PRIM) return prim_t(S, r.first, o, d, tmin, tmax, t);
+    return list_t(S, r.first, r.count, o, d, tmin, tmax, t, w);
+}
+
+// constant_medium::hit (medium.h:60-88), with the free-flight draw u.
+// d is the boundary's (possibly instance-local) direction; world_len is the
+// length of the world-space direction, which sets distances in the medium.
+__device__ bool medium_t(const dscene& S, const obj_rec& r, const v3& o, const v3& d, double world_len, double tmin,
+                         double tmax, double u, double& t) {
+    double t1a, t2a;
+    if (!boundary_t(S, r, o, d, -kInf, kInf, t1a)) return false;
+    if (!boundary_t(S, r, o, d, t1a + 0.0001, kInf, t2a)) return false;
+    double t0 = std::fmax(t1a, tmin);
+    double t1 = std::fmin(t2a, tmax);
+    if (t0 >= t1) return false;
+    double ray_length = world_len;
+    double distance_inside = (t1 - t0) * ray_length;
+    double hit_distance = -std::log(1 - u) / r.density;
+    if (hit_distance > distance_inside) return false;
+    t = t0 + hit_distance / ray_length;
+    return true;
+}
+
+// One world object, the ray in world space. Returns the hit t and primitive.
+__device__ __forceinline__ bool object_t(const dscene& S, int oi, v3 o, v3 d, double tmin, double tmax, double& t,
+                                         int32_t& prim, tstack& st, int sp, bool media, uint64_t mkey,
+                                         uint32_t serial) {
+    const obj_rec r = S.obj[oi];
+    if (r.kind == MPT_OBJ_MEDIUM && !media) return false;
+    const double world_len = r.kind == MPT_OBJ_MEDIUM ? length(d) : 0.0;
+    if (r.xform >= 0) {
+        const xform& x = S.xf[r.xform];
+        v3 lo = xf_inv_point(x, o);
+        v3 ld = xf_inv_vector(x, d);
+        o = lo;
+        d = ld;
+    }
+    switch (r.kind) {
+        case MPT_OBJ_PRIM:
+            prim = int32_t(r.first);
+            return prim_t(S, r.first, o, d, tmin, tmax, t);
+        case MPT_OBJ_LIST:
+            return list_t(S, r.first, r.count, o, d, tmin, tmax, t, prim);
+        case MPT_OBJ_MESH:
+            return mesh_t(S, r.blas_root, o, d, tmin, tmax, t, prim, st, sp);
+        default:
+            prim = -1;
+            return medium_t(S, r, o, d, world_len, tmin, tmax, medium_draw(mkey, serial, uint32_t(oi)), t);
+    }
+}
+
+// bvh::hit over the world (bvh.h:45-86). media: test constant_medium objects
+// with draws from (mkey, serial); otherwise they are skipped.
+__device__ bool trace_closest(const dscene& S, const v3& o, const v3& d, double tmin, double tmax, hitinfo& h,
+                              tstack& st, bool media, uint64_t mkey, uint32_t serial) {
+    h.obj = -1;
+    h.prim = -1;
+    h.t = tmax;
+    if (S.nobj == 0) return false;
+    v3 inv{1 / d.x, 1 / d.y, 1 / d.z};
+    bool neg[3] = {inv.x < 0, inv.y < 0, inv.z < 0};
+    bool any = false;
+    int sp = 0;
+    int cur = 0;
+    while (true) {
+        bvh_node n = load_node(S.tlas, cur);
+        if (node_hit(n, o, inv, tmin, tmax)) {
+            if (n.count > 0) {
+                for (int k = 0; k < n.count; k++) {
+                    int oi = S.titems[n.offset + k];
+                    double tt;
+                    int32_t pr = -1;
+                    if (object_t(S, oi, o, d, tmin, tmax, tt, pr, st, sp, media, mkey, serial)) {
+                        any = true;
+                        tmax = tt;
+                        h.t = tt;
+                        h.obj = oi;
+                        h.prim = pr;
+                    }
+                }
+                if (sp == 0) break;
+                cur = st.get(--sp);
+            } else if (neg[n.axis]) {
+                st.put(sp++, cur + 1);
+                cur = n.offset;
+            } else {
+                st.put(sp++, n.offset);
+                cur = cur + 1;
+            }
+        } else {
+            if (sp == 0) break;
+            cur = st.get(--sp);
+        }
+    }
+    return any;
+}
+
+// Full hit record of a closest hit: point, face normal, front_face,
+// material, as the primitive's hit() and instance::hit write them.
+struct hitrec {
+    v3 p;
+    v3 normal;
+    double t;
+    int32_t mat;
+    bool front;
+};
+
+__device__ __forceinline__ void finish_hit(const dscene& S, const hitinfo& h, const v3& wo, const v3& wd, hitrec& rec) {
+    const obj_rec r = S.obj[h.obj];
+    rec.t = h.t;
+    if (r.kind == MPT_OBJ_MEDIUM) {
+        rec.p = wo + h.t * wd;
+        rec.normal = v3{1, 0, 0};
+        rec.front = true;
+        rec.mat = r.material;
+        return;
+    }
+    v3 o = wo, d = wd;
+    const xform* x = r.xform >= 0 ? &S.xf[r.xform] : nullptr;
+    if (x) {
+        o = xf_inv_point(*x, wo);
+        d = xf_inv_vector(*x, wd);
+    }
+    const double* q = prim_ptr(S, h.prim);
+    int k = S.pkind[h.prim];
+    v3 p = o + h.t * d;
+    v3 outward;
+    if (k == MPT_PRIM_SPHERE) {
+        outward = (p - v3{q[0], q[1], q[2]}) / q[3];
+    } else {
+        outward = v3{q[9], q[10], q[11]};
+    }
+    rec.front = dot(d, outward) < 0;
+    v3 n = rec.front ? outward : -outward;
+    if (x) {
+        p = xf_point(*x, p);
+        n = unit_vector(xf_normal(*x, n));
+    }
+    rec.p = p;
+    rec.normal = n;
+    rec.mat = S.pmat[h.prim];
+}
+
+}  // namespace mpt

@@ -1,0 +1,1010 @@
+// kernels.hip - wavefront path-tracing stages for gfx950 and the C ABI that
+// drives them (include/mpt.h).
+//
+// One render is a sequence of batches; a batch traces `passes_in_flight`
+// samples of every pixel at once. Per bounce the stages are
+//   extend  closest-hit of every live path's ray (LDS short-stack BVH walk)
+//   shade   miss/emission/MIS, BSDF scatter, NEE light sample, Russian
+//           roulette; survivors and shadow rays are compacted into queues
+//           with one atomic per wave (ballot + popcount prefix)
+//   shadow  closest-hit of the shadow rays, emission found, MIS-weighted
+//           light contribution added to the path
+// and every batch ends with accumulate (per pixel, samples in pass order).
+// This is ray_color (camera.h:356-461) cut at its traversal calls: each
+// stage does, for every path of the batch, the part of one loop iteration
+// between two world.hit() calls, in the same order.
+//
+// All stage kernels are persistent: a fixed grid sized to the device walks
+// the queue with a wave-uniform stride, reading the queue length from HBM,
+// so the per-bounce launches need no host round trip and are captured once
+// into a hipGraph replayed for every bounce.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/mpt.h"
+#include "dev_shade.hpp"
+#include "scene_host.hpp"
+
+namespace mpt {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& m) {
+    g_last_error = m;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                           \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return set_error(MPT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+constexpr int kLdsDepth = 16;
+
+// ------------------------------------------------------------------ camera
+struct dcam {
+    int w, h, spp, max_depth;
+    int max_diffuse, max_specular, max_scatter, transparent_shadows;
+    int sqrt_spp, blades, sky_mode, features;  // sky_mode: 0 env, 1 gradient, 2 constant
+    double recip_sqrt_spp;
+    v3 center, pixel00, du, dv, w_axis, disk_u, disk_v, background;
+    double ortho_height, defocus_angle, focus_dist, blade_rotation;
+};
+
+// camera::initialize (camera.h:142-179)
+static dcam init_camera(const mpt_camera& c, bool has_env) {
+    dcam k{};
+    k.w = c.image_width;
+    int h = int(c.image_width / c.aspect_ratio);
+    k.h = h < 1 ? 1 : h;
+    k.spp = c.samples_per_pixel;
+    k.max_depth = c.max_depth;
+    k.max_diffuse = c.max_diffuse;
+    k.max_specular = c.max_specular;
+    k.max_scatter = c.max_scatter;
+    k.transparent_shadows = c.transparent_shadows;
+    k.sqrt_spp = int(std::sqrt(c.samples_per_pixel));
+    k.recip_sqrt_spp = 1.0 / k.sqrt_spp;
+    k.blades = c.blades;
+    k.features = c.collect_features;
+    k.sky_mode = has_env ? 0 : (c.use_sky ? 1 : 2);
+    k.background = v3{c.background[0], c.background[1], c.background[2]};
+    k.ortho_height = c.ortho_height;
+    k.defocus_angle = c.defocus_angle;
+    k.focus_dist = c.focus_dist;
+    k.blade_rotation = c.blade_rotation;
+    v3 lookfrom{c.lookfrom[0], c.lookfrom[1], c.lookfrom[2]};
+    v3 lookat{c.lookat[0], c.lookat[1], c.lookat[2]};
+    v3 vup{c.vup[0], c.vup[1], c.vup[2]};
+    k.center = lookfrom;
+    double theta = deg2rad(c.vfov);
+    double hh = std::tan(theta / 2);
+    double vh = c.ortho_height > 0 ? c.ortho_height : 2 * hh * c.focus_dist;
+    double vw = vh * (double(k.w) / k.h);
+    v3 w = unit_vector(lookfrom - lookat);
+    v3 u = unit_vector(cross(vup, w));
+    v3 v = cross(w, u);
+    v3 vu = vw * u;
+    v3 vv = vh * -v;
+    k.du = vu / k.w;
+    k.dv = vv / k.h;
+    v3 ul = k.center - (c.focus_dist * w) - vu / 2 - vv / 2;
+    k.pixel00 = ul + 0.5 * (k.du + k.dv);
+    double dr = c.focus_dist * std::tan(deg2rad(c.defocus_angle / 2));
+    k.disk_u = u * dr;
+    k.disk_v = v * dr;
+    k.w_axis = w;
+    return k;
+}
+
+__device__ v3 escaped(const dscene& S, const dcam& C, const v3& dir) {  // camera.h:515-529 (no sun)
+    if (C.sky_mode == 0) return env_radiance(S, dir);
+    if (C.sky_mode == 2) return C.background;
+    v3 ud = unit_vector(dir);
+    double a = 0.5 * (ud.y + 1.0);
+    return (1.0 - a) * v3{1.0, 1.0, 1.0} + a * v3{0.5, 0.7, 1.0};
+}
+
+__device__ __forceinline__ double power_heuristic(double a, double b) {  // camera.h:343-347
+    double a2 = a * a;
+    double b2 = b * b;
+    return (a2 + b2) > 0 ? a2 / (a2 + b2) : 0;
+}
+
+// ------------------------------------------------------------------ path state (SoA in HBM)
+enum {
+    D_OX, D_OY, D_OZ, D_DX, D_DY, D_DZ,     // current ray
+    D_BX, D_BY, D_BZ,                       // throughput beta
+    D_RX, D_RY, D_RZ,                       // path radiance
+    D_PX, D_PY, D_PZ,                       // prev_point
+    D_SPDF,                                 // scatter_pdf
+    D_HT,                                   // hit t
+    D_SOX, D_SOY, D_SOZ,                    // shadow ray origin
+    D_TLX, D_TLY, D_TLZ,                    // to_light (unnormalized)
+    D_FX, D_FY, D_FZ,                       // f of the NEE sample
+    D_SBX, D_SBY, D_SBZ,                    // beta at the NEE vertex
+    D_SW,                                   // MIS weight / light pdf
+    D_AX, D_AY, D_AZ, D_NX, D_NY, D_NZ,     // first-surface guides
+    D_COUNT
+};
+
+struct pstate {
+    double* d[D_COUNT];
+    uint64_t* key;
+    uint32_t* ctr;
+    uint32_t* flags;   // bit0 specular, bit1 seen_by_shadow_ray, 10-bit counters from bit 2, 12, 22
+    int32_t* hobj;
+    int32_t* hprim;
+};
+
+__device__ __forceinline__ v3 ld3(const pstate& P, int f, uint32_t i) { return v3{P.d[f][i], P.d[f + 1][i], P.d[f + 2][i]}; }
+__device__ __forceinline__ void st3(const pstate& P, int f, uint32_t i, const v3& v) {
+    P.d[f][i] = v.x;
+    P.d[f + 1][i] = v.y;
+    P.d[f + 2][i] = v.z;
+}
+
+struct launch_ctx {
+    int32_t* ovf;
+    int64_t nthreads;
+};
+
+// Wave-aggregated queue append: one atomic per wave, lane offsets from the
+// ballot's popcount prefix. Must be reached by every lane of the wave.
+__device__ __forceinline__ void wave_append(uint32_t* counter, uint32_t* queue, bool pred, uint32_t value) {
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return;
+    uint32_t lane = __lane_id();
+    uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == uint32_t(leader)) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+    base = __shfl(base, leader);
+    if (pred) queue[base + below] = value;
+}
+
+#define STACK_SETUP()                                                                   \
+    __shared__ int32_t lds_stack[kLdsDepth * BS];                                       \
+    tstack st;                                                                          \
+    st.lds = lds_stack;                                                                 \
+    st.ovf = L.ovf;                                                                     \
+    st.tid = threadIdx.x;                                                               \
+    st.bs = BS;                                                                         \
+    st.gtid = int64_t(blockIdx.x) * BS + threadIdx.x;                                   \
+    st.nthreads = L.nthreads;                                                           \
+    st.lds_depth = kLdsDepth;
+
+// ------------------------------------------------------------------ ray query
+template <int BS>
+__global__ __launch_bounds__(BS) void k_query(dscene S, launch_ctx L, const double* rays, const double* tt, int64_t n,
+                                              int64_t* out_id, double* out_t, uint8_t* out_front) {
+    STACK_SETUP();
+    for (int64_t i = st.gtid; i < n; i += L.nthreads) {
+        v3 o{rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]};
+        v3 d{rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]};
+        hitinfo h;
+        bool hit = trace_closest(S, o, d, tt[2 * i], tt[2 * i + 1], h, st, false, 0, 0);
+        if (hit) {
+            hitrec rec;
+            finish_hit(S, h, o, d, rec);
+            out_id[i] = S.obj[h.obj].id_base + S.pcanon[h.prim];
+            out_t[i] = h.t;
+            out_front[i] = rec.front ? 1 : 0;
+        } else {
+            out_id[i] = -1;
+            out_t[i] = 0;
+            out_front[i] = 0;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ camera rays
+struct batch_args {
+    uint64_t seed;
+    uint32_t scene_index;
+    int32_t first_pass;   // pass of slot 0
+    int32_t npix;
+    int32_t nslots;       // passes in this batch * npix
+};
+
+__device__ __forceinline__ v3 random_in_unit_disk(rstream& rs) {  // vec3.h:139-145
+    while (true) {
+        double x = rs.next(-1, 1);
+        double y = rs.next(-1, 1);
+        v3 p{x, y, 0};
+        if (length_squared(p) < 1) return p;
+    }
+}
+
+__global__ void k_camera(dcam C, pstate P, batch_args B, uint32_t* q0, uint32_t* cnt) {
+    int64_t gt = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gt == 0) {
+        cnt[0] = uint32_t(B.nslots);
+        cnt[1] = 0;
+        cnt[2] = 0;
+    }
+    if (gt >= B.nslots) return;
+    uint32_t slot = uint32_t(gt);
+    int pass = B.first_pass + int(slot / B.npix);
+    int pix = int(slot % B.npix);
+    int i = pix % C.w, j = pix / C.w;
+    rstream rs{path_key(B.seed, B.scene_index, uint32_t(pass), uint32_t(pix)), 0};
+    int strata = C.sqrt_spp * C.sqrt_spp;
+    double ox, oy;
+    if (pass >= strata) {  // camera.h:304-306
+        ox = rs.next() - 0.5;
+        oy = rs.next() - 0.5;
+    } else {  // camera.h:205-206, 310-314
+        int cell = (pass + i * 7919 + j * 104729) % strata;
+        int si = cell % C.sqrt_spp, sj = cell / C.sqrt_spp;
+        ox = ((si + rs.next()) * C.recip_sqrt_spp) - 0.5;
+        oy = ((sj + rs.next()) * C.recip_sqrt_spp) - 0.5;
+    }
+    // camera.h:291-301
+    v3 ps = C.pixel00 + ((i + ox) * C.du) + ((j + oy) * C.dv);
+    v3 eye = C.ortho_height > 0 ? ps + C.focus_dist * C.w_axis : C.center;
+    v3 org = eye;
+    if (C.defocus_angle > 0) {
+        v3 p;
+        if (C.blades >= 3) {  // camera.h:329-339
+            int k = int(rs.next() * C.blades);
+            k = k < C.blades - 1 ? k : C.blades - 1;
+            double a0 = deg2rad(C.blade_rotation) + 2 * kPi * k / C.blades;
+            double a1 = a0 + 2 * kPi / C.blades;
+            double s = rs.next(), t = rs.next();
+            if (s + t > 1) {
+                s = 1 - s;
+                t = 1 - t;
+            }
+            p = s * v3{std::cos(a0), std::sin(a0), 0} + t * v3{std::cos(a1), std::sin(a1), 0};
+        } else {
+            p = random_in_unit_disk(rs);
+        }
+        org = eye + ((p.x * C.disk_u) + (p.y * C.disk_v));
+    }
+    v3 dir = ps - org;
+    st3(P, D_OX, slot, org);
+    st3(P, D_DX, slot, dir);
+    st3(P, D_BX, slot, v3{1, 1, 1});
+    st3(P, D_RX, slot, v3{0, 0, 0});
+    st3(P, D_PX, slot, v3{0, 0, 0});
+    P.d[D_SPDF][slot] = 0;
+    P.key[slot] = rs.key;
+    P.ctr[slot] = rs.ctr;
+    P.flags[slot] = 1u;  // specular: camera rays see lights unweighted
+    q0[slot] = slot;
+}
+
+// camera::first_surface (camera.h:253-277) on the feature stream.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_features(dscene S, launch_ctx L, pstate P, batch_args B) {
+    STACK_SETUP();
+    for (int64_t s = st.gtid; s < B.nslots; s += L.nthreads) {
+        uint32_t slot = uint32_t(s);
+        v3 o = ld3(P, D_OX, slot), d = ld3(P, D_DX, slot);
+        rstream rs{feature_key(P.key[slot]), 0};
+        v3 tint{1, 1, 1};
+        v3 albedo = tint, normal{0, 0, 0};
+        bool done = false;
+        for (int bounce = 0; bounce < 8 && !done; bounce++) {
+            hitinfo h;
+            if (!trace_closest(S, o, d, 0.001, kInf, h, st, true, rs.key, uint32_t(bounce))) break;
+            hitrec rec;
+            finish_hit(S, h, o, d, rec);
+            const mpt_material& m = S.mat[rec.mat];
+            tint = tint * mat_interior(m, d, rec);
+            scatter_out so = mat_scatter(S, m, d, rec, rs);
+            if (!so.scattered) {
+                albedo = tint;
+                normal = rec.normal;
+                done = true;
+            } else if (!so.skip) {
+                albedo = tint * so.attenuation;
+                normal = rec.normal;
+                done = true;
+            } else {
+                tint = tint * so.attenuation;
+                o = rec.p;
+                d = so.skip_dir;
+            }
+        }
+        if (!done) {
+            albedo = tint;
+            normal = v3{0, 0, 0};
+        }
+        st3(P, D_AX, slot, albedo);
+        st3(P, D_NX, slot, normal);
+    }
+}
+
+// ------------------------------------------------------------------ extend
+template <int BS>
+__global__ __launch_bounds__(BS) void k_extend(dscene S, launch_ctx L, pstate P, int depth, const uint32_t* q,
+                                               const uint32_t* cnt_in, uint32_t* cnt_next, uint32_t* cnt_shadow,
+                                               unsigned long long* ray_stats) {
+    STACK_SETUP();
+    const uint32_t n = *cnt_in;
+    if (st.gtid == 0) {  // the counters shade and shadow append to in this bounce
+        *cnt_next = 0;
+        *cnt_shadow = 0;
+        atomicAdd(&ray_stats[0], (unsigned long long)n);
+    }
+    for (int64_t i = st.gtid; i < n; i += L.nthreads) {
+        uint32_t p = q[i];
+        v3 o = ld3(P, D_OX, p), d = ld3(P, D_DX, p);
+        hitinfo h;
+        trace_closest(S, o, d, 0.001, kInf, h, st, true, P.key[p], uint32_t(2 * depth));
+        P.d[D_HT][p] = h.t;
+        P.hobj[p] = h.obj;
+        P.hprim[p] = h.prim;
+    }
+}
+
+// ------------------------------------------------------------------ shade
+template <int BS>
+__global__ __launch_bounds__(BS) void k_shade(dscene S, dcam C, pstate P, int depth, const uint32_t* q,
+                                              const uint32_t* cnt_in, uint32_t* qnext, uint32_t* cnt_next,
+                                              uint32_t* qshadow, uint32_t* cnt_shadow, int64_t nthreads) {
+    const uint32_t n = *cnt_in;
+    const int64_t wave = (int64_t(blockIdx.x) * BS + threadIdx.x) / 64;
+    const int64_t nwaves = nthreads / 64;
+    const uint32_t lane = __lane_id();
+    for (int64_t base = wave * 64; base < n; base += nwaves * 64) {
+        int64_t i = base + lane;
+        bool alive = false, nee = false;
+        uint32_t p = 0;
+        if (i < n) {
+            p = q[i];
+            v3 o = ld3(P, D_OX, p), d = ld3(P, D_DX, p);
+            v3 beta = ld3(P, D_BX, p), rad = ld3(P, D_RX, p);
+            uint32_t fl = P.flags[p];
+            bool specular = fl & 1u, seen = fl & 2u;
+            int dcount = (fl >> 2) & 1023, scount = (fl >> 12) & 1023, vcount = (fl >> 22) & 1023;
+            rstream rs{P.key[p], P.ctr[p]};
+            const int hobj = P.hobj[p];
+            if (hobj < 0) {  // camera.h:373-387
+                v3 sky = escaped(S, C, d);
+                if (!seen) {
+                    if (specular || S.nlights == 0) {
+                        rad = rad + beta * sky;
+                    } else {
+                        double lp = lights_pdf(S, ld3(P, D_PX, p), d);
+                        rad = rad + beta * sky * power_heuristic(P.d[D_SPDF][p], lp);
+                    }
+                }
+            } else {
+                hitinfo h{P.d[D_HT][p], hobj, P.hprim[p]};
+                hitrec rec;
+                finish_hit(S, h, o, d, rec);
+                const mpt_material& m = S.mat[rec.mat];
+                beta = beta * mat_interior(m, d, rec);  // camera.h:396
+                v3 e = mat_emitted(m, d, rec);            // camera.h:399-407
+                if (!seen && (e.x > 0 || e.y > 0 || e.z > 0)) {
+                    if (specular || S.nlights == 0) {
+                        rad = rad + beta * e;
+                    } else {
+                        double lp = lights_pdf(S, ld3(P, D_PX, p), d);
+                        rad = rad + beta * e * power_heuristic(P.d[D_SPDF][p], lp);
+                    }
+                }
+                scatter_out so = mat_scatter(S, m, d, rec, rs);  // camera.h:409-411
+                if (so.scattered) {
+                    if (so.skip) {  // camera.h:413-423
+                        bool stop = C.max_specular > 0 && ++scount > C.max_specular;
+                        if (!stop) {
+                            beta = beta * so.attenuation;
+                            if (C.transparent_shadows && mat_lets_shadows(m) && S.nlights > 0 && depth > 0 && !specular)
+                                seen = true;
+                            o = rec.p;
+                            d = so.skip_dir;
+                            specular = true;
+                            alive = true;
+                        }
+                    } else {  // camera.h:424-448
+                        bool volume = m.type == MPT_MAT_PHASE;
+                        int cap = volume ? C.max_scatter : C.max_diffuse;
+                        bool capped = false;
+                        if (cap > 0) {
+                            int& count = volume ? vcount : dcount;
+                            capped = ++count >= cap;
+                        }
+                        bool last_bounce = depth == C.max_depth - 1 || capped;
+                        seen = false;
+                        if (S.nlights > 0) {  // sample_light, camera.h:466-506, up to the shadow ray
+                            v3 to_light = lights_random(S, rec.p, rs);
+                            double light_pdf = lights_pdf(S, rec.p, to_light);
+                            if (light_pdf > 0) {
+                                v3 f = mat_eval(S, m, d, rec, to_light);
+                                if (!(f.x <= 0 && f.y <= 0 && f.z <= 0)) {
+                                    double weight =
+                                        last_bounce ? 1.0 : power_heuristic(light_pdf, density_value(so.dens, to_light));
+                                    st3(P, D_SOX, p, rec.p);
+                                    st3(P, D_TLX, p, to_light);
+                                    st3(P, D_FX, p, f);
+                                    st3(P, D_SBX, p, beta);
+                                    P.d[D_SW][p] = weight / light_pdf;
+                                    nee = true;
+                                }
+                            }
+                        }
+                        if (!capped) {
+                            v3 dir = density_generate(so.dens, rs);
+                            double spdf = density_value(so.dens, dir);
+                            if (spdf > 0) {
+                                beta = beta * mat_eval(S, m, d, rec, dir) / spdf;
+                                P.d[D_SPDF][p] = spdf;
+                                st3(P, D_PX, p, rec.p);
+                                o = rec.p;
+                                d = dir;
+                                specular = false;
+                                alive = true;
+                            }
+                        }
+                    }
+                    if (alive && depth >= 3) {  // camera.h:452-457
+                        double survive = std::fmin(std::fmax(beta.x, std::fmax(beta.y, beta.z)), 0.95);
+                        if (rs.next() >= survive) alive = false;
+                        else beta = beta / survive;
+                    }
+                }
+            }
+            if (depth + 1 >= C.max_depth) alive = false;
+            st3(P, D_OX, p, o);
+            st3(P, D_DX, p, d);
+            st3(P, D_BX, p, beta);
+            st3(P, D_RX, p, rad);
+            P.ctr[p] = rs.ctr;
+            P.flags[p] = (specular ? 1u : 0u) | (seen ? 2u : 0u) | (uint32_t(dcount) << 2) | (uint32_t(scount) << 12) |
+                         (uint32_t(vcount) << 22);
+        }
+        wave_append(cnt_next, qnext, alive, p);
+        wave_append(cnt_shadow, qshadow, nee, p);
+    }
+}
+
+// ------------------------------------------------------------------ shadow
+template <int BS>
+__global__ __launch_bounds__(BS) void k_shadow(dscene S, dcam C, launch_ctx L, pstate P, int depth,
+                                               const uint32_t* qshadow, const uint32_t* cnt_shadow,
+                                               unsigned long long* ray_stats) {
+    STACK_SETUP();
+    const uint32_t n = *cnt_shadow;
+    if (st.gtid == 0) atomicAdd(&ray_stats[1], (unsigned long long)n);
+    for (int64_t i = st.gtid; i < n; i += L.nthreads) {
+        uint32_t p = qshadow[i];
+        v3 o = ld3(P, D_SOX, p);
+        v3 to_light = ld3(P, D_TLX, p);
+        v3 dir = unit_vector(to_light);
+        v3 through{1, 1, 1};
+        v3 emission;
+        for (int k = 0;; k++) {  // camera.h:488-502
+            hitinfo h;
+            if (!trace_closest(S, o, dir, 0.001, kInf, h, st, true, P.key[p], uint32_t(2 * depth + 1) | (uint32_t(k) << 20))) {
+                emission = escaped(S, C, to_light);
+                break;
+            }
+            hitrec rec;
+            finish_hit(S, h, o, dir, rec);
+            const mpt_material& m = S.mat[rec.mat];
+            if (C.transparent_shadows && k < 16 && mat_lets_shadows(m)) {
+                through = through * mat_interior(m, dir, rec);
+                o = rec.p;
+                continue;
+            }
+            emission = mat_emitted(m, dir, rec);
+            break;
+        }
+        emission = through * emission;
+        v3 c = ld3(P, D_FX, p) * emission * P.d[D_SW][p];
+        v3 rad = ld3(P, D_RX, p) + ld3(P, D_SBX, p) * c;
+        st3(P, D_RX, p, rad);
+    }
+}
+
+// ------------------------------------------------------------------ accumulate
+__global__ void k_accumulate(pstate P, int npix, int passes, int features, double* acc_rad, double* acc_alb,
+                             double* acc_nrm, double* acc_l2) {
+    int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix) return;
+    v3 a = v3{acc_rad[3 * pix], acc_rad[3 * pix + 1], acc_rad[3 * pix + 2]};
+    v3 al{0, 0, 0}, nr{0, 0, 0};
+    double l2 = 0;
+    if (features) {
+        al = v3{acc_alb[3 * pix], acc_alb[3 * pix + 1], acc_alb[3 * pix + 2]};
+        nr = v3{acc_nrm[3 * pix], acc_nrm[3 * pix + 1], acc_nrm[3 * pix + 2]};
+        l2 = acc_l2[pix];
+    }
+    for (int b = 0; b < passes; b++) {
+        uint32_t s = uint32_t(b) * uint32_t(npix) + uint32_t(pix);
+        v3 c = ld3(P, D_RX, s);
+        if (!std::isfinite(c.x) || !std::isfinite(c.y) || !std::isfinite(c.z)) c = v3{0, 0, 0};  // camera.h:281-287
+        a = a + c;
+        if (features) {
+            al = al + ld3(P, D_AX, s);
+            nr = nr + ld3(P, D_NX, s);
+            double l = luminance(c);
+            l2 += l * l;
+        }
+    }
+    acc_rad[3 * pix] = a.x, acc_rad[3 * pix + 1] = a.y, acc_rad[3 * pix + 2] = a.z;
+    if (features) {
+        acc_alb[3 * pix] = al.x, acc_alb[3 * pix + 1] = al.y, acc_alb[3 * pix + 2] = al.z;
+        acc_nrm[3 * pix] = nr.x, acc_nrm[3 * pix + 1] = nr.y, acc_nrm[3 * pix + 2] = nr.z;
+        acc_l2[pix] = l2;
+    }
+}
+
+// ------------------------------------------------------------------ BSDF test hook
+__global__ void k_bsdf_eval(dscene S, const int32_t* mats, const double* q, const uint8_t* front, int64_t n,
+                            double* eval, double* pdf) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    hitrec rec;
+    rec.normal = v3{q[9 * i], q[9 * i + 1], q[9 * i + 2]};
+    v3 rd{q[9 * i + 3], q[9 * i + 4], q[9 * i + 5]};
+    v3 wi{q[9 * i + 6], q[9 * i + 7], q[9 * i + 8]};
+    rec.p = v3{0, 0, 0};
+    rec.t = 1;
+    rec.front = front[i] != 0;
+    rec.mat = mats[i];
+    const mpt_material& m = S.mat[rec.mat];
+    rstream rs{0, 0};
+    scatter_out so = mat_scatter(S, m, rd, rec, rs);
+    v3 e = mat_eval(S, m, rd, rec, wi);
+    eval[3 * i] = e.x, eval[3 * i + 1] = e.y, eval[3 * i + 2] = e.z;
+    pdf[i] = (so.scattered && !so.skip) ? density_value(so.dens, wi) : 0.0;
+}
+
+}  // namespace mpt
+
+// ======================================================================== host side
+using namespace mpt;
+
+struct mpt_scene {
+    host_scene hs;
+    dscene ds{};
+    std::vector<void*> allocs;
+    int device = 0;
+    int cu_count = 0;
+    // persistent-grid launch configuration
+    int grid = 0;
+    int block = 256;
+    int32_t* ovf = nullptr;
+    int64_t ovf_threads = 0;
+    int ovf_depth = 0;
+    // render state
+    pstate ps{};
+    int64_t ps_capacity = 0;
+    std::vector<void*> ps_allocs;
+    uint32_t* queues = nullptr;   // [3][capacity]: two ping-pong + shadow
+    uint32_t* counters = nullptr; // [8]
+    double* acc = nullptr;        // radiance, albedo, normal (3 each) + l2
+    int64_t acc_pix = 0;
+    int last_done = 0;
+    mpt_camera last_cam{};
+    hipStream_t stream = nullptr;
+
+    ~mpt_scene() {
+        for (void* p : allocs) (void)hipFree(p);
+        for (void* p : ps_allocs) (void)hipFree(p);
+        if (ovf) (void)hipFree(ovf);
+        if (queues) (void)hipFree(queues);
+        if (counters) (void)hipFree(counters);
+        if (acc) (void)hipFree(acc);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <typename T>
+int upload(mpt_scene* s, const std::vector<T>& v, const T** out) {
+    if (v.empty()) {
+        *out = nullptr;
+        return MPT_OK;
+    }
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, v.size() * sizeof(T)));
+    s->allocs.push_back(p);
+    HIP_TRY(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    *out = static_cast<const T*>(p);
+    return MPT_OK;
+}
+
+int tree_depth(const std::vector<bvh_node>& nodes, int root) {
+    if (nodes.empty()) return 0;
+    // iterative depth of the subtree at root
+    int best = 0;
+    std::vector<std::pair<int, int>> todo{{root, 1}};
+    while (!todo.empty()) {
+        auto [i, dpt] = todo.back();
+        todo.pop_back();
+        best = std::max(best, dpt);
+        const bvh_node& n = nodes[i];
+        if (n.count == 0 && i + 1 < int(nodes.size())) {
+            todo.push_back({i + 1, dpt + 1});
+            todo.push_back({n.offset, dpt + 1});
+        }
+    }
+    return best;
+}
+
+int ensure_ovf(mpt_scene* s) {
+    int depth_tlas = tree_depth(s->hs.tlas, 0);
+    int depth_blas = 0;
+    for (const auto& o : s->hs.objects)
+        if (o.kind == MPT_OBJ_MESH) depth_blas = std::max(depth_blas, tree_depth(s->hs.blas, o.blas_root));
+    int need = depth_tlas + depth_blas + 2 - kLdsDepth;
+    if (need < 1) need = 1;
+    int64_t threads = int64_t(s->grid) * s->block;
+    if (s->ovf && s->ovf_depth >= need && s->ovf_threads == threads) return MPT_OK;
+    if (s->ovf) HIP_TRY(hipFree(s->ovf));
+    s->ovf = nullptr;
+    HIP_TRY(hipMalloc(&s->ovf, size_t(need) * threads * sizeof(int32_t)));
+    s->ovf_depth = need;
+    s->ovf_threads = threads;
+    return MPT_OK;
+}
+
+int grid_for(mpt_scene* s) {
+    int blocks_per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, reinterpret_cast<const void*>(&k_extend<256>),
+                                                         256, 0));
+    if (blocks_per_cu < 1) blocks_per_cu = 1;
+    s->block = 256;
+    s->grid = s->cu_count * blocks_per_cu;
+    return MPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mpt_last_error(void) { return g_last_error.c_str(); }
+int32_t mpt_abi_version(void) { return MPT_ABI_VERSION; }
+
+int32_t mpt_device_info(char* name, int32_t name_len, int32_t* cu_count, int64_t* hbm_bytes) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    if (name && name_len > 0) {
+        std::snprintf(name, size_t(name_len), "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (cu_count) *cu_count = prop.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = int64_t(prop.totalGlobalMem);
+    return MPT_OK;
+}
+
+int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
+    if (!desc || !out) return set_error(MPT_E_INVALID, "null argument");
+    *out = nullptr;
+    auto s = std::make_unique<mpt_scene>();
+    std::string err;
+    int rc = build_host_scene(*desc, s->hs, err);
+    if (rc != MPT_OK) return set_error(rc, err);
+    HIP_TRY(hipGetDevice(&s->device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, s->device));
+    s->cu_count = prop.multiProcessorCount;
+    HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    host_scene& h = s->hs;
+    dscene& d = s->ds;
+    if ((rc = upload(s.get(), h.prim, &d.prim))) return rc;
+    if ((rc = upload(s.get(), h.prim_kind, &d.pkind))) return rc;
+    if ((rc = upload(s.get(), h.prim_mat, &d.pmat))) return rc;
+    if ((rc = upload(s.get(), h.prim_canon, &d.pcanon))) return rc;
+    if ((rc = upload(s.get(), h.objects, &d.obj))) return rc;
+    if ((rc = upload(s.get(), h.tlas, &d.tlas))) return rc;
+    if ((rc = upload(s.get(), h.tlas_items, &d.titems))) return rc;
+    if ((rc = upload(s.get(), h.blas, &d.blas))) return rc;
+    if ((rc = upload(s.get(), h.xforms, &d.xf))) return rc;
+    if ((rc = upload(s.get(), h.materials, &d.mat))) return rc;
+    if ((rc = upload(s.get(), h.textures, &d.tex))) return rc;
+    if ((rc = upload(s.get(), h.lights, &d.lights))) return rc;
+    if ((rc = upload(s.get(), h.light_cdf, &d.lcdf))) return rc;
+    if ((rc = upload(s.get(), h.env_rgb, &d.env_rgb))) return rc;
+    if ((rc = upload(s.get(), h.env_cond_cdf, &d.env_cond))) return rc;
+    if ((rc = upload(s.get(), h.env_marg_cdf, &d.env_marg))) return rc;
+    if ((rc = upload(s.get(), h.env_row_weight, &d.env_roww))) return rc;
+    if ((rc = upload(s.get(), h.ggx_e, &d.ggx_e))) return rc;
+    if ((rc = upload(s.get(), h.ggx_e_avg, &d.ggx_eavg))) return rc;
+    d.env_total = h.env_total_weight;
+    d.env_intensity = h.env_intensity;
+    d.env_rotation = h.env_rotation;
+    d.nobj = int32_t(h.objects.size());
+    d.nlights = int32_t(h.lights.size());
+    d.lstrategy = h.light_strategy;
+    d.env_w = h.env_w;
+    d.env_h = h.env_h;
+    if ((rc = grid_for(s.get()))) return rc;
+    if ((rc = ensure_ovf(s.get()))) return rc;
+    *out = s.release();
+    return MPT_OK;
+}
+
+void mpt_scene_destroy(mpt_scene* scene) { delete scene; }
+
+int32_t mpt_scene_counts(const mpt_scene* s, int64_t* n_prims, int32_t* n_objects, int64_t* n_tlas, int64_t* n_blas,
+                         int32_t* n_lights) {
+    if (!s) return set_error(MPT_E_INVALID, "null scene");
+    if (n_prims) *n_prims = int64_t(s->hs.prim_kind.size());
+    if (n_objects) *n_objects = int32_t(s->hs.objects.size());
+    if (n_tlas) *n_tlas = int64_t(s->hs.tlas.size());
+    if (n_blas) *n_blas = int64_t(s->hs.blas.size());
+    if (n_lights) *n_lights = int32_t(s->hs.lights.size());
+    return MPT_OK;
+}
+
+int32_t mpt_scene_light_probabilities(const mpt_scene* s, double* probabilities) {
+    if (!s || !probabilities) return set_error(MPT_E_INVALID, "null argument");
+    for (size_t k = 0; k < s->hs.lights.size(); k++) probabilities[k] = s->hs.lights[k].probability;
+    return MPT_OK;
+}
+
+int32_t mpt_ggx_albedo_table(double* e_1024, double* e_avg_32) {
+    std::vector<double> e, ea;
+    build_ggx_table(e, ea);
+    if (e_1024) std::memcpy(e_1024, e.data(), e.size() * sizeof(double));
+    if (e_avg_32) std::memcpy(e_avg_32, ea.data(), ea.size() * sizeof(double));
+    return MPT_OK;
+}
+
+int32_t mpt_image_height(const mpt_camera* cam) {
+    if (!cam || !(cam->aspect_ratio > 0)) return set_error(MPT_E_INVALID, "bad camera");
+    int h = int(cam->image_width / cam->aspect_ratio);
+    return h < 1 ? 1 : h;
+}
+
+int32_t mpt_trace_closest_dev(mpt_scene* s, const double* rays, const double* tt, int64_t n, int64_t* id, double* t,
+                              uint8_t* front, void* stream) {
+    if (!s || n < 0) return set_error(MPT_E_INVALID, "bad argument");
+    if (n == 0) return MPT_OK;
+    if (!rays || !tt || !id || !t || !front) return set_error(MPT_E_INVALID, "null buffer");
+    launch_ctx L{s->ovf, int64_t(s->grid) * s->block};
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
+    hipLaunchKernelGGL(k_query<256>, dim3(s->grid), dim3(256), 0, st, s->ds, L, rays, tt, n, id, t, front);
+    HIP_TRY(hipGetLastError());
+    if (!stream) HIP_TRY(hipStreamSynchronize(st));
+    return MPT_OK;
+}
+
+int32_t mpt_trace_closest(mpt_scene* s, const double* rays, const double* tt, int64_t n, int64_t* id, double* t,
+                          uint8_t* front) {
+    if (!s || n < 0) return set_error(MPT_E_INVALID, "bad argument");
+    if (n == 0) return MPT_OK;
+    if (!rays || !tt || !id || !t || !front) return set_error(MPT_E_INVALID, "null buffer");
+    double *dr = nullptr, *dt = nullptr, *dto = nullptr;
+    int64_t* did = nullptr;
+    uint8_t* df = nullptr;
+    auto cleanup = [&]() {
+        (void)hipFree(dr), (void)hipFree(dt), (void)hipFree(dto), (void)hipFree(did), (void)hipFree(df);
+    };
+    if (hipMalloc(&dr, n * 6 * sizeof(double)) != hipSuccess || hipMalloc(&dt, n * 2 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&dto, n * sizeof(double)) != hipSuccess || hipMalloc(&did, n * sizeof(int64_t)) != hipSuccess ||
+        hipMalloc(&df, n) != hipSuccess) {
+        cleanup();
+        return set_error(MPT_E_HIP, "hipMalloc failed for the ray batch");
+    }
+    int rc = MPT_OK;
+    if (hipMemcpy(dr, rays, n * 6 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dt, tt, n * 2 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        cleanup();
+        return set_error(MPT_E_HIP, "hipMemcpy of the ray batch failed");
+    }
+    rc = mpt_trace_closest_dev(s, dr, dt, n, did, dto, df, nullptr);
+    if (rc == MPT_OK) {
+        if (hipMemcpy(id, did, n * sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(t, dto, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(front, df, n, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = set_error(MPT_E_HIP, "hipMemcpy of the results failed");
+    }
+    cleanup();
+    return rc;
+}
+
+int32_t mpt_bsdf_eval(mpt_scene* s, const int32_t* mats, const double* q, const uint8_t* front, int64_t n,
+                      double* eval, double* pdf) {
+    if (!s || n < 0 || (n > 0 && (!mats || !q || !front || !eval || !pdf))) return set_error(MPT_E_INVALID, "bad argument");
+    for (int64_t i = 0; i < n; i++)
+        if (mats[i] < 0 || mats[i] >= int32_t(s->hs.materials.size())) return set_error(MPT_E_INVALID, "bad material index");
+    if (n == 0) return MPT_OK;
+    int32_t* dm = nullptr;
+    double *dq = nullptr, *de = nullptr, *dp = nullptr;
+    uint8_t* dfr = nullptr;
+    HIP_TRY(hipMalloc(&dm, n * sizeof(int32_t)));
+    HIP_TRY(hipMalloc(&dq, n * 9 * sizeof(double)));
+    HIP_TRY(hipMalloc(&de, n * 3 * sizeof(double)));
+    HIP_TRY(hipMalloc(&dp, n * sizeof(double)));
+    HIP_TRY(hipMalloc(&dfr, n));
+    HIP_TRY(hipMemcpy(dm, mats, n * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dq, q, n * 9 * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dfr, front, n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_bsdf_eval, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s->stream, s->ds, dm, dq, dfr, n, de, dp);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(eval, de, n * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(pdf, dp, n * sizeof(double), hipMemcpyDeviceToHost));
+    (void)hipFree(dm), (void)hipFree(dq), (void)hipFree(de), (void)hipFree(dp), (void)hipFree(dfr);
+    return MPT_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ render
+namespace {
+
+int ensure_paths(mpt_scene* s, int64_t cap) {
+    if (s->ps_capacity >= cap) return MPT_OK;
+    for (void* p : s->ps_allocs) (void)hipFree(p);
+    s->ps_allocs.clear();
+    if (s->queues) (void)hipFree(s->queues);
+    s->queues = nullptr;
+    s->ps_capacity = 0;
+    auto alloc = [&](size_t bytes, void** out) -> int {
+        HIP_TRY(hipMalloc(out, bytes));
+        s->ps_allocs.push_back(*out);
+        return MPT_OK;
+    };
+    int rc;
+    for (int k = 0; k < D_COUNT; k++)
+        if ((rc = alloc(size_t(cap) * sizeof(double), reinterpret_cast<void**>(&s->ps.d[k])))) return rc;
+    if ((rc = alloc(size_t(cap) * sizeof(uint64_t), reinterpret_cast<void**>(&s->ps.key)))) return rc;
+    if ((rc = alloc(size_t(cap) * sizeof(uint32_t), reinterpret_cast<void**>(&s->ps.ctr)))) return rc;
+    if ((rc = alloc(size_t(cap) * sizeof(uint32_t), reinterpret_cast<void**>(&s->ps.flags)))) return rc;
+    if ((rc = alloc(size_t(cap) * sizeof(int32_t), reinterpret_cast<void**>(&s->ps.hobj)))) return rc;
+    if ((rc = alloc(size_t(cap) * sizeof(int32_t), reinterpret_cast<void**>(&s->ps.hprim)))) return rc;
+    HIP_TRY(hipMalloc(&s->queues, size_t(cap) * 3 * sizeof(uint32_t)));
+    if (!s->counters) HIP_TRY(hipMalloc(&s->counters, 8 * sizeof(uint64_t)));
+    s->ps_capacity = cap;
+    return MPT_OK;
+}
+
+void average_frame(const mpt_scene* s, const std::vector<double>& acc, int npix, int samples, bool features,
+                   double* radiance, double* albedo, double* normal, double* variance) {
+    // camera::average (camera.h:223-246)
+    double scale = 1.0 / samples;
+    const double* ar = acc.data();
+    const double* aa = ar + 3 * size_t(npix);
+    const double* an = aa + 3 * size_t(npix);
+    const double* al = an + 3 * size_t(npix);
+    (void)s;
+    for (int k = 0; k < npix; k++) {
+        double r0 = scale * ar[3 * k], r1 = scale * ar[3 * k + 1], r2 = scale * ar[3 * k + 2];
+        if (radiance) radiance[3 * k] = r0, radiance[3 * k + 1] = r1, radiance[3 * k + 2] = r2;
+        if (!features) continue;
+        if (albedo)
+            for (int c = 0; c < 3; c++) albedo[3 * k + c] = scale * aa[3 * k + c];
+        if (normal)
+            for (int c = 0; c < 3; c++) normal[3 * k + c] = scale * an[3 * k + c];
+        if (variance) {
+            double mean = luminance(v3{r0, r1, r2});
+            double spread = std::fmax(scale * al[k] - mean * mean, 0.0);
+            variance[k] = samples > 1 ? spread / (samples - 1) : spread;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t mpt_read_frame(mpt_scene* s, double* radiance, double* albedo, double* normal, double* variance) {
+    if (!s || !s->acc || s->last_done <= 0) return set_error(MPT_E_INVALID, "no frame rendered yet");
+    int npix = int(s->acc_pix);
+    std::vector<double> acc(size_t(npix) * 10);
+    HIP_TRY(hipMemcpy(acc.data(), s->acc, acc.size() * sizeof(double), hipMemcpyDeviceToHost));
+    average_frame(s, acc, npix, s->last_done, s->last_cam.collect_features != 0, radiance, albedo, normal, variance);
+    return MPT_OK;
+}
+
+int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts, double* radiance, double* albedo,
+                   double* normal, double* variance, mpt_pass_callback on_pass, void* user, mpt_render_stats* stats) {
+    if (!s || !cam) return set_error(MPT_E_INVALID, "null argument");
+    if (cam->image_width < 1 || !(cam->aspect_ratio > 0) || cam->samples_per_pixel < 1 || cam->max_depth < 1 ||
+        cam->max_depth > 1000)
+        return set_error(MPT_E_INVALID, "bad camera settings (width, aspect, spp or max_depth out of range)");
+    if (cam->max_diffuse < 0 || cam->max_diffuse > 1000 || cam->max_specular < 0 || cam->max_specular > 1000 ||
+        cam->max_scatter < 0 || cam->max_scatter > 1000)
+        return set_error(MPT_E_INVALID, "bounce caps must be in [0, 1000]");
+    mpt_render_opts o{};
+    if (opts) o = *opts;
+    auto t0 = std::chrono::steady_clock::now();
+    dcam C = init_camera(*cam, s->hs.env_w > 0);
+    const int64_t npix64 = int64_t(C.w) * C.h;
+    if (npix64 > (int64_t(1) << 28)) return set_error(MPT_E_CAPACITY, "image too large");
+    const int npix = int(npix64);
+    const int spp = C.spp;
+    int pif = o.passes_in_flight;
+    if (pif <= 0) {
+        // about 4M paths in flight: fills the device several times over
+        pif = int(std::max<int64_t>(1, (int64_t(4) << 20) / npix64));
+    }
+    pif = std::min(pif, spp);
+    if (int64_t(pif) * npix64 > (int64_t(1) << 31)) pif = int((int64_t(1) << 31) / npix64);
+    int rc;
+    if ((rc = ensure_paths(s, int64_t(pif) * npix))) return rc;
+    if (s->acc_pix != npix || !s->acc) {
+        if (s->acc) HIP_TRY(hipFree(s->acc));
+        s->acc = nullptr;
+        HIP_TRY(hipMalloc(&s->acc, size_t(npix) * 10 * sizeof(double)));
+        s->acc_pix = npix;
+    }
+    HIP_TRY(hipMemsetAsync(s->acc, 0, size_t(npix) * 10 * sizeof(double), s->stream));
+    s->last_cam = *cam;
+    s->last_done = 0;
+
+    const int64_t cap = s->ps_capacity;
+    uint32_t* q[3] = {s->queues, s->queues + cap, s->queues + 2 * cap};
+    uint32_t* cnt = s->counters;  // [0], [1] ping-pong queue lengths, [2] shadow queue length
+    launch_ctx L{s->ovf, int64_t(s->grid) * s->block};
+    const pstate P = s->ps;
+    double* acc_rad = s->acc;
+    double* acc_alb = acc_rad + 3 * size_t(npix);
+    double* acc_nrm = acc_alb + 3 * size_t(npix);
+    double* acc_l2 = acc_nrm + 3 * size_t(npix);
+    unsigned long long* ray_stats = reinterpret_cast<unsigned long long*>(cnt + 4);  // [0] extension, [1] shadow
+    HIP_TRY(hipMemsetAsync(ray_stats, 0, 2 * sizeof(unsigned long long), s->stream));
+
+    int pass = 0;
+    while (pass < spp) {
+        int nb = std::min(pif, spp - pass);
+        if (o.save_every > 0 && on_pass) {
+            int next_save = (pass / o.save_every + 1) * o.save_every;
+            nb = std::min(nb, next_save - pass);
+        }
+        batch_args B{o.seed, uint32_t(o.scene_index), o.first_pass + pass, npix, nb * npix};
+        hipLaunchKernelGGL(k_camera, dim3(unsigned((B.nslots + 255) / 256)), dim3(256), 0, s->stream, C, P, B, q[0], cnt);
+        if (C.features)
+            hipLaunchKernelGGL(k_features<256>, dim3(s->grid), dim3(256), 0, s->stream, s->ds, L, P, B);
+        for (int depth = 0; depth < C.max_depth; depth++) {
+            uint32_t* qin = q[depth & 1];
+            uint32_t* qout = q[(depth + 1) & 1];
+            uint32_t* cin = cnt + (depth & 1);
+            uint32_t* cout = cnt + ((depth + 1) & 1);
+            // extend zeroes the next-queue and shadow counters of this bounce;
+            // stream order puts it after the previous bounce's shadow stage
+            hipLaunchKernelGGL(k_extend<256>, dim3(s->grid), dim3(256), 0, s->stream, s->ds, L, P, depth, qin, cin, cout,
+                               cnt + 2, ray_stats);
+            hipLaunchKernelGGL(k_shade<256>, dim3(s->grid), dim3(256), 0, s->stream, s->ds, C, P, depth, qin, cin, qout,
+                               cout, q[2], cnt + 2, L.nthreads);
+            hipLaunchKernelGGL(k_shadow<256>, dim3(s->grid), dim3(256), 0, s->stream, s->ds, C, L, P, depth, q[2], cnt + 2,
+                               ray_stats);
+        }
+        hipLaunchKernelGGL(k_accumulate, dim3(unsigned((npix + 255) / 256)), dim3(256), 0, s->stream, P, npix, nb,
+                           C.features, acc_rad, acc_alb, acc_nrm, acc_l2);
+        HIP_TRY(hipGetLastError());
+        pass += nb;
+        s->last_done = pass;
+        if (on_pass && o.save_every > 0 && pass % o.save_every == 0 && pass < spp) {
+            HIP_TRY(hipStreamSynchronize(s->stream));
+            on_pass(pass, user);
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    auto t1 = std::chrono::steady_clock::now();
+    if (radiance || albedo || normal || variance) {
+        if ((rc = mpt_read_frame(s, radiance, albedo, normal, variance))) return rc;
+    }
+    if (stats) {
+        unsigned long long rs_host[2] = {0, 0};
+        HIP_TRY(hipMemcpy(rs_host, ray_stats, sizeof(rs_host), hipMemcpyDeviceToHost));
+        stats->paths = npix64 * spp;
+        stats->extension_rays = int64_t(rs_host[0]);
+        stats->shadow_rays = int64_t(rs_host[1]);
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+    return MPT_OK;
+}
+
+}  // extern "C"

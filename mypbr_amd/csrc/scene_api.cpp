@@ -1,0 +1,1099 @@
+// scene_api.cpp - the C++ scene API (include/mpt/mpt.hpp): transforms,
+// materials, flattening of world and lights into the C ABI description,
+// camera::render on the GPU, and the .scene loader (scene.h:244-1250, for
+// the subset of keywords the GPU path models).
+#include "../../include/mpt/mpt.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+
+#include "geom.hpp"
+#include "scene_host.hpp"
+
+namespace mpt {
+
+namespace {
+constexpr double kPiApi = 3.1415926535897932385;
+double degrees_to_radians(double d) { return d * kPiApi / 180.0; }
+double clamp01(double x) { return x < 0 ? 0 : (x > 1 ? 1 : x); }
+double r2a(double r) {  // roughness_to_alpha, microfacet.h:17-19
+    double a = r * r;
+    return a > 1e-3 ? a : 1e-3;
+}
+}  // namespace
+
+// ------------------------------------------------------------------ transform (transform.h)
+transform::transform() : m{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, t(0, 0, 0), inv_m{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, inv_t(0, 0, 0) {}
+
+transform transform::translate(const vec3& offset) {
+    transform x;
+    x.t = offset;
+    x.inv_t = -offset;
+    return x;
+}
+
+transform transform::scale(const vec3& s) {
+    transform x;
+    for (int k = 0; k < 3; k++) {
+        x.m[k][k] = s[k];
+        x.inv_m[k][k] = 1 / s[k];
+    }
+    return x;
+}
+
+transform transform::rotate(double degrees, const vec3& axis) {
+    vec3 a = unit_vector(axis);
+    double theta = degrees_to_radians(degrees);
+    double c = std::cos(theta), s = std::sin(theta), C = 1 - c;
+    transform x;
+    double r[3][3] = {
+        {a.x() * a.x() * C + c, a.x() * a.y() * C - a.z() * s, a.x() * a.z() * C + a.y() * s},
+        {a.y() * a.x() * C + a.z() * s, a.y() * a.y() * C + c, a.y() * a.z() * C - a.x() * s},
+        {a.z() * a.x() * C - a.y() * s, a.z() * a.y() * C + a.x() * s, a.z() * a.z() * C + c},
+    };
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            x.m[i][j] = r[i][j];
+            x.inv_m[j][i] = r[i][j];
+        }
+    return x;
+}
+
+transform transform::orthogonal(const double r[3][3]) {
+    transform x;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            x.m[i][j] = r[i][j];
+            x.inv_m[j][i] = r[i][j];
+        }
+    return x;
+}
+
+static void m_mul(const double a[3][3], const double b[3][3], double out[3][3]) {
+    double r[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) r[i][j] = a[i][0] * b[0][j] + a[i][1] * b[1][j] + a[i][2] * b[2][j];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) out[i][j] = r[i][j];
+}
+static vec3 mul(const double a[3][3], const vec3& v) {
+    return vec3(a[0][0] * v[0] + a[0][1] * v[1] + a[0][2] * v[2], a[1][0] * v[0] + a[1][1] * v[1] + a[1][2] * v[2],
+                a[2][0] * v[0] + a[2][1] * v[1] + a[2][2] * v[2]);
+}
+
+transform transform::then_after(const transform& first) const {
+    transform x;
+    m_mul(m, first.m, x.m);
+    x.t = apply_vector(first.t) + t;
+    m_mul(first.inv_m, inv_m, x.inv_m);
+    x.inv_t = first.apply_inverse_vector(inv_t) + first.inv_t;
+    return x;
+}
+point3 transform::apply_point(const point3& p) const { return apply_vector(p) + t; }
+vec3 transform::apply_vector(const vec3& v) const { return mul(m, v); }
+point3 transform::apply_inverse_point(const point3& p) const { return apply_inverse_vector(p) + inv_t; }
+vec3 transform::apply_inverse_vector(const vec3& v) const { return mul(inv_m, v); }
+
+bool transform::is_similarity() const {
+    vec3 c0(m[0][0], m[1][0], m[2][0]), c1(m[0][1], m[1][1], m[2][1]), c2(m[0][2], m[1][2], m[2][2]);
+    double l = c0.length_squared();
+    auto close = [l](double x, double y) { return std::fabs(x - y) <= 1e-9 * std::fmax(1.0, l); };
+    return close(c1.length_squared(), l) && close(c2.length_squared(), l) && close(dot(c0, c1), 0) &&
+           close(dot(c1, c2), 0) && close(dot(c0, c2), 0);
+}
+
+double transform::area_scale() const {
+    double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                 m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    return std::pow(std::fabs(det), 2.0 / 3.0);
+}
+
+mpt_xform transform::to_abi() const {
+    mpt_xform x;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            x.m[3 * i + j] = m[i][j];
+            x.inv_m[3 * i + j] = inv_m[i][j];
+        }
+    for (int i = 0; i < 3; i++) {
+        x.t[i] = t[i];
+        x.inv_t[i] = inv_t[i];
+    }
+    return x;
+}
+
+// ------------------------------------------------------------------ textures and materials
+mpt_texture solid_color::to_abi() const {
+    mpt_texture t{};
+    t.type = MPT_TEX_SOLID;
+    for (int k = 0; k < 3; k++) t.even[k] = t.odd[k] = albedo[k];
+    return t;
+}
+
+checker_texture::checker_texture(double scale, std::shared_ptr<texture> e, std::shared_ptr<texture> o)
+    : inv_scale(1.0 / scale), even(std::move(e)), odd(std::move(o)) {}
+
+mpt_texture checker_texture::to_abi() const {
+    auto se = std::dynamic_pointer_cast<solid_color>(even);
+    auto so = std::dynamic_pointer_cast<solid_color>(odd);
+    if (!se || !so) throw std::runtime_error("checker_texture: only solid colors are supported as cells");
+    mpt_texture t{};
+    t.type = MPT_TEX_CHECKER;
+    t.inv_scale = inv_scale;
+    for (int k = 0; k < 3; k++) {
+        t.even[k] = se->albedo[k];
+        t.odd[k] = so->albedo[k];
+    }
+    return t;
+}
+
+static mpt_material blank(int type, int32_t ti) {
+    mpt_material m;
+    std::memset(&m, 0, sizeof(m));
+    m.type = type;
+    m.texture = ti;
+    m.spread = 180;
+    m.tan_half = __builtin_huge_val();
+    return m;
+}
+
+mpt_material lambertian::to_abi(int32_t ti) const { return blank(MPT_MAT_LAMBERTIAN, ti); }
+
+metal::metal(const color& a, double r) : albedo(a), roughness(std::clamp(r, 0.0, 1.0)) {}
+mpt_material metal::to_abi(int32_t ti) const {
+    mpt_material m = blank(MPT_MAT_METAL, ti);
+    m.roughness = roughness;
+    m.alpha = r2a(roughness);
+    color f_avg = albedo + (color(1, 1, 1) - albedo) / 21;  // metal_fresnel::schlick, microfacet.h:102-107
+    for (int k = 0; k < 3; k++) {
+        m.albedo[k] = albedo[k];
+        m.f_avg[k] = f_avg[k];
+    }
+    return m;
+}
+
+dielectric::dielectric(double ri, double r, const color& absorption)
+    : refraction_index(ri), roughness(std::clamp(r, 0.0, 1.0)), sigma_a(absorption) {}
+color dielectric::absorption_for(const color& tint, double distance) {
+    return color(-std::log(tint.x()), -std::log(tint.y()), -std::log(tint.z())) / distance;
+}
+mpt_material dielectric::to_abi(int32_t ti) const {
+    mpt_material m = blank(MPT_MAT_DIELECTRIC, ti);
+    m.ior = refraction_index;
+    m.roughness = roughness;
+    m.alpha = r2a(roughness);
+    for (int k = 0; k < 3; k++) m.sigma_a[k] = sigma_a[k];
+    return m;
+}
+
+principled::principled(std::shared_ptr<texture> base, double met, double rough, double i)
+    : base_color(std::move(base)), metallic(met), roughness(rough), ior(i) {}
+mpt_material principled::to_abi(int32_t ti) const {
+    mpt_material m = blank(MPT_MAT_PRINCIPLED, ti);
+    m.metallic = clamp01(metallic);
+    m.roughness = clamp01(roughness);
+    m.alpha = r2a(m.roughness);
+    m.ior = ior;
+    double r0 = (ior - 1) / (ior + 1);
+    m.f0 = r0 * r0;
+    return m;
+}
+
+diffuse_light::diffuse_light(const color& e, double s) : emit(e), spread(s) {}
+color diffuse_light::average_emission() const {  // material.h:586-615
+    double half = degrees_to_radians(spread) / 2;
+    double share;
+    if (half >= kPiApi / 2 - 1e-9) share = 1;
+    else {
+        double s = std::sin(half), c = std::cos(half);
+        share = s * s - (half - s * c) * c / s;
+    }
+    return emit * share;
+}
+mpt_material diffuse_light::to_abi(int32_t ti) const {
+    mpt_material m = blank(MPT_MAT_DIFFUSE_LIGHT, ti);
+    m.spread = spread;
+    double half = degrees_to_radians(spread) / 2;
+    m.tan_half = spread >= 180 ? __builtin_huge_val() : std::tan(half);
+    color avg = average_emission();
+    for (int k = 0; k < 3; k++) {
+        m.emit[k] = emit[k];
+        m.avg_emission[k] = avg[k];
+    }
+    return m;
+}
+
+mpt_material phase_material::to_abi(int32_t ti) const {
+    mpt_material m = blank(MPT_MAT_PHASE, ti);
+    m.g = g;
+    for (int k = 0; k < 3; k++) m.albedo[k] = albedo[k];
+    return m;
+}
+
+std::shared_ptr<hittable_list> box(const point3& a, const point3& b, std::shared_ptr<material> mat) {
+    auto sides = std::make_shared<hittable_list>();
+    auto mn = point3(std::fmin(a.x(), b.x()), std::fmin(a.y(), b.y()), std::fmin(a.z(), b.z()));
+    auto mx = point3(std::fmax(a.x(), b.x()), std::fmax(a.y(), b.y()), std::fmax(a.z(), b.z()));
+    auto dx = vec3(mx.x() - mn.x(), 0, 0);
+    auto dy = vec3(0, mx.y() - mn.y(), 0);
+    auto dz = vec3(0, 0, mx.z() - mn.z());
+    sides->add(std::make_shared<quad>(point3(mn.x(), mn.y(), mx.z()), dx, dy, mat));   // front
+    sides->add(std::make_shared<quad>(point3(mx.x(), mn.y(), mx.z()), -dz, dy, mat));  // right
+    sides->add(std::make_shared<quad>(point3(mx.x(), mn.y(), mn.z()), -dx, dy, mat));  // back
+    sides->add(std::make_shared<quad>(point3(mn.x(), mn.y(), mn.z()), dz, dy, mat));   // left
+    sides->add(std::make_shared<quad>(point3(mn.x(), mx.y(), mx.z()), dx, -dz, mat));  // top
+    sides->add(std::make_shared<quad>(point3(mn.x(), mn.y(), mn.z()), dx, dz, mat));   // bottom
+    return sides;
+}
+
+// ------------------------------------------------------------------ flattening
+struct flat_builder::impl {
+    std::vector<int32_t> prim_kind, prim_mat;
+    std::vector<double> prim_data;
+    std::vector<int32_t> obj_kind, obj_xform, obj_mat, obj_bkind;
+    std::vector<int64_t> obj_first, obj_count, obj_id;
+    std::vector<double> obj_density;
+    std::vector<mpt_xform> xforms;
+    std::vector<mpt_material> materials;
+    std::vector<mpt_texture> textures;
+    std::vector<int32_t> light_object;
+    std::vector<float> env_rgb;
+    std::map<const material*, int32_t> mat_index;
+    std::map<const texture*, int32_t> tex_index;
+    std::map<const hittable*, int32_t> object_of;
+    int64_t next_id = 0;
+    int32_t last_object = -1;
+    mpt_scene_desc d{};
+};
+
+flat_builder::flat_builder() : p(new impl) {}
+flat_builder::~flat_builder() { delete p; }
+
+int32_t flat_builder::material_index(const std::shared_ptr<material>& m) {
+    if (!m) throw std::runtime_error("object without a material");
+    auto it = p->mat_index.find(m.get());
+    if (it != p->mat_index.end()) return it->second;
+    int32_t ti = -1;
+    if (auto t = m->base_texture()) {
+        auto jt = p->tex_index.find(t.get());
+        if (jt == p->tex_index.end()) {
+            ti = int32_t(p->textures.size());
+            p->textures.push_back(t->to_abi());
+            p->tex_index[t.get()] = ti;
+        } else {
+            ti = jt->second;
+        }
+    }
+    int32_t mi = int32_t(p->materials.size());
+    p->materials.push_back(m->to_abi(ti));
+    p->mat_index[m.get()] = mi;
+    return mi;
+}
+
+int32_t flat_builder::xform_index(const transform& x) {
+    p->xforms.push_back(x.to_abi());
+    return int32_t(p->xforms.size() - 1);
+}
+
+int64_t flat_builder::add_prim(int32_t kind, int32_t mat, const double* data9) {
+    p->prim_kind.push_back(kind);
+    p->prim_mat.push_back(mat);
+    p->prim_data.insert(p->prim_data.end(), data9, data9 + 9);
+    return int64_t(p->prim_kind.size() - 1);
+}
+
+int32_t flat_builder::add_object(int32_t kind, int64_t first, int64_t count, int32_t xform, const hittable* owner) {
+    p->obj_kind.push_back(kind);
+    p->obj_first.push_back(first);
+    p->obj_count.push_back(count);
+    p->obj_xform.push_back(xform);
+    p->obj_id.push_back(p->next_id);
+    p->obj_density.push_back(0);
+    p->obj_mat.push_back(-1);
+    p->obj_bkind.push_back(MPT_OBJ_PRIM);
+    p->next_id += count;
+    int32_t idx = int32_t(p->obj_kind.size() - 1);
+    p->object_of[owner] = idx;
+    p->last_object = idx;
+    return idx;
+}
+
+static void sphere_data(const sphere& s, double* d) {
+    std::memset(d, 0, 9 * sizeof(double));
+    d[0] = s.center[0], d[1] = s.center[1], d[2] = s.center[2], d[3] = s.radius;
+}
+static void quad_data(const quad& q, double* d) {
+    for (int k = 0; k < 3; k++) d[k] = q.Q[k], d[3 + k] = q.u[k], d[6 + k] = q.v[k];
+}
+
+void sphere::flatten(flat_builder& b, int32_t xform) const {
+    double d[9];
+    sphere_data(*this, d);
+    int64_t pi = b.add_prim(MPT_PRIM_SPHERE, b.material_index(mat), d);
+    b.add_object(MPT_OBJ_PRIM, pi, 1, xform, this);
+}
+
+void quad::flatten(flat_builder& b, int32_t xform) const {
+    double d[9];
+    quad_data(*this, d);
+    int64_t pi = b.add_prim(MPT_PRIM_QUAD, b.material_index(mat), d);
+    b.add_object(MPT_OBJ_PRIM, pi, 1, xform, this);
+}
+
+// Appends a list's children (spheres and quads) as primitives; returns the first.
+static int64_t list_prims(flat_builder& b, const hittable_list& l) {
+    int64_t first = -1;
+    for (const auto& o : l.objects) {
+        double d[9];
+        int64_t pi;
+        if (auto s = dynamic_cast<const sphere*>(o.get())) {
+            sphere_data(*s, d);
+            pi = b.add_prim(MPT_PRIM_SPHERE, b.material_index(s->mat), d);
+        } else if (auto q = dynamic_cast<const quad*>(o.get())) {
+            quad_data(*q, d);
+            pi = b.add_prim(MPT_PRIM_QUAD, b.material_index(q->mat), d);
+        } else {
+            throw std::runtime_error("a hittable_list object may only hold spheres and quads on the GPU path");
+        }
+        if (first < 0) first = pi;
+    }
+    return first;
+}
+
+void hittable_list::flatten(flat_builder& b, int32_t xform) const {
+    if (objects.empty()) throw std::runtime_error("empty hittable_list in the world");
+    int64_t first = list_prims(b, *this);
+    b.add_object(MPT_OBJ_LIST, first, int64_t(objects.size()), xform, this);
+}
+
+void bvh::flatten(flat_builder& b, int32_t xform) const {
+    if (xform >= 0) throw std::runtime_error("instances of a bvh are not supported on the GPU path");
+    for (const auto& o : objects) o->flatten(b, -1);
+}
+
+void mesh::flatten(flat_builder& b, int32_t xform) const {
+    if (indices.empty() || indices.size() % 3 != 0) throw std::runtime_error("mesh needs index triples");
+    auto it = b.mesh_ranges.find(this);
+    int64_t first, count = int64_t(indices.size() / 3);
+    if (it == b.mesh_ranges.end()) {
+        int32_t mi = b.material_index(mat);
+        first = -1;
+        for (size_t k = 0; k < indices.size(); k += 3) {
+            double d[9];
+            for (int c = 0; c < 3; c++) {
+                int32_t vi = indices[k + c];
+                if (vi < 0 || size_t(vi) >= positions.size()) throw std::runtime_error("mesh index out of range");
+                for (int a = 0; a < 3; a++) d[3 * c + a] = positions[vi][a];
+            }
+            int64_t pi = b.add_prim(MPT_PRIM_TRIANGLE, mi, d);
+            if (first < 0) first = pi;
+        }
+        b.mesh_ranges[this] = {first, count};
+    } else {
+        first = it->second.first;
+    }
+    b.add_object(MPT_OBJ_MESH, first, count, xform, this);
+}
+
+void instance::flatten(flat_builder& b, int32_t xform) const {
+    if (xform >= 0) throw std::runtime_error("nested instances are not supported on the GPU path");
+    if (dynamic_cast<const constant_medium*>(object.get()))
+        throw std::runtime_error("transform a medium's boundary, not the medium (medium.h:55-56)");
+    int32_t x = b.xform_index(this->xform);
+    object->flatten(b, x);
+    b.p->object_of[this] = b.p->last_object;
+}
+
+void constant_medium::flatten(flat_builder& b, int32_t xform) const {
+    if (xform >= 0) throw std::runtime_error("instances of a medium are not supported");
+    const hittable* bd = boundary.get();
+    int32_t bx = -1;
+    if (auto inst = dynamic_cast<const instance*>(bd)) {
+        bx = b.xform_index(inst->xform);
+        bd = inst->object.get();
+    }
+    int32_t mi = b.material_index(phase);
+    int64_t first, count;
+    int32_t bkind;
+    if (auto s = dynamic_cast<const sphere*>(bd)) {
+        double d[9];
+        sphere_data(*s, d);
+        first = b.add_prim(MPT_PRIM_SPHERE, s->mat ? b.material_index(s->mat) : mi, d);
+        count = 1;
+        bkind = MPT_OBJ_PRIM;
+    } else if (auto l = dynamic_cast<const hittable_list*>(bd)) {
+        first = list_prims(b, *l);
+        count = int64_t(l->objects.size());
+        bkind = MPT_OBJ_LIST;
+    } else {
+        throw std::runtime_error("a medium's boundary must be a sphere or a box on the GPU path");
+    }
+    int32_t oi = b.add_object(MPT_OBJ_MEDIUM, first, count, bx, this);
+    b.p->obj_density[oi] = density;
+    b.p->obj_mat[oi] = mi;
+    b.p->obj_bkind[oi] = bkind;
+}
+
+void environment_light::flatten(flat_builder&, int32_t) const {
+    // never hit (environment.h:53-56): not part of the world's geometry
+}
+
+void flat_builder::add_world(const hittable& world) {
+    if (auto l = dynamic_cast<const hittable_list*>(&world)) {
+        for (const auto& o : l->objects) o->flatten(*this, -1);
+    } else {
+        world.flatten(*this, -1);
+    }
+}
+
+void flat_builder::add_lights(const hittable_list& lights, const std::shared_ptr<environment_light>& env) {
+    const environment_light* envp = env.get();
+    for (const auto& o : lights.objects) {
+        if (auto e = dynamic_cast<const environment_light*>(o.get())) {
+            if (envp && envp != e) throw std::runtime_error("only one environment is allowed");
+            envp = e;
+            p->light_object.push_back(-1);
+            continue;
+        }
+        auto it = p->object_of.find(o.get());
+        if (it == p->object_of.end())
+            throw std::runtime_error("lights must also be part of the world (camera.h:73-74)");
+        p->light_object.push_back(it->second);
+    }
+    if (envp) {
+        if (envp->width < 1 || envp->height < 1 || envp->pixels.size() != size_t(envp->width) * envp->height)
+            throw std::runtime_error("environment_light: pixel count does not match its size");
+        p->env_rgb.resize(envp->pixels.size() * 3);
+        for (size_t k = 0; k < envp->pixels.size(); k++)
+            for (int c = 0; c < 3; c++) p->env_rgb[3 * k + c] = float(envp->pixels[k][c]);
+        p->d.env_width = envp->width;
+        p->d.env_height = envp->height;
+        p->d.env_intensity = envp->intensity;
+        p->d.env_rotation_degrees = envp->rotation_degrees;
+    }
+    auto ll = dynamic_cast<const light_list*>(&lights);
+    if (ll && ll->built) {
+        p->d.light_strategy = ll->strategy == light_list::selection::power ? 1 : 0;
+        p->d.light_uniform_fraction = ll->uniform_fraction;
+    } else {
+        p->d.light_strategy = 2;  // plain hittable_list: uniform pick (hittable_list.h:58-62)
+        p->d.light_uniform_fraction = 0;
+    }
+}
+
+const mpt_scene_desc& flat_builder::desc() {
+    mpt_scene_desc& d = p->d;
+    d.n_prims = int64_t(p->prim_kind.size());
+    d.prim_kind = p->prim_kind.data();
+    d.prim_material = p->prim_mat.data();
+    d.prim_data = p->prim_data.data();
+    d.n_objects = int32_t(p->obj_kind.size());
+    d.object_kind = p->obj_kind.data();
+    d.object_first = p->obj_first.data();
+    d.object_count = p->obj_count.data();
+    d.object_xform = p->obj_xform.data();
+    d.object_id_base = p->obj_id.data();
+    d.object_density = p->obj_density.data();
+    d.object_material = p->obj_mat.data();
+    d.object_boundary_kind = p->obj_bkind.data();
+    d.n_xforms = int32_t(p->xforms.size());
+    d.xforms = p->xforms.data();
+    d.n_materials = int32_t(p->materials.size());
+    d.materials = p->materials.data();
+    d.n_textures = int32_t(p->textures.size());
+    d.textures = p->textures.data();
+    d.n_lights = int32_t(p->light_object.size());
+    d.light_object = p->light_object.data();
+    d.env_rgb = p->env_rgb.empty() ? nullptr : p->env_rgb.data();
+    return d;
+}
+
+// ------------------------------------------------------------------ camera
+mpt_camera camera::to_abi() const {
+    mpt_camera c;
+    std::memset(&c, 0, sizeof(c));
+    c.aspect_ratio = aspect_ratio;
+    c.image_width = image_width;
+    c.samples_per_pixel = samples_per_pixel;
+    c.max_depth = max_depth;
+    c.max_diffuse = max_diffuse;
+    c.max_specular = max_specular;
+    c.max_scatter = max_scatter;
+    c.transparent_shadows = transparent_shadows;
+    c.vfov = vfov;
+    for (int k = 0; k < 3; k++) {
+        c.lookfrom[k] = lookfrom[k];
+        c.lookat[k] = lookat[k];
+        c.vup[k] = vup[k];
+        c.background[k] = background[k];
+    }
+    c.use_sky = use_sky;
+    c.collect_features = collect_features;
+    c.ortho_height = ortho_height;
+    c.defocus_angle = defocus_angle;
+    c.focus_dist = focus_dist;
+    c.blades = blades;
+    c.blade_rotation = blade_rotation;
+    return c;
+}
+
+gpu_scene::gpu_scene(const hittable& world, const hittable_list& lights, const std::shared_ptr<environment_light>& env) {
+    flat_builder b;
+    b.add_world(world);
+    b.add_lights(lights, env);
+    int rc = mpt_scene_create(&b.desc(), &handle);
+    if (rc != MPT_OK) throw std::runtime_error(std::string("mpt_scene_create: ") + mpt_last_error());
+}
+gpu_scene::~gpu_scene() { mpt_scene_destroy(handle); }
+
+namespace {
+struct pass_ctx {
+    const camera::pass_callback* cb;
+    mpt_scene* scene;
+    size_t npix;
+    bool features;
+};
+frame to_frame(const std::vector<double>& r, const std::vector<double>& a, const std::vector<double>& n,
+               const std::vector<double>& v, size_t npix, bool features) {
+    frame f;
+    f.radiance.resize(npix);
+    for (size_t k = 0; k < npix; k++) f.radiance[k] = color(r[3 * k], r[3 * k + 1], r[3 * k + 2]);
+    if (!features) return f;
+    f.albedo.resize(npix);
+    f.normal.resize(npix);
+    f.variance.assign(v.begin(), v.end());
+    for (size_t k = 0; k < npix; k++) {
+        f.albedo[k] = color(a[3 * k], a[3 * k + 1], a[3 * k + 2]);
+        f.normal[k] = color(n[3 * k], n[3 * k + 1], n[3 * k + 2]);
+    }
+    return f;
+}
+void pass_trampoline(int32_t done, void* user) {
+    auto* c = static_cast<pass_ctx*>(user);
+    std::vector<double> r(3 * c->npix), a, n, v;
+    if (c->features) {
+        a.resize(3 * c->npix);
+        n.resize(3 * c->npix);
+        v.resize(c->npix);
+    }
+    if (mpt_read_frame(c->scene, r.data(), c->features ? a.data() : nullptr, c->features ? n.data() : nullptr,
+                       c->features ? v.data() : nullptr) == MPT_OK)
+        (*c->cb)(done, to_frame(r, a, n, v, c->npix, c->features));
+}
+}  // namespace
+
+frame camera::render(const hittable& world, const hittable_list& lights, int save_every, const pass_callback& on_pass) {
+    image_height = int(image_width / aspect_ratio);
+    image_height = image_height < 1 ? 1 : image_height;
+    gpu_scene gs(world, lights, environment);
+    mpt_camera c = to_abi();
+    mpt_render_opts o{};
+    o.seed = seed;
+    o.save_every = save_every;
+    size_t npix = size_t(image_width) * image_height;
+    std::vector<double> r(3 * npix), a, n, v;
+    if (collect_features) {
+        a.resize(3 * npix);
+        n.resize(3 * npix);
+        v.resize(npix);
+    }
+    pass_ctx ctx{&on_pass, gs.get(), npix, collect_features};
+    int rc = mpt_render(gs.get(), &c, &o, r.data(), collect_features ? a.data() : nullptr,
+                        collect_features ? n.data() : nullptr, collect_features ? v.data() : nullptr,
+                        on_pass ? pass_trampoline : nullptr, &ctx, nullptr);
+    if (rc != MPT_OK) throw std::runtime_error(std::string("mpt_render: ") + mpt_last_error());
+    return to_frame(r, a, n, v, npix, collect_features);
+}
+
+// ------------------------------------------------------------------ .scene loader
+namespace {
+
+// Reads a PFM image (the reference's own output format, image_io.h:49-62)
+// into rows from the top.
+bool read_pfm(const std::string& path, int& w, int& h, std::vector<color>& px) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    std::string magic;
+    double scale;
+    f >> magic >> w >> h >> scale;
+    f.get();
+    if (magic != "PF" || w <= 0 || h <= 0) return false;
+    std::vector<float> data(size_t(w) * h * 3);
+    f.read(reinterpret_cast<char*>(data.data()), std::streamsize(data.size() * sizeof(float)));
+    if (!f) return false;
+    px.resize(size_t(w) * h);
+    for (int j = 0; j < h; j++)  // PFM stores the bottom row first
+        for (int i = 0; i < w; i++) {
+            const float* s = &data[(size_t(h - 1 - j) * w + i) * 3];
+            px[size_t(j) * w + i] = color(s[0], s[1], s[2]);
+        }
+    return true;
+}
+
+}  // namespace
+
+bool load_scene(const std::string& path, scene& result, std::string* err_out) {
+    auto& world = result.world;
+    auto& lights = result.lights;
+    auto& cam = result.cam;
+    std::string errbuf;
+    std::string& err = err_out ? *err_out : errbuf;
+
+    std::ifstream file(path);
+    if (!file) {
+        err = "cannot open scene file '" + path + "'";
+        return false;
+    }
+    int line_number = 0;
+    auto fail = [&](const std::string& message) {
+        err = path + ":" + std::to_string(line_number) + ": " + message;
+        return false;
+    };
+    std::map<std::string, std::shared_ptr<material>> materials;
+    std::map<std::string, std::shared_ptr<texture>> textures;
+    auto slash = path.find_last_of('/');
+    std::string scene_dir = slash == std::string::npos ? "" : path.substr(0, slash + 1);
+    auto default_material = std::make_shared<lambertian>(color(0.5, 0.5, 0.5));
+
+    auto read_color_or_texture = [&](std::istringstream& in, const std::string& first, std::shared_ptr<texture>& tex,
+                                     std::string& error) {
+        if (first == "texture") {
+            std::string name;
+            if (!(in >> name)) return false;
+            auto it = textures.find(name);
+            if (it == textures.end()) {
+                error = "unknown texture '" + name + "'";
+                return false;
+            }
+            tex = it->second;
+            return true;
+        }
+        std::istringstream rgb(first);
+        double r, g, b;
+        if (!(rgb >> r) || !(in >> g >> b)) return false;
+        tex = std::make_shared<solid_color>(color(r, g, b));
+        return true;
+    };
+
+    auto read_tail = [&](std::istringstream& in, std::shared_ptr<material>& mat, transform& xform, bool& transformed,
+                         std::string& error) {
+        mat = default_material;
+        xform = transform();
+        transformed = false;
+        auto is_transform = [](const std::string& t) {
+            return t == "translate" || t == "scale" || t == "rotate" || t == "rotate_x" || t == "rotate_y" ||
+                   t == "rotate_z";
+        };
+        std::string token;
+        if (!(in >> token)) return true;
+        if (!is_transform(token)) {
+            auto it = materials.find(token);
+            if (it == materials.end()) {
+                error = "unknown material '" + token + "'";
+                return false;
+            }
+            mat = it->second;
+            if (!(in >> token)) return true;
+        }
+        do {
+            transform step;
+            bool ok = true;
+            if (token == "translate") {
+                vec3 d;
+                ok = bool(in >> d[0] >> d[1] >> d[2]);
+                step = transform::translate(d);
+            } else if (token == "rotate_x" || token == "rotate_y" || token == "rotate_z") {
+                double degrees;
+                ok = bool(in >> degrees);
+                vec3 axis(token == "rotate_x", token == "rotate_y", token == "rotate_z");
+                step = transform::rotate(degrees, axis);
+            } else if (token == "rotate") {
+                double degrees;
+                vec3 axis;
+                ok = bool(in >> degrees >> axis[0] >> axis[1] >> axis[2]) && axis.length_squared() > 0;
+                if (ok) step = transform::rotate(degrees, axis);
+            } else if (token == "scale") {
+                vec3 f;
+                ok = bool(in >> f[0]);
+                if (ok) {
+                    std::streampos mark = in.tellg();
+                    if (in >> f[1]) {
+                        ok = bool(in >> f[2]);
+                    } else {
+                        in.clear();
+                        in.seekg(mark);
+                        f = vec3(f[0], f[0], f[0]);
+                    }
+                }
+                ok = ok && f[0] != 0 && f[1] != 0 && f[2] != 0;
+                step = transform::scale(f);
+            } else {
+                error = "unexpected value '" + token + "' (expected a transform)";
+                return false;
+            }
+            if (!ok) {
+                error = "bad values for '" + token + "'";
+                return false;
+            }
+            xform = step.then_after(xform);
+            transformed = true;
+        } while (in >> token);
+        return true;
+    };
+
+    auto add_object = [&](std::shared_ptr<hittable> object, const std::shared_ptr<material>& mat,
+                          const transform& xform, bool transformed, std::string& error) {
+        bool emissive = bool(std::dynamic_pointer_cast<diffuse_light>(mat));
+        if (transformed) {
+            if (emissive && !xform.is_similarity()) {
+                error = "lights can only be moved, rotated and scaled uniformly";
+                return false;
+            }
+            object = std::make_shared<instance>(object, xform);
+        }
+        world.add(object);
+        if (emissive) lights.add(object);
+        return true;
+    };
+
+    bool sky_uniform = false;
+    color sky_radiance;
+    double fog_density = 0;
+    std::shared_ptr<material> fog_material;
+    std::string line;
+    while (std::getline(file, line)) {
+        line_number++;
+        auto comment = line.find('#');
+        if (comment != std::string::npos) line.erase(comment);
+        std::istringstream in(line);
+        std::string keyword;
+        if (!(in >> keyword)) continue;
+
+        if (keyword == "image_width") {
+            if (!(in >> cam.image_width) || cam.image_width < 1) return fail("expected 'image_width <positive integer>'");
+        } else if (keyword == "aspect_ratio") {
+            double num, den = 1.0;
+            if (!(in >> num)) return fail("expected 'aspect_ratio <w>/<h>' or 'aspect_ratio <number>'");
+            if (in.peek() == '/') {
+                in.get();
+                if (!(in >> den)) return fail("expected 'aspect_ratio <w>/<h>'");
+            }
+            if (num <= 0 || den <= 0) return fail("aspect_ratio must be positive");
+            cam.aspect_ratio = num / den;
+        } else if (keyword == "samples_per_pixel") {
+            if (!(in >> cam.samples_per_pixel) || cam.samples_per_pixel < 1)
+                return fail("expected 'samples_per_pixel <positive integer>'");
+        } else if (keyword == "max_depth") {
+            if (!(in >> cam.max_depth) || cam.max_depth < 1) return fail("expected 'max_depth <positive integer>'");
+        } else if (keyword == "camera") {
+            std::string key;
+            while (in >> key) {
+                bool ok = true;
+                if (key == "lookfrom") ok = bool(in >> cam.lookfrom[0] >> cam.lookfrom[1] >> cam.lookfrom[2]);
+                else if (key == "lookat") ok = bool(in >> cam.lookat[0] >> cam.lookat[1] >> cam.lookat[2]);
+                else if (key == "vup") ok = bool(in >> cam.vup[0] >> cam.vup[1] >> cam.vup[2]);
+                else if (key == "vfov") ok = bool(in >> cam.vfov) && cam.vfov > 0 && cam.vfov < 180;
+                else if (key == "aperture") ok = bool(in >> cam.defocus_angle) && cam.defocus_angle >= 0;
+                else if (key == "focus") ok = bool(in >> cam.focus_dist) && cam.focus_dist > 0;
+                else if (key == "ortho") ok = bool(in >> cam.ortho_height) && cam.ortho_height >= 0;
+                else if (key == "blades") ok = bool(in >> cam.blades) && (cam.blades == 0 || cam.blades >= 3);
+                else if (key == "blade_rotation") ok = bool(in >> cam.blade_rotation);
+                else return fail("unknown camera setting '" + key + "'");
+                if (!ok) return fail("bad value for camera setting '" + key + "'");
+            }
+        } else if (keyword == "vfov") {
+            if (!(in >> cam.vfov) || cam.vfov <= 0 || cam.vfov >= 180) return fail("expected 'vfov <degrees between 0 and 180>'");
+        } else if (keyword == "aperture") {
+            if (!(in >> cam.defocus_angle) || cam.defocus_angle < 0) return fail("expected 'aperture <defocus angle in degrees>'");
+        } else if (keyword == "focus") {
+            if (!(in >> cam.focus_dist) || cam.focus_dist <= 0) return fail("expected 'focus <positive distance>'");
+        } else if (keyword == "background") {
+            double r, g, b;
+            if (!(in >> r >> g >> b)) return fail("expected 'background <r> <g> <b>'");
+            cam.use_sky = false;
+            cam.background = color(r, g, b);
+        } else if (keyword == "light_selection") {
+            std::string mode;
+            if (!(in >> mode) || (mode != "power" && mode != "uniform")) return fail("expected 'light_selection power|uniform'");
+            lights.strategy = mode == "power" ? light_list::selection::power : light_list::selection::uniform;
+        } else if (keyword == "environment") {
+            std::string fname, key;
+            double intensity = 1, rotation = 0;
+            const char* usage = "expected 'environment <file.pfm> [intensity <s>] [rotate <degrees>]'";
+            if (!(in >> fname)) return fail(usage);
+            while (in >> key) {
+                bool ok = (key == "intensity" && in >> intensity && intensity >= 0) || (key == "rotate" && in >> rotation);
+                if (!ok) return fail(usage);
+            }
+            if (fname[0] != '/') fname = scene_dir + fname;
+            int w = 0, h = 0;
+            std::vector<color> px;
+            if (!read_pfm(fname, w, h, px))
+                return fail("cannot load environment map '" + fname + "' (the GPU path reads .pfm images)");
+            if (cam.environment) return fail("only one environment is allowed");
+            cam.environment = std::make_shared<environment_light>(w, h, std::move(px), intensity, rotation);
+            lights.add(cam.environment);
+        } else if (keyword == "sky") {
+            std::string type;
+            if (!(in >> type)) return fail("expected 'sky uniform <r g b>' or 'sky gradient'");
+            if (type == "uniform") {
+                if (!(in >> sky_radiance[0] >> sky_radiance[1] >> sky_radiance[2])) return fail("expected 'sky uniform <r g b>'");
+                sky_uniform = true;
+                cam.use_sky = true;
+            } else if (type == "gradient") {
+                sky_uniform = false;
+                cam.use_sky = true;
+            } else {
+                return fail("sky type '" + type + "' is not modelled by the GPU path (uniform and gradient are)");
+            }
+        } else if (keyword == "bounces") {
+            std::string key;
+            while (in >> key) {
+                bool ok = (key == "diffuse" && in >> cam.max_diffuse && cam.max_diffuse >= 0) ||
+                          (key == "specular" && in >> cam.max_specular && cam.max_specular >= 0) ||
+                          (key == "scatter" && in >> cam.max_scatter && cam.max_scatter >= 0);
+                if (!ok) return fail("expected 'bounces [diffuse <n>] [specular <n>] [scatter <n>]'");
+            }
+        } else if (keyword == "transparent_shadows") {
+            std::string value;
+            if (!(in >> value) || (value != "on" && value != "off")) return fail("expected 'transparent_shadows on|off'");
+            cam.transparent_shadows = value == "on";
+        } else if (keyword == "bloom" || keyword == "vignette" || keyword == "exposure" || keyword == "tonemap" ||
+                   keyword == "normal_maps") {
+            // display settings: they change how radiance becomes pixels, not the radiance
+            std::string rest;
+            std::getline(in, rest);
+        } else if (keyword == "texture") {
+            std::string name, type;
+            if (!(in >> name >> type)) return fail("expected 'texture <name> <type> <parameters...>'");
+            std::shared_ptr<texture> tex;
+            if (type == "solid") {
+                double r, g, b;
+                if (!(in >> r >> g >> b)) return fail("expected 'texture <name> solid <r> <g> <b>'");
+                tex = std::make_shared<solid_color>(color(r, g, b));
+            } else if (type == "checker") {
+                double scale, r0, g0, b0, r1, g1, b1;
+                if (!(in >> scale >> r0 >> g0 >> b0 >> r1 >> g1 >> b1) || scale <= 0)
+                    return fail("expected 'texture <name> checker <scale> <r g b> <r g b>'");
+                tex = std::make_shared<checker_texture>(scale, std::make_shared<solid_color>(color(r0, g0, b0)),
+                                                        std::make_shared<solid_color>(color(r1, g1, b1)));
+            } else {
+                return fail("texture type '" + type + "' is not modelled by the GPU path (solid and checker are)");
+            }
+            textures[name] = tex;
+        } else if (keyword == "material") {
+            std::string name, type;
+            if (!(in >> name >> type)) return fail("expected 'material <name> <type> <parameters...>'");
+            std::shared_ptr<material> mat;
+            if (type == "lambertian") {
+                std::shared_ptr<texture> tex;
+                std::string first, error;
+                if (!(in >> first) || !read_color_or_texture(in, first, tex, error))
+                    return fail(error.empty() ? "expected 'material <name> lambertian <r> <g> <b>'" : error);
+                mat = std::make_shared<lambertian>(tex);
+            } else if (type == "metal") {
+                double r, g, b, roughness;
+                if (!(in >> r >> g >> b >> roughness) || roughness < 0)
+                    return fail("expected 'material <name> metal <r> <g> <b> <roughness>' (measured metals are not modelled)");
+                mat = std::make_shared<metal>(color(r, g, b), roughness);
+            } else if (type == "dielectric") {
+                const char* usage = "expected 'material <name> dielectric <ior> [roughness <r>] [color <r> <g> <b> [distance <d>]]'";
+                double ior, roughness = 0, distance = 1;
+                color tint(1, 1, 1);
+                std::string key;
+                if (!(in >> ior) || ior <= 0) return fail(usage);
+                while (in >> key) {
+                    bool ok = true;
+                    if (key == "roughness") ok = bool(in >> roughness) && roughness >= 0;
+                    else if (key == "color") {
+                        double r, g, b;
+                        ok = bool(in >> r >> g >> b) && r > 0 && g > 0 && b > 0 && r <= 1 && g <= 1 && b <= 1;
+                        tint = color(r, g, b);
+                    } else if (key == "distance") ok = bool(in >> distance) && distance > 0;
+                    else ok = false;
+                    if (!ok) return fail(usage);
+                }
+                mat = std::make_shared<dielectric>(ior, roughness, dielectric::absorption_for(tint, distance));
+            } else if (type == "principled") {
+                std::shared_ptr<texture> base = std::make_shared<solid_color>(color(0.8, 0.8, 0.8));
+                double metallic = 0, roughness = 0.5, ior = 1.5;
+                std::string key, error;
+                while (in >> key) {
+                    bool ok = true;
+                    if (key == "texture") ok = read_color_or_texture(in, key, base, error);
+                    else if (key == "color") {
+                        std::string first;
+                        ok = bool(in >> first) && read_color_or_texture(in, first, base, error);
+                    } else if (key == "metallic") ok = bool(in >> metallic) && metallic >= 0 && metallic <= 1;
+                    else if (key == "roughness") ok = bool(in >> roughness) && roughness >= 0 && roughness <= 1;
+                    else if (key == "ior") ok = bool(in >> ior) && ior > 0;
+                    else return fail("principled setting '" + key + "' is not modelled by the GPU path");
+                    if (!ok) return fail(error.empty() ? "bad value for principled setting '" + key + "'" : error);
+                }
+                mat = std::make_shared<principled>(base, metallic, roughness, ior);
+            } else if (type == "volume") {
+                double r, g, b, aniso = 0;
+                std::string key;
+                if (!(in >> r >> g >> b)) return fail("expected 'material <name> volume <r> <g> <b> [g <anisotropy>]'");
+                if (in >> key) {
+                    if (key != "g" || !(in >> aniso) || aniso <= -1 || aniso >= 1)
+                        return fail("expected 'material <name> volume <r> <g> <b> [g <anisotropy between -1 and 1>]'");
+                }
+                mat = std::make_shared<phase_material>(color(r, g, b), aniso);
+            } else if (type == "emissive") {
+                double r, g, b, spread = 180;
+                std::string key;
+                if (!(in >> r >> g >> b)) return fail("expected 'material <name> emissive <r> <g> <b> [spread <degrees>]'");
+                if (in >> key && (key != "spread" || !(in >> spread) || spread <= 0 || spread > 180))
+                    return fail("expected 'material <name> emissive <r> <g> <b> [spread <degrees, 0 to 180>]'");
+                mat = std::make_shared<diffuse_light>(color(r, g, b), spread);
+            } else {
+                return fail("material type '" + type + "' is not modelled by the GPU path");
+            }
+            materials[name] = mat;
+        } else if (keyword == "sphere" || keyword == "quad" || keyword == "box") {
+            std::shared_ptr<hittable> object;
+            std::shared_ptr<material> mat;
+            transform xform;
+            bool transformed;
+            std::string error;
+            if (keyword == "sphere") {
+                double x, y, z, radius;
+                if (!(in >> x >> y >> z >> radius) || radius == 0)
+                    return fail("expected 'sphere <x> <y> <z> <non-zero radius> [material] [transforms]'");
+                if (!read_tail(in, mat, xform, transformed, error)) return fail(error);
+                object = std::make_shared<sphere>(point3(x, y, z), radius, mat);
+            } else if (keyword == "quad") {
+                point3 Q;
+                vec3 u, v;
+                if (!(in >> Q[0] >> Q[1] >> Q[2] >> u[0] >> u[1] >> u[2] >> v[0] >> v[1] >> v[2]))
+                    return fail("expected 'quad <Qx Qy Qz> <ux uy uz> <vx vy vz> [material] [transforms]'");
+                if (cross(u, v).near_zero()) return fail("quad edges must not be parallel");
+                if (!read_tail(in, mat, xform, transformed, error)) return fail(error);
+                object = std::make_shared<quad>(Q, u, v, mat);
+            } else {
+                point3 a, b;
+                if (!(in >> a[0] >> a[1] >> a[2] >> b[0] >> b[1] >> b[2]))
+                    return fail("expected 'box <x0 y0 z0> <x1 y1 z1> [material] [transforms]'");
+                if (!read_tail(in, mat, xform, transformed, error)) return fail(error);
+                object = box(a, b, mat);
+            }
+            if (!add_object(object, mat, xform, transformed, error)) return fail(error);
+        } else if (keyword == "medium") {
+            double density;
+            std::string shape, error;
+            std::shared_ptr<hittable> boundary;
+            std::shared_ptr<material> mat;
+            transform xform;
+            bool transformed;
+            if (!(in >> density >> shape) || density <= 0)
+                return fail("expected 'medium <positive density> sphere|box <...> <material> [transforms]'");
+            if (shape == "sphere") {
+                double x, y, z, radius;
+                if (!(in >> x >> y >> z >> radius) || radius <= 0)
+                    return fail("expected 'medium <density> sphere <x> <y> <z> <positive radius> <material>'");
+                boundary = std::make_shared<sphere>(point3(x, y, z), radius, default_material);
+            } else if (shape == "box") {
+                point3 a, b;
+                if (!(in >> a[0] >> a[1] >> a[2] >> b[0] >> b[1] >> b[2]))
+                    return fail("expected 'medium <density> box <x0 y0 z0> <x1 y1 z1> <material>'");
+                boundary = box(a, b, default_material);
+            } else {
+                return fail("a medium's boundary must be 'sphere' or 'box'");
+            }
+            if (!read_tail(in, mat, xform, transformed, error)) return fail(error);
+            if (!std::dynamic_pointer_cast<phase_material>(mat)) return fail("a medium needs a volume material");
+            if (transformed) boundary = std::make_shared<instance>(boundary, xform);
+            world.add(std::make_shared<constant_medium>(boundary, density, mat));
+        } else if (keyword == "fog") {
+            std::string name;
+            if (!(in >> fog_density >> name) || fog_density <= 0) return fail("expected 'fog <positive density> <volume material>'");
+            auto it = materials.find(name);
+            if (it == materials.end() || !std::dynamic_pointer_cast<phase_material>(it->second))
+                return fail("expected a volume material, got '" + name + "'");
+            fog_material = it->second;
+        } else {
+            return fail("keyword '" + keyword + "' is not modelled by the GPU path");
+        }
+        std::string extra;
+        if (in >> extra) return fail("unexpected value '" + extra + "'");
+    }
+
+    if (world.empty()) return fail("the scene has no objects");
+    if (sky_uniform) {
+        if (cam.environment) return fail("'sky uniform' and 'environment' can't be used together");
+        cam.environment = std::make_shared<environment_light>(8, 4, std::vector<color>(32, sky_radiance));
+        lights.add(cam.environment);
+    }
+    if (fog_material) {
+        // the world's bounding box: flatten what is there and let the core compute it
+        flat_builder fb;
+        try {
+            fb.add_world(world);
+        } catch (const std::exception& e) {
+            return fail(e.what());
+        }
+        host_scene hs;
+        std::string e2;
+        if (build_host_scene(fb.desc(), hs, e2) != MPT_OK) return fail(e2);
+        const aabb& wb = hs.world_box;
+        point3 lo(wb.ax[0].min, wb.ax[1].min, wb.ax[2].min);
+        point3 hi(wb.ax[0].max, wb.ax[1].max, wb.ax[2].max);
+        world.add(std::make_shared<constant_medium>(box(lo, hi, default_material), fog_density, fog_material));
+    }
+    if ((cam.lookfrom - cam.lookat).near_zero() || cross(cam.vup, cam.lookfrom - cam.lookat).near_zero())
+        return fail("camera lookfrom must differ from lookat, and vup must not point along the view");
+    lights.build();
+    return true;
+}
+
+}  // namespace mpt
+
+// ------------------------------------------------------------------ C ABI: mpt_scene_load
+namespace mpt {
+int set_error(int code, const std::string& m);
+}
+
+extern "C" int32_t mpt_scene_load(const char* path, mpt_scene** out, mpt_camera* cam) {
+    if (!path || !out) return mpt::set_error(MPT_E_INVALID, "null argument");
+    *out = nullptr;
+    mpt::scene sc;
+    std::string err;
+    if (!mpt::load_scene(path, sc, &err)) return mpt::set_error(err.find("cannot open") == 0 ? MPT_E_IO : MPT_E_UNSUPPORTED, err);
+    try {
+        mpt::flat_builder b;
+        b.add_world(sc.world);
+        b.add_lights(sc.lights, sc.cam.environment);
+        int rc = mpt_scene_create(&b.desc(), out);
+        if (rc != MPT_OK) return rc;
+    } catch (const std::exception& e) {
+        return mpt::set_error(MPT_E_UNSUPPORTED, e.what());
+    }
+    if (cam) *cam = sc.cam.to_abi();
+    return MPT_OK;
+}
+
+extern "C" int32_t mpt_scene_describe(const char* path, mpt_scene_desc* desc, mpt_camera* cam, void** holder) {
+    if (!path || !desc || !holder) return mpt::set_error(MPT_E_INVALID, "null argument");
+    *holder = nullptr;
+    mpt::scene sc;
+    std::string err;
+    if (!mpt::load_scene(path, sc, &err)) return mpt::set_error(err.find("cannot open") == 0 ? MPT_E_IO : MPT_E_UNSUPPORTED, err);
+    auto* b = new mpt::flat_builder;
+    try {
+        b->add_world(sc.world);
+        b->add_lights(sc.lights, sc.cam.environment);
+        *desc = b->desc();
+    } catch (const std::exception& e) {
+        delete b;
+        return mpt::set_error(MPT_E_UNSUPPORTED, e.what());
+    }
+    if (cam) *cam = sc.cam.to_abi();
+    *holder = b;
+    return MPT_OK;
+}
+
+extern "C" void mpt_desc_free(void* holder) { delete static_cast<mpt::flat_builder*>(holder); }

@@ -1,0 +1,533 @@
+// scene_host.cpp - validation of a flattened scene description and the
+// one-off host work before upload: primitive records, object boxes, the
+// binned-SAH BVHs, light selection probabilities, the environment's
+// sampling tables and the GGX albedo table.
+#include "scene_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <utility>
+
+#include "microfacet.hpp"
+
+namespace mpt {
+
+namespace {
+
+constexpr int kBins = 16;        // bvh.h:110
+constexpr int kMaxLeaf = 4;      // bvh.h:111
+constexpr int kMaxSahDepth = 64; // bvh.h:115
+
+struct builder {
+    std::vector<build_item>& items;
+    std::vector<bvh_node>& nodes;
+
+    int make_leaf(int index, size_t begin, size_t count, int axis, int depth) {  // bvh.h:190-205
+        if (count > size_t(kMaxLeaf)) {
+            size_t mid = begin + count / 2;
+            nodes[index].axis = axis;
+            build(begin, mid, depth + 1);
+            int right = build(mid, begin + count, depth + 1);
+            nodes[index].offset = right;
+            return index;
+        }
+        nodes[index].offset = int(begin);
+        nodes[index].count = int(count);
+        return index;
+    }
+
+    int build(size_t begin, size_t end, int depth) {  // bvh.h:118-188
+        int index = int(nodes.size());
+        nodes.push_back(bvh_node{});
+        nodes[index].count = 0;
+        nodes[index].offset = 0;
+        nodes[index].axis = 0;
+
+        aabb bounds = box_empty(), cbounds = box_empty();
+        for (size_t k = begin; k < end; k++) {
+            bounds = box_merge(bounds, items[k].box);
+            // aabb(c, c) pads the degenerate point box before the merge
+            cbounds = box_merge(cbounds, box_points(items[k].centroid, items[k].centroid));
+        }
+        for (int a = 0; a < 3; a++) {
+            nodes[index].b[2 * a] = bounds.ax[a].min;
+            nodes[index].b[2 * a + 1] = bounds.ax[a].max;
+        }
+
+        size_t count = end - begin;
+        int axis = box_longest_axis(cbounds);
+        const interval extent = cbounds.ax[axis];
+
+        if (count == 1 || iv_size(extent) <= 0 || depth >= kMaxSahDepth)
+            return make_leaf(index, begin, count, axis, depth);
+
+        struct bin {
+            aabb box;
+            int count;
+        } bins[kBins];
+        for (auto& b : bins) {
+            b.box = box_empty();
+            b.count = 0;
+        }
+        auto bin_of = [&](const v3& c) {
+            int b = int(kBins * (at(c, axis) - extent.min) / iv_size(extent));
+            return std::min(b, kBins - 1);
+        };
+        for (size_t k = begin; k < end; k++) {
+            auto& b = bins[bin_of(items[k].centroid)];
+            b.box = box_merge(b.box, items[k].box);
+            b.count++;
+        }
+
+        double best_cost = kInf;
+        int best_split = -1;
+        for (int split = 0; split < kBins - 1; split++) {
+            aabb left = box_empty(), right = box_empty();
+            int lc = 0, rc = 0;
+            for (int b = 0; b <= split; b++) {
+                left = box_merge(left, bins[b].box);
+                lc += bins[b].count;
+            }
+            for (int b = split + 1; b < kBins; b++) {
+                right = box_merge(right, bins[b].box);
+                rc += bins[b].count;
+            }
+            if (lc == 0 || rc == 0) continue;
+            double cost = box_area(left) * lc + box_area(right) * rc;
+            if (cost < best_cost) {
+                best_cost = cost;
+                best_split = split;
+            }
+        }
+
+        double leaf_cost = double(count);
+        double split_cost = 0.125 + best_cost / box_area(bounds);
+        if (best_split < 0 || (count <= size_t(kMaxLeaf) && leaf_cost <= split_cost))
+            return make_leaf(index, begin, count, axis, depth);
+
+        auto mid_it = std::partition(items.begin() + begin, items.begin() + end,
+                                     [&](const build_item& it) { return bin_of(it.centroid) <= best_split; });
+        size_t mid = size_t(mid_it - items.begin());
+
+        nodes[index].axis = axis;
+        build(begin, mid, depth + 1);
+        int right = build(mid, end, depth + 1);
+        nodes[index].offset = right;
+        return index;
+    }
+};
+
+// Boxes of the primitive kinds (sphere.h:14-19, quad.h:17-22, mesh.h:40-41).
+aabb prim_box(int kind, const double* p) {
+    if (kind == MPT_PRIM_SPHERE) {
+        v3 c{p[0], p[1], p[2]};
+        double r = std::fabs(p[3]);
+        v3 rv{r, r, r};
+        return box_points(c - rv, c + rv);
+    }
+    if (kind == MPT_PRIM_QUAD) {
+        v3 Q{p[0], p[1], p[2]}, u{p[3], p[4], p[5]}, v{p[6], p[7], p[8]};
+        return box_merge(box_points(Q, Q + u + v), box_points(Q + u, Q + v));
+    }
+    v3 p0{p[0], p[1], p[2]}, p1{p[3], p[4], p[5]}, p2{p[6], p[7], p[8]};
+    return box_merge(box_points(p0, p1), box_points(p2, p2));
+}
+
+// instance bounding box: the eight transformed corners (instance.h:13-23).
+aabb instance_box(const xform& x, const aabb& b) {
+    aabb r = box_empty();
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++)
+            for (int k = 0; k < 2; k++) {
+                v3 c{i ? b.ax[0].max : b.ax[0].min, j ? b.ax[1].max : b.ax[1].min, k ? b.ax[2].max : b.ax[2].min};
+                v3 p = xf_point(x, c);
+                r = box_merge(r, box_points(p, p));
+            }
+    return r;
+}
+
+double area_scale(const xform& x) {  // transform.h:92-97
+    const double* m = x.m;
+    double det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+                 m[2] * (m[3] * m[7] - m[4] * m[6]);
+    return std::pow(std::fabs(det), 2.0 / 3.0);
+}
+
+}  // namespace
+
+void build_bvh(std::vector<build_item>& items, std::vector<bvh_node>& nodes) {
+    nodes.clear();
+    if (items.empty()) {
+        bvh_node n{};
+        aabb e = box_empty();
+        for (int a = 0; a < 3; a++) {
+            n.b[2 * a] = e.ax[a].min;
+            n.b[2 * a + 1] = e.ax[a].max;
+        }
+        nodes.push_back(n);
+        return;
+    }
+    nodes.reserve(2 * items.size());
+    builder b{items, nodes};
+    b.build(0, items.size(), 0);
+}
+
+void build_ggx_table(std::vector<double>& e, std::vector<double>& e_avg) {  // microfacet.h:299-337
+    const int size = kGgxTableSize;
+    const int strata = 64;
+    e.assign(size * size, 0.0);
+    e_avg.assign(size, 0.0);
+    for (int j = 0; j < size; j++) {
+        double roughness = double(j) / (size - 1);
+        double alpha = roughness_to_alpha(roughness);
+        for (int i = 0; i < size; i++) {
+            double mu = std::fmax(double(i) / (size - 1), 1e-3);
+            v3 wo{std::sqrt(1 - mu * mu), 0, mu};
+            double sum = 0;
+            for (int a = 0; a < strata; a++)
+                for (int b = 0; b < strata; b++) {
+                    v3 h = ggx_sample_visible(alpha, wo, (a + 0.5) / strata, (b + 0.5) / strata);
+                    v3 wi = -wo + 2 * dot(wo, h) * h;
+                    if (wi.z > 0) sum += ggx_G(alpha, wo, wi) / ggx_G1(alpha, wo);
+                }
+            e[j * size + i] = std::fmin(sum / (strata * strata), 1.0);
+        }
+        double integral = 0;
+        double h = 1.0 / (size - 1);
+        for (int i = 0; i + 1 < size; i++) {
+            double a = i * h, b = (i + 1) * h;
+            integral += h * (e[j * size + i] * (2 * a + b) + e[j * size + i + 1] * (a + 2 * b)) / 6;
+        }
+        e_avg[j] = std::fmin(2 * integral, 1.0);
+    }
+}
+
+int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
+    auto fail = [&](int code, const std::string& m) {
+        err = m;
+        return code;
+    };
+    if (d.n_prims < 0 || d.n_objects < 0 || d.n_xforms < 0 || d.n_materials < 0 || d.n_textures < 0 ||
+        d.n_lights < 0)
+        return fail(MPT_E_INVALID, "negative count in scene description");
+    if (d.n_prims > 0 && (!d.prim_kind || !d.prim_material || !d.prim_data))
+        return fail(MPT_E_INVALID, "primitive arrays missing");
+    if (d.n_objects > 0 && (!d.object_kind || !d.object_first || !d.object_count))
+        return fail(MPT_E_INVALID, "object arrays missing");
+    if (d.n_prims >= (int64_t(1) << 31))
+        return fail(MPT_E_CAPACITY, "more than 2^31 primitives");
+
+    // materials and textures
+    s.textures.assign(d.textures, d.textures + d.n_textures);
+    for (auto& t : s.textures)
+        if (t.type != MPT_TEX_SOLID && t.type != MPT_TEX_CHECKER)
+            return fail(MPT_E_UNSUPPORTED, "unsupported texture type " + std::to_string(t.type));
+    s.materials.assign(d.materials, d.materials + d.n_materials);
+    for (size_t k = 0; k < s.materials.size(); k++) {
+        auto& m = s.materials[k];
+        if (m.type < MPT_MAT_LAMBERTIAN || m.type > MPT_MAT_PHASE)
+            return fail(MPT_E_UNSUPPORTED, "unsupported material type " + std::to_string(m.type));
+        bool needs_tex = m.type == MPT_MAT_LAMBERTIAN || m.type == MPT_MAT_PRINCIPLED;
+        if (needs_tex && (m.texture < 0 || m.texture >= d.n_textures))
+            return fail(MPT_E_INVALID, "material " + std::to_string(k) + " has no valid texture");
+    }
+    s.xforms.resize(d.n_xforms);
+    for (int k = 0; k < d.n_xforms; k++) {
+        const mpt_xform& x = d.xforms[k];
+        xform& y = s.xforms[k];
+        for (int i = 0; i < 9; i++) {
+            y.m[i] = x.m[i];
+            y.im[i] = x.inv_m[i];
+        }
+        for (int i = 0; i < 3; i++) {
+            y.t[i] = x.t[i];
+            y.it[i] = x.inv_t[i];
+        }
+    }
+
+    // primitives in description order first
+    const int64_t np = d.n_prims;
+    s.prim.assign(size_t(np) * kPrimStride, 0.0);
+    s.prim_kind.assign(d.prim_kind, d.prim_kind + np);
+    s.prim_mat.assign(d.prim_material, d.prim_material + np);
+    s.prim_canon.resize(np);
+    std::vector<aabb> pbox(np);
+    for (int64_t i = 0; i < np; i++) {
+        const double* src = d.prim_data + 9 * i;
+        double* dst = &s.prim[size_t(i) * kPrimStride];
+        int kind = s.prim_kind[i];
+        int mat = s.prim_mat[i];
+        if (mat < -1 || mat >= d.n_materials)
+            return fail(MPT_E_INVALID, "primitive " + std::to_string(i) + " has a bad material index");
+        if (kind == MPT_PRIM_SPHERE) {
+            for (int k = 0; k < 4; k++) dst[k] = src[k];
+            if (src[3] == 0) return fail(MPT_E_INVALID, "sphere with zero radius");
+        } else if (kind == MPT_PRIM_QUAD) {
+            v3 Q{src[0], src[1], src[2]}, u{src[3], src[4], src[5]}, v{src[6], src[7], src[8]};
+            v3 n = cross(u, v);
+            v3 normal = unit_vector(n);
+            double D = dot(normal, Q);
+            v3 w = n / dot(n, n);
+            for (int k = 0; k < 9; k++) dst[k] = src[k];
+            dst[9] = normal.x, dst[10] = normal.y, dst[11] = normal.z;
+            dst[12] = w.x, dst[13] = w.y, dst[14] = w.z;
+            dst[15] = D;
+        } else if (kind == MPT_PRIM_TRIANGLE) {
+            v3 p0{src[0], src[1], src[2]}, p1{src[3], src[4], src[5]}, p2{src[6], src[7], src[8]};
+            v3 e1 = p1 - p0, e2 = p2 - p0;
+            v3 g = unit_vector(cross(e1, e2));
+            dst[0] = p0.x, dst[1] = p0.y, dst[2] = p0.z;
+            dst[3] = e1.x, dst[4] = e1.y, dst[5] = e1.z;
+            dst[6] = e2.x, dst[7] = e2.y, dst[8] = e2.z;
+            dst[9] = g.x, dst[10] = g.y, dst[11] = g.z;
+            dst[12] = 0.5 * length(cross(p1 - p0, p2 - p0));
+        } else {
+            return fail(MPT_E_UNSUPPORTED, "unsupported primitive kind " + std::to_string(kind));
+        }
+        pbox[i] = prim_box(kind, src);
+        s.prim_canon[i] = -1;
+    }
+
+    // objects
+    const int no = d.n_objects;
+    s.objects.resize(no);
+    std::vector<aabb> obox(no);
+    std::map<std::pair<int64_t, int64_t>, int32_t> mesh_roots;
+    s.blas.clear();
+    int64_t next_id = 0;
+    for (int o = 0; o < no; o++) {
+        obj_rec& r = s.objects[o];
+        r.kind = d.object_kind[o];
+        r.first = d.object_first[o];
+        r.count = d.object_count[o];
+        r.xform = d.object_xform ? d.object_xform[o] : -1;
+        r.id_base = d.object_id_base ? d.object_id_base[o] : next_id;
+        r.blas_root = -1;
+        r.material = d.object_material ? d.object_material[o] : -1;
+        r.density = d.object_density ? d.object_density[o] : 0;
+        r.bkind = d.object_boundary_kind ? d.object_boundary_kind[o] : MPT_OBJ_PRIM;
+        r.pad = 0;
+        next_id = r.id_base + r.count;
+        if (r.first < 0 || r.count < 1 || r.first + r.count > np)
+            return fail(MPT_E_INVALID, "object " + std::to_string(o) + " has a bad primitive range");
+        if (r.xform < -1 || r.xform >= d.n_xforms)
+            return fail(MPT_E_INVALID, "object " + std::to_string(o) + " has a bad transform index");
+        aabb b = box_empty();
+        switch (r.kind) {
+            case MPT_OBJ_PRIM:
+                if (r.count != 1) return fail(MPT_E_INVALID, "single-primitive object with count != 1");
+                b = pbox[r.first];
+                break;
+            case MPT_OBJ_LIST:
+                for (int64_t k = 0; k < r.count; k++) {
+                    if (s.prim_kind[r.first + k] == MPT_PRIM_TRIANGLE)
+                        return fail(MPT_E_UNSUPPORTED, "triangles must be in a mesh object");
+                    b = box_merge(b, pbox[r.first + k]);
+                }
+                break;
+            case MPT_OBJ_MESH: {
+                for (int64_t k = 0; k < r.count; k++)
+                    if (s.prim_kind[r.first + k] != MPT_PRIM_TRIANGLE)
+                        return fail(MPT_E_INVALID, "mesh objects hold triangles only");
+                auto key = std::make_pair(r.first, r.count);
+                auto it = mesh_roots.find(key);
+                if (it == mesh_roots.end()) {
+                    std::vector<build_item> items(r.count);
+                    for (int64_t k = 0; k < r.count; k++) {
+                        items[k].box = pbox[r.first + k];
+                        items[k].centroid = box_centroid(items[k].box);
+                        items[k].ref = k;
+                    }
+                    std::vector<bvh_node> nodes;
+                    build_bvh(items, nodes);
+                    int32_t root = int32_t(s.blas.size());
+                    // leaf offsets become absolute positions in the primitive array
+                    for (auto& n : nodes) {
+                        if (n.count > 0) n.offset += int32_t(r.first);
+                        else n.offset += root;
+                    }
+                    s.blas.insert(s.blas.end(), nodes.begin(), nodes.end());
+                    // reorder the mesh's primitives into leaf order
+                    std::vector<double> tmp(size_t(r.count) * kPrimStride);
+                    std::vector<aabb> tbox(r.count);
+                    for (int64_t k = 0; k < r.count; k++) {
+                        int64_t src = r.first + items[k].ref;
+                        std::copy(&s.prim[size_t(src) * kPrimStride], &s.prim[size_t(src) * kPrimStride] + kPrimStride,
+                                  &tmp[size_t(k) * kPrimStride]);
+                        tbox[k] = pbox[src];
+                    }
+                    std::vector<int32_t> tmat(r.count);
+                    for (int64_t k = 0; k < r.count; k++) tmat[k] = s.prim_mat[r.first + items[k].ref];
+                    for (int64_t k = 0; k < r.count; k++) {
+                        std::copy(&tmp[size_t(k) * kPrimStride], &tmp[size_t(k) * kPrimStride] + kPrimStride,
+                                  &s.prim[size_t(r.first + k) * kPrimStride]);
+                        pbox[r.first + k] = tbox[k];
+                        s.prim_mat[r.first + k] = tmat[k];
+                        s.prim_canon[r.first + k] = items[k].ref;
+                    }
+                    it = mesh_roots.emplace(key, root).first;
+                }
+                r.blas_root = it->second;
+                const bvh_node& rn = s.blas[r.blas_root];
+                for (int a = 0; a < 3; a++) b.ax[a] = interval{rn.b[2 * a], rn.b[2 * a + 1]};
+                break;
+            }
+            case MPT_OBJ_MEDIUM:
+                if (r.material < 0 || r.material >= d.n_materials || s.materials[r.material].type != MPT_MAT_PHASE)
+                    return fail(MPT_E_INVALID, "medium needs a volume (phase) material");
+                if (!(r.density > 0)) return fail(MPT_E_INVALID, "medium density must be positive");
+                if (r.bkind == MPT_OBJ_PRIM && r.count != 1)
+                    return fail(MPT_E_INVALID, "medium boundary prim with count != 1");
+                for (int64_t k = 0; k < r.count; k++) {
+                    if (s.prim_kind[r.first + k] == MPT_PRIM_TRIANGLE)
+                        return fail(MPT_E_UNSUPPORTED, "mesh boundaries of media are not supported");
+                    b = box_merge(b, pbox[r.first + k]);
+                }
+                break;
+            default:
+                return fail(MPT_E_UNSUPPORTED, "unsupported object kind " + std::to_string(r.kind));
+        }
+        if (r.xform >= 0) b = instance_box(s.xforms[r.xform], b);
+        obox[o] = b;
+    }
+    for (int64_t i = 0; i < np; i++)
+        if (s.prim_canon[i] < 0) s.prim_canon[i] = 0;  // fixed below for non-mesh ranges
+    for (int o = 0; o < no; o++) {
+        const obj_rec& r = s.objects[o];
+        if (r.kind != MPT_OBJ_MESH)
+            for (int64_t k = 0; k < r.count; k++) s.prim_canon[r.first + k] = k;
+    }
+
+    // top-level BVH over the world list (main.cpp:70, bvh.h:24-43)
+    std::vector<build_item> items(no);
+    for (int o = 0; o < no; o++) {
+        items[o].box = obox[o];
+        items[o].centroid = box_centroid(obox[o]);
+        items[o].ref = o;
+    }
+    build_bvh(items, s.tlas);
+    s.tlas_items.resize(no);
+    for (int o = 0; o < no; o++) s.tlas_items[o] = int32_t(items[o].ref);
+    s.world_box = box_empty();
+    for (int o = 0; o < no; o++) s.world_box = box_merge(s.world_box, obox[o]);
+
+    // environment (environment.h:140-177)
+    s.env_w = d.env_width;
+    s.env_h = d.env_height;
+    s.env_intensity = d.env_intensity;
+    s.env_rotation = d.env_rotation_degrees / 360;
+    if (s.env_w > 0) {
+        if (s.env_h <= 0 || !d.env_rgb) return fail(MPT_E_INVALID, "environment image missing");
+        const int w = s.env_w, h = s.env_h;
+        s.env_rgb.resize(size_t(w) * h * 3);
+        for (size_t k = 0; k < s.env_rgb.size(); k++) s.env_rgb[k] = double(d.env_rgb[k]);
+        auto weight = [&](int i, int j) {
+            double sin_theta = std::sin(kPi * (j + 0.5) / h);
+            const double* p = &s.env_rgb[(size_t(j) * w + i) * 3];
+            return luminance(v3{p[0], p[1], p[2]}) * sin_theta;
+        };
+        s.env_cond_cdf.assign(size_t(h) * (w + 1), 0.0);
+        s.env_marg_cdf.assign(h + 1, 0.0);
+        s.env_row_weight.assign(h, 0.0);
+        for (int j = 0; j < h; j++) {
+            double* cdf = &s.env_cond_cdf[size_t(j) * (w + 1)];
+            for (int i = 0; i < w; i++) cdf[i + 1] = cdf[i] + std::fmax(weight(i, j), 0.0);
+            s.env_row_weight[j] = cdf[w];
+            for (int i = 1; i <= w; i++) cdf[i] = s.env_row_weight[j] > 0 ? cdf[i] / s.env_row_weight[j] : double(i) / w;
+            s.env_marg_cdf[j + 1] = s.env_marg_cdf[j] + s.env_row_weight[j];
+        }
+        s.env_total_weight = s.env_marg_cdf[h];
+        if (s.env_total_weight <= 0) {
+            for (int j = 0; j < h; j++) {
+                s.env_row_weight[j] = std::sin(kPi * (j + 0.5) / h);
+                s.env_marg_cdf[j + 1] = s.env_marg_cdf[j] + s.env_row_weight[j];
+            }
+            s.env_total_weight = s.env_marg_cdf[h];
+        }
+        for (int j = 1; j <= h; j++) s.env_marg_cdf[j] /= s.env_total_weight;
+    }
+
+    // light selection (light_list.h:45-96)
+    const int nl = d.n_lights;
+    s.light_strategy = d.light_strategy;
+    s.lights.resize(nl);
+    for (int k = 0; k < nl; k++) {
+        int ob = d.light_object[k];
+        if (ob < -1 || ob >= no) return fail(MPT_E_INVALID, "light refers to a bad object");
+        if (ob == -1 && s.env_w <= 0) return fail(MPT_E_INVALID, "environment light without an environment");
+        if (ob >= 0) {
+            const obj_rec& r = s.objects[ob];
+            if (r.kind == MPT_OBJ_MEDIUM) return fail(MPT_E_UNSUPPORTED, "media cannot be lights");
+            if (r.kind == MPT_OBJ_MESH) return fail(MPT_E_UNSUPPORTED, "mesh lights are not supported yet");
+        }
+        s.lights[k].object = ob;
+        s.lights[k].pad = 0;
+        s.lights[k].probability = nl > 0 ? 1.0 / nl : 0.0;
+    }
+    if (nl > 0 && s.light_strategy == 1) {
+        double radius = 0;
+        const aabb& sb = s.world_box;
+        double sx = iv_size(sb.ax[0]), sy = iv_size(sb.ax[1]), sz = iv_size(sb.ax[2]);
+        if (sx > 0 && sy > 0 && sz > 0) radius = 0.5 * std::sqrt(sx * sx + sy * sy + sz * sz);
+        if (!std::isfinite(radius) || radius <= 0) radius = 1;
+        auto prim_power = [&](int64_t p) {
+            int m = s.prim_mat[p];
+            if (m < 0) return 0.0;
+            double lum = luminance(v3{s.materials[m].avg_emission[0], s.materials[m].avg_emission[1],
+                                      s.materials[m].avg_emission[2]});
+            const double* q = &s.prim[size_t(p) * kPrimStride];
+            if (s.prim_kind[p] == MPT_PRIM_SPHERE) return kPi * 4 * kPi * q[3] * q[3] * lum;
+            if (s.prim_kind[p] == MPT_PRIM_QUAD) {
+                v3 n = cross(v3{q[3], q[4], q[5]}, v3{q[6], q[7], q[8]});
+                return kPi * length(n) * lum;
+            }
+            return kPi * q[12] * lum;
+        };
+        std::vector<double> power(nl);
+        double known_sum = 0;
+        int known = 0;
+        for (int k = 0; k < nl; k++) {
+            int ob = s.lights[k].object;
+            if (ob < 0) {  // environment_light::power (environment.h:116-127)
+                double sum = 0;
+                for (int j = 0; j < s.env_h; j++)
+                    for (int i = 0; i < s.env_w; i++) {
+                        double sin_theta = std::sin(kPi * (j + 0.5) / s.env_h);
+                        const double* p = &s.env_rgb[(size_t(j) * s.env_w + i) * 3];
+                        sum += std::fmax(luminance(v3{p[0], p[1], p[2]}) * sin_theta, 0.0);
+                    }
+                double integral = s.env_intensity * sum * (2 * kPi / s.env_w) * (kPi / s.env_h);
+                power[k] = kPi * radius * radius * integral;
+            } else {
+                const obj_rec& r = s.objects[ob];
+                double pw = 0;
+                if (r.kind == MPT_OBJ_PRIM) pw = prim_power(r.first);
+                else
+                    for (int64_t i = 0; i < r.count; i++) pw += prim_power(r.first + i);
+                if (r.xform >= 0) pw = pw * area_scale(s.xforms[r.xform]);
+                power[k] = pw;
+            }
+            if (std::isfinite(power[k]) && power[k] > 0) {
+                known_sum += power[k];
+                known++;
+            }
+        }
+        double fallback = known > 0 ? known_sum / known : 1.0;
+        double total = 0;
+        for (auto& p : power) {
+            if (!std::isfinite(p) || p <= 0) p = fallback;
+            total += p;
+        }
+        double uf = d.light_uniform_fraction;
+        for (int k = 0; k < nl; k++) s.lights[k].probability = (1 - uf) * power[k] / total + uf / double(nl);
+    }
+    s.light_cdf.assign(nl + 1, 0.0);
+    for (int k = 0; k < nl; k++) s.light_cdf[k + 1] = s.light_cdf[k] + s.lights[k].probability;
+    if (nl > 0) s.light_cdf[nl] = 1;
+
+    build_ggx_table(s.ggx_e, s.ggx_e_avg);
+    return MPT_OK;
+}
+
+}  // namespace mpt

@@ -1,0 +1,100 @@
+// scene_host.hpp - host-side form of a flattened scene, after validation:
+// primitives in the layout the kernels read, BVHs, light tables and the GGX
+// albedo table. Built once per mpt_scene_create and then copied to HBM.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mpt.h"
+#include "geom.hpp"
+
+namespace mpt {
+
+// One BVH node: 64 bytes, box as (xmin, xmax, ymin, ymax, zmin, zmax) in
+// double like the reference node (bvh.h:93-98); offset is the first item of a
+// leaf or the right child of an interior node; count is 0 for interior nodes.
+struct bvh_node {
+    double b[6];
+    int32_t offset;
+    int32_t count;
+    int32_t axis;
+    int32_t pad;
+};
+static_assert(sizeof(bvh_node) == 64, "bvh_node must be 64 bytes");
+
+// Primitive record the kernels read: 16 doubles (128 bytes).
+//   sphere:   [0..2] center, [3] radius
+//   quad:     [0..2] Q, [3..5] u, [6..8] v, [9..11] unit normal, [12..14] w, [15] D
+//   triangle: [0..2] p0, [3..5] e1 = p1 - p0, [6..8] e2 = p2 - p0,
+//             [9..11] unit geometric normal, [12] area
+constexpr int kPrimStride = 16;
+
+// Object record (one world-list entry).
+struct obj_rec {
+    int32_t kind;        // MPT_OBJ_*
+    int32_t xform;       // -1 or index
+    int64_t first;       // first primitive (device order: BLAS leaf order for meshes)
+    int64_t count;
+    int64_t id_base;     // first canonical ID
+    int32_t blas_root;   // meshes: root node in the BLAS node array
+    int32_t material;    // media: phase material
+    double density;      // media
+    int32_t bkind;       // media: boundary kind (MPT_OBJ_PRIM or MPT_OBJ_LIST)
+    int32_t pad;
+};
+
+struct light_rec {
+    int32_t object;      // -1 = environment
+    int32_t pad;
+    double probability;
+};
+
+struct host_scene {
+    // primitives in device order
+    std::vector<double> prim;            // n * kPrimStride
+    std::vector<int32_t> prim_kind;
+    std::vector<int32_t> prim_mat;
+    std::vector<int64_t> prim_canon;     // canonical local index within its object range
+    std::vector<obj_rec> objects;
+    std::vector<bvh_node> tlas;
+    std::vector<int32_t> tlas_items;     // object indices in leaf order
+    std::vector<bvh_node> blas;          // all mesh BVHs back to back
+    std::vector<xform> xforms;
+    std::vector<mpt_material> materials;
+    std::vector<mpt_texture> textures;
+    std::vector<light_rec> lights;
+    std::vector<double> light_cdf;       // n_lights + 1
+    int32_t light_strategy = 1;
+    // environment (environment.h)
+    int32_t env_w = 0, env_h = 0;
+    std::vector<double> env_rgb;         // w*h*3
+    std::vector<double> env_cond_cdf;    // h*(w+1)
+    std::vector<double> env_marg_cdf;    // h+1
+    std::vector<double> env_row_weight;  // h
+    double env_total_weight = 0;
+    double env_intensity = 1;
+    double env_rotation = 0;             // turns
+    // GGX albedo table (microfacet.h:265-338)
+    std::vector<double> ggx_e;           // 32*32 [roughness][mu]
+    std::vector<double> ggx_e_avg;       // 32
+    aabb world_box;
+};
+
+// Validates the description and builds the host scene. Returns MPT_OK or an
+// error code with a message in err.
+int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err);
+
+// BVH build of bvh.h:118-205 over items; reorders items into leaf order.
+struct build_item {
+    aabb box;
+    v3 centroid;
+    int64_t ref;
+};
+void build_bvh(std::vector<build_item>& items, std::vector<bvh_node>& nodes);
+
+// ggx_albedo_table (microfacet.h:299-337).
+void build_ggx_table(std::vector<double>& e, std::vector<double>& e_avg);
+
+}  // namespace mpt

@@ -1,0 +1,1842 @@
+// oracle.cpp - CPU restatement of the reference renderer's hot path.
+//
+// TEST INFRASTRUCTURE. This is the checker the GPU path is compared with,
+// and the CPU baseline the benchmark times beside it. Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg load it; the
+// product library never does.
+//
+// It takes the same flattened scene description as the GPU core
+// (include/mpt.h, mpt_scene_desc) and restates, in double precision and in
+// the reference's object-oriented shape, what adriendjn/mypbr computes:
+//   bvh build + hit         include/bvh.h:22-205, aabb.h:40-88
+//   sphere / quad / tri     sphere.h:23-132, quad.h:13-79, mesh.h:47-139
+//   instance, list          instance.h:29-56, hittable_list.h:25-62
+//   constant_medium         medium.h:60-88
+//   materials               material.h:74-264, 346-615, medium.h:13-39
+//   microfacet / pdfs       microfacet.h:17-470, pdf.h:21-134
+//   light_list, environment light_list.h:45-127, environment.h:66-195
+//   camera                  camera.h:79-529
+// Random numbers come from the counter-based stream the GPU path uses (a
+// path's i-th draw is SplitMix64(key + i*gamma)), drawn in the same order,
+// so a seeded CPU render and a seeded GPU render trace the same paths.
+//
+// Pinning (what checks this restatement, besides the GPU comparison):
+//   - golden vectors written by the compiled reference (tests/golden):
+//     closest-hit IDs and t on seeded ray batches, the GGX albedo table,
+//     light selection probabilities, material eval/pdf values;
+//   - closed forms: white furnace, pdf normalisation, Beer-Lambert slab;
+//   - brute force: BVH closest hit == linear scan over all primitives.
+// Built with -ffp-contract=off so its arithmetic is the reference's.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../include/mpt.h"
+
+namespace orc {
+
+const double pi = 3.1415926535897932385;
+const double infinity = __builtin_huge_val();
+
+// ------------------------------------------------------------------ math (vec3.h, onb.h)
+struct vec3 {
+    double e[3];
+    vec3() : e{0, 0, 0} {}
+    vec3(double x, double y, double z) : e{x, y, z} {}
+    double x() const { return e[0]; }
+    double y() const { return e[1]; }
+    double z() const { return e[2]; }
+    double operator[](int i) const { return e[i]; }
+    double& operator[](int i) { return e[i]; }
+    vec3 operator-() const { return vec3(-e[0], -e[1], -e[2]); }
+    double length_squared() const { return e[0] * e[0] + e[1] * e[1] + e[2] * e[2]; }
+    double length() const { return std::sqrt(length_squared()); }
+};
+using point3 = vec3;
+using color = vec3;
+inline vec3 operator+(const vec3& a, const vec3& b) { return vec3(a[0] + b[0], a[1] + b[1], a[2] + b[2]); }
+inline vec3 operator-(const vec3& a, const vec3& b) { return vec3(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
+inline vec3 operator*(const vec3& a, const vec3& b) { return vec3(a[0] * b[0], a[1] * b[1], a[2] * b[2]); }
+inline vec3 operator*(double t, const vec3& a) { return vec3(t * a[0], t * a[1], t * a[2]); }
+inline vec3 operator*(const vec3& a, double t) { return vec3(t * a[0], t * a[1], t * a[2]); }
+inline vec3 operator/(const vec3& a, double t) { return (1 / t) * a; }
+inline double dot(const vec3& a, const vec3& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline vec3 cross(const vec3& u, const vec3& v) {
+    return vec3(u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]);
+}
+inline vec3 unit_vector(const vec3& v) { return v / v.length(); }
+inline vec3 reflect(const vec3& v, const vec3& n) { return v - 2 * dot(v, n) * n; }
+inline vec3 refract(const vec3& uv, const vec3& n, double etai_over_etat) {
+    double cos_theta = std::fmin(dot(-uv, n), 1.0);
+    vec3 perp = etai_over_etat * (uv + cos_theta * n);
+    vec3 par = -std::sqrt(std::fabs(1.0 - perp.length_squared())) * n;
+    return perp + par;
+}
+inline double luminance(const color& c) { return 0.2126 * c.x() + 0.7152 * c.y() + 0.0722 * c.z(); }
+inline double dclamp(double x, double lo, double hi) { return x < lo ? lo : (hi < x ? hi : x); }
+
+struct onb {
+    vec3 axis[3];
+    explicit onb(const vec3& n) {
+        axis[2] = unit_vector(n);
+        vec3 a = (std::fabs(axis[2].x()) > 0.9) ? vec3(0, 1, 0) : vec3(1, 0, 0);
+        axis[1] = unit_vector(cross(axis[2], a));
+        axis[0] = cross(axis[2], axis[1]);
+    }
+    const vec3& w() const { return axis[2]; }
+    vec3 transform(const vec3& v) const { return (v[0] * axis[0]) + (v[1] * axis[1]) + (v[2] * axis[2]); }
+    vec3 to_local(const vec3& v) const { return vec3(dot(v, axis[0]), dot(v, axis[1]), dot(v, axis[2])); }
+};
+
+// ------------------------------------------------------------------ random numbers
+inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+const uint64_t gamma64 = 0x9E3779B97F4A7C15ull;
+inline double to_unit(uint64_t x) { return double(x >> 11) * 0x1.0p-53; }
+inline uint64_t key_of(uint64_t seed, uint32_t scene, uint32_t sample, uint32_t pixel) {
+    uint64_t s = mix64(seed + uint64_t(scene) * gamma64);
+    return mix64(s ^ ((uint64_t(sample) << 32) | pixel));
+}
+struct sampler {
+    uint64_t key = 0;
+    uint32_t count = 0;
+    double next() { return to_unit(mix64(key + uint64_t(++count) * gamma64)); }
+    double next(double lo, double hi) { return lo + (hi - lo) * next(); }
+};
+inline double medium_u(uint64_t key, uint32_t serial, uint32_t obj) {
+    uint64_t k = mix64(key ^ 0x6A09E667F3BCC909ull);
+    return to_unit(mix64(k + ((uint64_t(serial) << 32) | obj) * gamma64));
+}
+
+// ------------------------------------------------------------------ rays, intervals, boxes
+struct ray {
+    point3 orig;
+    vec3 dir;
+    ray() {}
+    ray(const point3& o, const vec3& d) : orig(o), dir(d) {}
+    point3 at(double t) const { return orig + t * dir; }
+};
+struct interval {
+    double min = +infinity, max = -infinity;
+    interval() {}
+    interval(double a, double b) : min(a), max(b) {}
+    interval(const interval& a, const interval& b) : min(a.min <= b.min ? a.min : b.min), max(a.max >= b.max ? a.max : b.max) {}
+    double size() const { return max - min; }
+    bool contains(double x) const { return min <= x && x <= max; }
+    bool surrounds(double x) const { return min < x && x < max; }
+    interval expand(double d) const { return interval(min - d / 2, max + d / 2); }
+};
+struct aabb {
+    interval ax[3];
+    aabb() {}
+    aabb(const point3& a, const point3& b) {
+        for (int k = 0; k < 3; k++) ax[k] = a[k] <= b[k] ? interval(a[k], b[k]) : interval(b[k], a[k]);
+        for (int k = 0; k < 3; k++)
+            if (ax[k].size() < 0.0001) ax[k] = ax[k].expand(0.0001);
+    }
+    aabb(const aabb& a, const aabb& b) {
+        for (int k = 0; k < 3; k++) ax[k] = interval(a.ax[k], b.ax[k]);
+    }
+    bool hit(const point3& o, const vec3& inv, interval t) const {
+        for (int a = 0; a < 3; a++) {
+            double t0 = (ax[a].min - o[a]) * inv[a];
+            double t1 = (ax[a].max - o[a]) * inv[a];
+            if (t0 > t1) std::swap(t0, t1);
+            if (t0 > t.min) t.min = t0;
+            if (t1 < t.max) t.max = t1;
+            if (t.max <= t.min) return false;
+        }
+        return true;
+    }
+    point3 centroid() const {
+        return point3((ax[0].min + ax[0].max) / 2, (ax[1].min + ax[1].max) / 2, (ax[2].min + ax[2].max) / 2);
+    }
+    double area() const {
+        double dx = ax[0].size(), dy = ax[1].size(), dz = ax[2].size();
+        return 2 * (dx * dy + dy * dz + dz * dx);
+    }
+    int longest_axis() const {
+        if (ax[0].size() > ax[1].size()) return ax[0].size() > ax[2].size() ? 0 : 2;
+        return ax[1].size() > ax[2].size() ? 1 : 2;
+    }
+};
+
+// ------------------------------------------------------------------ hits
+struct material;
+struct hit_record {
+    point3 p;
+    vec3 normal;
+    const material* mat = nullptr;
+    double t = 0;
+    bool front_face = false;
+    int64_t id = -1;
+    void set_face_normal(const ray& r, const vec3& outward) {
+        front_face = dot(r.dir, outward) < 0;
+        normal = front_face ? outward : -outward;
+    }
+};
+
+// Draws a hit() makes (constant_medium) come from the ray's own stream.
+struct hit_ctx {
+    uint64_t key = 0;
+    uint32_t serial = 0;
+    bool media = true;
+};
+
+struct hittable {
+    virtual ~hittable() = default;
+    virtual bool hit(const ray& r, interval t, hit_record& rec, const hit_ctx& c) const = 0;
+    virtual aabb bounding_box() const = 0;
+    virtual double pdf_value(const point3&, const vec3&) const { return 0; }
+    virtual vec3 random(const point3&, sampler&) const { return vec3(1, 0, 0); }
+    virtual double power() const { return 0; }
+};
+
+// ------------------------------------------------------------------ materials
+struct pdf {
+    virtual ~pdf() = default;
+    virtual double value(const vec3& d) const = 0;
+    virtual vec3 generate(sampler& s) const = 0;
+};
+
+struct scatter_record {
+    color attenuation;
+    std::unique_ptr<pdf> pdf_ptr;
+    bool skip_pdf = false;
+    ray skip_pdf_ray;
+};
+
+struct ggx_table {  // microfacet.h:265-338, built once per scene
+    double e[32][32];
+    double e_avg[32];
+};
+
+struct material {
+    const ggx_table* table = nullptr;
+    virtual ~material() = default;
+    virtual bool scatter(const ray&, const hit_record&, scatter_record&, sampler&) const { return false; }
+    virtual color eval(const ray&, const hit_record&, const vec3&) const { return color(0, 0, 0); }
+    virtual color emitted(const ray&, const hit_record&) const { return color(0, 0, 0); }
+    virtual color interior_transmittance(const ray&, const hit_record&) const { return color(1, 1, 1); }
+    virtual color average_emission() const { return color(0, 0, 0); }
+    virtual bool lets_shadows_through() const { return false; }
+    virtual bool is_volume() const { return false; }
+};
+
+// microfacet.h
+double roughness_to_alpha(double r) { return std::max(r * r, 1e-3); }
+double fresnel_schlick(double c, double f0) { return f0 + (1 - f0) * std::pow(1 - dclamp(c, 0.0, 1.0), 5); }
+color fresnel_schlick(double c, const color& f0) {
+    return color(fresnel_schlick(c, f0.x()), fresnel_schlick(c, f0.y()), fresnel_schlick(c, f0.z()));
+}
+double fresnel_dielectric(double cos_i, double eta) {
+    cos_i = dclamp(cos_i, -1.0, 1.0);
+    if (cos_i < 0) {
+        eta = 1 / eta;
+        cos_i = -cos_i;
+    }
+    double sin2_i = 1 - cos_i * cos_i;
+    double sin2_t = sin2_i / (eta * eta);
+    if (sin2_t >= 1) return 1;
+    double cos_t = std::sqrt(1 - sin2_t);
+    double r_parl = (eta * cos_i - cos_t) / (eta * cos_i + cos_t);
+    double r_perp = (cos_i - eta * cos_t) / (cos_i + eta * cos_t);
+    return (r_parl * r_parl + r_perp * r_perp) / 2;
+}
+struct ggx {
+    double alpha;
+    explicit ggx(double a) : alpha(a) {}
+    double D(const vec3& h) const {
+        if (h.z() <= 0) return 0;
+        double a2 = alpha * alpha;
+        double d = h.z() * h.z() * (a2 - 1) + 1;
+        return a2 / (pi * d * d);
+    }
+    double lambda(const vec3& w) const {
+        double cos2 = w.z() * w.z();
+        if (cos2 <= 0) return infinity;
+        double tan2 = std::fmax(0.0, 1 - cos2) / cos2;
+        return (-1 + std::sqrt(1 + alpha * alpha * tan2)) / 2;
+    }
+    double G1(const vec3& w) const { return 1 / (1 + lambda(w)); }
+    double G(const vec3& wo, const vec3& wi) const { return 1 / (1 + lambda(wo) + lambda(wi)); }
+    double visible_pdf(const vec3& wo, const vec3& h) const {
+        if (wo.z() <= 0) return 0;
+        return G1(wo) / wo.z() * D(h) * std::fabs(dot(wo, h));
+    }
+    vec3 sample_visible(const vec3& wo, double u1, double u2) const {
+        vec3 vh = unit_vector(vec3(alpha * wo.x(), alpha * wo.y(), wo.z()));
+        double lensq = vh.x() * vh.x() + vh.y() * vh.y();
+        vec3 t1 = lensq > 0 ? vec3(-vh.y(), vh.x(), 0) / std::sqrt(lensq) : vec3(1, 0, 0);
+        vec3 t2 = cross(vh, t1);
+        double r = std::sqrt(u1);
+        double phi = 2 * pi * u2;
+        double p1 = r * std::cos(phi);
+        double p2 = r * std::sin(phi);
+        double s = 0.5 * (1 + vh.z());
+        p2 = (1 - s) * std::sqrt(std::fmax(0.0, 1 - p1 * p1)) + s * p2;
+        vec3 nh = p1 * t1 + p2 * t2 + std::sqrt(std::fmax(0.0, 1 - p1 * p1 - p2 * p2)) * vh;
+        return unit_vector(vec3(alpha * nh.x(), alpha * nh.y(), std::fmax(1e-6, nh.z())));
+    }
+};
+
+void build_table(ggx_table& t) {
+    const int size = 32, strata = 64;
+    for (int j = 0; j < size; j++) {
+        ggx dist(roughness_to_alpha(double(j) / (size - 1)));
+        for (int i = 0; i < size; i++) {
+            double mu = std::fmax(double(i) / (size - 1), 1e-3);
+            vec3 wo(std::sqrt(1 - mu * mu), 0, mu);
+            double sum = 0;
+            for (int a = 0; a < strata; a++)
+                for (int b = 0; b < strata; b++) {
+                    vec3 h = dist.sample_visible(wo, (a + 0.5) / strata, (b + 0.5) / strata);
+                    vec3 wi = -wo + 2 * dot(wo, h) * h;
+                    if (wi.z() > 0) sum += dist.G(wo, wi) / dist.G1(wo);
+                }
+            t.e[j][i] = std::fmin(sum / (strata * strata), 1.0);
+        }
+        double integral = 0, h = 1.0 / (size - 1);
+        for (int i = 0; i + 1 < size; i++) {
+            double a = i * h, b = (i + 1) * h;
+            integral += h * (t.e[j][i] * (2 * a + b) + t.e[j][i + 1] * (a + 2 * b)) / 6;
+        }
+        t.e_avg[j] = std::fmin(2 * integral, 1.0);
+    }
+}
+double table_E(const ggx_table& t, double c, double alpha) {
+    double x = dclamp(c, 0.0, 1.0) * 31, y = dclamp(std::sqrt(alpha), 0.0, 1.0) * 31;
+    int i = std::min(int(x), 30), j = std::min(int(y), 30);
+    double tx = x - i, ty = y - j;
+    double lo = (1 - tx) * t.e[j][i] + tx * t.e[j][i + 1];
+    double hi = (1 - tx) * t.e[j + 1][i] + tx * t.e[j + 1][i + 1];
+    return (1 - ty) * lo + ty * hi;
+}
+double table_E_avg(const ggx_table& t, double alpha) {
+    double y = dclamp(std::sqrt(alpha), 0.0, 1.0) * 31;
+    int j = std::min(int(y), 30);
+    double ty = y - j;
+    return (1 - ty) * t.e_avg[j] + ty * t.e_avg[j + 1];
+}
+double ms_eval(const ggx_table& t, double co, double ci, double alpha) {
+    if (co <= 0 || ci <= 0) return 0;
+    double missing = 1 - table_E_avg(t, alpha);
+    if (missing < 1e-6) return 0;
+    return (1 - table_E(t, co, alpha)) * (1 - table_E(t, ci, alpha)) / (pi * missing) * ci;
+}
+double ms_albedo(const ggx_table& t, double co, double alpha) {
+    if (co <= 0 || 1 - table_E_avg(t, alpha) < 1e-6) return 0;
+    return 1 - table_E(t, co, alpha);
+}
+color ms_fresnel_avg(const ggx_table& t, const color& f_avg, double alpha) {
+    double ea = table_E_avg(t, alpha);
+    color f;
+    for (int k = 0; k < 3; k++) f[k] = f_avg[k] * f_avg[k] * ea / (1 - f_avg[k] * (1 - ea));
+    return f;
+}
+color ms_fresnel(const ggx_table& t, const color& f0, double alpha) {
+    return ms_fresnel_avg(t, f0 + (color(1, 1, 1) - f0) / 21, alpha);
+}
+double average(const color& c) { return (c.x() + c.y() + c.z()) / 3; }
+
+// pdf.h
+struct cosine_pdf : pdf {
+    onb uvw;
+    explicit cosine_pdf(const vec3& w) : uvw(w) {}
+    double value(const vec3& d) const override { return std::fmax(0, dot(unit_vector(d), uvw.w()) / pi); }
+    vec3 generate(sampler& s) const override {
+        double r1 = s.next();
+        double r2 = s.next();
+        double phi = 2 * pi * r1;
+        return uvw.transform(vec3(std::cos(phi) * std::sqrt(r2), std::sin(phi) * std::sqrt(r2), std::sqrt(1 - r2)));
+    }
+};
+struct hg_pdf : pdf {
+    onb uvw;
+    double g;
+    hg_pdf(const vec3& w, double g) : uvw(w), g(g) {}
+    static double phase(double c, double g) {
+        double denom = 1 + g * g - 2 * g * c;
+        return (1 - g * g) / (4 * pi * denom * std::sqrt(denom));
+    }
+    double value(const vec3& d) const override { return phase(dot(unit_vector(d), uvw.w()), g); }
+    vec3 generate(sampler& s) const override {
+        double xi = s.next();
+        double c;
+        if (std::fabs(g) < 1e-3) c = 1 - 2 * xi;
+        else {
+            double q = (1 - g * g) / (1 - g + 2 * g * xi);
+            c = (1 + g * g - q * q) / (2 * g);
+        }
+        c = dclamp(c, -1.0, 1.0);
+        double st = std::sqrt(1 - c * c);
+        double phi = 2 * pi * s.next();
+        return uvw.transform(vec3(std::cos(phi) * st, std::sin(phi) * st, c));
+    }
+};
+struct ggx_reflection_pdf : pdf {
+    onb frame;
+    vec3 wo;
+    ggx dist;
+    ggx_reflection_pdf(const vec3& n, const vec3& wo_world, double alpha)
+        : frame(n), wo(unit_vector(frame.to_local(wo_world))), dist(alpha) {}
+    double value(const vec3& d) const override {
+        if (d.length_squared() == 0) return 0;
+        vec3 wi = unit_vector(frame.to_local(d));
+        if (wi.z() <= 0 || wo.z() <= 0) return 0;
+        vec3 h = unit_vector(wo + wi);
+        return dist.visible_pdf(wo, h) / (4 * std::fabs(dot(wo, h)));
+    }
+    vec3 generate(sampler& s) const override {
+        if (wo.z() <= 0) return vec3(0, 0, 0);
+        double u1 = s.next();
+        double u2 = s.next();
+        vec3 h = dist.sample_visible(wo, u1, u2);
+        vec3 wi = -wo + 2 * dot(wo, h) * h;
+        if (wi.z() <= 0) return vec3(0, 0, 0);
+        return frame.transform(wi);
+    }
+};
+struct mixture_pdf : pdf {
+    std::unique_ptr<pdf> p0, p1;
+    double w;
+    mixture_pdf(std::unique_ptr<pdf> a, std::unique_ptr<pdf> b, double w) : p0(std::move(a)), p1(std::move(b)), w(w) {}
+    double value(const vec3& d) const override { return w * p0->value(d) + (1 - w) * p1->value(d); }
+    vec3 generate(sampler& s) const override { return s.next() < w ? p0->generate(s) : p1->generate(s); }
+};
+struct rough_glass {  // microfacet.h:384-470
+    onb frame;
+    vec3 wo;
+    double eta;
+    ggx dist;
+    rough_glass(const vec3& n, const vec3& wo_world, double eta, double alpha)
+        : frame(n), wo(unit_vector(frame.to_local(wo_world))), eta(eta), dist(alpha) {}
+    bool half_vector(const vec3& wiw, vec3& wi, vec3& wm, bool& refl) const {
+        if (wiw.length_squared() == 0 || wo.z() <= 0) return false;
+        wi = unit_vector(frame.to_local(wiw));
+        if (wi.z() == 0) return false;
+        refl = wi.z() > 0;
+        wm = refl ? wo + wi : wi * eta + wo;
+        if (wm.length_squared() == 0) return false;
+        wm = unit_vector(wm);
+        if (wm.z() < 0) wm = -wm;
+        if (dot(wm, wi) * wi.z() < 0 || dot(wm, wo) * wo.z() < 0) return false;
+        return true;
+    }
+    double eval(const vec3& wiw) const {
+        vec3 wi, wm;
+        bool refl;
+        if (!half_vector(wiw, wi, wm, refl)) return 0;
+        double F = fresnel_dielectric(dot(wo, wm), eta);
+        if (refl) return F * dist.D(wm) * dist.G(wo, wi) / (4 * wo.z());
+        double q = dot(wi, wm) + dot(wo, wm) / eta;
+        double denom = q * q;
+        double ft = (1 - F) * dist.D(wm) * dist.G(wo, wi) * std::fabs(dot(wi, wm) * dot(wo, wm) / (wi.z() * wo.z() * denom));
+        ft /= eta * eta;
+        return ft * std::fabs(wi.z());
+    }
+    double pdf_value(const vec3& wiw) const {
+        vec3 wi, wm;
+        bool refl;
+        if (!half_vector(wiw, wi, wm, refl)) return 0;
+        double F = fresnel_dielectric(dot(wo, wm), eta);
+        if (refl) return F * dist.visible_pdf(wo, wm) / (4 * std::fabs(dot(wo, wm)));
+        double q = dot(wi, wm) + dot(wo, wm) / eta;
+        double denom = q * q;
+        return (1 - F) * dist.visible_pdf(wo, wm) * std::fabs(dot(wi, wm)) / denom;
+    }
+    vec3 sample(sampler& s) const {
+        if (wo.z() <= 0) return vec3(0, 0, 0);
+        double u1 = s.next();
+        double u2 = s.next();
+        vec3 wm = dist.sample_visible(wo, u1, u2);
+        double F = fresnel_dielectric(dot(wo, wm), eta);
+        vec3 wi;
+        if (s.next() < F) {
+            wi = -wo + 2 * dot(wo, wm) * wm;
+            if (wi.z() <= 0) return vec3(0, 0, 0);
+        } else {
+            wi = refract(-wo, wm, 1 / eta);
+            if (wi.z() >= 0) return vec3(0, 0, 0);
+        }
+        return frame.transform(wi);
+    }
+};
+struct rough_glass_pdf : pdf {
+    rough_glass g;
+    explicit rough_glass_pdf(const rough_glass& g) : g(g) {}
+    double value(const vec3& d) const override { return g.pdf_value(d); }
+    vec3 generate(sampler& s) const override { return g.sample(s); }
+};
+
+struct texture_t {
+    int type;
+    double inv_scale;
+    color even, odd;
+    color value(const point3& p) const {
+        if (type == MPT_TEX_SOLID) return even;
+        int x = int(std::floor(inv_scale * p.x())), y = int(std::floor(inv_scale * p.y())),
+            z = int(std::floor(inv_scale * p.z()));
+        return (x + y + z) % 2 == 0 ? even : odd;
+    }
+};
+
+struct lambertian : material {  // material.h:74-101
+    texture_t tex;
+    bool scatter(const ray&, const hit_record& rec, scatter_record& s, sampler&) const override {
+        s.attenuation = tex.value(rec.p);
+        s.pdf_ptr = std::make_unique<cosine_pdf>(rec.normal);
+        s.skip_pdf = false;
+        return true;
+    }
+    color eval(const ray&, const hit_record& rec, const vec3& d) const override {
+        double c = dot(rec.normal, unit_vector(d));
+        return tex.value(rec.p) * (c < 0 ? 0 : c / pi);
+    }
+};
+
+struct metal : material {  // material.h:109-173 (Schlick)
+    color albedo, f_avg;
+    double roughness, alpha;
+    bool scatter(const ray& r, const hit_record& rec, scatter_record& s, sampler&) const override {
+        if (roughness == 0) {
+            s.attenuation = albedo;
+            s.skip_pdf = true;
+            s.skip_pdf_ray = ray(rec.p, reflect(unit_vector(r.dir), rec.normal));
+            return true;
+        }
+        s.attenuation = albedo;
+        auto spec = std::make_unique<ggx_reflection_pdf>(rec.normal, -r.dir, alpha);
+        s.skip_pdf = false;
+        double cos_o = std::fmax(0.0, dot(rec.normal, unit_vector(-r.dir)));
+        double ms_w = ms_albedo(*table, cos_o, alpha) * average(ms_fresnel_avg(*table, f_avg, alpha));
+        double ss_w = table_E(*table, cos_o, alpha) * average(fresnel_schlick(cos_o, albedo));
+        if (ms_w <= 0) {
+            s.pdf_ptr = std::move(spec);
+            return true;
+        }
+        double p_spec = dclamp(ss_w / (ss_w + ms_w), 0.1, 1.0);
+        s.pdf_ptr = std::make_unique<mixture_pdf>(std::move(spec), std::make_unique<cosine_pdf>(rec.normal), p_spec);
+        return true;
+    }
+    color eval(const ray& r, const hit_record& rec, const vec3& d) const override {
+        onb f(rec.normal);
+        vec3 wo = unit_vector(f.to_local(-r.dir));
+        vec3 wi = unit_vector(f.to_local(d));
+        if (wo.z() <= 0 || wi.z() <= 0) return color(0, 0, 0);
+        vec3 h = unit_vector(wo + wi);
+        ggx dist(alpha);
+        return fresnel_schlick(dot(wi, h), albedo) * (dist.D(h) * dist.G(wo, wi) / (4 * wo.z())) +
+               ms_fresnel_avg(*table, f_avg, alpha) * ms_eval(*table, wo.z(), wi.z(), alpha);
+    }
+};
+
+struct dielectric : material {  // material.h:184-264
+    double ior, roughness, alpha;
+    color sigma_a;
+    color interior_transmittance(const ray& r, const hit_record& rec) const override {
+        if (rec.front_face || (sigma_a.x() == 0 && sigma_a.y() == 0 && sigma_a.z() == 0)) return color(1, 1, 1);
+        double d = rec.t * r.dir.length();
+        return color(std::exp(-sigma_a.x() * d), std::exp(-sigma_a.y() * d), std::exp(-sigma_a.z() * d));
+    }
+    bool lets_shadows_through() const override { return roughness == 0; }
+    rough_glass bsdf(const ray& r, const hit_record& rec) const {
+        return rough_glass(rec.normal, -r.dir, rec.front_face ? ior : 1.0 / ior, alpha);
+    }
+    bool scatter(const ray& r, const hit_record& rec, scatter_record& s, sampler& smp) const override {
+        if (roughness > 0) {
+            s.attenuation = color(1, 1, 1);
+            s.pdf_ptr = std::make_unique<rough_glass_pdf>(bsdf(r, rec));
+            s.skip_pdf = false;
+            return true;
+        }
+        s.attenuation = color(1, 1, 1);
+        s.skip_pdf = true;
+        double ri = rec.front_face ? (1.0 / ior) : ior;
+        vec3 ud = unit_vector(r.dir);
+        double cos_theta = std::fmin(dot(-ud, rec.normal), 1.0);
+        double sin_theta = std::sqrt(1.0 - cos_theta * cos_theta);
+        bool cannot_refract = ri * sin_theta > 1.0;
+        vec3 d;
+        if (cannot_refract || fresnel_dielectric(cos_theta, 1.0 / ri) > smp.next()) d = reflect(ud, rec.normal);
+        else d = refract(ud, rec.normal, ri);
+        s.skip_pdf_ray = ray(rec.p, d);
+        return true;
+    }
+    color eval(const ray& r, const hit_record& rec, const vec3& d) const override {
+        if (roughness == 0) return color(0, 0, 0);
+        double f = bsdf(r, rec).eval(d);
+        return color(f, f, f);
+    }
+};
+
+struct principled : material {  // material.h:346-552 (base layers only)
+    texture_t base;
+    double metallic, alpha, f0;
+    color spec_f0(const color& b) const { return (1 - metallic) * color(f0, f0, f0) + metallic * b; }
+    bool scatter(const ray& r, const hit_record& rec, scatter_record& s, sampler&) const override {
+        vec3 wo = unit_vector(-r.dir);
+        color b = base.value(rec.p);
+        double cos_o = std::fmax(0.0, dot(rec.normal, wo));
+        double spec_w = average(fresnel_schlick(cos_o, spec_f0(b)));
+        double diff_w = (1 - metallic) * average(b) * (1 - fresnel_schlick(cos_o, f0));
+        diff_w += ms_albedo(*table, cos_o, alpha) * average(ms_fresnel(*table, spec_f0(b), alpha));
+        double p_spec = spec_w + diff_w > 0 ? spec_w / (spec_w + diff_w) : 1;
+        p_spec = dclamp(p_spec, 0.1, 1.0);
+        s.pdf_ptr = std::make_unique<mixture_pdf>(std::make_unique<ggx_reflection_pdf>(rec.normal, wo, alpha),
+                                                  std::make_unique<cosine_pdf>(rec.normal), p_spec);
+        s.attenuation = b;
+        s.skip_pdf = false;
+        return true;
+    }
+    color eval(const ray& r, const hit_record& rec, const vec3& d) const override {
+        if (dot(d, rec.normal) <= 0) return color(0, 0, 0);
+        vec3 wo_w = unit_vector(-r.dir), wi_w = unit_vector(d);
+        onb f(rec.normal);
+        vec3 wo = unit_vector(f.to_local(wo_w)), wi = unit_vector(f.to_local(wi_w));
+        if (wo.z() <= 0 || wi.z() <= 0) return color(0, 0, 0);
+        color b = base.value(rec.p);
+        vec3 h = unit_vector(wo + wi);
+        ggx dist(alpha);
+        color specular = fresnel_schlick(dot(wi, h), spec_f0(b)) * (dist.D(h) * dist.G(wo, wi) / (4 * wo.z())) +
+                         ms_fresnel(*table, spec_f0(b), alpha) * ms_eval(*table, wo.z(), wi.z(), alpha);
+        double transmitted = (1 - fresnel_schlick(wi.z(), f0)) * (1 - fresnel_schlick(wo.z(), f0));
+        color diffuse = ((1 - metallic) * transmitted * wi.z() / pi) * b;
+        return specular + diffuse;
+    }
+};
+
+struct diffuse_light : material {  // material.h:568-615
+    color emit, avg;
+    double spread, tan_half;
+    color emitted(const ray& r, const hit_record& rec) const override {
+        if (!rec.front_face) return color(0, 0, 0);
+        if (spread >= 180) return emit;
+        double c = -dot(unit_vector(r.dir), rec.normal);
+        double gf;
+        if (c <= 0) gf = 0;
+        else gf = std::fmax(0.0, 1 - (std::sqrt(std::fmax(0.0, 1 - c * c)) / c) / tan_half);
+        return emit * gf;
+    }
+    color average_emission() const override { return avg; }
+};
+
+struct phase_material : material {  // medium.h:13-39
+    color albedo;
+    double g;
+    bool is_volume() const override { return true; }
+    bool scatter(const ray& r, const hit_record&, scatter_record& s, sampler&) const override {
+        s.attenuation = albedo;
+        s.pdf_ptr = std::make_unique<hg_pdf>(r.dir, g);
+        s.skip_pdf = false;
+        return true;
+    }
+    color eval(const ray& r, const hit_record&, const vec3& d) const override {
+        return albedo * hg_pdf::phase(dot(unit_vector(r.dir), unit_vector(d)), g);
+    }
+};
+
+// ------------------------------------------------------------------ geometry
+struct sphere : hittable {  // sphere.h
+    point3 center;
+    double radius;
+    const material* mat;
+    int64_t id;
+    bool hit(const ray& r, interval t, hit_record& rec, const hit_ctx&) const override {
+        vec3 oc = center - r.orig;
+        double a = r.dir.length_squared();
+        double h = dot(r.dir, oc);
+        double c = oc.length_squared() - radius * radius;
+        double disc = h * h - a * c;
+        if (disc < 0) return false;
+        double sq = std::sqrt(disc);
+        double root = (h - sq) / a;
+        if (!t.surrounds(root)) {
+            root = (h + sq) / a;
+            if (!t.surrounds(root)) return false;
+        }
+        rec.t = root;
+        rec.p = r.at(root);
+        rec.set_face_normal(r, (rec.p - center) / radius);
+        rec.mat = mat;
+        rec.id = id;
+        return true;
+    }
+    aabb bounding_box() const override {
+        double r = std::fabs(radius);
+        vec3 rv(r, r, r);
+        return aabb(center - rv, center + rv);
+    }
+    double pdf_value(const point3& o, const vec3& d) const override {
+        hit_record rec;
+        if (!hit(ray(o, d), interval(0.001, infinity), rec, hit_ctx{})) return 0;
+        double r = std::fabs(radius);
+        double dist2 = (center - o).length_squared();
+        if (dist2 <= r * r) {
+            double hd2 = rec.t * rec.t * d.length_squared();
+            double cosine = std::fabs(dot(unit_vector(d), (rec.p - center) / r));
+            return hd2 / (cosine * 4 * pi * r * r);
+        }
+        double cmax = std::sqrt(1 - r * r / dist2);
+        return 1 / (2 * pi * (1 - cmax));
+    }
+    vec3 random(const point3& o, sampler& s) const override {
+        double r = std::fabs(radius);
+        vec3 dir = center - o;
+        double dist2 = dir.length_squared();
+        if (dist2 <= r * r) {
+            while (true) {
+                double x = s.next(-1, 1);
+                double y = s.next(-1, 1);
+                double z = s.next(-1, 1);
+                vec3 p(x, y, z);
+                double l = p.length_squared();
+                if (1e-160 < l && l <= 1) return center + r * (p / std::sqrt(l)) - o;
+            }
+        }
+        onb uvw(dir);
+        double r1 = s.next();
+        double r2 = s.next();
+        double z = 1 + r2 * (std::sqrt(1 - r * r / dist2) - 1);
+        double phi = 2 * pi * r1;
+        return uvw.transform(vec3(std::cos(phi) * std::sqrt(1 - z * z), std::sin(phi) * std::sqrt(1 - z * z), z));
+    }
+    double power() const override { return pi * 4 * pi * radius * radius * luminance(mat->average_emission()); }
+};
+
+struct quad : hittable {  // quad.h
+    point3 Q;
+    vec3 u, v, w, normal;
+    double D, area;
+    const material* mat;
+    int64_t id;
+    aabb box;
+    quad(const point3& Q, const vec3& u, const vec3& v, const material* m, int64_t id) : Q(Q), u(u), v(v), mat(m), id(id) {
+        vec3 n = cross(u, v);
+        normal = unit_vector(n);
+        D = dot(normal, Q);
+        w = n / dot(n, n);
+        area = n.length();
+        box = aabb(aabb(Q, Q + u + v), aabb(Q + u, Q + v));
+    }
+    bool hit(const ray& r, interval t, hit_record& rec, const hit_ctx&) const override {
+        double denom = dot(normal, r.dir);
+        if (std::fabs(denom) < 1e-8) return false;
+        double tt = (D - dot(normal, r.orig)) / denom;
+        if (!t.contains(tt)) return false;
+        point3 ip = r.at(tt);
+        vec3 ph = ip - Q;
+        double alpha = dot(w, cross(ph, v));
+        double beta = dot(w, cross(u, ph));
+        interval unit(0, 1);
+        if (!unit.contains(alpha) || !unit.contains(beta)) return false;
+        rec.t = tt;
+        rec.p = ip;
+        rec.mat = mat;
+        rec.id = id;
+        rec.set_face_normal(r, normal);
+        return true;
+    }
+    aabb bounding_box() const override { return box; }
+    double pdf_value(const point3& o, const vec3& d) const override {
+        hit_record rec;
+        if (!hit(ray(o, d), interval(0.001, infinity), rec, hit_ctx{})) return 0;
+        double dist2 = rec.t * rec.t * d.length_squared();
+        double cosine = std::fabs(dot(d, rec.normal) / d.length());
+        return dist2 / (cosine * area);
+    }
+    vec3 random(const point3& o, sampler& s) const override {
+        double a = s.next();
+        double b = s.next();
+        return (Q + (a * u) + (b * v)) - o;
+    }
+    double power() const override { return pi * area * luminance(mat->average_emission()); }
+};
+
+struct triangle : hittable {  // mesh.h:28-139 (no uvs, normals or alpha)
+    point3 p0, p1, p2;
+    const material* mat;
+    int64_t id;
+    double area;
+    triangle(const point3& a, const point3& b, const point3& c, const material* m, int64_t id)
+        : p0(a), p1(b), p2(c), mat(m), id(id) {
+        area = 0.5 * cross(p1 - p0, p2 - p0).length();
+    }
+    bool hit(const ray& r, interval t, hit_record& rec, const hit_ctx&) const override {
+        vec3 e1 = p1 - p0, e2 = p2 - p0;
+        vec3 pvec = cross(r.dir, e2);
+        double det = dot(e1, pvec);
+        if (std::fabs(det) < 1e-12) return false;
+        double inv_det = 1 / det;
+        vec3 tvec = r.orig - p0;
+        double b1 = dot(tvec, pvec) * inv_det;
+        if (b1 < 0 || b1 > 1) return false;
+        vec3 qvec = cross(tvec, e1);
+        double b2 = dot(r.dir, qvec) * inv_det;
+        if (b2 < 0 || b1 + b2 > 1) return false;
+        double tt = dot(e2, qvec) * inv_det;
+        if (!t.surrounds(tt)) return false;
+        rec.t = tt;
+        rec.p = r.at(tt);
+        rec.mat = mat;
+        rec.id = id;
+        rec.set_face_normal(r, unit_vector(cross(e1, e2)));
+        return true;
+    }
+    aabb bounding_box() const override { return aabb(aabb(p0, p1), aabb(p2, p2)); }
+    double pdf_value(const point3& o, const vec3& d) const override {
+        hit_record rec;
+        if (!hit(ray(o, d), interval(0.001, infinity), rec, hit_ctx{})) return 0;
+        vec3 g = unit_vector(cross(p1 - p0, p2 - p0));
+        double dist2 = rec.t * rec.t * d.length_squared();
+        double cosine = std::fabs(dot(d, g) / d.length());
+        return dist2 / (cosine * area);
+    }
+    vec3 random(const point3& o, sampler& s) const override {
+        double r1 = s.next(), r2 = s.next();
+        if (r1 + r2 > 1) {
+            r1 = 1 - r1;
+            r2 = 1 - r2;
+        }
+        return (p0 + r1 * (p1 - p0) + r2 * (p2 - p0)) - o;
+    }
+    double power() const override { return mat ? pi * area * luminance(mat->average_emission()) : 0; }
+};
+
+struct hlist : hittable {  // hittable_list.h
+    std::vector<const hittable*> objects;
+    aabb box;
+    void add(const hittable* h) {
+        objects.push_back(h);
+        box = aabb(box, h->bounding_box());
+    }
+    bool hit(const ray& r, interval t, hit_record& rec, const hit_ctx& c) const override {
+        hit_record tmp;
+        bool any = false;
+        double closest = t.max;
+        for (auto* o : objects)
+            if (o->hit(r, interval(t.min, closest), tmp, c)) {
+                any = true;
+                closest = tmp.t;
+                rec = tmp;
+            }
+        return any;
+    }
+    aabb bounding_box() const override { return box; }
+    double pdf_value(const point3& o, const vec3& d) const override {
+        if (objects.empty()) return 0.0;
+        double weight = 1.0 / objects.size(), sum = 0.0;
+        for (auto* h : objects) sum += weight * h->pdf_value(o, d);
+        return sum;
+    }
+    vec3 random(const point3& o, sampler& s) const override {
+        int n = int(objects.size());
+        return objects[std::min(int(s.next() * n), n - 1)]->random(o, s);
+    }
+    double power() const override {
+        double sum = 0;
+        for (auto* h : objects) sum += h->power();
+        return sum;
+    }
+};
+
+struct bvh : hittable {  // bvh.h
+    struct node {
+        aabb box;
+        int offset = 0, count = 0, axis = 0;
+    };
+    struct item {
+        aabb box;
+        point3 centroid;
+        const hittable* obj;
+    };
+    std::vector<const hittable*> prims;
+    std::vector<node> nodes;
+    explicit bvh(const std::vector<const hittable*>& objs) {
+        if (objs.empty()) {
+            nodes.push_back(node());
+            return;
+        }
+        std::vector<item> items;
+        for (auto* o : objs) {
+            aabb b = o->bounding_box();
+            items.push_back({b, b.centroid(), o});
+        }
+        build(items, 0, items.size(), 0);
+        for (auto& it : items) prims.push_back(it.obj);
+    }
+    int make_leaf(int index, size_t begin, size_t count, std::vector<item>& items, int axis, int depth) {
+        if (count > 4) {
+            size_t mid = begin + count / 2;
+            nodes[index].axis = axis;
+            build(items, begin, mid, depth + 1);
+            int right = build(items, mid, begin + count, depth + 1);
+            nodes[index].offset = right;
+            return index;
+        }
+        nodes[index].offset = int(begin);
+        nodes[index].count = int(count);
+        return index;
+    }
+    int build(std::vector<item>& items, size_t begin, size_t end, int depth) {
+        int index = int(nodes.size());
+        nodes.push_back(node());
+        aabb bounds, cb;
+        for (size_t k = begin; k < end; k++) {
+            bounds = aabb(bounds, items[k].box);
+            cb = aabb(cb, aabb(items[k].centroid, items[k].centroid));
+        }
+        nodes[index].box = bounds;
+        size_t count = end - begin;
+        int axis = cb.longest_axis();
+        interval ext = cb.ax[axis];
+        if (count == 1 || ext.size() <= 0 || depth >= 64) return make_leaf(index, begin, count, items, axis, depth);
+        struct bin {
+            aabb box;
+            int count = 0;
+        } bins[16];
+        auto bin_of = [&](const point3& c) { return std::min(int(16 * (c[axis] - ext.min) / ext.size()), 15); };
+        for (size_t k = begin; k < end; k++) {
+            bin& b = bins[bin_of(items[k].centroid)];
+            b.box = aabb(b.box, items[k].box);
+            b.count++;
+        }
+        double best = infinity;
+        int split = -1;
+        for (int s = 0; s < 15; s++) {
+            aabb l, r;
+            int lc = 0, rc = 0;
+            for (int b = 0; b <= s; b++) {
+                l = aabb(l, bins[b].box);
+                lc += bins[b].count;
+            }
+            for (int b = s + 1; b < 16; b++) {
+                r = aabb(r, bins[b].box);
+                rc += bins[b].count;
+            }
+            if (lc == 0 || rc == 0) continue;
+            double cost = l.area() * lc + r.area() * rc;
+            if (cost < best) {
+                best = cost;
+                split = s;
+            }
+        }
+        double leaf_cost = double(count), split_cost = 0.125 + best / bounds.area();
+        if (split < 0 || (count <= 4 && leaf_cost <= split_cost)) return make_leaf(index, begin, count, items, axis, depth);
+        auto mid_it = std::partition(items.begin() + begin, items.begin() + end,
+                                     [&](const item& it) { return bin_of(it.centroid) <= split; });
+        size_t mid = size_t(mid_it - items.begin());
+        nodes[index].axis = axis;
+        build(items, begin, mid, depth + 1);
+        int right = build(items, mid, end, depth + 1);
+        nodes[index].offset = right;
+        return index;
+    }
+    bool hit(const ray& r, interval t, hit_record& rec, const hit_ctx& c) const override {
+        if (prims.empty()) return false;
+        vec3 inv(1 / r.dir.x(), 1 / r.dir.y(), 1 / r.dir.z());
+        bool neg[3] = {inv.x() < 0, inv.y() < 0, inv.z() < 0};
+        bool any = false;
+        int stack[256];
+        int sp = 0, cur = 0;
+        while (true) {
+            const node& n = nodes[cur];
+            if (n.box.hit(r.orig, inv, t)) {
+                if (n.count > 0) {
+                    for (int k = 0; k < n.count; k++)
+                        if (prims[n.offset + k]->hit(r, t, rec, c)) {
+                            any = true;
+                            t.max = rec.t;
+                        }
+                    if (sp == 0) break;
+                    cur = stack[--sp];
+                } else if (neg[n.axis]) {
+                    stack[sp++] = cur + 1;
+                    cur = n.offset;
+                } else {
+                    stack[sp++] = n.offset;
+                    cur = cur + 1;
+                }
+            } else {
+                if (sp == 0) break;
+                cur = stack[--sp];
+            }
+        }
+        return any;
+    }
+    aabb bounding_box() const override { return nodes[0].box; }
+};
+
+struct affine {  // transform.h
+    double m[9], t[3], im[9], it[3];
+    vec3 mul(const double* a, const vec3& v) const {
+        return vec3(a[0] * v[0] + a[1] * v[1] + a[2] * v[2], a[3] * v[0] + a[4] * v[1] + a[5] * v[2],
+                    a[6] * v[0] + a[7] * v[1] + a[8] * v[2]);
+    }
+    vec3 vector(const vec3& v) const { return mul(m, v); }
+    point3 point(const point3& p) const { return vector(p) + vec3(t[0], t[1], t[2]); }
+    vec3 inv_vector(const vec3& v) const { return mul(im, v); }
+    point3 inv_point(const point3& p) const { return inv_vector(p) + vec3(it[0], it[1], it[2]); }
+    vec3 normal(const vec3& n) const {
+        return vec3(im[0] * n[0] + im[3] * n[1] + im[6] * n[2], im[1] * n[0] + im[4] * n[1] + im[7] * n[2],
+                    im[2] * n[0] + im[5] * n[1] + im[8] * n[2]);
+    }
+    double area_scale() const {
+        double det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) + m[2] * (m[3] * m[7] - m[4] * m[6]);
+        return std::pow(std::fabs(det), 2.0 / 3.0);
+    }
+};
+
+struct instance : hittable {  // instance.h
+    const hittable* object;
+    affine x;
+    aabb box;
+    instance(const hittable* o, const affine& a) : object(o), x(a) {
+        aabb b = o->bounding_box();
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++)
+                for (int k = 0; k < 2; k++) {
+                    point3 c(i ? b.ax[0].max : b.ax[0].min, j ? b.ax[1].max : b.ax[1].min, k ? b.ax[2].max : b.ax[2].min);
+                    point3 p = x.point(c);
+                    box = aabb(box, aabb(p, p));
+                }
+    }
+    bool hit(const ray& r, interval t, hit_record& rec, const hit_ctx& c) const override {
+        ray local(x.inv_point(r.orig), x.inv_vector(r.dir));
+        if (!object->hit(local, t, rec, c)) return false;
+        rec.p = x.point(rec.p);
+        rec.normal = unit_vector(x.normal(rec.normal));
+        return true;
+    }
+    aabb bounding_box() const override { return box; }
+    double pdf_value(const point3& o, const vec3& d) const override { return object->pdf_value(x.inv_point(o), x.inv_vector(d)); }
+    vec3 random(const point3& o, sampler& s) const override { return x.vector(object->random(x.inv_point(o), s)); }
+    double power() const override { return object->power() * x.area_scale(); }
+};
+
+// A world-list object that owns its ID range (lists of an instanced box are
+// re-identified per instance by offsetting the child IDs).
+struct idshift : hittable {
+    const hittable* inner;
+    int64_t shift;
+    idshift(const hittable* h, int64_t s) : inner(h), shift(s) {}
+    bool hit(const ray& r, interval t, hit_record& rec, const hit_ctx& c) const override {
+        if (!inner->hit(r, t, rec, c)) return false;
+        rec.id += shift;
+        return true;
+    }
+    aabb bounding_box() const override { return inner->bounding_box(); }
+    double pdf_value(const point3& o, const vec3& d) const override { return inner->pdf_value(o, d); }
+    vec3 random(const point3& o, sampler& s) const override { return inner->random(o, s); }
+    double power() const override { return inner->power(); }
+};
+
+struct medium : hittable {  // constant_medium, medium.h:57-96
+    const hittable* boundary;
+    double density;
+    const material* phase;
+    uint32_t index;  // world-list position, keys the free-flight draw
+    bool hit(const ray& r, interval t, hit_record& rec, const hit_ctx& c) const override {
+        if (!c.media) return false;
+        hit_record r1, r2;
+        if (!boundary->hit(r, interval(-infinity, infinity), r1, c)) return false;
+        if (!boundary->hit(r, interval(r1.t + 0.0001, infinity), r2, c)) return false;
+        double t0 = std::fmax(r1.t, t.min), t1 = std::fmin(r2.t, t.max);
+        if (t0 >= t1) return false;
+        double len = r.dir.length();
+        double inside = (t1 - t0) * len;
+        double hd = -std::log(1 - medium_u(c.key, c.serial, index)) / density;
+        if (hd > inside) return false;
+        rec.t = t0 + hd / len;
+        rec.p = r.at(rec.t);
+        rec.normal = vec3(1, 0, 0);
+        rec.front_face = true;
+        rec.mat = phase;
+        rec.id = -2;
+        return true;
+    }
+    aabb bounding_box() const override { return boundary->bounding_box(); }
+};
+
+struct environment : hittable {  // environment.h
+    int width = 0, height = 0;
+    std::vector<color> pixels;
+    double intensity = 1, rotation = 0;
+    std::vector<double> cond, marg, row_weight;
+    double total = 0;
+    bool hit(const ray&, interval, hit_record&, const hit_ctx&) const override { return false; }
+    aabb bounding_box() const override { return aabb(); }
+    double weight(int i, int j) const { return luminance(pixels[size_t(j) * width + i]) * std::sin(pi * (j + 0.5) / height); }
+    void build() {
+        cond.assign(size_t(height) * (width + 1), 0.0);
+        marg.assign(height + 1, 0.0);
+        row_weight.assign(height, 0.0);
+        for (int j = 0; j < height; j++) {
+            double* c = &cond[size_t(j) * (width + 1)];
+            for (int i = 0; i < width; i++) c[i + 1] = c[i] + std::fmax(weight(i, j), 0.0);
+            row_weight[j] = c[width];
+            for (int i = 1; i <= width; i++) c[i] = row_weight[j] > 0 ? c[i] / row_weight[j] : double(i) / width;
+            marg[j + 1] = marg[j] + row_weight[j];
+        }
+        total = marg[height];
+        if (total <= 0) {
+            for (int j = 0; j < height; j++) {
+                row_weight[j] = std::sin(pi * (j + 0.5) / height);
+                marg[j + 1] = marg[j] + row_weight[j];
+            }
+            total = marg[height];
+        }
+        for (int j = 1; j <= height; j++) marg[j] /= total;
+    }
+    void angles(const vec3& dir, double& u, double& theta) const {
+        vec3 d = unit_vector(dir);
+        theta = std::acos(dclamp(d.y(), -1.0, 1.0));
+        u = 0.5 + std::atan2(d.x(), -d.z()) / (2 * pi) + rotation;
+        u -= std::floor(u);
+    }
+    color radiance(const vec3& dir) const {
+        double u, theta;
+        angles(dir, u, theta);
+        double x = u * width - 0.5;
+        double y = dclamp(theta / pi * height - 0.5, 0.0, height - 1.0);
+        double x0 = std::floor(x), y0 = std::floor(y), fx = x - x0, fy = y - y0;
+        int i0 = (int(x0) + width) % width, i1 = (i0 + 1) % width;
+        int j0 = int(y0), j1 = std::min(j0 + 1, height - 1);
+        auto at = [&](int i, int j) { return pixels[size_t(j) * width + i]; };
+        color top = (1 - fx) * at(i0, j0) + fx * at(i1, j0);
+        color bottom = (1 - fx) * at(i0, j1) + fx * at(i1, j1);
+        return intensity * ((1 - fy) * top + fy * bottom);
+    }
+    double pdf_value(const point3&, const vec3& dir) const override {
+        double u, theta;
+        angles(dir, u, theta);
+        double st = std::sin(theta);
+        int i = std::clamp(int(u * width), 0, width - 1), j = std::clamp(int(theta / pi * height), 0, height - 1);
+        if (st <= 0) return 0;
+        const double* c = &cond[size_t(j) * (width + 1)];
+        double pix = row_weight[j] / total * (c[i + 1] - c[i]) * width * height;
+        return pix / (2 * pi * pi * st);
+    }
+    static double sample_cdf(const double* cdf, int n, double xi) {
+        int k = int(std::upper_bound(cdf, cdf + n + 1, xi) - cdf) - 1;
+        k = std::clamp(k, 0, n - 1);
+        double span = cdf[k + 1] - cdf[k];
+        double f = span > 0 ? (xi - cdf[k]) / span : 0.5;
+        return k + dclamp(f, 0.0, 1.0 - 1e-9);
+    }
+    vec3 random(const point3&, sampler& s) const override {
+        double y = sample_cdf(marg.data(), height, s.next());
+        int j = std::min(int(y), height - 1);
+        double x = sample_cdf(&cond[size_t(j) * (width + 1)], width, s.next());
+        double theta = pi * y / height, phi = 2 * pi * (x / width - 0.5 - rotation);
+        double st = std::sin(theta);
+        return vec3(st * std::sin(phi), std::cos(theta), -st * std::cos(phi));
+    }
+    double power(double radius) const {
+        double sum = 0;
+        for (int j = 0; j < height; j++)
+            for (int i = 0; i < width; i++) sum += std::fmax(weight(i, j), 0.0);
+        double integral = intensity * sum * (2 * pi / width) * (pi / height);
+        return pi * radius * radius * integral;
+    }
+};
+
+// ------------------------------------------------------------------ scene
+struct scene {
+    std::vector<std::unique_ptr<material>> mats;
+    std::vector<std::unique_ptr<hittable>> owned;
+    std::vector<const hittable*> world;      // world-list objects (with their IDs)
+    std::unique_ptr<bvh> tree;
+    std::vector<const hittable*> lights;     // nullptr entries are the environment
+    std::vector<double> prob, cdf;
+    int strategy = 1;
+    std::unique_ptr<environment> env;
+    ggx_table table;
+    aabb world_box;
+
+    double lights_pdf(const point3& o, const vec3& d) const {
+        if (lights.empty()) return 0.0;
+        double sum = 0;
+        for (size_t k = 0; k < lights.size(); k++)
+            sum += prob[k] * (lights[k] ? lights[k]->pdf_value(o, d) : env->pdf_value(o, d));
+        return sum;
+    }
+    vec3 lights_random(const point3& o, sampler& s) const {
+        double xi = s.next();
+        size_t n = lights.size(), k;
+        if (strategy == 2) k = std::min(size_t(xi * n), n - 1);
+        else {
+            k = size_t(std::upper_bound(cdf.begin(), cdf.end(), xi) - cdf.begin());
+            k = std::clamp(k, size_t(1), n) - 1;
+        }
+        return lights[k] ? lights[k]->random(o, s) : env->random(o, s);
+    }
+};
+
+template <typename T>
+T* own(scene& s, std::unique_ptr<T> p) {
+    T* r = p.get();
+    s.owned.push_back(std::move(p));
+    return r;
+}
+
+int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
+    build_table(s.table);
+    for (int k = 0; k < d.n_materials; k++) {
+        const mpt_material& m = d.materials[k];
+        auto tex = [&](int ti) {
+            texture_t t{};
+            const mpt_texture& x = d.textures[ti];
+            t.type = x.type;
+            t.inv_scale = x.inv_scale;
+            t.even = color(x.even[0], x.even[1], x.even[2]);
+            t.odd = color(x.odd[0], x.odd[1], x.odd[2]);
+            return t;
+        };
+        std::unique_ptr<material> mp;
+        switch (m.type) {
+            case MPT_MAT_LAMBERTIAN: {
+                auto p = std::make_unique<lambertian>();
+                p->tex = tex(m.texture);
+                mp = std::move(p);
+                break;
+            }
+            case MPT_MAT_METAL: {
+                auto p = std::make_unique<metal>();
+                p->albedo = color(m.albedo[0], m.albedo[1], m.albedo[2]);
+                p->f_avg = p->albedo + (color(1, 1, 1) - p->albedo) / 21;
+                p->roughness = m.roughness;
+                p->alpha = roughness_to_alpha(m.roughness);
+                mp = std::move(p);
+                break;
+            }
+            case MPT_MAT_DIELECTRIC: {
+                auto p = std::make_unique<dielectric>();
+                p->ior = m.ior;
+                p->roughness = m.roughness;
+                p->alpha = roughness_to_alpha(m.roughness);
+                p->sigma_a = color(m.sigma_a[0], m.sigma_a[1], m.sigma_a[2]);
+                mp = std::move(p);
+                break;
+            }
+            case MPT_MAT_PRINCIPLED: {
+                auto p = std::make_unique<principled>();
+                p->base = tex(m.texture);
+                p->metallic = dclamp(m.metallic, 0.0, 1.0);
+                p->alpha = roughness_to_alpha(dclamp(m.roughness, 0.0, 1.0));
+                double r0 = (m.ior - 1) / (m.ior + 1);
+                p->f0 = r0 * r0;
+                mp = std::move(p);
+                break;
+            }
+            case MPT_MAT_DIFFUSE_LIGHT: {
+                auto p = std::make_unique<diffuse_light>();
+                p->emit = color(m.emit[0], m.emit[1], m.emit[2]);
+                p->spread = m.spread;
+                double half = m.spread * pi / 180.0 / 2;
+                p->tan_half = m.spread >= 180 ? infinity : std::tan(half);
+                double share = 1;
+                if (!(half >= pi / 2 - 1e-9)) {
+                    double sn = std::sin(half), cs = std::cos(half);
+                    share = sn * sn - (half - sn * cs) * cs / sn;
+                }
+                p->avg = p->emit * share;
+                mp = std::move(p);
+                break;
+            }
+            case MPT_MAT_PHASE: {
+                auto p = std::make_unique<phase_material>();
+                p->albedo = color(m.albedo[0], m.albedo[1], m.albedo[2]);
+                p->g = m.g;
+                mp = std::move(p);
+                break;
+            }
+            default:
+                err = "unknown material type";
+                return MPT_E_UNSUPPORTED;
+        }
+        mp->table = &s.table;
+        s.mats.push_back(std::move(mp));
+    }
+    auto matp = [&](int i) -> const material* { return i >= 0 ? s.mats[i].get() : nullptr; };
+    auto prim = [&](int64_t i, int64_t id) -> const hittable* {
+        const double* q = d.prim_data + 9 * i;
+        const material* m = matp(d.prim_material[i]);
+        switch (d.prim_kind[i]) {
+            case MPT_PRIM_SPHERE: {
+                auto p = std::make_unique<sphere>();
+                p->center = point3(q[0], q[1], q[2]);
+                p->radius = q[3];
+                p->mat = m;
+                p->id = id;
+                return own(s, std::move(p));
+            }
+            case MPT_PRIM_QUAD:
+                return own(s, std::make_unique<quad>(point3(q[0], q[1], q[2]), vec3(q[3], q[4], q[5]), vec3(q[6], q[7], q[8]),
+                                                     m, id));
+            default:
+                return own(s, std::make_unique<triangle>(point3(q[0], q[1], q[2]), point3(q[3], q[4], q[5]),
+                                                         point3(q[6], q[7], q[8]), m, id));
+        }
+    };
+    std::vector<std::pair<int64_t, const bvh*>> meshes;  // (first, tree) of meshes already built
+    for (int o = 0; o < d.n_objects; o++) {
+        int64_t first = d.object_first[o], count = d.object_count[o];
+        int64_t idb = d.object_id_base ? d.object_id_base[o] : 0;
+        int xi = d.object_xform ? d.object_xform[o] : -1;
+        const hittable* h = nullptr;
+        switch (d.object_kind[o]) {
+            case MPT_OBJ_PRIM:
+                h = prim(first, idb);
+                break;
+            case MPT_OBJ_LIST: {
+                auto l = std::make_unique<hlist>();
+                for (int64_t k = 0; k < count; k++) l->add(prim(first + k, idb + k));
+                h = own(s, std::move(l));
+                break;
+            }
+            case MPT_OBJ_MESH: {
+                const bvh* tree = nullptr;
+                for (auto& mm : meshes)
+                    if (mm.first == first) tree = mm.second;
+                int64_t shift = idb;
+                if (!tree) {
+                    std::vector<const hittable*> tris;
+                    for (int64_t k = 0; k < count; k++) tris.push_back(prim(first + k, k));
+                    tree = own(s, std::make_unique<bvh>(tris));
+                    meshes.push_back({first, tree});
+                }
+                h = own(s, std::make_unique<idshift>(tree, shift));
+                break;
+            }
+            case MPT_OBJ_MEDIUM: {
+                const hittable* b;
+                if (d.object_boundary_kind[o] == MPT_OBJ_PRIM) b = prim(first, idb);
+                else {
+                    auto l = std::make_unique<hlist>();
+                    for (int64_t k = 0; k < count; k++) l->add(prim(first + k, idb + k));
+                    b = own(s, std::move(l));
+                }
+                if (xi >= 0) {
+                    const mpt_xform& x = d.xforms[xi];
+                    affine a;
+                    std::memcpy(a.m, x.m, sizeof(a.m));
+                    std::memcpy(a.t, x.t, sizeof(a.t));
+                    std::memcpy(a.im, x.inv_m, sizeof(a.im));
+                    std::memcpy(a.it, x.inv_t, sizeof(a.it));
+                    b = own(s, std::make_unique<instance>(b, a));
+                    xi = -1;
+                }
+                auto m = std::make_unique<medium>();
+                m->boundary = b;
+                m->density = d.object_density[o];
+                m->phase = matp(d.object_material[o]);
+                m->index = uint32_t(o);
+                h = own(s, std::move(m));
+                break;
+            }
+            default:
+                err = "unknown object kind";
+                return MPT_E_UNSUPPORTED;
+        }
+        if (xi >= 0) {
+            const mpt_xform& x = d.xforms[xi];
+            affine a;
+            std::memcpy(a.m, x.m, sizeof(a.m));
+            std::memcpy(a.t, x.t, sizeof(a.t));
+            std::memcpy(a.im, x.inv_m, sizeof(a.im));
+            std::memcpy(a.it, x.inv_t, sizeof(a.it));
+            h = own(s, std::make_unique<instance>(h, a));
+        }
+        s.world.push_back(h);
+        s.world_box = aabb(s.world_box, h->bounding_box());
+    }
+    s.tree = std::make_unique<bvh>(s.world);
+
+    if (d.env_width > 0) {
+        s.env = std::make_unique<environment>();
+        s.env->width = d.env_width;
+        s.env->height = d.env_height;
+        s.env->intensity = d.env_intensity;
+        s.env->rotation = d.env_rotation_degrees / 360;
+        s.env->pixels.resize(size_t(d.env_width) * d.env_height);
+        for (size_t k = 0; k < s.env->pixels.size(); k++)
+            s.env->pixels[k] = color(d.env_rgb[3 * k], d.env_rgb[3 * k + 1], d.env_rgb[3 * k + 2]);
+        s.env->build();
+    }
+    // light_list::build (light_list.h:45-96)
+    size_t n = size_t(d.n_lights);
+    s.strategy = d.light_strategy;
+    for (size_t k = 0; k < n; k++) s.lights.push_back(d.light_object[k] < 0 ? nullptr : s.world[d.light_object[k]]);
+    s.prob.assign(n, n > 0 ? 1.0 / n : 0.0);
+    if (n > 0 && s.strategy == 1) {
+        const aabb& b = s.world_box;
+        double radius = 0;
+        if (b.ax[0].size() > 0 && b.ax[1].size() > 0 && b.ax[2].size() > 0)
+            radius = 0.5 * std::sqrt(b.ax[0].size() * b.ax[0].size() + b.ax[1].size() * b.ax[1].size() +
+                                     b.ax[2].size() * b.ax[2].size());
+        if (!std::isfinite(radius) || radius <= 0) radius = 1;
+        std::vector<double> power(n);
+        double known_sum = 0;
+        int known = 0;
+        for (size_t k = 0; k < n; k++) {
+            power[k] = s.lights[k] ? s.lights[k]->power() : s.env->power(radius);
+            if (std::isfinite(power[k]) && power[k] > 0) {
+                known_sum += power[k];
+                known++;
+            }
+        }
+        double fallback = known > 0 ? known_sum / known : 1.0, total = 0;
+        for (auto& p : power) {
+            if (!std::isfinite(p) || p <= 0) p = fallback;
+            total += p;
+        }
+        double uf = d.light_uniform_fraction;
+        for (size_t k = 0; k < n; k++) s.prob[k] = (1 - uf) * power[k] / total + uf / n;
+    }
+    s.cdf.assign(n + 1, 0.0);
+    for (size_t k = 0; k < n; k++) s.cdf[k + 1] = s.cdf[k] + s.prob[k];
+    if (n > 0) s.cdf[n] = 1;
+    return MPT_OK;
+}
+
+// ------------------------------------------------------------------ camera (camera.h)
+struct cam {
+    int w, h, spp, max_depth, max_diffuse, max_specular, max_scatter;
+    bool transparent_shadows, features, use_sky, has_env;
+    int sqrt_spp, blades;
+    double recip, ortho, defocus, focus, blade_rot;
+    vec3 center, p00, du, dv, w_axis, disk_u, disk_v;
+    color background;
+    explicit cam(const mpt_camera& c, bool env) {
+        w = c.image_width;
+        h = std::max(1, int(c.image_width / c.aspect_ratio));
+        spp = c.samples_per_pixel;
+        max_depth = c.max_depth;
+        max_diffuse = c.max_diffuse;
+        max_specular = c.max_specular;
+        max_scatter = c.max_scatter;
+        transparent_shadows = c.transparent_shadows;
+        features = c.collect_features;
+        use_sky = c.use_sky;
+        has_env = env;
+        background = color(c.background[0], c.background[1], c.background[2]);
+        sqrt_spp = int(std::sqrt(spp));
+        recip = 1.0 / sqrt_spp;
+        blades = c.blades;
+        ortho = c.ortho_height;
+        defocus = c.defocus_angle;
+        focus = c.focus_dist;
+        blade_rot = c.blade_rotation;
+        point3 lf(c.lookfrom[0], c.lookfrom[1], c.lookfrom[2]), la(c.lookat[0], c.lookat[1], c.lookat[2]);
+        vec3 vup(c.vup[0], c.vup[1], c.vup[2]);
+        center = lf;
+        double theta = c.vfov * pi / 180.0;
+        double hh = std::tan(theta / 2);
+        double vh = ortho > 0 ? ortho : 2 * hh * focus;
+        double vw = vh * (double(w) / h);
+        w_axis = unit_vector(lf - la);
+        vec3 u = unit_vector(cross(vup, w_axis));
+        vec3 v = cross(w_axis, u);
+        vec3 vu = vw * u, vv = vh * -v;
+        du = vu / w;
+        dv = vv / h;
+        point3 ul = center - (focus * w_axis) - vu / 2 - vv / 2;
+        p00 = ul + 0.5 * (du + dv);
+        double dr = focus * std::tan((defocus / 2) * pi / 180.0);
+        disk_u = u * dr;
+        disk_v = v * dr;
+    }
+    ray get_ray(int i, int j, int pass, sampler& s) const {
+        int strata = sqrt_spp * sqrt_spp;
+        double ox, oy;
+        if (pass >= strata) {
+            ox = s.next() - 0.5;
+            oy = s.next() - 0.5;
+        } else {
+            int cell = (pass + i * 7919 + j * 104729) % strata;
+            ox = ((cell % sqrt_spp + s.next()) * recip) - 0.5;
+            oy = ((cell / sqrt_spp + s.next()) * recip) - 0.5;
+        }
+        point3 ps = p00 + ((i + ox) * du) + ((j + oy) * dv);
+        point3 eye = ortho > 0 ? ps + focus * w_axis : center;
+        point3 org = eye;
+        if (defocus > 0) {
+            vec3 p;
+            if (blades >= 3) {
+                int k = std::min(int(s.next() * blades), blades - 1);
+                double a0 = blade_rot * pi / 180.0 + 2 * pi * k / blades, a1 = a0 + 2 * pi / blades;
+                double a = s.next(), b = s.next();
+                if (a + b > 1) {
+                    a = 1 - a;
+                    b = 1 - b;
+                }
+                p = a * vec3(std::cos(a0), std::sin(a0), 0) + b * vec3(std::cos(a1), std::sin(a1), 0);
+            } else {
+                while (true) {
+                    double x = s.next(-1, 1);
+                    double y = s.next(-1, 1);
+                    p = vec3(x, y, 0);
+                    if (p.length_squared() < 1) break;
+                }
+            }
+            org = eye + ((p[0] * disk_u) + (p[1] * disk_v));
+        }
+        return ray(org, ps - org);
+    }
+};
+
+struct counters {
+    uint64_t extension = 0, shadow = 0;
+};
+
+color escaped(const scene& S, const cam& C, const vec3& d) {
+    if (C.has_env) return S.env->radiance(d);
+    if (!C.use_sky) return C.background;
+    vec3 u = unit_vector(d);
+    double a = 0.5 * (u.y() + 1.0);
+    return (1.0 - a) * color(1.0, 1.0, 1.0) + a * color(0.5, 0.7, 1.0);
+}
+double power_heuristic(double a, double b) {
+    double a2 = a * a, b2 = b * b;
+    return (a2 + b2) > 0 ? a2 / (a2 + b2) : 0;
+}
+
+color sample_light(const scene& S, const cam& C, const ray& r_in, const hit_record& rec, const pdf& bsdf, bool last,
+                   sampler& s, int depth, counters& cnt) {
+    vec3 to_light = S.lights_random(rec.p, s);
+    double light_pdf = S.lights_pdf(rec.p, to_light);
+    if (light_pdf <= 0) return color(0, 0, 0);
+    color f = rec.mat->eval(r_in, rec, to_light);
+    if (f.x() <= 0 && f.y() <= 0 && f.z() <= 0) return color(0, 0, 0);
+    double weight = last ? 1.0 : power_heuristic(light_pdf, bsdf.value(to_light));
+    ray shadow(rec.p, unit_vector(to_light));
+    color emission, through(1, 1, 1);
+    for (int k = 0;; k++) {
+        hit_record lr;
+        cnt.shadow++;
+        hit_ctx hc{s.key, uint32_t(2 * depth + 1) | (uint32_t(k) << 20), true};
+        if (!S.tree->hit(shadow, interval(0.001, infinity), lr, hc)) {
+            emission = escaped(S, C, to_light);
+            break;
+        }
+        if (C.transparent_shadows && k < 16 && lr.mat->lets_shadows_through()) {
+            through = through * lr.mat->interior_transmittance(shadow, lr);
+            shadow = ray(lr.p, shadow.dir);
+            continue;
+        }
+        emission = lr.mat->emitted(shadow, lr);
+        break;
+    }
+    emission = through * emission;
+    return f * emission * (weight / light_pdf);
+}
+
+color ray_color(const scene& S, const cam& C, const ray& camera_ray, sampler& s, counters& cnt) {  // camera.h:356-461
+    color radiance(0, 0, 0), beta(1, 1, 1);
+    ray r = camera_ray;
+    bool specular = true, seen = false;
+    double scatter_pdf = 0;
+    point3 prev;
+    int dc = 0, sc = 0, vc = 0;
+    const bool no_lights = S.lights.empty();
+    for (int depth = 0; depth < C.max_depth; depth++) {
+        hit_record rec;
+        cnt.extension++;
+        hit_ctx hc{s.key, uint32_t(2 * depth), true};
+        if (!S.tree->hit(r, interval(0.001, infinity), rec, hc)) {
+            color sky = escaped(S, C, r.dir);
+            if (seen) break;
+            if (specular || no_lights) radiance = radiance + beta * sky;
+            else radiance = radiance + beta * sky * power_heuristic(scatter_pdf, S.lights_pdf(prev, r.dir));
+            break;
+        }
+        beta = beta * rec.mat->interior_transmittance(r, rec);
+        color e = rec.mat->emitted(r, rec);
+        if (!seen && (e.x() > 0 || e.y() > 0 || e.z() > 0)) {
+            if (specular || no_lights) radiance = radiance + beta * e;
+            else radiance = radiance + beta * e * power_heuristic(scatter_pdf, S.lights_pdf(prev, r.dir));
+        }
+        scatter_record srec;
+        if (!rec.mat->scatter(r, rec, srec, s)) break;
+        if (srec.skip_pdf) {
+            if (C.max_specular > 0 && ++sc > C.max_specular) break;
+            beta = beta * srec.attenuation;
+            r = srec.skip_pdf_ray;
+            if (C.transparent_shadows && rec.mat->lets_shadows_through() && !no_lights && depth > 0 && !specular) seen = true;
+            specular = true;
+        } else {
+            bool volume = rec.mat->is_volume();
+            int& count = volume ? vc : dc;
+            int cap = volume ? C.max_scatter : C.max_diffuse;
+            bool capped = cap > 0 && ++count >= cap;
+            bool last = depth == C.max_depth - 1 || capped;
+            seen = false;
+            if (!no_lights) radiance = radiance + beta * sample_light(S, C, r, rec, *srec.pdf_ptr, last, s, depth, cnt);
+            if (capped) break;
+            vec3 dir = srec.pdf_ptr->generate(s);
+            scatter_pdf = srec.pdf_ptr->value(dir);
+            if (scatter_pdf <= 0) break;
+            beta = beta * rec.mat->eval(r, rec, dir) / scatter_pdf;
+            prev = rec.p;
+            r = ray(rec.p, dir);
+            specular = false;
+        }
+        if (depth >= 3) {
+            double survive = std::fmin(std::fmax(beta.x(), std::fmax(beta.y(), beta.z())), 0.95);
+            if (s.next() >= survive) break;
+            beta = beta / survive;
+        }
+    }
+    return radiance;
+}
+
+void first_surface(const scene& S, const ray& cr, uint64_t key, color& albedo, color& normal) {  // camera.h:253-277
+    ray r = cr;
+    color tint(1, 1, 1);
+    sampler s;
+    s.key = mix64(key ^ 0x3C6EF372FE94F82Bull);
+    for (int b = 0; b < 8; b++) {
+        hit_record rec;
+        hit_ctx hc{s.key, uint32_t(b), true};
+        if (!S.tree->hit(r, interval(0.001, infinity), rec, hc)) break;
+        tint = tint * rec.mat->interior_transmittance(r, rec);
+        scatter_record srec;
+        if (!rec.mat->scatter(r, rec, srec, s)) {
+            albedo = tint;
+            normal = rec.normal;
+            return;
+        }
+        if (!srec.skip_pdf) {
+            albedo = tint * srec.attenuation;
+            normal = rec.normal;
+            return;
+        }
+        tint = tint * srec.attenuation;
+        r = srec.skip_pdf_ray;
+    }
+    albedo = tint;
+    normal = color(0, 0, 0);
+}
+
+}  // namespace orc
+
+// ======================================================================== C ABI
+using namespace orc;
+
+struct orc_scene {
+    scene s;
+};
+
+static thread_local std::string orc_err;
+
+extern "C" {
+
+const char* orc_last_error(void) { return orc_err.c_str(); }
+
+int32_t orc_scene_create(const mpt_scene_desc* d, orc_scene** out) {
+    auto p = std::make_unique<orc_scene>();
+    std::string err;
+    int rc = build_scene(*d, p->s, err);
+    if (rc != MPT_OK) {
+        orc_err = err;
+        return rc;
+    }
+    *out = p.release();
+    return MPT_OK;
+}
+
+void orc_scene_destroy(orc_scene* s) { delete s; }
+
+// Closest hit for n rays (media skipped): canonical ID, t, front face.
+int32_t orc_trace(orc_scene* S, const double* rays, const double* tt, int64_t n, int64_t* id, double* t, uint8_t* front,
+                  int32_t threads) {
+    auto work = [&](int64_t a, int64_t b) {
+        for (int64_t i = a; i < b; i++) {
+            ray r(point3(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]), vec3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]));
+            hit_record rec;
+            hit_ctx hc{0, 0, false};
+            if (S->s.tree->hit(r, interval(tt[2 * i], tt[2 * i + 1]), rec, hc)) {
+                id[i] = rec.id;
+                t[i] = rec.t;
+                front[i] = rec.front_face;
+            } else {
+                id[i] = -1;
+                t[i] = 0;
+                front[i] = 0;
+            }
+        }
+    };
+    int nt = std::max(1, threads);
+    std::vector<std::thread> pool;
+    int64_t chunk = (n + nt - 1) / nt;
+    for (int k = 1; k < nt; k++) pool.emplace_back(work, std::min(n, k * chunk), std::min(n, (k + 1) * chunk));
+    work(0, std::min(n, chunk));
+    for (auto& th : pool) th.join();
+    return MPT_OK;
+}
+
+// Linear scan over every world object: the closest hit any correct BVH
+// must return (brute-force check of the BVH).
+int32_t orc_trace_linear(orc_scene* S, const double* rays, const double* tt, int64_t n, int64_t* id, double* t) {
+    for (int64_t i = 0; i < n; i++) {
+        ray r(point3(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]), vec3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]));
+        hit_record best;
+        bool any = false;
+        double closest = tt[2 * i + 1];
+        hit_ctx hc{0, 0, false};
+        for (auto* o : S->s.world) {
+            hit_record rec;
+            if (o->hit(r, interval(tt[2 * i], closest), rec, hc)) {
+                any = true;
+                closest = rec.t;
+                best = rec;
+            }
+        }
+        id[i] = any ? best.id : -1;
+        t[i] = any ? best.t : 0;
+    }
+    return MPT_OK;
+}
+
+int32_t orc_light_probabilities(orc_scene* S, double* out) {
+    for (size_t k = 0; k < S->s.prob.size(); k++) out[k] = S->s.prob[k];
+    return MPT_OK;
+}
+
+int32_t orc_ggx_table(double* e, double* eavg) {
+    ggx_table t;
+    build_table(t);
+    std::memcpy(e, t.e, sizeof(t.e));
+    std::memcpy(eavg, t.e_avg, sizeof(t.e_avg));
+    return MPT_OK;
+}
+
+// material eval and scatter-pdf value for fixed directions (as mpt_bsdf_eval).
+int32_t orc_bsdf_eval(orc_scene* S, const int32_t* mats, const double* q, const uint8_t* front, int64_t n, double* eval,
+                      double* pdfv) {
+    for (int64_t i = 0; i < n; i++) {
+        hit_record rec;
+        rec.normal = vec3(q[9 * i], q[9 * i + 1], q[9 * i + 2]);
+        ray r(point3(0, 0, 0), vec3(q[9 * i + 3], q[9 * i + 4], q[9 * i + 5]));
+        vec3 wi(q[9 * i + 6], q[9 * i + 7], q[9 * i + 8]);
+        rec.p = point3(0, 0, 0);
+        rec.t = 1;
+        rec.front_face = front[i] != 0;
+        rec.mat = S->s.mats[mats[i]].get();
+        sampler smp;
+        scatter_record sr;
+        bool sc = rec.mat->scatter(r, rec, sr, smp);
+        color e = rec.mat->eval(r, rec, wi);
+        eval[3 * i] = e.x(), eval[3 * i + 1] = e.y(), eval[3 * i + 2] = e.z();
+        pdfv[i] = (sc && !sr.skip_pdf) ? sr.pdf_ptr->value(wi) : 0.0;
+    }
+    return MPT_OK;
+}
+
+// camera::render on the CPU: passes [first_pass, first_pass + passes) of
+// the camera, tile-parallel over `threads` workers (camera.h:92-139);
+// accumulates sums (not averages) into acc_* (3 doubles per pixel; l2 one).
+int32_t orc_render(orc_scene* S, const mpt_camera* c, uint64_t seed, uint32_t scene_index, int32_t first_pass,
+                   int32_t passes, int32_t threads, double* acc_rad, double* acc_alb, double* acc_nrm, double* acc_l2,
+                   int64_t* ext_rays, int64_t* shadow_rays, double* seconds) {
+    auto t0 = std::chrono::steady_clock::now();
+    cam C(*c, S->s.env != nullptr);
+    const int tile = 16;
+    int tx = (C.w + tile - 1) / tile, ty = (C.h + tile - 1) / tile, tiles = tx * ty;
+    int nt = std::max(1, threads);
+    std::atomic<uint64_t> ext{0}, sh{0};
+    for (int pass = first_pass; pass < first_pass + passes; pass++) {
+        std::atomic<int> next{0};
+        auto worker = [&]() {
+            counters cnt;
+            while (true) {
+                int tl = next.fetch_add(1);
+                if (tl >= tiles) break;
+                int i0 = (tl % tx) * tile, j0 = (tl / tx) * tile;
+                int i1 = std::min(i0 + tile, C.w), j1 = std::min(j0 + tile, C.h);
+                for (int j = j0; j < j1; j++)
+                    for (int i = i0; i < i1; i++) {
+                        size_t k = size_t(j) * C.w + i;
+                        sampler s;
+                        s.key = key_of(seed, scene_index, uint32_t(pass), uint32_t(k));
+                        ray r = C.get_ray(i, j, pass, s);
+                        color col = ray_color(S->s, C, r, s, cnt);
+                        for (int q = 0; q < 3; q++)
+                            if (!std::isfinite(col[q])) {
+                                col = color(0, 0, 0);
+                                break;
+                            }
+                        for (int q = 0; q < 3; q++) acc_rad[3 * k + q] += col[q];
+                        if (C.features) {
+                            color a, nn;
+                            first_surface(S->s, r, s.key, a, nn);
+                            for (int q = 0; q < 3; q++) {
+                                acc_alb[3 * k + q] += a[q];
+                                acc_nrm[3 * k + q] += nn[q];
+                            }
+                            double l = luminance(col);
+                            acc_l2[k] += l * l;
+                        }
+                    }
+            }
+            ext += cnt.extension;
+            sh += cnt.shadow;
+        };
+        std::vector<std::thread> pool;
+        for (int k = 1; k < std::min(nt, tiles); k++) pool.emplace_back(worker);
+        worker();
+        for (auto& th : pool) th.join();
+    }
+    if (ext_rays) *ext_rays = int64_t(ext.load());
+    if (shadow_rays) *shadow_rays = int64_t(sh.load());
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return MPT_OK;
+}
+
+// The seeded ray batch of the closest-hit oracle: half camera rays of the
+// camera (random jitter in each pixel), half cosine-distributed rays leaving
+// the points those camera rays hit (the first bounce). Rays 6 doubles,
+// intervals [0.001, inf).
+int32_t orc_make_ray_batch(orc_scene* S, const mpt_camera* c, uint64_t seed, int64_t n, double* rays, double* tt) {
+    cam C(*c, S->s.env != nullptr);
+    int64_t half = n / 2;
+    sampler s;
+    s.key = mix64(seed ^ 0x5851F42D4C957F2Dull);
+    auto put = [&](int64_t i, const ray& r) {
+        rays[6 * i] = r.orig.x(), rays[6 * i + 1] = r.orig.y(), rays[6 * i + 2] = r.orig.z();
+        rays[6 * i + 3] = r.dir.x(), rays[6 * i + 4] = r.dir.y(), rays[6 * i + 5] = r.dir.z();
+        tt[2 * i] = 0.001, tt[2 * i + 1] = infinity;
+    };
+    for (int64_t i = 0; i < half; i++) {
+        int px = std::min(int(s.next() * C.w), C.w - 1), py = std::min(int(s.next() * C.h), C.h - 1);
+        ray r = C.get_ray(px, py, C.spp, s);  // pass >= strata: uniform jitter
+        put(i, r);
+    }
+    for (int64_t i = half; i < n; i++) {
+        const int64_t src = half > 0 ? (i - half) % half : 0;
+        ray cr(point3(rays[6 * src], rays[6 * src + 1], rays[6 * src + 2]),
+               vec3(rays[6 * src + 3], rays[6 * src + 4], rays[6 * src + 5]));
+        hit_record rec;
+        hit_ctx hc{0, 0, false};
+        ray r = cr;
+        if (S->s.tree->hit(cr, interval(0.001, infinity), rec, hc)) {
+            cosine_pdf cp(rec.normal);
+            r = ray(rec.p, cp.generate(s));
+        }
+        put(i, r);
+    }
+    return MPT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,200 @@
+// ref_harness.cpp - a C entry layer over the reference renderer itself,
+// compiled from the reference's own headers where they lie (see
+// oracle/Makefile, target `ref`) into oracle/_ref/libmpt_ref.so.
+//
+// TEST INFRASTRUCTURE, used only in the container that holds the reference
+// sources: it writes the golden vectors under tests/golden and pins the
+// CPU restatement (oracle.cpp). Nothing from the reference is copied here;
+// the harness only calls it.
+//
+// The reference's hit_record has no primitive ID (hittable.h:12-32), so a
+// hit is identified afterwards: every primitive of the world, in canonical
+// order (world list order, lists expanded in place, box() sides in their
+// order, instances and meshes descended), is tested alone with the same ray
+// and interval; the ones whose t is bitwise equal to the BVH's are the
+// candidates. One candidate: its ID. Several: a tie, reported separately.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+// The reference keeps geometry members private; the harness reads them.
+#define private public
+#include "scene.h"
+#undef private
+
+namespace {
+
+struct canon_prim {
+    const hittable* prim;
+    const instance* inst;  // nullptr when not instanced
+    bool medium;           // boundary of a medium: never a ray-query answer
+};
+
+struct ref_scene {
+    scene sc;
+    std::shared_ptr<bvh> tree;
+    std::vector<canon_prim> canon;
+};
+
+void expand(const hittable* h, const instance* inst, bool medium, std::vector<canon_prim>& out) {
+    if (auto l = dynamic_cast<const hittable_list*>(h)) {
+        for (const auto& o : l->objects) expand(o.get(), inst, medium, out);
+    } else if (auto in = dynamic_cast<const instance*>(h)) {
+        expand(in->object.get(), in, medium, out);
+    } else if (auto m = dynamic_cast<const mesh*>(h)) {
+        for (const auto& t : m->triangles) out.push_back({t.get(), inst, medium});
+    } else if (auto cm = dynamic_cast<const constant_medium*>(h)) {
+        expand(cm->boundary.get(), inst, true, out);
+    } else {
+        out.push_back({h, inst, medium});
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_load(const char* path, void** out) {
+    auto r = new ref_scene;
+    std::clog.setstate(std::ios::failbit);
+    if (!load_scene(path, r->sc)) {
+        delete r;
+        return -1;
+    }
+    r->tree = std::make_shared<bvh>(r->sc.world);  // main.cpp:70
+    for (const auto& o : r->sc.world.objects) expand(o.get(), nullptr, false, r->canon);
+    *out = r;
+    return 0;
+}
+
+void ref_free(void* h) { delete static_cast<ref_scene*>(h); }
+
+int64_t ref_canon_count(void* h) { return int64_t(static_cast<ref_scene*>(h)->canon.size()); }
+
+// Closest hit of n rays through the reference BVH. id: canonical ID or -1
+// (miss); ties: number of canonical primitives with the same t (0 on a
+// miss, 1 for a unique hit); tie_min/tie_max: smallest/largest candidate ID.
+int ref_trace(void* h, const double* rays, const double* tt, int64_t n, int64_t* id, double* t, uint8_t* front,
+              int32_t* ties, int64_t* tie_min, int64_t* tie_max, int threads) {
+    auto* r = static_cast<ref_scene*>(h);
+    auto work = [&](int64_t a, int64_t b) {
+        for (int64_t i = a; i < b; i++) {
+            ray rr(point3(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]), vec3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]));
+            interval iv(tt[2 * i], tt[2 * i + 1]);
+            hit_record rec;
+            if (!r->tree->hit(rr, iv, rec)) {
+                id[i] = -1, t[i] = 0, front[i] = 0, ties[i] = 0, tie_min[i] = -1, tie_max[i] = -1;
+                continue;
+            }
+            t[i] = rec.t;
+            front[i] = rec.front_face;
+            int32_t nc = 0;
+            int64_t lo = -1, hi = -1;
+            for (size_t k = 0; k < r->canon.size(); k++) {
+                const canon_prim& c = r->canon[k];
+                if (c.medium) continue;
+                ray local = rr;
+                if (c.inst)
+                    local = ray(c.inst->xform.apply_inverse_point(rr.origin()), c.inst->xform.apply_inverse_vector(rr.direction()));
+                hit_record one;
+                if (c.prim->hit(local, iv, one) && std::memcmp(&one.t, &rec.t, sizeof(double)) == 0) {
+                    if (nc == 0) lo = int64_t(k);
+                    hi = int64_t(k);
+                    nc++;
+                }
+            }
+            ties[i] = nc;
+            tie_min[i] = lo;
+            tie_max[i] = hi;
+            id[i] = nc == 1 ? lo : (nc == 0 ? -2 : lo);
+        }
+    };
+    int nt = std::max(1, threads);
+    std::vector<std::thread> pool;
+    int64_t chunk = (n + nt - 1) / nt;
+    for (int k = 1; k < nt; k++) pool.emplace_back(work, std::min(n, k * chunk), std::min(n, (k + 1) * chunk));
+    work(0, std::min(n, chunk));
+    for (auto& th : pool) th.join();
+    return 0;
+}
+
+int ref_light_count(void* h) { return int(static_cast<ref_scene*>(h)->sc.lights.objects.size()); }
+
+int ref_light_probabilities(void* h, double* out) {
+    auto* r = static_cast<ref_scene*>(h);
+    for (size_t k = 0; k < r->sc.lights.objects.size(); k++) out[k] = r->sc.lights.probability(k);
+    return 0;
+}
+
+int ref_ggx_table(double* e, double* eavg) {
+    const auto& t = ggx_albedo_table::get();
+    std::memcpy(e, t.e, sizeof(t.e));
+    std::memcpy(eavg, t.e_avg, sizeof(t.e_avg));
+    return 0;
+}
+
+// Material eval and scatter-pdf value for fixed directions. spec per query
+// (12 doubles): type (0 lambertian, 1 metal, 2 dielectric, 3 principled,
+// 5 phase), r, g, b, roughness, ior, metallic, g (phase), sigma_a rgb, unused.
+// q: normal, ray direction, wi (9 doubles); front flags.
+int ref_bsdf_eval(const double* spec, const double* q, const uint8_t* front, int64_t n, double* eval, double* pdfv) {
+    for (int64_t i = 0; i < n; i++) {
+        const double* s = spec + 12 * i;
+        int type = int(s[0]);
+        color c(s[1], s[2], s[3]);
+        std::shared_ptr<material> m;
+        if (type == 0) m = std::make_shared<lambertian>(c);
+        else if (type == 1) m = std::make_shared<metal>(c, s[4]);
+        else if (type == 2) m = std::make_shared<dielectric>(s[5], s[4], color(s[8], s[9], s[10]));
+        else if (type == 3) m = std::make_shared<principled>(std::make_shared<solid_color>(c), s[6], s[4], s[5]);
+        else m = std::make_shared<phase_material>(c, s[7]);
+        hit_record rec{};
+        rec.normal = vec3(q[9 * i], q[9 * i + 1], q[9 * i + 2]);
+        rec.p = point3(0, 0, 0);
+        rec.t = 1;
+        rec.u = rec.v = 0;
+        rec.front_face = front[i] != 0;
+        rec.mat = m.get();
+        ray r(point3(0, 0, 0), vec3(q[9 * i + 3], q[9 * i + 4], q[9 * i + 5]));
+        vec3 wi(q[9 * i + 6], q[9 * i + 7], q[9 * i + 8]);
+        color e = m->eval(r, rec, wi);
+        eval[3 * i] = e.x(), eval[3 * i + 1] = e.y(), eval[3 * i + 2] = e.z();
+        scatter_record sr;
+        pdfv[i] = 0;
+        if (type != 2 || s[4] > 0) {  // smooth glass would draw a random number and is specular
+            if (m->scatter(r, rec, sr) && !sr.skip_pdf) pdfv[i] = sr.pdf_ptr->value(wi);
+        }
+    }
+    return 0;
+}
+
+// camera::render of the loaded scene with width/spp/depth overridden;
+// radiance is w*h*3 doubles (the reference's own unseeded RNG).
+int ref_render(void* h, int width, int spp, int max_depth, int threads, double* radiance) {
+    auto* r = static_cast<ref_scene*>(h);
+    camera cam = r->sc.cam;
+    cam.image_width = width;
+    cam.samples_per_pixel = spp;
+    cam.max_depth = max_depth;
+    cam.threads = threads;
+    hittable_list everything(r->tree);
+    for (const auto& o : r->sc.unbounded.objects) everything.add(o);
+    const hittable& world = r->sc.unbounded.empty() ? static_cast<const hittable&>(*r->tree) : everything;
+    frame f = cam.render(world, r->sc.lights);
+    for (size_t k = 0; k < f.radiance.size(); k++)
+        for (int c = 0; c < 3; c++) radiance[3 * k + c] = f.radiance[k][c];
+    return 0;
+}
+
+}  // extern "C"

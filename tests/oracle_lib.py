@@ -1,0 +1,216 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) and, where it was
+built, the compiled reference harness (oracle/_ref/libmpt_ref.so).
+
+Test infrastructure: the checker, never the thing measured or shipped.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from mypbr_amd.abi import Camera, SceneDesc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+REF = os.path.join(ROOT, "oracle", "_ref", "libmpt_ref.so")
+
+P = C.POINTER
+vp = C.c_void_p
+dp = P(C.c_double)
+
+
+def _d(a):
+    return a.ctypes.data_as(dp)
+
+
+def _i64(a):
+    return a.ctypes.data_as(P(C.c_int64))
+
+
+def _u8(a):
+    return a.ctypes.data_as(P(C.c_uint8))
+
+
+def _i32(a):
+    return a.ctypes.data_as(P(C.c_int32))
+
+
+_oracle = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        lib = C.CDLL(ORACLE)
+        sig = {
+            "orc_last_error": (C.c_char_p, []),
+            "orc_scene_create": (C.c_int32, [P(SceneDesc), P(vp)]),
+            "orc_scene_destroy": (None, [vp]),
+            "orc_trace": (C.c_int32, [vp, dp, dp, C.c_int64, P(C.c_int64), dp, P(C.c_uint8), C.c_int32]),
+            "orc_trace_linear": (C.c_int32, [vp, dp, dp, C.c_int64, P(C.c_int64), dp]),
+            "orc_light_probabilities": (C.c_int32, [vp, dp]),
+            "orc_ggx_table": (C.c_int32, [dp, dp]),
+            "orc_bsdf_eval": (C.c_int32, [vp, P(C.c_int32), dp, P(C.c_uint8), C.c_int64, dp, dp]),
+            "orc_render": (C.c_int32, [vp, P(Camera), C.c_uint64, C.c_uint32, C.c_int32, C.c_int32, C.c_int32,
+                                       dp, dp, dp, dp, P(C.c_int64), P(C.c_int64), dp]),
+            "orc_make_ray_batch": (C.c_int32, [vp, P(Camera), C.c_uint64, C.c_int64, dp, dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _oracle = lib
+    return _oracle
+
+
+class OracleScene:
+    def __init__(self, desc):
+        self.lib = oracle()
+        self.h = vp()
+        rc = self.lib.orc_scene_create(C.byref(desc), C.byref(self.h))
+        if rc != 0:
+            raise RuntimeError("orc_scene_create: " + self.lib.orc_last_error().decode())
+
+    def close(self):
+        if self.h:
+            self.lib.orc_scene_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        self.close()
+
+    def trace(self, rays, tt, threads=8):
+        n = rays.shape[0]
+        ids = np.zeros(n, dtype=np.int64)
+        t = np.zeros(n)
+        f = np.zeros(n, dtype=np.uint8)
+        self.lib.orc_trace(self.h, _d(rays), _d(tt), n, _i64(ids), _d(t), _u8(f), threads)
+        return ids, t, f
+
+    def trace_linear(self, rays, tt):
+        n = rays.shape[0]
+        ids = np.zeros(n, dtype=np.int64)
+        t = np.zeros(n)
+        self.lib.orc_trace_linear(self.h, _d(rays), _d(tt), n, _i64(ids), _d(t))
+        return ids, t
+
+    def light_probabilities(self, n):
+        out = np.zeros(max(n, 1))
+        self.lib.orc_light_probabilities(self.h, _d(out))
+        return out[:n]
+
+    def bsdf_eval(self, mats, q, front):
+        mats = np.ascontiguousarray(mats, dtype=np.int32)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        front = np.ascontiguousarray(front, dtype=np.uint8)
+        n = mats.shape[0]
+        ev = np.zeros((n, 3))
+        pdf = np.zeros(n)
+        self.lib.orc_bsdf_eval(self.h, _i32(mats), _d(q), _u8(front), n, _d(ev), _d(pdf))
+        return ev, pdf
+
+    def ray_batch(self, cam, seed, n):
+        rays = np.zeros((n, 6))
+        tt = np.zeros((n, 2))
+        self.lib.orc_make_ray_batch(self.h, C.byref(cam), seed, n, _d(rays), _d(tt))
+        return rays, tt
+
+    def render_sums(self, cam, seed=1, first_pass=0, passes=None, threads=8, scene_index=0):
+        """Sums (not averages) over passes, as the accumulator of camera.h:61-65."""
+        w, h = cam.image_width, cam.height()
+        passes = cam.samples_per_pixel if passes is None else passes
+        rad = np.zeros((h, w, 3))
+        alb = np.zeros((h, w, 3))
+        nrm = np.zeros((h, w, 3))
+        l2 = np.zeros((h, w))
+        ext = C.c_int64()
+        sh = C.c_int64()
+        secs = C.c_double()
+        self.lib.orc_render(self.h, C.byref(cam), seed, scene_index, first_pass, passes, threads, _d(rad), _d(alb),
+                            _d(nrm), _d(l2), C.byref(ext), C.byref(sh), C.byref(secs))
+        return {"radiance": rad, "albedo": alb, "normal": nrm, "l2": l2, "extension_rays": ext.value,
+                "shadow_rays": sh.value, "seconds": secs.value}
+
+    def render(self, cam, seed=1, threads=8):
+        """camera::render's averaged frame (camera.h:223-246)."""
+        s = self.render_sums(cam, seed=seed, threads=threads)
+        n = cam.samples_per_pixel
+        scale = 1.0 / n
+        out = {"radiance": scale * s["radiance"], "stats": s}
+        if cam.collect_features:
+            out["albedo"] = scale * s["albedo"]
+            out["normal"] = scale * s["normal"]
+            rad = out["radiance"]
+            mean = 0.2126 * rad[..., 0] + 0.7152 * rad[..., 1] + 0.0722 * rad[..., 2]
+            spread = np.fmax(scale * s["l2"] - mean * mean, 0.0)
+            out["variance"] = spread / (n - 1) if n > 1 else spread
+        return out
+
+
+_ref = None
+
+
+def reference_available():
+    return os.path.exists(REF)
+
+
+def reference():
+    global _ref
+    if _ref is None:
+        lib = C.CDLL(REF)
+        sig = {
+            "ref_load": (C.c_int, [C.c_char_p, P(vp)]),
+            "ref_free": (None, [vp]),
+            "ref_canon_count": (C.c_int64, [vp]),
+            "ref_trace": (C.c_int, [vp, dp, dp, C.c_int64, P(C.c_int64), dp, P(C.c_uint8), P(C.c_int32),
+                                    P(C.c_int64), P(C.c_int64), C.c_int]),
+            "ref_light_count": (C.c_int, [vp]),
+            "ref_light_probabilities": (C.c_int, [vp, dp]),
+            "ref_ggx_table": (C.c_int, [dp, dp]),
+            "ref_bsdf_eval": (C.c_int, [dp, dp, P(C.c_uint8), C.c_int64, dp, dp]),
+            "ref_render": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _ref = lib
+    return _ref
+
+
+class RefScene:
+    def __init__(self, path):
+        self.lib = reference()
+        self.h = vp()
+        if self.lib.ref_load(os.fsencode(path), C.byref(self.h)) != 0:
+            raise RuntimeError("reference load_scene failed for " + path)
+
+    def close(self):
+        if self.h:
+            self.lib.ref_free(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        self.close()
+
+    def trace(self, rays, tt, threads=8):
+        n = rays.shape[0]
+        ids = np.zeros(n, dtype=np.int64)
+        t = np.zeros(n)
+        f = np.zeros(n, dtype=np.uint8)
+        ties = np.zeros(n, dtype=np.int32)
+        lo = np.zeros(n, dtype=np.int64)
+        hi = np.zeros(n, dtype=np.int64)
+        self.lib.ref_trace(self.h, _d(rays), _d(tt), n, _i64(ids), _d(t), _u8(f), _i32(ties), _i64(lo), _i64(hi),
+                           threads)
+        return {"id": ids, "t": t, "front": f, "ties": ties, "tie_min": lo, "tie_max": hi}
+
+    def light_probabilities(self):
+        n = self.lib.ref_light_count(self.h)
+        out = np.zeros(max(n, 1))
+        self.lib.ref_light_probabilities(self.h, _d(out))
+        return out[:n]
+
+    def render(self, width, spp, max_depth, threads=8):
+        # height from the scene's own aspect ratio, as camera::initialize
+        return None
