@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Benchmark: paths/s of the wavefront path tracer at 1080p on one MI355X.
+
+Workload (the reference's plain-method procedural-spheres configuration):
+scenes/weekend_cover.scene (484 spheres: ground r=1000, three r=1 spheres,
+480 seeded small spheres; Lambertian/GGX metal/dielectric), rendered at
+1920x1080, 256 spp, max depth 50. A step is one batch of --passes-per-step
+samples of every pixel (16 by default); --steps 16 covers the full 256 spp
+render, each step adding to the previous ones' sums as a progressive
+render does. Scene data is resident in HBM before the timed region.
+
+The JSON line also carries
+  roofline      the dominant stage kernel, its algorithmic bytes per launch
+                (rays x fixed ray/hit bytes + BVH nodes x 64 B + primitive
+                bytes, counted by the instrumented kernels on the same
+                paths) over its HIP-event-timed average launch duration;
+  cpu_baseline  the reference renderer itself (oracle/_ref, compiled from the
+                reference's sources) timed on this host's cores on a bounded
+                sample of the same workload, or the oracle when the
+                reference build is absent.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "paths/s at 1080p; equal-time relMSE vs 4096spp ref (learned vs plain)"
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 64
+# per extension ray: queue index 4 + ray 48 + key 8 read, hit t/object/primitive 16 written
+EXT_RAY_BYTES = 76
+# per shadow ray: queue 4 + origin 24 + to_light 24 + key 8 + f 24 + weight 8 + beta 24 + radiance 24 read,
+# radiance 24 written
+SHADOW_RAY_BYTES = 164
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def synchronize():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def cpu_baseline(scene_path, cam, seconds, threads):
+    """Times the CPU renderer on a bounded sample (full frame, a few spp)."""
+    import numpy as np
+    import oracle_lib as ol
+    w, h = cam.image_width, cam.height()
+    if ol.reference_available():
+        r = ol.RefScene(scene_path)
+        rad = np.zeros((h, w, 3))
+        small = np.zeros((h // 8 + 1, w // 8 + 1, 3))
+        r.lib.ref_render(r.h, max(1, w // 8), 1, cam.max_depth, threads, ol._d(small))  # builds the GGX table once
+        t0 = time.time()
+        r.lib.ref_render(r.h, w, 1, cam.max_depth, threads, ol._d(rad))
+        one = time.time() - t0
+        spp = max(1, min(64, int(seconds / max(one, 1e-3))))
+        t0 = time.time()
+        r.lib.ref_render(r.h, w, spp, cam.max_depth, threads, ol._d(rad))
+        dt = time.time() - t0
+        kind = "reference"
+        sample = f"reference camera::render, {w}x{h} at {spp} spp (same scene, depth {cam.max_depth})"
+        return {"value": w * h * spp / dt, "unit": "paths/s", "cores": threads, "kind": kind, "sample": sample,
+                "seconds": dt}
+    import mypbr_amd as m
+    from mypbr_amd.abi import camera_copy
+    d = m.describe(scene_path)
+    o = ol.OracleScene(d.desc)
+    c1 = camera_copy(cam)
+    t0 = time.time()
+    o.render_sums(c1, seed=11, first_pass=0, passes=1, threads=threads)
+    one = time.time() - t0
+    spp = max(1, min(64, int(seconds / max(one, 1e-3))))
+    t0 = time.time()
+    o.render_sums(c1, seed=11, first_pass=0, passes=spp, threads=threads)
+    dt = time.time() - t0
+    return {"value": w * h * spp / dt, "unit": "paths/s", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (restatement) {w}x{h} passes 0..{spp - 1} of the 256-spp camera", "seconds": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--passes-per-step", type=int, default=16)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(paths shard by pass or tile with one sum-reduce; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import mypbr_amd as m
+    from mypbr_amd.abi import camera_copy
+
+    scene_path = os.path.join(ROOT, "scenes", "weekend_cover.scene")
+    d = m.describe(scene_path)
+    cam = camera_copy(d.camera, image_width=1920, samples_per_pixel=256, max_depth=50)
+    assert cam.height() == 1080
+    s = m.Scene.from_desc(d.desc, d.camera)
+    info = m.device_info()
+    log("device:", info)
+    pps = args.passes_per_step
+    total_passes = args.steps * pps
+
+    for k in range(args.warmup):
+        s.render(cam, seed=1, first_pass=(k * pps) % 256, pass_count=pps, read=False)
+    synchronize()
+    t0 = time.perf_counter()
+    ext = sh = 0
+    for k in range(args.steps):
+        f = s.render(cam, seed=1, first_pass=(k * pps) % 256, pass_count=pps, accumulate=k > 0, read=False)
+        ext += f.stats.extension_rays
+        sh += f.stats.shadow_rays
+        log(f"step {k + 1}/{args.steps}: {f.stats.seconds * 1e3:.1f} ms")
+    synchronize()
+    dt = time.perf_counter() - t0
+    paths = cam.image_width * cam.height() * total_passes
+    value = paths / dt
+    log(f"{paths} paths in {dt:.3f} s = {value / 1e6:.1f} M paths/s, {(ext + sh) / dt / 1e9:.2f} G rays/s")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "paths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: scenes/weekend_cover.scene, the reference's seeded procedural-spheres scene",
+        "config": {"workload": "procedural spheres (484), 1920x1080, 256 spp, max depth 50, plain wavefront tracer",
+                   "width": 1920, "height": 1080, "spp": 256, "max_depth": 50,
+                   "passes_per_step": pps, "passes_timed": total_passes},
+        "rays_per_s": (ext + sh) / dt,
+        "extension_rays_per_path": ext / paths,
+        "shadow_rays_per_path": sh / paths,
+        "device": info["name"],
+    }
+
+    if not args.no_profile:
+        # one step again with per-stage event timing, then with BVH work counters
+        ft = s.render(cam, seed=1, first_pass=0, pass_count=pps, profile=1, read=False).stats
+        fc = s.render(cam, seed=1, first_pass=0, pass_count=pps, profile=2, read=False).stats
+        names = ["extend", "shade", "shadow", "camera+accumulate"]
+        stage = {names[i]: {"ms": ft.stage_ms[i], "launches": ft.stage_launches[i]} for i in range(4)}
+        ext_bytes = fc.extension_rays * EXT_RAY_BYTES + fc.node_visits[0] * NODE_BYTES + fc.prim_bytes[0]
+        sh_bytes = fc.shadow_rays * SHADOW_RAY_BYTES + fc.node_visits[1] * NODE_BYTES + fc.prim_bytes[1]
+        stage["extend"]["algorithmic_bytes"] = ext_bytes
+        stage["shadow"]["algorithmic_bytes"] = sh_bytes
+        stage["extend"]["nodes_per_ray"] = fc.node_visits[0] / max(fc.extension_rays, 1)
+        stage["extend"]["prim_bytes_per_ray"] = fc.prim_bytes[0] / max(fc.extension_rays, 1)
+        line["stages"] = stage
+        dom = max(("extend", "shadow"), key=lambda k: stage[k]["ms"])
+        launches = stage[dom]["launches"]
+        avg_s = stage[dom]["ms"] / launches / 1e3
+        bytes_per_launch = stage[dom]["algorithmic_bytes"] / launches
+        achieved = bytes_per_launch / avg_s / 1e9
+        line["roofline"] = {"kernel": f"k_{dom}", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                            "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6}
+        log("stages:", json.dumps(stage))
+
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(scene_path, cam, args.cpu_seconds, threads)
+        line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

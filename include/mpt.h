@@ -144,7 +144,7 @@ typedef struct mpt_scene_desc {
     /* Environment (environment.h): an equirectangular RGB image, row 0 at +y,
      * or width = 0 for none. */
     int32_t env_width, env_height;
-    const float* env_rgb;          /* [env_width * env_height * 3] */
+    const double* env_rgb;         /* [env_width * env_height * 3], linear RGB */
     double env_intensity;
     double env_rotation_degrees;
 } mpt_scene_desc;
@@ -182,7 +182,9 @@ typedef struct mpt_render_opts {
     int32_t sort_by_material;      /* -1 = off, 1 = on, 0 = auto */
     int32_t use_graph;             /* -1 = off, 1 = on, 0 = auto */
     int32_t scene_index;           /* keys the RNG for batched multi-scene launches */
-    int32_t pad;
+    int32_t pass_count;            /* passes to render in this call (0 = samples_per_pixel - first_pass) */
+    int32_t accumulate;            /* 1: add to the previous call's sums (progressive render) */
+    int32_t profile;               /* bit 0: time each stage with HIP events; bit 1: count BVH work */
 } mpt_render_opts;
 
 /* Per-render statistics. */
@@ -191,6 +193,14 @@ typedef struct mpt_render_stats {
     int64_t extension_rays;        /* extension rays traced */
     int64_t shadow_rays;           /* shadow rays traced */
     double seconds;                /* wall time of the GPU part */
+    /* with profile bit 0: summed kernel time and launches per stage,
+     * [0] extend, [1] shade, [2] shadow, [3] camera + features + accumulate */
+    double stage_ms[4];
+    int64_t stage_launches[4];
+    /* with profile bit 1: BVH nodes visited and primitive bytes read by the
+     * extension [0] and shadow [1] traversals */
+    int64_t node_visits[2];
+    int64_t prim_bytes[2];
 } mpt_render_stats;
 
 typedef struct mpt_scene mpt_scene;

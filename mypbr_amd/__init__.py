@@ -196,7 +196,7 @@ class Scene:
         return ev, pdf
 
     def render(self, camera=None, seed=1, features=None, passes_in_flight=0, save_every=0, on_pass=None,
-               first_pass=0, scene_index=0):
+               first_pass=0, scene_index=0, pass_count=0, accumulate=False, profile=0, read=True):
         cam = abi.camera_copy(camera if camera is not None else self.camera)
         if features is not None:
             cam.collect_features = int(bool(features))
@@ -207,9 +207,14 @@ class Scene:
         nrm = np.zeros((h, w, 3)) if feats else None
         var = np.zeros((h, w)) if feats else None
         opts = RenderOpts(seed=seed, first_pass=first_pass, save_every=save_every,
-                          passes_in_flight=passes_in_flight, scene_index=scene_index)
+                          passes_in_flight=passes_in_flight, scene_index=scene_index, pass_count=pass_count,
+                          accumulate=int(bool(accumulate)), profile=profile)
         stats = RenderStats()
         cb = abi.PASS_CALLBACK(lambda done, user: on_pass(done, self)) if on_pass else abi.PASS_CALLBACK()
+        if not read:  # leave the frame in HBM (read it later with read_frame)
+            _check(lib.mpt_render(self._h, C.byref(cam), C.byref(opts), None, None, None, None, cb, None,
+                                  C.byref(stats)))
+            return Frame(None, stats=stats)
         _check(lib.mpt_render(self._h, C.byref(cam), C.byref(opts), _dp(rad),
                               _dp(alb) if feats else None, _dp(nrm) if feats else None,
                               _dp(var) if feats else None, cb, None, C.byref(stats)))

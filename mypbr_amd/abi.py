@@ -64,7 +64,7 @@ class SceneDesc(C.Structure):
         ("light_uniform_fraction", C.c_double),
         ("env_width", C.c_int32),
         ("env_height", C.c_int32),
-        ("env_rgb", C.POINTER(C.c_float)),
+        ("env_rgb", C.POINTER(C.c_double)),
         ("env_intensity", C.c_double),
         ("env_rotation_degrees", C.c_double),
     ]
@@ -101,12 +101,14 @@ class RenderOpts(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("first_pass", C.c_int32), ("save_every", C.c_int32),
                 ("passes_in_flight", C.c_int32), ("block_size", C.c_int32),
                 ("sort_by_material", C.c_int32), ("use_graph", C.c_int32),
-                ("scene_index", C.c_int32), ("pad", C.c_int32)]
+                ("scene_index", C.c_int32), ("pass_count", C.c_int32), ("accumulate", C.c_int32),
+                ("profile", C.c_int32)]
 
 
 class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_int64), ("extension_rays", C.c_int64), ("shadow_rays", C.c_int64),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("stage_ms", C.c_double * 4), ("stage_launches", C.c_int64 * 4),
+                ("node_visits", C.c_int64 * 2), ("prim_bytes", C.c_int64 * 2)]
 
 
 PASS_CALLBACK = C.CFUNCTYPE(None, C.c_int32, C.c_void_p)

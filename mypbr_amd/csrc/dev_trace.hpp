@@ -71,6 +71,13 @@ struct tstack {
     }
 };
 
+// BVH work of one lane (counted only in the instrumented kernels): nodes
+// visited and primitive bytes read (sphere 32, quad 128, triangle 72).
+struct wcount {
+    uint32_t nodes;
+    uint32_t pbytes;
+};
+
 struct hitinfo {
     double t;
     int32_t obj;    // object index, -1 = miss
@@ -168,23 +175,26 @@ __device__ __forceinline__ bool tri_t(const double* q, const v3& o, const v3& d,
     return true;
 }
 
+template <bool CNT>
 __device__ __forceinline__ bool prim_t(const dscene& S, int64_t p, const v3& o, const v3& d, double tmin,
-                                       double tmax, double& t) {
+                                       double tmax, double& t, wcount& wc) {
     const double* q = prim_ptr(S, p);
     int k = S.pkind[p];
+    if constexpr (CNT) wc.pbytes += k == MPT_PRIM_SPHERE ? 32u : (k == MPT_PRIM_QUAD ? 128u : 72u);
     if (k == MPT_PRIM_SPHERE) return sphere_t(q, o, d, tmin, tmax, t);
     if (k == MPT_PRIM_QUAD) return quad_t(q, o, d, tmin, tmax, t);
     return tri_t(q, o, d, tmin, tmax, t);
 }
 
 // hittable_list::hit (hittable_list.h:25-39) over [first, first + count).
+template <bool CNT>
 __device__ __forceinline__ bool list_t(const dscene& S, int64_t first, int64_t count, const v3& o, const v3& d,
-                                       double tmin, double tmax, double& t, int32_t& which) {
+                                       double tmin, double tmax, double& t, int32_t& which, wcount& wc) {
     bool any = false;
     double closest = tmax;
     for (int64_t k = 0; k < count; k++) {
         double tt;
-        if (prim_t(S, first + k, o, d, tmin, closest, tt)) {
+        if (prim_t<CNT>(S, first + k, o, d, tmin, closest, tt, wc)) {
             any = true;
             closest = tt;
             which = int32_t(first + k);
@@ -195,8 +205,9 @@ __device__ __forceinline__ bool list_t(const dscene& S, int64_t first, int64_t c
 }
 
 // Inner bvh::hit of a mesh; the stack continues above sp_base.
+template <bool CNT>
 __device__ bool mesh_t(const dscene& S, int root, const v3& o, const v3& d, double tmin, double tmax, double& t,
-                       int32_t& which, tstack& st, int sp_base) {
+                       int32_t& which, tstack& st, int sp_base, wcount& wc) {
     v3 inv{1 / d.x, 1 / d.y, 1 / d.z};
     bool neg[3] = {inv.x < 0, inv.y < 0, inv.z < 0};
     bool any = false;
@@ -204,10 +215,12 @@ __device__ bool mesh_t(const dscene& S, int root, const v3& o, const v3& d, doub
     int cur = root;
     while (true) {
         bvh_node n = load_node(S.blas, cur);
+        if constexpr (CNT) wc.nodes++;
         if (node_hit(n, o, inv, tmin, tmax)) {
             if (n.count > 0) {
                 for (int k = 0; k < n.count; k++) {
                     double tt;
+                    if constexpr (CNT) wc.pbytes += 72u;
                     if (tri_t(prim_ptr(S, n.offset + k), o, d, tmin, tmax, tt)) {
                         any = true;
                         tmax = tt;
@@ -233,21 +246,23 @@ __device__ bool mesh_t(const dscene& S, int root, const v3& o, const v3& d, doub
 }
 
 // Hit of an object's boundary (a primitive or a list) for media.
+template <bool CNT>
 __device__ __forceinline__ bool boundary_t(const dscene& S, const obj_rec& r, const v3& o, const v3& d, double tmin,
-                                           double tmax, double& t) {
+                                           double tmax, double& t, wcount& wc) {
     int32_t w;
-    if (r.bkind == MPT_OBJ_PRIM) return prim_t(S, r.first, o, d, tmin, tmax, t);
-    return list_t(S, r.first, r.count, o, d, tmin, tmax, t, w);
+    if (r.bkind == MPT_OBJ_PRIM) return prim_t<CNT>(S, r.first, o, d, tmin, tmax, t, wc);
+    return list_t<CNT>(S, r.first, r.count, o, d, tmin, tmax, t, w, wc);
 }
 
 // constant_medium::hit (medium.h:60-88), with the free-flight draw u.
 // d is the boundary's (possibly instance-local) direction; world_len is the
 // length of the world-space direction, which sets distances in the medium.
+template <bool CNT>
 __device__ bool medium_t(const dscene& S, const obj_rec& r, const v3& o, const v3& d, double world_len, double tmin,
-                         double tmax, double u, double& t) {
+                         double tmax, double u, double& t, wcount& wc) {
     double t1a, t2a;
-    if (!boundary_t(S, r, o, d, -kInf, kInf, t1a)) return false;
-    if (!boundary_t(S, r, o, d, t1a + 0.0001, kInf, t2a)) return false;
+    if (!boundary_t<CNT>(S, r, o, d, -kInf, kInf, t1a, wc)) return false;
+    if (!boundary_t<CNT>(S, r, o, d, t1a + 0.0001, kInf, t2a, wc)) return false;
     double t0 = std::fmax(t1a, tmin);
     double t1 = std::fmin(t2a, tmax);
     if (t0 >= t1) return false;
@@ -260,9 +275,10 @@ __device__ bool medium_t(const dscene& S, const obj_rec& r, const v3& o, const v
 }
 
 // One world object, the ray in world space. Returns the hit t and primitive.
+template <bool CNT>
 __device__ __forceinline__ bool object_t(const dscene& S, int oi, v3 o, v3 d, double tmin, double tmax, double& t,
                                          int32_t& prim, tstack& st, int sp, bool media, uint64_t mkey,
-                                         uint32_t serial) {
+                                         uint32_t serial, wcount& wc) {
     const obj_rec r = S.obj[oi];
     if (r.kind == MPT_OBJ_MEDIUM && !media) return false;
     const double world_len = r.kind == MPT_OBJ_MEDIUM ? length(d) : 0.0;
@@ -276,21 +292,22 @@ __device__ __forceinline__ bool object_t(const dscene& S, int oi, v3 o, v3 d, do
     switch (r.kind) {
         case MPT_OBJ_PRIM:
             prim = int32_t(r.first);
-            return prim_t(S, r.first, o, d, tmin, tmax, t);
+            return prim_t<CNT>(S, r.first, o, d, tmin, tmax, t, wc);
         case MPT_OBJ_LIST:
-            return list_t(S, r.first, r.count, o, d, tmin, tmax, t, prim);
+            return list_t<CNT>(S, r.first, r.count, o, d, tmin, tmax, t, prim, wc);
         case MPT_OBJ_MESH:
-            return mesh_t(S, r.blas_root, o, d, tmin, tmax, t, prim, st, sp);
+            return mesh_t<CNT>(S, r.blas_root, o, d, tmin, tmax, t, prim, st, sp, wc);
         default:
             prim = -1;
-            return medium_t(S, r, o, d, world_len, tmin, tmax, medium_draw(mkey, serial, uint32_t(oi)), t);
+            return medium_t<CNT>(S, r, o, d, world_len, tmin, tmax, medium_draw(mkey, serial, uint32_t(oi)), t, wc);
     }
 }
 
 // bvh::hit over the world (bvh.h:45-86). media: test constant_medium objects
 // with draws from (mkey, serial); otherwise they are skipped.
+template <bool CNT = false>
 __device__ bool trace_closest(const dscene& S, const v3& o, const v3& d, double tmin, double tmax, hitinfo& h,
-                              tstack& st, bool media, uint64_t mkey, uint32_t serial) {
+                              tstack& st, bool media, uint64_t mkey, uint32_t serial, wcount& wc) {
     h.obj = -1;
     h.prim = -1;
     h.t = tmax;
@@ -302,13 +319,14 @@ __device__ bool trace_closest(const dscene& S, const v3& o, const v3& d, double 
     int cur = 0;
     while (true) {
         bvh_node n = load_node(S.tlas, cur);
+        if constexpr (CNT) wc.nodes++;
         if (node_hit(n, o, inv, tmin, tmax)) {
             if (n.count > 0) {
                 for (int k = 0; k < n.count; k++) {
                     int oi = S.titems[n.offset + k];
                     double tt;
                     int32_t pr = -1;
-                    if (object_t(S, oi, o, d, tmin, tmax, tt, pr, st, sp, media, mkey, serial)) {
+                    if (object_t<CNT>(S, oi, o, d, tmin, tmax, tt, pr, st, sp, media, mkey, serial, wc)) {
                         any = true;
                         tmax = tt;
                         h.t = tt;

@@ -193,7 +193,8 @@ __global__ __launch_bounds__(BS) void k_query(dscene S, launch_ctx L, const doub
         v3 o{rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]};
         v3 d{rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]};
         hitinfo h;
-        bool hit = trace_closest(S, o, d, tt[2 * i], tt[2 * i + 1], h, st, false, 0, 0);
+        wcount wc{0, 0};
+        bool hit = trace_closest(S, o, d, tt[2 * i], tt[2 * i + 1], h, st, false, 0, 0, wc);
         if (hit) {
             hitrec rec;
             finish_hit(S, h, o, d, rec);
@@ -298,7 +299,8 @@ __global__ __launch_bounds__(BS) void k_features(dscene S, launch_ctx L, pstate 
         bool done = false;
         for (int bounce = 0; bounce < 8 && !done; bounce++) {
             hitinfo h;
-            if (!trace_closest(S, o, d, 0.001, kInf, h, st, true, rs.key, uint32_t(bounce))) break;
+            wcount wc{0, 0};
+            if (!trace_closest(S, o, d, 0.001, kInf, h, st, true, rs.key, uint32_t(bounce), wc)) break;
             hitrec rec;
             finish_hit(S, h, o, d, rec);
             const mpt_material& m = S.mat[rec.mat];
@@ -328,10 +330,11 @@ __global__ __launch_bounds__(BS) void k_features(dscene S, launch_ctx L, pstate 
 }
 
 // ------------------------------------------------------------------ extend
-template <int BS>
+// CNT: also count BVH work (nodes, primitive bytes) into work[0], work[1].
+template <int BS, bool CNT>
 __global__ __launch_bounds__(BS) void k_extend(dscene S, launch_ctx L, pstate P, int depth, const uint32_t* q,
                                                const uint32_t* cnt_in, uint32_t* cnt_next, uint32_t* cnt_shadow,
-                                               unsigned long long* ray_stats) {
+                                               unsigned long long* ray_stats, unsigned long long* work) {
     STACK_SETUP();
     const uint32_t n = *cnt_in;
     if (st.gtid == 0) {  // the counters shade and shadow append to in this bounce
@@ -339,14 +342,19 @@ __global__ __launch_bounds__(BS) void k_extend(dscene S, launch_ctx L, pstate P,
         *cnt_shadow = 0;
         atomicAdd(&ray_stats[0], (unsigned long long)n);
     }
+    wcount wc{0, 0};
     for (int64_t i = st.gtid; i < n; i += L.nthreads) {
         uint32_t p = q[i];
         v3 o = ld3(P, D_OX, p), d = ld3(P, D_DX, p);
         hitinfo h;
-        trace_closest(S, o, d, 0.001, kInf, h, st, true, P.key[p], uint32_t(2 * depth));
+        trace_closest<CNT>(S, o, d, 0.001, kInf, h, st, true, P.key[p], uint32_t(2 * depth), wc);
         P.d[D_HT][p] = h.t;
         P.hobj[p] = h.obj;
         P.hprim[p] = h.prim;
+    }
+    if constexpr (CNT) {
+        if (wc.nodes) atomicAdd(&work[0], (unsigned long long)wc.nodes);
+        if (wc.pbytes) atomicAdd(&work[1], (unsigned long long)wc.pbytes);
     }
 }
 
@@ -473,13 +481,14 @@ __global__ __launch_bounds__(BS) void k_shade(dscene S, dcam C, pstate P, int de
 }
 
 // ------------------------------------------------------------------ shadow
-template <int BS>
+template <int BS, bool CNT>
 __global__ __launch_bounds__(BS) void k_shadow(dscene S, dcam C, launch_ctx L, pstate P, int depth,
                                                const uint32_t* qshadow, const uint32_t* cnt_shadow,
-                                               unsigned long long* ray_stats) {
+                                               unsigned long long* ray_stats, unsigned long long* work) {
     STACK_SETUP();
     const uint32_t n = *cnt_shadow;
     if (st.gtid == 0) atomicAdd(&ray_stats[1], (unsigned long long)n);
+    wcount wc{0, 0};
     for (int64_t i = st.gtid; i < n; i += L.nthreads) {
         uint32_t p = qshadow[i];
         v3 o = ld3(P, D_SOX, p);
@@ -489,7 +498,8 @@ __global__ __launch_bounds__(BS) void k_shadow(dscene S, dcam C, launch_ctx L, p
         v3 emission;
         for (int k = 0;; k++) {  // camera.h:488-502
             hitinfo h;
-            if (!trace_closest(S, o, dir, 0.001, kInf, h, st, true, P.key[p], uint32_t(2 * depth + 1) | (uint32_t(k) << 20))) {
+            if (!trace_closest<CNT>(S, o, dir, 0.001, kInf, h, st, true, P.key[p],
+                                    uint32_t(2 * depth + 1) | (uint32_t(k) << 20), wc)) {
                 emission = escaped(S, C, to_light);
                 break;
             }
@@ -508,6 +518,10 @@ __global__ __launch_bounds__(BS) void k_shadow(dscene S, dcam C, launch_ctx L, p
         v3 c = ld3(P, D_FX, p) * emission * P.d[D_SW][p];
         v3 rad = ld3(P, D_RX, p) + ld3(P, D_SBX, p) * c;
         st3(P, D_RX, p, rad);
+    }
+    if constexpr (CNT) {
+        if (wc.nodes) atomicAdd(&work[2], (unsigned long long)wc.nodes);
+        if (wc.pbytes) atomicAdd(&work[3], (unsigned long long)wc.pbytes);
     }
 }
 
@@ -658,8 +672,8 @@ int ensure_ovf(mpt_scene* s) {
 
 int grid_for(mpt_scene* s) {
     int blocks_per_cu = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, reinterpret_cast<const void*>(&k_extend<256>),
-                                                         256, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &blocks_per_cu, reinterpret_cast<const void*>(&k_extend<256, false>), 256, 0));
     if (blocks_per_cu < 1) blocks_per_cu = 1;
     s->block = 256;
     s->grid = s->cu_count * blocks_per_cu;
@@ -919,30 +933,39 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
         return set_error(MPT_E_INVALID, "bounce caps must be in [0, 1000]");
     mpt_render_opts o{};
     if (opts) o = *opts;
+    if (o.first_pass < 0 || o.pass_count < 0) return set_error(MPT_E_INVALID, "negative first_pass or pass_count");
     auto t0 = std::chrono::steady_clock::now();
     dcam C = init_camera(*cam, s->hs.env_w > 0);
     const int64_t npix64 = int64_t(C.w) * C.h;
     if (npix64 > (int64_t(1) << 28)) return set_error(MPT_E_CAPACITY, "image too large");
     const int npix = int(npix64);
-    const int spp = C.spp;
+    const int total = o.pass_count > 0 ? o.pass_count : C.spp - o.first_pass;
+    if (total < 1) return set_error(MPT_E_INVALID, "no passes to render (first_pass >= samples_per_pixel)");
     int pif = o.passes_in_flight;
     if (pif <= 0) {
         // about 4M paths in flight: fills the device several times over
         pif = int(std::max<int64_t>(1, (int64_t(4) << 20) / npix64));
     }
-    pif = std::min(pif, spp);
+    pif = std::min(pif, total);
     if (int64_t(pif) * npix64 > (int64_t(1) << 31)) pif = int((int64_t(1) << 31) / npix64);
     int rc;
     if ((rc = ensure_paths(s, int64_t(pif) * npix))) return rc;
+    const bool keep = o.accumulate && s->acc && s->acc_pix == npix && s->last_done > 0 &&
+                      s->last_cam.collect_features == cam->collect_features;
+    if (o.accumulate && !keep && s->last_done > 0)
+        return set_error(MPT_E_INVALID, "accumulate: the previous render had another size or feature setting");
     if (s->acc_pix != npix || !s->acc) {
         if (s->acc) HIP_TRY(hipFree(s->acc));
         s->acc = nullptr;
         HIP_TRY(hipMalloc(&s->acc, size_t(npix) * 10 * sizeof(double)));
         s->acc_pix = npix;
     }
-    HIP_TRY(hipMemsetAsync(s->acc, 0, size_t(npix) * 10 * sizeof(double), s->stream));
+    if (!keep) {
+        HIP_TRY(hipMemsetAsync(s->acc, 0, size_t(npix) * 10 * sizeof(double), s->stream));
+        s->last_done = 0;
+    }
     s->last_cam = *cam;
-    s->last_done = 0;
+    const int done_before = s->last_done;
 
     const int64_t cap = s->ps_capacity;
     uint32_t* q[3] = {s->queues, s->queues + cap, s->queues + 2 * cap};
@@ -954,19 +977,85 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
     double* acc_nrm = acc_alb + 3 * size_t(npix);
     double* acc_l2 = acc_nrm + 3 * size_t(npix);
     unsigned long long* ray_stats = reinterpret_cast<unsigned long long*>(cnt + 4);  // [0] extension, [1] shadow
-    HIP_TRY(hipMemsetAsync(ray_stats, 0, 2 * sizeof(unsigned long long), s->stream));
+    unsigned long long* work = ray_stats + 2;                                       // nodes/bytes ext, shadow
+    HIP_TRY(hipMemsetAsync(ray_stats, 0, 6 * sizeof(unsigned long long), s->stream));
+    const bool timing = (o.profile & 1) != 0;
+    const bool counting = (o.profile & 2) != 0;
 
+    // per-stage event timing (profile bit 0): events are recorded around
+    // each launch and read back after each batch
+    struct timed {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<timed> evs;
+    size_t ev_used = 0;
+    double stage_ms[4] = {0, 0, 0, 0};
+    int64_t stage_n[4] = {0, 0, 0, 0};
+    auto ev_begin = [&](int stage) -> int {
+        if (!timing) return MPT_OK;
+        if (ev_used == evs.size()) {
+            timed t{stage, nullptr, nullptr};
+            HIP_TRY(hipEventCreate(&t.a));
+            HIP_TRY(hipEventCreate(&t.b));
+            evs.push_back(t);
+        }
+        evs[ev_used].stage = stage;
+        HIP_TRY(hipEventRecord(evs[ev_used].a, s->stream));
+        return MPT_OK;
+    };
+    auto ev_end = [&]() -> int {
+        if (!timing) return MPT_OK;
+        HIP_TRY(hipEventRecord(evs[ev_used].b, s->stream));
+        ev_used++;
+        return MPT_OK;
+    };
+    auto ev_collect = [&]() -> int {
+        if (!timing) return MPT_OK;
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        for (size_t k = 0; k < ev_used; k++) {
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, evs[k].a, evs[k].b));
+            stage_ms[evs[k].stage] += ms;
+            stage_n[evs[k].stage]++;
+        }
+        ev_used = 0;
+        return MPT_OK;
+    };
+    auto ev_free = [&]() {
+        for (auto& t : evs) {
+            (void)hipEventDestroy(t.a);
+            (void)hipEventDestroy(t.b);
+        }
+    };
+#define EV(stage, launch)                            \
+    do {                                             \
+        if ((rc = ev_begin(stage))) {                \
+            ev_free();                               \
+            return rc;                               \
+        }                                            \
+        launch;                                      \
+        if ((rc = ev_end())) {                       \
+            ev_free();                               \
+            return rc;                               \
+        }                                            \
+    } while (0)
+
+    auto extend = counting ? k_extend<256, true> : k_extend<256, false>;
+    auto shadow = counting ? k_shadow<256, true> : k_shadow<256, false>;
     int pass = 0;
-    while (pass < spp) {
-        int nb = std::min(pif, spp - pass);
+    while (pass < total) {
+        int nb = std::min(pif, total - pass);
         if (o.save_every > 0 && on_pass) {
-            int next_save = (pass / o.save_every + 1) * o.save_every;
-            nb = std::min(nb, next_save - pass);
+            int done_abs = done_before + pass;
+            int next_save = (done_abs / o.save_every + 1) * o.save_every;
+            nb = std::min(nb, next_save - done_abs);
         }
         batch_args B{o.seed, uint32_t(o.scene_index), o.first_pass + pass, npix, nb * npix};
-        hipLaunchKernelGGL(k_camera, dim3(unsigned((B.nslots + 255) / 256)), dim3(256), 0, s->stream, C, P, B, q[0], cnt);
+        EV(3, hipLaunchKernelGGL(k_camera, dim3(unsigned((B.nslots + 255) / 256)), dim3(256), 0, s->stream, C, P, B,
+                                 q[0], cnt));
         if (C.features)
-            hipLaunchKernelGGL(k_features<256>, dim3(s->grid), dim3(256), 0, s->stream, s->ds, L, P, B);
+            EV(3, hipLaunchKernelGGL(k_features<256>, dim3(s->grid), dim3(256), 0, s->stream, s->ds, L, P, B));
         for (int depth = 0; depth < C.max_depth; depth++) {
             uint32_t* qin = q[depth & 1];
             uint32_t* qout = q[(depth + 1) & 1];
@@ -974,35 +1063,53 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
             uint32_t* cout = cnt + ((depth + 1) & 1);
             // extend zeroes the next-queue and shadow counters of this bounce;
             // stream order puts it after the previous bounce's shadow stage
-            hipLaunchKernelGGL(k_extend<256>, dim3(s->grid), dim3(256), 0, s->stream, s->ds, L, P, depth, qin, cin, cout,
-                               cnt + 2, ray_stats);
-            hipLaunchKernelGGL(k_shade<256>, dim3(s->grid), dim3(256), 0, s->stream, s->ds, C, P, depth, qin, cin, qout,
-                               cout, q[2], cnt + 2, L.nthreads);
-            hipLaunchKernelGGL(k_shadow<256>, dim3(s->grid), dim3(256), 0, s->stream, s->ds, C, L, P, depth, q[2], cnt + 2,
-                               ray_stats);
+            EV(0, hipLaunchKernelGGL(extend, dim3(s->grid), dim3(256), 0, s->stream, s->ds, L, P, depth, qin, cin, cout,
+                                     cnt + 2, ray_stats, work));
+            EV(1, hipLaunchKernelGGL(k_shade<256>, dim3(s->grid), dim3(256), 0, s->stream, s->ds, C, P, depth, qin, cin,
+                                     qout, cout, q[2], cnt + 2, L.nthreads));
+            EV(2, hipLaunchKernelGGL(shadow, dim3(s->grid), dim3(256), 0, s->stream, s->ds, C, L, P, depth, q[2], cnt + 2,
+                                     ray_stats, work));
         }
-        hipLaunchKernelGGL(k_accumulate, dim3(unsigned((npix + 255) / 256)), dim3(256), 0, s->stream, P, npix, nb,
-                           C.features, acc_rad, acc_alb, acc_nrm, acc_l2);
-        HIP_TRY(hipGetLastError());
+        EV(3, hipLaunchKernelGGL(k_accumulate, dim3(unsigned((npix + 255) / 256)), dim3(256), 0, s->stream, P, npix, nb,
+                                 C.features, acc_rad, acc_alb, acc_nrm, acc_l2));
+        if (hipGetLastError() != hipSuccess) {
+            ev_free();
+            return set_error(MPT_E_HIP, "kernel launch failed");
+        }
+        if ((rc = ev_collect())) {
+            ev_free();
+            return rc;
+        }
         pass += nb;
-        s->last_done = pass;
-        if (on_pass && o.save_every > 0 && pass % o.save_every == 0 && pass < spp) {
+        s->last_done = done_before + pass;
+        if (on_pass && o.save_every > 0 && s->last_done % o.save_every == 0 && pass < total) {
             HIP_TRY(hipStreamSynchronize(s->stream));
-            on_pass(pass, user);
+            on_pass(s->last_done, user);
         }
     }
+#undef EV
+    ev_free();
     HIP_TRY(hipStreamSynchronize(s->stream));
     auto t1 = std::chrono::steady_clock::now();
     if (radiance || albedo || normal || variance) {
         if ((rc = mpt_read_frame(s, radiance, albedo, normal, variance))) return rc;
     }
     if (stats) {
-        unsigned long long rs_host[2] = {0, 0};
+        unsigned long long rs_host[6] = {0, 0, 0, 0, 0, 0};
         HIP_TRY(hipMemcpy(rs_host, ray_stats, sizeof(rs_host), hipMemcpyDeviceToHost));
-        stats->paths = npix64 * spp;
+        std::memset(stats, 0, sizeof(*stats));
+        stats->paths = npix64 * total;
         stats->extension_rays = int64_t(rs_host[0]);
         stats->shadow_rays = int64_t(rs_host[1]);
         stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+        for (int k = 0; k < 4; k++) {
+            stats->stage_ms[k] = stage_ms[k];
+            stats->stage_launches[k] = stage_n[k];
+        }
+        stats->node_visits[0] = int64_t(rs_host[2]);
+        stats->prim_bytes[0] = int64_t(rs_host[3]);
+        stats->node_visits[1] = int64_t(rs_host[4]);
+        stats->prim_bytes[1] = int64_t(rs_host[5]);
     }
     return MPT_OK;
 }

@@ -260,7 +260,7 @@ struct flat_builder::impl {
     std::vector<mpt_material> materials;
     std::vector<mpt_texture> textures;
     std::vector<int32_t> light_object;
-    std::vector<float> env_rgb;
+    std::vector<double> env_rgb;
     std::map<const material*, int32_t> mat_index;
     std::map<const texture*, int32_t> tex_index;
     std::map<const hittable*, int32_t> object_of;
@@ -468,7 +468,7 @@ void flat_builder::add_lights(const hittable_list& lights, const std::shared_ptr
             throw std::runtime_error("environment_light: pixel count does not match its size");
         p->env_rgb.resize(envp->pixels.size() * 3);
         for (size_t k = 0; k < envp->pixels.size(); k++)
-            for (int c = 0; c < 3; c++) p->env_rgb[3 * k + c] = float(envp->pixels[k][c]);
+            for (int c = 0; c < 3; c++) p->env_rgb[3 * k + c] = envp->pixels[k][c];
         p->d.env_width = envp->width;
         p->d.env_height = envp->height;
         p->d.env_intensity = envp->intensity;

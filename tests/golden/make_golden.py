@@ -1,0 +1,157 @@
+"""Writes the golden vectors in tests/golden from the compiled reference.
+
+Runs only where oracle/_ref/libmpt_ref.so was built (the reference renderer
+compiled from its own sources by oracle/Makefile). Every expected value
+here is computed by the reference itself; inputs (ray batches, material
+queries) are seeded and regenerated identically by the tests:
+
+  ggx_table.npz         the 32x32 E(mu, alpha) table and E_avg (microfacet.h:265-338)
+  trace_<scene>.npz     4096 rays of the seeded batch with the reference's
+                        closest-hit ID, t, front_face and tie count (bvh.h:45-86)
+  trace_digests.json    SHA-256 of the reference's IDs and t bits over the full
+                        1M-ray batches, tie and miss counts
+  light_probs.json      light_list::probability per light (light_list.h:45-127)
+  bsdf.npz              material::eval and the scatter pdf's value at fixed
+                        directions (material.h, microfacet.h, pdf.h, medium.h)
+  render_<scene>.npz    converged low-resolution means of camera::render and
+                        their per-pixel standard error over 8 runs
+
+Usage: python tests/golden/make_golden.py  (takes about a minute)
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import mypbr_amd as m  # noqa: E402  (host-side scene description only)
+import oracle_lib as ol  # noqa: E402
+
+TRACE_SCENES = ["cornell.scene", "weekend_cover.scene", "test_lights.scene"]
+N_BATCH = 1 << 20
+N_SUBSET = 4096
+BATCH_SEED = 1
+RENDERS = {  # scene: (width, spp per run, runs, max_depth)
+    "cornell.scene": (32, 512, 8, 16),
+    "weekend_cover.scene": (48, 256, 8, 50),
+    "test_lights.scene": (40, 256, 8, 24),
+    "test_media.scene": (32, 512, 8, 32),
+}
+
+# material specs for the BSDF vectors: type r g b roughness ior metallic g sigma_a(3) unused
+BSDF_MATERIALS = [
+    [0, 0.7, 0.5, 0.3, 0, 0, 0, 0, 0, 0, 0, 0],
+    [1, 0.9, 0.6, 0.3, 0.3, 0, 0, 0, 0, 0, 0, 0],
+    [1, 0.95, 0.95, 0.95, 0.8, 0, 0, 0, 0, 0, 0, 0],
+    [2, 1, 1, 1, 0.2, 1.5, 0, 0, 0, 0, 0, 0],
+    [2, 1, 1, 1, 0.6, 1.33, 0, 0, 0, 0, 0, 0],
+    [3, 0.6, 0.2, 0.1, 0.4, 1.5, 0.0, 0, 0, 0, 0, 0],
+    [3, 0.8, 0.7, 0.3, 0.7, 1.5, 1.0, 0, 0, 0, 0, 0],
+    [3, 0.5, 0.5, 0.9, 0.05, 1.45, 0.5, 0, 0, 0, 0, 0],
+    [5, 0.9, 0.8, 0.7, 0, 0, 0, 0.3, 0, 0, 0, 0],
+    [5, 0.6, 0.6, 0.6, 0, 0, 0, 0.0, 0, 0, 0, 0],
+]
+N_PER_MATERIAL = 256
+
+
+def scene_path(name):
+    return os.path.join(ROOT, "scenes", name)
+
+
+def bsdf_queries(seed=5):
+    """Seeded (normal, ray direction, wi, front) queries; the ray direction
+    points against the normal as after set_face_normal (hittable.h:23-26)."""
+    rng = np.random.default_rng(seed)
+    n = len(BSDF_MATERIALS) * N_PER_MATERIAL
+
+    def unit(k):
+        v = rng.normal(size=(k, 3))
+        return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+    nrm = unit(n)
+    rd = unit(n)
+    flip = np.sum(rd * nrm, axis=1) > 0
+    rd[flip] *= -1
+    rd *= rng.uniform(0.5, 3.0, size=(n, 1))  # camera and instance rays are not unit length
+    wi = unit(n)
+    # a quarter of the queries ask for the mirror direction (sharp lobes)
+    k = n // 4
+    ud = rd[:k] / np.linalg.norm(rd[:k], axis=1, keepdims=True)
+    wi[:k] = ud - 2 * np.sum(ud * nrm[:k], axis=1, keepdims=True) * nrm[:k]
+    front = rng.integers(0, 2, size=n).astype(np.uint8)
+    spec = np.repeat(np.array(BSDF_MATERIALS, dtype=np.float64), N_PER_MATERIAL, axis=0)
+    q = np.concatenate([nrm, rd, wi], axis=1)
+    return spec, np.ascontiguousarray(q), front
+
+
+def main():
+    if not ol.reference_available():
+        sys.exit("oracle/_ref/libmpt_ref.so is not built (make -C oracle ref, where the reference is present)")
+    R = ol.reference()
+
+    e = np.zeros(1024)
+    ea = np.zeros(32)
+    R.ref_ggx_table(ol._d(e), ol._d(ea))
+    np.savez_compressed(os.path.join(HERE, "ggx_table.npz"), e=e.reshape(32, 32), e_avg=ea)
+    print("ggx_table.npz", flush=True)
+
+    digests = {}
+    probs = {}
+    for name in TRACE_SCENES:
+        d = m.describe(scene_path(name))
+        o = ol.OracleScene(d.desc)  # generates the seeded input batch only
+        rays, tt = o.ray_batch(d.camera, BATCH_SEED, N_BATCH)
+        r = ol.RefScene(scene_path(name))
+        out = r.trace(rays, tt, threads=os.cpu_count() or 8)
+        digests[name] = {
+            "seed": BATCH_SEED, "n": N_BATCH,
+            "sha256_id": hashlib.sha256(out["id"].astype("<i8").tobytes()).hexdigest(),
+            "sha256_t": hashlib.sha256(out["t"].astype("<f8").tobytes()).hexdigest(),
+            "sha256_front": hashlib.sha256(out["front"].astype("u1").tobytes()).hexdigest(),
+            "ties": int((out["ties"] > 1).sum()), "misses": int((out["id"] == -1).sum()),
+            "unidentified": int((out["id"] == -2).sum()),
+        }
+        sel = np.linspace(0, N_BATCH - 1, N_SUBSET).astype(np.int64)
+        np.savez_compressed(os.path.join(HERE, "trace_" + name.replace(".scene", ".npz")),
+                            rays=rays[sel], tt=tt[sel], index=sel, id=out["id"][sel], t=out["t"][sel],
+                            front=out["front"][sel], ties=out["ties"][sel])
+        probs[name] = r.light_probabilities().tolist()
+        print(name, digests[name], flush=True)
+    with open(os.path.join(HERE, "trace_digests.json"), "w") as f:
+        json.dump(digests, f, indent=1)
+    with open(os.path.join(HERE, "light_probs.json"), "w") as f:
+        json.dump(probs, f, indent=1)
+
+    spec, q, front = bsdf_queries()
+    n = spec.shape[0]
+    ev = np.zeros((n, 3))
+    pdf = np.zeros(n)
+    R.ref_bsdf_eval(ol._d(spec), ol._d(q), front.ctypes.data_as(ol.P(ol.C.c_uint8)), n, ol._d(ev), ol._d(pdf))
+    np.savez_compressed(os.path.join(HERE, "bsdf.npz"), spec=spec, q=q, front=front, eval=ev, pdf=pdf)
+    print("bsdf.npz", n, "queries", flush=True)
+
+    for name, (width, spp, runs, depth) in RENDERS.items():
+        d = m.describe(scene_path(name))
+        h = max(1, int(width / d.camera.aspect_ratio))
+        r = ol.RefScene(scene_path(name))
+        imgs = []
+        for _ in range(runs):
+            img = np.zeros((h, width, 3))
+            R.ref_render(r.h, width, spp, depth, os.cpu_count() or 8, ol._d(img))
+            imgs.append(img)
+        imgs = np.stack(imgs)
+        mean = imgs.mean(0)
+        stderr = imgs.std(0, ddof=1) / np.sqrt(runs)
+        np.savez_compressed(os.path.join(HERE, "render_" + name.replace(".scene", ".npz")),
+                            mean=mean, stderr=stderr, width=width, height=h, spp=spp * runs, max_depth=depth)
+        print(name, "render", mean.mean((0, 1)), flush=True)
+
+
+if __name__ == "__main__":
+    main()
