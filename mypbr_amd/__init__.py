@@ -20,7 +20,8 @@ from . import abi
 from .abi import Camera, RenderOpts, RenderStats, SceneDesc
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmpt.so")
+# MPT_LIBRARY selects another build of the same library (tuning experiments)
+LIB_PATH = os.environ.get("MPT_LIBRARY", os.path.join(HERE, "libmpt.so"))
 
 __all__ = ["Scene", "Frame", "MptError", "lib", "abi", "Camera", "RenderOpts", "device_info",
            "ggx_albedo_table", "describe"]

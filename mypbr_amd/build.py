@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
-LIB = os.path.join(HERE, "libmpt.so")
+LIB = os.environ.get("MPT_LIB_OUT", os.path.join(HERE, "libmpt.so"))
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MPT_OFFLOAD_ARCH", "gfx950")
@@ -23,7 +23,7 @@ ARCH = os.environ.get("MPT_OFFLOAD_ARCH", "gfx950")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
           "-Wno-unused-parameter", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-munsafe-fp-atomics"]
+          "-munsafe-fp-atomics"] + os.environ.get("MPT_DEVICE_FLAGS", "").split()
 
 HOST_SOURCES = ["scene_host.cpp", "scene_api.cpp"]
 DEVICE_SOURCES = ["kernels.hip"]
@@ -35,14 +35,15 @@ def run(cmd):
 
 
 def build(verbose=True):
-    os.makedirs(BUILD, exist_ok=True)
+    build_dir = os.environ.get("MPT_BUILD_DIR", BUILD)
+    os.makedirs(build_dir, exist_ok=True)
     objs = []
     for src in HOST_SOURCES:
-        obj = os.path.join(BUILD, src + ".o")
+        obj = os.path.join(build_dir, src + ".o")
         run([HIPCC, "-x", "c++", *COMMON, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     for src in DEVICE_SOURCES:
-        obj = os.path.join(BUILD, src + ".o")
+        obj = os.path.join(build_dir, src + ".o")
         run([HIPCC, "-x", "hip", *COMMON, *DEVICE, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"

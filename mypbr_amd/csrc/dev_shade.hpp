@@ -53,9 +53,10 @@ __device__ __forceinline__ v3 cosine_generate(const v3& n, rstream& rs) {  // pd
     onb f = make_onb(n);
     double r1 = rs.next();
     double r2 = rs.next();
-    double phi = 2 * kPi * r1;
-    double x = std::cos(phi) * std::sqrt(r2);
-    double y = std::sin(phi) * std::sqrt(r2);
+    double sp, cp;
+    sin_cos_2pi(r1, sp, cp);  // phi = 2 pi r1
+    double x = cp * std::sqrt(r2);
+    double y = sp * std::sqrt(r2);
     double z = std::sqrt(1 - r2);
     return onb_transform(f, v3{x, y, z});
 }
@@ -152,45 +153,81 @@ __device__ __forceinline__ double hg_phase(double c, double g) {  // pdf.h:85-88
     return (1 - g * g) / (4 * kPi * denom * std::sqrt(denom));
 }
 
-__device__ __forceinline__ double density_value(const sdensity& s, const v3& dir) {
-    switch (s.kind) {
-        case LOBE_COSINE: return cosine_value(s.n, dir);
-        case LOBE_GGX: return ggx_value(s.n, s.wo_world, s.alpha, dir);
-        case LOBE_MIX:
-            return s.w0 * ggx_value(s.n, s.wo_world, s.alpha, dir) + (1 - s.w0) * cosine_value(s.n, dir);
-        case LOBE_ROUGH_GLASS: return rglass_pdf(make_rglass(s.n, s.wo_world, s.eta, s.alpha), dir);
-        default: {
-            onb f = make_onb(s.n);
-            return hg_phase(dot(unit_vector(dir), f.w), s.w0);
-        }
-    }
+__device__ __forceinline__ double hg_value(const sdensity& s, const v3& dir) {
+    onb f = make_onb(s.n);
+    return hg_phase(dot(unit_vector(dir), f.w), s.w0);
 }
 
-__device__ __forceinline__ v3 density_generate(const sdensity& s, rstream& rs) {
-    switch (s.kind) {
-        case LOBE_COSINE: return cosine_generate(s.n, rs);
-        case LOBE_GGX: return ggx_generate(s.n, s.wo_world, s.alpha, rs);
-        case LOBE_MIX:
-            if (rs.next() < s.w0) return ggx_generate(s.n, s.wo_world, s.alpha, rs);
-            return cosine_generate(s.n, rs);
-        case LOBE_ROUGH_GLASS: return rglass_sample(make_rglass(s.n, s.wo_world, s.eta, s.alpha), rs);
-        default: {  // henyey_greenstein_pdf::generate (pdf.h:67-80)
-            onb f = make_onb(s.n);
-            double g = s.w0;
-            double xi = rs.next();
-            double c;
-            if (std::fabs(g) < 1e-3) {
-                c = 1 - 2 * xi;
-            } else {
-                double sq = (1 - g * g) / (1 - g + 2 * g * xi);
-                c = (1 + g * g - sq * sq) / (2 * g);
-            }
-            c = clampd(c, -1.0, 1.0);
-            double st = std::sqrt(1 - c * c);
-            double phi = 2 * kPi * rs.next();
-            return onb_transform(f, v3{std::cos(phi) * st, std::sin(phi) * st, c});
+// The scatter density's value; MT (a material type known at compile time,
+// or -1) limits the lobes the code handles to those of that material.
+template <int MT = -1>
+__device__ __forceinline__ double density_value(const sdensity& s, const v3& dir) {
+    if constexpr (MT == MPT_MAT_LAMBERTIAN) return cosine_value(s.n, dir);
+    if constexpr (MT == MPT_MAT_METAL) {
+        if (s.kind == LOBE_GGX) return ggx_value(s.n, s.wo_world, s.alpha, dir);
+        return s.w0 * ggx_value(s.n, s.wo_world, s.alpha, dir) + (1 - s.w0) * cosine_value(s.n, dir);
+    }
+    if constexpr (MT == MPT_MAT_PRINCIPLED)
+        return s.w0 * ggx_value(s.n, s.wo_world, s.alpha, dir) + (1 - s.w0) * cosine_value(s.n, dir);
+    if constexpr (MT == MPT_MAT_DIELECTRIC) return rglass_pdf(make_rglass(s.n, s.wo_world, s.eta, s.alpha), dir);
+    if constexpr (MT == MPT_MAT_PHASE) return hg_value(s, dir);
+    if constexpr (MT < 0) {
+        switch (s.kind) {
+            case LOBE_COSINE: return cosine_value(s.n, dir);
+            case LOBE_GGX: return ggx_value(s.n, s.wo_world, s.alpha, dir);
+            case LOBE_MIX:
+                return s.w0 * ggx_value(s.n, s.wo_world, s.alpha, dir) + (1 - s.w0) * cosine_value(s.n, dir);
+            case LOBE_ROUGH_GLASS: return rglass_pdf(make_rglass(s.n, s.wo_world, s.eta, s.alpha), dir);
+            default: return hg_value(s, dir);
         }
     }
+    return 0;
+}
+
+__device__ __forceinline__ v3 hg_generate(const sdensity& s, rstream& rs) {  // pdf.h:67-80
+    onb f = make_onb(s.n);
+    double g = s.w0;
+    double xi = rs.next();
+    double c;
+    if (std::fabs(g) < 1e-3) {
+        c = 1 - 2 * xi;
+    } else {
+        double sq = (1 - g * g) / (1 - g + 2 * g * xi);
+        c = (1 + g * g - sq * sq) / (2 * g);
+    }
+    c = clampd(c, -1.0, 1.0);
+    double st = std::sqrt(1 - c * c);
+    double sp, cp;
+    sin_cos_2pi(rs.next(), sp, cp);  // phi = 2 pi xi
+    return onb_transform(f, v3{cp * st, sp * st, c});
+}
+
+template <int MT = -1>
+__device__ __forceinline__ v3 density_generate(const sdensity& s, rstream& rs) {
+    if constexpr (MT == MPT_MAT_LAMBERTIAN) return cosine_generate(s.n, rs);
+    if constexpr (MT == MPT_MAT_METAL) {
+        if (s.kind == LOBE_GGX) return ggx_generate(s.n, s.wo_world, s.alpha, rs);
+        if (rs.next() < s.w0) return ggx_generate(s.n, s.wo_world, s.alpha, rs);
+        return cosine_generate(s.n, rs);
+    }
+    if constexpr (MT == MPT_MAT_PRINCIPLED) {
+        if (rs.next() < s.w0) return ggx_generate(s.n, s.wo_world, s.alpha, rs);
+        return cosine_generate(s.n, rs);
+    }
+    if constexpr (MT == MPT_MAT_DIELECTRIC) return rglass_sample(make_rglass(s.n, s.wo_world, s.eta, s.alpha), rs);
+    if constexpr (MT == MPT_MAT_PHASE) return hg_generate(s, rs);
+    if constexpr (MT < 0) {
+        switch (s.kind) {
+            case LOBE_COSINE: return cosine_generate(s.n, rs);
+            case LOBE_GGX: return ggx_generate(s.n, s.wo_world, s.alpha, rs);
+            case LOBE_MIX:
+                if (rs.next() < s.w0) return ggx_generate(s.n, s.wo_world, s.alpha, rs);
+                return cosine_generate(s.n, rs);
+            case LOBE_ROUGH_GLASS: return rglass_sample(make_rglass(s.n, s.wo_world, s.eta, s.alpha), rs);
+            default: return hg_generate(s, rs);
+        }
+    }
+    return v3{0, 0, 0};
 }
 
 // ---------------------------------------------------------------- materials
@@ -206,9 +243,11 @@ __device__ __forceinline__ v3 principled_f0(const mpt_material& m, const v3& bas
     return (1 - m.metallic) * v3{m.f0, m.f0, m.f0} + m.metallic * base;
 }
 
-// material::scatter (material.h:79-85, 123-150, 210-242, 387-421; medium.h:21-27)
-__device__ scatter_out mat_scatter(const dscene& S, const mpt_material& m, const v3& rd, const hitrec& rec,
-                                   rstream& rs) {
+// material::scatter (material.h:79-85, 123-150, 210-242, 387-421; medium.h:21-27).
+// MT: the material type when known at compile time (-1: dispatch on m.type).
+template <int MT = -1>
+__device__ __forceinline__ scatter_out mat_scatter(const dscene& S, const mpt_material& m, const v3& rd,
+                                                   const hitrec& rec, rstream& rs) {
     scatter_out o;
     o.scattered = true;
     o.skip = false;
@@ -220,7 +259,7 @@ __device__ scatter_out mat_scatter(const dscene& S, const mpt_material& m, const
     o.dens.alpha = 0;
     o.dens.eta = 1;
     o.dens.w0 = 1;
-    switch (m.type) {
+    switch (MT >= 0 ? MT : m.type) {
         case MPT_MAT_LAMBERTIAN:
             o.attenuation = tex_value(S, m.texture, rec.p);
             break;
@@ -297,8 +336,10 @@ __device__ scatter_out mat_scatter(const dscene& S, const mpt_material& m, const
 }
 
 // material::eval: f * |cos| for a non-specular direction.
-__device__ v3 mat_eval(const dscene& S, const mpt_material& m, const v3& rd, const hitrec& rec, const v3& dir) {
-    switch (m.type) {
+template <int MT = -1>
+__device__ __forceinline__ v3 mat_eval(const dscene& S, const mpt_material& m, const v3& rd, const hitrec& rec,
+                                       const v3& dir) {
+    switch (MT >= 0 ? MT : m.type) {
         case MPT_MAT_LAMBERTIAN: {  // material.h:88-97
             double c = dot(rec.normal, unit_vector(dir));
             double pdf = c < 0 ? 0 : c / kPi;
@@ -348,8 +389,9 @@ __device__ v3 mat_eval(const dscene& S, const mpt_material& m, const v3& rd, con
     }
 }
 
+template <int MT = -1>
 __device__ __forceinline__ v3 mat_emitted(const mpt_material& m, const v3& rd, const hitrec& rec) {
-    if (m.type != MPT_MAT_DIFFUSE_LIGHT || !rec.front) return v3{0, 0, 0};  // material.h:578-584
+    if ((MT >= 0 ? MT : m.type) != MPT_MAT_DIFFUSE_LIGHT || !rec.front) return v3{0, 0, 0};  // material.h:578-584
     v3 e = m3(m.emit);
     if (m.spread >= 180) return e;
     double c = -dot(unit_vector(rd), rec.normal);
@@ -362,8 +404,9 @@ __device__ __forceinline__ v3 mat_emitted(const mpt_material& m, const v3& rd, c
     return e * gf;
 }
 
+template <int MT = -1>
 __device__ __forceinline__ v3 mat_interior(const mpt_material& m, const v3& rd, const hitrec& rec) {
-    if (m.type != MPT_MAT_DIELECTRIC) return v3{1, 1, 1};  // material.h:201-206
+    if ((MT >= 0 ? MT : m.type) != MPT_MAT_DIELECTRIC) return v3{1, 1, 1};  // material.h:201-206
     if (rec.front || (m.sigma_a[0] == 0 && m.sigma_a[1] == 0 && m.sigma_a[2] == 0)) return v3{1, 1, 1};
     double dd = rec.t * length(rd);
     return v3{std::exp(-m.sigma_a[0] * dd), std::exp(-m.sigma_a[1] * dd), std::exp(-m.sigma_a[2] * dd)};
@@ -436,16 +479,32 @@ __device__ v3 env_random(const dscene& S, rstream& rs) {  // environment.h:94-10
     double x = sample_cdf(&S.env_cond[size_t(j) * (w + 1)], w, rs.next());
     double theta = kPi * y / h;
     double phi = 2 * kPi * (x / w - 0.5 - S.env_rotation);
-    double st = std::sin(theta);
-    return v3{st * std::sin(phi), std::cos(theta), -st * std::cos(phi)};
+    double st, ct, sp, cp;
+    sin_cos(theta, st, ct);
+    sin_cos(phi, sp, cp);
+    return v3{st * sp, ct, -st * cp};
 }
 
 // ---------------------------------------------------------------- lights
+// Light kinds a scene's light list holds; the NEE code is compiled for the
+// kinds present (LK bit set), like the traversal for geometry.
+enum : int {
+    L_ENV = 1,
+    L_SPHERE = 2,
+    L_QUAD = 4,
+    L_TRI = 8,
+    L_LIST = 16,
+    L_XFORM = 32,
+    L_ALL = 63,
+};
+
+template <int LK>
 __device__ __forceinline__ double prim_light_pdf(const dscene& S, int64_t p, const v3& o, const v3& d) {
     const double* q = prim_ptr(S, p);
     int k = S.pkind[p];
     double t;
-    if (k == MPT_PRIM_QUAD) {  // quad.h:65-74
+    constexpr bool only_quads = (LK & (L_SPHERE | L_QUAD | L_TRI)) == L_QUAD;
+    if ((LK & L_QUAD) && (only_quads || k == MPT_PRIM_QUAD)) {  // quad.h:65-74
         if (!quad_t(q, o, d, 0.001, kInf, t)) return 0;
         double dist2 = t * t * length_squared(d);
         v3 nrm{q[9], q[10], q[11]};
@@ -454,7 +513,7 @@ __device__ __forceinline__ double prim_light_pdf(const dscene& S, int64_t p, con
         double area = length(cross(v3{q[3], q[4], q[5]}, v3{q[6], q[7], q[8]}));
         return dist2 / (cosine * area);
     }
-    if (k == MPT_PRIM_SPHERE) {  // sphere.h:56-73
+    if ((LK & L_SPHERE) && k == MPT_PRIM_SPHERE) {  // sphere.h:56-73
         if (!sphere_t(q, o, d, 0.001, kInf, t)) return 0;
         v3 c{q[0], q[1], q[2]};
         double r = std::fabs(q[3]);
@@ -469,6 +528,7 @@ __device__ __forceinline__ double prim_light_pdf(const dscene& S, int64_t p, con
         double solid_angle = 2 * kPi * (1 - cos_theta_max);
         return 1 / solid_angle;
     }
+    if constexpr (!(LK & L_TRI)) return 0;
     // triangle (mesh.h:115-124)
     if (!tri_t(q, o, d, 0.001, kInf, t)) return 0;
     double dist2 = t * t * length_squared(d);
@@ -488,16 +548,18 @@ __device__ __forceinline__ v3 random_unit_vector(rstream& rs) {  // vec3.h:116-1
     }
 }
 
+template <int LK>
 __device__ __forceinline__ v3 prim_light_random(const dscene& S, int64_t p, const v3& o, rstream& rs) {
     const double* q = prim_ptr(S, p);
     int k = S.pkind[p];
-    if (k == MPT_PRIM_QUAD) {  // quad.h:76-79
+    constexpr bool only_quads = (LK & (L_SPHERE | L_QUAD | L_TRI)) == L_QUAD;
+    if ((LK & L_QUAD) && (only_quads || k == MPT_PRIM_QUAD)) {  // quad.h:76-79
         double a = rs.next();
         double b = rs.next();
         v3 pt = v3{q[0], q[1], q[2]} + (a * v3{q[3], q[4], q[5]}) + (b * v3{q[6], q[7], q[8]});
         return pt - o;
     }
-    if (k == MPT_PRIM_SPHERE) {  // sphere.h:75-83, 122-132
+    if ((LK & L_SPHERE) && k == MPT_PRIM_SPHERE) {  // sphere.h:75-83, 122-132
         v3 c{q[0], q[1], q[2]};
         double r = std::fabs(q[3]);
         v3 dir = c - o;
@@ -507,11 +569,13 @@ __device__ __forceinline__ v3 prim_light_random(const dscene& S, int64_t p, cons
         double r1 = rs.next();
         double r2 = rs.next();
         double z = 1 + r2 * (std::sqrt(1 - r * r / dist2) - 1);
-        double phi = 2 * kPi * r1;
-        double x = std::cos(phi) * std::sqrt(1 - z * z);
-        double y = std::sin(phi) * std::sqrt(1 - z * z);
+        double sp, cp;
+        sin_cos_2pi(r1, sp, cp);  // phi = 2 pi r1
+        double x = cp * std::sqrt(1 - z * z);
+        double y = sp * std::sqrt(1 - z * z);
         return onb_transform(f, v3{x, y, z});
     }
+    if constexpr (!(LK & L_TRI)) return v3{0, 0, 0};
     // triangle (mesh.h:127-139)
     double r1 = rs.next(), r2 = rs.next();
     if (r1 + r2 > 1) {
@@ -524,47 +588,59 @@ __device__ __forceinline__ v3 prim_light_random(const dscene& S, int64_t p, cons
 }
 
 // hittable::pdf_value of light object ob (instance.h:47-50, hittable_list.h:46-56).
-__device__ double object_light_pdf(const dscene& S, int ob, v3 o, v3 d) {
+template <int LK>
+__device__ __forceinline__ double object_light_pdf(const dscene& S, int ob, v3 o, v3 d) {
     const obj_rec r = S.obj[ob];
-    if (r.xform >= 0) {
-        const xform& x = S.xf[r.xform];
-        v3 lo = xf_inv_point(x, o);
-        v3 ld = xf_inv_vector(x, d);
-        o = lo;
-        d = ld;
+    if constexpr (LK & L_XFORM) {
+        if (r.xform >= 0) {
+            const xform& x = S.xf[r.xform];
+            v3 lo = xf_inv_point(x, o);
+            v3 ld = xf_inv_vector(x, d);
+            o = lo;
+            d = ld;
+        }
     }
-    if (r.kind == MPT_OBJ_PRIM) return prim_light_pdf(S, r.first, o, d);
+    if (!(LK & L_LIST) || r.kind == MPT_OBJ_PRIM) return prim_light_pdf<LK>(S, r.first, o, d);
     double weight = 1.0 / double(r.count);
     double sum = 0.0;
-    for (int64_t k = 0; k < r.count; k++) sum += weight * prim_light_pdf(S, r.first + k, o, d);
+    for (int64_t k = 0; k < r.count; k++) sum += weight * prim_light_pdf<LK>(S, r.first + k, o, d);
     return sum;
 }
 
-__device__ v3 object_light_random(const dscene& S, int ob, const v3& o, rstream& rs) {
+template <int LK>
+__device__ __forceinline__ v3 object_light_random(const dscene& S, int ob, const v3& o, rstream& rs) {
     const obj_rec r = S.obj[ob];
     v3 lo = o;
-    const xform* x = r.xform >= 0 ? &S.xf[r.xform] : nullptr;
-    if (x) lo = xf_inv_point(*x, o);
+    const xform* x = nullptr;
+    if constexpr (LK & L_XFORM) {
+        x = r.xform >= 0 ? &S.xf[r.xform] : nullptr;
+        if (x) lo = xf_inv_point(*x, o);
+    }
     v3 res;
-    if (r.kind == MPT_OBJ_PRIM) {
-        res = prim_light_random(S, r.first, lo, rs);
+    if (!(LK & L_LIST) || r.kind == MPT_OBJ_PRIM) {
+        res = prim_light_random<LK>(S, r.first, lo, rs);
     } else {
         int n = int(r.count);
         int idx = int(rs.next() * n);
         idx = idx < n - 1 ? idx : n - 1;
-        res = prim_light_random(S, r.first + idx, lo, rs);
+        res = prim_light_random<LK>(S, r.first + idx, lo, rs);
     }
-    if (x) res = xf_vector(*x, res);
+    if constexpr (LK & L_XFORM)
+        if (x) res = xf_vector(*x, res);
     return res;
 }
 
 // light_list::pdf_value (light_list.h:106-113): sum of p_k pdf_k.
-__device__ double lights_pdf(const dscene& S, const v3& o, const v3& d) {
+template <int LK = L_ALL>
+__device__ __forceinline__ double lights_pdf(const dscene& S, const v3& o, const v3& d) {
     if (S.nlights == 0) return 0.0;
     double sum = 0;
     for (int k = 0; k < S.nlights; k++) {
         const light_rec L = S.lights[k];
-        double pk = L.object < 0 ? env_pdf(S, d) : object_light_pdf(S, L.object, o, d);
+        double pk;
+        if constexpr (LK == L_ENV) pk = env_pdf(S, d);
+        else if constexpr (!(LK & L_ENV)) pk = object_light_pdf<LK>(S, L.object, o, d);
+        else pk = L.object < 0 ? env_pdf(S, d) : object_light_pdf<LK>(S, L.object, o, d);
         sum += L.probability * pk;
     }
     return sum;
@@ -572,7 +648,8 @@ __device__ double lights_pdf(const dscene& S, const v3& o, const v3& d) {
 
 // light_list::random (light_list.h:115-127), or hittable_list::random for an
 // unbuilt list (strategy 2).
-__device__ v3 lights_random(const dscene& S, const v3& o, rstream& rs) {
+template <int LK = L_ALL>
+__device__ __forceinline__ v3 lights_random(const dscene& S, const v3& o, rstream& rs) {
     double xi = rs.next();
     int n = S.nlights;
     int k;
@@ -591,8 +668,12 @@ __device__ v3 lights_random(const dscene& S, const v3& o, rstream& rs) {
         k -= 1;
     }
     const light_rec L = S.lights[k];
-    if (L.object < 0) return env_random(S, rs);
-    return object_light_random(S, L.object, o, rs);
+    if constexpr (LK == L_ENV) return env_random(S, rs);
+    else if constexpr (!(LK & L_ENV)) return object_light_random<LK>(S, L.object, o, rs);
+    else {
+        if (L.object < 0) return env_random(S, rs);
+        return object_light_random<LK>(S, L.object, o, rs);
+    }
 }
 
 }  // namespace mpt

@@ -28,9 +28,11 @@ struct dscene {
     const int32_t* pmat;
     const int64_t* pcanon;
     const obj_rec* obj;
-    const bvh_node* tlas;
+    const cnode* tlas;        // child-pair records of the top-level BVH
     const int32_t* titems;
-    const bvh_node* blas;
+    const cnode* blas;        // child-pair records of all mesh BVHs
+    const croot* blas_roots;  // per mesh, indexed by obj_rec.blas_root
+    const double* sph;        // center xyz, radius per primitive (spheres)
     const xform* xf;
     const mpt_material* mat;
     const mpt_texture* tex;
@@ -50,6 +52,7 @@ struct dscene {
     int32_t lstrategy;
     int32_t env_w, env_h;
     int32_t pad;
+    croot tlas_root;
 };
 
 // Per-lane traversal stack: LDS for the first kLdsDepth entries, then HBM.
@@ -78,20 +81,34 @@ struct wcount {
     uint32_t pbytes;
 };
 
+// Geometry features a scene uses; traversal kernels are compiled for the
+// subset a scene needs so that unused primitive kinds and object kinds cost
+// neither instructions nor registers.
+enum : int {
+    F_SPHERE = 1,
+    F_QUAD = 2,
+    F_TRI = 4,
+    F_XFORM = 8,
+    F_LIST = 16,
+    F_MESH = 32,
+    F_MEDIUM = 64,
+    F_ALL = 127,
+};
+
 struct hitinfo {
     double t;
     int32_t obj;    // object index, -1 = miss
     int32_t prim;   // device primitive index, -1 for a medium
 };
 
-// Slab test of aabb::hit (aabb.h:40-54) on node n.
-__device__ __forceinline__ bool node_hit(const bvh_node& n, const v3& o, const v3& inv, double tmin, double tmax) {
+// Slab test of aabb::hit (aabb.h:40-54) on a box given as (xmin, xmax, ymin, ymax, zmin, zmax).
+__device__ __forceinline__ bool box_hit(const double* b, const v3& o, const v3& inv, double tmin, double tmax) {
     const double oo[3] = {o.x, o.y, o.z};
     const double ii[3] = {inv.x, inv.y, inv.z};
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        double t0 = (n.b[2 * a] - oo[a]) * ii[a];
-        double t1 = (n.b[2 * a + 1] - oo[a]) * ii[a];
+        double t0 = (b[2 * a] - oo[a]) * ii[a];
+        double t1 = (b[2 * a + 1] - oo[a]) * ii[a];
         if (t0 > t1) {
             double s = t0;
             t0 = t1;
@@ -104,14 +121,83 @@ __device__ __forceinline__ bool node_hit(const bvh_node& n, const v3& o, const v
     return true;
 }
 
-__device__ __forceinline__ bvh_node load_node(const bvh_node* nodes, int i) {
+__device__ __forceinline__ cnode load_cnode(const cnode* nodes, int i) {
     const double2* p = reinterpret_cast<const double2*>(nodes + i);
-    double2 a = p[0], b = p[1], c = p[2];
-    const int4 q = reinterpret_cast<const int4*>(nodes + i)[3];
-    bvh_node n;
-    n.b[0] = a.x, n.b[1] = a.y, n.b[2] = b.x, n.b[3] = b.y, n.b[4] = c.x, n.b[5] = c.y;
-    n.offset = q.x, n.count = q.y, n.axis = q.z, n.pad = 0;
+    cnode n;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        double2 v = p[k];
+        n.b[2 * k] = v.x;
+        n.b[2 * k + 1] = v.y;
+    }
+    const int4 a = reinterpret_cast<const int4*>(nodes + i)[6];
+    const int4 c = reinterpret_cast<const int4*>(nodes + i)[7];
+    n.off[0] = a.x, n.off[1] = a.y, n.cnt[0] = a.z, n.cnt[1] = a.w;
+    n.axis = c.x;
     return n;
+}
+
+// One child (0 = left, 1 = right) of record i: its box, item range or record.
+__device__ __forceinline__ void load_child(const cnode* nodes, int i, int w, double* b, int32_t& off, int32_t& cnt) {
+    const double2* p = reinterpret_cast<const double2*>(nodes + i) + 3 * w;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double2 v = p[k];
+        b[2 * k] = v.x;
+        b[2 * k + 1] = v.y;
+    }
+    const int4 a = reinterpret_cast<const int4*>(nodes + i)[6];
+    off = w ? a.y : a.x;
+    cnt = w ? a.w : a.z;
+}
+
+// Walk of bvh::hit (bvh.h:45-86) over child-pair records. At an interior
+// record both child boxes are tested with the current interval; the near
+// child (by the split axis sign) is entered, the far child pushed only if
+// it is hit. A popped child is tested again with the interval as it is then,
+// exactly as the reference tests a node when it pops it: a box missed with
+// a larger tmax is missed with a smaller one, so skipping its push changes
+// nothing, and the order of leaf visits is the reference's.
+// leaf(off, cnt, sp) tests a leaf's items and shrinks tmax on hits.
+// negmask: bit a set when the ray goes towards -a (inv_dir < 0).
+template <bool CNT, typename Leaf>
+__device__ __forceinline__ void walk(const cnode* nodes, const croot& root, const v3& o, const v3& inv, int negmask,
+                                     double tmin, double& tmax, tstack& st, int sp_base, wcount& wc, Leaf&& leaf) {
+    if constexpr (CNT) wc.nodes += 64u;
+    if (!box_hit(root.b, o, inv, tmin, tmax)) return;
+    if (root.cnt > 0) {
+        leaf(root.off, root.cnt, sp_base);
+        return;
+    }
+    int sp = sp_base;
+    int cur = root.off;
+    while (true) {
+        const cnode n = load_cnode(nodes, cur);
+        if constexpr (CNT) wc.nodes += 128u;
+        const bool h0 = box_hit(n.b, o, inv, tmin, tmax);
+        const bool h1 = box_hit(n.b + 6, o, inv, tmin, tmax);
+        const int nr = (negmask >> n.axis) & 1;  // the right child is nearer along -axis
+        if (nr ? h0 : h1) st.put(sp++, (cur << 1) | (1 - nr));
+        int next = -1;
+        if (nr ? h1 : h0) {
+            const int32_t noff = nr ? n.off[1] : n.off[0];
+            const int32_t ncnt = nr ? n.cnt[1] : n.cnt[0];
+            if (ncnt > 0) leaf(noff, ncnt, sp);
+            else next = noff;
+        }
+        while (next < 0) {
+            if (sp == sp_base) return;
+            const int e = st.get(--sp);
+            double b[6];
+            int32_t off, cnt;
+            load_child(nodes, e >> 1, e & 1, b, off, cnt);
+            if constexpr (CNT) wc.nodes += 64u;
+            if (!box_hit(b, o, inv, tmin, tmax)) continue;
+            if (cnt > 0) leaf(off, cnt, sp);
+            else next = off;
+        }
+        cur = next;
+    }
 }
 
 __device__ __forceinline__ const double* prim_ptr(const dscene& S, int64_t p) { return S.prim + p * kPrimStride; }
@@ -175,26 +261,37 @@ __device__ __forceinline__ bool tri_t(const double* q, const v3& o, const v3& d,
     return true;
 }
 
-template <bool CNT>
+template <bool CNT, int FEAT>
 __device__ __forceinline__ bool prim_t(const dscene& S, int64_t p, const v3& o, const v3& d, double tmin,
                                        double tmax, double& t, wcount& wc) {
+    constexpr int kinds = (FEAT & F_SPHERE ? 1 : 0) + (FEAT & F_QUAD ? 1 : 0) + (FEAT & F_TRI ? 1 : 0);
+    if constexpr (kinds == 1 && (FEAT & F_SPHERE)) {
+        if constexpr (CNT) wc.pbytes += 32u;
+        return sphere_t(S.sph + 4 * p, o, d, tmin, tmax, t);
+    }
     const double* q = prim_ptr(S, p);
     int k = S.pkind[p];
     if constexpr (CNT) wc.pbytes += k == MPT_PRIM_SPHERE ? 32u : (k == MPT_PRIM_QUAD ? 128u : 72u);
-    if (k == MPT_PRIM_SPHERE) return sphere_t(q, o, d, tmin, tmax, t);
-    if (k == MPT_PRIM_QUAD) return quad_t(q, o, d, tmin, tmax, t);
-    return tri_t(q, o, d, tmin, tmax, t);
+    if constexpr (kinds == 1) {
+        if constexpr (FEAT & F_SPHERE) return sphere_t(q, o, d, tmin, tmax, t);
+        if constexpr (FEAT & F_QUAD) return quad_t(q, o, d, tmin, tmax, t);
+        return tri_t(q, o, d, tmin, tmax, t);
+    }
+    if ((FEAT & F_SPHERE) && k == MPT_PRIM_SPHERE) return sphere_t(q, o, d, tmin, tmax, t);
+    if ((FEAT & F_QUAD) && k == MPT_PRIM_QUAD) return quad_t(q, o, d, tmin, tmax, t);
+    if constexpr (FEAT & F_TRI) return tri_t(q, o, d, tmin, tmax, t);
+    return false;
 }
 
 // hittable_list::hit (hittable_list.h:25-39) over [first, first + count).
-template <bool CNT>
+template <bool CNT, int FEAT>
 __device__ __forceinline__ bool list_t(const dscene& S, int64_t first, int64_t count, const v3& o, const v3& d,
                                        double tmin, double tmax, double& t, int32_t& which, wcount& wc) {
     bool any = false;
     double closest = tmax;
     for (int64_t k = 0; k < count; k++) {
         double tt;
-        if (prim_t<CNT>(S, first + k, o, d, tmin, closest, tt, wc)) {
+        if (prim_t<CNT, FEAT>(S, first + k, o, d, tmin, closest, tt, wc)) {
             any = true;
             closest = tt;
             which = int32_t(first + k);
@@ -205,64 +302,45 @@ __device__ __forceinline__ bool list_t(const dscene& S, int64_t first, int64_t c
 }
 
 // Inner bvh::hit of a mesh; the stack continues above sp_base.
-template <bool CNT>
+template <bool CNT, int FEAT>
 __device__ bool mesh_t(const dscene& S, int root, const v3& o, const v3& d, double tmin, double tmax, double& t,
                        int32_t& which, tstack& st, int sp_base, wcount& wc) {
     v3 inv{1 / d.x, 1 / d.y, 1 / d.z};
-    bool neg[3] = {inv.x < 0, inv.y < 0, inv.z < 0};
+    const int negmask = (inv.x < 0 ? 1 : 0) | (inv.y < 0 ? 2 : 0) | (inv.z < 0 ? 4 : 0);
     bool any = false;
-    int sp = sp_base;
-    int cur = root;
-    while (true) {
-        bvh_node n = load_node(S.blas, cur);
-        if constexpr (CNT) wc.nodes++;
-        if (node_hit(n, o, inv, tmin, tmax)) {
-            if (n.count > 0) {
-                for (int k = 0; k < n.count; k++) {
-                    double tt;
-                    if constexpr (CNT) wc.pbytes += 72u;
-                    if (tri_t(prim_ptr(S, n.offset + k), o, d, tmin, tmax, tt)) {
-                        any = true;
-                        tmax = tt;
-                        which = n.offset + k;
-                    }
-                }
-                if (sp == sp_base) break;
-                cur = st.get(--sp);
-            } else if (neg[n.axis]) {
-                st.put(sp++, cur + 1);
-                cur = n.offset;
-            } else {
-                st.put(sp++, n.offset);
-                cur = cur + 1;
+    walk<CNT>(S.blas, S.blas_roots[root], o, inv, negmask, tmin, tmax, st, sp_base, wc, [&](int off, int cnt, int) {
+        for (int k = 0; k < cnt; k++) {
+            double tt;
+            if constexpr (CNT) wc.pbytes += 72u;
+            if (tri_t(prim_ptr(S, off + k), o, d, tmin, tmax, tt)) {
+                any = true;
+                tmax = tt;
+                which = off + k;
             }
-        } else {
-            if (sp == sp_base) break;
-            cur = st.get(--sp);
         }
-    }
+    });
     t = tmax;
     return any;
 }
 
 // Hit of an object's boundary (a primitive or a list) for media.
-template <bool CNT>
+template <bool CNT, int FEAT>
 __device__ __forceinline__ bool boundary_t(const dscene& S, const obj_rec& r, const v3& o, const v3& d, double tmin,
                                            double tmax, double& t, wcount& wc) {
     int32_t w;
-    if (r.bkind == MPT_OBJ_PRIM) return prim_t<CNT>(S, r.first, o, d, tmin, tmax, t, wc);
-    return list_t<CNT>(S, r.first, r.count, o, d, tmin, tmax, t, w, wc);
+    if (r.bkind == MPT_OBJ_PRIM) return prim_t<CNT, FEAT>(S, r.first, o, d, tmin, tmax, t, wc);
+    return list_t<CNT, FEAT>(S, r.first, r.count, o, d, tmin, tmax, t, w, wc);
 }
 
 // constant_medium::hit (medium.h:60-88), with the free-flight draw u.
 // d is the boundary's (possibly instance-local) direction; world_len is the
 // length of the world-space direction, which sets distances in the medium.
-template <bool CNT>
+template <bool CNT, int FEAT>
 __device__ bool medium_t(const dscene& S, const obj_rec& r, const v3& o, const v3& d, double world_len, double tmin,
                          double tmax, double u, double& t, wcount& wc) {
     double t1a, t2a;
-    if (!boundary_t<CNT>(S, r, o, d, -kInf, kInf, t1a, wc)) return false;
-    if (!boundary_t<CNT>(S, r, o, d, t1a + 0.0001, kInf, t2a, wc)) return false;
+    if (!boundary_t<CNT, FEAT>(S, r, o, d, -kInf, kInf, t1a, wc)) return false;
+    if (!boundary_t<CNT, FEAT>(S, r, o, d, t1a + 0.0001, kInf, t2a, wc)) return false;
     double t0 = std::fmax(t1a, tmin);
     double t1 = std::fmin(t2a, tmax);
     if (t0 >= t1) return false;
@@ -275,37 +353,45 @@ __device__ bool medium_t(const dscene& S, const obj_rec& r, const v3& o, const v
 }
 
 // One world object, the ray in world space. Returns the hit t and primitive.
-template <bool CNT>
+template <bool CNT, int FEAT>
 __device__ __forceinline__ bool object_t(const dscene& S, int oi, v3 o, v3 d, double tmin, double tmax, double& t,
                                          int32_t& prim, tstack& st, int sp, bool media, uint64_t mkey,
                                          uint32_t serial, wcount& wc) {
+    if constexpr (!(FEAT & (F_XFORM | F_LIST | F_MESH | F_MEDIUM))) {
+        // primitives only: the object is its primitive
+        prim = oi;
+        return prim_t<CNT, FEAT>(S, oi, o, d, tmin, tmax, t, wc);
+    }
     const obj_rec r = S.obj[oi];
-    if (r.kind == MPT_OBJ_MEDIUM && !media) return false;
-    const double world_len = r.kind == MPT_OBJ_MEDIUM ? length(d) : 0.0;
-    if (r.xform >= 0) {
-        const xform& x = S.xf[r.xform];
-        v3 lo = xf_inv_point(x, o);
-        v3 ld = xf_inv_vector(x, d);
-        o = lo;
-        d = ld;
+    if ((FEAT & F_MEDIUM) && r.kind == MPT_OBJ_MEDIUM && !media) return false;
+    const double world_len = (FEAT & F_MEDIUM) && r.kind == MPT_OBJ_MEDIUM ? length(d) : 0.0;
+    if constexpr (FEAT & F_XFORM) {
+        if (r.xform >= 0) {
+            const xform& x = S.xf[r.xform];
+            v3 lo = xf_inv_point(x, o);
+            v3 ld = xf_inv_vector(x, d);
+            o = lo;
+            d = ld;
+        }
     }
-    switch (r.kind) {
-        case MPT_OBJ_PRIM:
-            prim = int32_t(r.first);
-            return prim_t<CNT>(S, r.first, o, d, tmin, tmax, t, wc);
-        case MPT_OBJ_LIST:
-            return list_t<CNT>(S, r.first, r.count, o, d, tmin, tmax, t, prim, wc);
-        case MPT_OBJ_MESH:
-            return mesh_t<CNT>(S, r.blas_root, o, d, tmin, tmax, t, prim, st, sp, wc);
-        default:
-            prim = -1;
-            return medium_t<CNT>(S, r, o, d, world_len, tmin, tmax, medium_draw(mkey, serial, uint32_t(oi)), t, wc);
+    if (r.kind == MPT_OBJ_PRIM) {
+        prim = int32_t(r.first);
+        return prim_t<CNT, FEAT>(S, r.first, o, d, tmin, tmax, t, wc);
     }
+    if constexpr (FEAT & F_LIST)
+        if (r.kind == MPT_OBJ_LIST) return list_t<CNT, FEAT>(S, r.first, r.count, o, d, tmin, tmax, t, prim, wc);
+    if constexpr (FEAT & F_MESH)
+        if (r.kind == MPT_OBJ_MESH) return mesh_t<CNT, FEAT>(S, r.blas_root, o, d, tmin, tmax, t, prim, st, sp, wc);
+    if constexpr (FEAT & F_MEDIUM) {
+        prim = -1;
+        return medium_t<CNT, FEAT>(S, r, o, d, world_len, tmin, tmax, medium_draw(mkey, serial, uint32_t(oi)), t, wc);
+    }
+    return false;
 }
 
 // bvh::hit over the world (bvh.h:45-86). media: test constant_medium objects
 // with draws from (mkey, serial); otherwise they are skipped.
-template <bool CNT = false>
+template <bool CNT = false, int FEAT = F_ALL>
 __device__ bool trace_closest(const dscene& S, const v3& o, const v3& d, double tmin, double tmax, hitinfo& h,
                               tstack& st, bool media, uint64_t mkey, uint32_t serial, wcount& wc) {
     h.obj = -1;
@@ -313,41 +399,22 @@ __device__ bool trace_closest(const dscene& S, const v3& o, const v3& d, double 
     h.t = tmax;
     if (S.nobj == 0) return false;
     v3 inv{1 / d.x, 1 / d.y, 1 / d.z};
-    bool neg[3] = {inv.x < 0, inv.y < 0, inv.z < 0};
+    const int negmask = (inv.x < 0 ? 1 : 0) | (inv.y < 0 ? 2 : 0) | (inv.z < 0 ? 4 : 0);
     bool any = false;
-    int sp = 0;
-    int cur = 0;
-    while (true) {
-        bvh_node n = load_node(S.tlas, cur);
-        if constexpr (CNT) wc.nodes++;
-        if (node_hit(n, o, inv, tmin, tmax)) {
-            if (n.count > 0) {
-                for (int k = 0; k < n.count; k++) {
-                    int oi = S.titems[n.offset + k];
-                    double tt;
-                    int32_t pr = -1;
-                    if (object_t<CNT>(S, oi, o, d, tmin, tmax, tt, pr, st, sp, media, mkey, serial, wc)) {
-                        any = true;
-                        tmax = tt;
-                        h.t = tt;
-                        h.obj = oi;
-                        h.prim = pr;
-                    }
-                }
-                if (sp == 0) break;
-                cur = st.get(--sp);
-            } else if (neg[n.axis]) {
-                st.put(sp++, cur + 1);
-                cur = n.offset;
-            } else {
-                st.put(sp++, n.offset);
-                cur = cur + 1;
+    walk<CNT>(S.tlas, S.tlas_root, o, inv, negmask, tmin, tmax, st, 0, wc, [&](int off, int cnt, int sp) {
+        for (int k = 0; k < cnt; k++) {
+            int oi = S.titems[off + k];
+            double tt;
+            int32_t pr = -1;
+            if (object_t<CNT, FEAT>(S, oi, o, d, tmin, tmax, tt, pr, st, sp, media, mkey, serial, wc)) {
+                any = true;
+                tmax = tt;
+                h.t = tt;
+                h.obj = oi;
+                h.prim = pr;
             }
-        } else {
-            if (sp == 0) break;
-            cur = st.get(--sp);
         }
-    }
+    });
     return any;
 }
 

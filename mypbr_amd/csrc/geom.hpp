@@ -12,7 +12,7 @@
 #include <cmath>
 #include <cstdint>
 
-#if defined(__HIPCC__)
+#if defined(__HIP__)
 #define MPT_HD __host__ __device__ __forceinline__
 #else
 #define MPT_HD inline
@@ -58,6 +58,28 @@ MPT_HD v3 refract(const v3& uv, const v3& n, double etai_over_etat) {      // ve
 MPT_HD double luminance(const v3& c) { return 0.2126 * c.x + 0.7152 * c.y + 0.0722 * c.z; }
 MPT_HD double clampd(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
 MPT_HD double deg2rad(double d) { return d * kPi / 180.0; }
+// sin and cos of one angle; the device computes both from one argument reduction.
+MPT_HD void sin_cos(double x, double& s, double& c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    sincos(x, &s, &c);
+#else
+    s = std::sin(x);
+    c = std::cos(x);
+#endif
+}
+// sin and cos of 2 pi u for u in [0, 1): the reference's sin(2 * pi * u). The
+// device evaluates sin(pi * 2u) with a bounded-argument reduction (2u is
+// exact), which differs from rounding 2 pi u first by about an ulp; the host
+// keeps the reference's expression (the GGX table must match it bit for bit).
+MPT_HD void sin_cos_2pi(double u, double& s, double& c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    sincospi(2 * u, &s, &c);
+#else
+    double phi = 2 * kPi * u;
+    s = std::sin(phi);
+    c = std::cos(phi);
+#endif
+}
 
 // interval.h: default-constructed intervals are empty (+inf, -inf).
 struct interval {

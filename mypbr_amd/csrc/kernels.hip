@@ -107,8 +107,12 @@ static dcam init_camera(const mpt_camera& c, bool has_env) {
     return k;
 }
 
-__device__ v3 escaped(const dscene& S, const dcam& C, const v3& dir) {  // camera.h:515-529 (no sun)
-    if (C.sky_mode == 0) return env_radiance(S, dir);
+// ENV: the scene has an environment map (sky mode 0); without one only the
+// gradient and constant skies are compiled in.
+template <bool ENV = true>
+__device__ __forceinline__ v3 escaped(const dscene& S, const dcam& C, const v3& dir) {  // camera.h:515-529 (no sun)
+    if constexpr (ENV)
+        if (C.sky_mode == 0) return env_radiance(S, dir);
     if (C.sky_mode == 2) return C.background;
     v3 ud = unit_vector(dir);
     double a = 0.5 * (ud.y + 1.0);
@@ -135,6 +139,7 @@ enum {
     D_SBX, D_SBY, D_SBZ,                    // beta at the NEE vertex
     D_SW,                                   // MIS weight / light pdf
     D_AX, D_AY, D_AZ, D_NX, D_NY, D_NZ,     // first-surface guides
+    D_QX, D_QY, D_QZ, D_MX, D_MY, D_MZ,     // hit point and face normal (hit record)
     D_COUNT
 };
 
@@ -145,6 +150,7 @@ struct pstate {
     uint32_t* flags;   // bit0 specular, bit1 seen_by_shadow_ray, 10-bit counters from bit 2, 12, 22
     int32_t* hobj;
     int32_t* hprim;
+    uint32_t* hmat;    // material of the hit, bit 31 = front face
 };
 
 __device__ __forceinline__ v3 ld3(const pstate& P, int f, uint32_t i) { return v3{P.d[f][i], P.d[f + 1][i], P.d[f + 2][i]}; }
@@ -158,6 +164,11 @@ struct launch_ctx {
     int32_t* ovf;
     int64_t nthreads;
 };
+
+// Shading classes: 0 = missed everything, 1 + t = a hit on material type t.
+constexpr int kClasses = 7;
+// geometry profile of scenes made of spheres and quads, in lists and instances
+constexpr int kQuadProfile = F_SPHERE | F_QUAD | F_LIST | F_XFORM;
 
 // Wave-aggregated queue append: one atomic per wave, lane offsets from the
 // ballot's popcount prefix. Must be reached by every lane of the wave.
@@ -185,7 +196,7 @@ __device__ __forceinline__ void wave_append(uint32_t* counter, uint32_t* queue, 
     st.lds_depth = kLdsDepth;
 
 // ------------------------------------------------------------------ ray query
-template <int BS>
+template <int BS, int FEAT>
 __global__ __launch_bounds__(BS) void k_query(dscene S, launch_ctx L, const double* rays, const double* tt, int64_t n,
                                               int64_t* out_id, double* out_t, uint8_t* out_front) {
     STACK_SETUP();
@@ -194,7 +205,7 @@ __global__ __launch_bounds__(BS) void k_query(dscene S, launch_ctx L, const doub
         v3 d{rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]};
         hitinfo h;
         wcount wc{0, 0};
-        bool hit = trace_closest(S, o, d, tt[2 * i], tt[2 * i + 1], h, st, false, 0, 0, wc);
+        bool hit = trace_closest<false, FEAT>(S, o, d, tt[2 * i], tt[2 * i + 1], h, st, false, 0, 0, wc);
         if (hit) {
             hitrec rec;
             finish_hit(S, h, o, d, rec);
@@ -231,8 +242,7 @@ __global__ void k_camera(dcam C, pstate P, batch_args B, uint32_t* q0, uint32_t*
     int64_t gt = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (gt == 0) {
         cnt[0] = uint32_t(B.nslots);
-        cnt[1] = 0;
-        cnt[2] = 0;
+        for (int k = 1; k < 3 + kClasses; k++) cnt[k] = 0;
     }
     if (gt >= B.nslots) return;
     uint32_t slot = uint32_t(gt);
@@ -330,10 +340,20 @@ __global__ __launch_bounds__(BS) void k_features(dscene S, launch_ctx L, pstate 
 }
 
 // ------------------------------------------------------------------ extend
+// Closest hit of every queued path, then binning by what shades the hit:
+// class 0 is a miss, class 1 + t a surface or medium of material type t.
+// Each class gets its own queue, appended with one atomic per wave.
 // CNT: also count BVH work (nodes, primitive bytes) into work[0], work[1].
-template <int BS, bool CNT>
-__global__ __launch_bounds__(BS) void k_extend(dscene S, launch_ctx L, pstate P, int depth, const uint32_t* q,
+// Waves per SIMD the spheres-only traversal is compiled for (its register
+// budget): 8 fits it in 64 VGPRs with a few spills.
+#ifndef MPT_SPHERE_WAVES
+#define MPT_SPHERE_WAVES 8
+#endif
+
+template <int BS, bool CNT, int FEAT>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_SPHERE ? MPT_SPHERE_WAVES : 1))) void k_extend(dscene S, launch_ctx L, pstate P, int depth, const uint32_t* q,
                                                const uint32_t* cnt_in, uint32_t* cnt_next, uint32_t* cnt_shadow,
+                                               uint32_t* qcls, uint32_t* ccls, int64_t cap,
                                                unsigned long long* ray_stats, unsigned long long* work) {
     STACK_SETUP();
     const uint32_t n = *cnt_in;
@@ -343,14 +363,36 @@ __global__ __launch_bounds__(BS) void k_extend(dscene S, launch_ctx L, pstate P,
         atomicAdd(&ray_stats[0], (unsigned long long)n);
     }
     wcount wc{0, 0};
-    for (int64_t i = st.gtid; i < n; i += L.nthreads) {
-        uint32_t p = q[i];
-        v3 o = ld3(P, D_OX, p), d = ld3(P, D_DX, p);
-        hitinfo h;
-        trace_closest<CNT>(S, o, d, 0.001, kInf, h, st, true, P.key[p], uint32_t(2 * depth), wc);
-        P.d[D_HT][p] = h.t;
-        P.hobj[p] = h.obj;
-        P.hprim[p] = h.prim;
+    const int64_t wave = st.gtid / 64;
+    const int64_t nwaves = L.nthreads / 64;
+    const uint32_t lane = __lane_id();
+    for (int64_t base = wave * 64; base < n; base += nwaves * 64) {
+        const int64_t i = base + lane;
+        int cls = -1;
+        uint32_t p = 0;
+        if (i < n) {
+            p = q[i];
+            v3 o = ld3(P, D_OX, p), d = ld3(P, D_DX, p);
+            hitinfo h;
+            trace_closest<CNT, FEAT>(S, o, d, 0.001, kInf, h, st, true, P.key[p], uint32_t(2 * depth), wc);
+            P.d[D_HT][p] = h.t;
+            P.hobj[p] = h.obj;
+            P.hprim[p] = h.prim;
+            if (h.obj < 0) {
+                cls = 0;
+            } else {  // the hit record shade reads: point, face normal, material
+                hitrec rec;
+                finish_hit(S, h, o, d, rec);
+                st3(P, D_QX, p, rec.p);
+                st3(P, D_MX, p, rec.normal);
+                P.hmat[p] = uint32_t(rec.mat) | (rec.front ? 0x80000000u : 0u);
+                cls = 1 + S.mat[rec.mat].type;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kClasses; c++) {
+            wave_append(ccls + c, qcls + c * cap, cls == c, p);
+        }
     }
     if constexpr (CNT) {
         if (wc.nodes) atomicAdd(&work[0], (unsigned long long)wc.nodes);
@@ -359,7 +401,126 @@ __global__ __launch_bounds__(BS) void k_extend(dscene S, launch_ctx L, pstate P,
 }
 
 // ------------------------------------------------------------------ shade
-template <int BS>
+// One kernel per shading class. MT is the material type (-1: the paths that
+// missed); LIGHTS: the scene has lights, so non-specular vertices do next
+// event estimation. Compiling each material apart keeps each kernel's
+// register footprint to what that material needs.
+template <int MT, int LK, bool ENV>
+__device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const pstate& P, int depth, uint32_t p,
+                                          bool& alive, bool& nee) {
+    constexpr bool LIGHTS = LK != 0;
+    v3 o = ld3(P, D_OX, p), d = ld3(P, D_DX, p);
+    v3 beta = ld3(P, D_BX, p), rad = ld3(P, D_RX, p);
+    uint32_t fl = P.flags[p];
+    bool specular = fl & 1u, seen = fl & 2u;
+    int dcount = (fl >> 2) & 1023, scount = (fl >> 12) & 1023, vcount = (fl >> 22) & 1023;
+    if constexpr (MT < 0) {  // camera.h:373-387
+        v3 sky = escaped<ENV>(S, C, d);
+        if (!seen) {
+            if (!LIGHTS || specular) {
+                rad = rad + beta * sky;
+            } else {
+                double lp = lights_pdf<LK>(S, ld3(P, D_PX, p), d);
+                rad = rad + beta * sky * power_heuristic(P.d[D_SPDF][p], lp);
+            }
+        }
+        st3(P, D_RX, p, rad);
+        return;
+    } else {
+        hitrec rec;
+        rec.t = P.d[D_HT][p];
+        rec.p = ld3(P, D_QX, p);
+        rec.normal = ld3(P, D_MX, p);
+        const uint32_t hm = P.hmat[p];
+        rec.mat = int32_t(hm & 0x7fffffffu);
+        rec.front = (hm >> 31) != 0;
+        const mpt_material& m = S.mat[rec.mat];
+        if constexpr (MT == MPT_MAT_DIELECTRIC) beta = beta * mat_interior<MT>(m, d, rec);  // camera.h:396
+        if constexpr (MT == MPT_MAT_DIFFUSE_LIGHT) {                                          // camera.h:399-407
+            v3 e = mat_emitted<MT>(m, d, rec);
+            if (!seen && (e.x > 0 || e.y > 0 || e.z > 0)) {
+                if (!LIGHTS || specular) {
+                    rad = rad + beta * e;
+                } else {
+                    double lp = lights_pdf<LK>(S, ld3(P, D_PX, p), d);
+                    rad = rad + beta * e * power_heuristic(P.d[D_SPDF][p], lp);
+                }
+            }
+            st3(P, D_RX, p, rad);
+            return;  // a light scatters nothing (material.h:28-31)
+        } else {
+            rstream rs{P.key[p], P.ctr[p]};
+            scatter_out so = mat_scatter<MT>(S, m, d, rec, rs);  // camera.h:409-411
+            if (so.skip) {                                        // camera.h:413-423
+                bool stop = C.max_specular > 0 && ++scount > C.max_specular;
+                if (!stop) {
+                    beta = beta * so.attenuation;
+                    if (LIGHTS && C.transparent_shadows && mat_lets_shadows(m) && depth > 0 && !specular) seen = true;
+                    o = rec.p;
+                    d = so.skip_dir;
+                    specular = true;
+                    alive = true;
+                }
+            } else {  // camera.h:424-448
+                constexpr bool volume = MT == MPT_MAT_PHASE;
+                const int cap = volume ? C.max_scatter : C.max_diffuse;
+                bool capped = false;
+                if (cap > 0) capped = ++(volume ? vcount : dcount) >= cap;
+                seen = false;
+                if constexpr (LIGHTS) {  // sample_light, camera.h:466-506, up to the shadow ray
+                    const bool last_bounce = depth == C.max_depth - 1 || capped;
+                    v3 to_light = lights_random<LK>(S, rec.p, rs);
+                    double light_pdf = lights_pdf<LK>(S, rec.p, to_light);
+                    if (light_pdf > 0) {
+                        v3 f = mat_eval<MT>(S, m, d, rec, to_light);
+                        if (!(f.x <= 0 && f.y <= 0 && f.z <= 0)) {
+                            double weight =
+                                last_bounce ? 1.0 : power_heuristic(light_pdf, density_value<MT>(so.dens, to_light));
+                            st3(P, D_SOX, p, rec.p);
+                            st3(P, D_TLX, p, to_light);
+                            st3(P, D_FX, p, f);
+                            st3(P, D_SBX, p, beta);
+                            P.d[D_SW][p] = weight / light_pdf;
+                            nee = true;
+                        }
+                    }
+                }
+                if (!capped) {
+                    v3 dir = density_generate<MT>(so.dens, rs);
+                    double spdf = density_value<MT>(so.dens, dir);
+                    if (spdf > 0) {
+                        beta = beta * mat_eval<MT>(S, m, d, rec, dir) / spdf;
+                        P.d[D_SPDF][p] = spdf;
+                        st3(P, D_PX, p, rec.p);
+                        o = rec.p;
+                        d = dir;
+                        specular = false;
+                        alive = true;
+                    }
+                }
+            }
+            if (alive && depth >= 3) {  // camera.h:452-457
+                double survive = std::fmin(std::fmax(beta.x, std::fmax(beta.y, beta.z)), 0.95);
+                if (rs.next() >= survive) alive = false;
+                else beta = beta / survive;
+            }
+            P.ctr[p] = rs.ctr;
+        }
+    }
+    if (depth + 1 >= C.max_depth) alive = false;
+    if (!alive) {  // the path ends here: only its radiance is read again
+        st3(P, D_RX, p, rad);
+        return;
+    }
+    st3(P, D_OX, p, o);
+    st3(P, D_DX, p, d);
+    st3(P, D_BX, p, beta);
+    st3(P, D_RX, p, rad);
+    P.flags[p] = (specular ? 1u : 0u) | (seen ? 2u : 0u) | (uint32_t(dcount) << 2) | (uint32_t(scount) << 12) |
+                 (uint32_t(vcount) << 22);
+}
+
+template <int BS, int MT, int LK, bool ENV>
 __global__ __launch_bounds__(BS) void k_shade(dscene S, dcam C, pstate P, int depth, const uint32_t* q,
                                               const uint32_t* cnt_in, uint32_t* qnext, uint32_t* cnt_next,
                                               uint32_t* qshadow, uint32_t* cnt_shadow, int64_t nthreads) {
@@ -373,121 +534,24 @@ __global__ __launch_bounds__(BS) void k_shade(dscene S, dcam C, pstate P, int de
         uint32_t p = 0;
         if (i < n) {
             p = q[i];
-            v3 o = ld3(P, D_OX, p), d = ld3(P, D_DX, p);
-            v3 beta = ld3(P, D_BX, p), rad = ld3(P, D_RX, p);
-            uint32_t fl = P.flags[p];
-            bool specular = fl & 1u, seen = fl & 2u;
-            int dcount = (fl >> 2) & 1023, scount = (fl >> 12) & 1023, vcount = (fl >> 22) & 1023;
-            rstream rs{P.key[p], P.ctr[p]};
-            const int hobj = P.hobj[p];
-            if (hobj < 0) {  // camera.h:373-387
-                v3 sky = escaped(S, C, d);
-                if (!seen) {
-                    if (specular || S.nlights == 0) {
-                        rad = rad + beta * sky;
-                    } else {
-                        double lp = lights_pdf(S, ld3(P, D_PX, p), d);
-                        rad = rad + beta * sky * power_heuristic(P.d[D_SPDF][p], lp);
-                    }
-                }
-            } else {
-                hitinfo h{P.d[D_HT][p], hobj, P.hprim[p]};
-                hitrec rec;
-                finish_hit(S, h, o, d, rec);
-                const mpt_material& m = S.mat[rec.mat];
-                beta = beta * mat_interior(m, d, rec);  // camera.h:396
-                v3 e = mat_emitted(m, d, rec);            // camera.h:399-407
-                if (!seen && (e.x > 0 || e.y > 0 || e.z > 0)) {
-                    if (specular || S.nlights == 0) {
-                        rad = rad + beta * e;
-                    } else {
-                        double lp = lights_pdf(S, ld3(P, D_PX, p), d);
-                        rad = rad + beta * e * power_heuristic(P.d[D_SPDF][p], lp);
-                    }
-                }
-                scatter_out so = mat_scatter(S, m, d, rec, rs);  // camera.h:409-411
-                if (so.scattered) {
-                    if (so.skip) {  // camera.h:413-423
-                        bool stop = C.max_specular > 0 && ++scount > C.max_specular;
-                        if (!stop) {
-                            beta = beta * so.attenuation;
-                            if (C.transparent_shadows && mat_lets_shadows(m) && S.nlights > 0 && depth > 0 && !specular)
-                                seen = true;
-                            o = rec.p;
-                            d = so.skip_dir;
-                            specular = true;
-                            alive = true;
-                        }
-                    } else {  // camera.h:424-448
-                        bool volume = m.type == MPT_MAT_PHASE;
-                        int cap = volume ? C.max_scatter : C.max_diffuse;
-                        bool capped = false;
-                        if (cap > 0) {
-                            int& count = volume ? vcount : dcount;
-                            capped = ++count >= cap;
-                        }
-                        bool last_bounce = depth == C.max_depth - 1 || capped;
-                        seen = false;
-                        if (S.nlights > 0) {  // sample_light, camera.h:466-506, up to the shadow ray
-                            v3 to_light = lights_random(S, rec.p, rs);
-                            double light_pdf = lights_pdf(S, rec.p, to_light);
-                            if (light_pdf > 0) {
-                                v3 f = mat_eval(S, m, d, rec, to_light);
-                                if (!(f.x <= 0 && f.y <= 0 && f.z <= 0)) {
-                                    double weight =
-                                        last_bounce ? 1.0 : power_heuristic(light_pdf, density_value(so.dens, to_light));
-                                    st3(P, D_SOX, p, rec.p);
-                                    st3(P, D_TLX, p, to_light);
-                                    st3(P, D_FX, p, f);
-                                    st3(P, D_SBX, p, beta);
-                                    P.d[D_SW][p] = weight / light_pdf;
-                                    nee = true;
-                                }
-                            }
-                        }
-                        if (!capped) {
-                            v3 dir = density_generate(so.dens, rs);
-                            double spdf = density_value(so.dens, dir);
-                            if (spdf > 0) {
-                                beta = beta * mat_eval(S, m, d, rec, dir) / spdf;
-                                P.d[D_SPDF][p] = spdf;
-                                st3(P, D_PX, p, rec.p);
-                                o = rec.p;
-                                d = dir;
-                                specular = false;
-                                alive = true;
-                            }
-                        }
-                    }
-                    if (alive && depth >= 3) {  // camera.h:452-457
-                        double survive = std::fmin(std::fmax(beta.x, std::fmax(beta.y, beta.z)), 0.95);
-                        if (rs.next() >= survive) alive = false;
-                        else beta = beta / survive;
-                    }
-                }
-            }
-            if (depth + 1 >= C.max_depth) alive = false;
-            st3(P, D_OX, p, o);
-            st3(P, D_DX, p, d);
-            st3(P, D_BX, p, beta);
-            st3(P, D_RX, p, rad);
-            P.ctr[p] = rs.ctr;
-            P.flags[p] = (specular ? 1u : 0u) | (seen ? 2u : 0u) | (uint32_t(dcount) << 2) | (uint32_t(scount) << 12) |
-                         (uint32_t(vcount) << 22);
+            shade_one<MT, LK, ENV>(S, C, P, depth, p, alive, nee);
         }
         wave_append(cnt_next, qnext, alive, p);
-        wave_append(cnt_shadow, qshadow, nee, p);
+        if constexpr (LK != 0) wave_append(cnt_shadow, qshadow, nee, p);
     }
 }
 
 // ------------------------------------------------------------------ shadow
-template <int BS, bool CNT>
+template <int BS, bool CNT, int FEAT, bool ENV>
 __global__ __launch_bounds__(BS) void k_shadow(dscene S, dcam C, launch_ctx L, pstate P, int depth,
-                                               const uint32_t* qshadow, const uint32_t* cnt_shadow,
+                                               const uint32_t* qshadow, const uint32_t* cnt_shadow, uint32_t* ccls,
                                                unsigned long long* ray_stats, unsigned long long* work) {
     STACK_SETUP();
     const uint32_t n = *cnt_shadow;
-    if (st.gtid == 0) atomicAdd(&ray_stats[1], (unsigned long long)n);
+    if (st.gtid == 0) {
+        atomicAdd(&ray_stats[1], (unsigned long long)n);
+        for (int c = 0; c < kClasses; c++) ccls[c] = 0;  // every shade kernel of this bounce has read them
+    }
     wcount wc{0, 0};
     for (int64_t i = st.gtid; i < n; i += L.nthreads) {
         uint32_t p = qshadow[i];
@@ -498,9 +562,9 @@ __global__ __launch_bounds__(BS) void k_shadow(dscene S, dcam C, launch_ctx L, p
         v3 emission;
         for (int k = 0;; k++) {  // camera.h:488-502
             hitinfo h;
-            if (!trace_closest<CNT>(S, o, dir, 0.001, kInf, h, st, true, P.key[p],
+            if (!trace_closest<CNT, FEAT>(S, o, dir, 0.001, kInf, h, st, true, P.key[p],
                                     uint32_t(2 * depth + 1) | (uint32_t(k) << 20), wc)) {
-                emission = escaped(S, C, to_light);
+                emission = escaped<ENV>(S, C, to_light);
                 break;
             }
             hitrec rec;
@@ -523,6 +587,11 @@ __global__ __launch_bounds__(BS) void k_shadow(dscene S, dcam C, launch_ctx L, p
         if (wc.nodes) atomicAdd(&work[2], (unsigned long long)wc.nodes);
         if (wc.pbytes) atomicAdd(&work[3], (unsigned long long)wc.pbytes);
     }
+}
+
+// Zeroes n counters (stands in for the shadow stage when a scene has no lights).
+__global__ void k_zero(uint32_t* c, int n) {
+    if (int(threadIdx.x) < n) c[threadIdx.x] = 0;
 }
 
 // ------------------------------------------------------------------ accumulate
@@ -601,7 +670,14 @@ struct mpt_scene {
     int64_t ps_capacity = 0;
     std::vector<void*> ps_allocs;
     uint32_t* queues = nullptr;   // [3][capacity]: two ping-pong + shadow
-    uint32_t* counters = nullptr; // [8]
+    uint32_t* counters = nullptr; // queue lengths [0..9], then 64-bit ray and work counters
+    // kernels compiled for this scene: geometry profile, material types
+    // present (bit t for type t), whether next event estimation runs
+    int profile = 0;
+    uint32_t mat_types = 0;
+    bool has_lights = false;
+    int light_profile = 0;  // compiled NEE variant: 1 one-primitive quads, 2 environment only, 3 all
+    int32_t* pinned_counts = nullptr;  // host copies of queue lengths for the early exit
     double* acc = nullptr;        // radiance, albedo, normal (3 each) + l2
     int64_t acc_pix = 0;
     int last_done = 0;
@@ -616,6 +692,7 @@ struct mpt_scene {
         if (counters) (void)hipFree(counters);
         if (acc) (void)hipFree(acc);
         if (stream) (void)hipStreamDestroy(stream);
+        if (pinned_counts) (void)hipHostFree(pinned_counts);
     }
 };
 
@@ -657,7 +734,8 @@ int ensure_ovf(mpt_scene* s) {
     int depth_tlas = tree_depth(s->hs.tlas, 0);
     int depth_blas = 0;
     for (const auto& o : s->hs.objects)
-        if (o.kind == MPT_OBJ_MESH) depth_blas = std::max(depth_blas, tree_depth(s->hs.blas, o.blas_root));
+        if (o.kind == MPT_OBJ_MESH)
+            depth_blas = std::max(depth_blas, tree_depth(s->hs.blas, s->hs.blas_root_node[o.blas_root]));
     int need = depth_tlas + depth_blas + 2 - kLdsDepth;
     if (need < 1) need = 1;
     int64_t threads = int64_t(s->grid) * s->block;
@@ -671,12 +749,12 @@ int ensure_ovf(mpt_scene* s) {
 }
 
 int grid_for(mpt_scene* s) {
-    int blocks_per_cu = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &blocks_per_cu, reinterpret_cast<const void*>(&k_extend<256, false>), 256, 0));
-    if (blocks_per_cu < 1) blocks_per_cu = 1;
+    // Persistent stage kernels walk their queue with a grid-wide stride. The
+    // grid is what the device holds when every kernel is at its best
+    // occupancy (8 blocks of 256 threads per CU); kernels that fit fewer
+    // blocks run the rest as a second round.
     s->block = 256;
-    s->grid = s->cu_count * blocks_per_cu;
+    s->grid = s->cu_count * 8;
     return MPT_OK;
 }
 
@@ -719,9 +797,12 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
     if ((rc = upload(s.get(), h.prim_mat, &d.pmat))) return rc;
     if ((rc = upload(s.get(), h.prim_canon, &d.pcanon))) return rc;
     if ((rc = upload(s.get(), h.objects, &d.obj))) return rc;
-    if ((rc = upload(s.get(), h.tlas, &d.tlas))) return rc;
+    if ((rc = upload(s.get(), h.ctlas, &d.tlas))) return rc;
     if ((rc = upload(s.get(), h.tlas_items, &d.titems))) return rc;
-    if ((rc = upload(s.get(), h.blas, &d.blas))) return rc;
+    if ((rc = upload(s.get(), h.cblas, &d.blas))) return rc;
+    if ((rc = upload(s.get(), h.cblas_roots, &d.blas_roots))) return rc;
+    if ((rc = upload(s.get(), h.spheres, &d.sph))) return rc;
+    d.tlas_root = h.ctlas_root;
     if ((rc = upload(s.get(), h.xforms, &d.xf))) return rc;
     if ((rc = upload(s.get(), h.materials, &d.mat))) return rc;
     if ((rc = upload(s.get(), h.textures, &d.tex))) return rc;
@@ -741,6 +822,38 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
     d.lstrategy = h.light_strategy;
     d.env_w = h.env_w;
     d.env_h = h.env_h;
+    // geometry profile: spheres only with object i = primitive i, quads and
+    // spheres in lists and instances, or everything
+    {
+        bool prims_only = true, quads_ok = true;
+        for (size_t o = 0; o < h.objects.size(); o++) {
+            const obj_rec& r = h.objects[o];
+            if (r.kind != MPT_OBJ_PRIM || r.xform >= 0 || r.first != int64_t(o) ||
+                h.prim_kind[r.first] != MPT_PRIM_SPHERE)
+                prims_only = false;
+            if (r.kind == MPT_OBJ_MESH || r.kind == MPT_OBJ_MEDIUM) quads_ok = false;
+        }
+        for (int32_t k : h.prim_kind)
+            if (k == MPT_PRIM_TRIANGLE) quads_ok = false;
+        s->profile = prims_only && !h.objects.empty() ? 1 : (quads_ok ? 2 : 0);
+        for (int32_t m : h.prim_mat)
+            if (m >= 0) s->mat_types |= 1u << h.materials[m].type;
+        for (const auto& r : h.objects)
+            if (r.kind == MPT_OBJ_MEDIUM) s->mat_types |= 1u << MPT_MAT_PHASE;
+        s->has_lights = !h.lights.empty();
+        bool quads = true, env_only = true;
+        for (const auto& L : h.lights) {
+            if (L.object < 0) {
+                quads = false;
+                continue;
+            }
+            env_only = false;
+            const obj_rec& r = h.objects[L.object];
+            if (r.kind != MPT_OBJ_PRIM || r.xform >= 0 || h.prim_kind[r.first] != MPT_PRIM_QUAD) quads = false;
+        }
+        s->light_profile = !s->has_lights ? 0 : (quads ? 1 : (env_only ? 2 : 3));
+    }
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_counts), 8 * sizeof(int32_t), hipHostMallocDefault));
     if ((rc = grid_for(s.get()))) return rc;
     if ((rc = ensure_ovf(s.get()))) return rc;
     *out = s.release();
@@ -787,7 +900,13 @@ int32_t mpt_trace_closest_dev(mpt_scene* s, const double* rays, const double* tt
     if (!rays || !tt || !id || !t || !front) return set_error(MPT_E_INVALID, "null buffer");
     launch_ctx L{s->ovf, int64_t(s->grid) * s->block};
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
-    hipLaunchKernelGGL(k_query<256>, dim3(s->grid), dim3(256), 0, st, s->ds, L, rays, tt, n, id, t, front);
+    if (s->profile == 1)
+        hipLaunchKernelGGL((k_query<256, F_SPHERE>), dim3(s->grid), dim3(256), 0, st, s->ds, L, rays, tt, n, id, t, front);
+    else if (s->profile == 2)
+        hipLaunchKernelGGL((k_query<256, kQuadProfile>), dim3(s->grid), dim3(256), 0, st, s->ds, L, rays, tt, n, id, t,
+                           front);
+    else
+        hipLaunchKernelGGL((k_query<256, F_ALL>), dim3(s->grid), dim3(256), 0, st, s->ds, L, rays, tt, n, id, t, front);
     HIP_TRY(hipGetLastError());
     if (!stream) HIP_TRY(hipStreamSynchronize(st));
     return MPT_OK;
@@ -878,8 +997,9 @@ int ensure_paths(mpt_scene* s, int64_t cap) {
     if ((rc = alloc(size_t(cap) * sizeof(uint32_t), reinterpret_cast<void**>(&s->ps.flags)))) return rc;
     if ((rc = alloc(size_t(cap) * sizeof(int32_t), reinterpret_cast<void**>(&s->ps.hobj)))) return rc;
     if ((rc = alloc(size_t(cap) * sizeof(int32_t), reinterpret_cast<void**>(&s->ps.hprim)))) return rc;
-    HIP_TRY(hipMalloc(&s->queues, size_t(cap) * 3 * sizeof(uint32_t)));
-    if (!s->counters) HIP_TRY(hipMalloc(&s->counters, 8 * sizeof(uint64_t)));
+    if ((rc = alloc(size_t(cap) * sizeof(uint32_t), reinterpret_cast<void**>(&s->ps.hmat)))) return rc;
+    HIP_TRY(hipMalloc(&s->queues, size_t(cap) * (3 + kClasses) * sizeof(uint32_t)));
+    if (!s->counters) HIP_TRY(hipMalloc(&s->counters, 32 * sizeof(uint64_t)));
     s->ps_capacity = cap;
     return MPT_OK;
 }
@@ -943,8 +1063,9 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
     if (total < 1) return set_error(MPT_E_INVALID, "no passes to render (first_pass >= samples_per_pixel)");
     int pif = o.passes_in_flight;
     if (pif <= 0) {
-        // about 4M paths in flight: fills the device several times over
-        pif = int(std::max<int64_t>(1, (int64_t(4) << 20) / npix64));
+        // about 16M paths in flight: many times what the device holds at once,
+        // so the tail of each bounce (few paths left) is a small share
+        pif = int(std::max<int64_t>(1, (int64_t(16) << 20) / npix64));
     }
     pif = std::min(pif, total);
     if (int64_t(pif) * npix64 > (int64_t(1) << 31)) pif = int((int64_t(1) << 31) / npix64);
@@ -969,15 +1090,17 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
 
     const int64_t cap = s->ps_capacity;
     uint32_t* q[3] = {s->queues, s->queues + cap, s->queues + 2 * cap};
-    uint32_t* cnt = s->counters;  // [0], [1] ping-pong queue lengths, [2] shadow queue length
+    uint32_t* qcls = s->queues + 3 * cap;  // kClasses queues of cap entries
+    uint32_t* cnt = s->counters;           // [0], [1] ping-pong queue lengths, [2] shadow, [3..9] classes
+    uint32_t* ccls = cnt + 3;
     launch_ctx L{s->ovf, int64_t(s->grid) * s->block};
     const pstate P = s->ps;
     double* acc_rad = s->acc;
     double* acc_alb = acc_rad + 3 * size_t(npix);
     double* acc_nrm = acc_alb + 3 * size_t(npix);
     double* acc_l2 = acc_nrm + 3 * size_t(npix);
-    unsigned long long* ray_stats = reinterpret_cast<unsigned long long*>(cnt + 4);  // [0] extension, [1] shadow
-    unsigned long long* work = ray_stats + 2;                                       // nodes/bytes ext, shadow
+    unsigned long long* ray_stats = reinterpret_cast<unsigned long long*>(s->counters) + 8;  // [0] ext, [1] shadow
+    unsigned long long* work = ray_stats + 2;                                               // nodes/bytes ext, shadow
     HIP_TRY(hipMemsetAsync(ray_stats, 0, 6 * sizeof(unsigned long long), s->stream));
     const bool timing = (o.profile & 1) != 0;
     const bool counting = (o.profile & 2) != 0;
@@ -1022,11 +1145,18 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
         ev_used = 0;
         return MPT_OK;
     };
+    // a small ring of events marks when each bounce's queue length has been
+    // copied to pinned host memory; the host looks two bounces back before
+    // launching the next one and stops once no path is left
+    constexpr int kRing = 4;
+    hipEvent_t ring[kRing] = {};
+    for (int k = 0; k < kRing; k++) HIP_TRY(hipEventCreateWithFlags(&ring[k], hipEventDisableTiming));
     auto ev_free = [&]() {
         for (auto& t : evs) {
             (void)hipEventDestroy(t.a);
             (void)hipEventDestroy(t.b);
         }
+        for (auto& e : ring) (void)hipEventDestroy(e);
     };
 #define EV(stage, launch)                            \
     do {                                             \
@@ -1041,8 +1171,80 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
         }                                            \
     } while (0)
 
-    auto extend = counting ? k_extend<256, true> : k_extend<256, false>;
-    auto shadow = counting ? k_shadow<256, true> : k_shadow<256, false>;
+    const dscene DS = s->ds;
+    const bool has_env = s->hs.env_w > 0;
+    const unsigned G = unsigned(s->grid);
+    hipStream_t stream = s->stream;
+    auto launch_extend = [&](int depth, uint32_t* qin, uint32_t* cin, uint32_t* cout) {
+#define EXT(FEAT)                                                                                                    \
+    do {                                                                                                             \
+        if (counting)                                                                                                \
+            hipLaunchKernelGGL((k_extend<256, true, FEAT>), dim3(G), dim3(256), 0, stream, DS, L, P, depth, qin, cin, \
+                               cout, cnt + 2, qcls, ccls, cap, ray_stats, work);                                     \
+        else                                                                                                         \
+            hipLaunchKernelGGL((k_extend<256, false, FEAT>), dim3(G), dim3(256), 0, stream, DS, L, P, depth, qin,    \
+                               cin, cout, cnt + 2, qcls, ccls, cap, ray_stats, work);                                \
+    } while (0)
+        if (s->profile == 1) EXT(F_SPHERE);
+        else if (s->profile == 2) EXT(kQuadProfile);
+        else EXT(F_ALL);
+#undef EXT
+    };
+    auto launch_shadow = [&](int depth) {
+#define SHD2(FEAT, ENV)                                                                                             \
+    do {                                                                                                            \
+        if (counting)                                                                                               \
+            hipLaunchKernelGGL((k_shadow<256, true, FEAT, ENV>), dim3(G), dim3(256), 0, stream, DS, C, L, P, depth,  \
+                               q[2], cnt + 2, ccls, ray_stats, work);                                               \
+        else                                                                                                        \
+            hipLaunchKernelGGL((k_shadow<256, false, FEAT, ENV>), dim3(G), dim3(256), 0, stream, DS, C, L, P, depth, \
+                               q[2], cnt + 2, ccls, ray_stats, work);                                               \
+    } while (0)
+#define SHD(FEAT)                       \
+    do {                                \
+        if (has_env) SHD2(FEAT, true);  \
+        else SHD2(FEAT, false);         \
+    } while (0)
+        if (s->profile == 1) SHD(F_SPHERE);
+        else if (s->profile == 2) SHD(kQuadProfile);
+        else SHD(F_ALL);
+#undef SHD
+#undef SHD2
+    };
+    auto launch_shade = [&](int cls, int depth, uint32_t* qout, uint32_t* cout) {
+        const uint32_t* qc = qcls + cls * cap;
+        const uint32_t* cc = ccls + cls;
+#define SHA2(MT, LK, ENV)                                                                                      \
+    hipLaunchKernelGGL((k_shade<256, MT, LK, ENV>), dim3(G), dim3(256), 0, stream, DS, C, P, depth, qc, cc, qout,  \
+                       cout, q[2], cnt + 2, L.nthreads)
+#define SHA1(MT, LK)                                          \
+    do {                                                      \
+        if (MT < 0 && has_env) SHA2(MT, LK, (MT < 0));        \
+        else SHA2(MT, LK, false);                             \
+    } while (0)
+#define SHA(MT)                                      \
+    do {                                             \
+        switch (s->light_profile) {                  \
+            case 0: SHA1(MT, 0); break;              \
+            case 1: SHA1(MT, L_QUAD); break;         \
+            case 2: SHA1(MT, L_ENV); break;          \
+            default: SHA1(MT, L_ALL); break;         \
+        }                                            \
+    } while (0)
+        switch (cls) {
+            case 0: SHA(-1); break;
+            case 1 + MPT_MAT_LAMBERTIAN: SHA(MPT_MAT_LAMBERTIAN); break;
+            case 1 + MPT_MAT_METAL: SHA(MPT_MAT_METAL); break;
+            case 1 + MPT_MAT_DIELECTRIC: SHA(MPT_MAT_DIELECTRIC); break;
+            case 1 + MPT_MAT_PRINCIPLED: SHA(MPT_MAT_PRINCIPLED); break;
+            case 1 + MPT_MAT_DIFFUSE_LIGHT: SHA(MPT_MAT_DIFFUSE_LIGHT); break;
+            default: SHA(MPT_MAT_PHASE); break;
+        }
+#undef SHA
+#undef SHA1
+#undef SHA2
+    };
+
     int pass = 0;
     while (pass < total) {
         int nb = std::min(pif, total - pass);
@@ -1052,25 +1254,34 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
             nb = std::min(nb, next_save - done_abs);
         }
         batch_args B{o.seed, uint32_t(o.scene_index), o.first_pass + pass, npix, nb * npix};
-        EV(3, hipLaunchKernelGGL(k_camera, dim3(unsigned((B.nslots + 255) / 256)), dim3(256), 0, s->stream, C, P, B,
-                                 q[0], cnt));
-        if (C.features)
-            EV(3, hipLaunchKernelGGL(k_features<256>, dim3(s->grid), dim3(256), 0, s->stream, s->ds, L, P, B));
+        EV(3, hipLaunchKernelGGL(k_camera, dim3(unsigned((B.nslots + 255) / 256)), dim3(256), 0, stream, C, P, B, q[0],
+                                 cnt));
+        if (C.features) EV(3, hipLaunchKernelGGL((k_features<256>), dim3(G), dim3(256), 0, stream, DS, L, P, B));
         for (int depth = 0; depth < C.max_depth; depth++) {
+            if (depth >= 2) {  // no path left two bounces ago: the rest is empty
+                if (hipEventSynchronize(ring[(depth - 2) % kRing]) != hipSuccess) {
+                    ev_free();
+                    return set_error(MPT_E_HIP, "hipEventSynchronize failed");
+                }
+                if (s->pinned_counts[(depth - 2) % kRing] == 0) break;
+            }
             uint32_t* qin = q[depth & 1];
             uint32_t* qout = q[(depth + 1) & 1];
             uint32_t* cin = cnt + (depth & 1);
             uint32_t* cout = cnt + ((depth + 1) & 1);
             // extend zeroes the next-queue and shadow counters of this bounce;
-            // stream order puts it after the previous bounce's shadow stage
-            EV(0, hipLaunchKernelGGL(extend, dim3(s->grid), dim3(256), 0, s->stream, s->ds, L, P, depth, qin, cin, cout,
-                                     cnt + 2, ray_stats, work));
-            EV(1, hipLaunchKernelGGL(k_shade<256>, dim3(s->grid), dim3(256), 0, s->stream, s->ds, C, P, depth, qin, cin,
-                                     qout, cout, q[2], cnt + 2, L.nthreads));
-            EV(2, hipLaunchKernelGGL(shadow, dim3(s->grid), dim3(256), 0, s->stream, s->ds, C, L, P, depth, q[2], cnt + 2,
-                                     ray_stats, work));
+            // shadow zeroes the class counters once every shade kernel read them
+            EV(0, launch_extend(depth, qin, cin, cout));
+            for (int cls = 0; cls < kClasses; cls++) {
+                if (cls > 0 && !(s->mat_types & (1u << (cls - 1)))) continue;
+                EV(1, launch_shade(cls, depth, qout, cout));
+            }
+            if (s->has_lights) EV(2, launch_shadow(depth));
+            else EV(2, hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, ccls, kClasses));
+            HIP_TRY(hipMemcpyAsync(&s->pinned_counts[depth % kRing], cout, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipEventRecord(ring[depth % kRing], stream));
         }
-        EV(3, hipLaunchKernelGGL(k_accumulate, dim3(unsigned((npix + 255) / 256)), dim3(256), 0, s->stream, P, npix, nb,
+        EV(3, hipLaunchKernelGGL(k_accumulate, dim3(unsigned((npix + 255) / 256)), dim3(256), 0, stream, P, npix, nb,
                                  C.features, acc_rad, acc_alb, acc_nrm, acc_l2));
         if (hipGetLastError() != hipSuccess) {
             ev_free();
@@ -1083,13 +1294,13 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
         pass += nb;
         s->last_done = done_before + pass;
         if (on_pass && o.save_every > 0 && s->last_done % o.save_every == 0 && pass < total) {
-            HIP_TRY(hipStreamSynchronize(s->stream));
+            HIP_TRY(hipStreamSynchronize(stream));
             on_pass(s->last_done, user);
         }
     }
 #undef EV
     ev_free();
-    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipStreamSynchronize(stream));
     auto t1 = std::chrono::steady_clock::now();
     if (radiance || albedo || normal || variance) {
         if ((rc = mpt_read_frame(s, radiance, albedo, normal, variance))) return rc;

@@ -67,9 +67,10 @@ MPT_HD v3 ggx_sample_visible(double alpha, const v3& wo, double u1, double u2) {
     v3 t1 = lensq > 0 ? v3{-vh.y, vh.x, 0} / std::sqrt(lensq) : v3{1, 0, 0};
     v3 t2 = cross(vh, t1);
     double r = std::sqrt(u1);
-    double phi = 2 * kPi * u2;
-    double p1 = r * std::cos(phi);
-    double p2 = r * std::sin(phi);
+    double sp, cp;
+    sin_cos_2pi(u2, sp, cp);  // phi = 2 pi u2
+    double p1 = r * cp;
+    double p2 = r * sp;
     double s = 0.5 * (1 + vh.z);
     p2 = (1 - s) * std::sqrt(std::fmax(0.0, 1 - p1 * p1)) + s * p2;
     v3 nh = p1 * t1 + p2 * t2 + std::sqrt(std::fmax(0.0, 1 - p1 * p1 - p2 * p2)) * vh;

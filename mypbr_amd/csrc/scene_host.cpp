@@ -173,6 +173,43 @@ void build_bvh(std::vector<build_item>& items, std::vector<bvh_node>& nodes) {
     b.build(0, items.size(), 0);
 }
 
+void pair_layout(const std::vector<bvh_node>& nodes, int root, std::vector<cnode>& out, croot& r) {
+    const bvh_node& R = nodes[root];
+    for (int k = 0; k < 6; k++) r.b[k] = R.b[k];
+    r.pad[0] = r.pad[1] = 0;
+    if (R.count > 0) {
+        r.off = R.offset;
+        r.cnt = R.count;
+        return;
+    }
+    r.cnt = 0;
+    r.off = int32_t(out.size());
+    // depth-first: a record's index is fixed before its children are made
+    std::vector<std::pair<int, int>> todo{{root, int(out.size())}};
+    out.push_back(cnode{});
+    while (!todo.empty()) {
+        auto [n, idx] = todo.back();
+        todo.pop_back();
+        const int child[2] = {n + 1, nodes[n].offset};
+        cnode c{};
+        c.axis = nodes[n].axis;
+        for (int w = 0; w < 2; w++) {
+            const bvh_node& ch = nodes[child[w]];
+            for (int k = 0; k < 6; k++) c.b[6 * w + k] = ch.b[k];
+            if (ch.count > 0) {
+                c.off[w] = ch.offset;
+                c.cnt[w] = ch.count;
+            } else {
+                c.off[w] = int32_t(out.size());
+                c.cnt[w] = 0;
+                out.push_back(cnode{});
+                todo.push_back({child[w], c.off[w]});
+            }
+        }
+        out[idx] = c;
+    }
+}
+
 void build_ggx_table(std::vector<double>& e, std::vector<double>& e_avg) {  // microfacet.h:299-337
     const int size = kGgxTableSize;
     const int strata = 64;
@@ -366,10 +403,14 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
                         s.prim_mat[r.first + k] = tmat[k];
                         s.prim_canon[r.first + k] = items[k].ref;
                     }
-                    it = mesh_roots.emplace(key, root).first;
+                    croot cr;
+                    pair_layout(s.blas, root, s.cblas, cr);
+                    s.cblas_roots.push_back(cr);
+                    s.blas_root_node.push_back(root);
+                    it = mesh_roots.emplace(key, int32_t(s.cblas_roots.size() - 1)).first;
                 }
                 r.blas_root = it->second;
-                const bvh_node& rn = s.blas[r.blas_root];
+                const bvh_node& rn = s.blas[s.blas_root_node[r.blas_root]];
                 for (int a = 0; a < 3; a++) b.ax[a] = interval{rn.b[2 * a], rn.b[2 * a + 1]};
                 break;
             }
@@ -407,6 +448,8 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         items[o].ref = o;
     }
     build_bvh(items, s.tlas);
+    s.ctlas.clear();
+    pair_layout(s.tlas, 0, s.ctlas, s.ctlas_root);
     s.tlas_items.resize(no);
     for (int o = 0; o < no; o++) s.tlas_items[o] = int32_t(items[o].ref);
     s.world_box = box_empty();
@@ -526,6 +569,9 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
     for (int k = 0; k < nl; k++) s.light_cdf[k + 1] = s.light_cdf[k] + s.lights[k].probability;
     if (nl > 0) s.light_cdf[nl] = 1;
 
+    s.spheres.assign(size_t(np) * 4, 0.0);
+    for (int64_t i = 0; i < np; i++)
+        for (int k = 0; k < 4; k++) s.spheres[size_t(i) * 4 + k] = s.prim[size_t(i) * kPrimStride + k];
     build_ggx_table(s.ggx_e, s.ggx_e_avg);
     return MPT_OK;
 }

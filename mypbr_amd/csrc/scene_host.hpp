@@ -24,6 +24,28 @@ struct bvh_node {
 };
 static_assert(sizeof(bvh_node) == 64, "bvh_node must be 64 bytes");
 
+// The same tree re-laid for the GPU walk: one record per interior node
+// holding both children's boxes (left = the reference's index + 1, right =
+// its offset) and what each child is: a leaf's item range (count > 0) or the
+// record of an interior child (count == 0). 128 bytes.
+struct cnode {
+    double b[12];      // left box (xmin, xmax, ymin, ymax, zmin, zmax), then right box
+    int32_t off[2];    // leaf: first item; interior: cnode index
+    int32_t cnt[2];    // leaf: item count; interior: 0
+    int32_t axis;      // split axis of the parent (near-first order)
+    int32_t pad[3];
+};
+static_assert(sizeof(cnode) == 128, "cnode must be 128 bytes");
+
+// Root of a re-laid tree: its box and what it is (as a cnode child).
+struct croot {
+    double b[6];
+    int32_t off;
+    int32_t cnt;
+    int32_t pad[2];
+};
+static_assert(sizeof(croot) == 64, "croot must be 64 bytes");
+
 // Primitive record the kernels read: 16 doubles (128 bytes).
 //   sphere:   [0..2] center, [3] radius
 //   quad:     [0..2] Q, [3..5] u, [6..8] v, [9..11] unit normal, [12..14] w, [15] D
@@ -38,7 +60,7 @@ struct obj_rec {
     int64_t first;       // first primitive (device order: BLAS leaf order for meshes)
     int64_t count;
     int64_t id_base;     // first canonical ID
-    int32_t blas_root;   // meshes: root node in the BLAS node array
+    int32_t blas_root;   // meshes: index of the BVH root (blas node array / cblas_roots)
     int32_t material;    // media: phase material
     double density;      // media
     int32_t bkind;       // media: boundary kind (MPT_OBJ_PRIM or MPT_OBJ_LIST)
@@ -61,6 +83,13 @@ struct host_scene {
     std::vector<bvh_node> tlas;
     std::vector<int32_t> tlas_items;     // object indices in leaf order
     std::vector<bvh_node> blas;          // all mesh BVHs back to back
+    // the same trees in child-pair records for the device walk
+    std::vector<cnode> ctlas;
+    croot ctlas_root{};
+    std::vector<cnode> cblas;
+    std::vector<croot> cblas_roots;      // indexed by obj_rec.blas_root
+    std::vector<int32_t> blas_root_node; // cblas_roots index -> root in blas (for depths)
+    std::vector<double> spheres;         // center xyz, radius of every primitive (spheres only meaningful)
     std::vector<xform> xforms;
     std::vector<mpt_material> materials;
     std::vector<mpt_texture> textures;
@@ -93,6 +122,9 @@ struct build_item {
     int64_t ref;
 };
 void build_bvh(std::vector<build_item>& items, std::vector<bvh_node>& nodes);
+
+// Re-lays the tree rooted at nodes[root] as child-pair records appended to out.
+void pair_layout(const std::vector<bvh_node>& nodes, int root, std::vector<cnode>& out, croot& r);
 
 // ggx_albedo_table (microfacet.h:299-337).
 void build_ggx_table(std::vector<double>& e, std::vector<double>& e_avg);
