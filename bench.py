@@ -11,9 +11,10 @@ render does. Scene data is resident in HBM before the timed region.
 
 The JSON line also carries
   roofline      the dominant stage kernel, its algorithmic bytes per launch
-                (rays x fixed ray/hit bytes + BVH nodes x 64 B + primitive
+                (rays x fixed ray/hit bytes + BVH node bytes + primitive
                 bytes, counted by the instrumented kernels on the same
-                paths) over its HIP-event-timed average launch duration;
+                paths) over its HIP-event-timed average launch duration,
+                and its measured HBM traffic per launch (profiles/);
   cpu_baseline  the reference renderer itself (oracle/_ref, compiled from the
                 reference's sources) timed on this host's cores on a bounded
                 sample of the same workload, or the oracle when the
@@ -31,9 +32,9 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 METRIC = "paths/s at 1080p; equal-time relMSE vs 4096spp ref (learned vs plain)"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 64
-# per extension ray: queue index 4 + ray 48 + key 8 read, hit t/object/primitive 16 written
-EXT_RAY_BYTES = 76
+# per extension ray: queue index 4 + ray 48 + key 8 read, hit t/object/primitive 16 written,
+# class-queue append 4 (the hit record written for hits is not counted)
+EXT_RAY_BYTES = 80
 # per shadow ray: queue 4 + origin 24 + to_light 24 + key 8 + f 24 + weight 8 + beta 24 + radiance 24 read,
 # radiance 24 written
 SHADOW_RAY_BYTES = 164
@@ -89,6 +90,31 @@ def cpu_baseline(scene_path, cam, seconds, threads):
             "sample": f"CPU oracle (restatement) {w}x{h} passes 0..{spp - 1} of the 256-spp camera", "seconds": dt}
 
 
+def copy_calibration(mib):
+    import torch
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(4):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "dispatches": 4}), flush=True)
+
+
+def traffic_per_launch(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary: FETCH_SIZE + WRITE_SIZE (KiB per dispatch, averaged over the
+    kernel's dispatches in one step), each scaled by the ratio of the known
+    bytes of a plain copy kernel to what the same counter reads for it."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)[kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return 1024.0 * (t["fetch_kib_per_dispatch"] * t["fetch_scale"] + t["write_kib_per_dispatch"] * t["write_scale"])
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,7 +125,13 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device 4 times (a plain copy kernel of known size, "
+                         "the scale for rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(paths shard by pass or tile with one sum-reduce; see the design notes)", file=sys.stderr)
@@ -162,11 +194,11 @@ def main():
         fc = s.render(cam, seed=1, first_pass=0, pass_count=pps, profile=2, read=False).stats
         names = ["extend", "shade", "shadow", "camera+accumulate"]
         stage = {names[i]: {"ms": ft.stage_ms[i], "launches": ft.stage_launches[i]} for i in range(4)}
-        ext_bytes = fc.extension_rays * EXT_RAY_BYTES + fc.node_visits[0] * NODE_BYTES + fc.prim_bytes[0]
-        sh_bytes = fc.shadow_rays * SHADOW_RAY_BYTES + fc.node_visits[1] * NODE_BYTES + fc.prim_bytes[1]
+        ext_bytes = fc.extension_rays * EXT_RAY_BYTES + fc.node_bytes[0] + fc.prim_bytes[0]
+        sh_bytes = fc.shadow_rays * SHADOW_RAY_BYTES + fc.node_bytes[1] + fc.prim_bytes[1]
         stage["extend"]["algorithmic_bytes"] = ext_bytes
         stage["shadow"]["algorithmic_bytes"] = sh_bytes
-        stage["extend"]["nodes_per_ray"] = fc.node_visits[0] / max(fc.extension_rays, 1)
+        stage["extend"]["node_bytes_per_ray"] = fc.node_bytes[0] / max(fc.extension_rays, 1)
         stage["extend"]["prim_bytes_per_ray"] = fc.prim_bytes[0] / max(fc.extension_rays, 1)
         line["stages"] = stage
         dom = max(("extend", "shadow"), key=lambda k: stage[k]["ms"])
@@ -175,7 +207,7 @@ def main():
         bytes_per_launch = stage[dom]["algorithmic_bytes"] / launches
         achieved = bytes_per_launch / avg_s / 1e9
         line["roofline"] = {"kernel": f"k_{dom}", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_per_launch(f"k_{dom}"),
                             "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6}
         log("stages:", json.dumps(stage))
 

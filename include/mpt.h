@@ -197,9 +197,10 @@ typedef struct mpt_render_stats {
      * [0] extend, [1] shade, [2] shadow, [3] camera + features + accumulate */
     double stage_ms[4];
     int64_t stage_launches[4];
-    /* with profile bit 1: BVH nodes visited and primitive bytes read by the
-     * extension [0] and shadow [1] traversals */
-    int64_t node_visits[2];
+    /* with profile bit 1: BVH node bytes and primitive bytes read by the
+     * extension [0] and shadow [1] traversals (a root box counts 64 bytes,
+     * an interior child-pair record 128, a child re-tested on pop 64) */
+    int64_t node_bytes[2];
     int64_t prim_bytes[2];
 } mpt_render_stats;
 

@@ -108,7 +108,7 @@ class RenderOpts(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_int64), ("extension_rays", C.c_int64), ("shadow_rays", C.c_int64),
                 ("seconds", C.c_double), ("stage_ms", C.c_double * 4), ("stage_launches", C.c_int64 * 4),
-                ("node_visits", C.c_int64 * 2), ("prim_bytes", C.c_int64 * 2)]
+                ("node_bytes", C.c_int64 * 2), ("prim_bytes", C.c_int64 * 2)]
 
 
 PASS_CALLBACK = C.CFUNCTYPE(None, C.c_int32, C.c_void_p)

@@ -1317,9 +1317,9 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
             stats->stage_ms[k] = stage_ms[k];
             stats->stage_launches[k] = stage_n[k];
         }
-        stats->node_visits[0] = int64_t(rs_host[2]);
+        stats->node_bytes[0] = int64_t(rs_host[2]);
         stats->prim_bytes[0] = int64_t(rs_host[3]);
-        stats->node_visits[1] = int64_t(rs_host[4]);
+        stats->node_bytes[1] = int64_t(rs_host[4]);
         stats->prim_bytes[1] = int64_t(rs_host[5]);
     }
     return MPT_OK;

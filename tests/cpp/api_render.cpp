@@ -1,0 +1,97 @@
+// api_render.cpp - a program written against the C++ scene API
+// (include/mpt/mpt.hpp) the way a program of the reference renderer is:
+// it loads a .scene file, or builds a world from objects, and calls
+// camera::render (camera.h:79-105). Used by tests/test_cpp_api.py.
+//
+//   api_render scene <file.scene> <width> <spp> <seed> <out.bin>
+//       renders the file's world and lights with its camera at the given
+//       width and spp; writes the radiance (h*w*3 doubles) to out.bin
+//   api_render objects <out.bin>
+//       renders a small world built from objects (spheres, a box() behind a
+//       rotate/translate instance, a quad light in a light_list, a checker
+//       floor) with save_every 2; prints the callback counts and writes the
+//       radiance of the returned frame followed by the last callback frame
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+
+#include "mpt/mpt.hpp"
+
+using namespace mpt;
+
+static bool write_frame(const char* path, const std::vector<color>& a, const std::vector<color>* b) {
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return false;
+    for (const auto& c : a) std::fwrite(c.v, sizeof(double), 3, f);
+    if (b)
+        for (const auto& c : *b) std::fwrite(c.v, sizeof(double), 3, f);
+    return std::fclose(f) == 0;
+}
+
+int main(int argc, char** argv) {
+    if (argc >= 7 && std::string(argv[1]) == "scene") {
+        scene sc;
+        std::string err;
+        if (!load_scene(argv[2], sc, &err)) {
+            std::fprintf(stderr, "%s\n", err.c_str());
+            return 1;
+        }
+        sc.cam.image_width = std::atoi(argv[3]);
+        sc.cam.samples_per_pixel = std::atoi(argv[4]);
+        sc.cam.seed = std::strtoull(argv[5], nullptr, 10);
+        try {
+            frame f = sc.cam.render(sc.world, sc.lights);
+            std::printf("height %d\n", sc.cam.height());
+            return write_frame(argv[6], f.radiance, nullptr) ? 0 : 1;
+        } catch (const std::exception& e) {
+            std::fprintf(stderr, "%s\n", e.what());
+            return 1;
+        }
+    }
+    if (argc >= 3 && std::string(argv[1]) == "objects") {
+        hittable_list world;
+        auto white = std::make_shared<lambertian>(color(0.73, 0.73, 0.73));
+        auto checker = std::make_shared<checker_texture>(0.5, std::make_shared<solid_color>(color(0.2, 0.3, 0.1)),
+                                                         std::make_shared<solid_color>(color(0.9, 0.9, 0.9)));
+        world.add(std::make_shared<quad>(point3(-5, 0, -5), vec3(10, 0, 0), vec3(0, 0, 10),
+                                         std::make_shared<lambertian>(checker)));
+        world.add(std::make_shared<sphere>(point3(0, 1, 0), 1.0, std::make_shared<dielectric>(1.5)));
+        world.add(std::make_shared<sphere>(point3(-2.2, 1, 0), 1.0, std::make_shared<metal>(color(0.8, 0.6, 0.4), 0.2)));
+        auto b = box(point3(0, 0, 0), point3(1, 1.5, 1), white);
+        transform x = transform::translate(vec3(1.6, 0, -0.5)).then_after(transform::rotate(25, vec3(0, 1, 0)));
+        world.add(std::make_shared<instance>(b, x));
+        auto light = std::make_shared<quad>(point3(-1, 4, -1), vec3(2, 0, 0), vec3(0, 0, 2),
+                                            std::make_shared<diffuse_light>(color(6, 6, 6)));
+        world.add(light);
+        light_list lights;
+        lights.add(light);
+        lights.build();
+
+        camera cam;
+        cam.aspect_ratio = 4.0 / 3.0;
+        cam.image_width = 48;
+        cam.samples_per_pixel = 6;
+        cam.max_depth = 8;
+        cam.vfov = 40;
+        cam.lookfrom = point3(0, 2.5, 7);
+        cam.lookat = point3(0, 0.8, 0);
+        cam.use_sky = true;
+        cam.seed = 7;
+        int calls = 0, last_done = 0;
+        std::vector<color> last;
+        try {
+            frame f = cam.render(bvh(world), lights, 2, [&](int done, const frame& fr) {
+                calls++;
+                last_done = done;
+                last = fr.radiance;
+            });
+            std::printf("height %d calls %d last %d\n", cam.height(), calls, last_done);
+            return write_frame(argv[2], f.radiance, &last) ? 0 : 1;
+        } catch (const std::exception& e) {
+            std::fprintf(stderr, "%s\n", e.what());
+            return 1;
+        }
+    }
+    std::fprintf(stderr, "usage: api_render scene <file> <width> <spp> <seed> <out> | objects <out>\n");
+    return 2;
+}
