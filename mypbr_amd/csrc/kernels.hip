@@ -167,22 +167,55 @@ struct launch_ctx {
 
 // Shading classes: 0 = missed everything, 1 + t = a hit on material type t.
 constexpr int kClasses = 7;
+// Counters, each on its own 256-byte line so that atomics on different
+// counters do not serialise on one cache line: counter k is word
+// k * kCtrStride of the counter array. 0, 1: ping-pong queue lengths; 2:
+// shadow queue; 3 .. 3 + kClasses - 1: class queues. The 64-bit ray and
+// work statistics follow at word kCtrWords.
+constexpr int kCtrStride = 64;
+constexpr int kCtrWords = 16 * kCtrStride;
+__host__ __device__ __forceinline__ uint32_t* ctr(uint32_t* base, int k) { return base + k * kCtrStride; }
+
+// Block-aggregated append to NQ queues: bit k of `bits` asks for `value` to
+// be appended to queue k (counter ctr_of(k), entries q_of(k)). One atomic per
+// queue per call for the whole block; lane offsets come from per-wave ballot
+// popcounts and a prefix over the block's waves in LDS. Every thread of the
+// block must make the call (block-uniform control flow).
+template <int BS, int NQ, typename CtrOf, typename QOf>
+__device__ __forceinline__ void block_append(uint32_t* lds, uint32_t bits, uint32_t value, CtrOf ctr_of, QOf q_of) {
+    constexpr int W = BS / 64;
+    const int wid = int(threadIdx.x) >> 6;
+    const uint32_t lane = __lane_id();
+    uint64_t mask[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+        mask[k] = __ballot((bits >> k) & 1u);
+        if (lane == 0) lds[k * W + wid] = uint32_t(__popcll(mask[k]));
+    }
+    __syncthreads();
+    if (int(threadIdx.x) < NQ) {
+        const int k = int(threadIdx.x);
+        uint32_t run = 0;
+        for (int w = 0; w < W; w++) {
+            const uint32_t c = lds[k * W + w];
+            lds[k * W + w] = run;
+            run += c;
+        }
+        lds[NQ * W + k] = run ? atomicAdd(ctr_of(k), run) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+        if ((bits >> k) & 1u) {
+            const uint32_t below =
+                __builtin_amdgcn_mbcnt_hi(uint32_t(mask[k] >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask[k]), 0u));
+            q_of(k)[lds[NQ * W + k] + lds[k * W + wid] + below] = value;
+        }
+    }
+    __syncthreads();
+}
 // geometry profile of scenes made of spheres and quads, in lists and instances
 constexpr int kQuadProfile = F_SPHERE | F_QUAD | F_LIST | F_XFORM;
-
-// Wave-aggregated queue append: one atomic per wave, lane offsets from the
-// ballot's popcount prefix. Must be reached by every lane of the wave.
-__device__ __forceinline__ void wave_append(uint32_t* counter, uint32_t* queue, bool pred, uint32_t value) {
-    uint64_t mask = __ballot(pred);
-    if (mask == 0) return;
-    uint32_t lane = __lane_id();
-    uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
-    int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == uint32_t(leader)) base = atomicAdd(counter, uint32_t(__popcll(mask)));
-    base = __shfl(base, leader);
-    if (pred) queue[base + below] = value;
-}
 
 #define STACK_SETUP()                                                                   \
     __shared__ int32_t lds_stack[kLdsDepth * BS];                                       \
@@ -241,8 +274,8 @@ __device__ __forceinline__ v3 random_in_unit_disk(rstream& rs) {  // vec3.h:139-
 __global__ void k_camera(dcam C, pstate P, batch_args B, uint32_t* q0, uint32_t* cnt) {
     int64_t gt = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (gt == 0) {
-        cnt[0] = uint32_t(B.nslots);
-        for (int k = 1; k < 3 + kClasses; k++) cnt[k] = 0;
+        *ctr(cnt, 0) = uint32_t(B.nslots);
+        for (int k = 1; k < 3 + kClasses; k++) *ctr(cnt, k) = 0;
     }
     if (gt >= B.nslots) return;
     uint32_t slot = uint32_t(gt);
@@ -363,11 +396,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_SP
         atomicAdd(&ray_stats[0], (unsigned long long)n);
     }
     wcount wc{0, 0};
-    const int64_t wave = st.gtid / 64;
-    const int64_t nwaves = L.nthreads / 64;
-    const uint32_t lane = __lane_id();
-    for (int64_t base = wave * 64; base < n; base += nwaves * 64) {
-        const int64_t i = base + lane;
+    // blocks walk the queue in block-sized steps; the class queues are
+    // appended once per block and step (block_append)
+    __shared__ uint32_t lds_app[kClasses * (BS / 64 + 1)];
+    for (int64_t base = int64_t(blockIdx.x) * BS; base < n; base += int64_t(gridDim.x) * BS) {
+        const int64_t i = base + threadIdx.x;
         int cls = -1;
         uint32_t p = 0;
         if (i < n) {
@@ -389,10 +422,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_SP
                 cls = 1 + S.mat[rec.mat].type;
             }
         }
-#pragma unroll
-        for (int c = 0; c < kClasses; c++) {
-            wave_append(ccls + c, qcls + c * cap, cls == c, p);
-        }
+        block_append<BS, kClasses>(
+            lds_app, cls >= 0 ? 1u << cls : 0u, p, [&](int k) { return ccls + k * kCtrStride; },
+            [&](int k) { return qcls + k * cap; });
     }
     if constexpr (CNT) {
         if (wc.nodes) atomicAdd(&work[0], (unsigned long long)wc.nodes);
@@ -523,21 +555,21 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
 template <int BS, int MT, int LK, bool ENV>
 __global__ __launch_bounds__(BS) void k_shade(dscene S, dcam C, pstate P, int depth, const uint32_t* q,
                                               const uint32_t* cnt_in, uint32_t* qnext, uint32_t* cnt_next,
-                                              uint32_t* qshadow, uint32_t* cnt_shadow, int64_t nthreads) {
+                                              uint32_t* qshadow, uint32_t* cnt_shadow) {
+    constexpr int NQ = LK != 0 ? 2 : 1;
+    __shared__ uint32_t lds[NQ * (BS / 64 + 1)];
     const uint32_t n = *cnt_in;
-    const int64_t wave = (int64_t(blockIdx.x) * BS + threadIdx.x) / 64;
-    const int64_t nwaves = nthreads / 64;
-    const uint32_t lane = __lane_id();
-    for (int64_t base = wave * 64; base < n; base += nwaves * 64) {
-        int64_t i = base + lane;
+    for (int64_t base = int64_t(blockIdx.x) * BS; base < n; base += int64_t(gridDim.x) * BS) {
+        const int64_t i = base + threadIdx.x;
         bool alive = false, nee = false;
         uint32_t p = 0;
         if (i < n) {
             p = q[i];
             shade_one<MT, LK, ENV>(S, C, P, depth, p, alive, nee);
         }
-        wave_append(cnt_next, qnext, alive, p);
-        if constexpr (LK != 0) wave_append(cnt_shadow, qshadow, nee, p);
+        block_append<BS, NQ>(
+            lds, (alive ? 1u : 0u) | (nee ? 2u : 0u), p, [&](int k) { return k ? cnt_shadow : cnt_next; },
+            [&](int k) { return k ? qshadow : qnext; });
     }
 }
 
@@ -550,7 +582,7 @@ __global__ __launch_bounds__(BS) void k_shadow(dscene S, dcam C, launch_ctx L, p
     const uint32_t n = *cnt_shadow;
     if (st.gtid == 0) {
         atomicAdd(&ray_stats[1], (unsigned long long)n);
-        for (int c = 0; c < kClasses; c++) ccls[c] = 0;  // every shade kernel of this bounce has read them
+        for (int c = 0; c < kClasses; c++) ccls[c * kCtrStride] = 0;  // every shade kernel of this bounce has read them
     }
     wcount wc{0, 0};
     for (int64_t i = st.gtid; i < n; i += L.nthreads) {
@@ -591,7 +623,7 @@ __global__ __launch_bounds__(BS) void k_shadow(dscene S, dcam C, launch_ctx L, p
 
 // Zeroes n counters (stands in for the shadow stage when a scene has no lights).
 __global__ void k_zero(uint32_t* c, int n) {
-    if (int(threadIdx.x) < n) c[threadIdx.x] = 0;
+    if (int(threadIdx.x) < n) c[threadIdx.x * kCtrStride] = 0;
 }
 
 // ------------------------------------------------------------------ accumulate
@@ -999,7 +1031,7 @@ int ensure_paths(mpt_scene* s, int64_t cap) {
     if ((rc = alloc(size_t(cap) * sizeof(int32_t), reinterpret_cast<void**>(&s->ps.hprim)))) return rc;
     if ((rc = alloc(size_t(cap) * sizeof(uint32_t), reinterpret_cast<void**>(&s->ps.hmat)))) return rc;
     HIP_TRY(hipMalloc(&s->queues, size_t(cap) * (3 + kClasses) * sizeof(uint32_t)));
-    if (!s->counters) HIP_TRY(hipMalloc(&s->counters, 32 * sizeof(uint64_t)));
+    if (!s->counters) HIP_TRY(hipMalloc(&s->counters, kCtrWords * sizeof(uint32_t) + 16 * sizeof(uint64_t)));
     s->ps_capacity = cap;
     return MPT_OK;
 }
@@ -1091,15 +1123,15 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
     const int64_t cap = s->ps_capacity;
     uint32_t* q[3] = {s->queues, s->queues + cap, s->queues + 2 * cap};
     uint32_t* qcls = s->queues + 3 * cap;  // kClasses queues of cap entries
-    uint32_t* cnt = s->counters;           // [0], [1] ping-pong queue lengths, [2] shadow, [3..9] classes
-    uint32_t* ccls = cnt + 3;
+    uint32_t* cnt = s->counters;           // counters 0, 1 ping-pong queue lengths, 2 shadow, 3.. classes
+    uint32_t* ccls = ctr(cnt, 3);
     launch_ctx L{s->ovf, int64_t(s->grid) * s->block};
     const pstate P = s->ps;
     double* acc_rad = s->acc;
     double* acc_alb = acc_rad + 3 * size_t(npix);
     double* acc_nrm = acc_alb + 3 * size_t(npix);
     double* acc_l2 = acc_nrm + 3 * size_t(npix);
-    unsigned long long* ray_stats = reinterpret_cast<unsigned long long*>(s->counters) + 8;  // [0] ext, [1] shadow
+    unsigned long long* ray_stats = reinterpret_cast<unsigned long long*>(s->counters + kCtrWords);  // [0] ext, [1] shadow
     unsigned long long* work = ray_stats + 2;                                               // nodes/bytes ext, shadow
     HIP_TRY(hipMemsetAsync(ray_stats, 0, 6 * sizeof(unsigned long long), s->stream));
     const bool timing = (o.profile & 1) != 0;
@@ -1180,10 +1212,10 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
     do {                                                                                                             \
         if (counting)                                                                                                \
             hipLaunchKernelGGL((k_extend<256, true, FEAT>), dim3(G), dim3(256), 0, stream, DS, L, P, depth, qin, cin, \
-                               cout, cnt + 2, qcls, ccls, cap, ray_stats, work);                                     \
+                               cout, ctr(cnt, 2), qcls, ccls, cap, ray_stats, work);                                     \
         else                                                                                                         \
             hipLaunchKernelGGL((k_extend<256, false, FEAT>), dim3(G), dim3(256), 0, stream, DS, L, P, depth, qin,    \
-                               cin, cout, cnt + 2, qcls, ccls, cap, ray_stats, work);                                \
+                               cin, cout, ctr(cnt, 2), qcls, ccls, cap, ray_stats, work);                                \
     } while (0)
         if (s->profile == 1) EXT(F_SPHERE);
         else if (s->profile == 2) EXT(kQuadProfile);
@@ -1195,10 +1227,10 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
     do {                                                                                                            \
         if (counting)                                                                                               \
             hipLaunchKernelGGL((k_shadow<256, true, FEAT, ENV>), dim3(G), dim3(256), 0, stream, DS, C, L, P, depth,  \
-                               q[2], cnt + 2, ccls, ray_stats, work);                                               \
+                               q[2], ctr(cnt, 2), ccls, ray_stats, work);                                               \
         else                                                                                                        \
             hipLaunchKernelGGL((k_shadow<256, false, FEAT, ENV>), dim3(G), dim3(256), 0, stream, DS, C, L, P, depth, \
-                               q[2], cnt + 2, ccls, ray_stats, work);                                               \
+                               q[2], ctr(cnt, 2), ccls, ray_stats, work);                                               \
     } while (0)
 #define SHD(FEAT)                       \
     do {                                \
@@ -1213,10 +1245,10 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
     };
     auto launch_shade = [&](int cls, int depth, uint32_t* qout, uint32_t* cout) {
         const uint32_t* qc = qcls + cls * cap;
-        const uint32_t* cc = ccls + cls;
+        const uint32_t* cc = ccls + cls * kCtrStride;
 #define SHA2(MT, LK, ENV)                                                                                      \
     hipLaunchKernelGGL((k_shade<256, MT, LK, ENV>), dim3(G), dim3(256), 0, stream, DS, C, P, depth, qc, cc, qout,  \
-                       cout, q[2], cnt + 2, L.nthreads)
+                       cout, q[2], ctr(cnt, 2))
 #define SHA1(MT, LK)                                          \
     do {                                                      \
         if (MT < 0 && has_env) SHA2(MT, LK, (MT < 0));        \
@@ -1267,8 +1299,8 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
             }
             uint32_t* qin = q[depth & 1];
             uint32_t* qout = q[(depth + 1) & 1];
-            uint32_t* cin = cnt + (depth & 1);
-            uint32_t* cout = cnt + ((depth + 1) & 1);
+            uint32_t* cin = ctr(cnt, depth & 1);
+            uint32_t* cout = ctr(cnt, (depth + 1) & 1);
             // extend zeroes the next-queue and shadow counters of this bounce;
             // shadow zeroes the class counters once every shade kernel read them
             EV(0, launch_extend(depth, qin, cin, cout));
