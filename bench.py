@@ -125,6 +125,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--tail-paths", type=int, default=0,
+                    help="live paths at which a batch's tail runs as one launch (0 = the library's default, -1 = never)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 4 times (a plain copy kernel of known size, "
                          "the scale for rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
@@ -151,12 +153,12 @@ def main():
     total_passes = args.steps * pps
 
     for k in range(args.warmup):
-        s.render(cam, seed=1, first_pass=(k * pps) % 256, pass_count=pps, read=False)
+        s.render(cam, seed=1, first_pass=(k * pps) % 256, pass_count=pps, read=False, tail_paths=args.tail_paths)
     synchronize()
     t0 = time.perf_counter()
     ext = sh = 0
     for k in range(args.steps):
-        f = s.render(cam, seed=1, first_pass=(k * pps) % 256, pass_count=pps, accumulate=k > 0, read=False)
+        f = s.render(cam, seed=1, first_pass=(k * pps) % 256, pass_count=pps, accumulate=k > 0, read=False, tail_paths=args.tail_paths)
         ext += f.stats.extension_rays
         sh += f.stats.shadow_rays
         log(f"step {k + 1}/{args.steps}: {f.stats.seconds * 1e3:.1f} ms")
@@ -190,16 +192,21 @@ def main():
 
     if not args.no_profile:
         # one step again with per-stage event timing, then with BVH work counters
-        ft = s.render(cam, seed=1, first_pass=0, pass_count=pps, profile=1, read=False).stats
-        fc = s.render(cam, seed=1, first_pass=0, pass_count=pps, profile=2, read=False).stats
-        names = ["extend", "shade", "shadow", "camera+accumulate"]
-        stage = {names[i]: {"ms": ft.stage_ms[i], "launches": ft.stage_launches[i]} for i in range(4)}
-        ext_bytes = fc.extension_rays * EXT_RAY_BYTES + fc.node_bytes[0] + fc.prim_bytes[0]
-        sh_bytes = fc.shadow_rays * SHADOW_RAY_BYTES + fc.node_bytes[1] + fc.prim_bytes[1]
+        ft = s.render(cam, seed=1, first_pass=0, pass_count=pps, profile=1, read=False, tail_paths=args.tail_paths).stats
+        fc = s.render(cam, seed=1, first_pass=0, pass_count=pps, profile=2, read=False, tail_paths=args.tail_paths).stats
+        names = ["extend", "shade", "shadow", "camera+accumulate", "finish"]
+        stage = {names[i]: {"ms": ft.stage_ms[i], "launches": ft.stage_launches[i]} for i in range(5)}
+        # rays of the stage kernels only: the batch tails (k_finish) neither count their BVH work
+        # nor are timed as part of k_extend / k_shadow
+        ext_rays = fc.extension_rays - fc.tail_rays[0]
+        sh_rays = fc.shadow_rays - fc.tail_rays[1]
+        ext_bytes = ext_rays * EXT_RAY_BYTES + fc.node_bytes[0] + fc.prim_bytes[0]
+        sh_bytes = sh_rays * SHADOW_RAY_BYTES + fc.node_bytes[1] + fc.prim_bytes[1]
         stage["extend"]["algorithmic_bytes"] = ext_bytes
         stage["shadow"]["algorithmic_bytes"] = sh_bytes
-        stage["extend"]["node_bytes_per_ray"] = fc.node_bytes[0] / max(fc.extension_rays, 1)
-        stage["extend"]["prim_bytes_per_ray"] = fc.prim_bytes[0] / max(fc.extension_rays, 1)
+        stage["extend"]["rays"] = ext_rays
+        stage["extend"]["node_bytes_per_ray"] = fc.node_bytes[0] / max(ext_rays, 1)
+        stage["extend"]["prim_bytes_per_ray"] = fc.prim_bytes[0] / max(ext_rays, 1)
         line["stages"] = stage
         dom = max(("extend", "shadow"), key=lambda k: stage[k]["ms"])
         launches = stage[dom]["launches"]

@@ -185,6 +185,8 @@ typedef struct mpt_render_opts {
     int32_t pass_count;            /* passes to render in this call (0 = samples_per_pixel - first_pass) */
     int32_t accumulate;            /* 1: add to the previous call's sums (progressive render) */
     int32_t profile;               /* bit 0: time each stage with HIP events; bit 1: count BVH work */
+    int32_t tail_paths;            /* once at most this many paths of a batch are left, the rest of the
+                                    * batch runs in one launch, a path per thread (0 = auto, -1 = never) */
 } mpt_render_opts;
 
 /* Per-render statistics. */
@@ -194,14 +196,18 @@ typedef struct mpt_render_stats {
     int64_t shadow_rays;           /* shadow rays traced */
     double seconds;                /* wall time of the GPU part */
     /* with profile bit 0: summed kernel time and launches per stage,
-     * [0] extend, [1] shade, [2] shadow, [3] camera + features + accumulate */
-    double stage_ms[4];
-    int64_t stage_launches[4];
+     * [0] extend, [1] shade, [2] shadow, [3] camera + features + accumulate,
+     * [4] the batch tails run as one launch each */
+    double stage_ms[5];
+    int64_t stage_launches[5];
     /* with profile bit 1: BVH node bytes and primitive bytes read by the
      * extension [0] and shadow [1] traversals (a root box counts 64 bytes,
      * an interior child-pair record 128, a child re-tested on pop 64) */
     int64_t node_bytes[2];
     int64_t prim_bytes[2];
+    /* extension [0] and shadow [1] rays traced by the batch tails (included
+     * in extension_rays and shadow_rays; their BVH work is not counted) */
+    int64_t tail_rays[2];
 } mpt_render_stats;
 
 typedef struct mpt_scene mpt_scene;

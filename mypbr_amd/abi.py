@@ -102,13 +102,14 @@ class RenderOpts(C.Structure):
                 ("passes_in_flight", C.c_int32), ("block_size", C.c_int32),
                 ("sort_by_material", C.c_int32), ("use_graph", C.c_int32),
                 ("scene_index", C.c_int32), ("pass_count", C.c_int32), ("accumulate", C.c_int32),
-                ("profile", C.c_int32)]
+                ("profile", C.c_int32), ("tail_paths", C.c_int32)]
 
 
 class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_int64), ("extension_rays", C.c_int64), ("shadow_rays", C.c_int64),
-                ("seconds", C.c_double), ("stage_ms", C.c_double * 4), ("stage_launches", C.c_int64 * 4),
-                ("node_bytes", C.c_int64 * 2), ("prim_bytes", C.c_int64 * 2)]
+                ("seconds", C.c_double), ("stage_ms", C.c_double * 5), ("stage_launches", C.c_int64 * 5),
+                ("node_bytes", C.c_int64 * 2), ("prim_bytes", C.c_int64 * 2),
+                ("tail_rays", C.c_int64 * 2)]
 
 
 PASS_CALLBACK = C.CFUNCTYPE(None, C.c_int32, C.c_void_p)

@@ -383,6 +383,26 @@ __global__ __launch_bounds__(BS) void k_features(dscene S, launch_ctx L, pstate 
 #define MPT_SPHERE_WAVES 8
 #endif
 
+// Closest hit of path p's ray and the hit record shade reads (point, face
+// normal, material); returns the path's shading class.
+template <bool CNT, int FEAT>
+__device__ __forceinline__ int extend_one(const dscene& S, const pstate& P, int depth, uint32_t p, tstack& st,
+                                          wcount& wc) {
+    const v3 o = ld3(P, D_OX, p), d = ld3(P, D_DX, p);
+    hitinfo h;
+    trace_closest<CNT, FEAT>(S, o, d, 0.001, kInf, h, st, true, P.key[p], uint32_t(2 * depth), wc);
+    P.d[D_HT][p] = h.t;
+    P.hobj[p] = h.obj;
+    P.hprim[p] = h.prim;
+    if (h.obj < 0) return 0;
+    hitrec rec;
+    finish_hit(S, h, o, d, rec);
+    st3(P, D_QX, p, rec.p);
+    st3(P, D_MX, p, rec.normal);
+    P.hmat[p] = uint32_t(rec.mat) | (rec.front ? 0x80000000u : 0u);
+    return 1 + S.mat[rec.mat].type;
+}
+
 template <int BS, bool CNT, int FEAT>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_SPHERE ? MPT_SPHERE_WAVES : 1))) void k_extend(dscene S, launch_ctx L, pstate P, int depth, const uint32_t* q,
                                                const uint32_t* cnt_in, uint32_t* cnt_next, uint32_t* cnt_shadow,
@@ -405,22 +425,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_SP
         uint32_t p = 0;
         if (i < n) {
             p = q[i];
-            v3 o = ld3(P, D_OX, p), d = ld3(P, D_DX, p);
-            hitinfo h;
-            trace_closest<CNT, FEAT>(S, o, d, 0.001, kInf, h, st, true, P.key[p], uint32_t(2 * depth), wc);
-            P.d[D_HT][p] = h.t;
-            P.hobj[p] = h.obj;
-            P.hprim[p] = h.prim;
-            if (h.obj < 0) {
-                cls = 0;
-            } else {  // the hit record shade reads: point, face normal, material
-                hitrec rec;
-                finish_hit(S, h, o, d, rec);
-                st3(P, D_QX, p, rec.p);
-                st3(P, D_MX, p, rec.normal);
-                P.hmat[p] = uint32_t(rec.mat) | (rec.front ? 0x80000000u : 0u);
-                cls = 1 + S.mat[rec.mat].type;
-            }
+            cls = extend_one<CNT, FEAT>(S, P, depth, p, st, wc);
         }
         block_append<BS, kClasses>(
             lds_app, cls >= 0 ? 1u << cls : 0u, p, [&](int k) { return ccls + k * kCtrStride; },
@@ -552,8 +557,13 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
                  (uint32_t(vcount) << 22);
 }
 
+// Waves per SIMD the shade kernels are compiled for (1: no register cap).
+#ifndef MPT_SHADE_WAVES
+#define MPT_SHADE_WAVES 1
+#endif
+
 template <int BS, int MT, int LK, bool ENV>
-__global__ __launch_bounds__(BS) void k_shade(dscene S, dcam C, pstate P, int depth, const uint32_t* q,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MPT_SHADE_WAVES))) void k_shade(dscene S, dcam C, pstate P, int depth, const uint32_t* q,
                                               const uint32_t* cnt_in, uint32_t* qnext, uint32_t* cnt_next,
                                               uint32_t* qshadow, uint32_t* cnt_shadow) {
     constexpr int NQ = LK != 0 ? 2 : 1;
@@ -574,19 +584,14 @@ __global__ __launch_bounds__(BS) void k_shade(dscene S, dcam C, pstate P, int de
 }
 
 // ------------------------------------------------------------------ shadow
-template <int BS, bool CNT, int FEAT, bool ENV>
-__global__ __launch_bounds__(BS) void k_shadow(dscene S, dcam C, launch_ctx L, pstate P, int depth,
-                                               const uint32_t* qshadow, const uint32_t* cnt_shadow, uint32_t* ccls,
-                                               unsigned long long* ray_stats, unsigned long long* work) {
-    STACK_SETUP();
-    const uint32_t n = *cnt_shadow;
-    if (st.gtid == 0) {
-        atomicAdd(&ray_stats[1], (unsigned long long)n);
-        for (int c = 0; c < kClasses; c++) ccls[c * kCtrStride] = 0;  // every shade kernel of this bounce has read them
-    }
-    wcount wc{0, 0};
-    for (int64_t i = st.gtid; i < n; i += L.nthreads) {
-        uint32_t p = qshadow[i];
+// The shadow ray of path p's NEE sample (sample_light, camera.h:480-506):
+// closest hit along the normalized direction, through surfaces that let
+// shadows through when transparent shadows are on, emission found (or the
+// sky), and the MIS-weighted contribution added to the path's radiance.
+template <bool CNT, int FEAT, bool ENV>
+__device__ __forceinline__ void shadow_one(const dscene& S, const dcam& C, const pstate& P, int depth, uint32_t p,
+                                           tstack& st, wcount& wc) {
+    {
         v3 o = ld3(P, D_SOX, p);
         v3 to_light = ld3(P, D_TLX, p);
         v3 dir = unit_vector(to_light);
@@ -615,9 +620,80 @@ __global__ __launch_bounds__(BS) void k_shadow(dscene S, dcam C, launch_ctx L, p
         v3 rad = ld3(P, D_RX, p) + ld3(P, D_SBX, p) * c;
         st3(P, D_RX, p, rad);
     }
+}
+
+template <int BS, bool CNT, int FEAT, bool ENV>
+__global__ __launch_bounds__(BS) void k_shadow(dscene S, dcam C, launch_ctx L, pstate P, int depth,
+                                               const uint32_t* qshadow, const uint32_t* cnt_shadow, uint32_t* ccls,
+                                               unsigned long long* ray_stats, unsigned long long* work) {
+    STACK_SETUP();
+    const uint32_t n = *cnt_shadow;
+    if (st.gtid == 0) {
+        atomicAdd(&ray_stats[1], (unsigned long long)n);
+        for (int c = 0; c < kClasses; c++) ccls[c * kCtrStride] = 0;  // every shade kernel of this bounce has read them
+    }
+    wcount wc{0, 0};
+    for (int64_t i = st.gtid; i < n; i += L.nthreads) shadow_one<CNT, FEAT, ENV>(S, C, P, depth, qshadow[i], st, wc);
     if constexpr (CNT) {
         if (wc.nodes) atomicAdd(&work[2], (unsigned long long)wc.nodes);
         if (wc.pbytes) atomicAdd(&work[3], (unsigned long long)wc.pbytes);
+    }
+}
+
+// ------------------------------------------------------------------ finish
+// The rest of a batch in one launch, for its tail: once few paths are left,
+// per-bounce stage launches cost more than the work in them, so each
+// remaining path runs all its bounces in one thread. A bounce is what the
+// stage kernels do for the path, in the same order (extend, shade by
+// material, shadow ray), so the results are the same.
+template <int BS, int LK, bool ENV>
+__global__ __launch_bounds__(BS) void k_finish(dscene S, dcam C, launch_ctx L, pstate P, int depth0,
+                                               const uint32_t* q, const uint32_t* cnt_in,
+                                               unsigned long long* ray_stats) {
+    STACK_SETUP();
+    const uint32_t n = *cnt_in;
+    unsigned long long ext = 0, shd = 0;
+    wcount wc{0, 0};
+    for (int64_t i = st.gtid; i < n; i += L.nthreads) {
+        const uint32_t p = q[i];
+        for (int depth = depth0; depth < C.max_depth; depth++) {
+            const int cls = extend_one<false, F_ALL>(S, P, depth, p, st, wc);
+            ext++;
+            bool alive = false, nee = false;
+            switch (cls) {
+                case 0: shade_one<-1, LK, ENV>(S, C, P, depth, p, alive, nee); break;
+                case 1 + MPT_MAT_LAMBERTIAN: shade_one<MPT_MAT_LAMBERTIAN, LK, false>(S, C, P, depth, p, alive, nee); break;
+                case 1 + MPT_MAT_METAL: shade_one<MPT_MAT_METAL, LK, false>(S, C, P, depth, p, alive, nee); break;
+                case 1 + MPT_MAT_DIELECTRIC: shade_one<MPT_MAT_DIELECTRIC, LK, false>(S, C, P, depth, p, alive, nee); break;
+                case 1 + MPT_MAT_PRINCIPLED: shade_one<MPT_MAT_PRINCIPLED, LK, false>(S, C, P, depth, p, alive, nee); break;
+                case 1 + MPT_MAT_DIFFUSE_LIGHT:
+                    shade_one<MPT_MAT_DIFFUSE_LIGHT, LK, false>(S, C, P, depth, p, alive, nee);
+                    break;
+                default: shade_one<MPT_MAT_PHASE, LK, false>(S, C, P, depth, p, alive, nee); break;
+            }
+            if constexpr (LK != 0) {
+                if (nee) {
+                    shadow_one<false, F_ALL, ENV>(S, C, P, depth, p, st, wc);
+                    shd++;
+                }
+            }
+            if (!alive) break;
+        }
+    }
+    // one atomic per wave for the ray counts
+    for (int off = 32; off > 0; off >>= 1) {
+        ext += __shfl_xor(ext, off);
+        shd += __shfl_xor(shd, off);
+    }
+    if (__lane_id() == 0) {
+        if (ext) {
+            atomicAdd(&ray_stats[0], ext);
+            atomicAdd(&ray_stats[6], ext);
+        }
+        if (shd) {
+            atomicAdd(&ray_stats[1], shd);
+            atomicAdd(&ray_stats[7], shd);
+        }
     }
 }
 
@@ -1133,7 +1209,7 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
     double* acc_l2 = acc_nrm + 3 * size_t(npix);
     unsigned long long* ray_stats = reinterpret_cast<unsigned long long*>(s->counters + kCtrWords);  // [0] ext, [1] shadow
     unsigned long long* work = ray_stats + 2;                                               // nodes/bytes ext, shadow
-    HIP_TRY(hipMemsetAsync(ray_stats, 0, 6 * sizeof(unsigned long long), s->stream));
+    HIP_TRY(hipMemsetAsync(ray_stats, 0, 8 * sizeof(unsigned long long), s->stream));
     const bool timing = (o.profile & 1) != 0;
     const bool counting = (o.profile & 2) != 0;
 
@@ -1145,8 +1221,8 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
     };
     std::vector<timed> evs;
     size_t ev_used = 0;
-    double stage_ms[4] = {0, 0, 0, 0};
-    int64_t stage_n[4] = {0, 0, 0, 0};
+    double stage_ms[5] = {0, 0, 0, 0, 0};
+    int64_t stage_n[5] = {0, 0, 0, 0, 0};
     auto ev_begin = [&](int stage) -> int {
         if (!timing) return MPT_OK;
         if (ev_used == evs.size()) {
@@ -1277,6 +1353,22 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
 #undef SHA2
     };
 
+    // the tail of a batch runs as one k_finish launch once at most `tail`
+    // paths are left; by default two paths per thread of the grid
+    const int64_t tail = o.tail_paths < 0 ? 0 : (o.tail_paths > 0 ? o.tail_paths : 2 * L.nthreads);
+    auto launch_finish = [&](int depth, uint32_t* qin, uint32_t* cin) {
+#define FIN(LK)                                                                                                 \
+    do {                                                                                                        \
+        if (has_env) hipLaunchKernelGGL((k_finish<256, LK, true>), dim3(G), dim3(256), 0, stream, DS, C, L, P,  \
+                                        depth, qin, cin, ray_stats);                                             \
+        else hipLaunchKernelGGL((k_finish<256, LK, false>), dim3(G), dim3(256), 0, stream, DS, C, L, P, depth,  \
+                                qin, cin, ray_stats);                                                            \
+    } while (0)
+        if (s->has_lights) FIN(L_ALL);
+        else FIN(0);
+#undef FIN
+    };
+
     int pass = 0;
     while (pass < total) {
         int nb = std::min(pif, total - pass);
@@ -1290,14 +1382,19 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
                                  cnt));
         if (C.features) EV(3, hipLaunchKernelGGL((k_features<256>), dim3(G), dim3(256), 0, stream, DS, L, P, B));
         for (int depth = 0; depth < C.max_depth; depth++) {
-            if (depth >= 2) {  // no path left two bounces ago: the rest is empty
+            uint32_t* qin = q[depth & 1];
+            if (depth >= 2) {  // the queue length two bounces ago bounds this one's
                 if (hipEventSynchronize(ring[(depth - 2) % kRing]) != hipSuccess) {
                     ev_free();
                     return set_error(MPT_E_HIP, "hipEventSynchronize failed");
                 }
-                if (s->pinned_counts[(depth - 2) % kRing] == 0) break;
+                const uint32_t left = uint32_t(s->pinned_counts[(depth - 2) % kRing]);
+                if (left == 0) break;
+                if (tail > 0 && left <= uint32_t(tail)) {
+                    EV(4, launch_finish(depth, qin, ctr(cnt, depth & 1)));
+                    break;
+                }
             }
-            uint32_t* qin = q[depth & 1];
             uint32_t* qout = q[(depth + 1) & 1];
             uint32_t* cin = ctr(cnt, depth & 1);
             uint32_t* cout = ctr(cnt, (depth + 1) & 1);
@@ -1338,14 +1435,14 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
         if ((rc = mpt_read_frame(s, radiance, albedo, normal, variance))) return rc;
     }
     if (stats) {
-        unsigned long long rs_host[6] = {0, 0, 0, 0, 0, 0};
+        unsigned long long rs_host[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         HIP_TRY(hipMemcpy(rs_host, ray_stats, sizeof(rs_host), hipMemcpyDeviceToHost));
         std::memset(stats, 0, sizeof(*stats));
         stats->paths = npix64 * total;
         stats->extension_rays = int64_t(rs_host[0]);
         stats->shadow_rays = int64_t(rs_host[1]);
         stats->seconds = std::chrono::duration<double>(t1 - t0).count();
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < 5; k++) {
             stats->stage_ms[k] = stage_ms[k];
             stats->stage_launches[k] = stage_n[k];
         }
@@ -1353,6 +1450,8 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
         stats->prim_bytes[0] = int64_t(rs_host[3]);
         stats->node_bytes[1] = int64_t(rs_host[4]);
         stats->prim_bytes[1] = int64_t(rs_host[5]);
+        stats->tail_rays[0] = int64_t(rs_host[6]);
+        stats->tail_rays[1] = int64_t(rs_host[7]);
     }
     return MPT_OK;
 }

@@ -59,3 +59,27 @@ def test_render_features_match_oracle(mpt):
     _compare(f.albedo, cpu["albedo"], 0.99, 1e-3)
     _compare(f.normal, cpu["normal"], 0.99, 1e-2)
     assert np.all(f.variance >= 0)
+
+
+@pytest.mark.parametrize("name,width,spp,depth", [
+    ("cornell.scene", 48, 8, 16),
+    ("weekend_cover.scene", 64, 4, 50),
+    ("test_lights.scene", 48, 4, 12),
+    ("test_media.scene", 48, 4, 12),
+])
+def test_batch_tail_in_one_launch_is_identical(mpt, name, width, spp, depth):
+    """k_finish runs a path's remaining bounces in one thread; the frame must
+    be bit-identical to the per-bounce stage kernels' (tail never used), for
+    a tail taken as early as bounce 2 and for the default threshold."""
+    from mypbr_amd.abi import camera_copy
+    d = mpt.describe(scene_path(name))
+    cam = camera_copy(d.camera, image_width=width, samples_per_pixel=spp, max_depth=depth)
+    s = mpt.Scene.from_desc(d.desc, d.camera)
+    staged = s.render(cam, seed=11, tail_paths=-1)
+    assert staged.stats.tail_rays[0] == 0
+    for tail in (1 << 30, 0):
+        f = s.render(cam, seed=11, tail_paths=tail)
+        assert f.stats.tail_rays[0] > 0
+        assert np.array_equal(f.radiance, staged.radiance)
+        assert f.stats.extension_rays == staged.stats.extension_rays
+        assert f.stats.shadow_rays == staged.stats.shadow_rays
