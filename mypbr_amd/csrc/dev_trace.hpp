@@ -102,6 +102,8 @@ struct hitinfo {
 };
 
 // Slab test of aabb::hit (aabb.h:40-54) on a box given as (xmin, xmax, ymin, ymax, zmin, zmax).
+// The early return per axis is kept: most boxes are missed on the first
+// axis or two, and a branch the whole wave takes skips the rest.
 __device__ __forceinline__ bool box_hit(const double* b, const v3& o, const v3& inv, double tmin, double tmax) {
     const double oo[3] = {o.x, o.y, o.z};
     const double ii[3] = {inv.x, inv.y, inv.z};

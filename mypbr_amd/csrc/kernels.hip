@@ -646,8 +646,15 @@ __global__ __launch_bounds__(BS) void k_shadow(dscene S, dcam C, launch_ctx L, p
 // remaining path runs all its bounces in one thread. A bounce is what the
 // stage kernels do for the path, in the same order (extend, shade by
 // material, shadow ray), so the results are the same.
-template <int BS, int LK, bool ENV>
-__global__ __launch_bounds__(BS) void k_finish(dscene S, dcam C, launch_ctx L, pstate P, int depth0,
+// Waves per SIMD k_finish is compiled for: it holds every material's code,
+// and without a cap its register footprint allows one wave per SIMD.
+#ifndef MPT_FINISH_WAVES
+#define MPT_FINISH_WAVES 2
+#endif
+
+template <int BS, int FEAT, int LK, bool ENV>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MPT_FINISH_WAVES))) void k_finish(
+    dscene S, dcam C, launch_ctx L, pstate P, int depth0,
                                                const uint32_t* q, const uint32_t* cnt_in,
                                                unsigned long long* ray_stats) {
     STACK_SETUP();
@@ -657,7 +664,7 @@ __global__ __launch_bounds__(BS) void k_finish(dscene S, dcam C, launch_ctx L, p
     for (int64_t i = st.gtid; i < n; i += L.nthreads) {
         const uint32_t p = q[i];
         for (int depth = depth0; depth < C.max_depth; depth++) {
-            const int cls = extend_one<false, F_ALL>(S, P, depth, p, st, wc);
+            const int cls = extend_one<false, FEAT>(S, P, depth, p, st, wc);
             ext++;
             bool alive = false, nee = false;
             switch (cls) {
@@ -673,7 +680,7 @@ __global__ __launch_bounds__(BS) void k_finish(dscene S, dcam C, launch_ctx L, p
             }
             if constexpr (LK != 0) {
                 if (nee) {
-                    shadow_one<false, F_ALL, ENV>(S, C, P, depth, p, st, wc);
+                    shadow_one<false, FEAT, ENV>(S, C, P, depth, p, st, wc);
                     shd++;
                 }
             }
@@ -1357,16 +1364,22 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
     // paths are left; by default two paths per thread of the grid
     const int64_t tail = o.tail_paths < 0 ? 0 : (o.tail_paths > 0 ? o.tail_paths : 2 * L.nthreads);
     auto launch_finish = [&](int depth, uint32_t* qin, uint32_t* cin) {
-#define FIN(LK)                                                                                                 \
-    do {                                                                                                        \
-        if (has_env) hipLaunchKernelGGL((k_finish<256, LK, true>), dim3(G), dim3(256), 0, stream, DS, C, L, P,  \
-                                        depth, qin, cin, ray_stats);                                             \
-        else hipLaunchKernelGGL((k_finish<256, LK, false>), dim3(G), dim3(256), 0, stream, DS, C, L, P, depth,  \
-                                qin, cin, ray_stats);                                                            \
+#define FIN2(FEAT, LK)                                                                                        \
+    do {                                                                                                      \
+        if (has_env) hipLaunchKernelGGL((k_finish<256, FEAT, LK, true>), dim3(G), dim3(256), 0, stream, DS, C, \
+                                        L, P, depth, qin, cin, ray_stats);                                     \
+        else hipLaunchKernelGGL((k_finish<256, FEAT, LK, false>), dim3(G), dim3(256), 0, stream, DS, C, L, P, \
+                                depth, qin, cin, ray_stats);                                                   \
     } while (0)
-        if (s->has_lights) FIN(L_ALL);
-        else FIN(0);
+#define FIN(FEAT)                         \
+    do {                                  \
+        if (s->has_lights) FIN2(FEAT, L_ALL); \
+        else FIN2(FEAT, 0);               \
+    } while (0)
+        if (s->profile == 1) FIN(F_SPHERE);
+        else FIN(F_ALL);
 #undef FIN
+#undef FIN2
     };
 
     int pass = 0;
