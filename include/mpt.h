@@ -291,6 +291,35 @@ int32_t mpt_scene_light_probabilities(const mpt_scene* scene, double* probabilit
 int32_t mpt_bsdf_eval(mpt_scene* scene, const int32_t* mats, const double* n_rin_wi,
                       const uint8_t* front_face, int64_t n, double* eval_rgb, double* pdf);
 
+/* ---- classical denoiser: denoiser::run (denoiser.h:45-193) on the GPU ---- */
+
+/* The denoiser's settings (denoiser.h:40-44). */
+typedef struct mpt_denoise_opts {
+    int32_t iterations;            /* a-trous passes, taps 2^it pixels apart (5) */
+    int32_t pad;
+    double sigma_luminance;        /* in standard deviations of the noise (4) */
+    double sigma_normal;           /* (0.1) */
+    double sigma_albedo;           /* (0.1) */
+} mpt_denoise_opts;
+
+/* Fills the reference's defaults. */
+void mpt_denoise_defaults(mpt_denoise_opts* opts);
+
+/* denoiser::run(frame, width, height): filters the frame's radiance guided by
+ * its albedo, normal and variance (interleaved RGB doubles and one double per
+ * pixel, row-major from the top-left, caller-owned host memory) into out.
+ * opts NULL = the defaults. */
+int32_t mpt_denoise(const double* radiance, const double* albedo, const double* normal, const double* variance,
+                    int32_t width, int32_t height, const mpt_denoise_opts* opts, double* out);
+
+/* The same on device buffers, enqueued on a HIP stream (NULL = the null
+ * stream) without synchronising; workspace is device memory of
+ * mpt_denoise_workspace_bytes(width, height) bytes. */
+size_t mpt_denoise_workspace_bytes(int32_t width, int32_t height);
+int32_t mpt_denoise_dev(const double* radiance, const double* albedo, const double* normal, const double* variance,
+                        int32_t width, int32_t height, const mpt_denoise_opts* opts, double* out, void* workspace,
+                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif

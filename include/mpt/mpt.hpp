@@ -17,6 +17,7 @@
 //   light_list             light_list.h:33-134 mpt::light_list
 //   camera::render         camera.h:79-105     mpt::camera::render -> mpt_render (include/mpt.h)
 //   frame                  denoiser.h:14-20    mpt::frame
+//   denoiser               denoiser.h:38-193   mpt::denoiser -> mpt_denoise (include/mpt.h)
 //   scene / load_scene     scene.h:32-244      mpt::scene / mpt::load_scene
 //
 // Objects are plain descriptions; camera::render flattens world and lights
@@ -320,6 +321,19 @@ public:
 
 private:
     int image_height = 1;
+};
+
+// denoiser (denoiser.h:38-193): the edge-avoiding a-trous filter, run on the
+// GPU (mpt_denoise). run() needs a frame rendered with collect_features.
+class denoiser {
+public:
+    int iterations = 5;
+    double sigma_luminance = 4;
+    double sigma_normal = 0.1;
+    double sigma_albedo = 0.1;
+    int threads = 0;  // kept for source compatibility; the GPU ignores it
+
+    std::vector<color> run(const frame& f, int width, int height) const;
 };
 
 // A flattened scene and the GPU scene made from it.

@@ -23,7 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # MPT_LIBRARY selects another build of the same library (tuning experiments)
 LIB_PATH = os.environ.get("MPT_LIBRARY", os.path.join(HERE, "libmpt.so"))
 
-__all__ = ["Scene", "Frame", "MptError", "lib", "abi", "Camera", "RenderOpts", "device_info",
+__all__ = ["Scene", "Frame", "MptError", "lib", "abi", "Camera", "RenderOpts", "device_info", "denoise",
            "ggx_albedo_table", "describe"]
 
 
@@ -61,6 +61,11 @@ def _load():
         "mpt_scene_light_probabilities": (C.c_int32, [vp, P(C.c_double)]),
         "mpt_bsdf_eval": (C.c_int32, [vp, P(C.c_int32), P(C.c_double), P(C.c_uint8), C.c_int64, P(C.c_double),
                                       P(C.c_double)]),
+        "mpt_denoise_defaults": (None, [P(abi.DenoiseOpts)]),
+        "mpt_denoise": (C.c_int32, [P(C.c_double)] * 4 + [C.c_int32, C.c_int32, P(abi.DenoiseOpts),
+                                                           P(C.c_double)]),
+        "mpt_denoise_workspace_bytes": (C.c_size_t, [C.c_int32, C.c_int32]),
+        "mpt_denoise_dev": (C.c_int32, [vp] * 4 + [C.c_int32, C.c_int32, P(abi.DenoiseOpts), vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -94,6 +99,25 @@ def ggx_albedo_table():
     ea = np.zeros(32)
     _check(lib.mpt_ggx_albedo_table(_dp(e), _dp(ea)))
     return e.reshape(32, 32), ea
+
+
+def denoise(frame, iterations=5, sigma_luminance=4.0, sigma_normal=0.1, sigma_albedo=0.1):
+    """denoiser::run (denoiser.h:45-78) on the GPU. `frame` is a Frame
+    rendered with features, or a dict with radiance, albedo, normal (h, w, 3)
+    and variance (h, w) float64 arrays. Returns the filtered radiance."""
+    get = (lambda k: getattr(frame, k)) if isinstance(frame, Frame) else (lambda k: frame[k])
+    raw = [get(k) for k in ("radiance", "albedo", "normal", "variance")]
+    if any(a is None for a in raw):
+        raise ValueError("denoise needs the frame's albedo, normal and variance (render with features)")
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in raw]
+    h, w = arrs[0].shape[:2]
+    if arrs[0].shape != (h, w, 3) or arrs[3].shape != (h, w) or any(a.shape != (h, w, 3) for a in arrs[1:3]):
+        raise ValueError("denoise needs radiance, albedo, normal (h, w, 3) and variance (h, w)")
+    o = abi.DenoiseOpts(iterations=iterations, pad=0, sigma_luminance=sigma_luminance, sigma_normal=sigma_normal,
+                        sigma_albedo=sigma_albedo)
+    out = np.zeros((h, w, 3))
+    _check(lib.mpt_denoise(*[_dp(a) for a in arrs], w, h, C.byref(o), _dp(out)))
+    return out
 
 
 class Description:

@@ -122,3 +122,9 @@ def camera_copy(cam, **overrides):
     for k, v in overrides.items():
         setattr(c, k, v)
     return c
+
+
+class DenoiseOpts(C.Structure):
+    """mpt_denoise_opts (denoiser.h:40-44)."""
+    _fields_ = [("iterations", C.c_int32), ("pad", C.c_int32), ("sigma_luminance", C.c_double),
+                ("sigma_normal", C.c_double), ("sigma_albedo", C.c_double)]

@@ -6,6 +6,7 @@ Geometry is IEEE double with FP contraction off everywhere, so that device
 and host arithmetic match the reference renderer's operation order bit for
 bit; packed-FP32 instructions are disabled for the device.
 """
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -26,7 +27,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
           "-munsafe-fp-atomics"] + os.environ.get("MPT_DEVICE_FLAGS", "").split()
 
 HOST_SOURCES = ["scene_host.cpp", "scene_api.cpp"]
-DEVICE_SOURCES = ["kernels.hip"]
+DEVICE_SOURCES = ["kernels.hip", "denoise.hip"]
 
 
 def run(cmd):
@@ -37,15 +38,19 @@ def run(cmd):
 def build(verbose=True):
     build_dir = os.environ.get("MPT_BUILD_DIR", BUILD)
     os.makedirs(build_dir, exist_ok=True)
-    objs = []
+    jobs = []
     for src in HOST_SOURCES:
         obj = os.path.join(build_dir, src + ".o")
-        run([HIPCC, "-x", "c++", *COMMON, "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+        jobs.append((obj, [HIPCC, "-x", "c++", *COMMON, "-c", os.path.join(CSRC, src), "-o", obj]))
     for src in DEVICE_SOURCES:
         obj = os.path.join(build_dir, src + ".o")
-        run([HIPCC, "-x", "hip", *COMMON, *DEVICE, "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+        jobs.append((obj, [HIPCC, "-x", "hip", *COMMON, *DEVICE, "-c", os.path.join(CSRC, src), "-o", obj]))
+    # the translation units compile independently; the largest (the stage
+    # kernels) sets the wall time
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 1)) as pool:
+        for f in [pool.submit(run, cmd) for _, cmd in jobs]:
+            f.result()
+    objs = [obj for obj, _ in jobs]
     tmp = LIB + ".tmp"
     run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, *objs, "-o", tmp, "-lpthread"])
     os.replace(tmp, LIB)

@@ -608,6 +608,31 @@ frame camera::render(const hittable& world, const hittable_list& lights, int sav
     return to_frame(r, a, n, v, npix, collect_features);
 }
 
+std::vector<color> denoiser::run(const frame& f, int width, int height) const {
+    const size_t n = size_t(width > 0 ? width : 0) * size_t(height > 0 ? height : 0);
+    if (f.radiance.size() != n || f.albedo.size() != n || f.normal.size() != n || f.variance.size() != n)
+        throw std::runtime_error("denoiser::run: the frame needs radiance, albedo, normal and variance for every pixel"
+                                 " (render with collect_features)");
+    std::vector<double> r(3 * n), a(3 * n), nn(3 * n), out(3 * n);
+    for (size_t k = 0; k < n; k++)
+        for (int c = 0; c < 3; c++) {
+            r[3 * k + c] = f.radiance[k][c];
+            a[3 * k + c] = f.albedo[k][c];
+            nn[3 * k + c] = f.normal[k][c];
+        }
+    mpt_denoise_opts o;
+    o.iterations = iterations;
+    o.pad = 0;
+    o.sigma_luminance = sigma_luminance;
+    o.sigma_normal = sigma_normal;
+    o.sigma_albedo = sigma_albedo;
+    if (mpt_denoise(r.data(), a.data(), nn.data(), f.variance.data(), width, height, &o, out.data()) != MPT_OK)
+        throw std::runtime_error(std::string("mpt_denoise: ") + mpt_last_error());
+    std::vector<color> res(n);
+    for (size_t k = 0; k < n; k++) res[k] = color(out[3 * k], out[3 * k + 1], out[3 * k + 2]);
+    return res;
+}
+
 // ------------------------------------------------------------------ .scene loader
 namespace {
 

@@ -1629,6 +1629,115 @@ void first_surface(const scene& S, const ray& cr, uint64_t key, color& albedo, c
     normal = color(0, 0, 0);
 }
 
+// ------------------------------------------------------------------ denoiser (denoiser.h:38-193)
+// Edge-avoiding a-trous filter on lighting = radiance / albedo, with SVGF's
+// variance-scaled luminance weight; a restatement of denoiser::run.
+struct atrous {
+    int iterations = 5;
+    double sigma_luminance = 4, sigma_normal = 0.1, sigma_albedo = 0.1;
+    static constexpr double eps = 1e-3;  // denoiser.h:95
+
+    std::vector<color> run(const std::vector<color>& rad, const std::vector<color>& alb, const std::vector<color>& nrm,
+                           const std::vector<double>& variance, int w, int h) const {
+        const size_t n = size_t(w) * h;
+        const color e(eps, eps, eps);
+        std::vector<color> light(n);
+        std::vector<double> var(n);
+        for (size_t k = 0; k < n; k++) {  // denoiser.h:48-55
+            color a = alb[k] + e;
+            light[k] = color(rad[k].x() / a.x(), rad[k].y() / a.y(), rad[k].z() / a.z());
+            double la = luminance(a);
+            var[k] = variance[k] / (la * la);
+        }
+        clamp_fireflies(light, var, w, h);
+        std::vector<color> light2(n);
+        std::vector<double> var2(n);
+        const double inv_n = 1 / (sigma_normal * sigma_normal);
+        const double inv_a = 1 / (sigma_albedo * sigma_albedo);
+        for (int it = 0; it < iterations; it++) {
+            std::vector<double> bvar = blur(var, w, h);
+            for (int j = 0; j < h; j++)
+                for (int i = 0; i < w; i++) pass(i, j, 1 << it, w, h, alb, nrm, light, var, bvar, inv_n, inv_a, light2, var2);
+            light.swap(light2);
+            var.swap(var2);
+        }
+        std::vector<color> out(n);
+        for (size_t k = 0; k < n; k++) out[k] = light[k] * (alb[k] + e);
+        return out;
+    }
+
+    // denoiser.h:98-124: a pixel brighter than its 8 neighbours whose
+    // deviation exceeds half its luminance drops to the brightest neighbour.
+    static void clamp_fireflies(std::vector<color>& light, std::vector<double>& var, int w, int h) {
+        const std::vector<color> src = light;
+        for (int j = 1; j < h - 1; j++)
+            for (int i = 1; i < w - 1; i++) {
+                const size_t p = size_t(j) * w + i;
+                const double lum = luminance(src[p]);
+                if (lum <= 0 || std::sqrt(var[p]) < 0.5 * lum) continue;
+                double top = 0;
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dx = -1; dx <= 1; dx++)
+                        if (dx || dy) top = std::fmax(top, luminance(src[p + dy * w + dx]));
+                if (lum > top) {
+                    const double f = top / lum;
+                    light[p] = f * src[p];
+                    var[p] *= f * f;
+                }
+            }
+    }
+
+    // denoiser.h:169-188: normalised 3x3 binomial blur, clipped at the borders.
+    static std::vector<double> blur(const std::vector<double>& v, int w, int h) {
+        const double k3[3] = {0.25, 0.5, 0.25};
+        std::vector<double> out(v.size());
+        for (int j = 0; j < h; j++)
+            for (int i = 0; i < w; i++) {
+                double acc = 0, wsum = 0;
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dx = -1; dx <= 1; dx++) {
+                        const int x = i + dx, y = j + dy;
+                        if (x < 0 || x >= w || y < 0 || y >= h) continue;
+                        const double wt = k3[dx + 1] * k3[dy + 1];
+                        acc += wt * v[size_t(y) * w + x];
+                        wsum += wt;
+                    }
+                out[size_t(j) * w + i] = acc / wsum;
+            }
+        return out;
+    }
+
+    // denoiser.h:126-167: one 5x5 tap pass at spacing `step`.
+    void pass(int i, int j, int step, int w, int h, const std::vector<color>& alb, const std::vector<color>& nrm,
+              const std::vector<color>& light, const std::vector<double>& var, const std::vector<double>& bvar,
+              double inv_n, double inv_a, std::vector<color>& out, std::vector<double>& out_var) const {
+        const double k5[5] = {1.0 / 16, 1.0 / 4, 3.0 / 8, 1.0 / 4, 1.0 / 16};
+        const size_t p = size_t(j) * w + i;
+        const double lp = luminance(light[p]);
+        const double scale = sigma_luminance * std::sqrt(std::fmax(bvar[p], 0.0)) + 1e-10;
+        color acc(0, 0, 0);
+        double wsum = 0, vsum = 0;
+        for (int dy = -2; dy <= 2; dy++) {
+            const int y = j + dy * step;
+            if (y < 0 || y >= h) continue;
+            for (int dx = -2; dx <= 2; dx++) {
+                const int x = i + dx * step;
+                if (x < 0 || x >= w) continue;
+                const size_t q = size_t(y) * w + x;
+                const double wn = std::exp(-(nrm[p] - nrm[q]).length_squared() * inv_n);
+                const double wa = std::exp(-(alb[p] - alb[q]).length_squared() * inv_a);
+                const double wl = std::exp(-std::fabs(lp - luminance(light[q])) / scale);
+                const double wt = k5[dx + 2] * k5[dy + 2] * wn * wa * wl;
+                acc = acc + wt * light[q];
+                wsum += wt;
+                vsum += wt * wt * var[q];
+            }
+        }
+        out[p] = acc / wsum;
+        out_var[p] = vsum / (wsum * wsum);
+    }
+};
+
 }  // namespace orc
 
 // ======================================================================== C ABI
@@ -1643,6 +1752,28 @@ static thread_local std::string orc_err;
 extern "C" {
 
 const char* orc_last_error(void) { return orc_err.c_str(); }
+
+// denoiser::run on interleaved RGB doubles (denoiser.h:45-78).
+int32_t orc_denoise(const double* rad, const double* alb, const double* nrm, const double* var, int32_t w, int32_t h,
+                    int32_t iterations, double sigma_l, double sigma_n, double sigma_a, double* out) {
+    const size_t n = size_t(w) * h;
+    std::vector<color> r(n), a(n), nn(n);
+    std::vector<double> v(var, var + n);
+    for (size_t k = 0; k < n; k++) {
+        r[k] = color(rad[3 * k], rad[3 * k + 1], rad[3 * k + 2]);
+        a[k] = color(alb[3 * k], alb[3 * k + 1], alb[3 * k + 2]);
+        nn[k] = color(nrm[3 * k], nrm[3 * k + 1], nrm[3 * k + 2]);
+    }
+    atrous f;
+    f.iterations = iterations;
+    f.sigma_luminance = sigma_l;
+    f.sigma_normal = sigma_n;
+    f.sigma_albedo = sigma_a;
+    std::vector<color> o = f.run(r, a, nn, v, w, h);
+    for (size_t k = 0; k < n; k++)
+        for (int c = 0; c < 3; c++) out[3 * k + c] = o[k][c];
+    return 0;
+}
 
 int32_t orc_scene_create(const mpt_scene_desc* d, orc_scene** out) {
     auto p = std::make_unique<orc_scene>();

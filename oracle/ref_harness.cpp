@@ -197,4 +197,27 @@ int ref_render(void* h, int width, int spp, int max_depth, int threads, double* 
     return 0;
 }
 
+// denoiser::run (denoiser.h:45-78) on interleaved RGB doubles.
+int ref_denoise(const double* rad, const double* alb, const double* nrm, const double* var, int w, int h,
+                int iterations, double sigma_l, double sigma_n, double sigma_a, double* out) {
+    const size_t n = size_t(w) * h;
+    frame f;
+    for (size_t k = 0; k < n; k++) {
+        f.radiance.push_back(color(rad[3 * k], rad[3 * k + 1], rad[3 * k + 2]));
+        f.albedo.push_back(color(alb[3 * k], alb[3 * k + 1], alb[3 * k + 2]));
+        f.normal.push_back(color(nrm[3 * k], nrm[3 * k + 1], nrm[3 * k + 2]));
+        f.variance.push_back(var[k]);
+    }
+    denoiser d;
+    d.iterations = iterations;
+    d.sigma_luminance = sigma_l;
+    d.sigma_normal = sigma_n;
+    d.sigma_albedo = sigma_a;
+    d.threads = 1;
+    std::vector<color> o = d.run(f, w, h);
+    for (size_t k = 0; k < n; k++)
+        for (int c = 0; c < 3; c++) out[3 * k + c] = o[k][c];
+    return 0;
+}
+
 }  // extern "C"

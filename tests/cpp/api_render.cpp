@@ -11,6 +11,10 @@
 //       rotate/translate instance, a quad light in a light_list, a checker
 //       floor) with save_every 2; prints the callback counts and writes the
 //       radiance of the returned frame followed by the last callback frame
+//   api_render denoise <file.scene> <out.bin>
+//       renders the file at width 40, 4 spp with collect_features and runs
+//       denoiser::run on the frame; writes radiance, albedo, normal, the
+//       denoised radiance (h*w*3 doubles each) and the variance (h*w)
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -92,6 +96,33 @@ int main(int argc, char** argv) {
             return 1;
         }
     }
-    std::fprintf(stderr, "usage: api_render scene <file> <width> <spp> <seed> <out> | objects <out>\n");
+    if (argc >= 4 && std::string(argv[1]) == "denoise") {
+        scene sc;
+        std::string err;
+        if (!load_scene(argv[2], sc, &err)) {
+            std::fprintf(stderr, "%s\n", err.c_str());
+            return 1;
+        }
+        sc.cam.image_width = 40;
+        sc.cam.samples_per_pixel = 4;
+        sc.cam.collect_features = true;
+        sc.cam.seed = 3;
+        try {
+            frame f = sc.cam.render(sc.world, sc.lights);
+            denoiser filter;
+            std::vector<color> out = filter.run(f, sc.cam.image_width, sc.cam.height());
+            std::printf("height %d\n", sc.cam.height());
+            FILE* fp = std::fopen(argv[3], "wb");
+            if (!fp) return 1;
+            for (const auto* v : {&f.radiance, &f.albedo, &f.normal, &out})
+                for (const auto& c : *v) std::fwrite(c.v, sizeof(double), 3, fp);
+            std::fwrite(f.variance.data(), sizeof(double), f.variance.size(), fp);
+            return std::fclose(fp) == 0 ? 0 : 1;
+        } catch (const std::exception& e) {
+            std::fprintf(stderr, "%s\n", e.what());
+            return 1;
+        }
+    }
+    std::fprintf(stderr, "usage: api_render scene <file> <width> <spp> <seed> <out> | objects <out> | denoise <file> <out>\n");
     return 2;
 }

@@ -15,8 +15,11 @@ queries) are seeded and regenerated identically by the tests:
                         directions (material.h, microfacet.h, pdf.h, medium.h)
   render_<scene>.npz    converged low-resolution means of camera::render and
                         their per-pixel standard error over 8 runs
+  denoise.npz           denoiser::run (denoiser.h:45-193) on the synthetic frames
+                        of tests/denoise_frames.py, for two settings
 
-Usage: python tests/golden/make_golden.py  (takes about a minute)
+Usage: python tests/golden/make_golden.py [step ...]  (steps: ggx trace bsdf render denoise; all by default)
+(all steps take about a minute)
 """
 import hashlib
 import json
@@ -90,10 +93,29 @@ def bsdf_queries(seed=5):
     return spec, np.ascontiguousarray(q), front
 
 
+def golden_denoise():
+    import denoise_frames as dfr
+    arrays = {}
+    for name, make in dfr.CASES.items():
+        f = make()
+        for k in ("radiance", "albedo", "normal", "variance"):
+            arrays[f"{name}/{k}"] = f[k]
+        for si, st in enumerate(dfr.SETTINGS):
+            arrays[f"{name}/out{si}"] = ol.reference_denoise(f, st)
+    arrays["settings"] = np.array(dfr.SETTINGS)
+    np.savez_compressed(os.path.join(HERE, "denoise.npz"), **arrays)
+    print("denoise.npz", len(dfr.CASES), "frames", flush=True)
+
+
 def main():
     if not ol.reference_available():
         sys.exit("oracle/_ref/libmpt_ref.so is not built (make -C oracle ref, where the reference is present)")
     R = ol.reference()
+    steps = set(sys.argv[1:]) or {"ggx", "trace", "bsdf", "render", "denoise"}
+    if "denoise" in steps:
+        golden_denoise()
+    if steps == {"denoise"}:
+        return
 
     e = np.zeros(1024)
     ea = np.zeros(32)

@@ -54,6 +54,8 @@ def oracle():
             "orc_render": (C.c_int32, [vp, P(Camera), C.c_uint64, C.c_uint32, C.c_int32, C.c_int32, C.c_int32,
                                        dp, dp, dp, dp, P(C.c_int64), P(C.c_int64), dp]),
             "orc_make_ray_batch": (C.c_int32, [vp, P(Camera), C.c_uint64, C.c_int64, dp, dp]),
+            "orc_denoise": (C.c_int32, [dp, dp, dp, dp, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                        C.c_double, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(lib, name)
@@ -169,6 +171,8 @@ def reference():
             "ref_ggx_table": (C.c_int, [dp, dp]),
             "ref_bsdf_eval": (C.c_int, [dp, dp, P(C.c_uint8), C.c_int64, dp, dp]),
             "ref_render": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp]),
+            "ref_denoise": (C.c_int, [dp, dp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(lib, name)
@@ -214,3 +218,26 @@ class RefScene:
     def render(self, width, spp, max_depth, threads=8):
         # height from the scene's own aspect ratio, as camera::initialize
         return None
+
+
+def _denoise(fn, frame, settings):
+    rad = np.ascontiguousarray(frame["radiance"], dtype=np.float64)
+    h, w = rad.shape[:2]
+    out = np.zeros((h, w, 3))
+    it, sl, sn, sa = settings
+    rc = fn(_d(rad), _d(np.ascontiguousarray(frame["albedo"], dtype=np.float64)),
+            _d(np.ascontiguousarray(frame["normal"], dtype=np.float64)),
+            _d(np.ascontiguousarray(frame["variance"], dtype=np.float64)), w, h, it, sl, sn, sa, _d(out))
+    if rc != 0:
+        raise RuntimeError("denoise failed")
+    return out
+
+
+def oracle_denoise(frame, settings=(5, 4.0, 0.1, 0.1)):
+    """The oracle's restatement of denoiser::run (denoiser.h:45-78)."""
+    return _denoise(oracle().orc_denoise, frame, settings)
+
+
+def reference_denoise(frame, settings=(5, 4.0, 0.1, 0.1)):
+    """The reference's own denoiser::run, through the compiled harness."""
+    return _denoise(reference().ref_denoise, frame, settings)

@@ -73,3 +73,19 @@ def test_cpp_objects_render_and_pass_callbacks(tmp_path):
     assert not np.array_equal(final, partial)
     r2 = subprocess.run([exe, "objects", out + "2"], capture_output=True, text=True, timeout=120)
     assert r2.returncode == 0 and np.array_equal(np.fromfile(out + "2", dtype=np.float64), data.ravel())
+
+
+@pytest.mark.gpu
+def test_cpp_denoiser_equals_c_abi_denoise(tmp_path):
+    import mypbr_amd
+    exe = _build(tmp_path)
+    out = str(tmp_path / "dn.bin")
+    r = subprocess.run([exe, "denoise", scene_path("cornell.scene"), out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    h = int(r.stdout.split()[1])
+    data = np.fromfile(out, dtype=np.float64)
+    n = h * 40
+    rad, alb, nrm, dn = (data[k * 3 * n:(k + 1) * 3 * n].reshape(h, 40, 3) for k in range(4))
+    var = data[12 * n:].reshape(h, 40)
+    want = mypbr_amd.denoise({"radiance": rad, "albedo": alb, "normal": nrm, "variance": var})
+    assert np.array_equal(dn, want)
