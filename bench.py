@@ -45,12 +45,8 @@ def log(*a):
 
 
 def synchronize():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+    import mypbr_amd as m
+    m.synchronize()
 
 
 def cpu_baseline(scene_path, cam, seconds, threads):

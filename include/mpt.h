@@ -223,6 +223,15 @@ int32_t mpt_abi_version(void);
 /* Device information for logs: name, CU count and HBM bytes. */
 int32_t mpt_device_info(char* name, int32_t name_len, int32_t* cu_count, int64_t* hbm_bytes);
 
+/* Device memory for callers that hand the _dev entry points their own
+ * buffers (no reference counterpart: the reference is CPU-only). alloc
+ * returns NULL on failure (mpt_last_error says why). */
+void* mpt_device_alloc(size_t bytes);
+void mpt_device_free(void* ptr);
+int32_t mpt_copy_to_device(void* dst, const void* src, size_t bytes);
+int32_t mpt_copy_to_host(void* dst, const void* src, size_t bytes);
+int32_t mpt_device_synchronize(void);
+
 /* Copies the description to HBM and builds BVHs, light CDFs and tables.
  * Replaces: bvh(list) construction (bvh.h:22-43, main.cpp:70), mesh's inner
  * BVH (mesh.h:177-179), light_list::build (light_list.h:45-96) and the
@@ -291,6 +300,12 @@ int32_t mpt_scene_light_probabilities(const mpt_scene* scene, double* probabilit
 int32_t mpt_bsdf_eval(mpt_scene* scene, const int32_t* mats, const double* n_rin_wi,
                       const uint8_t* front_face, int64_t n, double* eval_rgb, double* pdf);
 
+/* Depth of the first surface seen through every pixel centre (no jitter or
+ * defocus): t * |d| of the closest hit, 0 for the sky; h*w doubles. A guide
+ * for the learned denoiser; the reference's frame has no depth
+ * (denoiser.h:14-20). */
+int32_t mpt_primary_depth(mpt_scene* scene, const mpt_camera* cam, double* depth);
+
 /* ---- classical denoiser: denoiser::run (denoiser.h:45-193) on the GPU ---- */
 
 /* The denoiser's settings (denoiser.h:40-44). */
@@ -319,6 +334,42 @@ size_t mpt_denoise_workspace_bytes(int32_t width, int32_t height);
 int32_t mpt_denoise_dev(const double* radiance, const double* albedo, const double* normal, const double* variance,
                         int32_t width, int32_t height, const mpt_denoise_opts* opts, double* out, void* workspace,
                         void* stream);
+
+/* ---- learned denoiser: kernel-predicting CNN on BF16 matrix cores ----
+ * No counterpart in the reference (it has only the classical filter above);
+ * the learned mode the path tracer is compared against. */
+
+/* C = A B^T with BF16 operands and FP32 accumulation on the matrix cores:
+ * C[m][n] = (accumulate ? C[m][n] : 0) + sum_k A(m, k) B(n, k), where
+ * A(m, k) = A[m * lda + k] (a_t = 0) or A[k * lda + m] (a_t = 1), and B
+ * likewise. Device pointers; enqueued on `stream` (NULL = null stream). */
+int32_t mpt_gemm_bf16(int32_t a_t, int32_t b_t, int32_t M, int32_t N, int32_t K, const uint16_t* A, int64_t lda,
+                      const uint16_t* B, int64_t ldb, float* C, int64_t ldc, int32_t accumulate, void* stream);
+
+typedef struct mpt_kpcnn mpt_kpcnn;
+
+/* A network of `layers` 3x3 convolutions with `hidden` channels (a multiple
+ * of 16), He-initialised from `seed`. Parameters are FP32: per layer the
+ * weights [cout][9][cin] (tap = 3 * (dy + 1) + dx + 1) then the biases
+ * [cout]; mpt_kpcnn_layout gives each layer's sizes and offsets. */
+int32_t mpt_kpcnn_create(int32_t layers, int32_t hidden, uint64_t seed, mpt_kpcnn** out);
+void mpt_kpcnn_destroy(mpt_kpcnn* net);
+int64_t mpt_kpcnn_param_count(const mpt_kpcnn* net);
+int32_t mpt_kpcnn_layout(const mpt_kpcnn* net, int32_t* cin, int32_t* cout, int64_t* weight_offset,
+                         int64_t* bias_offset);
+int32_t mpt_kpcnn_get_params(mpt_kpcnn* net, float* params, float* grads);
+int32_t mpt_kpcnn_set_params(mpt_kpcnn* net, const float* params);
+
+/* Runs the network on B frames of H x W (device arrays, double: radiance,
+ * albedo, normal with 3 values per pixel, variance and depth with 1). out
+ * (device, FP32, 3 per pixel, or NULL) receives the denoised radiance. With
+ * a target (device, FP32 radiance, 3 per pixel) it also computes the relMSE
+ * loss (o - t)^2 / (t^2 + 0.01) averaged over pixels and channels, written
+ * to *loss (host), and the parameter gradients; lr > 0 then takes one Adam
+ * step. */
+int32_t mpt_kpcnn_run(mpt_kpcnn* net, const double* radiance, const double* albedo, const double* normal,
+                      const double* variance, const double* depth, int32_t B, int32_t H, int32_t W,
+                      const float* target, float lr, float* out, double* loss);
 
 #ifdef __cplusplus
 }

@@ -23,7 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # MPT_LIBRARY selects another build of the same library (tuning experiments)
 LIB_PATH = os.environ.get("MPT_LIBRARY", os.path.join(HERE, "libmpt.so"))
 
-__all__ = ["Scene", "Frame", "MptError", "lib", "abi", "Camera", "RenderOpts", "device_info", "denoise",
+__all__ = ["Scene", "Frame", "MptError", "lib", "abi", "Camera", "RenderOpts", "device_info", "denoise", "DeviceArray", "synchronize",
            "ggx_albedo_table", "describe"]
 
 
@@ -61,6 +61,21 @@ def _load():
         "mpt_scene_light_probabilities": (C.c_int32, [vp, P(C.c_double)]),
         "mpt_bsdf_eval": (C.c_int32, [vp, P(C.c_int32), P(C.c_double), P(C.c_uint8), C.c_int64, P(C.c_double),
                                       P(C.c_double)]),
+        "mpt_primary_depth": (C.c_int32, [vp, P(Camera), P(C.c_double)]),
+        "mpt_device_alloc": (vp, [C.c_size_t]),
+        "mpt_device_free": (None, [vp]),
+        "mpt_copy_to_device": (C.c_int32, [vp, vp, C.c_size_t]),
+        "mpt_copy_to_host": (C.c_int32, [vp, vp, C.c_size_t]),
+        "mpt_device_synchronize": (C.c_int32, []),
+        "mpt_gemm_bf16": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, C.c_int64, vp,
+                                      C.c_int64, vp, C.c_int64, C.c_int32, vp]),
+        "mpt_kpcnn_create": (C.c_int32, [C.c_int32, C.c_int32, C.c_uint64, P(vp)]),
+        "mpt_kpcnn_destroy": (None, [vp]),
+        "mpt_kpcnn_param_count": (C.c_int64, [vp]),
+        "mpt_kpcnn_layout": (C.c_int32, [vp, P(C.c_int32), P(C.c_int32), P(C.c_int64), P(C.c_int64)]),
+        "mpt_kpcnn_get_params": (C.c_int32, [vp, P(C.c_float), P(C.c_float)]),
+        "mpt_kpcnn_set_params": (C.c_int32, [vp, P(C.c_float)]),
+        "mpt_kpcnn_run": (C.c_int32, [vp] * 6 + [C.c_int32] * 3 + [vp, C.c_float, vp, P(C.c_double)]),
         "mpt_denoise_defaults": (None, [P(abi.DenoiseOpts)]),
         "mpt_denoise": (C.c_int32, [P(C.c_double)] * 4 + [C.c_int32, C.c_int32, P(abi.DenoiseOpts),
                                                            P(C.c_double)]),
@@ -99,6 +114,43 @@ def ggx_albedo_table():
     ea = np.zeros(32)
     _check(lib.mpt_ggx_albedo_table(_dp(e), _dp(ea)))
     return e.reshape(32, 32), ea
+
+
+class DeviceArray:
+    """A numpy-shaped buffer in HBM (mpt_device_alloc), for the entry points
+    that take device pointers."""
+
+    def __init__(self, shape, dtype):
+        self.shape, self.dtype = tuple(shape), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = lib.mpt_device_alloc(self.nbytes)
+        if not self.ptr:
+            raise MptError(-3, lib.mpt_last_error().decode())
+
+    @classmethod
+    def of(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(a.shape, a.dtype)
+        _check(lib.mpt_copy_to_device(d.ptr, a.ctypes.data, d.nbytes))
+        return d
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        _check(lib.mpt_copy_to_host(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    @property
+    def handle(self):
+        return C.c_void_p(self.ptr)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib.mpt_device_free(self.ptr)
+            self.ptr = None
+
+
+def synchronize():
+    _check(lib.mpt_device_synchronize())
 
 
 def denoise(frame, iterations=5, sigma_luminance=4.0, sigma_normal=0.1, sigma_albedo=0.1):
@@ -244,6 +296,13 @@ class Scene:
                               _dp(alb) if feats else None, _dp(nrm) if feats else None,
                               _dp(var) if feats else None, cb, None, C.byref(stats)))
         return Frame(rad, alb, nrm, var, stats)
+
+    def primary_depth(self, camera=None):
+        """Depth of the first surface through every pixel centre (0 for the sky)."""
+        cam = abi.camera_copy(camera if camera is not None else self.camera)
+        d = np.zeros((cam.height(), cam.image_width))
+        _check(lib.mpt_primary_depth(self._h, C.byref(cam), _dp(d)))
+        return d
 
     def read_frame(self, camera):
         w, h = camera.image_width, camera.height()

@@ -886,10 +886,40 @@ int32_t mpt_device_info(char* name, int32_t name_len, int32_t* cu_count, int64_t
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, dev));
     if (name && name_len > 0) {
-        std::snprintf(name, size_t(name_len), "%s (%s)", prop.name, prop.gcnArchName);
+        if (prop.name[0]) std::snprintf(name, size_t(name_len), "%s (%s)", prop.name, prop.gcnArchName);
+        else std::snprintf(name, size_t(name_len), "%s, %d CUs", prop.gcnArchName, prop.multiProcessorCount);
     }
     if (cu_count) *cu_count = prop.multiProcessorCount;
     if (hbm_bytes) *hbm_bytes = int64_t(prop.totalGlobalMem);
+    return MPT_OK;
+}
+
+void* mpt_device_alloc(size_t bytes) {
+    void* p = nullptr;
+    if (bytes == 0) bytes = 1;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        set_error(MPT_E_HIP, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        return nullptr;
+    }
+    return p;
+}
+
+void mpt_device_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+int32_t mpt_copy_to_device(void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return MPT_OK;
+}
+
+int32_t mpt_copy_to_host(void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return MPT_OK;
+}
+
+int32_t mpt_device_synchronize(void) {
+    HIP_TRY(hipDeviceSynchronize());
     return MPT_OK;
 }
 
@@ -1059,6 +1089,34 @@ int32_t mpt_trace_closest(mpt_scene* s, const double* rays, const double* tt, in
     }
     cleanup();
     return rc;
+}
+
+int32_t mpt_primary_depth(mpt_scene* s, const mpt_camera* cam, double* depth) {
+    if (!s || !cam || !depth) return set_error(MPT_E_INVALID, "null argument");
+    if (cam->image_width <= 0 || !(cam->aspect_ratio > 0)) return set_error(MPT_E_INVALID, "bad camera");
+    const dcam C = init_camera(*cam, s->hs.env_w > 0);
+    const int64_t n = int64_t(C.w) * C.h;
+    std::vector<double> rays(6 * n), tt(2 * n), t(n);
+    std::vector<int64_t> id(n);
+    std::vector<uint8_t> front(n);
+    for (int j = 0; j < C.h; j++)
+        for (int i = 0; i < C.w; i++) {  // pixel centre, no defocus (camera.h:291-301)
+            const int64_t k = int64_t(j) * C.w + i;
+            v3 ps = C.pixel00 + (double(i) * C.du) + (double(j) * C.dv);
+            v3 org = C.ortho_height > 0 ? ps + C.focus_dist * C.w_axis : C.center;
+            v3 d = ps - org;
+            rays[6 * k] = org.x, rays[6 * k + 1] = org.y, rays[6 * k + 2] = org.z;
+            rays[6 * k + 3] = d.x, rays[6 * k + 4] = d.y, rays[6 * k + 5] = d.z;
+            tt[2 * k] = 0.001;
+            tt[2 * k + 1] = kInf;
+        }
+    int rc = mpt_trace_closest(s, rays.data(), tt.data(), n, id.data(), t.data(), front.data());
+    if (rc) return rc;
+    for (int64_t k = 0; k < n; k++) {
+        const double dx = rays[6 * k + 3], dy = rays[6 * k + 4], dz = rays[6 * k + 5];
+        depth[k] = id[k] < 0 ? 0.0 : t[k] * std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    return MPT_OK;
 }
 
 int32_t mpt_bsdf_eval(mpt_scene* s, const int32_t* mats, const double* q, const uint8_t* front, int64_t n,
