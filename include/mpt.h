@@ -371,6 +371,59 @@ int32_t mpt_kpcnn_run(mpt_kpcnn* net, const double* radiance, const double* albe
                       const double* variance, const double* depth, int32_t B, int32_t H, int32_t W,
                       const float* target, float lr, float* out, double* loss);
 
+/* ---- neural radiance cache: hash-grid encoding + MLP on BF16 matrix cores ----
+ * No counterpart in the reference; the learned mode of the path tracer
+ * (the plain tracer, mpt_render, stays the fallback and the judge). */
+
+typedef struct mpt_nrc mpt_nrc;
+
+/* How a render uses the cache. */
+typedef struct mpt_nrc_opts {
+    int32_t query_depth;   /* paths are cut at the hit of this bounce (2) */
+    int32_t train_stride;  /* one path in this many (hash-chosen) is traced to the end to train (256) */
+    int32_t train_steps;   /* Adam steps per batch on its training paths (4) */
+    int32_t infer;         /* 1: cut paths and add beta * cache; 0: trace every path (train only) */
+    int32_t train;         /* 1: train after each batch */
+    float lr;              /* Adam learning rate (1e-3) */
+    double box_min[3];     /* the region the hash grid covers (all zero: the scene's bounds) */
+    double box_max[3];
+} mpt_nrc_opts;
+
+typedef struct mpt_nrc_stats {
+    int64_t queries;       /* paths that reached the query bounce */
+    int64_t train_samples; /* training samples used (summed over batches) */
+    double loss;           /* relative L2 loss of the last training step */
+} mpt_nrc_stats;
+
+void mpt_nrc_defaults(mpt_nrc_opts* opts);
+
+/* A cache with its hash tables and MLP initialised from seed. */
+int32_t mpt_nrc_create(uint64_t seed, mpt_nrc** out);
+void mpt_nrc_destroy(mpt_nrc* cache);
+/* MLP parameter count; *table_count receives the hash tables' (16 levels x 2^19 x 2). */
+int64_t mpt_nrc_param_count(const mpt_nrc* cache, int64_t* table_count);
+/* Grid resolution and dense-indexing flag of each of the 16 levels. */
+int32_t mpt_nrc_levels(const mpt_nrc* cache, uint32_t* resolution, uint32_t* dense);
+int32_t mpt_nrc_get_params(mpt_nrc* cache, float* mlp, float* mlp_grad, float* table, float* table_grad);
+int32_t mpt_nrc_set_params(mpt_nrc* cache, const float* mlp, const float* table);
+/* Predictions (3 per sample) for n samples: pos (3 per sample, in [0, 1]^3)
+ * and other inputs (10: normal, direction towards the viewer, roughness,
+ * albedo); device arrays. */
+int32_t mpt_nrc_infer_dev(mpt_nrc* cache, const float* pos, const float* other, int64_t n, float* pred);
+/* One training step towards target radiance (3 per sample): loss (host),
+ * gradients, and with lr > 0 an Adam step. */
+int32_t mpt_nrc_train_dev(mpt_nrc* cache, const float* pos, const float* other, const float* target, int64_t n,
+                          float lr, double* loss);
+
+/* camera::render with the cache: as mpt_render, and paths that reach the
+ * hit of bounce query_depth (all but the training paths) end there with
+ * radiance += beta * cache(hit); after each batch the cache trains on the
+ * training paths' radiance from that vertex on. nopts NULL = defaults;
+ * nstats may be NULL. */
+int32_t mpt_render_nrc(mpt_scene* scene, const mpt_camera* cam, const mpt_render_opts* opts, mpt_nrc* cache,
+                       const mpt_nrc_opts* nopts, double* radiance, double* albedo, double* normal, double* variance,
+                       mpt_pass_callback on_pass, void* user, mpt_render_stats* stats, mpt_nrc_stats* nstats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,17 @@ def _load():
         "mpt_kpcnn_get_params": (C.c_int32, [vp, P(C.c_float), P(C.c_float)]),
         "mpt_kpcnn_set_params": (C.c_int32, [vp, P(C.c_float)]),
         "mpt_kpcnn_run": (C.c_int32, [vp] * 6 + [C.c_int32] * 3 + [vp, C.c_float, vp, P(C.c_double)]),
+        "mpt_nrc_defaults": (None, [P(abi.NrcOpts)]),
+        "mpt_nrc_create": (C.c_int32, [C.c_uint64, P(vp)]),
+        "mpt_nrc_destroy": (None, [vp]),
+        "mpt_nrc_param_count": (C.c_int64, [vp, P(C.c_int64)]),
+        "mpt_nrc_levels": (C.c_int32, [vp, P(C.c_uint32), P(C.c_uint32)]),
+        "mpt_nrc_get_params": (C.c_int32, [vp] + [P(C.c_float)] * 4),
+        "mpt_nrc_set_params": (C.c_int32, [vp, P(C.c_float), P(C.c_float)]),
+        "mpt_nrc_infer_dev": (C.c_int32, [vp, vp, vp, C.c_int64, vp]),
+        "mpt_nrc_train_dev": (C.c_int32, [vp, vp, vp, vp, C.c_int64, C.c_float, P(C.c_double)]),
+        "mpt_render_nrc": (C.c_int32, [vp, P(Camera), P(RenderOpts), vp, P(abi.NrcOpts)] + [P(C.c_double)] * 4 +
+                           [abi.PASS_CALLBACK, vp, P(RenderStats), P(abi.NrcStats)]),
         "mpt_denoise_defaults": (None, [P(abi.DenoiseOpts)]),
         "mpt_denoise": (C.c_int32, [P(C.c_double)] * 4 + [C.c_int32, C.c_int32, P(abi.DenoiseOpts),
                                                            P(C.c_double)]),

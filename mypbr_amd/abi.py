@@ -128,3 +128,15 @@ class DenoiseOpts(C.Structure):
     """mpt_denoise_opts (denoiser.h:40-44)."""
     _fields_ = [("iterations", C.c_int32), ("pad", C.c_int32), ("sigma_luminance", C.c_double),
                 ("sigma_normal", C.c_double), ("sigma_albedo", C.c_double)]
+
+
+class NrcOpts(C.Structure):
+    """mpt_nrc_opts."""
+    _fields_ = [("query_depth", C.c_int32), ("train_stride", C.c_int32), ("train_steps", C.c_int32),
+                ("infer", C.c_int32), ("train", C.c_int32), ("lr", C.c_float), ("box_min", C.c_double * 3),
+                ("box_max", C.c_double * 3)]
+
+
+class NrcStats(C.Structure):
+    """mpt_nrc_stats."""
+    _fields_ = [("queries", C.c_int64), ("train_samples", C.c_int64), ("loss", C.c_double)]

@@ -27,7 +27,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
           "-munsafe-fp-atomics"] + os.environ.get("MPT_DEVICE_FLAGS", "").split()
 
 HOST_SOURCES = ["scene_host.cpp", "scene_api.cpp"]
-DEVICE_SOURCES = ["kernels.hip", "denoise.hip", "learn.hip"]
+DEVICE_SOURCES = ["kernels.hip", "denoise.hip", "learn.hip", "nrc.hip"]
 
 
 def run(cmd):

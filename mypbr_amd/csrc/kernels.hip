@@ -37,6 +37,11 @@ namespace mpt {
 
 thread_local std::string g_last_error;
 
+// the radiance cache's network (nrc.hip)
+int nrc_infer(mpt_nrc* r, const float* pos, const float* other, int64_t n, float* pred, hipStream_t s);
+int nrc_train(mpt_nrc* r, const float* pos, const float* other, const float* target, int64_t n, float lr,
+              hipStream_t s, double* loss);
+
 int set_error(int code, const std::string& m) {
     g_last_error = m;
     return code;
@@ -170,8 +175,11 @@ constexpr int kClasses = 7;
 // Counters, each on its own 256-byte line so that atomics on different
 // counters do not serialise on one cache line: counter k is word
 // k * kCtrStride of the counter array. 0, 1: ping-pong queue lengths; 2:
-// shadow queue; 3 .. 3 + kClasses - 1: class queues. The 64-bit ray and
-// work statistics follow at word kCtrWords.
+// shadow queue; 3 .. 3 + kClasses - 1: class queues; kNrcQueue,
+// kNrcTrain: the radiance cache's query queue and training paths. The
+// 64-bit ray and work statistics follow at word kCtrWords.
+constexpr int kNrcQueue = 10;
+constexpr int kNrcTrain = 11;
 constexpr int kCtrStride = 64;
 constexpr int kCtrWords = 16 * kCtrStride;
 __host__ __device__ __forceinline__ uint32_t* ctr(uint32_t* base, int k) { return base + k * kCtrStride; }
@@ -216,6 +224,18 @@ __device__ __forceinline__ void block_append(uint32_t* lds, uint32_t bits, uint3
 }
 // geometry profile of scenes made of spheres and quads, in lists and instances
 constexpr int kQuadProfile = F_SPHERE | F_QUAD | F_LIST | F_XFORM;
+
+// Where the radiance cache cuts paths (depth < 0: never): at the hit of
+// bounce `depth` every path is appended to q (counter cnt); only the
+// training paths go on to be shaded.
+struct nrc_route {
+    int depth;
+    int cut;          // 0: every path goes on (the cache only trains)
+    uint32_t stride;
+    uint32_t salt;
+    uint32_t* q;
+    uint32_t* cnt;
+};
 
 #define STACK_SETUP()                                                                   \
     __shared__ int32_t lds_stack[kLdsDepth * BS];                                       \
@@ -276,6 +296,8 @@ __global__ void k_camera(dcam C, pstate P, batch_args B, uint32_t* q0, uint32_t*
     if (gt == 0) {
         *ctr(cnt, 0) = uint32_t(B.nslots);
         for (int k = 1; k < 3 + kClasses; k++) *ctr(cnt, k) = 0;
+        *ctr(cnt, kNrcQueue) = 0;  // the radiance cache's query and training counts
+        *ctr(cnt, kNrcTrain) = 0;
     }
     if (gt >= B.nslots) return;
     uint32_t slot = uint32_t(gt);
@@ -403,11 +425,22 @@ __device__ __forceinline__ int extend_one(const dscene& S, const pstate& P, int 
     return 1 + S.mat[rec.mat].type;
 }
 
+// Paths a batch traces to the end to train the radiance cache: one in
+// `stride`, chosen by a hash of the slot and a per-batch salt.
+__device__ __forceinline__ bool nrc_training(uint32_t slot, const nrc_route& R) {
+    uint32_t h = (slot ^ R.salt) * 0x9E3779B1u;
+    h ^= h >> 15;
+    h *= 0x85EBCA77u;
+    h ^= h >> 13;
+    return h % R.stride == 0;
+}
+
 template <int BS, bool CNT, int FEAT>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_SPHERE ? MPT_SPHERE_WAVES : 1))) void k_extend(dscene S, launch_ctx L, pstate P, int depth, const uint32_t* q,
                                                const uint32_t* cnt_in, uint32_t* cnt_next, uint32_t* cnt_shadow,
                                                uint32_t* qcls, uint32_t* ccls, int64_t cap,
-                                               unsigned long long* ray_stats, unsigned long long* work) {
+                                               unsigned long long* ray_stats, unsigned long long* work,
+                                               nrc_route R) {
     STACK_SETUP();
     const uint32_t n = *cnt_in;
     if (st.gtid == 0) {  // the counters shade and shadow append to in this bounce
@@ -416,9 +449,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_SP
         atomicAdd(&ray_stats[0], (unsigned long long)n);
     }
     wcount wc{0, 0};
-    // blocks walk the queue in block-sized steps; the class queues are
-    // appended once per block and step (block_append)
-    __shared__ uint32_t lds_app[kClasses * (BS / 64 + 1)];
+    // blocks walk the queue in block-sized steps; the class queues (and the
+    // radiance cache's query queue) are appended once per block and step
+    __shared__ uint32_t lds_app[(kClasses + 1) * (BS / 64 + 1)];
+    const bool query = depth == R.depth;
     for (int64_t base = int64_t(blockIdx.x) * BS; base < n; base += int64_t(gridDim.x) * BS) {
         const int64_t i = base + threadIdx.x;
         int cls = -1;
@@ -427,9 +461,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_SP
             p = q[i];
             cls = extend_one<CNT, FEAT>(S, P, depth, p, st, wc);
         }
-        block_append<BS, kClasses>(
-            lds_app, cls >= 0 ? 1u << cls : 0u, p, [&](int k) { return ccls + k * kCtrStride; },
-            [&](int k) { return qcls + k * cap; });
+        uint32_t bits = cls >= 0 ? 1u << cls : 0u;
+        if (query && cls > 0) {  // a hit at the cache's bounce: queried; shaded only if it trains
+            bits |= 1u << kClasses;
+            if (R.cut && !nrc_training(p, R)) bits = 1u << kClasses;
+        }
+        block_append<BS, kClasses + 1>(
+            lds_app, bits, p, [&](int k) { return k < kClasses ? ccls + k * kCtrStride : R.cnt; },
+            [&](int k) { return k < kClasses ? qcls + k * cap : R.q; });
     }
     if constexpr (CNT) {
         if (wc.nodes) atomicAdd(&work[0], (unsigned long long)wc.nodes);
@@ -704,6 +743,108 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MPT_FINISH_W
     }
 }
 
+// ------------------------------------------------------------------ radiance cache
+// Buffers of the cache's query and training samples (capacity: the path
+// capacity for queries, tcap for training records).
+struct nrc_bufs {
+    float* pos;        // [cap][3] hit point in the cache's box, [0, 1]^3
+    float* other;      // [cap][10] normal, direction towards the viewer, roughness, albedo
+    float* pred;       // [cap][3] the cache's radiance
+    uint32_t* tidx;    // [cap] query-queue index of each training path
+    uint32_t* tcnt;    // their count
+    uint32_t* tslot;   // [tcap] training records: path slot,
+    double* trad;      //   radiance before the vertex,
+    double* tbeta;     //   throughput at the vertex,
+    float* tpos;       //   inputs,
+    float* tother;
+    float* ttarget;    //   and the path's radiance from the vertex on
+    int64_t tcap;
+    v3 box_min, inv_ext;
+};
+
+// The cache's inputs at every queried vertex; training paths' queue indices
+// are appended to tidx.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_nrc_prep(dscene S, pstate P, const uint32_t* q, const uint32_t* cnt,
+                                                 nrc_route R, nrc_bufs B) {
+    __shared__ uint32_t lds[BS / 64 + 1];
+    const uint32_t n = *cnt;
+    for (int64_t base = int64_t(blockIdx.x) * BS; base < n; base += int64_t(gridDim.x) * BS) {
+        const int64_t i = base + threadIdx.x;
+        bool train = false;
+        if (i < n) {
+            const uint32_t p = q[i];
+            const v3 x = ld3(P, D_QX, p), nrm = ld3(P, D_MX, p), d = ld3(P, D_DX, p);
+            const mpt_material& m = S.mat[P.hmat[p] & 0x7fffffffu];
+            const v3 u = (x - B.box_min) * B.inv_ext;
+            const v3 wo = -unit_vector(d);
+            double rough = 1.0;
+            v3 alb{1, 1, 1};
+            switch (m.type) {
+                case MPT_MAT_LAMBERTIAN: alb = tex_value(S, m.texture, x); break;
+                case MPT_MAT_METAL: alb = v3{m.albedo[0], m.albedo[1], m.albedo[2]}, rough = m.roughness; break;
+                case MPT_MAT_DIELECTRIC: rough = m.roughness; break;
+                case MPT_MAT_PRINCIPLED: alb = tex_value(S, m.texture, x), rough = m.roughness; break;
+                case MPT_MAT_PHASE: alb = v3{m.albedo[0], m.albedo[1], m.albedo[2]}; break;
+                default: break;
+            }
+            float* ps = B.pos + 3 * i;
+            ps[0] = float(u.x), ps[1] = float(u.y), ps[2] = float(u.z);
+            float* o = B.other + 10 * i;
+            o[0] = float(nrm.x), o[1] = float(nrm.y), o[2] = float(nrm.z);
+            o[3] = float(wo.x), o[4] = float(wo.y), o[5] = float(wo.z);
+            o[6] = float(rough);
+            o[7] = float(alb.x), o[8] = float(alb.y), o[9] = float(alb.z);
+            train = nrc_training(p, R);
+        }
+        block_append<BS, 1>(
+            lds, train ? 1u : 0u, uint32_t(i), [&](int) { return B.tcnt; }, [&](int) { return B.tidx; });
+    }
+}
+
+// Training records of the first tcap training paths: slot, radiance and throughput so far, inputs.
+__global__ void k_nrc_record(pstate P, const uint32_t* q, nrc_bufs B) {
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t n = *B.tcnt < B.tcap ? int64_t(*B.tcnt) : B.tcap;
+    if (j >= n) return;
+    const uint32_t i = B.tidx[j];
+    const uint32_t p = q[i];
+    B.tslot[j] = p;
+    const v3 r = ld3(P, D_RX, p), b = ld3(P, D_BX, p);
+    B.trad[3 * j] = r.x, B.trad[3 * j + 1] = r.y, B.trad[3 * j + 2] = r.z;
+    B.tbeta[3 * j] = b.x, B.tbeta[3 * j + 1] = b.y, B.tbeta[3 * j + 2] = b.z;
+    for (int c = 0; c < 3; c++) B.tpos[3 * j + c] = B.pos[3 * int64_t(i) + c];
+    for (int c = 0; c < 10; c++) B.tother[10 * j + c] = B.other[10 * int64_t(i) + c];
+}
+
+// Paths cut at the cache's bounce end here: radiance += beta * cache(x).
+__global__ void k_nrc_apply(pstate P, const uint32_t* q, const uint32_t* cnt, nrc_route R, const float* pred) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= *cnt) return;
+    const uint32_t p = q[i];
+    if (!R.cut || nrc_training(p, R)) return;
+    const v3 b = ld3(P, D_BX, p);
+    const v3 c{std::fmax(double(pred[3 * i]), 0.0), std::fmax(double(pred[3 * i + 1]), 0.0),
+               std::fmax(double(pred[3 * i + 2]), 0.0)};
+    st3(P, D_RX, p, ld3(P, D_RX, p) + b * c);
+}
+
+// Training targets once the batch is done: the radiance the path gathered
+// from the vertex on, per unit of throughput there.
+__global__ void k_nrc_collect(pstate P, nrc_bufs B) {
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t n = *B.tcnt < B.tcap ? int64_t(*B.tcnt) : B.tcap;
+    if (j >= n) return;
+    const v3 r = ld3(P, D_RX, B.tslot[j]);
+    const double rr[3] = {r.x, r.y, r.z};
+    for (int c = 0; c < 3; c++) {
+        const double b = B.tbeta[3 * j + c];
+        double t = b > 0 ? (rr[c] - B.trad[3 * j + c]) / b : 0.0;
+        if (!std::isfinite(t)) t = 0.0;
+        B.ttarget[3 * j + c] = float(t);
+    }
+}
+
 // Zeroes n counters (stands in for the shadow stage when a scene has no lights).
 __global__ void k_zero(uint32_t* c, int n) {
     if (int(threadIdx.x) < n) c[threadIdx.x * kCtrStride] = 0;
@@ -798,8 +939,14 @@ struct mpt_scene {
     int last_done = 0;
     mpt_camera last_cam{};
     hipStream_t stream = nullptr;
+    // radiance cache buffers (mpt_render_nrc)
+    std::vector<void*> nrc_allocs;
+    int64_t nrc_capacity = 0;
+    nrc_bufs nb{};
+    uint32_t* nrc_queue = nullptr;
 
     ~mpt_scene() {
+        for (void* p : nrc_allocs) (void)hipFree(p);
         for (void* p : allocs) (void)hipFree(p);
         for (void* p : ps_allocs) (void)hipFree(p);
         if (ovf) (void)hipFree(ovf);
@@ -1215,8 +1362,38 @@ int32_t mpt_read_frame(mpt_scene* s, double* radiance, double* albedo, double* n
     return MPT_OK;
 }
 
-int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts, double* radiance, double* albedo,
-                   double* normal, double* variance, mpt_pass_callback on_pass, void* user, mpt_render_stats* stats) {
+}  // extern "C"
+
+namespace {
+
+// Query and training buffers of the radiance cache for `cap` paths.
+int ensure_nrc(mpt_scene* s, int64_t cap) {
+    if (s->nrc_capacity >= cap) return MPT_OK;
+    for (void* p : s->nrc_allocs) (void)hipFree(p);
+    s->nrc_allocs.clear();
+    s->nrc_capacity = 0;
+    auto get = [&](void** out, size_t bytes) -> int {
+        HIP_TRY(hipMalloc(out, bytes));
+        s->nrc_allocs.push_back(*out);
+        return MPT_OK;
+    };
+    nrc_bufs& B = s->nb;
+    B.tcap = std::max<int64_t>(1 << 16, cap / 16);
+    int rc;
+    if ((rc = get((void**)&s->nrc_queue, cap * sizeof(uint32_t))) || (rc = get((void**)&B.pos, cap * 3 * sizeof(float))) ||
+        (rc = get((void**)&B.other, cap * 10 * sizeof(float))) || (rc = get((void**)&B.pred, cap * 3 * sizeof(float))) ||
+        (rc = get((void**)&B.tidx, cap * sizeof(uint32_t))) || (rc = get((void**)&B.tslot, B.tcap * sizeof(uint32_t))) ||
+        (rc = get((void**)&B.trad, B.tcap * 3 * sizeof(double))) || (rc = get((void**)&B.tbeta, B.tcap * 3 * sizeof(double))) ||
+        (rc = get((void**)&B.tpos, B.tcap * 3 * sizeof(float))) || (rc = get((void**)&B.tother, B.tcap * 10 * sizeof(float))) ||
+        (rc = get((void**)&B.ttarget, B.tcap * 3 * sizeof(float))))
+        return rc;
+    s->nrc_capacity = cap;
+    return MPT_OK;
+}
+
+int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts, double* radiance, double* albedo,
+                double* normal, double* variance, mpt_pass_callback on_pass, void* user, mpt_render_stats* stats,
+                mpt_nrc* nrc, const mpt_nrc_opts* nopts, mpt_nrc_stats* nstats) {
     if (!s || !cam) return set_error(MPT_E_INVALID, "null argument");
     if (cam->image_width < 1 || !(cam->aspect_ratio > 0) || cam->samples_per_pixel < 1 || cam->max_depth < 1 ||
         cam->max_depth > 1000)
@@ -1346,6 +1523,27 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
 
     const dscene DS = s->ds;
     const bool has_env = s->hs.env_w > 0;
+    // the radiance cache: where it cuts paths, its buffers
+    nrc_route R{-1, 0, 1, 0, nullptr, nullptr};
+    nrc_bufs NB{};
+    if (nrc) {
+        if (nopts->query_depth >= C.max_depth) return set_error(MPT_E_INVALID, "nrc: query_depth must be below max_depth");
+        if ((rc = ensure_nrc(s, s->ps_capacity))) return rc;
+        NB = s->nb;
+        NB.tcnt = ctr(cnt, kNrcTrain);
+        double lo[3], hi[3];
+        const bool given = nopts->box_max[0] > nopts->box_min[0] && nopts->box_max[1] > nopts->box_min[1] &&
+                           nopts->box_max[2] > nopts->box_min[2];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = given ? nopts->box_min[a] : s->hs.world_box.ax[a].min;
+            hi[a] = given ? nopts->box_max[a] : s->hs.world_box.ax[a].max;
+        }
+        NB.box_min = v3{lo[0], lo[1], lo[2]};
+        NB.inv_ext = v3{1 / (hi[0] - lo[0]), 1 / (hi[1] - lo[1]), 1 / (hi[2] - lo[2])};
+        R = nrc_route{nopts->query_depth, nopts->infer ? 1 : 0, uint32_t(nopts->train_stride), 0, s->nrc_queue,
+                      ctr(cnt, kNrcQueue)};
+        if (nstats) std::memset(nstats, 0, sizeof(*nstats));
+    }
     const unsigned G = unsigned(s->grid);
     hipStream_t stream = s->stream;
     auto launch_extend = [&](int depth, uint32_t* qin, uint32_t* cin, uint32_t* cout) {
@@ -1353,10 +1551,10 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
     do {                                                                                                             \
         if (counting)                                                                                                \
             hipLaunchKernelGGL((k_extend<256, true, FEAT>), dim3(G), dim3(256), 0, stream, DS, L, P, depth, qin, cin, \
-                               cout, ctr(cnt, 2), qcls, ccls, cap, ray_stats, work);                                     \
+                               cout, ctr(cnt, 2), qcls, ccls, cap, ray_stats, work, R);                                  \
         else                                                                                                         \
             hipLaunchKernelGGL((k_extend<256, false, FEAT>), dim3(G), dim3(256), 0, stream, DS, L, P, depth, qin,    \
-                               cin, cout, ctr(cnt, 2), qcls, ccls, cap, ray_stats, work);                                \
+                               cin, cout, ctr(cnt, 2), qcls, ccls, cap, ray_stats, work, R);                             \
     } while (0)
         if (s->profile == 1) EXT(F_SPHERE);
         else if (s->profile == 2) EXT(kQuadProfile);
@@ -1449,6 +1647,7 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
             nb = std::min(nb, next_save - done_abs);
         }
         batch_args B{o.seed, uint32_t(o.scene_index), o.first_pass + pass, npix, nb * npix};
+        R.salt = uint32_t(mix64(o.seed ^ (uint64_t(uint32_t(o.first_pass + pass)) << 32)));
         EV(3, hipLaunchKernelGGL(k_camera, dim3(unsigned((B.nslots + 255) / 256)), dim3(256), 0, stream, C, P, B, q[0],
                                  cnt));
         if (C.features) EV(3, hipLaunchKernelGGL((k_features<256>), dim3(G), dim3(256), 0, stream, DS, L, P, B));
@@ -1461,7 +1660,7 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
                 }
                 const uint32_t left = uint32_t(s->pinned_counts[(depth - 2) % kRing]);
                 if (left == 0) break;
-                if (tail > 0 && left <= uint32_t(tail)) {
+                if (tail > 0 && left <= uint32_t(tail) && depth > R.depth) {
                     EV(4, launch_finish(depth, qin, ctr(cnt, depth & 1)));
                     break;
                 }
@@ -1472,6 +1671,23 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
             // extend zeroes the next-queue and shadow counters of this bounce;
             // shadow zeroes the class counters once every shade kernel read them
             EV(0, launch_extend(depth, qin, cin, cout));
+            if (nrc && depth == R.depth) {  // the cache's inputs, then paths not training end with its radiance
+                hipLaunchKernelGGL((k_nrc_prep<256>), dim3(G), dim3(256), 0, stream, DS, P, R.q, R.cnt, R, NB);
+                hipLaunchKernelGGL(k_nrc_record, dim3(unsigned((NB.tcap + 255) / 256)), dim3(256), 0, stream, P, R.q,
+                                   NB);
+                uint32_t nq = 0;
+                HIP_TRY(hipMemcpyAsync(&nq, R.cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+                HIP_TRY(hipStreamSynchronize(stream));
+                if (nstats) nstats->queries += nq;
+                if (R.cut && nq > 0) {
+                    if ((rc = nrc_infer(nrc, NB.pos, NB.other, int64_t(nq), NB.pred, stream))) {
+                        ev_free();
+                        return rc;
+                    }
+                    hipLaunchKernelGGL(k_nrc_apply, dim3(unsigned((nq + 255) / 256)), dim3(256), 0, stream, P, R.q,
+                                       R.cnt, R, NB.pred);
+                }
+            }
             for (int cls = 0; cls < kClasses; cls++) {
                 if (cls > 0 && !(s->mat_types & (1u << (cls - 1)))) continue;
                 EV(1, launch_shade(cls, depth, qout, cout));
@@ -1480,6 +1696,25 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
             else EV(2, hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, ccls, kClasses));
             HIP_TRY(hipMemcpyAsync(&s->pinned_counts[depth % kRing], cout, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
             HIP_TRY(hipEventRecord(ring[depth % kRing], stream));
+        }
+        if (nrc && nopts->train && nopts->train_steps > 0) {  // train on this batch's path suffixes
+            hipLaunchKernelGGL(k_nrc_collect, dim3(unsigned((NB.tcap + 255) / 256)), dim3(256), 0, stream, P, NB);
+            uint32_t nt = 0;
+            HIP_TRY(hipMemcpyAsync(&nt, NB.tcnt, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            const int64_t ntrain = std::min<int64_t>(nt, NB.tcap);
+            double loss = 0;
+            for (int k = 0; k < nopts->train_steps && ntrain > 0; k++) {
+                if ((rc = nrc_train(nrc, NB.tpos, NB.tother, NB.ttarget, ntrain, nopts->lr, stream,
+                                    k + 1 == nopts->train_steps ? &loss : nullptr))) {
+                    ev_free();
+                    return rc;
+                }
+            }
+            if (nstats) {
+                nstats->train_samples += ntrain;
+                if (ntrain > 0) nstats->loss = loss;
+            }
         }
         EV(3, hipLaunchKernelGGL(k_accumulate, dim3(unsigned((npix + 255) / 256)), dim3(256), 0, stream, P, npix, nb,
                                  C.features, acc_rad, acc_alb, acc_nrm, acc_l2));
@@ -1527,4 +1762,38 @@ int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* o
     return MPT_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts, double* radiance, double* albedo,
+                   double* normal, double* variance, mpt_pass_callback on_pass, void* user, mpt_render_stats* stats) {
+    return render_impl(s, cam, opts, radiance, albedo, normal, variance, on_pass, user, stats, nullptr, nullptr,
+                       nullptr);
+}
+
+void mpt_nrc_defaults(mpt_nrc_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->query_depth = 2;
+    o->train_stride = 256;
+    o->train_steps = 4;
+    o->infer = 1;
+    o->train = 1;
+    o->lr = 1e-3f;
+}
+
+int32_t mpt_render_nrc(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts, mpt_nrc* nrc,
+                       const mpt_nrc_opts* nopts, double* radiance, double* albedo, double* normal, double* variance,
+                       mpt_pass_callback on_pass, void* user, mpt_render_stats* stats, mpt_nrc_stats* nstats) {
+    if (!nrc) return set_error(MPT_E_INVALID, "null radiance cache");
+    mpt_nrc_opts o;
+    if (nopts) o = *nopts;
+    else mpt_nrc_defaults(&o);
+    if (o.query_depth < 1 || o.train_stride < 1 || o.train_steps < 0 || !(o.lr >= 0))
+        return set_error(MPT_E_INVALID, "nrc: query_depth >= 1, train_stride >= 1, train_steps >= 0, lr >= 0");
+    return render_impl(s, cam, opts, radiance, albedo, normal, variance, on_pass, user, stats, nrc, &o, nstats);
+}
+
 }  // extern "C"
+
