@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-quality", action="store_true",
+                    help="skip the equal-time relMSE study of the learned forms against the plain tracer")
     ap.add_argument("--tail-paths", type=int, default=0,
                     help="live paths at which a batch's tail runs as one launch (0 = the library's default, -1 = never)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -213,6 +215,11 @@ def main():
                             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_per_launch(f"k_{dom}"),
                             "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6}
         log("stages:", json.dumps(stage))
+
+    if not args.no_quality:
+        # the metric's second half: equal-time relMSE (eps 0.01) against a 4096-spp render of the plain tracer
+        from mypbr_amd.quality import equal_time_study
+        line["quality"] = equal_time_study(scene_path, box=((-14, -1, -14), (14, 4, 14)), log=log)
 
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -358,6 +358,10 @@ int64_t mpt_kpcnn_param_count(const mpt_kpcnn* net);
 int32_t mpt_kpcnn_layout(const mpt_kpcnn* net, int32_t* cin, int32_t* cout, int64_t* weight_offset,
                          int64_t* bias_offset);
 int32_t mpt_kpcnn_get_params(mpt_kpcnn* net, float* params, float* grads);
+/* Training loss: 0 relMSE (o - t)^2 / (t^2 + 0.01) (the default); 1 the L1
+ * distance of log(1 + x), which does not let a few dark pixels next to
+ * bright ones dominate the gradient. */
+int32_t mpt_kpcnn_set_loss(mpt_kpcnn* net, int32_t mode);
 int32_t mpt_kpcnn_set_params(mpt_kpcnn* net, const float* params);
 
 /* Runs the network on B frames of H x W (device arrays, double: radiance,

@@ -75,6 +75,7 @@ def _load():
         "mpt_kpcnn_layout": (C.c_int32, [vp, P(C.c_int32), P(C.c_int32), P(C.c_int64), P(C.c_int64)]),
         "mpt_kpcnn_get_params": (C.c_int32, [vp, P(C.c_float), P(C.c_float)]),
         "mpt_kpcnn_set_params": (C.c_int32, [vp, P(C.c_float)]),
+        "mpt_kpcnn_set_loss": (C.c_int32, [vp, C.c_int32]),
         "mpt_kpcnn_run": (C.c_int32, [vp] * 6 + [C.c_int32] * 3 + [vp, C.c_float, vp, P(C.c_double)]),
         "mpt_nrc_defaults": (None, [P(abi.NrcOpts)]),
         "mpt_nrc_create": (C.c_int32, [C.c_uint64, P(vp)]),

@@ -104,9 +104,11 @@ __global__ void k_features(int64_t npix, const double* __restrict__ rad, const d
 // (coordinates clamped to the image), times albedo + eps. With a target:
 // loss += relMSE terms (o - t)^2 / (t^2 + 0.01) / (3 * npix) and the
 // gradient of the loss with respect to the logits (FP32, kOut per pixel).
+// Loss modes: 0 relMSE (o - t)^2 / (t^2 + 0.01); 1 L1 of log(1 + x),
+// |log(1 + max(o, 0)) - log(1 + max(t, 0))|; both averaged over pixels and channels.
 __global__ void k_apply(int64_t npix, int H, int W, const float* __restrict__ logits, const float* __restrict__ light,
                         const float* __restrict__ albe, const float* __restrict__ target, float* __restrict__ out,
-                        float* __restrict__ dlogits, double* __restrict__ loss) {
+                        float* __restrict__ dlogits, double* __restrict__ loss, int mode) {
     const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     double lsum = 0;
     if (p < npix) {
@@ -139,10 +141,18 @@ __global__ void k_apply(int64_t npix, int H, int W, const float* __restrict__ lo
             if (out) out[3 * p + c] = rad;
             if (target) {
                 const float t = target[3 * p + c];
-                const float den = t * t + 1e-2f;
-                const float d = rad - t;
-                lsum += double(d * d / den);
-                dl[c] = 2.f * d / den / float(3 * npix) * a;  // d loss / d o[c]
+                if (mode == 1) {
+                    const float ro = std::fmax(rad, 0.f);
+                    const float d = std::log1p(ro) - std::log1p(std::fmax(t, 0.f));
+                    lsum += double(std::fabs(d));
+                    const float sg = d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f);
+                    dl[c] = rad > 0.f ? sg / (1.f + ro) / float(3 * npix) * a : 0.f;
+                } else {
+                    const float den = t * t + 1e-2f;
+                    const float d = rad - t;
+                    lsum += double(d * d / den);
+                    dl[c] = 2.f * d / den / float(3 * npix) * a;  // d loss / d o[c]
+                }
             }
         }
         if (dlogits) {
@@ -181,6 +191,7 @@ struct mpt_kpcnn {
     float* m = nullptr;    // Adam moments
     float* v = nullptr;
     int64_t step = 0;
+    int loss_mode = 0;
     // activation workspace for a given pixel count
     int64_t cap = 0;
     std::vector<bf16*> act;  // act[0] = features, act[l + 1] = output of layer l (BF16), last unused
@@ -371,6 +382,12 @@ int32_t mpt_kpcnn_create(int32_t layers, int32_t hidden, uint64_t seed, mpt_kpcn
 
 void mpt_kpcnn_destroy(mpt_kpcnn* k) { delete k; }
 
+int32_t mpt_kpcnn_set_loss(mpt_kpcnn* k, int32_t mode) {
+    if (!k || mode < 0 || mode > 1) return set_error(MPT_E_INVALID, "kpcnn: loss mode 0 (relMSE) or 1 (log L1)");
+    k->loss_mode = mode;
+    return MPT_OK;
+}
+
 int64_t mpt_kpcnn_param_count(const mpt_kpcnn* k) { return k ? k->nparams : 0; }
 
 int32_t mpt_kpcnn_layout(const mpt_kpcnn* k, int32_t* cin, int32_t* cout, int64_t* woff, int64_t* boff) {
@@ -418,7 +435,7 @@ int32_t mpt_kpcnn_run(mpt_kpcnn* k, const double* rad, const double* alb, const 
     if ((rc = forward(k, B, H, W))) return rc;
     NN_TRY(hipMemsetAsync(k->loss, 0, sizeof(double), s));
     hipLaunchKernelGGL(k_apply, dim3(blocks(npix)), dim3(256), 0, s, npix, H, W, k->logits, k->light, k->albe,
-                       target, out, target ? k->dA : nullptr, target ? k->loss : nullptr);
+                       target, out, target ? k->dA : nullptr, target ? k->loss : nullptr, k->loss_mode);
     if (target) {
         if ((rc = backward(k, B, H, W))) return rc;
         if (lr > 0) {

@@ -36,6 +36,10 @@ class KPCNN:
             lib.mpt_kpcnn_destroy(self._h)
             self._h = None
 
+    def set_loss(self, mode):
+        """0: relMSE (o - t)^2 / (t^2 + 0.01); 1: L1 of log(1 + x)."""
+        _check(lib.mpt_kpcnn_set_loss(self._h, int(mode)))
+
     def layout(self):
         n = self.layers
         cin = (C.c_int32 * n)()

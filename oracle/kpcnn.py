@@ -108,15 +108,22 @@ def apply_kernel(logits, light, albe, B, H, W):
     return o * albe, w, q
 
 
-def loss_and_grad(logits, light, albe, target, B, H, W):
-    """relMSE (o - t)^2 / (t^2 + 0.01) averaged over pixels and channels, and
-    its gradient with respect to the logits (npix, 32)."""
+def loss_and_grad(logits, light, albe, target, B, H, W, mode=0):
+    """Mode 0: relMSE (o - t)^2 / (t^2 + 0.01); mode 1: |log(1 + max(o, 0)) -
+    log(1 + max(t, 0))|; averaged over pixels and channels, and the gradient
+    with respect to the logits (npix, 32)."""
     rad, w, q = apply_kernel(logits, light, albe, B, H, W)
-    den = target * target + 1e-2
-    d = rad - target
     npix = rad.shape[0]
-    loss = float((d * d / den).sum() / (3 * npix))
-    dl = 2 * d / den / (3 * npix) * albe              # d loss / d o
+    if mode == 1:
+        ro = np.maximum(rad, 0)
+        d = np.log1p(ro) - np.log1p(np.maximum(target, 0))
+        loss = float(np.abs(d).sum() / (3 * npix))
+        dl = np.where(rad > 0, np.sign(d) / (1 + ro), 0.0) / (3 * npix) * albe
+    else:
+        den = target * target + 1e-2
+        d = rad - target
+        loss = float((d * d / den).sum() / (3 * npix))
+        dl = 2 * d / den / (3 * npix) * albe          # d loss / d o
     gw = np.einsum("pc,pkc->pk", dl, light[q])        # d loss / d w_k
     dlog = np.zeros((npix, K_OUT))
     dlog[:, :K_TAPS] = w * (gw - (w * gw).sum(axis=1, keepdims=True))
@@ -135,12 +142,12 @@ def forward(net, feat, B, H, W, round_bf16=True):
         acts.append(r(np.maximum(y, 0.0)))
 
 
-def gradients(net, frame, target, B, H, W, round_bf16=True):
+def gradients(net, frame, target, B, H, W, round_bf16=True, mode=0):
     """Loss and the gradient of every parameter, in the parameter layout."""
     r = bf16 if round_bf16 else (lambda x: np.asarray(x, dtype=np.float64))
     feat, light, albe = frame
     acts, logits = forward(net, feat, B, H, W, round_bf16)
-    loss, dy, _ = loss_and_grad(logits, light, albe, target, B, H, W)
+    loss, dy, _ = loss_and_grad(logits, light, albe, target, B, H, W, mode)
     grad = np.zeros_like(net.p)
     for l in range(net.layers - 1, -1, -1):
         w, _ = net.layer(l)

@@ -85,7 +85,12 @@ def test_oracle_backward_equals_finite_differences():
     inp = oracle_inputs(frames)
     tgt = target.reshape(-1, 3)
     net = random_net(3, 16, 4)
-    loss, grad = kp.gradients(net, inp, tgt, B, H, W, round_bf16=False)
+    for mode in (0, 1):
+        _finite_differences(net, inp, tgt, B, H, W, mode)
+
+
+def _finite_differences(net, inp, tgt, B, H, W, mode):
+    loss, grad = kp.gradients(net, inp, tgt, B, H, W, round_bf16=False, mode=mode)
     rng = np.random.default_rng(5)
     # parameters that matter: used feature channels / logits of each layer
     picks = []
@@ -98,9 +103,9 @@ def test_oracle_backward_equals_finite_differences():
         h = 1e-6 * max(1.0, abs(net.p[i]))
         saved = net.p[i]
         net.p[i] = saved + h
-        lp, _ = kp.gradients(net, inp, tgt, B, H, W, round_bf16=False)
+        lp, _ = kp.gradients(net, inp, tgt, B, H, W, round_bf16=False, mode=mode)
         net.p[i] = saved - h
-        lm, _ = kp.gradients(net, inp, tgt, B, H, W, round_bf16=False)
+        lm, _ = kp.gradients(net, inp, tgt, B, H, W, round_bf16=False, mode=mode)
         net.p[i] = saved
         fd = (lp - lm) / (2 * h)
         err = abs(fd - grad[i]) / max(abs(fd), abs(grad[i]), 1e-9)
@@ -151,7 +156,8 @@ def test_gemm_bf16_matches_numpy(at, bt):
 
 
 @pytest.mark.gpu
-def test_gradients_match_oracle():
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gradients_match_oracle(mode):
     import mypbr_amd as m
     from mypbr_amd.learn import KPCNN, to_device
     B, H, W = 2, 12, 10
@@ -161,12 +167,14 @@ def test_gradients_match_oracle():
     assert (cin, cout) == layout(3, 16)[:2]
     p = np.random.default_rng(12).uniform(-0.3, 0.3, net.nparams).astype(np.float32)
     net.set_params(p)
+    net.set_loss(mode)
     dev = to_device(frames)
     tgt = m.DeviceArray.of(target, np.float32)
     loss = net.run(dev, target=tgt)
     g = net.grads().astype(np.float64)
     onet = kp.Net(p.astype(np.float64), cin, cout, woff, boff)
-    oloss, og = kp.gradients(onet, oracle_inputs(frames), target.reshape(-1, 3), B, H, W, round_bf16=True)
+    oloss, og = kp.gradients(onet, oracle_inputs(frames), target.reshape(-1, 3), B, H, W, round_bf16=True,
+                             mode=mode)
     print(f"loss gpu {loss:.6e} oracle {oloss:.6e}", flush=True)
     assert abs(loss - oloss) / oloss < 1e-3
     for l in range(3):
