@@ -278,6 +278,21 @@ int32_t mpt_render(mpt_scene* scene, const mpt_camera* cam, const mpt_render_opt
                    double* radiance, double* albedo, double* normal, double* variance,
                    mpt_pass_callback on_pass, void* user, mpt_render_stats* stats);
 
+/* What mpt_autotune chose. */
+typedef struct mpt_autotune_result {
+    int32_t passes_in_flight;  /* samples per pixel traced together */
+    int32_t trials;            /* timed trial renders */
+    int64_t tail_paths;        /* live paths at which a batch's tail runs in one launch (-1: never) */
+    double seconds_per_pass;   /* of the best trial */
+} mpt_autotune_result;
+
+/* Times trial renders of trial_passes passes (8 if < 1) of this camera's
+ * image for each combination of batch size (about 4, 8, 16 and 32 M paths
+ * in flight) and batch-tail size, keeps the fastest for the scene (used by
+ * later renders of the same image size whose options leave them at 0 =
+ * auto) and reports it. The image does not depend on either setting. */
+int32_t mpt_autotune(mpt_scene* scene, const mpt_camera* cam, int32_t trial_passes, mpt_autotune_result* out);
+
 /* Inside an on_pass callback: the frame averaged over the passes done. */
 int32_t mpt_read_frame(mpt_scene* scene, double* radiance, double* albedo, double* normal,
                        double* variance);

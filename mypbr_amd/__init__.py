@@ -62,6 +62,7 @@ def _load():
         "mpt_bsdf_eval": (C.c_int32, [vp, P(C.c_int32), P(C.c_double), P(C.c_uint8), C.c_int64, P(C.c_double),
                                       P(C.c_double)]),
         "mpt_primary_depth": (C.c_int32, [vp, P(Camera), P(C.c_double)]),
+        "mpt_autotune": (C.c_int32, [vp, P(Camera), C.c_int32, P(abi.AutotuneResult)]),
         "mpt_device_alloc": (vp, [C.c_size_t]),
         "mpt_device_free": (None, [vp]),
         "mpt_copy_to_device": (C.c_int32, [vp, vp, C.c_size_t]),
@@ -308,6 +309,14 @@ class Scene:
                               _dp(alb) if feats else None, _dp(nrm) if feats else None,
                               _dp(var) if feats else None, cb, None, C.byref(stats)))
         return Frame(rad, alb, nrm, var, stats)
+
+    def autotune(self, camera=None, trial_passes=8):
+        """mpt_autotune: picks the batch size and tail size for this image size."""
+        cam = abi.camera_copy(camera if camera is not None else self.camera)
+        r = abi.AutotuneResult()
+        _check(lib.mpt_autotune(self._h, C.byref(cam), trial_passes, C.byref(r)))
+        return {"passes_in_flight": r.passes_in_flight, "tail_paths": r.tail_paths, "trials": r.trials,
+                "seconds_per_pass": r.seconds_per_pass}
 
     def primary_depth(self, camera=None):
         """Depth of the first surface through every pixel centre (0 for the sky)."""

@@ -140,3 +140,9 @@ class NrcOpts(C.Structure):
 class NrcStats(C.Structure):
     """mpt_nrc_stats."""
     _fields_ = [("queries", C.c_int64), ("train_samples", C.c_int64), ("loss", C.c_double)]
+
+
+class AutotuneResult(C.Structure):
+    """mpt_autotune_result."""
+    _fields_ = [("passes_in_flight", C.c_int32), ("trials", C.c_int32), ("tail_paths", C.c_int64),
+                ("seconds_per_pass", C.c_double)]

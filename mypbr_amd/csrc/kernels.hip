@@ -130,23 +130,34 @@ __device__ __forceinline__ double power_heuristic(double a, double b) {  // came
     return (a2 + b2) > 0 ? a2 / (a2 + b2) : 0;
 }
 
-// ------------------------------------------------------------------ path state (SoA in HBM)
+// ------------------------------------------------------------------ path state in HBM
+// The double-precision fields of a path, grouped into records of 8 doubles
+// (64 bytes) by which stage reads them together. With MPT_PATH_RECORDS (the
+// default) a slot's record is contiguous (array of records), so the scattered
+// slots a queue names each cost whole 64-byte segments instead of one
+// partly used cache line per field; with 0 every field is its own array.
 enum {
-    D_OX, D_OY, D_OZ, D_DX, D_DY, D_DZ,     // current ray
-    D_BX, D_BY, D_BZ,                       // throughput beta
-    D_RX, D_RY, D_RZ,                       // path radiance
-    D_PX, D_PY, D_PZ,                       // prev_point
-    D_SPDF,                                 // scatter_pdf
-    D_HT,                                   // hit t
-    D_SOX, D_SOY, D_SOZ,                    // shadow ray origin
-    D_TLX, D_TLY, D_TLZ,                    // to_light (unnormalized)
-    D_FX, D_FY, D_FZ,                       // f of the NEE sample
-    D_SBX, D_SBY, D_SBZ,                    // beta at the NEE vertex
-    D_SW,                                   // MIS weight / light pdf
-    D_AX, D_AY, D_AZ, D_NX, D_NY, D_NZ,     // first-surface guides
-    D_QX, D_QY, D_QZ, D_MX, D_MY, D_MZ,     // hit point and face normal (hit record)
+    // record 0: the ray (extend reads it; shade rewrites it)
+    D_OX, D_OY, D_OZ, D_DX, D_DY, D_DZ, D_HT, D_SPDF,
+    // record 1: throughput, radiance, MIS weight of the pending shadow ray
+    D_BX, D_BY, D_BZ, D_RX, D_RY, D_RZ, D_SW, D_PAD1,
+    // record 2: the hit record shade reads (point, face normal)
+    D_QX, D_QY, D_QZ, D_MX, D_MY, D_MZ, D_PAD2A, D_PAD2B,
+    // record 3: previous vertex (MIS) and the shadow ray's origin
+    D_PX, D_PY, D_PZ, D_SOX, D_SOY, D_SOZ, D_PAD3A, D_PAD3B,
+    // record 4: the NEE sample (unnormalised direction to the light, f)
+    D_TLX, D_TLY, D_TLZ, D_FX, D_FY, D_FZ, D_PAD4A, D_PAD4B,
+    // record 5: throughput at the NEE vertex
+    D_SBX, D_SBY, D_SBZ, D_PAD5A, D_PAD5B, D_PAD5C, D_PAD5D, D_PAD5E,
+    // record 6: first-surface guides
+    D_AX, D_AY, D_AZ, D_NX, D_NY, D_NZ, D_PAD6A, D_PAD6B,
     D_COUNT
 };
+#ifndef MPT_PATH_RECORDS
+#define MPT_PATH_RECORDS 1
+#endif
+constexpr int kRecDoubles = 8;
+constexpr int64_t kFieldStride = MPT_PATH_RECORDS ? kRecDoubles : 1;  // doubles between slots of a field
 
 struct pstate {
     double* d[D_COUNT];
@@ -158,11 +169,26 @@ struct pstate {
     uint32_t* hmat;    // material of the hit, bit 31 = front face
 };
 
-__device__ __forceinline__ v3 ld3(const pstate& P, int f, uint32_t i) { return v3{P.d[f][i], P.d[f + 1][i], P.d[f + 2][i]}; }
+// Field f of slot i.
+__device__ __forceinline__ double& pd(const pstate& P, int f, uint32_t i) { return P.d[f][int64_t(i) * kFieldStride]; }
+__device__ __forceinline__ v3 ld3(const pstate& P, int f, uint32_t i) { return v3{pd(P, f, i), pd(P, f + 1, i), pd(P, f + 2, i)}; }
 __device__ __forceinline__ void st3(const pstate& P, int f, uint32_t i, const v3& v) {
-    P.d[f][i] = v.x;
-    P.d[f + 1][i] = v.y;
-    P.d[f + 2][i] = v.z;
+    pd(P, f, i) = v.x;
+    pd(P, f + 1, i) = v.y;
+    pd(P, f + 2, i) = v.z;
+}
+
+// A block's records r of its consecutive slots [s0, s0 + BS) staged through
+// LDS: each thread fills its record there, then the block moves them to HBM
+// as contiguous 16-byte chunks (slots beyond n are left alone).
+template <int BS>
+__device__ __forceinline__ void store_records(const pstate& P, int r, int64_t s0, int64_t n, double* lds) {
+    __syncthreads();
+    double2* dst = reinterpret_cast<double2*>(&pd(P, r * kRecDoubles, uint32_t(s0)));
+    const double2* src = reinterpret_cast<const double2*>(lds);
+    const int64_t slots = n - s0 < BS ? n - s0 : BS;
+    for (int64_t c = threadIdx.x; c < slots * (kRecDoubles / 2); c += BS) dst[c] = src[c];
+    __syncthreads();
 }
 
 struct launch_ctx {
@@ -291,16 +317,10 @@ __device__ __forceinline__ v3 random_in_unit_disk(rstream& rs) {  // vec3.h:139-
     }
 }
 
-__global__ void k_camera(dcam C, pstate P, batch_args B, uint32_t* q0, uint32_t* cnt) {
-    int64_t gt = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gt == 0) {
-        *ctr(cnt, 0) = uint32_t(B.nslots);
-        for (int k = 1; k < 3 + kClasses; k++) *ctr(cnt, k) = 0;
-        *ctr(cnt, kNrcQueue) = 0;  // the radiance cache's query and training counts
-        *ctr(cnt, kNrcTrain) = 0;
-    }
-    if (gt >= B.nslots) return;
-    uint32_t slot = uint32_t(gt);
+// The camera ray of slot `slot` (camera.h:286-302 get_ray); sets the
+// slot's random stream, flags and first-queue entry.
+__device__ __forceinline__ void camera_ray(const dcam& C, const pstate& P, const batch_args& B, uint32_t slot,
+                                           uint32_t* q0, v3& org_out, v3& dir_out) {
     int pass = B.first_pass + int(slot / B.npix);
     int pix = int(slot % B.npix);
     int i = pix % C.w, j = pix / C.w;
@@ -338,17 +358,49 @@ __global__ void k_camera(dcam C, pstate P, batch_args B, uint32_t* q0, uint32_t*
         }
         org = eye + ((p.x * C.disk_u) + (p.y * C.disk_v));
     }
-    v3 dir = ps - org;
+    org_out = org;
+    dir_out = ps - org;
+    P.key[slot] = rs.key;
+    P.ctr[slot] = rs.ctr;
+    P.flags[slot] = 1u;  // specular: camera rays see lights unweighted
+    q0[slot] = slot;
+}
+
+__global__ __launch_bounds__(256) void k_camera(dcam C, pstate P, batch_args B, uint32_t* q0, uint32_t* cnt) {
+    __shared__ double lds[256 * kRecDoubles];
+    int64_t gt = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gt == 0) {
+        *ctr(cnt, 0) = uint32_t(B.nslots);
+        for (int k = 1; k < 3 + kClasses; k++) *ctr(cnt, k) = 0;
+        *ctr(cnt, kNrcQueue) = 0;  // the radiance cache's query and training counts
+        *ctr(cnt, kNrcTrain) = 0;
+    }
+    if constexpr (MPT_PATH_RECORDS) {  // records are written block-wide (store_records); every thread takes part
+        const int64_t s0 = int64_t(blockIdx.x) * blockDim.x;
+        v3 org{0, 0, 0}, dir{0, 0, 0};
+        if (gt < B.nslots) camera_ray(C, P, B, uint32_t(gt), q0, org, dir);
+        double* my = lds + threadIdx.x * kRecDoubles;
+        // record 0: ray, hit t, scatter pdf
+        my[0] = org.x, my[1] = org.y, my[2] = org.z, my[3] = dir.x, my[4] = dir.y, my[5] = dir.z, my[6] = 0, my[7] = 0;
+        if (s0 < B.nslots) store_records<256>(P, 0, s0, B.nslots, lds);
+        // record 1: beta 1, radiance 0
+        my[0] = 1, my[1] = 1, my[2] = 1, my[3] = 0, my[4] = 0, my[5] = 0, my[6] = 0, my[7] = 0;
+        if (s0 < B.nslots) store_records<256>(P, 1, s0, B.nslots, lds);
+        // record 3: previous vertex 0
+        for (int k = 0; k < kRecDoubles; k++) my[k] = 0;
+        if (s0 < B.nslots) store_records<256>(P, 3, s0, B.nslots, lds);
+        return;
+    }
+    if (gt >= B.nslots) return;
+    const uint32_t slot = uint32_t(gt);
+    v3 org, dir;
+    camera_ray(C, P, B, slot, q0, org, dir);
     st3(P, D_OX, slot, org);
     st3(P, D_DX, slot, dir);
     st3(P, D_BX, slot, v3{1, 1, 1});
     st3(P, D_RX, slot, v3{0, 0, 0});
     st3(P, D_PX, slot, v3{0, 0, 0});
-    P.d[D_SPDF][slot] = 0;
-    P.key[slot] = rs.key;
-    P.ctr[slot] = rs.ctr;
-    P.flags[slot] = 1u;  // specular: camera rays see lights unweighted
-    q0[slot] = slot;
+    pd(P, D_SPDF, slot) = 0;
 }
 
 // camera::first_surface (camera.h:253-277) on the feature stream.
@@ -413,7 +465,7 @@ __device__ __forceinline__ int extend_one(const dscene& S, const pstate& P, int 
     const v3 o = ld3(P, D_OX, p), d = ld3(P, D_DX, p);
     hitinfo h;
     trace_closest<CNT, FEAT>(S, o, d, 0.001, kInf, h, st, true, P.key[p], uint32_t(2 * depth), wc);
-    P.d[D_HT][p] = h.t;
+    pd(P, D_HT, p) = h.t;
     P.hobj[p] = h.obj;
     P.hprim[p] = h.prim;
     if (h.obj < 0) return 0;
@@ -497,14 +549,14 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
                 rad = rad + beta * sky;
             } else {
                 double lp = lights_pdf<LK>(S, ld3(P, D_PX, p), d);
-                rad = rad + beta * sky * power_heuristic(P.d[D_SPDF][p], lp);
+                rad = rad + beta * sky * power_heuristic(pd(P, D_SPDF, p), lp);
             }
         }
         st3(P, D_RX, p, rad);
         return;
     } else {
         hitrec rec;
-        rec.t = P.d[D_HT][p];
+        rec.t = pd(P, D_HT, p);
         rec.p = ld3(P, D_QX, p);
         rec.normal = ld3(P, D_MX, p);
         const uint32_t hm = P.hmat[p];
@@ -519,7 +571,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
                     rad = rad + beta * e;
                 } else {
                     double lp = lights_pdf<LK>(S, ld3(P, D_PX, p), d);
-                    rad = rad + beta * e * power_heuristic(P.d[D_SPDF][p], lp);
+                    rad = rad + beta * e * power_heuristic(pd(P, D_SPDF, p), lp);
                 }
             }
             st3(P, D_RX, p, rad);
@@ -556,7 +608,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
                             st3(P, D_TLX, p, to_light);
                             st3(P, D_FX, p, f);
                             st3(P, D_SBX, p, beta);
-                            P.d[D_SW][p] = weight / light_pdf;
+                            pd(P, D_SW, p) = weight / light_pdf;
                             nee = true;
                         }
                     }
@@ -566,7 +618,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
                     double spdf = density_value<MT>(so.dens, dir);
                     if (spdf > 0) {
                         beta = beta * mat_eval<MT>(S, m, d, rec, dir) / spdf;
-                        P.d[D_SPDF][p] = spdf;
+                        pd(P, D_SPDF, p) = spdf;
                         st3(P, D_PX, p, rec.p);
                         o = rec.p;
                         d = dir;
@@ -655,7 +707,7 @@ __device__ __forceinline__ void shadow_one(const dscene& S, const dcam& C, const
             break;
         }
         emission = through * emission;
-        v3 c = ld3(P, D_FX, p) * emission * P.d[D_SW][p];
+        v3 c = ld3(P, D_FX, p) * emission * pd(P, D_SW, p);
         v3 rad = ld3(P, D_RX, p) + ld3(P, D_SBX, p) * c;
         st3(P, D_RX, p, rad);
     }
@@ -939,6 +991,10 @@ struct mpt_scene {
     int last_done = 0;
     mpt_camera last_cam{};
     hipStream_t stream = nullptr;
+    // mpt_autotune's choices for one image size (0: none)
+    int64_t tuned_pix = 0;
+    int tuned_pif = 0;
+    int64_t tuned_tail = 0;
     // radiance cache buffers (mpt_render_nrc)
     std::vector<void*> nrc_allocs;
     int64_t nrc_capacity = 0;
@@ -1310,8 +1366,11 @@ int ensure_paths(mpt_scene* s, int64_t cap) {
         return MPT_OK;
     };
     int rc;
-    for (int k = 0; k < D_COUNT; k++)
-        if ((rc = alloc(size_t(cap) * sizeof(double), reinterpret_cast<void**>(&s->ps.d[k])))) return rc;
+    double* base = nullptr;
+    if ((rc = alloc(size_t(cap) * D_COUNT * sizeof(double), reinterpret_cast<void**>(&base)))) return rc;
+    for (int f = 0; f < D_COUNT; f++)  // record r of slot i at base + (r * cap + i) * 8, or field f at base + f * cap
+        s->ps.d[f] = MPT_PATH_RECORDS ? base + (int64_t(f / kRecDoubles) * cap) * kRecDoubles + f % kRecDoubles
+                                      : base + int64_t(f) * cap;
     if ((rc = alloc(size_t(cap) * sizeof(uint64_t), reinterpret_cast<void**>(&s->ps.key)))) return rc;
     if ((rc = alloc(size_t(cap) * sizeof(uint32_t), reinterpret_cast<void**>(&s->ps.ctr)))) return rc;
     if ((rc = alloc(size_t(cap) * sizeof(uint32_t), reinterpret_cast<void**>(&s->ps.flags)))) return rc;
@@ -1412,6 +1471,7 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
     const int total = o.pass_count > 0 ? o.pass_count : C.spp - o.first_pass;
     if (total < 1) return set_error(MPT_E_INVALID, "no passes to render (first_pass >= samples_per_pixel)");
     int pif = o.passes_in_flight;
+    if (pif <= 0 && s->tuned_pix == npix64 && s->tuned_pif > 0) pif = s->tuned_pif;  // mpt_autotune's choice
     if (pif <= 0) {
         // about 16M paths in flight: many times what the device holds at once,
         // so the tail of each bounce (few paths left) is a small share
@@ -1618,7 +1678,9 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
 
     // the tail of a batch runs as one k_finish launch once at most `tail`
     // paths are left; by default two paths per thread of the grid
-    const int64_t tail = o.tail_paths < 0 ? 0 : (o.tail_paths > 0 ? o.tail_paths : 2 * L.nthreads);
+    int64_t tail_default = 2 * L.nthreads;
+    if (s->tuned_pix == npix64 && s->tuned_tail != 0) tail_default = s->tuned_tail < 0 ? 0 : s->tuned_tail;
+    const int64_t tail = o.tail_paths < 0 ? 0 : (o.tail_paths > 0 ? o.tail_paths : tail_default);
     auto launch_finish = [&](int depth, uint32_t* qin, uint32_t* cin) {
 #define FIN2(FEAT, LK)                                                                                        \
     do {                                                                                                      \
@@ -1793,6 +1855,56 @@ int32_t mpt_render_nrc(mpt_scene* s, const mpt_camera* cam, const mpt_render_opt
     if (o.query_depth < 1 || o.train_stride < 1 || o.train_steps < 0 || !(o.lr >= 0))
         return set_error(MPT_E_INVALID, "nrc: query_depth >= 1, train_stride >= 1, train_steps >= 0, lr >= 0");
     return render_impl(s, cam, opts, radiance, albedo, normal, variance, on_pass, user, stats, nrc, &o, nstats);
+}
+
+int32_t mpt_autotune(mpt_scene* s, const mpt_camera* cam, int32_t trial_passes, mpt_autotune_result* out) {
+    if (!s || !cam || !out) return set_error(MPT_E_INVALID, "null argument");
+    if (trial_passes < 1) trial_passes = 8;
+    const dcam C = init_camera(*cam, s->hs.env_w > 0);
+    const int64_t npix = int64_t(C.w) * C.h;
+    const int64_t nthreads = int64_t(s->grid) * s->block;
+    // candidates: paths in flight (passes per batch) around 4, 8, 16, 32 M
+    // paths, and the batch tail's size in paths per thread of the grid
+    std::vector<int> pifs;
+    for (int64_t target : {int64_t(4) << 20, int64_t(8) << 20, int64_t(16) << 20, int64_t(32) << 20}) {
+        int v = int(std::max<int64_t>(1, target / npix));
+        v = std::min(v, trial_passes);
+        if (int64_t(v) * npix > (int64_t(1) << 31)) continue;
+        if (std::find(pifs.begin(), pifs.end(), v) == pifs.end()) pifs.push_back(v);
+    }
+    const int64_t tails[] = {-1, nthreads / 2, nthreads, 2 * nthreads, 4 * nthreads};
+    mpt_render_opts o{};
+    o.seed = 0x5EED;
+    o.pass_count = trial_passes;
+    s->tuned_pix = 0;
+    double best = 1e300;
+    int best_pif = 0;
+    int64_t best_tail = 0;
+    out->trials = 0;
+    // a warm-up render sizes the buffers for the largest batch
+    o.passes_in_flight = *std::max_element(pifs.begin(), pifs.end());
+    int rc = render_impl(s, cam, &o, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                         nullptr);
+    if (rc) return rc;
+    for (int pif : pifs)
+        for (int64_t tail : tails) {
+            o.passes_in_flight = pif;
+            o.tail_paths = tail < 0 ? -1 : int32_t(std::min<int64_t>(tail, INT32_MAX));
+            auto t0 = std::chrono::steady_clock::now();
+            rc = render_impl(s, cam, &o, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                             nullptr, nullptr);
+            if (rc) return rc;
+            const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            out->trials++;
+            if (dt < best) best = dt, best_pif = pif, best_tail = tail;
+        }
+    s->tuned_pix = npix;
+    s->tuned_pif = best_pif;
+    s->tuned_tail = best_tail;
+    out->passes_in_flight = best_pif;
+    out->tail_paths = best_tail < 0 ? -1 : int64_t(best_tail);
+    out->seconds_per_pass = best / trial_passes;
+    return MPT_OK;
 }
 
 }  // extern "C"

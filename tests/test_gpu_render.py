@@ -83,3 +83,16 @@ def test_batch_tail_in_one_launch_is_identical(mpt, name, width, spp, depth):
         assert np.array_equal(f.radiance, staged.radiance)
         assert f.stats.extension_rays == staged.stats.extension_rays
         assert f.stats.shadow_rays == staged.stats.shadow_rays
+
+
+def test_autotune_picks_settings_that_leave_the_image_unchanged(mpt):
+    from mypbr_amd.abi import camera_copy
+    d = mpt.describe(scene_path("weekend_cover.scene"))
+    s = mpt.Scene.from_desc(d.desc, d.camera)
+    cam = camera_copy(d.camera, image_width=160, samples_per_pixel=12, max_depth=20)
+    r = s.autotune(cam, trial_passes=4)
+    print(r, flush=True)
+    assert r["passes_in_flight"] >= 1 and r["trials"] >= 5 and r["seconds_per_pass"] > 0
+    tuned = s.render(cam, seed=5).radiance
+    other = s.render(cam, seed=5, passes_in_flight=5, tail_paths=-1).radiance
+    assert np.array_equal(tuned, other)
