@@ -350,6 +350,67 @@ int32_t mpt_denoise_dev(const double* radiance, const double* albedo, const doub
                         int32_t width, int32_t height, const mpt_denoise_opts* opts, double* out, void* workspace,
                         void* stream);
 
+/* ---- display pipeline: postprocess.h, tonemap.h, color3.h, image_io.h ----
+ * How radiance becomes a display image: bloom and vignetting
+ * (postprocess(), postprocess.h:108-153), exposure and a tonemapper
+ * (display_settings::apply, tonemap.h:28-37), the sRGB curve and byte
+ * quantisation (to_byte, color3.h:30-33), and the writers of image_io.h
+ * (.png/.ppm display images, .hdr/.pfm raw radiance). */
+
+enum {
+    MPT_TONEMAP_NONE = 0,     /* clamp (tonemap.h:10) */
+    MPT_TONEMAP_REINHARD = 1, /* x / (1 + x) per channel */
+    MPT_TONEMAP_ACES = 2      /* Hill's fit of the ACES RRT + ODT */
+};
+
+/* display_settings + post_settings (tonemap.h:11-15, postprocess.h:12-21). */
+typedef struct mpt_display {
+    double exposure;        /* stops: display images are scaled by 2^exposure (0) */
+    int32_t tonemap;        /* MPT_TONEMAP_* (NONE) */
+    int32_t pad;
+    double bloom;           /* glow strength, 0 = off (0) */
+    double bloom_threshold; /* luminance above which pixels glow (1) */
+    double bloom_radius;    /* reach of the glow as a fraction of the width (0.04) */
+    double vignette;        /* corner darkening, 0 = off (0) */
+} mpt_display;
+
+/* Fills the reference's defaults. */
+void mpt_display_defaults(mpt_display* display);
+
+/* The display settings a .scene file sets (its bloom, vignette, exposure
+ * and tonemap lines, scene.h:667-688); the defaults where it sets none. */
+int32_t mpt_scene_display(const char* path, mpt_display* display);
+
+/* postprocess(pixels, width, height, post) (postprocess.h:108-153): bloom
+ * and vignetting on linear radiance (interleaved RGB doubles, row-major
+ * from the top-left, caller-owned host memory) into out. */
+int32_t mpt_postprocess(const double* radiance, int32_t width, int32_t height, const mpt_display* display,
+                        double* out);
+
+/* The bytes write_png and write_ppm emit (image_io.h:16-37): with post set
+ * and bloom or vignette on, the postprocessed image, then per channel
+ * to_byte(display.apply(pixel)); out holds width * height * 3 bytes. */
+int32_t mpt_display_bytes(const double* radiance, int32_t width, int32_t height, const mpt_display* display,
+                          int32_t post, uint8_t* rgb);
+
+/* The same two on device buffers, enqueued on a HIP stream (NULL = the null
+ * stream) without synchronising; workspace is device memory of
+ * mpt_display_workspace_bytes(width, height, display) bytes (0 when bloom
+ * is off; then workspace may be NULL). */
+size_t mpt_display_workspace_bytes(int32_t width, int32_t height, const mpt_display* display);
+int32_t mpt_postprocess_dev(const double* radiance, int32_t width, int32_t height, const mpt_display* display,
+                            double* out, void* workspace, void* stream);
+int32_t mpt_display_bytes_dev(const double* radiance, int32_t width, int32_t height, const mpt_display* display,
+                              int32_t post, uint8_t* rgb, void* workspace, void* stream);
+
+/* write_image(path, width, height, pixels, display) (image_io.h:81-106): the
+ * format follows the extension. .png and .ppm are display images (bloom,
+ * vignette, exposure, tonemap, sRGB bytes from the GPU); .hdr (Radiance
+ * RGBE) and .pfm (little-endian floats, bottom row first) keep the raw
+ * radiance. NULL path: PPM to stdout. */
+int32_t mpt_write_image(const char* path, int32_t width, int32_t height, const double* radiance,
+                        const mpt_display* display);
+
 /* ---- learned denoiser: kernel-predicting CNN on BF16 matrix cores ----
  * No counterpart in the reference (it has only the classical filter above);
  * the learned mode the path tracer is compared against. */

@@ -19,6 +19,9 @@
 //   frame                  denoiser.h:14-20    mpt::frame
 //   denoiser               denoiser.h:38-193   mpt::denoiser -> mpt_denoise (include/mpt.h)
 //   scene / load_scene     scene.h:32-244      mpt::scene / mpt::load_scene
+//   display_settings,      tonemap.h:11-37,    mpt::display_settings, mpt::post_settings,
+//   post_settings,         postprocess.h       mpt::postprocess -> mpt_postprocess,
+//   postprocess(), write_image()  image_io.h   mpt::write_image -> mpt_write_image
 //
 // Objects are plain descriptions; camera::render flattens world and lights
 // into the C ABI's structure-of-arrays description, uploads it once per
@@ -349,10 +352,45 @@ private:
     mpt_scene* handle = nullptr;
 };
 
-struct scene {  // scene.h:32-40 (display settings are parsed and ignored)
+// post_settings (postprocess.h:12-21): bloom and vignetting of display images.
+class post_settings {
+public:
+    double bloom = 0;            // glow strength (0 = off)
+    double bloom_threshold = 1;  // luminance above which pixels glow
+    double bloom_radius = 0.04;  // reach of the glow, as a fraction of the width
+    double vignette = 0;         // corner darkening (0 = off)
+
+    bool active() const { return bloom > 0 || vignette > 0; }
+};
+
+enum class tonemapper { none, reinhard, aces };  // tonemap.h:10
+
+// display_settings (tonemap.h:11-37): how radiance becomes PNG/PPM pixels.
+class display_settings {
+public:
+    double exposure = 0;              // stops: radiance is scaled by 2^exposure
+    tonemapper op = tonemapper::none; // none clamps
+    post_settings post;
+
+    // Parses "none", "reinhard" or "aces"; returns false if unknown.
+    bool set_tonemapper(const std::string& name);
+    mpt_display to_abi() const;
+};
+
+// postprocess(pixels, width, height, post) (postprocess.h:108-153) on the GPU.
+std::vector<color> postprocess(const std::vector<color>& pixels, int width, int height, const post_settings& post);
+
+// write_image(path, width, height, pixels, display) (image_io.h:81-106):
+// .png/.ppm display images (bytes from the GPU), .hdr/.pfm raw radiance; an
+// empty path writes PPM to stdout. Returns false and prints the error.
+bool write_image(const std::string& path, int width, int height, const std::vector<color>& pixels,
+                 const display_settings& display);
+
+struct scene {  // scene.h:32-40
     hittable_list world;
     light_list lights;
     camera cam;
+    display_settings display;  // how radiance becomes PNG/PPM pixels
 };
 
 // Loads a .scene file (scene.h:42-243) for the subset the GPU path models.

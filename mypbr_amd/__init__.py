@@ -94,6 +94,15 @@ def _load():
                                                            P(C.c_double)]),
         "mpt_denoise_workspace_bytes": (C.c_size_t, [C.c_int32, C.c_int32]),
         "mpt_denoise_dev": (C.c_int32, [vp] * 4 + [C.c_int32, C.c_int32, P(abi.DenoiseOpts), vp, vp, vp]),
+        "mpt_display_defaults": (None, [P(abi.Display)]),
+        "mpt_scene_display": (C.c_int32, [C.c_char_p, P(abi.Display)]),
+        "mpt_postprocess": (C.c_int32, [P(C.c_double), C.c_int32, C.c_int32, P(abi.Display), P(C.c_double)]),
+        "mpt_display_bytes": (C.c_int32, [P(C.c_double), C.c_int32, C.c_int32, P(abi.Display), C.c_int32,
+                                          P(C.c_uint8)]),
+        "mpt_display_workspace_bytes": (C.c_size_t, [C.c_int32, C.c_int32, P(abi.Display)]),
+        "mpt_postprocess_dev": (C.c_int32, [vp, C.c_int32, C.c_int32, P(abi.Display), vp, vp, vp]),
+        "mpt_display_bytes_dev": (C.c_int32, [vp, C.c_int32, C.c_int32, P(abi.Display), C.c_int32, vp, vp, vp]),
+        "mpt_write_image": (C.c_int32, [C.c_char_p, C.c_int32, C.c_int32, P(C.c_double), P(abi.Display)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -183,6 +192,58 @@ def denoise(frame, iterations=5, sigma_luminance=4.0, sigma_normal=0.1, sigma_al
     out = np.zeros((h, w, 3))
     _check(lib.mpt_denoise(*[_dp(a) for a in arrs], w, h, C.byref(o), _dp(out)))
     return out
+
+
+def display_settings(exposure=0.0, tonemap="none", bloom=0.0, bloom_threshold=1.0, bloom_radius=0.04, vignette=0.0):
+    """display_settings with its post_settings (tonemap.h:11-37,
+    postprocess.h:12-21) as an mpt_display."""
+    d = abi.Display()
+    lib.mpt_display_defaults(C.byref(d))
+    if tonemap not in abi.TONEMAPPERS:
+        raise ValueError("tonemap must be none, reinhard or aces")
+    d.exposure, d.tonemap = exposure, abi.TONEMAPPERS[tonemap]
+    d.bloom, d.bloom_threshold, d.bloom_radius, d.vignette = bloom, bloom_threshold, bloom_radius, vignette
+    return d
+
+
+def scene_display(path):
+    """The display settings a .scene file sets (scene.h:667-688)."""
+    d = abi.Display()
+    _check(lib.mpt_scene_display(os.fsencode(path), C.byref(d)))
+    return d
+
+
+def _image(radiance):
+    a = np.ascontiguousarray(radiance, dtype=np.float64)
+    if a.ndim != 3 or a.shape[2] != 3:
+        raise ValueError("an image is an (h, w, 3) array of linear radiance")
+    return a
+
+
+def postprocess(radiance, display):
+    """postprocess(pixels, width, height, post) (postprocess.h:108-153) on the
+    GPU: bloom and vignetting of an (h, w, 3) radiance image."""
+    a = _image(radiance)
+    out = np.zeros_like(a)
+    _check(lib.mpt_postprocess(_dp(a), a.shape[1], a.shape[0], C.byref(display), _dp(out)))
+    return out
+
+
+def display_bytes(radiance, display, post=True):
+    """The sRGB bytes write_png writes (image_io.h:25-37): postprocess (when
+    post and bloom/vignette are on), exposure, tonemap, to_byte. (h, w, 3) uint8."""
+    a = _image(radiance)
+    out = np.zeros(a.shape, dtype=np.uint8)
+    _check(lib.mpt_display_bytes(_dp(a), a.shape[1], a.shape[0], C.byref(display), int(post),
+                                 out.ctypes.data_as(C.POINTER(C.c_uint8))))
+    return out
+
+
+def write_image(path, radiance, display=None):
+    """write_image (image_io.h:81-106): .png/.ppm display images, .hdr/.pfm raw radiance."""
+    a = _image(radiance)
+    d = display if display is not None else display_settings()
+    _check(lib.mpt_write_image(os.fsencode(path) if path else None, a.shape[1], a.shape[0], _dp(a), C.byref(d)))
 
 
 class Description:

@@ -130,6 +130,16 @@ class DenoiseOpts(C.Structure):
                 ("sigma_normal", C.c_double), ("sigma_albedo", C.c_double)]
 
 
+TONEMAPPERS = {"none": 0, "reinhard": 1, "aces": 2}
+
+
+class Display(C.Structure):
+    """mpt_display: display_settings + post_settings (tonemap.h:11-15,
+    postprocess.h:12-21)."""
+    _fields_ = [("exposure", C.c_double), ("tonemap", C.c_int32), ("pad", C.c_int32), ("bloom", C.c_double),
+                ("bloom_threshold", C.c_double), ("bloom_radius", C.c_double), ("vignette", C.c_double)]
+
+
 class NrcOpts(C.Structure):
     """mpt_nrc_opts."""
     _fields_ = [("query_depth", C.c_int32), ("train_stride", C.c_int32), ("train_steps", C.c_int32),

@@ -26,8 +26,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"] + os.environ.get("MPT_DEVICE_FLAGS", "").split()
 
-HOST_SOURCES = ["scene_host.cpp", "scene_api.cpp"]
-DEVICE_SOURCES = ["kernels.hip", "denoise.hip", "learn.hip", "nrc.hip"]
+HOST_SOURCES = ["scene_host.cpp", "scene_api.cpp", "image_io.cpp"]
+DEVICE_SOURCES = ["kernels.hip", "denoise.hip", "learn.hip", "nrc.hip", "display.hip"]
 
 
 def run(cmd):
@@ -52,7 +52,7 @@ def build(verbose=True):
             f.result()
     objs = [obj for obj, _ in jobs]
     tmp = LIB + ".tmp"
-    run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, *objs, "-o", tmp, "-lpthread"])
+    run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, *objs, "-o", tmp, "-lpthread", "-lz"])
     os.replace(tmp, LIB)
     return LIB
 

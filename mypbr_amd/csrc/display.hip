@@ -1,0 +1,444 @@
+// display.hip - the display pipeline on the GPU: bloom, vignetting,
+// exposure, tonemapping and sRGB bytes.
+//
+// postprocess() (postprocess.h:108-153) keeps the part of each pixel above
+// the luminance threshold, builds a pyramid of half-size images of it
+// (downsample, postprocess.h:47-59), blurs every level with a separable
+// Gaussian of sigma 1.5 (postprocess.h:62-89) and adds back the average of
+// the blurred levels, sampled bilinearly at each full-size pixel, times the
+// bloom strength; vignetting divides by (1 + k r^2)^2. display_settings::
+// apply (tonemap.h:28-37) then scales by 2^exposure and tonemaps, and
+// to_byte (color3.h:30-33) encodes with the sRGB curve and quantises.
+//
+// Kernels, all in double precision in the reference's operation order:
+//   k_level     one launch per pyramid level: a block makes a 32 x 16 tile
+//               of the blurred level in LDS from the level above (for the
+//               first level, from the radiance with the threshold applied
+//               on the fly): the downsampled window with a 5-pixel halo,
+//               the horizontal pass, the vertical pass. It also stores the
+//               unblurred tile, the source of the next level.
+//   k_compose   one thread per full-size pixel: the glow from every level,
+//               vignetting, and either the postprocessed radiance or, fused,
+//               exposure, tonemap and the sRGB byte.
+// The byte is found by comparing with the 255 radiance thresholds at which
+// to_byte steps up, computed once on the host from the same formula and the
+// host's libm (the one the reference's own writer calls), so the bytes are
+// the reference's even where a GPU pow would round differently.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mpt.h"
+#include "geom.hpp"
+
+namespace mpt {
+
+int set_error(int code, const std::string& m);
+
+namespace {
+
+constexpr int kMaxLevels = 10;  // postprocess.h:124
+constexpr int kTaps = 11;       // blur radius ceil(3 * 1.5) = 5 (postprocess.h:63)
+constexpr int kR = 5;
+constexpr int kTX = 32, kTY = 16;  // k_level tile
+constexpr int kWX = kTX + 2 * kR, kWY = kTY + 2 * kR;
+
+struct level_desc {
+    double* level;    // unblurred (the next level's source); may be null for the last level
+    double* blurred;
+    int32_t w, h;
+};
+
+struct pyramid {
+    level_desc lv[kMaxLevels];
+    double sx[kMaxLevels], sy[kMaxLevels];  // double(b.w) / width, double(b.h) / height
+    int32_t count;
+};
+
+struct blur_weights {
+    double w[kTaps];
+};
+
+// 0 < T[1] <= ... <= T[255]: the smallest radiance to_byte maps to k or above.
+struct byte_thresholds {
+    double t[256];
+};
+
+struct display_args {
+    double scale;       // 2^exposure (std::pow on the host, as tonemap.h:29)
+    int32_t tonemap;
+    int32_t vignette_on;
+    double vignette;
+    double cx, cy, corner2;
+    double bloom_over_levels;  // post.bloom / blurred.size()
+};
+
+__device__ __forceinline__ v3 ld(const double* b, int64_t k) { return v3{b[3 * k], b[3 * k + 1], b[3 * k + 2]}; }
+__device__ __forceinline__ void st(double* b, int64_t k, const v3& v) {
+    b[3 * k] = v.x;
+    b[3 * k + 1] = v.y;
+    b[3 * k + 2] = v.z;
+}
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// The part of a pixel above the threshold (postprocess.h:113-119).
+__device__ __forceinline__ v3 bright(const v3& p, double threshold) {
+    const double l = luminance(p);
+    return (__builtin_isfinite(l) && l > threshold) ? p * ((l - threshold) / l) : v3{0, 0, 0};
+}
+
+// One pyramid level. src is the level above (sw x sh); bright_src: it is the
+// radiance and the threshold applies to every pixel read.
+__global__ __launch_bounds__(256) void k_level(const double* src, int sw, int sh, int bright_src, double threshold,
+                                               level_desc out, blur_weights bw) {
+    __shared__ double dwin[kWY][kWX][3];   // downsampled window, clamped coordinates
+    __shared__ double hwin[kWY][kTX][3];   // after the horizontal pass
+    const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY;
+    const int w = out.w, h = out.h;
+    auto src_at = [&](int i, int j) {  // image::at (postprocess.h:28-32)
+        const v3 p = ld(src, int64_t(clampi(j, 0, sh - 1)) * sw + clampi(i, 0, sw - 1));
+        return bright_src ? bright(p, threshold) : p;
+    };
+    for (int k = threadIdx.x; k < kWX * kWY; k += blockDim.x) {
+        const int c = k % kWX, r = k / kWX;
+        const int i = clampi(x0 - kR + c, 0, w - 1), j = clampi(y0 - kR + r, 0, h - 1);
+        // downsample (postprocess.h:53-56)
+        const v3 d = 0.25 * (src_at(2 * i, 2 * j) + src_at(2 * i + 1, 2 * j) + src_at(2 * i, 2 * j + 1) +
+                             src_at(2 * i + 1, 2 * j + 1));
+        dwin[r][c][0] = d.x, dwin[r][c][1] = d.y, dwin[r][c][2] = d.z;
+        // the unblurred level, for the next one
+        if (out.level && c >= kR && c < kR + kTX && r >= kR && r < kR + kTY) {
+            const int gi = x0 - kR + c, gj = y0 - kR + r;
+            if (gi < w && gj < h) st(out.level, int64_t(gj) * w + gi, d);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kTX * kWY; k += blockDim.x) {  // postprocess.h:72-79
+        const int c = k % kTX, r = k / kTX;
+        v3 s{0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < kTaps; t++) {
+            const v3 v{dwin[r][c + t][0], dwin[r][c + t][1], dwin[r][c + t][2]};
+            s = s + bw.w[t] * v;
+        }
+        hwin[r][c][0] = s.x, hwin[r][c][1] = s.y, hwin[r][c][2] = s.z;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kTX * kTY; k += blockDim.x) {  // postprocess.h:80-87
+        const int c = k % kTX, r = k / kTX;
+        const int gi = x0 + c, gj = y0 + r;
+        if (gi >= w || gj >= h) continue;
+        v3 s{0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < kTaps; t++) {
+            // tmp.at(i, j + t): rows clamp to the level, as the window's rows do
+            const int rr = clampi(gj - kR + t, 0, h - 1) - (y0 - kR);
+            const v3 v{hwin[rr][c][0], hwin[rr][c][1], hwin[rr][c][2]};
+            s = s + bw.w[t] * v;
+        }
+        st(out.blurred, int64_t(gj) * w + gi, s);
+    }
+}
+
+// image::sample (postprocess.h:35-42): bilinear, pixel centres at k + 0.5.
+__device__ __forceinline__ v3 sample(const double* img, int w, int h, double x, double y) {
+    x -= 0.5;
+    y -= 0.5;
+    const int i = int(std::floor(x)), j = int(std::floor(y));
+    const double fx = x - i, fy = y - j;
+    auto at = [&](int a, int b) { return ld(img, int64_t(clampi(b, 0, h - 1)) * w + clampi(a, 0, w - 1)); };
+    return (1 - fy) * ((1 - fx) * at(i, j) + fx * at(i + 1, j)) + fy * ((1 - fx) * at(i, j + 1) + fx * at(i + 1, j + 1));
+}
+
+__device__ __forceinline__ double rrt_odt_fit(double x) {  // tonemap.h:63-67
+    const double a = x * (x + 0.0245786) - 0.000090537;
+    const double b = x * (0.983729 * x + 0.4329510) + 0.238081;
+    return a / b;
+}
+__device__ __forceinline__ double clamp01(double x) { return x < 0 ? 0 : (x > 1 ? 1 : x); }
+
+// display_settings::apply (tonemap.h:28-37) after the exposure scale.
+__device__ __forceinline__ v3 tonemap(v3 c, int op) {
+    if (op == MPT_TONEMAP_REINHARD) return v3{c.x / (1 + c.x), c.y / (1 + c.y), c.z / (1 + c.z)};
+    if (op == MPT_TONEMAP_ACES) {  // tonemap.h:46-57
+        v3 v{0.59719 * c.x + 0.35458 * c.y + 0.04823 * c.z, 0.07600 * c.x + 0.90834 * c.y + 0.01566 * c.z,
+             0.02840 * c.x + 0.13383 * c.y + 0.83777 * c.z};
+        v = v3{rrt_odt_fit(v.x), rrt_odt_fit(v.y), rrt_odt_fit(v.z)};
+        v = v3{1.60475 * v.x + -0.53108 * v.y + -0.07367 * v.z, -0.10208 * v.x + 1.10813 * v.y + -0.00605 * v.z,
+               -0.00327 * v.x + -0.07276 * v.y + 1.07602 * v.z};
+        return v3{clamp01(v.x), clamp01(v.y), clamp01(v.z)};
+    }
+    return c;
+}
+
+// to_byte by the thresholds: the number of T[k] (k = 1..255) at or below x.
+// A NaN compares false everywhere and gives 0, as the reference's cast does
+// on x86-64.
+__device__ __forceinline__ uint32_t to_byte(const double* t, double x) {
+    uint32_t b = 0;
+#pragma unroll
+    for (uint32_t step = 128; step >= 1; step >>= 1)
+        if (t[b + step] <= x) b += step;
+    return b;
+}
+
+template <bool BYTES>
+__global__ __launch_bounds__(256) void k_compose(const double* rad, int width, int height, pyramid py,
+                                                 display_args a, byte_thresholds bt, double* out, uint8_t* rgb) {
+    __shared__ double thr[256];
+    if constexpr (BYTES) {
+        thr[threadIdx.x] = bt.t[threadIdx.x];
+        __syncthreads();
+    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= width) return;
+    const int64_t p = int64_t(j) * width + i;
+    v3 c = ld(rad, p);
+    if (py.count > 0) {  // postprocess.h:128-136
+        v3 glow{0, 0, 0};
+        for (int k = 0; k < py.count; k++)
+            glow = glow + sample(py.lv[k].blurred, py.lv[k].w, py.lv[k].h, (i + 0.5) * py.sx[k], (j + 0.5) * py.sy[k]);
+        c = c + a.bloom_over_levels * glow;
+    }
+    if (a.vignette_on) {  // postprocess.h:139-150
+        const double dx = i + 0.5 - a.cx, dy = j + 0.5 - a.cy;
+        const double f = 1 + a.vignette * (dx * dx + dy * dy) / a.corner2;
+        c = c / (f * f);
+    }
+    if constexpr (BYTES) {
+        c = tonemap(a.scale * c, a.tonemap);
+        rgb[3 * p] = uint8_t(to_byte(thr, c.x));
+        rgb[3 * p + 1] = uint8_t(to_byte(thr, c.y));
+        rgb[3 * p + 2] = uint8_t(to_byte(thr, c.z));
+    } else {
+        st(out, p, c);
+    }
+}
+
+// ---------------------------------------------------------------- host side
+
+// to_byte (color3.h:17-33) on the host, for the thresholds.
+uint8_t host_to_byte(double x) {
+    double s;
+    if (x <= 0) s = 0;
+    else if (x <= 0.0031308) s = 12.92 * x;
+    else s = 1.055 * std::pow(x, 1.0 / 2.4) - 0.055;
+    if (s < 0.000) s = 0.000;  // interval(0.000, 0.999).clamp (interval.h:33-37)
+    else if (s > 0.999) s = 0.999;
+    return uint8_t(256 * s);
+}
+
+double from_bits(uint64_t b) {
+    double d;
+    std::memcpy(&d, &b, sizeof d);
+    return d;
+}
+
+const byte_thresholds& thresholds() {
+    static byte_thresholds t;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        t.t[0] = -kInf;
+        const uint64_t inf_bits = 0x7FF0000000000000ull;  // positive doubles order as their bits
+        for (int k = 1; k < 256; k++) {
+            uint64_t lo = 0, hi = inf_bits;  // host_to_byte(lo) < k <= host_to_byte(hi)
+            while (hi - lo > 1) {
+                const uint64_t mid = lo + (hi - lo) / 2;
+                if (host_to_byte(from_bits(mid)) >= k) hi = mid;
+                else lo = mid;
+            }
+            t.t[k] = from_bits(hi);
+        }
+    });
+    return t;
+}
+
+blur_weights gaussian_weights() {  // blur(), postprocess.h:62-69, sigma 1.5
+    const double sigma = 1.5;
+    blur_weights bw;
+    double sum = 0;
+    for (int k = -kR; k <= kR; k++) sum += bw.w[k + kR] = std::exp(-0.5 * k * k / (sigma * sigma));
+    for (double& k : bw.w) k /= sum;
+    return bw;
+}
+
+// The pyramid's level sizes (postprocess.h:122-130).
+std::vector<std::pair<int, int>> level_sizes(int width, int height, const mpt_display& d) {
+    std::vector<std::pair<int, int>> out;
+    if (!(d.bloom > 0)) return out;
+    const double reach = std::fmax(d.bloom_radius * width, 1.0);
+    const int levels = std::clamp(int(std::ceil(std::log2(reach))), 1, kMaxLevels);
+    int w = width, h = height;
+    for (int k = 0; k < levels && w > 1 && h > 1; k++) {
+        w = std::max(1, w / 2);
+        h = std::max(1, h / 2);
+        out.push_back({w, h});
+    }
+    return out;
+}
+
+int check(int width, int height, const mpt_display* d) {
+    if (width <= 0 || height <= 0) return set_error(MPT_E_INVALID, "display: the image is empty");
+    if (!d) return MPT_OK;
+    if (d->tonemap < MPT_TONEMAP_NONE || d->tonemap > MPT_TONEMAP_ACES)
+        return set_error(MPT_E_INVALID, "display: unknown tonemapper");
+    if (!(d->bloom >= 0) || !(d->bloom_threshold >= 0) || !(d->bloom_radius > 0) || !(d->vignette >= 0))
+        return set_error(MPT_E_INVALID, "display: bloom and vignette need bloom >= 0, threshold >= 0, radius > 0, "
+                                        "vignette >= 0 (scene.h:667-681)");
+    return MPT_OK;
+}
+
+#define DP_TRY(x)                                                                                        \
+    do {                                                                                                 \
+        hipError_t e_ = (x);                                                                             \
+        if (e_ != hipSuccess) return set_error(MPT_E_HIP, std::string("display: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// The whole pipeline on device buffers. post: apply bloom/vignette.
+int run(const double* rad, int width, int height, const mpt_display* display, bool post, double* out, uint8_t* rgb,
+        void* workspace, hipStream_t s) {
+    mpt_display d;
+    if (display) d = *display;
+    else mpt_display_defaults(&d);
+    pyramid py{};
+    display_args a{};
+    a.scale = std::pow(2.0, d.exposure);
+    a.tonemap = d.tonemap;
+    if (post) {
+        const auto sizes = level_sizes(width, height, d);
+        py.count = int32_t(sizes.size());
+        double* ws = static_cast<double*>(workspace);
+        if (py.count && !ws) return set_error(MPT_E_INVALID, "display: bloom needs a workspace");
+        for (int k = 0; k < py.count; k++) {
+            const int64_t n = int64_t(sizes[k].first) * sizes[k].second;
+            py.lv[k].w = sizes[k].first;
+            py.lv[k].h = sizes[k].second;
+            py.lv[k].blurred = ws;
+            ws += 3 * n;
+            py.lv[k].level = nullptr;
+            if (k + 1 < py.count) {
+                py.lv[k].level = ws;
+                ws += 3 * n;
+            }
+            py.sx[k] = double(py.lv[k].w) / width;
+            py.sy[k] = double(py.lv[k].h) / height;
+        }
+        if (py.count) a.bloom_over_levels = d.bloom / double(py.count);
+        a.vignette_on = d.vignette > 0;
+        a.vignette = d.vignette;
+        a.cx = 0.5 * width;
+        a.cy = 0.5 * height;
+        a.corner2 = a.cx * a.cx + a.cy * a.cy;
+        const blur_weights bw = gaussian_weights();
+        const double* src = rad;
+        int sw = width, sh = height;
+        for (int k = 0; k < py.count; k++) {
+            const dim3 g(unsigned((py.lv[k].w + kTX - 1) / kTX), unsigned((py.lv[k].h + kTY - 1) / kTY));
+            hipLaunchKernelGGL(k_level, g, dim3(256), 0, s, src, sw, sh, int(k == 0), d.bloom_threshold, py.lv[k], bw);
+            src = py.lv[k].level;
+            sw = py.lv[k].w;
+            sh = py.lv[k].h;
+        }
+    }
+    const dim3 g(unsigned((width + 255) / 256), unsigned(height));
+    if (rgb) hipLaunchKernelGGL(k_compose<true>, g, dim3(256), 0, s, rad, width, height, py, a, thresholds(), out, rgb);
+    else hipLaunchKernelGGL(k_compose<false>, g, dim3(256), 0, s, rad, width, height, py, a, byte_thresholds{}, out, rgb);
+    DP_TRY(hipGetLastError());
+    return MPT_OK;
+}
+
+// Host buffers in and out around run().
+int run_host(const double* rad, int width, int height, const mpt_display* display, bool post, double* out,
+             uint8_t* rgb) {
+    const size_t n = size_t(width) * height;
+    const size_t img = n * 3 * sizeof(double);
+    const size_t ws = post ? mpt_display_workspace_bytes(width, height, display) : 0;
+    const size_t out_bytes = rgb ? n * 3 : img;
+    char* buf = nullptr;
+    DP_TRY(hipMalloc(&buf, img + ws + out_bytes));
+    auto done = [&](int code) {
+        (void)hipFree(buf);
+        return code;
+    };
+    double* d_rad = reinterpret_cast<double*>(buf);
+    void* d_ws = ws ? buf + img : nullptr;
+    char* d_out = buf + img + ws;
+    if (hipMemcpy(d_rad, rad, img, hipMemcpyHostToDevice) != hipSuccess)
+        return done(set_error(MPT_E_HIP, "display: copy to the device failed"));
+    int rc = run(d_rad, width, height, display, post, rgb ? nullptr : reinterpret_cast<double*>(d_out),
+                 rgb ? reinterpret_cast<uint8_t*>(d_out) : nullptr, d_ws, nullptr);
+    if (rc) return done(rc);
+    if (hipMemcpy(rgb ? static_cast<void*>(rgb) : static_cast<void*>(out), d_out, out_bytes, hipMemcpyDeviceToHost) !=
+        hipSuccess)
+        return done(set_error(MPT_E_HIP, "display: copy from the device failed"));
+    return done(MPT_OK);
+}
+
+}  // namespace
+}  // namespace mpt
+
+using namespace mpt;
+
+extern "C" {
+
+void mpt_display_defaults(mpt_display* d) {
+    if (!d) return;
+    d->exposure = 0;  // tonemap.h:13-15, postprocess.h:13-16
+    d->tonemap = MPT_TONEMAP_NONE;
+    d->pad = 0;
+    d->bloom = 0;
+    d->bloom_threshold = 1;
+    d->bloom_radius = 0.04;
+    d->vignette = 0;
+}
+
+size_t mpt_display_workspace_bytes(int32_t width, int32_t height, const mpt_display* display) {
+    if (width <= 0 || height <= 0) return 0;
+    mpt_display d;
+    if (display) d = *display;
+    else mpt_display_defaults(&d);
+    const auto sizes = level_sizes(width, height, d);
+    size_t doubles = 0;
+    for (size_t k = 0; k < sizes.size(); k++)
+        doubles += (k + 1 < sizes.size() ? 2 : 1) * 3 * size_t(sizes[k].first) * sizes[k].second;
+    return doubles * sizeof(double);
+}
+
+int32_t mpt_postprocess_dev(const double* radiance, int32_t width, int32_t height, const mpt_display* display,
+                            double* out, void* workspace, void* stream) {
+    int rc = check(width, height, display);
+    if (rc) return rc;
+    if (!radiance || !out) return set_error(MPT_E_INVALID, "postprocess: null buffer");
+    return run(radiance, width, height, display, true, out, nullptr, workspace, static_cast<hipStream_t>(stream));
+}
+
+int32_t mpt_display_bytes_dev(const double* radiance, int32_t width, int32_t height, const mpt_display* display,
+                              int32_t post, uint8_t* rgb, void* workspace, void* stream) {
+    int rc = check(width, height, display);
+    if (rc) return rc;
+    if (!radiance || !rgb) return set_error(MPT_E_INVALID, "display bytes: null buffer");
+    return run(radiance, width, height, display, post != 0, nullptr, rgb, workspace, static_cast<hipStream_t>(stream));
+}
+
+int32_t mpt_postprocess(const double* radiance, int32_t width, int32_t height, const mpt_display* display,
+                        double* out) {
+    int rc = check(width, height, display);
+    if (rc) return rc;
+    if (!radiance || !out) return set_error(MPT_E_INVALID, "postprocess: null buffer");
+    return run_host(radiance, width, height, display, true, out, nullptr);
+}
+
+int32_t mpt_display_bytes(const double* radiance, int32_t width, int32_t height, const mpt_display* display,
+                          int32_t post, uint8_t* rgb) {
+    int rc = check(width, height, display);
+    if (rc) return rc;
+    if (!radiance || !rgb) return set_error(MPT_E_INVALID, "display bytes: null buffer");
+    return run_host(radiance, width, height, display, post != 0, nullptr, rgb);
+}
+
+}  // extern "C"

@@ -5,6 +5,7 @@
 #include "../../include/mpt/mpt.hpp"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <iostream>
@@ -633,6 +634,65 @@ std::vector<color> denoiser::run(const frame& f, int width, int height) const {
     return res;
 }
 
+// ------------------------------------------------------------------ display
+bool display_settings::set_tonemapper(const std::string& name) {  // tonemap.h:18-25
+    if (name == "none") op = tonemapper::none;
+    else if (name == "reinhard") op = tonemapper::reinhard;
+    else if (name == "aces") op = tonemapper::aces;
+    else return false;
+    return true;
+}
+
+mpt_display display_settings::to_abi() const {
+    mpt_display d;
+    mpt_display_defaults(&d);
+    d.exposure = exposure;
+    d.tonemap = op == tonemapper::reinhard ? MPT_TONEMAP_REINHARD : (op == tonemapper::aces ? MPT_TONEMAP_ACES : MPT_TONEMAP_NONE);
+    d.bloom = post.bloom;
+    d.bloom_threshold = post.bloom_threshold;
+    d.bloom_radius = post.bloom_radius;
+    d.vignette = post.vignette;
+    return d;
+}
+
+namespace {
+std::vector<double> flatten_pixels(const std::vector<color>& pixels) {
+    std::vector<double> r(3 * pixels.size());
+    for (size_t k = 0; k < pixels.size(); k++)
+        for (int c = 0; c < 3; c++) r[3 * k + c] = pixels[k][c];
+    return r;
+}
+}  // namespace
+
+std::vector<color> postprocess(const std::vector<color>& pixels, int width, int height, const post_settings& post) {
+    if (pixels.size() != size_t(width > 0 ? width : 0) * size_t(height > 0 ? height : 0))
+        throw std::runtime_error("postprocess: the image needs width * height pixels");
+    display_settings d;
+    d.post = post;
+    const mpt_display abi = d.to_abi();
+    std::vector<double> r = flatten_pixels(pixels), out(r.size());
+    if (mpt_postprocess(r.data(), width, height, &abi, out.data()) != MPT_OK)
+        throw std::runtime_error(std::string("mpt_postprocess: ") + mpt_last_error());
+    std::vector<color> res(pixels.size());
+    for (size_t k = 0; k < res.size(); k++) res[k] = color(out[3 * k], out[3 * k + 1], out[3 * k + 2]);
+    return res;
+}
+
+bool write_image(const std::string& path, int width, int height, const std::vector<color>& pixels,
+                 const display_settings& display) {
+    if (pixels.size() != size_t(width > 0 ? width : 0) * size_t(height > 0 ? height : 0)) {
+        std::fprintf(stderr, "Error: write_image needs width * height pixels\n");
+        return false;
+    }
+    const std::vector<double> r = flatten_pixels(pixels);
+    const mpt_display abi = display.to_abi();
+    if (mpt_write_image(path.empty() ? nullptr : path.c_str(), width, height, r.data(), &abi) != MPT_OK) {
+        std::fprintf(stderr, "Error: %s\n", mpt_last_error());
+        return false;
+    }
+    return true;
+}
+
 // ------------------------------------------------------------------ .scene loader
 namespace {
 
@@ -886,9 +946,27 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
             std::string value;
             if (!(in >> value) || (value != "on" && value != "off")) return fail("expected 'transparent_shadows on|off'");
             cam.transparent_shadows = value == "on";
-        } else if (keyword == "bloom" || keyword == "vignette" || keyword == "exposure" || keyword == "tonemap" ||
-                   keyword == "normal_maps") {
-            // display settings: they change how radiance becomes pixels, not the radiance
+        } else if (keyword == "bloom") {  // scene.h:667-678
+            auto& post = result.display.post;
+            const char* usage = "expected 'bloom <strength> [threshold <radiance>] [radius <fraction of width>]'";
+            std::string key;
+            if (!(in >> post.bloom) || post.bloom < 0) return fail(usage);
+            while (in >> key) {
+                bool ok = (key == "threshold" && in >> post.bloom_threshold && post.bloom_threshold >= 0) ||
+                          (key == "radius" && in >> post.bloom_radius && post.bloom_radius > 0);
+                if (!ok) return fail(usage);
+            }
+        } else if (keyword == "vignette") {  // scene.h:679-681
+            if (!(in >> result.display.post.vignette) || result.display.post.vignette < 0)
+                return fail("expected 'vignette <strength>'");
+        } else if (keyword == "exposure") {  // scene.h:682-684
+            if (!(in >> result.display.exposure)) return fail("expected 'exposure <stops>'");
+        } else if (keyword == "tonemap") {  // scene.h:685-688
+            std::string name;
+            if (!(in >> name) || !result.display.set_tonemapper(name))
+                return fail("expected 'tonemap none|reinhard|aces'");
+        } else if (keyword == "normal_maps") {
+            // normal maps change shading normals of textured meshes, which the GPU path does not load
             std::string rest;
             std::getline(in, rest);
         } else if (keyword == "texture") {
@@ -1122,3 +1200,12 @@ extern "C" int32_t mpt_scene_describe(const char* path, mpt_scene_desc* desc, mp
 }
 
 extern "C" void mpt_desc_free(void* holder) { delete static_cast<mpt::flat_builder*>(holder); }
+
+extern "C" int32_t mpt_scene_display(const char* path, mpt_display* display) {
+    if (!path || !display) return mpt::set_error(MPT_E_INVALID, "null argument");
+    mpt::scene sc;
+    std::string err;
+    if (!mpt::load_scene(path, sc, &err)) return mpt::set_error(err.find("cannot open") == 0 ? MPT_E_IO : MPT_E_UNSUPPORTED, err);
+    *display = sc.display.to_abi();
+    return MPT_OK;
+}

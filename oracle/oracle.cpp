@@ -1629,6 +1629,140 @@ void first_surface(const scene& S, const ray& cr, uint64_t key, color& albedo, c
     normal = color(0, 0, 0);
 }
 
+// ------------------------------------------------------------------ display (postprocess.h, tonemap.h, color3.h)
+// Restatement of postprocess() (postprocess.h:108-153), display_settings::
+// apply (tonemap.h:28-67) and to_byte (color3.h:17-33), whole images at a
+// time. Pinned against the compiled reference (tests/test_display.py) and
+// by closed forms: no bloom or vignette is the identity, a uniform image
+// blurs to itself, the vignette factor at the centre and corners.
+struct display_img {
+    int w = 0, h = 0;
+    std::vector<color> px;
+    color at(int i, int j) const {  // postprocess.h:28-32
+        i = std::clamp(i, 0, w - 1);
+        j = std::clamp(j, 0, h - 1);
+        return px[size_t(j) * w + i];
+    }
+    color sample(double x, double y) const {  // postprocess.h:35-42
+        x -= 0.5;
+        y -= 0.5;
+        const int i = int(std::floor(x)), j = int(std::floor(y));
+        const double fx = x - i, fy = y - j;
+        return (1 - fy) * ((1 - fx) * at(i, j) + fx * at(i + 1, j)) + fy * ((1 - fx) * at(i, j + 1) + fx * at(i + 1, j + 1));
+    }
+};
+
+display_img half_size(const display_img& in) {  // postprocess.h:47-59
+    display_img out;
+    out.w = std::max(1, in.w / 2);
+    out.h = std::max(1, in.h / 2);
+    out.px.resize(size_t(out.w) * out.h);
+    for (int j = 0; j < out.h; j++)
+        for (int i = 0; i < out.w; i++)
+            out.px[size_t(j) * out.w + i] =
+                0.25 * (in.at(2 * i, 2 * j) + in.at(2 * i + 1, 2 * j) + in.at(2 * i, 2 * j + 1) + in.at(2 * i + 1, 2 * j + 1));
+    return out;
+}
+
+display_img gauss_blur(const display_img& in, double sigma) {  // postprocess.h:62-89
+    const int r = int(std::ceil(3 * sigma));
+    std::vector<double> k(2 * r + 1);
+    double sum = 0;
+    for (int t = -r; t <= r; t++) sum += k[t + r] = std::exp(-0.5 * t * t / (sigma * sigma));
+    for (double& x : k) x /= sum;
+    display_img tmp = in, out = in;
+    for (int j = 0; j < in.h; j++)
+        for (int i = 0; i < in.w; i++) {
+            color c(0, 0, 0);
+            for (int t = -r; t <= r; t++) c = c + k[t + r] * in.at(i + t, j);
+            tmp.px[size_t(j) * in.w + i] = c;
+        }
+    for (int j = 0; j < in.h; j++)
+        for (int i = 0; i < in.w; i++) {
+            color c(0, 0, 0);
+            for (int t = -r; t <= r; t++) c = c + k[t + r] * tmp.at(i, j + t);
+            out.px[size_t(j) * in.w + i] = c;
+        }
+    return out;
+}
+
+std::vector<color> post_image(const std::vector<color>& pixels, int width, int height, double bloom, double threshold,
+                              double radius, double vignette) {
+    std::vector<color> out = pixels;
+    if (bloom > 0) {
+        display_img bright;
+        bright.w = width;
+        bright.h = height;
+        bright.px.resize(pixels.size());
+        for (size_t k = 0; k < pixels.size(); k++) {
+            const double l = luminance(pixels[k]);
+            bright.px[k] = (std::isfinite(l) && l > threshold) ? pixels[k] * ((l - threshold) / l) : color(0, 0, 0);
+        }
+        const double reach = std::fmax(radius * width, 1.0);
+        const int levels = std::clamp(int(std::ceil(std::log2(reach))), 1, 10);
+        std::vector<display_img> blurred;
+        display_img level = bright;
+        for (int k = 0; k < levels && level.w > 1 && level.h > 1; k++) {
+            level = half_size(level);
+            blurred.push_back(gauss_blur(level, 1.5));
+        }
+        for (int j = 0; j < height; j++)
+            for (int i = 0; i < width; i++) {
+                color glow(0, 0, 0);
+                for (const auto& b : blurred) {
+                    const double sx = double(b.w) / width, sy = double(b.h) / height;
+                    glow = glow + b.sample((i + 0.5) * sx, (j + 0.5) * sy);
+                }
+                if (!blurred.empty()) {
+                    color& o = out[size_t(j) * width + i];
+                    o = o + bloom / blurred.size() * glow;
+                }
+            }
+    }
+    if (vignette > 0) {
+        const double cx = 0.5 * width, cy = 0.5 * height;
+        const double corner2 = cx * cx + cy * cy;
+        for (int j = 0; j < height; j++)
+            for (int i = 0; i < width; i++) {
+                const double dx = i + 0.5 - cx, dy = j + 0.5 - cy;
+                const double f = 1 + vignette * (dx * dx + dy * dy) / corner2;
+                out[size_t(j) * width + i] = out[size_t(j) * width + i] / (f * f);
+            }
+    }
+    return out;
+}
+
+double aces_curve(double x) {  // tonemap.h:63-67
+    const double a = x * (x + 0.0245786) - 0.000090537;
+    const double b = x * (0.983729 * x + 0.4329510) + 0.238081;
+    return a / b;
+}
+double unit_clamp(double x) { return x < 0 ? 0 : (x > 1 ? 1 : x); }
+
+color display_apply(const color& radiance, double exposure, int op) {  // tonemap.h:28-37
+    const color c = std::pow(2.0, exposure) * radiance;
+    if (op == 1) return color(c.x() / (1 + c.x()), c.y() / (1 + c.y()), c.z() / (1 + c.z()));
+    if (op == 2) {
+        color v(0.59719 * c.x() + 0.35458 * c.y() + 0.04823 * c.z(), 0.07600 * c.x() + 0.90834 * c.y() + 0.01566 * c.z(),
+                0.02840 * c.x() + 0.13383 * c.y() + 0.83777 * c.z());
+        v = color(aces_curve(v.x()), aces_curve(v.y()), aces_curve(v.z()));
+        v = color(1.60475 * v.x() + -0.53108 * v.y() + -0.07367 * v.z(),
+                  -0.10208 * v.x() + 1.10813 * v.y() + -0.00605 * v.z(),
+                  -0.00327 * v.x() + -0.07276 * v.y() + 1.07602 * v.z());
+        return color(unit_clamp(v.x()), unit_clamp(v.y()), unit_clamp(v.z()));
+    }
+    return c;
+}
+
+unsigned char srgb_byte(double x) {  // color3.h:17-33
+    double s;
+    if (x <= 0) s = 0;
+    else if (x <= 0.0031308) s = 12.92 * x;
+    else s = 1.055 * std::pow(x, 1.0 / 2.4) - 0.055;
+    s = s < 0.000 ? 0.000 : (s > 0.999 ? 0.999 : s);
+    return (unsigned char)(256 * s);
+}
+
 // ------------------------------------------------------------------ denoiser (denoiser.h:38-193)
 // Edge-avoiding a-trous filter on lighting = radiance / albedo, with SVGF's
 // variance-scaled luminance weight; a restatement of denoiser::run.
@@ -1967,6 +2101,37 @@ int32_t orc_make_ray_batch(orc_scene* S, const mpt_camera* c, uint64_t seed, int
         }
         put(i, r);
     }
+    return MPT_OK;
+}
+
+
+int32_t orc_postprocess(const double* rad, int32_t w, int32_t h, double bloom, double threshold, double radius,
+                        double vignette, double* out) {
+    std::vector<color> px(size_t(w) * h);
+    for (size_t k = 0; k < px.size(); k++) px[k] = color(rad[3 * k], rad[3 * k + 1], rad[3 * k + 2]);
+    const std::vector<color> o = post_image(px, w, h, bloom, threshold, radius, vignette);
+    for (size_t k = 0; k < o.size(); k++)
+        for (int c = 0; c < 3; c++) out[3 * k + c] = o[k][c];
+    return MPT_OK;
+}
+
+// The bytes write_png emits (image_io.h:25-37, 81-92): postprocess when
+// post is set (and bloom or vignette is on), then apply and to_byte.
+int32_t orc_display_bytes(const double* rad, int32_t w, int32_t h, double exposure, int32_t op, double bloom,
+                          double threshold, double radius, double vignette, int32_t post, uint8_t* out) {
+    std::vector<color> px(size_t(w) * h);
+    for (size_t k = 0; k < px.size(); k++) px[k] = color(rad[3 * k], rad[3 * k + 1], rad[3 * k + 2]);
+    if (post && (bloom > 0 || vignette > 0)) px = post_image(px, w, h, bloom, threshold, radius, vignette);
+    for (size_t k = 0; k < px.size(); k++) {
+        const color c = display_apply(px[k], exposure, op);
+        for (int q = 0; q < 3; q++) out[3 * k + q] = srgb_byte(c[q]);
+    }
+    return MPT_OK;
+}
+
+// to_byte alone, for direct checks of the sRGB quantisation.
+int32_t orc_to_byte(const double* x, int64_t n, uint8_t* out) {
+    for (int64_t k = 0; k < n; k++) out[k] = srgb_byte(x[k]);
     return MPT_OK;
 }
 

@@ -32,6 +32,7 @@
 #define private public
 #include "scene.h"
 #undef private
+#include "image_io.h"
 
 namespace {
 
@@ -217,6 +218,68 @@ int ref_denoise(const double* rad, const double* alb, const double* nrm, const d
     std::vector<color> o = d.run(f, w, h);
     for (size_t k = 0; k < n; k++)
         for (int c = 0; c < 3; c++) out[3 * k + c] = o[k][c];
+    return 0;
+}
+
+
+// Display settings from the harness's flat arguments.
+static display_settings make_display(double exposure, int op, double bloom, double threshold, double radius,
+                                     double vignette) {
+    display_settings d;
+    d.exposure = exposure;
+    d.op = op == 1 ? tonemapper::reinhard : (op == 2 ? tonemapper::aces : tonemapper::none);
+    d.post.bloom = bloom;
+    d.post.bloom_threshold = threshold;
+    d.post.bloom_radius = radius;
+    d.post.vignette = vignette;
+    return d;
+}
+
+static std::vector<color> to_colors(const double* rad, size_t n) {
+    std::vector<color> px(n);
+    for (size_t k = 0; k < n; k++) px[k] = color(rad[3 * k], rad[3 * k + 1], rad[3 * k + 2]);
+    return px;
+}
+
+int ref_postprocess(const double* rad, int w, int h, double bloom, double threshold, double radius, double vignette,
+                    double* out) {
+    const display_settings d = make_display(0, 0, bloom, threshold, radius, vignette);
+    const std::vector<color> o = postprocess(to_colors(rad, size_t(w) * h), w, h, d.post);
+    for (size_t k = 0; k < o.size(); k++)
+        for (int c = 0; c < 3; c++) out[3 * k + c] = o[k][c];
+    return 0;
+}
+
+// The bytes write_png builds (image_io.h:25-37 after write_image's postprocess, 81-92).
+int ref_display_bytes(const double* rad, int w, int h, double exposure, int op, double bloom, double threshold,
+                      double radius, double vignette, int post, uint8_t* out) {
+    const display_settings d = make_display(exposure, op, bloom, threshold, radius, vignette);
+    std::vector<color> px = to_colors(rad, size_t(w) * h);
+    if (post && d.post.active()) px = postprocess(px, w, h, d.post);
+    for (size_t k = 0; k < px.size(); k++) {
+        const color c = d.apply(px[k]);
+        for (int q = 0; q < 3; q++) out[3 * k + q] = to_byte(c[q]);
+    }
+    return 0;
+}
+
+int ref_write_image(const char* path, int w, int h, const double* rad, double exposure, int op, double bloom,
+                    double threshold, double radius, double vignette) {
+    const display_settings d = make_display(exposure, op, bloom, threshold, radius, vignette);
+    return write_image(path ? path : "", w, h, to_colors(rad, size_t(w) * h), d) ? 0 : -1;
+}
+
+// A .scene file's display settings: exposure, op, bloom, threshold, radius, vignette.
+int ref_scene_display(const char* path, double* out) {
+    scene sc;
+    if (!load_scene(path, sc)) return -1;
+    const display_settings& d = sc.display;
+    out[0] = d.exposure;
+    out[1] = d.op == tonemapper::reinhard ? 1 : (d.op == tonemapper::aces ? 2 : 0);
+    out[2] = d.post.bloom;
+    out[3] = d.post.bloom_threshold;
+    out[4] = d.post.bloom_radius;
+    out[5] = d.post.vignette;
     return 0;
 }
 

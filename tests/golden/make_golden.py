@@ -17,8 +17,11 @@ queries) are seeded and regenerated identically by the tests:
                         their per-pixel standard error over 8 runs
   denoise.npz           denoiser::run (denoiser.h:45-193) on the synthetic frames
                         of tests/denoise_frames.py, for two settings
+  display.npz           postprocess (postprocess.h:108-153) and the bytes write_png
+                        writes (display_settings::apply + to_byte, tonemap.h,
+                        color3.h) on the images and settings of tests/display_cases.py
 
-Usage: python tests/golden/make_golden.py [step ...]  (steps: ggx trace bsdf render denoise; all by default)
+Usage: python tests/golden/make_golden.py [step ...]  (steps: ggx trace bsdf render denoise display; all by default)
 (all steps take about a minute)
 """
 import hashlib
@@ -107,14 +110,31 @@ def golden_denoise():
     print("denoise.npz", len(dfr.CASES), "frames", flush=True)
 
 
+def golden_display():
+    import display_cases as dc
+    arrays = {}
+    for name, make in dc.CASES.items():
+        img = make()
+        arrays[f"{name}/radiance"] = img
+        for si, st in enumerate(dc.SETTINGS):
+            arrays[f"{name}/post{si}"] = ol.reference_postprocess(img, st)
+            arrays[f"{name}/bytes{si}"] = ol.reference_display_bytes(img, st)
+        arrays[f"{name}/bytes_nopost"] = ol.reference_display_bytes(img, dc.SETTINGS[2], post=False)
+    arrays["settings"] = np.array(dc.SETTINGS)
+    np.savez_compressed(os.path.join(HERE, "display.npz"), **arrays)
+    print("display.npz", len(dc.CASES), "images", flush=True)
+
+
 def main():
     if not ol.reference_available():
         sys.exit("oracle/_ref/libmpt_ref.so is not built (make -C oracle ref, where the reference is present)")
     R = ol.reference()
-    steps = set(sys.argv[1:]) or {"ggx", "trace", "bsdf", "render", "denoise"}
+    steps = set(sys.argv[1:]) or {"ggx", "trace", "bsdf", "render", "denoise", "display"}
     if "denoise" in steps:
         golden_denoise()
-    if steps == {"denoise"}:
+    if "display" in steps:
+        golden_display()
+    if steps <= {"denoise", "display"}:
         return
 
     e = np.zeros(1024)

@@ -56,6 +56,10 @@ def oracle():
             "orc_make_ray_batch": (C.c_int32, [vp, P(Camera), C.c_uint64, C.c_int64, dp, dp]),
             "orc_denoise": (C.c_int32, [dp, dp, dp, dp, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                         C.c_double, dp]),
+            "orc_postprocess": (C.c_int32, [dp, C.c_int32, C.c_int32] + [C.c_double] * 4 + [dp]),
+            "orc_display_bytes": (C.c_int32, [dp, C.c_int32, C.c_int32, C.c_double, C.c_int32] + [C.c_double] * 4 +
+                                  [C.c_int32, P(C.c_uint8)]),
+            "orc_to_byte": (C.c_int32, [dp, C.c_int64, P(C.c_uint8)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(lib, name)
@@ -173,6 +177,11 @@ def reference():
             "ref_render": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp]),
             "ref_denoise": (C.c_int, [dp, dp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                       dp]),
+            "ref_postprocess": (C.c_int, [dp, C.c_int, C.c_int] + [C.c_double] * 4 + [dp]),
+            "ref_display_bytes": (C.c_int, [dp, C.c_int, C.c_int, C.c_double, C.c_int] + [C.c_double] * 4 +
+                                  [C.c_int, P(C.c_uint8)]),
+            "ref_write_image": (C.c_int, [C.c_char_p, C.c_int, C.c_int, dp, C.c_double, C.c_int] + [C.c_double] * 4),
+            "ref_scene_display": (C.c_int, [C.c_char_p, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(lib, name)
@@ -241,3 +250,63 @@ def oracle_denoise(frame, settings=(5, 4.0, 0.1, 0.1)):
 def reference_denoise(frame, settings=(5, 4.0, 0.1, 0.1)):
     """The reference's own denoiser::run, through the compiled harness."""
     return _denoise(reference().ref_denoise, frame, settings)
+
+
+# ---- display pipeline (postprocess.h, tonemap.h, color3.h)
+
+def _img(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def oracle_postprocess(rad, setting):
+    _, _, bloom, thr, radius, vig = setting
+    a = _img(rad)
+    out = np.zeros_like(a)
+    oracle().orc_postprocess(_d(a), a.shape[1], a.shape[0], bloom, thr, radius, vig, _d(out))
+    return out
+
+
+def oracle_display_bytes(rad, setting, post=True):
+    exposure, op, bloom, thr, radius, vig = setting
+    a = _img(rad)
+    out = np.zeros(a.shape, dtype=np.uint8)
+    oracle().orc_display_bytes(_d(a), a.shape[1], a.shape[0], exposure, op, bloom, thr, radius, vig, int(post), _u8(out))
+    return out
+
+
+def oracle_to_byte(x):
+    x = _img(x).ravel()
+    out = np.zeros(x.shape, dtype=np.uint8)
+    oracle().orc_to_byte(_d(x), x.size, _u8(out))
+    return out
+
+
+def reference_postprocess(rad, setting):
+    _, _, bloom, thr, radius, vig = setting
+    a = _img(rad)
+    out = np.zeros_like(a)
+    reference().ref_postprocess(_d(a), a.shape[1], a.shape[0], bloom, thr, radius, vig, _d(out))
+    return out
+
+
+def reference_display_bytes(rad, setting, post=True):
+    exposure, op, bloom, thr, radius, vig = setting
+    a = _img(rad)
+    out = np.zeros(a.shape, dtype=np.uint8)
+    reference().ref_display_bytes(_d(a), a.shape[1], a.shape[0], exposure, op, bloom, thr, radius, vig, int(post),
+                                  _u8(out))
+    return out
+
+
+def reference_write_image(path, rad, setting):
+    exposure, op, bloom, thr, radius, vig = setting
+    a = _img(rad)
+    return reference().ref_write_image(os.fsencode(path), a.shape[1], a.shape[0], _d(a), exposure, op, bloom, thr,
+                                       radius, vig)
+
+
+def reference_scene_display(path):
+    out = np.zeros(6)
+    if reference().ref_scene_display(os.fsencode(path), _d(out)) != 0:
+        return None
+    return out
