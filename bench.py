@@ -116,6 +116,7 @@ def batch8(args):
     500 affine icosphere instances, sun + sky environment, fog) at uneven
     resolutions, 32 spp, max depth 12, rendered back to back with every
     scene resident in HBM; paths/s over the whole batch."""
+    sched = dict(use_graph=args.use_graph, sort_by_material=args.sort, blocks_per_cu=args.bpc)
     import mypbr_amd as m
     from mypbr_amd import workloads as W
     scenes = []
@@ -125,14 +126,14 @@ def batch8(args):
         scenes.append((m.Scene.from_desc(d, cam, keepalive=keep), cam))
     log(f"built 8 scenes in {time.perf_counter() - t0:.1f} s")
     for s, cam in scenes:  # warm-up: one pass each
-        s.render(cam, seed=1, pass_count=1, read=False)
+        s.render(cam, seed=1, pass_count=1, read=False, **sched)
     synchronize()
     t0 = time.perf_counter()
     paths = ext = sh = 0
     per = []
     for s, cam in scenes:
         t1 = time.perf_counter()
-        f = s.render(cam, seed=2, read=False)
+        f = s.render(cam, seed=2, read=False, **sched)
         per.append({"width": cam.image_width, "height": cam.height(), "seconds": time.perf_counter() - t1})
         paths += f.stats.paths
         ext += f.stats.extension_rays
@@ -161,6 +162,39 @@ def batch8(args):
     return line
 
 
+def display_bench(m, radiance, reps=50):
+    """The display pipeline at the frame's size, device-resident: bloom 0.25,
+    vignette 0.25, ACES (the reference's voxel-style preset, scene.h:631-635)
+    to sRGB bytes, and postprocess alone. Wall time per frame over `reps`."""
+    import ctypes as C
+    import numpy as np
+    h, w = radiance.shape[:2]
+    d = m.display_settings(tonemap="aces", bloom=0.25, vignette=0.25)
+    src = m.DeviceArray.of(radiance)
+    ws = m.DeviceArray((max(m.lib.mpt_display_workspace_bytes(w, h, C.byref(d)), 1),), np.uint8)
+    rgb = m.DeviceArray((h, w, 3), np.uint8)
+    out = m.DeviceArray((h, w, 3), np.float64)
+    res = {"width": w, "height": h, "settings": "bloom 0.25, vignette 0.25, tonemap aces, exposure 0"}
+    for name, call in (("to_bytes", lambda: m.lib.mpt_display_bytes_dev(src.handle, w, h, C.byref(d), 1, rgb.handle,
+                                                                          ws.handle, None)),
+                       ("postprocess", lambda: m.lib.mpt_postprocess_dev(src.handle, w, h, C.byref(d), out.handle,
+                                                                         ws.handle, None))):
+        for _ in range(3):
+            m._check(call())
+        synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            m._check(call())
+        synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        # algorithmic bytes: the radiance read once and the output written once
+        nbytes = w * h * (24 + (3 if name == "to_bytes" else 24))
+        res[name] = {"ms_per_frame": dt * 1e3, "mpix_per_s": w * h / dt / 1e6,
+                     "algorithmic_bytes": nbytes, "achieved_gbs": nbytes / dt / 1e9}
+    log("display:", json.dumps(res))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,8 +208,15 @@ def main():
     ap.add_argument("--workload", choices=["spheres", "batch8"], default="spheres",
                     help="spheres: the 1080p procedural-spheres render (the headline metric); batch8: the "
                          "8 generated terrain + instance scenes of uneven resolutions, 32 spp, depth 12")
+    ap.add_argument("--no-display", action="store_true",
+                    help="skip timing the display pipeline (bloom, vignette, tonemap, sRGB bytes) at 1080p")
     ap.add_argument("--no-quality", action="store_true",
                     help="skip the equal-time relMSE study of the learned forms against the plain tracer")
+    ap.add_argument("--use-graph", type=int, default=0, choices=[-1, 0, 1],
+                    help="bounce kernels replayed as hipGraphs (1), launched directly (-1), or the tuned choice (0)")
+    ap.add_argument("--sort", type=int, default=0, choices=[-1, 0, 1],
+                    help="bin hits by material into per-material shade kernels (1), one kernel per batch (-1), auto (0)")
+    ap.add_argument("--bpc", type=int, default=0, help="persistent grid blocks per CU (0 = auto)")
     ap.add_argument("--tail-paths", type=int, default=0,
                     help="live paths at which a batch's tail runs as one launch (0 = the library's default, -1 = never)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -206,14 +247,15 @@ def main():
     log("device:", info)
     pps = args.passes_per_step
     total_passes = args.steps * pps
+    sched = dict(use_graph=args.use_graph, sort_by_material=args.sort, blocks_per_cu=args.bpc)
 
     for k in range(args.warmup):
-        s.render(cam, seed=1, first_pass=(k * pps) % 256, pass_count=pps, read=False, tail_paths=args.tail_paths)
+        s.render(cam, seed=1, first_pass=(k * pps) % 256, pass_count=pps, read=False, tail_paths=args.tail_paths, **sched)
     synchronize()
     t0 = time.perf_counter()
     ext = sh = 0
     for k in range(args.steps):
-        f = s.render(cam, seed=1, first_pass=(k * pps) % 256, pass_count=pps, accumulate=k > 0, read=False, tail_paths=args.tail_paths)
+        f = s.render(cam, seed=1, first_pass=(k * pps) % 256, pass_count=pps, accumulate=k > 0, read=False, tail_paths=args.tail_paths, **sched)
         ext += f.stats.extension_rays
         sh += f.stats.shadow_rays
         log(f"step {k + 1}/{args.steps}: {f.stats.seconds * 1e3:.1f} ms")
@@ -247,8 +289,8 @@ def main():
 
     if not args.no_profile:
         # one step again with per-stage event timing, then with BVH work counters
-        ft = s.render(cam, seed=1, first_pass=0, pass_count=pps, profile=1, read=False, tail_paths=args.tail_paths).stats
-        fc = s.render(cam, seed=1, first_pass=0, pass_count=pps, profile=2, read=False, tail_paths=args.tail_paths).stats
+        ft = s.render(cam, seed=1, first_pass=0, pass_count=pps, profile=1, read=False, tail_paths=args.tail_paths, **sched).stats
+        fc = s.render(cam, seed=1, first_pass=0, pass_count=pps, profile=2, read=False, tail_paths=args.tail_paths, **sched).stats
         names = ["extend", "shade", "shadow", "camera+accumulate", "finish"]
         stage = {names[i]: {"ms": ft.stage_ms[i], "launches": ft.stage_launches[i]} for i in range(5)}
         # rays of the stage kernels only: the batch tails (k_finish) neither count their BVH work
@@ -272,6 +314,10 @@ def main():
                             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_per_launch(f"k_{dom}"),
                             "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6}
         log("stages:", json.dumps(stage))
+
+    if not args.no_display:
+        f = s.render(cam, seed=7, first_pass=0, pass_count=4)
+        line["display"] = display_bench(m, f.radiance)
 
     if not args.no_quality:
         # the metric's second half: equal-time relMSE (eps 0.01) against a 4096-spp render of the plain tracer

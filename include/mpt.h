@@ -178,9 +178,17 @@ typedef struct mpt_render_opts {
     int32_t first_pass;            /* pass index of the first sample rendered (resume) */
     int32_t save_every;            /* call on_pass every this many passes (0 = never) */
     int32_t passes_in_flight;      /* samples per pixel traced together (0 = auto) */
-    int32_t block_size;            /* threads per block of the stage kernels (0 = auto) */
-    int32_t sort_by_material;      /* -1 = off, 1 = on, 0 = auto */
-    int32_t use_graph;             /* -1 = off, 1 = on, 0 = auto */
+    int32_t blocks_per_cu;         /* 256-thread blocks per CU in the persistent stage grid, 1..64
+                                    * (0 = auto: the tuned choice, else 16) */
+    int32_t sort_by_material;      /* 1 = on: each bounce is a wavefront of stage kernels, hits binned
+                                    * into one queue per material and shaded by that material's
+                                    * kernel; -1 = off: each batch runs in one kernel, a path per
+                                    * thread through all its bounces, shading whatever it hits;
+                                    * 0 = auto (the tuned choice, else on) */
+    int32_t use_graph;             /* 1 = on: each bounce's kernels are captured once as a hipGraph
+                                    * and replayed (per bounce index, while the camera and buffers
+                                    * stay the same); -1 = off; 0 = auto (the tuned choice, else
+                                    * off). Profiling and the radiance cache launch directly. */
     int32_t scene_index;           /* keys the RNG for batched multi-scene launches */
     int32_t pass_count;            /* passes to render in this call (0 = samples_per_pixel - first_pass) */
     int32_t accumulate;            /* 1: add to the previous call's sums (progressive render) */
@@ -284,14 +292,30 @@ typedef struct mpt_autotune_result {
     int32_t trials;            /* timed trial renders */
     int64_t tail_paths;        /* live paths at which a batch's tail runs in one launch (-1: never) */
     double seconds_per_pass;   /* of the best trial */
+    int32_t blocks_per_cu;     /* persistent grid: 256-thread blocks per CU */
+    int32_t sort_by_material;  /* 1 = binned wavefront stages, -1 = one kernel per batch */
+    int32_t use_graph;         /* 1 = bounces replayed as hipGraphs, -1 = launched directly */
+    int32_t pad;
 } mpt_autotune_result;
 
 /* Times trial renders of trial_passes passes (8 if < 1) of this camera's
- * image for each combination of batch size (about 4, 8, 16 and 32 M paths
- * in flight) and batch-tail size, keeps the fastest for the scene (used by
- * later renders of the same image size whose options leave them at 0 =
- * auto) and reports it. The image does not depend on either setting. */
+ * image: every combination of batch size (about 4, 8, 16 and 32 M paths in
+ * flight) and batch-tail size, then, from the best, 8, 16 and 32 blocks per
+ * CU, sorting by material off, and bounce graphs on. Keeps the fastest for
+ * the scene (used by later renders of the same image size whose options
+ * leave them at 0 = auto) and reports it. The image does not depend on any
+ * of these settings. */
 int32_t mpt_autotune(mpt_scene* scene, const mpt_camera* cam, int32_t trial_passes, mpt_autotune_result* out);
+
+/* Installs settings found earlier (for instance read back from a cache of
+ * mpt_autotune results) for renders of this camera's image size, without
+ * timing anything. */
+int32_t mpt_autotune_apply(mpt_scene* scene, const mpt_camera* cam, const mpt_autotune_result* settings);
+
+/* A key for caching autotune results: the device, the scene's compiled
+ * kernel profile (geometry, material types, lights, environment) and the
+ * image size, as text in buf (NUL-terminated, truncated to len). */
+int32_t mpt_autotune_key(const mpt_scene* scene, const mpt_camera* cam, char* buf, int32_t len);
 
 /* Inside an on_pass callback: the frame averaged over the passes done. */
 int32_t mpt_read_frame(mpt_scene* scene, double* radiance, double* albedo, double* normal,

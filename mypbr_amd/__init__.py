@@ -63,6 +63,8 @@ def _load():
                                       P(C.c_double)]),
         "mpt_primary_depth": (C.c_int32, [vp, P(Camera), P(C.c_double)]),
         "mpt_autotune": (C.c_int32, [vp, P(Camera), C.c_int32, P(abi.AutotuneResult)]),
+        "mpt_autotune_apply": (C.c_int32, [vp, P(Camera), P(abi.AutotuneResult)]),
+        "mpt_autotune_key": (C.c_int32, [vp, P(Camera), C.c_char_p, C.c_int32]),
         "mpt_device_alloc": (vp, [C.c_size_t]),
         "mpt_device_free": (None, [vp]),
         "mpt_copy_to_device": (C.c_int32, [vp, vp, C.c_size_t]),
@@ -347,7 +349,8 @@ class Scene:
         return ev, pdf
 
     def render(self, camera=None, seed=1, features=None, passes_in_flight=0, save_every=0, on_pass=None,
-               first_pass=0, scene_index=0, pass_count=0, accumulate=False, profile=0, read=True, tail_paths=0):
+               first_pass=0, scene_index=0, pass_count=0, accumulate=False, profile=0, read=True, tail_paths=0,
+               blocks_per_cu=0, sort_by_material=0, use_graph=0):
         cam = abi.camera_copy(camera if camera is not None else self.camera)
         if features is not None:
             cam.collect_features = int(bool(features))
@@ -359,7 +362,8 @@ class Scene:
         var = np.zeros((h, w)) if feats else None
         opts = RenderOpts(seed=seed, first_pass=first_pass, save_every=save_every,
                           passes_in_flight=passes_in_flight, scene_index=scene_index, pass_count=pass_count,
-                          accumulate=int(bool(accumulate)), profile=profile, tail_paths=tail_paths)
+                          accumulate=int(bool(accumulate)), profile=profile, tail_paths=tail_paths,
+                          blocks_per_cu=blocks_per_cu, sort_by_material=sort_by_material, use_graph=use_graph)
         stats = RenderStats()
         cb = abi.PASS_CALLBACK(lambda done, user: on_pass(done, self)) if on_pass else abi.PASS_CALLBACK()
         if not read:  # leave the frame in HBM (read it later with read_frame)
@@ -371,13 +375,37 @@ class Scene:
                               _dp(var) if feats else None, cb, None, C.byref(stats)))
         return Frame(rad, alb, nrm, var, stats)
 
-    def autotune(self, camera=None, trial_passes=8):
-        """mpt_autotune: picks the batch size and tail size for this image size."""
+    _TUNED = ("passes_in_flight", "tail_paths", "blocks_per_cu", "sort_by_material", "use_graph")
+
+    def autotune(self, camera=None, trial_passes=8, cache=None):
+        """mpt_autotune: times batch size, tail size, grid, material binning and
+        bounce graphs for this image size and keeps the fastest. With `cache` (a
+        JSON file path) results are stored there keyed by device, kernel profile
+        and image size, and a stored result is applied without timing."""
+        import json
         cam = abi.camera_copy(camera if camera is not None else self.camera)
+        key = None
+        if cache:
+            buf = C.create_string_buffer(512)
+            _check(lib.mpt_autotune_key(self._h, C.byref(cam), buf, 512))
+            key = buf.value.decode()
+            table = json.load(open(cache)) if os.path.exists(cache) else {}
+            if key in table:
+                r = abi.AutotuneResult(**{k: table[key][k] for k in self._TUNED})
+                _check(lib.mpt_autotune_apply(self._h, C.byref(cam), C.byref(r)))
+                return dict(table[key], cached=True)
         r = abi.AutotuneResult()
         _check(lib.mpt_autotune(self._h, C.byref(cam), trial_passes, C.byref(r)))
-        return {"passes_in_flight": r.passes_in_flight, "tail_paths": r.tail_paths, "trials": r.trials,
-                "seconds_per_pass": r.seconds_per_pass}
+        out = {k: getattr(r, k) for k in self._TUNED}
+        out.update(trials=r.trials, seconds_per_pass=r.seconds_per_pass)
+        if cache:
+            table = json.load(open(cache)) if os.path.exists(cache) else {}
+            table[key] = out
+            tmp = cache + ".tmp"
+            with open(tmp, "w") as f:
+                json.dump(table, f, indent=1, sort_keys=True)
+            os.replace(tmp, cache)
+        return dict(out, cached=False)
 
     def primary_depth(self, camera=None):
         """Depth of the first surface through every pixel centre (0 for the sky)."""

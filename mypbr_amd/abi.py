@@ -99,7 +99,7 @@ class Camera(C.Structure):
 
 class RenderOpts(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("first_pass", C.c_int32), ("save_every", C.c_int32),
-                ("passes_in_flight", C.c_int32), ("block_size", C.c_int32),
+                ("passes_in_flight", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("sort_by_material", C.c_int32), ("use_graph", C.c_int32),
                 ("scene_index", C.c_int32), ("pass_count", C.c_int32), ("accumulate", C.c_int32),
                 ("profile", C.c_int32), ("tail_paths", C.c_int32)]
@@ -155,4 +155,5 @@ class NrcStats(C.Structure):
 class AutotuneResult(C.Structure):
     """mpt_autotune_result."""
     _fields_ = [("passes_in_flight", C.c_int32), ("trials", C.c_int32), ("tail_paths", C.c_int64),
-                ("seconds_per_pass", C.c_double)]
+                ("seconds_per_pass", C.c_double), ("blocks_per_cu", C.c_int32), ("sort_by_material", C.c_int32),
+                ("use_graph", C.c_int32), ("pad", C.c_int32)]

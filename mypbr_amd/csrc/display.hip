@@ -11,15 +11,21 @@
 // to_byte (color3.h:30-33) encodes with the sRGB curve and quantises.
 //
 // Kernels, all in double precision in the reference's operation order:
-//   k_level     one launch per pyramid level: a block makes a 32 x 16 tile
-//               of the blurred level in LDS from the level above (for the
-//               first level, from the radiance with the threshold applied
-//               on the fly): the downsampled window with a 5-pixel halo,
-//               the horizontal pass, the vertical pass. It also stores the
-//               unblurred tile, the source of the next level.
-//   k_compose   one thread per full-size pixel: the glow from every level,
-//               vignetting, and either the postprocessed radiance or, fused,
-//               exposure, tonemap and the sRGB byte.
+//   k_down      the first level, a thread per output pixel: the radiance's
+//               2 x 2 blocks with the threshold applied on the fly, averaged
+//               (a pure stream over the full-size image).
+//   k_level     one launch per level: a block makes a 32 x 16 tile of the
+//               blurred level in LDS: the downsampled window with a 5-pixel
+//               halo (for the first level, the window read from k_down's
+//               output), the horizontal pass, the vertical pass. It also
+//               stores the unblurred tile, the source of the next level.
+//   k_compose   a block per 32 x 8 tile of full-size pixels: first the texels
+//               of every level that the tile's bilinear samples touch are
+//               staged in LDS (at most 18 x 6 per level, since a level is at
+//               most half the width and height), then a thread per pixel
+//               adds the glow, vignettes, and writes either the
+//               postprocessed radiance or, fused, the exposed, tonemapped
+//               sRGB byte.
 // The byte is found by comparing with the 255 radiance thresholds at which
 // to_byte steps up, computed once on the host from the same formula and the
 // host's libm (the one the reference's own writer calls), so the bytes are
@@ -47,9 +53,11 @@ constexpr int kTaps = 11;       // blur radius ceil(3 * 1.5) = 5 (postprocess.h:
 constexpr int kR = 5;
 constexpr int kTX = 32, kTY = 16;  // k_level tile
 constexpr int kWX = kTX + 2 * kR, kWY = kTY + 2 * kR;
+constexpr int kCX = 32, kCY = 8;      // k_compose tile
+constexpr int kCWX = 20, kCWY = 8;    // its per-level texel window (18 x 6 needed)
 
 struct level_desc {
-    double* level;    // unblurred (the next level's source); may be null for the last level
+    double* level;    // unblurred (the next level's source)
     double* blurred;
     int32_t w, h;
 };
@@ -92,27 +100,40 @@ __device__ __forceinline__ v3 bright(const v3& p, double threshold) {
     return (__builtin_isfinite(l) && l > threshold) ? p * ((l - threshold) / l) : v3{0, 0, 0};
 }
 
-// One pyramid level. src is the level above (sw x sh); bright_src: it is the
-// radiance and the threshold applies to every pixel read.
-__global__ __launch_bounds__(256) void k_level(const double* src, int sw, int sh, int bright_src, double threshold,
-                                               level_desc out, blur_weights bw) {
+// downsample (postprocess.h:47-59) of src (sw x sh) at output pixel (i, j);
+// bright_src: src is the radiance and the threshold applies to each pixel read.
+__device__ __forceinline__ v3 down_at(const double* src, int sw, int sh, bool bright_src, double threshold, int i,
+                                      int j) {
+    auto at = [&](int a, int b) {  // image::at (postprocess.h:28-32)
+        const v3 p = ld(src, int64_t(clampi(b, 0, sh - 1)) * sw + clampi(a, 0, sw - 1));
+        return bright_src ? bright(p, threshold) : p;
+    };
+    return 0.25 * (at(2 * i, 2 * j) + at(2 * i + 1, 2 * j) + at(2 * i, 2 * j + 1) + at(2 * i + 1, 2 * j + 1));
+}
+
+// The first level, unblurred: a thread per pixel.
+__global__ __launch_bounds__(256) void k_down(const double* rad, int sw, int sh, double threshold, double* dst, int w,
+                                              int h) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= w) return;
+    st(dst, int64_t(j) * w + i, down_at(rad, sw, sh, true, threshold, i, j));
+}
+
+// One pyramid level. With src set, the level is downsampled from src (the
+// level above, sw x sh) on the fly and its unblurred tile stored; with src
+// null the unblurred level is out.level already.
+__global__ __launch_bounds__(256) void k_level(const double* src, int sw, int sh, level_desc out, blur_weights bw) {
     __shared__ double dwin[kWY][kWX][3];   // downsampled window, clamped coordinates
     __shared__ double hwin[kWY][kTX][3];   // after the horizontal pass
     const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY;
     const int w = out.w, h = out.h;
-    auto src_at = [&](int i, int j) {  // image::at (postprocess.h:28-32)
-        const v3 p = ld(src, int64_t(clampi(j, 0, sh - 1)) * sw + clampi(i, 0, sw - 1));
-        return bright_src ? bright(p, threshold) : p;
-    };
     for (int k = threadIdx.x; k < kWX * kWY; k += blockDim.x) {
         const int c = k % kWX, r = k / kWX;
         const int i = clampi(x0 - kR + c, 0, w - 1), j = clampi(y0 - kR + r, 0, h - 1);
-        // downsample (postprocess.h:53-56)
-        const v3 d = 0.25 * (src_at(2 * i, 2 * j) + src_at(2 * i + 1, 2 * j) + src_at(2 * i, 2 * j + 1) +
-                             src_at(2 * i + 1, 2 * j + 1));
+        const v3 d = src ? down_at(src, sw, sh, false, 0.0, i, j) : ld(out.level, int64_t(j) * w + i);
         dwin[r][c][0] = d.x, dwin[r][c][1] = d.y, dwin[r][c][2] = d.z;
         // the unblurred level, for the next one
-        if (out.level && c >= kR && c < kR + kTX && r >= kR && r < kR + kTY) {
+        if (src && c >= kR && c < kR + kTX && r >= kR && r < kR + kTY) {
             const int gi = x0 - kR + c, gj = y0 - kR + r;
             if (gi < w && gj < h) st(out.level, int64_t(gj) * w + gi, d);
         }
@@ -143,16 +164,6 @@ __global__ __launch_bounds__(256) void k_level(const double* src, int sw, int sh
         }
         st(out.blurred, int64_t(gj) * w + gi, s);
     }
-}
-
-// image::sample (postprocess.h:35-42): bilinear, pixel centres at k + 0.5.
-__device__ __forceinline__ v3 sample(const double* img, int w, int h, double x, double y) {
-    x -= 0.5;
-    y -= 0.5;
-    const int i = int(std::floor(x)), j = int(std::floor(y));
-    const double fx = x - i, fy = y - j;
-    auto at = [&](int a, int b) { return ld(img, int64_t(clampi(b, 0, h - 1)) * w + clampi(a, 0, w - 1)); };
-    return (1 - fy) * ((1 - fx) * at(i, j) + fx * at(i + 1, j)) + fy * ((1 - fx) * at(i, j + 1) + fx * at(i + 1, j + 1));
 }
 
 __device__ __forceinline__ double rrt_odt_fit(double x) {  // tonemap.h:63-67
@@ -191,18 +202,41 @@ template <bool BYTES>
 __global__ __launch_bounds__(256) void k_compose(const double* rad, int width, int height, pyramid py,
                                                  display_args a, byte_thresholds bt, double* out, uint8_t* rgb) {
     __shared__ double thr[256];
-    if constexpr (BYTES) {
-        thr[threadIdx.x] = bt.t[threadIdx.x];
-        __syncthreads();
+    __shared__ double win[kMaxLevels][kCWY][kCWX][3];
+    __shared__ int org[kMaxLevels][2];
+    if constexpr (BYTES) thr[threadIdx.x] = bt.t[threadIdx.x];
+    const int x0 = blockIdx.x * kCX, y0 = blockIdx.y * kCY;
+    // the sample coordinates of sample() (postprocess.h:35-42), as the pixels compute them
+    auto sx_of = [&](int k, int i) { return (i + 0.5) * py.sx[k] - 0.5; };
+    auto sy_of = [&](int k, int j) { return (j + 0.5) * py.sy[k] - 0.5; };
+    for (int k = 0; k < py.count; k++) {  // stage each level's window of texels (clamped as image::at)
+        const level_desc L = py.lv[k];
+        const int wx0 = int(std::floor(sx_of(k, x0))), wy0 = int(std::floor(sy_of(k, y0)));
+        const int nx = int(std::floor(sx_of(k, x0 + kCX - 1))) - wx0 + 2;
+        const int ny = int(std::floor(sy_of(k, y0 + kCY - 1))) - wy0 + 2;
+        if (threadIdx.x == 0) org[k][0] = wx0, org[k][1] = wy0;
+        for (int t = threadIdx.x; t < nx * ny; t += blockDim.x) {
+            const int c = t % nx, r = t / nx;
+            const v3 v = ld(L.blurred, int64_t(clampi(wy0 + r, 0, L.h - 1)) * L.w + clampi(wx0 + c, 0, L.w - 1));
+            win[k][r][c][0] = v.x, win[k][r][c][1] = v.y, win[k][r][c][2] = v.z;
+        }
     }
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= width) return;
+    __syncthreads();
+    const int i = x0 + int(threadIdx.x) % kCX, j = y0 + int(threadIdx.x) / kCX;
+    if (i >= width || j >= height) return;
     const int64_t p = int64_t(j) * width + i;
     v3 c = ld(rad, p);
     if (py.count > 0) {  // postprocess.h:128-136
         v3 glow{0, 0, 0};
-        for (int k = 0; k < py.count; k++)
-            glow = glow + sample(py.lv[k].blurred, py.lv[k].w, py.lv[k].h, (i + 0.5) * py.sx[k], (j + 0.5) * py.sy[k]);
+        for (int k = 0; k < py.count; k++) {
+            const double x = sx_of(k, i), y = sy_of(k, j);
+            const int ia = int(std::floor(x)), ja = int(std::floor(y));
+            const double fx = x - ia, fy = y - ja;
+            const int cc = ia - org[k][0], rr = ja - org[k][1];
+            auto at = [&](int r, int q) { return v3{win[k][r][q][0], win[k][r][q][1], win[k][r][q][2]}; };
+            glow = glow + ((1 - fy) * ((1 - fx) * at(rr, cc) + fx * at(rr, cc + 1)) +
+                           fy * ((1 - fx) * at(rr + 1, cc) + fx * at(rr + 1, cc + 1)));
+        }
         c = c + a.bloom_over_levels * glow;
     }
     if (a.vignette_on) {  // postprocess.h:139-150
@@ -319,15 +353,14 @@ int run(const double* rad, int width, int height, const mpt_display* display, bo
             py.lv[k].w = sizes[k].first;
             py.lv[k].h = sizes[k].second;
             py.lv[k].blurred = ws;
-            ws += 3 * n;
-            py.lv[k].level = nullptr;
-            if (k + 1 < py.count) {
-                py.lv[k].level = ws;
-                ws += 3 * n;
-            }
+            py.lv[k].level = ws + 3 * n;
+            ws += 6 * n;
             py.sx[k] = double(py.lv[k].w) / width;
             py.sy[k] = double(py.lv[k].h) / height;
         }
+        for (int k = 0; k < py.count; k++)
+            if (!(py.sx[k] <= 0.5 && py.sy[k] <= 0.5))  // a level is at most half the image: the windows fit
+                return set_error(MPT_E_INVALID, "display: pyramid level larger than half the image");
         if (py.count) a.bloom_over_levels = d.bloom / double(py.count);
         a.vignette_on = d.vignette > 0;
         a.vignette = d.vignette;
@@ -335,17 +368,18 @@ int run(const double* rad, int width, int height, const mpt_display* display, bo
         a.cy = 0.5 * height;
         a.corner2 = a.cx * a.cx + a.cy * a.cy;
         const blur_weights bw = gaussian_weights();
-        const double* src = rad;
-        int sw = width, sh = height;
         for (int k = 0; k < py.count; k++) {
-            const dim3 g(unsigned((py.lv[k].w + kTX - 1) / kTX), unsigned((py.lv[k].h + kTY - 1) / kTY));
-            hipLaunchKernelGGL(k_level, g, dim3(256), 0, s, src, sw, sh, int(k == 0), d.bloom_threshold, py.lv[k], bw);
-            src = py.lv[k].level;
-            sw = py.lv[k].w;
-            sh = py.lv[k].h;
+            level_desc L = py.lv[k];
+            if (k == 0) {  // the first level streams from the radiance, then blurs from its own buffer
+                hipLaunchKernelGGL(k_down, dim3(unsigned((L.w + 255) / 256), unsigned(L.h)), dim3(256), 0, s, rad, width,
+                                   height, d.bloom_threshold, L.level, L.w, L.h);
+            }
+            const dim3 g(unsigned((L.w + kTX - 1) / kTX), unsigned((L.h + kTY - 1) / kTY));
+            hipLaunchKernelGGL(k_level, g, dim3(256), 0, s, k ? py.lv[k - 1].level : nullptr, k ? py.lv[k - 1].w : 0,
+                               k ? py.lv[k - 1].h : 0, L, bw);
         }
     }
-    const dim3 g(unsigned((width + 255) / 256), unsigned(height));
+    const dim3 g(unsigned((width + kCX - 1) / kCX), unsigned((height + kCY - 1) / kCY));
     if (rgb) hipLaunchKernelGGL(k_compose<true>, g, dim3(256), 0, s, rad, width, height, py, a, thresholds(), out, rgb);
     else hipLaunchKernelGGL(k_compose<false>, g, dim3(256), 0, s, rad, width, height, py, a, byte_thresholds{}, out, rgb);
     DP_TRY(hipGetLastError());
@@ -404,8 +438,7 @@ size_t mpt_display_workspace_bytes(int32_t width, int32_t height, const mpt_disp
     else mpt_display_defaults(&d);
     const auto sizes = level_sizes(width, height, d);
     size_t doubles = 0;
-    for (size_t k = 0; k < sizes.size(); k++)
-        doubles += (k + 1 < sizes.size() ? 2 : 1) * 3 * size_t(sizes[k].first) * sizes[k].second;
+    for (size_t k = 0; k < sizes.size(); k++) doubles += 6 * size_t(sizes[k].first) * sizes[k].second;
     return doubles * sizeof(double);
 }
 

@@ -995,13 +995,27 @@ struct mpt_scene {
     int64_t tuned_pix = 0;
     int tuned_pif = 0;
     int64_t tuned_tail = 0;
+    int tuned_bpc = 0;    // blocks per CU
+    int tuned_sort = 0;   // 1 binned stages, -1 one kernel per batch
+    int tuned_graph = 0;  // 1 bounce graphs
+    // per-bounce graphs (use_graph), by bounce index, valid while graph_key holds
+    std::vector<hipGraphExec_t> graphs;
+    std::vector<unsigned char> graph_key;
     // radiance cache buffers (mpt_render_nrc)
     std::vector<void*> nrc_allocs;
     int64_t nrc_capacity = 0;
     nrc_bufs nb{};
     uint32_t* nrc_queue = nullptr;
 
+    void drop_graphs() {
+        for (hipGraphExec_t g : graphs)
+            if (g) (void)hipGraphExecDestroy(g);
+        graphs.clear();
+        graph_key.clear();
+    }
+
     ~mpt_scene() {
+        drop_graphs();
         for (void* p : nrc_allocs) (void)hipFree(p);
         for (void* p : allocs) (void)hipFree(p);
         for (void* p : ps_allocs) (void)hipFree(p);
@@ -1066,13 +1080,17 @@ int ensure_ovf(mpt_scene* s) {
     return MPT_OK;
 }
 
+constexpr int kDefaultBlocksPerCu = 16;
+
 int grid_for(mpt_scene* s) {
     // Persistent stage kernels walk their queue with a grid-wide stride. The
-    // grid is what the device holds when every kernel is at its best
-    // occupancy (8 blocks of 256 threads per CU); kernels that fit fewer
-    // blocks run the rest as a second round.
+    // default grid is 16 blocks of 256 threads per CU, twice what a CU holds
+    // at once for the traversal kernels: a second round of blocks starts as
+    // soon as the first ones run out of queue (2% faster than one round on
+    // the spheres scene, 3% on the generated batch); renders may set another
+    // size (blocks_per_cu) and the autotuner tries 8 and 32.
     s->block = 256;
-    s->grid = s->cu_count * 8;
+    s->grid = s->cu_count * kDefaultBlocksPerCu;
     return MPT_OK;
 }
 
@@ -1481,6 +1499,15 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
     if (int64_t(pif) * npix64 > (int64_t(1) << 31)) pif = int((int64_t(1) << 31) / npix64);
     int rc;
     if ((rc = ensure_paths(s, int64_t(pif) * npix))) return rc;
+    const bool tuned = s->tuned_pix == npix64;
+    int bpc = o.blocks_per_cu > 0 ? o.blocks_per_cu : (tuned && s->tuned_bpc > 0 ? s->tuned_bpc : kDefaultBlocksPerCu);
+    if (bpc > 64) return set_error(MPT_E_INVALID, "blocks_per_cu must be in [1, 64]");
+    if (s->grid != s->cu_count * bpc) {  // the persistent grid, and the stack spill area sized for it
+        s->grid = s->cu_count * bpc;
+        if ((rc = ensure_ovf(s))) return rc;
+    }
+    const bool sorted = o.sort_by_material > 0 || (o.sort_by_material == 0 && !(tuned && s->tuned_sort < 0));
+    if (!sorted && nrc) return set_error(MPT_E_INVALID, "the radiance cache needs sort_by_material (it cuts paths at a bounce)");
     const bool keep = o.accumulate && s->acc && s->acc_pix == npix && s->last_done > 0 &&
                       s->last_cam.collect_features == cam->collect_features;
     if (o.accumulate && !keep && s->last_done > 0)
@@ -1514,6 +1541,8 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
     HIP_TRY(hipMemsetAsync(ray_stats, 0, 8 * sizeof(unsigned long long), s->stream));
     const bool timing = (o.profile & 1) != 0;
     const bool counting = (o.profile & 2) != 0;
+    const bool graphs = (o.use_graph > 0 || (o.use_graph == 0 && tuned && s->tuned_graph > 0)) && !timing && !counting &&
+                        !nrc && sorted;
 
     // per-stage event timing (profile bit 0): events are recorded around
     // each launch and read back after each batch
@@ -1700,6 +1729,52 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
 #undef FIN2
     };
 
+    if (graphs) {  // graphs captured earlier hold kernel arguments: drop them if any has changed
+        std::vector<unsigned char> key(sizeof(mpt_camera));
+        std::memcpy(key.data(), cam, sizeof(mpt_camera));
+        const void* ptrs[] = {s->queues, s->counters, s->ovf, P.d[0], P.key};
+        const int64_t vals[] = {cap, int64_t(G), s->profile, s->light_profile, int64_t(s->mat_types), has_env,
+                                s->has_lights};
+        key.insert(key.end(), reinterpret_cast<const unsigned char*>(ptrs), reinterpret_cast<const unsigned char*>(ptrs) + sizeof(ptrs));
+        key.insert(key.end(), reinterpret_cast<const unsigned char*>(vals), reinterpret_cast<const unsigned char*>(vals) + sizeof(vals));
+        if (key != s->graph_key) {
+            s->drop_graphs();
+            s->graph_key = key;
+        }
+        if (int(s->graphs.size()) < C.max_depth) s->graphs.resize(size_t(C.max_depth), nullptr);
+    }
+    // one bounce's kernels: extend, shade per material class present, shadow
+    // (or the class-counter reset); extend zeroes the next-queue and shadow
+    // counters, shadow zeroes the class counters once every shade kernel read them
+    auto launch_bounce = [&](int depth, uint32_t* qin, uint32_t* qout, uint32_t* cin, uint32_t* cout) {
+        launch_extend(depth, qin, cin, cout);
+        for (int cls = 0; cls < kClasses; cls++) {
+            if (cls > 0 && !(s->mat_types & (1u << (cls - 1)))) continue;
+            launch_shade(cls, depth, qout, cout);
+        }
+        if (s->has_lights) launch_shadow(depth);
+        else hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, ccls, kClasses);
+    };
+    // the bounce as a graph: captured at its first use, replayed after
+    auto graph_bounce = [&](int depth, uint32_t* qin, uint32_t* qout, uint32_t* cin, uint32_t* cout) -> int {
+        hipGraphExec_t& ge = s->graphs[size_t(depth)];
+        if (!ge) {
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            launch_bounce(depth, qin, qout, cin, cout);
+            const hipError_t e = hipStreamEndCapture(stream, &g);
+            if (e != hipSuccess || !g) return set_error(MPT_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) {
+                ge = nullptr;
+                return set_error(MPT_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+            }
+        }
+        HIP_TRY(hipGraphLaunch(ge, stream));
+        return MPT_OK;
+    };
+
     int pass = 0;
     while (pass < total) {
         int nb = std::min(pif, total - pass);
@@ -1713,7 +1788,8 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
         EV(3, hipLaunchKernelGGL(k_camera, dim3(unsigned((B.nslots + 255) / 256)), dim3(256), 0, stream, C, P, B, q[0],
                                  cnt));
         if (C.features) EV(3, hipLaunchKernelGGL((k_features<256>), dim3(G), dim3(256), 0, stream, DS, L, P, B));
-        for (int depth = 0; depth < C.max_depth; depth++) {
+        if (!sorted) EV(4, launch_finish(0, q[0], ctr(cnt, 0)));  // every bounce in one kernel, no binning
+        for (int depth = 0; sorted && depth < C.max_depth; depth++) {
             uint32_t* qin = q[depth & 1];
             if (depth >= 2) {  // the queue length two bounces ago bounds this one's
                 if (hipEventSynchronize(ring[(depth - 2) % kRing]) != hipSuccess) {
@@ -1730,6 +1806,16 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
             uint32_t* qout = q[(depth + 1) & 1];
             uint32_t* cin = ctr(cnt, depth & 1);
             uint32_t* cout = ctr(cnt, (depth + 1) & 1);
+            if (graphs) {
+                if ((rc = graph_bounce(depth, qin, qout, cin, cout))) {
+                    ev_free();
+                    return rc;
+                }
+                HIP_TRY(hipMemcpyAsync(&s->pinned_counts[depth % kRing], cout, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                       stream));
+                HIP_TRY(hipEventRecord(ring[depth % kRing], stream));
+                continue;
+            }
             // extend zeroes the next-queue and shadow counters of this bounce;
             // shadow zeroes the class counters once every shade kernel read them
             EV(0, launch_extend(depth, qin, cin, cout));
@@ -1862,9 +1948,8 @@ int32_t mpt_autotune(mpt_scene* s, const mpt_camera* cam, int32_t trial_passes, 
     if (trial_passes < 1) trial_passes = 8;
     const dcam C = init_camera(*cam, s->hs.env_w > 0);
     const int64_t npix = int64_t(C.w) * C.h;
-    const int64_t nthreads = int64_t(s->grid) * s->block;
     // candidates: paths in flight (passes per batch) around 4, 8, 16, 32 M
-    // paths, and the batch tail's size in paths per thread of the grid
+    // paths, and the batch tail's size in paths per thread of the default grid
     std::vector<int> pifs;
     for (int64_t target : {int64_t(4) << 20, int64_t(8) << 20, int64_t(16) << 20, int64_t(32) << 20}) {
         int v = int(std::max<int64_t>(1, target / npix));
@@ -1872,15 +1957,35 @@ int32_t mpt_autotune(mpt_scene* s, const mpt_camera* cam, int32_t trial_passes, 
         if (int64_t(v) * npix > (int64_t(1) << 31)) continue;
         if (std::find(pifs.begin(), pifs.end(), v) == pifs.end()) pifs.push_back(v);
     }
+    const int64_t nthreads = int64_t(s->cu_count) * kDefaultBlocksPerCu * 256;
     const int64_t tails[] = {-1, nthreads / 2, nthreads, 2 * nthreads, 4 * nthreads};
     mpt_render_opts o{};
     o.seed = 0x5EED;
     o.pass_count = trial_passes;
+    o.blocks_per_cu = kDefaultBlocksPerCu;
+    o.sort_by_material = 1;
+    o.use_graph = -1;
     s->tuned_pix = 0;
-    double best = 1e300;
-    int best_pif = 0;
-    int64_t best_tail = 0;
+    mpt_autotune_result best{};
+    best.seconds_per_pass = 1e300;
     out->trials = 0;
+    auto trial = [&]() -> int {
+        auto t0 = std::chrono::steady_clock::now();
+        int rc = render_impl(s, cam, &o, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                             nullptr);
+        if (rc) return rc;
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / trial_passes;
+        out->trials++;
+        if (dt < best.seconds_per_pass) {
+            best.seconds_per_pass = dt;
+            best.passes_in_flight = o.passes_in_flight;
+            best.tail_paths = o.tail_paths;
+            best.blocks_per_cu = o.blocks_per_cu;
+            best.sort_by_material = o.sort_by_material;
+            best.use_graph = o.use_graph;
+        }
+        return MPT_OK;
+    };
     // a warm-up render sizes the buffers for the largest batch
     o.passes_in_flight = *std::max_element(pifs.begin(), pifs.end());
     int rc = render_impl(s, cam, &o, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
@@ -1890,20 +1995,48 @@ int32_t mpt_autotune(mpt_scene* s, const mpt_camera* cam, int32_t trial_passes, 
         for (int64_t tail : tails) {
             o.passes_in_flight = pif;
             o.tail_paths = tail < 0 ? -1 : int32_t(std::min<int64_t>(tail, INT32_MAX));
-            auto t0 = std::chrono::steady_clock::now();
-            rc = render_impl(s, cam, &o, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                             nullptr, nullptr);
-            if (rc) return rc;
-            const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            out->trials++;
-            if (dt < best) best = dt, best_pif = pif, best_tail = tail;
+            if ((rc = trial())) return rc;
         }
-    s->tuned_pix = npix;
-    s->tuned_pif = best_pif;
-    s->tuned_tail = best_tail;
-    out->passes_in_flight = best_pif;
-    out->tail_paths = best_tail < 0 ? -1 : int64_t(best_tail);
-    out->seconds_per_pass = best / trial_passes;
+    // from the best: the grid, then no binning, then graphs
+    o.passes_in_flight = best.passes_in_flight;
+    o.tail_paths = int32_t(best.tail_paths);
+    for (int bpc : {8, 32}) {
+        o.blocks_per_cu = bpc;
+        if ((rc = trial())) return rc;
+    }
+    o.blocks_per_cu = best.blocks_per_cu;
+    o.sort_by_material = -1;
+    if ((rc = trial())) return rc;
+    o.sort_by_material = 1;
+    o.use_graph = 1;
+    if ((rc = trial())) return rc;  // the first render captures; time the replays
+    if ((rc = trial())) return rc;
+    best.trials = out->trials;
+    *out = best;
+    return mpt_autotune_apply(s, cam, out);
+}
+
+int32_t mpt_autotune_apply(mpt_scene* s, const mpt_camera* cam, const mpt_autotune_result* r) {
+    if (!s || !cam || !r) return set_error(MPT_E_INVALID, "null argument");
+    if (r->passes_in_flight < 1 || r->blocks_per_cu < 1 || r->blocks_per_cu > 64)
+        return set_error(MPT_E_INVALID, "autotune settings out of range");
+    const dcam C = init_camera(*cam, s->hs.env_w > 0);
+    s->tuned_pix = int64_t(C.w) * C.h;
+    s->tuned_pif = r->passes_in_flight;
+    s->tuned_tail = r->tail_paths < 0 ? -1 : r->tail_paths;
+    s->tuned_bpc = r->blocks_per_cu;
+    s->tuned_sort = r->sort_by_material < 0 ? -1 : 1;
+    s->tuned_graph = r->use_graph > 0 ? 1 : -1;
+    return MPT_OK;
+}
+
+int32_t mpt_autotune_key(const mpt_scene* s, const mpt_camera* cam, char* buf, int32_t len) {
+    if (!s || !cam || !buf || len < 1) return set_error(MPT_E_INVALID, "null argument");
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, s->device));
+    const dcam C = init_camera(*cam, s->hs.env_w > 0);
+    std::snprintf(buf, size_t(len), "%s/%dcu/geometry%d/materials%x/lights%d/env%d/%dx%d", prop.gcnArchName,
+                  s->cu_count, s->profile, s->mat_types, s->light_profile, s->hs.env_w > 0 ? 1 : 0, C.w, C.h);
     return MPT_OK;
 }
 

@@ -266,6 +266,22 @@ def test_gpu_odd_sizes_and_many_levels():
             assert np.array_equal(m.display_bytes(img, d), ol.oracle_display_bytes(img, st)), (h, w, st)
 
 
+@pytest.mark.gpu
+def test_gpu_pyramid_splits():
+    """Pyramids whose first level is large and the rest small, and a single
+    large level: every split between the per-level kernels and the one-block
+    tail kernel."""
+    import mypbr_amd as m
+    rng = np.random.default_rng(15)
+    cases = [((300, 400), (0.0, 0, 0.4, 0.5, 0.04, 0.0)), ((1080, 1920), (0.0, 2, 0.3, 1.0, 0.001, 0.2)),
+             ((540, 960), (1.0, 1, 0.6, 0.8, 0.3, 0.0))]
+    for (h, w), st in cases:
+        img = rng.lognormal(-0.5, 1.5, (h, w, 3))
+        d = _display(m, st)
+        assert np.array_equal(_bits(m.postprocess(img, d)), _bits(ol.oracle_postprocess(img, st))), (h, w, st)
+        assert np.array_equal(m.display_bytes(img, d), ol.oracle_display_bytes(img, st)), (h, w, st)
+
+
 def _png_pixels(data):
     assert data[:8] == b"\x89PNG\r\n\x1a\n"
     p, idat = 8, b""

@@ -85,14 +85,51 @@ def test_batch_tail_in_one_launch_is_identical(mpt, name, width, spp, depth):
         assert f.stats.shadow_rays == staged.stats.shadow_rays
 
 
-def test_autotune_picks_settings_that_leave_the_image_unchanged(mpt):
+@pytest.mark.parametrize("name,width,spp,depth", [
+    ("cornell.scene", 48, 8, 16),
+    ("weekend_cover.scene", 64, 6, 50),
+    ("test_lights.scene", 48, 4, 12),
+    ("test_media.scene", 48, 4, 12),
+])
+def test_grid_binning_and_graphs_leave_the_frame_unchanged(mpt, name, width, spp, depth):
+    """The knobs of mpt_render_opts that only schedule the work: the persistent
+    grid (blocks per CU), material binning off (one kernel per batch) and
+    bounce graphs (captured on first use, replayed on later renders, dropped
+    when the camera changes). Every frame, guides included, must be
+    bit-identical to the default schedule's."""
+    from mypbr_amd.abi import camera_copy
+    d = mpt.describe(scene_path(name))
+    s = mpt.Scene.from_desc(d.desc, d.camera)
+    cams = [camera_copy(d.camera, image_width=width, samples_per_pixel=spp, max_depth=depth, collect_features=1),
+            camera_copy(d.camera, image_width=width + 8, samples_per_pixel=spp, max_depth=depth, collect_features=1)]
+    for cam in cams:
+        base = s.render(cam, seed=13, use_graph=-1, sort_by_material=1, blocks_per_cu=8, passes_in_flight=2)
+        variants = [dict(blocks_per_cu=4), dict(blocks_per_cu=16), dict(sort_by_material=-1),
+                    dict(use_graph=1), dict(use_graph=1), dict(use_graph=1, blocks_per_cu=4)]
+        for v in variants:
+            f = s.render(cam, seed=13, passes_in_flight=2, **v)
+            for k in ("radiance", "albedo", "normal", "variance"):
+                assert np.array_equal(getattr(f, k), getattr(base, k)), (name, cam.image_width, v, k)
+            assert f.stats.extension_rays == base.stats.extension_rays, v
+
+
+def test_autotune_picks_settings_that_leave_the_image_unchanged(mpt, tmp_path):
     from mypbr_amd.abi import camera_copy
     d = mpt.describe(scene_path("weekend_cover.scene"))
     s = mpt.Scene.from_desc(d.desc, d.camera)
     cam = camera_copy(d.camera, image_width=160, samples_per_pixel=12, max_depth=20)
-    r = s.autotune(cam, trial_passes=4)
+    cache = str(tmp_path / "tune.json")
+    r = s.autotune(cam, trial_passes=4, cache=cache)
     print(r, flush=True)
-    assert r["passes_in_flight"] >= 1 and r["trials"] >= 5 and r["seconds_per_pass"] > 0
+    assert not r["cached"]
+    assert r["passes_in_flight"] >= 1 and r["trials"] >= 9 and r["seconds_per_pass"] > 0
+    assert r["blocks_per_cu"] in (8, 16, 32) and r["sort_by_material"] in (-1, 1) and r["use_graph"] in (-1, 1)
     tuned = s.render(cam, seed=5).radiance
-    other = s.render(cam, seed=5, passes_in_flight=5, tail_paths=-1).radiance
+    other = s.render(cam, seed=5, passes_in_flight=5, tail_paths=-1, blocks_per_cu=8, sort_by_material=1,
+                     use_graph=-1).radiance
     assert np.array_equal(tuned, other)
+    # a second scene object of the same class finds the stored result and renders the same frame
+    s2 = mpt.Scene.from_desc(d.desc, d.camera)
+    r2 = s2.autotune(cam, cache=cache)
+    assert r2["cached"] and all(r2[k] == r[k] for k in s2._TUNED)
+    assert np.array_equal(s2.render(cam, seed=5).radiance, other)
