@@ -147,6 +147,14 @@ typedef struct mpt_scene_desc {
     const double* env_rgb;         /* [env_width * env_height * 3], linear RGB */
     double env_intensity;
     double env_rotation_degrees;
+
+    /* Per-vertex shading normals of triangles (mesh_data::normals, mesh.h:14-19):
+     * [n_prims * 9] as n0 xyz, n1 xyz, n2 xyz, or NULL for none. A triangle
+     * whose first value is NaN has none (flat shading); values of other
+     * primitive kinds are ignored. Interpolated at the hit with the
+     * barycentrics, normalised and turned to the geometric normal's side
+     * (mesh.h:93-101). */
+    const double* prim_normals;
 } mpt_scene_desc;
 
 /* Camera and render settings: the public fields of the reference camera

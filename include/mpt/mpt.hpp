@@ -240,7 +240,27 @@ public:
     std::vector<point3> positions;
     std::vector<int32_t> indices;
     std::shared_ptr<material> mat;
+    // optional per-vertex shading normals (mesh_data::normals, mesh.h:14-19):
+    // normal_indices parallels indices; a triangle with any -1 is flat
+    std::vector<vec3> normals;
+    std::vector<int32_t> normal_indices;
 };
+
+// A part of an OBJ file that uses one material (obj_loader.h:19-23).
+struct obj_part {
+    std::shared_ptr<material> mat;
+    std::shared_ptr<mesh> geometry;
+};
+
+// load_obj (obj_loader.h:170-340): v, vn, vt, f (fans; v, v/t, v//n, v/t/n;
+// negative indices), usemtl, mtllib with load_mtl's mapping (Kd, Ks/Ns/Ni,
+// Ke). Image textures, alpha masks and normal maps make it fail: the GPU
+// path does not model them (a texture file that does not exist falls back to
+// the plain color with a warning, as in the reference). Parts come in the
+// order of each material's first face.
+bool load_obj(const std::string& path, const std::map<std::string, std::shared_ptr<material>>& materials,
+              std::shared_ptr<material> default_mat, bool smooth, std::vector<obj_part>& parts, std::string& error,
+              bool use_normal_maps = true);
 
 class instance : public hittable {
 public:
@@ -411,6 +431,7 @@ public:
     int32_t material_index(const std::shared_ptr<material>& m);
     int32_t xform_index(const transform& x);
     int64_t add_prim(int32_t kind, int32_t mat, const double* data9);
+    void set_prim_normals(int64_t prim, const double* n9);  // a triangle's three vertex normals
     int32_t add_object(int32_t kind, int64_t first, int64_t count, int32_t xform, const hittable* owner);
     std::map<const void*, std::pair<int64_t, int64_t>> mesh_ranges;
     struct impl;

@@ -67,6 +67,7 @@ class SceneDesc(C.Structure):
         ("env_rgb", C.POINTER(C.c_double)),
         ("env_intensity", C.c_double),
         ("env_rotation_degrees", C.c_double),
+        ("prim_normals", C.POINTER(C.c_double)),
     ]
 
 

@@ -495,7 +495,8 @@ enum : int {
     L_TRI = 8,
     L_LIST = 16,
     L_XFORM = 32,
-    L_ALL = 63,
+    L_MESH = 64,
+    L_ALL = 127,
 };
 
 template <int LK>
@@ -600,7 +601,16 @@ __device__ __forceinline__ double object_light_pdf(const dscene& S, int ob, v3 o
             d = ld;
         }
     }
-    if (!(LK & L_LIST) || r.kind == MPT_OBJ_PRIM) return prim_light_pdf<LK>(S, r.first, o, d);
+    if (!(LK & (L_LIST | L_MESH)) || r.kind == MPT_OBJ_PRIM) return prim_light_pdf<LK>(S, r.first, o, d);
+    if ((LK & L_MESH) && r.kind == MPT_OBJ_MESH) {  // mesh::pdf_value (mesh.h:197-206): area-weighted over its triangles
+        const int64_t* tri = S.mlight_prim + r.light_table;
+        const double total = S.mlight_cdf[r.light_table + r.count - 1];
+        if (total <= 0) return 0;
+        double sum = 0;
+        for (int64_t k = 0; k < r.count; k++)
+            sum += (prim_ptr(S, tri[k])[12] / total) * prim_light_pdf<LK>(S, tri[k], o, d);
+        return sum;
+    }
     double weight = 1.0 / double(r.count);
     double sum = 0.0;
     for (int64_t k = 0; k < r.count; k++) sum += weight * prim_light_pdf<LK>(S, r.first + k, o, d);
@@ -617,8 +627,19 @@ __device__ __forceinline__ v3 object_light_random(const dscene& S, int ob, const
         if (x) lo = xf_inv_point(*x, o);
     }
     v3 res;
-    if (!(LK & L_LIST) || r.kind == MPT_OBJ_PRIM) {
+    if (!(LK & (L_LIST | L_MESH)) || r.kind == MPT_OBJ_PRIM) {
         res = prim_light_random<LK>(S, r.first, lo, rs);
+    } else if ((LK & L_MESH) && r.kind == MPT_OBJ_MESH) {  // mesh::random (mesh.h:208-213)
+        const double* cdf = S.mlight_cdf + r.light_table;
+        const int64_t n = r.count;
+        const double x = rs.next() * cdf[n - 1];
+        int64_t lo_i = 0, hi_i = n;  // upper_bound: the first running sum above x
+        while (lo_i < hi_i) {
+            const int64_t mid = (lo_i + hi_i) >> 1;
+            if (x < cdf[mid]) hi_i = mid;
+            else lo_i = mid + 1;
+        }
+        res = prim_light_random<LK>(S, S.mlight_prim[r.light_table + (lo_i < n - 1 ? lo_i : n - 1)], lo, rs);
     } else {
         int n = int(r.count);
         int idx = int(rs.next() * n);

@@ -38,6 +38,9 @@ struct dscene {
     const mpt_texture* tex;
     const light_rec* lights;
     const double* lcdf;
+    const double* vnrm;          // triangles' vertex normals, 9 per primitive (NaN: none), or null
+    const int64_t* mlight_prim;  // mesh lights: triangles in mesh order, and their area CDF
+    const double* mlight_cdf;
     const double* env_rgb;
     const double* env_cond;
     const double* env_marg;
@@ -430,6 +433,7 @@ struct hitrec {
     bool front;
 };
 
+template <int FEAT = F_ALL>
 __device__ __forceinline__ void finish_hit(const dscene& S, const hitinfo& h, const v3& wo, const v3& wd, hitrec& rec) {
     const obj_rec r = S.obj[h.obj];
     rec.t = h.t;
@@ -457,6 +461,23 @@ __device__ __forceinline__ void finish_hit(const dscene& S, const hitinfo& h, co
     }
     rec.front = dot(d, outward) < 0;
     v3 n = rec.front ? outward : -outward;
+    if ((FEAT & F_TRI) && S.vnrm && k == MPT_PRIM_TRIANGLE && !std::isnan(S.vnrm[9 * h.prim])) {
+        // interpolated shading normal on the geometric normal's side (mesh.h:93-101);
+        // the barycentrics by the arithmetic of the hit test
+        const double* vn = S.vnrm + 9 * h.prim;
+        const v3 p0{q[0], q[1], q[2]}, e1{q[3], q[4], q[5]}, e2{q[6], q[7], q[8]};
+        const v3 pvec = cross(d, e2);
+        const double inv_det = 1 / dot(e1, pvec);
+        const v3 tvec = o - p0;
+        const double b1 = dot(tvec, pvec) * inv_det;
+        const double b2 = dot(d, cross(tvec, e1)) * inv_det;
+        const double b0 = 1 - b1 - b2;
+        v3 sh = b0 * v3{vn[0], vn[1], vn[2]} + b1 * v3{vn[3], vn[4], vn[5]} + b2 * v3{vn[6], vn[7], vn[8]};
+        if (length_squared(sh) > 0) {
+            sh = unit_vector(sh);
+            n = dot(sh, n) < 0 ? -sh : sh;
+        }
+    }
     if (x) {
         p = xf_point(*x, p);
         n = unit_vector(xf_normal(*x, n));

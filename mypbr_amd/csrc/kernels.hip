@@ -287,7 +287,7 @@ __global__ __launch_bounds__(BS) void k_query(dscene S, launch_ctx L, const doub
         bool hit = trace_closest<false, FEAT>(S, o, d, tt[2 * i], tt[2 * i + 1], h, st, false, 0, 0, wc);
         if (hit) {
             hitrec rec;
-            finish_hit(S, h, o, d, rec);
+            finish_hit<FEAT>(S, h, o, d, rec);
             out_id[i] = S.obj[h.obj].id_base + S.pcanon[h.prim];
             out_t[i] = h.t;
             out_front[i] = rec.front ? 1 : 0;
@@ -470,7 +470,7 @@ __device__ __forceinline__ int extend_one(const dscene& S, const pstate& P, int 
     P.hprim[p] = h.prim;
     if (h.obj < 0) return 0;
     hitrec rec;
-    finish_hit(S, h, o, d, rec);
+    finish_hit<FEAT>(S, h, o, d, rec);
     st3(P, D_QX, p, rec.p);
     st3(P, D_MX, p, rec.normal);
     P.hmat[p] = uint32_t(rec.mat) | (rec.front ? 0x80000000u : 0u);
@@ -696,7 +696,7 @@ __device__ __forceinline__ void shadow_one(const dscene& S, const dcam& C, const
                 break;
             }
             hitrec rec;
-            finish_hit(S, h, o, dir, rec);
+            finish_hit<FEAT>(S, h, o, dir, rec);
             const mpt_material& m = S.mat[rec.mat];
             if (C.transparent_shadows && k < 16 && mat_lets_shadows(m)) {
                 through = through * mat_interior(m, dir, rec);
@@ -1174,6 +1174,9 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
     if ((rc = upload(s.get(), h.textures, &d.tex))) return rc;
     if ((rc = upload(s.get(), h.lights, &d.lights))) return rc;
     if ((rc = upload(s.get(), h.light_cdf, &d.lcdf))) return rc;
+    if ((rc = upload(s.get(), h.mlight_prim, &d.mlight_prim))) return rc;
+    if ((rc = upload(s.get(), h.prim_nrm, &d.vnrm))) return rc;
+    if ((rc = upload(s.get(), h.mlight_cdf, &d.mlight_cdf))) return rc;
     if ((rc = upload(s.get(), h.env_rgb, &d.env_rgb))) return rc;
     if ((rc = upload(s.get(), h.env_cond_cdf, &d.env_cond))) return rc;
     if ((rc = upload(s.get(), h.env_marg_cdf, &d.env_marg))) return rc;

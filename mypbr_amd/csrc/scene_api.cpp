@@ -5,11 +5,14 @@
 #include "../../include/mpt/mpt.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iostream>
 #include <sstream>
+#include <tuple>
 
 #include "geom.hpp"
 #include "scene_host.hpp"
@@ -254,6 +257,7 @@ std::shared_ptr<hittable_list> box(const point3& a, const point3& b, std::shared
 struct flat_builder::impl {
     std::vector<int32_t> prim_kind, prim_mat;
     std::vector<double> prim_data;
+    std::vector<double> prim_nrm;  // 9 per primitive once any triangle has vertex normals
     std::vector<int32_t> obj_kind, obj_xform, obj_mat, obj_bkind;
     std::vector<int64_t> obj_first, obj_count, obj_id;
     std::vector<double> obj_density;
@@ -303,7 +307,19 @@ int64_t flat_builder::add_prim(int32_t kind, int32_t mat, const double* data9) {
     p->prim_kind.push_back(kind);
     p->prim_mat.push_back(mat);
     p->prim_data.insert(p->prim_data.end(), data9, data9 + 9);
+    if (!p->prim_nrm.empty()) {
+        p->prim_nrm.push_back(std::nan(""));
+        p->prim_nrm.insert(p->prim_nrm.end(), 8, 0.0);
+    }
     return int64_t(p->prim_kind.size() - 1);
+}
+
+void flat_builder::set_prim_normals(int64_t prim, const double* n9) {
+    if (p->prim_nrm.empty()) {  // every primitive so far has none
+        p->prim_nrm.assign(9 * p->prim_kind.size(), 0.0);
+        for (size_t k = 0; k < p->prim_kind.size(); k++) p->prim_nrm[9 * k] = std::nan("");
+    }
+    std::copy(n9, n9 + 9, p->prim_nrm.begin() + 9 * prim);
 }
 
 int32_t flat_builder::add_object(int32_t kind, int64_t first, int64_t count, int32_t xform, const hittable* owner) {
@@ -391,6 +407,20 @@ void mesh::flatten(flat_builder& b, int32_t xform) const {
             }
             int64_t pi = b.add_prim(MPT_PRIM_TRIANGLE, mi, d);
             if (first < 0) first = pi;
+            if (!normal_indices.empty()) {
+                if (normal_indices.size() != indices.size()) throw std::runtime_error("mesh normal indices must parallel its indices");
+                double n[9];
+                bool all = true;
+                for (int c = 0; c < 3; c++) {
+                    int32_t ni = normal_indices[k + c];
+                    if (ni < 0 || size_t(ni) >= normals.size()) {
+                        all = false;
+                        break;
+                    }
+                    for (int a = 0; a < 3; a++) n[3 * c + a] = normals[ni][a];
+                }
+                if (all) b.set_prim_normals(pi, n);
+            }
         }
         b.mesh_ranges[this] = {first, count};
     } else {
@@ -491,6 +521,7 @@ const mpt_scene_desc& flat_builder::desc() {
     d.prim_kind = p->prim_kind.data();
     d.prim_material = p->prim_mat.data();
     d.prim_data = p->prim_data.data();
+    d.prim_normals = p->prim_nrm.empty() ? nullptr : p->prim_nrm.data();
     d.n_objects = int32_t(p->obj_kind.size());
     d.object_kind = p->obj_kind.data();
     d.object_first = p->obj_first.data();
@@ -693,6 +724,258 @@ bool write_image(const std::string& path, int width, int height, const std::vect
     return true;
 }
 
+// ------------------------------------------------------------------ OBJ / MTL (obj_loader.h)
+namespace {
+
+bool file_exists(const std::string& path) {
+    std::ifstream f(path);
+    return bool(f);
+}
+
+// load_mtl (obj_loader.h:61-168): Kd -> lambertian, a non-black Ks -> a
+// principled layer with roughness from the Phong exponent Ns and Ni, Ke ->
+// an emitter. Texture statements name images the GPU path cannot use: a file
+// that exists makes loading fail, a missing one is skipped with a warning (the
+// reference then falls back to the plain color too).
+bool load_mtl(const std::string& path, std::map<std::string, std::shared_ptr<material>>& result, std::string& error,
+              bool use_normal_maps) {
+    std::ifstream file(path);
+    if (!file) {
+        error = "cannot open '" + path + "'";
+        return false;
+    }
+    auto slash = path.find_last_of('/');
+    std::string dir = slash == std::string::npos ? "" : path.substr(0, slash + 1);
+    struct entry {
+        color kd{0.8, 0.8, 0.8}, ks{0, 0, 0}, ke{0, 0, 0};
+        double ns = 0, ni = 1.5;
+    };
+    std::vector<std::pair<std::string, entry>> entries;
+    std::string line;
+    while (std::getline(file, line)) {
+        std::istringstream in(line);
+        std::string keyword;
+        if (!(in >> keyword) || keyword[0] == '#') continue;
+        if (keyword == "newmtl") {
+            std::string name;
+            in >> name;
+            entries.push_back({name, entry()});
+            continue;
+        }
+        if (entries.empty()) continue;
+        entry& e = entries.back().second;
+        auto read_color = [&](color& c) {  // one value is grey
+            double r, g, b;
+            if (in >> r) {
+                c = color(r, r, r);
+                if (in >> g >> b) c = color(r, g, b);
+            }
+        };
+        auto texture_file = [&]() {  // options come first: the file is the last word
+            std::string word, last;
+            while (in >> word) last = word;
+            return last;
+        };
+        if (keyword == "Kd") read_color(e.kd);
+        else if (keyword == "Ks") read_color(e.ks);
+        else if (keyword == "Ke") read_color(e.ke);
+        else if (keyword == "Ns") in >> e.ns;
+        else if (keyword == "Ni") in >> e.ni;
+        else {
+            const bool color_map = keyword == "map_Kd" || keyword == "map_d";
+            const bool normal_map = keyword == "norm" || keyword == "map_Bump" || keyword == "map_bump" ||
+                                    keyword == "bump" || keyword == "map_Disp" || keyword == "disp";
+            if (!color_map && !normal_map) continue;  // Ka, illum, d, ...
+            std::string f = texture_file();
+            if (f.empty() || (normal_map && !use_normal_maps)) continue;
+            std::string full = f[0] == '/' ? f : dir + f;
+            if (!file_exists(full)) {
+                std::fprintf(stderr, "Warning: %s: cannot load texture '%s'\n", path.c_str(), full.c_str());
+                continue;
+            }
+            error = path + ": '" + f + "': image textures, alpha masks and normal maps are not modelled by the GPU path";
+            return false;
+        }
+    }
+    for (auto& [name, e] : entries) {
+        std::shared_ptr<material> m;
+        if (e.ke.length_squared() > 0) {
+            m = std::make_shared<diffuse_light>(e.ke);
+        } else if (e.ks.length_squared() > 0) {
+            // a Phong exponent n spreads like a microfacet alpha of sqrt(2 / (n + 2)); roughness is sqrt(alpha)
+            double roughness = std::sqrt(std::sqrt(2 / (std::fmax(e.ns, 0.0) + 2)));
+            double ior = e.ni > 1 ? e.ni : 1.5;
+            m = std::make_shared<principled>(std::make_shared<solid_color>(e.kd), 0.0, roughness, ior);
+        } else {
+            m = std::make_shared<lambertian>(e.kd);
+        }
+        result[name] = m;
+    }
+    return true;
+}
+
+}  // namespace
+
+bool load_obj(const std::string& path, const std::map<std::string, std::shared_ptr<material>>& materials,
+              std::shared_ptr<material> default_mat, bool smooth, std::vector<obj_part>& parts, std::string& error,
+              bool use_normal_maps) {
+    std::ifstream file(path);
+    if (!file) {
+        error = "cannot open '" + path + "'";
+        return false;
+    }
+    std::vector<point3> positions;
+    std::vector<vec3> normals;
+    size_t n_uvs = 0;  // texture coordinates are parsed for indexing; no material here reads them
+    struct face {
+        int v[3], n[3];
+    };
+    // faces per material, the materials in the order of their first face
+    std::vector<std::pair<std::shared_ptr<material>, std::vector<face>>> groups;
+    std::map<const material*, size_t> group_of;
+    std::shared_ptr<material> current = default_mat;
+    auto slash = path.find_last_of('/');
+    std::string dir = slash == std::string::npos ? "" : path.substr(0, slash + 1);
+    std::map<std::string, std::shared_ptr<material>> file_materials;
+    auto argument = [](const char* q) {  // the rest of the line without surrounding blanks
+        std::string arg(q);
+        arg.erase(0, arg.find_first_not_of(" \t"));
+        arg.erase(arg.find_last_not_of(" \t\r") + 1);
+        return arg;
+    };
+    auto resolve = [](long index, size_t count) -> int {  // 1-based, or negative from the end; -1 if absent
+        if (index > 0 && size_t(index) <= count) return int(index - 1);
+        if (index < 0 && size_t(-index) <= count) return int(long(count) + index);
+        return -1;
+    };
+    std::string line;
+    int line_number = 0;
+    while (std::getline(file, line)) {
+        line_number++;
+        const char* q = line.c_str();
+        while (*q == ' ' || *q == '\t') q++;
+        auto read_vec = [&](const char* r, int n) {
+            double x[3] = {0, 0, 0};
+            char* end;
+            for (int k = 0; k < n; k++) {
+                x[k] = std::strtod(r, &end);
+                r = end;
+            }
+            return vec3(x[0], x[1], x[2]);
+        };
+        const size_t at = size_t(q - line.c_str());
+        if (q[0] == 'v' && (q[1] == ' ' || q[1] == '\t')) {
+            positions.push_back(read_vec(q + 2, 3));
+        } else if (q[0] == 'v' && q[1] == 'n') {
+            normals.push_back(read_vec(q + 2, 3));
+        } else if (q[0] == 'v' && q[1] == 't') {
+            n_uvs++;
+        } else if (q[0] == 'f' && (q[1] == ' ' || q[1] == '\t')) {
+            std::vector<int> vs, ns;
+            const char* r = q + 2;
+            while (true) {
+                while (*r == ' ' || *r == '\t' || *r == '\r') r++;
+                if (*r == '\0') break;
+                char* end;
+                long v = std::strtol(r, &end, 10), t = 0, n = 0;
+                if (end == r) {
+                    error = path + ":" + std::to_string(line_number) + ": bad face";
+                    return false;
+                }
+                r = end;
+                if (*r == '/') {
+                    r++;
+                    if (*r != '/') {
+                        t = std::strtol(r, &end, 10);
+                        r = end;
+                    }
+                    if (*r == '/') {
+                        r++;
+                        n = std::strtol(r, &end, 10);
+                        r = end;
+                    }
+                }
+                (void)t;
+                int vi = resolve(v, positions.size());
+                if (vi < 0) {
+                    error = path + ":" + std::to_string(line_number) + ": vertex index out of range";
+                    return false;
+                }
+                vs.push_back(vi);
+                ns.push_back(resolve(n, normals.size()));
+            }
+            for (size_t k = 1; k + 1 < vs.size(); k++) {  // a fan; degenerate triangles are skipped
+                const size_t idx[3] = {0, k, k + 1};
+                face f;
+                for (int c = 0; c < 3; c++) {
+                    f.v[c] = vs[idx[c]];
+                    f.n[c] = ns[idx[c]];
+                }
+                const point3& p0 = positions[f.v[0]];
+                if (cross(positions[f.v[1]] - p0, positions[f.v[2]] - p0).near_zero()) continue;
+                auto g = group_of.find(current.get());
+                if (g == group_of.end()) {
+                    g = group_of.emplace(current.get(), groups.size()).first;
+                    groups.push_back({current, {}});
+                }
+                groups[g->second].second.push_back(f);
+            }
+        } else if (line.compare(at, 6, "usemtl") == 0) {
+            std::string name = argument(q + 6);
+            if (auto it = materials.find(name); it != materials.end()) current = it->second;
+            else if (auto jt = file_materials.find(name); jt != file_materials.end()) current = jt->second;
+            else current = default_mat;
+        } else if (line.compare(at, 6, "mtllib") == 0) {
+            std::istringstream names(argument(q + 6));
+            std::string name, mtl_error;
+            while (names >> name) {
+                const std::string full = name[0] == '/' ? name : dir + name;
+                if (!load_mtl(full, file_materials, mtl_error, use_normal_maps)) {
+                    if (mtl_error.find("cannot open") == 0)
+                        std::fprintf(stderr, "Warning: %s: %s\n", path.c_str(), mtl_error.c_str());
+                    else {
+                        error = mtl_error;
+                        return false;
+                    }
+                }
+            }
+        }
+    }
+    if (smooth && normals.empty()) {  // area-weighted vertex normals (obj_loader.h:311-324)
+        normals.assign(positions.size(), vec3(0, 0, 0));
+        for (auto& [mat, faces] : groups)
+            for (auto& f : faces) {
+                const point3& p0 = positions[f.v[0]];
+                vec3 n = cross(positions[f.v[1]] - p0, positions[f.v[2]] - p0);
+                for (int c = 0; c < 3; c++) {
+                    normals[f.v[c]] = normals[f.v[c]] + n;
+                    f.n[c] = f.v[c];
+                }
+            }
+    }
+    for (auto& [mat, faces] : groups) {
+        std::vector<int32_t> idx, nidx;
+        bool any_normals = false;
+        for (const auto& f : faces)
+            for (int c = 0; c < 3; c++) {
+                idx.push_back(f.v[c]);
+                nidx.push_back(f.n[c]);
+                any_normals |= f.n[c] >= 0;
+            }
+        auto m = std::make_shared<mesh>(positions, std::move(idx), mat);
+        if (any_normals) {
+            m->normals = normals;
+            m->normal_indices = std::move(nidx);
+        }
+        parts.push_back({mat, m});
+    }
+    if (parts.empty()) {
+        error = "'" + path + "' has no faces";
+        return false;
+    }
+    return true;
+}
+
 // ------------------------------------------------------------------ .scene loader
 namespace {
 
@@ -742,6 +1025,9 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
     auto slash = path.find_last_of('/');
     std::string scene_dir = slash == std::string::npos ? "" : path.substr(0, slash + 1);
     auto default_material = std::make_shared<lambertian>(color(0.5, 0.5, 0.5));
+    bool use_normal_maps = true;
+    // meshes loaded once per (file, smooth, material, normal maps) (scene.h:1167-1192)
+    std::map<std::tuple<std::string, bool, const material*, bool>, std::vector<obj_part>> mesh_cache;
 
     auto read_color_or_texture = [&](std::istringstream& in, const std::string& first, std::shared_ptr<texture>& tex,
                                      std::string& error) {
@@ -965,10 +1251,38 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
             std::string name;
             if (!(in >> name) || !result.display.set_tonemapper(name))
                 return fail("expected 'tonemap none|reinhard|aces'");
-        } else if (keyword == "normal_maps") {
-            // normal maps change shading normals of textured meshes, which the GPU path does not load
-            std::string rest;
+        } else if (keyword == "normal_maps") {  // scene.h:689-693
+            std::string state;
+            if (!(in >> state) || (state != "on" && state != "off")) return fail("expected 'normal_maps on|off'");
+            use_normal_maps = state == "on";
+        } else if (keyword == "mesh") {  // scene.h:1129-1197, Wavefront OBJ files
+            std::string file, error;
+            if (!(in >> file)) return fail("expected 'mesh <file.obj> [material] [transforms]'");
+            if (file[0] != '/') file = scene_dir + file;
+            std::string rest, first;
             std::getline(in, rest);
+            std::istringstream peek(rest);
+            const bool smooth = (peek >> first) && first == "smooth";
+            std::istringstream tail(smooth ? rest.substr(rest.find("smooth") + 6) : rest);
+            auto ends_with = [&](const std::string& suffix) {
+                return file.size() >= suffix.size() && file.compare(file.size() - suffix.size(), suffix.size(), suffix) == 0;
+            };
+            if (ends_with(".gltf") || ends_with(".glb")) return fail("glTF files are not modelled by the GPU path");
+            if (rest.find("replace") != std::string::npos)
+                return fail("'replace' is for glTF files (OBJ files take scene materials by usemtl name)");
+            std::shared_ptr<material> mat;
+            transform xform;
+            bool transformed;
+            if (!read_tail(tail, mat, xform, transformed, error)) return fail(error);
+            auto key = std::make_tuple(file, smooth, mat.get(), use_normal_maps);
+            auto cached = mesh_cache.find(key);
+            if (cached == mesh_cache.end()) {
+                std::vector<obj_part> parts;
+                if (!load_obj(file, materials, mat, smooth, parts, error, use_normal_maps)) return fail(error);
+                cached = mesh_cache.emplace(key, std::move(parts)).first;
+            }
+            for (const auto& part : cached->second)
+                if (!add_object(part.geometry, part.mat, xform, transformed, error)) return fail(error);
         } else if (keyword == "texture") {
             std::string name, type;
             if (!(in >> name >> type)) return fail("expected 'texture <name> <type> <parameters...>'");

@@ -325,6 +325,17 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         pbox[i] = prim_box(kind, src);
         s.prim_canon[i] = -1;
     }
+    s.prim_nrm.clear();
+    if (d.prim_normals) {
+        bool any = false;
+        for (int64_t i = 0; i < np && !any; i++)
+            any = s.prim_kind[i] == MPT_PRIM_TRIANGLE && !std::isnan(d.prim_normals[9 * i]);
+        if (any) {
+            s.prim_nrm.assign(d.prim_normals, d.prim_normals + 9 * np);
+            for (int64_t i = 0; i < np; i++)
+                if (s.prim_kind[i] != MPT_PRIM_TRIANGLE) s.prim_nrm[size_t(9 * i)] = std::nan("");
+        }
+    }
 
     // objects
     const int no = d.n_objects;
@@ -344,7 +355,7 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         r.material = d.object_material ? d.object_material[o] : -1;
         r.density = d.object_density ? d.object_density[o] : 0;
         r.bkind = d.object_boundary_kind ? d.object_boundary_kind[o] : MPT_OBJ_PRIM;
-        r.pad = 0;
+        r.light_table = -1;
         next_id = r.id_base + r.count;
         if (r.first < 0 || r.count < 1 || r.first + r.count > np)
             return fail(MPT_E_INVALID, "object " + std::to_string(o) + " has a bad primitive range");
@@ -396,6 +407,13 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
                     }
                     std::vector<int32_t> tmat(r.count);
                     for (int64_t k = 0; k < r.count; k++) tmat[k] = s.prim_mat[r.first + items[k].ref];
+                    if (!s.prim_nrm.empty()) {  // vertex normals follow their triangles
+                        std::vector<double> tn(size_t(r.count) * 9);
+                        for (int64_t k = 0; k < r.count; k++)
+                            std::copy(&s.prim_nrm[size_t(r.first + items[k].ref) * 9],
+                                      &s.prim_nrm[size_t(r.first + items[k].ref) * 9] + 9, &tn[size_t(k) * 9]);
+                        std::copy(tn.begin(), tn.end(), &s.prim_nrm[size_t(r.first) * 9]);
+                    }
                     for (int64_t k = 0; k < r.count; k++) {
                         std::copy(&tmp[size_t(k) * kPrimStride], &tmp[size_t(k) * kPrimStride] + kPrimStride,
                                   &s.prim[size_t(r.first + k) * kPrimStride]);
@@ -502,7 +520,26 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         if (ob >= 0) {
             const obj_rec& r = s.objects[ob];
             if (r.kind == MPT_OBJ_MEDIUM) return fail(MPT_E_UNSUPPORTED, "media cannot be lights");
-            if (r.kind == MPT_OBJ_MESH) return fail(MPT_E_UNSUPPORTED, "mesh lights are not supported yet");
+            if (r.kind == MPT_OBJ_MESH && s.objects[ob].light_table < 0) {
+                // the mesh's triangles in its own order and their area CDF
+                // (mesh.h:180-186); instances of one mesh share the table
+                int table = -1;
+                for (int o2 = 0; o2 < no && table < 0; o2++)
+                    if (s.objects[o2].kind == MPT_OBJ_MESH && s.objects[o2].first == r.first && s.objects[o2].light_table >= 0)
+                        table = s.objects[o2].light_table;
+                if (table < 0) {
+                    table = int32_t(s.mlight_prim.size());
+                    s.mlight_prim.resize(size_t(table + r.count));
+                    s.mlight_cdf.resize(size_t(table + r.count));
+                    for (int64_t k = 0; k < r.count; k++) s.mlight_prim[size_t(table + s.prim_canon[r.first + k])] = r.first + k;
+                    double total = 0;
+                    for (int64_t k = 0; k < r.count; k++) {
+                        total += s.prim[size_t(s.mlight_prim[size_t(table + k)]) * kPrimStride + 12];
+                        s.mlight_cdf[size_t(table + k)] = total;
+                    }
+                }
+                s.objects[ob].light_table = table;
+            }
         }
         s.lights[k].object = ob;
         s.lights[k].pad = 0;
@@ -546,6 +583,8 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
                 const obj_rec& r = s.objects[ob];
                 double pw = 0;
                 if (r.kind == MPT_OBJ_PRIM) pw = prim_power(r.first);
+                else if (r.kind == MPT_OBJ_MESH)  // mesh::power, in the mesh's triangle order
+                    for (int64_t i = 0; i < r.count; i++) pw += prim_power(s.mlight_prim[size_t(r.light_table + i)]);
                 else
                     for (int64_t i = 0; i < r.count; i++) pw += prim_power(r.first + i);
                 if (r.xform >= 0) pw = pw * area_scale(s.xforms[r.xform]);

@@ -64,7 +64,7 @@ struct obj_rec {
     int32_t material;    // media: phase material
     double density;      // media
     int32_t bkind;       // media: boundary kind (MPT_OBJ_PRIM or MPT_OBJ_LIST)
-    int32_t pad;
+    int32_t light_table; // mesh lights: offset of its triangles in mlight_prim / mlight_cdf (else -1)
 };
 
 struct light_rec {
@@ -79,6 +79,7 @@ struct host_scene {
     std::vector<int32_t> prim_kind;
     std::vector<int32_t> prim_mat;
     std::vector<int64_t> prim_canon;     // canonical local index within its object range
+    std::vector<double> prim_nrm;        // triangles' vertex normals, 9 per primitive (NaN: none); empty if none
     std::vector<obj_rec> objects;
     std::vector<bvh_node> tlas;
     std::vector<int32_t> tlas_items;     // object indices in leaf order
@@ -95,6 +96,11 @@ struct host_scene {
     std::vector<mpt_texture> textures;
     std::vector<light_rec> lights;
     std::vector<double> light_cdf;       // n_lights + 1
+    // mesh lights (mesh.h:175-216): per light mesh, its triangles in the
+    // mesh's own order (device primitive index) and the running sum of their
+    // areas in that order
+    std::vector<int64_t> mlight_prim;
+    std::vector<double> mlight_cdf;
     int32_t light_strategy = 1;
     // environment (environment.h)
     int32_t env_w = 0, env_h = 0;

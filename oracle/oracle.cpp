@@ -763,11 +763,13 @@ struct quad : hittable {  // quad.h
     double power() const override { return pi * area * luminance(mat->average_emission()); }
 };
 
-struct triangle : hittable {  // mesh.h:28-139 (no uvs, normals or alpha)
+struct triangle : hittable {  // mesh.h:28-139 (vertex normals; no uvs or alpha)
     point3 p0, p1, p2;
     const material* mat;
     int64_t id;
     double area;
+    bool has_normals = false;
+    vec3 n0, n1, n2;
     triangle(const point3& a, const point3& b, const point3& c, const material* m, int64_t id)
         : p0(a), p1(b), p2(c), mat(m), id(id) {
         area = 0.5 * cross(p1 - p0, p2 - p0).length();
@@ -791,6 +793,14 @@ struct triangle : hittable {  // mesh.h:28-139 (no uvs, normals or alpha)
         rec.mat = mat;
         rec.id = id;
         rec.set_face_normal(r, unit_vector(cross(e1, e2)));
+        if (has_normals) {  // mesh.h:93-101
+            double b0 = 1 - b1 - b2;
+            vec3 shading = b0 * n0 + b1 * n1 + b2 * n2;
+            if (shading.length_squared() > 0) {
+                shading = unit_vector(shading);
+                rec.normal = dot(shading, rec.normal) < 0 ? -shading : shading;
+            }
+        }
         return true;
     }
     aabb bounding_box() const override { return aabb(aabb(p0, p1), aabb(p2, p2)); }
@@ -1022,6 +1032,39 @@ struct instance : hittable {  // instance.h
     double pdf_value(const point3& o, const vec3& d) const override { return object->pdf_value(x.inv_point(o), x.inv_vector(d)); }
     vec3 random(const point3& o, sampler& s) const override { return x.vector(object->random(x.inv_point(o), s)); }
     double power() const override { return object->power() * x.area_scale(); }
+};
+
+// A triangle mesh (mesh.h:175-233): hits through its own BVH; as a light it
+// picks triangles by area in the mesh's own order.
+struct mesh_obj : hittable {
+    const bvh* tree;
+    std::vector<const hittable*> tris;
+    std::vector<double> area_cdf;
+    double total_area = 0;
+    mesh_obj(const bvh* t, std::vector<const hittable*> v) : tree(t), tris(std::move(v)) {
+        for (auto* h : tris) {
+            total_area += static_cast<const triangle*>(h)->area;
+            area_cdf.push_back(total_area);
+        }
+    }
+    bool hit(const ray& r, interval t, hit_record& rec, const hit_ctx& c) const override { return tree->hit(r, t, rec, c); }
+    aabb bounding_box() const override { return tree->bounding_box(); }
+    double pdf_value(const point3& o, const vec3& d) const override {
+        if (total_area <= 0) return 0;
+        double sum = 0;
+        for (auto* h : tris) sum += (static_cast<const triangle*>(h)->area / total_area) * h->pdf_value(o, d);
+        return sum;
+    }
+    vec3 random(const point3& o, sampler& s) const override {
+        double x = s.next() * total_area;
+        size_t i = size_t(std::upper_bound(area_cdf.begin(), area_cdf.end(), x) - area_cdf.begin());
+        return tris[std::min(i, tris.size() - 1)]->random(o, s);
+    }
+    double power() const override {
+        double sum = 0;
+        for (auto* h : tris) sum += h->power();
+        return sum;
+    }
 };
 
 // A world-list object that owns its ID range (lists of an instanced box are
@@ -1284,12 +1327,21 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
             case MPT_PRIM_QUAD:
                 return own(s, std::make_unique<quad>(point3(q[0], q[1], q[2]), vec3(q[3], q[4], q[5]), vec3(q[6], q[7], q[8]),
                                                      m, id));
-            default:
-                return own(s, std::make_unique<triangle>(point3(q[0], q[1], q[2]), point3(q[3], q[4], q[5]),
-                                                         point3(q[6], q[7], q[8]), m, id));
+            default: {
+                auto t = std::make_unique<triangle>(point3(q[0], q[1], q[2]), point3(q[3], q[4], q[5]),
+                                                    point3(q[6], q[7], q[8]), m, id);
+                const double* vn = d.prim_normals ? d.prim_normals + 9 * i : nullptr;
+                if (vn && !std::isnan(vn[0])) {
+                    t->has_normals = true;
+                    t->n0 = vec3(vn[0], vn[1], vn[2]);
+                    t->n1 = vec3(vn[3], vn[4], vn[5]);
+                    t->n2 = vec3(vn[6], vn[7], vn[8]);
+                }
+                return own(s, std::move(t));
+            }
         }
     };
-    std::vector<std::pair<int64_t, const bvh*>> meshes;  // (first, tree) of meshes already built
+    std::vector<std::pair<int64_t, const hittable*>> meshes;  // (first, mesh) of meshes already built
     for (int o = 0; o < d.n_objects; o++) {
         int64_t first = d.object_first[o], count = d.object_count[o];
         int64_t idb = d.object_id_base ? d.object_id_base[o] : 0;
@@ -1306,14 +1358,15 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
                 break;
             }
             case MPT_OBJ_MESH: {
-                const bvh* tree = nullptr;
+                const hittable* tree = nullptr;
                 for (auto& mm : meshes)
                     if (mm.first == first) tree = mm.second;
                 int64_t shift = idb;
                 if (!tree) {
                     std::vector<const hittable*> tris;
                     for (int64_t k = 0; k < count; k++) tris.push_back(prim(first + k, k));
-                    tree = own(s, std::make_unique<bvh>(tris));
+                    const bvh* b = own(s, std::make_unique<bvh>(tris));
+                    tree = own(s, std::make_unique<mesh_obj>(b, tris));
                     meshes.push_back({first, tree});
                 }
                 h = own(s, std::make_unique<idshift>(tree, shift));

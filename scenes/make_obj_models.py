@@ -1,0 +1,118 @@
+"""Writes the Wavefront OBJ/MTL test models under scenes/models (run from
+the repository root: python scenes/make_obj_models.py).
+
+room.obj / room.mtl  a room of MTL materials: lambertian walls (Kd, one as
+                     a single grey value), a glossy box (Ks/Ns/Ni ->
+                     principled) indexed with negative indices, an emissive
+                     ceiling panel (Ke) written as one quad face, a pentagon
+                     (fan triangulation), a degenerate face (skipped), and a
+                     faceted sphere with vertex normals (v//n) whose faces
+                     use v/t/n with texture coordinates;
+blob.obj             a low-poly sphere with no normals and a scene material
+                     (usemtl names a material of the .scene file), for
+                     `mesh ... smooth` and instancing.
+"""
+import math
+import os
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models")
+
+
+def sphere(nu, nv, r, c):
+    verts, faces = [], []
+    for j in range(nv + 1):
+        th = math.pi * j / nv
+        for i in range(nu):
+            ph = 2 * math.pi * i / nu
+            verts.append((c[0] + r * math.sin(th) * math.cos(ph), c[1] + r * math.cos(th),
+                          c[2] + r * math.sin(th) * math.sin(ph)))
+    for j in range(nv):
+        for i in range(nu):
+            a = j * nu + i
+            b = j * nu + (i + 1) % nu
+            faces.append((a, a + nu, b + nu, b))  # quads (degenerate at the poles: skipped or fanned)
+    return verts, faces
+
+
+def fmt(x):
+    return f"{x:.6f}"
+
+
+def room():
+    L = []
+    L.append("# test room for the OBJ loader")
+    L.append("mtllib room.mtl")
+    L.append("o room")
+    # floor / walls: 8 corners of a 4 x 3 x 4 box around the origin, floor at y = 0
+    c = [(-2, 0, -2), (2, 0, -2), (2, 0, 2), (-2, 0, 2), (-2, 3, -2), (2, 3, -2), (2, 3, 2), (-2, 3, 2)]
+    for p in c:
+        L.append("v " + " ".join(fmt(x) for x in p))
+    L.append("usemtl floor")
+    L.append("f 1 4 3 2")           # floor, facing up
+    L.append("usemtl wall_red")
+    L.append("f 1 5 8 4")           # left wall
+    L.append("usemtl wall_green")
+    L.append("f 2 3 7 6")           # right wall
+    L.append("usemtl grey")
+    L.append("f 1 2 6 5")           # back wall
+    L.append("f 5 6 7 8")           # ceiling
+    # ceiling light panel, slightly below the ceiling, facing down, one quad face
+    for p in [(-0.5, 2.99, -0.5), (0.5, 2.99, -0.5), (0.5, 2.99, 0.5), (-0.5, 2.99, 0.5)]:
+        L.append("v " + " ".join(fmt(x) for x in p))
+    L.append("usemtl lamp")
+    L.append("f 9 10 11 12")
+    # glossy box on the floor, faces with negative (relative) indices
+    x0, x1, z0, z1, h = -1.4, -0.4, -1.2, -0.2, 1.1
+    box = [(x0, 0, z0), (x1, 0, z0), (x1, 0, z1), (x0, 0, z1), (x0, h, z0), (x1, h, z0), (x1, h, z1), (x0, h, z1)]
+    for p in box:
+        L.append("v " + " ".join(fmt(x) for x in p))
+    L.append("usemtl gloss")
+    for f in [(1, 2, 6, 5), (2, 3, 7, 6), (3, 4, 8, 7), (4, 1, 5, 8), (5, 6, 7, 8)]:
+        L.append("f " + " ".join(str(k - 9) for k in f))
+    # a pentagon on the back wall and a degenerate face
+    for k in range(5):
+        a = 2 * math.pi * k / 5
+        L.append(f"v {fmt(1.0 + 0.5 * math.cos(a))} {fmt(1.8 + 0.5 * math.sin(a))} {fmt(-1.99)}")
+    L.append("usemtl wall_red")
+    L.append("f -5 -4 -3 -2 -1")
+    L.append("f 1 1 2")
+    # faceted sphere with vertex normals and texture coordinates
+    verts, faces = sphere(10, 6, 0.6, (0.9, 0.6, 0.6))
+    base_v = 12 + 8 + 5
+    for p in verts:
+        L.append("v " + " ".join(fmt(x) for x in p))
+    for p in verts:
+        n = [(p[k] - (0.9, 0.6, 0.6)[k]) / 0.6 for k in range(3)]
+        L.append("vn " + " ".join(fmt(x) for x in n))
+    for k in range(len(verts)):
+        L.append(f"vt {fmt((k % 10) / 10)} {fmt((k // 10) / 6)}")
+    L.append("usemtl ball")
+    for f in faces:
+        L.append("f " + " ".join(f"{base_v + a + 1}/{a + 1}/{a + 1}" for a in f))
+    open(os.path.join(HERE, "room.obj"), "w").write("\n".join(L) + "\n")
+    M = ["# materials of room.obj",
+         "newmtl floor", "Kd 0.7 0.65 0.6", "illum 2",
+         "newmtl wall_red", "Kd 0.63 0.07 0.05",
+         "newmtl wall_green", "Kd 0.14 0.45 0.09",
+         "newmtl grey", "Kd 0.6",
+         "newmtl lamp", "Kd 0 0 0", "Ke 12 11 9",
+         "newmtl gloss", "Kd 0.2 0.3 0.6", "Ks 0.5 0.5 0.5", "Ns 80", "Ni 1.45",
+         "newmtl ball", "Kd 0.8 0.6 0.2", "Ks 0.04", "Ns 10"]
+    open(os.path.join(HERE, "room.mtl"), "w").write("\n".join(M) + "\n")
+
+
+def blob():
+    verts, faces = sphere(12, 8, 0.35, (0, 0.35, 0))
+    L = ["# low-poly sphere without normals; its material comes from the scene"]
+    for p in verts:
+        L.append("v " + " ".join(fmt(x) for x in p))
+    L.append("usemtl jade")
+    for f in faces:
+        L.append("f " + " ".join(str(a + 1) for a in f))
+    open(os.path.join(HERE, "blob.obj"), "w").write("\n".join(L) + "\n")
+
+
+if __name__ == "__main__":
+    os.makedirs(HERE, exist_ok=True)
+    room()
+    blob()

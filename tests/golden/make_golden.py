@@ -17,11 +17,15 @@ queries) are seeded and regenerated identically by the tests:
                         their per-pixel standard error over 8 runs
   denoise.npz           denoiser::run (denoiser.h:45-193) on the synthetic frames
                         of tests/denoise_frames.py, for two settings
+  obj_room.npz          scenes/obj_room.scene (Wavefront OBJ/MTL models) loaded by the
+                        reference's load_obj: a 4096-ray subset of the seeded batch
+                        with the reference's hit IDs, t and front face, digests of
+                        t and front over the 1M rays, and converged means
   display.npz           postprocess (postprocess.h:108-153) and the bytes write_png
                         writes (display_settings::apply + to_byte, tonemap.h,
                         color3.h) on the images and settings of tests/display_cases.py
 
-Usage: python tests/golden/make_golden.py [step ...]  (steps: ggx trace bsdf render denoise display; all by default)
+Usage: python tests/golden/make_golden.py [step ...]  (steps: ggx trace bsdf render denoise display obj; all by default)
 (all steps take about a minute)
 """
 import hashlib
@@ -125,16 +129,53 @@ def golden_display():
     print("display.npz", len(dc.CASES), "images", flush=True)
 
 
+OBJ_SCENE = "obj_room.scene"
+OBJ_RENDER = (40, 256, 8, 16)  # width, spp per run, runs, max_depth
+
+
+def golden_obj():
+    """The OBJ scene through the reference's own loaders (load_obj,
+    load_mtl): the seeded 1M-ray batch's subset with the reference's hit IDs,
+    t and front face, and converged low-resolution means."""
+    R = ol.reference()
+    d = m.describe(scene_path(OBJ_SCENE))
+    o = ol.OracleScene(d.desc)
+    rays, tt = o.ray_batch(d.camera, BATCH_SEED, N_BATCH)
+    r = ol.RefScene(scene_path(OBJ_SCENE))
+    out = r.trace(rays, tt, threads=os.cpu_count() or 8)
+    sel = np.linspace(0, N_BATCH - 1, N_SUBSET).astype(np.int64)
+    digest = {"seed": BATCH_SEED, "n": N_BATCH,
+              "sha256_t": hashlib.sha256(out["t"].astype("<f8").tobytes()).hexdigest(),
+              "sha256_front": hashlib.sha256(out["front"].astype("u1").tobytes()).hexdigest(),
+              "ties": int((out["ties"] > 1).sum()), "misses": int((out["id"] == -1).sum()),
+              "unidentified": int((out["id"] == -2).sum())}
+    width, spp, runs, depth = OBJ_RENDER
+    h = max(1, int(width / d.camera.aspect_ratio))
+    imgs = []
+    for _ in range(runs):
+        img = np.zeros((h, width, 3))
+        R.ref_render(r.h, width, spp, depth, os.cpu_count() or 8, ol._d(img))
+        imgs.append(img)
+    imgs = np.stack(imgs)
+    np.savez_compressed(os.path.join(HERE, "obj_room.npz"), rays=rays[sel], tt=tt[sel], index=sel,
+                        id=out["id"][sel], t=out["t"][sel], front=out["front"][sel], ties=out["ties"][sel],
+                        digest=json.dumps(digest), mean=imgs.mean(0), stderr=imgs.std(0, ddof=1) / np.sqrt(runs),
+                        width=width, height=h, spp=spp * runs, max_depth=depth)
+    print("obj_room.npz", digest, flush=True)
+
+
 def main():
     if not ol.reference_available():
         sys.exit("oracle/_ref/libmpt_ref.so is not built (make -C oracle ref, where the reference is present)")
     R = ol.reference()
-    steps = set(sys.argv[1:]) or {"ggx", "trace", "bsdf", "render", "denoise", "display"}
+    steps = set(sys.argv[1:]) or {"ggx", "trace", "bsdf", "render", "denoise", "display", "obj"}
     if "denoise" in steps:
         golden_denoise()
     if "display" in steps:
         golden_display()
-    if steps <= {"denoise", "display"}:
+    if "obj" in steps:
+        golden_obj()
+    if steps <= {"denoise", "display", "obj"}:
         return
 
     e = np.zeros(1024)

@@ -66,6 +66,7 @@ def test_render_features_match_oracle(mpt):
     ("weekend_cover.scene", 64, 4, 50),
     ("test_lights.scene", 48, 4, 12),
     ("test_media.scene", 48, 4, 12),
+    ("obj_room.scene", 40, 4, 16),
 ])
 def test_batch_tail_in_one_launch_is_identical(mpt, name, width, spp, depth):
     """k_finish runs a path's remaining bounces in one thread; the frame must
@@ -90,6 +91,7 @@ def test_batch_tail_in_one_launch_is_identical(mpt, name, width, spp, depth):
     ("weekend_cover.scene", 64, 6, 50),
     ("test_lights.scene", 48, 4, 12),
     ("test_media.scene", 48, 4, 12),
+    ("obj_room.scene", 40, 4, 16),
 ])
 def test_grid_binning_and_graphs_leave_the_frame_unchanged(mpt, name, width, spp, depth):
     """The knobs of mpt_render_opts that only schedule the work: the persistent
