@@ -11,6 +11,7 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <iterator>
 #include <sstream>
 #include <tuple>
 
@@ -1001,6 +1002,77 @@ bool read_pfm(const std::string& path, int& w, int& h, std::vector<color>& px) {
     return true;
 }
 
+// Radiance RGBE (.hdr), as environment_light reads it through stb_image
+// 2.x (environment.h:30-41): the "#?RADIANCE" / "#?RGBE" header with
+// FORMAT=32-bit_rle_rgbe, a "-Y h +X w" resolution line, scanlines flat or
+// new-style run-length encoded (per component: count > 128 repeats the next
+// byte count - 128 times, else count literal bytes). A pixel is
+// float(m) * 2^(e - 136) per channel in float arithmetic, 0 when e = 0.
+bool read_hdr(const std::string& path, int& w, int& h, std::vector<color>& px) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    std::string line;
+    if (!std::getline(f, line) || (line != "#?RADIANCE" && line != "#?RGBE")) return false;
+    bool format = false;
+    while (std::getline(f, line) && !line.empty())
+        if (line == "FORMAT=32-bit_rle_rgbe") format = true;
+    if (!format || !std::getline(f, line)) return false;
+    char ya[3] = {0}, xa[3] = {0};
+    if (std::sscanf(line.c_str(), "%2s %d %2s %d", ya, &h, xa, &w) != 4 || std::string(ya) != "-Y" ||
+        std::string(xa) != "+X" || w <= 0 || h <= 0)
+        return false;
+    std::vector<unsigned char> data((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    std::vector<unsigned char> rgbe(size_t(w) * h * 4);
+    size_t at = 0;
+    auto flat_rest = [&](size_t from_pixel) {  // the rest of the image as plain RGBE
+        const size_t need = (size_t(w) * h - from_pixel) * 4;
+        if (at + need > data.size()) return false;
+        std::copy(data.begin() + long(at), data.begin() + long(at + need), rgbe.begin() + long(from_pixel * 4));
+        return true;
+    };
+    if (w < 8 || w >= 32768) {
+        if (!flat_rest(0)) return false;
+    } else {
+        for (int j = 0; j < h; j++) {
+            if (at + 4 > data.size()) return false;
+            const unsigned char c1 = data[at], c2 = data[at + 1], len = data[at + 2];
+            if (c1 != 2 || c2 != 2 || (len & 0x80)) {  // not run-length encoded: the file is flat from here
+                if (!flat_rest(size_t(j) * w)) return false;
+                break;
+            }
+            if (((int(len) << 8) | data[at + 3]) != w) return false;
+            at += 4;
+            for (int c = 0; c < 4; c++) {
+                int i = 0;
+                while (i < w) {
+                    if (at >= data.size()) return false;
+                    int count = data[at++];
+                    if (count > 128) {
+                        count -= 128;
+                        if (count > w - i || at >= data.size()) return false;
+                        const unsigned char v = data[at++];
+                        for (int k = 0; k < count; k++) rgbe[(size_t(j) * w + i++) * 4 + c] = v;
+                    } else {
+                        if (count == 0 || count > w - i || at + size_t(count) > data.size()) return false;
+                        for (int k = 0; k < count; k++) rgbe[(size_t(j) * w + i++) * 4 + c] = data[at++];
+                    }
+                }
+            }
+        }
+    }
+    px.resize(size_t(w) * h);
+    for (size_t k = 0; k < px.size(); k++) {
+        const unsigned char* e = &rgbe[4 * k];
+        if (e[3] == 0) {
+            px[k] = color(0, 0, 0);
+            continue;
+        }
+        const float f1 = float(std::ldexp(1.0f, int(e[3]) - (128 + 8)));
+        px[k] = color(float(e[0]) * f1, float(e[1]) * f1, float(e[2]) * f1);
+    }
+    return true;
+}
+
 }  // namespace
 
 bool load_scene(const std::string& path, scene& result, std::string* err_out) {
@@ -1193,7 +1265,7 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
         } else if (keyword == "environment") {
             std::string fname, key;
             double intensity = 1, rotation = 0;
-            const char* usage = "expected 'environment <file.pfm> [intensity <s>] [rotate <degrees>]'";
+            const char* usage = "expected 'environment <file.hdr|file.pfm> [intensity <s>] [rotate <degrees>]'";
             if (!(in >> fname)) return fail(usage);
             while (in >> key) {
                 bool ok = (key == "intensity" && in >> intensity && intensity >= 0) || (key == "rotate" && in >> rotation);
@@ -1202,8 +1274,9 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
             if (fname[0] != '/') fname = scene_dir + fname;
             int w = 0, h = 0;
             std::vector<color> px;
-            if (!read_pfm(fname, w, h, px))
-                return fail("cannot load environment map '" + fname + "' (the GPU path reads .pfm images)");
+            const bool hdr = fname.size() >= 4 && fname.compare(fname.size() - 4, 4, ".hdr") == 0;
+            if (!(hdr ? read_hdr(fname, w, h, px) : read_pfm(fname, w, h, px)))
+                return fail("cannot load environment map '" + fname + "' (the GPU path reads .hdr and .pfm images)");
             if (cam.environment) return fail("only one environment is allowed");
             cam.environment = std::make_shared<environment_light>(w, h, std::move(px), intensity, rotation);
             lights.add(cam.environment);

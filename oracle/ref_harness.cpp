@@ -283,4 +283,17 @@ int ref_scene_display(const char* path, double* out) {
     return 0;
 }
 
+
+// stb_image's decoding of an image as floats (environment_light's loader,
+// environment.h:30-41): width, height and up to cap floats of RGB.
+int ref_load_image_f(const char* path, int* w, int* h, float* out, int64_t cap) {
+    int c;
+    float* data = stbi_loadf(path, w, h, &c, 3);
+    if (!data) return -1;
+    const int64_t n = int64_t(*w) * *h * 3;
+    for (int64_t k = 0; k < n && k < cap; k++) out[k] = data[k];
+    stbi_image_free(data);
+    return 0;
+}
+
 }  // extern "C"

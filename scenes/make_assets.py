@@ -1,5 +1,5 @@
-"""Writes the Wavefront OBJ/MTL test models under scenes/models (run from
-the repository root: python scenes/make_obj_models.py).
+"""Writes the test assets under scenes/models (run from the repository root:
+python scenes/make_assets.py).
 
 room.obj / room.mtl  a room of MTL materials: lambertian walls (Kd, one as
                      a single grey value), a glossy box (Ks/Ns/Ni ->
@@ -10,7 +10,10 @@ room.obj / room.mtl  a room of MTL materials: lambertian walls (Kd, one as
                      use v/t/n with texture coordinates;
 blob.obj             a low-poly sphere with no normals and a scene material
                      (usemtl names a material of the .scene file), for
-                     `mesh ... smooth` and instancing.
+                     `mesh ... smooth` and instancing;
+sky_rle.hdr          a 256 x 128 sun-and-sky environment in Radiance RGBE with
+                     new-style run-length encoded scanlines (runs and
+                     literals), written here from the published format.
 """
 import math
 import os
@@ -112,7 +115,57 @@ def blob():
     open(os.path.join(HERE, "blob.obj"), "w").write("\n".join(L) + "\n")
 
 
+def rgbe(c):
+    """Ward's float2rgbe on float32 channels."""
+    import numpy as np
+    c = np.asarray(c, dtype=np.float32)
+    v = c.max()
+    if v < 1e-32:
+        return (0, 0, 0, 0)
+    m, e = math.frexp(float(v))
+    scale = np.float32(m) * np.float32(256) / v
+    return tuple(int(np.float32(x) * scale) for x in c) + (e + 128,)
+
+
+def rle_channel(vals):
+    """New-style RLE of one scanline channel: runs of 3+ equal bytes as
+    (128 + n, byte), everything else as literals of up to 128 bytes."""
+    out, i, n = [], 0, len(vals)
+    while i < n:
+        j = i
+        while j < n and j - i < 127 and vals[j] == vals[i]:
+            j += 1
+        if j - i >= 3:
+            out += [128 + (j - i), vals[i]]
+            i = j
+            continue
+        k = i
+        while k < n and k - i < 128:
+            if k + 2 < n and vals[k] == vals[k + 1] == vals[k + 2]:
+                break
+            k += 1
+        out += [k - i] + list(vals[i:k])
+        i = k
+    return out
+
+
+def sky_hdr(w=256, h=128):
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from mypbr_amd.workloads import sun_sky
+    img = sun_sky(w, h, sun_radiance=40.0, sun_angle_deg=4.0)
+    img[: h // 8, : w // 4] = 0.0  # a black patch: runs of zero bytes
+    data = bytearray(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n" + f"-Y {h} +X {w}\n".encode())
+    for j in range(h):
+        px = [rgbe(img[j, i]) for i in range(w)]
+        data += bytes([2, 2, w >> 8, w & 255])
+        for c in range(4):
+            data += bytes(rle_channel([p[c] for p in px]))
+    open(os.path.join(HERE, "sky_rle.hdr"), "wb").write(bytes(data))
+
+
 if __name__ == "__main__":
     os.makedirs(HERE, exist_ok=True)
     room()
     blob()
+    sky_hdr()

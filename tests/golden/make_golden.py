@@ -17,10 +17,11 @@ queries) are seeded and regenerated identically by the tests:
                         their per-pixel standard error over 8 runs
   denoise.npz           denoiser::run (denoiser.h:45-193) on the synthetic frames
                         of tests/denoise_frames.py, for two settings
-  obj_room.npz          scenes/obj_room.scene (Wavefront OBJ/MTL models) loaded by the
-                        reference's load_obj: a 4096-ray subset of the seeded batch
-                        with the reference's hit IDs, t and front face, digests of
-                        t and front over the 1M rays, and converged means
+  obj_room.npz,         scenes/obj_room.scene (Wavefront OBJ/MTL models) and
+  hdr_env.npz           scenes/hdr_env.scene (a run-length encoded .hdr environment)
+                        loaded by the reference's own loaders: a 4096-ray subset of
+                        the seeded batch with the reference's hit IDs, t and front
+                        face, digests over the 1M rays, and converged means
   display.npz           postprocess (postprocess.h:108-153) and the bytes write_png
                         writes (display_settings::apply + to_byte, tonemap.h,
                         color3.h) on the images and settings of tests/display_cases.py
@@ -129,27 +130,36 @@ def golden_display():
     print("display.npz", len(dc.CASES), "images", flush=True)
 
 
-OBJ_SCENE = "obj_room.scene"
-OBJ_RENDER = (40, 256, 8, 16)  # width, spp per run, runs, max_depth
+ASSET_SCENES = {  # scene: (width, spp per run, runs, max_depth) of the converged render
+    "obj_room.scene": (40, 256, 8, 16),
+    "hdr_env.scene": (48, 256, 8, 16),
+}
 
 
 def golden_obj():
-    """The OBJ scene through the reference's own loaders (load_obj,
-    load_mtl): the seeded 1M-ray batch's subset with the reference's hit IDs,
-    t and front face, and converged low-resolution means."""
+    for name, render in ASSET_SCENES.items():
+        golden_asset_scene(name, render)
+
+
+def golden_asset_scene(name, render):
+    """A scene of the repository's own assets through the reference's own
+    loaders (load_obj/load_mtl, stb_image's .hdr decoder): the seeded
+    1M-ray batch's subset with the reference's hit IDs, t and front face,
+    digests over the 1M rays, and converged low-resolution means."""
     R = ol.reference()
-    d = m.describe(scene_path(OBJ_SCENE))
+    d = m.describe(scene_path(name))
     o = ol.OracleScene(d.desc)
     rays, tt = o.ray_batch(d.camera, BATCH_SEED, N_BATCH)
-    r = ol.RefScene(scene_path(OBJ_SCENE))
+    r = ol.RefScene(scene_path(name))
     out = r.trace(rays, tt, threads=os.cpu_count() or 8)
     sel = np.linspace(0, N_BATCH - 1, N_SUBSET).astype(np.int64)
     digest = {"seed": BATCH_SEED, "n": N_BATCH,
+              "sha256_id": hashlib.sha256(out["id"].astype("<i8").tobytes()).hexdigest(),
               "sha256_t": hashlib.sha256(out["t"].astype("<f8").tobytes()).hexdigest(),
               "sha256_front": hashlib.sha256(out["front"].astype("u1").tobytes()).hexdigest(),
               "ties": int((out["ties"] > 1).sum()), "misses": int((out["id"] == -1).sum()),
               "unidentified": int((out["id"] == -2).sum())}
-    width, spp, runs, depth = OBJ_RENDER
+    width, spp, runs, depth = render
     h = max(1, int(width / d.camera.aspect_ratio))
     imgs = []
     for _ in range(runs):
@@ -157,11 +167,11 @@ def golden_obj():
         R.ref_render(r.h, width, spp, depth, os.cpu_count() or 8, ol._d(img))
         imgs.append(img)
     imgs = np.stack(imgs)
-    np.savez_compressed(os.path.join(HERE, "obj_room.npz"), rays=rays[sel], tt=tt[sel], index=sel,
+    np.savez_compressed(os.path.join(HERE, name.replace(".scene", ".npz")), rays=rays[sel], tt=tt[sel], index=sel,
                         id=out["id"][sel], t=out["t"][sel], front=out["front"][sel], ties=out["ties"][sel],
                         digest=json.dumps(digest), mean=imgs.mean(0), stderr=imgs.std(0, ddof=1) / np.sqrt(runs),
                         width=width, height=h, spp=spp * runs, max_depth=depth)
-    print("obj_room.npz", digest, flush=True)
+    print(name, digest, flush=True)
 
 
 def main():
