@@ -138,11 +138,11 @@ __device__ __forceinline__ double power_heuristic(double a, double b) {  // came
 // partly used cache line per field; with 0 every field is its own array.
 enum {
     // record 0: the ray (extend reads it; shade rewrites it)
-    D_OX, D_OY, D_OZ, D_DX, D_DY, D_DZ, D_HT, D_SPDF,
+    D_OX, D_OY, D_OZ, D_DX, D_DY, D_DZ, D_SPDF, D_PAD0,
     // record 1: throughput, radiance, MIS weight of the pending shadow ray
     D_BX, D_BY, D_BZ, D_RX, D_RY, D_RZ, D_SW, D_PAD1,
-    // record 2: the hit record shade reads (point, face normal)
-    D_QX, D_QY, D_QZ, D_MX, D_MY, D_MZ, D_PAD2A, D_PAD2B,
+    // record 2: the hit record shade reads (point, face normal, t)
+    D_QX, D_QY, D_QZ, D_MX, D_MY, D_MZ, D_HT, D_PAD2,
     // record 3: previous vertex (MIS) and the shadow ray's origin
     D_PX, D_PY, D_PZ, D_SOX, D_SOY, D_SOZ, D_PAD3A, D_PAD3B,
     // record 4: the NEE sample (unnormalised direction to the light, f)
@@ -176,6 +176,28 @@ __device__ __forceinline__ void st3(const pstate& P, int f, uint32_t i, const v3
     pd(P, f, i) = v.x;
     pd(P, f + 1, i) = v.y;
     pd(P, f + 2, i) = v.z;
+}
+
+// All 8 doubles of record r of slot i, written together: a record written
+// in part leaves its 64-byte segment partly written, which costs the memory
+// a read-modify-write (writing only 3 of record 3's doubles per bounce cost
+// 15% of the shade kernels' time).
+__device__ __forceinline__ void st_rec(const pstate& P, int r, uint32_t i, double a0, double a1, double a2, double a3,
+                                       double a4, double a5, double a6, double a7) {
+    if constexpr (MPT_PATH_RECORDS) {
+        double2* q = reinterpret_cast<double2*>(&pd(P, r * kRecDoubles, i));
+        q[0] = double2{a0, a1};
+        q[1] = double2{a2, a3};
+        q[2] = double2{a4, a5};
+        q[3] = double2{a6, a7};
+    } else {
+        const double a[8] = {a0, a1, a2, a3, a4, a5, a6, a7};
+        for (int k = 0; k < kRecDoubles; k++) pd(P, r * kRecDoubles + k, i) = a[k];
+    }
+}
+__device__ __forceinline__ void st_rec(const pstate& P, int r, uint32_t i, const v3& a, const v3& b, double c,
+                                       double d) {
+    st_rec(P, r, i, a.x, a.y, a.z, b.x, b.y, b.z, c, d);
 }
 
 // A block's records r of its consecutive slots [s0, s0 + BS) staged through
@@ -366,7 +388,8 @@ __device__ __forceinline__ void camera_ray(const dcam& C, const pstate& P, const
     q0[slot] = slot;
 }
 
-__global__ __launch_bounds__(256) void k_camera(dcam C, pstate P, batch_args B, uint32_t* q0, uint32_t* cnt) {
+__global__ __launch_bounds__(256) void k_camera(dcam C, pstate P, batch_args B, uint32_t* q0, uint32_t* cnt,
+                                                int lights) {
     __shared__ double lds[256 * kRecDoubles];
     int64_t gt = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (gt == 0) {
@@ -386,9 +409,9 @@ __global__ __launch_bounds__(256) void k_camera(dcam C, pstate P, batch_args B, 
         // record 1: beta 1, radiance 0
         my[0] = 1, my[1] = 1, my[2] = 1, my[3] = 0, my[4] = 0, my[5] = 0, my[6] = 0, my[7] = 0;
         if (s0 < B.nslots) store_records<256>(P, 1, s0, B.nslots, lds);
-        // record 3: previous vertex 0
+        // record 3: previous vertex 0 (read only by next-event estimation's MIS weights)
         for (int k = 0; k < kRecDoubles; k++) my[k] = 0;
-        if (s0 < B.nslots) store_records<256>(P, 3, s0, B.nslots, lds);
+        if (lights && s0 < B.nslots) store_records<256>(P, 3, s0, B.nslots, lds);
         return;
     }
     if (gt >= B.nslots) return;
@@ -399,7 +422,7 @@ __global__ __launch_bounds__(256) void k_camera(dcam C, pstate P, batch_args B, 
     st3(P, D_DX, slot, dir);
     st3(P, D_BX, slot, v3{1, 1, 1});
     st3(P, D_RX, slot, v3{0, 0, 0});
-    st3(P, D_PX, slot, v3{0, 0, 0});
+    if (lights) st3(P, D_PX, slot, v3{0, 0, 0});
     pd(P, D_SPDF, slot) = 0;
 }
 
@@ -441,8 +464,7 @@ __global__ __launch_bounds__(BS) void k_features(dscene S, launch_ctx L, pstate 
             albedo = tint;
             normal = v3{0, 0, 0};
         }
-        st3(P, D_AX, slot, albedo);
-        st3(P, D_NX, slot, normal);
+        st_rec(P, 6, slot, albedo, normal, 0.0, 0.0);
     }
 }
 
@@ -465,10 +487,13 @@ __device__ __forceinline__ int extend_one(const dscene& S, const pstate& P, int 
     const v3 o = ld3(P, D_OX, p), d = ld3(P, D_DX, p);
     hitinfo h;
     trace_closest<CNT, FEAT>(S, o, d, 0.001, kInf, h, st, true, P.key[p], uint32_t(2 * depth), wc);
-    pd(P, D_HT, p) = h.t;
     P.hobj[p] = h.obj;
     P.hprim[p] = h.prim;
     if (h.obj < 0) return 0;
+    // the hit record, every double of it (t and the pad first: t need not
+    // stay live through finish_hit)
+    pd(P, D_HT, p) = h.t;
+    pd(P, D_PAD2, p) = 0;
     hitrec rec;
     finish_hit<FEAT>(S, h, o, d, rec);
     st3(P, D_QX, p, rec.p);
@@ -539,6 +564,12 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
     constexpr bool LIGHTS = LK != 0;
     v3 o = ld3(P, D_OX, p), d = ld3(P, D_DX, p);
     v3 beta = ld3(P, D_BX, p), rad = ld3(P, D_RX, p);
+    // written at the end as whole records: the shadow ray's weight and the
+    // scatter pdf (next-event estimation's MIS), and whether this vertex is
+    // the previous vertex / shadow-ray origin of what follows
+    double sw = 0, spdf_out = 0;
+    bool vertex = false;
+    v3 vertex_p{0, 0, 0};
     uint32_t fl = P.flags[p];
     bool specular = fl & 1u, seen = fl & 2u;
     int dcount = (fl >> 2) & 1023, scount = (fl >> 12) & 1023, vcount = (fl >> 22) & 1023;
@@ -552,7 +583,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
                 rad = rad + beta * sky * power_heuristic(pd(P, D_SPDF, p), lp);
             }
         }
-        st3(P, D_RX, p, rad);
+        st_rec(P, 1, p, beta, rad, 0.0, 0.0);
         return;
     } else {
         hitrec rec;
@@ -574,7 +605,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
                     rad = rad + beta * e * power_heuristic(pd(P, D_SPDF, p), lp);
                 }
             }
-            st3(P, D_RX, p, rad);
+            st_rec(P, 1, p, beta, rad, 0.0, 0.0);
             return;  // a light scatters nothing (material.h:28-31)
         } else {
             rstream rs{P.key[p], P.ctr[p]};
@@ -604,11 +635,11 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
                         if (!(f.x <= 0 && f.y <= 0 && f.z <= 0)) {
                             double weight =
                                 last_bounce ? 1.0 : power_heuristic(light_pdf, density_value<MT>(so.dens, to_light));
-                            st3(P, D_SOX, p, rec.p);
-                            st3(P, D_TLX, p, to_light);
-                            st3(P, D_FX, p, f);
-                            st3(P, D_SBX, p, beta);
-                            pd(P, D_SW, p) = weight / light_pdf;
+                            st_rec(P, 4, p, to_light, f, 0.0, 0.0);
+                            st_rec(P, 5, p, beta, v3{0, 0, 0}, 0.0, 0.0);
+                            sw = weight / light_pdf;
+                            vertex = true;
+                            vertex_p = rec.p;
                             nee = true;
                         }
                     }
@@ -618,8 +649,11 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
                     double spdf = density_value<MT>(so.dens, dir);
                     if (spdf > 0) {
                         beta = beta * mat_eval<MT>(S, m, d, rec, dir) / spdf;
-                        pd(P, D_SPDF, p) = spdf;
-                        st3(P, D_PX, p, rec.p);
+                        if constexpr (LIGHTS) {  // read only by the MIS weights of next-event estimation
+                            spdf_out = spdf;
+                            vertex = true;
+                            vertex_p = rec.p;
+                        }
                         o = rec.p;
                         d = dir;
                         specular = false;
@@ -636,14 +670,12 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
         }
     }
     if (depth + 1 >= C.max_depth) alive = false;
-    if (!alive) {  // the path ends here: only its radiance is read again
-        st3(P, D_RX, p, rad);
-        return;
-    }
-    st3(P, D_OX, p, o);
-    st3(P, D_DX, p, d);
-    st3(P, D_BX, p, beta);
-    st3(P, D_RX, p, rad);
+    // record 3: this vertex as the next one's previous vertex (MIS) and as
+    // the shadow ray's origin; both are the hit point
+    if (LIGHTS && vertex) st_rec(P, 3, p, vertex_p, vertex_p, 0.0, 0.0);
+    st_rec(P, 1, p, beta, rad, sw, 0.0);  // a path that ends here keeps only its radiance (and a pending shadow ray)
+    if (!alive) return;
+    st_rec(P, 0, p, o, d, spdf_out, 0.0);
     P.flags[p] = (specular ? 1u : 0u) | (seen ? 2u : 0u) | (uint32_t(dcount) << 2) | (uint32_t(scount) << 12) |
                  (uint32_t(vcount) << 22);
 }
@@ -707,9 +739,10 @@ __device__ __forceinline__ void shadow_one(const dscene& S, const dcam& C, const
             break;
         }
         emission = through * emission;
-        v3 c = ld3(P, D_FX, p) * emission * pd(P, D_SW, p);
+        const double sw = pd(P, D_SW, p);
+        v3 c = ld3(P, D_FX, p) * emission * sw;
         v3 rad = ld3(P, D_RX, p) + ld3(P, D_SBX, p) * c;
-        st3(P, D_RX, p, rad);
+        st_rec(P, 1, p, ld3(P, D_BX, p), rad, sw, 0.0);
     }
 }
 
@@ -878,7 +911,7 @@ __global__ void k_nrc_apply(pstate P, const uint32_t* q, const uint32_t* cnt, nr
     const v3 b = ld3(P, D_BX, p);
     const v3 c{std::fmax(double(pred[3 * i]), 0.0), std::fmax(double(pred[3 * i + 1]), 0.0),
                std::fmax(double(pred[3 * i + 2]), 0.0)};
-    st3(P, D_RX, p, ld3(P, D_RX, p) + b * c);
+    st_rec(P, 1, p, b, ld3(P, D_RX, p) + b * c, pd(P, D_SW, p), 0.0);
 }
 
 // Training targets once the batch is done: the radiance the path gathered
@@ -1789,7 +1822,7 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
         batch_args B{o.seed, uint32_t(o.scene_index), o.first_pass + pass, npix, nb * npix};
         R.salt = uint32_t(mix64(o.seed ^ (uint64_t(uint32_t(o.first_pass + pass)) << 32)));
         EV(3, hipLaunchKernelGGL(k_camera, dim3(unsigned((B.nslots + 255) / 256)), dim3(256), 0, stream, C, P, B, q[0],
-                                 cnt));
+                                 cnt, int(s->has_lights)));
         if (C.features) EV(3, hipLaunchKernelGGL((k_features<256>), dim3(G), dim3(256), 0, stream, DS, L, P, B));
         if (!sorted) EV(4, launch_finish(0, q[0], ctr(cnt, 0)));  // every bounce in one kernel, no binning
         for (int depth = 0; sorted && depth < C.max_depth; depth++) {
