@@ -137,12 +137,14 @@ __device__ __forceinline__ double power_heuristic(double a, double b) {  // came
 // slots a queue names each cost whole 64-byte segments instead of one
 // partly used cache line per field; with 0 every field is its own array.
 enum {
-    // record 0: the ray (extend reads it; shade rewrites it)
-    D_OX, D_OY, D_OZ, D_DX, D_DY, D_DZ, D_SPDF, D_PAD0,
+    // record 0: the ray (extend reads it; shade rewrites it), and the path's
+    // flags and random-stream counter (the bits of one double: flags << 32 | counter)
+    D_OX, D_OY, D_OZ, D_DX, D_DY, D_DZ, D_SPDF, D_FLAGCTR,
     // record 1: throughput, radiance, MIS weight of the pending shadow ray
     D_BX, D_BY, D_BZ, D_RX, D_RY, D_RZ, D_SW, D_PAD1,
-    // record 2: the hit record shade reads (point, face normal, t)
-    D_QX, D_QY, D_QZ, D_MX, D_MY, D_MZ, D_HT, D_PAD2,
+    // record 2: the hit record shade reads (point, face normal, t, and the
+    // bits of the hit material with the front face in bit 31)
+    D_QX, D_QY, D_QZ, D_MX, D_MY, D_MZ, D_HT, D_HMAT,
     // record 3: previous vertex (MIS) and the shadow ray's origin
     D_PX, D_PY, D_PZ, D_SOX, D_SOY, D_SOZ, D_PAD3A, D_PAD3B,
     // record 4: the NEE sample (unnormalised direction to the light, f)
@@ -159,15 +161,15 @@ enum {
 constexpr int kRecDoubles = 8;
 constexpr int64_t kFieldStride = MPT_PATH_RECORDS ? kRecDoubles : 1;  // doubles between slots of a field
 
+// Flags (in D_FLAGCTR's high word): bit0 specular, bit1 seen_by_shadow_ray,
+// 10-bit bounce counters from bits 2, 12, 22.
 struct pstate {
     double* d[D_COUNT];
     uint64_t* key;
-    uint32_t* ctr;
-    uint32_t* flags;   // bit0 specular, bit1 seen_by_shadow_ray, 10-bit counters from bit 2, 12, 22
-    int32_t* hobj;
-    int32_t* hprim;
-    uint32_t* hmat;    // material of the hit, bit 31 = front face
 };
+
+__device__ __forceinline__ double as_double(uint64_t b) { return __longlong_as_double(static_cast<long long>(b)); }
+__device__ __forceinline__ uint64_t as_bits(double v) { return static_cast<uint64_t>(__double_as_longlong(v)); }
 
 // Field f of slot i.
 __device__ __forceinline__ double& pd(const pstate& P, int f, uint32_t i) { return P.d[f][int64_t(i) * kFieldStride]; }
@@ -342,7 +344,7 @@ __device__ __forceinline__ v3 random_in_unit_disk(rstream& rs) {  // vec3.h:139-
 // The camera ray of slot `slot` (camera.h:286-302 get_ray); sets the
 // slot's random stream, flags and first-queue entry.
 __device__ __forceinline__ void camera_ray(const dcam& C, const pstate& P, const batch_args& B, uint32_t slot,
-                                           uint32_t* q0, v3& org_out, v3& dir_out) {
+                                           uint32_t* q0, v3& org_out, v3& dir_out, double& flagctr) {
     int pass = B.first_pass + int(slot / B.npix);
     int pix = int(slot % B.npix);
     int i = pix % C.w, j = pix / C.w;
@@ -383,8 +385,7 @@ __device__ __forceinline__ void camera_ray(const dcam& C, const pstate& P, const
     org_out = org;
     dir_out = ps - org;
     P.key[slot] = rs.key;
-    P.ctr[slot] = rs.ctr;
-    P.flags[slot] = 1u;  // specular: camera rays see lights unweighted
+    flagctr = as_double((uint64_t(1u) << 32) | rs.ctr);  // specular: camera rays see lights unweighted
     q0[slot] = slot;
 }
 
@@ -401,10 +402,11 @@ __global__ __launch_bounds__(256) void k_camera(dcam C, pstate P, batch_args B, 
     if constexpr (MPT_PATH_RECORDS) {  // records are written block-wide (store_records); every thread takes part
         const int64_t s0 = int64_t(blockIdx.x) * blockDim.x;
         v3 org{0, 0, 0}, dir{0, 0, 0};
-        if (gt < B.nslots) camera_ray(C, P, B, uint32_t(gt), q0, org, dir);
+        double fc = 0;
+        if (gt < B.nslots) camera_ray(C, P, B, uint32_t(gt), q0, org, dir, fc);
         double* my = lds + threadIdx.x * kRecDoubles;
-        // record 0: ray, hit t, scatter pdf
-        my[0] = org.x, my[1] = org.y, my[2] = org.z, my[3] = dir.x, my[4] = dir.y, my[5] = dir.z, my[6] = 0, my[7] = 0;
+        // record 0: ray, scatter pdf, flags and counter
+        my[0] = org.x, my[1] = org.y, my[2] = org.z, my[3] = dir.x, my[4] = dir.y, my[5] = dir.z, my[6] = 0, my[7] = fc;
         if (s0 < B.nslots) store_records<256>(P, 0, s0, B.nslots, lds);
         // record 1: beta 1, radiance 0
         my[0] = 1, my[1] = 1, my[2] = 1, my[3] = 0, my[4] = 0, my[5] = 0, my[6] = 0, my[7] = 0;
@@ -417,9 +419,11 @@ __global__ __launch_bounds__(256) void k_camera(dcam C, pstate P, batch_args B, 
     if (gt >= B.nslots) return;
     const uint32_t slot = uint32_t(gt);
     v3 org, dir;
-    camera_ray(C, P, B, slot, q0, org, dir);
+    double fc;
+    camera_ray(C, P, B, slot, q0, org, dir, fc);
     st3(P, D_OX, slot, org);
     st3(P, D_DX, slot, dir);
+    pd(P, D_FLAGCTR, slot) = fc;
     st3(P, D_BX, slot, v3{1, 1, 1});
     st3(P, D_RX, slot, v3{0, 0, 0});
     if (lights) st3(P, D_PX, slot, v3{0, 0, 0});
@@ -474,9 +478,12 @@ __global__ __launch_bounds__(BS) void k_features(dscene S, launch_ctx L, pstate 
 // Each class gets its own queue, appended with one atomic per wave.
 // CNT: also count BVH work (nodes, primitive bytes) into work[0], work[1].
 // Waves per SIMD the spheres-only traversal is compiled for (its register
-// budget): 8 fits it in 64 VGPRs with a few spills.
+// budget): 6 gives it 80 VGPRs and no spills; at 8 (64 VGPRs) the slab
+// test's reciprocal direction spills inside the traversal loop, and that
+// costs more than the two waves it gains (1296 against 1410 M paths/s on
+// the 1080p Weekend cover).
 #ifndef MPT_SPHERE_WAVES
-#define MPT_SPHERE_WAVES 8
+#define MPT_SPHERE_WAVES 6
 #endif
 
 // Closest hit of path p's ray and the hit record shade reads (point, face
@@ -487,18 +494,14 @@ __device__ __forceinline__ int extend_one(const dscene& S, const pstate& P, int 
     const v3 o = ld3(P, D_OX, p), d = ld3(P, D_DX, p);
     hitinfo h;
     trace_closest<CNT, FEAT>(S, o, d, 0.001, kInf, h, st, true, P.key[p], uint32_t(2 * depth), wc);
-    P.hobj[p] = h.obj;
-    P.hprim[p] = h.prim;
     if (h.obj < 0) return 0;
-    // the hit record, every double of it (t and the pad first: t need not
-    // stay live through finish_hit)
+    // the hit record, every double of it (t first: it need not stay live through finish_hit)
     pd(P, D_HT, p) = h.t;
-    pd(P, D_PAD2, p) = 0;
     hitrec rec;
     finish_hit<FEAT>(S, h, o, d, rec);
     st3(P, D_QX, p, rec.p);
     st3(P, D_MX, p, rec.normal);
-    P.hmat[p] = uint32_t(rec.mat) | (rec.front ? 0x80000000u : 0u);
+    pd(P, D_HMAT, p) = as_double(uint32_t(rec.mat) | (rec.front ? 0x80000000u : 0u));
     return 1 + S.mat[rec.mat].type;
 }
 
@@ -570,7 +573,9 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
     double sw = 0, spdf_out = 0;
     bool vertex = false;
     v3 vertex_p{0, 0, 0};
-    uint32_t fl = P.flags[p];
+    const uint64_t fc = as_bits(pd(P, D_FLAGCTR, p));
+    const uint32_t fl = uint32_t(fc >> 32);
+    uint32_t ctr_out = uint32_t(fc);
     bool specular = fl & 1u, seen = fl & 2u;
     int dcount = (fl >> 2) & 1023, scount = (fl >> 12) & 1023, vcount = (fl >> 22) & 1023;
     if constexpr (MT < 0) {  // camera.h:373-387
@@ -590,7 +595,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
         rec.t = pd(P, D_HT, p);
         rec.p = ld3(P, D_QX, p);
         rec.normal = ld3(P, D_MX, p);
-        const uint32_t hm = P.hmat[p];
+        const uint32_t hm = uint32_t(as_bits(pd(P, D_HMAT, p)));
         rec.mat = int32_t(hm & 0x7fffffffu);
         rec.front = (hm >> 31) != 0;
         const mpt_material& m = S.mat[rec.mat];
@@ -608,7 +613,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
             st_rec(P, 1, p, beta, rad, 0.0, 0.0);
             return;  // a light scatters nothing (material.h:28-31)
         } else {
-            rstream rs{P.key[p], P.ctr[p]};
+            rstream rs{P.key[p], ctr_out};
             scatter_out so = mat_scatter<MT>(S, m, d, rec, rs);  // camera.h:409-411
             if (so.skip) {                                        // camera.h:413-423
                 bool stop = C.max_specular > 0 && ++scount > C.max_specular;
@@ -666,7 +671,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
                 if (rs.next() >= survive) alive = false;
                 else beta = beta / survive;
             }
-            P.ctr[p] = rs.ctr;
+            ctr_out = rs.ctr;
         }
     }
     if (depth + 1 >= C.max_depth) alive = false;
@@ -675,9 +680,9 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
     if (LIGHTS && vertex) st_rec(P, 3, p, vertex_p, vertex_p, 0.0, 0.0);
     st_rec(P, 1, p, beta, rad, sw, 0.0);  // a path that ends here keeps only its radiance (and a pending shadow ray)
     if (!alive) return;
-    st_rec(P, 0, p, o, d, spdf_out, 0.0);
-    P.flags[p] = (specular ? 1u : 0u) | (seen ? 2u : 0u) | (uint32_t(dcount) << 2) | (uint32_t(scount) << 12) |
-                 (uint32_t(vcount) << 22);
+    const uint32_t flags = (specular ? 1u : 0u) | (seen ? 2u : 0u) | (uint32_t(dcount) << 2) |
+                           (uint32_t(scount) << 12) | (uint32_t(vcount) << 22);
+    st_rec(P, 0, p, o, d, spdf_out, as_double((uint64_t(flags) << 32) | ctr_out));
 }
 
 // Waves per SIMD the shade kernels are compiled for (1: no register cap).
@@ -860,7 +865,7 @@ __global__ __launch_bounds__(BS) void k_nrc_prep(dscene S, pstate P, const uint3
         if (i < n) {
             const uint32_t p = q[i];
             const v3 x = ld3(P, D_QX, p), nrm = ld3(P, D_MX, p), d = ld3(P, D_DX, p);
-            const mpt_material& m = S.mat[P.hmat[p] & 0x7fffffffu];
+            const mpt_material& m = S.mat[uint32_t(as_bits(pd(P, D_HMAT, p))) & 0x7fffffffu];
             const v3 u = (x - B.box_min) * B.inv_ext;
             const v3 wo = -unit_vector(d);
             double rough = 1.0;
@@ -1426,11 +1431,6 @@ int ensure_paths(mpt_scene* s, int64_t cap) {
         s->ps.d[f] = MPT_PATH_RECORDS ? base + (int64_t(f / kRecDoubles) * cap) * kRecDoubles + f % kRecDoubles
                                       : base + int64_t(f) * cap;
     if ((rc = alloc(size_t(cap) * sizeof(uint64_t), reinterpret_cast<void**>(&s->ps.key)))) return rc;
-    if ((rc = alloc(size_t(cap) * sizeof(uint32_t), reinterpret_cast<void**>(&s->ps.ctr)))) return rc;
-    if ((rc = alloc(size_t(cap) * sizeof(uint32_t), reinterpret_cast<void**>(&s->ps.flags)))) return rc;
-    if ((rc = alloc(size_t(cap) * sizeof(int32_t), reinterpret_cast<void**>(&s->ps.hobj)))) return rc;
-    if ((rc = alloc(size_t(cap) * sizeof(int32_t), reinterpret_cast<void**>(&s->ps.hprim)))) return rc;
-    if ((rc = alloc(size_t(cap) * sizeof(uint32_t), reinterpret_cast<void**>(&s->ps.hmat)))) return rc;
     HIP_TRY(hipMalloc(&s->queues, size_t(cap) * (3 + kClasses) * sizeof(uint32_t)));
     if (!s->counters) HIP_TRY(hipMalloc(&s->counters, kCtrWords * sizeof(uint32_t) + 16 * sizeof(uint64_t)));
     s->ps_capacity = cap;
