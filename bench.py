@@ -32,9 +32,9 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 METRIC = "paths/s at 1080p; equal-time relMSE vs 4096spp ref (learned vs plain)"
 HBM_PEAK_GBS = 8000.0
-# per extension ray: queue index 4 + ray 48 + key 8 read, hit t/object/primitive 16 written,
-# class-queue append 4 (the hit record written for hits is not counted)
-EXT_RAY_BYTES = 80
+# per extension ray: queue index 4 + ray 48 read, class-queue append 4 (the 64-byte hit record
+# written for hits is not counted)
+EXT_RAY_BYTES = 56
 # per shadow ray: queue 4 + origin 24 + to_light 24 + key 8 + f 24 + weight 8 + beta 24 + radiance 24 read,
 # radiance 24 written
 SHADOW_RAY_BYTES = 164
