@@ -64,15 +64,23 @@ enum {
     MPT_MAT_PHASE = 5
 };
 
-/* Texture types (texture.h:24-56). */
-enum { MPT_TEX_SOLID = 0, MPT_TEX_CHECKER = 1 };
+/* Texture types (texture.h:24-105). */
+enum { MPT_TEX_SOLID = 0, MPT_TEX_CHECKER = 1, MPT_TEX_IMAGE = 2 };
+/* Image texture flags: 8-bit values are data (k / 255), not sRGB; (u, v)
+ * wraps around instead of clamping at the edge; the texels are linear
+ * floats (a Radiance .hdr file) instead of bytes. */
+enum { MPT_TEXF_LINEAR = 1, MPT_TEXF_REPEAT = 2, MPT_TEXF_HDR = 4 };
 
 typedef struct mpt_texture {
     int32_t type;        /* MPT_TEX_* */
-    int32_t pad;
+    int32_t flags;       /* image: MPT_TEXF_* */
     double inv_scale;    /* checker: 1 / cube side */
     double even[3];      /* solid: the color; checker: even cells */
     double odd[3];       /* checker: odd cells */
+    int32_t width;       /* image, in texels */
+    int32_t height;
+    int64_t offset;      /* image: first texel in mpt_scene_desc.texels (4 bytes, RGBA, per texel) or
+                            .texels_hdr (3 floats per texel), rows top first */
 } mpt_texture;
 
 typedef struct mpt_material {
@@ -91,6 +99,11 @@ typedef struct mpt_material {
     double emit[3];      /* diffuse_light radiance */
     double sigma_a[3];   /* dielectric absorption (Beer-Lambert) */
     double avg_emission[3]; /* diffuse_light: emit * directional share */
+    int32_t normal_map;        /* tangent-space normal map texture, -1 = none (normal_mapped, material.h:631-674;
+                                  principled's own, material.h:389) */
+    int32_t metallic_texture;  /* principled: metallic from this texture's red channel, -1 = the constant */
+    int32_t roughness_texture; /* principled: roughness from this texture's red channel, -1 = the constant */
+    int32_t uses_uv;           /* 1 when a texture above or the base texture is an image: hits need (u, v) */
 } mpt_material;
 
 /* An affine transform with its stored inverse, as transform.h:120-123:
@@ -155,6 +168,17 @@ typedef struct mpt_scene_desc {
      * barycentrics, normalised and turned to the geometric normal's side
      * (mesh.h:93-101). */
     const double* prim_normals;
+
+    /* Image texture storage: RGBA bytes and RGB floats, addressed by
+     * mpt_texture.offset. May be NULL when no texture uses them. */
+    const uint8_t* texels;
+    int64_t n_texel_bytes;
+    const float* texels_hdr;
+    int64_t n_texels_hdr;
+    /* Optional per-triangle texture coordinates, 6 per primitive (u, v at
+     * p0, p1, p2); NaN marks a triangle without them, whose (u, v) are its
+     * barycentrics b1, b2 (mesh.h:75-81). NULL: none. */
+    const double* prim_uvs;
 } mpt_scene_desc;
 
 /* Camera and render settings: the public fields of the reference camera

@@ -117,6 +117,22 @@ public:
     double inv_scale;
     std::shared_ptr<texture> even, odd;
 };
+// An image over the (u, v) unit square, v = 0 at the bottom row, looked up at
+// the nearest texel (texture.h:58-205). PNG and JPEG files are 8-bit: sRGB
+// encoded colors, or data taken as k / 255 with linear set; Radiance .hdr
+// files are linear floats. repeat wraps (u, v) around instead of clamping.
+class image_texture : public texture {
+public:
+    explicit image_texture(const std::string& path, bool repeat = false, bool linear = false);
+    bool loaded() const { return width > 0 && height > 0; }
+    mpt_texture to_abi() const override;  // offset 0: the builder places the texels
+    int width = 0, height = 0;
+    int channels = 0;           // of the file (1 grey, 2 grey + alpha, 3, 4)
+    bool repeat, linear;
+    std::string error;          // why loading failed
+    std::vector<uint8_t> ldr;   // RGBA bytes, rows top first
+    std::vector<float> hdr;     // RGB floats, rows top first
+};
 
 // ------------------------------------------------------------------ materials
 class material {
@@ -126,7 +142,25 @@ public:
     // material's texture (or -1).
     virtual mpt_material to_abi(int32_t texture_index) const = 0;
     virtual std::shared_ptr<texture> base_texture() const { return nullptr; }
+    // Tangent-space normal map and principled's metallic / roughness maps
+    // (null: none, or the material's constant).
+    virtual std::shared_ptr<texture> normal_map_texture() const { return nullptr; }
+    virtual std::shared_ptr<texture> metallic_texture() const { return nullptr; }
+    virtual std::shared_ptr<texture> roughness_texture() const { return nullptr; }
     virtual color average_emission() const { return color(0, 0, 0); }
+};
+// A material whose shading normal is bent by a tangent-space normal map
+// (normal_mapped, material.h:631-674); for lambertian and metal.
+class normal_mapped : public material {
+public:
+    normal_mapped(std::shared_ptr<material> inner, std::shared_ptr<texture> normal_map)
+        : inner(std::move(inner)), normal_map(std::move(normal_map)) {}
+    mpt_material to_abi(int32_t ti) const override { return inner->to_abi(ti); }
+    std::shared_ptr<texture> base_texture() const override { return inner->base_texture(); }
+    std::shared_ptr<texture> normal_map_texture() const override { return normal_map; }
+    color average_emission() const override { return inner->average_emission(); }
+    std::shared_ptr<material> inner;
+    std::shared_ptr<texture> normal_map;
 };
 class lambertian : public material {
 public:
@@ -156,8 +190,14 @@ public:
     principled(std::shared_ptr<texture> base_color, double metallic, double roughness, double ior);
     mpt_material to_abi(int32_t ti) const override;
     std::shared_ptr<texture> base_texture() const override { return base_color; }
+    std::shared_ptr<texture> normal_map_texture() const override { return normal_map; }
+    std::shared_ptr<texture> metallic_texture() const override { return metallic_map; }
+    std::shared_ptr<texture> roughness_texture() const override { return roughness_map; }
     std::shared_ptr<texture> base_color;
     double metallic, roughness, ior;
+    // read from the red channel at each hit when set (material.h:508-515)
+    std::shared_ptr<texture> metallic_map, roughness_map;
+    std::shared_ptr<texture> normal_map;  // bends the base layer (material.h:389)
 };
 class diffuse_light : public material {
 public:
@@ -244,6 +284,10 @@ public:
     // normal_indices parallels indices; a triangle with any -1 is flat
     std::vector<vec3> normals;
     std::vector<int32_t> normal_indices;
+    // optional per-vertex texture coordinates (mesh_data::uvs, x = u, y = v):
+    // uv_indices parallels indices; a triangle with any -1 has none
+    std::vector<vec3> uvs;
+    std::vector<int32_t> uv_indices;
 };
 
 // A part of an OBJ file that uses one material (obj_loader.h:19-23).
@@ -429,9 +473,11 @@ public:
 
     // used by hittable::flatten
     int32_t material_index(const std::shared_ptr<material>& m);
+    int32_t texture_index(const std::shared_ptr<texture>& t);  // -1 for null; image texels placed once
     int32_t xform_index(const transform& x);
     int64_t add_prim(int32_t kind, int32_t mat, const double* data9);
     void set_prim_normals(int64_t prim, const double* n9);  // a triangle's three vertex normals
+    void set_prim_uvs(int64_t prim, const double* uv6);     // a triangle's three texture coordinates
     int32_t add_object(int32_t kind, int64_t first, int64_t count, int32_t xform, const hittable* owner);
     std::map<const void*, std::pair<int64_t, int64_t>> mesh_ranges;
     struct impl;

@@ -15,14 +15,15 @@ MPT_E_CAPACITY = -5
 PRIM_SPHERE, PRIM_QUAD, PRIM_TRIANGLE = 0, 1, 2
 OBJ_PRIM, OBJ_LIST, OBJ_MESH, OBJ_MEDIUM = 0, 1, 2, 3
 MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_PRINCIPLED, MAT_DIFFUSE_LIGHT, MAT_PHASE = range(6)
-TEX_SOLID, TEX_CHECKER = 0, 1
+TEX_SOLID, TEX_CHECKER, TEX_IMAGE = 0, 1, 2
+TEXF_LINEAR, TEXF_REPEAT, TEXF_HDR = 1, 2, 4
 
 D3 = C.c_double * 3
 
 
 class Texture(C.Structure):
-    _fields_ = [("type", C.c_int32), ("pad", C.c_int32), ("inv_scale", C.c_double),
-                ("even", D3), ("odd", D3)]
+    _fields_ = [("type", C.c_int32), ("flags", C.c_int32), ("inv_scale", C.c_double),
+                ("even", D3), ("odd", D3), ("width", C.c_int32), ("height", C.c_int32), ("offset", C.c_int64)]
 
 
 class Material(C.Structure):
@@ -30,7 +31,8 @@ class Material(C.Structure):
                 ("alpha", C.c_double), ("ior", C.c_double), ("metallic", C.c_double),
                 ("f0", C.c_double), ("g", C.c_double), ("spread", C.c_double),
                 ("tan_half", C.c_double), ("albedo", D3), ("f_avg", D3), ("emit", D3),
-                ("sigma_a", D3), ("avg_emission", D3)]
+                ("sigma_a", D3), ("avg_emission", D3), ("normal_map", C.c_int32),
+                ("metallic_texture", C.c_int32), ("roughness_texture", C.c_int32), ("uses_uv", C.c_int32)]
 
 
 class Xform(C.Structure):
@@ -68,6 +70,11 @@ class SceneDesc(C.Structure):
         ("env_intensity", C.c_double),
         ("env_rotation_degrees", C.c_double),
         ("prim_normals", C.POINTER(C.c_double)),
+        ("texels", C.POINTER(C.c_uint8)),
+        ("n_texel_bytes", C.c_int64),
+        ("texels_hdr", C.POINTER(C.c_float)),
+        ("n_texels_hdr", C.c_int64),
+        ("prim_uvs", C.POINTER(C.c_double)),
     ]
 
 

@@ -19,14 +19,87 @@ struct rstream {
     __device__ __forceinline__ double next(double lo, double hi) { return lo + (hi - lo) * next(); }
 };
 
-__device__ __forceinline__ v3 tex_value(const dscene& S, int ti, const v3& p) {  // texture.h:24-56
+// texture::value(u, v, p) (texture.h:24-56, 119-127): solid, the 3D checker,
+// or the image texel nearest to (u, v) (texture.h:182-195). IMG false: the
+// scene has no image textures, and their code is left out.
+template <bool IMG = true>
+__device__ __forceinline__ v3 tex_value(const dscene& S, int ti, double u, double v, const v3& p) {
     const mpt_texture& t = S.tex[ti];
     if (t.type == MPT_TEX_SOLID) return v3{t.even[0], t.even[1], t.even[2]};
-    int x = int(std::floor(t.inv_scale * p.x));
-    int y = int(std::floor(t.inv_scale * p.y));
-    int z = int(std::floor(t.inv_scale * p.z));
-    bool even = (x + y + z) % 2 == 0;
-    return even ? v3{t.even[0], t.even[1], t.even[2]} : v3{t.odd[0], t.odd[1], t.odd[2]};
+    if (!IMG || t.type == MPT_TEX_CHECKER) {
+        int x = int(std::floor(t.inv_scale * p.x));
+        int y = int(std::floor(t.inv_scale * p.y));
+        int z = int(std::floor(t.inv_scale * p.z));
+        bool even = (x + y + z) % 2 == 0;
+        return even ? v3{t.even[0], t.even[1], t.even[2]} : v3{t.odd[0], t.odd[1], t.odd[2]};
+    }
+    if (t.flags & MPT_TEXF_REPEAT) {
+        u -= std::floor(u);
+        v -= std::floor(v);
+    } else {
+        u = std::fmin(std::fmax(u, 0.0), 1.0);
+        v = std::fmin(std::fmax(v, 0.0), 1.0);
+    }
+    v = 1.0 - v;  // image rows run down
+    const int i = min(max(int(u * t.width), 0), t.width - 1);
+    const int j = min(max(int(v * t.height), 0), t.height - 1);
+    const int64_t k = t.offset + int64_t(j) * t.width + i;
+    if (t.flags & MPT_TEXF_HDR) {
+        const float* c = S.texels_hdr + 3 * k;
+        return v3{double(c[0]), double(c[1]), double(c[2])};
+    }
+    const uchar4 c = *reinterpret_cast<const uchar4*>(S.texels + 4 * k);
+    if (t.flags & MPT_TEXF_LINEAR) return v3{double(c.x), double(c.y), double(c.z)} / 255.0;
+    return v3{S.srgb[c.x], S.srgb[c.y], S.srgb[c.z]};
+}
+template <bool IMG = true>
+__device__ __forceinline__ v3 tex_value(const dscene& S, int ti, const hitrec& rec) {
+    return tex_value<IMG>(S, ti, rec.u, rec.v, rec.p);
+}
+
+// normal_from_map (material.h:282-308): the tangent-space normal map's normal
+// in the frame of the outward normal and dp/du, back on the incoming ray's
+// side and tilted towards the viewer if it faces away (cosine at least 0.01).
+// A hit without a tangent keeps its normal.
+__device__ __forceinline__ v3 normal_from_map(const dscene& S, int ti, const v3& rd, const hitrec& rec) {
+    constexpr double min_cos_o = 0.01;
+    const v3 n = rec.front ? rec.normal : -rec.normal;
+    v3 t = rec.dpdu - dot(rec.dpdu, n) * n;
+    if (length_squared(t) <= 1e-12 * length_squared(rec.dpdu)) return rec.normal;
+    t = unit_vector(t);
+    v3 b = cross(n, t);
+    if (dot(b, rec.dpdv) < 0) b = -b;  // mirrored (u, v)
+    const v3 c = tex_value(S, ti, rec);
+    const double nz = std::fmax(2 * c.z - 1, 1e-3);
+    v3 m = unit_vector((2 * c.x - 1) * t + (2 * c.y - 1) * b + nz * n);
+    if (!rec.front) m = -m;
+    const v3 wo = unit_vector(-rd);
+    const double cos_o = dot(m, wo);
+    if (cos_o < min_cos_o) m = unit_vector(m + (min_cos_o - cos_o) * wo);
+    return m;
+}
+
+// normal_mapped::scatter's repair (material.h:640-645): a mirror direction on
+// the shading normal's side that went below the surface is reflected about
+// the hit's own normal gn.
+__device__ __forceinline__ v3 unbend(const v3& dir, const v3& shading, const v3& gn) {
+    if (dot(dir, shading) > 0 && dot(dir, gn) < 0) return dir - 2 * dot(dir, gn) * gn;
+    return dir;
+}
+
+// principled's metallic and GGX alpha at a hit: from the red channel of their
+// maps, clamped to [0, 1] (material.h:508-515), or the constants.
+struct pparams {
+    double metallic, alpha;
+};
+template <bool UV>
+__device__ __forceinline__ pparams principled_at(const dscene& S, const mpt_material& m, const hitrec& rec) {
+    pparams q{m.metallic, m.alpha};
+    if (!UV) return q;
+    if (m.metallic_texture >= 0) q.metallic = std::fmin(std::fmax(tex_value(S, m.metallic_texture, rec).x, 0.0), 1.0);
+    if (m.roughness_texture >= 0)
+        q.alpha = roughness_to_alpha(std::fmin(std::fmax(tex_value(S, m.roughness_texture, rec).x, 0.0), 1.0));
+    return q;
 }
 
 __device__ __forceinline__ double avg3(const v3& c) { return (c.x + c.y + c.z) / 3; }
@@ -239,13 +312,14 @@ struct scatter_out {
     sdensity dens;
 };
 
-__device__ __forceinline__ v3 principled_f0(const mpt_material& m, const v3& base) {
-    return (1 - m.metallic) * v3{m.f0, m.f0, m.f0} + m.metallic * base;
+__device__ __forceinline__ v3 principled_f0(const mpt_material& m, double metallic, const v3& base) {
+    return (1 - metallic) * v3{m.f0, m.f0, m.f0} + metallic * base;
 }
 
 // material::scatter (material.h:79-85, 123-150, 210-242, 387-421; medium.h:21-27).
 // MT: the material type when known at compile time (-1: dispatch on m.type).
-template <int MT = -1>
+// UV: the scene has image textures or texture maps (hits carry (u, v)).
+template <int MT = -1, bool UV = false>
 __device__ __forceinline__ scatter_out mat_scatter(const dscene& S, const mpt_material& m, const v3& rd,
                                                    const hitrec& rec, rstream& rs) {
     scatter_out o;
@@ -261,7 +335,7 @@ __device__ __forceinline__ scatter_out mat_scatter(const dscene& S, const mpt_ma
     o.dens.w0 = 1;
     switch (MT >= 0 ? MT : m.type) {
         case MPT_MAT_LAMBERTIAN:
-            o.attenuation = tex_value(S, m.texture, rec.p);
+            o.attenuation = tex_value<UV>(S, m.texture, rec);
             break;
         case MPT_MAT_METAL: {
             v3 albedo = m3(m.albedo);
@@ -306,11 +380,12 @@ __device__ __forceinline__ scatter_out mat_scatter(const dscene& S, const mpt_ma
         }
         case MPT_MAT_PRINCIPLED: {
             v3 wo = unit_vector(-rd);
-            v3 base = tex_value(S, m.texture, rec.p);
-            double metallic = m.metallic;
-            double alpha = m.alpha;
+            const pparams pp = principled_at<UV>(S, m, rec);
+            v3 base = tex_value<UV>(S, m.texture, rec);
+            double metallic = pp.metallic;
+            double alpha = pp.alpha;
             double cos_o = std::fmax(0.0, dot(rec.normal, wo));
-            v3 sf0 = principled_f0(m, base);
+            v3 sf0 = principled_f0(m, metallic, base);
             double spec_w = avg3(fresnel_schlick3(cos_o, sf0));
             double diff_w = (1 - metallic) * avg3(base) * (1 - fresnel_schlick(cos_o, m.f0));
             diff_w += ggx_ms_albedo(S.ggx_e, S.ggx_eavg, cos_o, alpha) * avg3(ggx_ms_fresnel(S.ggx_eavg, sf0, alpha));
@@ -335,15 +410,18 @@ __device__ __forceinline__ scatter_out mat_scatter(const dscene& S, const mpt_ma
     return o;
 }
 
-// material::eval: f * |cos| for a non-specular direction.
-template <int MT = -1>
+// material::eval: f * |cos| for a non-specular direction. rec.normal is the
+// shading normal (a normal map's, when the material has one) and gn the
+// hit's own: no light arrives from below gn (material.h:426, 651).
+template <int MT = -1, bool UV = false>
 __device__ __forceinline__ v3 mat_eval(const dscene& S, const mpt_material& m, const v3& rd, const hitrec& rec,
-                                       const v3& dir) {
+                                       const v3& gn, const v3& dir) {
+    if (UV && m.normal_map >= 0 && dot(dir, gn) <= 0) return v3{0, 0, 0};
     switch (MT >= 0 ? MT : m.type) {
         case MPT_MAT_LAMBERTIAN: {  // material.h:88-97
             double c = dot(rec.normal, unit_vector(dir));
             double pdf = c < 0 ? 0 : c / kPi;
-            return tex_value(S, m.texture, rec.p) * pdf;
+            return tex_value<UV>(S, m.texture, rec) * pdf;
         }
         case MPT_MAT_METAL: {  // material.h:154-164
             onb f = make_onb(rec.normal);
@@ -363,17 +441,18 @@ __device__ __forceinline__ v3 mat_eval(const dscene& S, const mpt_material& m, c
             return v3{f, f, f};
         }
         case MPT_MAT_PRINCIPLED: {  // material.h:423-431, 486-506
-            if (dot(dir, rec.normal) <= 0) return v3{0, 0, 0};
+            if (dot(dir, gn) <= 0) return v3{0, 0, 0};
             v3 wo_w = unit_vector(-rd);
             v3 wi_w = unit_vector(dir);
             onb f = make_onb(rec.normal);
             v3 wo = unit_vector(onb_to_local(f, wo_w));
             v3 wi = unit_vector(onb_to_local(f, wi_w));
             if (wo.z <= 0 || wi.z <= 0) return v3{0, 0, 0};
-            double metallic = m.metallic, alpha = m.alpha;
-            v3 base = tex_value(S, m.texture, rec.p);
+            const pparams pp = principled_at<UV>(S, m, rec);
+            double metallic = pp.metallic, alpha = pp.alpha;
+            v3 base = tex_value<UV>(S, m.texture, rec);
             v3 h = unit_vector(wo + wi);
-            v3 sf0 = principled_f0(m, base);
+            v3 sf0 = principled_f0(m, metallic, base);
             v3 specular = fresnel_schlick3(dot(wi, h), sf0) * (ggx_D(alpha, h) * ggx_G(alpha, wo, wi) / (4 * wo.z)) +
                           ggx_ms_fresnel(S.ggx_eavg, sf0, alpha) * ggx_ms_eval(S.ggx_e, S.ggx_eavg, wo.z, wi.z, alpha);
             double transmitted = (1 - fresnel_schlick(wi.z, m.f0)) * (1 - fresnel_schlick(wo.z, m.f0));

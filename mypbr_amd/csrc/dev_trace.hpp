@@ -39,6 +39,10 @@ struct dscene {
     const light_rec* lights;
     const double* lcdf;
     const double* vnrm;          // triangles' vertex normals, 9 per primitive (NaN: none), or null
+    const double* vuv;           // triangles' texture coordinates, 6 per primitive (NaN: none), or null
+    const uint8_t* texels;       // image textures: RGBA bytes
+    const float* texels_hdr;     // image textures: RGB floats
+    const double* srgb;          // 256: linear value of each 8-bit sRGB code
     const int64_t* mlight_prim;  // mesh lights: triangles in mesh order, and their area CDF
     const double* mlight_cdf;
     const double* env_rgb;
@@ -96,6 +100,8 @@ enum : int {
     F_MESH = 32,
     F_MEDIUM = 64,
     F_ALL = 127,
+    // not geometry: hits also get (u, v) and dp/du, dp/dv (image textures, normal maps)
+    F_UV = 128,
 };
 
 struct hitinfo {
@@ -431,6 +437,10 @@ struct hitrec {
     double t;
     int32_t mat;
     bool front;
+    // texture coordinates and how p moves with them (hittable.h:14-24); set
+    // by finish_hit when FEAT has F_UV, else zero
+    double u = 0, v = 0;
+    v3 dpdu{0, 0, 0}, dpdv{0, 0, 0};
 };
 
 template <int FEAT = F_ALL>
@@ -461,6 +471,63 @@ __device__ __forceinline__ void finish_hit(const dscene& S, const hitinfo& h, co
     }
     rec.front = dot(d, outward) < 0;
     v3 n = rec.front ? outward : -outward;
+    if constexpr (FEAT & F_UV) {
+        if (k == MPT_PRIM_SPHERE) {  // sphere.h:46-48, 99-120, on the unit sphere's point
+            const double r = q[3];
+            const v3 un = outward * (r < 0 ? -1.0 : 1.0);
+            const double theta = std::acos(std::fmin(std::fmax(-un.y, -1.0), 1.0));
+            const double phi = std::atan2(-un.z, un.x) + kPi;
+            rec.u = phi / (2 * kPi);
+            rec.v = theta / kPi;
+            const double ar = std::fabs(r);
+            const double sin_theta = std::sqrt(un.x * un.x + un.z * un.z);
+            if (sin_theta < 1e-8) {
+                rec.dpdu = rec.dpdv = v3{0, 0, 0};
+            } else {
+                rec.dpdu = (2 * kPi * ar) * v3{un.z, 0, -un.x};
+                rec.dpdv = (kPi * ar / sin_theta) * v3{-un.x * un.y, sin_theta * sin_theta, -un.y * un.z};
+            }
+        } else if (k == MPT_PRIM_QUAD) {  // quad.h:41-57
+            const v3 Q{q[0], q[1], q[2]}, qu{q[3], q[4], q[5]}, qv{q[6], q[7], q[8]}, w{q[12], q[13], q[14]};
+            const v3 ph = p - Q;
+            rec.u = dot(w, cross(ph, qv));
+            rec.v = dot(w, cross(qu, ph));
+            rec.dpdu = qu;
+            rec.dpdv = qv;
+        } else {  // mesh.h:71-93, 149-166: barycentrics by the hit test's arithmetic
+            const v3 p0{q[0], q[1], q[2]}, e1{q[3], q[4], q[5]}, e2{q[6], q[7], q[8]};
+            const v3 pvec = cross(d, e2);
+            const double inv_det = 1 / dot(e1, pvec);
+            const v3 tvec = o - p0;
+            const double b1 = dot(tvec, pvec) * inv_det;
+            const double b2 = dot(d, cross(tvec, e1)) * inv_det;
+            const double* t6 = S.vuv ? S.vuv + 6 * h.prim : nullptr;
+            if (t6 && !std::isnan(t6[0])) {
+                const double b0 = 1 - b1 - b2;
+                rec.u = b0 * t6[0] + b1 * t6[2] + b2 * t6[4];
+                rec.v = b0 * t6[1] + b1 * t6[3] + b2 * t6[5];
+                const double du1 = t6[2] - t6[0], dv1 = t6[3] - t6[1];
+                const double du2 = t6[4] - t6[0], dv2 = t6[5] - t6[1];
+                const double det = du1 * dv2 - du2 * dv1;
+                if (std::fabs(det) < 1e-12) {
+                    rec.dpdu = rec.dpdv = v3{0, 0, 0};
+                } else {
+                    const double inv = 1 / det;
+                    rec.dpdu = (dv2 * e1 - dv1 * e2) * inv;
+                    rec.dpdv = (du1 * e2 - du2 * e1) * inv;
+                }
+            } else {
+                rec.u = b1;
+                rec.v = b2;
+                rec.dpdu = e1;
+                rec.dpdv = e2;
+            }
+        }
+        if (x) {  // instance.h:38-39
+            rec.dpdu = xf_vector(*x, rec.dpdu);
+            rec.dpdv = xf_vector(*x, rec.dpdv);
+        }
+    }
     if ((FEAT & F_TRI) && S.vnrm && k == MPT_PRIM_TRIANGLE && !std::isnan(S.vnrm[9 * h.prim])) {
         // interpolated shading normal on the geometric normal's side (mesh.h:93-101);
         // the barycentrics by the arithmetic of the hit test

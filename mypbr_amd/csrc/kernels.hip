@@ -153,8 +153,12 @@ enum {
     D_SBX, D_SBY, D_SBZ, D_PAD5A, D_PAD5B, D_PAD5C, D_PAD5D, D_PAD5E,
     // record 6: first-surface guides
     D_AX, D_AY, D_AZ, D_NX, D_NY, D_NZ, D_PAD6A, D_PAD6B,
+    // record 7: texture coordinates and dp/du, dp/dv of the hit (allocated and
+    // written only in scenes whose materials read them)
+    D_U, D_V, D_DUX, D_DUY, D_DUZ, D_DVX, D_DVY, D_DVZ,
     D_COUNT
 };
+constexpr int kUvRecord = 7;
 #ifndef MPT_PATH_RECORDS
 #define MPT_PATH_RECORDS 1
 #endif
@@ -431,7 +435,7 @@ __global__ __launch_bounds__(256) void k_camera(dcam C, pstate P, batch_args B, 
 }
 
 // camera::first_surface (camera.h:253-277) on the feature stream.
-template <int BS>
+template <int BS, int FEAT>
 __global__ __launch_bounds__(BS) void k_features(dscene S, launch_ctx L, pstate P, batch_args B) {
     STACK_SETUP();
     for (int64_t s = st.gtid; s < B.nslots; s += L.nthreads) {
@@ -444,12 +448,16 @@ __global__ __launch_bounds__(BS) void k_features(dscene S, launch_ctx L, pstate 
         for (int bounce = 0; bounce < 8 && !done; bounce++) {
             hitinfo h;
             wcount wc{0, 0};
-            if (!trace_closest(S, o, d, 0.001, kInf, h, st, true, rs.key, uint32_t(bounce), wc)) break;
+            if (!trace_closest<false, FEAT>(S, o, d, 0.001, kInf, h, st, true, rs.key, uint32_t(bounce), wc)) break;
             hitrec rec;
-            finish_hit(S, h, o, d, rec);
+            finish_hit<FEAT>(S, h, o, d, rec);
             const mpt_material& m = S.mat[rec.mat];
             tint = tint * mat_interior(m, d, rec);
-            scatter_out so = mat_scatter(S, m, d, rec, rs);
+            constexpr bool kUV = (FEAT & F_UV) != 0;
+            hitrec shaded = rec;  // the guide keeps the hit's own normal
+            if (kUV && m.normal_map >= 0) shaded.normal = normal_from_map(S, m.normal_map, d, rec);
+            scatter_out so = mat_scatter<-1, kUV>(S, m, d, shaded, rs);
+            if (kUV && so.skip && m.normal_map >= 0) so.skip_dir = unbend(so.skip_dir, shaded.normal, rec.normal);
             if (!so.scattered) {
                 albedo = tint;
                 normal = rec.normal;
@@ -502,7 +510,11 @@ __device__ __forceinline__ int extend_one(const dscene& S, const pstate& P, int 
     st3(P, D_QX, p, rec.p);
     st3(P, D_MX, p, rec.normal);
     pd(P, D_HMAT, p) = as_double(uint32_t(rec.mat) | (rec.front ? 0x80000000u : 0u));
-    return 1 + S.mat[rec.mat].type;
+    const mpt_material& m = S.mat[rec.mat];
+    if constexpr (FEAT & F_UV)
+        if (m.uses_uv) st_rec(P, kUvRecord, p, rec.u, rec.v, rec.dpdu.x, rec.dpdu.y, rec.dpdu.z, rec.dpdv.x, rec.dpdv.y,
+                              rec.dpdv.z);
+    return 1 + m.type;
 }
 
 // Paths a batch traces to the end to train the radiance cache: one in
@@ -561,7 +573,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_SP
 // missed); LIGHTS: the scene has lights, so non-specular vertices do next
 // event estimation. Compiling each material apart keeps each kernel's
 // register footprint to what that material needs.
-template <int MT, int LK, bool ENV>
+template <int MT, int LK, bool ENV, bool UV = false>
 __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const pstate& P, int depth, uint32_t p,
                                           bool& alive, bool& nee) {
     constexpr bool LIGHTS = LK != 0;
@@ -599,6 +611,15 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
         rec.mat = int32_t(hm & 0x7fffffffu);
         rec.front = (hm >> 31) != 0;
         const mpt_material& m = S.mat[rec.mat];
+        if (UV && m.uses_uv) {
+            rec.u = pd(P, D_U, p);
+            rec.v = pd(P, D_V, p);
+            rec.dpdu = ld3(P, D_DUX, p);
+            rec.dpdv = ld3(P, D_DVX, p);
+        }
+        // the hit's own normal; a normal map bends the one the material shades with
+        const v3 gn = rec.normal;
+        if (UV && m.normal_map >= 0) rec.normal = normal_from_map(S, m.normal_map, d, rec);
         if constexpr (MT == MPT_MAT_DIELECTRIC) beta = beta * mat_interior<MT>(m, d, rec);  // camera.h:396
         if constexpr (MT == MPT_MAT_DIFFUSE_LIGHT) {                                          // camera.h:399-407
             v3 e = mat_emitted<MT>(m, d, rec);
@@ -614,7 +635,8 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
             return;  // a light scatters nothing (material.h:28-31)
         } else {
             rstream rs{P.key[p], ctr_out};
-            scatter_out so = mat_scatter<MT>(S, m, d, rec, rs);  // camera.h:409-411
+            scatter_out so = mat_scatter<MT, UV>(S, m, d, rec, rs);  // camera.h:409-411
+            if (UV && so.skip && m.normal_map >= 0) so.skip_dir = unbend(so.skip_dir, rec.normal, gn);
             if (so.skip) {                                        // camera.h:413-423
                 bool stop = C.max_specular > 0 && ++scount > C.max_specular;
                 if (!stop) {
@@ -636,7 +658,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
                     v3 to_light = lights_random<LK>(S, rec.p, rs);
                     double light_pdf = lights_pdf<LK>(S, rec.p, to_light);
                     if (light_pdf > 0) {
-                        v3 f = mat_eval<MT>(S, m, d, rec, to_light);
+                        v3 f = mat_eval<MT, UV>(S, m, d, rec, gn, to_light);
                         if (!(f.x <= 0 && f.y <= 0 && f.z <= 0)) {
                             double weight =
                                 last_bounce ? 1.0 : power_heuristic(light_pdf, density_value<MT>(so.dens, to_light));
@@ -653,7 +675,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
                     v3 dir = density_generate<MT>(so.dens, rs);
                     double spdf = density_value<MT>(so.dens, dir);
                     if (spdf > 0) {
-                        beta = beta * mat_eval<MT>(S, m, d, rec, dir) / spdf;
+                        beta = beta * mat_eval<MT, UV>(S, m, d, rec, gn, dir) / spdf;
                         if constexpr (LIGHTS) {  // read only by the MIS weights of next-event estimation
                             spdf_out = spdf;
                             vertex = true;
@@ -690,7 +712,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
 #define MPT_SHADE_WAVES 1
 #endif
 
-template <int BS, int MT, int LK, bool ENV>
+template <int BS, int MT, int LK, bool ENV, bool UV = false>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MPT_SHADE_WAVES))) void k_shade(dscene S, dcam C, pstate P, int depth, const uint32_t* q,
                                               const uint32_t* cnt_in, uint32_t* qnext, uint32_t* cnt_next,
                                               uint32_t* qshadow, uint32_t* cnt_shadow) {
@@ -703,7 +725,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MPT_SHADE_WA
         uint32_t p = 0;
         if (i < n) {
             p = q[i];
-            shade_one<MT, LK, ENV>(S, C, P, depth, p, alive, nee);
+            shade_one<MT, LK, ENV, UV>(S, C, P, depth, p, alive, nee);
         }
         block_append<BS, NQ>(
             lds, (alive ? 1u : 0u) | (nee ? 2u : 0u), p, [&](int k) { return k ? cnt_shadow : cnt_next; },
@@ -787,6 +809,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MPT_FINISH_W
                                                const uint32_t* q, const uint32_t* cnt_in,
                                                unsigned long long* ray_stats) {
     STACK_SETUP();
+    constexpr bool kUV = (FEAT & F_UV) != 0;
     const uint32_t n = *cnt_in;
     unsigned long long ext = 0, shd = 0;
     wcount wc{0, 0};
@@ -798,10 +821,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MPT_FINISH_W
             bool alive = false, nee = false;
             switch (cls) {
                 case 0: shade_one<-1, LK, ENV>(S, C, P, depth, p, alive, nee); break;
-                case 1 + MPT_MAT_LAMBERTIAN: shade_one<MPT_MAT_LAMBERTIAN, LK, false>(S, C, P, depth, p, alive, nee); break;
-                case 1 + MPT_MAT_METAL: shade_one<MPT_MAT_METAL, LK, false>(S, C, P, depth, p, alive, nee); break;
+                case 1 + MPT_MAT_LAMBERTIAN:
+                    shade_one<MPT_MAT_LAMBERTIAN, LK, false, kUV>(S, C, P, depth, p, alive, nee);
+                    break;
+                case 1 + MPT_MAT_METAL: shade_one<MPT_MAT_METAL, LK, false, kUV>(S, C, P, depth, p, alive, nee); break;
                 case 1 + MPT_MAT_DIELECTRIC: shade_one<MPT_MAT_DIELECTRIC, LK, false>(S, C, P, depth, p, alive, nee); break;
-                case 1 + MPT_MAT_PRINCIPLED: shade_one<MPT_MAT_PRINCIPLED, LK, false>(S, C, P, depth, p, alive, nee); break;
+                case 1 + MPT_MAT_PRINCIPLED:
+                    shade_one<MPT_MAT_PRINCIPLED, LK, false, kUV>(S, C, P, depth, p, alive, nee);
+                    break;
                 case 1 + MPT_MAT_DIFFUSE_LIGHT:
                     shade_one<MPT_MAT_DIFFUSE_LIGHT, LK, false>(S, C, P, depth, p, alive, nee);
                     break;
@@ -870,11 +897,12 @@ __global__ __launch_bounds__(BS) void k_nrc_prep(dscene S, pstate P, const uint3
             const v3 wo = -unit_vector(d);
             double rough = 1.0;
             v3 alb{1, 1, 1};
+            const double tu = m.uses_uv ? pd(P, D_U, p) : 0.0, tv = m.uses_uv ? pd(P, D_V, p) : 0.0;
             switch (m.type) {
-                case MPT_MAT_LAMBERTIAN: alb = tex_value(S, m.texture, x); break;
+                case MPT_MAT_LAMBERTIAN: alb = tex_value(S, m.texture, tu, tv, x); break;
                 case MPT_MAT_METAL: alb = v3{m.albedo[0], m.albedo[1], m.albedo[2]}, rough = m.roughness; break;
                 case MPT_MAT_DIELECTRIC: rough = m.roughness; break;
-                case MPT_MAT_PRINCIPLED: alb = tex_value(S, m.texture, x), rough = m.roughness; break;
+                case MPT_MAT_PRINCIPLED: alb = tex_value(S, m.texture, tu, tv, x), rough = m.roughness; break;
                 case MPT_MAT_PHASE: alb = v3{m.albedo[0], m.albedo[1], m.albedo[2]}; break;
                 default: break;
             }
@@ -989,7 +1017,7 @@ __global__ void k_bsdf_eval(dscene S, const int32_t* mats, const double* q, cons
     const mpt_material& m = S.mat[rec.mat];
     rstream rs{0, 0};
     scatter_out so = mat_scatter(S, m, rd, rec, rs);
-    v3 e = mat_eval(S, m, rd, rec, wi);
+    v3 e = mat_eval(S, m, rd, rec, rec.normal, wi);
     eval[3 * i] = e.x, eval[3 * i + 1] = e.y, eval[3 * i + 2] = e.z;
     pdf[i] = (so.scattered && !so.skip) ? density_value(so.dens, wi) : 0.0;
 }
@@ -1214,6 +1242,10 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
     if ((rc = upload(s.get(), h.light_cdf, &d.lcdf))) return rc;
     if ((rc = upload(s.get(), h.mlight_prim, &d.mlight_prim))) return rc;
     if ((rc = upload(s.get(), h.prim_nrm, &d.vnrm))) return rc;
+    if ((rc = upload(s.get(), h.prim_uv, &d.vuv))) return rc;
+    if ((rc = upload(s.get(), h.texels, &d.texels))) return rc;
+    if ((rc = upload(s.get(), h.texels_hdr, &d.texels_hdr))) return rc;
+    if ((rc = upload(s.get(), h.srgb, &d.srgb))) return rc;
     if ((rc = upload(s.get(), h.mlight_cdf, &d.mlight_cdf))) return rc;
     if ((rc = upload(s.get(), h.env_rgb, &d.env_rgb))) return rc;
     if ((rc = upload(s.get(), h.env_cond_cdf, &d.env_cond))) return rc;
@@ -1242,7 +1274,8 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
         }
         for (int32_t k : h.prim_kind)
             if (k == MPT_PRIM_TRIANGLE) quads_ok = false;
-        s->profile = prims_only && !h.objects.empty() ? 1 : (quads_ok ? 2 : 0);
+        // 3: everything, and hits carry (u, v) and tangents for textures and normal maps
+        s->profile = h.uses_uv ? 3 : (prims_only && !h.objects.empty() ? 1 : (quads_ok ? 2 : 0));
         for (int32_t m : h.prim_mat)
             if (m >= 0) s->mat_types |= 1u << h.materials[m].type;
         for (const auto& r : h.objects)
@@ -1426,10 +1459,13 @@ int ensure_paths(mpt_scene* s, int64_t cap) {
     };
     int rc;
     double* base = nullptr;
-    if ((rc = alloc(size_t(cap) * D_COUNT * sizeof(double), reinterpret_cast<void**>(&base)))) return rc;
+    // the (u, v) record only where a material reads it
+    const int fields = s->hs.uses_uv ? D_COUNT : kUvRecord * kRecDoubles;
+    if ((rc = alloc(size_t(cap) * fields * sizeof(double), reinterpret_cast<void**>(&base)))) return rc;
     for (int f = 0; f < D_COUNT; f++)  // record r of slot i at base + (r * cap + i) * 8, or field f at base + f * cap
-        s->ps.d[f] = MPT_PATH_RECORDS ? base + (int64_t(f / kRecDoubles) * cap) * kRecDoubles + f % kRecDoubles
-                                      : base + int64_t(f) * cap;
+        s->ps.d[f] = f >= fields ? nullptr
+                     : MPT_PATH_RECORDS ? base + (int64_t(f / kRecDoubles) * cap) * kRecDoubles + f % kRecDoubles
+                                        : base + int64_t(f) * cap;
     if ((rc = alloc(size_t(cap) * sizeof(uint64_t), reinterpret_cast<void**>(&s->ps.key)))) return rc;
     HIP_TRY(hipMalloc(&s->queues, size_t(cap) * (3 + kClasses) * sizeof(uint32_t)));
     if (!s->counters) HIP_TRY(hipMalloc(&s->counters, kCtrWords * sizeof(uint32_t) + 16 * sizeof(uint64_t)));
@@ -1683,6 +1719,7 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
     } while (0)
         if (s->profile == 1) EXT(F_SPHERE);
         else if (s->profile == 2) EXT(kQuadProfile);
+        else if (s->profile == 3) EXT(F_ALL | F_UV);
         else EXT(F_ALL);
 #undef EXT
     };
@@ -1710,9 +1747,16 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
     auto launch_shade = [&](int cls, int depth, uint32_t* qout, uint32_t* cout) {
         const uint32_t* qc = qcls + cls * cap;
         const uint32_t* cc = ccls + cls * kCtrStride;
-#define SHA2(MT, LK, ENV)                                                                                      \
-    hipLaunchKernelGGL((k_shade<256, MT, LK, ENV>), dim3(G), dim3(256), 0, stream, DS, C, P, depth, qc, cc, qout,  \
-                       cout, q[2], ctr(cnt, 2))
+#define SHA3(MT, LK, ENV, UV)                                                                                    \
+    hipLaunchKernelGGL((k_shade<256, MT, LK, ENV, UV>), dim3(G), dim3(256), 0, stream, DS, C, P, depth, qc, cc,    \
+                       qout, cout, q[2], ctr(cnt, 2))
+    // the materials that read (u, v) get the variant that loads it in scenes with image textures or maps
+#define SHA2(MT, LK, ENV)                                                                                           \
+    do {                                                                                                            \
+        constexpr bool kMapped = MT == MPT_MAT_LAMBERTIAN || MT == MPT_MAT_METAL || MT == MPT_MAT_PRINCIPLED;       \
+        if (kMapped && s->profile == 3) SHA3(MT, LK, ENV, kMapped);                                                \
+        else SHA3(MT, LK, ENV, false);                                                                              \
+    } while (0)
 #define SHA1(MT, LK)                                          \
     do {                                                      \
         if (MT < 0 && has_env) SHA2(MT, LK, (MT < 0));        \
@@ -1739,6 +1783,7 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
 #undef SHA
 #undef SHA1
 #undef SHA2
+#undef SHA3
     };
 
     // the tail of a batch runs as one k_finish launch once at most `tail`
@@ -1760,6 +1805,7 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
         else FIN2(FEAT, 0);               \
     } while (0)
         if (s->profile == 1) FIN(F_SPHERE);
+        else if (s->profile == 3) FIN(F_ALL | F_UV);
         else FIN(F_ALL);
 #undef FIN
 #undef FIN2
@@ -1823,7 +1869,10 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
         R.salt = uint32_t(mix64(o.seed ^ (uint64_t(uint32_t(o.first_pass + pass)) << 32)));
         EV(3, hipLaunchKernelGGL(k_camera, dim3(unsigned((B.nslots + 255) / 256)), dim3(256), 0, stream, C, P, B, q[0],
                                  cnt, int(s->has_lights)));
-        if (C.features) EV(3, hipLaunchKernelGGL((k_features<256>), dim3(G), dim3(256), 0, stream, DS, L, P, B));
+        if (C.features) {
+            if (s->profile == 3) EV(3, hipLaunchKernelGGL((k_features<256, F_ALL | F_UV>), dim3(G), dim3(256), 0, stream, DS, L, P, B));
+            else EV(3, hipLaunchKernelGGL((k_features<256, F_ALL>), dim3(G), dim3(256), 0, stream, DS, L, P, B));
+        }
         if (!sorted) EV(4, launch_finish(0, q[0], ctr(cnt, 0)));  // every bounce in one kernel, no binning
         for (int depth = 0; sorted && depth < C.max_depth; depth++) {
             uint32_t* qin = q[depth & 1];

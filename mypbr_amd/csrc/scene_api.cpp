@@ -16,6 +16,7 @@
 #include <tuple>
 
 #include "geom.hpp"
+#include "image_decode.hpp"
 #include "scene_host.hpp"
 
 namespace mpt {
@@ -155,6 +156,59 @@ mpt_texture checker_texture::to_abi() const {
     return t;
 }
 
+namespace {
+bool read_hdr(const std::string& path, int& w, int& h, std::vector<color>& px);
+}
+
+// image_texture's constructor (texture.h:74-87): a Radiance file by its
+// signature (stbi_is_hdr), else PNG or JPEG.
+image_texture::image_texture(const std::string& path, bool rep, bool lin) : repeat(rep), linear(lin) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) {
+        error = "cannot open '" + path + "'";
+        return;
+    }
+    std::vector<uint8_t> bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    auto starts = [&](const char* sig) {
+        const size_t n = std::strlen(sig);
+        return bytes.size() >= n && std::memcmp(bytes.data(), sig, n) == 0;
+    };
+    if (starts("#?RADIANCE\n") || starts("#?RGBE\n")) {
+        std::vector<color> px;
+        int w = 0, h = 0;
+        if (!read_hdr(path, w, h, px)) {
+            error = "cannot read the Radiance file '" + path + "'";
+            return;
+        }
+        hdr.resize(px.size() * 3);
+        for (size_t i = 0; i < px.size(); i++)
+            for (int k = 0; k < 3; k++) hdr[3 * i + k] = float(px[i][k]);  // exact: the decoder's values are floats
+        width = w;
+        height = h;
+        channels = 3;
+        return;
+    }
+    ldr_image im;
+    if (!decode_ldr_image(bytes, im, error)) {
+        error = path + ": " + error;
+        return;
+    }
+    width = im.width;
+    height = im.height;
+    channels = im.channels;
+    ldr = std::move(im.rgba);
+}
+
+mpt_texture image_texture::to_abi() const {
+    mpt_texture t{};
+    t.type = MPT_TEX_IMAGE;
+    t.flags = (linear ? MPT_TEXF_LINEAR : 0) | (repeat ? MPT_TEXF_REPEAT : 0) | (hdr.empty() ? 0 : MPT_TEXF_HDR);
+    t.width = width;
+    t.height = height;
+    t.offset = 0;
+    return t;
+}
+
 static mpt_material blank(int type, int32_t ti) {
     mpt_material m;
     std::memset(&m, 0, sizeof(m));
@@ -162,6 +216,7 @@ static mpt_material blank(int type, int32_t ti) {
     m.texture = ti;
     m.spread = 180;
     m.tan_half = __builtin_huge_val();
+    m.normal_map = m.metallic_texture = m.roughness_texture = -1;
     return m;
 }
 
@@ -265,6 +320,9 @@ struct flat_builder::impl {
     std::vector<mpt_xform> xforms;
     std::vector<mpt_material> materials;
     std::vector<mpt_texture> textures;
+    std::vector<uint8_t> texels;     // image textures' RGBA bytes
+    std::vector<float> texels_hdr;   // image textures' RGB floats
+    std::vector<double> prim_uv;     // 6 per primitive once any triangle has texture coordinates
     std::vector<int32_t> light_object;
     std::vector<double> env_rgb;
     std::map<const material*, int32_t> mat_index;
@@ -278,23 +336,40 @@ struct flat_builder::impl {
 flat_builder::flat_builder() : p(new impl) {}
 flat_builder::~flat_builder() { delete p; }
 
+int32_t flat_builder::texture_index(const std::shared_ptr<texture>& t) {
+    if (!t) return -1;
+    auto jt = p->tex_index.find(t.get());
+    if (jt != p->tex_index.end()) return jt->second;
+    mpt_texture a = t->to_abi();
+    if (auto im = dynamic_cast<const image_texture*>(t.get())) {
+        if (!im->loaded()) throw std::runtime_error("image texture not loaded: " + im->error);
+        if (!im->hdr.empty()) {
+            a.offset = int64_t(p->texels_hdr.size() / 3);
+            p->texels_hdr.insert(p->texels_hdr.end(), im->hdr.begin(), im->hdr.end());
+        } else {
+            a.offset = int64_t(p->texels.size() / 4);
+            p->texels.insert(p->texels.end(), im->ldr.begin(), im->ldr.end());
+        }
+    }
+    const int32_t ti = int32_t(p->textures.size());
+    p->textures.push_back(a);
+    p->tex_index[t.get()] = ti;
+    return ti;
+}
+
 int32_t flat_builder::material_index(const std::shared_ptr<material>& m) {
     if (!m) throw std::runtime_error("object without a material");
     auto it = p->mat_index.find(m.get());
     if (it != p->mat_index.end()) return it->second;
-    int32_t ti = -1;
-    if (auto t = m->base_texture()) {
-        auto jt = p->tex_index.find(t.get());
-        if (jt == p->tex_index.end()) {
-            ti = int32_t(p->textures.size());
-            p->textures.push_back(t->to_abi());
-            p->tex_index[t.get()] = ti;
-        } else {
-            ti = jt->second;
-        }
-    }
+    auto image = [](const std::shared_ptr<texture>& t) { return dynamic_cast<const image_texture*>(t.get()) != nullptr; };
+    mpt_material a = m->to_abi(texture_index(m->base_texture()));
+    a.normal_map = texture_index(m->normal_map_texture());
+    a.metallic_texture = texture_index(m->metallic_texture());
+    a.roughness_texture = texture_index(m->roughness_texture());
+    a.uses_uv = (a.normal_map >= 0 || image(m->base_texture()) || image(m->metallic_texture()) ||
+                 image(m->roughness_texture())) ? 1 : 0;
     int32_t mi = int32_t(p->materials.size());
-    p->materials.push_back(m->to_abi(ti));
+    p->materials.push_back(a);
     p->mat_index[m.get()] = mi;
     return mi;
 }
@@ -312,7 +387,19 @@ int64_t flat_builder::add_prim(int32_t kind, int32_t mat, const double* data9) {
         p->prim_nrm.push_back(std::nan(""));
         p->prim_nrm.insert(p->prim_nrm.end(), 8, 0.0);
     }
+    if (!p->prim_uv.empty()) {
+        p->prim_uv.push_back(std::nan(""));
+        p->prim_uv.insert(p->prim_uv.end(), 5, 0.0);
+    }
     return int64_t(p->prim_kind.size() - 1);
+}
+
+void flat_builder::set_prim_uvs(int64_t prim, const double* uv6) {
+    if (p->prim_uv.empty()) {  // every primitive so far has none
+        p->prim_uv.assign(6 * p->prim_kind.size(), 0.0);
+        for (size_t k = 0; k < p->prim_kind.size(); k++) p->prim_uv[6 * k] = std::nan("");
+    }
+    std::copy(uv6, uv6 + 6, p->prim_uv.begin() + 6 * prim);
 }
 
 void flat_builder::set_prim_normals(int64_t prim, const double* n9) {
@@ -421,6 +508,21 @@ void mesh::flatten(flat_builder& b, int32_t xform) const {
                     for (int a = 0; a < 3; a++) n[3 * c + a] = normals[ni][a];
                 }
                 if (all) b.set_prim_normals(pi, n);
+            }
+            if (!uv_indices.empty()) {
+                if (uv_indices.size() != indices.size()) throw std::runtime_error("mesh uv indices must parallel its indices");
+                double t[6];
+                bool all = true;
+                for (int c = 0; c < 3; c++) {
+                    int32_t ti = uv_indices[k + c];
+                    if (ti < 0 || size_t(ti) >= uvs.size()) {
+                        all = false;
+                        break;
+                    }
+                    t[2 * c] = uvs[ti][0];
+                    t[2 * c + 1] = uvs[ti][1];
+                }
+                if (all) b.set_prim_uvs(pi, t);
             }
         }
         b.mesh_ranges[this] = {first, count};
@@ -538,6 +640,11 @@ const mpt_scene_desc& flat_builder::desc() {
     d.materials = p->materials.data();
     d.n_textures = int32_t(p->textures.size());
     d.textures = p->textures.data();
+    d.texels = p->texels.empty() ? nullptr : p->texels.data();
+    d.n_texel_bytes = int64_t(p->texels.size());
+    d.texels_hdr = p->texels_hdr.empty() ? nullptr : p->texels_hdr.data();
+    d.n_texels_hdr = int64_t(p->texels_hdr.size());
+    d.prim_uvs = p->prim_uv.empty() ? nullptr : p->prim_uv.data();
     d.n_lights = int32_t(p->light_object.size());
     d.light_object = p->light_object.data();
     d.env_rgb = p->env_rgb.empty() ? nullptr : p->env_rgb.data();
@@ -1370,24 +1477,75 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
                     return fail("expected 'texture <name> checker <scale> <r g b> <r g b>'");
                 tex = std::make_shared<checker_texture>(scale, std::make_shared<solid_color>(color(r0, g0, b0)),
                                                         std::make_shared<solid_color>(color(r1, g1, b1)));
+            } else if (type == "image") {  // scene.h:712-727
+                std::string file, option;
+                if (!(in >> file)) return fail("expected 'texture <name> image <path> [linear]'");
+                bool linear = false;
+                if (in >> option) {
+                    if (option != "linear") return fail("expected 'texture <name> image <path> [linear]'");
+                    linear = true;
+                }
+                if (file[0] != '/') file = scene_dir + file;
+                auto image = std::make_shared<image_texture>(file, false, linear);
+                if (!image->loaded()) return fail("cannot load image '" + file + "': " + image->error);
+                tex = image;
             } else {
-                return fail("texture type '" + type + "' is not modelled by the GPU path (solid and checker are)");
+                return fail("unknown texture type '" + type + "'");
             }
             textures[name] = tex;
         } else if (keyword == "material") {
             std::string name, type;
             if (!(in >> name >> type)) return fail("expected 'material <name> <type> <parameters...>'");
             std::shared_ptr<material> mat;
+            std::shared_ptr<texture> normal_map;  // lambertian and metal: an optional "normal_map <texture>" last
+            auto read_texture_name = [&](std::shared_ptr<texture>& tex, std::string& error) {
+                std::string tname;
+                if (!(in >> tname)) {
+                    error = "expected a texture name";
+                    return false;
+                }
+                auto it = textures.find(tname);
+                if (it == textures.end()) {
+                    error = "unknown texture '" + tname + "'";
+                    return false;
+                }
+                tex = it->second;
+                return true;
+            };
+            auto read_normal_map = [&](std::string& error) {  // scene.h:740-749
+                std::string key;
+                if (!(in >> key)) return true;
+                if (key != "normal_map") {
+                    error = "unexpected value '" + key + "' (expected 'normal_map <texture>')";
+                    return false;
+                }
+                return read_texture_name(normal_map, error);
+            };
+            // "<number>" in [0, 1] or "texture <name>" (scene.h:308-322)
+            auto read_value_or_texture = [&](double& value, std::shared_ptr<texture>& tex, std::string& error) {
+                std::string first;
+                if (!(in >> first)) return false;
+                if (first == "texture") return read_texture_name(tex, error);
+                std::istringstream number(first);
+                double x;
+                if (!(number >> x) || x < 0 || x > 1) return false;
+                value = x;
+                tex = nullptr;
+                return true;
+            };
             if (type == "lambertian") {
                 std::shared_ptr<texture> tex;
                 std::string first, error;
                 if (!(in >> first) || !read_color_or_texture(in, first, tex, error))
                     return fail(error.empty() ? "expected 'material <name> lambertian <r> <g> <b>'" : error);
+                if (!read_normal_map(error)) return fail(error);
                 mat = std::make_shared<lambertian>(tex);
             } else if (type == "metal") {
                 double r, g, b, roughness;
+                std::string error;
                 if (!(in >> r >> g >> b >> roughness) || roughness < 0)
                     return fail("expected 'material <name> metal <r> <g> <b> <roughness>' (measured metals are not modelled)");
+                if (!read_normal_map(error)) return fail(error);
                 mat = std::make_shared<metal>(color(r, g, b), roughness);
             } else if (type == "dielectric") {
                 const char* usage = "expected 'material <name> dielectric <ior> [roughness <r>] [color <r> <g> <b> [distance <d>]]'";
@@ -1409,6 +1567,7 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
                 mat = std::make_shared<dielectric>(ior, roughness, dielectric::absorption_for(tint, distance));
             } else if (type == "principled") {
                 std::shared_ptr<texture> base = std::make_shared<solid_color>(color(0.8, 0.8, 0.8));
+                std::shared_ptr<texture> metallic_map, roughness_map, base_normals;
                 double metallic = 0, roughness = 0.5, ior = 1.5;
                 std::string key, error;
                 while (in >> key) {
@@ -1417,13 +1576,18 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
                     else if (key == "color") {
                         std::string first;
                         ok = bool(in >> first) && read_color_or_texture(in, first, base, error);
-                    } else if (key == "metallic") ok = bool(in >> metallic) && metallic >= 0 && metallic <= 1;
-                    else if (key == "roughness") ok = bool(in >> roughness) && roughness >= 0 && roughness <= 1;
+                    } else if (key == "metallic") ok = read_value_or_texture(metallic, metallic_map, error);
+                    else if (key == "roughness") ok = read_value_or_texture(roughness, roughness_map, error);
                     else if (key == "ior") ok = bool(in >> ior) && ior > 0;
+                    else if (key == "normal_map") ok = read_texture_name(base_normals, error);
                     else return fail("principled setting '" + key + "' is not modelled by the GPU path");
                     if (!ok) return fail(error.empty() ? "bad value for principled setting '" + key + "'" : error);
                 }
-                mat = std::make_shared<principled>(base, metallic, roughness, ior);
+                auto pm = std::make_shared<principled>(base, metallic, roughness, ior);
+                pm->metallic_map = metallic_map;
+                pm->roughness_map = roughness_map;
+                if (use_normal_maps) pm->normal_map = base_normals;  // principled bends its own normals
+                mat = pm;
             } else if (type == "volume") {
                 double r, g, b, aniso = 0;
                 std::string key;
@@ -1443,6 +1607,7 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
             } else {
                 return fail("material type '" + type + "' is not modelled by the GPU path");
             }
+            if (normal_map && use_normal_maps) mat = std::make_shared<normal_mapped>(mat, normal_map);
             materials[name] = mat;
         } else if (keyword == "sphere" || keyword == "quad" || keyword == "box") {
             std::shared_ptr<hittable> object;

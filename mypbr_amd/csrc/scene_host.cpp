@@ -257,10 +257,31 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
 
     // materials and textures
     s.textures.assign(d.textures, d.textures + d.n_textures);
-    for (auto& t : s.textures)
-        if (t.type != MPT_TEX_SOLID && t.type != MPT_TEX_CHECKER)
+    if (d.n_texel_bytes < 0 || d.n_texels_hdr < 0 || (d.n_texel_bytes % 4) != 0 || (d.n_texels_hdr % 3) != 0 ||
+        (d.n_texel_bytes > 0 && !d.texels) || (d.n_texels_hdr > 0 && !d.texels_hdr))
+        return fail(MPT_E_INVALID, "bad texel arrays");
+    for (size_t k = 0; k < s.textures.size(); k++) {
+        const mpt_texture& t = s.textures[k];
+        if (t.type == MPT_TEX_IMAGE) {
+            const bool hdr = t.flags & MPT_TEXF_HDR;
+            const int64_t have = hdr ? d.n_texels_hdr / 3 : d.n_texel_bytes / 4;
+            if (t.width < 1 || t.height < 1 || t.offset < 0 || t.offset + int64_t(t.width) * t.height > have)
+                return fail(MPT_E_INVALID, "image texture " + std::to_string(k) + " lies outside the texel arrays");
+        } else if (t.type != MPT_TEX_SOLID && t.type != MPT_TEX_CHECKER) {
             return fail(MPT_E_UNSUPPORTED, "unsupported texture type " + std::to_string(t.type));
+        }
+    }
+    s.texels.assign(d.texels, d.texels + d.n_texel_bytes);
+    s.texels_hdr.assign(d.texels_hdr, d.texels_hdr + d.n_texels_hdr);
+    s.srgb.resize(256);
+    for (int k = 0; k < 256; k++) {  // texture.h:197-208
+        const double encoded = k / 255.0;
+        s.srgb[size_t(k)] = encoded <= 0.04045 ? encoded / 12.92 : std::pow((encoded + 0.055) / 1.055, 2.4);
+    }
     s.materials.assign(d.materials, d.materials + d.n_materials);
+    s.uses_uv = false;
+    auto tex_ok = [&](int32_t t) { return t >= -1 && t < d.n_textures; };
+    auto is_image = [&](int32_t t) { return t >= 0 && s.textures[size_t(t)].type == MPT_TEX_IMAGE; };
     for (size_t k = 0; k < s.materials.size(); k++) {
         auto& m = s.materials[k];
         if (m.type < MPT_MAT_LAMBERTIAN || m.type > MPT_MAT_PHASE)
@@ -268,6 +289,17 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         bool needs_tex = m.type == MPT_MAT_LAMBERTIAN || m.type == MPT_MAT_PRINCIPLED;
         if (needs_tex && (m.texture < 0 || m.texture >= d.n_textures))
             return fail(MPT_E_INVALID, "material " + std::to_string(k) + " has no valid texture");
+        if (!tex_ok(m.normal_map) || !tex_ok(m.metallic_texture) || !tex_ok(m.roughness_texture))
+            return fail(MPT_E_INVALID, "material " + std::to_string(k) + " names a texture that does not exist");
+        const bool mapped = m.type == MPT_MAT_LAMBERTIAN || m.type == MPT_MAT_METAL || m.type == MPT_MAT_PRINCIPLED;
+        if (m.normal_map >= 0 && !mapped)
+            return fail(MPT_E_UNSUPPORTED, "material " + std::to_string(k) + ": normal maps apply to lambertian, metal and principled");
+        if ((m.metallic_texture >= 0 || m.roughness_texture >= 0) && m.type != MPT_MAT_PRINCIPLED)
+            return fail(MPT_E_UNSUPPORTED, "material " + std::to_string(k) + ": metallic and roughness maps are principled's");
+        // recomputed here: whatever the caller set, hits of this material need (u, v) exactly when
+        m.uses_uv = (m.normal_map >= 0 || (needs_tex && is_image(m.texture)) || is_image(m.metallic_texture) ||
+                     is_image(m.roughness_texture)) ? 1 : 0;
+        if (m.uses_uv) s.uses_uv = true;
     }
     s.xforms.resize(d.n_xforms);
     for (int k = 0; k < d.n_xforms; k++) {
@@ -334,6 +366,17 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
             s.prim_nrm.assign(d.prim_normals, d.prim_normals + 9 * np);
             for (int64_t i = 0; i < np; i++)
                 if (s.prim_kind[i] != MPT_PRIM_TRIANGLE) s.prim_nrm[size_t(9 * i)] = std::nan("");
+        }
+    }
+    s.prim_uv.clear();
+    if (d.prim_uvs) {
+        bool any = false;
+        for (int64_t i = 0; i < np && !any; i++)
+            any = s.prim_kind[i] == MPT_PRIM_TRIANGLE && !std::isnan(d.prim_uvs[6 * i]);
+        if (any) {
+            s.prim_uv.assign(d.prim_uvs, d.prim_uvs + 6 * np);
+            for (int64_t i = 0; i < np; i++)
+                if (s.prim_kind[i] != MPT_PRIM_TRIANGLE) s.prim_uv[size_t(6 * i)] = std::nan("");
         }
     }
 
@@ -413,6 +456,13 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
                             std::copy(&s.prim_nrm[size_t(r.first + items[k].ref) * 9],
                                       &s.prim_nrm[size_t(r.first + items[k].ref) * 9] + 9, &tn[size_t(k) * 9]);
                         std::copy(tn.begin(), tn.end(), &s.prim_nrm[size_t(r.first) * 9]);
+                    }
+                    if (!s.prim_uv.empty()) {  // and so do texture coordinates
+                        std::vector<double> tu(size_t(r.count) * 6);
+                        for (int64_t k = 0; k < r.count; k++)
+                            std::copy(&s.prim_uv[size_t(r.first + items[k].ref) * 6],
+                                      &s.prim_uv[size_t(r.first + items[k].ref) * 6] + 6, &tu[size_t(k) * 6]);
+                        std::copy(tu.begin(), tu.end(), &s.prim_uv[size_t(r.first) * 6]);
                     }
                     for (int64_t k = 0; k < r.count; k++) {
                         std::copy(&tmp[size_t(k) * kPrimStride], &tmp[size_t(k) * kPrimStride] + kPrimStride,

@@ -80,6 +80,7 @@ struct host_scene {
     std::vector<int32_t> prim_mat;
     std::vector<int64_t> prim_canon;     // canonical local index within its object range
     std::vector<double> prim_nrm;        // triangles' vertex normals, 9 per primitive (NaN: none); empty if none
+    std::vector<double> prim_uv;         // triangles' texture coordinates, 6 per primitive (NaN: none); empty if none
     std::vector<obj_rec> objects;
     std::vector<bvh_node> tlas;
     std::vector<int32_t> tlas_items;     // object indices in leaf order
@@ -94,6 +95,10 @@ struct host_scene {
     std::vector<xform> xforms;
     std::vector<mpt_material> materials;
     std::vector<mpt_texture> textures;
+    std::vector<uint8_t> texels;         // image textures' RGBA bytes
+    std::vector<float> texels_hdr;       // image textures' RGB floats
+    std::vector<double> srgb;            // 256: linear value of each 8-bit sRGB code (texture.h:197-208)
+    bool uses_uv = false;                // some material reads (u, v) or a tangent frame at its hits
     std::vector<light_rec> lights;
     std::vector<double> light_cdf;       // n_lights + 1
     // mesh lights (mesh.h:175-216): per light mesh, its triangles in the

@@ -56,6 +56,7 @@ class DescBuilder:
         m.texture = tex
         m.spread = 180
         m.tan_half = math.inf
+        m.normal_map = m.metallic_texture = m.roughness_texture = -1
         return m
 
     def _add_mat(self, m):

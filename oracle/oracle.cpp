@@ -12,7 +12,8 @@
 //   sphere / quad / tri     sphere.h:23-132, quad.h:13-79, mesh.h:47-139
 //   instance, list          instance.h:29-56, hittable_list.h:25-62
 //   constant_medium         medium.h:60-88
-//   materials               material.h:74-264, 346-615, medium.h:13-39
+//   materials               material.h:74-264, 282-308, 346-615, 631-674, medium.h:13-39
+//   textures                texture.h:24-208 (solid, checker, image)
 //   microfacet / pdfs       microfacet.h:17-470, pdf.h:21-134
 //   light_list, environment light_list.h:45-127, environment.h:66-195
 //   camera                  camera.h:79-529
@@ -180,6 +181,8 @@ struct hit_record {
     double t = 0;
     bool front_face = false;
     int64_t id = -1;
+    double u = 0, v = 0;  // texture coordinates and dp/du, dp/dv (hittable.h:14-24)
+    vec3 dpdu, dpdv;
     void set_face_normal(const ray& r, const vec3& outward) {
         front_face = dot(r.dir, outward) < 0;
         normal = front_face ? outward : -outward;
@@ -479,29 +482,72 @@ struct rough_glass_pdf : pdf {
     vec3 generate(sampler& s) const override { return g.sample(s); }
 };
 
+// solid_color, checker_texture and image_texture (texture.h:24-205)
 struct texture_t {
     int type;
     double inv_scale;
     color even, odd;
-    color value(const point3& p) const {
+    int width = 0, height = 0, flags = 0;
+    const uint8_t* ldr = nullptr;       // RGBA bytes of this image
+    const float* hdr = nullptr;         // or its RGB floats
+    const double* srgb = nullptr;       // 256 linear values of the sRGB codes
+    color value(double u, double v, const point3& p) const {
         if (type == MPT_TEX_SOLID) return even;
-        int x = int(std::floor(inv_scale * p.x())), y = int(std::floor(inv_scale * p.y())),
-            z = int(std::floor(inv_scale * p.z()));
-        return (x + y + z) % 2 == 0 ? even : odd;
+        if (type == MPT_TEX_CHECKER) {
+            int x = int(std::floor(inv_scale * p.x())), y = int(std::floor(inv_scale * p.y())),
+                z = int(std::floor(inv_scale * p.z()));
+            return (x + y + z) % 2 == 0 ? even : odd;
+        }
+        size_t k = pixel_index(u, v);
+        if (hdr) return color(hdr[3 * k], hdr[3 * k + 1], hdr[3 * k + 2]);
+        if (flags & MPT_TEXF_LINEAR) return color(ldr[4 * k], ldr[4 * k + 1], ldr[4 * k + 2]) / 255.0;
+        return color(srgb[ldr[4 * k]], srgb[ldr[4 * k + 1]], srgb[ldr[4 * k + 2]]);
+    }
+    size_t pixel_index(double u, double v) const {  // texture.h:182-195
+        if (flags & MPT_TEXF_REPEAT) {
+            u -= std::floor(u);
+            v -= std::floor(v);
+        } else {
+            u = std::fmin(std::fmax(u, 0.0), 1.0);
+            v = std::fmin(std::fmax(v, 0.0), 1.0);
+        }
+        v = 1.0 - v;
+        int i = std::clamp(int(u * width), 0, width - 1);
+        int j = std::clamp(int(v * height), 0, height - 1);
+        return size_t(j) * width + i;
     }
 };
+
+// normal_from_map (material.h:282-308)
+vec3 normal_from_map(const ray& r, const hit_record& rec, const texture_t& map) {
+    const double min_cos_o = 0.01;
+    vec3 n = rec.front_face ? rec.normal : -rec.normal;
+    vec3 t = rec.dpdu - dot(rec.dpdu, n) * n;
+    if (t.length_squared() <= 1e-12 * rec.dpdu.length_squared()) return rec.normal;
+    t = unit_vector(t);
+    vec3 b = cross(n, t);
+    if (dot(b, rec.dpdv) < 0) b = -b;
+    color c = map.value(rec.u, rec.v, rec.p);
+    double nz = std::fmax(2 * c.z() - 1, 1e-3);
+    vec3 m = unit_vector((2 * c.x() - 1) * t + (2 * c.y() - 1) * b + nz * n);
+    if (!rec.front_face) m = -m;
+    vec3 wo = unit_vector(-r.dir);
+    double cos_o = dot(m, wo);
+    if (cos_o < min_cos_o) m = unit_vector(m + (min_cos_o - cos_o) * wo);
+    return m;
+}
 
 struct lambertian : material {  // material.h:74-101
     texture_t tex;
     bool scatter(const ray&, const hit_record& rec, scatter_record& s, sampler&) const override {
-        s.attenuation = tex.value(rec.p);
+        s.attenuation = tex.value(rec.u, rec.v, rec.p);
         s.pdf_ptr = std::make_unique<cosine_pdf>(rec.normal);
         s.skip_pdf = false;
         return true;
     }
     color eval(const ray&, const hit_record& rec, const vec3& d) const override {
         double c = dot(rec.normal, unit_vector(d));
-        return tex.value(rec.p) * (c < 0 ? 0 : c / pi);
+        return tex.value(rec.u, rec.v, rec.p) * (c < 0 ? 0 : c / pi);
     }
 };
 
@@ -583,37 +629,78 @@ struct dielectric : material {  // material.h:184-264
 struct principled : material {  // material.h:346-552 (base layers only)
     texture_t base;
     double metallic, alpha, f0;
-    color spec_f0(const color& b) const { return (1 - metallic) * color(f0, f0, f0) + metallic * b; }
+    // metallic and roughness maps (red channel) and the base's normal map, when present
+    bool has_metallic_map = false, has_roughness_map = false, has_normal_map = false;
+    texture_t metallic_map, roughness_map, normal_map;
+    double metallic_at(const hit_record& rec) const {  // material.h:508-515
+        return has_metallic_map ? dclamp(metallic_map.value(rec.u, rec.v, rec.p).x(), 0.0, 1.0) : metallic;
+    }
+    double alpha_at(const hit_record& rec) const {
+        return has_roughness_map ? roughness_to_alpha(dclamp(roughness_map.value(rec.u, rec.v, rec.p).x(), 0.0, 1.0))
+                                 : alpha;
+    }
+    vec3 base_normal(const ray& r, const hit_record& rec) const {
+        return has_normal_map ? normal_from_map(r, rec, normal_map) : rec.normal;
+    }
+    color spec_f0(const color& b, double met) const { return (1 - met) * color(f0, f0, f0) + met * b; }
     bool scatter(const ray& r, const hit_record& rec, scatter_record& s, sampler&) const override {
         vec3 wo = unit_vector(-r.dir);
-        color b = base.value(rec.p);
-        double cos_o = std::fmax(0.0, dot(rec.normal, wo));
-        double spec_w = average(fresnel_schlick(cos_o, spec_f0(b)));
-        double diff_w = (1 - metallic) * average(b) * (1 - fresnel_schlick(cos_o, f0));
-        diff_w += ms_albedo(*table, cos_o, alpha) * average(ms_fresnel(*table, spec_f0(b), alpha));
+        vec3 n = base_normal(r, rec);
+        double met = metallic_at(rec), a = alpha_at(rec);
+        color b = base.value(rec.u, rec.v, rec.p);
+        double cos_o = std::fmax(0.0, dot(n, wo));
+        double spec_w = average(fresnel_schlick(cos_o, spec_f0(b, met)));
+        double diff_w = (1 - met) * average(b) * (1 - fresnel_schlick(cos_o, f0));
+        diff_w += ms_albedo(*table, cos_o, a) * average(ms_fresnel(*table, spec_f0(b, met), a));
         double p_spec = spec_w + diff_w > 0 ? spec_w / (spec_w + diff_w) : 1;
         p_spec = dclamp(p_spec, 0.1, 1.0);
-        s.pdf_ptr = std::make_unique<mixture_pdf>(std::make_unique<ggx_reflection_pdf>(rec.normal, wo, alpha),
-                                                  std::make_unique<cosine_pdf>(rec.normal), p_spec);
+        s.pdf_ptr = std::make_unique<mixture_pdf>(std::make_unique<ggx_reflection_pdf>(n, wo, a),
+                                                  std::make_unique<cosine_pdf>(n), p_spec);
         s.attenuation = b;
         s.skip_pdf = false;
         return true;
     }
     color eval(const ray& r, const hit_record& rec, const vec3& d) const override {
-        if (dot(d, rec.normal) <= 0) return color(0, 0, 0);
+        if (dot(d, rec.normal) <= 0) return color(0, 0, 0);  // the unbent normal
         vec3 wo_w = unit_vector(-r.dir), wi_w = unit_vector(d);
-        onb f(rec.normal);
+        onb f(base_normal(r, rec));
         vec3 wo = unit_vector(f.to_local(wo_w)), wi = unit_vector(f.to_local(wi_w));
         if (wo.z() <= 0 || wi.z() <= 0) return color(0, 0, 0);
-        color b = base.value(rec.p);
+        double met = metallic_at(rec), a = alpha_at(rec);
+        color b = base.value(rec.u, rec.v, rec.p);
         vec3 h = unit_vector(wo + wi);
-        ggx dist(alpha);
-        color specular = fresnel_schlick(dot(wi, h), spec_f0(b)) * (dist.D(h) * dist.G(wo, wi) / (4 * wo.z())) +
-                         ms_fresnel(*table, spec_f0(b), alpha) * ms_eval(*table, wo.z(), wi.z(), alpha);
+        ggx dist(a);
+        color specular = fresnel_schlick(dot(wi, h), spec_f0(b, met)) * (dist.D(h) * dist.G(wo, wi) / (4 * wo.z())) +
+                         ms_fresnel(*table, spec_f0(b, met), a) * ms_eval(*table, wo.z(), wi.z(), a);
         double transmitted = (1 - fresnel_schlick(wi.z(), f0)) * (1 - fresnel_schlick(wo.z(), f0));
-        color diffuse = ((1 - metallic) * transmitted * wi.z() / pi) * b;
+        color diffuse = ((1 - met) * transmitted * wi.z() / pi) * b;
         return specular + diffuse;
     }
+};
+
+struct normal_mapped : material {  // material.h:631-674
+    std::unique_ptr<material> inner;
+    texture_t map;
+    hit_record perturbed(const ray& r, const hit_record& rec) const {
+        hit_record shaded = rec;
+        shaded.normal = normal_from_map(r, rec, map);
+        return shaded;
+    }
+    bool scatter(const ray& r, const hit_record& rec, scatter_record& s, sampler& smp) const override {
+        hit_record shaded = perturbed(r, rec);
+        if (!inner->scatter(r, shaded, s, smp)) return false;
+        if (s.skip_pdf) {
+            vec3 d = s.skip_pdf_ray.dir;
+            if (dot(d, shaded.normal) > 0 && dot(d, rec.normal) < 0)
+                s.skip_pdf_ray = ray(s.skip_pdf_ray.orig, d - 2 * dot(d, rec.normal) * rec.normal);
+        }
+        return true;
+    }
+    color eval(const ray& r, const hit_record& rec, const vec3& d) const override {
+        if (dot(d, rec.normal) <= 0) return color(0, 0, 0);
+        return inner->eval(r, perturbed(r, rec), d);
+    }
+    color emitted(const ray& r, const hit_record& rec) const override { return inner->emitted(r, rec); }
 };
 
 struct diffuse_light : material {  // material.h:568-615
@@ -667,7 +754,22 @@ struct sphere : hittable {  // sphere.h
         }
         rec.t = root;
         rec.p = r.at(root);
-        rec.set_face_normal(r, (rec.p - center) / radius);
+        vec3 outward = (rec.p - center) / radius;
+        rec.set_face_normal(r, outward);
+        // get_sphere_uv and get_sphere_tangents (sphere.h:46-48, 99-120)
+        vec3 un = outward * (radius < 0 ? -1.0 : 1.0);
+        double theta = std::acos(std::clamp(-un.y(), -1.0, 1.0));
+        double phi = std::atan2(-un.z(), un.x()) + pi;
+        rec.u = phi / (2 * pi);
+        rec.v = theta / pi;
+        double ar = std::fabs(radius);
+        double sin_theta = std::sqrt(un.x() * un.x() + un.z() * un.z());
+        if (sin_theta < 1e-8) {
+            rec.dpdu = rec.dpdv = vec3(0, 0, 0);
+        } else {
+            rec.dpdu = (2 * pi * ar) * vec3(un.z(), 0, -un.x());
+            rec.dpdv = (pi * ar / sin_theta) * vec3(-un.x() * un.y(), sin_theta * sin_theta, -un.y() * un.z());
+        }
         rec.mat = mat;
         rec.id = id;
         return true;
@@ -742,6 +844,10 @@ struct quad : hittable {  // quad.h
         if (!unit.contains(alpha) || !unit.contains(beta)) return false;
         rec.t = tt;
         rec.p = ip;
+        rec.u = alpha;  // quad.h:53-57
+        rec.v = beta;
+        rec.dpdu = u;
+        rec.dpdv = v;
         rec.mat = mat;
         rec.id = id;
         rec.set_face_normal(r, normal);
@@ -763,13 +869,15 @@ struct quad : hittable {  // quad.h
     double power() const override { return pi * area * luminance(mat->average_emission()); }
 };
 
-struct triangle : hittable {  // mesh.h:28-139 (vertex normals; no uvs or alpha)
+struct triangle : hittable {  // mesh.h:28-166 (vertex normals and uvs; no alpha masks)
     point3 p0, p1, p2;
     const material* mat;
     int64_t id;
     double area;
     bool has_normals = false;
     vec3 n0, n1, n2;
+    bool has_uvs = false;
+    vec3 t0, t1, t2;  // (u, v, 0) at each vertex
     triangle(const point3& a, const point3& b, const point3& c, const material* m, int64_t id)
         : p0(a), p1(b), p2(c), mat(m), id(id) {
         area = 0.5 * cross(p1 - p0, p2 - p0).length();
@@ -792,6 +900,28 @@ struct triangle : hittable {  // mesh.h:28-139 (vertex normals; no uvs or alpha)
         rec.p = r.at(tt);
         rec.mat = mat;
         rec.id = id;
+        double b0 = 1 - b1 - b2;
+        rec.u = b1;
+        rec.v = b2;
+        if (has_uvs) {  // mesh.h:75-81
+            vec3 uv = b0 * t0 + b1 * t1 + b2 * t2;
+            rec.u = uv.x();
+            rec.v = uv.y();
+        }
+        if (!has_uvs) {  // set_tangents, mesh.h:149-166
+            rec.dpdu = e1;
+            rec.dpdv = e2;
+        } else {
+            vec3 duv1 = t1 - t0, duv2 = t2 - t0;
+            double det = duv1.x() * duv2.y() - duv2.x() * duv1.y();
+            if (std::fabs(det) < 1e-12) {
+                rec.dpdu = rec.dpdv = vec3(0, 0, 0);
+            } else {
+                double inv = 1 / det;
+                rec.dpdu = (duv2.y() * e1 - duv1.y() * e2) * inv;
+                rec.dpdv = (duv1.x() * e2 - duv2.x() * e1) * inv;
+            }
+        }
         rec.set_face_normal(r, unit_vector(cross(e1, e2)));
         if (has_normals) {  // mesh.h:93-101
             double b0 = 1 - b1 - b2;
@@ -1026,6 +1156,8 @@ struct instance : hittable {  // instance.h
         if (!object->hit(local, t, rec, c)) return false;
         rec.p = x.point(rec.p);
         rec.normal = unit_vector(x.normal(rec.normal));
+        rec.dpdu = x.vector(rec.dpdu);  // instance.h:38-39
+        rec.dpdv = x.vector(rec.dpdv);
         return true;
     }
     aabb bounding_box() const override { return box; }
@@ -1206,6 +1338,9 @@ struct scene {
     std::unique_ptr<environment> env;
     ggx_table table;
     aabb world_box;
+    std::vector<uint8_t> texels;    // image textures, copied from the description
+    std::vector<float> texels_hdr;
+    std::vector<double> srgb;       // image_texture::srgb_table (texture.h:197-208)
 
     double lights_pdf(const point3& o, const vec3& d) const {
         if (lights.empty()) return 0.0;
@@ -1235,6 +1370,13 @@ T* own(scene& s, std::unique_ptr<T> p) {
 
 int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
     build_table(s.table);
+    if (d.n_texel_bytes > 0) s.texels.assign(d.texels, d.texels + d.n_texel_bytes);
+    if (d.n_texels_hdr > 0) s.texels_hdr.assign(d.texels_hdr, d.texels_hdr + d.n_texels_hdr);
+    s.srgb.resize(256);
+    for (int k = 0; k < 256; k++) {
+        double encoded = k / 255.0;
+        s.srgb[k] = encoded <= 0.04045 ? encoded / 12.92 : std::pow((encoded + 0.055) / 1.055, 2.4);
+    }
     for (int k = 0; k < d.n_materials; k++) {
         const mpt_material& m = d.materials[k];
         auto tex = [&](int ti) {
@@ -1244,6 +1386,14 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
             t.inv_scale = x.inv_scale;
             t.even = color(x.even[0], x.even[1], x.even[2]);
             t.odd = color(x.odd[0], x.odd[1], x.odd[2]);
+            if (x.type == MPT_TEX_IMAGE) {
+                t.width = x.width;
+                t.height = x.height;
+                t.flags = x.flags;
+                if (x.flags & MPT_TEXF_HDR) t.hdr = s.texels_hdr.data() + 3 * x.offset;
+                else t.ldr = s.texels.data() + 4 * x.offset;
+                t.srgb = s.srgb.data();
+            }
             return t;
         };
         std::unique_ptr<material> mp;
@@ -1277,6 +1427,9 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
                 p->base = tex(m.texture);
                 p->metallic = dclamp(m.metallic, 0.0, 1.0);
                 p->alpha = roughness_to_alpha(dclamp(m.roughness, 0.0, 1.0));
+                if ((p->has_metallic_map = m.metallic_texture >= 0)) p->metallic_map = tex(m.metallic_texture);
+                if ((p->has_roughness_map = m.roughness_texture >= 0)) p->roughness_map = tex(m.roughness_texture);
+                if ((p->has_normal_map = m.normal_map >= 0)) p->normal_map = tex(m.normal_map);
                 double r0 = (m.ior - 1) / (m.ior + 1);
                 p->f0 = r0 * r0;
                 mp = std::move(p);
@@ -1309,6 +1462,13 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
                 return MPT_E_UNSUPPORTED;
         }
         mp->table = &s.table;
+        if (m.normal_map >= 0 && m.type != MPT_MAT_PRINCIPLED) {  // the scene file's normal_mapped wrapper
+            auto w = std::make_unique<normal_mapped>();
+            w->inner = std::move(mp);
+            w->map = tex(m.normal_map);
+            w->table = &s.table;
+            mp = std::move(w);
+        }
         s.mats.push_back(std::move(mp));
     }
     auto matp = [&](int i) -> const material* { return i >= 0 ? s.mats[i].get() : nullptr; };
@@ -1336,6 +1496,13 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
                     t->n0 = vec3(vn[0], vn[1], vn[2]);
                     t->n1 = vec3(vn[3], vn[4], vn[5]);
                     t->n2 = vec3(vn[6], vn[7], vn[8]);
+                }
+                const double* vt = d.prim_uvs ? d.prim_uvs + 6 * i : nullptr;
+                if (vt && !std::isnan(vt[0])) {
+                    t->has_uvs = true;
+                    t->t0 = vec3(vt[0], vt[1], 0);
+                    t->t1 = vec3(vt[2], vt[3], 0);
+                    t->t2 = vec3(vt[4], vt[5], 0);
                 }
                 return own(s, std::move(t));
             }
@@ -2001,6 +2168,40 @@ int32_t orc_trace(orc_scene* S, const double* rays, const double* tt, int64_t n,
     for (int k = 1; k < nt; k++) pool.emplace_back(work, std::min(n, k * chunk), std::min(n, (k + 1) * chunk));
     work(0, std::min(n, chunk));
     for (auto& th : pool) th.join();
+    return MPT_OK;
+}
+
+// What a hit hands its material, per ray (16 doubles, NaN on a miss): u, v,
+// dp/du, dp/dv, the base texture's value (lambertian, principled), the
+// shading normal after a normal map, principled's metallic and alpha.
+int32_t orc_surface(orc_scene* S, const double* rays, const double* tt, int64_t n, double* out) {
+    for (int64_t i = 0; i < n; i++) {
+        double* o = out + 16 * i;
+        for (int k = 0; k < 16; k++) o[k] = std::nan("");
+        ray r(point3(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]), vec3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]));
+        hit_record rec;
+        hit_ctx hc{0, 0, false};
+        if (!S->s.tree->hit(r, interval(tt[2 * i], tt[2 * i + 1]), rec, hc)) continue;
+        const material* m = rec.mat;
+        vec3 shading = rec.normal;
+        if (auto nm = dynamic_cast<const normal_mapped*>(m)) {
+            shading = nm->perturbed(r, rec).normal;
+            m = nm->inner.get();
+        }
+        color base(0, 0, 0);
+        double metallic = 0, alpha = 0;
+        if (auto l = dynamic_cast<const lambertian*>(m)) {
+            base = l->tex.value(rec.u, rec.v, rec.p);
+        } else if (auto p = dynamic_cast<const principled*>(m)) {
+            base = p->base.value(rec.u, rec.v, rec.p);
+            metallic = p->metallic_at(rec);
+            alpha = p->alpha_at(rec);
+            shading = p->base_normal(r, rec);
+        }
+        const double v[16] = {rec.u, rec.v, rec.dpdu.x(), rec.dpdu.y(), rec.dpdu.z(), rec.dpdv.x(), rec.dpdv.y(), rec.dpdv.z(),
+                              base.x(), base.y(), base.z(), shading.x(), shading.y(), shading.z(), metallic, alpha};
+        std::memcpy(o, v, sizeof(v));
+    }
     return MPT_OK;
 }
 

@@ -296,4 +296,58 @@ int ref_load_image_f(const char* path, int* w, int* h, float* out, int64_t cap) 
     return 0;
 }
 
+// What a hit hands its material, per ray (16 doubles): rec.u, rec.v,
+// rec.dpdu, rec.dpdv (world space), the base texture's value there
+// (lambertian's albedo, principled's base color; 0 otherwise), the normal
+// the material shades with (normal_from_map's through normal_mapped or
+// principled's own map; rec.normal without one), principled's metallic_at
+// and alpha_at (0 otherwise). A miss leaves its row NaN.
+int ref_surface(void* h, const double* rays, const double* tt, int64_t n, double* out) {
+    auto* r = static_cast<ref_scene*>(h);
+    for (int64_t i = 0; i < n; i++) {
+        double* o = out + 16 * i;
+        for (int k = 0; k < 16; k++) o[k] = std::nan("");
+        ray rr(point3(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]), vec3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]));
+        hit_record rec;
+        if (!r->tree->hit(rr, interval(tt[2 * i], tt[2 * i + 1]), rec)) continue;
+        const material* m = rec.mat;
+        vec3 shading = rec.normal;
+        if (auto nm = dynamic_cast<const normal_mapped*>(m)) {
+            shading = nm->perturbed(rr, rec).normal;
+            m = nm->inner.get();
+        }
+        color base(0, 0, 0);
+        double metallic = 0, alpha = 0;
+        if (auto l = dynamic_cast<const lambertian*>(m)) {
+            base = l->tex->value(rec.u, rec.v, rec.p);
+        } else if (auto p = dynamic_cast<const principled*>(m)) {
+            base = p->base_color->value(rec.u, rec.v, rec.p);
+            metallic = p->metallic_at(rec);
+            alpha = p->alpha_at(rec);
+            if (p->normal_map) shading = normal_from_map(rr, rec, *p->normal_map);
+        }
+        const double v[16] = {rec.u, rec.v, rec.dpdu.x(), rec.dpdu.y(), rec.dpdu.z(), rec.dpdv.x(), rec.dpdv.y(), rec.dpdv.z(),
+                              base.x(), base.y(), base.z(), shading.x(), shading.y(), shading.z(), metallic, alpha};
+        std::memcpy(o, v, sizeof(v));
+    }
+    return 0;
+}
+
+// stbi_load(path, ..., 4): what image_texture keeps of an 8-bit file
+// (texture.h:81-84), and the file's own channel count.
+int ref_load_image_u8(const char* path, int* w, int* h, int* channels, uint8_t* out, int64_t cap) {
+    unsigned char* data = stbi_load(path, w, h, channels, 4);
+    if (!data) return -1;
+    const int64_t n = int64_t(*w) * *h * 4;
+    for (int64_t k = 0; k < n && k < cap; k++) out[k] = data[k];
+    stbi_image_free(data);
+    return 0;
+}
+
+// stbi_write_jpg: baseline JPEG files for the decoder's tests (quality above
+// 90 keeps full-resolution chroma, at or below it chroma is subsampled 2x2).
+int ref_write_jpg(const char* path, int w, int h, int comp, const uint8_t* data, int quality) {
+    return stbi_write_jpg(path, w, h, comp, data, quality) ? 0 : -1;
+}
+
 }  // extern "C"

@@ -13,7 +13,16 @@ blob.obj             a low-poly sphere with no normals and a scene material
                      `mesh ... smooth` and instancing;
 sky_rle.hdr          a 256 x 128 sun-and-sky environment in Radiance RGBE with
                      new-style run-length encoded scanlines (runs and
-                     literals), written here from the published format.
+                     literals), written here from the published format;
+bumps_normal.png     a 64 x 64 tangent-space normal map of round bumps (RGB,
+                     8-bit), its rows written with all five PNG filters in turn;
+tiles_rough.png      a 48 x 48 roughness map of tiles, 16-bit grey;
+tiles_metal.png      a 48 x 48 metallic map, a 2-bit palette image;
+globe.jpg            a 128 x 64 continents-and-ocean color map, baseline JPEG
+                     with 2x2-subsampled chroma. JPEG needs an encoder: this
+                     one is written by stb_image_write through the compiled
+                     reference harness (oracle/_ref), so it is only rewritten
+                     where that is built.
 """
 import math
 import os
@@ -164,8 +173,121 @@ def sky_hdr(w=256, h=128):
     open(os.path.join(HERE, "sky_rle.hdr"), "wb").write(bytes(data))
 
 
+def png_write(path, rows, color_type, depth, palette=None):
+    """PNG from rows of samples (lists of ints), filters 0-4 in turn by row
+    (PNG spec, section 9: each filter's difference from the raw bytes)."""
+    import struct
+    import zlib
+    samples = {0: 1, 2: 3, 3: 1}[color_type]
+
+    def raw(row):  # samples to bytes, big-endian 16-bit, or packed low bit depths
+        if depth == 16:
+            return b"".join(struct.pack(">H", v) for v in row)
+        if depth == 8:
+            return bytes(row)
+        per = 8 // depth
+        out = bytearray()
+        for i in range(0, len(row), per):
+            b = 0
+            for k in range(per):
+                v = row[i + k] if i + k < len(row) else 0
+                b |= v << (8 - depth * (k + 1))
+            out.append(b)
+        return bytes(out)
+
+    bpp = max(1, samples * depth // 8)
+    prev = None
+    body = bytearray()
+    for y, row in enumerate(rows):
+        cur = raw(row)
+        up = prev if prev is not None else bytes(len(cur))
+        f = y % 5
+        enc = bytearray()
+        for i, x in enumerate(cur):
+            a = cur[i - bpp] if i >= bpp else 0
+            b = up[i]
+            c = up[i - bpp] if i >= bpp else 0
+            if f == 0:
+                pred = 0
+            elif f == 1:
+                pred = a
+            elif f == 2:
+                pred = b
+            elif f == 3:
+                pred = (a + b) >> 1
+            else:
+                pp = a + b - c
+                pa, pb, pc = abs(pp - a), abs(pp - b), abs(pp - c)
+                pred = a if pa <= pb and pa <= pc else (b if pb <= pc else c)
+            enc.append((x - pred) & 255)
+        body += bytes([f]) + enc
+        prev = cur
+
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
+
+    w = len(rows[0]) // samples
+    out = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, len(rows), depth, color_type, 0, 0, 0))
+    if palette is not None:
+        out += chunk(b"PLTE", bytes(v for rgb in palette for v in rgb))
+    out += chunk(b"IDAT", zlib.compress(bytes(body), 9)) + chunk(b"IEND", b"")
+    open(path, "wb").write(out)
+
+
+def texture_maps():
+    n = 64
+    rows = []
+    for j in range(n):
+        row = []
+        for i in range(n):
+            # bumps of radius 0.4 on an 8 x 8 cell grid; the normal as a color
+            cx, cy = (i % 8 + 0.5) / 8 - 0.5, (j % 8 + 0.5) / 8 - 0.5
+            r2 = (cx * cx + cy * cy) / 0.16
+            nx, ny = (cx / 0.4, -cy / 0.4) if r2 < 1 else (0.0, 0.0)
+            nz = math.sqrt(max(0.0, 1 - nx * nx - ny * ny))
+            row += [int(round(255 * (0.5 + 0.5 * v))) for v in (nx, ny, nz)]
+        rows.append(row)
+    png_write(os.path.join(HERE, "bumps_normal.png"), rows, 2, 8)
+    m = 48
+    rough = [[(12000 if ((i // 12) + (j // 12)) % 2 else 52000) + 37 * ((i * 7 + j * 3) % 11) for i in range(m)]
+             for j in range(m)]
+    png_write(os.path.join(HERE, "tiles_rough.png"), rough, 0, 16)
+    metal = [[(0 if (i // 6) % 2 else 3) if (j // 6) % 2 else (1 if (i // 6) % 2 else 2) for i in range(m)]
+             for j in range(m)]
+    png_write(os.path.join(HERE, "tiles_metal.png"), metal, 3, 2, palette=[(0, 0, 0), (80, 80, 80), (180, 0, 0), (255, 255, 255)])
+
+
+def globe_jpg(w=128, h=64):
+    import ctypes as C
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lib = os.path.join(root, "oracle", "_ref", "libmpt_ref.so")
+    if not os.path.exists(lib):
+        print("oracle/_ref is not built: globe.jpg kept as it is")
+        return
+    import numpy as np
+    img = np.zeros((h, w, 3), np.uint8)
+    for j in range(h):
+        for i in range(w):
+            lon, lat = 2 * math.pi * i / w, math.pi * (j / h - 0.5)
+            land = math.sin(3 * lon) * math.cos(2 * lat) + 0.5 * math.sin(5 * lon + 1.3) * math.sin(3 * lat + 0.4)
+            if abs(lat) > 1.25:
+                img[j, i] = (235, 240, 245)
+            elif land > 0.35:
+                img[j, i] = (60 + int(80 * (land - 0.35)), 120, 40)
+            else:
+                img[j, i] = (20, 50 + int(30 * (1 + land)), 140)
+    R = C.CDLL(lib)
+    R.ref_write_jpg.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8), C.c_int]
+    assert R.ref_write_jpg(os.fsencode(os.path.join(HERE, "globe.jpg")), w, h, 3,
+                           img.ctypes.data_as(C.POINTER(C.c_uint8)), 85) == 0
+    del sys
+
+
 if __name__ == "__main__":
     os.makedirs(HERE, exist_ok=True)
     room()
     blob()
     sky_hdr()
+    texture_maps()
+    globe_jpg()

@@ -17,6 +17,9 @@ queries) are seeded and regenerated identically by the tests:
                         their per-pixel standard error over 8 runs
   denoise.npz           denoiser::run (denoiser.h:45-193) on the synthetic frames
                         of tests/denoise_frames.py, for two settings
+  textured.npz          scenes/textured.scene (image textures, normal maps, texture
+                        maps) as below, plus the reference's surface record at each
+                        hit of the subset and stbi_load's bytes of its texture files
   obj_room.npz,         scenes/obj_room.scene (Wavefront OBJ/MTL models) and
   hdr_env.npz           scenes/hdr_env.scene (a run-length encoded .hdr environment)
                         loaded by the reference's own loaders: a 4096-ray subset of
@@ -26,7 +29,7 @@ queries) are seeded and regenerated identically by the tests:
                         writes (display_settings::apply + to_byte, tonemap.h,
                         color3.h) on the images and settings of tests/display_cases.py
 
-Usage: python tests/golden/make_golden.py [step ...]  (steps: ggx trace bsdf render denoise display obj; all by default)
+Usage: python tests/golden/make_golden.py [step ...]  (steps: ggx trace bsdf render denoise display obj textures; all by default)
 (all steps take about a minute)
 """
 import hashlib
@@ -134,11 +137,34 @@ ASSET_SCENES = {  # scene: (width, spp per run, runs, max_depth) of the converge
     "obj_room.scene": (40, 256, 8, 16),
     "hdr_env.scene": (48, 256, 8, 16),
 }
+TEXTURE_SCENES = {
+    "textured.scene": (48, 256, 8, 12),
+}
 
 
 def golden_obj():
     for name, render in ASSET_SCENES.items():
         golden_asset_scene(name, render)
+
+
+def golden_textures():
+    """textured.scene as golden_asset_scene writes it, plus what each hit of
+    the 4096-ray subset hands its material (the reference's rec.u, rec.v,
+    dp/du, dp/dv, base texture value, shading normal after normal maps,
+    principled metallic and alpha), and stbi_load's bytes of every texture
+    file the scene names."""
+    for name, render in TEXTURE_SCENES.items():
+        golden_asset_scene(name, render)
+        path = os.path.join(HERE, name.replace(".scene", ".npz"))
+        g = dict(np.load(path))
+        r = ol.RefScene(scene_path(name))
+        g["surface"] = r.surface(g["rays"], g["tt"])
+        for f in ("globe.jpg", "bumps_normal.png", "tiles_rough.png", "tiles_metal.png"):
+            px, ch = ol.reference_image_u8(os.path.join(ROOT, "scenes", "models", f))
+            g["image/" + f] = px
+            g["channels/" + f] = np.array(ch)
+        np.savez_compressed(path, **g)
+        print(name, "surface and images", flush=True)
 
 
 def golden_asset_scene(name, render):
@@ -178,14 +204,16 @@ def main():
     if not ol.reference_available():
         sys.exit("oracle/_ref/libmpt_ref.so is not built (make -C oracle ref, where the reference is present)")
     R = ol.reference()
-    steps = set(sys.argv[1:]) or {"ggx", "trace", "bsdf", "render", "denoise", "display", "obj"}
+    steps = set(sys.argv[1:]) or {"ggx", "trace", "bsdf", "render", "denoise", "display", "obj", "textures"}
     if "denoise" in steps:
         golden_denoise()
     if "display" in steps:
         golden_display()
     if "obj" in steps:
         golden_obj()
-    if steps <= {"denoise", "display", "obj"}:
+    if "textures" in steps:
+        golden_textures()
+    if steps <= {"denoise", "display", "obj", "textures"}:
         return
 
     e = np.zeros(1024)

@@ -48,6 +48,7 @@ def oracle():
             "orc_scene_destroy": (None, [vp]),
             "orc_trace": (C.c_int32, [vp, dp, dp, C.c_int64, P(C.c_int64), dp, P(C.c_uint8), C.c_int32]),
             "orc_trace_linear": (C.c_int32, [vp, dp, dp, C.c_int64, P(C.c_int64), dp]),
+            "orc_surface": (C.c_int32, [vp, dp, dp, C.c_int64, dp]),
             "orc_light_probabilities": (C.c_int32, [vp, dp]),
             "orc_ggx_table": (C.c_int32, [dp, dp]),
             "orc_bsdf_eval": (C.c_int32, [vp, P(C.c_int32), dp, P(C.c_uint8), C.c_int64, dp, dp]),
@@ -92,6 +93,13 @@ class OracleScene:
         f = np.zeros(n, dtype=np.uint8)
         self.lib.orc_trace(self.h, _d(rays), _d(tt), n, _i64(ids), _d(t), _u8(f), threads)
         return ids, t, f
+
+    def surface(self, rays, tt):
+        """Per ray: u, v, dp/du, dp/dv, base texture value, shading normal,
+        metallic, alpha (16 doubles; NaN on a miss)."""
+        out = np.zeros((rays.shape[0], 16))
+        self.lib.orc_surface(self.h, _d(rays), _d(tt), rays.shape[0], _d(out))
+        return out
 
     def trace_linear(self, rays, tt):
         n = rays.shape[0]
@@ -168,6 +176,9 @@ def reference():
             "ref_load": (C.c_int, [C.c_char_p, P(vp)]),
             "ref_free": (None, [vp]),
             "ref_canon_count": (C.c_int64, [vp]),
+            "ref_surface": (C.c_int, [vp, dp, dp, C.c_int64, dp]),
+            "ref_load_image_u8": (C.c_int, [C.c_char_p, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_uint8), C.c_int64]),
+            "ref_write_jpg": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, P(C.c_uint8), C.c_int]),
             "ref_trace": (C.c_int, [vp, dp, dp, C.c_int64, P(C.c_int64), dp, P(C.c_uint8), P(C.c_int32),
                                     P(C.c_int64), P(C.c_int64), C.c_int]),
             "ref_light_count": (C.c_int, [vp]),
@@ -205,6 +216,11 @@ class RefScene:
 
     def __del__(self):
         self.close()
+
+    def surface(self, rays, tt):
+        out = np.zeros((rays.shape[0], 16))
+        self.lib.ref_surface(self.h, _d(rays), _d(tt), rays.shape[0], _d(out))
+        return out
 
     def trace(self, rays, tt, threads=8):
         n = rays.shape[0]
@@ -310,3 +326,24 @@ def reference_scene_display(path):
     if reference().ref_scene_display(os.fsencode(path), _d(out)) != 0:
         return None
     return out
+
+
+def reference_image_u8(path):
+    """stb_image's stbi_load(path, ..., 4) through the compiled reference:
+    RGBA bytes (h, w, 4) and the file's channel count."""
+    R = reference()
+    w, h, c = C.c_int(), C.c_int(), C.c_int()
+    buf = np.zeros(4096 * 4096 * 4, np.uint8)
+    if R.ref_load_image_u8(os.fsencode(path), C.byref(w), C.byref(h), C.byref(c), buf.ctypes.data_as(P(C.c_uint8)),
+                           buf.size) != 0:
+        raise RuntimeError("stbi_load failed for " + path)
+    return buf[: w.value * h.value * 4].reshape(h.value, w.value, 4).copy(), c.value
+
+
+def reference_write_jpg(path, img, quality):
+    """stbi_write_jpg of an (h, w, comp) uint8 image through the compiled reference."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    h, w = img.shape[:2]
+    comp = 1 if img.ndim == 2 else img.shape[2]
+    if reference().ref_write_jpg(os.fsencode(path), w, h, comp, img.ctypes.data_as(P(C.c_uint8)), quality) != 0:
+        raise RuntimeError("stbi_write_jpg failed")

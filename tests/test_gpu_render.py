@@ -67,6 +67,7 @@ def test_render_features_match_oracle(mpt):
     ("test_lights.scene", 48, 4, 12),
     ("test_media.scene", 48, 4, 12),
     ("obj_room.scene", 40, 4, 16),
+    ("textured.scene", 48, 4, 12),
 ])
 def test_batch_tail_in_one_launch_is_identical(mpt, name, width, spp, depth):
     """k_finish runs a path's remaining bounces in one thread; the frame must
@@ -92,6 +93,7 @@ def test_batch_tail_in_one_launch_is_identical(mpt, name, width, spp, depth):
     ("test_lights.scene", 48, 4, 12),
     ("test_media.scene", 48, 4, 12),
     ("obj_room.scene", 40, 4, 16),
+    ("textured.scene", 48, 4, 12),
 ])
 def test_grid_binning_and_graphs_leave_the_frame_unchanged(mpt, name, width, spp, depth):
     """The knobs of mpt_render_opts that only schedule the work: the persistent
