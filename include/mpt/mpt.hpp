@@ -125,6 +125,10 @@ class image_texture : public texture {
 public:
     explicit image_texture(const std::string& path, bool repeat = false, bool linear = false);
     bool loaded() const { return width > 0 && height > 0; }
+    // a grey (1- or 2-channel) file: a height map rather than a normal map (texture.h:108-110)
+    bool greyscale() const { return hdr.empty() && channels <= 2; }
+    // every texel's coverage (alpha, or the grey level without alpha) is 1 (texture.h:101-105, 166-173)
+    bool fully_opaque() const;
     mpt_texture to_abi() const override;  // offset 0: the builder places the texels
     int width = 0, height = 0;
     int channels = 0;           // of the file (1 grey, 2 grey + alpha, 3, 4)
@@ -298,10 +302,10 @@ struct obj_part {
 
 // load_obj (obj_loader.h:170-340): v, vn, vt, f (fans; v, v/t, v//n, v/t/n;
 // negative indices), usemtl, mtllib with load_mtl's mapping (Kd, Ks/Ns/Ni,
-// Ke). Image textures, alpha masks and normal maps make it fail: the GPU
-// path does not model them (a texture file that does not exist falls back to
-// the plain color with a warning, as in the reference). Parts come in the
-// order of each material's first face.
+// Ke; map_Kd and normal maps as repeating image textures; a texture that
+// cannot be loaded falls back to the plain color with a warning, as in the
+// reference). Alpha masks make it fail: the GPU path does not model them.
+// Parts come in the order of each material's first face.
 bool load_obj(const std::string& path, const std::map<std::string, std::shared_ptr<material>>& materials,
               std::shared_ptr<material> default_mat, bool smooth, std::vector<obj_part>& parts, std::string& error,
               bool use_normal_maps = true);

@@ -199,6 +199,14 @@ image_texture::image_texture(const std::string& path, bool rep, bool lin) : repe
     ldr = std::move(im.rgba);
 }
 
+bool image_texture::fully_opaque() const {
+    if (!hdr.empty()) return true;
+    const size_t mask = (channels == 2 || channels == 4) ? 3 : 0;
+    for (size_t k = mask; k < ldr.size(); k += 4)
+        if (ldr[k] < 255) return false;
+    return true;
+}
+
 mpt_texture image_texture::to_abi() const {
     mpt_texture t{};
     t.type = MPT_TEX_IMAGE;
@@ -835,16 +843,13 @@ bool write_image(const std::string& path, int width, int height, const std::vect
 // ------------------------------------------------------------------ OBJ / MTL (obj_loader.h)
 namespace {
 
-bool file_exists(const std::string& path) {
-    std::ifstream f(path);
-    return bool(f);
-}
-
 // load_mtl (obj_loader.h:61-168): Kd -> lambertian, a non-black Ks -> a
 // principled layer with roughness from the Phong exponent Ns and Ni, Ke ->
-// an emitter. Texture statements name images the GPU path cannot use: a file
-// that exists makes loading fail, a missing one is skipped with a warning (the
-// reference then falls back to the plain color too).
+// an emitter; map_Kd replaces Kd, norm (or map_Bump, bump, map_Disp, disp
+// naming a color image) bends the normal. Textures wrap around; one that
+// cannot be loaded is skipped with a warning and the plain color stays. Alpha
+// masks (map_d, or a diffuse texture's alpha channel) are not modelled: a
+// material that would have one makes loading fail.
 bool load_mtl(const std::string& path, std::map<std::string, std::shared_ptr<material>>& result, std::string& error,
               bool use_normal_maps) {
     std::ifstream file(path);
@@ -854,9 +859,24 @@ bool load_mtl(const std::string& path, std::map<std::string, std::shared_ptr<mat
     }
     auto slash = path.find_last_of('/');
     std::string dir = slash == std::string::npos ? "" : path.substr(0, slash + 1);
+    // images shared between materials are loaded once per decoding (sRGB colors, linear normals)
+    std::map<std::pair<std::string, bool>, std::shared_ptr<image_texture>> images;
+    auto load_image = [&](const std::string& name, bool linear) -> std::shared_ptr<image_texture> {
+        const std::string full = name[0] == '/' ? name : dir + name;
+        auto it = images.find({full, linear});
+        if (it != images.end()) return it->second;
+        auto image = std::make_shared<image_texture>(full, true, linear);
+        if (!image->loaded()) {
+            std::fprintf(stderr, "Warning: %s: cannot load texture '%s'\n", path.c_str(), full.c_str());
+            image = nullptr;
+        }
+        images[{full, linear}] = image;
+        return image;
+    };
     struct entry {
         color kd{0.8, 0.8, 0.8}, ks{0, 0, 0}, ke{0, 0, 0};
         double ns = 0, ni = 1.5;
+        std::shared_ptr<image_texture> map_kd, map_d, normal_map;
     };
     std::vector<std::pair<std::string, entry>> entries;
     std::string line;
@@ -889,23 +909,28 @@ bool load_mtl(const std::string& path, std::map<std::string, std::shared_ptr<mat
         else if (keyword == "Ke") read_color(e.ke);
         else if (keyword == "Ns") in >> e.ns;
         else if (keyword == "Ni") in >> e.ni;
-        else {
-            const bool color_map = keyword == "map_Kd" || keyword == "map_d";
-            const bool normal_map = keyword == "norm" || keyword == "map_Bump" || keyword == "map_bump" ||
-                                    keyword == "bump" || keyword == "map_Disp" || keyword == "disp";
-            if (!color_map && !normal_map) continue;  // Ka, illum, d, ...
+        else if (keyword == "map_Kd") {
             std::string f = texture_file();
-            if (f.empty() || (normal_map && !use_normal_maps)) continue;
-            std::string full = f[0] == '/' ? f : dir + f;
-            if (!file_exists(full)) {
-                std::fprintf(stderr, "Warning: %s: cannot load texture '%s'\n", path.c_str(), full.c_str());
+            if (!f.empty()) e.map_kd = load_image(f, false);
+        } else if (keyword == "map_d") {
+            std::string f = texture_file();
+            if (!f.empty()) e.map_d = load_image(f, false);
+        } else if (keyword == "norm" || keyword == "map_Bump" || keyword == "map_bump" || keyword == "bump" ||
+                   keyword == "map_Disp" || keyword == "disp") {
+            std::string f = texture_file();
+            if (f.empty() || !use_normal_maps) continue;
+            auto image = load_image(f, true);
+            if (image && keyword != "norm" && image->greyscale()) {
+                std::fprintf(stderr, "Warning: %s: '%s' is a height map; only normal maps are supported\n", path.c_str(),
+                             f.c_str());
                 continue;
             }
-            error = path + ": '" + f + "': image textures, alpha masks and normal maps are not modelled by the GPU path";
-            return false;
+            if (image) e.normal_map = image;
         }
     }
     for (auto& [name, e] : entries) {
+        std::shared_ptr<texture> base =
+            e.map_kd ? std::static_pointer_cast<texture>(e.map_kd) : std::make_shared<solid_color>(e.kd);
         std::shared_ptr<material> m;
         if (e.ke.length_squared() > 0) {
             m = std::make_shared<diffuse_light>(e.ke);
@@ -913,9 +938,17 @@ bool load_mtl(const std::string& path, std::map<std::string, std::shared_ptr<mat
             // a Phong exponent n spreads like a microfacet alpha of sqrt(2 / (n + 2)); roughness is sqrt(alpha)
             double roughness = std::sqrt(std::sqrt(2 / (std::fmax(e.ns, 0.0) + 2)));
             double ior = e.ni > 1 ? e.ni : 1.5;
-            m = std::make_shared<principled>(std::make_shared<solid_color>(e.kd), 0.0, roughness, ior);
+            m = std::make_shared<principled>(base, 0.0, roughness, ior);
         } else {
-            m = std::make_shared<lambertian>(e.kd);
+            m = std::make_shared<lambertian>(base);
+        }
+        if (e.normal_map && e.ke.length_squared() == 0) m = std::make_shared<normal_mapped>(m, e.normal_map);
+        const bool masked = (e.map_d && !e.map_d->fully_opaque()) ||
+                            (!e.map_d && e.map_kd && (e.map_kd->channels == 2 || e.map_kd->channels == 4) &&
+                             !e.map_kd->fully_opaque());
+        if (masked) {
+            error = path + ": material '" + name + "': alpha masks are not modelled by the GPU path";
+            return false;
         }
         result[name] = m;
     }
@@ -934,9 +967,9 @@ bool load_obj(const std::string& path, const std::map<std::string, std::shared_p
     }
     std::vector<point3> positions;
     std::vector<vec3> normals;
-    size_t n_uvs = 0;  // texture coordinates are parsed for indexing; no material here reads them
+    std::vector<vec3> uvs;  // (u, v, 0)
     struct face {
-        int v[3], n[3];
+        int v[3], n[3], t[3];
     };
     // faces per material, the materials in the order of their first face
     std::vector<std::pair<std::shared_ptr<material>, std::vector<face>>> groups;
@@ -977,9 +1010,9 @@ bool load_obj(const std::string& path, const std::map<std::string, std::shared_p
         } else if (q[0] == 'v' && q[1] == 'n') {
             normals.push_back(read_vec(q + 2, 3));
         } else if (q[0] == 'v' && q[1] == 't') {
-            n_uvs++;
+            uvs.push_back(read_vec(q + 2, 2));
         } else if (q[0] == 'f' && (q[1] == ' ' || q[1] == '\t')) {
-            std::vector<int> vs, ns;
+            std::vector<int> vs, ns, ts;
             const char* r = q + 2;
             while (true) {
                 while (*r == ' ' || *r == '\t' || *r == '\r') r++;
@@ -1003,13 +1036,13 @@ bool load_obj(const std::string& path, const std::map<std::string, std::shared_p
                         r = end;
                     }
                 }
-                (void)t;
                 int vi = resolve(v, positions.size());
                 if (vi < 0) {
                     error = path + ":" + std::to_string(line_number) + ": vertex index out of range";
                     return false;
                 }
                 vs.push_back(vi);
+                ts.push_back(resolve(t, uvs.size()));
                 ns.push_back(resolve(n, normals.size()));
             }
             for (size_t k = 1; k + 1 < vs.size(); k++) {  // a fan; degenerate triangles are skipped
@@ -1018,6 +1051,7 @@ bool load_obj(const std::string& path, const std::map<std::string, std::shared_p
                 for (int c = 0; c < 3; c++) {
                     f.v[c] = vs[idx[c]];
                     f.n[c] = ns[idx[c]];
+                    f.t[c] = ts[idx[c]];
                 }
                 const point3& p0 = positions[f.v[0]];
                 if (cross(positions[f.v[1]] - p0, positions[f.v[2]] - p0).near_zero()) continue;
@@ -1062,18 +1096,24 @@ bool load_obj(const std::string& path, const std::map<std::string, std::shared_p
             }
     }
     for (auto& [mat, faces] : groups) {
-        std::vector<int32_t> idx, nidx;
-        bool any_normals = false;
+        std::vector<int32_t> idx, nidx, tidx;
+        bool any_normals = false, any_uvs = false;
         for (const auto& f : faces)
             for (int c = 0; c < 3; c++) {
                 idx.push_back(f.v[c]);
                 nidx.push_back(f.n[c]);
+                tidx.push_back(f.t[c]);
                 any_normals |= f.n[c] >= 0;
+                any_uvs |= f.t[c] >= 0;
             }
         auto m = std::make_shared<mesh>(positions, std::move(idx), mat);
         if (any_normals) {
             m->normals = normals;
             m->normal_indices = std::move(nidx);
+        }
+        if (any_uvs) {  // a triangle with all three uses them (mesh.h:75-81)
+            m->uvs = uvs;
+            m->uv_indices = std::move(tidx);
         }
         parts.push_back({mat, m});
     }

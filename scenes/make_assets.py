@@ -18,6 +18,10 @@ bumps_normal.png     a 64 x 64 tangent-space normal map of round bumps (RGB,
                      8-bit), its rows written with all five PNG filters in turn;
 tiles_rough.png      a 48 x 48 roughness map of tiles, 16-bit grey;
 tiles_metal.png      a 48 x 48 metallic map, a 2-bit palette image;
+crate.obj / .mtl     a box whose MTL materials take map_Kd (the JPEG below) and a
+                     norm normal map, with texture coordinates beyond [0, 1]
+                     (the textures repeat), mirrored on one side, and one
+                     face without them (its (u, v) are the barycentrics);
 globe.jpg            a 128 x 64 continents-and-ocean color map, baseline JPEG
                      with 2x2-subsampled chroma. JPEG needs an encoder: this
                      one is written by stb_image_write through the compiled
@@ -173,6 +177,30 @@ def sky_hdr(w=256, h=128):
     open(os.path.join(HERE, "sky_rle.hdr"), "wb").write(bytes(data))
 
 
+def crate():
+    # one material: parts of several materials come in heap order in the reference (a map keyed by pointers)
+    mtl = ["newmtl crate_wood", "Kd 0.8 0.8 0.8", "Ks 0.3 0.3 0.3", "Ns 60", "map_Kd -bm 1 globe.jpg",
+           "norm bumps_normal.png"]
+    open(os.path.join(HERE, "crate.mtl"), "w").write("\n".join(mtl) + "\n")
+    s = 0.45
+    v = [(-s, 0, -s), (s, 0, -s), (s, 2 * s, -s), (-s, 2 * s, -s), (-s, 0, s), (s, 0, s), (s, 2 * s, s), (-s, 2 * s, s)]
+    L = ["# a box of textured MTL materials", "mtllib crate.mtl"]
+    L += ["v " + " ".join(fmt(x) for x in p) for p in v]
+    # texture coordinates: a 2 x 2 repeat, an offset one outside [0, 1], and a mirrored one
+    vts = [(0, 0), (2, 0), (2, 2), (0, 2), (-0.5, 0.25), (0.75, 0.25), (0.75, 1.5), (-0.5, 1.5), (1, 0), (0, 0), (0, 1),
+           (1, 1)]
+    L += [f"vt {fmt(a)} {fmt(b)}" for a, b in vts]
+    L += ["vn 0 0 1", "vn 0 0 -1"]
+    L.append("usemtl crate_wood")
+    L.append("f 5/1/1 6/2/1 7/3/1 8/4/1")      # front (+z): repeated
+    L.append("f 2/5/2 1/6/2 4/7/2 3/8/2")      # back: offset outside [0, 1]
+    L.append("f 1/9 5/10 8/11 4/12")           # left: mirrored (u runs the other way)
+    L.append("f 6 2 3 7")                      # right: no texture coordinates
+    L.append("f 8/1 7/2 3/3 4/4")              # top
+    L.append("f 1/1 2/2 6/3 5/4")              # bottom
+    open(os.path.join(HERE, "crate.obj"), "w").write("\n".join(L) + "\n")
+
+
 def png_write(path, rows, color_type, depth, palette=None):
     """PNG from rows of samples (lists of ints), filters 0-4 in turn by row
     (PNG spec, section 9: each filter's difference from the raw bytes)."""
@@ -291,3 +319,4 @@ if __name__ == "__main__":
     sky_hdr()
     texture_maps()
     globe_jpg()
+    crate()

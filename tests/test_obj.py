@@ -150,9 +150,10 @@ TRI = "v 0 0 0\nv 1 0 0\nv 0 1 0\n"
     (TRI + "f 1 2 4\n", None, "", (), False),                                  # index out of range
     (TRI + "f 1 x 3\n", None, "", (), False),                                  # bad face
     (TRI + "f -3 -2 -1\n", None, "", (), True),                                # relative indices
-    ("mtllib m.mtl\nusemtl a\n" + TRI + "f 1 2 3\n", "newmtl a\nKd 1 0 0\nmap_Kd tex.png\n", "", ("tex.png",), False),
+    # an image that cannot be decoded falls back to the plain color with a warning (obj_loader.h:83-87)
+    ("mtllib m.mtl\nusemtl a\n" + TRI + "f 1 2 3\n", "newmtl a\nKd 1 0 0\nmap_Kd tex.png\n", "", ("tex.png",), True),
     ("mtllib m.mtl\nusemtl a\n" + TRI + "f 1 2 3\n", "newmtl a\nKd 1 0 0\nmap_Kd missing.png\n", "", (), True),
-    ("mtllib m.mtl\nusemtl a\n" + TRI + "f 1 2 3\n", "newmtl a\nKd 1 0 0\nnorm n.png\n", "", ("n.png",), False),
+    ("mtllib m.mtl\nusemtl a\n" + TRI + "f 1 2 3\n", "newmtl a\nKd 1 0 0\nnorm n.png\n", "", ("n.png",), True),
     ("mtllib nowhere.mtl\n" + TRI + "f 1 2 3\n", None, "", (), True),          # a missing MTL only warns
     (TRI + "f 1 2 3\n", None, "replace a b", (), False),                       # glTF-only syntax
     ("v 0 0 0\nv 1 0 0\nv 2 0 0\nf 1 2 3\n", None, "", (), False),             # only degenerate faces
