@@ -104,6 +104,13 @@ typedef struct mpt_material {
     int32_t metallic_texture;  /* principled: metallic from this texture's red channel, -1 = the constant */
     int32_t roughness_texture; /* principled: roughness from this texture's red channel, -1 = the constant */
     int32_t uses_uv;           /* 1 when a texture above or the base texture is an image: hits need (u, v) */
+    int32_t thin;              /* dielectric: a thin sheet (thin_dielectric, material.h:683-707): reflects
+                                  2R / (1 + R) of the light, lets the rest through unbent, tinted by albedo */
+    int32_t measured;          /* metal, principled: Fresnel of the complex index eta + i k (fresnel_conductor,
+                                  microfacet.h:57-66) instead of Schlick's; albedo (metal) and f_avg hold
+                                  its head-on value and hemispherical average (microfacet.h:109-121) */
+    double eta[3];             /* measured conductor: real and imaginary index of refraction, RGB */
+    double k[3];
 } mpt_material;
 
 /* An affine transform with its stored inverse, as transform.h:120-123:
@@ -436,6 +443,11 @@ void mpt_display_defaults(mpt_display* display);
 /* The display settings a .scene file sets (its bloom, vignette, exposure
  * and tonemap lines, scene.h:667-688); the defaults where it sets none. */
 int32_t mpt_scene_display(const char* path, mpt_display* display);
+
+/* find_conductor (microfacet.h:76-96): the RGB complex index of refraction
+ * (eta + i k) of a named metal (Al, Ti, Cu, Au, Ag, Cr, or their long names)
+ * for mpt_material.measured; MPT_E_INVALID for an unknown name. */
+int32_t mpt_find_conductor(const char* name, double* eta, double* k);
 
 /* postprocess(pixels, width, height, post) (postprocess.h:108-153): bloom
  * and vignetting on linear radiance (interleaved RGB doubles, row-major

@@ -174,12 +174,32 @@ public:
     std::shared_ptr<texture> base_texture() const override { return tex; }
     std::shared_ptr<texture> tex;
 };
+// A measured metal's complex index of refraction, RGB (microfacet.h:68-96).
+struct conductor_ior {
+    color eta, k;
+};
+// The named metals of the reference's table (Al, Ti, Cu, Au, Ag, Cr and their
+// long names); false for any other name.
+bool find_conductor(const std::string& name, conductor_ior& ior);
+
 class metal : public material {
 public:
     metal(const color& albedo, double roughness);
+    metal(const conductor_ior& ior, double roughness);  // exact Fresnel, also on the mirror
     mpt_material to_abi(int32_t ti) const override;
     color albedo;
     double roughness;
+    bool measured = false;
+    conductor_ior ior{};
+};
+// A thin sheet of glass (thin_dielectric, material.h:683-707): light is
+// reflected (2R / (1 + R) of it) or goes straight through, tinted.
+class thin_dielectric : public material {
+public:
+    explicit thin_dielectric(double ior, const color& tint = color(1, 1, 1)) : ior(ior), tint(tint) {}
+    mpt_material to_abi(int32_t ti) const override;
+    double ior;
+    color tint;
 };
 class dielectric : public material {
 public:
@@ -202,6 +222,9 @@ public:
     // read from the red channel at each hit when set (material.h:508-515)
     std::shared_ptr<texture> metallic_map, roughness_map;
     std::shared_ptr<texture> normal_map;  // bends the base layer (material.h:389)
+    // the metallic part as a measured metal (set_conductor, material.h:381-385)
+    bool measured = false;
+    conductor_ior conductor{};
 };
 class diffuse_light : public material {
 public:

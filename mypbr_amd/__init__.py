@@ -98,6 +98,7 @@ def _load():
         "mpt_denoise_dev": (C.c_int32, [vp] * 4 + [C.c_int32, C.c_int32, P(abi.DenoiseOpts), vp, vp, vp]),
         "mpt_display_defaults": (None, [P(abi.Display)]),
         "mpt_scene_display": (C.c_int32, [C.c_char_p, P(abi.Display)]),
+        "mpt_find_conductor": (C.c_int32, [C.c_char_p, P(C.c_double), P(C.c_double)]),
         "mpt_postprocess": (C.c_int32, [P(C.c_double), C.c_int32, C.c_int32, P(abi.Display), P(C.c_double)]),
         "mpt_display_bytes": (C.c_int32, [P(C.c_double), C.c_int32, C.c_int32, P(abi.Display), C.c_int32,
                                           P(C.c_uint8)]),

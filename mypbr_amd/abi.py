@@ -32,7 +32,8 @@ class Material(C.Structure):
                 ("f0", C.c_double), ("g", C.c_double), ("spread", C.c_double),
                 ("tan_half", C.c_double), ("albedo", D3), ("f_avg", D3), ("emit", D3),
                 ("sigma_a", D3), ("avg_emission", D3), ("normal_map", C.c_int32),
-                ("metallic_texture", C.c_int32), ("roughness_texture", C.c_int32), ("uses_uv", C.c_int32)]
+                ("metallic_texture", C.c_int32), ("roughness_texture", C.c_int32), ("uses_uv", C.c_int32),
+                ("thin", C.c_int32), ("measured", C.c_int32), ("eta", D3), ("k", D3)]
 
 
 class Xform(C.Structure):

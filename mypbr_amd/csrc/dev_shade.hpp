@@ -92,10 +92,10 @@ __device__ __forceinline__ v3 unbend(const v3& dir, const v3& shading, const v3&
 struct pparams {
     double metallic, alpha;
 };
-template <bool UV>
+template <bool RICH>
 __device__ __forceinline__ pparams principled_at(const dscene& S, const mpt_material& m, const hitrec& rec) {
     pparams q{m.metallic, m.alpha};
-    if (!UV) return q;
+    if (!RICH) return q;
     if (m.metallic_texture >= 0) q.metallic = std::fmin(std::fmax(tex_value(S, m.metallic_texture, rec).x, 0.0), 1.0);
     if (m.roughness_texture >= 0)
         q.alpha = roughness_to_alpha(std::fmin(std::fmax(tex_value(S, m.roughness_texture, rec).x, 0.0), 1.0));
@@ -316,10 +316,91 @@ __device__ __forceinline__ v3 principled_f0(const mpt_material& m, double metall
     return (1 - metallic) * v3{m.f0, m.f0, m.f0} + metallic * base;
 }
 
+// ---------------------------------------------------------------- measured metals
+// fresnel_conductor (microfacet.h:57-66) with the complex arithmetic written
+// out: products as (ac - bd, ad + bc); quotients by Smith's method, in the
+// branch order of the C runtime's complex division; the square root by the
+// half-angle identity on hypot, taking the root of the larger of d +- Re z
+// first so that nothing cancels.
+struct cpx {
+    double re, im;
+};
+__device__ __forceinline__ cpx cmul(cpx a, cpx b) { return cpx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ cpx cdiv(cpx a, cpx b) {
+    if (std::fabs(b.re) < std::fabs(b.im)) {
+        const double ratio = b.re / b.im;
+        const double denom = (b.re * ratio) + b.im;
+        return cpx{((a.re * ratio) + a.im) / denom, ((a.im * ratio) - a.re) / denom};
+    }
+    const double ratio = b.im / b.re;
+    const double denom = (b.im * ratio) + b.re;
+    return cpx{((a.im * ratio) + a.re) / denom, (a.im - (a.re * ratio)) / denom};
+}
+__device__ __forceinline__ cpx csqrt_(cpx z) {
+    if (z.im == 0) {
+        if (z.re < 0) return cpx{0, std::copysign(std::sqrt(-z.re), z.im)};
+        return cpx{std::fabs(std::sqrt(z.re)), std::copysign(0.0, z.im)};
+    }
+    if (z.re == 0) {
+        const double r = std::sqrt(0.5 * std::fabs(z.im));
+        return cpx{r, std::copysign(r, z.im)};
+    }
+    const double d = hypot(z.re, z.im);
+    double r, s;
+    if (z.re > 0) {
+        r = std::sqrt(0.5 * (d + z.re));
+        s = 0.5 * (z.im / r);
+    } else {
+        s = std::sqrt(0.5 * (d - z.re));
+        r = std::fabs(0.5 * (z.im / s));
+    }
+    return cpx{r, std::copysign(s, z.im)};
+}
+__device__ __forceinline__ double conductor_fresnel(double cos_i, double eta, double k) {
+    cos_i = std::fmin(std::fmax(cos_i, 0.0), 1.0);
+    const cpx n{eta, k};
+    const cpx sin2_t = cdiv(cpx{1 - cos_i * cos_i, 0}, cmul(n, n));
+    const cpx cos_t = csqrt_(cpx{1.0 - sin2_t.re, 0.0 - sin2_t.im});
+    const cpx nc{n.re * cos_i, n.im * cos_i};
+    const cpx r_parl = cdiv(cpx{nc.re - cos_t.re, nc.im - cos_t.im}, cpx{nc.re + cos_t.re, nc.im + cos_t.im});
+    const cpx nt = cmul(n, cos_t);
+    const cpx r_perp = cdiv(cpx{cos_i - nt.re, 0.0 - nt.im}, cpx{cos_i + nt.re, 0.0 + nt.im});
+    return ((r_parl.re * r_parl.re + r_parl.im * r_parl.im) + (r_perp.re * r_perp.re + r_perp.im * r_perp.im)) / 2;
+}
+__device__ __forceinline__ v3 conductor_at(const mpt_material& m, double c) {
+    return v3{conductor_fresnel(c, m.eta[0], m.k[0]), conductor_fresnel(c, m.eta[1], m.k[1]),
+              conductor_fresnel(c, m.eta[2], m.k[2])};
+}
+// metal_fresnel::at (microfacet.h:123-129): Schlick through albedo, or measured
+template <bool RICH>
+__device__ __forceinline__ v3 metal_fresnel(const mpt_material& m, double c) {
+    if (RICH && m.measured) return conductor_at(m, c);
+    return fresnel_schlick3(c, m3(m.albedo));
+}
+// principled::specular_fresnel and ms_fresnel (material.h:537-551)
+template <bool RICH>
+__device__ __forceinline__ v3 principled_fresnel(const mpt_material& m, double c, const v3& base, double metallic) {
+    if (RICH && m.measured) {
+        const double fd = fresnel_schlick(c, m.f0);
+        return (1 - metallic) * v3{fd, fd, fd} + metallic * conductor_at(m, c);
+    }
+    return fresnel_schlick3(c, principled_f0(m, metallic, base));
+}
+template <bool RICH>
+__device__ __forceinline__ v3 principled_ms_fresnel(const dscene& S, const mpt_material& m, const v3& base,
+                                                    double metallic, double alpha) {
+    if (RICH && m.measured) {
+        const double fd = m.f0 + (1 - m.f0) / 21;
+        return ggx_ms_fresnel_avg(S.ggx_eavg, (1 - metallic) * v3{fd, fd, fd} + metallic * m3(m.f_avg), alpha);
+    }
+    return ggx_ms_fresnel(S.ggx_eavg, principled_f0(m, metallic, base), alpha);
+}
+
 // material::scatter (material.h:79-85, 123-150, 210-242, 387-421; medium.h:21-27).
 // MT: the material type when known at compile time (-1: dispatch on m.type).
-// UV: the scene has image textures or texture maps (hits carry (u, v)).
-template <int MT = -1, bool UV = false>
+// RICH: the scene has image textures, texture or normal maps (hits carry
+// (u, v)), thin glass or measured metals; without it their code is left out.
+template <int MT = -1, bool RICH = false>
 __device__ __forceinline__ scatter_out mat_scatter(const dscene& S, const mpt_material& m, const v3& rd,
                                                    const hitrec& rec, rstream& rs) {
     scatter_out o;
@@ -335,12 +416,12 @@ __device__ __forceinline__ scatter_out mat_scatter(const dscene& S, const mpt_ma
     o.dens.w0 = 1;
     switch (MT >= 0 ? MT : m.type) {
         case MPT_MAT_LAMBERTIAN:
-            o.attenuation = tex_value<UV>(S, m.texture, rec);
+            o.attenuation = tex_value<RICH>(S, m.texture, rec);
             break;
         case MPT_MAT_METAL: {
             v3 albedo = m3(m.albedo);
-            if (m.roughness == 0) {
-                o.attenuation = albedo;
+            if (m.roughness == 0) {  // a Schlick mirror reflects albedo at every angle
+                o.attenuation = RICH && m.measured ? conductor_at(m, dot(rec.normal, unit_vector(-rd))) : albedo;
                 o.skip = true;
                 o.skip_dir = reflect(unit_vector(rd), rec.normal);
                 break;
@@ -352,13 +433,27 @@ __device__ __forceinline__ scatter_out mat_scatter(const dscene& S, const mpt_ma
             double cos_o = std::fmax(0.0, dot(rec.normal, unit_vector(-rd)));
             double ms_w = ggx_ms_albedo(S.ggx_e, S.ggx_eavg, cos_o, m.alpha) *
                           avg3(ggx_ms_fresnel_avg(S.ggx_eavg, m3(m.f_avg), m.alpha));
-            double ss_w = ggx_E(S.ggx_e, cos_o, m.alpha) * avg3(fresnel_schlick3(cos_o, albedo));
+            double ss_w = ggx_E(S.ggx_e, cos_o, m.alpha) * avg3(metal_fresnel<RICH>(m, cos_o));
             if (ms_w <= 0) break;
             o.dens.kind = LOBE_MIX;
             o.dens.w0 = clampd(ss_w / (ss_w + ms_w), 0.1, 1.0);
             break;
         }
         case MPT_MAT_DIELECTRIC: {
+            if (RICH && m.thin) {  // thin_dielectric::scatter (material.h:687-702)
+                v3 ud = unit_vector(rd);
+                double cos_theta = std::fmin(dot(-ud, rec.normal), 1.0);
+                double r = fresnel_dielectric(cos_theta, m.ior);
+                double reflectance = 2 * r / (1 + r);
+                o.skip = true;
+                if (rs.next() < reflectance) {
+                    o.skip_dir = reflect(ud, rec.normal);
+                } else {
+                    o.attenuation = m3(m.albedo);
+                    o.skip_dir = ud;
+                }
+                break;
+            }
             if (m.roughness > 0) {
                 o.dens.kind = LOBE_ROUGH_GLASS;
                 o.dens.wo_world = -rd;
@@ -380,15 +475,15 @@ __device__ __forceinline__ scatter_out mat_scatter(const dscene& S, const mpt_ma
         }
         case MPT_MAT_PRINCIPLED: {
             v3 wo = unit_vector(-rd);
-            const pparams pp = principled_at<UV>(S, m, rec);
-            v3 base = tex_value<UV>(S, m.texture, rec);
+            const pparams pp = principled_at<RICH>(S, m, rec);
+            v3 base = tex_value<RICH>(S, m.texture, rec);
             double metallic = pp.metallic;
             double alpha = pp.alpha;
             double cos_o = std::fmax(0.0, dot(rec.normal, wo));
-            v3 sf0 = principled_f0(m, metallic, base);
-            double spec_w = avg3(fresnel_schlick3(cos_o, sf0));
+            double spec_w = avg3(principled_fresnel<RICH>(m, cos_o, base, metallic));
             double diff_w = (1 - metallic) * avg3(base) * (1 - fresnel_schlick(cos_o, m.f0));
-            diff_w += ggx_ms_albedo(S.ggx_e, S.ggx_eavg, cos_o, alpha) * avg3(ggx_ms_fresnel(S.ggx_eavg, sf0, alpha));
+            diff_w += ggx_ms_albedo(S.ggx_e, S.ggx_eavg, cos_o, alpha) *
+                      avg3(principled_ms_fresnel<RICH>(S, m, base, metallic, alpha));
             double p_spec = spec_w + diff_w > 0 ? spec_w / (spec_w + diff_w) : 1;
             o.dens.kind = LOBE_MIX;
             o.dens.w0 = clampd(p_spec, 0.1, 1.0);
@@ -413,15 +508,15 @@ __device__ __forceinline__ scatter_out mat_scatter(const dscene& S, const mpt_ma
 // material::eval: f * |cos| for a non-specular direction. rec.normal is the
 // shading normal (a normal map's, when the material has one) and gn the
 // hit's own: no light arrives from below gn (material.h:426, 651).
-template <int MT = -1, bool UV = false>
+template <int MT = -1, bool RICH = false>
 __device__ __forceinline__ v3 mat_eval(const dscene& S, const mpt_material& m, const v3& rd, const hitrec& rec,
                                        const v3& gn, const v3& dir) {
-    if (UV && m.normal_map >= 0 && dot(dir, gn) <= 0) return v3{0, 0, 0};
+    if (RICH && m.normal_map >= 0 && dot(dir, gn) <= 0) return v3{0, 0, 0};
     switch (MT >= 0 ? MT : m.type) {
         case MPT_MAT_LAMBERTIAN: {  // material.h:88-97
             double c = dot(rec.normal, unit_vector(dir));
             double pdf = c < 0 ? 0 : c / kPi;
-            return tex_value<UV>(S, m.texture, rec) * pdf;
+            return tex_value<RICH>(S, m.texture, rec) * pdf;
         }
         case MPT_MAT_METAL: {  // material.h:154-164
             onb f = make_onb(rec.normal);
@@ -429,8 +524,7 @@ __device__ __forceinline__ v3 mat_eval(const dscene& S, const mpt_material& m, c
             v3 wi = unit_vector(onb_to_local(f, dir));
             if (wo.z <= 0 || wi.z <= 0) return v3{0, 0, 0};
             v3 h = unit_vector(wo + wi);
-            v3 spec = fresnel_schlick3(dot(wi, h), m3(m.albedo)) *
-                      (ggx_D(m.alpha, h) * ggx_G(m.alpha, wo, wi) / (4 * wo.z));
+            v3 spec = metal_fresnel<RICH>(m, dot(wi, h)) * (ggx_D(m.alpha, h) * ggx_G(m.alpha, wo, wi) / (4 * wo.z));
             return spec + ggx_ms_fresnel_avg(S.ggx_eavg, m3(m.f_avg), m.alpha) *
                               ggx_ms_eval(S.ggx_e, S.ggx_eavg, wo.z, wi.z, m.alpha);
         }
@@ -448,13 +542,14 @@ __device__ __forceinline__ v3 mat_eval(const dscene& S, const mpt_material& m, c
             v3 wo = unit_vector(onb_to_local(f, wo_w));
             v3 wi = unit_vector(onb_to_local(f, wi_w));
             if (wo.z <= 0 || wi.z <= 0) return v3{0, 0, 0};
-            const pparams pp = principled_at<UV>(S, m, rec);
+            const pparams pp = principled_at<RICH>(S, m, rec);
             double metallic = pp.metallic, alpha = pp.alpha;
-            v3 base = tex_value<UV>(S, m.texture, rec);
+            v3 base = tex_value<RICH>(S, m.texture, rec);
             v3 h = unit_vector(wo + wi);
-            v3 sf0 = principled_f0(m, metallic, base);
-            v3 specular = fresnel_schlick3(dot(wi, h), sf0) * (ggx_D(alpha, h) * ggx_G(alpha, wo, wi) / (4 * wo.z)) +
-                          ggx_ms_fresnel(S.ggx_eavg, sf0, alpha) * ggx_ms_eval(S.ggx_e, S.ggx_eavg, wo.z, wi.z, alpha);
+            v3 specular = principled_fresnel<RICH>(m, dot(wi, h), base, metallic) *
+                              (ggx_D(alpha, h) * ggx_G(alpha, wo, wi) / (4 * wo.z)) +
+                          principled_ms_fresnel<RICH>(S, m, base, metallic, alpha) *
+                              ggx_ms_eval(S.ggx_e, S.ggx_eavg, wo.z, wi.z, alpha);
             double transmitted = (1 - fresnel_schlick(wi.z, m.f0)) * (1 - fresnel_schlick(wo.z, m.f0));
             v3 diffuse = ((1 - metallic) * transmitted * wi.z / kPi) * base;
             return specular + diffuse;
@@ -492,7 +587,7 @@ __device__ __forceinline__ v3 mat_interior(const mpt_material& m, const v3& rd, 
 }
 
 __device__ __forceinline__ bool mat_lets_shadows(const mpt_material& m) {
-    return m.type == MPT_MAT_DIELECTRIC && m.roughness == 0;
+    return m.type == MPT_MAT_DIELECTRIC && m.roughness == 0 && !m.thin;  // thin_dielectric keeps the default
 }
 
 // ---------------------------------------------------------------- environment

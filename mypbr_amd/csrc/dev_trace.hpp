@@ -100,7 +100,8 @@ enum : int {
     F_MESH = 32,
     F_MEDIUM = 64,
     F_ALL = 127,
-    // not geometry: hits also get (u, v) and dp/du, dp/dv (image textures, normal maps)
+    // not geometry: hits also get (u, v) and dp/du, dp/dv, and shading has image
+    // textures, texture and normal maps, thin glass and measured metals
     F_UV = 128,
 };
 

@@ -573,7 +573,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_SP
 // missed); LIGHTS: the scene has lights, so non-specular vertices do next
 // event estimation. Compiling each material apart keeps each kernel's
 // register footprint to what that material needs.
-template <int MT, int LK, bool ENV, bool UV = false>
+template <int MT, int LK, bool ENV, bool RICH = false>
 __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const pstate& P, int depth, uint32_t p,
                                           bool& alive, bool& nee) {
     constexpr bool LIGHTS = LK != 0;
@@ -611,7 +611,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
         rec.mat = int32_t(hm & 0x7fffffffu);
         rec.front = (hm >> 31) != 0;
         const mpt_material& m = S.mat[rec.mat];
-        if (UV && m.uses_uv) {
+        if (RICH && m.uses_uv) {
             rec.u = pd(P, D_U, p);
             rec.v = pd(P, D_V, p);
             rec.dpdu = ld3(P, D_DUX, p);
@@ -619,7 +619,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
         }
         // the hit's own normal; a normal map bends the one the material shades with
         const v3 gn = rec.normal;
-        if (UV && m.normal_map >= 0) rec.normal = normal_from_map(S, m.normal_map, d, rec);
+        if (RICH && m.normal_map >= 0) rec.normal = normal_from_map(S, m.normal_map, d, rec);
         if constexpr (MT == MPT_MAT_DIELECTRIC) beta = beta * mat_interior<MT>(m, d, rec);  // camera.h:396
         if constexpr (MT == MPT_MAT_DIFFUSE_LIGHT) {                                          // camera.h:399-407
             v3 e = mat_emitted<MT>(m, d, rec);
@@ -635,8 +635,8 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
             return;  // a light scatters nothing (material.h:28-31)
         } else {
             rstream rs{P.key[p], ctr_out};
-            scatter_out so = mat_scatter<MT, UV>(S, m, d, rec, rs);  // camera.h:409-411
-            if (UV && so.skip && m.normal_map >= 0) so.skip_dir = unbend(so.skip_dir, rec.normal, gn);
+            scatter_out so = mat_scatter<MT, RICH>(S, m, d, rec, rs);  // camera.h:409-411
+            if (RICH && so.skip && m.normal_map >= 0) so.skip_dir = unbend(so.skip_dir, rec.normal, gn);
             if (so.skip) {                                        // camera.h:413-423
                 bool stop = C.max_specular > 0 && ++scount > C.max_specular;
                 if (!stop) {
@@ -658,7 +658,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
                     v3 to_light = lights_random<LK>(S, rec.p, rs);
                     double light_pdf = lights_pdf<LK>(S, rec.p, to_light);
                     if (light_pdf > 0) {
-                        v3 f = mat_eval<MT, UV>(S, m, d, rec, gn, to_light);
+                        v3 f = mat_eval<MT, RICH>(S, m, d, rec, gn, to_light);
                         if (!(f.x <= 0 && f.y <= 0 && f.z <= 0)) {
                             double weight =
                                 last_bounce ? 1.0 : power_heuristic(light_pdf, density_value<MT>(so.dens, to_light));
@@ -675,7 +675,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
                     v3 dir = density_generate<MT>(so.dens, rs);
                     double spdf = density_value<MT>(so.dens, dir);
                     if (spdf > 0) {
-                        beta = beta * mat_eval<MT, UV>(S, m, d, rec, gn, dir) / spdf;
+                        beta = beta * mat_eval<MT, RICH>(S, m, d, rec, gn, dir) / spdf;
                         if constexpr (LIGHTS) {  // read only by the MIS weights of next-event estimation
                             spdf_out = spdf;
                             vertex = true;
@@ -712,7 +712,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
 #define MPT_SHADE_WAVES 1
 #endif
 
-template <int BS, int MT, int LK, bool ENV, bool UV = false>
+template <int BS, int MT, int LK, bool ENV, bool RICH = false>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MPT_SHADE_WAVES))) void k_shade(dscene S, dcam C, pstate P, int depth, const uint32_t* q,
                                               const uint32_t* cnt_in, uint32_t* qnext, uint32_t* cnt_next,
                                               uint32_t* qshadow, uint32_t* cnt_shadow) {
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MPT_SHADE_WA
         uint32_t p = 0;
         if (i < n) {
             p = q[i];
-            shade_one<MT, LK, ENV, UV>(S, C, P, depth, p, alive, nee);
+            shade_one<MT, LK, ENV, RICH>(S, C, P, depth, p, alive, nee);
         }
         block_append<BS, NQ>(
             lds, (alive ? 1u : 0u) | (nee ? 2u : 0u), p, [&](int k) { return k ? cnt_shadow : cnt_next; },
@@ -825,7 +825,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MPT_FINISH_W
                     shade_one<MPT_MAT_LAMBERTIAN, LK, false, kUV>(S, C, P, depth, p, alive, nee);
                     break;
                 case 1 + MPT_MAT_METAL: shade_one<MPT_MAT_METAL, LK, false, kUV>(S, C, P, depth, p, alive, nee); break;
-                case 1 + MPT_MAT_DIELECTRIC: shade_one<MPT_MAT_DIELECTRIC, LK, false>(S, C, P, depth, p, alive, nee); break;
+                case 1 + MPT_MAT_DIELECTRIC:
+                    shade_one<MPT_MAT_DIELECTRIC, LK, false, kUV>(S, C, P, depth, p, alive, nee);
+                    break;
                 case 1 + MPT_MAT_PRINCIPLED:
                     shade_one<MPT_MAT_PRINCIPLED, LK, false, kUV>(S, C, P, depth, p, alive, nee);
                     break;
@@ -1016,8 +1018,8 @@ __global__ void k_bsdf_eval(dscene S, const int32_t* mats, const double* q, cons
     rec.mat = mats[i];
     const mpt_material& m = S.mat[rec.mat];
     rstream rs{0, 0};
-    scatter_out so = mat_scatter(S, m, rd, rec, rs);
-    v3 e = mat_eval(S, m, rd, rec, rec.normal, wi);
+    scatter_out so = mat_scatter<-1, true>(S, m, rd, rec, rs);
+    v3 e = mat_eval<-1, true>(S, m, rd, rec, rec.normal, wi);
     eval[3 * i] = e.x, eval[3 * i + 1] = e.y, eval[3 * i + 2] = e.z;
     pdf[i] = (so.scattered && !so.skip) ? density_value(so.dens, wi) : 0.0;
 }
@@ -1274,8 +1276,8 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
         }
         for (int32_t k : h.prim_kind)
             if (k == MPT_PRIM_TRIANGLE) quads_ok = false;
-        // 3: everything, and hits carry (u, v) and tangents for textures and normal maps
-        s->profile = h.uses_uv ? 3 : (prims_only && !h.objects.empty() ? 1 : (quads_ok ? 2 : 0));
+        // 3: everything, hits carry (u, v) and tangents, shading has textures, maps, thin glass, measured metals
+        s->profile = h.rich ? 3 : (prims_only && !h.objects.empty() ? 1 : (quads_ok ? 2 : 0));
         for (int32_t m : h.prim_mat)
             if (m >= 0) s->mat_types |= 1u << h.materials[m].type;
         for (const auto& r : h.objects)
@@ -1747,13 +1749,14 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
     auto launch_shade = [&](int cls, int depth, uint32_t* qout, uint32_t* cout) {
         const uint32_t* qc = qcls + cls * cap;
         const uint32_t* cc = ccls + cls * kCtrStride;
-#define SHA3(MT, LK, ENV, UV)                                                                                    \
-    hipLaunchKernelGGL((k_shade<256, MT, LK, ENV, UV>), dim3(G), dim3(256), 0, stream, DS, C, P, depth, qc, cc,    \
+#define SHA3(MT, LK, ENV, RICH)                                                                                  \
+    hipLaunchKernelGGL((k_shade<256, MT, LK, ENV, RICH>), dim3(G), dim3(256), 0, stream, DS, C, P, depth, qc, cc,    \
                        qout, cout, q[2], ctr(cnt, 2))
-    // the materials that read (u, v) get the variant that loads it in scenes with image textures or maps
+    // in scenes with textures, maps, thin glass or measured metals the surface materials get the variant with them
 #define SHA2(MT, LK, ENV)                                                                                           \
     do {                                                                                                            \
-        constexpr bool kMapped = MT == MPT_MAT_LAMBERTIAN || MT == MPT_MAT_METAL || MT == MPT_MAT_PRINCIPLED;       \
+        constexpr bool kMapped = MT == MPT_MAT_LAMBERTIAN || MT == MPT_MAT_METAL || MT == MPT_MAT_DIELECTRIC ||     \
+                                 MT == MPT_MAT_PRINCIPLED;                                                          \
         if (kMapped && s->profile == 3) SHA3(MT, LK, ENV, kMapped);                                                \
         else SHA3(MT, LK, ENV, false);                                                                              \
     } while (0)

@@ -230,7 +230,47 @@ static mpt_material blank(int type, int32_t ti) {
 
 mpt_material lambertian::to_abi(int32_t ti) const { return blank(MPT_MAT_LAMBERTIAN, ti); }
 
+// The table of find_conductor (microfacet.h:76-96): RGB (eta, k) of six
+// metals, data that has to equal the reference's for the same reflectance.
+bool find_conductor(const std::string& name, conductor_ior& ior) {
+    struct entry {
+        const char* names[3];
+        double eta[3], k[3];
+    };
+    static const entry table[] = {
+        {{"Al", "aluminium", "aluminum"}, {1.6575, 0.880405, 0.521244}, {9.22381, 6.2695, 4.837}},
+        {{"Ti", "titanium", "titanium"}, {3.2754, 2.37975, 2.02469}, {4.57233, 3.22453, 2.72781}},
+        {{"Cu", "copper", "copper"}, {0.201005, 0.92375, 1.10222}, {3.91326, 2.45305, 2.14209}},
+        {{"Au", "gold", "gold"}, {0.143036, 0.375307, 1.44205}, {3.983, 2.38556, 1.60336}},
+        {{"Ag", "silver", "silver"}, {0.155276, 0.116728, 0.138388}, {4.82835, 3.12222, 2.1469}},
+        {{"Cr", "chromium", "chrome"}, {4.36967, 2.91651, 1.6549}, {5.20655, 4.23131, 3.7547}},
+    };
+    for (const entry& e : table)
+        for (const char* n : e.names)
+            if (name == n) {
+                ior.eta = color(e.eta[0], e.eta[1], e.eta[2]);
+                ior.k = color(e.k[0], e.k[1], e.k[2]);
+                return true;
+            }
+    return false;
+}
+
+
+// A measured metal's index; the scene build derives its head-on color and
+// hemispherical average from it.
+static void set_conductor(mpt_material& m, const conductor_ior& ior) {
+    m.measured = 1;
+    for (int c = 0; c < 3; c++) {
+        m.eta[c] = ior.eta[c];
+        m.k[c] = ior.k[c];
+    }
+}
+
 metal::metal(const color& a, double r) : albedo(a), roughness(std::clamp(r, 0.0, 1.0)) {}
+metal::metal(const conductor_ior& i, double r) : metal(color(1, 1, 1), r) {
+    measured = true;
+    ior = i;
+}
 mpt_material metal::to_abi(int32_t ti) const {
     mpt_material m = blank(MPT_MAT_METAL, ti);
     m.roughness = roughness;
@@ -240,6 +280,15 @@ mpt_material metal::to_abi(int32_t ti) const {
         m.albedo[k] = albedo[k];
         m.f_avg[k] = f_avg[k];
     }
+    if (measured) set_conductor(m, ior);
+    return m;
+}
+
+mpt_material thin_dielectric::to_abi(int32_t ti) const {
+    mpt_material m = blank(MPT_MAT_DIELECTRIC, ti);
+    m.ior = ior;
+    m.thin = 1;
+    for (int k = 0; k < 3; k++) m.albedo[k] = tint[k];
     return m;
 }
 
@@ -267,6 +316,7 @@ mpt_material principled::to_abi(int32_t ti) const {
     m.ior = ior;
     double r0 = (ior - 1) / (ior + 1);
     m.f0 = r0 * r0;
+    if (measured) set_conductor(m, conductor);
     return m;
 }
 
@@ -1580,13 +1630,35 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
                     return fail(error.empty() ? "expected 'material <name> lambertian <r> <g> <b>'" : error);
                 if (!read_normal_map(error)) return fail(error);
                 mat = std::make_shared<lambertian>(tex);
-            } else if (type == "metal") {
-                double r, g, b, roughness;
-                std::string error;
-                if (!(in >> r >> g >> b >> roughness) || roughness < 0)
-                    return fail("expected 'material <name> metal <r> <g> <b> <roughness>' (measured metals are not modelled)");
+            } else if (type == "metal") {  // scene.h:758-780
+                const char* usage = "expected 'material <name> metal <r> <g> <b> <roughness>'"
+                                    " or '... metal <Al|Ti|Cu|Au|Ag|Cr> <roughness>'";
+                double r = 0, g = 0, b = 0, roughness;
+                std::string first, error;
+                conductor_ior ior;
+                if (!(in >> first)) return fail(usage);
+                const bool measured = find_conductor(first, ior);
+                if (measured) {
+                    if (!(in >> roughness) || roughness < 0) return fail(usage);
+                } else {
+                    std::istringstream number(first);
+                    if (!(number >> r) || !(in >> g >> b >> roughness) || roughness < 0) return fail(usage);
+                }
                 if (!read_normal_map(error)) return fail(error);
-                mat = std::make_shared<metal>(color(r, g, b), roughness);
+                mat = measured ? std::make_shared<metal>(ior, roughness) : std::make_shared<metal>(color(r, g, b), roughness);
+            } else if (type == "thin_glass") {  // scene.h:810-823
+                const char* usage = "expected 'material <name> thin_glass <index of refraction> [color <r> <g> <b>]'";
+                double ior;
+                color tint(1, 1, 1);
+                std::string key;
+                if (!(in >> ior) || ior <= 0) return fail(usage);
+                if (in >> key) {
+                    double r, g, b;
+                    if (key != "color" || !(in >> r >> g >> b) || r < 0 || g < 0 || b < 0 || r > 1 || g > 1 || b > 1)
+                        return fail(usage);
+                    tint = color(r, g, b);
+                }
+                mat = std::make_shared<thin_dielectric>(ior, tint);
             } else if (type == "dielectric") {
                 const char* usage = "expected 'material <name> dielectric <ior> [roughness <r>] [color <r> <g> <b> [distance <d>]]'";
                 double ior, roughness = 0, distance = 1;
@@ -1609,6 +1681,8 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
                 std::shared_ptr<texture> base = std::make_shared<solid_color>(color(0.8, 0.8, 0.8));
                 std::shared_ptr<texture> metallic_map, roughness_map, base_normals;
                 double metallic = 0, roughness = 0.5, ior = 1.5;
+                bool measured = false;
+                conductor_ior ior_metal;
                 std::string key, error;
                 while (in >> key) {
                     bool ok = true;
@@ -1620,12 +1694,18 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
                     else if (key == "roughness") ok = read_value_or_texture(roughness, roughness_map, error);
                     else if (key == "ior") ok = bool(in >> ior) && ior > 0;
                     else if (key == "normal_map") ok = read_texture_name(base_normals, error);
-                    else return fail("principled setting '" + key + "' is not modelled by the GPU path");
+                    else if (key == "conductor") {
+                        std::string metal_name;
+                        ok = bool(in >> metal_name) && (measured = find_conductor(metal_name, ior_metal));
+                        if (!ok) error = "unknown conductor '" + metal_name + "' (known: Al Ti Cu Au Ag Cr)";
+                    } else return fail("principled setting '" + key + "' is not modelled by the GPU path");
                     if (!ok) return fail(error.empty() ? "bad value for principled setting '" + key + "'" : error);
                 }
                 auto pm = std::make_shared<principled>(base, metallic, roughness, ior);
                 pm->metallic_map = metallic_map;
                 pm->roughness_map = roughness_map;
+                pm->measured = measured;
+                pm->conductor = ior_metal;
                 if (use_normal_maps) pm->normal_map = base_normals;  // principled bends its own normals
                 mat = pm;
             } else if (type == "volume") {
@@ -1792,6 +1872,16 @@ extern "C" int32_t mpt_scene_describe(const char* path, mpt_scene_desc* desc, mp
 }
 
 extern "C" void mpt_desc_free(void* holder) { delete static_cast<mpt::flat_builder*>(holder); }
+
+extern "C" int32_t mpt_find_conductor(const char* name, double* eta, double* k) {
+    mpt::conductor_ior ior;
+    if (!name || !eta || !k || !mpt::find_conductor(name, ior)) return MPT_E_INVALID;
+    for (int c = 0; c < 3; c++) {
+        eta[c] = ior.eta[c];
+        k[c] = ior.k[c];
+    }
+    return MPT_OK;
+}
 
 extern "C" int32_t mpt_scene_display(const char* path, mpt_display* display) {
     if (!path || !display) return mpt::set_error(MPT_E_INVALID, "null argument");

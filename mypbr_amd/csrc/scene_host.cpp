@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <map>
 #include <utility>
 
@@ -240,6 +241,34 @@ void build_ggx_table(std::vector<double>& e, std::vector<double>& e_avg) {  // m
     }
 }
 
+namespace {
+// fresnel_conductor (microfacet.h:57-66): unpolarised Fresnel reflectance of a
+// complex index eta + i k, in std::complex arithmetic as the reference's.
+double conductor_fresnel(double cos_i, double eta, double k) {
+    cos_i = std::clamp(cos_i, 0.0, 1.0);
+    std::complex<double> n(eta, k);
+    std::complex<double> sin2_t = (1 - cos_i * cos_i) / (n * n);
+    std::complex<double> cos_t = std::sqrt(1.0 - sin2_t);
+    std::complex<double> r_parl = (n * cos_i - cos_t) / (n * cos_i + cos_t);
+    std::complex<double> r_perp = (cos_i - n * cos_t) / (cos_i + n * cos_t);
+    return (std::norm(r_parl) + std::norm(r_perp)) / 2;
+}
+}  // namespace
+
+void conductor_averages(mpt_material& m) {
+    auto at = [&](double c, int ch) { return conductor_fresnel(c, m.eta[ch], m.k[ch]); };
+    const int n = 512;  // midpoint rule (F is smooth)
+    double sum[3] = {0, 0, 0};
+    for (int i = 0; i < n; i++) {
+        const double mu = (i + 0.5) / n;
+        for (int c = 0; c < 3; c++) sum[c] = sum[c] + at(mu, c) * mu;
+    }
+    for (int c = 0; c < 3; c++) {
+        if (m.type == MPT_MAT_METAL) m.albedo[c] = at(1, c);
+        m.f_avg[c] = (1 / double(n)) * (2 * sum[c]);
+    }
+}
+
 int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
     auto fail = [&](int code, const std::string& m) {
         err = m;
@@ -279,7 +308,10 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         s.srgb[size_t(k)] = encoded <= 0.04045 ? encoded / 12.92 : std::pow((encoded + 0.055) / 1.055, 2.4);
     }
     s.materials.assign(d.materials, d.materials + d.n_materials);
+    for (auto& m : s.materials)
+        if (m.measured) conductor_averages(m);
     s.uses_uv = false;
+    s.rich = false;
     auto tex_ok = [&](int32_t t) { return t >= -1 && t < d.n_textures; };
     auto is_image = [&](int32_t t) { return t >= 0 && s.textures[size_t(t)].type == MPT_TEX_IMAGE; };
     for (size_t k = 0; k < s.materials.size(); k++) {
@@ -300,6 +332,11 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         m.uses_uv = (m.normal_map >= 0 || (needs_tex && is_image(m.texture)) || is_image(m.metallic_texture) ||
                      is_image(m.roughness_texture)) ? 1 : 0;
         if (m.uses_uv) s.uses_uv = true;
+        if (m.thin && m.type != MPT_MAT_DIELECTRIC)
+            return fail(MPT_E_INVALID, "material " + std::to_string(k) + ": only a dielectric can be thin");
+        if (m.measured && m.type != MPT_MAT_METAL && m.type != MPT_MAT_PRINCIPLED)
+            return fail(MPT_E_INVALID, "material " + std::to_string(k) + ": only metal and principled can be measured");
+        if (m.uses_uv || m.thin || m.measured) s.rich = true;
     }
     s.xforms.resize(d.n_xforms);
     for (int k = 0; k < d.n_xforms; k++) {

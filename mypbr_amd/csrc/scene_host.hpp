@@ -99,6 +99,7 @@ struct host_scene {
     std::vector<float> texels_hdr;       // image textures' RGB floats
     std::vector<double> srgb;            // 256: linear value of each 8-bit sRGB code (texture.h:197-208)
     bool uses_uv = false;                // some material reads (u, v) or a tangent frame at its hits
+    bool rich = false;                   // uses_uv, or thin glass or a measured metal: the shading variant with them
     std::vector<light_rec> lights;
     std::vector<double> light_cdf;       // n_lights + 1
     // mesh lights (mesh.h:175-216): per light mesh, its triangles in the
@@ -121,6 +122,10 @@ struct host_scene {
     std::vector<double> ggx_e_avg;       // 32
     aabb world_box;
 };
+
+// A measured metal's head-on color (metal) and hemispherical average Fresnel
+// from its index (metal_fresnel::measured, microfacet.h:109-121).
+void conductor_averages(mpt_material& m);
 
 // Validates the description and builds the host scene. Returns MPT_OK or an
 // error code with a message in err.

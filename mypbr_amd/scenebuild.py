@@ -67,13 +67,35 @@ class DescBuilder:
         tex = texture if texture is not None else self.texture_solid(rgb)
         return self._add_mat(self._mat(abi.MAT_LAMBERTIAN, tex))
 
-    def metal(self, rgb, roughness):
+    def metal(self, rgb, roughness, conductor=None):
+        """conductor: a metal's name (Al, Ti, Cu, Au, Ag, Cr) for a measured
+        metal; rgb is then unused, the scene build derives the head-on color
+        from the index."""
         m = self._mat(abi.MAT_METAL)
         r = min(max(roughness, 0.0), 1.0)
         m.roughness, m.alpha = r, roughness_to_alpha(r)
         for k in range(3):
             m.albedo[k] = rgb[k]
             m.f_avg[k] = rgb[k] + (1 / 21) * (1 - rgb[k])  # f0 + (1 - f0) / 21, microfacet.h:102-107
+        if conductor is not None:
+            self._measure(m, conductor)
+        return self._add_mat(m)
+
+    @staticmethod
+    def _measure(m, conductor):
+        from . import lib  # the metal's index from the core's table (find_conductor)
+        eta, k = (C.c_double * 3)(), (C.c_double * 3)()
+        if lib.mpt_find_conductor(conductor.encode(), eta, k) != 0:
+            raise ValueError(f"unknown conductor {conductor!r}")
+        m.measured = 1
+        for c in range(3):
+            m.eta[c], m.k[c] = eta[c], k[c]
+
+    def thin_glass(self, ior, tint=(1, 1, 1)):
+        m = self._mat(abi.MAT_DIELECTRIC)
+        m.ior, m.thin = ior, 1
+        for k in range(3):
+            m.albedo[k] = tint[k]
         return self._add_mat(m)
 
     def dielectric(self, ior, roughness=0.0, sigma_a=(0, 0, 0)):
@@ -84,7 +106,7 @@ class DescBuilder:
             m.sigma_a[k] = sigma_a[k]
         return self._add_mat(m)
 
-    def principled(self, rgb, metallic, roughness, ior=1.5, texture=None):
+    def principled(self, rgb, metallic, roughness, ior=1.5, texture=None, conductor=None):
         tex = texture if texture is not None else self.texture_solid(rgb)
         m = self._mat(abi.MAT_PRINCIPLED, tex)
         r = min(max(roughness, 0.0), 1.0)
@@ -92,6 +114,8 @@ class DescBuilder:
         m.roughness, m.alpha, m.ior = r, roughness_to_alpha(r), ior
         r0 = (ior - 1) / (ior + 1)
         m.f0 = r0 * r0
+        if conductor is not None:
+            self._measure(m, conductor)
         return self._add_mat(m)
 
     def diffuse_light(self, rgb, spread=180.0):

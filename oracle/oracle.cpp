@@ -29,6 +29,7 @@
 //   - brute force: BVH closest hit == linear scan over all primitives.
 // Built with -ffp-contract=off so its arithmetic is the reference's.
 #include <algorithm>
+#include <complex>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -347,6 +348,49 @@ color ms_fresnel_avg(const ggx_table& t, const color& f_avg, double alpha) {
     for (int k = 0; k < 3; k++) f[k] = f_avg[k] * f_avg[k] * ea / (1 - f_avg[k] * (1 - ea));
     return f;
 }
+// fresnel_conductor (microfacet.h:57-66)
+double fresnel_conductor(double cos_i, double eta, double k) {
+    cos_i = std::clamp(cos_i, 0.0, 1.0);
+    std::complex<double> n(eta, k);
+    std::complex<double> sin2_t = (1 - cos_i * cos_i) / (n * n);
+    std::complex<double> cos_t = std::sqrt(1.0 - sin2_t);
+    std::complex<double> r_parl = (n * cos_i - cos_t) / (n * cos_i + cos_t);
+    std::complex<double> r_perp = (cos_i - n * cos_t) / (cos_i + n * cos_t);
+    return (std::norm(r_parl) + std::norm(r_perp)) / 2;
+}
+
+// metal_fresnel (microfacet.h:98-140): Schlick through f0, or a measured metal
+struct metal_fresnel {
+    bool measured = false;
+    color eta, k, f0{1, 1, 1}, f_avg{1, 1, 1};
+    static metal_fresnel schlick(const color& f0) {
+        metal_fresnel f;
+        f.f0 = f0;
+        f.f_avg = f0 + (color(1, 1, 1) - f0) / 21;
+        return f;
+    }
+    static metal_fresnel from_ior(const color& eta, const color& k) {
+        metal_fresnel f;
+        f.measured = true;
+        f.eta = eta;
+        f.k = k;
+        f.f0 = f.at(1);
+        const int n = 512;
+        color sum(0, 0, 0);
+        for (int i = 0; i < n; i++) {
+            double mu = (i + 0.5) / n;
+            sum = sum + f.at(mu) * mu;
+        }
+        f.f_avg = 2 * sum / n;
+        return f;
+    }
+    color at(double c) const {
+        if (!measured) return fresnel_schlick(c, f0);
+        return color(fresnel_conductor(c, eta.x(), k.x()), fresnel_conductor(c, eta.y(), k.y()),
+                     fresnel_conductor(c, eta.z(), k.z()));
+    }
+};
+
 color ms_fresnel(const ggx_table& t, const color& f0, double alpha) {
     return ms_fresnel_avg(t, f0 + (color(1, 1, 1) - f0) / 21, alpha);
 }
@@ -551,12 +595,13 @@ struct lambertian : material {  // material.h:74-101
     }
 };
 
-struct metal : material {  // material.h:109-173 (Schlick)
+struct metal : material {  // material.h:109-173
     color albedo, f_avg;
     double roughness, alpha;
+    metal_fresnel fresnel;
     bool scatter(const ray& r, const hit_record& rec, scatter_record& s, sampler&) const override {
         if (roughness == 0) {
-            s.attenuation = albedo;
+            s.attenuation = fresnel.measured ? fresnel.at(dot(rec.normal, unit_vector(-r.dir))) : albedo;
             s.skip_pdf = true;
             s.skip_pdf_ray = ray(rec.p, reflect(unit_vector(r.dir), rec.normal));
             return true;
@@ -566,7 +611,7 @@ struct metal : material {  // material.h:109-173 (Schlick)
         s.skip_pdf = false;
         double cos_o = std::fmax(0.0, dot(rec.normal, unit_vector(-r.dir)));
         double ms_w = ms_albedo(*table, cos_o, alpha) * average(ms_fresnel_avg(*table, f_avg, alpha));
-        double ss_w = table_E(*table, cos_o, alpha) * average(fresnel_schlick(cos_o, albedo));
+        double ss_w = table_E(*table, cos_o, alpha) * average(fresnel.at(cos_o));
         if (ms_w <= 0) {
             s.pdf_ptr = std::move(spec);
             return true;
@@ -582,8 +627,28 @@ struct metal : material {  // material.h:109-173 (Schlick)
         if (wo.z() <= 0 || wi.z() <= 0) return color(0, 0, 0);
         vec3 h = unit_vector(wo + wi);
         ggx dist(alpha);
-        return fresnel_schlick(dot(wi, h), albedo) * (dist.D(h) * dist.G(wo, wi) / (4 * wo.z())) +
+        return fresnel.at(dot(wi, h)) * (dist.D(h) * dist.G(wo, wi) / (4 * wo.z())) +
                ms_fresnel_avg(*table, f_avg, alpha) * ms_eval(*table, wo.z(), wi.z(), alpha);
+    }
+};
+
+struct thin_dielectric : material {  // material.h:683-707
+    double ior;
+    color tint;
+    bool scatter(const ray& r, const hit_record& rec, scatter_record& s, sampler& smp) const override {
+        vec3 ud = unit_vector(r.dir);
+        double cos_theta = std::fmin(dot(-ud, rec.normal), 1.0);
+        double f = fresnel_dielectric(cos_theta, ior);
+        double reflectance = 2 * f / (1 + f);
+        s.skip_pdf = true;
+        if (smp.next() < reflectance) {
+            s.attenuation = color(1, 1, 1);
+            s.skip_pdf_ray = ray(rec.p, reflect(ud, rec.normal));
+        } else {
+            s.attenuation = tint;
+            s.skip_pdf_ray = ray(rec.p, ud);
+        }
+        return true;
     }
 };
 
@@ -632,6 +697,18 @@ struct principled : material {  // material.h:346-552 (base layers only)
     // metallic and roughness maps (red channel) and the base's normal map, when present
     bool has_metallic_map = false, has_roughness_map = false, has_normal_map = false;
     texture_t metallic_map, roughness_map, normal_map;
+    bool measured = false;
+    metal_fresnel conductor;  // set_conductor (material.h:381-385)
+    color specular_fresnel(double c, const color& b, double met) const {  // material.h:537-543
+        if (!measured) return fresnel_schlick(c, spec_f0(b, met));
+        double fd = fresnel_schlick(c, f0);
+        return (1 - met) * color(fd, fd, fd) + met * conductor.at(c);
+    }
+    color ms_fresnel_of(const color& b, double met, double a) const {  // material.h:545-551
+        if (!measured) return ms_fresnel(*table, spec_f0(b, met), a);
+        double fd = f0 + (1 - f0) / 21;
+        return ms_fresnel_avg(*table, (1 - met) * color(fd, fd, fd) + met * conductor.f_avg, a);
+    }
     double metallic_at(const hit_record& rec) const {  // material.h:508-515
         return has_metallic_map ? dclamp(metallic_map.value(rec.u, rec.v, rec.p).x(), 0.0, 1.0) : metallic;
     }
@@ -649,9 +726,9 @@ struct principled : material {  // material.h:346-552 (base layers only)
         double met = metallic_at(rec), a = alpha_at(rec);
         color b = base.value(rec.u, rec.v, rec.p);
         double cos_o = std::fmax(0.0, dot(n, wo));
-        double spec_w = average(fresnel_schlick(cos_o, spec_f0(b, met)));
+        double spec_w = average(specular_fresnel(cos_o, b, met));
         double diff_w = (1 - met) * average(b) * (1 - fresnel_schlick(cos_o, f0));
-        diff_w += ms_albedo(*table, cos_o, a) * average(ms_fresnel(*table, spec_f0(b, met), a));
+        diff_w += ms_albedo(*table, cos_o, a) * average(ms_fresnel_of(b, met, a));
         double p_spec = spec_w + diff_w > 0 ? spec_w / (spec_w + diff_w) : 1;
         p_spec = dclamp(p_spec, 0.1, 1.0);
         s.pdf_ptr = std::make_unique<mixture_pdf>(std::make_unique<ggx_reflection_pdf>(n, wo, a),
@@ -670,8 +747,8 @@ struct principled : material {  // material.h:346-552 (base layers only)
         color b = base.value(rec.u, rec.v, rec.p);
         vec3 h = unit_vector(wo + wi);
         ggx dist(a);
-        color specular = fresnel_schlick(dot(wi, h), spec_f0(b, met)) * (dist.D(h) * dist.G(wo, wi) / (4 * wo.z())) +
-                         ms_fresnel(*table, spec_f0(b, met), a) * ms_eval(*table, wo.z(), wi.z(), a);
+        color specular = specular_fresnel(dot(wi, h), b, met) * (dist.D(h) * dist.G(wo, wi) / (4 * wo.z())) +
+                         ms_fresnel_of(b, met, a) * ms_eval(*table, wo.z(), wi.z(), a);
         double transmitted = (1 - fresnel_schlick(wi.z(), f0)) * (1 - fresnel_schlick(wo.z(), f0));
         color diffuse = ((1 - met) * transmitted * wi.z() / pi) * b;
         return specular + diffuse;
@@ -1406,14 +1483,27 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
             }
             case MPT_MAT_METAL: {
                 auto p = std::make_unique<metal>();
-                p->albedo = color(m.albedo[0], m.albedo[1], m.albedo[2]);
-                p->f_avg = p->albedo + (color(1, 1, 1) - p->albedo) / 21;
+                if (m.measured) {  // the measured metal's own head-on color and average (material.h:117-121)
+                    p->fresnel = metal_fresnel::from_ior(color(m.eta[0], m.eta[1], m.eta[2]), color(m.k[0], m.k[1], m.k[2]));
+                    p->albedo = p->fresnel.f0;
+                } else {
+                    p->albedo = color(m.albedo[0], m.albedo[1], m.albedo[2]);
+                    p->fresnel = metal_fresnel::schlick(p->albedo);
+                }
+                p->f_avg = p->fresnel.f_avg;
                 p->roughness = m.roughness;
                 p->alpha = roughness_to_alpha(m.roughness);
                 mp = std::move(p);
                 break;
             }
             case MPT_MAT_DIELECTRIC: {
+                if (m.thin) {
+                    auto t = std::make_unique<thin_dielectric>();
+                    t->ior = m.ior;
+                    t->tint = color(m.albedo[0], m.albedo[1], m.albedo[2]);
+                    mp = std::move(t);
+                    break;
+                }
                 auto p = std::make_unique<dielectric>();
                 p->ior = m.ior;
                 p->roughness = m.roughness;
@@ -1430,6 +1520,8 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
                 if ((p->has_metallic_map = m.metallic_texture >= 0)) p->metallic_map = tex(m.metallic_texture);
                 if ((p->has_roughness_map = m.roughness_texture >= 0)) p->roughness_map = tex(m.roughness_texture);
                 if ((p->has_normal_map = m.normal_map >= 0)) p->normal_map = tex(m.normal_map);
+                if ((p->measured = m.measured != 0))
+                    p->conductor = metal_fresnel::from_ior(color(m.eta[0], m.eta[1], m.eta[2]), color(m.k[0], m.k[1], m.k[2]));
                 double r0 = (m.ior - 1) / (m.ior + 1);
                 p->f0 = r0 * r0;
                 mp = std::move(p);

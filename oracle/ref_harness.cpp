@@ -147,7 +147,8 @@ int ref_ggx_table(double* e, double* eavg) {
 
 // Material eval and scatter-pdf value for fixed directions. spec per query
 // (12 doubles): type (0 lambertian, 1 metal, 2 dielectric, 3 principled,
-// 5 phase), r, g, b, roughness, ior, metallic, g (phase), sigma_a rgb, unused.
+// 5 phase), r, g, b, roughness, ior, metallic, g (phase), sigma_a rgb, and a
+// measured metal for metal and principled (0 none, 1.. Al Ti Cu Au Ag Cr).
 // q: normal, ray direction, wi (9 doubles); front flags.
 int ref_bsdf_eval(const double* spec, const double* q, const uint8_t* front, int64_t n, double* eval, double* pdfv) {
     for (int64_t i = 0; i < n; i++) {
@@ -155,11 +156,18 @@ int ref_bsdf_eval(const double* spec, const double* q, const uint8_t* front, int
         int type = int(s[0]);
         color c(s[1], s[2], s[3]);
         std::shared_ptr<material> m;
+        static const char* metals[] = {"Al", "Ti", "Cu", "Au", "Ag", "Cr"};
+        conductor_ior ior{};
+        const int measured = int(s[11]);
+        if (measured > 0) find_conductor(metals[measured - 1], ior);
         if (type == 0) m = std::make_shared<lambertian>(c);
-        else if (type == 1) m = std::make_shared<metal>(c, s[4]);
+        else if (type == 1) m = measured ? std::make_shared<metal>(ior, s[4]) : std::make_shared<metal>(c, s[4]);
         else if (type == 2) m = std::make_shared<dielectric>(s[5], s[4], color(s[8], s[9], s[10]));
-        else if (type == 3) m = std::make_shared<principled>(std::make_shared<solid_color>(c), s[6], s[4], s[5]);
-        else m = std::make_shared<phase_material>(c, s[7]);
+        else if (type == 3) {
+            auto p = std::make_shared<principled>(std::make_shared<solid_color>(c), s[6], s[4], s[5]);
+            if (measured) p->set_conductor(ior);
+            m = p;
+        } else m = std::make_shared<phase_material>(c, s[7]);
         hit_record rec{};
         rec.normal = vec3(q[9 * i], q[9 * i + 1], q[9 * i + 2]);
         rec.p = point3(0, 0, 0);

@@ -17,6 +17,7 @@ queries) are seeded and regenerated identically by the tests:
                         their per-pixel standard error over 8 runs
   denoise.npz           denoiser::run (denoiser.h:45-193) on the synthetic frames
                         of tests/denoise_frames.py, for two settings
+  metals.npz            scenes/metals.scene (measured metals, thin glass) as below
   textured.npz          scenes/textured.scene (image textures, normal maps, texture
                         maps) as below, plus the reference's surface record at each
                         hit of the subset and stbi_load's bytes of its texture files
@@ -29,7 +30,7 @@ queries) are seeded and regenerated identically by the tests:
                         writes (display_settings::apply + to_byte, tonemap.h,
                         color3.h) on the images and settings of tests/display_cases.py
 
-Usage: python tests/golden/make_golden.py [step ...]  (steps: ggx trace bsdf render denoise display obj textures; all by default)
+Usage: python tests/golden/make_golden.py [step ...]  (steps: ggx trace bsdf render denoise display obj textures materials; all by default)
 (all steps take about a minute)
 """
 import hashlib
@@ -70,6 +71,10 @@ BSDF_MATERIALS = [
     [3, 0.5, 0.5, 0.9, 0.05, 1.45, 0.5, 0, 0, 0, 0, 0],
     [5, 0.9, 0.8, 0.7, 0, 0, 0, 0.3, 0, 0, 0, 0],
     [5, 0.6, 0.6, 0.6, 0, 0, 0, 0.0, 0, 0, 0, 0],
+    # measured metals (find_conductor): gold and aluminium, principled over chromium
+    [1, 0, 0, 0, 0.3, 0, 0, 0, 0, 0, 0, 4],
+    [1, 0, 0, 0, 0.7, 0, 0, 0, 0, 0, 0, 1],
+    [3, 0.5, 0.4, 0.3, 0.35, 1.5, 0.8, 0, 0, 0, 0, 6],
 ]
 N_PER_MATERIAL = 256
 
@@ -140,6 +145,14 @@ ASSET_SCENES = {  # scene: (width, spp per run, runs, max_depth) of the converge
 TEXTURE_SCENES = {
     "textured.scene": (48, 256, 8, 12),
 }
+MATERIAL_SCENES = {
+    "metals.scene": (48, 256, 8, 16),
+}
+
+
+def golden_materials():
+    for name, render in MATERIAL_SCENES.items():
+        golden_asset_scene(name, render)
 
 
 def golden_obj():
@@ -204,7 +217,8 @@ def main():
     if not ol.reference_available():
         sys.exit("oracle/_ref/libmpt_ref.so is not built (make -C oracle ref, where the reference is present)")
     R = ol.reference()
-    steps = set(sys.argv[1:]) or {"ggx", "trace", "bsdf", "render", "denoise", "display", "obj", "textures"}
+    steps = set(sys.argv[1:]) or {"ggx", "trace", "bsdf", "render", "denoise", "display", "obj", "textures",
+                                  "materials"}
     if "denoise" in steps:
         golden_denoise()
     if "display" in steps:
@@ -213,15 +227,27 @@ def main():
         golden_obj()
     if "textures" in steps:
         golden_textures()
-    if steps <= {"denoise", "display", "obj", "textures"}:
-        return
+    if "materials" in steps:
+        golden_materials()
+    if "ggx" in steps:
+        golden_ggx(R)
+    if "trace" in steps:
+        golden_trace()
+    if "bsdf" in steps:
+        golden_bsdf(R)
+    if "render" in steps:
+        golden_render(R)
 
+
+def golden_ggx(R):
     e = np.zeros(1024)
     ea = np.zeros(32)
     R.ref_ggx_table(ol._d(e), ol._d(ea))
     np.savez_compressed(os.path.join(HERE, "ggx_table.npz"), e=e.reshape(32, 32), e_avg=ea)
     print("ggx_table.npz", flush=True)
 
+
+def golden_trace():
     digests = {}
     probs = {}
     for name in TRACE_SCENES:
@@ -249,6 +275,8 @@ def main():
     with open(os.path.join(HERE, "light_probs.json"), "w") as f:
         json.dump(probs, f, indent=1)
 
+
+def golden_bsdf(R):
     spec, q, front = bsdf_queries()
     n = spec.shape[0]
     ev = np.zeros((n, 3))
@@ -257,6 +285,8 @@ def main():
     np.savez_compressed(os.path.join(HERE, "bsdf.npz"), spec=spec, q=q, front=front, eval=ev, pdf=pdf)
     print("bsdf.npz", n, "queries", flush=True)
 
+
+def golden_render(R):
     for name, (width, spp, runs, depth) in RENDERS.items():
         d = m.describe(scene_path(name))
         h = max(1, int(width / d.camera.aspect_ratio))

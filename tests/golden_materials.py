@@ -1,9 +1,11 @@
 """Scene descriptions holding the materials of the golden BSDF vectors.
 
 A spec row is: type (0 lambertian, 1 metal, 2 dielectric, 3 principled,
-5 phase), r, g, b, roughness, ior, metallic, g, sigma_a (3), unused - the
-parameters make_golden.py gave the reference's material constructors.
+5 phase), r, g, b, roughness, ior, metallic, g, sigma_a (3), measured metal
+(0 none, 1.. Al Ti Cu Au Ag Cr) - the parameters make_golden.py gave the
+reference's material constructors.
 """
+METALS = ["Al", "Ti", "Cu", "Au", "Ag", "Cr"]
 import numpy as np
 
 from scenebuild import DescBuilder
@@ -21,11 +23,12 @@ def desc_for_specs(spec):
             if t == 0:
                 mi = b.lambertian(rgb)
             elif t == 1:
-                mi = b.metal(rgb, row[4])
+                mi = b.metal(rgb, row[4], conductor=METALS[int(row[11]) - 1] if row[11] > 0 else None)
             elif t == 2:
                 mi = b.dielectric(row[5], row[4], tuple(row[8:11]))
             elif t == 3:
-                mi = b.principled(rgb, row[6], row[4], row[5])
+                mi = b.principled(rgb, row[6], row[4], row[5],
+                                  conductor=METALS[int(row[11]) - 1] if row[11] > 0 else None)
             else:
                 mi = b.phase(rgb, row[7])
             index[key] = mi
