@@ -186,6 +186,14 @@ typedef struct mpt_scene_desc {
      * p0, p1, p2); NaN marks a triangle without them, whose (u, v) are its
      * barycentrics b1, b2 (mesh.h:75-81). NULL: none. */
     const double* prim_uvs;
+
+    /* With has_ground set, the infinite ground y = ground_height of material
+     * ground_material (ground_plane, plane.h:10-42), tested beside the BVH
+     * (main.cpp:70-74). A query hit on it reports the canonical ID one past
+     * the last primitive. */
+    int32_t has_ground;
+    int32_t ground_material;
+    double ground_height;
 } mpt_scene_desc;
 
 /* Camera and render settings: the public fields of the reference camera

@@ -477,8 +477,20 @@ std::vector<color> postprocess(const std::vector<color>& pixels, int width, int 
 bool write_image(const std::string& path, int width, int height, const std::vector<color>& pixels,
                  const display_settings& display);
 
+// The infinite horizontal plane y = height (plane.h:10-42). It has no finite
+// box: it goes in scene::unbounded and is tested beside the BVH, not in it;
+// (u, v) = (x, -z), so textures repeat over it and normal maps work.
+class ground_plane : public hittable {
+public:
+    ground_plane(double height, std::shared_ptr<material> mat) : height(height), mat(std::move(mat)) {}
+    void flatten(flat_builder& b, int32_t xform) const override;
+    double height;
+    std::shared_ptr<material> mat;
+};
+
 struct scene {  // scene.h:32-40
     hittable_list world;
+    hittable_list unbounded;  // objects without a finite box (a ground plane)
     light_list lights;
     camera cam;
     display_settings display;  // how radiance becomes PNG/PPM pixels
@@ -505,6 +517,7 @@ public:
     int64_t add_prim(int32_t kind, int32_t mat, const double* data9);
     void set_prim_normals(int64_t prim, const double* n9);  // a triangle's three vertex normals
     void set_prim_uvs(int64_t prim, const double* uv6);     // a triangle's three texture coordinates
+    void set_ground(double height, int32_t material);       // the one ground plane
     int32_t add_object(int32_t kind, int64_t first, int64_t count, int32_t xform, const hittable* owner);
     std::map<const void*, std::pair<int64_t, int64_t>> mesh_ranges;
     struct impl;

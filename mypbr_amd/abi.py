@@ -76,6 +76,9 @@ class SceneDesc(C.Structure):
         ("texels_hdr", C.POINTER(C.c_float)),
         ("n_texels_hdr", C.c_int64),
         ("prim_uvs", C.POINTER(C.c_double)),
+        ("has_ground", C.c_int32),
+        ("ground_material", C.c_int32),
+        ("ground_height", C.c_double),
     ]
 
 

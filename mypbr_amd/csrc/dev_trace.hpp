@@ -58,7 +58,9 @@ struct dscene {
     int32_t nlights;
     int32_t lstrategy;
     int32_t env_w, env_h;
-    int32_t pad;
+    int32_t ground_mat;  // the ground plane's material, -1 without one (hits on it name object nobj)
+    double ground_h;
+    int64_t ground_id;   // its canonical ID: one past the last primitive's
     croot tlas_root;
 };
 
@@ -401,6 +403,25 @@ __device__ __forceinline__ bool object_t(const dscene& S, int oi, v3 o, v3 d, do
     return false;
 }
 
+// ground_plane::hit (plane.h:17-34), t only; the interval is exclusive. Only
+// the kernels of scenes with rich shading (F_UV) carry it.
+template <int FEAT>
+__device__ __forceinline__ bool ground_t(const dscene& S, const v3& o, const v3& d, double tmin, double tmax,
+                                         hitinfo& h) {
+    if constexpr (FEAT & F_UV) {
+        if (S.ground_mat < 0) return false;
+        const double dy = d.y;
+        if (std::fabs(dy) < 1e-12) return false;
+        const double t = (S.ground_h - o.y) / dy;
+        if (!(tmin < t && t < tmax)) return false;
+        h.t = t;
+        h.obj = S.nobj;
+        h.prim = -1;
+        return true;
+    }
+    return false;
+}
+
 // bvh::hit over the world (bvh.h:45-86). media: test constant_medium objects
 // with draws from (mkey, serial); otherwise they are skipped.
 template <bool CNT = false, int FEAT = F_ALL>
@@ -409,7 +430,7 @@ __device__ bool trace_closest(const dscene& S, const v3& o, const v3& d, double 
     h.obj = -1;
     h.prim = -1;
     h.t = tmax;
-    if (S.nobj == 0) return false;
+    if (S.nobj == 0) return ground_t<FEAT>(S, o, d, tmin, tmax, h);
     v3 inv{1 / d.x, 1 / d.y, 1 / d.z};
     const int negmask = (inv.x < 0 ? 1 : 0) | (inv.y < 0 ? 2 : 0) | (inv.z < 0 ? 4 : 0);
     bool any = false;
@@ -427,6 +448,8 @@ __device__ bool trace_closest(const dscene& S, const v3& o, const v3& d, double 
             }
         }
     });
+    // the objects beside the BVH, after it with the closest t so far (main.cpp:70-74)
+    if (ground_t<FEAT>(S, o, d, tmin, tmax, h)) any = true;
     return any;
 }
 
@@ -446,6 +469,23 @@ struct hitrec {
 
 template <int FEAT = F_ALL>
 __device__ __forceinline__ void finish_hit(const dscene& S, const hitinfo& h, const v3& wo, const v3& wd, hitrec& rec) {
+    if constexpr (FEAT & F_UV) {
+        if (h.obj == S.nobj) {  // the ground (plane.h:23-33)
+            v3 p = wo + h.t * wd;
+            p.y = S.ground_h;  // exactly on the plane
+            rec.t = h.t;
+            rec.p = p;
+            rec.u = p.x;
+            rec.v = -p.z;
+            rec.dpdu = v3{1, 0, 0};
+            rec.dpdv = v3{0, 0, -1};
+            rec.mat = S.ground_mat;
+            const v3 up{0, 1, 0};
+            rec.front = dot(wd, up) < 0;
+            rec.normal = rec.front ? up : -up;
+            return;
+        }
+    }
     const obj_rec r = S.obj[h.obj];
     rec.t = h.t;
     if (r.kind == MPT_OBJ_MEDIUM) {

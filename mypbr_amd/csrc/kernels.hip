@@ -316,7 +316,7 @@ __global__ __launch_bounds__(BS) void k_query(dscene S, launch_ctx L, const doub
         if (hit) {
             hitrec rec;
             finish_hit<FEAT>(S, h, o, d, rec);
-            out_id[i] = S.obj[h.obj].id_base + S.pcanon[h.prim];
+            out_id[i] = h.obj == S.nobj ? S.ground_id : S.obj[h.obj].id_base + S.pcanon[h.prim];
             out_t[i] = h.t;
             out_front[i] = rec.front ? 1 : 0;
         } else {
@@ -1263,6 +1263,10 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
     d.lstrategy = h.light_strategy;
     d.env_w = h.env_w;
     d.env_h = h.env_h;
+    d.ground_mat = h.ground_mat;
+    d.ground_h = h.ground_h;
+    d.ground_id = 0;  // one past the last canonical ID of the world list
+    for (const obj_rec& r : h.objects) d.ground_id = std::max(d.ground_id, r.id_base + r.count);
     // geometry profile: spheres only with object i = primitive i, quads and
     // spheres in lists and instances, or everything
     {
@@ -1346,6 +1350,9 @@ int32_t mpt_trace_closest_dev(mpt_scene* s, const double* rays, const double* tt
         hipLaunchKernelGGL((k_query<256, F_SPHERE>), dim3(s->grid), dim3(256), 0, st, s->ds, L, rays, tt, n, id, t, front);
     else if (s->profile == 2)
         hipLaunchKernelGGL((k_query<256, kQuadProfile>), dim3(s->grid), dim3(256), 0, st, s->ds, L, rays, tt, n, id, t,
+                           front);
+    else if (s->profile == 3)
+        hipLaunchKernelGGL((k_query<256, F_ALL | F_UV>), dim3(s->grid), dim3(256), 0, st, s->ds, L, rays, tt, n, id, t,
                            front);
     else
         hipLaunchKernelGGL((k_query<256, F_ALL>), dim3(s->grid), dim3(256), 0, st, s->ds, L, rays, tt, n, id, t, front);
@@ -1742,6 +1749,7 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
     } while (0)
         if (s->profile == 1) SHD(F_SPHERE);
         else if (s->profile == 2) SHD(kQuadProfile);
+        else if (s->profile == 3) SHD(F_ALL | F_UV);  // the ground occludes too
         else SHD(F_ALL);
 #undef SHD
 #undef SHD2

@@ -381,6 +381,9 @@ struct flat_builder::impl {
     std::vector<uint8_t> texels;     // image textures' RGBA bytes
     std::vector<float> texels_hdr;   // image textures' RGB floats
     std::vector<double> prim_uv;     // 6 per primitive once any triangle has texture coordinates
+    bool has_ground = false;
+    int32_t ground_mat = -1;
+    double ground_height = 0;
     std::vector<int32_t> light_object;
     std::vector<double> env_rgb;
     std::map<const material*, int32_t> mat_index;
@@ -450,6 +453,18 @@ int64_t flat_builder::add_prim(int32_t kind, int32_t mat, const double* data9) {
         p->prim_uv.insert(p->prim_uv.end(), 5, 0.0);
     }
     return int64_t(p->prim_kind.size() - 1);
+}
+
+void flat_builder::set_ground(double height, int32_t material) {
+    if (p->has_ground) throw std::runtime_error("only one ground plane is supported");
+    p->has_ground = true;
+    p->ground_height = height;
+    p->ground_mat = material;
+}
+
+void ground_plane::flatten(flat_builder& b, int32_t xform) const {
+    if (xform >= 0) throw std::runtime_error("a ground plane cannot be instanced");
+    b.set_ground(height, b.material_index(mat));
 }
 
 void flat_builder::set_prim_uvs(int64_t prim, const double* uv6) {
@@ -703,6 +718,9 @@ const mpt_scene_desc& flat_builder::desc() {
     d.texels_hdr = p->texels_hdr.empty() ? nullptr : p->texels_hdr.data();
     d.n_texels_hdr = int64_t(p->texels_hdr.size());
     d.prim_uvs = p->prim_uv.empty() ? nullptr : p->prim_uv.data();
+    d.has_ground = p->has_ground ? 1 : 0;
+    d.ground_material = p->ground_mat;
+    d.ground_height = p->ground_height;
     d.n_lights = int32_t(p->light_object.size());
     d.light_object = p->light_object.data();
     d.env_rgb = p->env_rgb.empty() ? nullptr : p->env_rgb.data();
@@ -1295,6 +1313,10 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
     std::string scene_dir = slash == std::string::npos ? "" : path.substr(0, slash + 1);
     auto default_material = std::make_shared<lambertian>(color(0.5, 0.5, 0.5));
     bool use_normal_maps = true;
+    // the ground (scene.h:437-439, 596-611), made at the end of the file
+    bool ground = false;
+    double ground_height = 0;
+    std::shared_ptr<material> ground_material = default_material;
     // meshes loaded once per (file, smooth, material, normal maps) (scene.h:1167-1192)
     std::map<std::tuple<std::string, bool, const material*, bool>, std::vector<obj_part>> mesh_cache;
 
@@ -1521,6 +1543,20 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
             std::string name;
             if (!(in >> name) || !result.display.set_tonemapper(name))
                 return fail("expected 'tonemap none|reinhard|aces'");
+        } else if (keyword == "ground") {  // scene.h:596-611
+            std::string key;
+            ground = true;
+            while (in >> key) {
+                if (key == "off") {
+                    ground = false;
+                } else if (key == "height") {
+                    if (!(in >> ground_height)) return fail("expected 'ground [height <y>] [material]'");
+                } else {
+                    auto it = materials.find(key);
+                    if (it == materials.end()) return fail("unknown material '" + key + "'");
+                    ground_material = it->second;
+                }
+            }
         } else if (keyword == "normal_maps") {  // scene.h:689-693
             std::string state;
             if (!(in >> state) || (state != "on" && state != "off")) return fail("expected 'normal_maps on|off'");
@@ -1798,6 +1834,7 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
     }
 
     if (world.empty()) return fail("the scene has no objects");
+    if (ground) result.unbounded.add(std::make_shared<ground_plane>(ground_height, ground_material));  // scene.h:1237-1238
     if (sky_uniform) {
         if (cam.environment) return fail("'sky uniform' and 'environment' can't be used together");
         cam.environment = std::make_shared<environment_light>(8, 4, std::vector<color>(32, sky_radiance));
@@ -1841,6 +1878,7 @@ extern "C" int32_t mpt_scene_load(const char* path, mpt_scene** out, mpt_camera*
     try {
         mpt::flat_builder b;
         b.add_world(sc.world);
+        b.add_world(sc.unbounded);
         b.add_lights(sc.lights, sc.cam.environment);
         int rc = mpt_scene_create(&b.desc(), out);
         if (rc != MPT_OK) return rc;
@@ -1860,6 +1898,7 @@ extern "C" int32_t mpt_scene_describe(const char* path, mpt_scene_desc* desc, mp
     auto* b = new mpt::flat_builder;
     try {
         b->add_world(sc.world);
+        b->add_world(sc.unbounded);
         b->add_lights(sc.lights, sc.cam.environment);
         *desc = b->desc();
     } catch (const std::exception& e) {

@@ -312,6 +312,15 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         if (m.measured) conductor_averages(m);
     s.uses_uv = false;
     s.rich = false;
+    s.ground_mat = -1;
+    if (d.has_ground) {
+        if (d.ground_material < 0 || d.ground_material >= d.n_materials)
+            return fail(MPT_E_INVALID, "the ground plane's material does not exist");
+        if (!std::isfinite(d.ground_height)) return fail(MPT_E_INVALID, "the ground plane's height is not finite");
+        s.ground_mat = d.ground_material;
+        s.ground_h = d.ground_height;
+        s.rich = true;  // the ground is tested in the kernels of the variant that also carries (u, v)
+    }
     auto tex_ok = [&](int32_t t) { return t >= -1 && t < d.n_textures; };
     auto is_image = [&](int32_t t) { return t >= 0 && s.textures[size_t(t)].type == MPT_TEX_IMAGE; };
     for (size_t k = 0; k < s.materials.size(); k++) {

@@ -99,7 +99,9 @@ struct host_scene {
     std::vector<float> texels_hdr;       // image textures' RGB floats
     std::vector<double> srgb;            // 256: linear value of each 8-bit sRGB code (texture.h:197-208)
     bool uses_uv = false;                // some material reads (u, v) or a tangent frame at its hits
-    bool rich = false;                   // uses_uv, or thin glass or a measured metal: the shading variant with them
+    bool rich = false;                   // uses_uv, thin glass, a measured metal or the ground: the variant with them
+    int32_t ground_mat = -1;             // the ground plane's material, -1 without one
+    double ground_h = 0;
     std::vector<light_rec> lights;
     std::vector<double> light_cdf;       // n_lights + 1
     // mesh lights (mesh.h:175-216): per light mesh, its triangles in the

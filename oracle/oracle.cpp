@@ -1403,6 +1403,31 @@ struct environment : hittable {  // environment.h
     }
 };
 
+// The infinite ground y = height (plane.h:10-42).
+struct ground_plane : hittable {
+    double height;
+    const material* mat;
+    int64_t id;
+    bool hit(const ray& r, interval t, hit_record& rec, const hit_ctx&) const override {
+        double dy = r.dir.y();
+        if (std::fabs(dy) < 1e-12) return false;
+        double tt = (height - r.orig.y()) / dy;
+        if (!t.surrounds(tt)) return false;
+        rec.t = tt;
+        rec.p = r.at(tt);
+        rec.p[1] = height;  // exactly on the plane
+        rec.u = rec.p.x();
+        rec.v = -rec.p.z();
+        rec.dpdu = vec3(1, 0, 0);
+        rec.dpdv = vec3(0, 0, -1);
+        rec.mat = mat;
+        rec.id = id;
+        rec.set_face_normal(r, vec3(0, 1, 0));
+        return true;
+    }
+    aabb bounding_box() const override { return aabb(); }  // never in a BVH
+};
+
 // ------------------------------------------------------------------ scene
 struct scene {
     std::vector<std::unique_ptr<material>> mats;
@@ -1415,9 +1440,23 @@ struct scene {
     std::unique_ptr<environment> env;
     ggx_table table;
     aabb world_box;
+    const hittable* ground = nullptr;  // beside the BVH (scene::unbounded)
     std::vector<uint8_t> texels;    // image textures, copied from the description
     std::vector<float> texels_hdr;
     std::vector<double> srgb;       // image_texture::srgb_table (texture.h:197-208)
+
+    // the BVH, then the objects beside it, as one hittable_list (main.cpp:70-74)
+    bool hit(const ray& r, interval t, hit_record& rec, const hit_ctx& c) const {
+        bool any = tree->hit(r, t, rec, c);
+        if (ground) {
+            hit_record g;
+            if (ground->hit(r, interval(t.min, any ? rec.t : t.max), g, c)) {
+                rec = g;
+                any = true;
+            }
+        }
+        return any;
+    }
 
     double lights_pdf(const point3& o, const vec3& d) const {
         if (lights.empty()) return 0.0;
@@ -1674,6 +1713,14 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
         s.world_box = aabb(s.world_box, h->bounding_box());
     }
     s.tree = std::make_unique<bvh>(s.world);
+    if (d.has_ground) {
+        auto g = std::make_unique<ground_plane>();
+        g->height = d.ground_height;
+        g->mat = matp(d.ground_material);
+        g->id = 0;  // one past the last canonical ID
+        for (int o = 0; o < d.n_objects; o++) g->id = std::max(g->id, d.object_id_base[o] + d.object_count[o]);
+        s.ground = own(s, std::move(g));
+    }
 
     if (d.env_width > 0) {
         s.env = std::make_unique<environment>();
@@ -1838,7 +1885,7 @@ color sample_light(const scene& S, const cam& C, const ray& r_in, const hit_reco
         hit_record lr;
         cnt.shadow++;
         hit_ctx hc{s.key, uint32_t(2 * depth + 1) | (uint32_t(k) << 20), true};
-        if (!S.tree->hit(shadow, interval(0.001, infinity), lr, hc)) {
+        if (!S.hit(shadow, interval(0.001, infinity), lr, hc)) {
             emission = escaped(S, C, to_light);
             break;
         }
@@ -1866,7 +1913,7 @@ color ray_color(const scene& S, const cam& C, const ray& camera_ray, sampler& s,
         hit_record rec;
         cnt.extension++;
         hit_ctx hc{s.key, uint32_t(2 * depth), true};
-        if (!S.tree->hit(r, interval(0.001, infinity), rec, hc)) {
+        if (!S.hit(r, interval(0.001, infinity), rec, hc)) {
             color sky = escaped(S, C, r.dir);
             if (seen) break;
             if (specular || no_lights) radiance = radiance + beta * sky;
@@ -1921,7 +1968,7 @@ void first_surface(const scene& S, const ray& cr, uint64_t key, color& albedo, c
     for (int b = 0; b < 8; b++) {
         hit_record rec;
         hit_ctx hc{s.key, uint32_t(b), true};
-        if (!S.tree->hit(r, interval(0.001, infinity), rec, hc)) break;
+        if (!S.hit(r, interval(0.001, infinity), rec, hc)) break;
         tint = tint * rec.mat->interior_transmittance(r, rec);
         scatter_record srec;
         if (!rec.mat->scatter(r, rec, srec, s)) {
@@ -2243,7 +2290,7 @@ int32_t orc_trace(orc_scene* S, const double* rays, const double* tt, int64_t n,
             ray r(point3(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]), vec3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]));
             hit_record rec;
             hit_ctx hc{0, 0, false};
-            if (S->s.tree->hit(r, interval(tt[2 * i], tt[2 * i + 1]), rec, hc)) {
+            if (S->s.hit(r, interval(tt[2 * i], tt[2 * i + 1]), rec, hc)) {
                 id[i] = rec.id;
                 t[i] = rec.t;
                 front[i] = rec.front_face;
@@ -2273,7 +2320,7 @@ int32_t orc_surface(orc_scene* S, const double* rays, const double* tt, int64_t 
         ray r(point3(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]), vec3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]));
         hit_record rec;
         hit_ctx hc{0, 0, false};
-        if (!S->s.tree->hit(r, interval(tt[2 * i], tt[2 * i + 1]), rec, hc)) continue;
+        if (!S->s.hit(r, interval(tt[2 * i], tt[2 * i + 1]), rec, hc)) continue;
         const material* m = rec.mat;
         vec3 shading = rec.normal;
         if (auto nm = dynamic_cast<const normal_mapped*>(m)) {
@@ -2306,7 +2353,9 @@ int32_t orc_trace_linear(orc_scene* S, const double* rays, const double* tt, int
         bool any = false;
         double closest = tt[2 * i + 1];
         hit_ctx hc{0, 0, false};
-        for (auto* o : S->s.world) {
+        std::vector<const hittable*> all(S->s.world);
+        if (S->s.ground) all.push_back(S->s.ground);
+        for (auto* o : all) {
             hit_record rec;
             if (o->hit(r, interval(tt[2 * i], closest), rec, hc)) {
                 any = true;
@@ -2441,7 +2490,7 @@ int32_t orc_make_ray_batch(orc_scene* S, const mpt_camera* c, uint64_t seed, int
         hit_record rec;
         hit_ctx hc{0, 0, false};
         ray r = cr;
-        if (S->s.tree->hit(cr, interval(0.001, infinity), rec, hc)) {
+        if (S->s.hit(cr, interval(0.001, infinity), rec, hc)) {
             cosine_pdf cp(rec.normal);
             r = ray(rec.p, cp.generate(s));
         }

@@ -45,7 +45,12 @@ struct canon_prim {
 struct ref_scene {
     scene sc;
     std::shared_ptr<bvh> tree;
+    std::shared_ptr<hittable_list> everything;  // the BVH and the unbounded objects (main.cpp:70-74)
     std::vector<canon_prim> canon;
+    const hittable& world() const {
+        if (sc.unbounded.empty()) return *tree;
+        return *everything;
+    }
 };
 
 void expand(const hittable* h, const instance* inst, bool medium, std::vector<canon_prim>& out) {
@@ -74,7 +79,10 @@ int ref_load(const char* path, void** out) {
         return -1;
     }
     r->tree = std::make_shared<bvh>(r->sc.world);  // main.cpp:70
+    r->everything = std::make_shared<hittable_list>(r->tree);
+    for (const auto& o : r->sc.unbounded.objects) r->everything->add(o);
     for (const auto& o : r->sc.world.objects) expand(o.get(), nullptr, false, r->canon);
+    for (const auto& o : r->sc.unbounded.objects) expand(o.get(), nullptr, false, r->canon);  // IDs after the world's
     *out = r;
     return 0;
 }
@@ -94,7 +102,7 @@ int ref_trace(void* h, const double* rays, const double* tt, int64_t n, int64_t*
             ray rr(point3(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]), vec3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]));
             interval iv(tt[2 * i], tt[2 * i + 1]);
             hit_record rec;
-            if (!r->tree->hit(rr, iv, rec)) {
+            if (!r->world().hit(rr, iv, rec)) {
                 id[i] = -1, t[i] = 0, front[i] = 0, ties[i] = 0, tie_min[i] = -1, tie_max[i] = -1;
                 continue;
             }
@@ -317,7 +325,7 @@ int ref_surface(void* h, const double* rays, const double* tt, int64_t n, double
         for (int k = 0; k < 16; k++) o[k] = std::nan("");
         ray rr(point3(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]), vec3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]));
         hit_record rec;
-        if (!r->tree->hit(rr, interval(tt[2 * i], tt[2 * i + 1]), rec)) continue;
+        if (!r->world().hit(rr, interval(tt[2 * i], tt[2 * i + 1]), rec)) continue;
         const material* m = rec.mat;
         vec3 shading = rec.normal;
         if (auto nm = dynamic_cast<const normal_mapped*>(m)) {

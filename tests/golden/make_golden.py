@@ -18,6 +18,7 @@ queries) are seeded and regenerated identically by the tests:
   denoise.npz           denoiser::run (denoiser.h:45-193) on the synthetic frames
                         of tests/denoise_frames.py, for two settings
   metals.npz            scenes/metals.scene (measured metals, thin glass) as below
+  ground.npz            scenes/ground.scene (the ground plane beside the BVH) as textured.npz
   textured.npz          scenes/textured.scene (image textures, normal maps, texture
                         maps) as below, plus the reference's surface record at each
                         hit of the subset and stbi_load's bytes of its texture files
@@ -144,6 +145,7 @@ ASSET_SCENES = {  # scene: (width, spp per run, runs, max_depth) of the converge
 }
 TEXTURE_SCENES = {
     "textured.scene": (48, 256, 8, 12),
+    "ground.scene": (48, 256, 8, 12),
 }
 MATERIAL_SCENES = {
     "metals.scene": (48, 256, 8, 16),
