@@ -154,7 +154,8 @@ typedef struct mpt_scene_desc {
     const mpt_texture* textures;
 
     /* Lights for next event estimation, in light-list order
-     * (light_list.h:33-134): an object index, or -1 for the environment. */
+     * (light_list.h:33-134): an object index, -1 for the environment or -2
+     * for the sun. */
     int32_t n_lights;
     const int32_t* light_object;
     int32_t light_strategy;        /* 0 = uniform, 1 = power (light_list.h:35-38), 2 = a plain
@@ -194,6 +195,17 @@ typedef struct mpt_scene_desc {
     int32_t has_ground;
     int32_t ground_material;
     double ground_height;
+
+    /* With has_sun set, a distant light of small angular size (sun_light,
+     * sun.h:23-80): radiance irradiance / omega inside the cone of half-angle
+     * sun_angle_degrees / 2 around sun_direction, added to every escaping ray
+     * (camera.h:526-527) and sampled uniformly in the cone by next event
+     * estimation when a light_object entry is -2. */
+    int32_t has_sun;
+    int32_t sun_pad;
+    double sun_direction[3];
+    double sun_angle_degrees;      /* angular diameter, clamped below at 1e-3 (sun.h:29) */
+    double sun_irradiance[3];
 } mpt_scene_desc;
 
 /* Camera and render settings: the public fields of the reference camera
@@ -214,7 +226,8 @@ typedef struct mpt_camera {
     double defocus_angle;
     double focus_dist;
     int32_t blades;
-    int32_t pad;
+    int32_t sun_disk_hidden;       /* !sun_disk_visible (camera.h:46): camera rays do not see the
+                                      sun itself; zero keeps the reference default */
     double blade_rotation;
 } mpt_camera;
 

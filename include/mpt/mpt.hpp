@@ -362,6 +362,19 @@ public:
     double intensity, rotation_degrees;
 };
 
+// A distant light of small angular size (sun_light, sun.h:23-80): never hit,
+// added to escaping rays, sampled uniformly inside its cone. angular_diameter
+// in degrees; irradiance on a surface facing it.
+class sun_light : public hittable {
+public:
+    sun_light(const vec3& direction, double angular_diameter, const color& irradiance)
+        : direction(direction), angular_diameter(angular_diameter), irradiance(irradiance) {}
+    void flatten(flat_builder& b, int32_t xform) const override;
+    vec3 direction;
+    double angular_diameter;
+    color irradiance;
+};
+
 // light_list (light_list.h:33-134): the selection probabilities are computed
 // by the core on upload from this strategy; build() only records the bounds
 // flag, as the core recomputes the scene radius itself.
@@ -395,6 +408,8 @@ public:
     bool use_sky = true;
     color background = color(0, 0, 0);
     std::shared_ptr<environment_light> environment;
+    std::shared_ptr<sun_light> sun;  // added to escaping rays; must also be in the lights (camera.h:45)
+    bool sun_disk_visible = true;    // whether camera rays see the sun itself (camera.h:46)
     double ortho_height = 0;
     double defocus_angle = 0;
     double focus_dist = 10;

@@ -79,6 +79,11 @@ class SceneDesc(C.Structure):
         ("has_ground", C.c_int32),
         ("ground_material", C.c_int32),
         ("ground_height", C.c_double),
+        ("has_sun", C.c_int32),
+        ("sun_pad", C.c_int32),
+        ("sun_direction", D3),
+        ("sun_angle_degrees", C.c_double),
+        ("sun_irradiance", D3),
     ]
 
 
@@ -101,7 +106,7 @@ class Camera(C.Structure):
         ("defocus_angle", C.c_double),
         ("focus_dist", C.c_double),
         ("blades", C.c_int32),
-        ("pad", C.c_int32),
+        ("sun_disk_hidden", C.c_int32),
         ("blade_rotation", C.c_double),
     ]
 

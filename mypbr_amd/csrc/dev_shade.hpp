@@ -659,6 +659,25 @@ __device__ v3 env_random(const dscene& S, rstream& rs) {  // environment.h:94-10
     return v3{st * sp, ct, -st * cp};
 }
 
+// ---------------------------------------------------------------- sun
+__device__ __forceinline__ bool sun_in_cone(const dscene& S, const v3& dir) {  // sun.h:76-78
+    return dot(unit_vector(dir), S.sun_frame.w) >= S.sun_cos_max;
+}
+__device__ __forceinline__ v3 sun_radiance(const dscene& S, const v3& dir) {  // sun.h:46-48
+    return sun_in_cone(S, dir) ? S.sun_radiance : v3{0, 0, 0};
+}
+__device__ __forceinline__ double sun_pdf(const dscene& S, const v3& dir) {  // sun.h:50-53
+    return sun_in_cone(S, dir) ? 1 / S.sun_solid_angle : 0;
+}
+__device__ __forceinline__ v3 sun_random(const dscene& S, rstream& rs) {  // sun.h:56-62: cos(theta) uniform in the cone
+    double cos_theta = 1 - rs.next() * (1 - S.sun_cos_max);
+    double sin_theta = std::sqrt(std::fmax(0.0, 1 - cos_theta * cos_theta));
+    double phi = 2 * kPi * rs.next();
+    double sp, cp;
+    sin_cos(phi, sp, cp);
+    return onb_transform(S.sun_frame, v3{cp * sin_theta, sp * sin_theta, cos_theta});
+}
+
 // ---------------------------------------------------------------- lights
 // Light kinds a scene's light list holds; the NEE code is compiled for the
 // kinds present (LK bit set), like the traversal for geometry.
@@ -835,7 +854,7 @@ __device__ __forceinline__ double lights_pdf(const dscene& S, const v3& o, const
         double pk;
         if constexpr (LK == L_ENV) pk = env_pdf(S, d);
         else if constexpr (!(LK & L_ENV)) pk = object_light_pdf<LK>(S, L.object, o, d);
-        else pk = L.object < 0 ? env_pdf(S, d) : object_light_pdf<LK>(S, L.object, o, d);
+        else pk = L.object == -2 ? sun_pdf(S, d) : (L.object < 0 ? env_pdf(S, d) : object_light_pdf<LK>(S, L.object, o, d));
         sum += L.probability * pk;
     }
     return sum;
@@ -866,6 +885,7 @@ __device__ __forceinline__ v3 lights_random(const dscene& S, const v3& o, rstrea
     if constexpr (LK == L_ENV) return env_random(S, rs);
     else if constexpr (!(LK & L_ENV)) return object_light_random<LK>(S, L.object, o, rs);
     else {
+        if (L.object == -2) return sun_random(S, rs);
         if (L.object < 0) return env_random(S, rs);
         return object_light_random<LK>(S, L.object, o, rs);
     }

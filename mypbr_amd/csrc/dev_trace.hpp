@@ -62,6 +62,11 @@ struct dscene {
     double ground_h;
     int64_t ground_id;   // its canonical ID: one past the last primitive's
     croot tlas_root;
+    // the sun (sun.h): on, its frame (w towards the sun), cone and radiance
+    int32_t has_sun;
+    onb sun_frame;
+    double sun_cos_max, sun_solid_angle;
+    v3 sun_radiance;
 };
 
 // Per-lane traversal stack: LDS for the first kLdsDepth entries, then HBM.

@@ -61,6 +61,7 @@ struct dcam {
     int w, h, spp, max_depth;
     int max_diffuse, max_specular, max_scatter, transparent_shadows;
     int sqrt_spp, blades, sky_mode, features;  // sky_mode: 0 env, 1 gradient, 2 constant
+    int sun_disk_hidden, pad_;                 // camera rays do not see the sun itself (camera.h:46)
     double recip_sqrt_spp;
     v3 center, pixel00, du, dv, w_axis, disk_u, disk_v, background;
     double ortho_height, defocus_angle, focus_dist, blade_rotation;
@@ -83,6 +84,7 @@ static dcam init_camera(const mpt_camera& c, bool has_env) {
     k.blades = c.blades;
     k.features = c.collect_features;
     k.sky_mode = has_env ? 0 : (c.use_sky ? 1 : 2);
+    k.sun_disk_hidden = c.sun_disk_hidden != 0;
     k.background = v3{c.background[0], c.background[1], c.background[2]};
     k.ortho_height = c.ortho_height;
     k.defocus_angle = c.defocus_angle;
@@ -113,15 +115,22 @@ static dcam init_camera(const mpt_camera& c, bool has_env) {
 }
 
 // ENV: the scene has an environment map (sky mode 0); without one only the
-// gradient and constant skies are compiled in.
+// gradient and constant skies are compiled in. The sun is added to every
+// escaping ray, to camera rays (depth 0) only while its disk is visible.
 template <bool ENV = true>
-__device__ __forceinline__ v3 escaped(const dscene& S, const dcam& C, const v3& dir) {  // camera.h:515-529 (no sun)
-    if constexpr (ENV)
-        if (C.sky_mode == 0) return env_radiance(S, dir);
-    if (C.sky_mode == 2) return C.background;
-    v3 ud = unit_vector(dir);
-    double a = 0.5 * (ud.y + 1.0);
-    return (1.0 - a) * v3{1.0, 1.0, 1.0} + a * v3{0.5, 0.7, 1.0};
+__device__ __forceinline__ v3 escaped(const dscene& S, const dcam& C, const v3& dir, bool camera_ray) {  // camera.h:515-529
+    v3 sky;
+    if (ENV && C.sky_mode == 0) {
+        sky = env_radiance(S, dir);
+    } else if (C.sky_mode == 2) {
+        sky = C.background;
+    } else {
+        v3 ud = unit_vector(dir);
+        double a = 0.5 * (ud.y + 1.0);
+        sky = (1.0 - a) * v3{1.0, 1.0, 1.0} + a * v3{0.5, 0.7, 1.0};
+    }
+    if (S.has_sun && (!C.sun_disk_hidden || !camera_ray)) sky = sky + sun_radiance(S, dir);
+    return sky;
 }
 
 __device__ __forceinline__ double power_heuristic(double a, double b) {  // camera.h:343-347
@@ -591,7 +600,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
     bool specular = fl & 1u, seen = fl & 2u;
     int dcount = (fl >> 2) & 1023, scount = (fl >> 12) & 1023, vcount = (fl >> 22) & 1023;
     if constexpr (MT < 0) {  // camera.h:373-387
-        v3 sky = escaped<ENV>(S, C, d);
+        v3 sky = escaped<ENV>(S, C, d, depth == 0);
         if (!seen) {
             if (!LIGHTS || specular) {
                 rad = rad + beta * sky;
@@ -751,7 +760,7 @@ __device__ __forceinline__ void shadow_one(const dscene& S, const dcam& C, const
             hitinfo h;
             if (!trace_closest<CNT, FEAT>(S, o, dir, 0.001, kInf, h, st, true, P.key[p],
                                     uint32_t(2 * depth + 1) | (uint32_t(k) << 20), wc)) {
-                emission = escaped<ENV>(S, C, to_light);
+                emission = escaped<ENV>(S, C, to_light, false);
                 break;
             }
             hitrec rec;
@@ -1265,6 +1274,11 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
     d.env_h = h.env_h;
     d.ground_mat = h.ground_mat;
     d.ground_h = h.ground_h;
+    d.has_sun = h.has_sun;
+    d.sun_frame = h.sun_frame;
+    d.sun_cos_max = h.sun_cos_max;
+    d.sun_solid_angle = h.sun_solid_angle;
+    d.sun_radiance = h.sun_radiance;
     d.ground_id = 0;  // one past the last canonical ID of the world list
     for (const obj_rec& r : h.objects) d.ground_id = std::max(d.ground_id, r.id_base + r.count);
     // geometry profile: spheres only with object i = primitive i, quads and
@@ -1289,6 +1303,10 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
         s->has_lights = !h.lights.empty();
         bool quads = true, env_only = true;
         for (const auto& L : h.lights) {
+            if (L.object == -2) {  // the sun: only the general light code samples it
+                quads = env_only = false;
+                continue;
+            }
             if (L.object < 0) {
                 quads = false;
                 continue;

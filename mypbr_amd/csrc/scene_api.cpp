@@ -648,6 +648,10 @@ void environment_light::flatten(flat_builder&, int32_t) const {
     // never hit (environment.h:53-56): not part of the world's geometry
 }
 
+void sun_light::flatten(flat_builder&, int32_t) const {
+    // never hit (sun.h:38-41): not part of the world's geometry
+}
+
 void flat_builder::add_world(const hittable& world) {
     if (auto l = dynamic_cast<const hittable_list*>(&world)) {
         for (const auto& o : l->objects) o->flatten(*this, -1);
@@ -663,6 +667,17 @@ void flat_builder::add_lights(const hittable_list& lights, const std::shared_ptr
             if (envp && envp != e) throw std::runtime_error("only one environment is allowed");
             envp = e;
             p->light_object.push_back(-1);
+            continue;
+        }
+        if (auto sun = dynamic_cast<const sun_light*>(o.get())) {
+            if (p->d.has_sun) throw std::runtime_error("only one sun is allowed");
+            p->d.has_sun = 1;
+            for (int k = 0; k < 3; k++) {
+                p->d.sun_direction[k] = sun->direction[k];
+                p->d.sun_irradiance[k] = sun->irradiance[k];
+            }
+            p->d.sun_angle_degrees = sun->angular_diameter;
+            p->light_object.push_back(-2);
             continue;
         }
         auto it = p->object_of.find(o.get());
@@ -752,6 +767,7 @@ mpt_camera camera::to_abi() const {
     c.defocus_angle = defocus_angle;
     c.focus_dist = focus_dist;
     c.blades = blades;
+    c.sun_disk_hidden = !sun_disk_visible;
     c.blade_rotation = blade_rotation;
     return c;
 }
@@ -1423,6 +1439,10 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
 
     bool sky_uniform = false;
     color sky_radiance;
+    // the sun line's settings, applied at the end of the file (scene.h:422-432)
+    bool sun_on = false, sun_disk = true;
+    double sun_elevation = 45, sun_azimuth = 45, sun_intensity = 3, sun_angle = 4;
+    color sun_tint(1, 1, 1);
     double fog_density = 0;
     std::shared_ptr<material> fog_material;
     std::string line;
@@ -1499,6 +1519,26 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
             if (cam.environment) return fail("only one environment is allowed");
             cam.environment = std::make_shared<environment_light>(w, h, std::move(px), intensity, rotation);
             lights.add(cam.environment);
+        } else if (keyword == "sun") {  // scene.h:543-562
+            const char* usage = "expected 'sun elevation <deg> azimuth <deg> [intensity <E>] [color <r g b>] "
+                                "[angle <deg>] [disk on|off]' or 'sun off'";
+            std::string key;
+            sun_on = true;
+            while (in >> key) {
+                bool ok = true;
+                if (key == "off") sun_on = false;
+                else if (key == "elevation") ok = bool(in >> sun_elevation) && sun_elevation >= -90 && sun_elevation <= 90;
+                else if (key == "azimuth") ok = bool(in >> sun_azimuth);
+                else if (key == "intensity") ok = bool(in >> sun_intensity) && sun_intensity >= 0;
+                else if (key == "color") ok = bool(in >> sun_tint[0] >> sun_tint[1] >> sun_tint[2]);
+                else if (key == "angle") ok = bool(in >> sun_angle) && sun_angle > 0 && sun_angle < 90;
+                else if (key == "disk") {
+                    std::string value;
+                    ok = bool(in >> value) && (value == "on" || value == "off");
+                    sun_disk = value == "on";
+                } else ok = false;
+                if (!ok) return fail(usage);
+            }
         } else if (keyword == "sky") {
             std::string type;
             if (!(in >> type)) return fail("expected 'sky uniform <r g b>' or 'sky gradient'");
@@ -1839,6 +1879,13 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
         if (cam.environment) return fail("'sky uniform' and 'environment' can't be used together");
         cam.environment = std::make_shared<environment_light>(8, 4, std::vector<color>(32, sky_radiance));
         lights.add(cam.environment);
+    }
+    if (sun_on) {  // scene.h:1217-1235: after the environment in the light list
+        double el = degrees_to_radians(sun_elevation), az = degrees_to_radians(sun_azimuth);
+        vec3 dir(std::cos(el) * std::sin(az), std::sin(el), -std::cos(el) * std::cos(az));
+        cam.sun = std::make_shared<sun_light>(dir, sun_angle, sun_intensity * sun_tint);
+        cam.sun_disk_visible = sun_disk;
+        lights.add(cam.sun);
     }
     if (fog_material) {
         // the world's bounding box: flatten what is there and let the core compute it

@@ -605,13 +605,28 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         for (int j = 1; j <= h; j++) s.env_marg_cdf[j] /= s.env_total_weight;
     }
 
+    // the sun (sun_light's constructor, sun.h:26-34)
+    s.has_sun = d.has_sun != 0;
+    if (s.has_sun) {
+        v3 dir{d.sun_direction[0], d.sun_direction[1], d.sun_direction[2]};
+        if (!(length_squared(dir) > 0) || !std::isfinite(length_squared(dir)))
+            return fail(MPT_E_INVALID, "sun direction must be finite and non-zero");
+        s.sun_frame = make_onb(dir);
+        double half = std::fmax(d.sun_angle_degrees, 1e-3) * kPi / 180.0 / 2;
+        s.sun_cos_max = std::cos(half);
+        s.sun_solid_angle = 2 * kPi * (1 - s.sun_cos_max);
+        s.sun_irradiance = v3{d.sun_irradiance[0], d.sun_irradiance[1], d.sun_irradiance[2]};
+        s.sun_radiance = s.sun_irradiance / s.sun_solid_angle;
+    }
+
     // light selection (light_list.h:45-96)
     const int nl = d.n_lights;
     s.light_strategy = d.light_strategy;
     s.lights.resize(nl);
     for (int k = 0; k < nl; k++) {
         int ob = d.light_object[k];
-        if (ob < -1 || ob >= no) return fail(MPT_E_INVALID, "light refers to a bad object");
+        if (ob < -2 || ob >= no) return fail(MPT_E_INVALID, "light refers to a bad object");
+        if (ob == -2 && !s.has_sun) return fail(MPT_E_INVALID, "sun light without a sun");
         if (ob == -1 && s.env_w <= 0) return fail(MPT_E_INVALID, "environment light without an environment");
         if (ob >= 0) {
             const obj_rec& r = s.objects[ob];
@@ -665,7 +680,9 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         int known = 0;
         for (int k = 0; k < nl; k++) {
             int ob = s.lights[k].object;
-            if (ob < 0) {  // environment_light::power (environment.h:116-127)
+            if (ob == -2) {  // sun_light::power (sun.h:66-71)
+                power[k] = kPi * radius * radius * luminance(s.sun_irradiance);
+            } else if (ob < 0) {  // environment_light::power (environment.h:116-127)
                 double sum = 0;
                 for (int j = 0; j < s.env_h; j++)
                     for (int i = 0; i < s.env_w; i++) {

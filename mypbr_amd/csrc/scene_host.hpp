@@ -102,6 +102,12 @@ struct host_scene {
     bool rich = false;                   // uses_uv, thin glass, a measured metal or the ground: the variant with them
     int32_t ground_mat = -1;             // the ground plane's material, -1 without one
     double ground_h = 0;
+    // the sun (sun.h:26-34): its frame around the direction, the cone's
+    // cosine and solid angle, and radiance = irradiance / solid angle
+    bool has_sun = false;
+    onb sun_frame{};
+    double sun_cos_max = 1, sun_solid_angle = 0;
+    v3 sun_irradiance{0, 0, 0}, sun_radiance{0, 0, 0};
     std::vector<light_rec> lights;
     std::vector<double> light_cdf;       // n_lights + 1
     // mesh lights (mesh.h:175-216): per light mesh, its triangles in the

@@ -1320,6 +1320,33 @@ struct medium : hittable {  // constant_medium, medium.h:57-96
     aabb bounding_box() const override { return boundary->bounding_box(); }
 };
 
+// A distant light of small angular size (sun.h:23-80): never hit; radiance
+// E / omega inside the cone, sampled uniformly in it.
+struct sun_light : hittable {
+    onb frame;
+    double cos_max, solid_angle;
+    color irradiance, radiance_in_cone;
+    sun_light(const vec3& direction, double angular_diameter, const color& e) : frame(direction) {
+        double half = std::fmax(angular_diameter, 1e-3) * pi / 180.0 / 2;  // sun.h:28-33
+        cos_max = std::cos(half);
+        solid_angle = 2 * pi * (1 - cos_max);
+        irradiance = e;
+        radiance_in_cone = e / solid_angle;
+    }
+    bool hit(const ray&, interval, hit_record&, const hit_ctx&) const override { return false; }
+    aabb bounding_box() const override { return aabb(); }
+    bool in_cone(const vec3& d) const { return dot(unit_vector(d), frame.w()) >= cos_max; }  // sun.h:76-78
+    color radiance(const vec3& d) const { return in_cone(d) ? radiance_in_cone : color(0, 0, 0); }
+    double pdf_value(const point3&, const vec3& d) const override { return in_cone(d) ? 1 / solid_angle : 0; }
+    vec3 random(const point3&, sampler& s) const override {  // sun.h:56-62
+        double cos_theta = 1 - s.next() * (1 - cos_max);
+        double sin_theta = std::sqrt(std::fmax(0.0, 1 - cos_theta * cos_theta));
+        double phi = 2 * pi * s.next();
+        return frame.transform(vec3(std::cos(phi) * sin_theta, std::sin(phi) * sin_theta, cos_theta));
+    }
+    double power(double radius) const { return pi * radius * radius * luminance(irradiance); }  // sun.h:66-71
+};
+
 struct environment : hittable {  // environment.h
     int width = 0, height = 0;
     std::vector<color> pixels;
@@ -1438,6 +1465,7 @@ struct scene {
     std::vector<double> prob, cdf;
     int strategy = 1;
     std::unique_ptr<environment> env;
+    std::unique_ptr<sun_light> sun;    // also in lights, when the scene has one
     ggx_table table;
     aabb world_box;
     const hittable* ground = nullptr;  // beside the BVH (scene::unbounded)
@@ -1736,7 +1764,15 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
     // light_list::build (light_list.h:45-96)
     size_t n = size_t(d.n_lights);
     s.strategy = d.light_strategy;
-    for (size_t k = 0; k < n; k++) s.lights.push_back(d.light_object[k] < 0 ? nullptr : s.world[d.light_object[k]]);
+    if (d.has_sun)
+        s.sun = std::make_unique<sun_light>(vec3(d.sun_direction[0], d.sun_direction[1], d.sun_direction[2]),
+                                            d.sun_angle_degrees,
+                                            color(d.sun_irradiance[0], d.sun_irradiance[1], d.sun_irradiance[2]));
+    for (size_t k = 0; k < n; k++) {
+        const int ob = d.light_object[k];
+        if (ob == -2 && !s.sun) return MPT_E_INVALID;  // a sun light without a sun
+        s.lights.push_back(ob == -2 ? s.sun.get() : ob < 0 ? nullptr : s.world[ob]);
+    }
     s.prob.assign(n, n > 0 ? 1.0 / n : 0.0);
     if (n > 0 && s.strategy == 1) {
         const aabb& b = s.world_box;
@@ -1749,7 +1785,9 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
         double known_sum = 0;
         int known = 0;
         for (size_t k = 0; k < n; k++) {
-            power[k] = s.lights[k] ? s.lights[k]->power() : s.env->power(radius);
+            power[k] = d.light_object[k] == -2 ? s.sun->power(radius)
+                       : s.lights[k]               ? s.lights[k]->power()
+                                                   : s.env->power(radius);
             if (std::isfinite(power[k]) && power[k] > 0) {
                 known_sum += power[k];
                 known++;
@@ -1772,7 +1810,7 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
 // ------------------------------------------------------------------ camera (camera.h)
 struct cam {
     int w, h, spp, max_depth, max_diffuse, max_specular, max_scatter;
-    bool transparent_shadows, features, use_sky, has_env;
+    bool transparent_shadows, features, use_sky, has_env, sun_disk_visible;
     int sqrt_spp, blades;
     double recip, ortho, defocus, focus, blade_rot;
     vec3 center, p00, du, dv, w_axis, disk_u, disk_v;
@@ -1789,6 +1827,7 @@ struct cam {
         features = c.collect_features;
         use_sky = c.use_sky;
         has_env = env;
+        sun_disk_visible = !c.sun_disk_hidden;
         background = color(c.background[0], c.background[1], c.background[2]);
         sqrt_spp = int(std::sqrt(spp));
         recip = 1.0 / sqrt_spp;
@@ -1859,12 +1898,21 @@ struct counters {
     uint64_t extension = 0, shadow = 0;
 };
 
-color escaped(const scene& S, const cam& C, const vec3& d) {
-    if (C.has_env) return S.env->radiance(d);
-    if (!C.use_sky) return C.background;
-    vec3 u = unit_vector(d);
-    double a = 0.5 * (u.y() + 1.0);
-    return (1.0 - a) * color(1.0, 1.0, 1.0) + a * color(0.5, 0.7, 1.0);
+// camera::escaped_radiance (camera.h:515-529): the sky, plus the sun for
+// every ray but a camera ray while the sun's disk is hidden
+color escaped(const scene& S, const cam& C, const vec3& d, bool camera_ray) {
+    color sky;
+    if (C.has_env) {
+        sky = S.env->radiance(d);
+    } else if (!C.use_sky) {
+        sky = C.background;
+    } else {
+        vec3 u = unit_vector(d);
+        double a = 0.5 * (u.y() + 1.0);
+        sky = (1.0 - a) * color(1.0, 1.0, 1.0) + a * color(0.5, 0.7, 1.0);
+    }
+    if (S.sun && (C.sun_disk_visible || !camera_ray)) sky = sky + S.sun->radiance(d);
+    return sky;
 }
 double power_heuristic(double a, double b) {
     double a2 = a * a, b2 = b * b;
@@ -1886,7 +1934,7 @@ color sample_light(const scene& S, const cam& C, const ray& r_in, const hit_reco
         cnt.shadow++;
         hit_ctx hc{s.key, uint32_t(2 * depth + 1) | (uint32_t(k) << 20), true};
         if (!S.hit(shadow, interval(0.001, infinity), lr, hc)) {
-            emission = escaped(S, C, to_light);
+            emission = escaped(S, C, to_light, false);
             break;
         }
         if (C.transparent_shadows && k < 16 && lr.mat->lets_shadows_through()) {
@@ -1914,7 +1962,7 @@ color ray_color(const scene& S, const cam& C, const ray& camera_ray, sampler& s,
         cnt.extension++;
         hit_ctx hc{s.key, uint32_t(2 * depth), true};
         if (!S.hit(r, interval(0.001, infinity), rec, hc)) {
-            color sky = escaped(S, C, r.dir);
+            color sky = escaped(S, C, r.dir, depth == 0);
             if (seen) break;
             if (specular || no_lights) radiance = radiance + beta * sky;
             else radiance = radiance + beta * sky * power_heuristic(scatter_pdf, S.lights_pdf(prev, r.dir));

@@ -19,6 +19,8 @@ queries) are seeded and regenerated identically by the tests:
                         of tests/denoise_frames.py, for two settings
   metals.npz            scenes/metals.scene (measured metals, thin glass) as below
   ground.npz            scenes/ground.scene (the ground plane beside the BVH) as textured.npz
+  sun.npz               scenes/sun.scene (sun_light, sun.h) as below, plus the reference's
+                        light_list::probability per light
   textured.npz          scenes/textured.scene (image textures, normal maps, texture
                         maps) as below, plus the reference's surface record at each
                         hit of the subset and stbi_load's bytes of its texture files
@@ -150,6 +152,19 @@ TEXTURE_SCENES = {
 MATERIAL_SCENES = {
     "metals.scene": (48, 256, 8, 16),
 }
+SUN_SCENES = {
+    "sun.scene": (48, 256, 8, 12),
+}
+
+
+def golden_sun():
+    for name, render in SUN_SCENES.items():
+        golden_asset_scene(name, render)
+        path = os.path.join(HERE, name.replace(".scene", ".npz"))
+        g = dict(np.load(path))
+        g["light_probabilities"] = ol.RefScene(scene_path(name)).light_probabilities()
+        np.savez_compressed(path, **g)
+        print(name, "light probabilities", g["light_probabilities"], flush=True)
 
 
 def golden_materials():
@@ -220,7 +235,7 @@ def main():
         sys.exit("oracle/_ref/libmpt_ref.so is not built (make -C oracle ref, where the reference is present)")
     R = ol.reference()
     steps = set(sys.argv[1:]) or {"ggx", "trace", "bsdf", "render", "denoise", "display", "obj", "textures",
-                                  "materials"}
+                                  "materials", "sun"}
     if "denoise" in steps:
         golden_denoise()
     if "display" in steps:
@@ -231,6 +246,8 @@ def main():
         golden_textures()
     if "materials" in steps:
         golden_materials()
+    if "sun" in steps:
+        golden_sun()
     if "ggx" in steps:
         golden_ggx(R)
     if "trace" in steps:
