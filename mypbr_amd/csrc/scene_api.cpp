@@ -648,6 +648,91 @@ void environment_light::flatten(flat_builder&, int32_t) const {
     // never hit (environment.h:53-56): not part of the world's geometry
 }
 
+// ------------------------------------------------------------------ sky atmosphere
+// Single-scattering sky over a spherical Earth (atmosphere.h:26-165): a
+// viewer 200 m above the ground looks along d; 32 midpoint steps to the top
+// of the air (or the ground) gather sunlight scattered by air (Rayleigh) and
+// haze (Mie, Henyey-Greenstein phase), each dimmed by the optical depth back
+// to the viewer and, towards the sun, by a 16-step march to the top. The
+// densities fall off with height (scale heights 8 km and 1.2 km; ozone a tent
+// around 25 km). The sums keep the reference's operand order, so the baked
+// image equals the reference's bit for bit.
+namespace {
+struct sky_air {
+    double rayleigh = 1, mie = 1, mie_g = 0.8, ozone = 1, intensity = 3;  // atmosphere.h:13-24
+    static constexpr double r_ground = 6360e3, r_top = 6460e3, r_view = 6360e3 + 200;
+
+    // extinction (or, with scatter_only, scattering) coefficients at height h, per channel
+    void coefficients(double h, double ext[3]) const {
+        static const double br[3] = {5.802e-6, 13.558e-6, 33.1e-6}, bo[3] = {0.650e-6, 1.881e-6, 0.085e-6};
+        const double rr = std::exp(-h / 8000), rm = std::exp(-h / 1200);
+        const double ro = std::fmax(0.0, 1 - std::fabs(h - 25000) / 15000);
+        for (int c = 0; c < 3; c++) ext[c] = rr * (rayleigh * br[c]) + rm * (mie * 4.40e-6) + ro * (ozone * bo[c]);
+    }
+    static double to_top(const vec3& o, const vec3& d) {
+        double b = dot(o, d), c = o.length_squared() - r_top * r_top;
+        return -b + std::sqrt(std::fmax(0.0, b * b - c));
+    }
+    static double to_ground(const vec3& o, const vec3& d) {  // -1 when the ray misses the ground
+        double b = dot(o, d), c = o.length_squared() - r_ground * r_ground;
+        double disc = b * b - c;
+        if (disc < 0) return -1;
+        double t = -b - std::sqrt(disc);
+        return t > 0 ? t : -1;
+    }
+    color transmittance(const vec3& p, const vec3& d) const {  // atmosphere.h:146-160
+        if (to_ground(p, d) > 0) return color(0, 0, 0);
+        const double dt = to_top(p, d) / 16;
+        double tau[3] = {0, 0, 0}, ext[3];
+        for (int k = 0; k < 16; k++) {
+            coefficients((p + (k + 0.5) * dt * d).length() - r_ground, ext);
+            for (int c = 0; c < 3; c++) tau[c] += dt * ext[c];
+        }
+        return color(std::exp(-1.0 * tau[0]), std::exp(-1.0 * tau[1]), std::exp(-1.0 * tau[2]));
+    }
+    color radiance(const vec3& direction, const vec3& sun_direction, const color& e) const {  // atmosphere.h:58-95
+        static const double br[3] = {5.802e-6, 13.558e-6, 33.1e-6};
+        const vec3 d = unit_vector(direction), s = unit_vector(sun_direction);
+        const vec3 o(0, r_view, 0);
+        double t_end = to_top(o, d);
+        const double tg = to_ground(o, d);
+        if (tg > 0) t_end = std::fmin(t_end, tg);
+        const double mu = dot(d, s);
+        const double phase_r = 3 / (16 * kPiApi) * (1 + mu * mu);
+        const double g = mie_g, denom = 1 + g * g - 2 * g * mu;
+        const double phase_m = (1 - g * g) / (4 * kPiApi * denom * std::sqrt(denom));
+        const double dt = t_end / 32;
+        double tau[3] = {0, 0, 0}, sum[3] = {0, 0, 0}, ext[3];
+        for (int k = 0; k < 32; k++) {
+            const vec3 p = o + (k + 0.5) * dt * d;
+            const double h = p.length() - r_ground;
+            coefficients(h, ext);
+            const double rr = std::exp(-h / 8000), rm = std::exp(-h / 1200);
+            for (int c = 0; c < 3; c++) tau[c] += 0.5 * dt * ext[c];  // midpoint: half a step to p
+            const color sun_t = transmittance(p, s);
+            for (int c = 0; c < 3; c++) {
+                const double scatter = rr * phase_r * (rayleigh * br[c]) + rm * phase_m * (mie * 3.996e-6);
+                sum[c] += std::exp(-1.0 * tau[c]) * sun_t[c] * scatter * dt;
+                tau[c] += 0.5 * dt * ext[c];
+            }
+        }
+        return color(intensity * sum[0] * e[0], intensity * sum[1] * e[1], intensity * sum[2] * e[2]);
+    }
+    std::vector<color> bake(int w, int h, const vec3& sun_direction, const color& e) const {  // atmosphere.h:100-114
+        std::vector<color> px(size_t(w) * h);
+        for (int j = 0; j < h; j++) {
+            const double theta = kPiApi * (j + 0.5) / h;
+            for (int i = 0; i < w; i++) {
+                const double phi = 2 * kPiApi * ((i + 0.5) / w - 0.5);
+                const vec3 d(std::sin(theta) * std::sin(phi), std::cos(theta), -std::sin(theta) * std::cos(phi));
+                px[size_t(j) * w + i] = radiance(d, sun_direction, e);
+            }
+        }
+        return px;
+    }
+};
+}  // namespace
+
 void sun_light::flatten(flat_builder&, int32_t) const {
     // never hit (sun.h:38-41): not part of the world's geometry
 }
@@ -1440,7 +1525,8 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
     bool sky_uniform = false;
     color sky_radiance;
     // the sun line's settings, applied at the end of the file (scene.h:422-432)
-    bool sun_on = false, sun_disk = true;
+    bool sun_on = false, sun_disk = true, sky_atmosphere = false;
+    sky_air air;
     double sun_elevation = 45, sun_azimuth = 45, sun_intensity = 3, sun_angle = 4;
     color sun_tint(1, 1, 1);
     double fog_density = 0;
@@ -1546,11 +1632,27 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
                 if (!(in >> sky_radiance[0] >> sky_radiance[1] >> sky_radiance[2])) return fail("expected 'sky uniform <r g b>'");
                 sky_uniform = true;
                 cam.use_sky = true;
-            } else if (type == "gradient") {
+                sky_atmosphere = false;
+            } else if (type == "atmosphere") {  // scene.h:574-588
+                const char* usage = "expected 'sky atmosphere [rayleigh <x>] [mie <x>] [mie_g <g>] [ozone <x>] [intensity <s>]'";
+                std::string key;
+                while (in >> key) {
+                    bool ok = (key == "rayleigh" && in >> air.rayleigh && air.rayleigh >= 0) ||
+                              (key == "mie" && in >> air.mie && air.mie >= 0) ||
+                              (key == "mie_g" && in >> air.mie_g && air.mie_g > -1 && air.mie_g < 1) ||
+                              (key == "ozone" && in >> air.ozone && air.ozone >= 0) ||
+                              (key == "intensity" && in >> air.intensity && air.intensity >= 0);
+                    if (!ok) return fail(usage);
+                }
+                sky_atmosphere = true;
                 sky_uniform = false;
                 cam.use_sky = true;
+            } else if (type == "gradient") {
+                sky_uniform = false;
+                sky_atmosphere = false;
+                cam.use_sky = true;
             } else {
-                return fail("sky type '" + type + "' is not modelled by the GPU path (uniform and gradient are)");
+                return fail("unknown sky type '" + type + "' (expected uniform, atmosphere or gradient)");
             }
         } else if (keyword == "bounces") {
             std::string key;
@@ -1880,10 +1982,21 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
         cam.environment = std::make_shared<environment_light>(8, 4, std::vector<color>(32, sky_radiance));
         lights.add(cam.environment);
     }
-    if (sun_on) {  // scene.h:1217-1235: after the environment in the light list
-        double el = degrees_to_radians(sun_elevation), az = degrees_to_radians(sun_azimuth);
-        vec3 dir(std::cos(el) * std::sin(az), std::sin(el), -std::cos(el) * std::cos(az));
-        cam.sun = std::make_shared<sun_light>(dir, sun_angle, sun_intensity * sun_tint);
+    // scene.h:1217-1235: the sky bake, then the sun after the environment in the light list
+    double el = degrees_to_radians(sun_elevation), az = degrees_to_radians(sun_azimuth);
+    vec3 sun_dir(std::cos(el) * std::sin(az), std::sin(el), -std::cos(el) * std::cos(az));
+    color sun_e = sun_intensity * sun_tint;
+    if (sky_atmosphere) {
+        // lit by the sun's settings even with "sun off"; the sun's own light
+        // then reaches the ground through the air
+        if (cam.environment) return fail("'sky atmosphere' and 'environment' can't be used together");
+        cam.environment = std::make_shared<environment_light>(256, 128, air.bake(256, 128, sun_dir, sun_e));
+        lights.add(cam.environment);
+        color tr = air.transmittance(vec3(0, sky_air::r_view, 0), unit_vector(sun_dir));
+        sun_e = color(sun_e[0] * tr[0], sun_e[1] * tr[1], sun_e[2] * tr[2]);
+    }
+    if (sun_on) {
+        cam.sun = std::make_shared<sun_light>(sun_dir, sun_angle, sun_e);
         cam.sun_disk_visible = sun_disk;
         lights.add(cam.sun);
     }

@@ -146,6 +146,19 @@ int ref_light_probabilities(void* h, double* out) {
     return 0;
 }
 
+// The scene camera's environment (an image file, a uniform sky or the baked
+// atmosphere) looked up by environment_light::radiance (environment.h:66-79)
+// along n directions; returns -1 without one.
+int ref_env_radiance(void* h, const double* dirs, int64_t n, double* out) {
+    auto* r = static_cast<ref_scene*>(h);
+    if (!r->sc.cam.environment) return -1;
+    for (int64_t k = 0; k < n; k++) {
+        color c = r->sc.cam.environment->radiance(vec3(dirs[3 * k], dirs[3 * k + 1], dirs[3 * k + 2]));
+        for (int i = 0; i < 3; i++) out[3 * k + i] = c[i];
+    }
+    return 0;
+}
+
 int ref_ggx_table(double* e, double* eavg) {
     const auto& t = ggx_albedo_table::get();
     std::memcpy(e, t.e, sizeof(t.e));

@@ -21,6 +21,9 @@ queries) are seeded and regenerated identically by the tests:
   ground.npz            scenes/ground.scene (the ground plane beside the BVH) as textured.npz
   sun.npz               scenes/sun.scene (sun_light, sun.h) as below, plus the reference's
                         light_list::probability per light
+  sky_atmosphere.npz    scenes/sky_atmosphere.scene (the sky bake, atmosphere.h) as sun.npz,
+                        plus the reference's environment_light::radiance of its baked sky
+                        along 4096 seeded directions
   textured.npz          scenes/textured.scene (image textures, normal maps, texture
                         maps) as below, plus the reference's surface record at each
                         hit of the subset and stbi_load's bytes of its texture files
@@ -154,6 +157,7 @@ MATERIAL_SCENES = {
 }
 SUN_SCENES = {
     "sun.scene": (48, 256, 8, 12),
+    "sky_atmosphere.scene": (48, 256, 8, 12),
 }
 
 
@@ -162,7 +166,12 @@ def golden_sun():
         golden_asset_scene(name, render)
         path = os.path.join(HERE, name.replace(".scene", ".npz"))
         g = dict(np.load(path))
-        g["light_probabilities"] = ol.RefScene(scene_path(name)).light_probabilities()
+        r = ol.RefScene(scene_path(name))
+        g["light_probabilities"] = r.light_probabilities()
+        if name == "sky_atmosphere.scene":
+            dirs = np.random.default_rng(11).normal(size=(4096, 3))
+            g["sky_dirs"] = dirs
+            g["sky_radiance"] = r.env_radiance(dirs)
         np.savez_compressed(path, **g)
         print(name, "light probabilities", g["light_probabilities"], flush=True)
 

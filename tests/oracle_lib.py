@@ -183,6 +183,7 @@ def reference():
                                     P(C.c_int64), P(C.c_int64), C.c_int]),
             "ref_light_count": (C.c_int, [vp]),
             "ref_light_probabilities": (C.c_int, [vp, dp]),
+            "ref_env_radiance": (C.c_int, [vp, dp, C.c_int64, dp]),
             "ref_ggx_table": (C.c_int, [dp, dp]),
             "ref_bsdf_eval": (C.c_int, [dp, dp, P(C.c_uint8), C.c_int64, dp, dp]),
             "ref_render": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp]),
@@ -233,6 +234,13 @@ class RefScene:
         self.lib.ref_trace(self.h, _d(rays), _d(tt), n, _i64(ids), _d(t), _u8(f), _i32(ties), _i64(lo), _i64(hi),
                            threads)
         return {"id": ids, "t": t, "front": f, "ties": ties, "tie_min": lo, "tie_max": hi}
+
+    def env_radiance(self, dirs):
+        dirs = np.ascontiguousarray(dirs, dtype=np.float64)
+        out = np.zeros((len(dirs), 3))
+        if self.lib.ref_env_radiance(self.h, _d(dirs), len(dirs), _d(out)) != 0:
+            raise RuntimeError("the reference scene has no environment")
+        return out
 
     def light_probabilities(self):
         n = self.lib.ref_light_count(self.h)
