@@ -492,6 +492,21 @@ int32_t mpt_postprocess_dev(const double* radiance, int32_t width, int32_t heigh
 int32_t mpt_display_bytes_dev(const double* radiance, int32_t width, int32_t height, const mpt_display* display,
                               int32_t post, uint8_t* rgb, void* workspace, void* stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mpt_display_bytes_dev with the bloom pyramid in a fixed
+ * array inside the library instead of a workspace argument. It calls no
+ * hipMalloc, hipFree or allocating library routine. The array holds
+ * mpt_display_noalloc_max_workspace() bytes (48 MiB: enough for 1920 x 1080
+ * at the default bloom radius); a pyramid larger than that is refused with
+ * MPT_E_CAPACITY. Each device has its own copy, held until the process
+ * ends. Every call on a device shares that device's array: calls that
+ * overlap on one device (on two streams, or from two threads) race, so
+ * serialise them. Results are identical to mpt_display_bytes_dev's. */
+size_t mpt_display_noalloc_max_workspace(void);
+int32_t mpt_display_bytes_dev_noalloc(const double* radiance, int32_t width, int32_t height, const mpt_display* display,
+                                      int32_t post, uint8_t* rgb, void* stream);
+
 /* write_image(path, width, height, pixels, display) (image_io.h:81-106): the
  * format follows the extension. .png and .ppm are display images (bloom,
  * vignette, exposure, tonemap, sRGB bytes from the GPU); .hdr (Radiance

@@ -105,6 +105,8 @@ def _load():
         "mpt_display_workspace_bytes": (C.c_size_t, [C.c_int32, C.c_int32, P(abi.Display)]),
         "mpt_postprocess_dev": (C.c_int32, [vp, C.c_int32, C.c_int32, P(abi.Display), vp, vp, vp]),
         "mpt_display_bytes_dev": (C.c_int32, [vp, C.c_int32, C.c_int32, P(abi.Display), C.c_int32, vp, vp, vp]),
+        "mpt_display_noalloc_max_workspace": (C.c_size_t, []),
+        "mpt_display_bytes_dev_noalloc": (C.c_int32, [vp, C.c_int32, C.c_int32, P(abi.Display), C.c_int32, vp, vp]),
         "mpt_write_image": (C.c_int32, [C.c_char_p, C.c_int32, C.c_int32, P(C.c_double), P(abi.Display)]),
     }
     for name, (res, args) in sig.items():

@@ -413,6 +413,38 @@ int run_host(const double* rad, int width, int height, const mpt_display* displa
     return done(MPT_OK);
 }
 
+// The allocation-free entry point's bloom pyramid: a fixed array in device
+// memory, one copy per device, held until the process ends. 48 MiB covers
+// the pyramid of a 1920 x 1080 image at the default bloom radius (33.2 MB,
+// 7 levels) with room to spare. Host code never passes `scratch` itself to a
+// kernel: in host code the name stands for a host-side shadow, so every use
+// goes through the device address hipGetSymbolAddress returns for it.
+constexpr size_t kScratchBytes = size_t(48) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of the current device's scratch array, looked up once
+// per device (with that device current) and kept.
+int scratch_for_current_device(void** out) {
+    int dev = 0;
+    DP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_error(MPT_E_INVALID, "display: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(MPT_E_HIP, std::string("display: hipGetSymbolAddress of the scratch array: ") +
+                                            hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return MPT_OK;
+}
+
 }  // namespace
 }  // namespace mpt
 
@@ -456,6 +488,26 @@ int32_t mpt_display_bytes_dev(const double* radiance, int32_t width, int32_t hei
     if (rc) return rc;
     if (!radiance || !rgb) return set_error(MPT_E_INVALID, "display bytes: null buffer");
     return run(radiance, width, height, display, post != 0, nullptr, rgb, workspace, static_cast<hipStream_t>(stream));
+}
+
+size_t mpt_display_noalloc_max_workspace(void) { return kScratchBytes; }
+
+int32_t mpt_display_bytes_dev_noalloc(const double* radiance, int32_t width, int32_t height, const mpt_display* display,
+                                      int32_t post, uint8_t* rgb, void* stream) {
+    int rc = check(width, height, display);
+    if (rc) return rc;
+    if (!radiance || !rgb) return set_error(MPT_E_INVALID, "display bytes: null buffer");
+    const size_t need = post ? mpt_display_workspace_bytes(width, height, display) : 0;
+    if (need > kScratchBytes)
+        return set_error(MPT_E_CAPACITY, "display bytes (no allocation): the bloom pyramid of a " +
+                                             std::to_string(width) + " x " + std::to_string(height) + " image needs " +
+                                             std::to_string(need) + " bytes, more than the fixed scratch of " +
+                                             std::to_string(kScratchBytes) + " bytes");
+    void* ws = nullptr;
+    if (need > 0 && (rc = scratch_for_current_device(&ws))) return rc;
+    // the pyramid kernels write every texel before any kernel reads it, so the
+    // array needs no clearing between calls
+    return run(radiance, width, height, display, post != 0, nullptr, rgb, ws, static_cast<hipStream_t>(stream));
 }
 
 int32_t mpt_postprocess(const double* radiance, int32_t width, int32_t height, const mpt_display* display,
