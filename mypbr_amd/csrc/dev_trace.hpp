@@ -3,11 +3,20 @@
 // Two-level BVH in the reference's layout: a top-level BVH over the world
 // list (main.cpp:70) whose leaves are objects (a primitive, a linear list, a
 // mesh with its own BVH, or a medium), each optionally behind an affine
-// transform (instance.h:29-41). Traversal follows bvh::hit (bvh.h:45-86)
-// step for step in double precision: the same slab test (aabb.h:40-54),
-// near-first order by the split axis sign, leaf tests that shrink the
-// interval on every hit. Identical node order and identical arithmetic give
+// transform (instance.h:29-41). Traversal follows bvh::hit (bvh.h:45-86):
+// near-first order by the split axis sign, leaf tests in double precision
+// that shrink the interval on every hit, the leaves' slab test (aabb.h:40-54)
+// in double precision. Identical leaf order and identical arithmetic give
 // identical closest hits, ties included.
+//
+// Interior boxes are culled in single precision against 64-byte records
+// whose boxes are rounded outward, with the ray origin moved by a bound on
+// the rounding error: such a cull never rejects a box the double test
+// accepts, so it only adds visits. A leaf's double box is tested, with the
+// interval the leaf was entered with, when one of its items first reports a
+// hit; if the double test rejects the box, the reference never visits the
+// leaf and its hits are dropped. Which leaves count, and in which order, is
+// therefore the reference's.
 //
 // The traversal stack lives in LDS (a short stack of kLdsDepth entries per
 // lane, strided by the block size so lanes hit distinct banks) and spills to
@@ -29,8 +38,10 @@ struct dscene {
     const int64_t* pcanon;
     const obj_rec* obj;
     const cnode* tlas;        // child-pair records of the top-level BVH
+    const fnode* ftlas;       // the same in single precision (box culls)
     const int32_t* titems;
     const cnode* blas;        // child-pair records of all mesh BVHs
+    const fnode* fblas;
     const croot* blas_roots;  // per mesh, indexed by obj_rec.blas_root
     const double* sph;        // center xyz, radius per primitive (spheres)
     const xform* xf;
@@ -140,79 +151,175 @@ __device__ __forceinline__ bool box_hit(const double* b, const v3& o, const v3& 
     return true;
 }
 
-__device__ __forceinline__ cnode load_cnode(const cnode* nodes, int i) {
-    const double2* p = reinterpret_cast<const double2*>(nodes + i);
-    cnode n;
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-        double2 v = p[k];
-        n.b[2 * k] = v.x;
-        n.b[2 * k + 1] = v.y;
+// A ray in single precision for the box culls. Per axis the origin is moved
+// by delta = 2^-20 (|o| + the root box's largest coordinate magnitude) in
+// both directions: o + delta for the planes at box minima, o - delta for the
+// maxima. delta bounds (with a factor 4 to spare) what rounding the box, the
+// origin, the reciprocal direction and the two operations to single
+// precision can move a slab distance, in units of the inverse direction, so
+// every single-precision interval contains the double one. An axis whose
+// reciprocal direction is infinite behaves as in aabb::hit: the planes then
+// give +-inf (or NaN, which the comparisons ignore).
+struct fray {
+    float olo[3], ohi[3], inv[3];
+    float tmin;  // the interval's start, rounded down
+};
+
+// x rounded to single precision towards +inf / -inf.
+__device__ __forceinline__ float f32_up(double x) {
+    float f = float(x);
+    if (double(f) < x) {  // f is finite here: step to the next float above
+        const int32_t i = __float_as_int(f);
+        f = f == 0.0f ? __int_as_float(1) : __int_as_float(i >= 0 ? i + 1 : i - 1);
     }
-    const int4 a = reinterpret_cast<const int4*>(nodes + i)[6];
-    const int4 c = reinterpret_cast<const int4*>(nodes + i)[7];
-    n.off[0] = a.x, n.off[1] = a.y, n.cnt[0] = a.z, n.cnt[1] = a.w;
-    n.axis = c.x;
+    return f;
+}
+__device__ __forceinline__ float f32_down(double x) {
+    float f = float(x);
+    if (double(f) > x) {
+        const int32_t i = __float_as_int(f);
+        f = f == 0.0f ? -__int_as_float(1) : __int_as_float(i >= 0 ? i - 1 : i + 1);
+    }
+    return f;
+}
+
+__device__ __forceinline__ fray make_fray(const v3& o, const v3& inv, double tmin, const double* root_box) {
+    fray r;
+    const double oo[3] = {o.x, o.y, o.z};
+    const double ii[3] = {inv.x, inv.y, inv.z};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const double bm = std::fmax(std::fabs(root_box[2 * a]), std::fabs(root_box[2 * a + 1]));
+        const double delta = (std::fabs(oo[a]) + bm) * 0x1p-20;
+        r.olo[a] = float(oo[a] + delta);
+        r.ohi[a] = float(oo[a] - delta);
+        r.inv[a] = float(ii[a]);
+    }
+    r.tmin = f32_down(tmin);
+    return r;
+}
+
+// Conservative single-precision slab test: true whenever the double test of
+// aabb::hit (aabb.h:40-54) on the unrounded box is true. tmax rounded up.
+__device__ __forceinline__ bool fbox_hit(const float* b, const fray& r, float tmax) {
+    float lo = r.tmin, hi = tmax;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float t0 = (b[2 * a] - r.olo[a]) * r.inv[a];
+        float t1 = (b[2 * a + 1] - r.ohi[a]) * r.inv[a];
+        if (t0 > t1) {
+            const float t = t0;
+            t0 = t1;
+            t1 = t;
+        }
+        if (t0 > lo) lo = t0;
+        if (t1 < hi) hi = t1;
+    }
+    return hi >= lo;
+}
+
+__device__ __forceinline__ fnode load_fnode(const fnode* nodes, int i) {
+    const float4* p = reinterpret_cast<const float4*>(nodes + i);
+    fnode n;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float4 v = p[k];
+        n.b[4 * k] = v.x, n.b[4 * k + 1] = v.y, n.b[4 * k + 2] = v.z, n.b[4 * k + 3] = v.w;
+    }
+    const int4 c = reinterpret_cast<const int4*>(nodes + i)[3];
+    n.off[0] = c.x, n.off[1] = c.y, n.cnt[0] = c.z, n.cnt[1] = c.w;
     return n;
 }
 
-// One child (0 = left, 1 = right) of record i: its box, item range or record.
-__device__ __forceinline__ void load_child(const cnode* nodes, int i, int w, double* b, int32_t& off, int32_t& cnt) {
-    const double2* p = reinterpret_cast<const double2*>(nodes + i) + 3 * w;
+// One child (0 = left, 1 = right) of single-precision record i.
+__device__ __forceinline__ void load_fchild(const fnode* nodes, int i, int w, float* b, int32_t& off, int32_t& cnt) {
+    const float2* p = reinterpret_cast<const float2*>(nodes + i) + 3 * w;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        double2 v = p[k];
+        const float2 v = p[k];
         b[2 * k] = v.x;
         b[2 * k + 1] = v.y;
     }
-    const int4 a = reinterpret_cast<const int4*>(nodes + i)[6];
-    off = w ? a.y : a.x;
-    cnt = w ? a.w : a.z;
+    const int4 c = reinterpret_cast<const int4*>(nodes + i)[3];
+    off = w ? c.y : c.x;
+    cnt = (w ? c.w : c.z) & 0x3fffffff;
+}
+
+// The double-precision test of leaf e (child e & 1 of record e >> 1; e < 0:
+// the root, already tested exactly) with the interval the leaf was entered
+// with: the test the reference makes when it visits the leaf.
+// The reciprocal direction is recomputed here rather than kept live through
+// the walk: the check is rare and the registers are not.
+template <bool CNT>
+__device__ __forceinline__ bool leaf_exact(const cnode* nodes, int e, const v3& o, const v3& d, double tmin,
+                                           double tmax_at_entry, wcount& wc) {
+    if (e < 0) return true;
+    if constexpr (CNT) wc.nodes += 48u;
+    const v3 inv{1 / d.x, 1 / d.y, 1 / d.z};
+    const double2* p = reinterpret_cast<const double2*>(nodes + (e >> 1)) + 3 * (e & 1);
+    double b[6];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double2 v = p[k];
+        b[2 * k] = v.x;
+        b[2 * k + 1] = v.y;
+    }
+    return box_hit(b, o, inv, tmin, tmax_at_entry);
 }
 
 // Walk of bvh::hit (bvh.h:45-86) over child-pair records. At an interior
-// record both child boxes are tested with the current interval; the near
-// child (by the split axis sign) is entered, the far child pushed only if
-// it is hit. A popped child is tested again with the interval as it is then,
-// exactly as the reference tests a node when it pops it: a box missed with
-// a larger tmax is missed with a smaller one, so skipping its push changes
-// nothing, and the order of leaf visits is the reference's.
-// leaf(off, cnt, sp) tests a leaf's items and shrinks tmax on hits.
-// negmask: bit a set when the ray goes towards -a (inv_dir < 0).
+// record both child boxes are culled with the current interval; the near
+// child (by the split axis sign) is entered, the far child pushed only if it
+// passes. A popped child is culled again with the interval as it is then,
+// as the reference tests a node when it pops it: a box missed with a larger
+// tmax is missed with a smaller one, so skipping its push changes nothing.
+// The culls are conservative, so every leaf the reference visits is reached,
+// in the reference's order; leaf(off, cnt, sp, e) tests a leaf's items,
+// shrinks tmax on hits, and keeps a hit only if leaf_exact(e) holds.
 template <bool CNT, typename Leaf>
-__device__ __forceinline__ void walk(const cnode* nodes, const croot& root, const v3& o, const v3& inv, int negmask,
-                                     double tmin, double& tmax, tstack& st, int sp_base, wcount& wc, Leaf&& leaf) {
+__device__ __forceinline__ void walk(const fnode* fnodes, const croot& root, const v3& o, const v3& d, double tmin,
+                                     double& tmax, tstack& st, int sp_base, wcount& wc, Leaf&& leaf) {
     if constexpr (CNT) wc.nodes += 64u;
-    if (!box_hit(root.b, o, inv, tmin, tmax)) return;
-    if (root.cnt > 0) {
-        leaf(root.off, root.cnt, sp_base);
-        return;
+    fray fr;
+    {
+        const v3 inv{1 / d.x, 1 / d.y, 1 / d.z};
+        if (!box_hit(root.b, o, inv, tmin, tmax)) return;
+        if (root.cnt > 0) {
+            leaf(root.off, root.cnt, sp_base, -1);
+            return;
+        }
+        fr = make_fray(o, inv, tmin, root.b);
     }
+    // bit a set when the ray goes towards -a (inv_dir < 0; the single-precision
+    // reciprocal has the double's sign)
+    const int negmask = (fr.inv[0] < 0 ? 1 : 0) | (fr.inv[1] < 0 ? 2 : 0) | (fr.inv[2] < 0 ? 4 : 0);
     int sp = sp_base;
     int cur = root.off;
     while (true) {
-        const cnode n = load_cnode(nodes, cur);
-        if constexpr (CNT) wc.nodes += 128u;
-        const bool h0 = box_hit(n.b, o, inv, tmin, tmax);
-        const bool h1 = box_hit(n.b + 6, o, inv, tmin, tmax);
-        const int nr = (negmask >> n.axis) & 1;  // the right child is nearer along -axis
+        const fnode n = load_fnode(fnodes, cur);
+        if constexpr (CNT) wc.nodes += 64u;
+        const float tf = f32_up(tmax);
+        const bool h0 = fbox_hit(n.b, fr, tf);
+        const bool h1 = fbox_hit(n.b + 6, fr, tf);
+        const int axis = int(uint32_t(n.cnt[0]) >> 30);
+        const int nr = (negmask >> axis) & 1;  // the right child is nearer along -axis
         if (nr ? h0 : h1) st.put(sp++, (cur << 1) | (1 - nr));
         int next = -1;
         if (nr ? h1 : h0) {
             const int32_t noff = nr ? n.off[1] : n.off[0];
-            const int32_t ncnt = nr ? n.cnt[1] : n.cnt[0];
-            if (ncnt > 0) leaf(noff, ncnt, sp);
+            const int32_t ncnt = nr ? n.cnt[1] : (n.cnt[0] & 0x3fffffff);
+            if (ncnt > 0) leaf(noff, ncnt, sp, (cur << 1) | nr);
             else next = noff;
         }
         while (next < 0) {
             if (sp == sp_base) return;
             const int e = st.get(--sp);
-            double b[6];
+            float b[6];
             int32_t off, cnt;
-            load_child(nodes, e >> 1, e & 1, b, off, cnt);
-            if constexpr (CNT) wc.nodes += 64u;
-            if (!box_hit(b, o, inv, tmin, tmax)) continue;
-            if (cnt > 0) leaf(off, cnt, sp);
+            load_fchild(fnodes, e >> 1, e & 1, b, off, cnt);
+            if constexpr (CNT) wc.nodes += 32u;
+            if (!fbox_hit(b, fr, f32_up(tmax))) continue;
+            if (cnt > 0) leaf(off, cnt, sp, e);
             else next = off;
         }
         cur = next;
@@ -324,20 +431,23 @@ __device__ __forceinline__ bool list_t(const dscene& S, int64_t first, int64_t c
 template <bool CNT, int FEAT>
 __device__ bool mesh_t(const dscene& S, int root, const v3& o, const v3& d, double tmin, double tmax, double& t,
                        int32_t& which, tstack& st, int sp_base, wcount& wc) {
-    v3 inv{1 / d.x, 1 / d.y, 1 / d.z};
-    const int negmask = (inv.x < 0 ? 1 : 0) | (inv.y < 0 ? 2 : 0) | (inv.z < 0 ? 4 : 0);
     bool any = false;
-    walk<CNT>(S.blas, S.blas_roots[root], o, inv, negmask, tmin, tmax, st, sp_base, wc, [&](int off, int cnt, int) {
-        for (int k = 0; k < cnt; k++) {
-            double tt;
-            if constexpr (CNT) wc.pbytes += 72u;
-            if (tri_t(prim_ptr(S, off + k), o, d, tmin, tmax, tt)) {
-                any = true;
-                tmax = tt;
-                which = off + k;
-            }
-        }
-    });
+    walk<CNT>(S.fblas, S.blas_roots[root], o, d, tmin, tmax, st, sp_base, wc,
+              [&](int off, int cnt, int, int e) {
+                  const double tleaf = tmax;
+                  int ok = -1;  // the leaf's double test, made at its first hit
+                  for (int k = 0; k < cnt; k++) {
+                      double tt;
+                      if constexpr (CNT) wc.pbytes += 72u;
+                      if (tri_t(prim_ptr(S, off + k), o, d, tmin, tmax, tt)) {
+                          if (ok < 0) ok = leaf_exact<CNT>(S.blas, e, o, d, tmin, tleaf, wc) ? 1 : 0;
+                          if (!ok) break;
+                          any = true;
+                          tmax = tt;
+                          which = off + k;
+                      }
+                  }
+              });
     t = tmax;
     return any;
 }
@@ -436,15 +546,17 @@ __device__ bool trace_closest(const dscene& S, const v3& o, const v3& d, double 
     h.prim = -1;
     h.t = tmax;
     if (S.nobj == 0) return ground_t<FEAT>(S, o, d, tmin, tmax, h);
-    v3 inv{1 / d.x, 1 / d.y, 1 / d.z};
-    const int negmask = (inv.x < 0 ? 1 : 0) | (inv.y < 0 ? 2 : 0) | (inv.z < 0 ? 4 : 0);
     bool any = false;
-    walk<CNT>(S.tlas, S.tlas_root, o, inv, negmask, tmin, tmax, st, 0, wc, [&](int off, int cnt, int sp) {
+    walk<CNT>(S.ftlas, S.tlas_root, o, d, tmin, tmax, st, 0, wc, [&](int off, int cnt, int sp, int e) {
+        const double tleaf = tmax;
+        int ok = -1;  // the leaf's double test, made at its first hit
         for (int k = 0; k < cnt; k++) {
             int oi = S.titems[off + k];
             double tt;
             int32_t pr = -1;
             if (object_t<CNT, FEAT>(S, oi, o, d, tmin, tmax, tt, pr, st, sp, media, mkey, serial, wc)) {
+                if (ok < 0) ok = leaf_exact<CNT>(S.tlas, e, o, d, tmin, tleaf, wc) ? 1 : 0;
+                if (!ok) break;
                 any = true;
                 tmax = tt;
                 h.t = tt;

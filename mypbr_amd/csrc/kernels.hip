@@ -176,8 +176,13 @@ constexpr int64_t kFieldStride = MPT_PATH_RECORDS ? kRecDoubles : 1;  // doubles
 
 // Flags (in D_FLAGCTR's high word): bit0 specular, bit1 seen_by_shadow_ray,
 // 10-bit bounce counters from bits 2, 12, 22.
+// One allocation holds every field: with records, record r of slot i at
+// base + (r * cap + i) * 8; without, field f of slot i at base + f * cap + i.
+// (A pointer per field would cost the stage kernels 128 scalar registers of
+// arguments.)
 struct pstate {
-    double* d[D_COUNT];
+    double* base;
+    int64_t cap;
     uint64_t* key;
 };
 
@@ -185,7 +190,10 @@ __device__ __forceinline__ double as_double(uint64_t b) { return __longlong_as_d
 __device__ __forceinline__ uint64_t as_bits(double v) { return static_cast<uint64_t>(__double_as_longlong(v)); }
 
 // Field f of slot i.
-__device__ __forceinline__ double& pd(const pstate& P, int f, uint32_t i) { return P.d[f][int64_t(i) * kFieldStride]; }
+__device__ __forceinline__ double& pd(const pstate& P, int f, uint32_t i) {
+    if constexpr (MPT_PATH_RECORDS) return P.base[(int64_t(f / kRecDoubles) * P.cap + i) * kRecDoubles + f % kRecDoubles];
+    return P.base[int64_t(f) * P.cap + i];
+}
 __device__ __forceinline__ v3 ld3(const pstate& P, int f, uint32_t i) { return v3{pd(P, f, i), pd(P, f + 1, i), pd(P, f + 2, i)}; }
 __device__ __forceinline__ void st3(const pstate& P, int f, uint32_t i, const v3& v) {
     pd(P, f, i) = v.x;
@@ -495,12 +503,11 @@ __global__ __launch_bounds__(BS) void k_features(dscene S, launch_ctx L, pstate 
 // Each class gets its own queue, appended with one atomic per wave.
 // CNT: also count BVH work (nodes, primitive bytes) into work[0], work[1].
 // Waves per SIMD the spheres-only traversal is compiled for (its register
-// budget): 6 gives it 80 VGPRs and no spills; at 8 (64 VGPRs) the slab
-// test's reciprocal direction spills inside the traversal loop, and that
-// costs more than the two waves it gains (1296 against 1410 M paths/s on
-// the 1080p Weekend cover).
+// budget). With the single-precision culls the kernel needs about 90 VGPRs
+// without spills: 5 waves (96 VGPRs) gave 1587 M paths/s on the 1080p
+// Weekend cover, 6 waves (80 VGPRs, 13 spilled) 1575 M and 7 waves 1457 M.
 #ifndef MPT_SPHERE_WAVES
-#define MPT_SPHERE_WAVES 6
+#define MPT_SPHERE_WAVES 5
 #endif
 
 // Closest hit of path p's ray and the hit record shade reads (point, face
@@ -1243,6 +1250,8 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
     if ((rc = upload(s.get(), h.ctlas, &d.tlas))) return rc;
     if ((rc = upload(s.get(), h.tlas_items, &d.titems))) return rc;
     if ((rc = upload(s.get(), h.cblas, &d.blas))) return rc;
+    if ((rc = upload(s.get(), h.ftlas, &d.ftlas))) return rc;
+    if ((rc = upload(s.get(), h.fblas, &d.fblas))) return rc;
     if ((rc = upload(s.get(), h.cblas_roots, &d.blas_roots))) return rc;
     if ((rc = upload(s.get(), h.spheres, &d.sph))) return rc;
     d.tlas_root = h.ctlas_root;
@@ -1489,10 +1498,8 @@ int ensure_paths(mpt_scene* s, int64_t cap) {
     // the (u, v) record only where a material reads it
     const int fields = s->hs.uses_uv ? D_COUNT : kUvRecord * kRecDoubles;
     if ((rc = alloc(size_t(cap) * fields * sizeof(double), reinterpret_cast<void**>(&base)))) return rc;
-    for (int f = 0; f < D_COUNT; f++)  // record r of slot i at base + (r * cap + i) * 8, or field f at base + f * cap
-        s->ps.d[f] = f >= fields ? nullptr
-                     : MPT_PATH_RECORDS ? base + (int64_t(f / kRecDoubles) * cap) * kRecDoubles + f % kRecDoubles
-                                        : base + int64_t(f) * cap;
+    s->ps.base = base;  // fields beyond `fields` are never touched
+    s->ps.cap = cap;
     if ((rc = alloc(size_t(cap) * sizeof(uint64_t), reinterpret_cast<void**>(&s->ps.key)))) return rc;
     HIP_TRY(hipMalloc(&s->queues, size_t(cap) * (3 + kClasses) * sizeof(uint32_t)));
     if (!s->counters) HIP_TRY(hipMalloc(&s->counters, kCtrWords * sizeof(uint32_t) + 16 * sizeof(uint64_t)));
@@ -1843,7 +1850,7 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
     if (graphs) {  // graphs captured earlier hold kernel arguments: drop them if any has changed
         std::vector<unsigned char> key(sizeof(mpt_camera));
         std::memcpy(key.data(), cam, sizeof(mpt_camera));
-        const void* ptrs[] = {s->queues, s->counters, s->ovf, P.d[0], P.key};
+        const void* ptrs[] = {s->queues, s->counters, s->ovf, P.base, P.key};
         const int64_t vals[] = {cap, int64_t(G), s->profile, s->light_profile, int64_t(s->mat_types), has_env,
                                 s->has_lights};
         key.insert(key.end(), reinterpret_cast<const unsigned char*>(ptrs), reinterpret_cast<const unsigned char*>(ptrs) + sizeof(ptrs));

@@ -211,6 +211,31 @@ void pair_layout(const std::vector<bvh_node>& nodes, int root, std::vector<cnode
     }
 }
 
+namespace {
+// x rounded to single precision towards -inf / +inf
+float round_down(double x) {
+    float f = float(x);
+    return double(f) > x ? std::nextafterf(f, -INFINITY) : f;
+}
+float round_up(double x) {
+    float f = float(x);
+    return double(f) < x ? std::nextafterf(f, INFINITY) : f;
+}
+}  // namespace
+
+void float_layout(const std::vector<cnode>& in, std::vector<fnode>& out) {
+    out.resize(in.size());
+    for (size_t i = 0; i < in.size(); i++) {
+        const cnode& c = in[i];
+        fnode& f = out[i];
+        for (int k = 0; k < 12; k++) f.b[k] = (k & 1) ? round_up(c.b[k]) : round_down(c.b[k]);
+        f.off[0] = c.off[0];
+        f.off[1] = c.off[1];
+        f.cnt[0] = c.cnt[0] | (c.axis << 30);
+        f.cnt[1] = c.cnt[1];
+    }
+}
+
 void build_ggx_table(std::vector<double>& e, std::vector<double>& e_avg) {  // microfacet.h:299-337
     const int size = kGgxTableSize;
     const int strata = 64;
@@ -564,6 +589,8 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
     build_bvh(items, s.tlas);
     s.ctlas.clear();
     pair_layout(s.tlas, 0, s.ctlas, s.ctlas_root);
+    float_layout(s.ctlas, s.ftlas);
+    float_layout(s.cblas, s.fblas);
     s.tlas_items.resize(no);
     for (int o = 0; o < no; o++) s.tlas_items[o] = int32_t(items[o].ref);
     s.world_box = box_empty();

@@ -37,6 +37,23 @@ struct cnode {
 };
 static_assert(sizeof(cnode) == 128, "cnode must be 128 bytes");
 
+// The same records in single precision for the traversal's box culls, 64
+// bytes: both children's boxes rounded outward (minima down, maxima up), the
+// children's item ranges or records, and the split axis in the top two bits
+// of cnt[0]. A cull against these boxes never rejects a box the double
+// precision test accepts; the double boxes of the cnode records decide the
+// leaves (the traversal checks a leaf's double box when the leaf first
+// yields a hit).
+struct fnode {
+    float b[12];
+    int32_t off[2];
+    int32_t cnt[2];    // cnt[0]: left count | axis << 30; cnt[1]: right count
+};
+static_assert(sizeof(fnode) == 64, "fnode must be 64 bytes");
+
+// Single-precision copies of child-pair records (same indices).
+void float_layout(const std::vector<cnode>& in, std::vector<fnode>& out);
+
 // Root of a re-laid tree: its box and what it is (as a cnode child).
 struct croot {
     double b[6];
@@ -90,6 +107,7 @@ struct host_scene {
     croot ctlas_root{};
     std::vector<cnode> cblas;
     std::vector<croot> cblas_roots;      // indexed by obj_rec.blas_root
+    std::vector<fnode> ftlas, fblas;     // ctlas / cblas in single precision, for the culls
     std::vector<int32_t> blas_root_node; // cblas_roots index -> root in blas (for depths)
     std::vector<double> spheres;         // center xyz, radius of every primitive (spheres only meaningful)
     std::vector<xform> xforms;
