@@ -270,7 +270,8 @@ typedef struct mpt_render_stats {
     int64_t stage_launches[5];
     /* with profile bit 1: BVH node bytes and primitive bytes read by the
      * extension [0] and shadow [1] traversals (a root box counts 64 bytes,
-     * an interior child-pair record 128, a child re-tested on pop 64) */
+     * an interior child-pair record 64, a child re-tested on pop 32, a
+     * leaf's double-precision box checked at its first hit 48) */
     int64_t node_bytes[2];
     int64_t prim_bytes[2];
     /* extension [0] and shadow [1] rays traced by the batch tails (included
@@ -345,6 +346,21 @@ int32_t mpt_trace_closest_dev(mpt_scene* scene, const double* rays_dev, const do
 int32_t mpt_render(mpt_scene* scene, const mpt_camera* cam, const mpt_render_opts* opts,
                    double* radiance, double* albedo, double* normal, double* variance,
                    mpt_pass_callback on_pass, void* user, mpt_render_stats* stats);
+
+/* Several scenes rendered together: every stage launch (camera, extend,
+ * shade, shadow, tail, accumulate) serves the paths of all n scenes (1 to
+ * 16), scene k with cameras[k]. It replaces rendering the scenes one after
+ * another with camera::render (camera.h:79-105, whose tile loop over one
+ * image, camera.h:92-138, becomes one pool of paths over all the images).
+ * Scene k's paths use scene index opts->scene_index + k, so its frame is the
+ * frame mpt_render of that scene alone gives with that scene_index; read it
+ * with mpt_read_frame(scenes[k], ...). Every scene renders the same number
+ * of passes. Feature guides, save_every, the one-kernel schedule and scenes
+ * with image textures, maps, thin glass, measured metals or the ground plane
+ * are not supported here (MPT_E_UNSUPPORTED); the scenes must share their
+ * kind of lights. stats covers the whole batch. */
+int32_t mpt_render_batch(mpt_scene* const* scenes, const mpt_camera* cameras, int32_t n,
+                         const mpt_render_opts* opts, mpt_render_stats* stats);
 
 /* What mpt_autotune chose. */
 typedef struct mpt_autotune_result {

@@ -23,7 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # MPT_LIBRARY selects another build of the same library (tuning experiments)
 LIB_PATH = os.environ.get("MPT_LIBRARY", os.path.join(HERE, "libmpt.so"))
 
-__all__ = ["Scene", "Frame", "MptError", "lib", "abi", "Camera", "RenderOpts", "device_info", "denoise", "DeviceArray", "synchronize",
+__all__ = ["Scene", "Frame", "render_batch", "MptError", "lib", "abi", "Camera", "RenderOpts", "device_info", "denoise", "DeviceArray", "synchronize",
            "ggx_albedo_table", "describe"]
 
 
@@ -56,6 +56,7 @@ def _load():
         "mpt_render": (C.c_int32, [vp, P(Camera), P(RenderOpts), P(C.c_double), P(C.c_double), P(C.c_double),
                                    P(C.c_double), abi.PASS_CALLBACK, vp, P(RenderStats)]),
         "mpt_read_frame": (C.c_int32, [vp, P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double)]),
+        "mpt_render_batch": (C.c_int32, [P(vp), P(Camera), C.c_int32, P(RenderOpts), P(RenderStats)]),
         "mpt_image_height": (C.c_int32, [P(Camera)]),
         "mpt_ggx_albedo_table": (C.c_int32, [P(C.c_double), P(C.c_double)]),
         "mpt_scene_light_probabilities": (C.c_int32, [vp, P(C.c_double)]),
@@ -273,6 +274,27 @@ class Description:
 
 def describe(path):
     return Description(path)
+
+
+def render_batch(scenes, cameras=None, seed=1, first_pass=0, scene_index=0, pass_count=0, accumulate=False,
+                 passes_in_flight=0, profile=0, tail_paths=0, blocks_per_cu=0, read=True):
+    """mpt_render_batch: the scenes rendered together, every stage launch
+    serving all of them; scene k keyed with scene index scene_index + k.
+    Returns (frames, stats): frames[k] is scene k's averaged radiance (or
+    None with read=False; read it later with Scene.read_frame)."""
+    scenes = list(scenes)
+    cams = [abi.camera_copy(c if c is not None else s.camera)
+            for s, c in zip(scenes, cameras if cameras is not None else [None] * len(scenes))]
+    n = len(scenes)
+    handles = (C.c_void_p * n)(*[s._h for s in scenes])
+    cam_arr = (Camera * n)(*cams)
+    opts = RenderOpts(seed=seed, first_pass=first_pass, scene_index=scene_index, pass_count=pass_count,
+                      accumulate=int(bool(accumulate)), passes_in_flight=passes_in_flight, profile=profile,
+                      tail_paths=tail_paths, blocks_per_cu=blocks_per_cu)
+    stats = RenderStats()
+    _check(lib.mpt_render_batch(handles, cam_arr, n, C.byref(opts), C.byref(stats)))
+    frames = [s.read_frame(c) for s, c in zip(scenes, cams)] if read else [None] * n
+    return frames, stats
 
 
 class Frame:

@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -308,6 +309,67 @@ struct nrc_route {
     uint32_t* cnt;
 };
 
+// Several scenes in one batch (mpt_render_batch): every stage launch serves
+// the paths of all of them. Scene k owns slots [base[k], base[k + 1]) of the
+// batch (its passes of this batch, pixel-major within a pass, as in a
+// one-scene batch) and pixels [pbase[k], pbase[k + 1]) of the accumulation
+// launch; a path's scene and camera are looked up from its slot.
+constexpr int kMaxBatch = 16;
+struct mbatch {
+    const dscene* S;      // device arrays of n scenes and cameras
+    const dcam* C;
+    int32_t n;
+    int32_t npix[kMaxBatch];
+    uint32_t base[kMaxBatch + 1];
+    uint32_t pbase[kMaxBatch + 1];
+    double* acc[kMaxBatch];  // each scene's running sums (radiance, albedo, normal, l2)
+    __device__ __forceinline__ int scene_of(uint32_t slot) const {
+        int k = 0;
+#pragma unroll
+        for (int j = 1; j < kMaxBatch; j++) k += (j < n && slot >= base[j]) ? 1 : 0;
+        return k;
+    }
+    __device__ __forceinline__ int scene_of_pixel(uint32_t g) const {
+        int k = 0;
+#pragma unroll
+        for (int j = 1; j < kMaxBatch; j++) k += (j < n && g >= pbase[j]) ? 1 : 0;
+        return k;
+    }
+};
+
+// The scene and camera tables read through the constant address space: a
+// load at a wave-uniform index then goes to scalar registers, as the fields
+// of a kernel argument do.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(4))) const dscene const_dscene;
+typedef __attribute__((address_space(4))) const dcam const_dcam;
+#else  // the host pass only parses device code; address spaces are device-side
+typedef const dscene const_dscene;
+typedef const dcam const_dcam;
+#endif
+
+// f(S, C) for the scene of this lane's slot, run for the wave's scenes one
+// at a time (usually one: scenes own contiguous slot ranges) with the scene
+// index uniform across the lanes that run together. The index is taken from
+// the lowest waiting lane (ballot, readlane) and the scene loaded before the
+// lane test, so the compiler cannot substitute the lane's own index.
+template <typename F>
+__device__ __forceinline__ void with_scene(const mbatch& M, uint32_t slot, F&& f) {
+    const int k = M.scene_of(slot);
+    bool pending = true;
+    while (true) {
+        const uint64_t m = __ballot(pending);
+        if (!m) break;
+        const int k0 = __builtin_amdgcn_readlane(k, __builtin_ctzll(m));
+        const dscene S = ((const_dscene*)M.S)[k0];
+        const dcam C = ((const_dcam*)M.C)[k0];
+        if (pending && k == k0) {
+            f(S, C, k0);
+            pending = false;
+        }
+    }
+}
+
 #define STACK_SETUP()                                                                   \
     __shared__ int32_t lds_stack[kLdsDepth * BS];                                       \
     tstack st;                                                                          \
@@ -362,12 +424,13 @@ __device__ __forceinline__ v3 random_in_unit_disk(rstream& rs) {  // vec3.h:139-
     }
 }
 
-// The camera ray of slot `slot` (camera.h:286-302 get_ray); sets the
-// slot's random stream, flags and first-queue entry.
+// The camera ray of slot `slot` (camera.h:286-302 get_ray), the `local`-th
+// path of its scene's part of the batch; sets the slot's random stream,
+// flags and first-queue entry.
 __device__ __forceinline__ void camera_ray(const dcam& C, const pstate& P, const batch_args& B, uint32_t slot,
-                                           uint32_t* q0, v3& org_out, v3& dir_out, double& flagctr) {
-    int pass = B.first_pass + int(slot / B.npix);
-    int pix = int(slot % B.npix);
+                                           uint32_t local, uint32_t* q0, v3& org_out, v3& dir_out, double& flagctr) {
+    int pass = B.first_pass + int(local / B.npix);
+    int pix = int(local % B.npix);
     int i = pix % C.w, j = pix / C.w;
     rstream rs{path_key(B.seed, B.scene_index, uint32_t(pass), uint32_t(pix)), 0};
     int strata = C.sqrt_spp * C.sqrt_spp;
@@ -410,8 +473,9 @@ __device__ __forceinline__ void camera_ray(const dcam& C, const pstate& P, const
     q0[slot] = slot;
 }
 
+template <bool MULTI = false>
 __global__ __launch_bounds__(256) void k_camera(dcam C, pstate P, batch_args B, uint32_t* q0, uint32_t* cnt,
-                                                int lights) {
+                                                int lights, mbatch M) {
     __shared__ double lds[256 * kRecDoubles];
     int64_t gt = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (gt == 0) {
@@ -424,7 +488,18 @@ __global__ __launch_bounds__(256) void k_camera(dcam C, pstate P, batch_args B, 
         const int64_t s0 = int64_t(blockIdx.x) * blockDim.x;
         v3 org{0, 0, 0}, dir{0, 0, 0};
         double fc = 0;
-        if (gt < B.nslots) camera_ray(C, P, B, uint32_t(gt), q0, org, dir, fc);
+        if (gt < B.nslots) {
+            if constexpr (MULTI) {  // slot gt is path gt - base[k] of scene k, whose key has scene index + k
+                with_scene(M, uint32_t(gt), [&](const dscene&, const dcam& CC, int k) {
+                    batch_args Bk = B;
+                    Bk.scene_index = B.scene_index + uint32_t(k);
+                    Bk.npix = M.npix[k];
+                    camera_ray(CC, P, Bk, uint32_t(gt), uint32_t(gt) - M.base[k], q0, org, dir, fc);
+                });
+            } else {
+                camera_ray(C, P, B, uint32_t(gt), uint32_t(gt), q0, org, dir, fc);
+            }
+        }
         double* my = lds + threadIdx.x * kRecDoubles;
         // record 0: ray, scatter pdf, flags and counter
         my[0] = org.x, my[1] = org.y, my[2] = org.z, my[3] = dir.x, my[4] = dir.y, my[5] = dir.z, my[6] = 0, my[7] = fc;
@@ -441,7 +516,7 @@ __global__ __launch_bounds__(256) void k_camera(dcam C, pstate P, batch_args B, 
     const uint32_t slot = uint32_t(gt);
     v3 org, dir;
     double fc;
-    camera_ray(C, P, B, slot, q0, org, dir, fc);
+    camera_ray(C, P, B, slot, slot, q0, org, dir, fc);
     st3(P, D_OX, slot, org);
     st3(P, D_DX, slot, dir);
     pd(P, D_FLAGCTR, slot) = fc;
@@ -509,6 +584,14 @@ __global__ __launch_bounds__(BS) void k_features(dscene S, launch_ctx L, pstate 
 #ifndef MPT_SPHERE_WAVES
 #define MPT_SPHERE_WAVES 5
 #endif
+// Waves per SIMD of the general-geometry traversal kernels (meshes,
+// instances, media). Uncapped they take about 200 VGPRs (255 with an
+// environment in the shadow kernel) and run 1-2 waves per SIMD; capped at 3
+// the batch of 8 generated scenes went from 9.7 to 15.7 M paths/s (4 waves:
+// 15.5 M), since their traversal waits on HBM and needs waves in flight.
+#ifndef MPT_GENERAL_WAVES
+#define MPT_GENERAL_WAVES 3
+#endif
 
 // Closest hit of path p's ray and the hit record shade reads (point, face
 // normal, material); returns the path's shading class.
@@ -543,12 +626,12 @@ __device__ __forceinline__ bool nrc_training(uint32_t slot, const nrc_route& R) 
     return h % R.stride == 0;
 }
 
-template <int BS, bool CNT, int FEAT>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_SPHERE ? MPT_SPHERE_WAVES : 1))) void k_extend(dscene S, launch_ctx L, pstate P, int depth, const uint32_t* q,
+template <int BS, bool CNT, int FEAT, bool MULTI = false>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_SPHERE ? MPT_SPHERE_WAVES : (FEAT == F_ALL ? MPT_GENERAL_WAVES : 1)))) void k_extend(dscene S, launch_ctx L, pstate P, int depth, const uint32_t* q,
                                                const uint32_t* cnt_in, uint32_t* cnt_next, uint32_t* cnt_shadow,
                                                uint32_t* qcls, uint32_t* ccls, int64_t cap,
                                                unsigned long long* ray_stats, unsigned long long* work,
-                                               nrc_route R) {
+                                               nrc_route R, mbatch M) {
     STACK_SETUP();
     const uint32_t n = *cnt_in;
     if (st.gtid == 0) {  // the counters shade and shadow append to in this bounce
@@ -567,7 +650,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_SP
         uint32_t p = 0;
         if (i < n) {
             p = q[i];
-            cls = extend_one<CNT, FEAT>(S, P, depth, p, st, wc);
+            if constexpr (MULTI)
+                with_scene(M, p, [&](const dscene& SS, const dcam&, int) { cls = extend_one<CNT, FEAT>(SS, P, depth, p, st, wc); });
+            else cls = extend_one<CNT, FEAT>(S, P, depth, p, st, wc);
         }
         uint32_t bits = cls >= 0 ? 1u << cls : 0u;
         if (query && cls > 0) {  // a hit at the cache's bounce: queried; shaded only if it trains
@@ -728,10 +813,10 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
 #define MPT_SHADE_WAVES 1
 #endif
 
-template <int BS, int MT, int LK, bool ENV, bool RICH = false>
+template <int BS, int MT, int LK, bool ENV, bool RICH = false, bool MULTI = false>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MPT_SHADE_WAVES))) void k_shade(dscene S, dcam C, pstate P, int depth, const uint32_t* q,
                                               const uint32_t* cnt_in, uint32_t* qnext, uint32_t* cnt_next,
-                                              uint32_t* qshadow, uint32_t* cnt_shadow) {
+                                              uint32_t* qshadow, uint32_t* cnt_shadow, mbatch M) {
     constexpr int NQ = LK != 0 ? 2 : 1;
     __shared__ uint32_t lds[NQ * (BS / 64 + 1)];
     const uint32_t n = *cnt_in;
@@ -741,7 +826,13 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MPT_SHADE_WA
         uint32_t p = 0;
         if (i < n) {
             p = q[i];
-            shade_one<MT, LK, ENV, RICH>(S, C, P, depth, p, alive, nee);
+            if constexpr (MULTI) {
+                with_scene(M, p, [&](const dscene& SS, const dcam& CC, int) {
+                    shade_one<MT, LK, ENV, RICH>(SS, CC, P, depth, p, alive, nee);
+                });
+            } else {
+                shade_one<MT, LK, ENV, RICH>(S, C, P, depth, p, alive, nee);
+            }
         }
         block_append<BS, NQ>(
             lds, (alive ? 1u : 0u) | (nee ? 2u : 0u), p, [&](int k) { return k ? cnt_shadow : cnt_next; },
@@ -789,10 +880,10 @@ __device__ __forceinline__ void shadow_one(const dscene& S, const dcam& C, const
     }
 }
 
-template <int BS, bool CNT, int FEAT, bool ENV>
-__global__ __launch_bounds__(BS) void k_shadow(dscene S, dcam C, launch_ctx L, pstate P, int depth,
+template <int BS, bool CNT, int FEAT, bool ENV, bool MULTI = false>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_ALL ? MPT_GENERAL_WAVES : 1))) void k_shadow(dscene S, dcam C, launch_ctx L, pstate P, int depth,
                                                const uint32_t* qshadow, const uint32_t* cnt_shadow, uint32_t* ccls,
-                                               unsigned long long* ray_stats, unsigned long long* work) {
+                                               unsigned long long* ray_stats, unsigned long long* work, mbatch M) {
     STACK_SETUP();
     const uint32_t n = *cnt_shadow;
     if (st.gtid == 0) {
@@ -800,7 +891,16 @@ __global__ __launch_bounds__(BS) void k_shadow(dscene S, dcam C, launch_ctx L, p
         for (int c = 0; c < kClasses; c++) ccls[c * kCtrStride] = 0;  // every shade kernel of this bounce has read them
     }
     wcount wc{0, 0};
-    for (int64_t i = st.gtid; i < n; i += L.nthreads) shadow_one<CNT, FEAT, ENV>(S, C, P, depth, qshadow[i], st, wc);
+    for (int64_t i = st.gtid; i < n; i += L.nthreads) {
+        const uint32_t p = qshadow[i];
+        if constexpr (MULTI) {
+            with_scene(M, p, [&](const dscene& SS, const dcam& CC, int) {
+                shadow_one<CNT, FEAT, ENV>(SS, CC, P, depth, p, st, wc);
+            });
+        } else {
+            shadow_one<CNT, FEAT, ENV>(S, C, P, depth, p, st, wc);
+        }
+    }
     if constexpr (CNT) {
         if (wc.nodes) atomicAdd(&work[2], (unsigned long long)wc.nodes);
         if (wc.pbytes) atomicAdd(&work[3], (unsigned long long)wc.pbytes);
@@ -819,46 +919,59 @@ __global__ __launch_bounds__(BS) void k_shadow(dscene S, dcam C, launch_ctx L, p
 #define MPT_FINISH_WAVES 2
 #endif
 
-template <int BS, int FEAT, int LK, bool ENV>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MPT_FINISH_WAVES))) void k_finish(
-    dscene S, dcam C, launch_ctx L, pstate P, int depth0,
-                                               const uint32_t* q, const uint32_t* cnt_in,
-                                               unsigned long long* ray_stats) {
-    STACK_SETUP();
+// Every remaining bounce of path p from depth0 on (one thread).
+template <int FEAT, int LK, bool ENV>
+__device__ __forceinline__ void finish_path(const dscene& S, const dcam& C, const pstate& P, int depth0, uint32_t p,
+                                            tstack& st, wcount& wc, unsigned long long& ext,
+                                            unsigned long long& shd) {
     constexpr bool kUV = (FEAT & F_UV) != 0;
+    for (int depth = depth0; depth < C.max_depth; depth++) {
+        const int cls = extend_one<false, FEAT>(S, P, depth, p, st, wc);
+        ext++;
+        bool alive = false, nee = false;
+        switch (cls) {
+            case 0: shade_one<-1, LK, ENV>(S, C, P, depth, p, alive, nee); break;
+            case 1 + MPT_MAT_LAMBERTIAN:
+                shade_one<MPT_MAT_LAMBERTIAN, LK, false, kUV>(S, C, P, depth, p, alive, nee);
+                break;
+            case 1 + MPT_MAT_METAL: shade_one<MPT_MAT_METAL, LK, false, kUV>(S, C, P, depth, p, alive, nee); break;
+            case 1 + MPT_MAT_DIELECTRIC:
+                shade_one<MPT_MAT_DIELECTRIC, LK, false, kUV>(S, C, P, depth, p, alive, nee);
+                break;
+            case 1 + MPT_MAT_PRINCIPLED:
+                shade_one<MPT_MAT_PRINCIPLED, LK, false, kUV>(S, C, P, depth, p, alive, nee);
+                break;
+            case 1 + MPT_MAT_DIFFUSE_LIGHT:
+                shade_one<MPT_MAT_DIFFUSE_LIGHT, LK, false>(S, C, P, depth, p, alive, nee);
+                break;
+            default: shade_one<MPT_MAT_PHASE, LK, false>(S, C, P, depth, p, alive, nee); break;
+        }
+        if constexpr (LK != 0) {
+            if (nee) {
+                shadow_one<false, FEAT, ENV>(S, C, P, depth, p, st, wc);
+                shd++;
+            }
+        }
+        if (!alive) break;
+    }
+}
+
+template <int BS, int FEAT, int LK, bool ENV, bool MULTI = false>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MPT_FINISH_WAVES))) void k_finish(
+    dscene S, dcam C, launch_ctx L, pstate P, int depth0, const uint32_t* q, const uint32_t* cnt_in,
+    unsigned long long* ray_stats, mbatch M) {
+    STACK_SETUP();
     const uint32_t n = *cnt_in;
     unsigned long long ext = 0, shd = 0;
     wcount wc{0, 0};
     for (int64_t i = st.gtid; i < n; i += L.nthreads) {
         const uint32_t p = q[i];
-        for (int depth = depth0; depth < C.max_depth; depth++) {
-            const int cls = extend_one<false, FEAT>(S, P, depth, p, st, wc);
-            ext++;
-            bool alive = false, nee = false;
-            switch (cls) {
-                case 0: shade_one<-1, LK, ENV>(S, C, P, depth, p, alive, nee); break;
-                case 1 + MPT_MAT_LAMBERTIAN:
-                    shade_one<MPT_MAT_LAMBERTIAN, LK, false, kUV>(S, C, P, depth, p, alive, nee);
-                    break;
-                case 1 + MPT_MAT_METAL: shade_one<MPT_MAT_METAL, LK, false, kUV>(S, C, P, depth, p, alive, nee); break;
-                case 1 + MPT_MAT_DIELECTRIC:
-                    shade_one<MPT_MAT_DIELECTRIC, LK, false, kUV>(S, C, P, depth, p, alive, nee);
-                    break;
-                case 1 + MPT_MAT_PRINCIPLED:
-                    shade_one<MPT_MAT_PRINCIPLED, LK, false, kUV>(S, C, P, depth, p, alive, nee);
-                    break;
-                case 1 + MPT_MAT_DIFFUSE_LIGHT:
-                    shade_one<MPT_MAT_DIFFUSE_LIGHT, LK, false>(S, C, P, depth, p, alive, nee);
-                    break;
-                default: shade_one<MPT_MAT_PHASE, LK, false>(S, C, P, depth, p, alive, nee); break;
-            }
-            if constexpr (LK != 0) {
-                if (nee) {
-                    shadow_one<false, FEAT, ENV>(S, C, P, depth, p, st, wc);
-                    shd++;
-                }
-            }
-            if (!alive) break;
+        if constexpr (MULTI) {
+            with_scene(M, p, [&](const dscene& SS, const dcam& CC, int) {
+                finish_path<FEAT, LK, ENV>(SS, CC, P, depth0, p, st, wc, ext, shd);
+            });
+        } else {
+            finish_path<FEAT, LK, ENV>(S, C, P, depth0, p, st, wc, ext, shd);
         }
     }
     // one atomic per wave for the ray counts
@@ -1019,6 +1132,25 @@ __global__ void k_accumulate(pstate P, int npix, int passes, int features, doubl
     }
 }
 
+// The per-pixel sums of a batch of several scenes (features off): a thread
+// per pixel of every scene, samples added in pass order as k_accumulate does.
+__global__ void k_accumulate_multi(pstate P, int passes, mbatch M) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= M.pbase[M.n]) return;
+    const int k = M.scene_of_pixel(g);
+    const uint32_t pix = g - M.pbase[k];
+    const uint32_t npix = uint32_t(M.npix[k]);
+    double* acc_rad = M.acc[k];
+    v3 a = v3{acc_rad[3 * pix], acc_rad[3 * pix + 1], acc_rad[3 * pix + 2]};
+    for (int b = 0; b < passes; b++) {
+        const uint32_t slot = M.base[k] + uint32_t(b) * npix + pix;
+        v3 c = ld3(P, D_RX, slot);
+        if (!std::isfinite(c.x) || !std::isfinite(c.y) || !std::isfinite(c.z)) c = v3{0, 0, 0};  // camera.h:281-287
+        a = a + c;
+    }
+    acc_rad[3 * pix] = a.x, acc_rad[3 * pix + 1] = a.y, acc_rad[3 * pix + 2] = a.z;
+}
+
 // ------------------------------------------------------------------ BSDF test hook
 __global__ void k_bsdf_eval(dscene S, const int32_t* mats, const double* q, const uint8_t* front, int64_t n,
                             double* eval, double* pdf) {
@@ -1090,6 +1222,9 @@ struct mpt_scene {
     int64_t nrc_capacity = 0;
     nrc_bufs nb{};
     uint32_t* nrc_queue = nullptr;
+    // the scene and camera tables of batched renders (mpt_render_batch)
+    void* batch_tab = nullptr;
+    size_t batch_tab_bytes = 0;
 
     void drop_graphs() {
         for (hipGraphExec_t g : graphs)
@@ -1107,6 +1242,7 @@ struct mpt_scene {
         if (queues) (void)hipFree(queues);
         if (counters) (void)hipFree(counters);
         if (acc) (void)hipFree(acc);
+        if (batch_tab) (void)hipFree(batch_tab);
         if (stream) (void)hipStreamDestroy(stream);
         if (pinned_counts) (void)hipHostFree(pinned_counts);
     }
@@ -1741,15 +1877,16 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
     }
     const unsigned G = unsigned(s->grid);
     hipStream_t stream = s->stream;
+    const mbatch MB{};  // one scene: the kernels read S and C
     auto launch_extend = [&](int depth, uint32_t* qin, uint32_t* cin, uint32_t* cout) {
 #define EXT(FEAT)                                                                                                    \
     do {                                                                                                             \
         if (counting)                                                                                                \
             hipLaunchKernelGGL((k_extend<256, true, FEAT>), dim3(G), dim3(256), 0, stream, DS, L, P, depth, qin, cin, \
-                               cout, ctr(cnt, 2), qcls, ccls, cap, ray_stats, work, R);                                  \
+                               cout, ctr(cnt, 2), qcls, ccls, cap, ray_stats, work, R, MB);                              \
         else                                                                                                         \
             hipLaunchKernelGGL((k_extend<256, false, FEAT>), dim3(G), dim3(256), 0, stream, DS, L, P, depth, qin,    \
-                               cin, cout, ctr(cnt, 2), qcls, ccls, cap, ray_stats, work, R);                             \
+                               cin, cout, ctr(cnt, 2), qcls, ccls, cap, ray_stats, work, R, MB);                         \
     } while (0)
         if (s->profile == 1) EXT(F_SPHERE);
         else if (s->profile == 2) EXT(kQuadProfile);
@@ -1762,10 +1899,10 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
     do {                                                                                                            \
         if (counting)                                                                                               \
             hipLaunchKernelGGL((k_shadow<256, true, FEAT, ENV>), dim3(G), dim3(256), 0, stream, DS, C, L, P, depth,  \
-                               q[2], ctr(cnt, 2), ccls, ray_stats, work);                                               \
+                               q[2], ctr(cnt, 2), ccls, ray_stats, work, MB);                                               \
         else                                                                                                        \
             hipLaunchKernelGGL((k_shadow<256, false, FEAT, ENV>), dim3(G), dim3(256), 0, stream, DS, C, L, P, depth, \
-                               q[2], ctr(cnt, 2), ccls, ray_stats, work);                                               \
+                               q[2], ctr(cnt, 2), ccls, ray_stats, work, MB);                                               \
     } while (0)
 #define SHD(FEAT)                       \
     do {                                \
@@ -1784,7 +1921,7 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
         const uint32_t* cc = ccls + cls * kCtrStride;
 #define SHA3(MT, LK, ENV, RICH)                                                                                  \
     hipLaunchKernelGGL((k_shade<256, MT, LK, ENV, RICH>), dim3(G), dim3(256), 0, stream, DS, C, P, depth, qc, cc,    \
-                       qout, cout, q[2], ctr(cnt, 2))
+                       qout, cout, q[2], ctr(cnt, 2), MB)
     // in scenes with textures, maps, thin glass or measured metals the surface materials get the variant with them
 #define SHA2(MT, LK, ENV)                                                                                           \
     do {                                                                                                            \
@@ -1831,9 +1968,9 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
 #define FIN2(FEAT, LK)                                                                                        \
     do {                                                                                                      \
         if (has_env) hipLaunchKernelGGL((k_finish<256, FEAT, LK, true>), dim3(G), dim3(256), 0, stream, DS, C, \
-                                        L, P, depth, qin, cin, ray_stats);                                     \
+                                        L, P, depth, qin, cin, ray_stats, MB);                                     \
         else hipLaunchKernelGGL((k_finish<256, FEAT, LK, false>), dim3(G), dim3(256), 0, stream, DS, C, L, P, \
-                                depth, qin, cin, ray_stats);                                                   \
+                                depth, qin, cin, ray_stats, MB);                                                   \
     } while (0)
 #define FIN(FEAT)                         \
     do {                                  \
@@ -1903,8 +2040,8 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
         }
         batch_args B{o.seed, uint32_t(o.scene_index), o.first_pass + pass, npix, nb * npix};
         R.salt = uint32_t(mix64(o.seed ^ (uint64_t(uint32_t(o.first_pass + pass)) << 32)));
-        EV(3, hipLaunchKernelGGL(k_camera, dim3(unsigned((B.nslots + 255) / 256)), dim3(256), 0, stream, C, P, B, q[0],
-                                 cnt, int(s->has_lights)));
+        EV(3, hipLaunchKernelGGL(k_camera<false>, dim3(unsigned((B.nslots + 255) / 256)), dim3(256), 0, stream, C, P, B,
+                                 q[0], cnt, int(s->has_lights), MB));
         if (C.features) {
             if (s->profile == 3) EV(3, hipLaunchKernelGGL((k_features<256, F_ALL | F_UV>), dim3(G), dim3(256), 0, stream, DS, L, P, B));
             else EV(3, hipLaunchKernelGGL((k_features<256, F_ALL>), dim3(G), dim3(256), 0, stream, DS, L, P, B));
@@ -2033,12 +2170,352 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
 
 }  // namespace
 
+namespace {
+
+// Several scenes in one batch of launches (mpt_render_batch). The first
+// scene's path state, queues, counters, stack spill area and stream carry
+// the batch; each scene keeps its own running sums. A batch holds `nb`
+// passes of every scene; scene k's keys use scene index scene_index + k, so
+// its paths are the ones a render of that scene alone would trace.
+int render_batch_impl(mpt_scene* const* ss, const mpt_camera* cams, int n, const mpt_render_opts* opts,
+                      mpt_render_stats* stats) {
+    if (!ss || !cams || n < 1 || n > kMaxBatch)
+        return set_error(MPT_E_INVALID, "render batch: 1 to " + std::to_string(kMaxBatch) + " scenes");
+    if (!MPT_PATH_RECORDS) return set_error(MPT_E_UNSUPPORTED, "render batch: needs the path-record layout");
+    mpt_render_opts o{};
+    if (opts) o = *opts;
+    if (o.first_pass < 0 || o.pass_count < 0) return set_error(MPT_E_INVALID, "negative first_pass or pass_count");
+    if (o.save_every > 0) return set_error(MPT_E_UNSUPPORTED, "render batch: save_every is not supported");
+    if (o.sort_by_material < 0) return set_error(MPT_E_UNSUPPORTED, "render batch: runs binned stages only");
+    mpt_scene* s = ss[0];
+    for (int k = 0; k < n; k++) {
+        const mpt_scene* sk = ss[k];
+        const mpt_camera* c = &cams[k];
+        if (!sk) return set_error(MPT_E_INVALID, "render batch: null scene");
+        for (int j = 0; j < k; j++)
+            if (ss[j] == sk) return set_error(MPT_E_INVALID, "render batch: a scene appears twice");
+        if (c->image_width < 1 || !(c->aspect_ratio > 0) || c->samples_per_pixel < 1 || c->max_depth < 1 ||
+            c->max_depth > 1000 || c->max_diffuse < 0 || c->max_diffuse > 1000 || c->max_specular < 0 ||
+            c->max_specular > 1000 || c->max_scatter < 0 || c->max_scatter > 1000)
+            return set_error(MPT_E_INVALID, "render batch: bad camera settings for scene " + std::to_string(k));
+        if (c->collect_features) return set_error(MPT_E_UNSUPPORTED, "render batch: feature guides are not supported");
+        if (sk->profile == 3)
+            return set_error(MPT_E_UNSUPPORTED, "render batch: scenes with image textures, maps, thin glass, measured "
+                                                "metals or the ground plane render one at a time");
+        if (sk->device != s->device) return set_error(MPT_E_INVALID, "render batch: scenes on different devices");
+        if (sk->light_profile != s->light_profile || (sk->hs.env_w > 0) != (s->hs.env_w > 0))
+            return set_error(MPT_E_UNSUPPORTED, "render batch: scenes must share their light setup (same kind of "
+                                                "lights, all with or all without an environment)");
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<dcam> C(static_cast<size_t>(n));
+    std::vector<int64_t> npix(static_cast<size_t>(n));
+    int64_t pix_total = 0;
+    int total = -1;
+    uint32_t mat_types = 0;
+    for (int k = 0; k < n; k++) {
+        C[k] = init_camera(cams[k], ss[k]->hs.env_w > 0);
+        npix[k] = int64_t(C[k].w) * C[k].h;
+        pix_total += npix[k];
+        const int tk = o.pass_count > 0 ? o.pass_count : C[k].spp - o.first_pass;
+        if (tk < 1) return set_error(MPT_E_INVALID, "no passes to render (first_pass >= samples_per_pixel)");
+        if (total >= 0 && tk != total)
+            return set_error(MPT_E_INVALID, "render batch: every scene renders the same number of passes");
+        total = tk;
+        mat_types |= ss[k]->mat_types;
+    }
+    if (pix_total > (int64_t(1) << 28)) return set_error(MPT_E_CAPACITY, "render batch: too many pixels");
+    int pif = o.passes_in_flight > 0 ? o.passes_in_flight : int(std::max<int64_t>(1, (int64_t(16) << 20) / pix_total));
+    pif = std::min(pif, total);
+    if (int64_t(pif) * pix_total > (int64_t(1) << 31)) pif = int((int64_t(1) << 31) / pix_total);
+    int rc;
+    if ((rc = ensure_paths(s, int64_t(pif) * pix_total))) return rc;
+    const int bpc = o.blocks_per_cu > 0 ? o.blocks_per_cu : kDefaultBlocksPerCu;
+    if (bpc > 64) return set_error(MPT_E_INVALID, "blocks_per_cu must be in [1, 64]");
+    if (s->grid != s->cu_count * bpc) {
+        s->grid = s->cu_count * bpc;
+        if ((rc = ensure_ovf(s))) return rc;
+    }
+    for (int k = 1; k < n; k++) {  // the spill area must hold the deepest scene's stack
+        const int64_t threads = int64_t(s->grid) * s->block;
+        ss[k]->grid = s->grid;
+        if ((rc = ensure_ovf(ss[k]))) return rc;
+        if (ss[k]->ovf_depth > s->ovf_depth) {
+            HIP_TRY(hipFree(s->ovf));
+            s->ovf = nullptr;
+            HIP_TRY(hipMalloc(&s->ovf, size_t(ss[k]->ovf_depth) * threads * sizeof(int32_t)));
+            s->ovf_depth = ss[k]->ovf_depth;
+            s->ovf_threads = threads;
+        }
+    }
+    // each scene's sums, and the scene and camera tables the kernels read
+    for (int k = 0; k < n; k++) {
+        mpt_scene* sk = ss[k];
+        const bool keep = o.accumulate && sk->acc && sk->acc_pix == npix[k] && sk->last_done > 0 &&
+                          !sk->last_cam.collect_features;
+        if (o.accumulate && !keep && sk->last_done > 0)
+            return set_error(MPT_E_INVALID, "accumulate: the previous render had another size or feature setting");
+        if (sk->acc_pix != npix[k] || !sk->acc) {
+            if (sk->acc) HIP_TRY(hipFree(sk->acc));
+            sk->acc = nullptr;
+            HIP_TRY(hipMalloc(&sk->acc, size_t(npix[k]) * 10 * sizeof(double)));
+            sk->acc_pix = npix[k];
+        }
+        if (!keep) {
+            HIP_TRY(hipMemsetAsync(sk->acc, 0, size_t(npix[k]) * 10 * sizeof(double), s->stream));
+            sk->last_done = 0;
+        }
+    }
+    const size_t tab_bytes = size_t(n) * (sizeof(dscene) + sizeof(dcam));
+    if (s->batch_tab_bytes < tab_bytes) {
+        if (s->batch_tab) HIP_TRY(hipFree(s->batch_tab));
+        s->batch_tab = nullptr;
+        s->batch_tab_bytes = 0;
+        HIP_TRY(hipMalloc(&s->batch_tab, tab_bytes));
+        s->batch_tab_bytes = tab_bytes;
+    }
+    {
+        std::vector<unsigned char> tab(tab_bytes);
+        for (int k = 0; k < n; k++) std::memcpy(tab.data() + k * sizeof(dscene), &ss[k]->ds, sizeof(dscene));
+        std::memcpy(tab.data() + n * sizeof(dscene), C.data(), n * sizeof(dcam));
+        HIP_TRY(hipStreamSynchronize(s->stream));  // the table may still be read by the previous batch
+        HIP_TRY(hipMemcpy(s->batch_tab, tab.data(), tab_bytes, hipMemcpyHostToDevice));
+    }
+    mbatch M{};
+    M.S = static_cast<const dscene*>(s->batch_tab);
+    M.C = reinterpret_cast<const dcam*>(static_cast<const unsigned char*>(s->batch_tab) + n * sizeof(dscene));
+    M.n = n;
+    for (int k = 0; k < n; k++) {
+        M.npix[k] = int32_t(npix[k]);
+        M.pbase[k + 1] = M.pbase[k] + uint32_t(npix[k]);
+        M.acc[k] = ss[k]->acc;
+    }
+
+    const int64_t cap = s->ps_capacity;
+    uint32_t* q[3] = {s->queues, s->queues + cap, s->queues + 2 * cap};
+    uint32_t* qcls = s->queues + 3 * cap;
+    uint32_t* cnt = s->counters;
+    uint32_t* ccls = ctr(cnt, 3);
+    launch_ctx L{s->ovf, int64_t(s->grid) * s->block};
+    const pstate P = s->ps;
+    unsigned long long* ray_stats = reinterpret_cast<unsigned long long*>(s->counters + kCtrWords);
+    unsigned long long* work = ray_stats + 2;
+    HIP_TRY(hipMemsetAsync(ray_stats, 0, 8 * sizeof(unsigned long long), s->stream));
+    const bool timing = (o.profile & 1) != 0;
+    const bool counting = (o.profile & 2) != 0;
+    const bool has_env = s->hs.env_w > 0;
+    const bool has_lights = s->has_lights;
+    const int light_profile = s->light_profile;
+    const unsigned G = unsigned(s->grid);
+    hipStream_t stream = s->stream;
+    const dscene DS = s->ds;  // unused by the batch kernels (they read M.S)
+    const dcam C0 = C[0];
+    const nrc_route R{-1, 0, 1, 0, nullptr, nullptr};
+
+    // per-stage event timing, as render_impl
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
+    size_t ev_used = 0;
+    double stage_ms[5] = {0, 0, 0, 0, 0};
+    int64_t stage_n[5] = {0, 0, 0, 0, 0};
+    constexpr int kRing = 4;
+    hipEvent_t ring[kRing] = {};
+    auto ev_free = [&]() {
+        for (auto& e : evs) {
+            (void)hipEventDestroy(e.second.first);
+            (void)hipEventDestroy(e.second.second);
+        }
+        for (auto& e : ring)
+            if (e) (void)hipEventDestroy(e);
+    };
+    struct guard_t {
+        std::function<void()> f;
+        ~guard_t() { f(); }
+    } guard{ev_free};
+    for (int k = 0; k < kRing; k++) HIP_TRY(hipEventCreateWithFlags(&ring[k], hipEventDisableTiming));
+    auto ev_begin = [&](int stage) -> int {
+        if (!timing) return MPT_OK;
+        if (ev_used == evs.size()) {
+            hipEvent_t a = nullptr, b = nullptr;
+            HIP_TRY(hipEventCreate(&a));
+            HIP_TRY(hipEventCreate(&b));
+            evs.push_back({stage, {a, b}});
+        }
+        evs[ev_used].first = stage;
+        HIP_TRY(hipEventRecord(evs[ev_used].second.first, stream));
+        return MPT_OK;
+    };
+    auto ev_end = [&]() -> int {
+        if (!timing) return MPT_OK;
+        HIP_TRY(hipEventRecord(evs[ev_used].second.second, stream));
+        ev_used++;
+        return MPT_OK;
+    };
+    auto ev_collect = [&]() -> int {
+        if (!timing) return MPT_OK;
+        HIP_TRY(hipStreamSynchronize(stream));
+        for (size_t k = 0; k < ev_used; k++) {
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, evs[k].second.first, evs[k].second.second));
+            stage_ms[evs[k].first] += ms;
+            stage_n[evs[k].first]++;
+        }
+        ev_used = 0;
+        return MPT_OK;
+    };
+#define BEV(stage, launch)                    \
+    do {                                      \
+        if ((rc = ev_begin(stage))) return rc; \
+        launch;                               \
+        if ((rc = ev_end())) return rc;       \
+    } while (0)
+
+    // the batch kernels: general geometry, the scenes' shared light profile
+    auto launch_extend = [&](int depth, uint32_t* qin, uint32_t* cin, uint32_t* cout) {
+        if (counting)
+            hipLaunchKernelGGL((k_extend<256, true, F_ALL, true>), dim3(G), dim3(256), 0, stream, DS, L, P, depth, qin,
+                               cin, cout, ctr(cnt, 2), qcls, ccls, cap, ray_stats, work, R, M);
+        else
+            hipLaunchKernelGGL((k_extend<256, false, F_ALL, true>), dim3(G), dim3(256), 0, stream, DS, L, P, depth, qin,
+                               cin, cout, ctr(cnt, 2), qcls, ccls, cap, ray_stats, work, R, M);
+    };
+    auto launch_shadow = [&](int depth) {
+#define BSHD(CNT, ENV)                                                                                             \
+    hipLaunchKernelGGL((k_shadow<256, CNT, F_ALL, ENV, true>), dim3(G), dim3(256), 0, stream, DS, C0, L, P, depth, \
+                       q[2], ctr(cnt, 2), ccls, ray_stats, work, M)
+        if (counting) {
+            if (has_env) BSHD(true, true);
+            else BSHD(true, false);
+        } else {
+            if (has_env) BSHD(false, true);
+            else BSHD(false, false);
+        }
+#undef BSHD
+    };
+    auto launch_shade = [&](int cls, int depth, uint32_t* qout, uint32_t* cout) {
+        const uint32_t* qc = qcls + cls * cap;
+        const uint32_t* cc = ccls + cls * kCtrStride;
+#define BSHA2(MT, LK, ENV)                                                                                    \
+    hipLaunchKernelGGL((k_shade<256, MT, LK, ENV, false, true>), dim3(G), dim3(256), 0, stream, DS, C0, P, depth, \
+                       qc, cc, qout, cout, q[2], ctr(cnt, 2), M)
+#define BSHA1(MT, LK)                               \
+    do {                                            \
+        if (MT < 0 && has_env) BSHA2(MT, LK, (MT < 0)); \
+        else BSHA2(MT, LK, false);                  \
+    } while (0)
+#define BSHA(MT)                                  \
+    do {                                          \
+        switch (light_profile) {                  \
+            case 0: BSHA1(MT, 0); break;          \
+            case 1: BSHA1(MT, L_QUAD); break;     \
+            case 2: BSHA1(MT, L_ENV); break;      \
+            default: BSHA1(MT, L_ALL); break;     \
+        }                                         \
+    } while (0)
+        switch (cls) {
+            case 0: BSHA(-1); break;
+            case 1 + MPT_MAT_LAMBERTIAN: BSHA(MPT_MAT_LAMBERTIAN); break;
+            case 1 + MPT_MAT_METAL: BSHA(MPT_MAT_METAL); break;
+            case 1 + MPT_MAT_DIELECTRIC: BSHA(MPT_MAT_DIELECTRIC); break;
+            case 1 + MPT_MAT_PRINCIPLED: BSHA(MPT_MAT_PRINCIPLED); break;
+            case 1 + MPT_MAT_DIFFUSE_LIGHT: BSHA(MPT_MAT_DIFFUSE_LIGHT); break;
+            default: BSHA(MPT_MAT_PHASE); break;
+        }
+#undef BSHA
+#undef BSHA1
+#undef BSHA2
+    };
+    auto launch_finish = [&](int depth, uint32_t* qin, uint32_t* cin) {
+        if (has_lights) {
+            if (has_env) hipLaunchKernelGGL((k_finish<256, F_ALL, L_ALL, true, true>), dim3(G), dim3(256), 0, stream, DS, C0, L, P, depth, qin, cin, ray_stats, M);
+            else hipLaunchKernelGGL((k_finish<256, F_ALL, L_ALL, false, true>), dim3(G), dim3(256), 0, stream, DS, C0, L, P, depth, qin, cin, ray_stats, M);
+        } else {
+            if (has_env) hipLaunchKernelGGL((k_finish<256, F_ALL, 0, true, true>), dim3(G), dim3(256), 0, stream, DS, C0, L, P, depth, qin, cin, ray_stats, M);
+            else hipLaunchKernelGGL((k_finish<256, F_ALL, 0, false, true>), dim3(G), dim3(256), 0, stream, DS, C0, L, P, depth, qin, cin, ray_stats, M);
+        }
+    };
+    int max_depth = 0;
+    for (int k = 0; k < n; k++) max_depth = std::max(max_depth, C[k].max_depth);
+    const int64_t tail = o.tail_paths < 0 ? 0 : (o.tail_paths > 0 ? o.tail_paths : 2 * L.nthreads);
+
+    int pass = 0;
+    while (pass < total) {
+        const int nb = std::min(pif, total - pass);
+        for (int k = 0; k <= n; k++) M.base[k] = uint32_t(int64_t(nb) * M.pbase[k]);
+        batch_args B{o.seed, uint32_t(o.scene_index), o.first_pass + pass, 0, int32_t(M.base[n])};
+        BEV(3, hipLaunchKernelGGL(k_camera<true>, dim3(unsigned((B.nslots + 255) / 256)), dim3(256), 0, stream, C0, P,
+                                  B, q[0], cnt, int(has_lights), M));
+        for (int depth = 0; depth < max_depth; depth++) {
+            uint32_t* qin = q[depth & 1];
+            if (depth >= 2) {  // the queue length two bounces ago bounds this one's
+                HIP_TRY(hipEventSynchronize(ring[(depth - 2) % kRing]));
+                const uint32_t left = uint32_t(s->pinned_counts[(depth - 2) % kRing]);
+                if (left == 0) break;
+                if (tail > 0 && left <= uint32_t(tail)) {
+                    BEV(4, launch_finish(depth, qin, ctr(cnt, depth & 1)));
+                    break;
+                }
+            }
+            uint32_t* qout = q[(depth + 1) & 1];
+            uint32_t* cin = ctr(cnt, depth & 1);
+            uint32_t* cout = ctr(cnt, (depth + 1) & 1);
+            BEV(0, launch_extend(depth, qin, cin, cout));
+            for (int cls = 0; cls < kClasses; cls++) {
+                if (cls > 0 && !(mat_types & (1u << (cls - 1)))) continue;
+                BEV(1, launch_shade(cls, depth, qout, cout));
+            }
+            if (has_lights) BEV(2, launch_shadow(depth));
+            else BEV(2, hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, ccls, kClasses));
+            HIP_TRY(hipMemcpyAsync(&s->pinned_counts[depth % kRing], cout, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipEventRecord(ring[depth % kRing], stream));
+        }
+        BEV(3, hipLaunchKernelGGL(k_accumulate_multi, dim3(unsigned((M.pbase[n] + 255) / 256)), dim3(256), 0, stream, P,
+                                  nb, M));
+        if (hipGetLastError() != hipSuccess) return set_error(MPT_E_HIP, "kernel launch failed");
+        if ((rc = ev_collect())) return rc;
+        pass += nb;
+    }
+#undef BEV
+    HIP_TRY(hipStreamSynchronize(stream));
+    auto t1 = std::chrono::steady_clock::now();
+    for (int k = 0; k < n; k++) {
+        ss[k]->last_done += total;
+        ss[k]->last_cam = cams[k];
+    }
+    if (stats) {
+        unsigned long long rs_host[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        HIP_TRY(hipMemcpy(rs_host, ray_stats, sizeof(rs_host), hipMemcpyDeviceToHost));
+        std::memset(stats, 0, sizeof(*stats));
+        stats->paths = pix_total * total;
+        stats->extension_rays = int64_t(rs_host[0]);
+        stats->shadow_rays = int64_t(rs_host[1]);
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+        for (int k = 0; k < 5; k++) {
+            stats->stage_ms[k] = stage_ms[k];
+            stats->stage_launches[k] = stage_n[k];
+        }
+        stats->node_bytes[0] = int64_t(rs_host[2]);
+        stats->prim_bytes[0] = int64_t(rs_host[3]);
+        stats->node_bytes[1] = int64_t(rs_host[4]);
+        stats->prim_bytes[1] = int64_t(rs_host[5]);
+        stats->tail_rays[0] = int64_t(rs_host[6]);
+        stats->tail_rays[1] = int64_t(rs_host[7]);
+    }
+    return MPT_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int32_t mpt_render(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts, double* radiance, double* albedo,
                    double* normal, double* variance, mpt_pass_callback on_pass, void* user, mpt_render_stats* stats) {
     return render_impl(s, cam, opts, radiance, albedo, normal, variance, on_pass, user, stats, nullptr, nullptr,
                        nullptr);
+}
+
+int32_t mpt_render_batch(mpt_scene* const* scenes, const mpt_camera* cameras, int32_t n, const mpt_render_opts* opts,
+                         mpt_render_stats* stats) {
+    return render_batch_impl(scenes, cameras, n, opts, stats);
 }
 
 void mpt_nrc_defaults(mpt_nrc_opts* o) {

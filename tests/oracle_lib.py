@@ -161,17 +161,65 @@ class OracleScene:
         return out
 
 
-_ref = None
+_ref = {}
+# timing-only builds of the reference harness with the reference's Release flags (oracle/Makefile ref_timing)
+TIMING_REFS = [("x86-64-v4", os.path.join(ROOT, "oracle", "_ref", "libmpt_ref_time_v4.so"),
+                ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl")),
+               ("x86-64-v3", os.path.join(ROOT, "oracle", "_ref", "libmpt_ref_time_v3.so"),
+                ("avx2", "fma", "bmi2"))]
+TIMING_FLAGS = "-O3 -DNDEBUG -std=c++17 -mtune=znver3, gcc default FP contraction"
 
 
 def reference_available():
     return os.path.exists(REF)
 
 
-def reference():
-    global _ref
-    if _ref is None:
-        lib = C.CDLL(REF)
+def _cpu_flags():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return set()
+
+
+def timing_reference():
+    """(library, -march level, flags) of the fastest timing build of the
+    reference the host can run, or None when none is built."""
+    have = _cpu_flags()
+    for march, path, need in TIMING_REFS:
+        if os.path.exists(path) and all(f in have for f in need):
+            return reference(path), march, f"-march={march} {TIMING_FLAGS}"
+    return None
+
+
+def cpu_description():
+    """CPU model, logical CPUs and physical cores of this host (from /proc/cpuinfo)."""
+    model, logical, cores = "unknown", 0, set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name":
+                    model = v
+                    logical += 1
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                    cores.add((phys, core))
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": logical, "physical_cores": len(cores) or logical}
+
+
+def reference(path=REF):
+    if path not in _ref:
+        lib = C.CDLL(path)
         sig = {
             "ref_load": (C.c_int, [C.c_char_p, P(vp)]),
             "ref_free": (None, [vp]),
@@ -199,13 +247,13 @@ def reference():
             f = getattr(lib, name)
             f.restype = res
             f.argtypes = args
-        _ref = lib
-    return _ref
+        _ref[path] = lib
+    return _ref[path]
 
 
 class RefScene:
-    def __init__(self, path):
-        self.lib = reference()
+    def __init__(self, path, lib=None):
+        self.lib = lib or reference()
         self.h = vp()
         if self.lib.ref_load(os.fsencode(path), C.byref(self.h)) != 0:
             raise RuntimeError("reference load_scene failed for " + path)
