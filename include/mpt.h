@@ -270,8 +270,8 @@ typedef struct mpt_render_stats {
     int64_t stage_launches[5];
     /* with profile bit 1: BVH node bytes and primitive bytes read by the
      * extension [0] and shadow [1] traversals (a root box counts 64 bytes,
-     * an interior child-pair record 64, a child re-tested on pop 32, a
-     * leaf's double-precision box checked at its first hit 48) */
+     * a four-wide record 144, a slot re-tested on pop 32, a leaf's
+     * double-precision box checked at its first hit 48) */
     int64_t node_bytes[2];
     int64_t prim_bytes[2];
     /* extension [0] and shadow [1] rays traced by the batch tails (included

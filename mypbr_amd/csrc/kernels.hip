@@ -1288,7 +1288,8 @@ int ensure_ovf(mpt_scene* s) {
     for (const auto& o : s->hs.objects)
         if (o.kind == MPT_OBJ_MESH)
             depth_blas = std::max(depth_blas, tree_depth(s->hs.blas, s->hs.blas_root_node[o.blas_root]));
-    int need = depth_tlas + depth_blas + 2 - kLdsDepth;
+    // the walk pushes at most three entries per four-wide record (two levels)
+    int need = 2 * (depth_tlas + depth_blas) + 4 - kLdsDepth;
     if (need < 1) need = 1;
     int64_t threads = int64_t(s->grid) * s->block;
     if (s->ovf && s->ovf_depth >= need && s->ovf_threads == threads) return MPT_OK;
@@ -1386,8 +1387,8 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
     if ((rc = upload(s.get(), h.ctlas, &d.tlas))) return rc;
     if ((rc = upload(s.get(), h.tlas_items, &d.titems))) return rc;
     if ((rc = upload(s.get(), h.cblas, &d.blas))) return rc;
-    if ((rc = upload(s.get(), h.ftlas, &d.ftlas))) return rc;
-    if ((rc = upload(s.get(), h.fblas, &d.fblas))) return rc;
+    if ((rc = upload(s.get(), h.qtlas, &d.qtlas))) return rc;
+    if ((rc = upload(s.get(), h.qblas, &d.qblas))) return rc;
     if ((rc = upload(s.get(), h.cblas_roots, &d.blas_roots))) return rc;
     if ((rc = upload(s.get(), h.spheres, &d.sph))) return rc;
     d.tlas_root = h.ctlas_root;

@@ -177,7 +177,7 @@ void build_bvh(std::vector<build_item>& items, std::vector<bvh_node>& nodes) {
 void pair_layout(const std::vector<bvh_node>& nodes, int root, std::vector<cnode>& out, croot& r) {
     const bvh_node& R = nodes[root];
     for (int k = 0; k < 6; k++) r.b[k] = R.b[k];
-    r.pad[0] = r.pad[1] = 0;
+    r.qoff = r.pad = 0;
     if (R.count > 0) {
         r.off = R.offset;
         r.cnt = R.count;
@@ -211,29 +211,52 @@ void pair_layout(const std::vector<bvh_node>& nodes, int root, std::vector<cnode
     }
 }
 
-namespace {
-// x rounded to single precision towards -inf / +inf
-float round_down(double x) {
-    float f = float(x);
-    return double(f) > x ? std::nextafterf(f, -INFINITY) : f;
-}
-float round_up(double x) {
-    float f = float(x);
-    return double(f) < x ? std::nextafterf(f, INFINITY) : f;
-}
-}  // namespace
-
-void float_layout(const std::vector<cnode>& in, std::vector<fnode>& out) {
-    out.resize(in.size());
-    for (size_t i = 0; i < in.size(); i++) {
-        const cnode& c = in[i];
-        fnode& f = out[i];
-        for (int k = 0; k < 12; k++) f.b[k] = (k & 1) ? round_up(c.b[k]) : round_down(c.b[k]);
-        f.off[0] = c.off[0];
-        f.off[1] = c.off[1];
-        f.cnt[0] = c.cnt[0] | (c.axis << 30);
-        f.cnt[1] = c.cnt[1];
+int quad_layout(const std::vector<cnode>& pairs, int root_record, std::vector<qnode>& out) {
+    const int root = int(out.size());
+    out.push_back(qnode{});
+    std::vector<std::pair<int, int>> todo{{root_record, root}};  // (pair record, its qnode)
+    while (!todo.empty()) {
+        auto [r, qi] = todo.back();
+        todo.pop_back();
+        const cnode& c = pairs[r];
+        qnode q{};
+        int slot = 0;
+        q.info = c.axis & 3;
+        // slot from child w of pair record pr
+        auto put = [&](int pr, int w) {
+            const cnode& p = pairs[pr];
+            for (int k = 0; k < 6; k++) {
+                const double v = p.b[6 * w + k];
+                float f = float(v);
+                if ((k & 1) ? double(f) < v : double(f) > v) f = std::nextafterf(f, (k & 1) ? INFINITY : -INFINITY);
+                q.b[6 * slot + k] = f;
+            }
+            if (p.cnt[w] > 0) {
+                q.off[slot] = p.off[w];
+                q.meta[slot] = ((pr << 1) | w) | (p.cnt[w] << 28);
+            } else {
+                q.off[slot] = int32_t(out.size());
+                q.meta[slot] = 0;
+                out.push_back(qnode{});
+                todo.push_back({p.off[w], q.off[slot]});
+            }
+            slot++;
+        };
+        for (int w = 0; w < 2; w++) {
+            if (c.cnt[w] > 0) {
+                put(r, w);  // a leaf child: one slot
+            } else {
+                const int cr = c.off[w];  // an interior child: its two children
+                q.info |= (pairs[cr].axis & 3) << (2 + 2 * w);
+                q.info |= 1 << (6 + w);
+                put(cr, 0);
+                put(cr, 1);
+            }
+        }
+        for (int k = slot; k < 4; k++) q.meta[k] = 0;
+        out[qi] = q;
     }
+    return root;
 }
 
 void build_ggx_table(std::vector<double>& e, std::vector<double>& e_avg) {  // microfacet.h:299-337
@@ -589,8 +612,11 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
     build_bvh(items, s.tlas);
     s.ctlas.clear();
     pair_layout(s.tlas, 0, s.ctlas, s.ctlas_root);
-    float_layout(s.ctlas, s.ftlas);
-    float_layout(s.cblas, s.fblas);
+    s.qtlas.clear();
+    s.qblas.clear();
+    if (s.ctlas_root.cnt == 0) s.ctlas_root.qoff = quad_layout(s.ctlas, s.ctlas_root.off, s.qtlas);
+    for (croot& r : s.cblas_roots)
+        if (r.cnt == 0) r.qoff = quad_layout(s.cblas, r.off, s.qblas);
     s.tlas_items.resize(no);
     for (int o = 0; o < no; o++) s.tlas_items[o] = int32_t(items[o].ref);
     s.world_box = box_empty();

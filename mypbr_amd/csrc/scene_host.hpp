@@ -37,29 +37,41 @@ struct cnode {
 };
 static_assert(sizeof(cnode) == 128, "cnode must be 128 bytes");
 
-// The same records in single precision for the traversal's box culls, 64
-// bytes: both children's boxes rounded outward (minima down, maxima up), the
-// children's item ranges or records, and the split axis in the top two bits
-// of cnt[0]. A cull against these boxes never rejects a box the double
-// precision test accepts; the double boxes of the cnode records decide the
-// leaves (the traversal checks a leaf's double box when the leaf first
-// yields a hit).
-struct fnode {
-    float b[12];
-    int32_t off[2];
-    int32_t cnt[2];    // cnt[0]: left count | axis << 30; cnt[1]: right count
+// Four-wide single-precision records for the traversal's culls, 144 bytes.
+// A cull against their boxes never rejects a box the double precision test
+// accepts; the double boxes of the cnode records decide the leaves (the
+// traversal checks a leaf's double box when the leaf first yields a hit).
+// A record stands for one interior node N of the binary tree and holds, per
+// child of N, either the child itself (a leaf) or the child's two children
+// (an interior child is expanded), so up to four slots: left child's part
+// first, then the right child's. Slot boxes are rounded outward. A leaf
+// slot holds its item range and where its double box is (the cnode record
+// and child it is in); an interior slot holds the record of that node.
+// `info` gives what visit order needs: N's split axis (bits 0-1), the left
+// and right children's split axes when expanded (bits 2-3, 4-5), and which
+// children are expanded (bits 6, 7). The binary tree's near-first order is
+// then N's near child's part first, each expanded part near child first.
+struct qnode {
+    float b[24];       // slot s: xmin, xmax, ymin, ymax, zmin, zmax at b[6 s]
+    int32_t off[4];    // leaf slot: first item; interior slot: qnode index
+    int32_t meta[4];   // leaf slot: (cnode record << 1 | child) | count << 28; interior slot: 0
+    int32_t info;
+    int32_t pad[3];
 };
-static_assert(sizeof(fnode) == 64, "fnode must be 64 bytes");
+static_assert(sizeof(qnode) == 144, "qnode must be 144 bytes");
 
-// Single-precision copies of child-pair records (same indices).
-void float_layout(const std::vector<cnode>& in, std::vector<fnode>& out);
+// Four-wide records of the pair tree whose root record is `root_record`,
+// appended to out; returns the index of the root's record.
+int quad_layout(const std::vector<cnode>& pairs, int root_record, std::vector<qnode>& out);
 
-// Root of a re-laid tree: its box and what it is (as a cnode child).
+// Root of a re-laid tree: its box and what it is (as a cnode child); for an
+// interior root, qoff is its four-wide record.
 struct croot {
     double b[6];
     int32_t off;
     int32_t cnt;
-    int32_t pad[2];
+    int32_t qoff;
+    int32_t pad;
 };
 static_assert(sizeof(croot) == 64, "croot must be 64 bytes");
 
@@ -107,7 +119,7 @@ struct host_scene {
     croot ctlas_root{};
     std::vector<cnode> cblas;
     std::vector<croot> cblas_roots;      // indexed by obj_rec.blas_root
-    std::vector<fnode> ftlas, fblas;     // ctlas / cblas in single precision, for the culls
+    std::vector<qnode> qtlas, qblas;     // the same trees four-wide, single precision (qoff of the roots)
     std::vector<int32_t> blas_root_node; // cblas_roots index -> root in blas (for depths)
     std::vector<double> spheres;         // center xyz, radius of every primitive (spheres only meaningful)
     std::vector<xform> xforms;
