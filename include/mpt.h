@@ -206,6 +206,14 @@ typedef struct mpt_scene_desc {
     double sun_direction[3];
     double sun_angle_degrees;      /* angular diameter, clamped below at 1e-3 (sun.h:29) */
     double sun_irradiance[3];
+
+    /* With has_light_bounds set, the scene bounds light selection takes its
+     * radius from (light_list::build(scene_bounds), light_list.h:45-58),
+     * as (xmin, xmax, ymin, ymax, zmin, zmax); otherwise the world list's
+     * bounding box, as scene.h:1248 passes it. */
+    int32_t has_light_bounds;
+    int32_t light_bounds_pad;
+    double light_bounds[6];
 } mpt_scene_desc;
 
 /* Camera and render settings: the public fields of the reference camera
@@ -406,6 +414,20 @@ int32_t mpt_ggx_albedo_table(double* e_1024, double* e_avg_32);
 
 /* Light selection probabilities light_list::build assigned (light_list.h:45-96). */
 int32_t mpt_scene_light_probabilities(const mpt_scene* scene, double* probabilities);
+
+/* Host-only forms of the same computations on a description (no GPU): the
+ * bounding box of the world list, as the BVH over it holds it
+ * (bvh::bounding_box, bvh.h:88: the union of the objects' padded boxes,
+ * aabb.h:83-88; the ground plane, beside the BVH, is not in it), as
+ * (xmin, xmax, ymin, ymax, zmin, zmax); each light's power (hittable::power,
+ * hittable.h:59; environment_light::power and sun_light::power with the
+ * scene radius, environment.h:116-127, sun.h:66-71) before unknown powers
+ * get the mean; and the selection probabilities light_list::build gives
+ * (light_list.h:45-96), from scene_bounds (NULL: the description's own
+ * light bounds or world box). powers and probabilities hold n_lights. */
+int32_t mpt_desc_bounds(const mpt_scene_desc* desc, double* box6);
+int32_t mpt_desc_light_powers(const mpt_scene_desc* desc, const double* scene_bounds6, double* powers);
+int32_t mpt_desc_light_probabilities(const mpt_scene_desc* desc, const double* scene_bounds6, double* probabilities);
 
 /* BSDF evaluation on the GPU for fixed directions (test hook for the
  * material::eval / pdf value pairs, material.h:22-72): for each of n queries

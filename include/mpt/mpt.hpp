@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstdint>
 #include <functional>
+#include <limits>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -245,12 +246,38 @@ public:
 // ------------------------------------------------------------------ geometry
 class flat_builder;
 
+// interval (interval.h): default-constructed intervals are empty (+inf, -inf).
+struct interval {
+    double min = std::numeric_limits<double>::infinity();
+    double max = -std::numeric_limits<double>::infinity();
+    interval() = default;
+    interval(double min, double max) : min(min), max(max) {}
+    double size() const { return max - min; }
+    bool contains(double x) const { return min <= x && x <= max; }
+};
+
+// aabb (aabb.h): three intervals; default-constructed boxes are empty.
+struct aabb {
+    interval x, y, z;
+    aabb() = default;
+    aabb(const interval& x, const interval& y, const interval& z) : x(x), y(y), z(z) {}
+    const interval& axis_interval(int n) const { return n == 1 ? y : n == 2 ? z : x; }
+};
+
 class hittable {
 public:
     virtual ~hittable() = default;
     // Appends this object to the world list of the description; xform is the
     // index of the instance transform it sits behind (-1 for none).
     virtual void flatten(flat_builder& b, int32_t xform) const = 0;
+    // hittable::bounding_box (hittable.h:40): the box the core's BVH build
+    // gives this object (aabb.h padding included); for a list, the union.
+    virtual aabb bounding_box() const;
+    // hittable::power (hittable.h:54-59): luminance of the emitted power,
+    // pi * area * luminance(average emission) for emitters (sums for lists
+    // and meshes, times the area scale behind an instance), 0 otherwise.
+    // The core's own light-power code computes it.
+    virtual double power() const;
 };
 
 class sphere : public hittable {
@@ -357,6 +384,11 @@ public:
     environment_light(int width, int height, std::vector<color> pixels, double intensity = 1, double rotation_degrees = 0)
         : width(width), height(height), pixels(std::move(pixels)), intensity(intensity), rotation_degrees(rotation_degrees) {}
     void flatten(flat_builder& b, int32_t xform) const override;
+    aabb bounding_box() const override { return aabb(); }  // environment.h:59
+    double power() const override { return 0; }           // needs the radius: power(scene_radius)
+    // environment_light::power (environment.h:116-127): pi R^2 times the
+    // luminance of the radiance integrated over the sphere.
+    double power(double scene_radius) const;
     int width, height;
     std::vector<color> pixels;
     double intensity, rotation_degrees;
@@ -370,21 +402,64 @@ public:
     sun_light(const vec3& direction, double angular_diameter, const color& irradiance)
         : direction(direction), angular_diameter(angular_diameter), irradiance(irradiance) {}
     void flatten(flat_builder& b, int32_t xform) const override;
+    aabb bounding_box() const override { return aabb(); }  // sun.h:41
+    double power() const override { return 0; }
+    // sun_light::power (sun.h:66-71): pi R^2 luminance(irradiance).
+    double power(double scene_radius) const;
     vec3 direction;
     double angular_diameter;
     color irradiance;
 };
 
-// light_list (light_list.h:33-134): the selection probabilities are computed
-// by the core on upload from this strategy; build() only records the bounds
-// flag, as the core recomputes the scene radius itself.
+// atmosphere_settings and atmosphere (atmosphere.h:36-165): the
+// single-scattering sky a `sky atmosphere` line bakes, for scenes built in
+// memory. bake() gives the pixels of an equirectangular environment_light
+// (row 0 at +y) lit by a sun of the given direction and irradiance above
+// the air; sun_transmittance() is the share of that irradiance reaching the
+// ground (0 below the horizon), by which the loader dims the sun. The values
+// are the loader's, bit for bit.
+struct atmosphere_settings {
+    double rayleigh = 1;   // air density, relative to Earth's
+    double mie = 1;        // aerosol (haze) density
+    double mie_g = 0.8;    // forward scattering of the haze
+    double ozone = 1;      // ozone density
+    double intensity = 3;  // overall scale of the sky's radiance
+};
+class atmosphere {
+public:
+    explicit atmosphere(const atmosphere_settings& s) : settings(s) {}
+    color sun_transmittance(const vec3& direction) const;
+    color radiance(const vec3& direction, const vec3& sun_direction, const color& sun_irradiance) const;
+    std::vector<color> bake(int width, int height, const vec3& sun_direction, const color& sun_irradiance) const;
+    atmosphere_settings settings;
+};
+
+// light_list (light_list.h:33-134): picks light k with probability p_k,
+// in proportion to its power with a uniform_fraction share kept uniform.
+// build(scene_bounds) computes the probabilities with the core's light
+// code (the radius of scene_bounds sets the environment's and the sun's
+// power) and renders use the same bounds; until it is called, selection is
+// uniform as in the reference. build() without bounds uses the world's own
+// box, as scene.h:1248 passes it.
 class light_list : public hittable_list {
 public:
     enum class selection { uniform, power };
     selection strategy = selection::power;
     double uniform_fraction = 0.2;
-    void build() { built = true; }
+    void build(const aabb& scene_bounds);
+    void build() {
+        built = true;
+        has_bounds = false;
+    }
+    // Probability that the sampler picks light k (light_list.h:98-103).
+    double probability(size_t k) const;
+    // The light a uniform xi in [0, 1) selects (light_list.h:118-127).
+    size_t pick(double xi) const;
     bool built = false;
+    bool has_bounds = false;
+    aabb bounds;
+private:
+    std::vector<double> probabilities_, cdf_;
 };
 
 // ------------------------------------------------------------------ camera
@@ -499,6 +574,7 @@ class ground_plane : public hittable {
 public:
     ground_plane(double height, std::shared_ptr<material> mat) : height(height), mat(std::move(mat)) {}
     void flatten(flat_builder& b, int32_t xform) const override;
+    aabb bounding_box() const override;  // plane.h:36: the whole plane y = height
     double height;
     std::shared_ptr<material> mat;
 };

@@ -60,6 +60,9 @@ def _load():
         "mpt_image_height": (C.c_int32, [P(Camera)]),
         "mpt_ggx_albedo_table": (C.c_int32, [P(C.c_double), P(C.c_double)]),
         "mpt_scene_light_probabilities": (C.c_int32, [vp, P(C.c_double)]),
+        "mpt_desc_bounds": (C.c_int32, [P(SceneDesc), P(C.c_double)]),
+        "mpt_desc_light_powers": (C.c_int32, [P(SceneDesc), P(C.c_double), P(C.c_double)]),
+        "mpt_desc_light_probabilities": (C.c_int32, [P(SceneDesc), P(C.c_double), P(C.c_double)]),
         "mpt_bsdf_eval": (C.c_int32, [vp, P(C.c_int32), P(C.c_double), P(C.c_uint8), C.c_int64, P(C.c_double),
                                       P(C.c_double)]),
         "mpt_primary_depth": (C.c_int32, [vp, P(Camera), P(C.c_double)]),

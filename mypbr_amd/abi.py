@@ -84,6 +84,9 @@ class SceneDesc(C.Structure):
         ("sun_direction", D3),
         ("sun_angle_degrees", C.c_double),
         ("sun_irradiance", D3),
+        ("has_light_bounds", C.c_int32),
+        ("light_bounds_pad", C.c_int32),
+        ("light_bounds", C.c_double * 6),
     ]
 
 

@@ -82,49 +82,49 @@ MPT_HD void sin_cos_2pi(double u, double& s, double& c) {
 }
 
 // interval.h: default-constructed intervals are empty (+inf, -inf).
-struct interval {
+struct span {
     double min, max;
 };
-MPT_HD interval iv_empty() { return interval{kInf, -kInf}; }
-MPT_HD interval iv_merge(const interval& a, const interval& b) {
-    return interval{a.min <= b.min ? a.min : b.min, a.max >= b.max ? a.max : b.max};
+MPT_HD span iv_empty() { return span{kInf, -kInf}; }
+MPT_HD span iv_merge(const span& a, const span& b) {
+    return span{a.min <= b.min ? a.min : b.min, a.max >= b.max ? a.max : b.max};
 }
-MPT_HD double iv_size(const interval& a) { return a.max - a.min; }
-MPT_HD interval iv_expand(const interval& a, double delta) {
+MPT_HD double iv_size(const span& a) { return a.max - a.min; }
+MPT_HD span iv_expand(const span& a, double delta) {
     double padding = delta / 2;
-    return interval{a.min - padding, a.max + padding};
+    return span{a.min - padding, a.max + padding};
 }
 
-struct aabb {
-    interval ax[3];
+struct box3 {
+    span ax[3];
 };
-MPT_HD aabb box_empty() { return aabb{{iv_empty(), iv_empty(), iv_empty()}}; }
+MPT_HD box3 box_empty() { return box3{{iv_empty(), iv_empty(), iv_empty()}}; }
 // aabb(interval x, y, z) and aabb(point a, point b) pad thin sides (aabb.h:83-88).
-MPT_HD void box_pad(aabb& b) {
+MPT_HD void box_pad(box3& b) {
     const double delta = 0.0001;
     for (int k = 0; k < 3; k++)
         if (iv_size(b.ax[k]) < delta) b.ax[k] = iv_expand(b.ax[k], delta);
 }
-MPT_HD aabb box_points(const v3& a, const v3& b) {
-    aabb r;
-    r.ax[0] = a.x <= b.x ? interval{a.x, b.x} : interval{b.x, a.x};
-    r.ax[1] = a.y <= b.y ? interval{a.y, b.y} : interval{b.y, a.y};
-    r.ax[2] = a.z <= b.z ? interval{a.z, b.z} : interval{b.z, a.z};
+MPT_HD box3 box_points(const v3& a, const v3& b) {
+    box3 r;
+    r.ax[0] = a.x <= b.x ? span{a.x, b.x} : span{b.x, a.x};
+    r.ax[1] = a.y <= b.y ? span{a.y, b.y} : span{b.y, a.y};
+    r.ax[2] = a.z <= b.z ? span{a.z, b.z} : span{b.z, a.z};
     box_pad(r);
     return r;
 }
-MPT_HD aabb box_merge(const aabb& a, const aabb& b) {
-    return aabb{{iv_merge(a.ax[0], b.ax[0]), iv_merge(a.ax[1], b.ax[1]), iv_merge(a.ax[2], b.ax[2])}};
+MPT_HD box3 box_merge(const box3& a, const box3& b) {
+    return box3{{iv_merge(a.ax[0], b.ax[0]), iv_merge(a.ax[1], b.ax[1]), iv_merge(a.ax[2], b.ax[2])}};
 }
-MPT_HD v3 box_centroid(const aabb& b) {
+MPT_HD v3 box_centroid(const box3& b) {
     return v3{(b.ax[0].min + b.ax[0].max) / 2, (b.ax[1].min + b.ax[1].max) / 2,
               (b.ax[2].min + b.ax[2].max) / 2};
 }
-MPT_HD double box_area(const aabb& b) {
+MPT_HD double box_area(const box3& b) {
     double dx = iv_size(b.ax[0]), dy = iv_size(b.ax[1]), dz = iv_size(b.ax[2]);
     return 2 * (dx * dy + dy * dz + dz * dx);
 }
-MPT_HD int box_longest_axis(const aabb& b) {
+MPT_HD int box_longest_axis(const box3& b) {
     double sx = iv_size(b.ax[0]), sy = iv_size(b.ax[1]), sz = iv_size(b.ax[2]);
     if (sx > sy) return sx > sz ? 0 : 2;
     return sy > sz ? 1 : 2;

@@ -733,8 +733,129 @@ struct sky_air {
 };
 }  // namespace
 
+namespace {
+sky_air air_of(const atmosphere_settings& s) {
+    sky_air a;
+    a.rayleigh = s.rayleigh;
+    a.mie = s.mie;
+    a.mie_g = s.mie_g;
+    a.ozone = s.ozone;
+    a.intensity = s.intensity;
+    return a;
+}
+}  // namespace
+
+color atmosphere::sun_transmittance(const vec3& direction) const {  // atmosphere.h:52-56
+    return air_of(settings).transmittance(vec3(0, sky_air::r_view, 0), unit_vector(direction));
+}
+color atmosphere::radiance(const vec3& direction, const vec3& sun_direction, const color& sun_irradiance) const {
+    return air_of(settings).radiance(direction, sun_direction, sun_irradiance);
+}
+std::vector<color> atmosphere::bake(int width, int height, const vec3& sun_direction, const color& sun_irradiance) const {
+    if (width < 1 || height < 1) throw std::runtime_error("atmosphere::bake: width and height must be positive");
+    return air_of(settings).bake(width, height, sun_direction, sun_irradiance);
+}
+
 void sun_light::flatten(flat_builder&, int32_t) const {
     // never hit (sun.h:38-41): not part of the world's geometry
+}
+
+// ------------------------------------------------------------------ boxes, powers, light selection
+namespace {
+// A non-owning handle on an object the caller keeps alive (for the one-object
+// descriptions below).
+std::shared_ptr<hittable> borrowed(const hittable* h) {
+    return std::shared_ptr<hittable>(std::shared_ptr<hittable>(), const_cast<hittable*>(h));
+}
+}  // namespace
+
+aabb hittable::bounding_box() const {
+    flat_builder b;
+    b.add_world(*this);  // a list is expanded in place, as a world list
+    double box[6];
+    if (mpt_desc_bounds(&b.desc(), box) != MPT_OK)
+        throw std::runtime_error(std::string("bounding_box: ") + mpt_last_error());
+    return aabb(interval(box[0], box[1]), interval(box[2], box[3]), interval(box[4], box[5]));
+}
+
+aabb ground_plane::bounding_box() const {
+    const double inf = std::numeric_limits<double>::infinity();
+    return aabb(interval(-inf, inf), interval(height, height), interval(-inf, inf));
+}
+
+double hittable::power() const {
+    if (auto l = dynamic_cast<const hittable_list*>(this)) {  // hittable_list::power (hittable_list.h:65-70)
+        double sum = 0;
+        for (const auto& o : l->objects) sum += o->power();
+        return sum;
+    }
+    hittable_list w;
+    w.add(borrowed(this));
+    flat_builder b;
+    try {
+        b.add_world(w);
+        b.add_lights(w, nullptr);
+    } catch (const std::exception&) {
+        return 0;  // nothing the light code can weigh (a sky, a sun, a plane)
+    }
+    double pw = 0;
+    if (mpt_desc_light_powers(&b.desc(), nullptr, &pw) != MPT_OK) return 0;  // media are never lights
+    return pw;
+}
+
+double environment_light::power(double scene_radius) const {  // environment.h:116-127
+    if (pixels.empty()) return 0;
+    double sum = 0;
+    for (int j = 0; j < height; j++) {
+        const double sin_theta = std::sin(kPi * (j + 0.5) / height);
+        for (int i = 0; i < width; i++) {
+            const color& c = pixels[size_t(j) * width + i];
+            sum += std::fmax(luminance(v3{c[0], c[1], c[2]}) * sin_theta, 0.0);
+        }
+    }
+    const double integral = intensity * sum * (2 * kPi / width) * (kPi / height);
+    return kPi * scene_radius * scene_radius * integral;
+}
+
+double sun_light::power(double scene_radius) const {  // sun.h:66-71
+    return kPi * scene_radius * scene_radius * luminance(v3{irradiance[0], irradiance[1], irradiance[2]});
+}
+
+void light_list::build(const aabb& scene_bounds) {
+    built = true;
+    has_bounds = true;
+    bounds = scene_bounds;
+    const size_t n = objects.size();
+    probabilities_.assign(n, n ? 1.0 / double(n) : 0.0);
+    cdf_.assign(n + 1, 0.0);
+    if (n == 0) return;
+    // the lights alone as the world (the sky and the sun are lights only), so
+    // that the core's light code weighs them with these bounds
+    hittable_list w;
+    for (const auto& o : objects)
+        if (!dynamic_cast<const environment_light*>(o.get()) && !dynamic_cast<const sun_light*>(o.get())) w.add(o);
+    flat_builder b;
+    b.add_world(w);
+    b.add_lights(*this, nullptr);
+    const double b6[6] = {scene_bounds.x.min, scene_bounds.x.max, scene_bounds.y.min,
+                          scene_bounds.y.max, scene_bounds.z.min, scene_bounds.z.max};
+    if (mpt_desc_light_probabilities(&b.desc(), b6, probabilities_.data()) != MPT_OK)
+        throw std::runtime_error(std::string("light_list::build: ") + mpt_last_error());
+    for (size_t k = 0; k < n; k++) cdf_[k + 1] = cdf_[k] + probabilities_[k];
+    cdf_[n] = 1;  // light_list.h:92: no rounding gap at the top
+}
+
+double light_list::probability(size_t k) const {
+    if (built && has_bounds && probabilities_.size() == objects.size()) return probabilities_[k];
+    return 1.0 / double(objects.size());
+}
+
+size_t light_list::pick(double xi) const {
+    const size_t n = objects.size();
+    if (!(built && has_bounds && cdf_.size() == n + 1)) return std::min(size_t(xi * double(n)), n - 1);
+    const auto it = std::upper_bound(cdf_.begin(), cdf_.end(), xi);
+    const size_t k = size_t(it - cdf_.begin());
+    return std::clamp(k, size_t(1), n) - 1;
 }
 
 void flat_builder::add_world(const hittable& world) {
@@ -785,6 +906,12 @@ void flat_builder::add_lights(const hittable_list& lights, const std::shared_ptr
     if (ll && ll->built) {
         p->d.light_strategy = ll->strategy == light_list::selection::power ? 1 : 0;
         p->d.light_uniform_fraction = ll->uniform_fraction;
+        p->d.has_light_bounds = ll->has_bounds ? 1 : 0;
+        if (ll->has_bounds) {
+            const aabb& lb = ll->bounds;
+            const double v[6] = {lb.x.min, lb.x.max, lb.y.min, lb.y.max, lb.z.min, lb.z.max};
+            std::copy(v, v + 6, p->d.light_bounds);
+        }
     } else {
         p->d.light_strategy = 2;  // plain hittable_list: uniform pick (hittable_list.h:58-62)
         p->d.light_uniform_fraction = 0;
@@ -905,6 +1032,17 @@ void pass_trampoline(int32_t done, void* user) {
 frame camera::render(const hittable& world, const hittable_list& lights, int save_every, const pass_callback& on_pass) {
     image_height = int(image_width / aspect_ratio);
     image_height = image_height < 1 ? 1 : image_height;
+    // camera::sun is added to every escaping ray (camera.h:526-527) and the
+    // lights sample it (camera.h:45); the core models the two together, so
+    // they must name the same sun
+    const sun_light* listed = nullptr;
+    for (const auto& o : lights.objects)
+        if (auto s = dynamic_cast<const sun_light*>(o.get())) listed = s;
+    if (sun && listed != sun.get())
+        throw std::runtime_error("camera::sun must also be in the lights, and be the only sun there (camera.h:45)");
+    if (!sun && listed)
+        throw std::runtime_error("a sun_light in the lights needs camera::sun set to it: escaping rays add "
+                                 "camera::sun (camera.h:526-527)");
     gpu_scene gs(world, lights, environment);
     mpt_camera c = to_abi();
     mpt_render_opts o{};
@@ -2011,14 +2149,18 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
         host_scene hs;
         std::string e2;
         if (build_host_scene(fb.desc(), hs, e2) != MPT_OK) return fail(e2);
-        const aabb& wb = hs.world_box;
+        const box3& wb = hs.world_box;
         point3 lo(wb.ax[0].min, wb.ax[1].min, wb.ax[2].min);
         point3 hi(wb.ax[0].max, wb.ax[1].max, wb.ax[2].max);
         world.add(std::make_shared<constant_medium>(box(lo, hi, default_material), fog_density, fog_material));
     }
     if ((cam.lookfrom - cam.lookat).near_zero() || cross(cam.vup, cam.lookfrom - cam.lookat).near_zero())
         return fail("camera lookfrom must differ from lookat, and vup must not point along the view");
-    lights.build();
+    try {
+        lights.build(world.bounding_box());  // scene.h:1248
+    } catch (const std::exception& e) {
+        return fail(e.what());
+    }
     return true;
 }
 
@@ -2071,6 +2213,57 @@ extern "C" int32_t mpt_scene_describe(const char* path, mpt_scene_desc* desc, mp
 }
 
 extern "C" void mpt_desc_free(void* holder) { delete static_cast<mpt::flat_builder*>(holder); }
+
+namespace {
+// The host scene of a description (validation, boxes, BVHs, light tables), without the GPU.
+int host_of(const mpt_scene_desc* d, mpt::host_scene& hs) {
+    if (!d) return mpt::set_error(MPT_E_INVALID, "null description");
+    std::string err;
+    const int rc = mpt::build_host_scene(*d, hs, err);
+    return rc == MPT_OK ? MPT_OK : mpt::set_error(rc, err);
+}
+mpt::box3 bounds_of(const mpt::host_scene& hs, const mpt_scene_desc* d, const double* b6) {
+    mpt::box3 b = hs.world_box;
+    const double* src = b6 ? b6 : (d->has_light_bounds ? d->light_bounds : nullptr);
+    if (src)
+        for (int a = 0; a < 3; a++) b.ax[a] = mpt::span{src[2 * a], src[2 * a + 1]};
+    return b;
+}
+}  // namespace
+
+extern "C" int32_t mpt_desc_bounds(const mpt_scene_desc* desc, double* box6) {
+    if (!box6) return mpt::set_error(MPT_E_INVALID, "null argument");
+    mpt::host_scene hs;
+    int rc = host_of(desc, hs);
+    if (rc) return rc;
+    for (int a = 0; a < 3; a++) {
+        box6[2 * a] = hs.world_box.ax[a].min;
+        box6[2 * a + 1] = hs.world_box.ax[a].max;
+    }
+    return MPT_OK;
+}
+
+extern "C" int32_t mpt_desc_light_powers(const mpt_scene_desc* desc, const double* scene_bounds6, double* powers) {
+    mpt::host_scene hs;
+    int rc = host_of(desc, hs);
+    if (rc) return rc;
+    if (!powers && !hs.lights.empty()) return mpt::set_error(MPT_E_INVALID, "null argument");
+    std::vector<double> pw;
+    mpt::light_selection(hs, bounds_of(hs, desc, scene_bounds6), desc->light_uniform_fraction, &pw);
+    for (size_t k = 0; k < pw.size(); k++) powers[k] = pw[k];
+    return MPT_OK;
+}
+
+extern "C" int32_t mpt_desc_light_probabilities(const mpt_scene_desc* desc, const double* scene_bounds6,
+                                                double* probabilities) {
+    mpt::host_scene hs;
+    int rc = host_of(desc, hs);
+    if (rc) return rc;
+    if (!probabilities && !hs.lights.empty()) return mpt::set_error(MPT_E_INVALID, "null argument");
+    mpt::light_selection(hs, bounds_of(hs, desc, scene_bounds6), desc->light_uniform_fraction, nullptr);
+    for (size_t k = 0; k < hs.lights.size(); k++) probabilities[k] = hs.lights[k].probability;
+    return MPT_OK;
+}
 
 extern "C" int32_t mpt_find_conductor(const char* name, double* eta, double* k) {
     mpt::conductor_ior ior;

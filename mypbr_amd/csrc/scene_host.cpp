@@ -45,7 +45,7 @@ struct builder {
         nodes[index].offset = 0;
         nodes[index].axis = 0;
 
-        aabb bounds = box_empty(), cbounds = box_empty();
+        box3 bounds = box_empty(), cbounds = box_empty();
         for (size_t k = begin; k < end; k++) {
             bounds = box_merge(bounds, items[k].box);
             // aabb(c, c) pads the degenerate point box before the merge
@@ -58,13 +58,13 @@ struct builder {
 
         size_t count = end - begin;
         int axis = box_longest_axis(cbounds);
-        const interval extent = cbounds.ax[axis];
+        const span extent = cbounds.ax[axis];
 
         if (count == 1 || iv_size(extent) <= 0 || depth >= kMaxSahDepth)
             return make_leaf(index, begin, count, axis, depth);
 
         struct bin {
-            aabb box;
+            box3 box;
             int count;
         } bins[kBins];
         for (auto& b : bins) {
@@ -84,7 +84,7 @@ struct builder {
         double best_cost = kInf;
         int best_split = -1;
         for (int split = 0; split < kBins - 1; split++) {
-            aabb left = box_empty(), right = box_empty();
+            box3 left = box_empty(), right = box_empty();
             int lc = 0, rc = 0;
             for (int b = 0; b <= split; b++) {
                 left = box_merge(left, bins[b].box);
@@ -120,7 +120,7 @@ struct builder {
 };
 
 // Boxes of the primitive kinds (sphere.h:14-19, quad.h:17-22, mesh.h:40-41).
-aabb prim_box(int kind, const double* p) {
+box3 prim_box(int kind, const double* p) {
     if (kind == MPT_PRIM_SPHERE) {
         v3 c{p[0], p[1], p[2]};
         double r = std::fabs(p[3]);
@@ -136,8 +136,8 @@ aabb prim_box(int kind, const double* p) {
 }
 
 // instance bounding box: the eight transformed corners (instance.h:13-23).
-aabb instance_box(const xform& x, const aabb& b) {
-    aabb r = box_empty();
+box3 instance_box(const xform& x, const box3& b) {
+    box3 r = box_empty();
     for (int i = 0; i < 2; i++)
         for (int j = 0; j < 2; j++)
             for (int k = 0; k < 2; k++) {
@@ -161,7 +161,7 @@ void build_bvh(std::vector<build_item>& items, std::vector<bvh_node>& nodes) {
     nodes.clear();
     if (items.empty()) {
         bvh_node n{};
-        aabb e = box_empty();
+        box3 e = box_empty();
         for (int a = 0; a < 3; a++) {
             n.b[2 * a] = e.ax[a].min;
             n.b[2 * a + 1] = e.ax[a].max;
@@ -317,6 +317,78 @@ void conductor_averages(mpt_material& m) {
     }
 }
 
+void light_selection(host_scene& s, const box3& bounds, double uniform_fraction, std::vector<double>* power_out) {
+    const int nl = int(s.lights.size());
+    if (power_out) power_out->assign(size_t(nl), 0.0);
+    if (nl > 0 && (s.light_strategy == 1 || power_out)) {
+        double radius = 0;
+        const box3& sb = bounds;
+        double sx = iv_size(sb.ax[0]), sy = iv_size(sb.ax[1]), sz = iv_size(sb.ax[2]);
+        if (sx > 0 && sy > 0 && sz > 0) radius = 0.5 * std::sqrt(sx * sx + sy * sy + sz * sz);
+        if (!std::isfinite(radius) || radius <= 0) radius = 1;
+        auto prim_power = [&](int64_t p) {
+            int m = s.prim_mat[p];
+            if (m < 0) return 0.0;
+            double lum = luminance(v3{s.materials[m].avg_emission[0], s.materials[m].avg_emission[1],
+                                      s.materials[m].avg_emission[2]});
+            const double* q = &s.prim[size_t(p) * kPrimStride];
+            if (s.prim_kind[p] == MPT_PRIM_SPHERE) return kPi * 4 * kPi * q[3] * q[3] * lum;
+            if (s.prim_kind[p] == MPT_PRIM_QUAD) {
+                v3 n = cross(v3{q[3], q[4], q[5]}, v3{q[6], q[7], q[8]});
+                return kPi * length(n) * lum;
+            }
+            return kPi * q[12] * lum;
+        };
+        std::vector<double> power(nl);
+        double known_sum = 0;
+        int known = 0;
+        for (int k = 0; k < nl; k++) {
+            int ob = s.lights[k].object;
+            if (ob == -2) {  // sun_light::power (sun.h:66-71)
+                power[k] = kPi * radius * radius * luminance(s.sun_irradiance);
+            } else if (ob < 0) {  // environment_light::power (environment.h:116-127)
+                double sum = 0;
+                for (int j = 0; j < s.env_h; j++)
+                    for (int i = 0; i < s.env_w; i++) {
+                        double sin_theta = std::sin(kPi * (j + 0.5) / s.env_h);
+                        const double* p = &s.env_rgb[(size_t(j) * s.env_w + i) * 3];
+                        sum += std::fmax(luminance(v3{p[0], p[1], p[2]}) * sin_theta, 0.0);
+                    }
+                double integral = s.env_intensity * sum * (2 * kPi / s.env_w) * (kPi / s.env_h);
+                power[k] = kPi * radius * radius * integral;
+            } else {
+                const obj_rec& r = s.objects[ob];
+                double pw = 0;
+                if (r.kind == MPT_OBJ_PRIM) pw = prim_power(r.first);
+                else if (r.kind == MPT_OBJ_MESH)  // mesh::power, in the mesh's triangle order
+                    for (int64_t i = 0; i < r.count; i++) pw += prim_power(s.mlight_prim[size_t(r.light_table + i)]);
+                else
+                    for (int64_t i = 0; i < r.count; i++) pw += prim_power(r.first + i);
+                if (r.xform >= 0) pw = pw * area_scale(s.xforms[r.xform]);
+                power[k] = pw;
+            }
+            if (std::isfinite(power[k]) && power[k] > 0) {
+                known_sum += power[k];
+                known++;
+            }
+        }
+        if (power_out) *power_out = power;
+        double fallback = known > 0 ? known_sum / known : 1.0;
+        double total = 0;
+        for (auto& p : power) {
+            if (!std::isfinite(p) || p <= 0) p = fallback;
+            total += p;
+        }
+        const double uf = uniform_fraction;
+        if (s.light_strategy == 1)
+            for (int k = 0; k < nl; k++) s.lights[k].probability = (1 - uf) * power[k] / total + uf / double(nl);
+    }
+    s.light_cdf.assign(nl + 1, 0.0);
+    for (int k = 0; k < nl; k++) s.light_cdf[k + 1] = s.light_cdf[k] + s.lights[k].probability;
+    if (nl > 0) s.light_cdf[nl] = 1;
+
+}
+
 int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
     auto fail = [&](int code, const std::string& m) {
         err = m;
@@ -415,7 +487,7 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
     s.prim_kind.assign(d.prim_kind, d.prim_kind + np);
     s.prim_mat.assign(d.prim_material, d.prim_material + np);
     s.prim_canon.resize(np);
-    std::vector<aabb> pbox(np);
+    std::vector<box3> pbox(np);
     for (int64_t i = 0; i < np; i++) {
         const double* src = d.prim_data + 9 * i;
         double* dst = &s.prim[size_t(i) * kPrimStride];
@@ -477,7 +549,7 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
     // objects
     const int no = d.n_objects;
     s.objects.resize(no);
-    std::vector<aabb> obox(no);
+    std::vector<box3> obox(no);
     std::map<std::pair<int64_t, int64_t>, int32_t> mesh_roots;
     s.blas.clear();
     int64_t next_id = 0;
@@ -498,7 +570,7 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
             return fail(MPT_E_INVALID, "object " + std::to_string(o) + " has a bad primitive range");
         if (r.xform < -1 || r.xform >= d.n_xforms)
             return fail(MPT_E_INVALID, "object " + std::to_string(o) + " has a bad transform index");
-        aabb b = box_empty();
+        box3 b = box_empty();
         switch (r.kind) {
             case MPT_OBJ_PRIM:
                 if (r.count != 1) return fail(MPT_E_INVALID, "single-primitive object with count != 1");
@@ -535,7 +607,7 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
                     s.blas.insert(s.blas.end(), nodes.begin(), nodes.end());
                     // reorder the mesh's primitives into leaf order
                     std::vector<double> tmp(size_t(r.count) * kPrimStride);
-                    std::vector<aabb> tbox(r.count);
+                    std::vector<box3> tbox(r.count);
                     for (int64_t k = 0; k < r.count; k++) {
                         int64_t src = r.first + items[k].ref;
                         std::copy(&s.prim[size_t(src) * kPrimStride], &s.prim[size_t(src) * kPrimStride] + kPrimStride,
@@ -573,7 +645,7 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
                 }
                 r.blas_root = it->second;
                 const bvh_node& rn = s.blas[s.blas_root_node[r.blas_root]];
-                for (int a = 0; a < 3; a++) b.ax[a] = interval{rn.b[2 * a], rn.b[2 * a + 1]};
+                for (int a = 0; a < 3; a++) b.ax[a] = span{rn.b[2 * a], rn.b[2 * a + 1]};
                 break;
             }
             case MPT_OBJ_MEDIUM:
@@ -609,12 +681,15 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         items[o].centroid = box_centroid(obox[o]);
         items[o].ref = o;
     }
-    build_bvh(items, s.tlas);
     s.ctlas.clear();
-    pair_layout(s.tlas, 0, s.ctlas, s.ctlas_root);
+    s.tlas.clear();
+    if (no > 0) {  // an empty world has no tree: the walk never starts (nobj = 0)
+        build_bvh(items, s.tlas);
+        pair_layout(s.tlas, 0, s.ctlas, s.ctlas_root);
+    }
     s.qtlas.clear();
     s.qblas.clear();
-    if (s.ctlas_root.cnt == 0) s.ctlas_root.qoff = quad_layout(s.ctlas, s.ctlas_root.off, s.qtlas);
+    if (no > 0 && s.ctlas_root.cnt == 0) s.ctlas_root.qoff = quad_layout(s.ctlas, s.ctlas_root.off, s.qtlas);
     for (croot& r : s.cblas_roots)
         if (r.cnt == 0) r.qoff = quad_layout(s.cblas, r.off, s.qblas);
     s.tlas_items.resize(no);
@@ -709,70 +784,10 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         s.lights[k].pad = 0;
         s.lights[k].probability = nl > 0 ? 1.0 / nl : 0.0;
     }
-    if (nl > 0 && s.light_strategy == 1) {
-        double radius = 0;
-        const aabb& sb = s.world_box;
-        double sx = iv_size(sb.ax[0]), sy = iv_size(sb.ax[1]), sz = iv_size(sb.ax[2]);
-        if (sx > 0 && sy > 0 && sz > 0) radius = 0.5 * std::sqrt(sx * sx + sy * sy + sz * sz);
-        if (!std::isfinite(radius) || radius <= 0) radius = 1;
-        auto prim_power = [&](int64_t p) {
-            int m = s.prim_mat[p];
-            if (m < 0) return 0.0;
-            double lum = luminance(v3{s.materials[m].avg_emission[0], s.materials[m].avg_emission[1],
-                                      s.materials[m].avg_emission[2]});
-            const double* q = &s.prim[size_t(p) * kPrimStride];
-            if (s.prim_kind[p] == MPT_PRIM_SPHERE) return kPi * 4 * kPi * q[3] * q[3] * lum;
-            if (s.prim_kind[p] == MPT_PRIM_QUAD) {
-                v3 n = cross(v3{q[3], q[4], q[5]}, v3{q[6], q[7], q[8]});
-                return kPi * length(n) * lum;
-            }
-            return kPi * q[12] * lum;
-        };
-        std::vector<double> power(nl);
-        double known_sum = 0;
-        int known = 0;
-        for (int k = 0; k < nl; k++) {
-            int ob = s.lights[k].object;
-            if (ob == -2) {  // sun_light::power (sun.h:66-71)
-                power[k] = kPi * radius * radius * luminance(s.sun_irradiance);
-            } else if (ob < 0) {  // environment_light::power (environment.h:116-127)
-                double sum = 0;
-                for (int j = 0; j < s.env_h; j++)
-                    for (int i = 0; i < s.env_w; i++) {
-                        double sin_theta = std::sin(kPi * (j + 0.5) / s.env_h);
-                        const double* p = &s.env_rgb[(size_t(j) * s.env_w + i) * 3];
-                        sum += std::fmax(luminance(v3{p[0], p[1], p[2]}) * sin_theta, 0.0);
-                    }
-                double integral = s.env_intensity * sum * (2 * kPi / s.env_w) * (kPi / s.env_h);
-                power[k] = kPi * radius * radius * integral;
-            } else {
-                const obj_rec& r = s.objects[ob];
-                double pw = 0;
-                if (r.kind == MPT_OBJ_PRIM) pw = prim_power(r.first);
-                else if (r.kind == MPT_OBJ_MESH)  // mesh::power, in the mesh's triangle order
-                    for (int64_t i = 0; i < r.count; i++) pw += prim_power(s.mlight_prim[size_t(r.light_table + i)]);
-                else
-                    for (int64_t i = 0; i < r.count; i++) pw += prim_power(r.first + i);
-                if (r.xform >= 0) pw = pw * area_scale(s.xforms[r.xform]);
-                power[k] = pw;
-            }
-            if (std::isfinite(power[k]) && power[k] > 0) {
-                known_sum += power[k];
-                known++;
-            }
-        }
-        double fallback = known > 0 ? known_sum / known : 1.0;
-        double total = 0;
-        for (auto& p : power) {
-            if (!std::isfinite(p) || p <= 0) p = fallback;
-            total += p;
-        }
-        double uf = d.light_uniform_fraction;
-        for (int k = 0; k < nl; k++) s.lights[k].probability = (1 - uf) * power[k] / total + uf / double(nl);
-    }
-    s.light_cdf.assign(nl + 1, 0.0);
-    for (int k = 0; k < nl; k++) s.light_cdf[k + 1] = s.light_cdf[k] + s.lights[k].probability;
-    if (nl > 0) s.light_cdf[nl] = 1;
+    box3 lb = s.world_box;
+    if (d.has_light_bounds)
+        for (int a = 0; a < 3; a++) lb.ax[a] = span{d.light_bounds[2 * a], d.light_bounds[2 * a + 1]};
+    light_selection(s, lb, d.light_uniform_fraction, nullptr);
 
     s.spheres.assign(size_t(np) * 4, 0.0);
     for (int64_t i = 0; i < np; i++)

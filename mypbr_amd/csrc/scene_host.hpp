@@ -158,7 +158,7 @@ struct host_scene {
     // GGX albedo table (microfacet.h:265-338)
     std::vector<double> ggx_e;           // 32*32 [roughness][mu]
     std::vector<double> ggx_e_avg;       // 32
-    aabb world_box;
+    box3 world_box;
 };
 
 // A measured metal's head-on color (metal) and hemispherical average Fresnel
@@ -169,9 +169,15 @@ void conductor_averages(mpt_material& m);
 // error code with a message in err.
 int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err);
 
+// light_list::build (light_list.h:45-96) on a built host scene: each light's
+// power with the radius of `bounds` (half its diagonal), and the selection
+// probabilities and CDF from them (uniform_fraction as given); stores them
+// in s.lights and s.light_cdf. power_out, when set, gets the raw powers.
+void light_selection(host_scene& s, const box3& bounds, double uniform_fraction, std::vector<double>* power_out);
+
 // BVH build of bvh.h:118-205 over items; reorders items into leaf order.
 struct build_item {
-    aabb box;
+    box3 box;
     v3 centroid;
     int64_t ref;
 };

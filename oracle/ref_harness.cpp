@@ -2,10 +2,13 @@
 // compiled from the reference's own headers where they lie (see
 // oracle/Makefile, target `ref`) into oracle/_ref/libmpt_ref.so.
 //
-// TEST INFRASTRUCTURE, used only in the container that holds the reference
-// sources: it writes the golden vectors under tests/golden and pins the
-// CPU restatement (oracle.cpp). Nothing from the reference is copied here;
-// the harness only calls it.
+// TEST INFRASTRUCTURE. It is built only where the reference sources are
+// (their headers are compiled in place); the built library then travels
+// with the tests. It writes the golden vectors under tests/golden, pins the
+// CPU restatement (oracle.cpp), is compared with the GPU in tests where it
+// is present, and is the CPU baseline bench.py times (the timing builds,
+// Makefile target ref_timing). The product library never loads it.
+// Nothing from the reference is copied here; the harness only calls it.
 //
 // The reference's hit_record has no primitive ID (hittable.h:12-32), so a
 // hit is identified afterwards: every primitive of the world, in canonical

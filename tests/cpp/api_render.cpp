@@ -11,10 +11,22 @@
 //       rotate/translate instance, a quad light in a light_list, a checker
 //       floor) with save_every 2; prints the callback counts and writes the
 //       radiance of the returned frame followed by the last callback frame
+//   api_render lights <file.scene>
+//       loads the file (its loader calls lights.build(world.bounding_box()),
+//       scene.h:1248), builds the list again from world.bounding_box() as a
+//       program would, and prints the world box, each light's
+//       probability(k) and power(), the picks of a few xi, and the core's
+//       probabilities for the uploaded scene (mpt_scene_light_probabilities)
+//   api_render atmosphere <out.bin>
+//       bakes mpt::atmosphere with the settings of scenes/sky_atmosphere.scene
+//       (mie 1.5, intensity 3; sun at elevation 18, azimuth 35, intensity 4)
+//       at 256 x 128 as its loader does; writes the pixels (h*w*3 doubles)
+//       and then the sun's irradiance dimmed by sun_transmittance (3 doubles)
 //   api_render denoise <file.scene> <out.bin>
 //       renders the file at width 40, 4 spp with collect_features and runs
 //       denoiser::run on the frame; writes radiance, albedo, normal, the
 //       denoised radiance (h*w*3 doubles each) and the variance (h*w)
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -52,6 +64,51 @@ int main(int argc, char** argv) {
             return 1;
         }
     }
+    if (argc >= 3 && std::string(argv[1]) == "atmosphere") {
+        atmosphere_settings st;
+        st.mie = 1.5;
+        st.intensity = 3;
+        atmosphere air(st);
+        const double el = 18 * 3.1415926535897932385 / 180.0, az = 35 * 3.1415926535897932385 / 180.0;
+        const vec3 sun_dir(std::cos(el) * std::sin(az), std::sin(el), -std::cos(el) * std::cos(az));
+        const color e = 4.0 * color(1, 1, 1);
+        std::vector<color> px = air.bake(256, 128, sun_dir, e);
+        const color tr = air.sun_transmittance(sun_dir);
+        px.push_back(color(e[0] * tr[0], e[1] * tr[1], e[2] * tr[2]));
+        return write_frame(argv[2], px, nullptr) ? 0 : 1;
+    }
+    if (argc >= 3 && std::string(argv[1]) == "lights") {
+        scene sc;
+        std::string err;
+        if (!load_scene(argv[2], sc, &err)) {
+            std::fprintf(stderr, "%s\n", err.c_str());
+            return 1;
+        }
+        try {
+            const aabb box = sc.world.bounding_box();
+            std::printf("box %.17g %.17g %.17g %.17g %.17g %.17g\n", box.x.min, box.x.max, box.y.min, box.y.max,
+                        box.z.min, box.z.max);
+            std::vector<double> loaded;
+            for (size_t k = 0; k < sc.lights.objects.size(); k++) loaded.push_back(sc.lights.probability(k));
+            sc.lights.build(box);
+            for (size_t k = 0; k < sc.lights.objects.size(); k++)
+                std::printf("light %zu %.17g %.17g %.17g\n", k, sc.lights.probability(k), loaded[k],
+                            sc.lights.objects[k]->power());
+            for (double xi : {0.0, 0.05, 0.1, 0.15, 0.5, 0.999999})
+                std::printf("pick %.17g %zu\n", xi, sc.lights.objects.empty() ? size_t(0) : sc.lights.pick(xi));
+            gpu_scene gs(sc.world, sc.lights, sc.cam.environment);
+            std::vector<double> core(sc.lights.objects.size());
+            if (mpt_scene_light_probabilities(gs.get(), core.data()) != MPT_OK) {
+                std::fprintf(stderr, "%s\n", mpt_last_error());
+                return 1;
+            }
+            for (size_t k = 0; k < core.size(); k++) std::printf("core %zu %.17g\n", k, core[k]);
+            return 0;
+        } catch (const std::exception& e) {
+            std::fprintf(stderr, "%s\n", e.what());
+            return 1;
+        }
+    }
     if (argc >= 3 && std::string(argv[1]) == "objects") {
         hittable_list world;
         auto white = std::make_shared<lambertian>(color(0.73, 0.73, 0.73));
@@ -69,7 +126,7 @@ int main(int argc, char** argv) {
         world.add(light);
         light_list lights;
         lights.add(light);
-        lights.build();
+        lights.build(world.bounding_box());
 
         camera cam;
         cam.aspect_ratio = 4.0 / 3.0;

@@ -89,3 +89,94 @@ def test_cpp_denoiser_equals_c_abi_denoise(tmp_path):
     var = data[12 * n:].reshape(h, 40)
     want = mypbr_amd.denoise({"radiance": rad, "albedo": alb, "normal": nrm, "variance": var})
     assert np.array_equal(dn, want)
+
+
+def _light_lines(out):
+    box = None
+    mirror, loaded, power, picks, core = [], [], [], [], []
+    for line in out.splitlines():
+        f = line.split()
+        if f[0] == "box":
+            box = [float(x) for x in f[1:]]
+        elif f[0] == "light":
+            mirror.append(float(f[2]))
+            loaded.append(float(f[3]))
+            power.append(float(f[4]))
+        elif f[0] == "pick":
+            picks.append((float(f[1]), int(f[2])))
+        elif f[0] == "core":
+            core.append(float(f[2]))
+    return box, mirror, loaded, power, picks, core
+
+
+@pytest.mark.parametrize("name", ["cornell.scene", "test_lights.scene"])
+def test_cpp_light_list_matches_reference_probabilities(tmp_path, name):
+    """light_list::build(world.bounding_box()), probability(k) and pick(xi)
+    (light_list.h:45-127), and hittable::bounding_box / power, against the
+    probabilities the compiled reference's light_list gives for the same file
+    (tests/golden/light_probs.json) bit for bit. The GPU part (the uploaded
+    scene's probabilities) runs where a GPU is present."""
+    import json
+    import mypbr_amd as m
+    exe = _build(tmp_path)
+    r = subprocess.run([exe, "lights", scene_path(name)], capture_output=True, text=True, timeout=120)
+    box, mirror, loaded, power, picks, core = _light_lines(r.stdout)
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "light_probs.json")))[name]
+    assert mirror == gold and loaded == gold  # the loader's build and a program's build agree
+    # hittable::power of the emitters; the environment has none without a radius (environment.h:115)
+    assert [p > 0 for p in power] == {"cornell.scene": [True], "test_lights.scene": [True, True, True, False]}[name]
+    # the world box is the one the core builds its BVH from
+    d = m.describe(scene_path(name))
+    import ctypes as Cc
+    b6 = (Cc.c_double * 6)()
+    m._check(m.lib.mpt_desc_bounds(Cc.byref(d.desc), b6))
+    assert box == list(b6)
+    cdf = np.concatenate([[0.0], np.cumsum(gold)])
+    cdf[-1] = 1.0
+    for xi, k in picks:
+        assert k == min(max(int(np.searchsorted(cdf, xi, side="right")), 1), len(gold)) - 1
+    if r.returncode == 0:  # the GPU part ran
+        assert core == gold
+
+
+def test_desc_light_entry_points_match_reference_probabilities():
+    import ctypes as Cc
+    import json
+    import mypbr_amd as m
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "light_probs.json")))
+    for name, probs in gold.items():
+        d = m.describe(scene_path(name))
+        n = len(probs)
+        out = (Cc.c_double * max(n, 1))()
+        m._check(m.lib.mpt_desc_light_probabilities(Cc.byref(d.desc), None, out))
+        assert list(out)[:n] == probs, name
+        pw = (Cc.c_double * max(n, 1))()
+        m._check(m.lib.mpt_desc_light_powers(Cc.byref(d.desc), None, pw))
+        assert all(x > 0 for x in list(pw)[:n]), name
+    # bounds given explicitly: a bigger scene radius raises the environment's share (light_list.h:52-58)
+    d = m.describe(scene_path("test_lights.scene"))
+    b = (Cc.c_double * 6)()
+    m._check(m.lib.mpt_desc_bounds(Cc.byref(d.desc), b))
+    big = (Cc.c_double * 6)(*[x * 10 for x in b])
+    out = (Cc.c_double * 4)()
+    m._check(m.lib.mpt_desc_light_probabilities(Cc.byref(d.desc), big, out))
+    assert out[3] > gold["test_lights.scene"][3] and abs(sum(out) - 1) < 1e-12
+
+
+def test_cpp_atmosphere_bake_equals_the_loaders(tmp_path):
+    """mpt::atmosphere (atmosphere.h:36-165) for in-memory scenes: its bake and
+    sun_transmittance equal, bit for bit, what the .scene loader makes of
+    scenes/sky_atmosphere.scene (the environment's pixels and the dimmed sun),
+    which the sun tests check against the compiled reference."""
+    import mypbr_amd as m
+    exe = _build(tmp_path)
+    out = str(tmp_path / "air.bin")
+    r = subprocess.run([exe, "atmosphere", out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(out, dtype=np.float64)
+    keep = m.describe(scene_path("sky_atmosphere.scene"))  # owns the arrays d points to
+    d = keep.desc
+    assert (d.env_width, d.env_height) == (256, 128)
+    env = np.ctypeslib.as_array(d.env_rgb, (256 * 128 * 3,))
+    assert np.array_equal(got[:-3].view(np.uint64), env.view(np.uint64))
+    assert np.array_equal(got[-3:].view(np.uint64), np.array(list(d.sun_irradiance)).view(np.uint64))
