@@ -346,6 +346,21 @@ int32_t mpt_trace_closest_dev(mpt_scene* scene, const double* rays_dev, const do
                               int64_t n, int64_t* prim_id_dev, double* t_dev, uint8_t* front_face_dev,
                               void* hip_stream);
 
+/* The closest-hit query of mpt_trace_closest with grazing ties listed
+ * separately. The reference's hit_record names no primitive, so its tests
+ * identify a hit by testing each primitive alone (hittable::hit,
+ * hittable.h:34-38, with the ray moved into instance space as
+ * instance::hit does, instance.h:29-42) over the ray's whole interval;
+ * primitives whose t is bitwise equal to the closest hit's are the
+ * candidates. ties[i]: their number (0 on a miss, 1 for a unique hit, 2 or
+ * more for a tie); tie_min[i] / tie_max[i]: their smallest and largest
+ * canonical ID (-1 on a miss). prim_id is the primitive the BVH walk took,
+ * one of the candidates. Media are not tested. Every primitive is tested
+ * for every ray that hits: O(rays x primitives), for query batches. */
+int32_t mpt_trace_ties(mpt_scene* scene, const double* rays, const double* tmin_tmax, int64_t n,
+                       int64_t* prim_id, double* t, uint8_t* front_face, int32_t* ties, int64_t* tie_min,
+                       int64_t* tie_max);
+
 /* camera::render (camera.h:79-105): renders cam.samples_per_pixel passes
  * and writes the averaged frame (denoiser.h:14-20), row-major from the top
  * left, height = max(1, int(width / aspect)). radiance/albedo/normal are

@@ -53,6 +53,8 @@ def _load():
         "mpt_trace_closest": (C.c_int32, [vp, P(C.c_double), P(C.c_double), C.c_int64, P(C.c_int64),
                                           P(C.c_double), P(C.c_uint8)]),
         "mpt_trace_closest_dev": (C.c_int32, [vp, vp, vp, C.c_int64, vp, vp, vp, vp]),
+        "mpt_trace_ties": (C.c_int32, [vp, P(C.c_double), P(C.c_double), C.c_int64, P(C.c_int64), P(C.c_double),
+                                       P(C.c_uint8), P(C.c_int32), P(C.c_int64), P(C.c_int64)]),
         "mpt_render": (C.c_int32, [vp, P(Camera), P(RenderOpts), P(C.c_double), P(C.c_double), P(C.c_double),
                                    P(C.c_double), abi.PASS_CALLBACK, vp, P(RenderStats)]),
         "mpt_read_frame": (C.c_int32, [vp, P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double)]),
@@ -364,6 +366,23 @@ class Scene:
         _check(lib.mpt_trace_closest(self._h, _dp(rays), _dp(tt), n, ids.ctypes.data_as(C.POINTER(C.c_int64)),
                                      _dp(t), front.ctypes.data_as(C.POINTER(C.c_uint8))))
         return ids, t, front
+
+    def trace_ties(self, rays, tt):
+        """trace_closest plus the grazing ties: a dict of id, t, front, ties
+        (candidates with the closest t bitwise), tie_min and tie_max."""
+        rays = np.ascontiguousarray(rays, dtype=np.float64)
+        tt = np.ascontiguousarray(tt, dtype=np.float64)
+        n = rays.shape[0]
+        assert rays.shape == (n, 6) and tt.shape == (n, 2)
+        out = {"id": np.zeros(n, dtype=np.int64), "t": np.zeros(n), "front": np.zeros(n, dtype=np.uint8),
+               "ties": np.zeros(n, dtype=np.int32), "tie_min": np.zeros(n, dtype=np.int64),
+               "tie_max": np.zeros(n, dtype=np.int64)}
+        i64 = C.POINTER(C.c_int64)
+        _check(lib.mpt_trace_ties(self._h, _dp(rays), _dp(tt), n, out["id"].ctypes.data_as(i64), _dp(out["t"]),
+                                  out["front"].ctypes.data_as(C.POINTER(C.c_uint8)),
+                                  out["ties"].ctypes.data_as(C.POINTER(C.c_int32)), out["tie_min"].ctypes.data_as(i64),
+                                  out["tie_max"].ctypes.data_as(i64)))
+        return out
 
     def bsdf_eval(self, mats, q, front):
         mats = np.ascontiguousarray(mats, dtype=np.int32)

@@ -406,6 +406,56 @@ __global__ __launch_bounds__(BS) void k_query(dscene S, launch_ctx L, const doub
     }
 }
 
+// Grazing ties of closest hits found by k_query: every primitive is tested
+// alone over the ray's whole interval (its object's transform applied, media
+// left out, the ground plane last), and those whose t is bitwise equal to
+// the closest hit's are counted with their smallest and largest canonical
+// ID. One ray per thread over all primitives: a diagnostic for query
+// batches, O(rays x primitives).
+__global__ __launch_bounds__(256) void k_ties(dscene S, const double* rays, const double* tt, int64_t n,
+                                              const int64_t* id, const double* tc, int32_t* ties, int64_t* tie_min,
+                                              int64_t* tie_max) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        int32_t count = 0;
+        int64_t lo = -1, hi = -1;
+        if (id[i] >= 0) {
+            const v3 o{rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]};
+            const v3 d{rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]};
+            const double tmin = tt[2 * i], tmax = tt[2 * i + 1];
+            const uint64_t best = __double_as_longlong(tc[i]);
+            auto take = [&](int64_t c) {
+                lo = count == 0 ? c : (c < lo ? c : lo);
+                hi = count == 0 ? c : (c > hi ? c : hi);
+                count++;
+            };
+            for (int oi = 0; oi < S.nobj; oi++) {
+                const obj_rec r = S.obj[oi];
+                if (r.kind == MPT_OBJ_MEDIUM) continue;
+                v3 lo_o = o, lo_d = d;
+                if (r.xform >= 0) {
+                    lo_o = xf_inv_point(S.xf[r.xform], o);
+                    lo_d = xf_inv_vector(S.xf[r.xform], d);
+                }
+                const int64_t cnt = r.kind == MPT_OBJ_PRIM ? 1 : r.count;
+                for (int64_t k = 0; k < cnt; k++) {
+                    const int64_t p = r.first + k;
+                    double t;
+                    wcount wc{0, 0};
+                    if (prim_t<false, F_ALL>(S, p, lo_o, lo_d, tmin, tmax, t, wc) &&
+                        uint64_t(__double_as_longlong(t)) == best)
+                        take(r.id_base + S.pcanon[p]);
+                }
+            }
+            hitinfo g;
+            if (ground_t<F_ALL | F_UV>(S, o, d, tmin, tmax, g) && uint64_t(__double_as_longlong(g.t)) == best)
+                take(S.ground_id);
+        }
+        ties[i] = count;
+        tie_min[i] = lo;
+        tie_max[i] = hi;
+    }
+}
+
 // ------------------------------------------------------------------ camera rays
 struct batch_args {
     uint64_t seed;
@@ -1553,6 +1603,53 @@ int32_t mpt_trace_closest(mpt_scene* s, const double* rays, const double* tt, in
         if (hipMemcpy(id, did, n * sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(t, dto, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(front, df, n, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = set_error(MPT_E_HIP, "hipMemcpy of the results failed");
+    }
+    cleanup();
+    return rc;
+}
+
+int32_t mpt_trace_ties(mpt_scene* s, const double* rays, const double* tt, int64_t n, int64_t* id, double* t,
+                       uint8_t* front, int32_t* ties, int64_t* tie_min, int64_t* tie_max) {
+    if (!s || n < 0) return set_error(MPT_E_INVALID, "bad argument");
+    if (n == 0) return MPT_OK;
+    if (!rays || !tt || !id || !t || !front || !ties || !tie_min || !tie_max)
+        return set_error(MPT_E_INVALID, "null buffer");
+    double *dr = nullptr, *dt = nullptr, *dto = nullptr;
+    int64_t *did = nullptr, *dlo = nullptr, *dhi = nullptr;
+    int32_t* dties = nullptr;
+    uint8_t* df = nullptr;
+    auto cleanup = [&]() {
+        (void)hipFree(dr), (void)hipFree(dt), (void)hipFree(dto), (void)hipFree(did), (void)hipFree(df);
+        (void)hipFree(dlo), (void)hipFree(dhi), (void)hipFree(dties);
+    };
+    if (hipMalloc(&dr, n * 6 * sizeof(double)) != hipSuccess || hipMalloc(&dt, n * 2 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&dto, n * sizeof(double)) != hipSuccess || hipMalloc(&did, n * sizeof(int64_t)) != hipSuccess ||
+        hipMalloc(&df, n) != hipSuccess || hipMalloc(&dlo, n * sizeof(int64_t)) != hipSuccess ||
+        hipMalloc(&dhi, n * sizeof(int64_t)) != hipSuccess || hipMalloc(&dties, n * sizeof(int32_t)) != hipSuccess) {
+        cleanup();
+        return set_error(MPT_E_HIP, "hipMalloc failed for the ray batch");
+    }
+    if (hipMemcpy(dr, rays, n * 6 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dt, tt, n * 2 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        cleanup();
+        return set_error(MPT_E_HIP, "hipMemcpy of the ray batch failed");
+    }
+    int rc = mpt_trace_closest_dev(s, dr, dt, n, did, dto, df, nullptr);
+    if (rc == MPT_OK) {
+        const int64_t blocks = std::min<int64_t>((n + 255) / 256, int64_t(s->grid));
+        hipLaunchKernelGGL(k_ties, dim3(unsigned(blocks)), dim3(256), 0, s->stream, s->ds, dr, dt, n, did, dto, dties,
+                           dlo, dhi);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess)
+            rc = set_error(MPT_E_HIP, "tie listing kernel failed");
+    }
+    if (rc == MPT_OK) {
+        if (hipMemcpy(id, did, n * sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(t, dto, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(front, df, n, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(ties, dties, n * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(tie_min, dlo, n * sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(tie_max, dhi, n * sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess)
             rc = set_error(MPT_E_HIP, "hipMemcpy of the results failed");
     }
     cleanup();

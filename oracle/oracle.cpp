@@ -2417,6 +2417,66 @@ int32_t orc_trace_linear(orc_scene* S, const double* rays, const double* tt, int
     return MPT_OK;
 }
 
+// Grazing ties, listed as the reference's test harness lists them: after the
+// closest hit, every primitive of the world is tested alone over the ray's
+// whole interval (the instance transforms above it applied, medium
+// boundaries left out); those whose t is bitwise equal to the closest hit's
+// are the candidates. ties: their number (0 on a miss); tie_min/tie_max:
+// their smallest and largest canonical ID (-1 on a miss).
+struct leaf_prim {
+    const hittable* prim;
+    std::vector<const affine*> xf;  // outermost first
+    int64_t shift;
+};
+
+static void expand_leaves(const hittable* h, std::vector<const affine*> xf, int64_t shift, std::vector<leaf_prim>& out) {
+    if (auto l = dynamic_cast<const hlist*>(h)) {
+        for (auto* o : l->objects) expand_leaves(o, xf, shift, out);
+    } else if (auto in = dynamic_cast<const instance*>(h)) {
+        xf.push_back(&in->x);
+        expand_leaves(in->object, xf, shift, out);
+    } else if (auto m = dynamic_cast<const mesh_obj*>(h)) {
+        for (auto* t : m->tris) out.push_back({t, xf, shift});
+    } else if (auto s = dynamic_cast<const idshift*>(h)) {
+        expand_leaves(s->inner, xf, shift + s->shift, out);
+    } else if (dynamic_cast<const medium*>(h)) {
+        return;
+    } else {
+        out.push_back({h, xf, shift});
+    }
+}
+
+int32_t orc_trace_ties(orc_scene* S, const double* rays, const double* tt, int64_t n, int64_t* id, double* t,
+                       int32_t* ties, int64_t* tie_min, int64_t* tie_max) {
+    std::vector<leaf_prim> leaves;
+    for (auto* o : S->s.world) expand_leaves(o, {}, 0, leaves);
+    if (S->s.ground) leaves.push_back({S->s.ground, {}, 0});
+    for (int64_t i = 0; i < n; i++) {
+        ray r(point3(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]), vec3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]));
+        interval iv(tt[2 * i], tt[2 * i + 1]);
+        hit_record rec;
+        hit_ctx hc{0, 0, false};
+        ties[i] = 0, tie_min[i] = -1, tie_max[i] = -1;
+        if (!S->s.hit(r, iv, rec, hc)) {
+            id[i] = -1, t[i] = 0;
+            continue;
+        }
+        id[i] = rec.id;
+        t[i] = rec.t;
+        for (const leaf_prim& lp : leaves) {
+            ray local = r;
+            for (const affine* x : lp.xf) local = ray(x->inv_point(local.orig), x->inv_vector(local.dir));
+            hit_record one;
+            if (!lp.prim->hit(local, iv, one, hc) || std::memcmp(&one.t, &rec.t, sizeof(double)) != 0) continue;
+            const int64_t c = one.id + lp.shift;
+            tie_min[i] = ties[i] == 0 ? c : std::min(tie_min[i], c);
+            tie_max[i] = ties[i] == 0 ? c : std::max(tie_max[i], c);
+            ties[i]++;
+        }
+    }
+    return MPT_OK;
+}
+
 int32_t orc_light_probabilities(orc_scene* S, double* out) {
     for (size_t k = 0; k < S->s.prob.size(); k++) out[k] = S->s.prob[k];
     return MPT_OK;

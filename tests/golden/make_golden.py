@@ -8,6 +8,9 @@ queries) are seeded and regenerated identically by the tests:
   ggx_table.npz         the 32x32 E(mu, alpha) table and E_avg (microfacet.h:265-338)
   trace_<scene>.npz     4096 rays of the seeded batch with the reference's
                         closest-hit ID, t, front_face and tie count (bvh.h:45-86)
+  trace_ties.npz        rays aimed at where primitives meet, and rays whose interval
+                        ends or starts at the hit t, with the reference's closest hit
+                        and its tie candidates (count, smallest and largest ID)
   trace_digests.json    SHA-256 of the reference's IDs and t bits over the full
                         1M-ray batches, tie and miss counts
   light_probs.json      light_list::probability per light (light_list.h:45-127)
@@ -36,7 +39,7 @@ queries) are seeded and regenerated identically by the tests:
                         writes (display_settings::apply + to_byte, tonemap.h,
                         color3.h) on the images and settings of tests/display_cases.py
 
-Usage: python tests/golden/make_golden.py [step ...]  (steps: ggx trace bsdf render denoise display obj textures materials; all by default)
+Usage: python tests/golden/make_golden.py [step ...]  (steps: ggx trace ties bsdf render denoise display obj textures materials; all by default)
 (all steps take about a minute)
 """
 import hashlib
@@ -158,6 +161,7 @@ MATERIAL_SCENES = {
 SUN_SCENES = {
     "sun.scene": (48, 256, 8, 12),
     "sky_atmosphere.scene": (48, 256, 8, 12),
+    "sun_disk.scene": (48, 256, 8, 12),
 }
 
 
@@ -243,8 +247,8 @@ def main():
     if not ol.reference_available():
         sys.exit("oracle/_ref/libmpt_ref.so is not built (make -C oracle ref, where the reference is present)")
     R = ol.reference()
-    steps = set(sys.argv[1:]) or {"ggx", "trace", "bsdf", "render", "denoise", "display", "obj", "textures",
-                                  "materials", "sun"}
+    steps = set(sys.argv[1:]) or {"ggx", "trace", "ties", "bsdf", "render", "denoise", "display", "obj",
+                                  "textures", "materials", "sun"}
     if "denoise" in steps:
         golden_denoise()
     if "display" in steps:
@@ -261,6 +265,8 @@ def main():
         golden_ggx(R)
     if "trace" in steps:
         golden_trace()
+    if "ties" in steps:
+        golden_ties()
     if "bsdf" in steps:
         golden_bsdf(R)
     if "render" in steps:
@@ -302,6 +308,108 @@ def golden_trace():
         json.dump(digests, f, indent=1)
     with open(os.path.join(HERE, "light_probs.json"), "w") as f:
         json.dump(probs, f, indent=1)
+
+
+TIE_SCENES = ["cornell.scene", "test_lights.scene", "weekend_cover.scene"]
+N_TIE_AIMED = 3072
+N_TIE_BOUND = 1024
+
+
+def _desc_arrays(desc):
+    """Primitive and object arrays of a scene description as numpy."""
+    n = desc.n_prims
+    kind = np.ctypeslib.as_array(desc.prim_kind, (n,)).copy()
+    data = np.ctypeslib.as_array(desc.prim_data, (n * 9,)).reshape(n, 9).copy()
+    no = desc.n_objects
+    okind = np.ctypeslib.as_array(desc.object_kind, (no,)).copy()
+    first = np.ctypeslib.as_array(desc.object_first, (no,)).copy()
+    count = np.ctypeslib.as_array(desc.object_count, (no,)).copy()
+    oxf = np.ctypeslib.as_array(desc.object_xform, (no,)).copy()
+    xfs = [(np.array(desc.xforms[k].m[:]).reshape(3, 3), np.array(desc.xforms[k].t[:])) for k in range(desc.n_xforms)]
+    return kind, data, okind, first, count, oxf, xfs
+
+
+def tie_targets(desc):
+    """Points where primitives meet or end: corners and dyadic edge points of
+    quads, the lowest point of spheres (where they rest on a floor), vertices
+    and edge midpoints of triangles; an instanced object's points are mapped
+    into the world by its transform."""
+    kind, data, okind, first, count, oxf, xfs = _desc_arrays(desc)
+    pts = []
+    for o in range(len(okind)):
+        local = []
+        for p in range(first[o], first[o] + (1 if okind[o] == 0 else count[o])):
+            q = data[p]
+            if kind[p] == 1:  # quad: Q, u, v
+                Q, u, v = q[0:3], q[3:6], q[6:9]
+                for s in np.arange(17) / 16.0:
+                    local += [Q + s * u, Q + s * v, Q + u + s * v, Q + v + s * u]
+            elif kind[p] == 0:  # sphere: its lowest point
+                local.append(q[0:3] - np.array([0.0, q[3], 0.0]))
+            else:  # triangle
+                a, b, c = q[0:3], q[3:6], q[6:9]
+                local += [a, b, c, 0.5 * (a + b), 0.5 * (b + c), 0.5 * (a + c)]
+        if oxf[o] >= 0:
+            mm, t = xfs[oxf[o]]
+            local = [mm @ x + t for x in local]
+        pts += local
+    return np.unique(np.array(pts), axis=0)
+
+
+def tie_batch(name, R, rng):
+    """Rays from dyadic origins (and the camera) aimed exactly at tie_targets,
+    so that the planes of primitives meeting there give bitwise-equal t; and
+    seeded-batch rays whose interval ends or starts exactly at the reference's
+    hit t (inclusive quads, exclusive spheres and triangles)."""
+    d = m.describe(scene_path(name))
+    targets = tie_targets(d.desc)
+    lo, hi = targets.min(0), targets.max(0)
+    span = np.maximum(hi - lo, 1.0)
+    idx = rng.integers(0, len(targets), N_TIE_AIMED)
+    P = targets[idx]
+    org = np.round((lo + (0.05 + 0.9 * rng.random((N_TIE_AIMED, 3))) * span) * 8) / 8  # dyadic interior points
+    cam = np.array(d.camera.lookfrom[:])
+    org[: N_TIE_AIMED // 4] = cam
+    dirs = P - org
+    keep = np.linalg.norm(dirs, axis=1) > 1e-9
+    aimed = np.concatenate([org[keep], dirs[keep]], axis=1)
+    aimed_tt = np.tile([0.001, np.inf], (aimed.shape[0], 1))
+    o = ol.OracleScene(d.desc)
+    rays, tt = o.ray_batch(d.camera, BATCH_SEED + 7, 4 * N_TIE_BOUND)
+    first = R_trace(name, rays, tt)
+    hit = np.nonzero(first["id"] >= 0)[0][: 2 * (N_TIE_BOUND // 2)]
+    half = len(hit) // 2
+    tmax_rays, tmax_tt = rays[hit[:half]], tt[hit[:half]].copy()
+    tmax_tt[:, 1] = first["t"][hit[:half]]  # the interval ends at the hit
+    tmin_rays, tmin_tt = rays[hit[half:]], tt[hit[half:]].copy()
+    tmin_tt[:, 0] = first["t"][hit[half:]]  # the interval starts at the hit
+    tmin_tt[:, 1] = np.inf
+    all_rays = np.concatenate([aimed, tmax_rays, tmin_rays])
+    all_tt = np.concatenate([aimed_tt, tmax_tt, tmin_tt])
+    kind = np.concatenate([np.zeros(len(aimed), np.int8), np.ones(half, np.int8), np.full(len(hit) - half, 2, np.int8)])
+    return all_rays, all_tt, kind
+
+
+def R_trace(name, rays, tt):
+    return ol.RefScene(scene_path(name)).trace(rays, tt, threads=os.cpu_count() or 8)
+
+
+def golden_ties():
+    """trace_ties.npz: per scene, rays aimed at where primitives meet plus
+    rays whose interval bound is the hit t, with the reference's closest-hit
+    t and front face, candidate count and smallest/largest candidate ID."""
+    rng = np.random.default_rng(11)
+    out = {}
+    for name in TIE_SCENES:
+        rays, tt, kind = tie_batch(name, None, rng)
+        r = R_trace(name, rays, tt)
+        key = name.replace(".scene", "")
+        out.update({f"{key}_rays": rays, f"{key}_tt": tt, f"{key}_kind": kind, f"{key}_id": r["id"],
+                    f"{key}_t": r["t"], f"{key}_front": r["front"], f"{key}_ties": r["ties"],
+                    f"{key}_tie_min": r["tie_min"], f"{key}_tie_max": r["tie_max"]})
+        print(name, "rays", len(rays), "ties", int((r["ties"] > 1).sum()), "misses", int((r["id"] == -1).sum()),
+              flush=True)
+    np.savez_compressed(os.path.join(HERE, "trace_ties.npz"), **out)
 
 
 def golden_bsdf(R):

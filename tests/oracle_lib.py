@@ -48,6 +48,8 @@ def oracle():
             "orc_scene_destroy": (None, [vp]),
             "orc_trace": (C.c_int32, [vp, dp, dp, C.c_int64, P(C.c_int64), dp, P(C.c_uint8), C.c_int32]),
             "orc_trace_linear": (C.c_int32, [vp, dp, dp, C.c_int64, P(C.c_int64), dp]),
+            "orc_trace_ties": (C.c_int32, [vp, dp, dp, C.c_int64, P(C.c_int64), dp, P(C.c_int32), P(C.c_int64),
+                                           P(C.c_int64)]),
             "orc_surface": (C.c_int32, [vp, dp, dp, C.c_int64, dp]),
             "orc_light_probabilities": (C.c_int32, [vp, dp]),
             "orc_ggx_table": (C.c_int32, [dp, dp]),
@@ -107,6 +109,15 @@ class OracleScene:
         t = np.zeros(n)
         self.lib.orc_trace_linear(self.h, _d(rays), _d(tt), n, _i64(ids), _d(t))
         return ids, t
+
+    def trace_ties(self, rays, tt):
+        """Closest hit plus the candidates with its t bitwise (ties, tie_min, tie_max)."""
+        n = rays.shape[0]
+        out = {"id": np.zeros(n, dtype=np.int64), "t": np.zeros(n), "ties": np.zeros(n, dtype=np.int32),
+               "tie_min": np.zeros(n, dtype=np.int64), "tie_max": np.zeros(n, dtype=np.int64)}
+        self.lib.orc_trace_ties(self.h, _d(rays), _d(tt), n, _i64(out["id"]), _d(out["t"]), _i32(out["ties"]),
+                                _i64(out["tie_min"]), _i64(out["tie_max"]))
+        return out
 
     def light_probabilities(self, n):
         out = np.zeros(max(n, 1))

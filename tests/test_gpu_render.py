@@ -3,8 +3,10 @@
 Both sides draw the same counter-based random numbers in the same order
 and compute in double, so paths agree until a transcendental (cos, sin,
 exp, pow) rounds differently on the GPU; such a path then wanders off and
-a handful of pixels differ. The tests bound that: almost every pixel is
-equal to 1e-12 relative, and the image means agree to 1e-3 relative."""
+a handful of pixels differ. The tests bound that: at least 99.5% of the
+pixels are equal to 1e-12 relative (measured on MI355X: every pixel of
+these scenes, 99.96% of the variance guide's), and the image means agree
+to 1e-3 relative."""
 import numpy as np
 import pytest
 
@@ -42,7 +44,7 @@ def test_render_matches_oracle(mpt, name, width, spp, depth):
     cpu = OracleScene(d.desc).render(cam, seed=7)["radiance"]
     s = mpt.Scene.from_desc(d.desc, d.camera)
     f = s.render(cam, seed=7)
-    _compare(f.radiance, cpu, 0.97, 1e-3)
+    _compare(f.radiance, cpu, 0.995, 1e-3)
     st = f.stats
     assert st.paths == width * cam.height() * spp
     assert st.extension_rays >= st.paths
@@ -55,9 +57,9 @@ def test_render_features_match_oracle(mpt):
     cam = camera_copy(d.camera, image_width=64, samples_per_pixel=4, max_depth=20, collect_features=1)
     cpu = OracleScene(d.desc).render(cam, seed=3)
     f = mpt.Scene.from_desc(d.desc, d.camera).render(cam, seed=3)
-    _compare(f.radiance, cpu["radiance"], 0.97, 1e-3)
-    _compare(f.albedo, cpu["albedo"], 0.99, 1e-3)
-    _compare(f.normal, cpu["normal"], 0.99, 1e-2)
+    _compare(f.radiance, cpu["radiance"], 0.995, 1e-3)
+    _compare(f.albedo, cpu["albedo"], 0.995, 1e-3)
+    _compare(f.normal, cpu["normal"], 0.995, 1e-3)
     assert np.all(f.variance >= 0)
 
 
@@ -137,3 +139,68 @@ def test_autotune_picks_settings_that_leave_the_image_unchanged(mpt, tmp_path):
     r2 = s2.autotune(cam, cache=cache)
     assert r2["cached"] and all(r2[k] == r[k] for k in s2._TUNED)
     assert np.array_equal(s2.render(cam, seed=5).radiance, other)
+
+
+def test_variance_matches_oracle(mpt):
+    """frame.variance (camera.h:240-244): the spread of the pixel's sample
+    luminances over n - 1; per pixel against the oracle at equal seed."""
+    from mypbr_amd.abi import camera_copy
+    from oracle_lib import OracleScene
+    for name, width, spp, depth in (("weekend_cover.scene", 64, 8, 20), ("cornell.scene", 48, 16, 16)):
+        d = mpt.describe(scene_path(name))
+        cam = camera_copy(d.camera, image_width=width, samples_per_pixel=spp, max_depth=depth, collect_features=1)
+        cpu = OracleScene(d.desc).render(cam, seed=5)
+        f = mpt.Scene.from_desc(d.desc, d.camera).render(cam, seed=5)
+        assert f.variance.shape == cpu["variance"].shape
+        _compare(f.variance[..., None], cpu["variance"][..., None], 0.995, 1e-2)
+        assert np.all(f.variance >= 0) and f.variance.max() > 0
+
+
+@pytest.mark.parametrize("name,overrides", [
+    ("cornell.scene", dict(ortho_height=600.0)),
+    ("cornell.scene", dict(ortho_height=450.0, defocus_angle=1.5, focus_dist=800.0)),
+    ("weekend_cover.scene", dict(blades=6, blade_rotation=20.0)),
+    ("test_lights.scene", dict(blades=3, defocus_angle=2.0)),
+])
+def test_camera_projections_match_oracle(mpt, name, overrides):
+    """Orthographic rays (camera.h:291-301: each from its image-plane point
+    moved back by focus_dist) and the polygon aperture (camera.h:329-339:
+    a blade triangle picked uniformly, a point in it), per pixel against the
+    oracle at equal seed."""
+    from mypbr_amd.abi import camera_copy
+    from oracle_lib import OracleScene
+    d = mpt.describe(scene_path(name))
+    cam = camera_copy(d.camera, image_width=56, samples_per_pixel=8, max_depth=12, **overrides)
+    cpu = OracleScene(d.desc).render(cam, seed=9)["radiance"]
+    plain = OracleScene(d.desc).render(camera_copy(d.camera, image_width=56, samples_per_pixel=8, max_depth=12),
+                                       seed=9)["radiance"]
+    assert not np.array_equal(cpu, plain), "the camera override changed nothing"
+    f = mpt.Scene.from_desc(d.desc, d.camera).render(cam, seed=9)
+    _compare(f.radiance, cpu, 0.995, 1e-3)
+
+
+def test_non_finite_samples_are_dropped_like_the_oracle(mpt):
+    """sample_radiance (camera.h:281-287) counts a path whose radiance has a
+    NaN or infinite component as black. test_lights.scene with its box
+    light's emission made infinite: every path that reaches that light,
+    directly or by light sampling, is non-finite; the frame must stay finite
+    and equal the oracle's per pixel at equal seed."""
+    from mypbr_amd.abi import MAT_DIFFUSE_LIGHT, camera_copy
+    from oracle_lib import OracleScene
+    d = mpt.describe(scene_path("test_lights.scene"))
+    warm = [k for k in range(d.desc.n_materials) if d.desc.materials[k].type == MAT_DIFFUSE_LIGHT
+            and list(d.desc.materials[k].emit) == [6.0, 3.0, 1.0]]
+    assert len(warm) == 1
+    cam = camera_copy(d.camera, image_width=48, samples_per_pixel=8, max_depth=12, collect_features=1)
+    finite = OracleScene(d.desc).render(cam, seed=3)["radiance"]
+    m = d.desc.materials[warm[0]]
+    m.emit[0] = m.emit[1] = m.emit[2] = float("inf")
+    m.avg_emission[0] = m.avg_emission[1] = m.avg_emission[2] = float("inf")
+    cpu = OracleScene(d.desc).render(cam, seed=3)
+    f = mpt.Scene.from_desc(d.desc, d.camera).render(cam, seed=3)
+    assert np.all(np.isfinite(cpu["radiance"])) and np.all(np.isfinite(cpu["variance"]))
+    _compare(f.radiance, cpu["radiance"], 0.995, 1e-3)
+    _compare(f.variance[..., None], cpu["variance"][..., None], 0.995, 1e-2)
+    assert cpu["radiance"].max() > 0
+    # the dropped paths darken a good share of the pixels against the finite light
+    assert np.mean(np.any(cpu["radiance"] < finite, axis=-1)) > 0.1

@@ -280,9 +280,12 @@ def test_gpu_render_matches_oracle():
     cam = camera_copy(d.camera, image_width=64, samples_per_pixel=8, max_depth=12, collect_features=1)
     cpu = ol.OracleScene(d.desc).render(cam, seed=7)
     f = m.Scene.from_desc(d.desc, d.camera).render(cam, seed=7)
+    # the radiance bar is lower here: bilinear texel weights and the normal
+    # maps' renormalisation send about 2% of this image's paths apart in the
+    # last bits (97.7% of pixels equal on MI355X); the guides are exact
     _compare(f.radiance, cpu["radiance"], 0.97, 1e-3)
-    _compare(f.albedo, cpu["albedo"], 0.97, 1e-3)
-    _compare(f.normal, cpu["normal"], 0.97, 1e-3)
+    _compare(f.albedo, cpu["albedo"], 0.995, 1e-3)
+    _compare(f.normal, cpu["normal"], 0.995, 1e-3)
 
 
 @pytest.mark.gpu
