@@ -357,6 +357,64 @@ def display_bench(m, radiance, reps=50):
     return res
 
 
+def network_bench(m, s, cam, reps=20):
+    """The learned denoiser's network at 1080p on the matrix cores: the
+    six-layer, 64-channel KPCNN's forward pass (implicit-GEMM 3x3
+    convolutions, k_conv3x3) on a frame already in HBM, and one training
+    step (forward, backward, Adam) on a batch of 16 crops of 256^2. FLOPs
+    are the convolutions' multiply-adds x 2 as computed (padded channels
+    included); the kernel application is not counted."""
+    from mypbr_amd.learn import KPCNN
+    from mypbr_amd.abi import camera_copy
+    import ctypes as C
+    import numpy as np
+    c = camera_copy(cam, samples_per_pixel=4, collect_features=1)
+    s.render(c, seed=5, read=False)
+    dev = s.frame_dev(c)
+    net = KPCNN(layers=6, hidden=64, seed=1)
+    cin, cout, _, _ = net.layout()
+    h, w = c.height(), c.image_width
+    flops = sum(2.0 * w * h * 9 * ci * co for ci, co in zip(cin, cout))
+    out = m.DeviceArray((h, w, 3), np.float32)
+    loss = C.c_double(0)
+    vp = lambda t: t.handle if t is not None else None  # noqa: E731
+    run = lambda: m._check(m.lib.mpt_kpcnn_run(net._h, *[vp(dev[k]) for k in ("radiance", "albedo", "normal",  # noqa: E731
+                                                                              "variance", "depth")],
+                                                1, h, w, None, 0.0, out.handle, C.byref(loss)))
+    for _ in range(3):
+        run()
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run()
+    synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    # one training step on a batch of 16 crops of 256^2 (synthetic frames): forward, backward, Adam
+    from mypbr_amd.learn import to_device
+    rng = np.random.default_rng(3)
+    B, ch = 16, 256
+    frames = [dict(radiance=rng.uniform(0, 2, (ch, ch, 3)), albedo=rng.uniform(0.05, 0.95, (ch, ch, 3)),
+                   normal=rng.uniform(-1, 1, (ch, ch, 3)), variance=rng.exponential(0.05, (ch, ch)),
+                   depth=rng.uniform(0, 20, (ch, ch))) for _ in range(B)]
+    tdev = to_device(frames)
+    tgt = m.DeviceArray.of(rng.uniform(0, 2, (B, ch, ch, 3)), np.float32)
+    for _ in range(2):
+        net.run(tdev, target=tgt, lr=1e-4)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.run(tdev, target=tgt, lr=1e-4)
+    step = (time.perf_counter() - t0) / reps
+    train_flops = 3 * sum(2.0 * B * ch * ch * 9 * ci * co for ci, co in zip(cin, cout))
+    res = {"network": "KPCNN 6 layers x 64 channels, 3x3 convolutions, %d parameters" % net.nparams,
+           "width": w, "height": h, "forward_ms": dt * 1e3, "forward_flops": flops,
+           "train_step_ms": step * 1e3, "train_step_batch": f"{B} x {ch}x{ch}",
+           "train_step_tflops": train_flops / step / 1e12,
+           "roofline": {"kernel": "k_conv3x3 (whole forward pass)", "bound": "mfma", "achieved": flops / dt / 1e12,
+                        "peak": 2500.0, "unit": "TFLOP/s", "frac": flops / dt / 1e12 / 2500.0, "traffic": None}}
+    log("network:", json.dumps(res))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -377,6 +435,7 @@ def main():
                     help="batch8: all 8 scenes in one batch of launches (default), or one render per scene")
     ap.add_argument("--no-display", action="store_true",
                     help="skip timing the display pipeline (bloom, vignette, tonemap, sRGB bytes) at 1080p")
+    ap.add_argument("--no-network", action="store_true", help="skip the learned denoiser's network timing")
     ap.add_argument("--no-quality", action="store_true",
                     help="skip the equal-time relMSE study of the learned forms against the plain tracer")
     ap.add_argument("--use-graph", type=int, default=0, choices=[-1, 0, 1],
@@ -502,6 +561,9 @@ def main():
     if not args.no_display:
         f = s.render(cam, seed=7, first_pass=0, pass_count=4)
         line["display"] = display_bench(m, f.radiance)
+
+    if not args.no_network:
+        line["network"] = network_bench(m, s, cam)
 
     if not args.no_quality:
         # the metric's second half: equal-time relMSE (eps 0.01) against a 4096-spp render of the plain tracer

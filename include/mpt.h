@@ -214,6 +214,16 @@ typedef struct mpt_scene_desc {
     int32_t has_light_bounds;
     int32_t light_bounds_pad;
     double light_bounds[6];
+
+    /* Height fog (height_fog, medium.h:98-155): n_fogs unbounded media,
+     * tested beside the BVH after it and before the ground plane, in this
+     * order (scene.h:656-666, 1237-1238). fog_params holds 3 doubles per fog:
+     * density at y = base, scale height (the density falls by e per scale
+     * height up), base; fog_material the phase (volume) material. */
+    int32_t n_fogs;
+    int32_t fog_pad;
+    const double* fog_params;      /* [n_fogs * 3] */
+    const int32_t* fog_material;   /* [n_fogs] */
 } mpt_scene_desc;
 
 /* Camera and render settings: the public fields of the reference camera
@@ -420,6 +430,13 @@ int32_t mpt_autotune_key(const mpt_scene* scene, const mpt_camera* cam, char* bu
 int32_t mpt_read_frame(mpt_scene* scene, double* radiance, double* albedo, double* normal,
                        double* variance);
 
+/* mpt_read_frame into device buffers (any may be NULL): the averaged frame
+ * of the last render computed on the GPU from its running sums, for callers
+ * that keep render -> guides -> denoiser on the device. Guides only when the
+ * last render collected them. stream may be NULL (then it synchronises). */
+int32_t mpt_read_frame_dev(mpt_scene* scene, double* radiance_dev, double* albedo_dev, double* normal_dev,
+                           double* variance_dev, void* hip_stream);
+
 /* Image height camera::initialize computes (camera.h:143-144). */
 int32_t mpt_image_height(const mpt_camera* cam);
 
@@ -457,6 +474,10 @@ int32_t mpt_bsdf_eval(mpt_scene* scene, const int32_t* mats, const double* n_rin
  * for the learned denoiser; the reference's frame has no depth
  * (denoiser.h:14-20). */
 int32_t mpt_primary_depth(mpt_scene* scene, const mpt_camera* cam, double* depth);
+
+/* mpt_primary_depth into a device buffer (h*w doubles), traced on the GPU
+ * from the pixel centres; stream may be NULL (then it synchronises). */
+int32_t mpt_primary_depth_dev(mpt_scene* scene, const mpt_camera* cam, double* depth_dev, void* hip_stream);
 
 /* ---- classical denoiser: denoiser::run (denoiser.h:45-193) on the GPU ---- */
 

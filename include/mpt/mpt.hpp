@@ -579,9 +579,23 @@ public:
     std::shared_ptr<material> mat;
 };
 
+// Fog whose density falls off with height, density(y) = density *
+// exp(-(y - base) / scale_height), filling all of space (height_fog,
+// medium.h:98-155): free flights drawn by inverting the closed-form optical
+// depth along the ray. No finite box: it goes in scene::unbounded.
+class height_fog : public hittable {
+public:
+    height_fog(double density, double scale_height, double base, std::shared_ptr<material> phase)
+        : density(density), scale_height(scale_height), base(base), phase(std::move(phase)) {}
+    void flatten(flat_builder& b, int32_t xform) const override;
+    aabb bounding_box() const override;  // medium.h:151: the universe
+    double density, scale_height, base;
+    std::shared_ptr<material> phase;
+};
+
 struct scene {  // scene.h:32-40
     hittable_list world;
-    hittable_list unbounded;  // objects without a finite box (a ground plane)
+    hittable_list unbounded;  // objects without a finite box (height fogs, then a ground plane)
     light_list lights;
     camera cam;
     display_settings display;  // how radiance becomes PNG/PPM pixels
@@ -609,6 +623,7 @@ public:
     void set_prim_normals(int64_t prim, const double* n9);  // a triangle's three vertex normals
     void set_prim_uvs(int64_t prim, const double* uv6);     // a triangle's three texture coordinates
     void set_ground(double height, int32_t material);       // the one ground plane
+    void add_fog(double density, double scale_height, double base, int32_t material);  // a height fog
     int32_t add_object(int32_t kind, int64_t first, int64_t count, int32_t xform, const hittable* owner);
     std::map<const void*, std::pair<int64_t, int64_t>> mesh_ranges;
     struct impl;

@@ -58,6 +58,8 @@ def _load():
         "mpt_render": (C.c_int32, [vp, P(Camera), P(RenderOpts), P(C.c_double), P(C.c_double), P(C.c_double),
                                    P(C.c_double), abi.PASS_CALLBACK, vp, P(RenderStats)]),
         "mpt_read_frame": (C.c_int32, [vp, P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double)]),
+        "mpt_read_frame_dev": (C.c_int32, [vp, vp, vp, vp, vp, vp]),
+        "mpt_primary_depth_dev": (C.c_int32, [vp, P(Camera), vp, vp]),
         "mpt_render_batch": (C.c_int32, [P(vp), P(Camera), C.c_int32, P(RenderOpts), P(RenderStats)]),
         "mpt_image_height": (C.c_int32, [P(Camera)]),
         "mpt_ggx_albedo_table": (C.c_int32, [P(C.c_double), P(C.c_double)]),
@@ -460,6 +462,29 @@ class Scene:
         d = np.zeros((cam.height(), cam.image_width))
         _check(lib.mpt_primary_depth(self._h, C.byref(cam), _dp(d)))
         return d
+
+    def primary_depth_dev(self, camera=None):
+        """primary_depth traced on the GPU into a DeviceArray (h, w)."""
+        cam = abi.camera_copy(camera if camera is not None else self.camera)
+        d = DeviceArray((cam.height(), cam.image_width), np.float64)
+        _check(lib.mpt_primary_depth_dev(self._h, C.byref(cam), d.handle, None))
+        return d
+
+    def frame_dev(self, camera, features=True, depth=True):
+        """The last render's averaged frame as DeviceArrays (radiance, and with
+        features albedo, normal, variance; with depth the primary depth), the
+        inputs of the learned denoiser without a host round trip."""
+        w, h = camera.image_width, camera.height()
+        out = {"radiance": DeviceArray((h, w, 3), np.float64)}
+        if features:
+            out.update(albedo=DeviceArray((h, w, 3), np.float64), normal=DeviceArray((h, w, 3), np.float64),
+                       variance=DeviceArray((h, w), np.float64))
+        _check(lib.mpt_read_frame_dev(self._h, out["radiance"].handle, out["albedo"].handle if features else None,
+                                      out["normal"].handle if features else None,
+                                      out["variance"].handle if features else None, None))
+        if depth:
+            out["depth"] = self.primary_depth_dev(camera)
+        return out
 
     def read_frame(self, camera):
         w, h = camera.image_width, camera.height()

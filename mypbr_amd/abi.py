@@ -87,6 +87,10 @@ class SceneDesc(C.Structure):
         ("has_light_bounds", C.c_int32),
         ("light_bounds_pad", C.c_int32),
         ("light_bounds", C.c_double * 6),
+        ("n_fogs", C.c_int32),
+        ("fog_pad", C.c_int32),
+        ("fog_params", C.POINTER(C.c_double)),
+        ("fog_material", C.POINTER(C.c_int32)),
     ]
 
 

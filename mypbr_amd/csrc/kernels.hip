@@ -456,6 +456,49 @@ __global__ __launch_bounds__(256) void k_ties(dscene S, const double* rays, cons
     }
 }
 
+// Depth of the first surface seen through each pixel centre (no jitter or
+// defocus, the ray camera::get_ray makes for offset (0, 0) without the lens,
+// camera.h:291-301): t * |d| of the closest hit, 0 for the sky.
+template <int BS, int FEAT>
+__global__ __launch_bounds__(BS) void k_depth(dscene S, launch_ctx L, v3 pixel00, v3 du, v3 dv, v3 center, v3 w_axis,
+                                              double ortho, double focus, int w, int h, double* depth) {
+    STACK_SETUP();
+    for (int64_t k = st.gtid; k < int64_t(w) * h; k += L.nthreads) {
+        const int i = int(k % w), j = int(k / w);
+        const v3 ps = pixel00 + (double(i) * du) + (double(j) * dv);
+        const v3 org = ortho > 0 ? ps + focus * w_axis : center;
+        const v3 d = ps - org;
+        hitinfo hi;
+        wcount wc{0, 0};
+        const bool hit = trace_closest<false, FEAT>(S, org, d, 0.001, kInf, hi, st, false, 0, 0, wc);
+        depth[k] = hit ? hi.t * std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z) : 0.0;
+    }
+}
+
+// camera::average (camera.h:223-246) of the running sums on the device.
+__global__ void k_average(const double* __restrict__ acc, int64_t npix, int samples, int features, double* radiance,
+                          double* albedo, double* normal, double* variance) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= npix) return;
+    const double scale = 1.0 / samples;
+    const double* ar = acc;
+    const double* aa = ar + 3 * npix;
+    const double* an = aa + 3 * npix;
+    const double* al = an + 3 * npix;
+    const double r0 = scale * ar[3 * k], r1 = scale * ar[3 * k + 1], r2 = scale * ar[3 * k + 2];
+    if (radiance) radiance[3 * k] = r0, radiance[3 * k + 1] = r1, radiance[3 * k + 2] = r2;
+    if (!features) return;
+    if (albedo)
+        for (int c = 0; c < 3; c++) albedo[3 * k + c] = scale * aa[3 * k + c];
+    if (normal)
+        for (int c = 0; c < 3; c++) normal[3 * k + c] = scale * an[3 * k + c];
+    if (variance) {
+        const double mean = luminance(v3{r0, r1, r2});
+        const double spread = std::fmax(scale * al[k] - mean * mean, 0.0);
+        variance[k] = samples > 1 ? spread / (samples - 1) : spread;
+    }
+}
+
 // ------------------------------------------------------------------ camera rays
 struct batch_args {
     uint64_t seed;
@@ -1275,6 +1318,9 @@ struct mpt_scene {
     // the scene and camera tables of batched renders (mpt_render_batch)
     void* batch_tab = nullptr;
     size_t batch_tab_bytes = 0;
+    // the averaged frame on the device, staged for mpt_read_frame's copies
+    double* frame_buf = nullptr;
+    int64_t frame_buf_pix = 0;
 
     void drop_graphs() {
         for (hipGraphExec_t g : graphs)
@@ -1293,6 +1339,7 @@ struct mpt_scene {
         if (counters) (void)hipFree(counters);
         if (acc) (void)hipFree(acc);
         if (batch_tab) (void)hipFree(batch_tab);
+        if (frame_buf) (void)hipFree(frame_buf);
         if (stream) (void)hipStreamDestroy(stream);
         if (pinned_counts) (void)hipHostFree(pinned_counts);
     }
@@ -1741,30 +1788,6 @@ int ensure_paths(mpt_scene* s, int64_t cap) {
     return MPT_OK;
 }
 
-void average_frame(const mpt_scene* s, const std::vector<double>& acc, int npix, int samples, bool features,
-                   double* radiance, double* albedo, double* normal, double* variance) {
-    // camera::average (camera.h:223-246)
-    double scale = 1.0 / samples;
-    const double* ar = acc.data();
-    const double* aa = ar + 3 * size_t(npix);
-    const double* an = aa + 3 * size_t(npix);
-    const double* al = an + 3 * size_t(npix);
-    (void)s;
-    for (int k = 0; k < npix; k++) {
-        double r0 = scale * ar[3 * k], r1 = scale * ar[3 * k + 1], r2 = scale * ar[3 * k + 2];
-        if (radiance) radiance[3 * k] = r0, radiance[3 * k + 1] = r1, radiance[3 * k + 2] = r2;
-        if (!features) continue;
-        if (albedo)
-            for (int c = 0; c < 3; c++) albedo[3 * k + c] = scale * aa[3 * k + c];
-        if (normal)
-            for (int c = 0; c < 3; c++) normal[3 * k + c] = scale * an[3 * k + c];
-        if (variance) {
-            double mean = luminance(v3{r0, r1, r2});
-            double spread = std::fmax(scale * al[k] - mean * mean, 0.0);
-            variance[k] = samples > 1 ? spread / (samples - 1) : spread;
-        }
-    }
-}
 
 }  // namespace
 
@@ -1772,10 +1795,67 @@ extern "C" {
 
 int32_t mpt_read_frame(mpt_scene* s, double* radiance, double* albedo, double* normal, double* variance) {
     if (!s || !s->acc || s->last_done <= 0) return set_error(MPT_E_INVALID, "no frame rendered yet");
-    int npix = int(s->acc_pix);
-    std::vector<double> acc(size_t(npix) * 10);
-    HIP_TRY(hipMemcpy(acc.data(), s->acc, acc.size() * sizeof(double), hipMemcpyDeviceToHost));
-    average_frame(s, acc, npix, s->last_done, s->last_cam.collect_features != 0, radiance, albedo, normal, variance);
+    // camera::average on the device (k_average), then only the requested
+    // outputs cross PCIe
+    const int64_t npix = s->acc_pix;
+    const bool features = s->last_cam.collect_features != 0;
+    if (s->frame_buf_pix < npix) {
+        if (s->frame_buf) HIP_TRY(hipFree(s->frame_buf));
+        s->frame_buf = nullptr;
+        s->frame_buf_pix = 0;
+        HIP_TRY(hipMalloc(&s->frame_buf, size_t(npix) * 10 * sizeof(double)));
+        s->frame_buf_pix = npix;
+    }
+    double* fr = s->frame_buf;
+    double* fa = fr + 3 * npix;
+    double* fn = fa + 3 * npix;
+    double* fv = fn + 3 * npix;
+    hipLaunchKernelGGL(k_average, dim3(unsigned((npix + 255) / 256)), dim3(256), 0, s->stream, s->acc, npix,
+                       s->last_done, features ? 1 : 0, fr, fa, fn, fv);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (radiance) HIP_TRY(hipMemcpy(radiance, fr, size_t(npix) * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (features) {
+        if (albedo) HIP_TRY(hipMemcpy(albedo, fa, size_t(npix) * 3 * sizeof(double), hipMemcpyDeviceToHost));
+        if (normal) HIP_TRY(hipMemcpy(normal, fn, size_t(npix) * 3 * sizeof(double), hipMemcpyDeviceToHost));
+        if (variance) HIP_TRY(hipMemcpy(variance, fv, size_t(npix) * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return MPT_OK;
+}
+
+int32_t mpt_read_frame_dev(mpt_scene* s, double* radiance, double* albedo, double* normal, double* variance,
+                           void* stream) {
+    if (!s || !s->acc || s->last_done <= 0) return set_error(MPT_E_INVALID, "no frame rendered yet");
+    const bool features = s->last_cam.collect_features != 0;
+    if (!features && (albedo || normal || variance))
+        return set_error(MPT_E_INVALID, "the last render collected no guides");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
+    const int64_t npix = s->acc_pix;
+    hipLaunchKernelGGL(k_average, dim3(unsigned((npix + 255) / 256)), dim3(256), 0, st, s->acc, npix, s->last_done,
+                       features ? 1 : 0, radiance, albedo, normal, variance);
+    HIP_TRY(hipGetLastError());
+    if (!stream) HIP_TRY(hipStreamSynchronize(st));
+    return MPT_OK;
+}
+
+int32_t mpt_primary_depth_dev(mpt_scene* s, const mpt_camera* cam, double* depth, void* stream) {
+    if (!s || !cam || !depth) return set_error(MPT_E_INVALID, "null argument");
+    if (cam->image_width <= 0 || !(cam->aspect_ratio > 0)) return set_error(MPT_E_INVALID, "bad camera");
+    const dcam C = init_camera(*cam, s->hs.env_w > 0);
+    launch_ctx L{s->ovf, int64_t(s->grid) * s->block};
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
+#define DEPTH_ARGS s->ds, L, C.pixel00, C.du, C.dv, C.center, C.w_axis, C.ortho_height, C.focus_dist, C.w, C.h, depth
+    if (s->profile == 1)
+        hipLaunchKernelGGL((k_depth<256, F_SPHERE>), dim3(s->grid), dim3(256), 0, st, DEPTH_ARGS);
+    else if (s->profile == 2)
+        hipLaunchKernelGGL((k_depth<256, kQuadProfile>), dim3(s->grid), dim3(256), 0, st, DEPTH_ARGS);
+    else if (s->profile == 3)
+        hipLaunchKernelGGL((k_depth<256, F_ALL | F_UV>), dim3(s->grid), dim3(256), 0, st, DEPTH_ARGS);
+    else
+        hipLaunchKernelGGL((k_depth<256, F_ALL>), dim3(s->grid), dim3(256), 0, st, DEPTH_ARGS);
+#undef DEPTH_ARGS
+    HIP_TRY(hipGetLastError());
+    if (!stream) HIP_TRY(hipStreamSynchronize(st));
     return MPT_OK;
 }
 

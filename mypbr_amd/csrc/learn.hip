@@ -9,14 +9,16 @@
 // neighbourhood of the noisy lighting (clamped at the border), which is then
 // multiplied back by the albedo (Bako et al. 2017, kernel prediction).
 //
-// Every convolution is an im2col + GEMM on the matrix cores:
-//   forward    Y[p][co]   = sum_k Xcol[p][k] W[co][k]          (k = tap * Cin + ci)
-//   weights    dW[co][k]  = sum_p G[p][co] Xcol[p][k]
-//   inputs     dXcol[p][k]= sum_co G[p][co] W[co][k], then col2im over the taps
-// with one GEMM kernel, k_gemm, on v_mfma_f32_16x16x16_bf16: BF16 operands,
-// FP32 accumulation. Activations are stored BF16 (NHWC), weights kept in
-// FP32 (Adam state too) with a BF16 copy for the GEMMs; the kernel
-// application, loss and their gradients are FP32.
+// Every convolution is an implicit GEMM on the matrix cores (k_conv3x3:
+// the input tile and its halo staged in LDS, the nine taps read at shifted
+// offsets, no im2col buffer):
+//   forward    Y[p][co]   = sum_{t,ci} X[p + off(t)][ci] W[co][t][ci]
+//   inputs     dX[p][ci]  = sum_{t,co} G[p + off(t)][co] W[co][8 - t][ci]  (the same kernel)
+//   weights    dW[co][t][ci] = sum_p G[p][co] X[p + off(t)][ci]            (k_conv3x3_wgrad)
+// on v_mfma_f32_16x16x32_bf16: BF16 operands, FP32 accumulation.
+// Activations are stored BF16 (NHWC), weights kept in FP32 (Adam state too)
+// with a BF16 copy for the products; the kernel application, loss and their
+// gradients are FP32.
 #include <hip/hip_runtime.h>
 
 #include <memory>
@@ -27,43 +29,225 @@
 namespace mpt {
 namespace nn {
 
-// ------------------------------------------------------------------ conv helpers
-// im2col of X [B][H][W][Cin] (BF16) into Xcol [B*H*W][9*Cin]: tap t = (dy + 1) * 3 + (dx + 1),
-// zero outside the image.
-__global__ void k_im2col(int64_t npix, int H, int W, int Cin, const bf16* __restrict__ X, bf16* __restrict__ Xcol) {
-    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;  // one (pixel, tap) pair
-    if (e >= npix * 9) return;
-    const int64_t p = e / 9;
-    const int t = int(e % 9);
-    const int x = int(p % W), y = int((p / W) % H);
-    const int64_t b = p / (int64_t(W) * H);
-    const int yy = y + t / 3 - 1, xx = x + t % 3 - 1;
-    // Cin is a multiple of 8: whole 16-byte groups
-    uint4* dst = reinterpret_cast<uint4*>(Xcol + p * 9 * Cin + t * Cin);
-    if (yy < 0 || yy >= H || xx < 0 || xx >= W) {
-        for (int c = 0; c < Cin / 8; c++) dst[c] = uint4{0, 0, 0, 0};
-        return;
+// ------------------------------------------------------------------ implicit-GEMM convolution
+// Y[p][co] = sum over taps t and channels ci of X[p + off(t)][ci] W[co][t * CIN + ci]
+// (off(t) = (t / 3 - 1, t % 3 - 1) rows and columns, zero outside the image),
+// computed on the matrix cores without an im2col buffer: a block stages its
+// weights [COUT][9 CIN] in LDS once, then walks tiles of kTileH rows x
+// kTileW columns of one image; each tile's input halo ((kTileH + 2) x
+// (kTileW + 2) pixels x CIN) is staged in LDS, and the nine taps read it at
+// shifted offsets. Wave w computes tile row w: 64 pixels = 4 fragments of 16
+// against COUT / 16 fragments of 16 output channels, 32 k per MFMA step.
+// The next tile's halo is loaded into registers while the current one is
+// multiplied. One kernel serves the forward pass (EPI 2: + bias, ReLU,
+// BF16; EPI 3: + bias, FP32 logits) and the input gradient (EPI 0: FP32),
+// which is the same convolution of the output gradient with the weights
+// flipped in the taps and transposed in the channels (k_flip_weights).
+constexpr int kTileW = 64;
+constexpr int kTileH = 4;
+constexpr int kHaloW = kTileW + 2;
+constexpr int kHaloH = kTileH + 2;
+constexpr int kMaxCin = 64;  // input and output channels of one convolution
+constexpr int kHaloChunks = (kHaloH * kHaloW * kMaxCin / 8 + 255) / 256;  // 16-byte loads per thread
+
+struct conv_shape {
+    int B, H, W, CIN;
+    int tiles_x, tiles_y;
+    __host__ __device__ int64_t ntiles() const { return int64_t(B) * tiles_x * tiles_y; }
+};
+
+__device__ __forceinline__ void halo_load(const conv_shape& S, const bf16* __restrict__ X, int64_t tile,
+                                          uint4 (&pre)[kHaloChunks]) {
+    const int tx = int(tile % S.tiles_x);
+    const int ty = int((tile / S.tiles_x) % S.tiles_y);
+    const int64_t b = tile / (int64_t(S.tiles_x) * S.tiles_y);
+    const int c8n = S.CIN / 8;
+    const int n = kHaloH * kHaloW * c8n;
+#pragma unroll
+    for (int r = 0; r < kHaloChunks; r++) {
+        const int e = int(threadIdx.x) + 256 * r;
+        uint4 v{0, 0, 0, 0};
+        if (e < n) {
+            const int c8 = e % c8n, pix = e / c8n;
+            const int y = ty * kTileH + pix / kHaloW - 1, x = tx * kTileW + pix % kHaloW - 1;
+            if (y >= 0 && y < S.H && x >= 0 && x < S.W)
+                v = *reinterpret_cast<const uint4*>(X + ((b * S.H + y) * S.W + x) * S.CIN + c8 * 8);
+        }
+        pre[r] = v;
     }
-    const uint4* src = reinterpret_cast<const uint4*>(X + ((b * H + yy) * W + xx) * Cin);
-    for (int c = 0; c < Cin / 8; c++) dst[c] = src[c];
 }
 
-// dX[p][c] = sum over taps of dXcol[q][t * Cin + c] where q is the pixel whose
-// tap t reads p (q = p - offset(t)); FP32.
-__global__ void k_col2im(int64_t npix, int H, int W, int Cin, const float* __restrict__ dXcol, float* __restrict__ dX) {
-    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;  // one (pixel, channel) pair
-    if (e >= npix * Cin) return;
-    const int64_t p = e / Cin;
-    const int c = int(e % Cin);
-    const int x = int(p % W), y = int((p / W) % H);
-    const int64_t b = p / (int64_t(W) * H);
-    float s = 0.f;
-    for (int t = 0; t < 9; t++) {
-        const int qy = y - (t / 3 - 1), qx = x - (t % 3 - 1);
-        if (qy < 0 || qy >= H || qx < 0 || qx >= W) continue;
-        s += dXcol[((b * H + qy) * W + qx) * 9 * Cin + t * Cin + c];
+template <int NF, int EPI>
+__global__ __launch_bounds__(256) void k_conv3x3(conv_shape S, const bf16* __restrict__ X, const bf16* __restrict__ Wt,
+                                                 const float* __restrict__ bias, void* __restrict__ Yv) {
+    constexpr int COUT = 16 * NF;
+    constexpr int kMaxKP = 9 * kMaxCin + 8;
+    __shared__ __attribute__((aligned(16))) bf16 lw[COUT * kMaxKP];
+    __shared__ __attribute__((aligned(16))) bf16 lx[kHaloH * kHaloW * (kMaxCin + 8)];
+    const int CIN = S.CIN;
+    const int K = 9 * CIN;
+    const int KP = (K + 31) / 32 * 32 + 8;  // weight row stride in LDS (the pad k read as zero)
+    const int XS = CIN + 8;                 // halo pixel stride in LDS
+    for (int e = threadIdx.x; e < COUT * (KP / 8); e += 256) {
+        const int co = e / (KP / 8), k8 = e % (KP / 8) * 8;
+        uint4 v{0, 0, 0, 0};
+        if (k8 < K) v = *reinterpret_cast<const uint4*>(Wt + int64_t(co) * K + k8);
+        *reinterpret_cast<uint4*>(lw + co * KP + k8) = v;
     }
-    dX[e] = s;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r16 = lane & 15, g = lane >> 4;
+    const int c8n = CIN / 8;
+    const int nchunks = kHaloH * kHaloW * c8n;
+    const int64_t ntiles = S.ntiles();
+    uint4 pre[kHaloChunks];
+    if (int64_t(blockIdx.x) < ntiles) halo_load(S, X, blockIdx.x, pre);
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        __syncthreads();  // the previous tile's reads of the halo are done (and the weights are staged)
+#pragma unroll
+        for (int r = 0; r < kHaloChunks; r++) {
+            const int e = int(threadIdx.x) + 256 * r;
+            if (e < nchunks) *reinterpret_cast<uint4*>(lx + (e / c8n) * XS + (e % c8n) * 8) = pre[r];
+        }
+        __syncthreads();
+        if (tile + gridDim.x < ntiles) halo_load(S, X, tile + gridDim.x, pre);
+        float4_t acc[4][NF];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < NF; j++) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+        for (int kc = 0; kc < K; kc += 32) {
+            const int k = kc + 8 * g;
+            const int t = k / CIN, ci = k - t * CIN;
+            const bool tap = t < 9;
+            const int dy = tap ? t / 3 : 0, dx = tap ? t % 3 : 0;
+            bf16x8_t a[4], b[NF];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                uint4 u = *reinterpret_cast<const uint4*>(lx + ((w + dy) * kHaloW + 16 * i + r16 + dx) * XS +
+                                                          (tap ? ci : 0));
+                if (!tap) u = uint4{0, 0, 0, 0};
+                a[i] = as_bf16x8(u);
+            }
+#pragma unroll
+            for (int j = 0; j < NF; j++) b[j] = as_bf16x8(*reinterpret_cast<const uint4*>(lw + (16 * j + r16) * KP + k));
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < NF; j++) acc[i][j] = mfma32(a[i], b[j], acc[i][j]);
+        }
+        const int tx = int(tile % S.tiles_x);
+        const int ty = int((tile / S.tiles_x) % S.tiles_y);
+        const int64_t bimg = tile / (int64_t(S.tiles_x) * S.tiles_y);
+        const int y = ty * kTileH + w;
+        if (y >= S.H) continue;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int x = tx * kTileW + 16 * i + 4 * g + q;
+                if (x >= S.W) continue;
+                const int64_t p = (bimg * S.H + y) * S.W + x;
+#pragma unroll
+                for (int j = 0; j < NF; j++) {
+                    const int co = 16 * j + r16;
+                    float v = acc[i][j][q];
+                    if constexpr (EPI == 0) {
+                        static_cast<float*>(Yv)[p * COUT + co] = v;
+                    } else if constexpr (EPI == 3) {
+                        static_cast<float*>(Yv)[p * COUT + co] = v + bias[co];
+                    } else {
+                        v += bias[co];
+                        if (!(v > 0.f)) v = 0.f;
+                        static_cast<bf16*>(Yv)[p * COUT + co] = to_bf16(v);
+                    }
+                }
+            }
+    }
+}
+
+// Wf[ci][t][co] = W[co][8 - t][ci]: the weights of the input-gradient
+// convolution dX[p][ci] = sum_t sum_co G[p - off(t)][co] W[co][t][ci]
+// = sum_t' sum_co G[p + off(t')][co] W[co][8 - t'][ci].
+__global__ void k_flip_weights(int COUT, int CIN, const bf16* __restrict__ W, bf16* __restrict__ Wf) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;  // index into Wf
+    if (e >= int64_t(COUT) * 9 * CIN) return;
+    const int co = int(e % COUT);
+    const int t = int((e / COUT) % 9);
+    const int ci = int(e / (int64_t(COUT) * 9));
+    Wf[e] = W[(int64_t(co) * 9 + (8 - t)) * CIN + ci];
+}
+
+// dW[co][t * CIN + ci] += sum_p G[p][co] X[p + off(t)][ci]: the weight
+// gradient as a GEMM over pixels (M = COUT, N = CIN, K = pixels), one tap per
+// blockIdx.y. A block walks chunks of 256 pixels, stages the chunk's output
+// gradient and the tap-shifted inputs transposed ([channel][pixel]) in LDS,
+// so that a lane's eight consecutive pixels are one 16-byte read, and adds
+// its partial sums into dW atomically at the end. FPW: 16 x 16 fragments of
+// dW per wave.
+template <int FPW>
+__global__ __launch_bounds__(256) void k_conv3x3_wgrad(conv_shape S, int COUT, const bf16* __restrict__ G,
+                                                       const bf16* __restrict__ X, float* __restrict__ dW) {
+    constexpr int kChunk = 256, kRow = kChunk + 8;
+    __shared__ __attribute__((aligned(16))) bf16 gt[kMaxCin * kRow];
+    __shared__ __attribute__((aligned(16))) bf16 xt[kMaxCin * kRow];
+    const int CIN = S.CIN;
+    const int t = blockIdx.y, dy = t / 3 - 1, dx = t % 3 - 1;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r16 = lane & 15, g = lane >> 4;
+    const int nfr = CIN / 16, nfrags = (COUT / 16) * nfr;
+    const int64_t npix = int64_t(S.B) * S.H * S.W;
+    float4_t acc[FPW];
+#pragma unroll
+    for (int q = 0; q < FPW; q++) acc[q] = float4_t{0.f, 0.f, 0.f, 0.f};
+    for (int64_t p0 = int64_t(blockIdx.x) * kChunk; p0 < npix; p0 += int64_t(gridDim.x) * kChunk) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < kChunk * (COUT / 8); e += 256) {
+            const int px = e % kChunk, c8 = e / kChunk;
+            const int64_t p = p0 + px;
+            uint4 v{0, 0, 0, 0};
+            if (p < npix) v = *reinterpret_cast<const uint4*>(G + p * COUT + c8 * 8);
+            const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+            for (int q = 0; q < 8; q++) gt[(c8 * 8 + q) * kRow + px] = h[q];
+        }
+        for (int e = threadIdx.x; e < kChunk * (CIN / 8); e += 256) {
+            const int px = e % kChunk, c8 = e / kChunk;
+            const int64_t p = p0 + px;
+            uint4 v{0, 0, 0, 0};
+            if (p < npix) {
+                const int x = int(p % S.W) + dx, y = int((p / S.W) % S.H) + dy;
+                if (x >= 0 && x < S.W && y >= 0 && y < S.H)
+                    v = *reinterpret_cast<const uint4*>(X + (p + int64_t(dy) * S.W + dx) * CIN + c8 * 8);
+            }
+            const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+            for (int q = 0; q < 8; q++) xt[(c8 * 8 + q) * kRow + px] = h[q];
+        }
+        __syncthreads();
+        for (int kc = 0; kc < kChunk; kc += 32) {
+#pragma unroll
+            for (int q = 0; q < FPW; q++) {
+                const int f = w + 4 * q;
+                if (f >= nfrags) break;
+                const int mi = f / nfr, ni = f % nfr;
+                const bf16x8_t a = as_bf16x8(*reinterpret_cast<const uint4*>(gt + (16 * mi + r16) * kRow + kc + 8 * g));
+                const bf16x8_t b = as_bf16x8(*reinterpret_cast<const uint4*>(xt + (16 * ni + r16) * kRow + kc + 8 * g));
+                acc[q] = mfma32(a, b, acc[q]);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < FPW; q++) {
+        const int f = w + 4 * q;
+        if (f >= nfrags) break;
+        const int mi = f / nfr, ni = f % nfr;
+        const int ci = 16 * ni + r16;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int co = 16 * mi + 4 * g + r;
+            atomicAdd(dW + (int64_t(co) * 9 + t) * CIN + ci, acc[q][r]);
+        }
+    }
 }
 
 // ------------------------------------------------------------------ features and kernel application
@@ -195,8 +379,7 @@ struct mpt_kpcnn {
     // activation workspace for a given pixel count
     int64_t cap = 0;
     std::vector<bf16*> act;  // act[0] = features, act[l + 1] = output of layer l (BF16), last unused
-    bf16* xcol = nullptr;
-    float* dcol = nullptr;
+    bf16* wflip = nullptr;   // one layer's weights flipped for its input gradient
     float* logits = nullptr;
     float* dA = nullptr;   // FP32 gradient w.r.t. the current layer's output
     float* dB = nullptr;
@@ -205,6 +388,7 @@ struct mpt_kpcnn {
     float* albe = nullptr;
     double* loss = nullptr;
     hipStream_t stream = nullptr;
+    int cu_count = 256;
     ~mpt_kpcnn() {
         for (void* p : {(void*)w, (void*)wb, (void*)g, (void*)m, (void*)v}) (void)hipFree(p);
         release();
@@ -213,10 +397,10 @@ struct mpt_kpcnn {
     void release() {
         for (bf16* a : act) (void)hipFree(a);
         act.clear();
-        for (void* p : {(void*)xcol, (void*)dcol, (void*)logits, (void*)dA, (void*)dB, (void*)G, (void*)light,
-                        (void*)albe, (void*)loss})
+        for (void* p : {(void*)wflip, (void*)logits, (void*)dA, (void*)dB, (void*)G, (void*)light, (void*)albe,
+                        (void*)loss})
             (void)hipFree(p);
-        xcol = nullptr, dcol = nullptr, logits = nullptr, dA = nullptr, dB = nullptr, G = nullptr, light = nullptr;
+        wflip = nullptr, logits = nullptr, dA = nullptr, dB = nullptr, G = nullptr, light = nullptr;
         albe = nullptr, loss = nullptr;
         cap = 0;
     }
@@ -228,18 +412,18 @@ namespace {
 int ensure(mpt_kpcnn* k, int64_t npix) {
     if (k->cap >= npix) return MPT_OK;
     k->release();
-    int maxc = kFeat, maxk = 0;
+    int maxc = kFeat;
+    int64_t maxw = 0;
     for (int l = 0; l < k->layers; l++) {
         maxc = std::max(maxc, std::max(k->cin[l], k->cout[l]));
-        maxk = std::max(maxk, 9 * k->cin[l]);
+        maxw = std::max(maxw, k->wcount(l));
     }
     k->act.assign(k->layers + 1, nullptr);
     for (int l = 0; l <= k->layers; l++) {
         const int c = l == 0 ? kFeat : k->cout[l - 1];
         NN_TRY(hipMalloc(&k->act[l], size_t(npix) * c * sizeof(bf16)));
     }
-    NN_TRY(hipMalloc(&k->xcol, size_t(npix) * maxk * sizeof(bf16)));
-    NN_TRY(hipMalloc(&k->dcol, size_t(npix) * maxk * sizeof(float)));
+    NN_TRY(hipMalloc(&k->wflip, size_t(maxw) * sizeof(bf16)));
     NN_TRY(hipMalloc(&k->logits, size_t(npix) * kOut * sizeof(float)));
     NN_TRY(hipMalloc(&k->dA, size_t(npix) * maxc * sizeof(float)));
     NN_TRY(hipMalloc(&k->dB, size_t(npix) * maxc * sizeof(float)));
@@ -251,16 +435,52 @@ int ensure(mpt_kpcnn* k, int64_t npix) {
     return MPT_OK;
 }
 
+conv_shape shape_of(int B, int H, int W, int cin) {
+    return conv_shape{B, H, W, cin, (W + kTileW - 1) / kTileW, (H + kTileH - 1) / kTileH};
+}
+
+// One implicit-GEMM convolution (k_conv3x3) of cout output channels; a
+// block per CU walks the tiles (each block holds ~130 KB of LDS).
+int conv(mpt_kpcnn* k, const conv_shape& S, int cout, int epi, const bf16* X, const bf16* Wt, const float* bias,
+         void* Y) {
+    const unsigned grid = unsigned(std::max<int64_t>(1, std::min<int64_t>(S.ntiles(), k->cu_count)));
+    hipStream_t s = k->stream;
+#define CONV_CASE(NF, E)                                                                                     \
+    if (cout == 16 * NF && epi == E) {                                                                       \
+        hipLaunchKernelGGL((k_conv3x3<NF, E>), dim3(grid), dim3(256), 0, s, S, X, Wt, bias, Y);              \
+        return MPT_OK;                                                                                       \
+    }
+    CONV_CASE(1, 0) CONV_CASE(2, 0) CONV_CASE(3, 0) CONV_CASE(4, 0)
+    CONV_CASE(1, 2) CONV_CASE(2, 2) CONV_CASE(3, 2) CONV_CASE(4, 2)
+    CONV_CASE(2, 3)
+#undef CONV_CASE
+    return set_error(MPT_E_INVALID, "kpcnn: unsupported convolution width");
+}
+
+// dW of one layer: k_conv3x3_wgrad over every pixel, one tap per grid row.
+int conv_wgrad(mpt_kpcnn* k, const conv_shape& S, int cout, const bf16* G, const bf16* X, float* dW) {
+    const int64_t npix = int64_t(S.B) * S.H * S.W;
+    const int64_t chunks = (npix + 255) / 256;
+    const unsigned gx = unsigned(std::max<int64_t>(1, std::min<int64_t>(chunks, (4 * int64_t(k->cu_count) + 8) / 9)));
+    const int fpw = ((cout / 16) * (S.CIN / 16) + 3) / 4;
+    hipStream_t s = k->stream;
+    if (fpw <= 1)
+        hipLaunchKernelGGL((k_conv3x3_wgrad<1>), dim3(gx, 9), dim3(256), 0, s, S, cout, G, X, dW);
+    else if (fpw <= 2)
+        hipLaunchKernelGGL((k_conv3x3_wgrad<2>), dim3(gx, 9), dim3(256), 0, s, S, cout, G, X, dW);
+    else if (fpw <= 4)
+        hipLaunchKernelGGL((k_conv3x3_wgrad<4>), dim3(gx, 9), dim3(256), 0, s, S, cout, G, X, dW);
+    else
+        return set_error(MPT_E_INVALID, "kpcnn: unsupported convolution width");
+    return MPT_OK;
+}
+
 // Forward through every layer from act[0]; logits in k->logits.
 int forward(mpt_kpcnn* k, int B, int H, int W) {
-    const int64_t npix = int64_t(B) * H * W;
-    hipStream_t s = k->stream;
     for (int l = 0; l < k->layers; l++) {
-        const int ci = k->cin[l], co = k->cout[l];
-        hipLaunchKernelGGL(k_im2col, dim3(blocks(npix * 9)), dim3(256), 0, s, npix, H, W, ci, k->act[l], k->xcol);
         const bool last = l == k->layers - 1;
-        int rc = gemm(false, false, last ? 3 : 2, int(npix), co, 9 * ci, k->xcol, 9 * ci, k->wb + k->woff[l], 9 * ci,
-                      last ? (void*)k->logits : (void*)k->act[l + 1], co, k->w + k->boff[l], false, s);
+        int rc = conv(k, shape_of(B, H, W, k->cin[l]), k->cout[l], last ? 3 : 2, k->act[l], k->wb + k->woff[l],
+                      k->w + k->boff[l], last ? (void*)k->logits : (void*)k->act[l + 1]);
         if (rc) return rc;
     }
     return MPT_OK;
@@ -275,21 +495,21 @@ int backward(mpt_kpcnn* k, int B, int H, int W) {
     for (int l = k->layers - 1; l >= 0; l--) {
         const int ci = k->cin[l], co = k->cout[l];
         const bool last = l == k->layers - 1;
-        hipLaunchKernelGGL(k_relu_grad, dim3(blocks(npix * co)), dim3(256), 0, s, npix * co, dY, k->act[l + 1],
-                           last ? 0 : 1, k->G);
-        hipLaunchKernelGGL(k_bias_grad, dim3(co), dim3(256), 0, s, npix, co, k->G, k->g + k->boff[l]);
-        hipLaunchKernelGGL(k_im2col, dim3(blocks(npix * 9)), dim3(256), 0, s, npix, H, W, ci, k->act[l], k->xcol);
-        // dW[co][k] = sum_p G[p][co] Xcol[p][k], K = pixels split over blocks
-        NN_TRY(hipMemsetAsync(k->g + k->woff[l], 0, k->wcount(l) * sizeof(float), s));
-        int rc = gemm(true, true, 4, co, 9 * ci, int(npix), k->G, co, k->xcol, 9 * ci, k->g + k->woff[l], 9 * ci,
-                      nullptr, false, s);
+        // weights and bias of the layer are contiguous in the gradient vector: zero both
+        NN_TRY(hipMemsetAsync(k->g + k->woff[l], 0, (k->wcount(l) + co) * sizeof(float), s));
+        const int bs = (256 / co) * co;
+        const unsigned gb = unsigned(std::max<int64_t>(1, std::min<int64_t>(int64_t(k->cu_count) * 8,
+                                                                             (npix * co + bs - 1) / bs)));
+        hipLaunchKernelGGL(k_relu_grad, dim3(gb), dim3(bs), 0, s, npix * co, co, dY, k->act[l + 1], last ? 0 : 1,
+                           k->G, k->g + k->boff[l]);
+        int rc = conv_wgrad(k, shape_of(B, H, W, ci), co, k->G, k->act[l], k->g + k->woff[l]);
         if (rc) return rc;
         if (l == 0) break;
-        // dXcol[p][k] = sum_co G[p][co] W[co][k]
-        rc = gemm(false, true, 0, int(npix), 9 * ci, co, k->G, co, k->wb + k->woff[l], 9 * ci, k->dcol, 9 * ci,
-                  nullptr, false, s);
+        // dX = the convolution of G with the flipped, transposed weights
+        hipLaunchKernelGGL(k_flip_weights, dim3(blocks(k->wcount(l))), dim3(256), 0, s, co, ci, k->wb + k->woff[l],
+                           k->wflip);
+        rc = conv(k, shape_of(B, H, W, co), ci, 0, k->G, k->wflip, nullptr, dX);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_col2im, dim3(blocks(npix * ci)), dim3(256), 0, s, npix, H, W, ci, k->dcol, dX);
         std::swap(dY, dX);
     }
     return MPT_OK;
@@ -324,8 +544,8 @@ int32_t mpt_gemm_bf16(int32_t a_t, int32_t b_t, int32_t M, int32_t N, int32_t K,
 
 int32_t mpt_kpcnn_create(int32_t layers, int32_t hidden, uint64_t seed, mpt_kpcnn** out) {
     if (!out) return set_error(MPT_E_INVALID, "kpcnn: null output");
-    if (layers < 2 || layers > 16 || hidden < 16 || hidden > 256 || hidden % 16)
-        return set_error(MPT_E_INVALID, "kpcnn: layers in [2, 16], hidden a multiple of 16 in [16, 256]");
+    if (layers < 2 || layers > 16 || hidden < 16 || hidden > kMaxCin || hidden % 16)
+        return set_error(MPT_E_INVALID, "kpcnn: layers in [2, 16], hidden a multiple of 16 in [16, 64]");
     auto k = std::make_unique<mpt_kpcnn>();
     k->layers = layers;
     k->hidden = hidden;
@@ -365,6 +585,9 @@ int32_t mpt_kpcnn_create(int32_t layers, int32_t hidden, uint64_t seed, mpt_kpcn
                 }
     }
     NN_TRY(hipStreamCreate(&k->stream));
+    int dev = 0;
+    NN_TRY(hipGetDevice(&dev));
+    NN_TRY(hipDeviceGetAttribute(&k->cu_count, hipDeviceAttributeMultiprocessorCount, dev));
     NN_TRY(hipMalloc(&k->w, off * sizeof(float)));
     NN_TRY(hipMalloc(&k->wb, off * sizeof(bf16)));
     NN_TRY(hipMalloc(&k->g, off * sizeof(float)));

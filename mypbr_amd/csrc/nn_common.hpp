@@ -42,41 +42,53 @@ __host__ __device__ __forceinline__ float from_bf16(bf16 h) {
 }
 
 // ------------------------------------------------------------------ GEMM
+// The matrix-core instruction: v_mfma_f32_16x16x32_bf16 (gfx950), a 16 x 16
+// FP32 tile += A (16 x 32, BF16) x B (32 x 16, BF16). Lane l supplies row
+// (l % 16) of A and column (l % 16) of B, eight BF16 each, and receives
+// column (l % 16), rows 4 * (l / 16) .. + 3 of the result. The eight values a
+// lane supplies are the k of its slot group l / 16; the instruction pairs A's
+// and B's slots position by position, so any assignment of k to the slots
+// that A and B share gives the same sum. Here slot group g holds the eight
+// consecutive k = 8 g .. 8 g + 7 of the 32, one 16-byte load.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ float4_t mfma32(const bf16x8_t& a, const bf16x8_t& b, const float4_t& c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8_t as_bf16x8(const uint4& u) {
+    bf16x8_t v;
+    __builtin_memcpy(&v, &u, 16);
+    return v;
+}
+
 // C[m][n] = (accumulate ? C : 0) + sum_k A(m, k) B(n, k), FP32 accumulation.
 // A(m, k) is A[m * lda + k] (AT = false) or A[k * lda + m] (AT = true); the
 // same for B. A block of 256 threads computes a 64 x 64 tile of C, each wave
-// 32 x 32 as 2 x 2 MFMA 16x16x16 tiles. For 16x16x16 BF16 the lane l holds
-// row (l % 16) of A and column (l % 16) of B, each with the 4 consecutive k
-// 4 * (l / 16) .. + 3, and receives column (l % 16), rows 4 * (l / 16) .. + 3
-// of the 16 x 16 result.
+// 32 x 32 as 2 x 2 MFMA tiles, 32 k per step.
 // Epilogue (EPI): 0 store FP32; 1 add bias[n], store BF16; 2 add bias[n],
 // ReLU, store BF16; 3 add bias[n], store FP32; 4 atomically add FP32 (the
 // K range is split over gridDim.z blocks of kchunk each).
-// Fragment loads: 4 consecutive k of one row are one 8-byte load when the
+// Fragment loads: 8 consecutive k of one row are one 16-byte load when the
 // operand is k-contiguous and K and its leading dimension are multiples of
-// 16 and 4; otherwise 4 element loads with bounds checks.
+// 32 and 8; otherwise 8 element loads with bounds checks.
 template <bool T>
-__device__ __forceinline__ short4_t frag(const bf16* __restrict__ X, int64_t ld, int64_t row, int64_t nrows, int k,
+__device__ __forceinline__ bf16x8_t frag(const bf16* __restrict__ X, int64_t ld, int64_t row, int64_t nrows, int k,
                                          int K, bool fast) {
-    short4_t f;
     if (!T && fast) {
-        if (row < nrows) {
-            const uint2 u = *reinterpret_cast<const uint2*>(X + row * ld + k);
-            f[0] = short(u.x & 0xffffu);
-            f[1] = short(u.x >> 16);
-            f[2] = short(u.y & 0xffffu);
-            f[3] = short(u.y >> 16);
-        } else {
-            f = short4_t{0, 0, 0, 0};
-        }
-        return f;
+        uint4 u{0, 0, 0, 0};
+        if (row < nrows) u = *reinterpret_cast<const uint4*>(X + row * ld + k);
+        return as_bf16x8(u);
     }
+    uint16_t v[8];
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-        bf16 v = 0;
-        if (row < nrows && k + t < K) v = T ? X[int64_t(k + t) * ld + row] : X[row * ld + k + t];
-        f[t] = short(v);
+    for (int t = 0; t < 8; t++) {
+        bf16 x = 0;
+        if (row < nrows && k + t < K) x = T ? X[int64_t(k + t) * ld + row] : X[row * ld + k + t];
+        v[t] = x;
     }
+    bf16x8_t f;
+    __builtin_memcpy(&f, v, 16);
     return f;
 }
 
@@ -90,17 +102,17 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16* _
     const int64_t m0 = int64_t(blockIdx.x) * 64 + (wv & 1) * 32;
     const int64_t n0 = int64_t(blockIdx.y) * 64 + (wv >> 1) * 32;
     const int r = lane & 15;
-    const int kq = (lane >> 4) * 4;
+    const int kq = (lane >> 4) * 8;
     const int kb = int(blockIdx.z) * kchunk;
     const int ke = K - kb < kchunk ? K : kb + kchunk;
-    const bool fa = (K % 16) == 0 && (lda % 4) == 0, fb = (K % 16) == 0 && (ldb % 4) == 0;
+    const bool fa = (K % 32) == 0 && (lda % 8) == 0, fb = (K % 32) == 0 && (ldb % 8) == 0;
     float4_t acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = kb; k0 < ke; k0 += 16) {
-        short4_t a[2], b[2];
+    for (int k0 = kb; k0 < ke; k0 += 32) {
+        bf16x8_t a[2], b[2];
 #pragma unroll
         for (int i = 0; i < 2; i++) a[i] = frag<AT>(A, lda, m0 + 16 * i + r, M, k0 + kq, ke, fa);
 #pragma unroll
@@ -108,7 +120,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16* _
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
-            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+            for (int j = 0; j < 2; j++) acc[i][j] = mfma32(a[i], b[j], acc[i][j]);
     }
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -137,29 +149,34 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16* _
         }
 }
 
-// G = bf16(dY * (Y > 0)) for a ReLU layer (relu = 1), or bf16(dY).
-__global__ void k_relu_grad(int64_t n, const float* __restrict__ dY, const bf16* __restrict__ Y, int relu,
-                            bf16* __restrict__ G) {
-    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    float g = dY[e];
-    if (relu && !(from_bf16(Y[e]) > 0.f)) g = 0.f;
-    G[e] = to_bf16(g);
-}
-
-// db[c] = sum_p G[p][c] (FP32 over the BF16 gradient), one block per channel.
-__global__ void k_bias_grad(int64_t npix, int C, const bf16* __restrict__ G, float* __restrict__ db) {
+// G = bf16(dY * (Y > 0)) for a ReLU layer (relu = 1), or bf16(dY), over
+// n = npix * C elements [pixel][channel]; with db, also the bias gradient
+// db[c] += sum_p G[p][c] (FP32 over the BF16 values; db zeroed by the
+// caller). The block holds (256 / C) * C threads and the grid stride is a
+// multiple of C, so each thread keeps one channel: its partial sum stays in
+// a register, the block adds them per channel in LDS, and one atomic per
+// channel and block goes to db.
+__global__ __launch_bounds__(256) void k_relu_grad(int64_t n, int C, const float* __restrict__ dY,
+                                                   const bf16* __restrict__ Y, int relu, bf16* __restrict__ G,
+                                                   float* __restrict__ db) {
     __shared__ float part[256];
-    const int c = blockIdx.x;
     float s = 0.f;
-    for (int64_t p = threadIdx.x; p < npix; p += blockDim.x) s += from_bf16(G[p * C + c]);
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += stride) {
+        float g = dY[e];
+        if (relu && !(from_bf16(Y[e]) > 0.f)) g = 0.f;
+        const bf16 b = to_bf16(g);
+        G[e] = b;
+        s += from_bf16(b);
+    }
+    if (!db) return;
     part[threadIdx.x] = s;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (int(threadIdx.x) < o) part[threadIdx.x] += part[threadIdx.x + o];
-        __syncthreads();
+    if (int(threadIdx.x) < C) {
+        float t = 0.f;
+        for (int k = int(threadIdx.x); k < int(blockDim.x); k += C) t += part[k];
+        atomicAdd(db + threadIdx.x, t);
     }
-    if (threadIdx.x == 0) db[c] = part[0];
 }
 
 __global__ void k_to_bf16(int64_t n, const float* __restrict__ x, bf16* __restrict__ y) {
@@ -208,7 +225,7 @@ int gemm(bool at, bool bt, int epi, int M, int N, int K, const bf16* A, int64_t 
     int kchunk = K, splits = 1;
     if (epi == 4) {
         const int64_t want = std::max<int64_t>(1, 4096 / (int64_t(gm) * gn));
-        kchunk = int(std::max<int64_t>(256, ((int64_t(K) + want - 1) / want + 15) / 16 * 16));
+        kchunk = int(std::max<int64_t>(256, ((int64_t(K) + want - 1) / want + 31) / 32 * 32));
         splits = int((int64_t(K) + kchunk - 1) / kchunk);
     }
     dim3 g(gm, gn, unsigned(splits));

@@ -257,9 +257,11 @@ int backward(mpt_nrc* r, int64_t n, const float* pos, hipStream_t s) {
     for (int l = kLayers - 1; l >= 0; l--) {
         const int ci = r->cin[l], co = r->cout[l];
         const bool last = l == kLayers - 1;
-        hipLaunchKernelGGL(nn::k_relu_grad, dim3(blocks(n * co)), dim3(256), 0, s, n * co, dY, r->act[l + 1],
-                           last ? 0 : 1, r->G);
-        hipLaunchKernelGGL(nn::k_bias_grad, dim3(co), dim3(256), 0, s, n, co, r->G, r->g + r->boff[l]);
+        NN_TRY(hipMemsetAsync(r->g + r->boff[l], 0, size_t(co) * sizeof(float), s));
+        const int bs = (256 / co) * co;
+        const unsigned gb = unsigned(std::max<int64_t>(1, std::min<int64_t>(2048, (n * co + bs - 1) / bs)));
+        hipLaunchKernelGGL(nn::k_relu_grad, dim3(gb), dim3(bs), 0, s, n * co, co, dY, r->act[l + 1], last ? 0 : 1,
+                           r->G, r->g + r->boff[l]);
         NN_TRY(hipMemsetAsync(r->g + r->woff[l], 0, size_t(co) * ci * sizeof(float), s));
         int rc = nn::gemm(true, true, 4, co, ci, int(n), r->G, co, r->act[l], ci, r->g + r->woff[l], ci, nullptr,
                           false, s);

@@ -73,6 +73,16 @@ class KPCNN:
                                  _vp(out), C.byref(loss)))
         return loss.value
 
+    def denoise_dev(self, dev):
+        """Denoised radiance (float32 DeviceArray (h, w, 3)) of one frame that
+        is already in HBM (Scene.frame_dev): no host round trip."""
+        h, w = dev["radiance"].shape[:2]
+        out = DeviceArray((h, w, 3), np.float32)
+        loss = C.c_double(0)
+        _check(lib.mpt_kpcnn_run(self._h, *[_vp(dev[k]) for k in FRAME_KEYS], 1, h, w, None, 0.0, _vp(out),
+                                 C.byref(loss)))
+        return out
+
     def denoise(self, frames):
         """Denoised radiance (numpy, (B, h, w, 3) float32) of host frames."""
         dev = to_device(frames)

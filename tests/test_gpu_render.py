@@ -204,3 +204,22 @@ def test_non_finite_samples_are_dropped_like_the_oracle(mpt):
     assert cpu["radiance"].max() > 0
     # the dropped paths darken a good share of the pixels against the finite light
     assert np.mean(np.any(cpu["radiance"] < finite, axis=-1)) > 0.1
+
+
+def test_device_frame_and_depth_equal_the_host_ones(mpt):
+    """mpt_read_frame_dev averages the running sums on the GPU as
+    mpt_read_frame does on the host (camera.h:223-246); mpt_primary_depth_dev
+    traces the pixel-centre rays on the GPU that mpt_primary_depth builds on
+    the host. Both bit for bit."""
+    from mypbr_amd.abi import camera_copy
+    for name in ("weekend_cover.scene", "textured.scene"):
+        d = mpt.describe(scene_path(name))
+        cam = camera_copy(d.camera, image_width=72, samples_per_pixel=6, max_depth=12, collect_features=1)
+        s = mpt.Scene.from_desc(d.desc, d.camera)
+        f = s.render(cam, seed=21)
+        dev = s.frame_dev(cam)
+        for k in ("radiance", "albedo", "normal", "variance"):
+            assert np.array_equal(dev[k].numpy(), getattr(f, k)), (name, k)
+        depth = s.primary_depth(cam)
+        assert np.array_equal(dev["depth"].numpy(), depth)
+        assert depth.max() > 0
