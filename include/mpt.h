@@ -69,7 +69,10 @@ enum { MPT_TEX_SOLID = 0, MPT_TEX_CHECKER = 1, MPT_TEX_IMAGE = 2 };
 /* Image texture flags: 8-bit values are data (k / 255), not sRGB; (u, v)
  * wraps around instead of clamping at the edge; the texels are linear
  * floats (a Radiance .hdr file) instead of bytes. */
-enum { MPT_TEXF_LINEAR = 1, MPT_TEXF_REPEAT = 2, MPT_TEXF_HDR = 4 };
+/* MPT_TEXF_NO_ALPHA: the file had no alpha channel, so as an alpha mask the
+ * texture's coverage is its first byte (the grey level), as
+ * image_texture::alpha_value reads it (texture.h:130-136, 166-173). */
+enum { MPT_TEXF_LINEAR = 1, MPT_TEXF_REPEAT = 2, MPT_TEXF_HDR = 4, MPT_TEXF_NO_ALPHA = 8 };
 
 typedef struct mpt_texture {
     int32_t type;        /* MPT_TEX_* */
@@ -111,6 +114,12 @@ typedef struct mpt_material {
                                   its head-on value and hemispherical average (microfacet.h:109-121) */
     double eta[3];             /* measured conductor: real and imaginary index of refraction, RGB */
     double k[3];
+    int32_t coat;              /* principled: 1 = a clear coat on top (set_clearcoat, material.h:326-345, 363-379):
+                                  a colorless GGX layer of ior 1.5 that passes the rest to the base, twice */
+    int32_t coat_weight_texture;     /* its weight: red channel, clamped to [0, 1] (a solid texture for a constant) */
+    int32_t coat_roughness_texture;  /* its roughness: red channel, clamped to [0, 1] */
+    int32_t coat_normal_map;         /* its own tangent-space normal map, -1: the unperturbed normal */
+    double coat_tint[3];       /* what the coat lets through head-on, in and out (set_clearcoat_color); 1: clear */
 } mpt_material;
 
 /* An affine transform with its stored inverse, as transform.h:120-123:
@@ -224,6 +233,13 @@ typedef struct mpt_scene_desc {
     int32_t fog_pad;
     const double* fog_params;      /* [n_fogs * 3] */
     const int32_t* fog_material;   /* [n_fogs] */
+
+    /* Alpha masks (mesh.h:24-32, 82-86): per object, an image texture whose
+     * coverage (alpha byte / 255, nearest texel at the hit's (u, v)) cuts
+     * holes in a mesh's triangles: a hit with coverage c < 1 is kept when a
+     * uniform draw falls below c. -1: opaque. NULL: no object has one. Only
+     * mesh objects may have one. */
+    const int32_t* object_alpha;   /* [n_objects] */
 } mpt_scene_desc;
 
 /* Camera and render settings: the public fields of the reference camera
@@ -273,6 +289,12 @@ typedef struct mpt_render_opts {
     int32_t profile;               /* bit 0: time each stage with HIP events; bit 1: count BVH work */
     int32_t tail_paths;            /* once at most this many paths of a batch are left, the rest of the
                                     * batch runs in one launch, a path per thread (0 = auto, -1 = never) */
+    int32_t guide_passes;          /* with collect_features: trace the albedo and normal guides
+                                    * (camera::first_surface) only for passes with index below this, and
+                                    * average them over those passes; the variance guide still takes
+                                    * every pass. 0 = every pass, the reference's frame (camera.h:230-244).
+                                    * For denoising inputs, whose guides converge long before the
+                                    * radiance does. The radiance is unchanged bit for bit. */
 } mpt_render_opts;
 
 /* Per-render statistics. */

@@ -152,6 +152,10 @@ public:
     virtual std::shared_ptr<texture> normal_map_texture() const { return nullptr; }
     virtual std::shared_ptr<texture> metallic_texture() const { return nullptr; }
     virtual std::shared_ptr<texture> roughness_texture() const { return nullptr; }
+    // principled's clear coat: weight, roughness and its own normal map (null: none)
+    virtual std::shared_ptr<texture> coat_weight_texture() const { return nullptr; }
+    virtual std::shared_ptr<texture> coat_roughness_texture() const { return nullptr; }
+    virtual std::shared_ptr<texture> coat_normal_texture() const { return nullptr; }
     virtual color average_emission() const { return color(0, 0, 0); }
 };
 // A material whose shading normal is bent by a tangent-space normal map
@@ -226,6 +230,18 @@ public:
     // the metallic part as a measured metal (set_conductor, material.h:381-385)
     bool measured = false;
     conductor_ior conductor{};
+    // a clear coat of weight w and roughness (red channels, clamped to [0, 1])
+    // and an optional normal map of its own (set_clearcoat, material.h:363-371);
+    // a null weight removes it
+    void set_clearcoat(std::shared_ptr<texture> weight, std::shared_ptr<texture> roughness,
+                       std::shared_ptr<texture> coat_normals = nullptr);
+    // what the coat lets through head-on, in and back out (set_clearcoat_color, material.h:373-379)
+    void set_clearcoat_color(const color& tint) { coat_tint = tint; }
+    std::shared_ptr<texture> coat_weight_texture() const override { return coat_map; }
+    std::shared_ptr<texture> coat_roughness_texture() const override { return coat_map ? coat_roughness_map : nullptr; }
+    std::shared_ptr<texture> coat_normal_texture() const override { return coat_map ? coat_normal_map : nullptr; }
+    std::shared_ptr<texture> coat_map, coat_roughness_map, coat_normal_map;
+    color coat_tint{1, 1, 1};
 };
 class diffuse_light : public material {
 public:
@@ -342,6 +358,9 @@ public:
     // uv_indices parallels indices; a triangle with any -1 has none
     std::vector<vec3> uvs;
     std::vector<int32_t> uv_indices;
+    // optional alpha mask (mesh.h:24-32): its coverage at a hit's (u, v) cuts
+    // holes in the triangles; null or fully opaque: none
+    std::shared_ptr<image_texture> alpha_mask;
 };
 
 // A part of an OBJ file that uses one material (obj_loader.h:19-23).

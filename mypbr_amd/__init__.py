@@ -399,7 +399,7 @@ class Scene:
 
     def render(self, camera=None, seed=1, features=None, passes_in_flight=0, save_every=0, on_pass=None,
                first_pass=0, scene_index=0, pass_count=0, accumulate=False, profile=0, read=True, tail_paths=0,
-               blocks_per_cu=0, sort_by_material=0, use_graph=0):
+               blocks_per_cu=0, sort_by_material=0, use_graph=0, guide_passes=0):
         cam = abi.camera_copy(camera if camera is not None else self.camera)
         if features is not None:
             cam.collect_features = int(bool(features))
@@ -412,7 +412,8 @@ class Scene:
         opts = RenderOpts(seed=seed, first_pass=first_pass, save_every=save_every,
                           passes_in_flight=passes_in_flight, scene_index=scene_index, pass_count=pass_count,
                           accumulate=int(bool(accumulate)), profile=profile, tail_paths=tail_paths,
-                          blocks_per_cu=blocks_per_cu, sort_by_material=sort_by_material, use_graph=use_graph)
+                          blocks_per_cu=blocks_per_cu, sort_by_material=sort_by_material, use_graph=use_graph,
+                          guide_passes=guide_passes)
         stats = RenderStats()
         cb = abi.PASS_CALLBACK(lambda done, user: on_pass(done, self)) if on_pass else abi.PASS_CALLBACK()
         if not read:  # leave the frame in HBM (read it later with read_frame)

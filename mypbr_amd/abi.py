@@ -16,7 +16,7 @@ PRIM_SPHERE, PRIM_QUAD, PRIM_TRIANGLE = 0, 1, 2
 OBJ_PRIM, OBJ_LIST, OBJ_MESH, OBJ_MEDIUM = 0, 1, 2, 3
 MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_PRINCIPLED, MAT_DIFFUSE_LIGHT, MAT_PHASE = range(6)
 TEX_SOLID, TEX_CHECKER, TEX_IMAGE = 0, 1, 2
-TEXF_LINEAR, TEXF_REPEAT, TEXF_HDR = 1, 2, 4
+TEXF_LINEAR, TEXF_REPEAT, TEXF_HDR, TEXF_NO_ALPHA = 1, 2, 4, 8
 
 D3 = C.c_double * 3
 
@@ -33,7 +33,9 @@ class Material(C.Structure):
                 ("tan_half", C.c_double), ("albedo", D3), ("f_avg", D3), ("emit", D3),
                 ("sigma_a", D3), ("avg_emission", D3), ("normal_map", C.c_int32),
                 ("metallic_texture", C.c_int32), ("roughness_texture", C.c_int32), ("uses_uv", C.c_int32),
-                ("thin", C.c_int32), ("measured", C.c_int32), ("eta", D3), ("k", D3)]
+                ("thin", C.c_int32), ("measured", C.c_int32), ("eta", D3), ("k", D3),
+                ("coat", C.c_int32), ("coat_weight_texture", C.c_int32), ("coat_roughness_texture", C.c_int32),
+                ("coat_normal_map", C.c_int32), ("coat_tint", D3)]
 
 
 class Xform(C.Structure):
@@ -91,6 +93,7 @@ class SceneDesc(C.Structure):
         ("fog_pad", C.c_int32),
         ("fog_params", C.POINTER(C.c_double)),
         ("fog_material", C.POINTER(C.c_int32)),
+        ("object_alpha", C.POINTER(C.c_int32)),
     ]
 
 
@@ -126,7 +129,7 @@ class RenderOpts(C.Structure):
                 ("passes_in_flight", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("sort_by_material", C.c_int32), ("use_graph", C.c_int32),
                 ("scene_index", C.c_int32), ("pass_count", C.c_int32), ("accumulate", C.c_int32),
-                ("profile", C.c_int32), ("tail_paths", C.c_int32)]
+                ("profile", C.c_int32), ("tail_paths", C.c_int32), ("guide_passes", C.c_int32)]
 
 
 class RenderStats(C.Structure):

@@ -106,7 +106,7 @@ __device__ __forceinline__ double avg3(const v3& c) { return (c.x + c.y + c.z) /
 __device__ __forceinline__ v3 m3(const double* a) { return v3{a[0], a[1], a[2]}; }
 
 // ---------------------------------------------------------------- densities
-enum { LOBE_COSINE = 0, LOBE_GGX = 1, LOBE_MIX = 2, LOBE_ROUGH_GLASS = 3, LOBE_HG = 4 };
+enum { LOBE_COSINE = 0, LOBE_GGX = 1, LOBE_MIX = 2, LOBE_ROUGH_GLASS = 3, LOBE_HG = 4, LOBE_COAT = 5 };
 
 struct sdensity {
     int kind;
@@ -115,6 +115,11 @@ struct sdensity {
     double alpha;
     double eta;    // rough glass
     double w0;     // mixture weight of the GGX lobe; HG g
+    // LOBE_COAT (principled with a clear coat, material.h:409-419): the coat's
+    // GGX lobe (normal n2, alpha2) with weight w1 against the base's mixture
+    v3 n2;
+    double alpha2;
+    double w1;
 };
 
 __device__ __forceinline__ double cosine_value(const v3& n, const v3& dir) {  // pdf.h:41-44
@@ -240,8 +245,11 @@ __device__ __forceinline__ double density_value(const sdensity& s, const v3& dir
         if (s.kind == LOBE_GGX) return ggx_value(s.n, s.wo_world, s.alpha, dir);
         return s.w0 * ggx_value(s.n, s.wo_world, s.alpha, dir) + (1 - s.w0) * cosine_value(s.n, dir);
     }
-    if constexpr (MT == MPT_MAT_PRINCIPLED)
-        return s.w0 * ggx_value(s.n, s.wo_world, s.alpha, dir) + (1 - s.w0) * cosine_value(s.n, dir);
+    if constexpr (MT == MPT_MAT_PRINCIPLED) {
+        const double base = s.w0 * ggx_value(s.n, s.wo_world, s.alpha, dir) + (1 - s.w0) * cosine_value(s.n, dir);
+        if (s.kind != LOBE_COAT) return base;
+        return s.w1 * ggx_value(s.n2, s.wo_world, s.alpha2, dir) + (1 - s.w1) * base;  // pdf.h:120-122
+    }
     if constexpr (MT == MPT_MAT_DIELECTRIC) return rglass_pdf(make_rglass(s.n, s.wo_world, s.eta, s.alpha), dir);
     if constexpr (MT == MPT_MAT_PHASE) return hg_value(s, dir);
     if constexpr (MT < 0) {
@@ -251,6 +259,9 @@ __device__ __forceinline__ double density_value(const sdensity& s, const v3& dir
             case LOBE_MIX:
                 return s.w0 * ggx_value(s.n, s.wo_world, s.alpha, dir) + (1 - s.w0) * cosine_value(s.n, dir);
             case LOBE_ROUGH_GLASS: return rglass_pdf(make_rglass(s.n, s.wo_world, s.eta, s.alpha), dir);
+            case LOBE_COAT:
+                return s.w1 * ggx_value(s.n2, s.wo_world, s.alpha2, dir) +
+                       (1 - s.w1) * (s.w0 * ggx_value(s.n, s.wo_world, s.alpha, dir) + (1 - s.w0) * cosine_value(s.n, dir));
             default: return hg_value(s, dir);
         }
     }
@@ -284,6 +295,7 @@ __device__ __forceinline__ v3 density_generate(const sdensity& s, rstream& rs) {
         return cosine_generate(s.n, rs);
     }
     if constexpr (MT == MPT_MAT_PRINCIPLED) {
+        if (s.kind == LOBE_COAT && rs.next() < s.w1) return ggx_generate(s.n2, s.wo_world, s.alpha2, rs);  // pdf.h:124-129
         if (rs.next() < s.w0) return ggx_generate(s.n, s.wo_world, s.alpha, rs);
         return cosine_generate(s.n, rs);
     }
@@ -297,6 +309,10 @@ __device__ __forceinline__ v3 density_generate(const sdensity& s, rstream& rs) {
                 if (rs.next() < s.w0) return ggx_generate(s.n, s.wo_world, s.alpha, rs);
                 return cosine_generate(s.n, rs);
             case LOBE_ROUGH_GLASS: return rglass_sample(make_rglass(s.n, s.wo_world, s.eta, s.alpha), rs);
+            case LOBE_COAT:
+                if (rs.next() < s.w1) return ggx_generate(s.n2, s.wo_world, s.alpha2, rs);
+                if (rs.next() < s.w0) return ggx_generate(s.n, s.wo_world, s.alpha, rs);
+                return cosine_generate(s.n, rs);
             default: return hg_generate(s, rs);
         }
     }
@@ -397,12 +413,37 @@ __device__ __forceinline__ v3 principled_ms_fresnel(const dscene& S, const mpt_m
 }
 
 // material::scatter (material.h:79-85, 123-150, 210-242, 387-421; medium.h:21-27).
+// principled's clear coat (material.h:460-527): weight and GGX alpha at the
+// hit from the red channels of their textures, clamped to [0, 1]; and what a
+// coat of weight w lets through to the base and back for cosines cos_o and
+// cos_i to its normal (coat_transmittance): Beer-Lambert through the tinted
+// layer along the refracted directions into ior 1.5, blended by w.
+constexpr double kCoatF0 = 0.04;
+__device__ __forceinline__ double coat_weight(const dscene& S, const mpt_material& m, const hitrec& rec) {
+    return std::fmin(std::fmax(tex_value(S, m.coat_weight_texture, rec).x, 0.0), 1.0);
+}
+__device__ __forceinline__ double coat_alpha(const dscene& S, const mpt_material& m, const hitrec& rec) {
+    return roughness_to_alpha(std::fmin(std::fmax(tex_value(S, m.coat_roughness_texture, rec).x, 0.0), 1.0));
+}
+__device__ __forceinline__ v3 coat_transmittance(const mpt_material& m, double w, double cos_o, double cos_i) {
+    if (m.coat_tint[0] == 1 && m.coat_tint[1] == 1 && m.coat_tint[2] == 1) return v3{1, 1, 1};
+    auto cos_t = [](double c) {
+        c = std::fmin(std::fmax(c, 0.0), 1.0);
+        return std::sqrt(1 - (1 - c * c) / (1.5 * 1.5));
+    };
+    const double e = (1 / cos_t(cos_o) + 1 / cos_t(cos_i)) / 2;
+    const v3 t{std::pow(m.coat_tint[0], e), std::pow(m.coat_tint[1], e), std::pow(m.coat_tint[2], e)};
+    return (1 - w) * v3{1, 1, 1} + w * t;
+}
+
 // MT: the material type when known at compile time (-1: dispatch on m.type).
 // RICH: the scene has image textures, texture or normal maps (hits carry
 // (u, v)), thin glass or measured metals; without it their code is left out.
+// gn: the hit's own normal when rec carries a normal map's (null: rec.normal),
+// which a clear coat without a map of its own keeps.
 template <int MT = -1, bool RICH = false>
 __device__ __forceinline__ scatter_out mat_scatter(const dscene& S, const mpt_material& m, const v3& rd,
-                                                   const hitrec& rec, rstream& rs) {
+                                                   const hitrec& rec, rstream& rs, const v3* gn = nullptr) {
     scatter_out o;
     o.scattered = true;
     o.skip = false;
@@ -490,6 +531,23 @@ __device__ __forceinline__ scatter_out mat_scatter(const dscene& S, const mpt_ma
             o.dens.wo_world = wo;
             o.dens.alpha = alpha;
             o.attenuation = base;
+            if constexpr (RICH) {
+                if (m.coat) {  // material.h:409-419
+                    const double w = coat_weight(S, m, rec);
+                    if (w > 0) {
+                        hitrec geo = rec;
+                        if (gn) geo.normal = *gn;
+                        const v3 n_coat = m.coat_normal_map >= 0 ? normal_from_map(S, m.coat_normal_map, rd, geo)
+                                                                 : geo.normal;
+                        const double cw = w * fresnel_schlick(dot(n_coat, wo), kCoatF0);
+                        const double below = (1 - cw) * (spec_w + diff_w) * avg3(coat_transmittance(m, w, 1, 1));
+                        o.dens.kind = LOBE_COAT;
+                        o.dens.n2 = n_coat;
+                        o.dens.alpha2 = coat_alpha(S, m, rec);
+                        o.dens.w1 = clampd(cw / (cw + below), 0.1, 0.9);
+                    }
+                }
+            }
             break;
         }
         case MPT_MAT_PHASE:
@@ -534,25 +592,54 @@ __device__ __forceinline__ v3 mat_eval(const dscene& S, const mpt_material& m, c
             double f = rglass_eval(make_rglass(rec.normal, -rd, eta, m.alpha), dir);
             return v3{f, f, f};
         }
-        case MPT_MAT_PRINCIPLED: {  // material.h:423-431, 486-506
+        case MPT_MAT_PRINCIPLED: {  // material.h:423-456, 486-506
             if (dot(dir, gn) <= 0) return v3{0, 0, 0};
             v3 wo_w = unit_vector(-rd);
             v3 wi_w = unit_vector(dir);
             onb f = make_onb(rec.normal);
             v3 wo = unit_vector(onb_to_local(f, wo_w));
             v3 wi = unit_vector(onb_to_local(f, wi_w));
-            if (wo.z <= 0 || wi.z <= 0) return v3{0, 0, 0};
-            const pparams pp = principled_at<RICH>(S, m, rec);
-            double metallic = pp.metallic, alpha = pp.alpha;
-            v3 base = tex_value<RICH>(S, m.texture, rec);
-            v3 h = unit_vector(wo + wi);
-            v3 specular = principled_fresnel<RICH>(m, dot(wi, h), base, metallic) *
-                              (ggx_D(alpha, h) * ggx_G(alpha, wo, wi) / (4 * wo.z)) +
-                          principled_ms_fresnel<RICH>(S, m, base, metallic, alpha) *
-                              ggx_ms_eval(S.ggx_e, S.ggx_eavg, wo.z, wi.z, alpha);
-            double transmitted = (1 - fresnel_schlick(wi.z, m.f0)) * (1 - fresnel_schlick(wo.z, m.f0));
-            v3 diffuse = ((1 - metallic) * transmitted * wi.z / kPi) * base;
-            return specular + diffuse;
+            v3 below{0, 0, 0};
+            if (wo.z > 0 && wi.z > 0) {
+                const pparams pp = principled_at<RICH>(S, m, rec);
+                double metallic = pp.metallic, alpha = pp.alpha;
+                v3 base = tex_value<RICH>(S, m.texture, rec);
+                v3 h = unit_vector(wo + wi);
+                v3 specular = principled_fresnel<RICH>(m, dot(wi, h), base, metallic) *
+                                  (ggx_D(alpha, h) * ggx_G(alpha, wo, wi) / (4 * wo.z)) +
+                              principled_ms_fresnel<RICH>(S, m, base, metallic, alpha) *
+                                  ggx_ms_eval(S.ggx_e, S.ggx_eavg, wo.z, wi.z, alpha);
+                double transmitted = (1 - fresnel_schlick(wi.z, m.f0)) * (1 - fresnel_schlick(wo.z, m.f0));
+                v3 diffuse = ((1 - metallic) * transmitted * wi.z / kPi) * base;
+                below = specular + diffuse;
+            }
+            if constexpr (RICH) {
+                if (m.coat) {
+                    const double w = coat_weight(S, m, rec);
+                    if (!(w <= 0)) {  // the coat: a colorless GGX reflection, and what it lets through twice
+                        hitrec geo = rec;
+                        geo.normal = gn;
+                        const v3 n_coat = m.coat_normal_map >= 0 ? normal_from_map(S, m.coat_normal_map, rd, geo) : gn;
+                        const double cos_o = dot(n_coat, wo_w), cos_i = dot(n_coat, wi_w);
+                        const double through = (1 - w * fresnel_schlick(cos_o, kCoatF0)) *
+                                               (1 - w * fresnel_schlick(cos_i, kCoatF0));
+                        v3 result = (through * coat_transmittance(m, w, cos_o, cos_i)) * below;
+                        if (cos_o > 0 && cos_i > 0) {
+                            const onb cf = make_onb(n_coat);
+                            const v3 lo = unit_vector(onb_to_local(cf, wo_w)), li = unit_vector(onb_to_local(cf, wi_w));
+                            const v3 h = unit_vector(lo + li);
+                            const double ca = coat_alpha(S, m, rec);
+                            double coat = fresnel_schlick(dot(li, h), kCoatF0) * ggx_D(ca, h) * ggx_G(ca, lo, li) /
+                                          (4 * lo.z);
+                            coat += ggx_ms_fresnel(S.ggx_eavg, v3{kCoatF0, kCoatF0, kCoatF0}, ca).x *
+                                    ggx_ms_eval(S.ggx_e, S.ggx_eavg, lo.z, li.z, ca);
+                            result = result + v3{w * coat, w * coat, w * coat};
+                        }
+                        return result;
+                    }
+                }
+            }
+            return below;
         }
         case MPT_MAT_PHASE: {  // medium.h:33-36
             double c = dot(unit_vector(rd), unit_vector(dir));
@@ -586,7 +673,7 @@ __device__ __forceinline__ v3 mat_interior(const mpt_material& m, const v3& rd, 
     return v3{std::exp(-m.sigma_a[0] * dd), std::exp(-m.sigma_a[1] * dd), std::exp(-m.sigma_a[2] * dd)};
 }
 
-__device__ __forceinline__ bool mat_lets_shadows(const mpt_material& m) {
+__host__ __device__ inline bool mat_lets_shadows(const mpt_material& m) {
     return m.type == MPT_MAT_DIELECTRIC && m.roughness == 0 && !m.thin;  // thin_dielectric keeps the default
 }
 

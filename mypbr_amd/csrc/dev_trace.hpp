@@ -71,6 +71,11 @@ struct dscene {
     int32_t env_w, env_h;
     int32_t ground_mat;  // the ground plane's material, -1 without one (hits on it name object nobj)
     double ground_h;
+    int32_t nfog;        // height fogs (hits on fog k name object nobj + 1 + k)
+    const fog_rec* fog;
+    // 1 when no material emits or lets a shadow ray through: a shadow ray then
+    // only asks whether anything is in the way (an occlusion query)
+    int32_t shadow_occlusion;
     int64_t ground_id;   // its canonical ID: one past the last primitive's
     croot tlas_root;
     // the sun (sun.h): on, its frame (w towards the sun), cone and radiance
@@ -318,8 +323,12 @@ __device__ __forceinline__ void walk4(const qnode* qnodes, const croot& root, co
             const int32_t off = first == 0 ? offs.x : first == 1 ? offs.y : first == 2 ? offs.z : offs.w;
             const int32_t meta = first == 0 ? metas.x : first == 1 ? metas.y : first == 2 ? metas.z : metas.w;
             const int32_t cnt = int32_t(uint32_t(meta) >> 28);
-            if (cnt > 0) leaf(off, cnt, sp, meta & 0x0fffffff);
-            else next = off;
+            if (cnt > 0) {
+                leaf(off, cnt, sp, meta & 0x0fffffff);
+                if (tmax < tmin) return;  // an occlusion query found its hit
+            } else {
+                next = off;
+            }
         }
         while (next < 0) {
             if (sp == sp_base) return;
@@ -339,8 +348,12 @@ __device__ __forceinline__ void walk4(const qnode* qnodes, const croot& root, co
             if constexpr (CNT) wc.nodes += 32u;
             if (!fbox_hit(bb, fr, f32_up(tmax))) continue;
             const int32_t cnt = int32_t(uint32_t(meta) >> 28);
-            if (cnt > 0) leaf(off, cnt, sp, meta & 0x0fffffff);
-            else next = off;
+            if (cnt > 0) {
+                leaf(off, cnt, sp, meta & 0x0fffffff);
+                if (tmax < tmin) return;
+            } else {
+                next = off;
+            }
         }
         cur = next;
     }
@@ -447,11 +460,63 @@ __device__ __forceinline__ bool list_t(const dscene& S, int64_t first, int64_t c
     return any;
 }
 
-// Inner bvh::hit of a mesh; the stack continues above sp_base.
-template <bool CNT, int FEAT>
+// Coverage of an alpha mask at (u, v): the nearest texel's alpha byte / 255,
+// or its first byte for an image without alpha (image_texture::alpha_value,
+// texture.h:130-136, 182-195).
+__device__ __forceinline__ double mask_coverage(const dscene& S, int ti, double u, double v) {
+    const mpt_texture& t = S.tex[ti];
+    if (t.flags & MPT_TEXF_REPEAT) {
+        u -= std::floor(u);
+        v -= std::floor(v);
+    } else {
+        u = std::fmin(std::fmax(u, 0.0), 1.0);
+        v = std::fmin(std::fmax(v, 0.0), 1.0);
+    }
+    v = 1.0 - v;  // image rows run down
+    const int i = min(max(int(u * t.width), 0), t.width - 1);
+    const int j = min(max(int(v * t.height), 0), t.height - 1);
+    const int64_t k = t.offset + int64_t(j) * t.width + i;
+    return double(S.texels[4 * k + ((t.flags & MPT_TEXF_NO_ALPHA) ? 0 : 3)]) / 255.0;
+}
+
+// triangle::hit's alpha test (mesh.h:72-86) for device primitive p of a mesh
+// with mask ti, hit by the ray (o, d) (instance space): (u, v) from the
+// triangle's texture coordinates, else its barycentrics; a coverage c < 1
+// keeps the hit when the draw is below c. The draw is keyed by the ray's
+// medium stream and the triangle (its mesh's first primitive plus its index
+// in the mesh, shared by every instance), so it does not depend on the order
+// in which the walk meets the triangles.
+__device__ __forceinline__ bool alpha_keeps(const dscene& S, int ti, int64_t mesh_first, int64_t p, const v3& o,
+                                            const v3& d, uint64_t mkey, uint32_t serial) {
+    const double* q = prim_ptr(S, p);
+    const v3 p0{q[0], q[1], q[2]}, e1{q[3], q[4], q[5]}, e2{q[6], q[7], q[8]};
+    const v3 pvec = cross(d, e2);
+    const double inv_det = 1 / dot(e1, pvec);
+    const v3 tvec = o - p0;
+    const double b1 = dot(tvec, pvec) * inv_det;
+    const double b2 = dot(d, cross(tvec, e1)) * inv_det;
+    double u = b1, v = b2;
+    const double* t6 = S.vuv ? S.vuv + 6 * p : nullptr;
+    if (t6 && !std::isnan(t6[0])) {
+        const double b0 = 1 - b1 - b2;
+        u = b0 * t6[0] + b1 * t6[2] + b2 * t6[4];
+        v = b0 * t6[1] + b1 * t6[3] + b2 * t6[5];
+    }
+    const double c = mask_coverage(S, ti, u, v);
+    if (!(c < 1)) return true;
+    const uint32_t salt = 0x80000000u | uint32_t(mesh_first + S.pcanon[p]);
+    return medium_draw(mkey, serial, salt) < c;
+}
+
+// Inner bvh::hit of a mesh; the stack continues above sp_base. alpha: the
+// mesh's mask (-1: opaque), mesh_first its first primitive, (mkey, serial)
+// the ray's medium stream for the mask's draws.
+template <bool CNT, int FEAT, bool ANY = false>
 __device__ bool mesh_t(const dscene& S, int root, const v3& o, const v3& d, double tmin, double tmax, double& t,
-                       int32_t& which, tstack& st, int sp_base, wcount& wc) {
+                       int32_t& which, tstack& st, int sp_base, wcount& wc, int alpha, int64_t mesh_first,
+                       uint64_t mkey, uint32_t serial) {
     bool any = false;
+    double thit = tmax;
     walk4<CNT>(S.qblas, S.blas_roots[root], o, d, tmin, tmax, st, sp_base, wc,
               [&](int off, int cnt, int, int e) {
                   const double tleaf = tmax;
@@ -459,16 +524,22 @@ __device__ bool mesh_t(const dscene& S, int root, const v3& o, const v3& d, doub
                   for (int k = 0; k < cnt; k++) {
                       double tt;
                       if constexpr (CNT) wc.pbytes += 72u;
-                      if (tri_t(prim_ptr(S, off + k), o, d, tmin, tmax, tt)) {
+                      if (tri_t(prim_ptr(S, off + k), o, d, tmin, tmax, tt) &&
+                          (alpha < 0 || alpha_keeps(S, alpha, mesh_first, off + k, o, d, mkey, serial))) {
                           if (ok < 0) ok = leaf_exact<CNT>(S.blas, e, o, d, tmin, tleaf, wc) ? 1 : 0;
                           if (!ok) break;
                           any = true;
                           tmax = tt;
+                          thit = tt;
                           which = off + k;
+                          if constexpr (ANY) {  // the first hit answers an occlusion query: stop the walk
+                              tmax = -kInf;
+                              break;
+                          }
                       }
                   }
               });
-    t = tmax;
+    t = thit;
     return any;
 }
 
@@ -502,7 +573,7 @@ __device__ bool medium_t(const dscene& S, const obj_rec& r, const v3& o, const v
 }
 
 // One world object, the ray in world space. Returns the hit t and primitive.
-template <bool CNT, int FEAT>
+template <bool CNT, int FEAT, bool ANY = false>
 __device__ __forceinline__ bool object_t(const dscene& S, int oi, v3 o, v3 d, double tmin, double tmax, double& t,
                                          int32_t& prim, tstack& st, int sp, bool media, uint64_t mkey,
                                          uint32_t serial, wcount& wc) {
@@ -530,7 +601,9 @@ __device__ __forceinline__ bool object_t(const dscene& S, int oi, v3 o, v3 d, do
     if constexpr (FEAT & F_LIST)
         if (r.kind == MPT_OBJ_LIST) return list_t<CNT, FEAT>(S, r.first, r.count, o, d, tmin, tmax, t, prim, wc);
     if constexpr (FEAT & F_MESH)
-        if (r.kind == MPT_OBJ_MESH) return mesh_t<CNT, FEAT>(S, r.blas_root, o, d, tmin, tmax, t, prim, st, sp, wc);
+        if (r.kind == MPT_OBJ_MESH)
+            return mesh_t<CNT, FEAT, ANY>(S, r.blas_root, o, d, tmin, tmax, t, prim, st, sp, wc, r.alpha, r.first,
+                                          mkey, serial);
     if constexpr (FEAT & F_MEDIUM) {
         prim = -1;
         return medium_t<CNT, FEAT>(S, r, o, d, world_len, tmin, tmax, medium_draw(mkey, serial, uint32_t(oi)), t, wc);
@@ -557,16 +630,59 @@ __device__ __forceinline__ bool ground_t(const dscene& S, const v3& o, const v3&
     return false;
 }
 
-// bvh::hit over the world (bvh.h:45-86). media: test constant_medium objects
-// with draws from (mkey, serial); otherwise they are skipped.
-template <bool CNT = false, int FEAT = F_ALL>
+// height_fog::hit (medium.h:116-146): the free flight of draw u along the
+// ray, inverting the optical depth, which is exponential in t.
+__device__ __forceinline__ bool fog_t(const fog_rec& f, const v3& o, const v3& d, double tmin, double tmax, double u,
+                                      double& t) {
+    const double len = length(d);
+    const double k = d.y / f.scale;
+    const double t0 = tmin;
+    const double start = f.density * len * std::exp(-(o.y + t0 * d.y - f.base) / f.scale);
+    if (!(start > 0)) return false;
+    const double tau = -std::log(1 - u);
+    double tt;
+    if (std::fabs(k) < 1e-12) {
+        tt = t0 + tau / start;
+    } else {
+        const double x = 1 - tau * k / start;
+        if (x <= 0) return false;  // leaves upwards before the collision
+        tt = t0 - std::log(x) / k;
+    }
+    if (!(tt < tmax) || !std::isfinite(tt)) return false;
+    t = tt;
+    return true;
+}
+
+// bvh::hit over the world (bvh.h:45-86), then the unbounded objects beside
+// it in order (main.cpp:70-74): the height fogs, the ground. media: test
+// constant_medium objects and fogs with draws from (mkey, serial); otherwise
+// they are skipped. ANY: an occlusion query, which only asks whether a hit
+// exists: the walk is the closest-hit walk up to its first accepted hit and
+// stops there. Every test the closest-hit walk makes before that hit is the
+// same, and a hit it would accept later exists only if one was accepted
+// first (intervals only shrink on hits; media, fogs and masks draw by key,
+// not by order), so the answer is the closest-hit walk's; h is that first
+// hit, not the closest.
+template <bool CNT = false, int FEAT = F_ALL, bool ANY = false>
 __device__ bool trace_closest(const dscene& S, const v3& o, const v3& d, double tmin, double tmax, hitinfo& h,
                               tstack& st, bool media, uint64_t mkey, uint32_t serial, wcount& wc) {
     h.obj = -1;
     h.prim = -1;
     h.t = tmax;
-    if (S.nobj == 0) return ground_t<FEAT>(S, o, d, tmin, tmax, h);
     bool any = false;
+    if (S.nobj == 0) {
+        if constexpr (FEAT & F_MEDIUM) {
+            if (media)
+                for (int k = 0; k < S.nfog; k++) {
+                    double tf;
+                    if (fog_t(S.fog[k], o, d, tmin, tmax, medium_draw(mkey, serial, uint32_t(S.nobj + 1 + k)), tf)) {
+                        tmax = tf, h.t = tf, h.obj = S.nobj + 1 + k, h.prim = -1, any = true;
+                    }
+                }
+        }
+        if (ground_t<FEAT>(S, o, d, tmin, tmax, h)) any = true;
+        return any;
+    }
     walk4<CNT>(S.qtlas, S.tlas_root, o, d, tmin, tmax, st, 0, wc, [&](int off, int cnt, int sp, int e) {
         const double tleaf = tmax;
         int ok = -1;  // the leaf's double test, made at its first hit
@@ -574,7 +690,7 @@ __device__ bool trace_closest(const dscene& S, const v3& o, const v3& d, double 
             int oi = S.titems[off + k];
             double tt;
             int32_t pr = -1;
-            if (object_t<CNT, FEAT>(S, oi, o, d, tmin, tmax, tt, pr, st, sp, media, mkey, serial, wc)) {
+            if (object_t<CNT, FEAT, ANY>(S, oi, o, d, tmin, tmax, tt, pr, st, sp, media, mkey, serial, wc)) {
                 if (ok < 0) ok = leaf_exact<CNT>(S.tlas, e, o, d, tmin, tleaf, wc) ? 1 : 0;
                 if (!ok) break;
                 any = true;
@@ -582,10 +698,25 @@ __device__ bool trace_closest(const dscene& S, const v3& o, const v3& d, double 
                 h.t = tt;
                 h.obj = oi;
                 h.prim = pr;
+                if constexpr (ANY) {
+                    tmax = -kInf;
+                    break;
+                }
             }
         }
     });
+    if constexpr (ANY)
+        if (any) return true;
     // the objects beside the BVH, after it with the closest t so far (main.cpp:70-74)
+    if constexpr (FEAT & F_MEDIUM) {
+        if (media)
+            for (int k = 0; k < S.nfog; k++) {
+                double tf;
+                if (fog_t(S.fog[k], o, d, tmin, tmax, medium_draw(mkey, serial, uint32_t(S.nobj + 1 + k)), tf)) {
+                    tmax = tf, h.t = tf, h.obj = S.nobj + 1 + k, h.prim = -1, any = true;
+                }
+            }
+    }
     if (ground_t<FEAT>(S, o, d, tmin, tmax, h)) any = true;
     return any;
 }
@@ -620,6 +751,16 @@ __device__ __forceinline__ void finish_hit(const dscene& S, const hitinfo& h, co
             const v3 up{0, 1, 0};
             rec.front = dot(wd, up) < 0;
             rec.normal = rec.front ? up : -up;
+            return;
+        }
+    }
+    if constexpr (FEAT & F_MEDIUM) {
+        if (h.obj > S.nobj) {  // a height fog (medium.h:139-145)
+            rec.t = h.t;
+            rec.p = wo + h.t * wd;
+            rec.normal = v3{1, 0, 0};
+            rec.front = true;
+            rec.mat = S.fog[h.obj - S.nobj - 1].material;
             return;
         }
     }

@@ -476,11 +476,13 @@ __global__ __launch_bounds__(BS) void k_depth(dscene S, launch_ctx L, v3 pixel00
 }
 
 // camera::average (camera.h:223-246) of the running sums on the device.
-__global__ void k_average(const double* __restrict__ acc, int64_t npix, int samples, int features, double* radiance,
-                          double* albedo, double* normal, double* variance) {
+// guide_samples: the passes whose albedo and normal guides are in the sums.
+__global__ void k_average(const double* __restrict__ acc, int64_t npix, int samples, int guide_samples, int features,
+                          double* radiance, double* albedo, double* normal, double* variance) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= npix) return;
     const double scale = 1.0 / samples;
+    const double gscale = guide_samples == samples ? scale : (guide_samples > 0 ? 1.0 / guide_samples : 0.0);
     const double* ar = acc;
     const double* aa = ar + 3 * npix;
     const double* an = aa + 3 * npix;
@@ -489,9 +491,9 @@ __global__ void k_average(const double* __restrict__ acc, int64_t npix, int samp
     if (radiance) radiance[3 * k] = r0, radiance[3 * k + 1] = r1, radiance[3 * k + 2] = r2;
     if (!features) return;
     if (albedo)
-        for (int c = 0; c < 3; c++) albedo[3 * k + c] = scale * aa[3 * k + c];
+        for (int c = 0; c < 3; c++) albedo[3 * k + c] = gscale * aa[3 * k + c];
     if (normal)
-        for (int c = 0; c < 3; c++) normal[3 * k + c] = scale * an[3 * k + c];
+        for (int c = 0; c < 3; c++) normal[3 * k + c] = gscale * an[3 * k + c];
     if (variance) {
         const double mean = luminance(v3{r0, r1, r2});
         const double spread = std::fmax(scale * al[k] - mean * mean, 0.0);
@@ -641,7 +643,7 @@ __global__ __launch_bounds__(BS) void k_features(dscene S, launch_ctx L, pstate 
             constexpr bool kUV = (FEAT & F_UV) != 0;
             hitrec shaded = rec;  // the guide keeps the hit's own normal
             if (kUV && m.normal_map >= 0) shaded.normal = normal_from_map(S, m.normal_map, d, rec);
-            scatter_out so = mat_scatter<-1, kUV>(S, m, d, shaded, rs);
+            scatter_out so = mat_scatter<-1, kUV>(S, m, d, shaded, rs, &rec.normal);
             if (kUV && so.skip && m.normal_map >= 0) so.skip_dir = unbend(so.skip_dir, shaded.normal, rec.normal);
             if (!so.scattered) {
                 albedo = tint;
@@ -829,7 +831,7 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
             return;  // a light scatters nothing (material.h:28-31)
         } else {
             rstream rs{P.key[p], ctr_out};
-            scatter_out so = mat_scatter<MT, RICH>(S, m, d, rec, rs);  // camera.h:409-411
+            scatter_out so = mat_scatter<MT, RICH>(S, m, d, rec, rs, &gn);  // camera.h:409-411
             if (RICH && so.skip && m.normal_map >= 0) so.skip_dir = unbend(so.skip_dir, rec.normal, gn);
             if (so.skip) {                                        // camera.h:413-423
                 bool stop = C.max_specular > 0 && ++scount > C.max_specular;
@@ -949,8 +951,17 @@ __device__ __forceinline__ void shadow_one(const dscene& S, const dcam& C, const
         v3 emission;
         for (int k = 0;; k++) {  // camera.h:488-502
             hitinfo h;
-            if (!trace_closest<CNT, FEAT>(S, o, dir, 0.001, kInf, h, st, true, P.key[p],
-                                    uint32_t(2 * depth + 1) | (uint32_t(k) << 20), wc)) {
+            const uint32_t serial = uint32_t(2 * depth + 1) | (uint32_t(k) << 20);
+            if (S.shadow_occlusion) {
+                // nothing emits and nothing lets it through: whatever the ray hits
+                // first gives no emission, so only whether it hits anything matters
+                if (!trace_closest<CNT, FEAT, true>(S, o, dir, 0.001, kInf, h, st, true, P.key[p], serial, wc))
+                    emission = escaped<ENV>(S, C, to_light, false);
+                else
+                    emission = v3{0, 0, 0};
+                break;
+            }
+            if (!trace_closest<CNT, FEAT>(S, o, dir, 0.001, kInf, h, st, true, P.key[p], serial, wc)) {
                 emission = escaped<ENV>(S, C, to_light, false);
                 break;
             }
@@ -1193,8 +1204,9 @@ __global__ void k_zero(uint32_t* c, int n) {
 }
 
 // ------------------------------------------------------------------ accumulate
-__global__ void k_accumulate(pstate P, int npix, int passes, int features, double* acc_rad, double* acc_alb,
-                             double* acc_nrm, double* acc_l2) {
+// guide_passes: the batch's leading passes whose albedo and normal guides count.
+__global__ void k_accumulate(pstate P, int npix, int passes, int features, int guide_passes, double* acc_rad,
+                             double* acc_alb, double* acc_nrm, double* acc_l2) {
     int pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
     v3 a = v3{acc_rad[3 * pix], acc_rad[3 * pix + 1], acc_rad[3 * pix + 2]};
@@ -1211,8 +1223,10 @@ __global__ void k_accumulate(pstate P, int npix, int passes, int features, doubl
         if (!std::isfinite(c.x) || !std::isfinite(c.y) || !std::isfinite(c.z)) c = v3{0, 0, 0};  // camera.h:281-287
         a = a + c;
         if (features) {
-            al = al + ld3(P, D_AX, s);
-            nr = nr + ld3(P, D_NX, s);
+            if (b < guide_passes) {
+                al = al + ld3(P, D_AX, s);
+                nr = nr + ld3(P, D_NX, s);
+            }
             double l = luminance(c);
             l2 += l * l;
         }
@@ -1298,6 +1312,7 @@ struct mpt_scene {
     double* acc = nullptr;        // radiance, albedo, normal (3 each) + l2
     int64_t acc_pix = 0;
     int last_done = 0;
+    int guide_done = 0;           // passes whose albedo and normal guides are in the sums
     mpt_camera last_cam{};
     hipStream_t stream = nullptr;
     // mpt_autotune's choices for one image size (0: none)
@@ -1517,6 +1532,11 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
     d.env_h = h.env_h;
     d.ground_mat = h.ground_mat;
     d.ground_h = h.ground_h;
+    if ((rc = upload(s.get(), h.fogs, &d.fog))) return rc;
+    d.nfog = int32_t(h.fogs.size());
+    d.shadow_occlusion = 1;
+    for (const mpt_material& m : h.materials)
+        if (m.type == MPT_MAT_DIFFUSE_LIGHT || mat_lets_shadows(m)) d.shadow_occlusion = 0;
     d.has_sun = h.has_sun;
     d.sun_frame = h.sun_frame;
     d.sun_cos_max = h.sun_cos_max;
@@ -1538,11 +1558,13 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
         for (int32_t k : h.prim_kind)
             if (k == MPT_PRIM_TRIANGLE) quads_ok = false;
         // 3: everything, hits carry (u, v) and tangents, shading has textures, maps, thin glass, measured metals
-        s->profile = h.rich ? 3 : (prims_only && !h.objects.empty() ? 1 : (quads_ok ? 2 : 0));
+        // height fogs are media: only the general variants carry them
+        s->profile = h.rich ? 3 : (!h.fogs.empty() ? 0 : (prims_only && !h.objects.empty() ? 1 : (quads_ok ? 2 : 0)));
         for (int32_t m : h.prim_mat)
             if (m >= 0) s->mat_types |= 1u << h.materials[m].type;
         for (const auto& r : h.objects)
             if (r.kind == MPT_OBJ_MEDIUM) s->mat_types |= 1u << MPT_MAT_PHASE;
+        if (!h.fogs.empty()) s->mat_types |= 1u << MPT_MAT_PHASE;  // fog hits shade as a phase material
         s->has_lights = !h.lights.empty();
         bool quads = true, env_only = true;
         for (const auto& L : h.lights) {
@@ -1811,7 +1833,7 @@ int32_t mpt_read_frame(mpt_scene* s, double* radiance, double* albedo, double* n
     double* fn = fa + 3 * npix;
     double* fv = fn + 3 * npix;
     hipLaunchKernelGGL(k_average, dim3(unsigned((npix + 255) / 256)), dim3(256), 0, s->stream, s->acc, npix,
-                       s->last_done, features ? 1 : 0, fr, fa, fn, fv);
+                       s->last_done, s->guide_done, features ? 1 : 0, fr, fa, fn, fv);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));
     if (radiance) HIP_TRY(hipMemcpy(radiance, fr, size_t(npix) * 3 * sizeof(double), hipMemcpyDeviceToHost));
@@ -1832,7 +1854,7 @@ int32_t mpt_read_frame_dev(mpt_scene* s, double* radiance, double* albedo, doubl
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
     const int64_t npix = s->acc_pix;
     hipLaunchKernelGGL(k_average, dim3(unsigned((npix + 255) / 256)), dim3(256), 0, st, s->acc, npix, s->last_done,
-                       features ? 1 : 0, radiance, albedo, normal, variance);
+                       s->guide_done, features ? 1 : 0, radiance, albedo, normal, variance);
     HIP_TRY(hipGetLastError());
     if (!stream) HIP_TRY(hipStreamSynchronize(st));
     return MPT_OK;
@@ -1941,9 +1963,11 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
     if (!keep) {
         HIP_TRY(hipMemsetAsync(s->acc, 0, size_t(npix) * 10 * sizeof(double), s->stream));
         s->last_done = 0;
+        s->guide_done = 0;
     }
     s->last_cam = *cam;
     const int done_before = s->last_done;
+    if (o.guide_passes < 0) return set_error(MPT_E_INVALID, "guide_passes must be >= 0");
 
     const int64_t cap = s->ps_capacity;
     uint32_t* q[3] = {s->queues, s->queues + cap, s->queues + 2 * cap};
@@ -2220,9 +2244,16 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
         R.salt = uint32_t(mix64(o.seed ^ (uint64_t(uint32_t(o.first_pass + pass)) << 32)));
         EV(3, hipLaunchKernelGGL(k_camera<false>, dim3(unsigned((B.nslots + 255) / 256)), dim3(256), 0, stream, C, P, B,
                                  q[0], cnt, int(s->has_lights), MB));
-        if (C.features) {
-            if (s->profile == 3) EV(3, hipLaunchKernelGGL((k_features<256, F_ALL | F_UV>), dim3(G), dim3(256), 0, stream, DS, L, P, B));
-            else EV(3, hipLaunchKernelGGL((k_features<256, F_ALL>), dim3(G), dim3(256), 0, stream, DS, L, P, B));
+        // passes of this batch whose albedo and normal guides are traced (the leading ones)
+        const int guide_nb = o.guide_passes > 0 ? std::max(0, std::min(nb, o.guide_passes - (o.first_pass + pass))) : nb;
+        if (C.features && guide_nb > 0) {
+            batch_args BG = B;
+            BG.nslots = guide_nb * npix;  // slots run pass-major
+            // the guides' walk, compiled for the scene's geometry as the extension kernels are
+            if (s->profile == 1) EV(3, hipLaunchKernelGGL((k_features<256, F_SPHERE>), dim3(G), dim3(256), 0, stream, DS, L, P, BG));
+            else if (s->profile == 2) EV(3, hipLaunchKernelGGL((k_features<256, kQuadProfile>), dim3(G), dim3(256), 0, stream, DS, L, P, BG));
+            else if (s->profile == 3) EV(3, hipLaunchKernelGGL((k_features<256, F_ALL | F_UV>), dim3(G), dim3(256), 0, stream, DS, L, P, BG));
+            else EV(3, hipLaunchKernelGGL((k_features<256, F_ALL>), dim3(G), dim3(256), 0, stream, DS, L, P, BG));
         }
         if (!sorted) EV(4, launch_finish(0, q[0], ctr(cnt, 0)));  // every bounce in one kernel, no binning
         for (int depth = 0; sorted && depth < C.max_depth; depth++) {
@@ -2301,7 +2332,8 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
             }
         }
         EV(3, hipLaunchKernelGGL(k_accumulate, dim3(unsigned((npix + 255) / 256)), dim3(256), 0, stream, P, npix, nb,
-                                 C.features, acc_rad, acc_alb, acc_nrm, acc_l2));
+                                 C.features, guide_nb, acc_rad, acc_alb, acc_nrm, acc_l2));
+        if (C.features) s->guide_done += guide_nb;
         if (hipGetLastError() != hipSuccess) {
             ev_free();
             return set_error(MPT_E_HIP, "kernel launch failed");
@@ -2657,6 +2689,7 @@ int render_batch_impl(mpt_scene* const* ss, const mpt_camera* cams, int n, const
     auto t1 = std::chrono::steady_clock::now();
     for (int k = 0; k < n; k++) {
         ss[k]->last_done += total;
+        ss[k]->guide_done = ss[k]->last_done;
         ss[k]->last_cam = cams[k];
     }
     if (stats) {

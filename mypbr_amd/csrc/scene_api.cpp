@@ -210,7 +210,8 @@ bool image_texture::fully_opaque() const {
 mpt_texture image_texture::to_abi() const {
     mpt_texture t{};
     t.type = MPT_TEX_IMAGE;
-    t.flags = (linear ? MPT_TEXF_LINEAR : 0) | (repeat ? MPT_TEXF_REPEAT : 0) | (hdr.empty() ? 0 : MPT_TEXF_HDR);
+    t.flags = (linear ? MPT_TEXF_LINEAR : 0) | (repeat ? MPT_TEXF_REPEAT : 0) | (hdr.empty() ? 0 : MPT_TEXF_HDR) |
+              (channels == 2 || channels == 4 ? 0 : MPT_TEXF_NO_ALPHA);
     t.width = width;
     t.height = height;
     t.offset = 0;
@@ -225,6 +226,9 @@ static mpt_material blank(int type, int32_t ti) {
     m.spread = 180;
     m.tan_half = __builtin_huge_val();
     m.normal_map = m.metallic_texture = m.roughness_texture = -1;
+    m.coat = 0;
+    m.coat_weight_texture = m.coat_roughness_texture = m.coat_normal_map = -1;
+    m.coat_tint[0] = m.coat_tint[1] = m.coat_tint[2] = 1;
     return m;
 }
 
@@ -308,6 +312,13 @@ mpt_material dielectric::to_abi(int32_t ti) const {
 
 principled::principled(std::shared_ptr<texture> base, double met, double rough, double i)
     : base_color(std::move(base)), metallic(met), roughness(rough), ior(i) {}
+void principled::set_clearcoat(std::shared_ptr<texture> weight, std::shared_ptr<texture> roughness,
+                               std::shared_ptr<texture> coat_normals) {
+    coat_map = std::move(weight);
+    coat_roughness_map = roughness ? std::move(roughness) : std::make_shared<solid_color>(color(0, 0, 0));
+    coat_normal_map = std::move(coat_normals);
+}
+
 mpt_material principled::to_abi(int32_t ti) const {
     mpt_material m = blank(MPT_MAT_PRINCIPLED, ti);
     m.metallic = clamp01(metallic);
@@ -317,6 +328,10 @@ mpt_material principled::to_abi(int32_t ti) const {
     double r0 = (ior - 1) / (ior + 1);
     m.f0 = r0 * r0;
     if (measured) set_conductor(m, conductor);
+    if (coat_map) {
+        m.coat = 1;
+        for (int k = 0; k < 3; k++) m.coat_tint[k] = coat_tint[k];
+    }
     return m;
 }
 
@@ -372,7 +387,8 @@ struct flat_builder::impl {
     std::vector<int32_t> prim_kind, prim_mat;
     std::vector<double> prim_data;
     std::vector<double> prim_nrm;  // 9 per primitive once any triangle has vertex normals
-    std::vector<int32_t> obj_kind, obj_xform, obj_mat, obj_bkind;
+    std::vector<int32_t> obj_kind, obj_xform, obj_mat, obj_bkind, obj_alpha;
+    bool any_alpha = false;
     std::vector<int64_t> obj_first, obj_count, obj_id;
     std::vector<double> obj_density;
     std::vector<mpt_xform> xforms;
@@ -384,6 +400,8 @@ struct flat_builder::impl {
     bool has_ground = false;
     int32_t ground_mat = -1;
     double ground_height = 0;
+    std::vector<double> fog_params;   // 3 per height fog
+    std::vector<int32_t> fog_material;
     std::vector<int32_t> light_object;
     std::vector<double> env_rgb;
     std::map<const material*, int32_t> mat_index;
@@ -427,8 +445,15 @@ int32_t flat_builder::material_index(const std::shared_ptr<material>& m) {
     a.normal_map = texture_index(m->normal_map_texture());
     a.metallic_texture = texture_index(m->metallic_texture());
     a.roughness_texture = texture_index(m->roughness_texture());
-    a.uses_uv = (a.normal_map >= 0 || image(m->base_texture()) || image(m->metallic_texture()) ||
-                 image(m->roughness_texture())) ? 1 : 0;
+    if (a.coat) {
+        a.coat_weight_texture = texture_index(m->coat_weight_texture());
+        auto cr = m->coat_roughness_texture();
+        a.coat_roughness_texture = texture_index(cr ? cr : std::make_shared<solid_color>(color(0, 0, 0)));
+        a.coat_normal_map = texture_index(m->coat_normal_texture());
+    }
+    a.uses_uv = (a.normal_map >= 0 || a.coat_normal_map >= 0 || image(m->base_texture()) ||
+                 image(m->metallic_texture()) || image(m->roughness_texture()) ||
+                 (a.coat && (image(m->coat_weight_texture()) || image(m->coat_roughness_texture())))) ? 1 : 0;
     int32_t mi = int32_t(p->materials.size());
     p->materials.push_back(a);
     p->mat_index[m.get()] = mi;
@@ -462,6 +487,22 @@ void flat_builder::set_ground(double height, int32_t material) {
     p->ground_mat = material;
 }
 
+void flat_builder::add_fog(double density, double scale_height, double base, int32_t material) {
+    p->fog_params.insert(p->fog_params.end(), {density, scale_height, base});
+    p->fog_material.push_back(material);
+}
+
+void height_fog::flatten(flat_builder& b, int32_t xform) const {
+    if (xform >= 0) throw std::runtime_error("a height fog cannot be instanced");
+    if (!std::dynamic_pointer_cast<phase_material>(phase)) throw std::runtime_error("a height fog needs a volume material");
+    b.add_fog(density, scale_height, base, b.material_index(phase));
+}
+
+aabb height_fog::bounding_box() const {
+    const double inf = std::numeric_limits<double>::infinity();
+    return aabb(interval(-inf, inf), interval(-inf, inf), interval(-inf, inf));
+}
+
 void ground_plane::flatten(flat_builder& b, int32_t xform) const {
     if (xform >= 0) throw std::runtime_error("a ground plane cannot be instanced");
     b.set_ground(height, b.material_index(mat));
@@ -492,6 +533,7 @@ int32_t flat_builder::add_object(int32_t kind, int64_t first, int64_t count, int
     p->obj_density.push_back(0);
     p->obj_mat.push_back(-1);
     p->obj_bkind.push_back(MPT_OBJ_PRIM);
+    p->obj_alpha.push_back(-1);
     p->next_id += count;
     int32_t idx = int32_t(p->obj_kind.size() - 1);
     p->object_of[owner] = idx;
@@ -602,7 +644,12 @@ void mesh::flatten(flat_builder& b, int32_t xform) const {
     } else {
         first = it->second.first;
     }
-    b.add_object(MPT_OBJ_MESH, first, count, xform, this);
+    const int32_t oi = b.add_object(MPT_OBJ_MESH, first, count, xform, this);
+    if (alpha_mask && !alpha_mask->fully_opaque()) {
+        if (!alpha_mask->hdr.empty()) throw std::runtime_error("an alpha mask must be an 8-bit image");
+        b.p->obj_alpha[size_t(oi)] = b.texture_index(alpha_mask);
+        b.p->any_alpha = true;
+    }
 }
 
 void instance::flatten(flat_builder& b, int32_t xform) const {
@@ -948,6 +995,10 @@ const mpt_scene_desc& flat_builder::desc() {
     d.has_ground = p->has_ground ? 1 : 0;
     d.ground_material = p->ground_mat;
     d.ground_height = p->ground_height;
+    d.object_alpha = p->any_alpha ? p->obj_alpha.data() : nullptr;
+    d.n_fogs = int32_t(p->fog_material.size());
+    d.fog_params = p->fog_params.empty() ? nullptr : p->fog_params.data();
+    d.fog_material = p->fog_material.empty() ? nullptr : p->fog_material.data();
     d.n_lights = int32_t(p->light_object.size());
     d.light_object = p->light_object.data();
     d.env_rgb = p->env_rgb.empty() ? nullptr : p->env_rgb.data();
@@ -1154,10 +1205,11 @@ namespace {
 // principled layer with roughness from the Phong exponent Ns and Ni, Ke ->
 // an emitter; map_Kd replaces Kd, norm (or map_Bump, bump, map_Disp, disp
 // naming a color image) bends the normal. Textures wrap around; one that
-// cannot be loaded is skipped with a warning and the plain color stays. Alpha
-// masks (map_d, or a diffuse texture's alpha channel) are not modelled: a
-// material that would have one makes loading fail.
-bool load_mtl(const std::string& path, std::map<std::string, std::shared_ptr<material>>& result, std::string& error,
+// cannot be loaded is skipped with a warning and the plain color stays. The
+// alpha mask (obj_loader.h:169-172): map_d when it is not fully opaque, else
+// without map_d the diffuse texture when it has a non-opaque alpha channel.
+bool load_mtl(const std::string& path, std::map<std::string, std::shared_ptr<material>>& result,
+              std::map<std::string, std::shared_ptr<image_texture>>& masks, std::string& error,
               bool use_normal_maps) {
     std::ifstream file(path);
     if (!file) {
@@ -1250,14 +1302,14 @@ bool load_mtl(const std::string& path, std::map<std::string, std::shared_ptr<mat
             m = std::make_shared<lambertian>(base);
         }
         if (e.normal_map && e.ke.length_squared() == 0) m = std::make_shared<normal_mapped>(m, e.normal_map);
-        const bool masked = (e.map_d && !e.map_d->fully_opaque()) ||
-                            (!e.map_d && e.map_kd && (e.map_kd->channels == 2 || e.map_kd->channels == 4) &&
-                             !e.map_kd->fully_opaque());
-        if (masked) {
-            error = path + ": material '" + name + "': alpha masks are not modelled by the GPU path";
-            return false;
-        }
+        std::shared_ptr<image_texture> mask;
+        if (e.map_d && !e.map_d->fully_opaque())
+            mask = e.map_d;
+        else if (!e.map_d && e.map_kd && (e.map_kd->channels == 2 || e.map_kd->channels == 4) &&
+                 !e.map_kd->fully_opaque())
+            mask = e.map_kd;
         result[name] = m;
+        masks[name] = mask;
     }
     return true;
 }
@@ -1278,13 +1330,20 @@ bool load_obj(const std::string& path, const std::map<std::string, std::shared_p
     struct face {
         int v[3], n[3], t[3];
     };
-    // faces per material, the materials in the order of their first face
-    std::vector<std::pair<std::shared_ptr<material>, std::vector<face>>> groups;
-    std::map<const material*, size_t> group_of;
+    // faces per material and alpha mask (obj_loader.h:196-200), in the order of their first face
+    struct group {
+        std::shared_ptr<material> mat;
+        std::shared_ptr<image_texture> mask;
+        std::vector<face> faces;
+    };
+    std::vector<group> groups;
+    std::map<std::pair<const material*, const image_texture*>, size_t> group_of;
     std::shared_ptr<material> current = default_mat;
+    std::shared_ptr<image_texture> current_mask;
     auto slash = path.find_last_of('/');
     std::string dir = slash == std::string::npos ? "" : path.substr(0, slash + 1);
     std::map<std::string, std::shared_ptr<material>> file_materials;
+    std::map<std::string, std::shared_ptr<image_texture>> file_masks;
     auto argument = [](const char* q) {  // the rest of the line without surrounding blanks
         std::string arg(q);
         arg.erase(0, arg.find_first_not_of(" \t"));
@@ -1362,24 +1421,30 @@ bool load_obj(const std::string& path, const std::map<std::string, std::shared_p
                 }
                 const point3& p0 = positions[f.v[0]];
                 if (cross(positions[f.v[1]] - p0, positions[f.v[2]] - p0).near_zero()) continue;
-                auto g = group_of.find(current.get());
+                auto g = group_of.find({current.get(), current_mask.get()});
                 if (g == group_of.end()) {
-                    g = group_of.emplace(current.get(), groups.size()).first;
-                    groups.push_back({current, {}});
+                    g = group_of.emplace(std::make_pair(current.get(), current_mask.get()), groups.size()).first;
+                    groups.push_back({current, current_mask, {}});
                 }
-                groups[g->second].second.push_back(f);
+                groups[g->second].faces.push_back(f);
             }
         } else if (line.compare(at, 6, "usemtl") == 0) {
             std::string name = argument(q + 6);
-            if (auto it = materials.find(name); it != materials.end()) current = it->second;
-            else if (auto jt = file_materials.find(name); jt != file_materials.end()) current = jt->second;
-            else current = default_mat;
+            current_mask = nullptr;  // a scene material or the default has no mask
+            if (auto it = materials.find(name); it != materials.end()) {
+                current = it->second;
+            } else if (auto jt = file_materials.find(name); jt != file_materials.end()) {
+                current = jt->second;
+                current_mask = file_masks[name];
+            } else {
+                current = default_mat;
+            }
         } else if (line.compare(at, 6, "mtllib") == 0) {
             std::istringstream names(argument(q + 6));
             std::string name, mtl_error;
             while (names >> name) {
                 const std::string full = name[0] == '/' ? name : dir + name;
-                if (!load_mtl(full, file_materials, mtl_error, use_normal_maps)) {
+                if (!load_mtl(full, file_materials, file_masks, mtl_error, use_normal_maps)) {
                     if (mtl_error.find("cannot open") == 0)
                         std::fprintf(stderr, "Warning: %s: %s\n", path.c_str(), mtl_error.c_str());
                     else {
@@ -1392,8 +1457,8 @@ bool load_obj(const std::string& path, const std::map<std::string, std::shared_p
     }
     if (smooth && normals.empty()) {  // area-weighted vertex normals (obj_loader.h:311-324)
         normals.assign(positions.size(), vec3(0, 0, 0));
-        for (auto& [mat, faces] : groups)
-            for (auto& f : faces) {
+        for (auto& gr : groups)
+            for (auto& f : gr.faces) {
                 const point3& p0 = positions[f.v[0]];
                 vec3 n = cross(positions[f.v[1]] - p0, positions[f.v[2]] - p0);
                 for (int c = 0; c < 3; c++) {
@@ -1402,10 +1467,11 @@ bool load_obj(const std::string& path, const std::map<std::string, std::shared_p
                 }
             }
     }
-    for (auto& [mat, faces] : groups) {
+    for (auto& gr : groups) {
+        const auto& mat = gr.mat;
         std::vector<int32_t> idx, nidx, tidx;
         bool any_normals = false, any_uvs = false;
-        for (const auto& f : faces)
+        for (const auto& f : gr.faces)
             for (int c = 0; c < 3; c++) {
                 idx.push_back(f.v[c]);
                 nidx.push_back(f.n[c]);
@@ -1422,6 +1488,7 @@ bool load_obj(const std::string& path, const std::map<std::string, std::shared_p
             m->uvs = uvs;
             m->uv_indices = std::move(tidx);
         }
+        m->alpha_mask = gr.mask;
         parts.push_back({mat, m});
     }
     if (parts.empty()) {
@@ -1996,8 +2063,22 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
             } else if (type == "principled") {
                 std::shared_ptr<texture> base = std::make_shared<solid_color>(color(0.8, 0.8, 0.8));
                 std::shared_ptr<texture> metallic_map, roughness_map, base_normals;
+                std::shared_ptr<texture> coat, coat_roughness, coat_normals;  // no coat unless given
+                color coat_tint(1, 1, 1);
+                bool coat_tinted = false;
                 double metallic = 0, roughness = 0.5, ior = 1.5;
                 bool measured = false;
+                // "<number in [0, 1]>" or "texture <name>" as a texture (scene.h:310-322)
+                auto value_texture = [&](std::shared_ptr<texture>& tex, std::string& err) {
+                    std::string first;
+                    if (!(in >> first)) return false;
+                    if (first == "texture") return read_texture_name(tex, err);
+                    std::istringstream number(first);
+                    double x;
+                    if (!(number >> x) || x < 0 || x > 1) return false;
+                    tex = std::make_shared<solid_color>(color(x, x, x));
+                    return true;
+                };
                 conductor_ior ior_metal;
                 std::string key, error;
                 while (in >> key) {
@@ -2010,7 +2091,15 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
                     else if (key == "roughness") ok = read_value_or_texture(roughness, roughness_map, error);
                     else if (key == "ior") ok = bool(in >> ior) && ior > 0;
                     else if (key == "normal_map") ok = read_texture_name(base_normals, error);
-                    else if (key == "conductor") {
+                    else if (key == "clearcoat") ok = value_texture(coat, error);
+                    else if (key == "clearcoat_roughness") ok = value_texture(coat_roughness, error);
+                    else if (key == "clearcoat_normal_map") ok = read_texture_name(coat_normals, error);
+                    else if (key == "clearcoat_color") {
+                        double r, g, b;  // 0 would need infinite absorption, > 1 would add energy
+                        ok = bool(in >> r >> g >> b) && r > 0 && g > 0 && b > 0 && r <= 1 && g <= 1 && b <= 1;
+                        coat_tint = color(r, g, b);
+                        coat_tinted = true;
+                    } else if (key == "conductor") {
                         std::string metal_name;
                         ok = bool(in >> metal_name) && (measured = find_conductor(metal_name, ior_metal));
                         if (!ok) error = "unknown conductor '" + metal_name + "' (known: Al Ti Cu Au Ag Cr)";
@@ -2023,6 +2112,12 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
                 pm->measured = measured;
                 pm->conductor = ior_metal;
                 if (use_normal_maps) pm->normal_map = base_normals;  // principled bends its own normals
+                if ((coat_roughness || coat_normals || coat_tinted) && !coat)
+                    return fail("clearcoat_roughness, clearcoat_normal_map and clearcoat_color need 'clearcoat <weight>'");
+                if (coat) {
+                    pm->set_clearcoat(coat, coat_roughness, use_normal_maps ? coat_normals : nullptr);
+                    pm->set_clearcoat_color(coat_tint);
+                }
                 mat = pm;
             } else if (type == "volume") {
                 double r, g, b, aniso = 0;
@@ -2099,6 +2194,16 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
             if (!std::dynamic_pointer_cast<phase_material>(mat)) return fail("a medium needs a volume material");
             if (transformed) boundary = std::make_shared<instance>(boundary, xform);
             world.add(std::make_shared<constant_medium>(boundary, density, mat));
+        } else if (keyword == "height_fog") {  // scene.h:656-666
+            const char* usage = "expected 'height_fog <density> <scale height> <volume material> [base <y>]'";
+            double density, scale, base = 0;
+            std::string name, key;
+            if (!(in >> density >> scale >> name) || density <= 0 || scale <= 0) return fail(usage);
+            if (in >> key && (key != "base" || !(in >> base))) return fail(usage);
+            auto it = materials.find(name);
+            if (it == materials.end() || !std::dynamic_pointer_cast<phase_material>(it->second))
+                return fail("expected a volume material, got '" + name + "'");
+            result.unbounded.add(std::make_shared<height_fog>(density, scale, base, it->second));
         } else if (keyword == "fog") {
             std::string name;
             if (!(in >> fog_density >> name) || fog_density <= 0) return fail("expected 'fog <positive density> <volume material>'");

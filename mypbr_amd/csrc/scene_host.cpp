@@ -441,6 +441,18 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         s.ground_h = d.ground_height;
         s.rich = true;  // the ground is tested in the kernels of the variant that also carries (u, v)
     }
+    s.fogs.clear();
+    if (d.n_fogs < 0 || (d.n_fogs > 0 && (!d.fog_params || !d.fog_material)))
+        return fail(MPT_E_INVALID, "height fogs without their parameters");
+    for (int32_t k = 0; k < d.n_fogs; k++) {
+        const double* q = d.fog_params + 3 * k;
+        const int32_t m = d.fog_material[k];
+        if (!(q[0] > 0) || !(q[1] > 0) || !std::isfinite(q[0]) || !std::isfinite(q[1]) || !std::isfinite(q[2]))
+            return fail(MPT_E_INVALID, "a height fog needs a positive density and scale height");
+        if (m < 0 || m >= d.n_materials || d.materials[m].type != MPT_MAT_PHASE)
+            return fail(MPT_E_INVALID, "a height fog needs a volume material");
+        s.fogs.push_back(fog_rec{q[0], q[1], q[2], m, 0});
+    }
     auto tex_ok = [&](int32_t t) { return t >= -1 && t < d.n_textures; };
     auto is_image = [&](int32_t t) { return t >= 0 && s.textures[size_t(t)].type == MPT_TEX_IMAGE; };
     for (size_t k = 0; k < s.materials.size(); k++) {
@@ -457,15 +469,28 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
             return fail(MPT_E_UNSUPPORTED, "material " + std::to_string(k) + ": normal maps apply to lambertian, metal and principled");
         if ((m.metallic_texture >= 0 || m.roughness_texture >= 0) && m.type != MPT_MAT_PRINCIPLED)
             return fail(MPT_E_UNSUPPORTED, "material " + std::to_string(k) + ": metallic and roughness maps are principled's");
+        if (m.coat) {
+            if (m.type != MPT_MAT_PRINCIPLED)
+                return fail(MPT_E_INVALID, "material " + std::to_string(k) + ": only principled takes a clear coat");
+            if (m.coat_weight_texture < 0 || m.coat_weight_texture >= d.n_textures || m.coat_roughness_texture < 0 ||
+                m.coat_roughness_texture >= d.n_textures || !tex_ok(m.coat_normal_map))
+                return fail(MPT_E_INVALID, "material " + std::to_string(k) + ": the clear coat names a texture that does not exist");
+            for (int c = 0; c < 3; c++)
+                if (!(m.coat_tint[c] > 0 && m.coat_tint[c] <= 1))
+                    return fail(MPT_E_INVALID, "material " + std::to_string(k) + ": the coat's tint must be in (0, 1]");
+        } else {
+            m.coat_weight_texture = m.coat_roughness_texture = m.coat_normal_map = -1;
+        }
         // recomputed here: whatever the caller set, hits of this material need (u, v) exactly when
         m.uses_uv = (m.normal_map >= 0 || (needs_tex && is_image(m.texture)) || is_image(m.metallic_texture) ||
-                     is_image(m.roughness_texture)) ? 1 : 0;
+                     is_image(m.roughness_texture) || m.coat_normal_map >= 0 || is_image(m.coat_weight_texture) ||
+                     is_image(m.coat_roughness_texture)) ? 1 : 0;
         if (m.uses_uv) s.uses_uv = true;
         if (m.thin && m.type != MPT_MAT_DIELECTRIC)
             return fail(MPT_E_INVALID, "material " + std::to_string(k) + ": only a dielectric can be thin");
         if (m.measured && m.type != MPT_MAT_METAL && m.type != MPT_MAT_PRINCIPLED)
             return fail(MPT_E_INVALID, "material " + std::to_string(k) + ": only metal and principled can be measured");
-        if (m.uses_uv || m.thin || m.measured) s.rich = true;
+        if (m.uses_uv || m.thin || m.measured || m.coat) s.rich = true;
     }
     s.xforms.resize(d.n_xforms);
     for (int k = 0; k < d.n_xforms; k++) {
@@ -565,7 +590,15 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         r.density = d.object_density ? d.object_density[o] : 0;
         r.bkind = d.object_boundary_kind ? d.object_boundary_kind[o] : MPT_OBJ_PRIM;
         r.light_table = -1;
+        r.alpha = d.object_alpha ? d.object_alpha[o] : -1;
+        r.pad = 0;
         next_id = r.id_base + r.count;
+        if (r.alpha != -1) {
+            if (r.kind != MPT_OBJ_MESH) return fail(MPT_E_INVALID, "only meshes take alpha masks");
+            if (r.alpha < 0 || r.alpha >= d.n_textures || d.textures[r.alpha].type != MPT_TEX_IMAGE ||
+                (d.textures[r.alpha].flags & MPT_TEXF_HDR))
+                return fail(MPT_E_INVALID, "an alpha mask must be an 8-bit image texture");
+        }
         if (r.first < 0 || r.count < 1 || r.first + r.count > np)
             return fail(MPT_E_INVALID, "object " + std::to_string(o) + " has a bad primitive range");
         if (r.xform < -1 || r.xform >= d.n_xforms)

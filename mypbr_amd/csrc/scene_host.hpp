@@ -94,6 +94,18 @@ struct obj_rec {
     double density;      // media
     int32_t bkind;       // media: boundary kind (MPT_OBJ_PRIM or MPT_OBJ_LIST)
     int32_t light_table; // mesh lights: offset of its triangles in mlight_prim / mlight_cdf (else -1)
+    int32_t alpha;       // meshes: alpha-mask image texture (mesh.h:82-86), -1 when opaque
+    int32_t pad;
+};
+
+// A height fog (height_fog, medium.h:98-155): density at y = base, scale
+// height, base and its phase material.
+struct fog_rec {
+    double density;
+    double scale;
+    double base;
+    int32_t material;
+    int32_t pad;
 };
 
 struct light_rec {
@@ -132,6 +144,7 @@ struct host_scene {
     bool rich = false;                   // uses_uv, thin glass, a measured metal or the ground: the variant with them
     int32_t ground_mat = -1;             // the ground plane's material, -1 without one
     double ground_h = 0;
+    std::vector<fog_rec> fogs;           // height fogs, tested after the BVH and before the ground
     // the sun (sun.h:26-34): its frame around the direction, the cone's
     // cosine and solid angle, and radiance = irradiance / solid angle
     bool has_sun = false;

@@ -39,11 +39,16 @@ def ssim(x, ref, win=8):
     return float(s.mean())
 
 
+# the denoiser's albedo and normal guides come from the first GUIDE_PASSES passes
+# (mpt_render_opts.guide_passes): they converge long before the radiance does
+GUIDE_PASSES = 16
+
+
 def render(scene, cam, spp, seed, features=False):
     """Frame (radiance, and guides with features) and its wall time."""
     c = camera_copy(cam, samples_per_pixel=spp, collect_features=int(features))
     t0 = time.perf_counter()
-    f = scene.render(c, seed=seed)
+    f = scene.render(c, seed=seed, guide_passes=GUIDE_PASSES if features else 0)
     return f, time.perf_counter() - t0
 
 
@@ -95,13 +100,14 @@ def crop_camera(base, lookfrom, lookat, full_width, crop, cx, cy):
     return c
 
 
-def train_kpcnn(scene, base_cam, n_views=8, width=1920, spp_in=(32, 64, 128, 192), spp_ref=4096, steps=1500,
-                lr=2e-3, lr_end=1e-4, seed=1, hidden=64, crop=256, crops_per_view=2, relmse_share=1 / 3, log=None):
-    """A kernel-predicting denoiser trained on the tracer's own renders: a
-    batch of n_views x crops_per_view crop x crop windows of width-wide views
-    (random camera positions, windows at the full view's pixel scale),
-    inputs at each spp of spp_in with their guides, targets at spp_ref. Each
-    Adam step takes the whole batch at one input spp, cycling through them;
+def train_kpcnn(scene, base_cam, n_views=16, width=1920, spp_in=(32, 64, 128, 192), spp_ref=4096, steps=2000,
+                lr=2e-3, lr_end=1e-4, seed=1, hidden=64, crop=256, crops_per_view=4, batch=16, relmse_share=1 / 3,
+                log=None):
+    """A kernel-predicting denoiser trained on the tracer's own renders:
+    n_views x crops_per_view crop x crop windows of width-wide views (random
+    camera positions, windows at the full view's pixel scale), inputs at each
+    spp of spp_in with their guides, targets at spp_ref. Each Adam step takes
+    a batch of `batch` windows at one input spp, cycling through both;
     the learning rate falls from lr to lr_end on a cosine. The loss is the
     L1 of log(1 + x) (robust to the inputs' fireflies), then for the last
     relmse_share of the steps relMSE itself, the metric it is judged by."""
@@ -117,13 +123,16 @@ def train_kpcnn(scene, base_cam, n_views=8, width=1920, spp_in=(32, 64, 128, 192
     t0 = time.perf_counter()
     refs = np.stack([render(scene, c, spp_ref, 10_000 + k)[0].radiance for k, c in enumerate(cams)])
     t_ref = time.perf_counter() - t0
-    batches = []
-    for spp in spp_in:
-        frames = [frame_dict(scene, render(scene, c, spp, 100 * spp + k, features=True)[0], c)
-                  for k, c in enumerate(cams)]
-        batches.append(to_device(frames))
+    groups = [list(range(g, min(g + batch, len(cams)))) for g in range(0, len(cams), batch)]
+    batches = []  # (device inputs, device targets), spp level fastest
+    inputs = {spp: [frame_dict(scene, render(scene, c, spp, 100 * spp + k, features=True)[0], c)
+                    for k, c in enumerate(cams)] for spp in spp_in}
+    for g in groups:
+        tgt = DeviceArray.of(refs[g], np.float32)
+        for spp in spp_in:
+            batches.append((to_device([inputs[spp][k] for k in g]), tgt))
+    del inputs
     net = KPCNN(layers=6, hidden=hidden, seed=seed)
-    tgt = DeviceArray.of(refs, np.float32)
     switch = int(steps * (1 - relmse_share))
     t0 = time.perf_counter()
     losses = []
@@ -131,13 +140,14 @@ def train_kpcnn(scene, base_cam, n_views=8, width=1920, spp_in=(32, 64, 128, 192
         if i in (0, switch):
             net.set_loss(1 if i < switch else 0)
         rate = lr_end + 0.5 * (lr - lr_end) * (1 + np.cos(np.pi * i / steps))
-        losses.append(net.run(batches[i % len(batches)], target=tgt, lr=float(rate)))
+        dev, tgt = batches[i % len(batches)]
+        losses.append(net.run(dev, target=tgt, lr=float(rate)))
     train_s = time.perf_counter() - t0
     if log:
-        log(f"kpcnn: {net.nparams} parameters, {steps} steps of {len(cams)} {crop}^2 crops in {train_s:.1f} s, "
+        log(f"kpcnn: {net.nparams} parameters, {steps} steps of {batch} of {len(cams)} {crop}^2 crops in {train_s:.1f} s, "
             f"log-L1 loss {losses[0]:.4f} -> {losses[switch - 1]:.4f}, relMSE loss {losses[switch]:.5f} -> "
             f"{losses[-1]:.5f}")
-    return net, {"views": n_views, "width": width, "crop": crop, "batch": len(cams), "spp_in": list(spp_in),
+    return net, {"views": n_views, "width": width, "crop": crop, "crops": len(cams), "batch": batch, "spp_in": list(spp_in),
                  "spp_target": spp_ref, "target_render_s": t_ref, "steps": steps, "lr": lr, "hidden": hidden,
                  "layers": 6, "parameters": int(net.nparams), "loss": "log_l1, then relmse for the last %.2f" % relmse_share,
                  "lr_end": lr_end, "loss_first": losses[0], "loss_last_log_l1": losses[switch - 1],
@@ -177,7 +187,7 @@ class Cache:
 
 
 def equal_time_study(scene_path, width=1920, ref_spp=4096, plain_spp=(4, 16, 64, 128, 192, 256, 1024),
-                     kpcnn_spp=(4, 16, 64, 128, 192), nrc_frames=256, box=None, log=print):
+                     kpcnn_spp=(4, 16, 64, 128, 192, 224), nrc_frames=256, box=None, log=print):
     """Plain tracer at several spp against the learned forms (kernel-predicting
     denoiser on kpcnn_spp inputs, radiance cache) and the classical filter,
     all against one ref_spp render of the plain tracer; times are wall clock
@@ -202,7 +212,7 @@ def equal_time_study(scene_path, width=1920, ref_spp=4096, plain_spp=(4, 16, 64,
         c = camera_copy(cam, samples_per_pixel=spp, collect_features=1)
         synchronize()
         t0 = time.perf_counter()
-        scene.render(c, seed=2, read=False)
+        scene.render(c, seed=2, read=False, guide_passes=GUIDE_PASSES)
         dev = scene.frame_dev(c)
         t1 = time.perf_counter()
         den = net.denoise_dev(dev).numpy()

@@ -106,7 +106,10 @@ class DescBuilder:
             m.sigma_a[k] = sigma_a[k]
         return self._add_mat(m)
 
-    def principled(self, rgb, metallic, roughness, ior=1.5, texture=None, conductor=None):
+    def principled(self, rgb, metallic, roughness, ior=1.5, texture=None, conductor=None, coat=None,
+                   coat_roughness=0.0, coat_tint=(1.0, 1.0, 1.0)):
+        """coat: a clear coat of that weight (set_clearcoat, material.h:363-379),
+        with its roughness and tint, as solid textures."""
         tex = texture if texture is not None else self.texture_solid(rgb)
         m = self._mat(abi.MAT_PRINCIPLED, tex)
         r = min(max(roughness, 0.0), 1.0)
@@ -116,6 +119,13 @@ class DescBuilder:
         m.f0 = r0 * r0
         if conductor is not None:
             self._measure(m, conductor)
+        if coat is not None:
+            m.coat = 1
+            m.coat_weight_texture = self.texture_solid((coat, coat, coat))
+            m.coat_roughness_texture = self.texture_solid((coat_roughness,) * 3)
+            m.coat_normal_map = -1
+            for k in range(3):
+                m.coat_tint[k] = coat_tint[k]
         return self._add_mat(m)
 
     def diffuse_light(self, rgb, spread=180.0):

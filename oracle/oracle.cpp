@@ -547,6 +547,11 @@ struct texture_t {
         if (flags & MPT_TEXF_LINEAR) return color(ldr[4 * k], ldr[4 * k + 1], ldr[4 * k + 2]) / 255.0;
         return color(srgb[ldr[4 * k]], srgb[ldr[4 * k + 1]], srgb[ldr[4 * k + 2]]);
     }
+    // image_texture::alpha_value (texture.h:130-136): the alpha byte, or the
+    // first byte of an image without an alpha channel, over 255
+    double alpha_value(double u, double v) const {
+        return ldr[4 * pixel_index(u, v) + ((flags & MPT_TEXF_NO_ALPHA) ? 0 : 3)] / 255.0;
+    }
     size_t pixel_index(double u, double v) const {  // texture.h:182-195
         if (flags & MPT_TEXF_REPEAT) {
             u -= std::floor(u);
@@ -691,9 +696,34 @@ struct dielectric : material {  // material.h:184-264
     }
 };
 
-struct principled : material {  // material.h:346-552 (base layers only)
+struct principled : material {  // material.h:346-552 (base layers and clear coat)
     texture_t base;
     double metallic, alpha, f0;
+    // clear coat (material.h:326-345, 460-527): weight and roughness from the
+    // red channels of their textures, an optional normal map of its own, a tint
+    bool has_coat = false, has_coat_normal_map = false;
+    texture_t coat_map, coat_roughness_map, coat_normal_map;
+    color coat_tint = color(1, 1, 1);
+    static constexpr double coat_f0 = 0.04;
+    double coat_at(const hit_record& rec) const {
+        return has_coat ? dclamp(coat_map.value(rec.u, rec.v, rec.p).x(), 0.0, 1.0) : 0.0;
+    }
+    double coat_alpha_at(const hit_record& rec) const {
+        return roughness_to_alpha(dclamp(coat_roughness_map.value(rec.u, rec.v, rec.p).x(), 0.0, 1.0));
+    }
+    vec3 coat_normal(const ray& r, const hit_record& rec) const {
+        return has_coat_normal_map ? normal_from_map(r, rec, coat_normal_map) : rec.normal;
+    }
+    color coat_transmittance(double w, double cos_o, double cos_i) const {
+        if (coat_tint.x() == 1 && coat_tint.y() == 1 && coat_tint.z() == 1) return color(1, 1, 1);
+        auto cos_t = [](double c) {
+            c = dclamp(c, 0.0, 1.0);
+            return std::sqrt(1 - (1 - c * c) / (1.5 * 1.5));
+        };
+        double e = (1 / cos_t(cos_o) + 1 / cos_t(cos_i)) / 2;
+        color t(std::pow(coat_tint.x(), e), std::pow(coat_tint.y(), e), std::pow(coat_tint.z(), e));
+        return (1 - w) * color(1, 1, 1) + w * t;
+    }
     // metallic and roughness maps (red channel) and the base's normal map, when present
     bool has_metallic_map = false, has_roughness_map = false, has_normal_map = false;
     texture_t metallic_map, roughness_map, normal_map;
@@ -735,12 +765,20 @@ struct principled : material {  // material.h:346-552 (base layers only)
                                                   std::make_unique<cosine_pdf>(n), p_spec);
         s.attenuation = b;
         s.skip_pdf = false;
+        double w = coat_at(rec);
+        if (w > 0) {  // the coat's sharp lobe gets its share of the samples (material.h:409-419)
+            vec3 n_coat = coat_normal(r, rec);
+            double coat_weight = w * fresnel_schlick(dot(n_coat, wo), coat_f0);
+            double below = (1 - coat_weight) * (spec_w + diff_w) * average(coat_transmittance(w, 1, 1));
+            double p_coat = dclamp(coat_weight / (coat_weight + below), 0.1, 0.9);
+            s.pdf_ptr = std::make_unique<mixture_pdf>(std::make_unique<ggx_reflection_pdf>(n_coat, wo, coat_alpha_at(rec)),
+                                                      std::move(s.pdf_ptr), p_coat);
+        }
         return true;
     }
-    color eval(const ray& r, const hit_record& rec, const vec3& d) const override {
-        if (dot(d, rec.normal) <= 0) return color(0, 0, 0);  // the unbent normal
-        vec3 wo_w = unit_vector(-r.dir), wi_w = unit_vector(d);
-        onb f(base_normal(r, rec));
+    // the base layers with normal n (material.h:486-506)
+    color base_eval(const hit_record& rec, const vec3& n, const vec3& wo_w, const vec3& wi_w) const {
+        onb f(n);
         vec3 wo = unit_vector(f.to_local(wo_w)), wi = unit_vector(f.to_local(wi_w));
         if (wo.z() <= 0 || wi.z() <= 0) return color(0, 0, 0);
         double met = metallic_at(rec), a = alpha_at(rec);
@@ -752,6 +790,28 @@ struct principled : material {  // material.h:346-552 (base layers only)
         double transmitted = (1 - fresnel_schlick(wi.z(), f0)) * (1 - fresnel_schlick(wo.z(), f0));
         color diffuse = ((1 - met) * transmitted * wi.z() / pi) * b;
         return specular + diffuse;
+    }
+    color eval(const ray& r, const hit_record& rec, const vec3& d) const override {  // material.h:423-456
+        if (dot(d, rec.normal) <= 0) return color(0, 0, 0);  // the unbent normal
+        vec3 wo_w = unit_vector(-r.dir), wi_w = unit_vector(d);
+        color below = base_eval(rec, base_normal(r, rec), wo_w, wi_w);
+        double w = coat_at(rec);
+        if (w <= 0) return below;
+        vec3 n_coat = coat_normal(r, rec);
+        double cos_o = dot(n_coat, wo_w), cos_i = dot(n_coat, wi_w);
+        double through = (1 - w * fresnel_schlick(cos_o, coat_f0)) * (1 - w * fresnel_schlick(cos_i, coat_f0));
+        color result = through * coat_transmittance(w, cos_o, cos_i) * below;
+        if (cos_o > 0 && cos_i > 0) {
+            onb frame(n_coat);
+            vec3 lo = unit_vector(frame.to_local(wo_w)), li = unit_vector(frame.to_local(wi_w));
+            vec3 h = unit_vector(lo + li);
+            double ca = coat_alpha_at(rec);
+            ggx dist(ca);
+            double coat = fresnel_schlick(dot(li, h), coat_f0) * dist.D(h) * dist.G(lo, li) / (4 * lo.z());
+            coat += (ms_fresnel(*table, color(coat_f0, coat_f0, coat_f0), ca) * ms_eval(*table, lo.z(), li.z(), ca)).x();
+            result = result + color(w * coat, w * coat, w * coat);
+        }
+        return result;
     }
 };
 
@@ -946,7 +1006,7 @@ struct quad : hittable {  // quad.h
     double power() const override { return pi * area * luminance(mat->average_emission()); }
 };
 
-struct triangle : hittable {  // mesh.h:28-166 (vertex normals and uvs; no alpha masks)
+struct triangle : hittable {  // mesh.h:28-166 (vertex normals, uvs, alpha masks)
     point3 p0, p1, p2;
     const material* mat;
     int64_t id;
@@ -955,11 +1015,16 @@ struct triangle : hittable {  // mesh.h:28-166 (vertex normals and uvs; no alpha
     vec3 n0, n1, n2;
     bool has_uvs = false;
     vec3 t0, t1, t2;  // (u, v, 0) at each vertex
+    // the mesh's alpha mask (mesh.h:82-86) and its first primitive: the draw
+    // of the mask test is keyed by mesh_first + id (the triangle's index in
+    // its mesh), as on the GPU
+    const texture_t* alpha = nullptr;
+    int64_t mesh_first = 0;
     triangle(const point3& a, const point3& b, const point3& c, const material* m, int64_t id)
         : p0(a), p1(b), p2(c), mat(m), id(id) {
         area = 0.5 * cross(p1 - p0, p2 - p0).length();
     }
-    bool hit(const ray& r, interval t, hit_record& rec, const hit_ctx&) const override {
+    bool hit(const ray& r, interval t, hit_record& rec, const hit_ctx& c) const override {
         vec3 e1 = p1 - p0, e2 = p2 - p0;
         vec3 pvec = cross(r.dir, e2);
         double det = dot(e1, pvec);
@@ -973,11 +1038,22 @@ struct triangle : hittable {  // mesh.h:28-166 (vertex normals and uvs; no alpha
         if (b2 < 0 || b1 + b2 > 1) return false;
         double tt = dot(e2, qvec) * inv_det;
         if (!t.surrounds(tt)) return false;
+        double b0 = 1 - b1 - b2;
+        if (alpha) {  // checked before rec is touched (mesh.h:82-86)
+            double u = b1, v = b2;
+            if (has_uvs) {
+                vec3 uv = b0 * t0 + b1 * t1 + b2 * t2;
+                u = uv.x();
+                v = uv.y();
+            }
+            double coverage = alpha->alpha_value(u, v);
+            if (coverage < 1 && medium_u(c.key, c.serial, 0x80000000u | uint32_t(mesh_first + id)) >= coverage)
+                return false;
+        }
         rec.t = tt;
         rec.p = r.at(tt);
         rec.mat = mat;
         rec.id = id;
-        double b0 = 1 - b1 - b2;
         rec.u = b1;
         rec.v = b2;
         if (has_uvs) {  // mesh.h:75-81
@@ -1455,6 +1531,42 @@ struct ground_plane : hittable {
     aabb bounding_box() const override { return aabb(); }  // never in a BVH
 };
 
+// height_fog (medium.h:98-155): density * exp(-(y - base) / scale) over all
+// of space; the free flight inverts the closed-form optical depth along the
+// ray. Drawn from the medium stream with object index `index` (the world
+// list's size plus one plus the fog's position), as the GPU draws it.
+struct height_fog : hittable {
+    double density, scale, base;
+    const material* phase;
+    uint32_t index;
+    bool hit(const ray& r, interval t, hit_record& rec, const hit_ctx& c) const override {
+        if (!c.media) return false;
+        double length = r.dir.length();
+        double k = r.dir.y() / scale;
+        double t0 = t.min;
+        double start = density * length * std::exp(-(r.orig.y() + t0 * r.dir.y() - base) / scale);
+        if (!(start > 0)) return false;
+        double tau = -std::log(1 - medium_u(c.key, c.serial, index));
+        double tt;
+        if (std::fabs(k) < 1e-12) {
+            tt = t0 + tau / start;
+        } else {
+            double x = 1 - tau * k / start;
+            if (x <= 0) return false;
+            tt = t0 - std::log(x) / k;
+        }
+        if (!(tt < t.max) || !std::isfinite(tt)) return false;
+        rec.t = tt;
+        rec.p = r.at(tt);
+        rec.normal = vec3(1, 0, 0);
+        rec.front_face = true;
+        rec.mat = phase;
+        rec.id = -2;
+        return true;
+    }
+    aabb bounding_box() const override { return aabb(); }  // never in a BVH
+};
+
 // ------------------------------------------------------------------ scene
 struct scene {
     std::vector<std::unique_ptr<material>> mats;
@@ -1468,6 +1580,8 @@ struct scene {
     std::unique_ptr<sun_light> sun;    // also in lights, when the scene has one
     ggx_table table;
     aabb world_box;
+    std::vector<const hittable*> fogs; // beside the BVH, before the ground (scene::unbounded)
+    std::vector<std::unique_ptr<texture_t>> masks;  // meshes' alpha masks
     const hittable* ground = nullptr;  // beside the BVH (scene::unbounded)
     std::vector<uint8_t> texels;    // image textures, copied from the description
     std::vector<float> texels_hdr;
@@ -1476,6 +1590,13 @@ struct scene {
     // the BVH, then the objects beside it, as one hittable_list (main.cpp:70-74)
     bool hit(const ray& r, interval t, hit_record& rec, const hit_ctx& c) const {
         bool any = tree->hit(r, t, rec, c);
+        for (const hittable* f : fogs) {
+            hit_record g;
+            if (f->hit(r, interval(t.min, any ? rec.t : t.max), g, c)) {
+                rec = g;
+                any = true;
+            }
+        }
         if (ground) {
             hit_record g;
             if (ground->hit(r, interval(t.min, any ? rec.t : t.max), g, c)) {
@@ -1521,25 +1642,25 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
         double encoded = k / 255.0;
         s.srgb[k] = encoded <= 0.04045 ? encoded / 12.92 : std::pow((encoded + 0.055) / 1.055, 2.4);
     }
+    auto tex = [&](int ti) {
+        texture_t t{};
+        const mpt_texture& x = d.textures[ti];
+        t.type = x.type;
+        t.inv_scale = x.inv_scale;
+        t.even = color(x.even[0], x.even[1], x.even[2]);
+        t.odd = color(x.odd[0], x.odd[1], x.odd[2]);
+        if (x.type == MPT_TEX_IMAGE) {
+            t.width = x.width;
+            t.height = x.height;
+            t.flags = x.flags;
+            if (x.flags & MPT_TEXF_HDR) t.hdr = s.texels_hdr.data() + 3 * x.offset;
+            else t.ldr = s.texels.data() + 4 * x.offset;
+            t.srgb = s.srgb.data();
+        }
+        return t;
+    };
     for (int k = 0; k < d.n_materials; k++) {
         const mpt_material& m = d.materials[k];
-        auto tex = [&](int ti) {
-            texture_t t{};
-            const mpt_texture& x = d.textures[ti];
-            t.type = x.type;
-            t.inv_scale = x.inv_scale;
-            t.even = color(x.even[0], x.even[1], x.even[2]);
-            t.odd = color(x.odd[0], x.odd[1], x.odd[2]);
-            if (x.type == MPT_TEX_IMAGE) {
-                t.width = x.width;
-                t.height = x.height;
-                t.flags = x.flags;
-                if (x.flags & MPT_TEXF_HDR) t.hdr = s.texels_hdr.data() + 3 * x.offset;
-                else t.ldr = s.texels.data() + 4 * x.offset;
-                t.srgb = s.srgb.data();
-            }
-            return t;
-        };
         std::unique_ptr<material> mp;
         switch (m.type) {
             case MPT_MAT_LAMBERTIAN: {
@@ -1591,6 +1712,12 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
                     p->conductor = metal_fresnel::from_ior(color(m.eta[0], m.eta[1], m.eta[2]), color(m.k[0], m.k[1], m.k[2]));
                 double r0 = (m.ior - 1) / (m.ior + 1);
                 p->f0 = r0 * r0;
+                if ((p->has_coat = m.coat != 0)) {
+                    p->coat_map = tex(m.coat_weight_texture);
+                    p->coat_roughness_map = tex(m.coat_roughness_texture);
+                    if ((p->has_coat_normal_map = m.coat_normal_map >= 0)) p->coat_normal_map = tex(m.coat_normal_map);
+                    p->coat_tint = color(m.coat_tint[0], m.coat_tint[1], m.coat_tint[2]);
+                }
                 mp = std::move(p);
                 break;
             }
@@ -1690,7 +1817,21 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
                 int64_t shift = idb;
                 if (!tree) {
                     std::vector<const hittable*> tris;
-                    for (int64_t k = 0; k < count; k++) tris.push_back(prim(first + k, k));
+                    const int32_t am = d.object_alpha ? d.object_alpha[o] : -1;
+                    const texture_t* mask = nullptr;
+                    if (am >= 0) {
+                        s.masks.push_back(std::make_unique<texture_t>(tex(am)));
+                        mask = s.masks.back().get();
+                    }
+                    for (int64_t k = 0; k < count; k++) {
+                        const hittable* t = prim(first + k, k);
+                        if (mask) {
+                            auto* tri = const_cast<triangle*>(static_cast<const triangle*>(t));
+                            tri->alpha = mask;
+                            tri->mesh_first = first;
+                        }
+                        tris.push_back(t);
+                    }
                     const bvh* b = own(s, std::make_unique<bvh>(tris));
                     tree = own(s, std::make_unique<mesh_obj>(b, tris));
                     meshes.push_back({first, tree});
@@ -1741,6 +1882,15 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
         s.world_box = aabb(s.world_box, h->bounding_box());
     }
     s.tree = std::make_unique<bvh>(s.world);
+    for (int k = 0; k < d.n_fogs; k++) {
+        auto f = std::make_unique<height_fog>();
+        f->density = d.fog_params[3 * k];
+        f->scale = d.fog_params[3 * k + 1];
+        f->base = d.fog_params[3 * k + 2];
+        f->phase = matp(d.fog_material[k]);
+        f->index = uint32_t(d.n_objects + 1 + k);
+        s.fogs.push_back(own(s, std::move(f)));
+    }
     if (d.has_ground) {
         auto g = std::make_unique<ground_plane>();
         g->height = d.ground_height;

@@ -174,9 +174,15 @@ int ref_ggx_table(double* e, double* eavg) {
 // 5 phase), r, g, b, roughness, ior, metallic, g (phase), sigma_a rgb, and a
 // measured metal for metal and principled (0 none, 1.. Al Ti Cu Au Ag Cr).
 // q: normal, ray direction, wi (9 doubles); front flags.
-int ref_bsdf_eval(const double* spec, const double* q, const uint8_t* front, int64_t n, double* eval, double* pdfv) {
+// material::eval and the scatter pdf's value of the materials spec rows
+// describe (stride doubles each: type, r, g, b, roughness, ior, metallic, g,
+// sigma_a (3), measured metal; with stride 17 also principled's clear coat:
+// weight (0: none), roughness, tint (3)), at the queries' normals and
+// directions.
+int ref_bsdf_eval_rows(const double* spec, int stride, const double* q, const uint8_t* front, int64_t n, double* eval,
+                       double* pdfv) {
     for (int64_t i = 0; i < n; i++) {
-        const double* s = spec + 12 * i;
+        const double* s = spec + stride * i;
         int type = int(s[0]);
         color c(s[1], s[2], s[3]);
         std::shared_ptr<material> m;
@@ -190,6 +196,11 @@ int ref_bsdf_eval(const double* spec, const double* q, const uint8_t* front, int
         else if (type == 3) {
             auto p = std::make_shared<principled>(std::make_shared<solid_color>(c), s[6], s[4], s[5]);
             if (measured) p->set_conductor(ior);
+            if (stride >= 17 && s[12] > 0) {
+                p->set_clearcoat(std::make_shared<solid_color>(color(s[12], s[12], s[12])),
+                                 std::make_shared<solid_color>(color(s[13], s[13], s[13])));
+                p->set_clearcoat_color(color(s[14], s[15], s[16]));
+            }
             m = p;
         } else m = std::make_shared<phase_material>(c, s[7]);
         hit_record rec{};
@@ -210,6 +221,10 @@ int ref_bsdf_eval(const double* spec, const double* q, const uint8_t* front, int
         }
     }
     return 0;
+}
+
+int ref_bsdf_eval(const double* spec, const double* q, const uint8_t* front, int64_t n, double* eval, double* pdfv) {
+    return ref_bsdf_eval_rows(spec, 12, q, front, n, eval, pdfv);
 }
 
 // camera::render of the loaded scene with width/spp/depth overridden;

@@ -22,6 +22,12 @@ crate.obj / .mtl     a box whose MTL materials take map_Kd (the JPEG below) and 
                      norm normal map, with texture coordinates beyond [0, 1]
                      (the textures repeat), mirrored on one side, and one
                      face without them (its (u, v) are the barycentrics);
+leaves.obj / .mtl    alpha-masked quads: "leaf" takes map_d leaf_mask.png (grey +
+                     alpha, a leaf outline with a half-transparent rim), "fence"
+                     a diffuse map with an alpha channel and no map_d (the
+                     diffuse texture's alpha is the mask), "board" stays opaque;
+leaf_mask.png        64 x 64 grey + alpha (PNG color type 4);
+fence_rgba.png       32 x 32 RGBA slats with transparent gaps (PNG color type 6);
 globe.jpg            a 128 x 64 continents-and-ocean color map, baseline JPEG
                      with 2x2-subsampled chroma. JPEG needs an encoder: this
                      one is written by stb_image_write through the compiled
@@ -206,7 +212,7 @@ def png_write(path, rows, color_type, depth, palette=None):
     (PNG spec, section 9: each filter's difference from the raw bytes)."""
     import struct
     import zlib
-    samples = {0: 1, 2: 3, 3: 1}[color_type]
+    samples = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[color_type]
 
     def raw(row):  # samples to bytes, big-endian 16-bit, or packed low bit depths
         if depth == 16:
@@ -285,6 +291,60 @@ def texture_maps():
     png_write(os.path.join(HERE, "tiles_metal.png"), metal, 3, 2, palette=[(0, 0, 0), (80, 80, 80), (180, 0, 0), (255, 255, 255)])
 
 
+def leaves():
+    n = 64
+    mask = []
+    for j in range(n):
+        row = []
+        for i in range(n):
+            x, y = (i + 0.5) / n - 0.5, (j + 0.5) / n - 0.5
+            r = (x / 0.45) ** 2 + (y / 0.3) ** 2  # an ellipse: opaque inside, a soft rim, clear outside
+            a = 255 if r < 0.7 else (128 if r < 1 else (40 if r < 1.15 else 0))
+            row += [200, a]
+        mask.append(row)
+    png_write(os.path.join(HERE, "leaf_mask.png"), mask, 4, 8)
+    m = 32
+    fence = []
+    for j in range(m):
+        row = []
+        for i in range(m):
+            slat = (i // 4) % 2 == 0
+            row += [150, 110, 70, 255 if slat else (96 if (j // 8) % 2 else 0)]
+        fence.append(row)
+    png_write(os.path.join(HERE, "fence_rgba.png"), fence, 6, 8)
+    with open(os.path.join(HERE, "leaves.mtl"), "w") as f:
+        f.write("# alpha-masked materials (see scenes/make_assets.py)\n"
+                "newmtl leaf\nKd 0.25 0.55 0.15\nmap_d leaf_mask.png\n\n"
+                "newmtl fence\nKd 1 1 1\nmap_Kd fence_rgba.png\n\n"
+                "newmtl board\nKd 0.6 0.6 0.55\n")
+    v, vt, faces = [], [], []
+
+    def quad(c, u, w, mat, uv_scale=1.0):
+        base = len(v)
+        for a, b in ((0, 0), (1, 0), (1, 1), (0, 1)):
+            v.append([c[k] + a * u[k] + b * w[k] for k in range(3)])
+            vt.append([a * uv_scale, b * uv_scale])
+        faces.append((mat, base))
+
+    quad((-1.6, 0.2, 0.0), (1.2, 0, 0.3), (0, 1.0, 0), "leaf")
+    quad((-0.9, 0.5, 0.6), (1.1, 0.1, -0.2), (0, 0.9, 0.1), "leaf")
+    quad((0.3, 0.0, -0.2), (1.4, 0, 0), (0, 1.4, 0), "fence", 2.0)
+    quad((-2.0, 0.0, -1.0), (4.0, 0, 0), (0, 2.0, 0), "board")
+    with open(os.path.join(HERE, "leaves.obj"), "w") as f:
+        f.write("# two leaves, a fence panel and an opaque board behind them (see scenes/make_assets.py)\n"
+                "mtllib leaves.mtl\n")
+        for p in v:
+            f.write("v %s %s %s\n" % tuple(fmt(x) for x in p))
+        for t in vt:
+            f.write("vt %s %s\n" % tuple(fmt(x) for x in t))
+        cur = None
+        for mat, b in faces:
+            if mat != cur:
+                f.write("usemtl %s\n" % mat)
+                cur = mat
+            f.write("f %d/%d %d/%d %d/%d %d/%d\n" % (b + 1, b + 1, b + 2, b + 2, b + 3, b + 3, b + 4, b + 4))
+
+
 def globe_jpg(w=128, h=64):
     import ctypes as C
     import sys
@@ -320,3 +380,4 @@ if __name__ == "__main__":
     texture_maps()
     globe_jpg()
     crate()
+    leaves()

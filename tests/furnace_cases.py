@@ -98,6 +98,17 @@ for _r in (0.1, 0.5, 1.0):
         (f"principled metallic, roughness {_r}", sphere_case(lambda b, r=_r: b.principled((1, 1, 1), 1.0, r))),
     ]
 
+# a clear coat never adds energy (material.h:334-341): it takes F_c into its own lobe and passes
+# (1 - w F_c) twice, times the tint's transmittance, to the layers below
+for _r in (0.05, 0.4):
+    BOUNDED += [
+        (f"principled metallic under a clear coat, coat roughness {_r}",
+         sphere_case(lambda b, r=_r: b.principled((1, 1, 1), 1.0, 0.3, coat=1.0, coat_roughness=r))),
+        (f"principled dielectric under a tinted clear coat, coat roughness {_r}",
+         sphere_case(lambda b, r=_r: b.principled((1, 1, 1), 0.0, 0.6, coat=0.8, coat_roughness=r,
+                                                   coat_tint=(0.6, 0.3, 0.2)))),
+    ]
+
 # furnace_test.cpp:237-248: white rough conductors keep all the energy
 ENERGY = []
 for _r in (0.5, 1.0):

@@ -14,6 +14,8 @@ queries) are seeded and regenerated identically by the tests:
   trace_digests.json    SHA-256 of the reference's IDs and t bits over the full
                         1M-ray batches, tie and miss counts
   light_probs.json      light_list::probability per light (light_list.h:45-127)
+  bsdf_coat.npz         the same for principled with a clear coat (set_clearcoat,
+                        set_clearcoat_color, material.h:326-552)
   bsdf.npz              material::eval and the scatter pdf's value at fixed
                         directions (material.h, microfacet.h, pdf.h, medium.h)
   render_<scene>.npz    converged low-resolution means of camera::render and
@@ -24,6 +26,10 @@ queries) are seeded and regenerated identically by the tests:
   ground.npz            scenes/ground.scene (the ground plane beside the BVH) as textured.npz
   sun.npz               scenes/sun.scene (sun_light, sun.h) as below, plus the reference's
                         light_list::probability per light
+  height_fog.npz        scenes/height_fog.scene (height_fog, medium.h:98-155) as sun.npz
+  alpha_leaves.npz      scenes/alpha_leaves.scene (alpha masks, mesh.h:82-86, through the
+                        reference's load_obj/load_mtl) as sun.npz
+  clearcoat.npz         scenes/clearcoat.scene (principled's clear coat) as sun.npz
   sky_atmosphere.npz    scenes/sky_atmosphere.scene (the sky bake, atmosphere.h) as sun.npz,
                         plus the reference's environment_light::radiance of its baked sky
                         along 4096 seeded directions
@@ -39,7 +45,7 @@ queries) are seeded and regenerated identically by the tests:
                         writes (display_settings::apply + to_byte, tonemap.h,
                         color3.h) on the images and settings of tests/display_cases.py
 
-Usage: python tests/golden/make_golden.py [step ...]  (steps: ggx trace ties bsdf render denoise display obj textures materials; all by default)
+Usage: python tests/golden/make_golden.py [step ...]  (steps: ggx trace ties bsdf render denoise display obj textures materials sun fog alpha coat; all by default)
 (all steps take about a minute)
 """
 import hashlib
@@ -92,11 +98,21 @@ def scene_path(name):
     return os.path.join(ROOT, "scenes", name)
 
 
-def bsdf_queries(seed=5):
+# principled with a clear coat: the 12 columns above, then coat weight, coat roughness, tint (3)
+COAT_MATERIALS = [
+    [3, 0.6, 0.2, 0.1, 0.4, 1.5, 0.0, 0, 0, 0, 0, 0, 1.0, 0.0, 1, 1, 1],
+    [3, 0.8, 0.7, 0.3, 0.5, 1.5, 1.0, 0, 0, 0, 0, 0, 0.7, 0.1, 0.6, 0.3, 0.2],
+    [3, 0.5, 0.4, 0.3, 0.3, 1.5, 1.0, 0, 0, 0, 0, 6, 1.0, 0.05, 0.6, 0.35, 0.28],
+    [3, 0.2, 0.5, 0.8, 0.7, 1.45, 0.3, 0, 0, 0, 0, 0, 0.3, 0.3, 1, 1, 1],
+]
+
+
+def bsdf_queries(seed=5, materials=None):
     """Seeded (normal, ray direction, wi, front) queries; the ray direction
     points against the normal as after set_face_normal (hittable.h:23-26)."""
+    materials = BSDF_MATERIALS if materials is None else materials
     rng = np.random.default_rng(seed)
-    n = len(BSDF_MATERIALS) * N_PER_MATERIAL
+    n = len(materials) * N_PER_MATERIAL
 
     def unit(k):
         v = rng.normal(size=(k, 3))
@@ -113,7 +129,7 @@ def bsdf_queries(seed=5):
     ud = rd[:k] / np.linalg.norm(rd[:k], axis=1, keepdims=True)
     wi[:k] = ud - 2 * np.sum(ud * nrm[:k], axis=1, keepdims=True) * nrm[:k]
     front = rng.integers(0, 2, size=n).astype(np.uint8)
-    spec = np.repeat(np.array(BSDF_MATERIALS, dtype=np.float64), N_PER_MATERIAL, axis=0)
+    spec = np.repeat(np.array(materials, dtype=np.float64), N_PER_MATERIAL, axis=0)
     q = np.concatenate([nrm, rd, wi], axis=1)
     return spec, np.ascontiguousarray(q), front
 
@@ -163,6 +179,25 @@ SUN_SCENES = {
     "sky_atmosphere.scene": (48, 256, 8, 12),
     "sun_disk.scene": (48, 256, 8, 12),
 }
+
+
+FOG_SCENES = {
+    "height_fog.scene": (48, 256, 8, 12),
+}
+ALPHA_SCENES = {
+    "alpha_leaves.scene": (48, 256, 8, 12),
+    "clearcoat.scene": (48, 256, 8, 12),
+}
+
+
+def golden_alpha():
+    for name, render in ALPHA_SCENES.items():
+        golden_asset_scene(name, render)
+
+
+def golden_fog():
+    for name, render in FOG_SCENES.items():
+        golden_asset_scene(name, render)
 
 
 def golden_sun():
@@ -248,7 +283,7 @@ def main():
         sys.exit("oracle/_ref/libmpt_ref.so is not built (make -C oracle ref, where the reference is present)")
     R = ol.reference()
     steps = set(sys.argv[1:]) or {"ggx", "trace", "ties", "bsdf", "render", "denoise", "display", "obj",
-                                  "textures", "materials", "sun"}
+                                  "textures", "materials", "sun", "fog", "alpha", "coat"}
     if "denoise" in steps:
         golden_denoise()
     if "display" in steps:
@@ -261,6 +296,10 @@ def main():
         golden_materials()
     if "sun" in steps:
         golden_sun()
+    if "fog" in steps:
+        golden_fog()
+    if "alpha" in steps:
+        golden_alpha()
     if "ggx" in steps:
         golden_ggx(R)
     if "trace" in steps:
@@ -269,6 +308,8 @@ def main():
         golden_ties()
     if "bsdf" in steps:
         golden_bsdf(R)
+    if "coat" in steps:
+        golden_coat(R)
     if "render" in steps:
         golden_render(R)
 
@@ -420,6 +461,22 @@ def golden_bsdf(R):
     R.ref_bsdf_eval(ol._d(spec), ol._d(q), front.ctypes.data_as(ol.P(ol.C.c_uint8)), n, ol._d(ev), ol._d(pdf))
     np.savez_compressed(os.path.join(HERE, "bsdf.npz"), spec=spec, q=q, front=front, eval=ev, pdf=pdf)
     print("bsdf.npz", n, "queries", flush=True)
+
+
+def golden_coat(R):
+    """bsdf_coat.npz: principled::eval and its scatter pdf's value with a clear
+    coat (material.h:326-552), through the reference's set_clearcoat."""
+    spec, q, front = bsdf_queries(seed=9, materials=COAT_MATERIALS)
+    n = spec.shape[0]
+    ev = np.zeros((n, 3))
+    pdf = np.zeros(n)
+    f = R.ref_bsdf_eval_rows
+    f.restype = ol.C.c_int
+    f.argtypes = [ol.P(ol.C.c_double), ol.C.c_int, ol.P(ol.C.c_double), ol.P(ol.C.c_uint8), ol.C.c_int64,
+                  ol.P(ol.C.c_double), ol.P(ol.C.c_double)]
+    f(ol._d(spec), 17, ol._d(q), front.ctypes.data_as(ol.P(ol.C.c_uint8)), n, ol._d(ev), ol._d(pdf))
+    np.savez_compressed(os.path.join(HERE, "bsdf_coat.npz"), spec=spec, q=q, front=front, eval=ev, pdf=pdf)
+    print("bsdf_coat.npz", n, "queries", flush=True)
 
 
 def golden_render(R):

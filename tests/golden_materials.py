@@ -3,7 +3,8 @@
 A spec row is: type (0 lambertian, 1 metal, 2 dielectric, 3 principled,
 5 phase), r, g, b, roughness, ior, metallic, g, sigma_a (3), measured metal
 (0 none, 1.. Al Ti Cu Au Ag Cr) - the parameters make_golden.py gave the
-reference's material constructors.
+reference's material constructors. Rows of 17 add principled's clear coat:
+weight (0 none), roughness, tint (3).
 """
 METALS = ["Al", "Ti", "Cu", "Au", "Ag", "Cr"]
 import numpy as np
@@ -27,8 +28,11 @@ def desc_for_specs(spec):
             elif t == 2:
                 mi = b.dielectric(row[5], row[4], tuple(row[8:11]))
             elif t == 3:
+                coat = {}
+                if len(row) >= 17 and row[12] > 0:
+                    coat = dict(coat=row[12], coat_roughness=row[13], coat_tint=tuple(row[14:17]))
                 mi = b.principled(rgb, row[6], row[4], row[5],
-                                  conductor=METALS[int(row[11]) - 1] if row[11] > 0 else None)
+                                  conductor=METALS[int(row[11]) - 1] if row[11] > 0 else None, **coat)
             else:
                 mi = b.phase(rgb, row[7])
             index[key] = mi

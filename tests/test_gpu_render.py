@@ -223,3 +223,21 @@ def test_device_frame_and_depth_equal_the_host_ones(mpt):
         depth = s.primary_depth(cam)
         assert np.array_equal(dev["depth"].numpy(), depth)
         assert depth.max() > 0
+
+
+def test_guide_passes_leave_radiance_and_variance_unchanged(mpt):
+    """mpt_render_opts.guide_passes traces the albedo and normal guides for
+    the leading passes only: the radiance and variance equal the full
+    render's bit for bit, and the guides equal those of the first that many
+    passes of the same render (pass_count; the strata of a pass depend on the
+    render's spp), also when batches straddle the boundary."""
+    from mypbr_amd.abi import camera_copy
+    d = mpt.describe(scene_path("textured.scene"))
+    s = mpt.Scene.from_desc(d.desc, d.camera)
+    cam = camera_copy(d.camera, image_width=48, samples_per_pixel=12, max_depth=12, collect_features=1)
+    full = s.render(cam, seed=4, passes_in_flight=5)
+    part = s.render(cam, seed=4, passes_in_flight=5, guide_passes=7)
+    few = s.render(cam, seed=4, passes_in_flight=5, pass_count=7)
+    assert np.array_equal(part.radiance, full.radiance) and np.array_equal(part.variance, full.variance)
+    assert np.array_equal(part.albedo, few.albedo) and np.array_equal(part.normal, few.normal)
+    assert not np.array_equal(part.albedo, full.albedo)
