@@ -255,8 +255,9 @@ __device__ __forceinline__ bool leaf_exact(const cnode* nodes, int e, const v3& 
 // pops it. A box the double test rejects with a larger tmax it rejects with
 // a smaller one, and the culls are conservative, so every leaf the reference
 // visits is reached, in the reference's order; leaf(off, cnt, sp, e) tests a
-// leaf's items, shrinks tmax on hits, and keeps a hit only if the leaf's
-// double box test (leaf_exact) holds. Against a binary walk over the same
+// leaf's items, shrinks tmax on hits, keeps a hit only if the leaf's
+// double box test (leaf_exact) holds, and returns true to end the walk (an
+// occlusion query's first hit). Against a binary walk over the same
 // tree in 64-byte single-precision records this took the spheres scene from
 // 1449 to 1721 M paths/s and the generated batch from 15.6 to 21.9 M.
 
@@ -269,7 +270,7 @@ __device__ __forceinline__ void walk4(const qnode* qnodes, const croot& root, co
         const v3 inv{1 / d.x, 1 / d.y, 1 / d.z};
         if (!box_hit(root.b, o, inv, tmin, tmax)) return;
         if (root.cnt > 0) {
-            leaf(root.off, root.cnt, sp_base, -1);
+            (void)leaf(root.off, root.cnt, sp_base, -1);
             return;
         }
         fr = make_fray(o, inv, tmin, root.b);
@@ -324,8 +325,7 @@ __device__ __forceinline__ void walk4(const qnode* qnodes, const croot& root, co
             const int32_t meta = first == 0 ? metas.x : first == 1 ? metas.y : first == 2 ? metas.z : metas.w;
             const int32_t cnt = int32_t(uint32_t(meta) >> 28);
             if (cnt > 0) {
-                leaf(off, cnt, sp, meta & 0x0fffffff);
-                if (tmax < tmin) return;  // an occlusion query found its hit
+                if (leaf(off, cnt, sp, meta & 0x0fffffff)) return;  // an occlusion query found its hit
             } else {
                 next = off;
             }
@@ -349,8 +349,7 @@ __device__ __forceinline__ void walk4(const qnode* qnodes, const croot& root, co
             if (!fbox_hit(bb, fr, f32_up(tmax))) continue;
             const int32_t cnt = int32_t(uint32_t(meta) >> 28);
             if (cnt > 0) {
-                leaf(off, cnt, sp, meta & 0x0fffffff);
-                if (tmax < tmin) return;
+                if (leaf(off, cnt, sp, meta & 0x0fffffff)) return;
             } else {
                 next = off;
             }
@@ -510,13 +509,14 @@ __device__ __forceinline__ bool alpha_keeps(const dscene& S, int ti, int64_t mes
 
 // Inner bvh::hit of a mesh; the stack continues above sp_base. alpha: the
 // mesh's mask (-1: opaque), mesh_first its first primitive, (mkey, serial)
-// the ray's medium stream for the mask's draws.
-template <bool CNT, int FEAT, bool ANY = false>
+// the ray's medium stream for the mask's draws; masks read (u, v), so only
+// the variants that carry it (F_UV) test them (the host puts every masked
+// scene there). any_hit: stop at the first accepted hit.
+template <bool CNT, int FEAT>
 __device__ bool mesh_t(const dscene& S, int root, const v3& o, const v3& d, double tmin, double tmax, double& t,
                        int32_t& which, tstack& st, int sp_base, wcount& wc, int alpha, int64_t mesh_first,
-                       uint64_t mkey, uint32_t serial) {
+                       uint64_t mkey, uint32_t serial, bool any_hit) {
     bool any = false;
-    double thit = tmax;
     walk4<CNT>(S.qblas, S.blas_roots[root], o, d, tmin, tmax, st, sp_base, wc,
               [&](int off, int cnt, int, int e) {
                   const double tleaf = tmax;
@@ -524,22 +524,19 @@ __device__ bool mesh_t(const dscene& S, int root, const v3& o, const v3& d, doub
                   for (int k = 0; k < cnt; k++) {
                       double tt;
                       if constexpr (CNT) wc.pbytes += 72u;
-                      if (tri_t(prim_ptr(S, off + k), o, d, tmin, tmax, tt) &&
-                          (alpha < 0 || alpha_keeps(S, alpha, mesh_first, off + k, o, d, mkey, serial))) {
-                          if (ok < 0) ok = leaf_exact<CNT>(S.blas, e, o, d, tmin, tleaf, wc) ? 1 : 0;
-                          if (!ok) break;
-                          any = true;
-                          tmax = tt;
-                          thit = tt;
-                          which = off + k;
-                          if constexpr (ANY) {  // the first hit answers an occlusion query: stop the walk
-                              tmax = -kInf;
-                              break;
-                          }
-                      }
+                      if (!tri_t(prim_ptr(S, off + k), o, d, tmin, tmax, tt)) continue;
+                      if constexpr (FEAT & F_UV)
+                          if (alpha >= 0 && !alpha_keeps(S, alpha, mesh_first, off + k, o, d, mkey, serial)) continue;
+                      if (ok < 0) ok = leaf_exact<CNT>(S.blas, e, o, d, tmin, tleaf, wc) ? 1 : 0;
+                      if (!ok) break;
+                      any = true;
+                      tmax = tt;
+                      which = off + k;
+                      if (any_hit) return true;  // the first hit answers an occlusion query: stop the walk
                   }
+                  return false;
               });
-    t = thit;
+    t = tmax;
     return any;
 }
 
@@ -573,10 +570,10 @@ __device__ bool medium_t(const dscene& S, const obj_rec& r, const v3& o, const v
 }
 
 // One world object, the ray in world space. Returns the hit t and primitive.
-template <bool CNT, int FEAT, bool ANY = false>
+template <bool CNT, int FEAT>
 __device__ __forceinline__ bool object_t(const dscene& S, int oi, v3 o, v3 d, double tmin, double tmax, double& t,
                                          int32_t& prim, tstack& st, int sp, bool media, uint64_t mkey,
-                                         uint32_t serial, wcount& wc) {
+                                         uint32_t serial, wcount& wc, bool any_hit) {
     if constexpr (!(FEAT & (F_XFORM | F_LIST | F_MESH | F_MEDIUM))) {
         // primitives only: the object is its primitive
         prim = oi;
@@ -602,8 +599,8 @@ __device__ __forceinline__ bool object_t(const dscene& S, int oi, v3 o, v3 d, do
         if (r.kind == MPT_OBJ_LIST) return list_t<CNT, FEAT>(S, r.first, r.count, o, d, tmin, tmax, t, prim, wc);
     if constexpr (FEAT & F_MESH)
         if (r.kind == MPT_OBJ_MESH)
-            return mesh_t<CNT, FEAT, ANY>(S, r.blas_root, o, d, tmin, tmax, t, prim, st, sp, wc, r.alpha, r.first,
-                                          mkey, serial);
+            return mesh_t<CNT, FEAT>(S, r.blas_root, o, d, tmin, tmax, t, prim, st, sp, wc, r.alpha, r.first, mkey,
+                                     serial, any_hit);
     if constexpr (FEAT & F_MEDIUM) {
         prim = -1;
         return medium_t<CNT, FEAT>(S, r, o, d, world_len, tmin, tmax, medium_draw(mkey, serial, uint32_t(oi)), t, wc);
@@ -656,41 +653,29 @@ __device__ __forceinline__ bool fog_t(const fog_rec& f, const v3& o, const v3& d
 // bvh::hit over the world (bvh.h:45-86), then the unbounded objects beside
 // it in order (main.cpp:70-74): the height fogs, the ground. media: test
 // constant_medium objects and fogs with draws from (mkey, serial); otherwise
-// they are skipped. ANY: an occlusion query, which only asks whether a hit
-// exists: the walk is the closest-hit walk up to its first accepted hit and
-// stops there. Every test the closest-hit walk makes before that hit is the
+// they are skipped. any_hit: an occlusion query, which only asks whether a
+// hit exists: the walk is the closest-hit walk up to its first accepted hit
+// and stops there (one walk for both, the flag uniform across the wave). Every test the closest-hit walk makes before that hit is the
 // same, and a hit it would accept later exists only if one was accepted
 // first (intervals only shrink on hits; media, fogs and masks draw by key,
 // not by order), so the answer is the closest-hit walk's; h is that first
 // hit, not the closest.
-template <bool CNT = false, int FEAT = F_ALL, bool ANY = false>
+template <bool CNT = false, int FEAT = F_ALL>
 __device__ bool trace_closest(const dscene& S, const v3& o, const v3& d, double tmin, double tmax, hitinfo& h,
-                              tstack& st, bool media, uint64_t mkey, uint32_t serial, wcount& wc) {
+                              tstack& st, bool media, uint64_t mkey, uint32_t serial, wcount& wc,
+                              bool any_hit = false) {
     h.obj = -1;
     h.prim = -1;
     h.t = tmax;
     bool any = false;
-    if (S.nobj == 0) {
-        if constexpr (FEAT & F_MEDIUM) {
-            if (media)
-                for (int k = 0; k < S.nfog; k++) {
-                    double tf;
-                    if (fog_t(S.fog[k], o, d, tmin, tmax, medium_draw(mkey, serial, uint32_t(S.nobj + 1 + k)), tf)) {
-                        tmax = tf, h.t = tf, h.obj = S.nobj + 1 + k, h.prim = -1, any = true;
-                    }
-                }
-        }
-        if (ground_t<FEAT>(S, o, d, tmin, tmax, h)) any = true;
-        return any;
-    }
-    walk4<CNT>(S.qtlas, S.tlas_root, o, d, tmin, tmax, st, 0, wc, [&](int off, int cnt, int sp, int e) {
+    if (S.nobj > 0) walk4<CNT>(S.qtlas, S.tlas_root, o, d, tmin, tmax, st, 0, wc, [&](int off, int cnt, int sp, int e) {
         const double tleaf = tmax;
         int ok = -1;  // the leaf's double test, made at its first hit
         for (int k = 0; k < cnt; k++) {
             int oi = S.titems[off + k];
             double tt;
             int32_t pr = -1;
-            if (object_t<CNT, FEAT, ANY>(S, oi, o, d, tmin, tmax, tt, pr, st, sp, media, mkey, serial, wc)) {
+            if (object_t<CNT, FEAT>(S, oi, o, d, tmin, tmax, tt, pr, st, sp, media, mkey, serial, wc, any_hit)) {
                 if (ok < 0) ok = leaf_exact<CNT>(S.tlas, e, o, d, tmin, tleaf, wc) ? 1 : 0;
                 if (!ok) break;
                 any = true;
@@ -698,17 +683,15 @@ __device__ bool trace_closest(const dscene& S, const v3& o, const v3& d, double 
                 h.t = tt;
                 h.obj = oi;
                 h.prim = pr;
-                if constexpr (ANY) {
-                    tmax = -kInf;
-                    break;
-                }
+                if (any_hit) return true;
             }
         }
+        return false;
     });
-    if constexpr (ANY)
-        if (any) return true;
-    // the objects beside the BVH, after it with the closest t so far (main.cpp:70-74)
-    if constexpr (FEAT & F_MEDIUM) {
+    if (any_hit && any) return true;
+    // the objects beside the BVH, after it with the closest t so far
+    // (main.cpp:70-74); only the variant that carries (u, v) has them
+    if constexpr (FEAT & F_UV) {
         if (media)
             for (int k = 0; k < S.nfog; k++) {
                 double tf;
@@ -716,8 +699,8 @@ __device__ bool trace_closest(const dscene& S, const v3& o, const v3& d, double 
                     tmax = tf, h.t = tf, h.obj = S.nobj + 1 + k, h.prim = -1, any = true;
                 }
             }
+        if (ground_t<FEAT>(S, o, d, tmin, tmax, h)) any = true;
     }
-    if (ground_t<FEAT>(S, o, d, tmin, tmax, h)) any = true;
     return any;
 }
 
@@ -753,8 +736,6 @@ __device__ __forceinline__ void finish_hit(const dscene& S, const hitinfo& h, co
             rec.normal = rec.front ? up : -up;
             return;
         }
-    }
-    if constexpr (FEAT & F_MEDIUM) {
         if (h.obj > S.nobj) {  // a height fog (medium.h:139-145)
             rec.t = h.t;
             rec.p = wo + h.t * wd;

@@ -622,9 +622,14 @@ __global__ __launch_bounds__(256) void k_camera(dcam C, pstate P, batch_args B, 
 }
 
 // camera::first_surface (camera.h:253-277) on the feature stream.
-template <int BS, int FEAT>
+// FIRST: the camera rays were extended already and their hit records (or the
+// miss mark) are in record 2, so the walk starts from them instead of tracing
+// the camera ray again; valid where a ray's closest hit does not depend on
+// the stream that traces it (no media, fogs or alpha masks).
+template <int BS, int FEAT, bool FIRST = false>
 __global__ __launch_bounds__(BS) void k_features(dscene S, launch_ctx L, pstate P, batch_args B) {
     STACK_SETUP();
+    constexpr bool kUV = (FEAT & F_UV) != 0;
     for (int64_t s = st.gtid; s < B.nslots; s += L.nthreads) {
         uint32_t slot = uint32_t(s);
         v3 o = ld3(P, D_OX, slot), d = ld3(P, D_DX, slot);
@@ -633,14 +638,29 @@ __global__ __launch_bounds__(BS) void k_features(dscene S, launch_ctx L, pstate 
         v3 albedo = tint, normal{0, 0, 0};
         bool done = false;
         for (int bounce = 0; bounce < 8 && !done; bounce++) {
-            hitinfo h;
-            wcount wc{0, 0};
-            if (!trace_closest<false, FEAT>(S, o, d, 0.001, kInf, h, st, true, rs.key, uint32_t(bounce), wc)) break;
             hitrec rec;
-            finish_hit<FEAT>(S, h, o, d, rec);
+            if (FIRST && bounce == 0) {  // the extension's record of the camera ray's closest hit
+                rec.t = pd(P, D_HT, slot);
+                if (rec.t < 0) break;
+                rec.p = ld3(P, D_QX, slot);
+                rec.normal = ld3(P, D_MX, slot);
+                const uint32_t hm = uint32_t(as_bits(pd(P, D_HMAT, slot)));
+                rec.mat = int32_t(hm & 0x7fffffffu);
+                rec.front = (hm >> 31) != 0;
+                if (kUV && S.mat[rec.mat].uses_uv) {
+                    rec.u = pd(P, D_U, slot);
+                    rec.v = pd(P, D_V, slot);
+                    rec.dpdu = ld3(P, D_DUX, slot);
+                    rec.dpdv = ld3(P, D_DVX, slot);
+                }
+            } else {
+                hitinfo h;
+                wcount wc{0, 0};
+                if (!trace_closest<false, FEAT>(S, o, d, 0.001, kInf, h, st, true, rs.key, uint32_t(bounce), wc)) break;
+                finish_hit<FEAT>(S, h, o, d, rec);
+            }
             const mpt_material& m = S.mat[rec.mat];
             tint = tint * mat_interior(m, d, rec);
-            constexpr bool kUV = (FEAT & F_UV) != 0;
             hitrec shaded = rec;  // the guide keeps the hit's own normal
             if (kUV && m.normal_map >= 0) shaded.normal = normal_from_map(S, m.normal_map, d, rec);
             scatter_out so = mat_scatter<-1, kUV>(S, m, d, shaded, rs, &rec.normal);
@@ -696,7 +716,10 @@ __device__ __forceinline__ int extend_one(const dscene& S, const pstate& P, int 
     const v3 o = ld3(P, D_OX, p), d = ld3(P, D_DX, p);
     hitinfo h;
     trace_closest<CNT, FEAT>(S, o, d, 0.001, kInf, h, st, true, P.key[p], uint32_t(2 * depth), wc);
-    if (h.obj < 0) return 0;
+    if (h.obj < 0) {
+        if (depth == 0) pd(P, D_HT, p) = -1.0;  // the camera ray's miss, for the guides' walk
+        return 0;
+    }
     // the hit record, every double of it (t first: it need not stay live through finish_hit)
     pd(P, D_HT, p) = h.t;
     hitrec rec;
@@ -949,20 +972,18 @@ __device__ __forceinline__ void shadow_one(const dscene& S, const dcam& C, const
         v3 dir = unit_vector(to_light);
         v3 through{1, 1, 1};
         v3 emission;
+        // nothing emits and nothing lets shadows through: whatever the ray hits
+        // first gives no emission, so only whether it hits anything matters
+        const bool occlusion = S.shadow_occlusion != 0;
         for (int k = 0;; k++) {  // camera.h:488-502
             hitinfo h;
             const uint32_t serial = uint32_t(2 * depth + 1) | (uint32_t(k) << 20);
-            if (S.shadow_occlusion) {
-                // nothing emits and nothing lets it through: whatever the ray hits
-                // first gives no emission, so only whether it hits anything matters
-                if (!trace_closest<CNT, FEAT, true>(S, o, dir, 0.001, kInf, h, st, true, P.key[p], serial, wc))
-                    emission = escaped<ENV>(S, C, to_light, false);
-                else
-                    emission = v3{0, 0, 0};
+            if (!trace_closest<CNT, FEAT>(S, o, dir, 0.001, kInf, h, st, true, P.key[p], serial, wc, occlusion)) {
+                emission = escaped<ENV>(S, C, to_light, false);
                 break;
             }
-            if (!trace_closest<CNT, FEAT>(S, o, dir, 0.001, kInf, h, st, true, P.key[p], serial, wc)) {
-                emission = escaped<ENV>(S, C, to_light, false);
+            if (occlusion) {
+                emission = v3{0, 0, 0};
                 break;
             }
             hitrec rec;
@@ -1305,6 +1326,10 @@ struct mpt_scene {
     // kernels compiled for this scene: geometry profile, material types
     // present (bit t for type t), whether next event estimation runs
     int profile = 0;
+    // the closest hit of a ray depends on the path's key (media and height
+    // fogs draw free flights, alpha masks draw coverage): a path's first hit
+    // cannot stand in for the guides' own
+    bool keyed_hits = false;
     uint32_t mat_types = 0;
     bool has_lights = false;
     int light_profile = 0;  // compiled NEE variant: 1 one-primitive quads, 2 environment only, 3 all
@@ -1557,14 +1582,17 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
         }
         for (int32_t k : h.prim_kind)
             if (k == MPT_PRIM_TRIANGLE) quads_ok = false;
-        // 3: everything, hits carry (u, v) and tangents, shading has textures, maps, thin glass, measured metals
-        // height fogs are media: only the general variants carry them
-        s->profile = h.rich ? 3 : (!h.fogs.empty() ? 0 : (prims_only && !h.objects.empty() ? 1 : (quads_ok ? 2 : 0)));
+        // 3: everything, hits carry (u, v) and tangents, shading has textures, maps, thin glass, measured
+        // metals, and the objects beside the BVH (ground, height fogs) and alpha masks are tested
+        s->profile = h.rich ? 3 : (prims_only && !h.objects.empty() ? 1 : (quads_ok ? 2 : 0));
         for (int32_t m : h.prim_mat)
             if (m >= 0) s->mat_types |= 1u << h.materials[m].type;
         for (const auto& r : h.objects)
             if (r.kind == MPT_OBJ_MEDIUM) s->mat_types |= 1u << MPT_MAT_PHASE;
         if (!h.fogs.empty()) s->mat_types |= 1u << MPT_MAT_PHASE;  // fog hits shade as a phase material
+        s->keyed_hits = !h.fogs.empty();
+        for (const auto& r : h.objects)
+            if (r.kind == MPT_OBJ_MEDIUM || r.alpha != -1) s->keyed_hits = true;
         s->has_lights = !h.lights.empty();
         bool quads = true, env_only = true;
         for (const auto& L : h.lights) {
@@ -2246,15 +2274,22 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
                                  q[0], cnt, int(s->has_lights), MB));
         // passes of this batch whose albedo and normal guides are traced (the leading ones)
         const int guide_nb = o.guide_passes > 0 ? std::max(0, std::min(nb, o.guide_passes - (o.first_pass + pass))) : nb;
-        if (C.features && guide_nb > 0) {
-            batch_args BG = B;
-            BG.nslots = guide_nb * npix;  // slots run pass-major
-            // the guides' walk, compiled for the scene's geometry as the extension kernels are
-            if (s->profile == 1) EV(3, hipLaunchKernelGGL((k_features<256, F_SPHERE>), dim3(G), dim3(256), 0, stream, DS, L, P, BG));
-            else if (s->profile == 2) EV(3, hipLaunchKernelGGL((k_features<256, kQuadProfile>), dim3(G), dim3(256), 0, stream, DS, L, P, BG));
-            else if (s->profile == 3) EV(3, hipLaunchKernelGGL((k_features<256, F_ALL | F_UV>), dim3(G), dim3(256), 0, stream, DS, L, P, BG));
-            else EV(3, hipLaunchKernelGGL((k_features<256, F_ALL>), dim3(G), dim3(256), 0, stream, DS, L, P, BG));
-        }
+        batch_args BG = B;
+        BG.nslots = guide_nb * npix;  // slots run pass-major
+        // the guides' walk, compiled for the scene's geometry as the extension kernels are
+        auto launch_features = [&](auto first) {
+            constexpr bool F = decltype(first)::value;
+            if (s->profile == 1) EV(3, hipLaunchKernelGGL((k_features<256, F_SPHERE, F>), dim3(G), dim3(256), 0, stream, DS, L, P, BG));
+            else if (s->profile == 2) EV(3, hipLaunchKernelGGL((k_features<256, kQuadProfile, F>), dim3(G), dim3(256), 0, stream, DS, L, P, BG));
+            else if (s->profile == 3) EV(3, hipLaunchKernelGGL((k_features<256, F_ALL | F_UV, F>), dim3(G), dim3(256), 0, stream, DS, L, P, BG));
+            else EV(3, hipLaunchKernelGGL((k_features<256, F_ALL, F>), dim3(G), dim3(256), 0, stream, DS, L, P, BG));
+            return int(MPT_OK);  // EV returns its error, the events already freed
+        };
+        // where the camera rays' closest hits do not depend on the stream, the
+        // guides start from the first extension's records (after it, before
+        // shading rewrites the rays); else they trace the camera rays themselves
+        const bool guides_after_extend = sorted && !graphs && !s->keyed_hits && C.max_depth > 0;
+        if (C.features && guide_nb > 0 && !guides_after_extend && (rc = launch_features(std::false_type{}))) return rc;
         if (!sorted) EV(4, launch_finish(0, q[0], ctr(cnt, 0)));  // every bounce in one kernel, no binning
         for (int depth = 0; sorted && depth < C.max_depth; depth++) {
             uint32_t* qin = q[depth & 1];
@@ -2286,6 +2321,9 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
             // extend zeroes the next-queue and shadow counters of this bounce;
             // shadow zeroes the class counters once every shade kernel read them
             EV(0, launch_extend(depth, qin, cin, cout));
+            if (depth == 0 && C.features && guide_nb > 0 && guides_after_extend &&
+                (rc = launch_features(std::true_type{})))
+                return rc;
             if (nrc && depth == R.depth) {  // the cache's inputs, then paths not training end with its radiance
                 hipLaunchKernelGGL((k_nrc_prep<256>), dim3(G), dim3(256), 0, stream, DS, P, R.q, R.cnt, R, NB);
                 hipLaunchKernelGGL(k_nrc_record, dim3(unsigned((NB.tcap + 255) / 256)), dim3(256), 0, stream, P, R.q,
@@ -2410,8 +2448,9 @@ int render_batch_impl(mpt_scene* const* ss, const mpt_camera* cams, int n, const
             return set_error(MPT_E_INVALID, "render batch: bad camera settings for scene " + std::to_string(k));
         if (c->collect_features) return set_error(MPT_E_UNSUPPORTED, "render batch: feature guides are not supported");
         if (sk->profile == 3)
-            return set_error(MPT_E_UNSUPPORTED, "render batch: scenes with image textures, maps, thin glass, measured "
-                                                "metals or the ground plane render one at a time");
+            return set_error(MPT_E_UNSUPPORTED, "render batch: scenes with image textures, maps, alpha masks, thin "
+                                                "glass, measured metals, the ground plane or height fogs render one "
+                                                "at a time");
         if (sk->device != s->device) return set_error(MPT_E_INVALID, "render batch: scenes on different devices");
         if (sk->light_profile != s->light_profile || (sk->hs.env_w > 0) != (s->hs.env_w > 0))
             return set_error(MPT_E_UNSUPPORTED, "render batch: scenes must share their light setup (same kind of "

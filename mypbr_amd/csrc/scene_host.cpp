@@ -452,6 +452,7 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         if (m < 0 || m >= d.n_materials || d.materials[m].type != MPT_MAT_PHASE)
             return fail(MPT_E_INVALID, "a height fog needs a volume material");
         s.fogs.push_back(fog_rec{q[0], q[1], q[2], m, 0});
+        s.rich = true;  // tested beside the BVH, as the ground is, by the variant that carries (u, v)
     }
     auto tex_ok = [&](int32_t t) { return t >= -1 && t < d.n_textures; };
     auto is_image = [&](int32_t t) { return t >= 0 && s.textures[size_t(t)].type == MPT_TEX_IMAGE; };
@@ -594,6 +595,7 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         r.pad = 0;
         next_id = r.id_base + r.count;
         if (r.alpha != -1) {
+            s.rich = true;  // a mask is read at the hit's (u, v): the variant that carries it tests masks
             if (r.kind != MPT_OBJ_MESH) return fail(MPT_E_INVALID, "only meshes take alpha masks");
             if (r.alpha < 0 || r.alpha >= d.n_textures || d.textures[r.alpha].type != MPT_TEX_IMAGE ||
                 (d.textures[r.alpha].flags & MPT_TEXF_HDR))

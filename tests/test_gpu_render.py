@@ -241,3 +241,21 @@ def test_guide_passes_leave_radiance_and_variance_unchanged(mpt):
     assert np.array_equal(part.radiance, full.radiance) and np.array_equal(part.variance, full.variance)
     assert np.array_equal(part.albedo, few.albedo) and np.array_equal(part.normal, few.normal)
     assert not np.array_equal(part.albedo, full.albedo)
+
+
+@pytest.mark.parametrize("scene", ["weekend_cover.scene", "cornell.scene", "textured.scene", "obj_room.scene"])
+def test_guides_from_the_first_extension_equal_their_own_trace(mpt, scene):
+    """Where a camera ray's closest hit does not depend on the stream that
+    traces it, the guides start from the first extension's hit records
+    instead of tracing the camera rays again. The guides must equal, bit for
+    bit, those of a render that traces them itself (hipGraph bounces, where
+    the guides run before the first bounce), for each geometry profile."""
+    from mypbr_amd.abi import camera_copy
+    d = mpt.describe(scene_path(scene))
+    s = mpt.Scene.from_desc(d.desc, d.camera)
+    cam = camera_copy(d.camera, image_width=64, samples_per_pixel=6, max_depth=8, collect_features=1)
+    fused = s.render(cam, seed=9, passes_in_flight=3)
+    traced = s.render(cam, seed=9, passes_in_flight=3, use_graph=1)
+    assert np.array_equal(fused.radiance, traced.radiance)
+    assert np.array_equal(fused.albedo, traced.albedo) and np.array_equal(fused.normal, traced.normal)
+    assert np.abs(fused.normal).sum() > 0
