@@ -43,7 +43,9 @@ enum {
 /* ---------------------------------------------------------------------- */
 
 /* Primitive kinds (geometry a leaf tests). */
-enum { MPT_PRIM_SPHERE = 0, MPT_PRIM_QUAD = 1, MPT_PRIM_TRIANGLE = 2 };
+enum { MPT_PRIM_SPHERE = 0, MPT_PRIM_QUAD = 1, MPT_PRIM_TRIANGLE = 2,
+       MPT_PRIM_VOXBLOCK = 3 /* a block of a voxel model (voxel_grid, voxel.h:168-337): prim_data lo xyz,
+                                hi xyz (cells [lo, hi), whole numbers), the model's index, 2 unused */ };
 
 /* Object kinds: one entry of the world list the top-level BVH is built on
  * (scene.h:406-420 adds each scene line as one such object). */
@@ -51,8 +53,30 @@ enum {
     MPT_OBJ_PRIM = 0,   /* a single primitive (sphere.h, quad.h) */
     MPT_OBJ_LIST = 1,   /* hittable_list tested linearly (hittable_list.h:25-39), e.g. box() */
     MPT_OBJ_MESH = 2,   /* triangle mesh with its own BVH (mesh.h:175-191) */
-    MPT_OBJ_MEDIUM = 3  /* constant_medium over a list/sphere boundary (medium.h:60-88) */
+    MPT_OBJ_MEDIUM = 3, /* constant_medium over a list/sphere boundary (medium.h:60-88) */
+    MPT_OBJ_VOXEL = 4   /* voxel_blocks (voxel.h:341-382): voxel blocks of one model with their own BVH */
 };
+
+/* object_flags bits. */
+enum {
+    MPT_OBJF_LIGHT_ONLY = 1  /* sampled as a light, never hit: not in the world BVH (the emissive faces
+                                of a voxel model, which the model's grid is hit for; scene.h:1118-1128) */
+};
+
+/* A voxel model (vox_model, vox_loader.h:32-44) and its palette in the
+ * scene's materials (voxel_palette, voxel.h:23-31): cells are palette
+ * indices, x fastest, then y, then z, 0 = empty; kind[i] is 0 empty, 1
+ * opaque, 2 transparent (voxel_kind, voxel.h:16-21); material[i] the
+ * material of index i (-1 where no cell uses it). With edge_width > 0 the
+ * outlines of its shapes are drawn with edge_material (voxel.h:290-311). */
+typedef struct mpt_vox_model {
+    int32_t nx, ny, nz;
+    int32_t edge_material;         /* -1: no outlines */
+    int64_t cells;                 /* offset of cell (0, 0, 0) in vox_cells */
+    double edge_width;             /* in voxels, 0 or (0, 0.5) */
+    int32_t material[256];
+    uint8_t kind[256];
+} mpt_vox_model;
 
 /* Material types (material.h, medium.h). */
 enum {
@@ -240,6 +264,18 @@ typedef struct mpt_scene_desc {
      * uniform draw falls below c. -1: opaque. NULL: no object has one. Only
      * mesh objects may have one. */
     const int32_t* object_alpha;   /* [n_objects] */
+
+    /* Per object, MPT_OBJF_* bits; NULL: all 0. */
+    const int32_t* object_flags;   /* [n_objects] */
+
+    /* Voxel models, read by MPT_PRIM_VOXBLOCK primitives and MPT_OBJ_VOXEL
+     * objects (a block's model index names one). The grids are walked only by
+     * the kernels that carry (u, v) and the ground plane. */
+    int32_t n_vox_models;
+    int32_t vox_pad;
+    const mpt_vox_model* vox_models;  /* [n_vox_models] */
+    const uint8_t* vox_cells;         /* every model's cells */
+    int64_t n_vox_cells;
 } mpt_scene_desc;
 
 /* Camera and render settings: the public fields of the reference camera

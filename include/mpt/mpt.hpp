@@ -29,6 +29,7 @@
 // model make render() throw std::runtime_error instead of falling back.
 #pragma once
 
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <functional>
@@ -612,6 +613,84 @@ public:
     std::shared_ptr<material> phase;
 };
 
+// ------------------------------------------------------------------ voxels
+// MagicaVoxel .vox files (vox_loader.h) and voxel grids walked with a 3D DDA
+// (voxel.h). One voxel is one unit; cell (x, y, z) covers [x, x+1] x
+// [y, y+1] x [z, z+1] of its model; files are z-up.
+
+// A dense grid of palette indices, x fastest, then y, then z; 0 is empty.
+struct vox_model {
+    int nx = 0, ny = 0, nz = 0;
+    std::vector<uint8_t> cells;
+    uint8_t at(int x, int y, int z) const {  // 0 outside the model
+        if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return 0;
+        return cells[(size_t(z) * ny + y) * nx + x];
+    }
+};
+// A model placed by the file's scene graph: xform takes model coordinates to
+// the file's (still z-up) world.
+struct vox_instance {
+    int model = 0;
+    transform xform;
+};
+struct vox_file {
+    std::vector<vox_model> models;
+    std::vector<vox_instance> instances;
+    color palette[256];                                 // sRGB-encoded, in [0, 1], as stored
+    std::map<std::string, std::string> materials[256];  // MATL properties per palette index
+};
+// load_vox (vox_loader.h:176-327): versions 150 and 200; SIZE/XYZI models,
+// RGBA (MagicaVoxel's default palette without one), MATL, and the
+// nTRN/nGRP/nSHP scene graph (first animation frame; hidden nodes skipped).
+bool load_vox(const std::string& path, vox_file& out, std::string& error);
+
+// How a palette index takes part in surfaces (voxel.h:16-21).
+enum class voxel_kind : uint8_t { empty, opaque, transparent };
+// A surface is where two neighbouring cells differ in kind (voxel.h:103-111).
+bool voxel_face_visible(voxel_kind k, voxel_kind n);
+
+// The 256 palette entries as materials (make_voxel_palette, voxel.h:49-99):
+// _diffuse lambertian, _metal principled, _glass tinted dielectric
+// (transparent), _emit diffuse light; colours sRGB-decoded.
+struct voxel_palette {
+    std::array<std::shared_ptr<material>, 256> materials;
+    std::array<voxel_kind, 256> kinds;
+    std::array<bool, 256> emissive;
+    voxel_kind kind(uint8_t index) const { return kinds[index]; }
+};
+voxel_palette make_voxel_palette(const vox_file& vox, double emission_scale);
+
+// The visible faces of a model as quads with their cells' materials; the
+// faces of emissive cells also go to emitters (voxel_model_faces, voxel.h:113-152).
+void voxel_model_faces(const vox_model& m, const voxel_palette& pal, hittable_list& faces, hittable_list& emitters);
+
+// The cells [lo, hi) of a model, walked cell by cell with a 3D DDA
+// (voxel_grid, voxel.h:168-337); with an edge material the outlines of the
+// model's shapes are drawn with it, edge_width wide (in voxels).
+class voxel_grid : public hittable {
+public:
+    voxel_grid(std::shared_ptr<const vox_model> model, std::shared_ptr<const voxel_palette> palette, const int lo[3],
+               const int hi[3], std::shared_ptr<material> edge_material = nullptr, double edge_width = 0);
+    void flatten(flat_builder& b, int32_t xform) const override;
+    std::shared_ptr<const vox_model> model;
+    std::shared_ptr<const voxel_palette> palette;
+    std::shared_ptr<material> edge_material;
+    double edge_width;
+    int lo[3], hi[3];
+};
+// A model as voxel_grid blocks of up to block^3 cells, each shrunk to the
+// box of its voxels, blocks without voxels skipped, in one BVH (voxel_blocks,
+// voxel.h:339-382). On the GPU path the blocks are one object with its own
+// BVH, instanced by instance like a mesh.
+class voxel_block_set : public hittable {
+public:
+    void flatten(flat_builder& b, int32_t xform) const override;
+    std::vector<std::shared_ptr<voxel_grid>> blocks;
+};
+std::shared_ptr<hittable> voxel_blocks(std::shared_ptr<const vox_model> model,
+                                       std::shared_ptr<const voxel_palette> palette, int block = 8,
+                                       std::shared_ptr<material> edge_material = nullptr, double edge_width = 0);
+
 struct scene {  // scene.h:32-40
     hittable_list world;
     hittable_list unbounded;  // objects without a finite box (height fogs, then a ground plane)
@@ -644,6 +723,9 @@ public:
     void set_ground(double height, int32_t material);       // the one ground plane
     void add_fog(double density, double scale_height, double base, int32_t material);  // a height fog
     int32_t add_object(int32_t kind, int64_t first, int64_t count, int32_t xform, const hittable* owner);
+    // a voxel model with its palette and edges in the description's tables (once per combination)
+    int32_t vox_model_index(const vox_model& m, const voxel_palette& pal, const std::shared_ptr<material>& edge_material,
+                            double edge_width);
     std::map<const void*, std::pair<int64_t, int64_t>> mesh_ranges;
     struct impl;
     impl* p;

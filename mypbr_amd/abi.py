@@ -12,8 +12,10 @@ MPT_E_HIP = -3
 MPT_E_IO = -4
 MPT_E_CAPACITY = -5
 
-PRIM_SPHERE, PRIM_QUAD, PRIM_TRIANGLE = 0, 1, 2
-OBJ_PRIM, OBJ_LIST, OBJ_MESH, OBJ_MEDIUM = 0, 1, 2, 3
+PRIM_SPHERE, PRIM_QUAD, PRIM_TRIANGLE, PRIM_VOXBLOCK = 0, 1, 2, 3
+OBJ_PRIM, OBJ_LIST, OBJ_MESH, OBJ_MEDIUM, OBJ_VOXEL = 0, 1, 2, 3, 4
+OBJF_LIGHT_ONLY = 1
+VOX_EMPTY, VOX_OPAQUE, VOX_TRANSPARENT = 0, 1, 2
 MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_PRINCIPLED, MAT_DIFFUSE_LIGHT, MAT_PHASE = range(6)
 TEX_SOLID, TEX_CHECKER, TEX_IMAGE = 0, 1, 2
 TEXF_LINEAR, TEXF_REPEAT, TEXF_HDR, TEXF_NO_ALPHA = 1, 2, 4, 8
@@ -40,6 +42,12 @@ class Material(C.Structure):
 
 class Xform(C.Structure):
     _fields_ = [("m", C.c_double * 9), ("t", D3), ("inv_m", C.c_double * 9), ("inv_t", D3)]
+
+
+class VoxModel(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("edge_material", C.c_int32),
+                ("cells", C.c_int64), ("edge_width", C.c_double), ("material", C.c_int32 * 256),
+                ("kind", C.c_uint8 * 256)]
 
 
 class SceneDesc(C.Structure):
@@ -94,6 +102,12 @@ class SceneDesc(C.Structure):
         ("fog_params", C.POINTER(C.c_double)),
         ("fog_material", C.POINTER(C.c_int32)),
         ("object_alpha", C.POINTER(C.c_int32)),
+        ("object_flags", C.POINTER(C.c_int32)),
+        ("n_vox_models", C.c_int32),
+        ("vox_pad", C.c_int32),
+        ("vox_models", C.POINTER(VoxModel)),
+        ("vox_cells", C.POINTER(C.c_uint8)),
+        ("n_vox_cells", C.c_int64),
     ]
 
 

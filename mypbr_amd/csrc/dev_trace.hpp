@@ -76,6 +76,9 @@ struct dscene {
     // 1 when no material emits or lets a shadow ray through: a shadow ray then
     // only asks whether anything is in the way (an occlusion query)
     int32_t shadow_occlusion;
+    int32_t nworld;      // objects in the top-level BVH (the others are only lights)
+    const mpt_vox_model* vox;  // voxel models and their cells
+    const uint8_t* vcells;
     int64_t ground_id;   // its canonical ID: one past the last primitive's
     croot tlas_root;
     // the sun (sun.h): on, its frame (w towards the sun), cone and radiance
@@ -569,6 +572,206 @@ __device__ bool medium_t(const dscene& S, const obj_rec& r, const v3& o, const v
     return true;
 }
 
+// ---- voxel grids (voxel_grid, voxel.h:126-337)
+// Cells of a block are addressed as three scalars, and axes picked with
+// selects, so that nothing on the walk is an array indexed at run time
+// (which would live in scratch memory).
+
+__device__ __forceinline__ int isel(int x, int y, int z, int a) { return a == 0 ? x : (a == 1 ? y : z); }
+__device__ __forceinline__ double dsel(const v3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+
+// The palette index of cell (x, y, z) of a model, 0 outside it (voxel.h:276-281).
+__device__ __forceinline__ int vox_at(const mpt_vox_model& m, const uint8_t* cells, int x, int y, int z) {
+    if (unsigned(x) >= unsigned(m.nx) || unsigned(y) >= unsigned(m.ny) || unsigned(z) >= unsigned(m.nz)) return 0;
+    return cells[(int64_t(z) * m.ny + y) * m.nx + x];
+}
+
+// Whether the face of a cell of kind k towards a neighbour of kind n is a
+// surface (voxel_face_visible, voxel.h:103-111): 0 empty, 1 opaque, 2
+// transparent.
+__device__ __forceinline__ bool vox_face(int k, int n) { return k != 0 && (n == 0 || (k == 1 && n == 2)); }
+
+// The face a walk stopped on: the plane perpendicular to axis, the outward
+// normal's sign along it, the palette index and cell of the voxel it bounds.
+struct vox_hit {
+    int axis, sign, index;
+    int x, y, z;
+};
+
+// voxel_grid::hit (voxel.h:193-267) over block q (lo xyz, hi xyz, model) for
+// the ray (o, d) in model space: a 3D DDA (Amanatides and Woo 1987) from
+// where the ray enters the block (or from tmin inside it), stopping at the
+// first plane crossing where the two cells differ in kind (crossing,
+// voxel.h:283-298), within tmax. The entry, the cell clamping and the plane
+// times follow the reference's arithmetic, so t is bitwise the reference's.
+// With REC the face is written to f.
+template <bool REC>
+__device__ bool vox_walk(const dscene& S, const double* q, const v3& o, const v3& d, double tmin, double tmax,
+                         double& t_hit, vox_hit* f) {
+    const int lo0 = int(q[0]), lo1 = int(q[1]), lo2 = int(q[2]);
+    const int hi0 = int(q[3]), hi1 = int(q[4]), hi2 = int(q[5]);
+    const mpt_vox_model& m = S.vox[int(q[6])];
+    const uint8_t* cells = S.vcells + m.cells;
+    const uint8_t* kind = m.kind;
+    // where the ray's line enters and leaves the block
+    double t_enter = -kInf, t_exit = kInf;
+    int enter_axis = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const double oa = dsel(o, a), da = dsel(d, a);
+        const int la = a == 0 ? lo0 : (a == 1 ? lo1 : lo2), ha = a == 0 ? hi0 : (a == 1 ? hi1 : hi2);
+        if (da == 0) {
+            if (oa < la || oa > ha) return false;
+            continue;
+        }
+        double t0 = (la - oa) / da, t1 = (ha - oa) / da;
+        if (t0 > t1) {
+            const double tt = t0;
+            t0 = t1;
+            t1 = tt;
+        }
+        if (t0 > t_enter) {
+            t_enter = t0;
+            enter_axis = a;
+        }
+        t_exit = std::fmin(t_exit, t1);
+    }
+    const bool outside = t_enter >= tmin;
+    const double t_start = outside ? t_enter : tmin;
+    if (t_start > std::fmin(t_exit, tmax)) return false;
+    // the first cell and the DDA state per axis: step, next plane's t, t between planes
+    int c[3], stp[3];
+    double tn[3], td[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const double oa = dsel(o, a), da = dsel(d, a);
+        const int la = a == 0 ? lo0 : (a == 1 ? lo1 : lo2), ha = a == 0 ? hi0 : (a == 1 ? hi1 : hi2);
+        const double sa = oa + t_start * da;  // ray::at
+        int ca = int(std::floor(sa));
+        ca = ca < la ? la : (ca > ha - 1 ? ha - 1 : ca);
+        if (outside && a == enter_axis) ca = da > 0 ? la : ha - 1;  // the layer just inside the entry face
+        c[a] = ca;
+        if (da > 0) {
+            stp[a] = 1;
+            tn[a] = (ca + 1 - oa) / da;
+            td[a] = 1 / da;
+        } else if (da < 0) {
+            stp[a] = -1;
+            tn[a] = (ca - oa) / da;
+            td[a] = -1 / da;
+        } else {
+            stp[a] = 0;
+            tn[a] = td[a] = kInf;
+        }
+    }
+    // crossing (voxel.h:283-298): into cell (x, y, z) of palette index to from
+    // one of index from, across the plane of axis a at step s
+    auto crossing = [&](double t, int a, int s, int from, int to, int x, int y, int z) {
+        const int kf = kind[from], kt = kind[to];
+        if (kt != 0 && vox_face(kt, kf)) {
+            if (REC) *f = vox_hit{a, -s, to, x, y, z};
+            t_hit = t;
+            return true;
+        }
+        if (vox_face(kf, kt)) {
+            if (REC) *f = vox_hit{a, s, from, a == 0 ? x - s : x, a == 1 ? y - s : y, a == 2 ? z - s : z};
+            t_hit = t;
+            return true;
+        }
+        return false;
+    };
+    int index = vox_at(m, cells, c[0], c[1], c[2]);
+    if (outside) {  // from the cell behind the entry face
+        const int s = isel(stp[0], stp[1], stp[2], enter_axis);
+        const int bx = enter_axis == 0 ? c[0] - s : c[0], by = enter_axis == 1 ? c[1] - s : c[1];
+        const int bz = enter_axis == 2 ? c[2] - s : c[2];
+        if (crossing(t_enter, enter_axis, s, vox_at(m, cells, bx, by, bz), index, c[0], c[1], c[2])) return true;
+    }
+    // every step moves one cell towards the block's exit, so the walk ends;
+    // a nearest plane without a step (a zero or NaN direction component, t
+    // infinite) ends it too, as no hit: the reference would loop there
+    while (true) {
+        // the nearest plane ahead decides the next cell
+        const int a = tn[0] < tn[1] ? (tn[0] < tn[2] ? 0 : 2) : (tn[1] < tn[2] ? 1 : 2);
+        double t;
+        int s;
+        bool out;
+        if (a == 0) {
+            t = tn[0], s = stp[0];
+            if (t > tmax || s == 0) return false;
+            c[0] += s;
+            tn[0] += td[0];
+            out = c[0] < lo0 || c[0] >= hi0;
+        } else if (a == 1) {
+            t = tn[1], s = stp[1];
+            if (t > tmax || s == 0) return false;
+            c[1] += s;
+            tn[1] += td[1];
+            out = c[1] < lo1 || c[1] >= hi1;
+        } else {
+            t = tn[2], s = stp[2];
+            if (t > tmax || s == 0) return false;
+            c[2] += s;
+            tn[2] += td[2];
+            out = c[2] < lo2 || c[2] >= hi2;
+        }
+        const int next = vox_at(m, cells, c[0], c[1], c[2]);  // 0 outside the model
+        if (crossing(t, a, s, index, next, c[0], c[1], c[2])) return true;
+        if (out) return false;  // left the block
+        index = next;
+    }
+}
+
+// on_edge (voxel.h:300-322): whether a hit at p on the face of cell (x, y,
+// z) whose outward normal runs along axis (sign) lies within the model's
+// edge width of an outline: a convex edge (the face around the corner is a
+// surface too) or a concave one (a voxel stands beside, in front).
+__device__ __forceinline__ bool vox_on_edge(const dscene& S, const mpt_vox_model& m, const v3& p, const vox_hit& f) {
+    const uint8_t* cells = S.vcells + m.cells;
+    const int k = m.kind[vox_at(m, cells, f.x, f.y, f.z)];
+    for (int side = 1; side <= 2; side++) {
+        const int tangent = (f.axis + side) % 3;
+        const double fr = dsel(p, tangent) - isel(f.x, f.y, f.z, tangent);
+        for (int dir = -1; dir <= 1; dir += 2) {
+            if ((dir > 0 ? 1 - fr : fr) >= m.edge_width) continue;
+            const int bx = f.x + (tangent == 0 ? dir : 0), by = f.y + (tangent == 1 ? dir : 0);
+            const int bz = f.z + (tangent == 2 ? dir : 0);
+            const int ax = bx + (f.axis == 0 ? f.sign : 0), ay = by + (f.axis == 1 ? f.sign : 0);
+            const int az = bz + (f.axis == 2 ? f.sign : 0);
+            if (vox_face(k, m.kind[vox_at(m, cells, bx, by, bz)]) || m.kind[vox_at(m, cells, ax, ay, az)] != 0)
+                return true;
+        }
+    }
+    return false;
+}
+
+// voxel_blocks' BVH (voxel.h:341-382): the model's blocks, each walked by
+// vox_walk; the stack continues above sp_base. any_hit: stop at the first
+// accepted hit.
+template <bool CNT, int FEAT>
+__device__ bool voxel_t(const dscene& S, int root, const v3& o, const v3& d, double tmin, double tmax, double& t,
+                        int32_t& which, tstack& st, int sp_base, wcount& wc, bool any_hit) {
+    bool any = false;
+    walk4<CNT>(S.qblas, S.blas_roots[root], o, d, tmin, tmax, st, sp_base, wc, [&](int off, int cnt, int, int e) {
+        const double tleaf = tmax;
+        int ok = -1;  // the leaf's double test, made at its first hit
+        for (int k = 0; k < cnt; k++) {
+            double tt;
+            if constexpr (CNT) wc.pbytes += 56u;
+            if (!vox_walk<false>(S, prim_ptr(S, off + k), o, d, tmin, tmax, tt, nullptr)) continue;
+            if (ok < 0) ok = leaf_exact<CNT>(S.blas, e, o, d, tmin, tleaf, wc) ? 1 : 0;
+            if (!ok) break;
+            any = true;
+            tmax = tt;
+            which = off + k;
+            if (any_hit) return true;
+        }
+        return false;
+    });
+    t = tmax;
+    return any;
+}
+
 // One world object, the ray in world space. Returns the hit t and primitive.
 template <bool CNT, int FEAT>
 __device__ __forceinline__ bool object_t(const dscene& S, int oi, v3 o, v3 d, double tmin, double tmax, double& t,
@@ -597,6 +800,9 @@ __device__ __forceinline__ bool object_t(const dscene& S, int oi, v3 o, v3 d, do
     }
     if constexpr (FEAT & F_LIST)
         if (r.kind == MPT_OBJ_LIST) return list_t<CNT, FEAT>(S, r.first, r.count, o, d, tmin, tmax, t, prim, wc);
+    if constexpr (FEAT & F_UV)  // voxel scenes run the variant that carries (u, v)
+        if (r.kind == MPT_OBJ_VOXEL)
+            return voxel_t<CNT, FEAT>(S, r.blas_root, o, d, tmin, tmax, t, prim, st, sp, wc, any_hit);
     if constexpr (FEAT & F_MESH)
         if (r.kind == MPT_OBJ_MESH)
             return mesh_t<CNT, FEAT>(S, r.blas_root, o, d, tmin, tmax, t, prim, st, sp, wc, r.alpha, r.first, mkey,
@@ -668,7 +874,7 @@ __device__ bool trace_closest(const dscene& S, const v3& o, const v3& d, double 
     h.prim = -1;
     h.t = tmax;
     bool any = false;
-    if (S.nobj > 0) walk4<CNT>(S.qtlas, S.tlas_root, o, d, tmin, tmax, st, 0, wc, [&](int off, int cnt, int sp, int e) {
+    if (S.nworld > 0) walk4<CNT>(S.qtlas, S.tlas_root, o, d, tmin, tmax, st, 0, wc, [&](int off, int cnt, int sp, int e) {
         const double tleaf = tmax;
         int ok = -1;  // the leaf's double test, made at its first hit
         for (int k = 0; k < cnt; k++) {
@@ -718,8 +924,11 @@ struct hitrec {
     v3 dpdu{0, 0, 0}, dpdv{0, 0, 0};
 };
 
+// tmin: the interval's start the hit was found with (a voxel hit's face is
+// found again by walking its block up to the hit).
 template <int FEAT = F_ALL>
-__device__ __forceinline__ void finish_hit(const dscene& S, const hitinfo& h, const v3& wo, const v3& wd, hitrec& rec) {
+__device__ __forceinline__ void finish_hit(const dscene& S, const hitinfo& h, const v3& wo, const v3& wd, hitrec& rec,
+                                           double tmin = 0.001) {
     if constexpr (FEAT & F_UV) {
         if (h.obj == S.nobj) {  // the ground (plane.h:23-33)
             v3 p = wo + h.t * wd;
@@ -759,6 +968,38 @@ __device__ __forceinline__ void finish_hit(const dscene& S, const hitinfo& h, co
     if (x) {
         o = xf_inv_point(*x, wo);
         d = xf_inv_vector(*x, wd);
+    }
+    if constexpr (FEAT & F_UV) {
+        if (r.kind == MPT_OBJ_VOXEL) {  // record (voxel.h:324-335) of the face the block's walk stops on
+            const double* vq = prim_ptr(S, h.prim);
+            vox_hit f{0, 1, 0, 0, 0, 0};
+            double tt;
+            vox_walk<true>(S, vq, o, d, tmin, h.t, tt, &f);
+            const mpt_vox_model& m = S.vox[int(vq[6])];
+            v3 p = o + h.t * d;
+            const int b = (f.axis + 1) % 3, c = (f.axis + 2) % 3;
+            const v3 outward{f.axis == 0 ? double(f.sign) : 0.0, f.axis == 1 ? double(f.sign) : 0.0,
+                             f.axis == 2 ? double(f.sign) : 0.0};
+            const v3 tb{b == 0 ? 1.0 : 0.0, b == 1 ? 1.0 : 0.0, b == 2 ? 1.0 : 0.0};
+            const v3 tc{c == 0 ? 1.0 : 0.0, c == 1 ? 1.0 : 0.0, c == 2 ? 1.0 : 0.0};
+            rec.dpdu = f.sign > 0 ? tb : tc;  // cross(dpdu, dpdv) = the outward normal
+            rec.dpdv = f.sign > 0 ? tc : tb;
+            const double pu = dot(p, rec.dpdu), pv = dot(p, rec.dpdv);
+            rec.u = pu - std::floor(pu);
+            rec.v = pv - std::floor(pv);
+            rec.mat = m.edge_width > 0 && vox_on_edge(S, m, p, f) ? m.edge_material : m.material[f.index];
+            rec.front = dot(d, outward) < 0;
+            v3 n = rec.front ? outward : -outward;
+            if (x) {  // instance.h:33-42
+                p = xf_point(*x, p);
+                n = unit_vector(xf_normal(*x, n));
+                rec.dpdu = xf_vector(*x, rec.dpdu);
+                rec.dpdv = xf_vector(*x, rec.dpdv);
+            }
+            rec.p = p;
+            rec.normal = n;
+            return;
+        }
     }
     const double* q = prim_ptr(S, h.prim);
     int k = S.pkind[h.prim];

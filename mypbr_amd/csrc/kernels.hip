@@ -394,7 +394,7 @@ __global__ __launch_bounds__(BS) void k_query(dscene S, launch_ctx L, const doub
         bool hit = trace_closest<false, FEAT>(S, o, d, tt[2 * i], tt[2 * i + 1], h, st, false, 0, 0, wc);
         if (hit) {
             hitrec rec;
-            finish_hit<FEAT>(S, h, o, d, rec);
+            finish_hit<FEAT>(S, h, o, d, rec, tt[2 * i]);
             out_id[i] = h.obj == S.nobj ? S.ground_id : S.obj[h.obj].id_base + S.pcanon[h.prim];
             out_t[i] = h.t;
             out_front[i] = rec.front ? 1 : 0;
@@ -430,7 +430,7 @@ __global__ __launch_bounds__(256) void k_ties(dscene S, const double* rays, cons
             };
             for (int oi = 0; oi < S.nobj; oi++) {
                 const obj_rec r = S.obj[oi];
-                if (r.kind == MPT_OBJ_MEDIUM) continue;
+                if (r.kind == MPT_OBJ_MEDIUM || (r.flags & MPT_OBJF_LIGHT_ONLY)) continue;
                 v3 lo_o = o, lo_d = d;
                 if (r.xform >= 0) {
                     lo_o = xf_inv_point(S.xf[r.xform], o);
@@ -441,7 +441,10 @@ __global__ __launch_bounds__(256) void k_ties(dscene S, const double* rays, cons
                     const int64_t p = r.first + k;
                     double t;
                     wcount wc{0, 0};
-                    if (prim_t<false, F_ALL>(S, p, lo_o, lo_d, tmin, tmax, t, wc) &&
+                    const bool hit = r.kind == MPT_OBJ_VOXEL
+                                         ? vox_walk<false>(S, prim_ptr(S, p), lo_o, lo_d, tmin, tmax, t, nullptr)
+                                         : prim_t<false, F_ALL>(S, p, lo_o, lo_d, tmin, tmax, t, wc);
+                    if (hit &&
                         uint64_t(__double_as_longlong(t)) == best)
                         take(r.id_base + S.pcanon[p]);
                 }
@@ -1551,6 +1554,9 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
     d.env_intensity = h.env_intensity;
     d.env_rotation = h.env_rotation;
     d.nobj = int32_t(h.objects.size());
+    d.nworld = int32_t(h.tlas_items.size());
+    if ((rc = upload(s.get(), h.vox, &d.vox))) return rc;
+    if ((rc = upload(s.get(), h.vox_cells, &d.vcells))) return rc;
     d.nlights = int32_t(h.lights.size());
     d.lstrategy = h.light_strategy;
     d.env_w = h.env_w;
@@ -1568,7 +1574,8 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
     d.sun_solid_angle = h.sun_solid_angle;
     d.sun_radiance = h.sun_radiance;
     d.ground_id = 0;  // one past the last canonical ID of the world list
-    for (const obj_rec& r : h.objects) d.ground_id = std::max(d.ground_id, r.id_base + r.count);
+    for (const obj_rec& r : h.objects)
+        if (!(r.flags & MPT_OBJF_LIGHT_ONLY)) d.ground_id = std::max(d.ground_id, r.id_base + r.count);
     // geometry profile: spheres only with object i = primitive i, quads and
     // spheres in lists and instances, or everything
     {
@@ -1590,6 +1597,12 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
         for (const auto& r : h.objects)
             if (r.kind == MPT_OBJ_MEDIUM) s->mat_types |= 1u << MPT_MAT_PHASE;
         if (!h.fogs.empty()) s->mat_types |= 1u << MPT_MAT_PHASE;  // fog hits shade as a phase material
+        if (h.ground_mat >= 0) s->mat_types |= 1u << h.materials[h.ground_mat].type;
+        for (const mpt_vox_model& vm : h.vox) {  // voxel hits take their palette's (or the outline's) material
+            for (int i = 0; i < 256; i++)
+                if (vm.material[i] >= 0) s->mat_types |= 1u << h.materials[vm.material[i]].type;
+            if (vm.edge_width > 0) s->mat_types |= 1u << h.materials[vm.edge_material].type;
+        }
         s->keyed_hits = !h.fogs.empty();
         for (const auto& r : h.objects)
             if (r.kind == MPT_OBJ_MEDIUM || r.alpha != -1) s->keyed_hits = true;

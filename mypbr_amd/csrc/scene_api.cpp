@@ -387,8 +387,11 @@ struct flat_builder::impl {
     std::vector<int32_t> prim_kind, prim_mat;
     std::vector<double> prim_data;
     std::vector<double> prim_nrm;  // 9 per primitive once any triangle has vertex normals
-    std::vector<int32_t> obj_kind, obj_xform, obj_mat, obj_bkind, obj_alpha;
-    bool any_alpha = false;
+    std::vector<int32_t> obj_kind, obj_xform, obj_mat, obj_bkind, obj_alpha, obj_flags;
+    bool any_alpha = false, any_flags = false;
+    std::vector<mpt_vox_model> vox_models;
+    std::vector<uint8_t> vox_cells;
+    std::map<std::tuple<const void*, const void*, const void*, double>, int32_t> vox_index;
     std::vector<int64_t> obj_first, obj_count, obj_id;
     std::vector<double> obj_density;
     std::vector<mpt_xform> xforms;
@@ -534,6 +537,7 @@ int32_t flat_builder::add_object(int32_t kind, int64_t first, int64_t count, int
     p->obj_mat.push_back(-1);
     p->obj_bkind.push_back(MPT_OBJ_PRIM);
     p->obj_alpha.push_back(-1);
+    p->obj_flags.push_back(0);
     p->next_id += count;
     int32_t idx = int32_t(p->obj_kind.size() - 1);
     p->object_of[owner] = idx;
@@ -934,8 +938,17 @@ void flat_builder::add_lights(const hittable_list& lights, const std::shared_ptr
             continue;
         }
         auto it = p->object_of.find(o.get());
-        if (it == p->object_of.end())
-            throw std::runtime_error("lights must also be part of the world (camera.h:73-74)");
+        if (it == p->object_of.end()) {
+            // sampled only, never hit: the emissive faces of a voxel model, whose
+            // grid the rays hit instead (scene.h:1118-1128)
+            const size_t before = p->obj_kind.size();
+            o->flatten(*this, -1);
+            if (p->obj_kind.size() != before + 1)
+                throw std::runtime_error("a light outside the world must flatten to one object");
+            p->obj_flags[before] = MPT_OBJF_LIGHT_ONLY;
+            p->any_flags = true;
+            it = p->object_of.find(o.get());
+        }
         p->light_object.push_back(it->second);
     }
     if (envp) {
@@ -963,6 +976,417 @@ void flat_builder::add_lights(const hittable_list& lights, const std::shared_ptr
         p->d.light_strategy = 2;  // plain hittable_list: uniform pick (hittable_list.h:58-62)
         p->d.light_uniform_fraction = 0;
     }
+}
+
+// ------------------------------------------------------------------ voxels
+namespace {
+
+// sRGB-encoded [0, 1] to linear (color3.h:26-30: the IEC 61966-2-1 curve).
+double srgb_decode(double encoded) {
+    return encoded <= 0.04045 ? encoded / 12.92 : std::pow((encoded + 0.055) / 1.055, 2.4);
+}
+
+// MagicaVoxel's default palette, for files without an RGBA chunk
+// (vox_loader.h:62-81), as its structure gives it: index 0 empty; 1-215 the
+// 6x6x6 cube of levels ff, cc, 99, 66, 33, 00 (blue fastest, then green, then
+// red) without black; then ramps of ten levels (ee, dd, bb, aa, 88, 77, 55,
+// 44, 22, 11) in red, green, blue and grey. Colours are in [0, 1], sRGB as stored.
+void default_vox_palette(color out[256]) {
+    static const int cube[6] = {0xff, 0xcc, 0x99, 0x66, 0x33, 0x00};
+    static const int ramp[10] = {0xee, 0xdd, 0xbb, 0xaa, 0x88, 0x77, 0x55, 0x44, 0x22, 0x11};
+    out[0] = color(0, 0, 0);
+    for (int i = 0; i < 215; i++)
+        out[1 + i] = color(cube[i / 36] / 255.0, cube[(i / 6) % 6] / 255.0, cube[i % 6] / 255.0);
+    for (int j = 0; j < 10; j++) {
+        const double v = ramp[j] / 255.0;
+        out[216 + j] = color(v, 0, 0);
+        out[226 + j] = color(0, v, 0);
+        out[236 + j] = color(0, 0, v);
+        out[246 + j] = color(v, v, v);
+    }
+}
+
+// Little-endian fields of one chunk's content; a read past its end marks the
+// chunk corrupt.
+struct vox_reader {
+    const uint8_t* p;
+    const uint8_t* end;
+    bool ok = true;
+    int32_t i32() {
+        if (end - p < 4) {
+            ok = false;
+            p = end;
+            return 0;
+        }
+        const uint32_t v = uint32_t(p[0]) | uint32_t(p[1]) << 8 | uint32_t(p[2]) << 16 | uint32_t(p[3]) << 24;
+        p += 4;
+        return int32_t(v);
+    }
+    std::string str() {
+        const int32_t n = i32();
+        if (n < 0 || n > end - p) {
+            ok = false;
+            p = end;
+            return std::string();
+        }
+        std::string r(reinterpret_cast<const char*>(p), size_t(n));
+        p += n;
+        return r;
+    }
+    std::map<std::string, std::string> dict() {
+        std::map<std::string, std::string> d;
+        const int32_t n = i32();
+        for (int32_t k = 0; k < n && ok; k++) {
+            std::string key = str();
+            d[key] = str();
+        }
+        return d;
+    }
+};
+
+struct vox_node {
+    char type = 0;  // 'T' transform, 'G' group, 'S' shape
+    bool hidden = false;
+    std::vector<int> children;
+    int model = -1;  // a shape's (first frame) model
+    transform xform;  // a transform's (first frame)
+};
+
+}  // namespace
+
+bool load_vox(const std::string& path, vox_file& out, std::string& error) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) {
+        error = "cannot open '" + path + "'";
+        return false;
+    }
+    const std::vector<uint8_t> bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    if (bytes.size() < 20 || std::memcmp(bytes.data(), "VOX ", 4) != 0) {
+        error = "'" + path + "' is not a MagicaVoxel .vox file";
+        return false;
+    }
+    default_vox_palette(out.palette);
+    out.models.clear();
+    out.instances.clear();
+    for (auto& m : out.materials) m.clear();
+    const uint8_t* p = bytes.data() + 8;  // past "VOX " and the version
+    const uint8_t* end = bytes.data() + bytes.size();
+    if (end - p < 12 || std::memcmp(p, "MAIN", 4) != 0) {
+        error = "'" + path + "' has no MAIN chunk";
+        return false;
+    }
+    {
+        vox_reader h{p + 4, p + 12};
+        p += 12 + std::max(h.i32(), 0);  // MAIN's own content; its children follow as one flat run
+    }
+    std::map<int, vox_node> nodes;
+    int sx = 0, sy = 0, sz = 0;
+    while (end - p >= 12) {
+        const std::string id(reinterpret_cast<const char*>(p), 4);
+        vox_reader h{p + 4, p + 12};
+        const int32_t content = h.i32(), children = h.i32();
+        if (content < 0 || children < 0 || int64_t(content) + children > end - p - 12) {
+            error = "'" + path + "' is truncated or corrupt (chunk " + id + ")";
+            return false;
+        }
+        const uint8_t* body = p + 12;
+        vox_reader in{body, body + content};
+        p += 12 + int64_t(content) + children;
+        if (id == "SIZE") {
+            sx = in.i32(), sy = in.i32(), sz = in.i32();
+            if (sx <= 0 || sy <= 0 || sz <= 0 || sx > 2048 || sy > 2048 || sz > 2048) {
+                error = "'" + path + "' has a model of invalid size";
+                return false;
+            }
+        } else if (id == "XYZI") {
+            if (sx <= 0) {
+                error = "'" + path + "' has voxels before their SIZE";
+                return false;
+            }
+            vox_model m;
+            m.nx = sx, m.ny = sy, m.nz = sz;
+            m.cells.assign(size_t(sx) * sy * sz, 0);
+            const int32_t n = in.i32();
+            if (n < 0 || int64_t(n) * 4 > in.end - in.p) {
+                error = "'" + path + "' has a truncated voxel list";
+                return false;
+            }
+            for (int32_t k = 0; k < n; k++) {
+                const uint8_t* v = in.p + 4 * int64_t(k);  // x, y, z, palette index
+                if (v[0] < sx && v[1] < sy && v[2] < sz) m.cells[(size_t(v[2]) * sy + v[1]) * sx + v[0]] = v[3];
+            }
+            out.models.push_back(std::move(m));
+            sx = sy = sz = 0;
+        } else if (id == "RGBA") {
+            if (content < 1024) {
+                error = "'" + path + "' has a short palette";
+                return false;
+            }
+            for (int k = 0; k < 255; k++)  // entry k is palette index k + 1
+                out.palette[k + 1] = color(body[4 * k] / 255.0, body[4 * k + 1] / 255.0, body[4 * k + 2] / 255.0);
+        } else if (id == "MATL") {
+            const int index = in.i32();
+            auto props = in.dict();
+            if (in.ok && index >= 0 && index < 256) out.materials[index] = props;
+        } else if (id == "nTRN") {
+            vox_node& t = nodes[in.i32()];
+            auto attr = in.dict();
+            t.type = 'T';
+            t.hidden = attr["_hidden"] == "1";
+            t.children.push_back(in.i32());
+            in.i32();  // reserved
+            in.i32();  // layer
+            if (in.i32() > 0) {  // frames: the first one places the child
+                auto frame = in.dict();
+                transform rotation;
+                if (frame.count("_r")) {
+                    // a signed permutation: bits 0-1 and 2-3 the columns of rows 0 and
+                    // 1 (row 2 takes the other), bits 4-6 the rows' signs
+                    int r = 4;
+                    try {
+                        r = std::stoi(frame["_r"]);
+                    } catch (...) {
+                        r = 4;
+                    }
+                    const int c0 = r & 3, c1 = (r >> 2) & 3, c2 = 3 - c0 - c1;
+                    if (c0 > 2 || c1 > 2 || c0 == c1) {
+                        error = "'" + path + "' has an invalid rotation in an nTRN chunk";
+                        return false;
+                    }
+                    double m[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                    m[0][c0] = (r & 16) ? -1 : 1;
+                    m[1][c1] = (r & 32) ? -1 : 1;
+                    m[2][c2] = (r & 64) ? -1 : 1;
+                    rotation = transform::orthogonal(m);
+                }
+                vec3 offset(0, 0, 0);
+                if (frame.count("_t")) {
+                    std::istringstream xyz(frame["_t"]);
+                    xyz >> offset[0] >> offset[1] >> offset[2];
+                }
+                t.xform = transform::translate(offset).then_after(rotation);
+            }
+        } else if (id == "nGRP") {
+            vox_node& g = nodes[in.i32()];
+            auto attr = in.dict();
+            g.type = 'G';
+            g.hidden = attr["_hidden"] == "1";
+            const int32_t n = in.i32();
+            for (int32_t k = 0; k < n && in.ok; k++) g.children.push_back(in.i32());
+        } else if (id == "nSHP") {
+            vox_node& sh = nodes[in.i32()];
+            in.dict();
+            sh.type = 'S';
+            if (in.i32() > 0) {
+                sh.model = in.i32();
+                in.dict();
+            }
+        }
+        if (!in.ok) {
+            error = "'" + path + "' has a corrupt " + id + " chunk";
+            return false;
+        }
+    }
+    if (out.models.empty()) {
+        error = "'" + path + "' has no voxels";
+        return false;
+    }
+    // a model's cells sit around its centre, rounded down, as MagicaVoxel places them
+    auto centred = [&](int k) {
+        const vox_model& m = out.models[size_t(k)];
+        return transform::translate(-vec3(m.nx / 2, m.ny / 2, m.nz / 2));
+    };
+    if (!nodes.count(0)) {  // no scene graph (version 150): each model on z = 0, centred in x and y
+        for (int k = 0; k < int(out.models.size()); k++)
+            out.instances.push_back({k, transform::translate(vec3(0, 0, out.models[size_t(k)].nz / 2)).then_after(centred(k))});
+        return true;
+    }
+    // the graph from node 0, transforms composed on the way down; a stack (so
+    // children are taken last first) with a depth cap for cyclic files
+    struct item {
+        int id, depth;
+        transform xform;
+    };
+    std::vector<item> stack{{0, 0, transform()}};
+    while (!stack.empty()) {
+        const item it = stack.back();
+        stack.pop_back();
+        auto nt = nodes.find(it.id);
+        if (nt == nodes.end() || nt->second.hidden || it.depth > 64) continue;
+        const vox_node& n = nt->second;
+        const transform here = n.type == 'T' ? it.xform.then_after(n.xform) : it.xform;
+        for (int child : n.children) stack.push_back({child, it.depth + 1, here});
+        if (n.model >= 0 && n.model < int(out.models.size())) out.instances.push_back({n.model, here.then_after(centred(n.model))});
+    }
+    return true;
+}
+
+bool voxel_face_visible(voxel_kind k, voxel_kind n) {
+    if (k == voxel_kind::empty) return false;
+    return n == voxel_kind::empty || (k == voxel_kind::opaque && n == voxel_kind::transparent);
+}
+
+voxel_palette make_voxel_palette(const vox_file& vox, double emission_scale) {
+    voxel_palette pal;
+    bool warned = false;
+    auto number = [](const std::map<std::string, std::string>& props, const char* key, double fallback) {
+        auto it = props.find(key);
+        if (it == props.end()) return fallback;
+        try {
+            return std::stod(it->second);
+        } catch (...) {
+            return fallback;
+        }
+    };
+    for (int k = 0; k < 256; k++) {
+        const color& c = vox.palette[k];
+        const color albedo(srgb_decode(c[0]), srgb_decode(c[1]), srgb_decode(c[2]));
+        const auto& props = vox.materials[k];
+        auto ti = props.find("_type");
+        const std::string type = ti == props.end() ? "_diffuse" : ti->second;
+        const double roughness = std::clamp(number(props, "_rough", 0.1), 0.0, 1.0);
+        double ior = number(props, "_ri", 1 + number(props, "_ior", 0.5));
+        if (ior <= 1) ior = 1.5;
+        pal.kinds[k] = k == 0 ? voxel_kind::empty : voxel_kind::opaque;
+        pal.emissive[k] = false;
+        if (type == "_metal") {
+            const double metallic = number(props, "_plastic", 0) > 0 ? 0 : std::clamp(number(props, "_metal", 1), 0.0, 1.0);
+            pal.materials[k] = std::make_shared<principled>(std::make_shared<solid_color>(albedo), metallic, roughness, ior);
+        } else if (type == "_glass") {
+            // light crossing one voxel keeps mix(white, colour, _att) of itself
+            const double att = std::clamp(number(props, "_att", 0.5), 0.0, 1.0);
+            color tint = (1 - att) * color(1, 1, 1) + att * albedo;
+            for (int i = 0; i < 3; i++) tint[i] = std::clamp(tint[i], 1e-3, 1.0);
+            pal.materials[k] = std::make_shared<dielectric>(ior, roughness <= 0.05 ? 0 : roughness,
+                                                            dielectric::absorption_for(tint, 1));
+            pal.kinds[k] = voxel_kind::transparent;
+        } else if (type == "_emit") {
+            const double strength = number(props, "_emit", 1) * std::pow(10.0, number(props, "_flux", 0)) * emission_scale;
+            pal.materials[k] = std::make_shared<diffuse_light>(strength * albedo);
+            pal.emissive[k] = strength > 0;
+        } else {
+            if (type != "_diffuse" && !warned && k > 0) {
+                std::fprintf(stderr, "Warning: .vox material type %s is rendered as diffuse\n", type.c_str());
+                warned = true;
+            }
+            pal.materials[k] = std::make_shared<lambertian>(albedo);
+        }
+    }
+    return pal;
+}
+
+void voxel_model_faces(const vox_model& m, const voxel_palette& pal, hittable_list& faces, hittable_list& emitters) {
+    // the six sides in the order +x, -x, +y, -y, +z, -z; a + side's corner is
+    // one step along its axis, and its edges (u, v) are the next two axes in
+    // turn, swapped on the - side, so that cross(u, v) points out of the cell
+    for (int z = 0; z < m.nz; z++)
+        for (int y = 0; y < m.ny; y++)
+            for (int x = 0; x < m.nx; x++) {
+                const uint8_t index = m.at(x, y, z);
+                const voxel_kind k = pal.kind(index);
+                if (k == voxel_kind::empty) continue;
+                for (int side = 0; side < 6; side++) {
+                    const int a = side / 2, s = side % 2 == 0 ? 1 : -1;
+                    int nb[3] = {x, y, z};
+                    nb[a] += s;
+                    if (!voxel_face_visible(k, pal.kind(m.at(nb[0], nb[1], nb[2])))) continue;
+                    point3 Q(x, y, z);
+                    if (s > 0) Q[a] += 1;
+                    vec3 e1(0, 0, 0), e2(0, 0, 0);
+                    e1[(a + 1) % 3] = 1;
+                    e2[(a + 2) % 3] = 1;
+                    auto face = std::make_shared<quad>(Q, s > 0 ? e1 : e2, s > 0 ? e2 : e1, pal.materials[index]);
+                    faces.add(face);
+                    if (pal.emissive[index]) emitters.add(face);
+                }
+            }
+}
+
+voxel_grid::voxel_grid(std::shared_ptr<const vox_model> model, std::shared_ptr<const voxel_palette> palette,
+                       const int lo_[3], const int hi_[3], std::shared_ptr<material> edge_material, double edge_width)
+    : model(std::move(model)), palette(std::move(palette)), edge_material(std::move(edge_material)),
+      edge_width(this->edge_material ? edge_width : 0) {
+    for (int a = 0; a < 3; a++) lo[a] = lo_[a], hi[a] = hi_[a];
+}
+
+int32_t flat_builder::vox_model_index(const vox_model& m, const voxel_palette& pal,
+                                      const std::shared_ptr<material>& edge_material, double edge_width) {
+    const auto key = std::make_tuple(static_cast<const void*>(&m), static_cast<const void*>(&pal),
+                                     static_cast<const void*>(edge_material.get()), edge_width);
+    auto it = p->vox_index.find(key);
+    if (it != p->vox_index.end()) return it->second;
+    if (m.nx < 1 || m.ny < 1 || m.nz < 1 || m.cells.size() != size_t(m.nx) * m.ny * m.nz)
+        throw std::runtime_error("a voxel model's cells do not match its size");
+    mpt_vox_model v;
+    std::memset(&v, 0, sizeof(v));
+    v.nx = m.nx, v.ny = m.ny, v.nz = m.nz;
+    v.cells = int64_t(p->vox_cells.size());
+    v.edge_width = edge_material ? edge_width : 0;
+    v.edge_material = v.edge_width > 0 ? material_index(edge_material) : -1;
+    bool used[256] = {};
+    for (uint8_t c : m.cells) used[c] = true;
+    for (int i = 0; i < 256; i++) {
+        v.kind[i] = uint8_t(pal.kinds[size_t(i)]);
+        // materials only for the indices cells use: an unused emitter or glass entry
+        // would change which shading kernels and shadow queries the scene gets
+        v.material[i] = used[i] && pal.kinds[size_t(i)] != voxel_kind::empty ? material_index(pal.materials[size_t(i)]) : -1;
+    }
+    p->vox_cells.insert(p->vox_cells.end(), m.cells.begin(), m.cells.end());
+    p->vox_models.push_back(v);
+    const int32_t idx = int32_t(p->vox_models.size() - 1);
+    p->vox_index[key] = idx;
+    return idx;
+}
+
+// The blocks as MPT_PRIM_VOXBLOCK primitives (once per set) and one voxel
+// object over them, behind the instance transform if any.
+static void flatten_blocks(flat_builder& b, int32_t xform, const void* key,
+                           const std::vector<std::shared_ptr<voxel_grid>>& blocks, const hittable* owner) {
+    if (blocks.empty()) throw std::runtime_error("a voxel model without voxels");
+    auto it = b.mesh_ranges.find(key);
+    if (it == b.mesh_ranges.end()) {
+        int64_t first = -1;
+        for (const auto& g : blocks) {
+            const int32_t mi = b.vox_model_index(*g->model, *g->palette, g->edge_material, g->edge_width);
+            double d[9] = {double(g->lo[0]), double(g->lo[1]), double(g->lo[2]), double(g->hi[0]), double(g->hi[1]),
+                           double(g->hi[2]), double(mi), 0, 0};
+            const int64_t pi = b.add_prim(MPT_PRIM_VOXBLOCK, -1, d);
+            if (first < 0) first = pi;
+        }
+        it = b.mesh_ranges.emplace(key, std::make_pair(first, int64_t(blocks.size()))).first;
+    }
+    b.add_object(MPT_OBJ_VOXEL, it->second.first, it->second.second, xform, owner);
+}
+
+void voxel_grid::flatten(flat_builder& b, int32_t xform) const {
+    std::vector<std::shared_ptr<voxel_grid>> one{std::make_shared<voxel_grid>(*this)};
+    flatten_blocks(b, xform, this, one, this);
+}
+
+void voxel_block_set::flatten(flat_builder& b, int32_t xform) const { flatten_blocks(b, xform, this, blocks, this); }
+
+std::shared_ptr<hittable> voxel_blocks(std::shared_ptr<const vox_model> model,
+                                       std::shared_ptr<const voxel_palette> palette, int block,
+                                       std::shared_ptr<material> edge_material, double edge_width) {
+    if (block < 1) throw std::invalid_argument("voxel_blocks: block must be at least 1");
+    auto set = std::make_shared<voxel_block_set>();
+    for (int z0 = 0; z0 < model->nz; z0 += block)
+        for (int y0 = 0; y0 < model->ny; y0 += block)
+            for (int x0 = 0; x0 < model->nx; x0 += block) {
+                // the box of the block's voxels: every surface lies on a voxel's faces
+                int lo[3] = {x0 + block, y0 + block, z0 + block}, hi[3] = {x0, y0, z0};
+                for (int z = z0; z < std::min(z0 + block, model->nz); z++)
+                    for (int y = y0; y < std::min(y0 + block, model->ny); y++)
+                        for (int x = x0; x < std::min(x0 + block, model->nx); x++)
+                            if (palette->kind(model->at(x, y, z)) != voxel_kind::empty) {
+                                lo[0] = std::min(lo[0], x), hi[0] = std::max(hi[0], x + 1);
+                                lo[1] = std::min(lo[1], y), hi[1] = std::max(hi[1], y + 1);
+                                lo[2] = std::min(lo[2], z), hi[2] = std::max(hi[2], z + 1);
+                            }
+                if (hi[0] > lo[0])
+                    set->blocks.push_back(std::make_shared<voxel_grid>(model, palette, lo, hi, edge_material, edge_width));
+            }
+    return set;
 }
 
 const mpt_scene_desc& flat_builder::desc() {
@@ -1002,6 +1426,11 @@ const mpt_scene_desc& flat_builder::desc() {
     d.n_lights = int32_t(p->light_object.size());
     d.light_object = p->light_object.data();
     d.env_rgb = p->env_rgb.empty() ? nullptr : p->env_rgb.data();
+    d.object_flags = p->any_flags ? p->obj_flags.data() : nullptr;
+    d.n_vox_models = int32_t(p->vox_models.size());
+    d.vox_models = p->vox_models.empty() ? nullptr : p->vox_models.data();
+    d.vox_cells = p->vox_cells.empty() ? nullptr : p->vox_cells.data();
+    d.n_vox_cells = int64_t(p->vox_cells.size());
     return d;
 }
 
@@ -1908,6 +2337,108 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
             std::string state;
             if (!(in >> state) || (state != "on" && state != "off")) return fail("expected 'normal_maps on|off'");
             use_normal_maps = state == "on";
+        } else if (keyword == "style") {  // scene.h:612-645: a preset of other lines, which later lines can change
+            std::string name;
+            if (!(in >> name) || (name != "voxel" && name != "classic"))
+                return fail("expected 'style voxel' or 'style classic'");
+            const bool voxel = name == "voxel";
+            auto& post = result.display.post;
+            result.display.op = voxel ? tonemapper::aces : tonemapper::none;
+            post = post_settings();
+            sun_on = false, sun_disk = true;
+            sun_elevation = 45, sun_azimuth = 45, sun_intensity = 3, sun_angle = 4;
+            sun_tint = color(1, 1, 1);
+            sky_uniform = false;
+            sky_atmosphere = voxel;
+            air = sky_air();
+            cam.use_sky = true;
+            ground = voxel;
+            ground_material = voxel ? std::make_shared<lambertian>(color(0.3, 0.3, 0.3)) : default_material;
+            cam.max_diffuse = voxel ? 2 : 0;
+            cam.max_specular = voxel ? 5 : 0;
+            cam.max_scatter = voxel ? 5 : 0;
+            cam.transparent_shadows = voxel;
+            if (voxel) {
+                post.bloom = 0.25;
+                post.vignette = 0.25;
+                sun_on = true;
+                sun_elevation = 50, sun_azimuth = 50, sun_intensity = 4, sun_angle = 3;
+            }
+        } else if (keyword == "vox") {  // scene.h:1040-1128, MagicaVoxel models
+            const char* usage = "expected 'vox <file.vox> [quads] [emission <scale>] [edges <width> [<r g b>]] [transforms]'";
+            std::string file, error, token;
+            if (!(in >> file)) return fail(usage);
+            if (file[0] != '/') file = scene_dir + file;
+            // options first, then the transforms
+            bool quads = false;
+            double emission = 1, edge_width = 0;
+            color edge_color(0, 0, 0);
+            std::vector<std::string> rest;
+            while (in >> token) rest.push_back(token);
+            size_t k = 0;
+            for (; k < rest.size(); k++) {
+                if (rest[k] == "quads") {
+                    quads = true;
+                } else if (rest[k] == "emission" && k + 1 < rest.size()) {
+                    std::istringstream number(rest[++k]);
+                    if (!(number >> emission) || emission < 0) return fail(usage);
+                } else if (rest[k] == "edges" && k + 1 < rest.size()) {
+                    std::istringstream number(rest[++k]);
+                    if (!(number >> edge_width) || edge_width <= 0 || edge_width >= 0.5) return fail(usage);
+                    if (k + 3 < rest.size()) {  // an optional colour
+                        std::istringstream rgb(rest[k + 1] + ' ' + rest[k + 2] + ' ' + rest[k + 3]);
+                        color c;
+                        if (rgb >> c[0] >> c[1] >> c[2]) {
+                            edge_color = c;
+                            k += 3;
+                        }
+                    }
+                } else {
+                    break;
+                }
+            }
+            std::string tail_text;
+            for (; k < rest.size(); k++) tail_text += rest[k] + ' ';
+            std::istringstream tail(tail_text);
+            std::shared_ptr<material> mat;
+            transform xform;
+            bool transformed;
+            if (!read_tail(tail, mat, xform, transformed, error)) return fail(error);
+            if (mat != default_material) return fail("vox models take their materials from the file's palette");
+            vox_file vox;
+            if (!load_vox(file, vox, error)) return fail(error);
+            auto palette = std::make_shared<voxel_palette>(make_voxel_palette(vox, emission));
+            // per model: a grid of blocks (or with "quads" its visible faces), shared
+            // by the instances that place it; emissive faces are lights either way,
+            // and with a grid only sampled: the grid is what rays hit
+            std::vector<std::shared_ptr<hittable>> geometry(vox.models.size());
+            std::vector<std::shared_ptr<hittable_list>> emitters(vox.models.size());
+            std::vector<std::shared_ptr<hittable_list>> face_lists(vox.models.size());
+            for (size_t m = 0; m < vox.models.size(); m++) {
+                face_lists[m] = std::make_shared<hittable_list>();
+                emitters[m] = std::make_shared<hittable_list>();
+                voxel_model_faces(vox.models[m], *palette, *face_lists[m], *emitters[m]);
+                if (!quads)
+                    geometry[m] = voxel_blocks(std::make_shared<vox_model>(vox.models[m]), palette, 8,
+                                               edge_width > 0 ? std::make_shared<lambertian>(edge_color) : nullptr,
+                                               edge_width);
+            }
+            // MagicaVoxel is z-up: its +z becomes +y (and +y becomes -z)
+            const transform z_up = transform::rotate(-90, vec3(1, 0, 0));
+            for (const auto& placed : vox.instances) {
+                const transform to_world = xform.then_after(z_up.then_after(placed.xform));
+                if (quads) {  // each face behind the instance's transform (the GPU path instances no BVH)
+                    for (const auto& face : face_lists[size_t(placed.model)]->objects)
+                        world.add(std::make_shared<instance>(face, to_world));
+                } else {
+                    world.add(std::make_shared<instance>(geometry[size_t(placed.model)], to_world));
+                }
+                if (!emitters[size_t(placed.model)]->empty()) {
+                    if (!to_world.is_similarity())
+                        return fail("a vox model with emissive voxels can only be scaled uniformly");
+                    lights.add(std::make_shared<instance>(emitters[size_t(placed.model)], to_world));
+                }
+            }
         } else if (keyword == "mesh") {  // scene.h:1129-1197, Wavefront OBJ files
             std::string file, error;
             if (!(in >> file)) return fail("expected 'mesh <file.obj> [material] [transforms]'");

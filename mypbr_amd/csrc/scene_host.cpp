@@ -131,6 +131,11 @@ box3 prim_box(int kind, const double* p) {
         v3 Q{p[0], p[1], p[2]}, u{p[3], p[4], p[5]}, v{p[6], p[7], p[8]};
         return box_merge(box_points(Q, Q + u + v), box_points(Q + u, Q + v));
     }
+    if (kind == MPT_PRIM_VOXBLOCK) {  // the block's cells (voxel.h:187)
+        box3 b;
+        for (int a = 0; a < 3; a++) b.ax[a] = span{p[a], p[3 + a]};
+        return b;
+    }
     v3 p0{p[0], p[1], p[2]}, p1{p[3], p[4], p[5]}, p2{p[6], p[7], p[8]};
     return box_merge(box_points(p0, p1), box_points(p2, p2));
 }
@@ -507,6 +512,37 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         }
     }
 
+    // voxel models (vox_loader.h:32-44, voxel.h:23-31)
+    s.vox.clear();
+    s.vox_cells.clear();
+    if (d.n_vox_models < 0 || (d.n_vox_models > 0 && (!d.vox_models || !d.vox_cells || d.n_vox_cells < 0)))
+        return fail(MPT_E_INVALID, "voxel models without their cells");
+    for (int32_t k = 0; k < d.n_vox_models; k++) {
+        const mpt_vox_model& m = d.vox_models[k];
+        if (m.nx < 1 || m.ny < 1 || m.nz < 1 || m.nx > 2048 || m.ny > 2048 || m.nz > 2048)
+            return fail(MPT_E_INVALID, "a voxel model's size must be 1 to 2048 cells a side");
+        if (m.cells < 0 || m.cells + int64_t(m.nx) * m.ny * m.nz > d.n_vox_cells)
+            return fail(MPT_E_INVALID, "a voxel model's cells lie outside vox_cells");
+        if (m.kind[0] != 0) return fail(MPT_E_INVALID, "palette index 0 of a voxel model must be empty");
+        for (int i = 0; i < 256; i++) {
+            if (m.kind[i] > 2) return fail(MPT_E_INVALID, "a voxel palette kind must be 0, 1 or 2");
+            if (m.material[i] < -1 || m.material[i] >= d.n_materials)
+                return fail(MPT_E_INVALID, "a voxel palette entry names a bad material");
+        }
+        const uint8_t* c = d.vox_cells + m.cells;
+        for (int64_t i = 0, n = int64_t(m.nx) * m.ny * m.nz; i < n; i++)
+            if (m.kind[c[i]] != 0 && m.material[c[i]] < 0)
+                return fail(MPT_E_INVALID, "a voxel cell's palette entry has no material");
+        if (!(m.edge_width >= 0 && m.edge_width < 0.5))
+            return fail(MPT_E_INVALID, "a voxel model's edge width must be in [0, 0.5)");
+        if (m.edge_width > 0 && (m.edge_material < 0 || m.edge_material >= d.n_materials))
+            return fail(MPT_E_INVALID, "a voxel model with edges needs an edge material");
+        mpt_vox_model h = m;
+        h.cells = int64_t(s.vox_cells.size());
+        s.vox_cells.insert(s.vox_cells.end(), c, c + int64_t(m.nx) * m.ny * m.nz);
+        s.vox.push_back(h);
+    }
+
     // primitives in description order first
     const int64_t np = d.n_prims;
     s.prim.assign(size_t(np) * kPrimStride, 0.0);
@@ -543,6 +579,17 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
             dst[6] = e2.x, dst[7] = e2.y, dst[8] = e2.z;
             dst[9] = g.x, dst[10] = g.y, dst[11] = g.z;
             dst[12] = 0.5 * length(cross(p1 - p0, p2 - p0));
+        } else if (kind == MPT_PRIM_VOXBLOCK) {
+            const int32_t model = int32_t(src[6]);
+            if (!(src[6] >= 0 && src[6] < d.n_vox_models && double(model) == src[6]))
+                return fail(MPT_E_INVALID, "a voxel block names a bad model");
+            const int32_t dims[3] = {s.vox[model].nx, s.vox[model].ny, s.vox[model].nz};
+            for (int a = 0; a < 3; a++)
+                if (!(src[a] >= 0 && src[a] < src[3 + a] && src[3 + a] <= dims[a]) || src[a] != std::floor(src[a]) ||
+                    src[3 + a] != std::floor(src[3 + a]))
+                    return fail(MPT_E_INVALID, "a voxel block must be whole cells inside its model");
+            for (int k = 0; k < 7; k++) dst[k] = src[k];
+            if (mat != -1) return fail(MPT_E_INVALID, "a voxel block takes its materials from its palette");
         } else {
             return fail(MPT_E_UNSUPPORTED, "unsupported primitive kind " + std::to_string(kind));
         }
@@ -592,7 +639,9 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         r.bkind = d.object_boundary_kind ? d.object_boundary_kind[o] : MPT_OBJ_PRIM;
         r.light_table = -1;
         r.alpha = d.object_alpha ? d.object_alpha[o] : -1;
-        r.pad = 0;
+        r.flags = d.object_flags ? d.object_flags[o] : 0;
+        if (r.flags & ~int32_t(MPT_OBJF_LIGHT_ONLY)) return fail(MPT_E_INVALID, "unknown object flags");
+        if (r.kind == MPT_OBJ_VOXEL) s.rich = true;  // grids are walked by the variant that carries (u, v)
         next_id = r.id_base + r.count;
         if (r.alpha != -1) {
             s.rich = true;  // a mask is read at the hit's (u, v): the variant that carries it tests masks
@@ -613,15 +662,20 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
                 break;
             case MPT_OBJ_LIST:
                 for (int64_t k = 0; k < r.count; k++) {
-                    if (s.prim_kind[r.first + k] == MPT_PRIM_TRIANGLE)
-                        return fail(MPT_E_UNSUPPORTED, "triangles must be in a mesh object");
+                    if (s.prim_kind[r.first + k] == MPT_PRIM_TRIANGLE || s.prim_kind[r.first + k] == MPT_PRIM_VOXBLOCK)
+                        return fail(MPT_E_UNSUPPORTED, "triangles and voxel blocks must be in a mesh or voxel object");
                     b = box_merge(b, pbox[r.first + k]);
                 }
                 break;
-            case MPT_OBJ_MESH: {
+            case MPT_OBJ_MESH:
+            case MPT_OBJ_VOXEL: {  // a BVH of triangles (mesh.h:175-191) or of voxel blocks (voxel.h:379)
+                const int32_t want = r.kind == MPT_OBJ_MESH ? MPT_PRIM_TRIANGLE : MPT_PRIM_VOXBLOCK;
                 for (int64_t k = 0; k < r.count; k++)
-                    if (s.prim_kind[r.first + k] != MPT_PRIM_TRIANGLE)
-                        return fail(MPT_E_INVALID, "mesh objects hold triangles only");
+                    if (s.prim_kind[r.first + k] != want)
+                        return fail(MPT_E_INVALID, r.kind == MPT_OBJ_MESH ? "mesh objects hold triangles only"
+                                                                        : "voxel objects hold voxel blocks only");
+                if (r.first < 0 || r.count < 1 || r.first + r.count > np)
+                    return fail(MPT_E_INVALID, "object " + std::to_string(o) + " has a bad primitive range");
                 auto key = std::make_pair(r.first, r.count);
                 auto it = mesh_roots.find(key);
                 if (it == mesh_roots.end()) {
@@ -690,8 +744,8 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
                 if (r.bkind == MPT_OBJ_PRIM && r.count != 1)
                     return fail(MPT_E_INVALID, "medium boundary prim with count != 1");
                 for (int64_t k = 0; k < r.count; k++) {
-                    if (s.prim_kind[r.first + k] == MPT_PRIM_TRIANGLE)
-                        return fail(MPT_E_UNSUPPORTED, "mesh boundaries of media are not supported");
+                    if (s.prim_kind[r.first + k] == MPT_PRIM_TRIANGLE || s.prim_kind[r.first + k] == MPT_PRIM_VOXBLOCK)
+                        return fail(MPT_E_UNSUPPORTED, "mesh and voxel boundaries of media are not supported");
                     b = box_merge(b, pbox[r.first + k]);
                 }
                 break;
@@ -705,32 +759,37 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         if (s.prim_canon[i] < 0) s.prim_canon[i] = 0;  // fixed below for non-mesh ranges
     for (int o = 0; o < no; o++) {
         const obj_rec& r = s.objects[o];
-        if (r.kind != MPT_OBJ_MESH)
+        if (r.kind != MPT_OBJ_MESH && r.kind != MPT_OBJ_VOXEL)
             for (int64_t k = 0; k < r.count; k++) s.prim_canon[r.first + k] = k;
     }
 
-    // top-level BVH over the world list (main.cpp:70, bvh.h:24-43)
-    std::vector<build_item> items(no);
+    // top-level BVH over the world list (main.cpp:70, bvh.h:24-43); objects
+    // that are only lights stay out of it
+    std::vector<build_item> items;
     for (int o = 0; o < no; o++) {
-        items[o].box = obox[o];
-        items[o].centroid = box_centroid(obox[o]);
-        items[o].ref = o;
+        if (s.objects[o].flags & MPT_OBJF_LIGHT_ONLY) continue;
+        build_item it;
+        it.box = obox[o];
+        it.centroid = box_centroid(obox[o]);
+        it.ref = o;
+        items.push_back(it);
     }
+    const int nw = int(items.size());
     s.ctlas.clear();
     s.tlas.clear();
-    if (no > 0) {  // an empty world has no tree: the walk never starts (nobj = 0)
+    if (nw > 0) {  // an empty world has no tree: the walk never starts
         build_bvh(items, s.tlas);
         pair_layout(s.tlas, 0, s.ctlas, s.ctlas_root);
     }
     s.qtlas.clear();
     s.qblas.clear();
-    if (no > 0 && s.ctlas_root.cnt == 0) s.ctlas_root.qoff = quad_layout(s.ctlas, s.ctlas_root.off, s.qtlas);
+    if (nw > 0 && s.ctlas_root.cnt == 0) s.ctlas_root.qoff = quad_layout(s.ctlas, s.ctlas_root.off, s.qtlas);
     for (croot& r : s.cblas_roots)
         if (r.cnt == 0) r.qoff = quad_layout(s.cblas, r.off, s.qblas);
-    s.tlas_items.resize(no);
-    for (int o = 0; o < no; o++) s.tlas_items[o] = int32_t(items[o].ref);
+    s.tlas_items.resize(nw);
+    for (int k = 0; k < nw; k++) s.tlas_items[k] = int32_t(items[k].ref);
     s.world_box = box_empty();
-    for (int o = 0; o < no; o++) s.world_box = box_merge(s.world_box, obox[o]);
+    for (int k = 0; k < nw; k++) s.world_box = box_merge(s.world_box, obox[items[k].ref]);
 
     // environment (environment.h:140-177)
     s.env_w = d.env_width;
@@ -794,6 +853,8 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         if (ob >= 0) {
             const obj_rec& r = s.objects[ob];
             if (r.kind == MPT_OBJ_MEDIUM) return fail(MPT_E_UNSUPPORTED, "media cannot be lights");
+            if (r.kind == MPT_OBJ_VOXEL)
+                return fail(MPT_E_UNSUPPORTED, "a voxel grid cannot be a light: its emissive faces are (scene.h:1118-1128)");
             if (r.kind == MPT_OBJ_MESH && s.objects[ob].light_table < 0) {
                 // the mesh's triangles in its own order and their area CDF
                 // (mesh.h:180-186); instances of one mesh share the table

@@ -95,7 +95,7 @@ struct obj_rec {
     int32_t bkind;       // media: boundary kind (MPT_OBJ_PRIM or MPT_OBJ_LIST)
     int32_t light_table; // mesh lights: offset of its triangles in mlight_prim / mlight_cdf (else -1)
     int32_t alpha;       // meshes: alpha-mask image texture (mesh.h:82-86), -1 when opaque
-    int32_t pad;
+    int32_t flags;       // MPT_OBJF_*
 };
 
 // A height fog (height_fog, medium.h:98-155): density at y = base, scale
@@ -144,6 +144,8 @@ struct host_scene {
     bool rich = false;                   // uses_uv, thin glass, a measured metal or the ground: the variant with them
     int32_t ground_mat = -1;             // the ground plane's material, -1 without one
     double ground_h = 0;
+    std::vector<mpt_vox_model> vox;      // voxel models (their palettes in scene materials)
+    std::vector<uint8_t> vox_cells;      // every model's cells
     std::vector<fog_rec> fogs;           // height fogs, tested after the BVH and before the ground
     // the sun (sun.h:26-34): its frame around the direction, the cone's
     // cosine and solid angle, and radiance = irradiance / solid angle

@@ -22,7 +22,8 @@ class DescBuilder:
     def __init__(self):
         self.prim_kind, self.prim_mat, self.prim_data = [], [], []
         self.obj_kind, self.obj_first, self.obj_count, self.obj_xform = [], [], [], []
-        self.obj_density, self.obj_mat, self.obj_bkind = [], [], []
+        self.obj_density, self.obj_mat, self.obj_bkind, self.obj_flags = [], [], [], []
+        self.vox_models, self.vox_cells = [], []
         self.xforms, self.materials, self.textures = [], [], []
         self.lights = []
         self.light_strategy = 1
@@ -166,6 +167,7 @@ class DescBuilder:
         self.obj_density.append(density)
         self.obj_mat.append(mat)
         self.obj_bkind.append(bkind)
+        self.obj_flags.append(0)
         return len(self.obj_kind) - 1
 
     def xform(self, m, t):
@@ -234,6 +236,58 @@ class DescBuilder:
     def light(self, obj):
         self.lights.append(obj)
 
+    def light_only(self, obj):
+        """Marks obj as sampled by next event estimation but never hit (it
+        stays out of the world BVH) and adds it to the lights."""
+        self.obj_flags[obj] |= abi.OBJF_LIGHT_ONLY
+        self.lights.append(obj)
+
+    # ---------------------------------------------------------------- voxels (voxel.h)
+    def vox_model(self, cells, kinds, materials, edge_material=-1, edge_width=0.0):
+        """A voxel model: cells a (nz, ny, nx) array of palette indices (x
+        fastest in memory), kinds[256] 0 empty / 1 opaque / 2 transparent,
+        materials[256] material indices (-1 where no cell uses the entry).
+        Returns its index."""
+        c = np.ascontiguousarray(np.asarray(cells, dtype=np.uint8))
+        assert c.ndim == 3
+        m = abi.VoxModel()
+        m.nz, m.ny, m.nx = c.shape
+        m.cells = sum(x.size for x in self.vox_cells)
+        m.edge_material = edge_material
+        m.edge_width = edge_width
+        for i in range(256):
+            m.material[i] = int(materials[i])
+            m.kind[i] = int(kinds[i])
+        self.vox_cells.append(c.ravel())
+        self.vox_models.append(m)
+        return len(self.vox_models) - 1
+
+    def voxel_blocks(self, model, block=8):
+        """voxel_blocks (voxel.h:341-382): the model cut into blocks of up to
+        block^3 cells, each shrunk to the box of its voxels, empty blocks
+        skipped, as MPT_PRIM_VOXBLOCK primitives; returns (first, count)."""
+        m = self.vox_models[model]
+        off = m.cells
+        cells = np.concatenate(self.vox_cells)[off:off + m.nx * m.ny * m.nz].reshape(m.nz, m.ny, m.nx)
+        kinds = np.array([m.kind[i] for i in range(256)])
+        solid = kinds[cells] != 0
+        first, count = len(self.prim_kind), 0
+        for z0 in range(0, m.nz, block):
+            for y0 in range(0, m.ny, block):
+                for x0 in range(0, m.nx, block):
+                    sub = solid[z0:z0 + block, y0:y0 + block, x0:x0 + block]
+                    if not sub.any():
+                        continue
+                    zz, yy, xx = np.nonzero(sub)
+                    lo = [x0 + xx.min(), y0 + yy.min(), z0 + zz.min()]
+                    hi = [x0 + xx.max() + 1, y0 + yy.max() + 1, z0 + zz.max() + 1]
+                    self._prim(abi.PRIM_VOXBLOCK, -1, [*lo, *hi, model])
+                    count += 1
+        return first, count
+
+    def voxel(self, first, count, xform=-1):
+        return self._obj(abi.OBJ_VOXEL, first, count, xform)
+
     def environment(self, rgb_image, intensity=1.0, rotation=0.0, as_light=True):
         img = np.ascontiguousarray(rgb_image, dtype=np.float64)
         assert img.ndim == 3 and img.shape[2] == 3
@@ -270,6 +324,16 @@ class DescBuilder:
         d.object_density = arr("od", self.obj_density, np.float64, C.c_double)
         d.object_material = arr("om", self.obj_mat, np.int32, C.c_int32)
         d.object_boundary_kind = arr("ob", self.obj_bkind, np.int32, C.c_int32)
+        if any(self.obj_flags):
+            d.object_flags = arr("ofl", self.obj_flags, np.int32, C.c_int32)
+        if self.vox_models:
+            vm = (abi.VoxModel * len(self.vox_models))(*self.vox_models)
+            keep["vm"] = vm
+            d.n_vox_models = len(self.vox_models)
+            d.vox_models = C.cast(vm, C.POINTER(abi.VoxModel))
+            cells = np.concatenate(self.vox_cells)
+            d.vox_cells = arr("vc", cells, np.uint8, C.c_uint8)
+            d.n_vox_cells = cells.size
         xf = (abi.Xform * max(1, len(self.xforms)))(*self.xforms)
         mats = (abi.Material * max(1, len(self.materials)))(*self.materials)
         texs = (abi.Texture * max(1, len(self.textures)))(*self.textures)

@@ -1352,6 +1352,127 @@ struct mesh_obj : hittable {
     }
 };
 
+// A block of a voxel model (voxel_grid, voxel.h:168-337): the cells [lo, hi)
+// walked along the ray with a 3D DDA (Amanatides and Woo 1987); a surface is
+// a plane crossing between cells of different kinds (voxel.h:103-111). The
+// model's whole grid is looked up, so a block boundary is no surface.
+struct vox_block : hittable {
+    const mpt_vox_model* m = nullptr;
+    const uint8_t* cells = nullptr;
+    const material* mats[256] = {};
+    const material* edge = nullptr;
+    double edge_width = 0;
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    int64_t id = 0;
+
+    int at(const int c[3]) const {  // palette index, 0 outside the model
+        if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= m->nx || c[1] >= m->ny || c[2] >= m->nz) return 0;
+        return cells[(size_t(c[2]) * m->ny + c[1]) * m->nx + c[0]];
+    }
+    static bool face(int k, int n) { return k != 0 && (n == 0 || (k == 1 && n == 2)); }
+
+    bool hit(const ray& r, interval ray_t, hit_record& rec, const hit_ctx&) const override {
+        const point3& o = r.orig;
+        const vec3& d = r.dir;
+        double t_enter = -infinity, t_exit = infinity;
+        int enter_axis = 0;
+        for (int a = 0; a < 3; a++) {
+            if (d[a] == 0) {
+                if (o[a] < lo[a] || o[a] > hi[a]) return false;
+                continue;
+            }
+            double t0 = (lo[a] - o[a]) / d[a], t1 = (hi[a] - o[a]) / d[a];
+            if (t0 > t1) std::swap(t0, t1);
+            if (t0 > t_enter) {
+                t_enter = t0;
+                enter_axis = a;
+            }
+            t_exit = std::fmin(t_exit, t1);
+        }
+        const bool outside = t_enter >= ray_t.min;
+        const double t_start = outside ? t_enter : ray_t.min;
+        if (t_start > std::fmin(t_exit, ray_t.max)) return false;
+        const point3 start = r.at(t_start);
+        int c[3], step[3];
+        double t_next[3], t_delta[3];
+        for (int a = 0; a < 3; a++) {
+            c[a] = std::clamp(int(std::floor(start[a])), lo[a], hi[a] - 1);
+            if (outside && a == enter_axis) c[a] = d[a] > 0 ? lo[a] : hi[a] - 1;
+            if (d[a] > 0) {
+                step[a] = 1, t_next[a] = (c[a] + 1 - o[a]) / d[a], t_delta[a] = 1 / d[a];
+            } else if (d[a] < 0) {
+                step[a] = -1, t_next[a] = (c[a] - o[a]) / d[a], t_delta[a] = -1 / d[a];
+            } else {
+                step[a] = 0, t_next[a] = t_delta[a] = infinity;
+            }
+        }
+        int index = at(c);
+        if (outside) {
+            int behind[3] = {c[0], c[1], c[2]};
+            behind[enter_axis] -= step[enter_axis];
+            if (crossing(r, t_enter, enter_axis, step[enter_axis], at(behind), index, c, rec)) return true;
+        }
+        for (;;) {
+            const int a = t_next[0] < t_next[1] ? (t_next[0] < t_next[2] ? 0 : 2) : (t_next[1] < t_next[2] ? 1 : 2);
+            const double t = t_next[a];
+            if (t > ray_t.max || step[a] == 0) return false;  // no plane ahead: the reference would loop
+            c[a] += step[a];
+            t_next[a] += t_delta[a];
+            const int next = at(c);
+            if (crossing(r, t, a, step[a], index, next, c, rec)) return true;
+            if (c[a] < lo[a] || c[a] >= hi[a]) return false;
+            index = next;
+        }
+    }
+    // into cell to_cell (index to) from index from across the plane of axis at t
+    bool crossing(const ray& r, double t, int axis, int step, int from, int to, const int to_cell[3],
+                  hit_record& rec) const {
+        const int kf = m->kind[from], kt = m->kind[to];
+        if (kt != 0 && face(kt, kf)) return record(r, t, axis, -step, to, to_cell, rec);
+        if (face(kf, kt)) {
+            int from_cell[3] = {to_cell[0], to_cell[1], to_cell[2]};
+            from_cell[axis] -= step;
+            return record(r, t, axis, step, from, from_cell, rec);
+        }
+        return false;
+    }
+    bool on_edge(const point3& p, int axis, int sign, const int cell[3]) const {
+        const int k = m->kind[at(cell)];
+        for (int side = 1; side <= 2; side++) {
+            const int tg = (axis + side) % 3;
+            const double f = p[tg] - cell[tg];
+            for (int dir : {-1, 1}) {
+                if ((dir > 0 ? 1 - f : f) >= edge_width) continue;
+                int beside[3] = {cell[0], cell[1], cell[2]};
+                beside[tg] += dir;
+                int above[3] = {beside[0], beside[1], beside[2]};
+                above[axis] += sign;
+                if (face(k, m->kind[at(beside)]) || m->kind[at(above)] != 0) return true;
+            }
+        }
+        return false;
+    }
+    bool record(const ray& r, double t, int axis, int sign, int index, const int cell[3], hit_record& rec) const {
+        rec.t = t;
+        rec.p = r.at(t);
+        vec3 outward(0, 0, 0), tb(0, 0, 0), tc(0, 0, 0);
+        outward[axis] = sign;
+        tb[(axis + 1) % 3] = 1;
+        tc[(axis + 2) % 3] = 1;
+        rec.dpdu = sign > 0 ? tb : tc;
+        rec.dpdv = sign > 0 ? tc : tb;
+        rec.u = dot(rec.p, rec.dpdu) - std::floor(dot(rec.p, rec.dpdu));
+        rec.v = dot(rec.p, rec.dpdv) - std::floor(dot(rec.p, rec.dpdv));
+        rec.mat = edge_width > 0 && on_edge(rec.p, axis, sign, cell) ? edge : mats[index];
+        rec.set_face_normal(r, outward);
+        rec.id = id;
+        return true;
+    }
+    aabb bounding_box() const override {
+        return aabb(point3(lo[0], lo[1], lo[2]), point3(hi[0], hi[1], hi[2]));
+    }
+};
+
 // A world-list object that owns its ID range (lists of an instanced box are
 // re-identified per instance by offsetting the child IDs).
 struct idshift : hittable {
@@ -1572,6 +1693,7 @@ struct scene {
     std::vector<std::unique_ptr<material>> mats;
     std::vector<std::unique_ptr<hittable>> owned;
     std::vector<const hittable*> world;      // world-list objects (with their IDs)
+    std::vector<const hittable*> items;      // those the BVH holds (not the lights that are only sampled)
     std::unique_ptr<bvh> tree;
     std::vector<const hittable*> lights;     // nullptr entries are the environment
     std::vector<double> prob, cdf;
@@ -1795,6 +1917,7 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
         }
     };
     std::vector<std::pair<int64_t, const hittable*>> meshes;  // (first, mesh) of meshes already built
+    std::vector<const hittable*> world_items;                   // the objects the BVH is built over
     for (int o = 0; o < d.n_objects; o++) {
         int64_t first = d.object_first[o], count = d.object_count[o];
         int64_t idb = d.object_id_base ? d.object_id_base[o] : 0;
@@ -1839,6 +1962,34 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
                 h = own(s, std::make_unique<idshift>(tree, shift));
                 break;
             }
+            case MPT_OBJ_VOXEL: {  // voxel_blocks (voxel.h:341-382): its blocks in a BVH
+                const hittable* tree = nullptr;
+                for (auto& mm : meshes)
+                    if (mm.first == first) tree = mm.second;
+                if (!tree) {
+                    std::vector<const hittable*> blocks;
+                    for (int64_t k = 0; k < count; k++) {
+                        const double* q = d.prim_data + 9 * (first + k);
+                        if (d.prim_kind[first + k] != MPT_PRIM_VOXBLOCK || q[6] < 0 || q[6] >= d.n_vox_models) {
+                            err = "voxel objects hold voxel blocks of known models";
+                            return MPT_E_INVALID;
+                        }
+                        auto vb = std::make_unique<vox_block>();
+                        vb->m = &d.vox_models[int(q[6])];
+                        vb->cells = d.vox_cells + vb->m->cells;
+                        for (int i = 0; i < 256; i++) vb->mats[i] = matp(vb->m->material[i]);
+                        vb->edge_width = vb->m->edge_width;
+                        vb->edge = vb->edge_width > 0 ? matp(vb->m->edge_material) : nullptr;
+                        for (int a = 0; a < 3; a++) vb->lo[a] = int(q[a]), vb->hi[a] = int(q[3 + a]);
+                        vb->id = k;
+                        blocks.push_back(own(s, std::move(vb)));
+                    }
+                    tree = own(s, std::make_unique<bvh>(blocks));
+                    meshes.push_back({first, tree});
+                }
+                h = own(s, std::make_unique<idshift>(tree, idb));
+                break;
+            }
             case MPT_OBJ_MEDIUM: {
                 const hittable* b;
                 if (d.object_boundary_kind[o] == MPT_OBJ_PRIM) b = prim(first, idb);
@@ -1879,9 +2030,13 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
             h = own(s, std::make_unique<instance>(h, a));
         }
         s.world.push_back(h);
-        s.world_box = aabb(s.world_box, h->bounding_box());
+        if (!(d.object_flags && (d.object_flags[o] & MPT_OBJF_LIGHT_ONLY))) {  // sampled only, never hit
+            world_items.push_back(h);
+            s.world_box = aabb(s.world_box, h->bounding_box());
+        }
     }
-    s.tree = std::make_unique<bvh>(s.world);
+    s.tree = std::make_unique<bvh>(world_items);
+    s.items = world_items;
     for (int k = 0; k < d.n_fogs; k++) {
         auto f = std::make_unique<height_fog>();
         f->density = d.fog_params[3 * k];
@@ -1896,7 +2051,9 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
         g->height = d.ground_height;
         g->mat = matp(d.ground_material);
         g->id = 0;  // one past the last canonical ID
-        for (int o = 0; o < d.n_objects; o++) g->id = std::max(g->id, d.object_id_base[o] + d.object_count[o]);
+        for (int o = 0; o < d.n_objects; o++)
+            if (!(d.object_flags && (d.object_flags[o] & MPT_OBJF_LIGHT_ONLY)))
+                g->id = std::max(g->id, d.object_id_base[o] + d.object_count[o]);
         s.ground = own(s, std::move(g));
     }
 
@@ -2551,7 +2708,7 @@ int32_t orc_trace_linear(orc_scene* S, const double* rays, const double* tt, int
         bool any = false;
         double closest = tt[2 * i + 1];
         hit_ctx hc{0, 0, false};
-        std::vector<const hittable*> all(S->s.world);
+        std::vector<const hittable*> all(S->s.items);
         if (S->s.ground) all.push_back(S->s.ground);
         for (auto* o : all) {
             hit_record rec;
@@ -2589,6 +2746,8 @@ static void expand_leaves(const hittable* h, std::vector<const affine*> xf, int6
         for (auto* t : m->tris) out.push_back({t, xf, shift});
     } else if (auto s = dynamic_cast<const idshift*>(h)) {
         expand_leaves(s->inner, xf, shift + s->shift, out);
+    } else if (auto b = dynamic_cast<const bvh*>(h)) {  // a voxel model's blocks
+        for (auto* o : b->prims) expand_leaves(o, xf, shift, out);
     } else if (dynamic_cast<const medium*>(h)) {
         return;
     } else {
@@ -2599,7 +2758,7 @@ static void expand_leaves(const hittable* h, std::vector<const affine*> xf, int6
 int32_t orc_trace_ties(orc_scene* S, const double* rays, const double* tt, int64_t n, int64_t* id, double* t,
                        int32_t* ties, int64_t* tie_min, int64_t* tie_max) {
     std::vector<leaf_prim> leaves;
-    for (auto* o : S->s.world) expand_leaves(o, {}, 0, leaves);
+    for (auto* o : S->s.items) expand_leaves(o, {}, 0, leaves);
     if (S->s.ground) leaves.push_back({S->s.ground, {}, 0});
     for (int64_t i = 0; i < n; i++) {
         ray r(point3(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]), vec3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]));

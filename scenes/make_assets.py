@@ -28,6 +28,15 @@ leaves.obj / .mtl    alpha-masked quads: "leaf" takes map_d leaf_mask.png (grey 
                      diffuse texture's alpha is the mask), "board" stays opaque;
 leaf_mask.png        64 x 64 grey + alpha (PNG color type 4);
 fence_rgba.png       32 x 32 RGBA slats with transparent gaps (PNG color type 6);
+island.vox           a MagicaVoxel (version 200) island: a 40 x 40 x 18 terrain
+                     model (sand, grass, rock, a glass pond, a metal post,
+                     an emissive lantern) and a 5 x 5 x 6 tower placed three
+                     times by the scene graph (nTRN translations and ROTATION
+                     bytes, one mirrored, one hidden), an RGBA palette and
+                     MATL materials (_glass, _metal, _emit);
+plain150.vox         a version-150 file with two models (a row of 255 voxels of
+                     palette indices 1-255, and a small column), no scene graph
+                     and no RGBA chunk (MagicaVoxel's default palette);
 globe.jpg            a 128 x 64 continents-and-ocean color map, baseline JPEG
                      with 2x2-subsampled chroma. JPEG needs an encoder: this
                      one is written by stb_image_write through the compiled
@@ -372,6 +381,117 @@ def globe_jpg(w=128, h=64):
     del sys
 
 
+def _chunk(cid, content=b"", children=b""):
+    import struct
+    return cid + struct.pack("<ii", len(content), len(children)) + content + children
+
+
+def _vstr(x):
+    import struct
+    b = x.encode()
+    return struct.pack("<i", len(b)) + b
+
+
+def _vdict(d):
+    import struct
+    return struct.pack("<i", len(d)) + b"".join(_vstr(k) + _vstr(v) for k, v in d.items())
+
+
+def vox_write(path, models, palette=None, materials=None, graph=None, version=200):
+    """A .vox file from the published format (ephtracy/voxel-model):
+    models is a list of ((nx, ny, nz), [(x, y, z, index), ...]); palette 256
+    RGBA tuples (entry k is palette index k + 1) or None for none; materials
+    {index: {key: value}}; graph a list of (node id, chunk id, content bytes)
+    for nTRN / nGRP / nSHP, or None."""
+    import struct
+    body = b""
+    for (nx, ny, nz), vox in models:
+        body += _chunk(b"SIZE", struct.pack("<iii", nx, ny, nz))
+        body += _chunk(b"XYZI", struct.pack("<i", len(vox)) + b"".join(bytes(v) for v in vox))
+    for _, cid, content in graph or []:
+        body += _chunk(cid, content)
+    if palette is not None:
+        body += _chunk(b"RGBA", b"".join(bytes(c) for c in palette))
+    for idx, props in sorted((materials or {}).items()):
+        body += _chunk(b"MATL", struct.pack("<i", idx) + _vdict(props))
+    with open(path, "wb") as f:
+        f.write(b"VOX " + struct.pack("<i", version) + _chunk(b"MAIN", b"", body))
+
+
+def _ntrn(nid, child, t=None, r=None, hidden=False):
+    import struct
+    frame = {}
+    if r is not None:
+        frame["_r"] = str(r)
+    if t is not None:
+        frame["_t"] = " ".join(str(v) for v in t)
+    attrs = {"_hidden": "1"} if hidden else {}
+    return (nid, b"nTRN", struct.pack("<i", nid) + _vdict(attrs) + struct.pack("<iiii", child, -1, 0, 1) + _vdict(frame))
+
+
+def _ngrp(nid, children):
+    import struct
+    return (nid, b"nGRP", struct.pack("<i", nid) + _vdict({}) + struct.pack("<i", len(children)) +
+            b"".join(struct.pack("<i", c) for c in children))
+
+
+def _nshp(nid, model):
+    import struct
+    return (nid, b"nSHP", struct.pack("<i", nid) + _vdict({}) + struct.pack("<ii", 1, model) + _vdict({}))
+
+
+def island():
+    """scenes/models/island.vox and plain150.vox (see the module text)."""
+    # palette: 1 sand, 2 grass, 3 rock, 4 dark rock, 5 glass water, 6 metal,
+    # 7 lantern (emissive), 8 wood, 9 roof, 10 tinted glass window
+    pal = [(0, 0, 0, 255)] * 256
+    cols = {1: (218, 196, 140), 2: (86, 150, 62), 3: (128, 124, 118), 4: (84, 80, 78), 5: (70, 140, 200),
+            6: (200, 200, 210), 7: (255, 200, 120), 8: (120, 80, 50), 9: (170, 60, 50), 10: (180, 220, 255)}
+    for k, c in cols.items():
+        pal[k - 1] = (*c, 255)  # entry k - 1 is palette index k
+    mats = {5: {"_type": "_glass", "_rough": "0.0", "_ior": "0.33", "_att": "0.6"},
+            6: {"_type": "_metal", "_rough": "0.25", "_metal": "1"},
+            7: {"_type": "_emit", "_emit": "2", "_flux": "1"},
+            9: {"_type": "_metal", "_plastic": "1", "_rough": "0.5"},
+            10: {"_type": "_glass", "_rough": "0.2", "_ri": "1.45"},
+            3: {"_type": "_blend"}}  # an unsupported type: diffuse, with a warning
+    nx, ny, nz = 40, 40, 18
+    vox = []
+    for y in range(ny):
+        for x in range(nx):
+            r = math.hypot(x - 19.5, y - 19.5)
+            h = int(10 - 0.45 * r + 2.5 * math.sin(0.45 * x) * math.cos(0.35 * y))
+            for z in range(max(h, 0)):
+                top = z == h - 1
+                idx = 3 if z < h - 3 else (4 if r > 16 else (2 if top and r < 13 else 1))
+                if 9 <= x <= 14 and 22 <= y <= 27 and z >= h - 2 and r < 14:
+                    idx = 5  # a glass pond sunk into the top
+                vox.append((x, y, z, idx))
+    for z in range(8, 14):  # a metal post with a lantern on top
+        vox.append((20, 12, z, 6))
+    vox.append((20, 12, 14, 7))
+    vox.append((21, 12, 14, 7))
+    tower = []
+    for z in range(6):
+        for y in range(5):
+            for x in range(5):
+                edge = x in (0, 4) or y in (0, 4)
+                if z == 5:
+                    tower.append((x, y, z, 9))
+                elif edge:
+                    tower.append((x, y, z, 10 if (z == 3 and x == 2) else 8))
+    graph = [_ntrn(0, 1), _ngrp(1, [2, 4, 6, 8, 10]),
+             _ntrn(2, 3), _nshp(3, 0),
+             _ntrn(4, 5, t=(9, -8, 9), r=4), _nshp(5, 1),               # the tower, upright
+             _ntrn(6, 7, t=(-10, 6, 8), r=1 | (2 << 2)), _nshp(7, 1),   # turned about z (a signed permutation)
+             _ntrn(8, 9, t=(0, 0, 20), r=4 | 16, hidden=True), _nshp(9, 1),  # hidden: not placed
+             _ntrn(10, 11, t=(-4, -12, 9), r=4 | 32), _nshp(11, 1)]    # mirrored in y
+    vox_write(os.path.join(HERE, "island.vox"), [((nx, ny, nz), vox), ((5, 5, 6), tower)], pal, mats, graph)
+    small = [((255, 1, 1), [(x, 0, 0, x + 1) for x in range(255)]),  # every palette index in a row
+             ((2, 2, 3), [(1, 1, z, 200) for z in range(3)])]
+    vox_write(os.path.join(HERE, "plain150.vox"), small, None, None, None, version=150)
+
+
 if __name__ == "__main__":
     os.makedirs(HERE, exist_ok=True)
     room()
@@ -381,3 +501,4 @@ if __name__ == "__main__":
     globe_jpg()
     crate()
     leaves()
+    island()

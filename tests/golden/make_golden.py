@@ -190,6 +190,39 @@ ALPHA_SCENES = {
 }
 
 
+VOXEL_SCENES = {"voxel_island.scene": (64, 128, 8, 12), "voxel_quads.scene": (64, 128, 8, 12)}
+
+
+def golden_voxel():
+    """The voxel scenes through the reference's own .vox loader, palette
+    and voxel_grid walk (or its faces, voxel_quads.scene): the hit subset
+    with t and front face, the surface records (u, v, tangents, base colour:
+    palette or outline material) and the converged means; and MagicaVoxel's
+    default palette as the reference holds it (vox_default_palette), read
+    back through a version-150 file without an RGBA chunk."""
+    for name, render in VOXEL_SCENES.items():
+        # the reference's tie count tests every face of the quads scene per ray:
+        # the subset is drawn from a batch of 2^15
+        golden_asset_scene(name, render, n_batch=1 << 15)
+        path = os.path.join(HERE, name.replace(".scene", ".npz"))
+        g = dict(np.load(path))
+        r = ol.RefScene(scene_path(name))
+        g["surface"] = r.surface(g["rays"], g["tt"])
+        g["light_probabilities"] = r.light_probabilities()
+        np.savez_compressed(path, **g)
+        print(name, "surface", flush=True)
+    # the default palette: straight down onto the row of voxels 1-255 of
+    # plain150.vox (cell x spans world x in [x - 127, x - 126], y in [0, 1])
+    r = ol.RefScene(scene_path("vox_palette.scene"))
+    xs = np.arange(255) - 127 + 0.5
+    rays = np.zeros((255, 6))
+    rays[:, 0], rays[:, 1], rays[:, 2], rays[:, 4] = xs, 5.0, -0.5, -1.0
+    tt = np.tile([0.001, np.inf], (255, 1))
+    np.savez_compressed(os.path.join(HERE, "vox_palette.npz"), rays=rays, tt=tt, surface=r.surface(rays, tt),
+                        trace_t=r.trace(rays, tt)["t"])
+    print("vox_palette", flush=True)
+
+
 def golden_alpha():
     for name, render in ALPHA_SCENES.items():
         golden_asset_scene(name, render)
@@ -245,7 +278,7 @@ def golden_textures():
         print(name, "surface and images", flush=True)
 
 
-def golden_asset_scene(name, render):
+def golden_asset_scene(name, render, n_batch=None):
     """A scene of the repository's own assets through the reference's own
     loaders (load_obj/load_mtl, stb_image's .hdr decoder): the seeded
     1M-ray batch's subset with the reference's hit IDs, t and front face,
@@ -253,11 +286,12 @@ def golden_asset_scene(name, render):
     R = ol.reference()
     d = m.describe(scene_path(name))
     o = ol.OracleScene(d.desc)
-    rays, tt = o.ray_batch(d.camera, BATCH_SEED, N_BATCH)
+    n_batch = n_batch or N_BATCH
+    rays, tt = o.ray_batch(d.camera, BATCH_SEED, n_batch)
     r = ol.RefScene(scene_path(name))
     out = r.trace(rays, tt, threads=os.cpu_count() or 8)
-    sel = np.linspace(0, N_BATCH - 1, N_SUBSET).astype(np.int64)
-    digest = {"seed": BATCH_SEED, "n": N_BATCH,
+    sel = np.linspace(0, n_batch - 1, N_SUBSET).astype(np.int64)
+    digest = {"seed": BATCH_SEED, "n": n_batch,
               "sha256_id": hashlib.sha256(out["id"].astype("<i8").tobytes()).hexdigest(),
               "sha256_t": hashlib.sha256(out["t"].astype("<f8").tobytes()).hexdigest(),
               "sha256_front": hashlib.sha256(out["front"].astype("u1").tobytes()).hexdigest(),
@@ -283,7 +317,7 @@ def main():
         sys.exit("oracle/_ref/libmpt_ref.so is not built (make -C oracle ref, where the reference is present)")
     R = ol.reference()
     steps = set(sys.argv[1:]) or {"ggx", "trace", "ties", "bsdf", "render", "denoise", "display", "obj",
-                                  "textures", "materials", "sun", "fog", "alpha", "coat"}
+                                  "textures", "materials", "sun", "fog", "alpha", "coat", "voxel"}
     if "denoise" in steps:
         golden_denoise()
     if "display" in steps:
@@ -300,6 +334,8 @@ def main():
         golden_fog()
     if "alpha" in steps:
         golden_alpha()
+    if "voxel" in steps:
+        golden_voxel()
     if "ggx" in steps:
         golden_ggx(R)
     if "trace" in steps:

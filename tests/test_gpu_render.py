@@ -259,3 +259,20 @@ def test_guides_from_the_first_extension_equal_their_own_trace(mpt, scene):
     assert np.array_equal(fused.radiance, traced.radiance)
     assert np.array_equal(fused.albedo, traced.albedo) and np.array_equal(fused.normal, traced.normal)
     assert np.abs(fused.normal).sum() > 0
+
+
+def test_materials_only_the_ground_uses_are_shaded(mpt, tmp_path):
+    """The shading kernels a scene launches follow the materials its hits
+    can take, the ground plane's included: a ground whose material no
+    primitive shares renders as the oracle does."""
+    from mypbr_amd.abi import camera_copy
+    from oracle_lib import OracleScene
+    s = tmp_path / "g.scene"
+    s.write_text("image_width 48\naspect_ratio 1\nsamples_per_pixel 4\nmax_depth 8\nbackground 0.7 0.8 1\n"
+                 "camera lookfrom 0 2 6 lookat 0 0.5 0 vfov 40\nmaterial floor lambertian 0.6 0.5 0.4\n"
+                 "material chrome metal 0.9 0.9 0.9 0.1\nground floor\nsphere 0 1 0 1 chrome\n")
+    d = mpt.describe(str(s))
+    cam = camera_copy(d.camera)
+    cpu = OracleScene(d.desc).render(cam, seed=3)["radiance"]
+    f = mpt.Scene.from_desc(d.desc, d.camera).render(cam, seed=3)
+    _compare(f.radiance, cpu, 0.995, 1e-3)
