@@ -89,7 +89,12 @@ enum {
 };
 
 /* Texture types (texture.h:24-105). */
-enum { MPT_TEX_SOLID = 0, MPT_TEX_CHECKER = 1, MPT_TEX_IMAGE = 2 };
+enum { MPT_TEX_SOLID = 0, MPT_TEX_CHECKER = 1, MPT_TEX_IMAGE = 2,
+       /* wrappers of another texture (inner), texture.h:212-282: */
+       MPT_TEX_SCALED = 3,       /* inner's value times even[3] (scaled_texture) */
+       MPT_TEX_CHANNEL = 4,      /* grey of inner's channel `channel` times scale (channel_texture) */
+       MPT_TEX_UV_TRANSFORM = 5, /* inner at (u, v) scaled, rotated, offset (uv_transform_texture) */
+       MPT_TEX_NORMAL_SCALE = 6  /* inner's encoded normal with its tilt times scale (normal_scale_texture) */ };
 /* Image texture flags: 8-bit values are data (k / 255), not sRGB; (u, v)
  * wraps around instead of clamping at the edge; the texels are linear
  * floats (a Radiance .hdr file) instead of bytes. */
@@ -108,6 +113,13 @@ typedef struct mpt_texture {
     int32_t height;
     int64_t offset;      /* image: first texel in mpt_scene_desc.texels (4 bytes, RGBA, per texel) or
                             .texels_hdr (3 floats per texel), rows top first */
+    /* Wrappers: the texture read, and their parameters. A chain is at most a
+     * value wrapper (scaled, channel, normal scale) over uv transforms over a
+     * plain texture, as glTF's texture views give it. */
+    int32_t inner;
+    int32_t channel;     /* channel: 0 red, 1 green, 2 blue */
+    double scale;        /* channel, normal scale */
+    double uv[6];        /* uv transform: offset u, offset v, scale u, scale v, cos and sin of the rotation */
 } mpt_texture;
 
 typedef struct mpt_material {
@@ -143,6 +155,11 @@ typedef struct mpt_material {
     int32_t coat_weight_texture;     /* its weight: red channel, clamped to [0, 1] (a solid texture for a constant) */
     int32_t coat_roughness_texture;  /* its roughness: red channel, clamped to [0, 1] */
     int32_t coat_normal_map;         /* its own tangent-space normal map, -1: the unperturbed normal */
+    /* emissive_surface (material.h:714-741): with glow set, the front face
+     * glows with glow_texture's value while the material scatters as it is;
+     * not a light. */
+    int32_t glow;
+    int32_t glow_texture;
     double coat_tint[3];       /* what the coat lets through head-on, in and out (set_clearcoat_color); 1: clear */
 } mpt_material;
 

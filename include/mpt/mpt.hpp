@@ -105,6 +105,7 @@ class texture {
 public:
     virtual ~texture() = default;
     virtual mpt_texture to_abi() const = 0;
+    virtual std::shared_ptr<texture> inner_texture() const { return nullptr; }  // wrappers: what they read
 };
 class solid_color : public texture {
 public:
@@ -126,6 +127,12 @@ public:
 class image_texture : public texture {
 public:
     explicit image_texture(const std::string& path, bool repeat = false, bool linear = false);
+    // a PNG or JPEG file's bytes in memory (an image inside a glTF file)
+    image_texture(const uint8_t* data, size_t size, bool repeat, bool linear);
+    // a 1 x 1 white image whose alpha is the coverage rounded to a byte: a
+    // constant mask (texture.h:98-106)
+    explicit image_texture(double coverage);
+    bool has_alpha_channel() const { return hdr.empty() && (channels == 2 || channels == 4); }
     bool loaded() const { return width > 0 && height > 0; }
     // a grey (1- or 2-channel) file: a height map rather than a normal map (texture.h:108-110)
     bool greyscale() const { return hdr.empty() && channels <= 2; }
@@ -138,6 +145,45 @@ public:
     std::string error;          // why loading failed
     std::vector<uint8_t> ldr;   // RGBA bytes, rows top first
     std::vector<float> hdr;     // RGB floats, rows top first
+};
+
+// Wrappers of another texture (texture.h:212-282). The core reads at most
+// one value wrapper (scaled, channel, normal scale) over uv transforms over
+// a plain texture: the chains glTF's texture views give.
+class scaled_texture : public texture {  // inner times a colour
+public:
+    scaled_texture(std::shared_ptr<texture> inner, const color& scale) : inner(std::move(inner)), scale(scale) {}
+    mpt_texture to_abi() const override;
+    std::shared_ptr<texture> inner_texture() const override { return inner; }
+    std::shared_ptr<texture> inner;
+    color scale;
+};
+class channel_texture : public texture {  // grey of one channel (0 red, 1 green, 2 blue) times a factor
+public:
+    channel_texture(std::shared_ptr<texture> inner, int channel, double scale)
+        : inner(std::move(inner)), channel(channel), scale(scale) {}
+    mpt_texture to_abi() const override;
+    std::shared_ptr<texture> inner_texture() const override { return inner; }
+    std::shared_ptr<texture> inner;
+    int channel;
+    double scale;
+};
+class uv_transform_texture : public texture {  // KHR_texture_transform: scale, rotate, offset
+public:
+    uv_transform_texture(std::shared_ptr<texture> inner, double offset_u, double offset_v, double rotation,
+                         double scale_u, double scale_v);
+    mpt_texture to_abi() const override;
+    std::shared_ptr<texture> inner_texture() const override { return inner; }
+    std::shared_ptr<texture> inner;
+    double offset_u, offset_v, scale_u, scale_v, cos_r, sin_r;
+};
+class normal_scale_texture : public texture {  // a normal map's tilt times a factor
+public:
+    normal_scale_texture(std::shared_ptr<texture> inner, double scale) : inner(std::move(inner)), scale(scale) {}
+    mpt_texture to_abi() const override;
+    std::shared_ptr<texture> inner_texture() const override { return inner; }
+    std::shared_ptr<texture> inner;
+    double scale;
 };
 
 // ------------------------------------------------------------------ materials
@@ -158,6 +204,8 @@ public:
     virtual std::shared_ptr<texture> coat_roughness_texture() const { return nullptr; }
     virtual std::shared_ptr<texture> coat_normal_texture() const { return nullptr; }
     virtual color average_emission() const { return color(0, 0, 0); }
+    // emissive_surface's glow (null: none)
+    virtual std::shared_ptr<texture> glow_texture() const { return nullptr; }
 };
 // A material whose shading normal is bent by a tangent-space normal map
 // (normal_mapped, material.h:631-674); for lambertian and metal.
@@ -251,6 +299,24 @@ public:
     color average_emission() const override;
     color emit;
     double spread;
+};
+// A material whose front face glows with a texture while it scatters as its
+// inner material (emissive_surface, material.h:714-741): not a light.
+class emissive_surface : public material {
+public:
+    emissive_surface(std::shared_ptr<material> inner, std::shared_ptr<texture> emission)
+        : inner(std::move(inner)), emission(std::move(emission)) {}
+    mpt_material to_abi(int32_t ti) const override { return inner->to_abi(ti); }
+    std::shared_ptr<texture> base_texture() const override { return inner->base_texture(); }
+    std::shared_ptr<texture> normal_map_texture() const override { return inner->normal_map_texture(); }
+    std::shared_ptr<texture> metallic_texture() const override { return inner->metallic_texture(); }
+    std::shared_ptr<texture> roughness_texture() const override { return inner->roughness_texture(); }
+    std::shared_ptr<texture> coat_weight_texture() const override { return inner->coat_weight_texture(); }
+    std::shared_ptr<texture> coat_roughness_texture() const override { return inner->coat_roughness_texture(); }
+    std::shared_ptr<texture> coat_normal_texture() const override { return inner->coat_normal_texture(); }
+    std::shared_ptr<texture> glow_texture() const override { return emission; }
+    std::shared_ptr<material> inner;
+    std::shared_ptr<texture> emission;
 };
 class phase_material : public material {
 public:
@@ -379,6 +445,16 @@ struct obj_part {
 bool load_obj(const std::string& path, const std::map<std::string, std::shared_ptr<material>>& materials,
               std::shared_ptr<material> default_mat, bool smooth, std::vector<obj_part>& parts, std::string& error,
               bool use_normal_maps = true);
+
+// load_gltf (gltf_loader.h:21-399): a glTF 2.0 file (.gltf with its buffers
+// and images, or .glb) as one mesh part per material and coverage mask, the
+// node tree flattened into world-space vertices. override_mat, when set,
+// replaces every material; replace swaps materials by their name in the file.
+// Returns false with a message in error when the file cannot be read or has
+// no triangles.
+bool load_gltf(const std::string& path, std::shared_ptr<material> override_mat, std::vector<obj_part>& parts,
+               std::string& error, bool use_normal_maps = true,
+               const std::map<std::string, std::shared_ptr<material>>& replace = {});
 
 class instance : public hittable {
 public:

@@ -17,7 +17,7 @@ OBJ_PRIM, OBJ_LIST, OBJ_MESH, OBJ_MEDIUM, OBJ_VOXEL = 0, 1, 2, 3, 4
 OBJF_LIGHT_ONLY = 1
 VOX_EMPTY, VOX_OPAQUE, VOX_TRANSPARENT = 0, 1, 2
 MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_PRINCIPLED, MAT_DIFFUSE_LIGHT, MAT_PHASE = range(6)
-TEX_SOLID, TEX_CHECKER, TEX_IMAGE = 0, 1, 2
+TEX_SOLID, TEX_CHECKER, TEX_IMAGE, TEX_SCALED, TEX_CHANNEL, TEX_UV_TRANSFORM, TEX_NORMAL_SCALE = range(7)
 TEXF_LINEAR, TEXF_REPEAT, TEXF_HDR, TEXF_NO_ALPHA = 1, 2, 4, 8
 
 D3 = C.c_double * 3
@@ -25,7 +25,8 @@ D3 = C.c_double * 3
 
 class Texture(C.Structure):
     _fields_ = [("type", C.c_int32), ("flags", C.c_int32), ("inv_scale", C.c_double),
-                ("even", D3), ("odd", D3), ("width", C.c_int32), ("height", C.c_int32), ("offset", C.c_int64)]
+                ("even", D3), ("odd", D3), ("width", C.c_int32), ("height", C.c_int32), ("offset", C.c_int64),
+                ("inner", C.c_int32), ("channel", C.c_int32), ("scale", C.c_double), ("uv", C.c_double * 6)]
 
 
 class Material(C.Structure):
@@ -37,7 +38,8 @@ class Material(C.Structure):
                 ("metallic_texture", C.c_int32), ("roughness_texture", C.c_int32), ("uses_uv", C.c_int32),
                 ("thin", C.c_int32), ("measured", C.c_int32), ("eta", D3), ("k", D3),
                 ("coat", C.c_int32), ("coat_weight_texture", C.c_int32), ("coat_roughness_texture", C.c_int32),
-                ("coat_normal_map", C.c_int32), ("coat_tint", D3)]
+                ("coat_normal_map", C.c_int32), ("glow", C.c_int32), ("glow_texture", C.c_int32),
+                ("coat_tint", D3)]
 
 
 class Xform(C.Structure):

@@ -26,7 +26,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"] + os.environ.get("MPT_DEVICE_FLAGS", "").split()
 
-HOST_SOURCES = ["scene_host.cpp", "scene_api.cpp", "image_io.cpp", "image_decode.cpp"]
+HOST_SOURCES = ["scene_host.cpp", "scene_api.cpp", "gltf_loader.cpp", "image_io.cpp", "image_decode.cpp"]
 DEVICE_SOURCES = ["kernels.hip", "denoise.hip", "learn.hip", "nrc.hip", "display.hip"]
 
 

@@ -19,12 +19,12 @@ struct rstream {
     __device__ __forceinline__ double next(double lo, double hi) { return lo + (hi - lo) * next(); }
 };
 
-// texture::value(u, v, p) (texture.h:24-56, 119-127): solid, the 3D checker,
-// or the image texel nearest to (u, v) (texture.h:182-195). IMG false: the
-// scene has no image textures, and their code is left out.
+// texture::value(u, v, p) (texture.h:24-56, 119-127) of a plain texture:
+// solid, the 3D checker, or the image texel nearest to (u, v)
+// (texture.h:182-195). IMG false: the scene has no image textures, and their
+// code is left out.
 template <bool IMG = true>
-__device__ __forceinline__ v3 tex_value(const dscene& S, int ti, double u, double v, const v3& p) {
-    const mpt_texture& t = S.tex[ti];
+__device__ __forceinline__ v3 tex_plain(const dscene& S, const mpt_texture& t, double u, double v, const v3& p) {
     if (t.type == MPT_TEX_SOLID) return v3{t.even[0], t.even[1], t.even[2]};
     if (!IMG || t.type == MPT_TEX_CHECKER) {
         int x = int(std::floor(t.inv_scale * p.x));
@@ -51,6 +51,32 @@ __device__ __forceinline__ v3 tex_value(const dscene& S, int ti, double u, doubl
     const uchar4 c = *reinterpret_cast<const uchar4*>(S.texels + 4 * k);
     if (t.flags & MPT_TEXF_LINEAR) return v3{double(c.x), double(c.y), double(c.z)} / 255.0;
     return v3{S.srgb[c.x], S.srgb[c.y], S.srgb[c.z]};
+}
+
+// A texture's value, through its wrappers (texture.h:212-282): a value
+// wrapper (scaled, one channel, a normal's tilt) over uv transforms over a
+// plain texture, the chain the host checked.
+template <bool IMG = true>
+__device__ __forceinline__ v3 tex_value(const dscene& S, int ti, double u, double v, const v3& p) {
+    const mpt_texture& t = S.tex[ti];
+    if (!IMG || t.type < MPT_TEX_SCALED) return tex_plain<IMG>(S, t, u, v, p);
+    const mpt_texture* x = t.type == MPT_TEX_UV_TRANSFORM ? &t : &S.tex[t.inner];
+    while (x->type == MPT_TEX_UV_TRANSFORM) {  // scale, rotate, offset in glTF's v-down coordinates
+        const double su = x->uv[2] * u, sv = x->uv[3] * (1 - v);
+        const double tu = x->uv[4] * su + x->uv[5] * sv + x->uv[0];
+        const double tv = -x->uv[5] * su + x->uv[4] * sv + x->uv[1];
+        u = tu;
+        v = 1 - tv;
+        x = &S.tex[x->inner];
+    }
+    const v3 c = tex_plain<IMG>(S, *x, u, v, p);
+    if (t.type == MPT_TEX_SCALED) return v3{t.even[0] * c.x, t.even[1] * c.y, t.even[2] * c.z};
+    if (t.type == MPT_TEX_CHANNEL) {
+        const double g = t.scale * (t.channel == 0 ? c.x : (t.channel == 1 ? c.y : c.z));
+        return v3{g, g, g};
+    }
+    if (t.type == MPT_TEX_NORMAL_SCALE) return v3{0.5 + t.scale * (c.x - 0.5), 0.5 + t.scale * (c.y - 0.5), c.z};
+    return c;
 }
 template <bool IMG = true>
 __device__ __forceinline__ v3 tex_value(const dscene& S, int ti, const hitrec& rec) {

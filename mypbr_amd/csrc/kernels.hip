@@ -843,6 +843,21 @@ __device__ __forceinline__ void shade_one(const dscene& S, const dcam& C, const 
         const v3 gn = rec.normal;
         if (RICH && m.normal_map >= 0) rec.normal = normal_from_map(S, m.normal_map, d, rec);
         if constexpr (MT == MPT_MAT_DIELECTRIC) beta = beta * mat_interior<MT>(m, d, rec);  // camera.h:396
+        if constexpr (RICH && MT != MPT_MAT_DIFFUSE_LIGHT) {
+            // emissive_surface::emitted (material.h:731-736) added as camera.h:399-407 adds
+            // any emission; the surface then scatters as its inner material
+            if (m.glow && rec.front && !seen) {
+                const v3 e = tex_value(S, m.glow_texture, rec);
+                if (e.x > 0 || e.y > 0 || e.z > 0) {
+                    if (!LIGHTS || specular) {
+                        rad = rad + beta * e;
+                    } else {
+                        double lp = lights_pdf<LK>(S, ld3(P, D_PX, p), d);
+                        rad = rad + beta * e * power_heuristic(pd(P, D_SPDF, p), lp);
+                    }
+                }
+            }
+        }
         if constexpr (MT == MPT_MAT_DIFFUSE_LIGHT) {                                          // camera.h:399-407
             v3 e = mat_emitted<MT>(m, d, rec);
             if (!seen && (e.x > 0 || e.y > 0 || e.z > 0)) {
@@ -998,6 +1013,8 @@ __device__ __forceinline__ void shadow_one(const dscene& S, const dcam& C, const
                 continue;
             }
             emission = mat_emitted(m, dir, rec);
+            if constexpr ((FEAT & F_UV) != 0)  // a glowing surface in the way gives its glow (camera.h:500)
+                if (m.glow && rec.front) emission = emission + tex_value(S, m.glow_texture, rec);
             break;
         }
         emission = through * emission;
@@ -1567,7 +1584,7 @@ int32_t mpt_scene_create(const mpt_scene_desc* desc, mpt_scene** out) {
     d.nfog = int32_t(h.fogs.size());
     d.shadow_occlusion = 1;
     for (const mpt_material& m : h.materials)
-        if (m.type == MPT_MAT_DIFFUSE_LIGHT || mat_lets_shadows(m)) d.shadow_occlusion = 0;
+        if (m.type == MPT_MAT_DIFFUSE_LIGHT || m.glow || mat_lets_shadows(m)) d.shadow_occlusion = 0;
     d.has_sun = h.has_sun;
     d.sun_frame = h.sun_frame;
     d.sun_cos_max = h.sun_cos_max;

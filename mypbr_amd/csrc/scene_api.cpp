@@ -199,6 +199,54 @@ image_texture::image_texture(const std::string& path, bool rep, bool lin) : repe
     ldr = std::move(im.rgba);
 }
 
+image_texture::image_texture(const uint8_t* data, size_t size, bool rep, bool lin) : repeat(rep), linear(lin) {
+    const std::vector<uint8_t> bytes(data, data + size);
+    ldr_image im;
+    if (!decode_ldr_image(bytes, im, error)) return;
+    width = im.width;
+    height = im.height;
+    channels = im.channels;
+    ldr = std::move(im.rgba);
+}
+
+image_texture::image_texture(double coverage) : repeat(false), linear(false) {
+    const auto a = static_cast<uint8_t>(std::lround(255 * std::clamp(coverage, 0.0, 1.0)));
+    width = height = 1;
+    channels = 4;
+    ldr = {255, 255, 255, a};
+}
+
+mpt_texture scaled_texture::to_abi() const {
+    mpt_texture t{};
+    t.type = MPT_TEX_SCALED;
+    for (int k = 0; k < 3; k++) t.even[k] = scale[k];
+    return t;
+}
+mpt_texture channel_texture::to_abi() const {
+    mpt_texture t{};
+    t.type = MPT_TEX_CHANNEL;
+    t.channel = channel;
+    t.scale = scale;
+    return t;
+}
+uv_transform_texture::uv_transform_texture(std::shared_ptr<texture> in, double ou, double ov, double rotation, double su,
+                                           double sv)
+    : inner(std::move(in)), offset_u(ou), offset_v(ov), scale_u(su), scale_v(sv), cos_r(std::cos(rotation)),
+      sin_r(std::sin(rotation)) {}
+mpt_texture uv_transform_texture::to_abi() const {
+    mpt_texture t{};
+    t.type = MPT_TEX_UV_TRANSFORM;
+    const double uv[6] = {offset_u, offset_v, scale_u, scale_v, cos_r, sin_r};
+    for (int k = 0; k < 6; k++) t.uv[k] = uv[k];
+    return t;
+}
+mpt_texture normal_scale_texture::to_abi() const {
+    mpt_texture t{};
+    t.type = MPT_TEX_NORMAL_SCALE;
+    t.scale = scale;
+    return t;
+}
+
 bool image_texture::fully_opaque() const {
     if (!hdr.empty()) return true;
     const size_t mask = (channels == 2 || channels == 4) ? 3 : 0;
@@ -423,6 +471,8 @@ int32_t flat_builder::texture_index(const std::shared_ptr<texture>& t) {
     auto jt = p->tex_index.find(t.get());
     if (jt != p->tex_index.end()) return jt->second;
     mpt_texture a = t->to_abi();
+    a.inner = -1;
+    if (auto in = t->inner_texture()) a.inner = texture_index(in);  // placed before the wrapper
     if (auto im = dynamic_cast<const image_texture*>(t.get())) {
         if (!im->loaded()) throw std::runtime_error("image texture not loaded: " + im->error);
         if (!im->hdr.empty()) {
@@ -453,6 +503,13 @@ int32_t flat_builder::material_index(const std::shared_ptr<material>& m) {
         auto cr = m->coat_roughness_texture();
         a.coat_roughness_texture = texture_index(cr ? cr : std::make_shared<solid_color>(color(0, 0, 0)));
         a.coat_normal_map = texture_index(m->coat_normal_texture());
+    }
+    if (auto g = m->glow_texture()) {
+        a.glow = 1;
+        a.glow_texture = texture_index(g);
+    } else {
+        a.glow = 0;
+        a.glow_texture = -1;
     }
     a.uses_uv = (a.normal_map >= 0 || a.coat_normal_map >= 0 || image(m->base_texture()) ||
                  image(m->metallic_texture()) || image(m->roughness_texture()) ||
@@ -2451,18 +2508,38 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
             auto ends_with = [&](const std::string& suffix) {
                 return file.size() >= suffix.size() && file.compare(file.size() - suffix.size(), suffix.size(), suffix) == 0;
             };
-            if (ends_with(".gltf") || ends_with(".glb")) return fail("glTF files are not modelled by the GPU path");
-            if (rest.find("replace") != std::string::npos)
-                return fail("'replace' is for glTF files (OBJ files take scene materials by usemtl name)");
+            const bool gltf = ends_with(".gltf") || ends_with(".glb");
+            // then any number of "replace <material in the file> <scene material>" (glTF)
+            std::map<std::string, std::shared_ptr<material>> replace;
+            std::string replace_key, token;
+            std::streampos mark = tail.tellg();
+            while (tail >> token && token == "replace") {
+                std::string from, to;
+                if (!(tail >> from >> to)) return fail("expected 'replace <material in the file> <scene material>'");
+                auto it = materials.find(to);
+                if (it == materials.end()) return fail("unknown material '" + to + "'");
+                replace[from] = it->second;
+                replace_key += from + '\n' + to + '\n';
+                mark = tail.tellg();
+            }
+            tail.clear();
+            tail.seekg(mark);
             std::shared_ptr<material> mat;
             transform xform;
             bool transformed;
             if (!read_tail(tail, mat, xform, transformed, error)) return fail(error);
-            auto key = std::make_tuple(file, smooth, mat.get(), use_normal_maps);
+            if (!replace.empty() && mat != default_material)
+                return fail("'replace' and a material for the whole mesh can't be used together");
+            if (!gltf && !replace.empty())
+                return fail("'replace' is for glTF files (OBJ files take scene materials by usemtl name)");
+            auto key = std::make_tuple(file + '\n' + replace_key, smooth, mat.get(), use_normal_maps);
             auto cached = mesh_cache.find(key);
             if (cached == mesh_cache.end()) {
                 std::vector<obj_part> parts;
-                if (!load_obj(file, materials, mat, smooth, parts, error, use_normal_maps)) return fail(error);
+                // glTF files bring their own materials: one named on the line replaces them all
+                if (gltf ? !load_gltf(file, mat == default_material ? nullptr : mat, parts, error, use_normal_maps, replace)
+                         : !load_obj(file, materials, mat, smooth, parts, error, use_normal_maps))
+                    return fail(error);
                 cached = mesh_cache.emplace(key, std::move(parts)).first;
             }
             for (const auto& part : cached->second)

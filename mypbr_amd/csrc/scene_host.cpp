@@ -421,6 +421,21 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
             const int64_t have = hdr ? d.n_texels_hdr / 3 : d.n_texel_bytes / 4;
             if (t.width < 1 || t.height < 1 || t.offset < 0 || t.offset + int64_t(t.width) * t.height > have)
                 return fail(MPT_E_INVALID, "image texture " + std::to_string(k) + " lies outside the texel arrays");
+        } else if (t.type >= MPT_TEX_SCALED && t.type <= MPT_TEX_NORMAL_SCALE) {
+            // a value wrapper over uv transforms over a plain texture, at most four deep
+            int32_t cur = int32_t(k), depth = 0;
+            bool value_seen = false;
+            while (s.textures[size_t(cur)].type >= MPT_TEX_SCALED) {
+                const mpt_texture& w = s.textures[size_t(cur)];
+                const bool value = w.type != MPT_TEX_UV_TRANSFORM;
+                if ((value && (value_seen || depth > 0)) || ++depth > 4 || w.inner < 0 || w.inner >= d.n_textures)
+                    return fail(MPT_E_INVALID, "texture " + std::to_string(k) + ": a wrapper chain must be one value "
+                                               "wrapper over uv transforms over a texture, at most 4 deep");
+                if (w.type == MPT_TEX_CHANNEL && (w.channel < 0 || w.channel > 2))
+                    return fail(MPT_E_INVALID, "texture " + std::to_string(k) + ": channel must be 0, 1 or 2");
+                value_seen = value_seen || value;
+                cur = w.inner;
+            }
         } else if (t.type != MPT_TEX_SOLID && t.type != MPT_TEX_CHECKER) {
             return fail(MPT_E_UNSUPPORTED, "unsupported texture type " + std::to_string(t.type));
         }
@@ -460,7 +475,11 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         s.rich = true;  // tested beside the BVH, as the ground is, by the variant that carries (u, v)
     }
     auto tex_ok = [&](int32_t t) { return t >= -1 && t < d.n_textures; };
-    auto is_image = [&](int32_t t) { return t >= 0 && s.textures[size_t(t)].type == MPT_TEX_IMAGE; };
+    auto is_image = [&](int32_t t) {  // an image, or wrappers of one: read at (u, v)
+        if (t < 0) return false;
+        while (s.textures[size_t(t)].type >= MPT_TEX_SCALED) t = s.textures[size_t(t)].inner;
+        return s.textures[size_t(t)].type == MPT_TEX_IMAGE;
+    };
     for (size_t k = 0; k < s.materials.size(); k++) {
         auto& m = s.materials[k];
         if (m.type < MPT_MAT_LAMBERTIAN || m.type > MPT_MAT_PHASE)
@@ -487,16 +506,24 @@ int build_host_scene(const mpt_scene_desc& d, host_scene& s, std::string& err) {
         } else {
             m.coat_weight_texture = m.coat_roughness_texture = m.coat_normal_map = -1;
         }
+        if (m.glow) {
+            if (m.type == MPT_MAT_DIFFUSE_LIGHT || m.type == MPT_MAT_PHASE)
+                return fail(MPT_E_INVALID, "material " + std::to_string(k) + ": lights and volumes take no glow");
+            if (m.glow_texture < 0 || m.glow_texture >= d.n_textures)
+                return fail(MPT_E_INVALID, "material " + std::to_string(k) + ": its glow names a texture that does not exist");
+        } else {
+            m.glow_texture = -1;
+        }
         // recomputed here: whatever the caller set, hits of this material need (u, v) exactly when
         m.uses_uv = (m.normal_map >= 0 || (needs_tex && is_image(m.texture)) || is_image(m.metallic_texture) ||
                      is_image(m.roughness_texture) || m.coat_normal_map >= 0 || is_image(m.coat_weight_texture) ||
-                     is_image(m.coat_roughness_texture)) ? 1 : 0;
+                     is_image(m.coat_roughness_texture) || is_image(m.glow_texture)) ? 1 : 0;
         if (m.uses_uv) s.uses_uv = true;
         if (m.thin && m.type != MPT_MAT_DIELECTRIC)
             return fail(MPT_E_INVALID, "material " + std::to_string(k) + ": only a dielectric can be thin");
         if (m.measured && m.type != MPT_MAT_METAL && m.type != MPT_MAT_PRINCIPLED)
             return fail(MPT_E_INVALID, "material " + std::to_string(k) + ": only metal and principled can be measured");
-        if (m.uses_uv || m.thin || m.measured || m.coat) s.rich = true;
+        if (m.uses_uv || m.thin || m.measured || m.coat || m.glow) s.rich = true;
     }
     s.xforms.resize(d.n_xforms);
     for (int k = 0; k < d.n_xforms; k++) {

@@ -33,6 +33,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <functional>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -535,7 +536,30 @@ struct texture_t {
     const uint8_t* ldr = nullptr;       // RGBA bytes of this image
     const float* hdr = nullptr;         // or its RGB floats
     const double* srgb = nullptr;       // 256 linear values of the sRGB codes
+    // wrappers (texture.h:212-282): what they read and their parameters
+    std::shared_ptr<const texture_t> inner;
+    int channel = 0;
+    double scale = 1;
+    double offset_u = 0, offset_v = 0, scale_u = 1, scale_v = 1, cos_r = 1, sin_r = 0;
     color value(double u, double v, const point3& p) const {
+        switch (type) {
+            case MPT_TEX_SCALED: return even * inner->value(u, v, p);
+            case MPT_TEX_CHANNEL: {
+                const double x = scale * inner->value(u, v, p)[channel];
+                return color(x, x, x);
+            }
+            case MPT_TEX_UV_TRANSFORM: {
+                const double su = scale_u * u, sv = scale_v * (1 - v);
+                const double tu = cos_r * su + sin_r * sv + offset_u;
+                const double tv = -sin_r * su + cos_r * sv + offset_v;
+                return inner->value(tu, 1 - tv, p);
+            }
+            case MPT_TEX_NORMAL_SCALE: {
+                const color c = inner->value(u, v, p);
+                return color(0.5 + scale * (c.x() - 0.5), 0.5 + scale * (c.y() - 0.5), c.z());
+            }
+            default: break;
+        }
         if (type == MPT_TEX_SOLID) return even;
         if (type == MPT_TEX_CHECKER) {
             int x = int(std::floor(inv_scale * p.x())), y = int(std::floor(inv_scale * p.y())),
@@ -838,6 +862,27 @@ struct normal_mapped : material {  // material.h:631-674
         return inner->eval(r, perturbed(r, rec), d);
     }
     color emitted(const ray& r, const hit_record& rec) const override { return inner->emitted(r, rec); }
+};
+
+// emissive_surface (material.h:714-741): the inner material's scattering,
+// and a glow of the texture's value on the front face; not a light.
+struct glowing : material {
+    std::unique_ptr<material> inner;
+    texture_t glow;
+    bool scatter(const ray& r, const hit_record& rec, scatter_record& s, sampler& smp) const override {
+        return inner->scatter(r, rec, s, smp);
+    }
+    color eval(const ray& r, const hit_record& rec, const vec3& d) const override { return inner->eval(r, rec, d); }
+    color emitted(const ray&, const hit_record& rec) const override {
+        if (!rec.front_face) return color(0, 0, 0);
+        return glow.value(rec.u, rec.v, rec.p);
+    }
+    color interior_transmittance(const ray& r, const hit_record& rec) const override {
+        return inner->interior_transmittance(r, rec);
+    }
+    color average_emission() const override { return inner->average_emission(); }
+    bool lets_shadows_through() const override { return inner->lets_shadows_through(); }
+    bool is_volume() const override { return inner->is_volume(); }
 };
 
 struct diffuse_light : material {  // material.h:568-615
@@ -1764,13 +1809,20 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
         double encoded = k / 255.0;
         s.srgb[k] = encoded <= 0.04045 ? encoded / 12.92 : std::pow((encoded + 0.055) / 1.055, 2.4);
     }
-    auto tex = [&](int ti) {
+    std::function<texture_t(int)> tex = [&](int ti) {
         texture_t t{};
         const mpt_texture& x = d.textures[ti];
         t.type = x.type;
         t.inv_scale = x.inv_scale;
         t.even = color(x.even[0], x.even[1], x.even[2]);
         t.odd = color(x.odd[0], x.odd[1], x.odd[2]);
+        if (x.type >= MPT_TEX_SCALED && x.type <= MPT_TEX_NORMAL_SCALE) {
+            t.inner = std::make_shared<texture_t>(tex(x.inner));
+            t.channel = x.channel;
+            t.scale = x.scale;
+            t.offset_u = x.uv[0], t.offset_v = x.uv[1], t.scale_u = x.uv[2], t.scale_v = x.uv[3];
+            t.cos_r = x.uv[4], t.sin_r = x.uv[5];
+        }
         if (x.type == MPT_TEX_IMAGE) {
             t.width = x.width;
             t.height = x.height;
@@ -1874,6 +1926,13 @@ int build_scene(const mpt_scene_desc& d, scene& s, std::string& err) {
             auto w = std::make_unique<normal_mapped>();
             w->inner = std::move(mp);
             w->map = tex(m.normal_map);
+            w->table = &s.table;
+            mp = std::move(w);
+        }
+        if (m.glow && m.glow_texture >= 0) {
+            auto w = std::make_unique<glowing>();
+            w->inner = std::move(mp);
+            w->glow = tex(m.glow_texture);
             w->table = &s.table;
             mp = std::move(w);
         }

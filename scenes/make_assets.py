@@ -34,6 +34,23 @@ island.vox           a MagicaVoxel (version 200) island: a 40 x 40 x 18 terrain
                      times by the scene graph (nTRN translations and ROTATION
                      bytes, one mirrored, one hidden), an RGBA palette and
                      MATL materials (_glass, _metal, _emit);
+props.gltf / .bin    a glTF 2.0 model: a node tree of matrices and TRS (a
+                     rotated parent, a mirroring node, a mesh placed twice);
+                     a box with normals and texture coordinates (u16
+                     indices), a quad (u8 indices, normalized u16 texture
+                     coordinates), an unindexed pyramid without normals; the
+                     materials of the reference's mapping: base colour
+                     texture with KHR_texture_transform and clamping,
+                     metallic-roughness and scaled normal textures, thin
+                     glass (KHR_materials_transmission, _ior), clear coat,
+                     an emissive light (emissive_strength), an emissive
+                     texture (emissive_surface), MASK by a texture's alpha
+                     (0 or 255: no random draw decides a hit), and a
+                     primitive with no material; one image inside the .bin,
+                     one a file whose name needs percent-decoding ("props
+                     grid.png"), one buffer a data: URI;
+props_box.glb        a binary glTF: a box with an embedded PNG texture, and a
+                     box of BLEND coverage by a constant alpha (0.46);
 plain150.vox         a version-150 file with two models (a row of 255 voxels of
                      palette indices 1-255, and a small column), no scene graph
                      and no RGBA chunk (MagicaVoxel's default palette);
@@ -492,6 +509,206 @@ def island():
     vox_write(os.path.join(HERE, "plain150.vox"), small, None, None, None, version=150)
 
 
+def _png_bytes(rows, color_type):
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "t.png")
+        png_write(path, rows, color_type, 8)
+        return open(path, "rb").read()
+
+
+def gltf_props():
+    """scenes/models/props.gltf, props.bin, props grid.png, props_box.glb
+    (see the module text)."""
+    import base64
+    import json
+    import struct
+    import numpy as np
+
+    def box(h):
+        """24 vertices (4 per side) with normals and uvs, 36 indices."""
+        pos, nrm, uv, idx = [], [], [], []
+        for a in range(3):
+            for s in (1, -1):
+                n = [0.0, 0.0, 0.0]
+                n[a] = s
+                b, c = (a + 1) % 3, (a + 2) % 3
+                base = len(pos)
+                for (i, j) in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                    p = [0.0, 0.0, 0.0]
+                    p[a] = s * h
+                    p[b] = (2 * i - 1) * h
+                    p[c] = (2 * j - 1) * h * s
+                    pos.append(p)
+                    nrm.append(n)
+                    uv.append([i * 1.5, j * 1.5])  # past 1: repeats (or clamps)
+                idx += [base, base + 1, base + 2, base, base + 2, base + 3]
+        return (np.array(pos, np.float32), np.array(nrm, np.float32), np.array(uv, np.float32),
+                np.array(idx, np.uint16))
+
+    blob = bytearray()
+    views, accessors = [], []
+
+    def add_view(data, target=None, stride=None):
+        while len(blob) % 4:
+            blob.append(0)
+        v = {"buffer": 0, "byteOffset": len(blob), "byteLength": len(data)}
+        if target:
+            v["target"] = target
+        if stride:
+            v["byteStride"] = stride
+        blob.extend(data)
+        views.append(v)
+        return len(views) - 1
+
+    def add_acc(arr, type_, comp, target=None, normalized=False, minmax=False):
+        vi = add_view(arr.tobytes(), target)
+        a = {"bufferView": vi, "componentType": comp, "count": int(arr.shape[0]), "type": type_}
+        if normalized:
+            a["normalized"] = True
+        if minmax:
+            a["min"] = [float(x) for x in arr.min(0)]
+            a["max"] = [float(x) for x in arr.max(0)]
+        accessors.append(a)
+        return len(accessors) - 1
+
+    P, N, T, I = box(0.5)
+    box_prim = {"attributes": {"POSITION": add_acc(P, "VEC3", 5126, 34962, minmax=True),
+                               "NORMAL": add_acc(N, "VEC3", 5126, 34962),
+                               "TEXCOORD_0": add_acc(T, "VEC2", 5126, 34962)},
+                "indices": add_acc(I, "SCALAR", 5123, 34963)}
+    qp = np.array([[-1, 0, -1], [1, 0, -1], [1, 0, 1], [-1, 0, 1]], np.float32)
+    qt = np.array([[0, 0], [65535, 0], [65535, 65535], [0, 65535]], np.uint16)  # normalized u16
+    qi = np.array([0, 2, 1, 0, 3, 2], np.uint8)
+    quad = {"POSITION": add_acc(qp, "VEC3", 5126, 34962, minmax=True), "TEXCOORD_0": add_acc(qt, "VEC2", 5123, 34962, True)}
+    quad_idx = add_acc(qi, "SCALAR", 5121, 34963)
+    apex = [0, 0.8, 0]
+    corners = [[-0.5, 0, -0.5], [0.5, 0, -0.5], [0.5, 0, 0.5], [-0.5, 0, 0.5]]
+    tris = []
+    for k in range(4):
+        tris += [corners[k], corners[(k + 1) % 4], apex]
+    pyr = {"POSITION": add_acc(np.array(tris, np.float32), "VEC3", 5126, 34962, minmax=True)}
+    pyr2 = {"POSITION": add_acc(np.array(tris, np.float32) * [0.6, 0.6, 0.6] + [1.1, 0, 0], "VEC3", 5126, 34962,
+                                minmax=True)}  # a second, smaller pyramid beside it
+    # images: a checker in the .bin (sRGB base colour), a metallic-roughness map
+    # (G roughness, B metal), a normal map of ridges, an RGBA leaf mask, a glow panel
+    checker = [[c for i in range(16) for c in ((230, 60, 40) if (i // 4 + j // 4) % 2 else (240, 230, 200))]
+               for j in range(16)]
+    mr = [[c for i in range(16) for c in (0, 40 + 12 * j, 255 if i < 8 else 0)] for j in range(16)]
+    ridges = [[c for i in range(16) for c in (int(128 + 90 * math.sin(i * 0.8)), 128, 230)] for j in range(16)]
+    leaf = [[c for i in range(16) for c in (60, 160, 50, 255 if (i - 7.5) ** 2 + (j - 7.5) ** 2 < 40 else 0)]
+            for j in range(16)]
+    panel = [[c for i in range(8) for c in ((255, 255, 255) if (i + j) % 3 else (20, 80, 255))] for j in range(8)]
+    img_checker = add_view(_png_bytes(checker, 2))
+    png_write(os.path.join(HERE, "props grid.png"), mr, 2, 8)
+    img_ridges = add_view(_png_bytes(ridges, 2))
+    img_leaf = add_view(_png_bytes(leaf, 6))
+    img_panel = add_view(_png_bytes(panel, 2))
+    images = [{"bufferView": img_checker, "mimeType": "image/png"}, {"uri": "props%20grid.png"},
+              {"bufferView": img_ridges, "mimeType": "image/png"}, {"bufferView": img_leaf, "mimeType": "image/png"},
+              {"bufferView": img_panel, "mimeType": "image/png"}]
+    samplers = [{}, {"wrapS": 33071, "wrapT": 33071}]
+    textures = [{"source": 0, "sampler": 1}, {"source": 1}, {"source": 2}, {"source": 3}, {"source": 4}]
+    tt = {"KHR_texture_transform": {"offset": [0.25, -0.1], "rotation": 0.4, "scale": [2.0, 1.5]}}
+    materials = [
+        {"name": "painted", "pbrMetallicRoughness": {"baseColorFactor": [0.9, 0.8, 0.7, 1],
+                                                     "baseColorTexture": {"index": 0, "extensions": tt},
+                                                     "metallicFactor": 0, "roughnessFactor": 0.6}},
+        {"name": "metal_rough", "pbrMetallicRoughness": {"baseColorFactor": [0.8, 0.8, 0.85, 1],
+                                                         "metallicRoughnessTexture": {"index": 1},
+                                                         "metallicFactor": 1, "roughnessFactor": 0.9},
+         "normalTexture": {"index": 2, "scale": 0.5}},
+        {"name": "glass", "pbrMetallicRoughness": {"baseColorFactor": [0.8, 1.0, 0.9, 1], "roughnessFactor": 0},
+         "extensions": {"KHR_materials_transmission": {"transmissionFactor": 1},
+                        "KHR_materials_ior": {"ior": 1.45}}},
+        {"name": "coated", "pbrMetallicRoughness": {"baseColorFactor": [0.1, 0.3, 0.8, 1], "metallicFactor": 0,
+                                                    "roughnessFactor": 0.5},
+         "extensions": {"KHR_materials_clearcoat": {"clearcoatFactor": 0.8, "clearcoatRoughnessFactor": 0.1}}},
+        {"name": "lamp", "emissiveFactor": [1, 0.8, 0.5],
+         "extensions": {"KHR_materials_emissive_strength": {"emissiveStrength": 4}}},
+        {"name": "screen", "pbrMetallicRoughness": {"baseColorFactor": [0.05, 0.05, 0.05, 1], "metallicFactor": 0},
+         "emissiveFactor": [1, 1, 1], "emissiveTexture": {"index": 4}},
+        {"name": "leafy", "alphaMode": "MASK", "pbrMetallicRoughness": {"baseColorTexture": {"index": 3},
+                                                                      "metallicFactor": 0}},
+        {"name": "ghost", "alphaMode": "BLEND", "pbrMetallicRoughness": {"baseColorFactor": [0.9, 0.9, 1.0, 0.46],
+                                                                       "metallicFactor": 0}},
+        {"name": "sheeny", "pbrMetallicRoughness": {"baseColorFactor": [0.7, 0.2, 0.6, 1], "metallicFactor": 0},
+         "extensions": {"KHR_materials_sheen": {"sheenColorFactor": [1, 1, 1]}}},
+    ]
+    meshes = [
+        {"name": "painted box", "primitives": [dict(box_prim, material=0)]},
+        {"name": "metal box", "primitives": [dict(box_prim, material=1)]},
+        {"name": "glass box", "primitives": [dict(box_prim, material=2)]},
+        {"name": "coated box", "primitives": [dict(box_prim, material=3)]},
+        {"name": "lamp", "primitives": [{"attributes": quad, "indices": quad_idx, "material": 4}]},
+        {"name": "screen", "primitives": [{"attributes": quad, "indices": quad_idx, "material": 5}]},
+        {"name": "leaf", "primitives": [{"attributes": quad, "indices": quad_idx, "material": 6}]},
+        {"name": "ghost", "primitives": [dict(box_prim, material=7)]},
+        {"name": "pyramid", "primitives": [{"attributes": pyr}, {"attributes": pyr2, "material": 8}]},
+    ]
+    s2 = math.sqrt(0.5)
+    nodes = [
+        {"name": "root", "children": [1, 2, 3, 4, 5, 6, 7, 8, 9], "translation": [0, 0.5, 0],
+         "rotation": [0, math.sin(0.15), 0, math.cos(0.15)], "scale": [1.0, 1.0, 1.0]},
+        {"mesh": 0, "translation": [-2.2, 0, 0]},
+        {"mesh": 1, "translation": [-0.8, 0, 0], "rotation": [0, 0.3826834, 0, 0.9238795]},
+        {"mesh": 2, "translation": [0.7, 0, 0], "scale": [0.8, 1.2, 0.8]},
+        {"mesh": 3, "matrix": [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 2.1, 0, 0, 1]},
+        {"mesh": 4, "translation": [0, 2.4, 0], "rotation": [1, 0, 0, 0], "scale": [0.6, 1, 0.6]},  # facing down
+        {"mesh": 5, "translation": [0, 0.6, -2.0], "rotation": [s2, 0, 0, s2], "scale": [1.6, 1, 0.9]},
+        {"mesh": 6, "translation": [1.4, 0.2, 1.4], "rotation": [s2, 0, 0, s2], "scale": [0.5, 1, 0.5]},
+        {"mesh": 8, "translation": [0.0, -0.5, 1.6], "scale": [-1, 1, 1]},  # mirroring: the winding flips
+        {"mesh": 0, "translation": [2.0, 0.6, 1.8], "scale": [0.4, 0.4, 0.4]},  # the painted box again
+    ]
+    # one more buffer as a data: URI: a second copy of the quad's positions
+    extra = qp.tobytes()
+    doc = {"asset": {"version": "2.0", "generator": "scenes/make_assets.py"}, "scene": 0,
+           "scenes": [{"nodes": [0]}], "nodes": nodes, "meshes": meshes, "materials": materials,
+           "textures": textures, "images": images, "samplers": samplers, "accessors": accessors,
+           "bufferViews": views,
+           "buffers": [{"uri": "props.bin", "byteLength": len(blob)},
+                       {"uri": "data:application/octet-stream;base64," + base64.b64encode(extra).decode(),
+                        "byteLength": len(extra)}],
+           "extensionsUsed": ["KHR_texture_transform", "KHR_materials_transmission", "KHR_materials_ior",
+                              "KHR_materials_clearcoat", "KHR_materials_emissive_strength", "KHR_materials_sheen"]}
+    accessors.append({"bufferView": len(views), "componentType": 5126, "count": 4, "type": "VEC3"})
+    views.append({"buffer": 1, "byteOffset": 0, "byteLength": len(extra)})
+    doc["meshes"].append({"name": "floor tile", "primitives": [{"attributes": {"POSITION": len(accessors) - 1},
+                                                               "indices": quad_idx, "material": 1}]})
+    nodes[0]["children"].append(len(nodes))
+    nodes.append({"mesh": len(doc["meshes"]) - 1, "translation": [0, -0.49, 0], "scale": [3, 1, 3]})
+    open(os.path.join(HERE, "props.bin"), "wb").write(bytes(blob))
+    with open(os.path.join(HERE, "props.gltf"), "w") as f:
+        json.dump(doc, f, indent=1)
+
+    # the binary container: one box with an embedded texture
+    blob = bytearray()
+    views, accessors = [], []
+    P, N, T, I = box(0.7)
+    prim = {"attributes": {"POSITION": add_acc(P, "VEC3", 5126, 34962, minmax=True),
+                           "NORMAL": add_acc(N, "VEC3", 5126, 34962), "TEXCOORD_0": add_acc(T, "VEC2", 5126, 34962)},
+            "indices": add_acc(I, "SCALAR", 5123, 34963), "material": 0}
+    iv = add_view(_png_bytes(checker, 2))
+    doc = {"asset": {"version": "2.0"}, "scenes": [{"nodes": [0, 1]}],
+           "nodes": [{"mesh": 0, "rotation": [0, 0.2588190, 0, 0.9659258]},
+                     {"mesh": 1, "translation": [0, 0, 1.6], "scale": [0.6, 0.6, 0.6]}],
+           "meshes": [{"primitives": [prim]}, {"primitives": [dict(prim, material=1)]}],
+           "materials": [{"name": "crate", "pbrMetallicRoughness": {"baseColorTexture": {"index": 0},
+                                                                    "metallicFactor": 0.0, "roughnessFactor": 0.8}},
+                         {"name": "ghost", "alphaMode": "BLEND",
+                          "pbrMetallicRoughness": {"baseColorFactor": [0.9, 0.9, 1.0, 0.46], "metallicFactor": 0}}],
+           "textures": [{"source": 0}], "images": [{"bufferView": iv, "mimeType": "image/png"}],
+           "accessors": accessors, "bufferViews": views, "buffers": [{"byteLength": 0}]}
+    while len(blob) % 4:
+        blob.append(0)
+    doc["buffers"][0]["byteLength"] = len(blob)
+    js = json.dumps(doc).encode()
+    js += b" " * ((4 - len(js) % 4) % 4)
+    body = struct.pack("<II", len(js), 0x4E4F534A) + js + struct.pack("<II", len(blob), 0x004E4942) + bytes(blob)
+    with open(os.path.join(HERE, "props_box.glb"), "wb") as f:
+        f.write(b"glTF" + struct.pack("<II", 2, 12 + len(body)) + body)
+
+
 if __name__ == "__main__":
     os.makedirs(HERE, exist_ok=True)
     room()
@@ -502,3 +719,4 @@ if __name__ == "__main__":
     crate()
     leaves()
     island()
+    gltf_props()

@@ -223,6 +223,38 @@ def golden_voxel():
     print("vox_palette", flush=True)
 
 
+def golden_gltf():
+    """The glTF scenes through the reference's own load_gltf (cgltf) and
+    material mapping: for gltf_props.scene the hit subset and each hit's
+    surface record (u, v, tangents, base colour through the texture
+    transform and scale, the normal after a scaled normal map, principled's
+    metallic and alpha from the channels of the metallic-roughness map), and
+    converged means; for gltf_glb.scene (a constant coverage, whose hit test
+    draws a random number) the converged means only."""
+    golden_asset_scene("gltf_props.scene", (64, 128, 8, 12), n_batch=1 << 16)
+    path = os.path.join(HERE, "gltf_props.npz")
+    g = dict(np.load(path))
+    r = ol.RefScene(scene_path("gltf_props.scene"))
+    g["surface"] = r.surface(g["rays"], g["tt"])
+    g["light_probabilities"] = r.light_probabilities()
+    np.savez_compressed(path, **g)
+    R = ol.reference()
+    d = m.describe(scene_path("gltf_glb.scene"))
+    width, spp, runs, depth = 64, 128, 8, 8
+    h = max(1, int(width / d.camera.aspect_ratio))
+    r = ol.RefScene(scene_path("gltf_glb.scene"))
+    imgs = []
+    for _ in range(runs):
+        img = np.zeros((h, width, 3))
+        R.ref_render(r.h, width, spp, depth, os.cpu_count() or 8, ol._d(img))
+        imgs.append(img)
+    imgs = np.stack(imgs)
+    np.savez_compressed(os.path.join(HERE, "gltf_glb.npz"), mean=imgs.mean(0),
+                        stderr=imgs.std(0, ddof=1) / np.sqrt(runs), width=width, height=h, spp=spp * runs,
+                        max_depth=depth)
+    print("gltf", flush=True)
+
+
 def golden_alpha():
     for name, render in ALPHA_SCENES.items():
         golden_asset_scene(name, render)
@@ -317,7 +349,7 @@ def main():
         sys.exit("oracle/_ref/libmpt_ref.so is not built (make -C oracle ref, where the reference is present)")
     R = ol.reference()
     steps = set(sys.argv[1:]) or {"ggx", "trace", "ties", "bsdf", "render", "denoise", "display", "obj",
-                                  "textures", "materials", "sun", "fog", "alpha", "coat", "voxel"}
+                                  "textures", "materials", "sun", "fog", "alpha", "coat", "voxel", "gltf"}
     if "denoise" in steps:
         golden_denoise()
     if "display" in steps:
@@ -336,6 +368,8 @@ def main():
         golden_alpha()
     if "voxel" in steps:
         golden_voxel()
+    if "gltf" in steps:
+        golden_gltf()
     if "ggx" in steps:
         golden_ggx(R)
     if "trace" in steps:

@@ -87,7 +87,7 @@ def write(tmp_path, text):
     ("sphere 0 0 -1 0.5 nosuch\n", ":1: unknown material 'nosuch'"),
     ("material a lambertian 1 1 1\n\nquad 0 0 0 1 0 0 2 0 0 a\n", ":3: quad edges must not be parallel"),
     ("vox models/island.vox\n", ":1: cannot open"),
-    ("mesh models/house.gltf\n", "not modelled by the GPU path"),
+    ("mesh models/house.gltf\n", ":1: cannot read glTF file"),
     ("material a lambertian 1 1 1\nsphere 0 0 -1 0.5 a extra\n", "unexpected value 'extra'"),
     ("# nothing\n", "the scene has no objects"),
     ("material e emissive 1 1 1\nbox 0 0 0 1 1 1 e scale 1 2 1\n", "lights can only be moved"),
