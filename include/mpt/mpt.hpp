@@ -456,6 +456,24 @@ bool load_gltf(const std::string& path, std::shared_ptr<material> override_mat, 
                std::string& error, bool use_normal_maps = true,
                const std::map<std::string, std::shared_ptr<material>>& replace = {});
 
+// Photo studio pieces (studio.h). The shape of a cyclorama in its own frame:
+// floor y = 0 from z = front back to z = -back, then a quarter circle of the
+// radius up into a wall at z = -back - radius, height high, width wide
+// about x = 0.
+struct cyclorama_settings {
+    double width = 20, front = 10, back = 4, height = 8, radius = 3;
+    int segments = 32;  // flat strips along the curve
+};
+// A cyclorama as one mesh (make_cyclorama, studio.h:39-87): profile vertices
+// carry the circle's true normal, uv runs along x and along the profile by
+// arc length.
+std::shared_ptr<mesh> make_cyclorama(const cyclorama_settings& s, std::shared_ptr<material> mat);
+// A width x height rectangle light centred on center whose front faces aim,
+// its width horizontal, turned by roll degrees about the aiming axis
+// (make_softbox, studio.h:89-101).
+std::shared_ptr<quad> make_softbox(const point3& center, const point3& aim, double width, double height,
+                                   double roll_degrees, std::shared_ptr<material> mat);
+
 class instance : public hittable {
 public:
     instance(std::shared_ptr<hittable> object, const transform& xform) : object(std::move(object)), xform(xform) {}

@@ -1035,6 +1035,55 @@ void flat_builder::add_lights(const hittable_list& lights, const std::shared_ptr
     }
 }
 
+// ------------------------------------------------------------------ studio
+std::shared_ptr<mesh> make_cyclorama(const cyclorama_settings& s, std::shared_ptr<material> mat) {
+    // the profile in the (y, z) plane, from the floor's front to the wall's top;
+    // the curve's centre is (y = radius, z = -back) and its normals point at it
+    std::vector<double> ys, zs, nys, nzs;
+    auto add = [&](double y, double z, double ny, double nz) {
+        ys.push_back(y), zs.push_back(z), nys.push_back(ny), nzs.push_back(nz);
+    };
+    add(0, s.front, 1, 0);
+    for (int i = 0; i <= s.segments; i++) {
+        const double theta = (kPiApi / 2) * i / s.segments;
+        add(s.radius * (1 - std::cos(theta)), -s.back - s.radius * std::sin(theta), std::cos(theta), std::sin(theta));
+    }
+    if (s.height > s.radius) add(s.height, -s.back - s.radius, 0, 1);
+    std::vector<double> arc(ys.size(), 0);
+    for (size_t i = 1; i < ys.size(); i++) arc[i] = arc[i - 1] + std::hypot(ys[i] - ys[i - 1], zs[i] - zs[i - 1]);
+    std::vector<point3> positions;
+    auto m = std::make_shared<mesh>(positions, std::vector<int32_t>(), mat);
+    for (size_t i = 0; i < ys.size(); i++)
+        for (int side = 0; side < 2; side++) {
+            m->positions.push_back(point3(side == 0 ? -s.width / 2 : s.width / 2, ys[i], zs[i]));
+            m->normals.push_back(vec3(0, nys[i], nzs[i]));
+            m->uvs.push_back(vec3(side, arc[i] / arc.back(), 0));
+        }
+    // two triangles per strip between rows i and i + 1, counterclockwise seen from the front
+    for (int i = 0; i + 1 < int(ys.size()); i++) {
+        const int l0 = 2 * i, r0 = 2 * i + 1, l1 = 2 * i + 2, r1 = 2 * i + 3;
+        for (int v : {l0, r0, r1, l0, r1, l1}) {
+            m->indices.push_back(v);
+            m->normal_indices.push_back(v);
+            m->uv_indices.push_back(v);
+        }
+    }
+    return m;
+}
+
+std::shared_ptr<quad> make_softbox(const point3& center, const point3& aim, double width, double height,
+                                   double roll_degrees, std::shared_ptr<material> mat) {
+    const vec3 dir = unit_vector(aim - center);
+    const vec3 up = std::fabs(dir.y()) > 0.999 ? vec3(0, 0, 1) : vec3(0, 1, 0);
+    const vec3 u = unit_vector(cross(up, dir));
+    const vec3 v = cross(dir, u);  // cross(u, v) = dir: the front faces the aim point
+    const double roll = degrees_to_radians(roll_degrees);
+    const vec3 ur = std::cos(roll) * u + std::sin(roll) * v;
+    const vec3 vr = -std::sin(roll) * u + std::cos(roll) * v;
+    const point3 corner = center - (width / 2) * ur - (height / 2) * vr;
+    return std::make_shared<quad>(corner, width * ur, height * vr, std::move(mat));
+}
+
 // ------------------------------------------------------------------ voxels
 namespace {
 
@@ -2394,6 +2443,62 @@ bool load_scene(const std::string& path, scene& result, std::string* err_out) {
             std::string state;
             if (!(in >> state) || (state != "on" && state != "off")) return fail("expected 'normal_maps on|off'");
             use_normal_maps = state == "on";
+        } else if (keyword == "cyclorama") {  // scene.h:935-968
+            const char* usage = "expected 'cyclorama [width <w>] [front <f>] [back <b>] [height <h>] [radius <r>]"
+                                " [segments <n>] [material] [transforms]'";
+            cyclorama_settings cyc;
+            std::string key, error;
+            std::streampos mark = in.tellg();
+            while (in >> key) {  // settings first, as key value pairs
+                bool ok = true;
+                if (key == "width") ok = bool(in >> cyc.width) && cyc.width > 0;
+                else if (key == "front") ok = bool(in >> cyc.front);
+                else if (key == "back") ok = bool(in >> cyc.back);
+                else if (key == "height") ok = bool(in >> cyc.height) && cyc.height > 0;
+                else if (key == "radius") ok = bool(in >> cyc.radius) && cyc.radius > 0;
+                else if (key == "segments") ok = bool(in >> cyc.segments) && cyc.segments > 0;
+                else break;
+                if (!ok) return fail(usage);
+                mark = in.tellg();
+            }
+            if (cyc.front < -cyc.back) return fail("a cyclorama's front must be in front of its back (front >= -back)");
+            in.clear();
+            in.seekg(mark);
+            std::shared_ptr<material> mat;
+            transform xform;
+            bool transformed;
+            if (!read_tail(in, mat, xform, transformed, error)) return fail(error);
+            if (std::dynamic_pointer_cast<diffuse_light>(mat))
+                return fail("a cyclorama cannot be a light (mesh light sampling is linear in triangles)");
+            if (!add_object(make_cyclorama(cyc, mat), mat, xform, transformed, error)) return fail(error);
+        } else if (keyword == "softbox") {  // scene.h:969-1001
+            const char* usage = "expected 'softbox center <x y z> aim <x y z> size <width> <height> [roll <degrees>]"
+                                " <emissive material> [transforms]'";
+            point3 center, aim;
+            double width = 0, height = 0, roll = 0;
+            bool has_center = false, has_aim = false;
+            std::string key, error;
+            std::streampos mark = in.tellg();
+            while (in >> key) {
+                bool ok = true;
+                if (key == "center") ok = has_center = bool(in >> center[0] >> center[1] >> center[2]);
+                else if (key == "aim") ok = has_aim = bool(in >> aim[0] >> aim[1] >> aim[2]);
+                else if (key == "size") ok = bool(in >> width >> height) && width > 0 && height > 0;
+                else if (key == "roll") ok = bool(in >> roll);
+                else break;
+                if (!ok) return fail(usage);
+                mark = in.tellg();
+            }
+            if (!has_center || !has_aim || width <= 0 || (aim - center).near_zero()) return fail(usage);
+            in.clear();
+            in.seekg(mark);
+            std::shared_ptr<material> mat;
+            transform xform;
+            bool transformed;
+            if (!read_tail(in, mat, xform, transformed, error)) return fail(error);
+            if (!std::dynamic_pointer_cast<diffuse_light>(mat)) return fail("a softbox needs an emissive material");
+            if (!add_object(make_softbox(center, aim, width, height, roll, mat), mat, xform, transformed, error))
+                return fail(error);
         } else if (keyword == "style") {  // scene.h:612-645: a preset of other lines, which later lines can change
             std::string name;
             if (!(in >> name) || (name != "voxel" && name != "classic"))

@@ -255,6 +255,20 @@ def golden_gltf():
     print("gltf", flush=True)
 
 
+def golden_studio():
+    """studio.scene through the reference's make_cyclorama / make_softbox:
+    the hit subset, its surface records (the cyclorama's interpolated
+    normals and arc-length uv) and converged means."""
+    golden_asset_scene("studio.scene", (64, 128, 8, 8), n_batch=1 << 16)
+    path = os.path.join(HERE, "studio.npz")
+    g = dict(np.load(path))
+    r = ol.RefScene(scene_path("studio.scene"))
+    g["surface"] = r.surface(g["rays"], g["tt"])
+    g["light_probabilities"] = r.light_probabilities()
+    np.savez_compressed(path, **g)
+    print("studio", flush=True)
+
+
 def golden_alpha():
     for name, render in ALPHA_SCENES.items():
         golden_asset_scene(name, render)
@@ -349,7 +363,7 @@ def main():
         sys.exit("oracle/_ref/libmpt_ref.so is not built (make -C oracle ref, where the reference is present)")
     R = ol.reference()
     steps = set(sys.argv[1:]) or {"ggx", "trace", "ties", "bsdf", "render", "denoise", "display", "obj",
-                                  "textures", "materials", "sun", "fog", "alpha", "coat", "voxel", "gltf"}
+                                  "textures", "materials", "sun", "fog", "alpha", "coat", "voxel", "gltf", "studio"}
     if "denoise" in steps:
         golden_denoise()
     if "display" in steps:
@@ -370,6 +384,8 @@ def main():
         golden_voxel()
     if "gltf" in steps:
         golden_gltf()
+    if "studio" in steps:
+        golden_studio()
     if "ggx" in steps:
         golden_ggx(R)
     if "trace" in steps:
