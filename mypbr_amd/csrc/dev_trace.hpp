@@ -264,6 +264,84 @@ __device__ __forceinline__ bool leaf_exact(const cnode* nodes, int e, const v3& 
 // tree in 64-byte single-precision records this took the spheres scene from
 // 1449 to 1721 M paths/s and the generated batch from 15.6 to 21.9 M.
 
+__device__ __forceinline__ int neg_mask(const fray& fr) {
+    return (fr.inv[0] < 0 ? 1 : 0) | (fr.inv[1] < 0 ? 2 : 0) | (fr.inv[2] < 0 ? 4 : 0);
+}
+
+// One record visit of the four-wide walk: record cur's slots culled with the
+// interval up to tmax; every slot that passes but the first in visit order
+// is pushed (farthest first); the first's (off, meta) are returned, or false
+// when no slot passes.
+template <bool CNT>
+__device__ __forceinline__ bool visit_record(const qnode* qnodes, int cur, const fray& fr, int negmask, double tmax,
+                                             tstack& st, int& sp, int32_t& off, int32_t& meta, wcount& wc) {
+    const qnode* q = qnodes + cur;
+    float b[24];
+    {
+        const float4* p = reinterpret_cast<const float4*>(q);
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            const float4 v = p[k];
+            b[4 * k] = v.x, b[4 * k + 1] = v.y, b[4 * k + 2] = v.z, b[4 * k + 3] = v.w;
+        }
+    }
+    const int4 offs = reinterpret_cast<const int4*>(q)[6];
+    const int4 metas = reinterpret_cast<const int4*>(q)[7];
+    const int info = reinterpret_cast<const int*>(q)[32];
+    if constexpr (CNT) wc.nodes += 144u;
+    // visit order: slot positions, two bits each
+    const int nN = (negmask >> (info & 3)) & 1;
+    const int nL = (negmask >> ((info >> 2) & 3)) & 1;
+    const int nR = (negmask >> ((info >> 4) & 3)) & 1;
+    const bool eL = (info >> 6) & 1, eR = (info >> 7) & 1;
+    const int ln = eL ? 2 : 1, rn = eR ? 2 : 1, rb = ln;
+    const uint32_t lord = eL ? (nL ? 1u : 4u) : 0u;  // [1, 0] or [0, 1] or [0]
+    const uint32_t rord = eR ? (nR ? uint32_t(rb + 1) | uint32_t(rb) << 2 : uint32_t(rb) | uint32_t(rb + 1) << 2)
+                             : uint32_t(rb);
+    const uint32_t ord = nN ? (rord | lord << (2 * rn)) : (lord | rord << (2 * ln));
+    const int nslots = ln + rn;
+    const float tf = f32_up(tmax);
+    uint32_t hits = 0;
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++)
+        if (sl < nslots && fbox_hit(b + 6 * sl, fr, tf)) hits |= 1u << sl;
+    // the first slot that passes is taken now; the others are pushed, farthest first
+    int first = -1;
+#pragma unroll
+    for (int j = 3; j >= 0; j--) {
+        const int sl = int(ord >> (2 * j)) & 3;
+        if (j < nslots && ((hits >> sl) & 1)) {
+            if (first >= 0) st.put(sp++, (cur << 2) | first);
+            first = sl;
+        }
+    }
+    if (first < 0) return false;
+    off = first == 0 ? offs.x : first == 1 ? offs.y : first == 2 ? offs.z : offs.w;
+    meta = first == 0 ? metas.x : first == 1 ? metas.y : first == 2 ? metas.z : metas.w;
+    return true;
+}
+
+// A pushed slot e (record e >> 2, slot e & 3) culled again with the interval
+// as it is now; its (off, meta) when it passes.
+template <bool CNT>
+__device__ __forceinline__ bool pop_slot(const qnode* qnodes, int e, const fray& fr, double tmax, int32_t& off,
+                                         int32_t& meta, wcount& wc) {
+    const qnode* pq = qnodes + (e >> 2);
+    const int sl = e & 3;
+    float bb[6];
+    const float2* pb = reinterpret_cast<const float2*>(pq->b + 6 * sl);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float2 v = pb[k];
+        bb[2 * k] = v.x;
+        bb[2 * k + 1] = v.y;
+    }
+    off = pq->off[sl];
+    meta = pq->meta[sl];
+    if constexpr (CNT) wc.nodes += 32u;
+    return fbox_hit(bb, fr, f32_up(tmax));
+}
+
 template <bool CNT, typename Leaf>
 __device__ __forceinline__ void walk4(const qnode* qnodes, const croot& root, const v3& o, const v3& d, double tmin,
                                       double& tmax, tstack& st, int sp_base, wcount& wc, Leaf&& leaf) {
@@ -278,54 +356,13 @@ __device__ __forceinline__ void walk4(const qnode* qnodes, const croot& root, co
         }
         fr = make_fray(o, inv, tmin, root.b);
     }
-    const int negmask = (fr.inv[0] < 0 ? 1 : 0) | (fr.inv[1] < 0 ? 2 : 0) | (fr.inv[2] < 0 ? 4 : 0);
+    const int negmask = neg_mask(fr);
     int sp = sp_base;
     int cur = root.qoff;
     while (true) {
-        const qnode* q = qnodes + cur;
-        float b[24];
-        {
-            const float4* p = reinterpret_cast<const float4*>(q);
-#pragma unroll
-            for (int k = 0; k < 6; k++) {
-                const float4 v = p[k];
-                b[4 * k] = v.x, b[4 * k + 1] = v.y, b[4 * k + 2] = v.z, b[4 * k + 3] = v.w;
-            }
-        }
-        const int4 offs = reinterpret_cast<const int4*>(q)[6];
-        const int4 metas = reinterpret_cast<const int4*>(q)[7];
-        const int info = reinterpret_cast<const int*>(q)[32];
-        if constexpr (CNT) wc.nodes += 144u;
-        // visit order: slot positions, two bits each
-        const int nN = (negmask >> (info & 3)) & 1;
-        const int nL = (negmask >> ((info >> 2) & 3)) & 1;
-        const int nR = (negmask >> ((info >> 4) & 3)) & 1;
-        const bool eL = (info >> 6) & 1, eR = (info >> 7) & 1;
-        const int ln = eL ? 2 : 1, rn = eR ? 2 : 1, rb = ln;
-        const uint32_t lord = eL ? (nL ? 1u : 4u) : 0u;  // [1, 0] or [0, 1] or [0]
-        const uint32_t rord = eR ? (nR ? uint32_t(rb + 1) | uint32_t(rb) << 2 : uint32_t(rb) | uint32_t(rb + 1) << 2)
-                                 : uint32_t(rb);
-        const uint32_t ord = nN ? (rord | lord << (2 * rn)) : (lord | rord << (2 * ln));
-        const int nslots = ln + rn;
-        const float tf = f32_up(tmax);
-        uint32_t hits = 0;
-#pragma unroll
-        for (int sl = 0; sl < 4; sl++)
-            if (sl < nslots && fbox_hit(b + 6 * sl, fr, tf)) hits |= 1u << sl;
-        // the first slot that passes is taken now; the others are pushed, farthest first
-        int first = -1;
-#pragma unroll
-        for (int j = 3; j >= 0; j--) {
-            const int sl = int(ord >> (2 * j)) & 3;
-            if (j < nslots && ((hits >> sl) & 1)) {
-                if (first >= 0) st.put(sp++, (cur << 2) | first);
-                first = sl;
-            }
-        }
         int next = -1;
-        if (first >= 0) {
-            const int32_t off = first == 0 ? offs.x : first == 1 ? offs.y : first == 2 ? offs.z : offs.w;
-            const int32_t meta = first == 0 ? metas.x : first == 1 ? metas.y : first == 2 ? metas.z : metas.w;
+        int32_t off, meta;
+        if (visit_record<CNT>(qnodes, cur, fr, negmask, tmax, st, sp, off, meta, wc)) {
             const int32_t cnt = int32_t(uint32_t(meta) >> 28);
             if (cnt > 0) {
                 if (leaf(off, cnt, sp, meta & 0x0fffffff)) return;  // an occlusion query found its hit
@@ -336,20 +373,7 @@ __device__ __forceinline__ void walk4(const qnode* qnodes, const croot& root, co
         while (next < 0) {
             if (sp == sp_base) return;
             const int e = st.get(--sp);
-            const qnode* pq = qnodes + (e >> 2);
-            const int sl = e & 3;
-            float bb[6];
-            const float2* pb = reinterpret_cast<const float2*>(pq->b + 6 * sl);
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const float2 v = pb[k];
-                bb[2 * k] = v.x;
-                bb[2 * k + 1] = v.y;
-            }
-            const int32_t off = pq->off[sl];
-            const int32_t meta = pq->meta[sl];
-            if constexpr (CNT) wc.nodes += 32u;
-            if (!fbox_hit(bb, fr, f32_up(tmax))) continue;
+            if (!pop_slot<CNT>(qnodes, e, fr, tmax, off, meta, wc)) continue;
             const int32_t cnt = int32_t(uint32_t(meta) >> 28);
             if (cnt > 0) {
                 if (leaf(off, cnt, sp, meta & 0x0fffffff)) return;
@@ -515,30 +539,41 @@ __device__ __forceinline__ bool alpha_keeps(const dscene& S, int ti, int64_t mes
 // the ray's medium stream for the mask's draws; masks read (u, v), so only
 // the variants that carry it (F_UV) test them (the host puts every masked
 // scene there). any_hit: stop at the first accepted hit.
+// The triangles of a mesh leaf (off, cnt; e its child-pair record for the
+// double test, made at the leaf's first hit): hits shrink tmax and set which
+// and any; returns true when any_hit asks to stop at the first accepted hit.
+template <bool CNT, int FEAT>
+__device__ __forceinline__ bool mesh_leaf(const dscene& S, int off, int cnt, int e, const v3& o, const v3& d,
+                                          double tmin, double& tmax, int32_t& which, bool& any, int alpha,
+                                          int64_t mesh_first, uint64_t mkey, uint32_t serial, bool any_hit,
+                                          wcount& wc) {
+    const double tleaf = tmax;
+    int ok = -1;  // the leaf's double test, made at its first hit
+    for (int k = 0; k < cnt; k++) {
+        double tt;
+        if constexpr (CNT) wc.pbytes += 72u;
+        if (!tri_t(prim_ptr(S, off + k), o, d, tmin, tmax, tt)) continue;
+        if constexpr (FEAT & F_UV)
+            if (alpha >= 0 && !alpha_keeps(S, alpha, mesh_first, off + k, o, d, mkey, serial)) continue;
+        if (ok < 0) ok = leaf_exact<CNT>(S.blas, e, o, d, tmin, tleaf, wc) ? 1 : 0;
+        if (!ok) break;
+        any = true;
+        tmax = tt;
+        which = off + k;
+        if (any_hit) return true;  // the first hit answers an occlusion query: stop the walk
+    }
+    return false;
+}
+
 template <bool CNT, int FEAT>
 __device__ bool mesh_t(const dscene& S, int root, const v3& o, const v3& d, double tmin, double tmax, double& t,
                        int32_t& which, tstack& st, int sp_base, wcount& wc, int alpha, int64_t mesh_first,
                        uint64_t mkey, uint32_t serial, bool any_hit) {
     bool any = false;
-    walk4<CNT>(S.qblas, S.blas_roots[root], o, d, tmin, tmax, st, sp_base, wc,
-              [&](int off, int cnt, int, int e) {
-                  const double tleaf = tmax;
-                  int ok = -1;  // the leaf's double test, made at its first hit
-                  for (int k = 0; k < cnt; k++) {
-                      double tt;
-                      if constexpr (CNT) wc.pbytes += 72u;
-                      if (!tri_t(prim_ptr(S, off + k), o, d, tmin, tmax, tt)) continue;
-                      if constexpr (FEAT & F_UV)
-                          if (alpha >= 0 && !alpha_keeps(S, alpha, mesh_first, off + k, o, d, mkey, serial)) continue;
-                      if (ok < 0) ok = leaf_exact<CNT>(S.blas, e, o, d, tmin, tleaf, wc) ? 1 : 0;
-                      if (!ok) break;
-                      any = true;
-                      tmax = tt;
-                      which = off + k;
-                      if (any_hit) return true;  // the first hit answers an occlusion query: stop the walk
-                  }
-                  return false;
-              });
+    walk4<CNT>(S.qblas, S.blas_roots[root], o, d, tmin, tmax, st, sp_base, wc, [&](int off, int cnt, int, int e) {
+        return mesh_leaf<CNT, FEAT>(S, off, cnt, e, o, d, tmin, tmax, which, any, alpha, mesh_first, mkey, serial,
+                                    any_hit, wc);
+    });
     t = tmax;
     return any;
 }
@@ -856,6 +891,182 @@ __device__ __forceinline__ bool fog_t(const fog_rec& f, const v3& o, const v3& d
     return true;
 }
 
+// bvh::hit over the world with the meshes' own bvh::hit (bvh.h:45-86,
+// instance.h, mesh.h) in one loop: a wave's lanes visit records of the world
+// tree and of any mesh tree in the same step, where a walk nested in the
+// world leaf's test would run each mesh tree only for the lanes that reached
+// it together. Each tree is walked as walk4 walks it, in the same order: a
+// world leaf's items are taken one per step; a mesh item enters the mesh's
+// tree (its ray in instance space, its stack part above the world's) and the
+// walk returns to the leaf's next item when that part is empty. A mesh's
+// hits shrink the interval while its tree is walked, as in the nested walk;
+// at its end the world leaf's double test (made at its first hit) accepts
+// them or restores the interval and closes the leaf. tmax: in, the
+// interval's end; out, the closest accepted hit's t.
+// 0 builds the nested walk for scenes with meshes instead (for comparison).
+#ifndef MPT_FLAT_WALK
+#define MPT_FLAT_WALK 1
+#endif
+template <bool CNT, int FEAT>
+__device__ __forceinline__ bool walk_world(const dscene& S, const v3& o, const v3& d, double tmin, double& tmax,
+                                           hitinfo& h, tstack& st, bool media, uint64_t mkey, uint32_t serial,
+                                           wcount& wc, bool any_hit) {
+    if constexpr (CNT) wc.nodes += 64u;
+    const croot& root = S.tlas_root;
+    fray fr;
+    int cur = -1;
+    // the open world leaf: items [l_off + l_k, l_off + l_cnt), its child-pair
+    // record l_e (-1: the root), the interval's end when it was entered and
+    // its double test (-1: not made yet)
+    int l_off = 0, l_cnt = 0, l_k = 0, l_e = -1, l_ok = -1;
+    double l_tleaf = tmax;
+    {
+        const v3 inv{1 / d.x, 1 / d.y, 1 / d.z};
+        if (!box_hit(root.b, o, inv, tmin, tmax)) return false;
+        fr = make_fray(o, inv, tmin, root.b);
+        if (root.cnt > 0) {
+            l_off = root.off;
+            l_cnt = root.cnt;
+        } else {
+            cur = root.qoff;
+        }
+    }
+    int negmask = neg_mask(fr);
+    bool any = false;
+    // the mesh whose tree is being walked: object, stack base, instance-space
+    // ray, hit so far, the interval's end when it was entered
+    bool inmesh = false;
+    int m_oi = 0, m_alpha = -1, bsp = 0;
+    int64_t m_first = 0;
+    int32_t m_prim = -1;
+    bool m_any = false;
+    double m_tw = tmax;
+    v3 lo = o, ld = d;
+    int sp = 0;
+    // a mesh's hit goes to the world leaf's test; true: the walk stops
+    auto mesh_end = [&]() -> bool {
+        if (!m_any) return false;
+        if (l_ok < 0) l_ok = leaf_exact<CNT>(S.tlas, l_e, o, d, tmin, l_tleaf, wc) ? 1 : 0;
+        if (!l_ok) {
+            tmax = m_tw;
+            l_k = l_cnt;
+            return false;
+        }
+        any = true;
+        h.t = tmax;
+        h.obj = m_oi;
+        h.prim = m_prim;
+        return any_hit;
+    };
+    while (true) {
+        if (!inmesh && l_k < l_cnt) {  // the open world leaf's next item
+            const int oi = S.titems[l_off + l_k];
+            l_k++;
+            const obj_rec r = S.obj[oi];
+            if (r.kind == MPT_OBJ_MESH) {
+                if constexpr (CNT) wc.nodes += 64u;
+                v3 mo = o, md = d;
+                if constexpr (FEAT & F_XFORM) {
+                    if (r.xform >= 0) {
+                        const xform& x = S.xf[r.xform];
+                        mo = xf_inv_point(x, o);
+                        md = xf_inv_vector(x, d);
+                    }
+                }
+                const croot br = S.blas_roots[r.blas_root];
+                const v3 minv{1 / md.x, 1 / md.y, 1 / md.z};
+                if (!box_hit(br.b, mo, minv, tmin, tmax)) continue;
+                m_oi = oi;
+                m_alpha = r.alpha;
+                m_first = r.first;
+                m_prim = -1;
+                m_any = false;
+                m_tw = tmax;
+                lo = mo;
+                ld = md;
+                if (br.cnt > 0) {  // a one-leaf tree
+                    (void)mesh_leaf<CNT, FEAT>(S, br.off, br.cnt, -1, lo, ld, tmin, tmax, m_prim, m_any, m_alpha,
+                                               m_first, mkey, serial, any_hit, wc);
+                    if (mesh_end()) return true;
+                    continue;
+                }
+                fr = make_fray(lo, minv, tmin, br.b);
+                negmask = neg_mask(fr);
+                cur = br.qoff;
+                bsp = sp;
+                inmesh = true;
+                continue;
+            }
+            double tt;
+            int32_t pr = -1;  // a mesh never gets here: the object test is compiled without them
+            if (object_t<CNT, FEAT & ~F_MESH>(S, oi, o, d, tmin, tmax, tt, pr, st, sp, media, mkey, serial, wc,
+                                              any_hit)) {
+                if (l_ok < 0) l_ok = leaf_exact<CNT>(S.tlas, l_e, o, d, tmin, l_tleaf, wc) ? 1 : 0;
+                if (!l_ok) {
+                    l_k = l_cnt;
+                    continue;
+                }
+                any = true;
+                tmax = tt;
+                h.t = tt;
+                h.obj = oi;
+                h.prim = pr;
+                if (any_hit) return true;
+            }
+            continue;
+        }
+        // a record visit, then pops until an interior record (walk4's step);
+        // ev: 0 walking on, 1 a world leaf was opened (its items come next),
+        // 2 this tree's stack part is empty, 3 an occlusion query's first hit
+        // in the mesh
+        const qnode* qn = inmesh ? S.qblas : S.qtlas;
+        int next = -1, ev = 0;
+        auto take = [&](int32_t off, int32_t meta) {
+            const int cnt = int(uint32_t(meta) >> 28);
+            if (cnt == 0) {
+                next = off;
+            } else if (inmesh) {
+                if (mesh_leaf<CNT, FEAT>(S, off, cnt, meta & 0x0fffffff, lo, ld, tmin, tmax, m_prim, m_any, m_alpha,
+                                         m_first, mkey, serial, any_hit, wc))
+                    ev = 3;
+            } else {
+                l_off = off;
+                l_cnt = cnt;
+                l_k = 0;
+                l_e = meta & 0x0fffffff;
+                l_ok = -1;
+                l_tleaf = tmax;
+                ev = 1;
+            }
+        };
+        if (cur >= 0) {
+            int32_t off, meta;
+            if (visit_record<CNT>(qn, cur, fr, negmask, tmax, st, sp, off, meta, wc)) take(off, meta);
+        }
+        while (next < 0 && ev == 0) {
+            if (sp == (inmesh ? bsp : 0)) {
+                ev = 2;
+                break;
+            }
+            int32_t off, meta;
+            if (pop_slot<CNT>(qn, st.get(--sp), fr, tmax, off, meta, wc)) take(off, meta);
+        }
+        cur = next;
+        if (ev >= 2) {
+            if (!inmesh) break;  // the world's walk is done
+            // the mesh's walk is done (or stopped at an occlusion query's
+            // hit): back to the world leaf's items
+            sp = bsp;
+            inmesh = false;
+            if (mesh_end()) return true;
+            const v3 inv{1 / d.x, 1 / d.y, 1 / d.z};
+            fr = make_fray(o, inv, tmin, root.b);
+            negmask = neg_mask(fr);
+        }
+    }
+    return any;
+}
+
 // bvh::hit over the world (bvh.h:45-86), then the unbounded objects beside
 // it in order (main.cpp:70-74): the height fogs, the ground. media: test
 // constant_medium objects and fogs with draws from (mkey, serial); otherwise
@@ -874,7 +1085,9 @@ __device__ bool trace_closest(const dscene& S, const v3& o, const v3& d, double 
     h.prim = -1;
     h.t = tmax;
     bool any = false;
-    if (S.nworld > 0) walk4<CNT>(S.qtlas, S.tlas_root, o, d, tmin, tmax, st, 0, wc, [&](int off, int cnt, int sp, int e) {
+    if constexpr ((FEAT & F_MESH) && MPT_FLAT_WALK) {
+        if (S.nworld > 0) any = walk_world<CNT, FEAT>(S, o, d, tmin, tmax, h, st, media, mkey, serial, wc, any_hit);
+    } else if (S.nworld > 0) walk4<CNT>(S.qtlas, S.tlas_root, o, d, tmin, tmax, st, 0, wc, [&](int off, int cnt, int sp, int e) {
         const double tleaf = tmax;
         int ok = -1;  // the leaf's double test, made at its first hit
         for (int k = 0; k < cnt; k++) {

@@ -704,11 +704,12 @@ __global__ __launch_bounds__(BS) void k_features(dscene S, launch_ctx L, pstate 
 #endif
 // Waves per SIMD of the general-geometry traversal kernels (meshes,
 // instances, media). Uncapped they take about 200 VGPRs (255 with an
-// environment in the shadow kernel) and run 1-2 waves per SIMD; capped at 3
-// the batch of 8 generated scenes went from 9.7 to 15.7 M paths/s (4 waves:
-// 15.5 M), since their traversal waits on HBM and needs waves in flight.
+// environment in the shadow kernel) and run 1-2 waves per SIMD. With the
+// world and mesh trees walked in one loop, the batch of 8 generated scenes
+// ran 24.0 / 28.4 / 31.8 / 18.8 / 13.0 M paths/s at 2 / 3 / 4 / 5 / 6 waves
+// (from 5 on, spills take over).
 #ifndef MPT_GENERAL_WAVES
-#define MPT_GENERAL_WAVES 3
+#define MPT_GENERAL_WAVES 4
 #endif
 
 // Closest hit of path p's ray and the hit record shade reads (point, face
