@@ -982,7 +982,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MPT_SHADE_WA
 // closest hit along the normalized direction, through surfaces that let
 // shadows through when transparent shadows are on, emission found (or the
 // sky), and the MIS-weighted contribution added to the path's radiance.
-template <bool CNT, int FEAT, bool ENV>
+// OCC: compiled for scenes whose shadow rays are occlusion queries only (no
+// emitter, glow or shadow-passing material), so the kernel carries neither
+// the hit record nor the materials of the general loop.
+template <bool CNT, int FEAT, bool ENV, bool OCC = false>
 __device__ __forceinline__ void shadow_one(const dscene& S, const dcam& C, const pstate& P, int depth, uint32_t p,
                                            tstack& st, wcount& wc) {
     {
@@ -993,30 +996,38 @@ __device__ __forceinline__ void shadow_one(const dscene& S, const dcam& C, const
         v3 emission;
         // nothing emits and nothing lets shadows through: whatever the ray hits
         // first gives no emission, so only whether it hits anything matters
-        const bool occlusion = S.shadow_occlusion != 0;
-        for (int k = 0;; k++) {  // camera.h:488-502
+        if constexpr (OCC) {
             hitinfo h;
-            const uint32_t serial = uint32_t(2 * depth + 1) | (uint32_t(k) << 20);
-            if (!trace_closest<CNT, FEAT>(S, o, dir, 0.001, kInf, h, st, true, P.key[p], serial, wc, occlusion)) {
-                emission = escaped<ENV>(S, C, to_light, false);
+            emission = trace_closest<CNT, FEAT>(S, o, dir, 0.001, kInf, h, st, true, P.key[p], uint32_t(2 * depth + 1),
+                                                wc, true)
+                           ? v3{0, 0, 0}
+                           : escaped<ENV>(S, C, to_light, false);
+        } else {
+            const bool occlusion = S.shadow_occlusion != 0;
+            for (int k = 0;; k++) {  // camera.h:488-502
+                hitinfo h;
+                const uint32_t serial = uint32_t(2 * depth + 1) | (uint32_t(k) << 20);
+                if (!trace_closest<CNT, FEAT>(S, o, dir, 0.001, kInf, h, st, true, P.key[p], serial, wc, occlusion)) {
+                    emission = escaped<ENV>(S, C, to_light, false);
+                    break;
+                }
+                if (occlusion) {
+                    emission = v3{0, 0, 0};
+                    break;
+                }
+                hitrec rec;
+                finish_hit<FEAT>(S, h, o, dir, rec);
+                const mpt_material& m = S.mat[rec.mat];
+                if (C.transparent_shadows && k < 16 && mat_lets_shadows(m)) {
+                    through = through * mat_interior(m, dir, rec);
+                    o = rec.p;
+                    continue;
+                }
+                emission = mat_emitted(m, dir, rec);
+                if constexpr ((FEAT & F_UV) != 0)  // a glowing surface in the way gives its glow (camera.h:500)
+                    if (m.glow && rec.front) emission = emission + tex_value(S, m.glow_texture, rec);
                 break;
             }
-            if (occlusion) {
-                emission = v3{0, 0, 0};
-                break;
-            }
-            hitrec rec;
-            finish_hit<FEAT>(S, h, o, dir, rec);
-            const mpt_material& m = S.mat[rec.mat];
-            if (C.transparent_shadows && k < 16 && mat_lets_shadows(m)) {
-                through = through * mat_interior(m, dir, rec);
-                o = rec.p;
-                continue;
-            }
-            emission = mat_emitted(m, dir, rec);
-            if constexpr ((FEAT & F_UV) != 0)  // a glowing surface in the way gives its glow (camera.h:500)
-                if (m.glow && rec.front) emission = emission + tex_value(S, m.glow_texture, rec);
-            break;
         }
         emission = through * emission;
         const double sw = pd(P, D_SW, p);
@@ -1026,7 +1037,7 @@ __device__ __forceinline__ void shadow_one(const dscene& S, const dcam& C, const
     }
 }
 
-template <int BS, bool CNT, int FEAT, bool ENV, bool MULTI = false>
+template <int BS, bool CNT, int FEAT, bool ENV, bool MULTI = false, bool OCC = false>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_ALL ? MPT_GENERAL_WAVES : 1))) void k_shadow(dscene S, dcam C, launch_ctx L, pstate P, int depth,
                                                const uint32_t* qshadow, const uint32_t* cnt_shadow, uint32_t* ccls,
                                                unsigned long long* ray_stats, unsigned long long* work, mbatch M) {
@@ -1041,10 +1052,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_AL
         const uint32_t p = qshadow[i];
         if constexpr (MULTI) {
             with_scene(M, p, [&](const dscene& SS, const dcam& CC, int) {
-                shadow_one<CNT, FEAT, ENV>(SS, CC, P, depth, p, st, wc);
+                shadow_one<CNT, FEAT, ENV, OCC>(SS, CC, P, depth, p, st, wc);
             });
         } else {
-            shadow_one<CNT, FEAT, ENV>(S, C, P, depth, p, st, wc);
+            shadow_one<CNT, FEAT, ENV, OCC>(S, C, P, depth, p, st, wc);
         }
     }
     if constexpr (CNT) {
@@ -2170,10 +2181,22 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
         if (has_env) SHD2(FEAT, true);  \
         else SHD2(FEAT, false);         \
     } while (0)
+#define SHD_OCC(CNT, ENV)                                                                                          \
+    hipLaunchKernelGGL((k_shadow<256, CNT, F_ALL, ENV, false, true>), dim3(G), dim3(256), 0, stream, DS, C, L, P, \
+                       depth, q[2], ctr(cnt, 2), ccls, ray_stats, work, MB)
         if (s->profile == 1) SHD(F_SPHERE);
         else if (s->profile == 2) SHD(kQuadProfile);
         else if (s->profile == 3) SHD(F_ALL | F_UV);  // the ground occludes too
-        else SHD(F_ALL);
+        else if (DS.shadow_occlusion) {  // occlusion queries only: the kernel without the general loop
+            if (counting) {
+                if (has_env) SHD_OCC(true, true);
+                else SHD_OCC(true, false);
+            } else {
+                if (has_env) SHD_OCC(false, true);
+                else SHD_OCC(false, false);
+            }
+        } else SHD(F_ALL);
+#undef SHD_OCC
 #undef SHD
 #undef SHD2
     };
@@ -2589,6 +2612,8 @@ int render_batch_impl(mpt_scene* const* ss, const mpt_camera* cams, int n, const
     const unsigned G = unsigned(s->grid);
     hipStream_t stream = s->stream;
     const dscene DS = s->ds;  // unused by the batch kernels (they read M.S)
+    bool all_occlusion = true;
+    for (int k = 0; k < n; k++) all_occlusion = all_occlusion && ss[k]->ds.shadow_occlusion != 0;
     const dcam C0 = C[0];
     const nrc_route R{-1, 0, 1, 0, nullptr, nullptr};
 
@@ -2659,16 +2684,24 @@ int render_batch_impl(mpt_scene* const* ss, const mpt_camera* cams, int n, const
                                cin, cout, ctr(cnt, 2), qcls, ccls, cap, ray_stats, work, R, M);
     };
     auto launch_shadow = [&](int depth) {
-#define BSHD(CNT, ENV)                                                                                             \
-    hipLaunchKernelGGL((k_shadow<256, CNT, F_ALL, ENV, true>), dim3(G), dim3(256), 0, stream, DS, C0, L, P, depth, \
+#define BSHD(CNT, ENV, OCC)                                                                                          \
+    hipLaunchKernelGGL((k_shadow<256, CNT, F_ALL, ENV, true, OCC>), dim3(G), dim3(256), 0, stream, DS, C0, L, P, depth, \
                        q[2], ctr(cnt, 2), ccls, ray_stats, work, M)
-        if (counting) {
-            if (has_env) BSHD(true, true);
-            else BSHD(true, false);
+#define BSHD2(ENV, OCC)                    \
+    do {                                   \
+        if (counting) BSHD(true, ENV, OCC); \
+        else BSHD(false, ENV, OCC);        \
+    } while (0)
+        // occlusion queries only when no scene of the batch has an emitter,
+        // a glow or a material that lets shadows through
+        if (all_occlusion) {
+            if (has_env) BSHD2(true, true);
+            else BSHD2(false, true);
         } else {
-            if (has_env) BSHD(false, true);
-            else BSHD(false, false);
+            if (has_env) BSHD2(true, false);
+            else BSHD2(false, false);
         }
+#undef BSHD2
 #undef BSHD
     };
     auto launch_shade = [&](int cls, int depth, uint32_t* qout, uint32_t* cout) {
