@@ -1071,9 +1071,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_AL
 // stage kernels do for the path, in the same order (extend, shade by
 // material, shadow ray), so the results are the same.
 // Waves per SIMD k_finish is compiled for: it holds every material's code,
-// and without a cap its register footprint allows one wave per SIMD.
+// and without a cap its register footprint allows one wave per SIMD. The
+// general-geometry variant is capped at 4: the batch of 8 generated scenes
+// ran 32.2 / 33.2 / 33.6 M paths/s at 2 / 3 / 4 waves.
 #ifndef MPT_FINISH_WAVES
 #define MPT_FINISH_WAVES 2
+#endif
+#ifndef MPT_GENERAL_FINISH_WAVES
+#define MPT_GENERAL_FINISH_WAVES 4
 #endif
 
 // Every remaining bounce of path p from depth0 on (one thread).
@@ -1114,7 +1119,7 @@ __device__ __forceinline__ void finish_path(const dscene& S, const dcam& C, cons
 }
 
 template <int BS, int FEAT, int LK, bool ENV, bool MULTI = false>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MPT_FINISH_WAVES))) void k_finish(
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FEAT == F_ALL ? MPT_GENERAL_FINISH_WAVES : MPT_FINISH_WAVES))) void k_finish(
     dscene S, dcam C, launch_ctx L, pstate P, int depth0, const uint32_t* q, const uint32_t* cnt_in,
     unsigned long long* ray_stats, mbatch M) {
     STACK_SETUP();
