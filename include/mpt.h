@@ -326,7 +326,9 @@ typedef struct mpt_render_opts {
     int32_t save_every;            /* call on_pass every this many passes (0 = never) */
     int32_t passes_in_flight;      /* samples per pixel traced together (0 = auto) */
     int32_t blocks_per_cu;         /* 256-thread blocks per CU in the persistent stage grid, 1..64
-                                    * (0 = auto: the tuned choice, else 16) */
+                                    * (0 = auto: the tuned choice, else 16; 4, one resident
+                                    * round, for scenes on the general-geometry kernels and for
+                                    * mpt_render_batch) */
     int32_t sort_by_material;      /* 1 = on: each bounce is a wavefront of stage kernels, hits binned
                                     * into one queue per material and shaded by that material's
                                     * kernel; -1 = off: each batch runs in one kernel, a path per

@@ -1477,15 +1477,21 @@ int ensure_ovf(mpt_scene* s) {
 
 constexpr int kDefaultBlocksPerCu = 16;
 
+// Persistent stage kernels walk their queue with a grid-wide stride. The
+// default grid is 16 blocks of 256 threads per CU, more than a CU holds at
+// once: a second round of blocks starts as soon as the first ones run out of
+// queue (2% faster than one round on the spheres scene). Scenes on the
+// general-geometry kernels (and batches, which always run them) take one
+// round instead, MPT_GENERAL_WAVES blocks per CU, as many as their 4-wave
+// cap keeps resident: their walks are long and uneven, and a late round
+// leaves a tail (the batch of 8 generated scenes: 36.6 M paths/s against
+// 33.7 M with 16 blocks, 31.1 M with 6). Renders may set another size
+// (blocks_per_cu) and the autotuner tries 8, 16 and 32 besides the default.
+int default_blocks_per_cu(const mpt_scene* s) { return s->profile == 0 ? MPT_GENERAL_WAVES : kDefaultBlocksPerCu; }
+
 int grid_for(mpt_scene* s) {
-    // Persistent stage kernels walk their queue with a grid-wide stride. The
-    // default grid is 16 blocks of 256 threads per CU, twice what a CU holds
-    // at once for the traversal kernels: a second round of blocks starts as
-    // soon as the first ones run out of queue (2% faster than one round on
-    // the spheres scene, 3% on the generated batch); renders may set another
-    // size (blocks_per_cu) and the autotuner tries 8 and 32.
     s->block = 256;
-    s->grid = s->cu_count * kDefaultBlocksPerCu;
+    s->grid = s->cu_count * default_blocks_per_cu(s);
     return MPT_OK;
 }
 
@@ -2017,7 +2023,8 @@ int render_impl(mpt_scene* s, const mpt_camera* cam, const mpt_render_opts* opts
     int rc;
     if ((rc = ensure_paths(s, int64_t(pif) * npix))) return rc;
     const bool tuned = s->tuned_pix == npix64;
-    int bpc = o.blocks_per_cu > 0 ? o.blocks_per_cu : (tuned && s->tuned_bpc > 0 ? s->tuned_bpc : kDefaultBlocksPerCu);
+    int bpc = o.blocks_per_cu > 0 ? o.blocks_per_cu
+                                  : (tuned && s->tuned_bpc > 0 ? s->tuned_bpc : default_blocks_per_cu(s));
     if (bpc > 64) return set_error(MPT_E_INVALID, "blocks_per_cu must be in [1, 64]");
     if (s->grid != s->cu_count * bpc) {  // the persistent grid, and the stack spill area sized for it
         s->grid = s->cu_count * bpc;
@@ -2538,7 +2545,7 @@ int render_batch_impl(mpt_scene* const* ss, const mpt_camera* cams, int n, const
     if (int64_t(pif) * pix_total > (int64_t(1) << 31)) pif = int((int64_t(1) << 31) / pix_total);
     int rc;
     if ((rc = ensure_paths(s, int64_t(pif) * pix_total))) return rc;
-    const int bpc = o.blocks_per_cu > 0 ? o.blocks_per_cu : kDefaultBlocksPerCu;
+    const int bpc = o.blocks_per_cu > 0 ? o.blocks_per_cu : MPT_GENERAL_WAVES;  // the batch kernels' one round
     if (bpc > 64) return set_error(MPT_E_INVALID, "blocks_per_cu must be in [1, 64]");
     if (s->grid != s->cu_count * bpc) {
         s->grid = s->cu_count * bpc;
@@ -2874,12 +2881,13 @@ int32_t mpt_autotune(mpt_scene* s, const mpt_camera* cam, int32_t trial_passes, 
         if (int64_t(v) * npix > (int64_t(1) << 31)) continue;
         if (std::find(pifs.begin(), pifs.end(), v) == pifs.end()) pifs.push_back(v);
     }
-    const int64_t nthreads = int64_t(s->cu_count) * kDefaultBlocksPerCu * 256;
+    const int bpc0 = default_blocks_per_cu(s);
+    const int64_t nthreads = int64_t(s->cu_count) * bpc0 * 256;
     const int64_t tails[] = {-1, nthreads / 2, nthreads, 2 * nthreads, 4 * nthreads};
     mpt_render_opts o{};
     o.seed = 0x5EED;
     o.pass_count = trial_passes;
-    o.blocks_per_cu = kDefaultBlocksPerCu;
+    o.blocks_per_cu = bpc0;
     o.sort_by_material = 1;
     o.use_graph = -1;
     s->tuned_pix = 0;
@@ -2917,7 +2925,8 @@ int32_t mpt_autotune(mpt_scene* s, const mpt_camera* cam, int32_t trial_passes, 
     // from the best: the grid, then no binning, then graphs
     o.passes_in_flight = best.passes_in_flight;
     o.tail_paths = int32_t(best.tail_paths);
-    for (int bpc : {8, 32}) {
+    for (int bpc : {8, 16, 32}) {
+        if (bpc == bpc0) continue;
         o.blocks_per_cu = bpc;
         if ((rc = trial())) return rc;
     }
