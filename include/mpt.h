@@ -328,7 +328,8 @@ typedef struct mpt_render_opts {
     int32_t blocks_per_cu;         /* 256-thread blocks per CU in the persistent stage grid, 1..64
                                     * (0 = auto: the tuned choice, else 16; 4, one resident
                                     * round, for scenes on the general-geometry kernels and for
-                                    * mpt_render_batch) */
+                                    * mpt_render_batch; one resident round for scenes with quads,
+                                    * lists or instances and no meshes or media) */
     int32_t sort_by_material;      /* 1 = on: each bounce is a wavefront of stage kernels, hits binned
                                     * into one queue per material and shaded by that material's
                                     * kernel; -1 = off: each batch runs in one kernel, a path per
